@@ -1,0 +1,159 @@
+"""ctypes binding of libdust_gpu.so (the C ABI declared in include/dust_gpu.h).
+
+The library is loaded from this package directory only.  There is no fallback:
+if the shared object is missing or fails to load, every entry point raises.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdust_gpu.so")
+
+DUST_OK = 0
+DUST_E_ARG = -1
+DUST_E_HIP = -2
+DUST_E_OUTSIDE = -3
+DUST_E_CAPACITY = -4
+DUST_E_NOMEM = -5
+DUST_E_UNSUPPORTED = -6
+
+FMM_FP64 = 0
+FMM_TUNED = 1
+
+LU_FP64 = 0
+LU_FP32 = 1
+LU_FP16S = 2
+
+STAGES = ("sort", "p2m", "m2m", "m2l", "l2l", "p2p")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+
+class FmmParams(C.Structure):
+    _fields_ = [
+        ("box_length", C.c_double),
+        ("nbox", C.c_int * 3),
+        ("origin", C.c_double * 3),
+        ("nlevels", C.c_int),
+        ("min_part", C.c_int),
+        ("degree", C.c_int),
+        ("delta", C.c_double),
+        ("use_vs", C.c_int),
+        ("use_vd", C.c_int),
+        ("use_tv", C.c_int),
+        ("nu_inf", C.c_double),
+        ("u_inf", C.c_double * 3),
+        ("precision", C.c_int),
+        ("reserved", C.c_int * 7),
+    ]
+
+
+class AicParams(C.Structure):
+    _fields_ = [
+        ("ff_ratio_doublet", C.c_double),
+        ("rankine_rad", C.c_double),
+        ("cutoff_rad", C.c_double),
+        ("reserved", C.c_int * 4),
+    ]
+
+
+# (name, restype, argtypes) of every exported function in include/dust_gpu.h
+SIGNATURES = [
+    ("dust_last_error", C.c_char_p, []),
+    ("dust_version", C.c_char_p, []),
+    ("dust_fmm_create", C.c_int, [C.POINTER(FmmParams), C.c_int, C.POINTER(_vp)]),
+    ("dust_fmm_destroy", C.c_int, [_vp]),
+    ("dust_fmm_set_particles", C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
+    ("dust_fmm_set_particles_dev", C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
+    ("dust_fmm_sort_particles", C.c_int, [_vp, _ip]),
+    ("dust_fmm_calculate_multipole", C.c_int, [_vp]),
+    ("dust_fmm_apply_multipole", C.c_int, [_vp, _vp, _vp, _vp]),
+    ("dust_fmm_apply_multipole_dev", C.c_int, [_vp]),
+    ("dust_fmm_results_dev", C.c_int, [_vp, _vp, _vp, _vp]),
+    ("dust_fmm_pass_dev", C.c_int, [_vp, _ip]),
+    ("dust_fmm_stream", _vp, [_vp]),
+    ("dust_fmm_synchronize", C.c_int, [_vp]),
+    ("dust_fmm_leaves", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    ("dust_fmm_cell_index", C.c_int, [_vp, _vp]),
+    ("dust_fmm_work", C.c_int, [_vp, _dp, _dp]),
+    ("dust_fmm_set_timing", C.c_int, [_vp, C.c_int]),
+    ("dust_fmm_stage_times", C.c_int, [_vp, _vp, _vp]),
+    ("dust_vortpart_direct", C.c_int,
+     [C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_double, _vp, _vp, _vp]),
+    ("dust_aic_surfpan", C.c_int,
+     [C.c_int, _vp, _vp, C.POINTER(AicParams), _vp, C.c_int, _vp, C.c_int]),
+    ("dust_aic_surfpan_dev", C.c_int,
+     [C.c_int, _vp, _vp, C.POINTER(AicParams), _vp, C.c_int, _vp, C.c_int, _vp]),
+    ("dust_aic_vortlatt", C.c_int,
+     [C.c_int, _vp, _vp, C.POINTER(AicParams), _vp, C.c_int, _vp, C.c_int]),
+    ("dust_aic_rhs", C.c_int,
+     [C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int]),
+    ("dust_dgetrf", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp]),
+    ("dust_dgetrs", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int]),
+    ("dust_dgetrf_dev", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp]),
+    ("dust_dgetrs_dev", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, _vp]),
+    ("dust_dgesv_mixed", C.c_int,
+     [C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _dp]),
+    ("dust_dgesv_mixed_dev", C.c_int,
+     [C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _dp,
+      _vp]),
+    ("dust_solve_linsys", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp]),
+    ("dust_test_gemm_sub_dev", C.c_int,
+     [C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int]),
+    ("dust_test_trsm_llnu_dev", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int]),
+]
+
+_lib = None
+
+
+class DustError(RuntimeError):
+    def __init__(self, code, where):
+        msg = ""
+        if _lib is not None:
+            raw = _lib.dust_last_error()
+            msg = raw.decode() if raw else ""
+        super().__init__(f"{where} failed with status {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libdust_gpu.so (no fallback: raises if it is absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python dust_amd/build.py` "
+                "(or __graft_entry__.build()); there is no CPU fallback")
+        # PyTorch ships its own HIP runtime under the same soname
+        # (libamdhip64.so.7).  Whichever is loaded first serves the whole
+        # process, and torch fails to initialise on a runtime other than its
+        # own, so torch (when installed) is loaded before libdust_gpu.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code, where):
+    """Raise DustError for a negative status; return LAPACK-style info (>= 0)."""
+    if code < 0:
+        raise DustError(code, where)
+    return code
+
+
+def ptr(a):
+    """Address of a numpy array or torch tensor (None stays NULL)."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data

@@ -1,0 +1,70 @@
+// Internal helpers shared by the libdust_gpu translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "../../include/dust_gpu.h"
+
+namespace dust {
+
+void set_error(const char *fmt, ...);
+
+// Evaluate a HIP call; on failure record the error text and return DUST_E_HIP
+// from the enclosing function.
+#define DUST_HIP(call)                                                            \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      ::dust::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,                 \
+                        hipGetErrorString(e_));                                   \
+      return DUST_E_HIP;                                                          \
+    }                                                                             \
+  } while (0)
+
+#define DUST_CHECK(cond, code, ...)                                               \
+  do {                                                                            \
+    if (!(cond)) {                                                                \
+      ::dust::set_error(__VA_ARGS__);                                             \
+      return (code);                                                              \
+    }                                                                             \
+  } while (0)
+
+// Propagate a non-zero status from a helper.
+#define DUST_TRY(expr)                                                            \
+  do {                                                                            \
+    int s_ = (expr);                                                              \
+    if (s_ != 0) return s_;                                                       \
+  } while (0)
+
+constexpr double kPi = 3.1415926535897932384626433832795028841971693993751;
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Device buffer that grows on demand (no shrinking).
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  int reserve(size_t count) {
+    if (count <= n) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    size_t bytes = (count ? count : 1) * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      set_error("hipMalloc of %zu bytes failed", bytes);
+      return DUST_E_NOMEM;
+    }
+    n = count;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace dust

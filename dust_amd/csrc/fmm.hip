@@ -1,0 +1,1529 @@
+// Octree FMM for vortex particles on gfx950: tree build, expansions and the
+// plain FP64 evaluation, plus the C ABI of dust_fmm_*.
+//
+// Reference behaviour: src/fmm/mod_octree.f90 (initialize_octree 251-518,
+// sort_particles 703-996, calculate_multipole 1002-1133, apply_multipole
+// 1139-1429) and src/fmm/mod_multipole.f90 (P2M 224, M2M 256, M2L 282,
+// L2L 360, compute_der_ker 388, set_degree_polyexp 454).  Particle kernels:
+// src/aeroel/mod_vortpart.f90:139-336.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+
+#include "fmm_internal.h"
+
+namespace dust {
+
+// ---------------------------------------------------------------------------
+// Error text
+// ---------------------------------------------------------------------------
+static thread_local char g_err[512] = "";
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+// ---------------------------------------------------------------------------
+// Graded monomial numbering of set_degree_polyexp (mod_multipole.f90:465-488):
+// order o = i+j+k, then i descending, then j descending.  Closed form of the
+// position: o(o+1)(o+2)/6 monomials of lower order, (o-i)(o-i+1)/2 with a
+// larger i, and o-i-j with the same i and a larger j.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int mono_index(int i, int j, int k) {
+  int o = i + j + k;
+  return o * (o + 1) * (o + 2) / 6 + (o - i) * (o - i + 1) / 2 + (o - i - j);
+}
+inline int mono_count(int deg) { return (deg + 1) * (deg + 2) * (deg + 3) / 6; }
+
+static void mono_powers(int deg, std::vector<int> &pwr) {
+  pwr.clear();
+  for (int o = 0; o <= deg; ++o)
+    for (int i = o; i >= 0; --i)
+      for (int j = o - i; j >= 0; --j) {
+        pwr.push_back(i);
+        pwr.push_back(j);
+        pwr.push_back(o - i - j);
+      }
+}
+
+// Taylor coefficients of the Rosenhead-regularised Green's function
+// 1/(4 pi sqrt(|x|^2 + delta^2)) at x = diff, by the Lindsay-Krasny recurrence
+// (mod_multipole.f90:409-434), and the derived velocity (D) and gradient (Dc)
+// tables (436-449).  Writes the first nU entries of D and Dc in the SoA order
+// used on the device: out[e*nU + m] for e = 0..2 (D) and e = 3 + c + 3 i (Dc).
+static void kernel_derivatives(int p, const double diff[3], double delta, int nU, double *out) {
+  int deg = 2 * p + 2;
+  int nb = mono_count(deg);
+  std::vector<double> bk(nb, 0.0);
+  double R2 = diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2] + delta * delta;
+  for (int k = 0; k <= deg; ++k)
+    for (int j = 0; j <= deg - k; ++j)
+      for (int i = 0; i <= deg - j - k; ++i) {
+        if (i == 0 && j == 0 && k == 0) {
+          bk[0] = 1.0 / (4.0 * kPi * std::sqrt(R2));
+          continue;
+        }
+        double km = (double)(i + j + k), s1 = 0.0, s2 = 0.0;
+        if (i >= 1) s1 += diff[0] * bk[mono_index(i - 1, j, k)];
+        if (j >= 1) s1 += diff[1] * bk[mono_index(i, j - 1, k)];
+        if (k >= 1) s1 += diff[2] * bk[mono_index(i, j, k - 1)];
+        if (i >= 2) s2 += bk[mono_index(i - 2, j, k)];
+        if (j >= 2) s2 += bk[mono_index(i, j - 2, k)];
+        if (k >= 2) s2 += bk[mono_index(i, j, k - 2)];
+        bk[mono_index(i, j, k)] = 1.0 / (R2 * km) * ((2.0 * km - 1.0) * s1 - (km - 1.0) * s2);
+      }
+  int nD = mono_count(deg - 1);
+  std::vector<double> D(3 * nD, 0.0);
+  for (int k = 0; k <= deg - 1; ++k)
+    for (int j = 0; j <= deg - 1 - k; ++j)
+      for (int i = 0; i <= deg - 1 - j - k; ++i) {
+        int m = mono_index(i, j, k);
+        D[3 * m + 0] = -(1.0 + i) * bk[mono_index(i + 1, j, k)];
+        D[3 * m + 1] = -(1.0 + j) * bk[mono_index(i, j + 1, k)];
+        D[3 * m + 2] = -(1.0 + k) * bk[mono_index(i, j, k + 1)];
+      }
+  for (int m = 0; m < nU; ++m)
+    for (int c = 0; c < 3; ++c) out[c * nU + m] = D[3 * m + c];
+  // Dc(:,ii,m) = -(1+pwr_ii(m)) D(:, m + e_ii); only orders <= 2p are needed.
+  std::vector<int> pw;
+  mono_powers(2 * p, pw);
+  for (int m = 0; m < nU; ++m) {
+    int i = pw[3 * m], j = pw[3 * m + 1], k = pw[3 * m + 2];
+    int mx = mono_index(i + 1, j, k), my = mono_index(i, j + 1, k), mz = mono_index(i, j, k + 1);
+    for (int c = 0; c < 3; ++c) {
+      out[(3 + c + 0) * nU + m] = -(1.0 + i) * D[3 * mx + c];
+      out[(3 + c + 3) * nU + m] = -(1.0 + j) * D[3 * my + c];
+      out[(3 + c + 6) * nU + m] = -(1.0 + k) * D[3 * mz + c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Device helpers for the cell numbering
+// ---------------------------------------------------------------------------
+struct TreeGeom {
+  int nlev;
+  int nbox[3];
+  int ncl[kMaxLevels][3];
+  long off[kMaxLevels];
+  int kdl[kMaxLevels][3];
+  long kd_base[kMaxLevels];
+  double xmin[3], xmax[3];
+  double csize_fine;
+};
+
+__host__ __device__ inline unsigned spread3(unsigned v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__host__ __device__ inline unsigned compact3(unsigned v) {
+  v &= 0x09249249u;
+  v = (v | (v >> 2)) & 0x030C30C3u;
+  v = (v | (v >> 4)) & 0x0300F00Fu;
+  v = (v | (v >> 8)) & 0x030000FFu;
+  v = (v | (v >> 16)) & 0x000003FFu;
+  return v;
+}
+
+// 0-based cartesian cell coordinates -> cell number within level lvl.
+__host__ __device__ inline long cell_encode(const TreeGeom &G, int lvl, int ix, int iy, int iz) {
+  int bx = ix >> lvl, by = iy >> lvl, bz = iz >> lvl;
+  unsigned msk = (1u << lvl) - 1u;
+  long box = bx + (long)G.nbox[0] * (by + (long)G.nbox[1] * bz);
+  unsigned m = spread3(ix & msk) | (spread3(iy & msk) << 1) | (spread3(iz & msk) << 2);
+  return (box << (3 * lvl)) | (long)m;
+}
+__host__ __device__ inline void cell_decode(const TreeGeom &G, int lvl, long c, int &ix, int &iy,
+                                            int &iz) {
+  long box = c >> (3 * lvl);
+  unsigned m = (unsigned)(c & ((1L << (3 * lvl)) - 1));
+  int bx = (int)(box % G.nbox[0]);
+  int by = (int)((box / G.nbox[0]) % G.nbox[1]);
+  int bz = (int)(box / ((long)G.nbox[0] * G.nbox[1]));
+  ix = (bx << lvl) | (int)compact3(m);
+  iy = (by << lvl) | (int)compact3(m >> 1);
+  iz = (bz << lvl) | (int)compact3(m >> 2);
+}
+// Cell centre, evaluated as mod_octree.f90:332-334,363-366.
+__host__ __device__ inline double cell_center(const TreeGeom &G, int lvl, int d, int i0) {
+  return G.xmin[d] + ((double)(i0 + 1) - 0.5) * (G.xmax[d] - G.xmin[d]) / (double)(G.nbox[d] << lvl);
+}
+__device__ inline int lvl_of(const TreeGeom &G, long g) {
+  int l = 0;
+  while (l + 1 < G.nlev && g >= G.off[l + 1]) ++l;
+  return l;
+}
+// Particle range [s, e) of cell c at level lvl in the sorted arrays.
+__device__ inline int2 cell_range(const TreeGeom &G, const int *fstart, int lvl, long c) {
+  int sh = 3 * (G.nlev - 1 - lvl);
+  return make_int2(fstart[c << sh], fstart[(c + 1) << sh]);
+}
+
+static TreeGeom make_geom(const FmmHandle *h) {
+  TreeGeom G;
+  memset(&G, 0, sizeof(G));
+  G.nlev = h->nlev;
+  for (int d = 0; d < 3; ++d) {
+    G.nbox[d] = h->prm.nbox[d];
+    G.xmin[d] = h->xmin[d];
+    G.xmax[d] = h->xmax[d];
+  }
+  for (int l = 0; l < h->nlev; ++l) {
+    for (int d = 0; d < 3; ++d) {
+      G.ncl[l][d] = h->lev[l].ncl[d];
+      G.kdl[l][d] = h->lev[l].kdl[d];
+    }
+    G.off[l] = h->lev[l].off;
+    G.kd_base[l] = h->lev[l].kd_base;
+  }
+  G.csize_fine = h->lev[h->nlev - 1].csize;
+  return G;
+}
+
+// ---------------------------------------------------------------------------
+// Tree build kernels (sort_particles, mod_octree.f90:703-996)
+// ---------------------------------------------------------------------------
+
+// Finest-level cell of every particle: idx = ceiling((x - xmin)/csize)
+// (mod_octree.f90:755), with the out-of-box check of 758-762.
+__global__ void k_keys(TreeGeom G, int n, const double *__restrict__ cen, unsigned *key, int *idx,
+                       int *err) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int L = G.nlev - 1;
+  int id[3];
+  bool bad = false;
+  for (int d = 0; d < 3; ++d) {
+    double q = ceil((cen[3 * i + d] - G.xmin[d]) / G.csize_fine);
+    // compare in floating point first so huge values cannot overflow the int
+    if (!(q >= 1.0 && q <= (double)G.ncl[L][d])) bad = true;
+    id[d] = bad ? 1 : (int)q;
+  }
+  if (bad) {
+    atomicMin(err, i);
+    key[i] = 0;
+  } else {
+    key[i] = (unsigned)cell_encode(G, L, id[0] - 1, id[1] - 1, id[2] - 1);
+  }
+  idx[i] = i;
+}
+
+// Gather the particle data into sorted SoA arrays.
+__global__ void k_gather(int n, const int *__restrict__ perm, const double *__restrict__ cen,
+                         const double *__restrict__ dir, const double *__restrict__ mag,
+                         const double *__restrict__ rv, double *px, double *py, double *pz,
+                         double *dx, double *dy, double *dz, double *pm, double *pr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int q = perm[i];
+  px[i] = cen[3 * q];
+  py[i] = cen[3 * q + 1];
+  pz[i] = cen[3 * q + 2];
+  dx[i] = dir[3 * q];
+  dy[i] = dir[3 * q + 1];
+  dz[i] = dir[3 * q + 2];
+  pm[i] = mag[q];
+  pr[i] = rv[q];
+}
+
+// fstart[c] = first sorted particle with key >= c, for c = 0..nfine.
+__global__ void k_fstart(long nfine, int n, const unsigned *__restrict__ ks, int *fstart) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > nfine) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if ((long)ks[mid] < c) lo = mid + 1; else hi = mid;
+  }
+  fstart[c] = lo;
+}
+
+// Finest level: leaf iff npart >= min_part, or the tree has one level
+// (mod_octree.f90:898-907).
+__global__ void k_flags_fine(TreeGeom G, const int *__restrict__ fstart, int min_part,
+                             unsigned char *flags) {
+  int L = G.nlev - 1;
+  long nc = (long)G.ncl[L][0] * G.ncl[L][1] * G.ncl[L][2];
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  int np = fstart[c + 1] - fstart[c];
+  flags[G.off[L] + c] = (np >= min_part || G.nlev == 1) ? kLeaf : 0;
+}
+
+// One level upwards (mod_octree.f90:911-961): a cell with a leaf or branch
+// child becomes a branch and its other children become leaves; otherwise its
+// children are de-activated and it is a leaf if it holds min_part particles
+// (always on level 1).
+__global__ void k_flags_level(TreeGeom G, int lvl, const int *__restrict__ fstart, int min_part,
+                              unsigned char *flags) {
+  long nc = (long)G.ncl[lvl][0] * G.ncl[lvl][1] * G.ncl[lvl][2];
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  unsigned char *ch = &flags[G.off[lvl + 1] + 8 * c];
+  bool got = false;
+  for (int p = 0; p < 8; ++p) got |= (ch[p] != 0);
+  if (got) {
+    for (int p = 0; p < 8; ++p)
+      if (ch[p] == 0) ch[p] = kLeaf;
+    flags[G.off[lvl] + c] = kBranch;
+  } else {
+    for (int p = 0; p < 8; ++p) ch[p] = 0;
+    int2 r = cell_range(G, fstart, lvl, c);
+    flags[G.off[lvl] + c] = (r.y - r.x >= min_part || lvl == 0) ? kLeaf : 0;
+  }
+}
+
+__global__ void k_fill_int(long n, int *p, int v) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_slots(int n_active, const int *__restrict__ active, int *slot) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_active) slot[active[i]] = i;
+}
+
+// First position of each level in the ascending active list.
+__global__ void k_level_starts(TreeGeom G, int n_active, const int *__restrict__ active, int *out) {
+  int l = threadIdx.x;
+  if (l > G.nlev) return;
+  long key = (l == G.nlev) ? 0x7fffffffffffL : G.off[l];
+  int lo = 0, hi = n_active;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if ((long)active[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  out[l] = lo;
+}
+
+// P2P source list of every leaf, in the order of apply_multipole
+// (mod_octree.f90:1258-1348): the 26 same-level neighbours that are active
+// (their whole subtree), then leaf_neigh = leaves among the neighbours of
+// every ancestor from level 1 down to the parent (964-988), then the leaf
+// itself (the target is skipped by index).  Each entry is a range of sorted
+// particles.
+__global__ void k_p2p_lists(TreeGeom G, int n_leaves, const int *__restrict__ leaves,
+                            const unsigned char *__restrict__ flags,
+                            const int *__restrict__ fstart, int cap, int2 *rng, int *cnt) {
+  int lv = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lv >= n_leaves) return;
+  long g = leaves[lv];
+  int lvl = lvl_of(G, g);
+  long c = g - G.off[lvl];
+  int ix, iy, iz;
+  cell_decode(G, lvl, c, ix, iy, iz);
+  int2 *out = &rng[(long)lv * cap];
+  int w = 0;
+  for (int kk = -1; kk <= 1; ++kk)
+    for (int jj = -1; jj <= 1; ++jj)
+      for (int ii = -1; ii <= 1; ++ii) {
+        if (!ii && !jj && !kk) continue;
+        int a = ix + ii, b = iy + jj, d = iz + kk;
+        if (a < 0 || b < 0 || d < 0 || a >= G.ncl[lvl][0] || b >= G.ncl[lvl][1] ||
+            d >= G.ncl[lvl][2])
+          continue;
+        long nc = cell_encode(G, lvl, a, b, d);
+        if (flags[G.off[lvl] + nc] == 0) continue;
+        int2 r = cell_range(G, fstart, lvl, nc);
+        if (r.y > r.x) out[w++] = r;
+      }
+  for (int al = 0; al < lvl; ++al) {
+    int sh = lvl - al;
+    int ax = ix >> sh, ay = iy >> sh, az = iz >> sh;
+    for (int kk = -1; kk <= 1; ++kk)
+      for (int jj = -1; jj <= 1; ++jj)
+        for (int ii = -1; ii <= 1; ++ii) {
+          if (!ii && !jj && !kk) continue;
+          int a = ax + ii, b = ay + jj, d = az + kk;
+          if (a < 0 || b < 0 || d < 0 || a >= G.ncl[al][0] || b >= G.ncl[al][1] ||
+              d >= G.ncl[al][2])
+            continue;
+          long nc = cell_encode(G, al, a, b, d);
+          if (flags[G.off[al] + nc] != kLeaf) continue;
+          int2 r = cell_range(G, fstart, al, nc);
+          if (r.y > r.x) out[w++] = r;
+        }
+  }
+  out[w++] = cell_range(G, fstart, lvl, c);
+  cnt[lv] = w;
+}
+
+// ---------------------------------------------------------------------------
+// Expansions (calculate_multipole, mod_octree.f90:1002-1133)
+// ---------------------------------------------------------------------------
+__device__ inline double ipowd(double x, int e) {
+  double r = 1.0;
+  for (int q = 0; q < e; ++q) r *= x;
+  return r;
+}
+
+// P2M on every leaf (leaf_M_multipole, mod_multipole.f90:224-251):
+// a(:,m) = sum_i mag_i (x_i - c)^pwr(m) dir_i, particles in list order.
+// One block per leaf, one thread per monomial; particle powers staged in LDS.
+constexpr int kP2MChunk = 64;
+__global__ void k_p2m(TreeGeom G, int deg, int nmon, const int *__restrict__ pwr,
+                      const int *__restrict__ leaves, const int *__restrict__ slot,
+                      const int *__restrict__ fstart, const double *__restrict__ px,
+                      const double *__restrict__ py, const double *__restrict__ pz,
+                      const double *__restrict__ dx, const double *__restrict__ dy,
+                      const double *__restrict__ dz, const double *__restrict__ pm, double *mp_a) {
+  extern __shared__ double sm[];
+  const int P1 = deg + 1;
+  double *pw = sm;                         // [kP2MChunk][3][P1]
+  double *al = sm + kP2MChunk * 3 * P1;    // [kP2MChunk][4] mag, dir
+  long g = leaves[blockIdx.x];
+  int lvl = lvl_of(G, g);
+  long c = g - G.off[lvl];
+  int ix, iy, iz;
+  cell_decode(G, lvl, c, ix, iy, iz);
+  double cx = cell_center(G, lvl, 0, ix), cy = cell_center(G, lvl, 1, iy),
+         cz = cell_center(G, lvl, 2, iz);
+  int2 r = cell_range(G, fstart, lvl, c);
+  int m = threadIdx.x;
+  int e0 = 0, e1 = 0, e2 = 0;
+  if (m < nmon) { e0 = pwr[3 * m]; e1 = pwr[3 * m + 1]; e2 = pwr[3 * m + 2]; }
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int base = r.x; base < r.y; base += kP2MChunk) {
+    int cnt = min(kP2MChunk, r.y - base);
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+      int q = base + t;
+      double d0 = px[q] - cx, d1 = py[q] - cy, d2 = pz[q] - cz;
+      double v0 = 1.0, v1 = 1.0, v2 = 1.0;
+      for (int e = 0; e < P1; ++e) {
+        pw[(t * 3 + 0) * P1 + e] = v0;
+        pw[(t * 3 + 1) * P1 + e] = v1;
+        pw[(t * 3 + 2) * P1 + e] = v2;
+        v0 *= d0; v1 *= d1; v2 *= d2;
+      }
+      al[4 * t] = pm[q];
+      al[4 * t + 1] = dx[q];
+      al[4 * t + 2] = dy[q];
+      al[4 * t + 3] = dz[q];
+    }
+    __syncthreads();
+    if (m < nmon) {
+      for (int t = 0; t < cnt; ++t) {
+        double pr = pw[(t * 3 + 0) * P1 + e0] * pw[(t * 3 + 1) * P1 + e1] * pw[(t * 3 + 2) * P1 + e2];
+        double f = al[4 * t] * pr;
+        a0 += f * al[4 * t + 1];
+        a1 += f * al[4 * t + 2];
+        a2 += f * al[4 * t + 3];
+      }
+    }
+    __syncthreads();
+  }
+  if (m < nmon) {
+    double *a = &mp_a[(long)slot[g] * 3 * nmon];
+    a[3 * m] = a0; a[3 * m + 1] = a1; a[3 * m + 2] = a2;
+  }
+}
+
+// M2M into the branches of one level (M2M_multipole, mod_multipole.f90:256-278):
+// a(:,m) = sum_children sum_{s<=m} C(m,s) (c_child - c)^(m-s) a_child(:,s).
+__global__ void k_m2m(TreeGeom G, int lvl, int deg, int nmon, const int *__restrict__ pwr,
+                      const double *__restrict__ fact, const int *__restrict__ active, int a0i,
+                      const unsigned char *__restrict__ flags, const int *__restrict__ slot,
+                      double *mp_a) {
+  __shared__ double dp[8][3][11];
+  long g = active[a0i + blockIdx.x];
+  if (flags[g] != kBranch) return;
+  long c = g - G.off[lvl];
+  int ix, iy, iz;
+  cell_decode(G, lvl, c, ix, iy, iz);
+  double cc[3] = {cell_center(G, lvl, 0, ix), cell_center(G, lvl, 1, iy),
+                  cell_center(G, lvl, 2, iz)};
+  for (int t = threadIdx.x; t < 8; t += blockDim.x) {
+    int p = t;
+    int ci[3] = {2 * ix + (p & 1), 2 * iy + ((p >> 1) & 1), 2 * iz + ((p >> 2) & 1)};
+    for (int d = 0; d < 3; ++d) {
+      double dd = cell_center(G, lvl + 1, d, ci[d]) - cc[d];
+      double v = 1.0;
+      for (int e = 0; e <= deg; ++e) { dp[p][d][e] = v; v *= dd; }
+    }
+  }
+  __syncthreads();
+  int m = threadIdx.x;
+  if (m >= nmon) return;
+  int i = pwr[3 * m], j = pwr[3 * m + 1], k = pwr[3 * m + 2];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int p = 0; p < 8; ++p) {
+    long gc = G.off[lvl + 1] + 8 * c + p;
+    const double *ac = &mp_a[(long)slot[gc] * 3 * nmon];
+    for (int ks = 0; ks <= k; ++ks)
+      for (int js = 0; js <= j; ++js)
+        for (int is = 0; is <= i; ++is) {
+          int s = mono_index(is, js, ks);
+          double nb = (fact[i] / (fact[is] * fact[i - is])) * (fact[j] / (fact[js] * fact[j - js])) *
+                      (fact[k] / (fact[ks] * fact[k - ks]));
+          double f = nb * (dp[p][0][i - is] * dp[p][1][j - js] * dp[p][2][k - ks]);
+          s0 += f * ac[3 * s];
+          s1 += f * ac[3 * s + 1];
+          s2 += f * ac[3 * s + 2];
+        }
+  }
+  double *a = &mp_a[(long)slot[g] * 3 * nmon];
+  a[3 * m] = s0; a[3 * m + 1] = s1; a[3 * m + 2] = s2;
+}
+
+// Offset of the kernel-derivative table for displacement (di,dj,dk) on level lvl.
+__device__ inline long kd_table(const TreeGeom &G, int lvl, int di, int dj, int dk) {
+  int w0 = 2 * G.kdl[lvl][0] + 1, w1 = 2 * G.kdl[lvl][1] + 1;
+  return G.kd_base[lvl] + (di + G.kdl[lvl][0]) + w0 * ((dj + G.kdl[lvl][1]) + (long)w1 * (dk + G.kdl[lvl][2]));
+}
+
+// M2L on the active cells of one level (M2L_multipole, mod_multipole.f90:282-356),
+// plain FP64.  Interaction list in the order of initialize_octree
+// (mod_octree.f90:422-476).  One block per target, one thread per output
+// monomial m; the derivative table of each source's offset and the source
+// expansion are staged in LDS.
+//   b(:,m) += (-1)^|m| sum_n mult(m,n) D(:,m+n) x a(:,n)
+//   c(i,:,m) += (-1)^|m| sum_n mult(m,n) Dc(:,i,m+n) x a(:,n)
+__global__ void k_m2l_fp64(TreeGeom G, int lvl, int nmon, int nU, int use_vs,
+                           const int *__restrict__ pwr, const double *__restrict__ multT,
+                           const int *__restrict__ mnT, const double *__restrict__ kd,
+                           const int *__restrict__ active, int a0i,
+                           const unsigned char *__restrict__ flags, const int *__restrict__ slot,
+                           const double *__restrict__ mp_a, double *mp_b, double *mp_c) {
+  extern __shared__ double sm[];
+  double *sD = sm;                 // [12][nU]
+  double *sa = sm + 12 * nU;       // [3*nmon]
+  const int ntab = use_vs ? 12 : 3;
+  long g = active[a0i + blockIdx.x];
+  long c = g - G.off[lvl];
+  int ix, iy, iz;
+  cell_decode(G, lvl, c, ix, iy, iz);
+  int m = threadIdx.x;
+  double b0 = 0, b1 = 0, b2 = 0;
+  double cg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // enumerate the interaction list; all threads walk the same sequence
+  int px_ = ix >> 1, py_ = iy >> 1, pz_ = iz >> 1;
+  int total = (lvl == 0) ? G.ncl[0][0] * G.ncl[0][1] * G.ncl[0][2] : 27 * 8;
+  for (int t = 0; t < total; ++t) {
+    int sx, sy, sz;
+    if (lvl == 0) {
+      sx = t % G.ncl[0][0];
+      sy = (t / G.ncl[0][0]) % G.ncl[0][1];
+      sz = t / (G.ncl[0][0] * G.ncl[0][1]);
+    } else {
+      int nb = t >> 3, p = t & 7;
+      int ic = nb % 3 - 1, jc = (nb / 3) % 3 - 1, kc = nb / 9 - 1;
+      if (!ic && !jc && !kc) continue;
+      int a = px_ + ic, b = py_ + jc, d = pz_ + kc;
+      if (a < 0 || b < 0 || d < 0 || a >= G.ncl[lvl - 1][0] || b >= G.ncl[lvl - 1][1] ||
+          d >= G.ncl[lvl - 1][2])
+        continue;
+      sx = 2 * a + (p & 1);
+      sy = 2 * b + ((p >> 1) & 1);
+      sz = 2 * d + ((p >> 2) & 1);
+    }
+    int di = sx - ix, dj = sy - iy, dk = sz - iz;
+    if (abs(di) <= 1 && abs(dj) <= 1 && abs(dk) <= 1) continue;
+    long gs = G.off[lvl] + cell_encode(G, lvl, sx, sy, sz);
+    if (flags[gs] == 0) continue;
+    const double *tab = &kd[kd_table(G, lvl, di, dj, dk) * 12 * (long)nU];
+    const double *as = &mp_a[(long)slot[gs] * 3 * nmon];
+    __syncthreads();
+    for (int q = threadIdx.x; q < ntab * nU; q += blockDim.x) sD[q] = tab[q];
+    for (int q = threadIdx.x; q < 3 * nmon; q += blockDim.x) sa[q] = as[q];
+    __syncthreads();
+    if (m < nmon) {
+      double v0 = 0, v1 = 0, v2 = 0;
+      double gg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int n = 0; n < nmon; ++n) {
+        int id = mnT[n * nmon + m];
+        double mu = multT[n * nmon + m];
+        double A0 = sa[3 * n], A1 = sa[3 * n + 1], A2 = sa[3 * n + 2];
+        double D0 = sD[id], D1 = sD[nU + id], D2 = sD[2 * nU + id];
+        v0 += mu * (D1 * A2 - D2 * A1);
+        v1 += mu * (D2 * A0 - D0 * A2);
+        v2 += mu * (D0 * A1 - D1 * A0);
+        if (use_vs) {
+#pragma unroll
+          for (int ii = 0; ii < 3; ++ii) {
+            double E0 = sD[(3 + 0 + 3 * ii) * nU + id], E1 = sD[(3 + 1 + 3 * ii) * nU + id],
+                   E2 = sD[(3 + 2 + 3 * ii) * nU + id];
+            gg[ii + 0] += mu * (E1 * A2 - E2 * A1);
+            gg[ii + 3] += mu * (E2 * A0 - E0 * A2);
+            gg[ii + 6] += mu * (E0 * A1 - E1 * A0);
+          }
+        }
+      }
+      b0 += v0; b1 += v1; b2 += v2;
+      for (int q = 0; q < 9; ++q) cg[q] += gg[q];
+    }
+  }
+  if (m < nmon) {
+    int i = pwr[3 * m], j = pwr[3 * m + 1], k = pwr[3 * m + 2];
+    double sgn = ((i + j + k) & 1) ? -1.0 : 1.0;
+    double *b = &mp_b[(long)slot[g] * 3 * nmon];
+    b[3 * m] += sgn * b0; b[3 * m + 1] += sgn * b1; b[3 * m + 2] += sgn * b2;
+    if (use_vs) {
+      double *cc = &mp_c[(long)slot[g] * 9 * nmon];
+      for (int q = 0; q < 9; ++q) cc[9 * m + q] += sgn * cg[q];
+    }
+  }
+}
+
+// L2L from level lvl to its active children (L2L_multipole,
+// mod_multipole.f90:360-384):
+// b_child(:,m) += sum_{s>=m} C(s,m) (c_child - c_parent)^(s-m) b_parent(:,s)
+__global__ void k_l2l(TreeGeom G, int lvl, int deg, int nmon, int use_vs,
+                      const int *__restrict__ pwr, const double *__restrict__ fact,
+                      const int *__restrict__ active, int a0i, const int *__restrict__ slot,
+                      double *mp_b, double *mp_c) {
+  __shared__ double dp[3][11];
+  long g = active[a0i + blockIdx.x];  // child at level lvl+1
+  long c = g - G.off[lvl + 1];
+  int ix, iy, iz;
+  cell_decode(G, lvl + 1, c, ix, iy, iz);
+  long gp = G.off[lvl] + (c >> 3);
+  if (threadIdx.x < 3) {
+    int d = threadIdx.x;
+    int ci = d == 0 ? ix : (d == 1 ? iy : iz);
+    double dd = cell_center(G, lvl + 1, d, ci) - cell_center(G, lvl, d, ci >> 1);
+    double v = 1.0;
+    for (int e = 0; e <= deg; ++e) { dp[d][e] = v; v *= dd; }
+  }
+  __syncthreads();
+  int m = threadIdx.x;
+  if (m >= nmon) return;
+  int i = pwr[3 * m], j = pwr[3 * m + 1], k = pwr[3 * m + 2];
+  const double *bp = &mp_b[(long)slot[gp] * 3 * nmon];
+  const double *cp = &mp_c[(long)slot[gp] * 9 * nmon];
+  double s0 = 0, s1 = 0, s2 = 0, sc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int ks = k; ks <= deg; ++ks)
+    for (int js = j; js <= deg - ks; ++js)
+      for (int is = i; is <= deg - ks - js; ++is) {
+        int s = mono_index(is, js, ks);
+        double nb = (fact[is] / (fact[i] * fact[is - i])) * (fact[js] / (fact[j] * fact[js - j])) *
+                    (fact[ks] / (fact[k] * fact[ks - k]));
+        double f = nb * (dp[0][is - i] * dp[1][js - j] * dp[2][ks - k]);
+        s0 += f * bp[3 * s];
+        s1 += f * bp[3 * s + 1];
+        s2 += f * bp[3 * s + 2];
+        if (use_vs)
+          for (int q = 0; q < 9; ++q) sc[q] += f * cp[9 * s + q];
+      }
+  double *b = &mp_b[(long)slot[g] * 3 * nmon];
+  b[3 * m] += s0; b[3 * m + 1] += s1; b[3 * m + 2] += s2;
+  if (use_vs) {
+    double *cc = &mp_c[(long)slot[g] * 9 * nmon];
+    for (int q = 0; q < 9; ++q) cc[9 * m + q] += sc[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Evaluation: L2P + P2P, plain FP64 (apply_multipole, mod_octree.f90:1139-1429)
+// ---------------------------------------------------------------------------
+struct EvalArgs {
+  TreeGeom G;
+  int deg, nmon, use_vs, use_vd, use_tv;
+  double nu_inf, uinf[3];
+  const int *pwr, *leaves, *slot, *fstart;
+  const double *mp_b, *mp_c;
+  const double *px, *py, *pz, *dx, *dy, *dz, *pm, *pr;
+  const int2 *rng;
+  const int *cnt;
+  int cap;
+  double *vel, *str, *tv;
+};
+
+constexpr int kEvalThreads = 128;
+
+// L2P of one target: vel = sum b(:,m) d^m, grad(i,j) = sum c(i,j,m) d^m.
+__device__ inline void l2p_point(const EvalArgs &A, const double *sb, const double *sc,
+                                 const int *spw, double d0, double d1, double d2, double v[3],
+                                 double g[9]) {
+  double p0[11], p1[11], p2[11];
+  p0[0] = p1[0] = p2[0] = 1.0;
+  for (int e = 1; e <= A.deg; ++e) {
+    p0[e] = p0[e - 1] * d0;
+    p1[e] = p1[e - 1] * d1;
+    p2[e] = p2[e - 1] * d2;
+  }
+  v[0] = v[1] = v[2] = 0.0;
+  for (int q = 0; q < 9; ++q) g[q] = 0.0;
+  for (int m = 0; m < A.nmon; ++m) {
+    double pr = p0[spw[3 * m]] * p1[spw[3 * m + 1]] * p2[spw[3 * m + 2]];
+    v[0] += sb[3 * m] * pr;
+    v[1] += sb[3 * m + 1] * pr;
+    v[2] += sb[3 * m + 2] * pr;
+    if (A.use_vs)
+      for (int q = 0; q < 9; ++q) g[q] += sc[9 * m + q] * pr;
+  }
+}
+
+__global__ void __launch_bounds__(kEvalThreads) k_eval_fp64(EvalArgs A) {
+  extern __shared__ double sm[];
+  const int nmon = A.nmon;
+  double *sb = sm;                       // 3*nmon
+  double *sc = sb + 3 * nmon;            // 9*nmon
+  double *ss = sc + 9 * nmon;            // [8][kEvalThreads] source tile
+  int *spw = (int *)(ss + 8 * kEvalThreads);
+  __shared__ double red[kEvalThreads / 64];
+  const int lv = blockIdx.x;
+  long g = A.leaves[lv];
+  int lvl = lvl_of(A.G, g);
+  long c = g - A.G.off[lvl];
+  int ix, iy, iz;
+  cell_decode(A.G, lvl, c, ix, iy, iz);
+  double cx = cell_center(A.G, lvl, 0, ix), cy = cell_center(A.G, lvl, 1, iy),
+         cz = cell_center(A.G, lvl, 2, iz);
+  int2 own = cell_range(A.G, A.fstart, lvl, c);
+  long sl = A.slot[g];
+  for (int q = threadIdx.x; q < 3 * nmon; q += blockDim.x) sb[q] = A.mp_b[sl * 3 * nmon + q];
+  if (A.use_vs)
+    for (int q = threadIdx.x; q < 9 * nmon; q += blockDim.x) sc[q] = A.mp_c[sl * 9 * nmon + q];
+  for (int q = threadIdx.x; q < 3 * nmon; q += blockDim.x) spw[q] = A.pwr[q];
+  __syncthreads();
+  const double fpi_inv = 1.0 / (4.0 * kPi);
+  // leaf-averaged rate of strain of the L2P gradient (mod_octree.f90:1214-1224)
+  double ave_ros = 0.0;
+  if (A.use_vd && A.use_tv) {
+    double acc = 0.0;
+    for (int t = own.x + threadIdx.x; t < own.y; t += blockDim.x) {
+      double v[3], gr[9];
+      l2p_point(A, sb, sc, spw, A.px[t] - cx, A.py[t] - cy, A.pz[t] - cz, v, gr);
+      for (int ii = 0; ii < 3; ++ii)
+        for (int jj = 0; jj < 3; ++jj) {
+          double e = 0.5 * (gr[ii + 3 * jj] + gr[jj + 3 * ii]);
+          acc += e * e;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int w = 0; w < kEvalThreads / 64; ++w) s += red[w];
+      red[0] = s;
+    }
+    __syncthreads();
+    ave_ros = red[0];
+    if (own.y > own.x) ave_ros /= (double)(own.y - own.x);
+  }
+  const int nr = A.cnt[lv];
+  const int2 *rl = &A.rng[(long)lv * A.cap];
+  for (int tb = own.x; tb < own.y; tb += blockDim.x) {
+    int t = tb + threadIdx.x;
+    bool valid = t < own.y;
+    double x = 0, y = 0, z = 0, al0 = 0, al1 = 0, al2 = 0, tvp = 0.0;
+    double v[3] = {0, 0, 0}, st[3] = {0, 0, 0};
+    if (valid) {
+      x = A.px[t]; y = A.py[t]; z = A.pz[t];
+      double mg = A.pm[t];
+      al0 = mg * A.dx[t]; al1 = mg * A.dy[t]; al2 = mg * A.dz[t];
+      double gr[9];
+      l2p_point(A, sb, sc, spw, x - cx, y - cy, z - cz, v, gr);
+      if (A.use_vs) {
+        // str = matmul(alpha, transpose(grad)): str_j = sum_i grad(j,i) alpha_i
+        for (int jj = 0; jj < 3; ++jj)
+          st[jj] = gr[jj] * al0 + gr[jj + 3] * al1 + gr[jj + 6] * al2;
+      }
+      if (A.use_tv) {
+        double rr = 0.11 * A.pr[t];
+        tvp = rr * rr * sqrt(2.0 * ave_ros);
+      }
+    }
+    double pv0 = 0, pv1 = 0, pv2 = 0, ps0 = 0, ps1 = 0, ps2 = 0, pd0 = 0, pd1 = 0, pd2 = 0;
+    for (int ri = 0; ri < nr; ++ri) {
+      int2 r = rl[ri];
+      for (int sb0 = r.x; sb0 < r.y; sb0 += kEvalThreads) {
+        int ns = min(kEvalThreads, r.y - sb0);
+        __syncthreads();
+        if (threadIdx.x < ns) {
+          int q = sb0 + threadIdx.x;
+          ss[0 * kEvalThreads + threadIdx.x] = A.px[q];
+          ss[1 * kEvalThreads + threadIdx.x] = A.py[q];
+          ss[2 * kEvalThreads + threadIdx.x] = A.pz[q];
+          ss[3 * kEvalThreads + threadIdx.x] = A.dx[q];
+          ss[4 * kEvalThreads + threadIdx.x] = A.dy[q];
+          ss[5 * kEvalThreads + threadIdx.x] = A.dz[q];
+          ss[6 * kEvalThreads + threadIdx.x] = A.pm[q];
+          ss[7 * kEvalThreads + threadIdx.x] = A.pr[q];
+        }
+        __syncthreads();
+        if (valid) {
+          for (int s = 0; s < ns; ++s) {
+            if (sb0 + s == t) continue;
+            double r0 = x - ss[s], r1 = y - ss[kEvalThreads + s], r2 = z - ss[2 * kEvalThreads + s];
+            double sdx = ss[3 * kEvalThreads + s], sdy = ss[4 * kEvalThreads + s],
+                   sdz = ss[5 * kEvalThreads + s];
+            double smg = ss[6 * kEvalThreads + s], srv = ss[7 * kEvalThreads + s];
+            double d2 = r0 * r0 + r1 * r1 + r2 * r2;
+            // kernel_coeffs (mod_vortpart.f90:269-271)
+            double R2 = d2 + srv * srv;
+            double R = sqrt(R2);
+            double R3 = R2 * R;
+            double cf = -1.0 / R3;
+            // compute_vel: c * (dist x dir) * mag
+            pv0 += cf * (r1 * sdz - r2 * sdy) * smg;
+            pv1 += cf * (r2 * sdx - r0 * sdz) * smg;
+            pv2 += cf * (r0 * sdy - r1 * sdx) * smg;
+            if (A.use_vs) {
+              double df = 3.0 / (R3 * R2);
+              double am0 = sdx * smg, am1 = sdy * smg, am2 = sdz * smg;
+              double vp0 = al1 * am2 - al2 * am1, vp1 = al2 * am0 - al0 * am2,
+                     vp2 = al0 * am1 - al1 * am0;
+              double dv = r0 * vp0 + r1 * vp1 + r2 * vp2;
+              ps0 -= cf * vp0 + df * r0 * dv;
+              ps1 -= cf * vp1 + df * r1 * dv;
+              ps2 -= cf * vp2 + df * r2 * dv;
+            }
+            if (A.use_vd) {
+              // compute_diffusion with etaeps (mod_vortpart.f90:305-336)
+              double vol = 4.0 / 3.0 * kPi * srv * srv * srv;
+              double qq = d2 / (srv * srv) + 1.0;
+              double q2 = qq * qq;
+              double eta = 105.0 / (8.0 * kPi) / (q2 * q2 * sqrt(qq)) / (srv * srv * srv);
+              double f = 1.0 / (srv * srv) * eta * (A.nu_inf + tvp);
+              pd0 += f * (vol * sdx * smg - vol * al0);
+              pd1 += f * (vol * sdy * smg - vol * al1);
+              pd2 += f * (vol * sdz * smg - vol * al2);
+            }
+          }
+        }
+      }
+    }
+    if (valid) {
+      A.vel[3 * t] = v[0] + pv0 * fpi_inv + A.uinf[0];
+      A.vel[3 * t + 1] = v[1] + pv1 * fpi_inv + A.uinf[1];
+      A.vel[3 * t + 2] = v[2] + pv2 * fpi_inv + A.uinf[2];
+      A.str[3 * t] = st[0] + ps0 * fpi_inv + pd0;
+      A.str[3 * t + 1] = st[1] + ps1 * fpi_inv + pd1;
+      A.str[3 * t + 2] = st[2] + ps2 * fpi_inv + pd2;
+      A.tv[t] = tvp;
+    }
+  }
+}
+
+// Scatter results from sorted to input order.
+__global__ void k_scatter(int n, const int *__restrict__ perm, const double *__restrict__ vs,
+                          const double *__restrict__ ss, const double *__restrict__ ts,
+                          double *v, double *s, double *tv) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int q = perm[i];
+  v[3 * q] = vs[3 * i];
+  v[3 * q + 1] = vs[3 * i + 1];
+  v[3 * q + 2] = vs[3 * i + 2];
+  s[3 * q] = ss[3 * i];
+  s[3 * q + 1] = ss[3 * i + 1];
+  s[3 * q + 2] = ss[3 * i + 2];
+  tv[q] = ts[i];
+}
+
+// Direct O(N^2) particle terms, the fmm=F branch (mod_wake.f90:1050-1113).
+__global__ void __launch_bounds__(kEvalThreads)
+    k_direct(int n, const double *__restrict__ cen, const double *__restrict__ dir,
+             const double *__restrict__ mag, const double *__restrict__ rv, int use_vs, int use_vd,
+             double nu_inf, double u0, double u1, double u2, double *vel, double *str) {
+  __shared__ double ss[8][kEvalThreads];
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  bool valid = t < n;
+  double x = 0, y = 0, z = 0, al0 = 0, al1 = 0, al2 = 0;
+  if (valid) {
+    x = cen[3 * t]; y = cen[3 * t + 1]; z = cen[3 * t + 2];
+    al0 = dir[3 * t] * mag[t]; al1 = dir[3 * t + 1] * mag[t]; al2 = dir[3 * t + 2] * mag[t];
+  }
+  double pv0 = 0, pv1 = 0, pv2 = 0, ps0 = 0, ps1 = 0, ps2 = 0, pd0 = 0, pd1 = 0, pd2 = 0;
+  for (int b0 = 0; b0 < n; b0 += kEvalThreads) {
+    int ns = min(kEvalThreads, n - b0);
+    __syncthreads();
+    if (threadIdx.x < ns) {
+      int q = b0 + threadIdx.x;
+      ss[0][threadIdx.x] = cen[3 * q];
+      ss[1][threadIdx.x] = cen[3 * q + 1];
+      ss[2][threadIdx.x] = cen[3 * q + 2];
+      ss[3][threadIdx.x] = dir[3 * q];
+      ss[4][threadIdx.x] = dir[3 * q + 1];
+      ss[5][threadIdx.x] = dir[3 * q + 2];
+      ss[6][threadIdx.x] = mag[q];
+      ss[7][threadIdx.x] = rv[q];
+    }
+    __syncthreads();
+    if (!valid) continue;
+    for (int s = 0; s < ns; ++s) {
+      if (b0 + s == t) continue;
+      double r0 = x - ss[0][s], r1 = y - ss[1][s], r2 = z - ss[2][s];
+      double sdx = ss[3][s], sdy = ss[4][s], sdz = ss[5][s], smg = ss[6][s], srv = ss[7][s];
+      double d2 = r0 * r0 + r1 * r1 + r2 * r2;
+      double R2 = d2 + srv * srv;
+      double R3 = R2 * sqrt(R2);
+      double cf = -1.0 / R3;
+      pv0 += cf * (r1 * sdz - r2 * sdy) * smg;
+      pv1 += cf * (r2 * sdx - r0 * sdz) * smg;
+      pv2 += cf * (r0 * sdy - r1 * sdx) * smg;
+      if (use_vs) {
+        double df = 3.0 / (R3 * R2);
+        double am0 = sdx * smg, am1 = sdy * smg, am2 = sdz * smg;
+        double vp0 = al1 * am2 - al2 * am1, vp1 = al2 * am0 - al0 * am2, vp2 = al0 * am1 - al1 * am0;
+        double dv = r0 * vp0 + r1 * vp1 + r2 * vp2;
+        ps0 -= cf * vp0 + df * r0 * dv;
+        ps1 -= cf * vp1 + df * r1 * dv;
+        ps2 -= cf * vp2 + df * r2 * dv;
+      }
+      if (use_vd) {
+        double vol = 4.0 / 3.0 * kPi * srv * srv * srv;
+        double qq = d2 / (srv * srv) + 1.0;
+        double q2 = qq * qq;
+        double eta = 105.0 / (8.0 * kPi) / (q2 * q2 * sqrt(qq)) / (srv * srv * srv);
+        double f = 1.0 / (srv * srv) * eta * nu_inf;
+        pd0 += f * (vol * sdx * smg - vol * al0);
+        pd1 += f * (vol * sdy * smg - vol * al1);
+        pd2 += f * (vol * sdz * smg - vol * al2);
+      }
+    }
+  }
+  if (valid) {
+    const double fi = 1.0 / (4.0 * kPi);
+    vel[3 * t] = pv0 * fi + u0;
+    vel[3 * t + 1] = pv1 * fi + u1;
+    vel[3 * t + 2] = pv2 * fi + u2;
+    if (str) {
+      str[3 * t] = ps0 * fi + pd0;
+      str[3 * t + 1] = ps1 * fi + pd1;
+      str[3 * t + 2] = ps2 * fi + pd2;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage timing
+// ---------------------------------------------------------------------------
+void timer_begin(FmmHandle *h, int stage) {
+  if (!h->timer.on) return;
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  h->timer.pool.push_back(a);
+  h->timer.pool.push_back(b);
+  h->timer.marks.push_back({stage, (int)h->timer.pool.size() - 2});
+  (void)hipEventRecord(a, h->stream);
+}
+void timer_end(FmmHandle *h, int stage) {
+  if (!h->timer.on || h->timer.marks.empty()) return;
+  (void)stage;
+  (void)hipEventRecord(h->timer.pool[h->timer.marks.back().second + 1], h->stream);
+}
+static void timer_collect(FmmHandle *h) {
+  if (h->timer.marks.empty()) return;
+  (void)hipStreamSynchronize(h->stream);
+  for (auto &mk : h->timer.marks) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->timer.pool[mk.second], h->timer.pool[mk.second + 1]) ==
+        hipSuccess) {
+      h->timer.ms[mk.first] += ms;
+      h->timer.launches[mk.first] += 1;
+    }
+  }
+  for (auto e : h->timer.pool) (void)hipEventDestroy(e);
+  h->timer.pool.clear();
+  h->timer.marks.clear();
+}
+
+// ---------------------------------------------------------------------------
+// Host orchestration
+// ---------------------------------------------------------------------------
+static int fmm_create(const dust_fmm_params *prm, int capacity, FmmHandle **out) {
+  DUST_CHECK(prm && out, DUST_E_ARG, "null argument");
+  DUST_CHECK(prm->nlevels >= 1 && prm->nlevels <= kMaxLevels, DUST_E_ARG,
+             "nlevels must be in 1..%d", kMaxLevels);
+  DUST_CHECK(prm->degree >= 0 && prm->degree <= 8, DUST_E_ARG, "degree must be in 0..8");
+  DUST_CHECK(prm->nbox[0] >= 1 && prm->nbox[1] >= 1 && prm->nbox[2] >= 1, DUST_E_ARG,
+             "nbox must be positive");
+  DUST_CHECK(prm->box_length > 0.0, DUST_E_ARG, "box_length must be positive");
+  DUST_CHECK(capacity >= 1, DUST_E_ARG, "capacity must be positive");
+  DUST_CHECK(prm->precision == DUST_FMM_FP64 || prm->precision == DUST_FMM_TUNED, DUST_E_ARG,
+             "unknown precision %d", prm->precision);
+  long nb = (long)prm->nbox[0] * prm->nbox[1] * prm->nbox[2];
+  long nfine = nb << (3 * (prm->nlevels - 1));
+  DUST_CHECK(nfine < (1L << 31) - 1, DUST_E_ARG, "finest level has too many cells");
+  FmmHandle *h = new FmmHandle();
+  h->prm = *prm;
+  h->cap = capacity;
+  h->nlev = prm->nlevels;
+  h->deg = prm->degree;
+  h->nmon = mono_count(h->deg);
+  h->nU = mono_count(2 * h->deg);
+  for (int d = 0; d < 3; ++d) {
+    h->xmin[d] = prm->origin[d];
+    h->xmax[d] = prm->origin[d] + (double)prm->nbox[d] * prm->box_length;
+  }
+  long off = 0, kdb = 0;
+  for (int l = 0; l < h->nlev; ++l) {
+    Level &L = h->lev[l];
+    for (int d = 0; d < 3; ++d) {
+      L.ncl[d] = prm->nbox[d] << l;
+      L.kdl[d] = (l == 0) ? std::max(prm->nbox[d], 3) : 3;
+    }
+    L.ncells = (long)L.ncl[0] * L.ncl[1] * L.ncl[2];
+    L.off = off;
+    off += L.ncells;
+    L.csize = prm->box_length / (double)(1L << l);
+    L.kd_base = kdb;
+    kdb += (long)(2 * L.kdl[0] + 1) * (2 * L.kdl[1] + 1) * (2 * L.kdl[2] + 1);
+  }
+  h->ncells_tot = off;
+  h->nfine = h->lev[h->nlev - 1].ncells;
+  h->kd_tables = kdb;
+  int rc = 0;
+  auto fail = [&](int code) {
+    delete h;
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    return fail(DUST_E_HIP);
+  }
+  // polynomial tables
+  std::vector<int> pw;
+  mono_powers(h->deg, pw);
+  int nm = h->nmon;
+  std::vector<double> multT((size_t)nm * nm);
+  std::vector<int> mnT((size_t)nm * nm);
+  std::vector<double> fact(2 * h->deg + 3);
+  fact[0] = 1.0;
+  for (size_t q = 1; q < fact.size(); ++q) fact[q] = fact[q - 1] * (double)q;
+  for (int m = 0; m < nm; ++m)
+    for (int n = 0; n < nm; ++n) {
+      int a = pw[3 * m] + pw[3 * n], b = pw[3 * m + 1] + pw[3 * n + 1], c = pw[3 * m + 2] + pw[3 * n + 2];
+      // mult = nfact_der(m+n)/(nfact(m) nfact(n)), mod_multipole.f90:304-306,
+      // with exact factorials
+      multT[(size_t)n * nm + m] = (fact[a] * fact[b] * fact[c]) /
+                                  ((fact[pw[3 * m]] * fact[pw[3 * m + 1]] * fact[pw[3 * m + 2]]) *
+                                   (fact[pw[3 * n]] * fact[pw[3 * n + 1]] * fact[pw[3 * n + 2]]));
+      mnT[(size_t)n * nm + m] = mono_index(a, b, c);
+    }
+  // kernel derivative tables, initialize_octree (mod_octree.f90:486-516):
+  // offset (i,j,k) has displacement cell_size * (-i,-j,-k)
+  h->h_kd.assign((size_t)kdb * 12 * h->nU, 0.0);
+  for (int l = 0; l < h->nlev; ++l) {
+    Level &L = h->lev[l];
+    int w0 = 2 * L.kdl[0] + 1, w1 = 2 * L.kdl[1] + 1;
+    for (int k = -L.kdl[2]; k <= L.kdl[2]; ++k)
+      for (int j = -L.kdl[1]; j <= L.kdl[1]; ++j)
+        for (int i = -L.kdl[0]; i <= L.kdl[0]; ++i) {
+          if (std::abs(i) < 2 && std::abs(j) < 2 && std::abs(k) < 2) continue;
+          long o = L.kd_base + (i + L.kdl[0]) + w0 * ((j + L.kdl[1]) + (long)w1 * (k + L.kdl[2]));
+          double diff[3] = {L.csize * (double)(-i), L.csize * (double)(-j), L.csize * (double)(-k)};
+          kernel_derivatives(h->deg, diff, prm->delta, h->nU, &h->h_kd[(size_t)o * 12 * h->nU]);
+        }
+  }
+  if ((rc = h->d_pwr.reserve(pw.size())) || (rc = h->d_multT.reserve(multT.size())) ||
+      (rc = h->d_mnT.reserve(mnT.size())) || (rc = h->d_kd.reserve(h->h_kd.size())) ||
+      (rc = h->d_fact.reserve(fact.size())))
+    return fail(rc);
+  if (hipMemcpy(h->d_pwr.p, pw.data(), pw.size() * sizeof(int), hipMemcpyHostToDevice) ||
+      hipMemcpy(h->d_multT.p, multT.data(), multT.size() * sizeof(double), hipMemcpyHostToDevice) ||
+      hipMemcpy(h->d_mnT.p, mnT.data(), mnT.size() * sizeof(int), hipMemcpyHostToDevice) ||
+      hipMemcpy(h->d_kd.p, h->h_kd.data(), h->h_kd.size() * sizeof(double), hipMemcpyHostToDevice) ||
+      hipMemcpy(h->d_fact.p, fact.data(), fact.size() * sizeof(double), hipMemcpyHostToDevice)) {
+    set_error("table upload failed");
+    return fail(DUST_E_HIP);
+  }
+  // particle and tree storage sized for the capacity
+  size_t C = (size_t)capacity;
+  if ((rc = h->cen_in.reserve(3 * C)) || (rc = h->dir_in.reserve(3 * C)) ||
+      (rc = h->mag_in.reserve(C)) || (rc = h->rv_in.reserve(C)) || (rc = h->key.reserve(C)) ||
+      (rc = h->key_s.reserve(C)) || (rc = h->idx_in.reserve(C)) || (rc = h->perm.reserve(C)) ||
+      (rc = h->px.reserve(C)) || (rc = h->py.reserve(C)) || (rc = h->pz.reserve(C)) ||
+      (rc = h->dx.reserve(C)) || (rc = h->dy.reserve(C)) || (rc = h->dz.reserve(C)) ||
+      (rc = h->pm.reserve(C)) || (rc = h->pr.reserve(C)) || (rc = h->vel_s.reserve(3 * C)) ||
+      (rc = h->str_s.reserve(3 * C)) || (rc = h->tv_s.reserve(C)) || (rc = h->vel.reserve(3 * C)) ||
+      (rc = h->str.reserve(3 * C)) || (rc = h->tv.reserve(C)) ||
+      (rc = h->fstart.reserve(h->nfine + 1)) || (rc = h->flags.reserve(h->ncells_tot)) ||
+      (rc = h->slot.reserve(h->ncells_tot)) || (rc = h->active.reserve(h->ncells_tot)) ||
+      (rc = h->leaves.reserve(h->ncells_tot)) || (rc = h->counts.reserve(kMaxLevels + 4)) ||
+      (rc = h->d_err.reserve(1)))
+    return fail(rc);
+  // scratch for the device-wide primitives, sized once so that later calls
+  // never reallocate while work is in flight
+  if ((rc = prim_reserve(h))) return fail(rc);
+  {
+    size_t m2l_lds = sizeof(double) * (12 * h->nU + 3 * nm);
+    if (m2l_lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void *)k_m2l_fp64, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)m2l_lds) != hipSuccess) {
+      set_error("cannot reserve %zu bytes of LDS for M2L", m2l_lds);
+      return fail(DUST_E_HIP);
+    }
+  }
+  h->p2p_cap = 27 + 26 * (h->nlev - 1);
+  if (prm->precision == DUST_FMM_TUNED && (rc = tuned_prepare(h))) return fail(rc);
+  *out = h;
+  return 0;
+}
+
+static void fmm_destroy(FmmHandle *h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  tuned_release(h);
+  DevBuf<double> *dbl[] = {&h->d_multT, &h->d_kd, &h->d_fact, &h->cen_in, &h->dir_in, &h->mag_in,
+                           &h->rv_in, &h->px, &h->py, &h->pz, &h->dx, &h->dy, &h->dz, &h->pm,
+                           &h->pr, &h->mp_a, &h->mp_b, &h->mp_c, &h->vel_s, &h->str_s, &h->tv_s,
+                           &h->vel, &h->str, &h->tv};
+  for (auto b : dbl) b->release();
+  DevBuf<int> *ints[] = {&h->d_pwr, &h->d_mnT, &h->idx_in, &h->perm, &h->fstart, &h->slot,
+                         &h->active, &h->leaves, &h->counts, &h->p2p_cnt, &h->d_err};
+  for (auto b : ints) b->release();
+  h->key.release();
+  h->key_s.release();
+  h->tmp.release();
+  h->flags.release();
+  h->p2p_rng.release();
+  for (auto e : h->timer.pool) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+static int fmm_set_particles(FmmHandle *h, int n, const double *cen, const double *dir,
+                             const double *mag, const double *rv, hipMemcpyKind kind) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  DUST_CHECK(n >= 0, DUST_E_ARG, "negative particle count");
+  DUST_CHECK(n <= h->cap, DUST_E_CAPACITY, "%d particles exceed the handle capacity %d", n, h->cap);
+  DUST_CHECK(n == 0 || (cen && dir && mag && rv), DUST_E_ARG, "null particle array");
+  h->n = n;
+  h->sorted = h->computed = false;
+  if (n == 0) return 0;
+  DUST_HIP(hipMemcpyAsync(h->cen_in.p, cen, sizeof(double) * 3 * n, kind, h->stream));
+  DUST_HIP(hipMemcpyAsync(h->dir_in.p, dir, sizeof(double) * 3 * n, kind, h->stream));
+  DUST_HIP(hipMemcpyAsync(h->mag_in.p, mag, sizeof(double) * n, kind, h->stream));
+  DUST_HIP(hipMemcpyAsync(h->rv_in.p, rv, sizeof(double) * n, kind, h->stream));
+  if (kind == hipMemcpyHostToDevice) DUST_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// sort_particles (mod_octree.f90:703-996)
+static int fmm_sort(FmmHandle *h, int *nleaves_out) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  const int n = h->n;
+  TreeGeom G = make_geom(h);
+  const int TB = 256;
+  timer_begin(h, DUST_FMM_STAGE_SORT);
+  const int init_err = 0x7fffffff;
+  DUST_HIP(hipMemcpyAsync(h->d_err.p, &init_err, sizeof(int), hipMemcpyHostToDevice, h->stream));
+  if (n > 0) {
+    k_keys<<<ceil_div(n, TB), TB, 0, h->stream>>>(G, n, h->cen_in.p, h->key.p, h->idx_in.p,
+                                                  h->d_err.p);
+    int bits = 0;
+    while ((1L << bits) < h->nfine) ++bits;
+    if (bits == 0) bits = 1;
+    DUST_TRY(prim_sort_pairs(h, n, bits));
+    k_gather<<<ceil_div(n, TB), TB, 0, h->stream>>>(n, h->perm.p, h->cen_in.p, h->dir_in.p,
+                                                    h->mag_in.p, h->rv_in.p, h->px.p, h->py.p,
+                                                    h->pz.p, h->dx.p, h->dy.p, h->dz.p, h->pm.p,
+                                                    h->pr.p);
+  }
+  k_fstart<<<ceil_div(h->nfine + 1, TB), TB, 0, h->stream>>>(h->nfine, n, h->key_s.p,
+                                                             h->fstart.p);
+  k_flags_fine<<<ceil_div(h->nfine, TB), TB, 0, h->stream>>>(G, h->fstart.p, h->prm.min_part,
+                                                             h->flags.p);
+  for (int l = h->nlev - 2; l >= 0; --l)
+    k_flags_level<<<ceil_div(h->lev[l].ncells, TB), TB, 0, h->stream>>>(G, l, h->fstart.p,
+                                                                        h->prm.min_part, h->flags.p);
+  DUST_TRY(prim_select_flagged_cells(h, 0, h->active.p, h->counts.p));
+  DUST_TRY(prim_select_flagged_cells(h, 1, h->leaves.p, h->counts.p + 1));
+  int hc[kMaxLevels + 4];
+  int herr = 0;
+  DUST_HIP(hipMemcpyAsync(hc, h->counts.p, sizeof(int) * 2, hipMemcpyDeviceToHost, h->stream));
+  DUST_HIP(hipMemcpyAsync(&herr, h->d_err.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  DUST_HIP(hipStreamSynchronize(h->stream));
+  if (herr != init_err) {
+    set_error("particle %d resulted outside the octree box", herr);
+    timer_end(h, DUST_FMM_STAGE_SORT);
+    return DUST_E_OUTSIDE;
+  }
+  h->n_active = hc[0];
+  h->n_leaves = hc[1];
+  k_level_starts<<<1, 64, 0, h->stream>>>(G, h->n_active, h->active.p, h->counts.p + 2);
+  DUST_HIP(hipMemcpyAsync(h->act_start, h->counts.p + 2, sizeof(int) * (h->nlev + 1),
+                          hipMemcpyDeviceToHost, h->stream));
+  k_fill_int<<<ceil_div(h->ncells_tot, TB), TB, 0, h->stream>>>(h->ncells_tot, h->slot.p, -1);
+  if (h->n_active > 0)
+    k_slots<<<ceil_div(h->n_active, TB), TB, 0, h->stream>>>(h->n_active, h->active.p, h->slot.p);
+  // P2P lists
+  DUST_TRY(h->p2p_rng.reserve((size_t)std::max(h->n_leaves, 1) * h->p2p_cap));
+  DUST_TRY(h->p2p_cnt.reserve(std::max(h->n_leaves, 1)));
+  if (h->n_leaves > 0)
+    k_p2p_lists<<<ceil_div(h->n_leaves, TB), TB, 0, h->stream>>>(
+        G, h->n_leaves, h->leaves.p, h->flags.p, h->fstart.p, h->p2p_cap, h->p2p_rng.p,
+        h->p2p_cnt.p);
+  // expansion storage
+  size_t na = (size_t)std::max(h->n_active, 1);
+  DUST_TRY(h->mp_a.reserve(na * 3 * h->nmon));
+  DUST_TRY(h->mp_b.reserve(na * 3 * h->nmon));
+  DUST_TRY(h->mp_c.reserve(na * 9 * h->nmon));
+  DUST_HIP(hipStreamSynchronize(h->stream));
+  timer_end(h, DUST_FMM_STAGE_SORT);
+  h->sorted = true;
+  h->computed = false;
+  if (nleaves_out) *nleaves_out = h->n_leaves;
+  return 0;
+}
+
+// calculate_multipole (mod_octree.f90:1002-1133)
+static int fmm_multipole(FmmHandle *h) {
+  DUST_CHECK(h && h->sorted, DUST_E_ARG, "sort_particles must precede calculate_multipole");
+  TreeGeom G = make_geom(h);
+  const int nm = h->nmon;
+  const int tpb = ((nm + 63) / 64) * 64;
+  const int use_vs = h->prm.use_vs;
+  if (h->n_active == 0) { h->computed = true; return 0; }
+  // local expansions start from zero (set_leaf/set_branch reset, 1596-1603)
+  DUST_HIP(hipMemsetAsync(h->mp_b.p, 0, sizeof(double) * 3 * nm * h->n_active, h->stream));
+  if (use_vs)
+    DUST_HIP(hipMemsetAsync(h->mp_c.p, 0, sizeof(double) * 9 * nm * h->n_active, h->stream));
+  timer_begin(h, DUST_FMM_STAGE_P2M);
+  if (h->n_leaves > 0) {
+    size_t sh = sizeof(double) * (kP2MChunk * 3 * (h->deg + 1) + kP2MChunk * 4);
+    k_p2m<<<h->n_leaves, tpb, sh, h->stream>>>(G, h->deg, nm, h->d_pwr.p, h->leaves.p, h->slot.p,
+                                               h->fstart.p, h->px.p, h->py.p, h->pz.p, h->dx.p,
+                                               h->dy.p, h->dz.p, h->pm.p, h->mp_a.p);
+  }
+  timer_end(h, DUST_FMM_STAGE_P2M);
+  timer_begin(h, DUST_FMM_STAGE_M2M);
+  for (int l = h->nlev - 2; l >= 0; --l) {
+    int na = h->act_start[l + 1] - h->act_start[l];
+    if (na > 0)
+      k_m2m<<<na, tpb, 0, h->stream>>>(G, l, h->deg, nm, h->d_pwr.p, h->d_fact.p, h->active.p,
+                                       h->act_start[l], h->flags.p, h->slot.p, h->mp_a.p);
+  }
+  timer_end(h, DUST_FMM_STAGE_M2M);
+  for (int l = 0; l < h->nlev; ++l) {
+    int na = h->act_start[l + 1] - h->act_start[l];
+    timer_begin(h, DUST_FMM_STAGE_M2L);
+    if (na > 0) {
+      if (h->prm.precision == DUST_FMM_TUNED) {
+        DUST_TRY(tuned_m2l_level(h, l));
+      } else {
+        size_t sh = sizeof(double) * (12 * h->nU + 3 * nm);
+        k_m2l_fp64<<<na, tpb, sh, h->stream>>>(G, l, nm, h->nU, use_vs, h->d_pwr.p, h->d_multT.p,
+                                               h->d_mnT.p, h->d_kd.p, h->active.p, h->act_start[l],
+                                               h->flags.p, h->slot.p, h->mp_a.p, h->mp_b.p,
+                                               h->mp_c.p);
+      }
+    }
+    timer_end(h, DUST_FMM_STAGE_M2L);
+    if (l + 1 < h->nlev) {
+      int nc = h->act_start[l + 2] - h->act_start[l + 1];
+      timer_begin(h, DUST_FMM_STAGE_L2L);
+      if (nc > 0)
+        k_l2l<<<nc, tpb, 0, h->stream>>>(G, l, h->deg, nm, use_vs, h->d_pwr.p, h->d_fact.p,
+                                         h->active.p, h->act_start[l + 1], h->slot.p, h->mp_b.p,
+                                         h->mp_c.p);
+      timer_end(h, DUST_FMM_STAGE_L2L);
+    }
+  }
+  h->computed = true;
+  return 0;
+}
+
+// apply_multipole (mod_octree.f90:1139-1429), particle terms + free stream
+static int fmm_apply(FmmHandle *h) {
+  DUST_CHECK(h && h->computed, DUST_E_ARG, "calculate_multipole must precede apply_multipole");
+  const int n = h->n;
+  if (n == 0) return 0;
+  timer_begin(h, DUST_FMM_STAGE_P2P);
+  if (h->prm.precision == DUST_FMM_TUNED) {
+    DUST_TRY(tuned_eval(h));
+  } else {
+    EvalArgs A;
+    A.G = make_geom(h);
+    A.deg = h->deg; A.nmon = h->nmon;
+    A.use_vs = h->prm.use_vs; A.use_vd = h->prm.use_vd; A.use_tv = h->prm.use_tv;
+    A.nu_inf = h->prm.nu_inf;
+    for (int d = 0; d < 3; ++d) A.uinf[d] = h->prm.u_inf[d];
+    A.pwr = h->d_pwr.p; A.leaves = h->leaves.p; A.slot = h->slot.p; A.fstart = h->fstart.p;
+    A.mp_b = h->mp_b.p; A.mp_c = h->mp_c.p;
+    A.px = h->px.p; A.py = h->py.p; A.pz = h->pz.p; A.dx = h->dx.p; A.dy = h->dy.p; A.dz = h->dz.p;
+    A.pm = h->pm.p; A.pr = h->pr.p; A.rng = h->p2p_rng.p; A.cnt = h->p2p_cnt.p; A.cap = h->p2p_cap;
+    A.vel = h->vel_s.p; A.str = h->str_s.p; A.tv = h->tv_s.p;
+    size_t sh = sizeof(double) * (12 * h->nmon + 8 * kEvalThreads) + sizeof(int) * 3 * h->nmon;
+    if (h->n_leaves > 0) k_eval_fp64<<<h->n_leaves, kEvalThreads, sh, h->stream>>>(A);
+  }
+  timer_end(h, DUST_FMM_STAGE_P2P);
+  k_scatter<<<ceil_div(n, 256), 256, 0, h->stream>>>(n, h->perm.p, h->vel_s.p, h->str_s.p,
+                                                     h->tv_s.p, h->vel.p, h->str.p, h->tv.p);
+  return 0;
+}
+
+}  // namespace dust
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace dust;
+
+struct dust_fmm_s {
+  FmmHandle *h;
+};
+
+extern "C" {
+
+const char *dust_last_error(void) { return g_err; }
+
+const char *dust_version(void) { return "libdust_gpu 0.1 (gfx950)"; }
+
+int dust_fmm_create(const dust_fmm_params *prm, int capacity, dust_fmm_t *out) {
+  DUST_CHECK(out, DUST_E_ARG, "null output handle");
+  FmmHandle *h = nullptr;
+  int rc = fmm_create(prm, capacity, &h);
+  if (rc) return rc;
+  *out = new dust_fmm_s{h};
+  return 0;
+}
+
+int dust_fmm_destroy(dust_fmm_t h) {
+  if (!h) return 0;
+  fmm_destroy(h->h);
+  delete h;
+  return 0;
+}
+
+int dust_fmm_set_particles(dust_fmm_t h, int n, const double *cen, const double *dir,
+                           const double *mag, const double *rv) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  return fmm_set_particles(h->h, n, cen, dir, mag, rv, hipMemcpyHostToDevice);
+}
+
+int dust_fmm_set_particles_dev(dust_fmm_t h, int n, const double *cen, const double *dir,
+                               const double *mag, const double *rv) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  return fmm_set_particles(h->h, n, cen, dir, mag, rv, hipMemcpyDeviceToDevice);
+}
+
+int dust_fmm_sort_particles(dust_fmm_t h, int *nleaves) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  return fmm_sort(h->h, nleaves);
+}
+
+int dust_fmm_calculate_multipole(dust_fmm_t h) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  DUST_TRY(fmm_multipole(h->h));
+  return 0;
+}
+
+int dust_fmm_apply_multipole_dev(dust_fmm_t h) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  return fmm_apply(h->h);
+}
+
+int dust_fmm_apply_multipole(dust_fmm_t hh, double *vel, double *stretch, double *turbvisc) {
+  DUST_CHECK(hh, DUST_E_ARG, "null handle");
+  FmmHandle *h = hh->h;
+  DUST_TRY(fmm_apply(h));
+  int n = h->n;
+  if (n > 0) {
+    if (vel) DUST_HIP(hipMemcpyAsync(vel, h->vel.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, h->stream));
+    if (stretch)
+      DUST_HIP(hipMemcpyAsync(stretch, h->str.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, h->stream));
+    if (turbvisc)
+      DUST_HIP(hipMemcpyAsync(turbvisc, h->tv.p, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+  }
+  DUST_HIP(hipStreamSynchronize(h->stream));
+  timer_collect(h);
+  return 0;
+}
+
+int dust_fmm_results_dev(dust_fmm_t hh, double *vel, double *stretch, double *turbvisc) {
+  DUST_CHECK(hh, DUST_E_ARG, "null handle");
+  FmmHandle *h = hh->h;
+  int n = h->n;
+  if (n == 0) return 0;
+  if (vel) DUST_HIP(hipMemcpyAsync(vel, h->vel.p, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, h->stream));
+  if (stretch)
+    DUST_HIP(hipMemcpyAsync(stretch, h->str.p, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, h->stream));
+  if (turbvisc)
+    DUST_HIP(hipMemcpyAsync(turbvisc, h->tv.p, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+  return 0;
+}
+
+int dust_fmm_pass_dev(dust_fmm_t hh, int *nleaves) {
+  DUST_CHECK(hh, DUST_E_ARG, "null handle");
+  DUST_TRY(fmm_sort(hh->h, nleaves));
+  DUST_TRY(fmm_multipole(hh->h));
+  DUST_TRY(fmm_apply(hh->h));
+  return 0;
+}
+
+void *dust_fmm_stream(dust_fmm_t h) { return h ? (void *)h->h->stream : nullptr; }
+
+int dust_fmm_synchronize(dust_fmm_t h) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  DUST_HIP(hipStreamSynchronize(h->h->stream));
+  timer_collect(h->h);
+  return 0;
+}
+
+int dust_fmm_set_timing(dust_fmm_t h, int enable) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  timer_collect(h->h);
+  h->h->timer.on = enable != 0;
+  for (int s = 0; s < DUST_FMM_NSTAGES; ++s) {
+    h->h->timer.ms[s] = 0.0;
+    h->h->timer.launches[s] = 0;
+  }
+  return 0;
+}
+
+int dust_fmm_stage_times(dust_fmm_t h, double *ms, int *launches) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  timer_collect(h->h);
+  for (int s = 0; s < DUST_FMM_NSTAGES; ++s) {
+    if (ms) ms[s] = h->h->timer.ms[s];
+    if (launches) launches[s] = h->h->timer.launches[s];
+  }
+  return 0;
+}
+
+int dust_fmm_leaves(dust_fmm_t hh, int *level, int *ijk, int *npart, int *members) {
+  DUST_CHECK(hh, DUST_E_ARG, "null handle");
+  FmmHandle *h = hh->h;
+  DUST_CHECK(h->sorted, DUST_E_ARG, "sort_particles must be called first");
+  TreeGeom G = make_geom(h);
+  std::vector<int> lv(h->n_leaves), fs(h->nfine + 1), perm(h->n);
+  DUST_HIP(hipMemcpy(lv.data(), h->leaves.p, sizeof(int) * h->n_leaves, hipMemcpyDeviceToHost));
+  DUST_HIP(hipMemcpy(fs.data(), h->fstart.p, sizeof(int) * (h->nfine + 1), hipMemcpyDeviceToHost));
+  if (h->n > 0)
+    DUST_HIP(hipMemcpy(perm.data(), h->perm.p, sizeof(int) * h->n, hipMemcpyDeviceToHost));
+  int w = 0;
+  for (int q = 0; q < h->n_leaves; ++q) {
+    long g = lv[q];
+    int l = 0;
+    while (l + 1 < h->nlev && g >= h->lev[l + 1].off) ++l;
+    long c = g - h->lev[l].off;
+    int ix, iy, iz;
+    cell_decode(G, l, c, ix, iy, iz);
+    int sh = 3 * (h->nlev - 1 - l);
+    int s = fs[c << sh], e = fs[(c + 1) << sh];
+    if (level) level[q] = l + 1;
+    if (ijk) { ijk[3 * q] = ix + 1; ijk[3 * q + 1] = iy + 1; ijk[3 * q + 2] = iz + 1; }
+    if (npart) npart[q] = e - s;
+    if (members)
+      for (int t = s; t < e; ++t) members[w++] = perm[t];
+  }
+  return 0;
+}
+
+int dust_fmm_cell_index(dust_fmm_t hh, int *ijk) {
+  DUST_CHECK(hh && ijk, DUST_E_ARG, "null argument");
+  FmmHandle *h = hh->h;
+  DUST_CHECK(h->sorted, DUST_E_ARG, "sort_particles must be called first");
+  TreeGeom G = make_geom(h);
+  std::vector<unsigned> ks(h->n);
+  std::vector<int> perm(h->n);
+  if (h->n == 0) return 0;
+  DUST_HIP(hipMemcpy(ks.data(), h->key_s.p, sizeof(unsigned) * h->n, hipMemcpyDeviceToHost));
+  DUST_HIP(hipMemcpy(perm.data(), h->perm.p, sizeof(int) * h->n, hipMemcpyDeviceToHost));
+  for (int t = 0; t < h->n; ++t) {
+    int ix, iy, iz;
+    cell_decode(G, h->nlev - 1, (long)ks[t], ix, iy, iz);
+    int q = perm[t];
+    ijk[3 * q] = ix + 1; ijk[3 * q + 1] = iy + 1; ijk[3 * q + 2] = iz + 1;
+  }
+  return 0;
+}
+
+int dust_fmm_work(dust_fmm_t hh, double *p2p_pairs, double *m2l_pairs) {
+  DUST_CHECK(hh, DUST_E_ARG, "null handle");
+  FmmHandle *h = hh->h;
+  DUST_CHECK(h->sorted, DUST_E_ARG, "sort_particles must be called first");
+  TreeGeom G = make_geom(h);
+  std::vector<int> cnt(h->n_leaves), lv(h->n_leaves), fs(h->nfine + 1);
+  std::vector<int2> rng((size_t)h->n_leaves * h->p2p_cap);
+  std::vector<unsigned char> fl(h->ncells_tot);
+  if (h->n_leaves > 0) {
+    DUST_HIP(hipMemcpy(cnt.data(), h->p2p_cnt.p, sizeof(int) * h->n_leaves, hipMemcpyDeviceToHost));
+    DUST_HIP(hipMemcpy(lv.data(), h->leaves.p, sizeof(int) * h->n_leaves, hipMemcpyDeviceToHost));
+    DUST_HIP(hipMemcpy(rng.data(), h->p2p_rng.p, sizeof(int2) * rng.size(), hipMemcpyDeviceToHost));
+  }
+  DUST_HIP(hipMemcpy(fs.data(), h->fstart.p, sizeof(int) * (h->nfine + 1), hipMemcpyDeviceToHost));
+  DUST_HIP(hipMemcpy(fl.data(), h->flags.p, h->ncells_tot, hipMemcpyDeviceToHost));
+  double pp = 0.0;
+  for (int q = 0; q < h->n_leaves; ++q) {
+    const int2 *r = &rng[(size_t)q * h->p2p_cap];
+    double src = 0.0;
+    for (int t = 0; t + 1 < cnt[q]; ++t) src += r[t].y - r[t].x;
+    double np = r[cnt[q] - 1].y - r[cnt[q] - 1].x;
+    pp += np * (src + np - 1.0);
+  }
+  double mp = 0.0;
+  for (int l = 0; l < h->nlev; ++l) {
+    const Level &L = h->lev[l];
+    for (long c = 0; c < L.ncells; ++c) {
+      if (!fl[L.off + c]) continue;
+      int ix, iy, iz;
+      cell_decode(G, l, c, ix, iy, iz);
+      if (l == 0) {
+        for (long s = 0; s < L.ncells; ++s) {
+          int sx, sy, sz;
+          cell_decode(G, 0, s, sx, sy, sz);
+          if ((std::abs(sx - ix) > 1 || std::abs(sy - iy) > 1 || std::abs(sz - iz) > 1) && fl[L.off + s])
+            mp += 1.0;
+        }
+        continue;
+      }
+      int px = ix >> 1, py = iy >> 1, pz = iz >> 1;
+      for (int kc = -1; kc <= 1; ++kc)
+        for (int jc = -1; jc <= 1; ++jc)
+          for (int ic = -1; ic <= 1; ++ic) {
+            if (!ic && !jc && !kc) continue;
+            int a = px + ic, b = py + jc, d = pz + kc;
+            const Level &P = h->lev[l - 1];
+            if (a < 0 || b < 0 || d < 0 || a >= P.ncl[0] || b >= P.ncl[1] || d >= P.ncl[2]) continue;
+            for (int p = 0; p < 8; ++p) {
+              int sx = 2 * a + (p & 1), sy = 2 * b + ((p >> 1) & 1), sz = 2 * d + ((p >> 2) & 1);
+              if (std::abs(sx - ix) <= 1 && std::abs(sy - iy) <= 1 && std::abs(sz - iz) <= 1) continue;
+              if (fl[L.off + cell_encode(G, l, sx, sy, sz)]) mp += 1.0;
+            }
+          }
+    }
+  }
+  if (p2p_pairs) *p2p_pairs = pp;
+  if (m2l_pairs) *m2l_pairs = mp;
+  return 0;
+}
+
+int dust_vortpart_direct(int n, const double *cen, const double *dir, const double *mag,
+                         const double *rv, int use_vs, int use_vd, double nu_inf,
+                         const double u_inf[3], double *vel, double *stretch) {
+  DUST_CHECK(n >= 0, DUST_E_ARG, "negative particle count");
+  if (n == 0) return 0;
+  DUST_CHECK(cen && dir && mag && rv && vel, DUST_E_ARG, "null argument");
+  double *d = nullptr;
+  size_t tot = (size_t)n * 14;
+  DUST_HIP(hipMalloc(&d, sizeof(double) * tot));
+  double *dc = d, *dd = d + 3 * n, *dm = d + 6 * n, *dr = d + 7 * n, *dv = d + 8 * n,
+         *ds = d + 11 * n;
+  int rc = 0;
+  if (hipMemcpy(dc, cen, sizeof(double) * 3 * n, hipMemcpyHostToDevice) ||
+      hipMemcpy(dd, dir, sizeof(double) * 3 * n, hipMemcpyHostToDevice) ||
+      hipMemcpy(dm, mag, sizeof(double) * n, hipMemcpyHostToDevice) ||
+      hipMemcpy(dr, rv, sizeof(double) * n, hipMemcpyHostToDevice)) {
+    set_error("upload failed");
+    rc = DUST_E_HIP;
+  }
+  if (!rc) {
+    double u0 = u_inf ? u_inf[0] : 0.0, u1 = u_inf ? u_inf[1] : 0.0, u2 = u_inf ? u_inf[2] : 0.0;
+    k_direct<<<ceil_div(n, kEvalThreads), kEvalThreads>>>(n, dc, dd, dm, dr, use_vs, use_vd,
+                                                          nu_inf, u0, u1, u2, dv, ds);
+    if (hipMemcpy(vel, dv, sizeof(double) * 3 * n, hipMemcpyDeviceToHost) ||
+        (stretch && hipMemcpy(stretch, ds, sizeof(double) * 3 * n, hipMemcpyDeviceToHost))) {
+      set_error("direct evaluation failed: %s", hipGetErrorString(hipGetLastError()));
+      rc = DUST_E_HIP;
+    }
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+}  // extern "C"

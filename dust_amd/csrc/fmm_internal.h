@@ -1,0 +1,107 @@
+// Internal state of an FMM handle (dust_fmm_t) and the launch interfaces
+// shared between the FMM translation units.
+#pragma once
+#include <vector>
+
+#include "common.h"
+
+namespace dust {
+
+constexpr int kMaxLevels = 10;
+
+// Cell flags, the state sort_particles leaves in t_cell%leaf/%branch
+// (src/fmm/mod_octree.f90:93-100); active == leaf or branch.
+constexpr unsigned char kLeaf = 1;
+constexpr unsigned char kBranch = 2;
+
+// Geometry of one octree level.  Cells of a level are numbered
+//   c = box * 8^lvl + morton(ix mod 2^lvl, iy mod 2^lvl, iz mod 2^lvl)
+// with box the level-1 box (x fastest) and morton bits interleaved x,y,z from
+// the least significant end.  Children of c are 8c+p with p = ic + 2 jc + 4 kc,
+// the child order of mod_octree.f90:349-361, and every cell is a contiguous
+// range of the finest-level keys, hence of the sorted particles.
+struct Level {
+  int ncl[3];        // cells per direction
+  long ncells;       // cells in the level
+  long off;          // first global cell id of the level
+  double csize;      // cell edge
+  int kdl[3];        // half-extent of the kernel-derivative offset table
+  long kd_base;      // first table of the level in the derivative array
+};
+
+// Per-stage HIP event pairs for live timing.
+struct StageTimer {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, int>> marks;  // (stage, event index pairs)
+  double ms[DUST_FMM_NSTAGES] = {0};
+  int launches[DUST_FMM_NSTAGES] = {0};
+};
+
+struct FmmHandle {
+  dust_fmm_params prm{};
+  int cap = 0, n = 0;
+  int nlev = 0;
+  Level lev[kMaxLevels];
+  long ncells_tot = 0;
+  long nfine = 0;
+  double xmin[3], xmax[3];
+  // polynomial tables
+  int deg = 0, nmon = 0;
+  int nU = 0;  // derivative entries used by M2L: monomials of order <= 2p
+  DevBuf<int> d_pwr;          // (3, nmon)
+  DevBuf<double> d_multT;     // (nmon, nmon): multT[n*nmon+m]
+  DevBuf<int> d_mnT;          // (nmon, nmon): derivative index of pwr(m)+pwr(n)
+  DevBuf<double> d_kd;        // 12*nU per (level, offset): D(3,:) then Dc(3,3,:), SoA
+  DevBuf<double> d_fact;      // factorials 0..2p+2
+  std::vector<double> h_kd;   // host copy (used by the tuned M2L operator build)
+  long kd_tables = 0;
+  // particles, input order
+  DevBuf<double> cen_in, dir_in, mag_in, rv_in;
+  // sort
+  DevBuf<unsigned> key, key_s;
+  DevBuf<int> idx_in, perm;
+  DevBuf<unsigned char> tmp;
+  // particles, sorted (SoA)
+  DevBuf<double> px, py, pz, dx, dy, dz, pm, pr;
+  // tree
+  DevBuf<int> fstart;        // nfine + 1
+  DevBuf<unsigned char> flags;
+  DevBuf<int> slot;
+  DevBuf<int> active;        // global ids of active cells, ascending
+  DevBuf<int> leaves;        // global ids of leaves, ascending
+  DevBuf<int> counts;        // device scratch for list sizes
+  int n_active = 0, n_leaves = 0;
+  int act_start[kMaxLevels + 1];
+  // P2P lists
+  int p2p_cap = 0;
+  DevBuf<int2> p2p_rng;
+  DevBuf<int> p2p_cnt;
+  // expansions
+  DevBuf<double> mp_a, mp_b, mp_c;
+  // outputs
+  DevBuf<double> vel_s, str_s, tv_s, vel, str, tv;
+  DevBuf<int> d_err;
+  // tuned form state
+  void *tuned = nullptr;
+  hipStream_t stream = nullptr;
+  StageTimer timer;
+  bool sorted = false, computed = false;
+};
+
+// fmm_prim.hip: device-wide primitives (radix sort, selection).
+int prim_reserve(FmmHandle *h);
+int prim_sort_pairs(FmmHandle *h, int n, int key_bits);
+int prim_select_flagged_cells(FmmHandle *h, int want_leaf_only, int *out, int *d_count);
+
+// fmm_tuned.hip
+int tuned_prepare(FmmHandle *h);
+int tuned_m2l_level(FmmHandle *h, int lvl);
+int tuned_eval(FmmHandle *h);
+void tuned_release(FmmHandle *h);
+
+// timing helpers
+void timer_begin(FmmHandle *h, int stage);
+void timer_end(FmmHandle *h, int stage);
+
+}  // namespace dust

@@ -1,0 +1,112 @@
+// solve_linsys on the GPU: the block static/moving elimination of
+// src/linsys/mod_linsys.f90:352-483 (dlaswp 373, dtrsm 385 and 399, dgemm 416,
+// dgetrf 435, dlaswp 456, pivot offset 464-465, dgetrs 473) expressed with the
+// device LU building blocks.
+#include <algorithm>
+#include <vector>
+
+#include "lu_internal.h"
+
+namespace dust {
+
+// X U = B for the rows of B (m x n), U upper non-unit (n x n): the
+// dtrsm('R','U','N','N') of mod_linsys.f90:399.  One thread per row for a
+// 64-column diagonal block, GEMM for the rest.
+__global__ void k_trsm_runn_block(const double *__restrict__ U, long ldu, double *B, long ldb,
+                                  int m, int k, int jb) {
+  __shared__ double sU[64][65];
+  for (int q = threadIdx.x; q < jb * jb; q += blockDim.x) {
+    int r = q % jb, c = q / jb;
+    sU[r][c] = U[(k + r) + ldu * (k + c)];
+  }
+  __syncthreads();
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double x[64];
+  for (int c = 0; c < jb; ++c) x[c] = B[i + ldb * (k + c)];
+  for (int c = 0; c < jb; ++c) {
+    double s = x[c];
+    for (int t = 0; t < c; ++t) s -= x[t] * sU[t][c];
+    x[c] = s / sU[c][c];
+  }
+  for (int c = 0; c < jb; ++c) B[i + ldb * (k + c)] = x[c];
+}
+
+__global__ void k_add_offset(int *p, int n, int off) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] += off;
+}
+
+static int solve_linsys_dev(int rank, int ns, double *dA, long lda, int *dp, double *dres,
+                            hipStream_t st, int *info) {
+  const int nm = rank - ns;
+  *info = 0;
+  if (ns > 0 && nm > 0) {
+    // Pss Asd (373): interchanges of the static pivots on the upper-right block
+    DUST_TRY(lu_laswp_dev(dA + lda * ns, lda, nm, 0, ns, dp, st));
+    // Usd = Lss^-1 Pss Asd (385)
+    DUST_TRY(lu_trsm_llnu_dev(ns, nm, dA, lda, dA + lda * ns, lda, st));
+    // Lds = Ads Uss^-1 (399)
+    for (int k = 0; k < ns; k += 64) {
+      int jb = std::min(64, ns - k);
+      if (k > 0)
+        DUST_TRY(lu_gemm_sub_dev(nm, jb, k, dA + ns, lda, dA + lda * k, lda, dA + ns + lda * k, lda,
+                                 st));
+      k_trsm_runn_block<<<ceil_div(nm, 64), 64, 0, st>>>(dA, lda, dA + ns, lda, nm, k, jb);
+    }
+    // Add -= Lds Usd (416)
+    DUST_TRY(lu_gemm_sub_dev(nm, nm, ns, dA + ns, lda, dA + lda * ns, lda, dA + ns + lda * ns, lda,
+                             st));
+  }
+  if (nm > 0) {
+    // factor the moving block (435)
+    DUST_TRY(lu_getrf_dev(nm, nm, dA + ns + lda * ns, lda, dp + ns, st, 64, info));
+    if (*info > 0) return 0;
+    // interchanges of the moving pivots on the lower-left block (456)
+    if (ns > 0) DUST_TRY(lu_laswp_dev(dA + ns, lda, ns, 0, nm, dp + ns, st));
+    // global pivot numbering (464-465)
+    k_add_offset<<<ceil_div(nm, 256), 256, 0, st>>>(dp + ns, nm, ns);
+  }
+  // dgetrs with one right-hand side (473)
+  DUST_TRY(lu_getrs_dev(rank, 1, dA, lda, dp, dres, rank, st));
+  DUST_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+}  // namespace dust
+
+using namespace dust;
+
+extern "C" {
+
+int dust_solve_linsys(int rank, int nstatic, double *A, int lda, int *ipiv, const double *b,
+                      double *res) {
+  DUST_CHECK(rank >= 1 && nstatic >= 0 && nstatic <= rank && lda >= rank && A && ipiv && b && res,
+             DUST_E_ARG, "invalid solve_linsys arguments");
+  double *dA = nullptr, *dr = nullptr;
+  int *dp = nullptr;
+  DUST_HIP(hipMalloc(&dA, sizeof(double) * (size_t)rank * rank));
+  DUST_HIP(hipMalloc(&dr, sizeof(double) * rank));
+  DUST_HIP(hipMalloc(&dp, sizeof(int) * rank));
+  DUST_HIP(hipMemcpy2D(dA, sizeof(double) * rank, A, sizeof(double) * lda, sizeof(double) * rank,
+                       rank, hipMemcpyHostToDevice));
+  DUST_HIP(hipMemcpy(dp, ipiv, sizeof(int) * rank, hipMemcpyHostToDevice));
+  DUST_HIP(hipMemcpy(dr, b, sizeof(double) * rank, hipMemcpyHostToDevice));
+  int info = 0;
+  int rc = solve_linsys_dev(rank, nstatic, dA, rank, dp, dr, nullptr, &info);
+  if (!rc) {
+    if (hipMemcpy2D(A, sizeof(double) * lda, dA, sizeof(double) * rank, sizeof(double) * rank, rank,
+                    hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(ipiv, dp, sizeof(int) * rank, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(res, dr, sizeof(double) * rank, hipMemcpyDeviceToHost) != hipSuccess) {
+      set_error("download failed");
+      rc = DUST_E_HIP;
+    }
+  }
+  (void)hipFree(dA);
+  (void)hipFree(dr);
+  (void)hipFree(dp);
+  return rc ? rc : info;
+}
+
+}  // extern "C"

@@ -1,0 +1,606 @@
+// Dense LU with partial pivoting (plain FP64) and the triangular solves, with
+// LAPACK xGETRF/xGETRS semantics, plus the block static/moving update of
+// solve_linsys.
+//
+// Reference call sites: src/linsys/mod_linsys.f90:236 (dgetrf, static block),
+// 373-465 (dlaswp/dtrsm/dtrsm/dgemm/dgetrf/dlaswp, mixed static/moving),
+// 473 (dgetrs, one right-hand side); the pressure system repeats them
+// (src/linsys/mod_pressure_equation.f90:251-390).
+//
+// Structure: right-looking blocked LU.  Each panel is factored by one
+// cooperative launch whose blocks keep their rows of the panel in LDS and
+// meet at a grid barrier once per column (pivot search across blocks);
+// the row interchanges are then applied to the rest of the matrix, U12 is
+// obtained by a unit-lower triangular solve in LDS, and the trailing matrix
+// is updated by an FP64 MFMA GEMM.
+#include <hip/hip_cooperative_groups.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+#include "lu_internal.h"
+
+namespace dust {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Grid barrier for cooperative launches: arrival counter + generation word.
+// Spins are bounded; a timed-out barrier raises flag[2] and every block
+// leaves the kernel at the next check.
+// ---------------------------------------------------------------------------
+__device__ inline bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &gen) {
+  __shared__ int ok;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ok = 1;
+    __threadfence();
+    unsigned prev = atomicAdd(&bar[0], 1u);
+    if (prev == nblocks - 1) {
+      atomicExch(&bar[0], 0u);
+      __threadfence();
+      atomicAdd(&bar[1], 1u);
+    } else {
+      long spins = 0;
+      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > (1L << 26)) {
+          atomicExch(&bar[2], 1u);
+          ok = 0;
+          break;
+        }
+        if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          ok = 0;
+          break;
+        }
+      }
+    }
+    __threadfence();
+    gen += 1;
+  }
+  __syncthreads();
+  __threadfence();
+  return ok != 0;
+}
+
+// ---------------------------------------------------------------------------
+// Panel factorisation: columns [k, k+jb) of A, rows [k, m).  Block b owns
+// rows [k + b R, k + (b+1) R).  Per column: local first-maximum of |a| ->
+// candidates published -> barrier -> global pivot (max value, smallest row
+// on ties, as idamax) -> swap inside the panel -> scale and rank-1 update.
+// ---------------------------------------------------------------------------
+struct PanelArgs {
+  double *A;
+  long lda;
+  int m, k, jb, R;
+  int *ipiv;      // 1-based, global row numbers
+  int *info;
+  unsigned *bar;  // [3]
+  double *cval;   // [2][G]
+  int *crow;      // [2][G]
+  double *crowv;  // [2][G][jb]
+  double *topv;   // [2][jb]
+};
+
+__global__ void __launch_bounds__(256) k_panel(PanelArgs P) {
+  extern __shared__ double sp[];  // [R][jb] row-major panel rows + pivot rows
+  const int G = gridDim.x, b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  const int jb = P.jb, R = P.R;
+  const int r0 = P.k + b * R;
+  const int r1 = min(P.m, r0 + R);
+  const int nr = max(0, r1 - r0);
+  double *rows = sp;               // nr x jb
+  double *prow = sp + R * jb;      // jb
+  double *trow = prow + jb;        // jb
+  __shared__ double rv[256];
+  __shared__ int ri[256];
+  __shared__ int win;
+  unsigned gen = 0;
+  if (tid == 0) gen = __hip_atomic_load(&P.bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int q = tid; q < nr * jb; q += nt) {
+    int r = q / jb, c = q % jb;
+    rows[q] = P.A[(long)(r0 + r) + P.lda * (P.k + c)];
+  }
+  __syncthreads();
+  for (int jj = 0; jj < jb; ++jj) {
+    const int par = jj & 1;
+    const int gr = P.k + jj;  // global row of the diagonal
+    // local first maximum over rows >= gr
+    double best = -1.0;
+    int bi = 0x7fffffff;
+    for (int r = tid; r < nr; r += nt) {
+      if (r0 + r < gr) continue;
+      double a = fabs(rows[r * jb + jj]);
+      if (a > best) { best = a; bi = r0 + r; }
+    }
+    rv[tid] = best;
+    ri[tid] = bi;
+    __syncthreads();
+    for (int s = nt / 2; s > 0; s >>= 1) {
+      if (tid < s) {
+        double o = rv[tid + s];
+        int oi = ri[tid + s];
+        if (o > rv[tid] || (o == rv[tid] && oi < ri[tid])) { rv[tid] = o; ri[tid] = oi; }
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      P.cval[par * G + b] = rv[0];
+      P.crow[par * G + b] = ri[0];
+    }
+    if (ri[0] != 0x7fffffff)
+      for (int c = tid; c < jb; c += nt) P.crowv[((long)par * G + b) * jb + c] = rows[(ri[0] - r0) * jb + c];
+    if (gr >= r0 && gr < r1)
+      for (int c = tid; c < jb; c += nt) P.topv[par * jb + c] = rows[(gr - r0) * jb + c];
+    if (!grid_sync(P.bar, G, gen)) return;
+    // global pivot
+    double gb = -1.0;
+    int gi = 0x7fffffff, gw = 0;
+    for (int q = tid; q < G; q += nt) {
+      double v = __hip_atomic_load(&P.cval[par * G + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      int i = __hip_atomic_load(&P.crow[par * G + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (v > gb || (v == gb && i < gi)) { gb = v; gi = i; gw = q; }
+    }
+    rv[tid] = gb;
+    ri[tid] = gi;
+    __syncthreads();
+    if (tid == 0) {
+      double bv = rv[0];
+      int bidx = ri[0], bw = 0;
+      // recover the winner block: candidates are owned by row ranges
+      for (int t = 1; t < nt; ++t)
+        if (rv[t] > bv || (rv[t] == bv && ri[t] < bidx)) { bv = rv[t]; bidx = ri[t]; }
+      bw = (bidx - P.k) / R;
+      win = bidx;
+      ri[0] = bw;
+    }
+    __syncthreads();
+    (void)gw;
+    const int p = win, pw = ri[0];
+    for (int c = tid; c < jb; c += nt) {
+      prow[c] = __hip_atomic_load(&P.crowv[((long)par * G + pw) * jb + c], __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+      trow[c] = __hip_atomic_load(&P.topv[par * jb + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (p != gr) {
+      if (p >= r0 && p < r1)
+        for (int c = tid; c < jb; c += nt) rows[(p - r0) * jb + c] = trow[c];
+      if (gr >= r0 && gr < r1)
+        for (int c = tid; c < jb; c += nt) rows[(gr - r0) * jb + c] = prow[c];
+    }
+    __syncthreads();
+    const double piv = prow[jj];
+    if (b == 0 && tid == 0) {
+      P.ipiv[gr] = p + 1;
+      if (piv == 0.0 && *P.info == 0) *P.info = gr + 1;
+    }
+    if (piv != 0.0) {
+      // dgetf2: scale by the reciprocal when it is representable
+      const bool recip = fabs(piv) >= 2.2250738585072014e-308;
+      const double rp = 1.0 / piv;
+      for (int r = tid; r < nr; r += nt) {
+        if (r0 + r <= gr) continue;
+        double l = recip ? rows[r * jb + jj] * rp : rows[r * jb + jj] / piv;
+        rows[r * jb + jj] = l;
+      }
+      __syncthreads();
+      const int w = jb - jj - 1;
+      for (int q = tid; q < nr * w; q += nt) {
+        int r = q / w, c = jj + 1 + q % w;
+        if (r0 + r <= gr) continue;
+        rows[r * jb + c] -= rows[r * jb + jj] * prow[c];
+      }
+    }
+    __syncthreads();
+  }
+  for (int q = tid; q < nr * jb; q += nt) {
+    int r = q / jb, c = q % jb;
+    P.A[(long)(r0 + r) + P.lda * (P.k + c)] = rows[q];
+  }
+}
+
+// Row interchanges ipiv[k..k+jb) applied to the columns outside the panel
+// (dlaswp): one thread per column.
+__global__ void k_laswp(double *A, long lda, int ncol, int c0, int c1, int k, int jb,
+                        const int *__restrict__ ipiv) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol || (c >= c0 && c < c1)) return;
+  double *col = A + lda * c;
+  for (int i = k; i < k + jb; ++i) {
+    int p = ipiv[i] - 1;
+    if (p != i) {
+      double t = col[i];
+      col[i] = col[p];
+      col[p] = t;
+    }
+  }
+}
+
+// U12 = L11^-1 A12 for 64 columns per block; L11 unit lower (jb x jb) in LDS.
+__global__ void __launch_bounds__(256) k_trsm_llnu(const double *__restrict__ L, long ldl, double *B,
+                                                   long ldb, int jb, int ncol) {
+  extern __shared__ double st[];
+  double *sL = st;              // jb x jb, column-major
+  double *sX = st + jb * jb;    // jb x 64, column-major
+  const int c0 = blockIdx.x * 64;
+  const int nc = min(64, ncol - c0);
+  for (int q = threadIdx.x; q < jb * jb; q += blockDim.x) sL[q] = L[(q % jb) + ldl * (q / jb)];
+  for (int q = threadIdx.x; q < jb * nc; q += blockDim.x)
+    sX[q] = B[(q % jb) + ldb * (long)(c0 + q / jb)];
+  __syncthreads();
+  for (int i = 0; i < jb - 1; ++i) {
+    const int w = jb - 1 - i;
+    for (int q = threadIdx.x; q < w * nc; q += blockDim.x) {
+      int r = i + 1 + q % w, c = q / w;
+      sX[r + jb * c] -= sL[r + jb * i] * sX[i + jb * c];
+    }
+    __syncthreads();
+  }
+  for (int q = threadIdx.x; q < jb * nc; q += blockDim.x)
+    B[(q % jb) + ldb * (long)(c0 + q / jb)] = sX[q];
+}
+
+// C(M,N) -= A(M,K) B(K,N), FP64 on the matrix cores (v_mfma_f64_16x16x4f64).
+// Block tile 64x64, four waves in 2x2, each wave a 32x32 tile of 2x2 MFMA
+// tiles; K staged through LDS in chunks of 16.
+__global__ void __launch_bounds__(256) k_gemm_f64_sub(int M, int N, int K, const double *__restrict__ A,
+                                                      long lda, const double *__restrict__ B, long ldb,
+                                                      double *C, long ldc) {
+  __shared__ double sA[16][64 + 1];  // [k][m]
+  __shared__ double sB[16][64 + 1];  // [k][n]
+  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wr = wv & 1, wc = wv >> 1;
+  d4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    // A chunk 64 x 16: thread -> (m = tid % 64, k = tid / 64 + 4 t)
+    for (int t = 0; t < 4; ++t) {
+      int kk = tid / 64 + 4 * t, mm = tid % 64;
+      int gm = m0 + mm, gk = k0 + kk;
+      sA[kk][mm] = (gm < M && gk < K) ? A[gm + lda * gk] : 0.0;
+    }
+    // B chunk 16 x 64: thread -> (k = tid % 16, n = tid / 16 + 16 t)
+    for (int t = 0; t < 4; ++t) {
+      int kk = tid % 16, nn = tid / 16 + 16 * t;
+      int gn = n0 + nn, gk = k0 + kk;
+      sB[kk][nn] = (gn < N && gk < K) ? B[gk + ldb * gn] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const int kk = ks * 4 + (lane >> 4);
+      double af[2], bf[2];
+      for (int a = 0; a < 2; ++a) af[a] = sA[kk][wr * 32 + a * 16 + (lane & 15)];
+      for (int b = 0; b < 2; ++b) bf[b] = sB[kk][wc * 32 + b * 16 + (lane & 15)];
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // FP64 accumulator layout of v_mfma_f64_16x16x4f64: lane holds column
+  // (lane % 16) and rows 4 i + lane / 16 (i = register index); the FP32
+  // variants use rows 4 (lane / 16) + i instead.
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      int gn = n0 + wc * 32 + b * 16 + (lane & 15);
+      if (gn >= N) continue;
+      for (int i = 0; i < 4; ++i) {
+        int gm = m0 + wr * 32 + a * 16 + 4 * i + (lane >> 4);
+        if (gm < M) C[gm + ldc * gn] -= acc[a][b][i];
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// getrs pieces: row interchanges on B, blocked unit-lower and upper solves.
+// ---------------------------------------------------------------------------
+__global__ void k_laswp_rhs(double *B, long ldb, int nrhs, int n, const int *__restrict__ ipiv,
+                            int reverse) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrhs) return;
+  double *b = B + ldb * r;
+  if (!reverse) {
+    for (int i = 0; i < n; ++i) {
+      int p = ipiv[i] - 1;
+      if (p != i) { double t = b[i]; b[i] = b[p]; b[p] = t; }
+    }
+  } else {
+    for (int i = n - 1; i >= 0; --i) {
+      int p = ipiv[i] - 1;
+      if (p != i) { double t = b[i]; b[i] = b[p]; b[p] = t; }
+    }
+  }
+}
+
+// Solve the diagonal block [k, k+jb) in place (one block per RHS set).
+__global__ void k_trsv_block(const double *__restrict__ A, long lda, double *B, long ldb, int nrhs,
+                             int k, int jb, int upper) {
+  extern __shared__ double sx[];  // jb x nrhs
+  for (int q = threadIdx.x; q < jb * nrhs; q += blockDim.x)
+    sx[q] = B[(k + q % jb) + ldb * (q / jb)];
+  __syncthreads();
+  if (!upper) {
+    for (int i = 0; i < jb; ++i) {
+      for (int q = threadIdx.x; q < (jb - 1 - i) * nrhs; q += blockDim.x) {
+        int w = jb - 1 - i, r = i + 1 + q % w, c = q / w;
+        sx[r + jb * c] -= A[(k + r) + lda * (k + i)] * sx[i + jb * c];
+      }
+      __syncthreads();
+    }
+  } else {
+    for (int i = jb - 1; i >= 0; --i) {
+      if (threadIdx.x < nrhs) sx[i + jb * threadIdx.x] /= A[(k + i) + lda * (k + i)];
+      __syncthreads();
+      for (int q = threadIdx.x; q < i * nrhs; q += blockDim.x) {
+        int r = q % i, c = q / i;
+        sx[r + jb * c] -= A[(k + r) + lda * (k + i)] * sx[i + jb * c];
+      }
+      __syncthreads();
+    }
+  }
+  for (int q = threadIdx.x; q < jb * nrhs; q += blockDim.x)
+    B[(k + q % jb) + ldb * (q / jb)] = sx[q];
+}
+
+// B[rows, :] -= A[rows, k:k+jb] X[k:k+jb, :], rows given by [r0, r1).
+__global__ void k_gemv_update(const double *__restrict__ A, long lda, double *B, long ldb,
+                              int nrhs, int k, int jb, int r0, int r1) {
+  int r = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= r1) return;
+  for (int c = 0; c < nrhs; ++c) {
+    double acc = 0.0;
+    for (int t = 0; t < jb; ++t) acc += A[r + lda * (k + t)] * B[(k + t) + ldb * c];
+    B[r + ldb * c] -= acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host drivers
+// ---------------------------------------------------------------------------
+static int g_cus = 0;
+static int device_cus() {
+  if (g_cus == 0) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, dev) == hipSuccess) g_cus = pr.multiProcessorCount;
+    if (g_cus <= 0) g_cus = 64;
+  }
+  return g_cus;
+}
+
+int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t st, int nb,
+                 int *info_out) {
+  DUST_CHECK(m >= 0 && n >= 0 && lda >= std::max(1, m), DUST_E_ARG, "invalid getrf dimensions");
+  *info_out = 0;
+  if (m == 0 || n == 0) return 0;
+  const int cus = device_cus();
+  const int Gmax = cus;
+  // workspace: barrier + candidates
+  size_t wbytes = 256 + sizeof(double) * 2 * Gmax + sizeof(int) * 2 * Gmax +
+                  sizeof(double) * 2 * (size_t)Gmax * nb + sizeof(double) * 2 * nb + 64;
+  char *ws = nullptr;
+  DUST_HIP(hipMalloc(&ws, wbytes));
+  DUST_HIP(hipMemsetAsync(ws, 0, wbytes, st));
+  unsigned *bar = (unsigned *)ws;
+  int *dinfo = (int *)(ws + 16);
+  double *cval = (double *)(ws + 256);
+  int *crow = (int *)(cval + 2 * Gmax);
+  double *crowv = (double *)(crow + 2 * Gmax + 2);
+  crowv = (double *)(((uintptr_t)crowv + 15) & ~(uintptr_t)15);
+  double *topv = crowv + 2 * (size_t)Gmax * nb;
+  const int mn = std::min(m, n);
+  int rc = 0;
+  for (int k = 0; k < mn && !rc; k += nb) {
+    const int jb = std::min(nb, mn - k);
+    const int mrows = m - k;
+    int G = std::min(Gmax, std::max(1, (mrows + 31) / 32));
+    int R = (mrows + G - 1) / G;
+    G = (mrows + R - 1) / R;
+    size_t lds = sizeof(double) * ((size_t)R * jb + 2 * jb);
+    if (lds > 150 * 1024) {
+      set_error("panel of %d x %d does not fit in LDS", R, jb);
+      rc = DUST_E_UNSUPPORTED;
+      break;
+    }
+    PanelArgs P{dA, lda, m, k, jb, R, dipiv, dinfo, bar, cval, crow, crowv, topv};
+    void *args[] = {&P};
+    if (lds > 64 * 1024)
+      (void)hipFuncSetAttribute((const void *)k_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+    hipError_t e = hipLaunchCooperativeKernel((const void *)k_panel, dim3(G), dim3(256), args, lds, st);
+    if (e != hipSuccess) {
+      set_error("cooperative panel launch failed (G=%d, lds=%zu): %s", G, lds, hipGetErrorString(e));
+      rc = DUST_E_HIP;
+      break;
+    }
+    // interchanges on the other columns
+    k_laswp<<<ceil_div(n, 256), 256, 0, st>>>(dA, lda, n, k, k + jb, k, jb, dipiv);
+    if ((e = hipGetLastError()) != hipSuccess) {
+      set_error("laswp launch failed: %s", hipGetErrorString(e));
+      rc = DUST_E_HIP;
+      break;
+    }
+    const int nrest = n - k - jb;
+    if (nrest > 0) {
+      size_t tl = sizeof(double) * ((size_t)jb * jb + (size_t)jb * 64);
+      if (tl > 64 * 1024)
+        (void)hipFuncSetAttribute((const void *)k_trsm_llnu,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)tl);
+      k_trsm_llnu<<<ceil_div(nrest, 64), 256, tl, st>>>(dA + k + lda * k, lda,
+                                                        dA + k + lda * (k + jb), lda, jb, nrest);
+      const int mrest = m - k - jb;
+      if (mrest > 0) {
+        dim3 grid(ceil_div(mrest, 64), ceil_div(nrest, 64));
+        k_gemm_f64_sub<<<grid, 256, 0, st>>>(mrest, nrest, jb, dA + (k + jb) + lda * k, lda,
+                                             dA + k + lda * (k + jb), lda,
+                                             dA + (k + jb) + lda * (k + jb), lda);
+      }
+      if ((e = hipGetLastError()) != hipSuccess) {
+        set_error("trailing update launch failed: %s", hipGetErrorString(e));
+        rc = DUST_E_HIP;
+        break;
+      }
+    }
+  }
+  unsigned hb[3] = {0, 0, 0};
+  int hinfo = 0;
+  if (!rc) {
+    DUST_HIP(hipMemcpyAsync(hb, bar, sizeof(hb), hipMemcpyDeviceToHost, st));
+    DUST_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, st));
+  }
+  DUST_HIP(hipStreamSynchronize(st));
+  (void)hipFree(ws);
+  if (rc) return rc;
+  DUST_CHECK(hb[2] == 0, DUST_E_HIP, "panel grid barrier timed out");
+  *info_out = hinfo;
+  return 0;
+}
+
+int lu_getrs_dev(int n, int nrhs, const double *dA, long lda, const int *dipiv, double *dB,
+                 long ldb, hipStream_t st) {
+  DUST_CHECK(n >= 0 && nrhs >= 0 && lda >= std::max(1, n) && ldb >= std::max(1, n), DUST_E_ARG,
+             "invalid getrs dimensions");
+  if (n == 0 || nrhs == 0) return 0;
+  const int jb = 64;
+  k_laswp_rhs<<<ceil_div(nrhs, 64), 64, 0, st>>>(dB, ldb, nrhs, n, dipiv, 0);
+  size_t sh = sizeof(double) * jb * nrhs;
+  DUST_CHECK(sh <= 64 * 1024, DUST_E_ARG, "too many right-hand sides for one solve");
+  for (int k = 0; k < n; k += jb) {
+    int w = std::min(jb, n - k);
+    k_trsv_block<<<1, 256, sizeof(double) * w * nrhs, st>>>(dA, lda, dB, ldb, nrhs, k, w, 0);
+    if (k + w < n)
+      k_gemv_update<<<ceil_div(n - k - w, 256), 256, 0, st>>>(dA, lda, dB, ldb, nrhs, k, w, k + w, n);
+  }
+  for (int k = ((n - 1) / jb) * jb; k >= 0; k -= jb) {
+    int w = std::min(jb, n - k);
+    k_trsv_block<<<1, 256, sizeof(double) * w * nrhs, st>>>(dA, lda, dB, ldb, nrhs, k, w, 1);
+    if (k > 0) k_gemv_update<<<ceil_div(k, 256), 256, 0, st>>>(dA, lda, dB, ldb, nrhs, k, w, 0, k);
+  }
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
+
+int lu_gemm_sub_dev(int M, int N, int K, const double *A, long lda, const double *B, long ldb,
+                    double *C, long ldc, hipStream_t st) {
+  if (M <= 0 || N <= 0 || K <= 0) return 0;
+  dim3 grid(ceil_div(M, 64), ceil_div(N, 64));
+  k_gemm_f64_sub<<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
+
+int lu_laswp_dev(double *A, long lda, int ncol, int k, int jb, const int *dipiv, hipStream_t st) {
+  if (ncol <= 0 || jb <= 0) return 0;
+  k_laswp<<<ceil_div(ncol, 256), 256, 0, st>>>(A, lda, ncol, 0, 0, k, jb, dipiv);
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
+
+int lu_trsm_llnu_dev(int m, int n, const double *L, long ldl, double *B, long ldb, hipStream_t st) {
+  // blocked: diagonal blocks of 64 solved in LDS, off-diagonal updates by GEMM
+  for (int k = 0; k < m; k += 64) {
+    int jb = std::min(64, m - k);
+    size_t tl = sizeof(double) * ((size_t)jb * jb + (size_t)jb * 64);
+    k_trsm_llnu<<<ceil_div(n, 64), 256, tl, st>>>(L + k + ldl * k, ldl, B + k, ldb, jb, n);
+    if (k + jb < m) DUST_TRY(lu_gemm_sub_dev(m - k - jb, n, jb, L + (k + jb) + ldl * k, ldl, B + k, ldb,
+                                             B + k + jb, ldb, st));
+  }
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace dust
+
+using namespace dust;
+
+extern "C" {
+
+int dust_dgetrf_dev(int m, int n, double *dA, int lda, int *dipiv, void *stream) {
+  int info = 0;
+  DUST_TRY(lu_getrf_dev(m, n, dA, lda, dipiv, (hipStream_t)stream, 64, &info));
+  return info;
+}
+
+int dust_dgetrs_dev(int n, int nrhs, const double *dA, int lda, const int *dipiv, double *dB,
+                    int ldb, void *stream) {
+  DUST_TRY(lu_getrs_dev(n, nrhs, dA, lda, dipiv, dB, ldb, (hipStream_t)stream));
+  DUST_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return 0;
+}
+
+int dust_dgetrf(int m, int n, double *A, int lda, int *ipiv) {
+  DUST_CHECK(A && ipiv && m >= 0 && n >= 0 && lda >= std::max(1, m), DUST_E_ARG,
+             "invalid dgetrf arguments");
+  if (m == 0 || n == 0) return 0;
+  double *dA = nullptr;
+  int *dp = nullptr;
+  int mn = std::min(m, n);
+  DUST_HIP(hipMalloc(&dA, sizeof(double) * (size_t)m * n));
+  DUST_HIP(hipMalloc(&dp, sizeof(int) * mn));
+  DUST_HIP(hipMemcpy2D(dA, sizeof(double) * m, A, sizeof(double) * lda, sizeof(double) * m, n,
+                       hipMemcpyHostToDevice));
+  int info = 0;
+  int rc = lu_getrf_dev(m, n, dA, m, dp, nullptr, 64, &info);
+  if (!rc) {
+    if (hipMemcpy2D(A, sizeof(double) * lda, dA, sizeof(double) * m, sizeof(double) * m, n,
+                    hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(ipiv, dp, sizeof(int) * mn, hipMemcpyDeviceToHost) != hipSuccess) {
+      set_error("download failed");
+      rc = DUST_E_HIP;
+    }
+  }
+  (void)hipFree(dA);
+  (void)hipFree(dp);
+  return rc ? rc : info;
+}
+
+// Test hooks on device pointers: the trailing-update GEMM, the unit-lower
+// triangular solve and the row interchanges used inside dgetrf.
+int dust_test_gemm_sub_dev(int M, int N, int K, const double *A, int lda, const double *B, int ldb,
+                           double *C, int ldc) {
+  DUST_TRY(lu_gemm_sub_dev(M, N, K, A, lda, B, ldb, C, ldc, nullptr));
+  DUST_HIP(hipDeviceSynchronize());
+  return 0;
+}
+int dust_test_trsm_llnu_dev(int m, int n, const double *L, int ldl, double *B, int ldb) {
+  DUST_TRY(lu_trsm_llnu_dev(m, n, L, ldl, B, ldb, nullptr));
+  DUST_HIP(hipDeviceSynchronize());
+  return 0;
+}
+
+int dust_dgetrs(int n, int nrhs, const double *A, int lda, const int *ipiv, double *B, int ldb) {
+  DUST_CHECK(A && ipiv && B && n >= 0 && nrhs >= 0 && lda >= std::max(1, n) &&
+                 ldb >= std::max(1, n),
+             DUST_E_ARG, "invalid dgetrs arguments");
+  if (n == 0 || nrhs == 0) return 0;
+  double *dA = nullptr, *dB = nullptr;
+  int *dp = nullptr;
+  DUST_HIP(hipMalloc(&dA, sizeof(double) * (size_t)n * n));
+  DUST_HIP(hipMalloc(&dB, sizeof(double) * (size_t)n * nrhs));
+  DUST_HIP(hipMalloc(&dp, sizeof(int) * n));
+  DUST_HIP(hipMemcpy2D(dA, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, n,
+                       hipMemcpyHostToDevice));
+  DUST_HIP(hipMemcpy2D(dB, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
+                       hipMemcpyHostToDevice));
+  DUST_HIP(hipMemcpy(dp, ipiv, sizeof(int) * n, hipMemcpyHostToDevice));
+  int rc = lu_getrs_dev(n, nrhs, dA, n, dp, dB, n, nullptr);
+  if (!rc && hipMemcpy2D(B, sizeof(double) * ldb, dB, sizeof(double) * n, sizeof(double) * n, nrhs,
+                         hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("download failed");
+    rc = DUST_E_HIP;
+  }
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dp);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+"""Host mirror of DUST's src/linsys dense kernels on top of libdust_gpu.
+
+    aic_surfpan(ver, n_ver)     static AIC of surface panels: build_row_static_surfpan
+                                with compute_pot_surfpan (src/aeroel/mod_surfpan.f90:448-481,
+                                579-601)  -> (A, B)
+    aic_vortlatt(ver, n_ver)    static AIC of vortex rings: build_row_static_vortlatt with
+                                compute_psi_vortlatt (src/aeroel/mod_vortlatt.f90:201-237,
+                                365-391)  -> (A, B)
+    rhs_static(...)             static RHS (mod_surfpan.f90:400-404)
+    dgetrf / dgetrs             the LAPACK call sites of src/linsys/mod_linsys.f90 (236, 435,
+                                473): same arguments, 1-based ipiv, LAPACK info
+    solve_linsys(...)           the block static/moving path, mod_linsys.f90:352-483
+    dgesv_mixed(...)            tuned mixed-precision LU + FP64 iterative refinement
+
+Matrices are numpy float64 arrays in Fortran (column-major) order.  A LAPACK
+info > 0 is returned, as the reference receives it before calling error()
+(mod_linsys.f90:241-244); API misuse raises DustError.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _params(ff_ratio_doublet=10.0, rankine_rad=0.1, cutoff_rad=0.001):
+    p = L.AicParams()
+    p.ff_ratio_doublet = float(ff_ratio_doublet)
+    p.rankine_rad = float(rankine_rad)
+    p.cutoff_rad = float(cutoff_rad)
+    return p
+
+
+def _panels(ver, n_ver):
+    n = len(n_ver)
+    v = np.zeros((n, 4, 3))
+    ver = np.asarray(ver, dtype=np.float64)
+    v[:, : ver.shape[1], :] = ver[:, :4, :]
+    return np.ascontiguousarray(v), np.ascontiguousarray(n_ver, dtype=np.int32)
+
+
+def aic_surfpan(ver, n_ver, with_source=True, **prm):
+    v, nv = _panels(ver, n_ver)
+    n = len(nv)
+    A = np.zeros((n, n), order="F")
+    B = np.zeros((n, n), order="F") if with_source else None
+    L.check(L.lib().dust_aic_surfpan(n, L.ptr(nv), L.ptr(v), C.byref(_params(**prm)), L.ptr(A), n,
+                                     L.ptr(B), n), "aic_surfpan")
+    return A, B
+
+
+def aic_vortlatt(ver, n_ver, **prm):
+    v, nv = _panels(ver, n_ver)
+    n = len(nv)
+    A = np.zeros((n, n), order="F")
+    B = np.zeros((n, n), order="F")
+    L.check(L.lib().dust_aic_vortlatt(n, L.ptr(nv), L.ptr(v), C.byref(_params(**prm)), L.ptr(A), n,
+                                      L.ptr(B), n), "aic_vortlatt")
+    return A, B
+
+
+def rhs_static(ver, n_ver, B, u_inf, uvort=None):
+    """b(:, r) = B n.(-u_inf(:, r) - uvort) for each free-stream vector."""
+    v, nv = _panels(ver, n_ver)
+    n = len(nv)
+    B = np.asfortranarray(B, dtype=np.float64)
+    u = np.ascontiguousarray(np.atleast_2d(u_inf), dtype=np.float64)
+    nrhs = u.shape[0]
+    b = np.zeros((n, nrhs), order="F")
+    uv = None if uvort is None else np.ascontiguousarray(uvort, dtype=np.float64)
+    L.check(L.lib().dust_aic_rhs(n, L.ptr(nv), L.ptr(v), L.ptr(B), n, nrhs, L.ptr(u), L.ptr(uv),
+                                 L.ptr(b), n), "rhs_static")
+    return b
+
+
+def dgetrf(A):
+    """In-place LU of a Fortran-ordered float64 array.  Returns (ipiv, info)."""
+    if not (A.flags.f_contiguous and A.dtype == np.float64):
+        raise ValueError("A must be a Fortran-ordered float64 array")
+    m, n = A.shape
+    ipiv = np.zeros(min(m, n), np.int32)
+    info = L.check(L.lib().dust_dgetrf(m, n, L.ptr(A), m, L.ptr(ipiv)), "dgetrf")
+    return ipiv, info
+
+
+def dgetrs(LU, ipiv, B):
+    """In-place solve with the factors of dgetrf; B is (n,) or (n, nrhs)."""
+    n = LU.shape[0]
+    Bv = B.reshape(n, -1, order="F")
+    if not (B.flags.f_contiguous and B.dtype == np.float64):
+        raise ValueError("B must be a Fortran-ordered float64 array")
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32)
+    L.check(L.lib().dust_dgetrs(n, Bv.shape[1], L.ptr(LU), n, L.ptr(ipiv), L.ptr(B), n), "dgetrs")
+    return B
+
+
+def solve_linsys(A, ipiv, nstatic, b):
+    """solve_linsys (mod_linsys.f90:352-483).  A(1:nstatic,1:nstatic) holds the
+    static factors and ipiv(1:nstatic) their pivots; A and ipiv are updated in
+    place as the reference leaves them.  Returns (res, info)."""
+    rank = A.shape[0]
+    res = np.zeros(rank)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    info = L.check(L.lib().dust_solve_linsys(rank, nstatic, L.ptr(A), rank, L.ptr(ipiv), L.ptr(b),
+                                             L.ptr(res)), "solve_linsys")
+    return res, info
+
+
+def dgesv_mixed(A, B, prec="fp16s", tol=1e-12, max_iter=30):
+    """Tuned solve of A X = B: low-precision LU on the matrix cores, FP64
+    residuals and iterative refinement.  A is not modified; B is overwritten
+    with X.  Returns (X, iters, backward_error)."""
+    n = A.shape[0]
+    Bv = B.reshape(n, -1, order="F")
+    it = C.c_int()
+    res = C.c_double()
+    code = {"fp64": L.LU_FP64, "fp32": L.LU_FP32, "fp16s": L.LU_FP16S}[prec]
+    info = L.check(L.lib().dust_dgesv_mixed(n, Bv.shape[1], L.ptr(A), n, L.ptr(B), n, code,
+                                            float(tol), int(max_iter), C.byref(it), C.byref(res)),
+                   "dgesv_mixed")
+    if info > 0:
+        raise np.linalg.LinAlgError(f"dgesv_mixed: info = {info}")
+    return B, it.value, res.value

@@ -1,0 +1,198 @@
+"""Host mirror of DUST's src/fmm public interface on top of libdust_gpu.
+
+Names, argument meaning and error behaviour follow mod_octree
+(src/fmm/mod_octree.f90:78-79):
+
+    initialize_octree(box_length, nbox, origin, nlevels, min_part, degree,
+                      delta)                          -> Octree   (251-518)
+    sort_particles(parts, octree)                     -> n_leaves (703-996)
+    calculate_multipole(parts, octree)                            (1002-1133)
+    apply_multipole(parts, octree)                                (1139-1429)
+    destroy_octree(octree)                                        (694)
+
+`parts` is a VortParts, the SoA image of t_vortpart (src/aeroel/mod_vortpart.f90:85-97):
+cen (n,3), dir (n,3), mag (n), r_vortex (n); apply_multipole fills vel, stretch
+and turbvisc in place, as the reference writes through t_vortpart pointers.
+Where the reference stops with error() (particle outside the box,
+mod_octree.f90:757-762) a DustError is raised.
+
+The sim_param switches the reference FMM reads (use_vs, use_vd, use_tv,
+nu_inf, u_inf) are keyword arguments of initialize_octree.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class VortParts:
+    """Vortex particles (t_vortpart fields used by the FMM)."""
+
+    def __init__(self, cen, dir, mag, r_vortex):
+        self.cen = np.ascontiguousarray(cen, dtype=np.float64).reshape(-1, 3)
+        n = self.cen.shape[0]
+        self.dir = np.ascontiguousarray(dir, dtype=np.float64).reshape(n, 3)
+        self.mag = np.ascontiguousarray(np.broadcast_to(mag, (n,)), dtype=np.float64)
+        self.r_vortex = np.ascontiguousarray(np.broadcast_to(r_vortex, (n,)), dtype=np.float64)
+        self.vel = np.zeros((n, 3))
+        self.stretch = np.zeros((n, 3))
+        self.turbvisc = np.zeros(n)
+
+    @classmethod
+    def from_alpha(cls, cen, alpha, r_vortex):
+        """Build from vorticity vectors alpha = mag * dir."""
+        alpha = np.asarray(alpha, dtype=np.float64).reshape(-1, 3)
+        mag = np.linalg.norm(alpha, axis=1)
+        dir = np.where(mag[:, None] > 0, alpha / np.where(mag > 0, mag, 1.0)[:, None], 0.0)
+        return cls(cen, dir, mag, r_vortex)
+
+    @property
+    def n(self):
+        return self.cen.shape[0]
+
+    @property
+    def alpha(self):
+        return self.dir * self.mag[:, None]
+
+
+class Octree:
+    """Handle on a device octree (the t_octree of mod_octree.f90:174-223)."""
+
+    def __init__(self, params, capacity):
+        self.params = params
+        self.capacity = capacity
+        h = C.c_void_p()
+        L.check(L.lib().dust_fmm_create(C.byref(params), capacity, C.byref(h)), "initialize_octree")
+        self._h = h
+        self.n_leaves = 0
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise RuntimeError("octree destroyed")
+        return self._h
+
+    @property
+    def degree(self):
+        return self.params.degree
+
+    @property
+    def nlevels(self):
+        return self.params.nlevels
+
+    def destroy(self):
+        if self._h is not None:
+            L.lib().dust_fmm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    # -- introspection -----------------------------------------------------
+    def leaves(self, n):
+        """Leaf membership: (level (1-based), ijk (1-based), npart, members)."""
+        nl = self.n_leaves
+        lev = np.zeros(nl, np.int32)
+        ijk = np.zeros((nl, 3), np.int32)
+        npart = np.zeros(nl, np.int32)
+        mem = np.zeros(max(n, 1), np.int32)
+        L.check(L.lib().dust_fmm_leaves(self.handle, L.ptr(lev), L.ptr(ijk), L.ptr(npart),
+                                        L.ptr(mem)), "leaves")
+        return lev, ijk, npart, mem[:n]
+
+    def cell_index(self, n):
+        ijk = np.zeros((max(n, 1), 3), np.int32)
+        L.check(L.lib().dust_fmm_cell_index(self.handle, L.ptr(ijk)), "cell_index")
+        return ijk[:n]
+
+    def work(self):
+        p = C.c_double()
+        m = C.c_double()
+        L.check(L.lib().dust_fmm_work(self.handle, C.byref(p), C.byref(m)), "work")
+        return p.value, m.value
+
+    def set_timing(self, on):
+        L.check(L.lib().dust_fmm_set_timing(self.handle, int(bool(on))), "set_timing")
+
+    def stage_times(self):
+        ms = np.zeros(6)
+        cnt = np.zeros(6, np.int32)
+        L.check(L.lib().dust_fmm_stage_times(self.handle, L.ptr(ms), L.ptr(cnt)), "stage_times")
+        return {s: (ms[i], int(cnt[i])) for i, s in enumerate(L.STAGES)}
+
+    @property
+    def stream(self):
+        return L.lib().dust_fmm_stream(self.handle)
+
+
+def make_params(box_length, nbox, origin, nlevels, min_part, degree, delta, use_vs=True,
+                use_vd=False, use_tv=False, nu_inf=0.0, u_inf=(0.0, 0.0, 0.0),
+                precision="fp64"):
+    p = L.FmmParams()
+    p.box_length = float(box_length)
+    for d in range(3):
+        p.nbox[d] = int(nbox[d])
+        p.origin[d] = float(origin[d])
+        p.u_inf[d] = float(u_inf[d])
+    p.nlevels = int(nlevels)
+    p.min_part = int(min_part)
+    p.degree = int(degree)
+    p.delta = float(delta)
+    p.use_vs = int(bool(use_vs))
+    p.use_vd = int(bool(use_vd))
+    p.use_tv = int(bool(use_tv))
+    p.nu_inf = float(nu_inf)
+    p.precision = {"fp64": L.FMM_FP64, "tuned": L.FMM_TUNED}[precision]
+    return p
+
+
+def initialize_octree(box_length, nbox, origin, nlevels, min_part, degree, delta, *,
+                      capacity=1 << 20, **flags):
+    """initialize_octree (mod_octree.f90:251-518)."""
+    return Octree(make_params(box_length, nbox, origin, nlevels, min_part, degree, delta, **flags),
+                  capacity)
+
+
+def sort_particles(parts, octree):
+    """sort_particles (mod_octree.f90:703-996); returns the number of leaves."""
+    lib = L.lib()
+    L.check(lib.dust_fmm_set_particles(octree.handle, parts.n, L.ptr(parts.cen), L.ptr(parts.dir),
+                                       L.ptr(parts.mag), L.ptr(parts.r_vortex)), "sort_particles")
+    nl = C.c_int()
+    L.check(lib.dust_fmm_sort_particles(octree.handle, C.byref(nl)), "sort_particles")
+    octree.n_leaves = nl.value
+    return nl.value
+
+
+def calculate_multipole(parts, octree):
+    """calculate_multipole (mod_octree.f90:1002-1133); `parts` is unused as in
+    the reference."""
+    L.check(L.lib().dust_fmm_calculate_multipole(octree.handle), "calculate_multipole")
+
+
+def apply_multipole(parts, octree):
+    """apply_multipole (mod_octree.f90:1139-1429), particle terms + free stream."""
+    L.check(L.lib().dust_fmm_apply_multipole(octree.handle, L.ptr(parts.vel), L.ptr(parts.stretch),
+                                             L.ptr(parts.turbvisc)), "apply_multipole")
+
+
+def destroy_octree(octree):
+    """destroy_octree (mod_octree.f90:694)."""
+    octree.destroy()
+
+
+def direct(parts, use_vs=True, use_vd=False, nu_inf=0.0, u_inf=(0.0, 0.0, 0.0)):
+    """Direct O(N^2) particle terms, the fmm=F branch of update_wake
+    (src/aeroel/mod_wake.f90:1050-1113).  Returns (vel, stretch)."""
+    vel = np.zeros((parts.n, 3))
+    st = np.zeros((parts.n, 3))
+    u = np.asarray(u_inf, dtype=np.float64)
+    L.check(L.lib().dust_vortpart_direct(parts.n, L.ptr(parts.cen), L.ptr(parts.dir),
+                                         L.ptr(parts.mag), L.ptr(parts.r_vortex), int(use_vs),
+                                         int(use_vd), float(nu_inf), L.ptr(u), L.ptr(vel),
+                                         L.ptr(st)), "direct")
+    return vel, st

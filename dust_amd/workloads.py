@@ -1,0 +1,169 @@
+"""Seeded synthetic inputs for the hot path (numpy only, no device work).
+
+helix_wake   -- vortex-particle wake: three helical tubes plus a uniform
+                background (the 1,000,000-particle FMM workload, seed 20261203)
+ellipsoid    -- perturbed ellipsoid surface mesh for the dense AIC
+                (128 x 128 panels = 16,384, seed 20261204)
+wing_case    -- rectangular NACA0012 wing (40 chordwise per side x 20 spanwise)
+                with a trailing vortex-particle sheet (1,600 panels, 8,000
+                particles)
+Each generator returns plain numpy arrays plus the octree settings that go
+with them.
+"""
+import numpy as np
+
+HELIX_SEED = 20261203
+ELLIPSOID_SEED = 20261204
+
+
+def helix_wake(n=1_000_000, seed=HELIX_SEED, tube_fraction=0.9, radius=1.0, pitch=0.5,
+               sigma=0.05, core=0.02):
+    """Three helical vortex tubes (radius 1, pitch 0.5, Gaussian cross-section
+    sigma) with tangent strengths ~N(1e-3, 2e-4), plus uniform background
+    particles in [-2,2]^3 with isotropic N(0, 1e-4^2) strengths.
+
+    Returns (cen (n,3), alpha (n,3), r_vortex (n,), octree dict)."""
+    rng = np.random.default_rng(seed)
+    nt = int(round(n * tube_fraction))
+    nb = n - nt
+    k = rng.integers(0, 3, nt)
+    phi = 2.0 * np.pi * k / 3.0
+    s = rng.uniform(0.0, 14.0 * np.pi, nt)
+    c = pitch / (2.0 * np.pi)
+    a = s + phi
+    ca, sa = np.cos(a), np.sin(a)
+    centre = np.stack([radius * ca, radius * sa, -1.75 + c * s], axis=1)
+    t = np.stack([-radius * sa, radius * ca, np.full(nt, c)], axis=1)
+    t /= np.linalg.norm(t, axis=1)[:, None]
+    nrm = np.stack([-ca, -sa, np.zeros(nt)], axis=1)
+    bn = np.cross(t, nrm)
+    g = rng.standard_normal((nt, 2))
+    pos_t = centre + sigma * (g[:, :1] * nrm + g[:, 1:] * bn)
+    m = rng.normal(1e-3, 2e-4, nt)
+    alpha_t = m[:, None] * t
+    pos_b = rng.uniform(-2.0, 2.0, (nb, 3))
+    alpha_b = rng.normal(0.0, 1e-4, (nb, 3))
+    cen = np.concatenate([pos_t, pos_b])
+    alpha = np.concatenate([alpha_t, alpha_b])
+    perm = rng.permutation(n)
+    cen = np.ascontiguousarray(cen[perm])
+    alpha = np.ascontiguousarray(alpha[perm])
+    rv = np.full(n, core)
+    tree = dict(box_length=4.0 + 2e-6, nbox=(1, 1, 1), origin=(-2.0 - 1e-6,) * 3, nlevels=7,
+                min_part=64, delta=core)
+    return cen, alpha, rv, tree
+
+
+def ellipsoid(nt=128, npf=128, axes=(4.0, 1.0, 0.5), jitter=0.01, seed=ELLIPSOID_SEED):
+    """Closed ellipsoid surface, nt rows in theta x npf columns in phi; the
+    two pole rows are triangles (n_ver = 3), the rest quads; outward normals.
+    Non-pole vertices are jittered by jitter * (local minimum edge) * U[-1,1]^3.
+
+    Returns (ver (n,4,3), n_ver (n,))."""
+    rng = np.random.default_rng(seed)
+    th = np.pi * np.arange(nt + 1) / nt
+    ph = 2.0 * np.pi * np.arange(npf) / npf
+    T, P = np.meshgrid(th[1:-1], ph, indexing="ij")  # interior rings
+    ring = np.stack([axes[0] * np.sin(T) * np.cos(P), axes[1] * np.sin(T) * np.sin(P),
+                     axes[2] * np.cos(T)], axis=-1)  # (nt-1, npf, 3)
+    # local minimum edge of every interior vertex
+    d_phi = np.linalg.norm(ring - np.roll(ring, 1, axis=1), axis=-1)
+    d_phi = np.minimum(d_phi, np.roll(d_phi, -1, axis=1))
+    north = np.array([0.0, 0.0, axes[2]])
+    south = np.array([0.0, 0.0, -axes[2]])
+    up = np.concatenate([np.linalg.norm(ring[:1] - north, axis=-1),
+                         np.linalg.norm(ring[1:] - ring[:-1], axis=-1)])
+    dn = np.concatenate([np.linalg.norm(ring[1:] - ring[:-1], axis=-1),
+                         np.linalg.norm(ring[-1:] - south, axis=-1)])
+    lmin = np.minimum(np.minimum(d_phi, up), dn)
+    ring = ring + jitter * lmin[..., None] * rng.uniform(-1.0, 1.0, ring.shape)
+    ver = np.zeros((nt * npf, 4, 3))
+    nv = np.zeros(nt * npf, np.int32)
+    q = 0
+    for i in range(nt):
+        for j in range(npf):
+            j1 = (j + 1) % npf
+            if i == 0:
+                v = [north, ring[0, j], ring[0, j1]]
+            elif i == nt - 1:
+                v = [ring[nt - 2, j], south, ring[nt - 2, j1]]
+            else:
+                v = [ring[i - 1, j], ring[i, j], ring[i, j1], ring[i - 1, j1]]
+            ver[q, : len(v)] = v
+            nv[q] = len(v)
+            q += 1
+    # orient outward: normal as in calc_geo_data_surfpan, flipped if inward
+    for p in range(len(nv)):
+        v = ver[p, : nv[p]]
+        if nv[p] == 4:
+            nr = np.cross(v[2] - v[0], v[3] - v[1])
+        else:
+            nr = np.cross(v[2] - v[1], v[0] - v[1])
+        if np.dot(nr, v.mean(axis=0)) < 0.0:
+            ver[p, : nv[p]] = v[::-1]
+    return ver, nv
+
+
+def freestreams(nrhs=8, seed=ELLIPSOID_SEED):
+    """Unit free-stream directions for the right-hand sides."""
+    rng = np.random.default_rng(seed + 1)
+    u = rng.standard_normal((nrhs, 3))
+    return u / np.linalg.norm(u, axis=1)[:, None]
+
+
+def naca0012(x):
+    """Half-thickness of the NACA 0012 section (chord 1)."""
+    return 0.6 * (0.2969 * np.sqrt(x) - 0.1260 * x - 0.3516 * x**2 + 0.2843 * x**3
+                  - 0.1015 * x**4)
+
+
+def wing_case(nc=40, ns=20, aspect=8.0, alpha_deg=5.0, u=30.0, nrows=200, seed=7):
+    """Rectangular NACA0012 wing (chord 1, span = aspect) of 2*nc*ns quads with
+    cosine chordwise spacing, pitched by alpha, and a trailing sheet of
+    nrows x (2*ns) particles spaced by dt*U = chord/20 behind the trailing
+    edge with an elliptic circulation distribution.  Core radius 0.1 chord."""
+    xc = 0.5 * (1.0 - np.cos(np.linspace(0.0, np.pi, nc + 1)))
+    ys = np.linspace(-aspect / 2, aspect / 2, ns + 1)
+    zt = naca0012(xc)
+    ca, sa = np.cos(np.radians(alpha_deg)), np.sin(np.radians(alpha_deg))
+
+    def pt(x, y, z):
+        return np.array([x * ca + z * sa, y, -x * sa + z * ca])
+
+    ver = []
+    for j in range(ns):
+        for i in range(nc):  # upper surface, trailing edge -> leading edge order
+            a, b = nc - i, nc - i - 1
+            ver.append([pt(xc[a], ys[j], zt[a]), pt(xc[b], ys[j], zt[b]),
+                        pt(xc[b], ys[j + 1], zt[b]), pt(xc[a], ys[j + 1], zt[a])])
+        for i in range(nc):  # lower surface
+            a, b = i, i + 1
+            ver.append([pt(xc[a], ys[j], -zt[a]), pt(xc[b], ys[j], -zt[b]),
+                        pt(xc[b], ys[j + 1], -zt[b]), pt(xc[a], ys[j + 1], -zt[a])])
+    ver = np.array(ver)
+    nv = np.full(len(ver), 4, np.int32)
+    # particle sheet
+    rng = np.random.default_rng(seed)
+    dx = 1.0 / 20.0
+    nsp = 2 * ns
+    yp = (np.arange(nsp) + 0.5) / nsp * aspect - aspect / 2
+    gam = np.sqrt(np.maximum(1.0 - (2.0 * yp / aspect) ** 2, 0.0))
+    dgam = -np.gradient(gam, yp)
+    cen, alpha = [], []
+    te = pt(1.0, 0.0, 0.0)
+    for r in range(nrows):
+        x = te[0] + dx * (r + 0.5)
+        for s in range(nsp):
+            cen.append([x, yp[s], te[2] + 1e-3 * rng.standard_normal()])
+            # trailing vorticity along x: -dGamma/dy * dy * dx (Gamma ~ 0.1 U c)
+            alpha.append([-0.1 * u * dgam[s] * (aspect / nsp) * dx, 0.0, 0.0])
+    cen = np.array(cen)
+    alpha = np.array(alpha)
+    alpha[:, 1] = 1e-3 * rng.standard_normal(len(cen))
+    rv = np.full(len(cen), 0.1)
+    lo = np.minimum(cen.min(axis=0), ver.reshape(-1, 3).min(axis=0)) - 0.5
+    hi = np.maximum(cen.max(axis=0), ver.reshape(-1, 3).max(axis=0)) + 0.5
+    box = float((hi - lo).max())
+    tree = dict(box_length=box, nbox=(1, 1, 1), origin=tuple(lo), nlevels=5, min_part=64,
+                delta=0.1)
+    return ver, nv, cen, alpha, rv, tree

@@ -1,0 +1,231 @@
+/*
+ * dust_gpu.h -- C ABI of libdust_gpu.so, the MI355X (gfx950) implementation of
+ * DUST's per-timestep hot path: the octree FMM evaluation of the vortex-particle
+ * wake (src/fmm) and the dense AIC assembly plus LU solve (src/linsys).
+ *
+ * Plain C types only (pointers, sizes, int status codes) so that Fortran binds
+ * it with ISO_C_BINDING and Python with ctypes.  Every function returns an int
+ * status: 0 on success, > 0 a LAPACK-style `info`, < 0 one of DUST_E_*.
+ *
+ * Array conventions follow the Fortran side: 3-vectors per particle or panel
+ * are stored as Fortran real(wp) :: x(3,n), i.e. interleaved (x1,y1,z1,x2,...);
+ * matrices are column-major with a leading dimension; pivot indices are
+ * 1-based as returned by LAPACK xGETRF.
+ *
+ * Host-pointer entry points are synchronous: outputs are on the host when the
+ * call returns.  Entry points with the _dev suffix take device pointers, are
+ * asynchronous on the handle's stream (dust_fmm_stream) and leave results in
+ * device memory.  A handle is single-threaded and not re-entrant.
+ */
+#ifndef DUST_GPU_H
+#define DUST_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define DUST_OK 0
+#define DUST_E_ARG (-1)        /* invalid argument / API misuse              */
+#define DUST_E_HIP (-2)        /* HIP runtime error                          */
+#define DUST_E_OUTSIDE (-3)    /* particle outside the octree box            */
+#define DUST_E_CAPACITY (-4)   /* size beyond what the handle was built for  */
+#define DUST_E_NOMEM (-5)      /* device allocation failed                   */
+#define DUST_E_UNSUPPORTED (-6)
+
+/* Human-readable text of the last error raised on the calling thread. */
+const char *dust_last_error(void);
+
+/* ======================================================================== */
+/* FMM: src/fmm/mod_octree.f90 + src/fmm/mod_multipole.f90                   */
+/* ======================================================================== */
+
+/* Evaluation forms (dust_fmm_params.precision). */
+#define DUST_FMM_FP64 0   /* plain FP64 path: the drop-in                    */
+#define DUST_FMM_TUNED 1  /* FP32 P2P with FP64 near pairs + GEMM M2L        */
+
+typedef struct {
+  /* initialize_octree arguments, mod_octree.f90:251-260 */
+  double box_length;     /* edge of a level-1 box                           */
+  int nbox[3];           /* level-1 boxes per direction                      */
+  double origin[3];      /* octree origin (xmin)                             */
+  int nlevels;           /* uniform tree depth                               */
+  int min_part;          /* min_octree_part                                  */
+  int degree;            /* multipole expansion order p                      */
+  double delta;          /* regularisation radius of the M2L kernel          */
+                         /* derivatives (sim_param%RankineRad, dust.f90:374)  */
+  /* sim_param flags read by the reference FMM (mod_octree.f90:1190-1408) */
+  int use_vs;            /* vortex stretching                               */
+  int use_vd;            /* PSE vortex diffusion                             */
+  int use_tv;            /* turbulent viscosity (needs use_vd)               */
+  double nu_inf;         /* kinematic viscosity                              */
+  double u_inf[3];       /* uniform free stream (variable_wind without gust) */
+  int precision;         /* DUST_FMM_FP64 or DUST_FMM_TUNED                  */
+  int reserved[7];
+} dust_fmm_params;
+
+typedef struct dust_fmm_s *dust_fmm_t;
+
+/* initialize_octree (mod_octree.f90:251): builds the per-level kernel
+ * derivative tables (compute_der_ker, mod_multipole.f90:388) and the device
+ * state.  `capacity` is the maximum particle count the handle accepts. */
+int dust_fmm_create(const dust_fmm_params *prm, int capacity, dust_fmm_t *out);
+
+/* destroy_octree (mod_octree.f90:694). */
+int dust_fmm_destroy(dust_fmm_t h);
+
+/* Particle input, the SoA image of t_vortpart (mod_vortpart.f90:85-97):
+ * cen(3,n), dir(3,n), mag(n), r_Vortex(n).  Host pointers. */
+int dust_fmm_set_particles(dust_fmm_t h, int n, const double *cen, const double *dir,
+                           const double *mag, const double *r_vortex);
+/* Same with device pointers; the data are copied on the handle's stream. */
+int dust_fmm_set_particles_dev(dust_fmm_t h, int n, const double *cen, const double *dir,
+                               const double *mag, const double *r_vortex);
+
+/* sort_particles (mod_octree.f90:703): bins the particles on the finest
+ * level, flags leaves/branches, builds the P2P lists.  Returns the number of
+ * leaves in *nleaves (may be NULL).  DUST_E_OUTSIDE if a particle lies
+ * outside the box (mod_octree.f90:757-762). */
+int dust_fmm_sort_particles(dust_fmm_t h, int *nleaves);
+
+/* calculate_multipole (mod_octree.f90:1002): P2M, M2M, M2L, L2L. */
+int dust_fmm_calculate_multipole(dust_fmm_t h);
+
+/* apply_multipole (mod_octree.f90:1139), particle terms plus free stream:
+ * L2P + P2P, writes vel(3,n), stretch(3,n) (zero unless use_vs/use_vd) and
+ * turbvisc(n) in input particle order.  Host pointers (any may be NULL). */
+int dust_fmm_apply_multipole(dust_fmm_t h, double *vel, double *stretch, double *turbvisc);
+/* Asynchronous form: results stay on the device (see dust_fmm_results_dev). */
+int dust_fmm_apply_multipole_dev(dust_fmm_t h);
+/* Device-to-device copy of the last results (input particle order). */
+int dust_fmm_results_dev(dust_fmm_t h, double *vel, double *stretch, double *turbvisc);
+
+/* One full pass (sort + multipole + apply) on device-resident particles. */
+int dust_fmm_pass_dev(dust_fmm_t h, int *nleaves);
+
+/* HIP stream (hipStream_t) the handle launches on. */
+void *dust_fmm_stream(dust_fmm_t h);
+int dust_fmm_synchronize(dust_fmm_t h);
+
+/* Leaf membership dump after sort_particles: per leaf its level (1-based),
+ * cartesian index (1-based, (3,nleaves)) and particle count; `members` gets
+ * the input indices (0-based) of each leaf's particles, leaf after leaf.
+ * Any pointer may be NULL. */
+int dust_fmm_leaves(dust_fmm_t h, int *level, int *ijk, int *npart, int *members);
+
+/* Finest-level cell index (1-based, (3,n)) of every particle, i.e.
+ * ceiling((x - xmin)/cell_size) of mod_octree.f90:755. */
+int dust_fmm_cell_index(dust_fmm_t h, int *ijk);
+
+/* Algorithmic work of the last sort: P2P pairs and active M2L pairs. */
+int dust_fmm_work(dust_fmm_t h, double *p2p_pairs, double *m2l_pairs);
+
+/* Stage timing with HIP events recorded on the handle's stream around every
+ * stage of a pass: enable, then read accumulated milliseconds and launch
+ * counts per stage (DUST_FMM_STAGE_*). */
+#define DUST_FMM_STAGE_SORT 0
+#define DUST_FMM_STAGE_P2M 1
+#define DUST_FMM_STAGE_M2M 2
+#define DUST_FMM_STAGE_M2L 3
+#define DUST_FMM_STAGE_L2L 4
+#define DUST_FMM_STAGE_P2P 5
+#define DUST_FMM_NSTAGES 6
+int dust_fmm_set_timing(dust_fmm_t h, int enable);
+int dust_fmm_stage_times(dust_fmm_t h, double *ms, int *launches);
+
+/* Direct O(N^2) evaluation of the same particle terms, the reference's
+ * fmm=F branch (mod_wake.f90:1050-1113).  Host pointers, synchronous. */
+int dust_vortpart_direct(int n, const double *cen, const double *dir, const double *mag,
+                         const double *r_vortex, int use_vs, int use_vd, double nu_inf,
+                         const double u_inf[3], double *vel, double *stretch);
+
+/* ======================================================================== */
+/* AIC assembly: src/aeroel/mod_surfpan.f90, mod_vortlatt.f90, mod_doublet   */
+/* ======================================================================== */
+
+/* Panels are given by their vertices ver(3,4,n) (triangles leave the 4th
+ * vertex unused) and n_ver(n) in {3,4}, as in calc_geo_data_surfpan
+ * (mod_surfpan.f90:890) / calc_geo_data_vortlatt (mod_vortlatt.f90:601).   */
+typedef struct {
+  double ff_ratio_doublet;  /* sim_param%FarFieldRatioDoublet (default 10) */
+  double rankine_rad;       /* sim_param%RankineRad                        */
+  double cutoff_rad;        /* sim_param%CutoffRad                         */
+  int reserved[4];
+} dust_aic_params;
+
+/* Static surface-panel system, build_row_static_surfpan with
+ * compute_pot_surfpan (mod_surfpan.f90:448-481, 579-601):
+ *   A(i,j) = -Omega_j(cen_i), A(i,i) = 2*pi;  B(i,j) = source potential.
+ * Column-major A(lda,n), B(ldb,n); B may be NULL.  Host pointers. */
+int dust_aic_surfpan(int n, const int *n_ver, const double *ver, const dust_aic_params *prm,
+                     double *A, int lda, double *B, int ldb);
+int dust_aic_surfpan_dev(int n, const int *n_ver, const double *ver,
+                         const dust_aic_params *prm, double *A, int lda, double *B, int ldb,
+                         void *stream);
+
+/* Static vortex-lattice system, build_row_static_vortlatt with
+ * compute_psi_vortlatt (mod_vortlatt.f90:201-237, 365-391):
+ *   A(i,j) = v_ring_j(cen_i).nor_i;  B(i,i) = 4*pi, else 0. */
+int dust_aic_vortlatt(int n, const int *n_ver, const double *ver, const dust_aic_params *prm,
+                      double *A, int lda, double *B, int ldb);
+
+/* Static part of the right-hand side (mod_surfpan.f90:400-404,
+ * mod_vortlatt.f90:158-164): b(i) = sum_j B(i,j) nor_j.(-u_inf - uvort_j)
+ * for nrhs free-stream vectors u_inf(3,nrhs); uvort(3,n) may be NULL.
+ * b is (ldb_out, nrhs).  Host pointers. */
+int dust_aic_rhs(int n, const int *n_ver, const double *ver, const double *B, int ldb,
+                 int nrhs, const double *u_inf, const double *uvort, double *b, int ldbo);
+
+/* ======================================================================== */
+/* LU: the LAPACK call sites of src/linsys/mod_linsys.f90 (236, 435, 473)    */
+/* ======================================================================== */
+
+/* dgetrf / dgetrs with LAPACK semantics (column-major, 1-based ipiv, info),
+ * plain FP64.  Host pointers, synchronous. */
+int dust_dgetrf(int m, int n, double *A, int lda, int *ipiv);
+int dust_dgetrs(int n, int nrhs, const double *A, int lda, const int *ipiv, double *B,
+                int ldb);
+/* Device-pointer forms on `stream` (hipStream_t, NULL = default).  info is
+ * returned through the function value after an internal synchronisation. */
+int dust_dgetrf_dev(int m, int n, double *dA, int lda, int *dipiv, void *stream);
+int dust_dgetrs_dev(int n, int nrhs, const double *dA, int lda, const int *dipiv, double *dB,
+                    int ldb, void *stream);
+
+/* Tuned mixed-precision solve of A X = B (tuned form of the same call
+ * sites): LU in `prec` (DUST_LU_FP32, DUST_LU_FP16S = FP16 with scaling)
+ * on the matrix cores, FP64 residuals and iterative refinement until
+ * ||b - A x||_inf / (||A||_inf ||x||_inf) <= tol or max_iter sweeps.
+ * A is not modified.  Host pointers.  *iters and *resid report the
+ * refinement sweeps and the final backward error (may be NULL). */
+#define DUST_LU_FP64 0
+#define DUST_LU_FP32 1
+#define DUST_LU_FP16S 2
+int dust_dgesv_mixed(int n, int nrhs, const double *A, int lda, double *B, int ldb, int prec,
+                     double tol, int max_iter, int *iters, double *resid);
+int dust_dgesv_mixed_dev(int n, int nrhs, const double *dA, int lda, double *dB, int ldb,
+                         int prec, double tol, int max_iter, int *iters, double *resid,
+                         void *stream);
+
+/* solve_linsys block path (mod_linsys.f90:352-483): the leading nstatic
+ * block of A(lda,rank) is already factored (ipiv(1:nstatic) from dgetrf);
+ * the moving rows/columns are eliminated with laswp/trsm/gemm, the moving
+ * block is factored, the pivots are made global, and res = A^-1 b is
+ * solved with one right-hand side.  Host pointers; A and ipiv are updated
+ * in place exactly as the reference leaves them. */
+int dust_solve_linsys(int rank, int nstatic, double *A, int lda, int *ipiv, const double *b,
+                      double *res);
+
+/* Test hooks (no counterpart in the reference): the building blocks of
+ * dust_dgetrf on device pointers, synchronous.  C(M,N) -= A(M,K) B(K,N) in
+ * FP64 on the matrix cores; B(m,n) <- L^-1 B with L unit lower (m x m). */
+int dust_test_gemm_sub_dev(int M, int N, int K, const double *A, int lda, const double *B,
+                           int ldb, double *C, int ldc);
+int dust_test_trsm_llnu_dev(int m, int n, const double *L, int ldl, double *B, int ldb);
+
+/* Library build/version information. */
+const char *dust_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DUST_GPU_H */

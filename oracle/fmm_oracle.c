@@ -1,0 +1,1102 @@
+/*
+ * fmm_oracle.c -- CPU restatement of DUST's Cartesian FMM for vortex particles.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP path: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ * The product library (dust_amd/libdust_gpu.so) never links or calls it.
+ *
+ * It follows the reference loops literally (FP64, same ordering of monomials,
+ * same interaction lists, same summation order of the P2P lists), with two
+ * deliberate deviations, both documented at the place they occur:
+ *   - factorials are exact doubles (the reference uses default int32, which
+ *     overflows for expansion order >= 7, src/fmm/mod_multipole.f90:152,490-494);
+ *   - leaves are enumerated deterministically (the reference collects them
+ *     under an OpenMP critical section, src/fmm/mod_octree.f90:1591-1594).
+ *
+ * Reference files (paths relative to the DUST source tree):
+ *   src/fmm/mod_multipole.f90   polynomial tables, P2M, M2M, M2L, L2L, kernel derivatives
+ *   src/fmm/mod_octree.f90      octree build, sort, upward/downward pass, L2P+P2P
+ *   src/aeroel/mod_vortpart.f90 particle kernels (Rosenhead), PSE diffusion
+ *   src/aeroel/mod_wake.f90     direct O(N^2) branch (1050-1113)
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_PI 3.1415926535897932384626433832795028841971693993751058209749445923
+/* mod_param.f90:92 defines pi = 4*atan(1) in working precision */
+
+/* ------------------------------------------------------------------ */
+/* Polynomial expansion tables: mod_multipole.f90:454-536              */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int deg;
+  int n_mon;
+  int *n_mon_d;  /* [deg+1], cumulative monomial count up to order o      */
+  int *idx;      /* [(deg+1)^3], 0-based monomial index of (i,j,k), -1 if  */
+                 /* i+j+k > deg                                            */
+  int *pwr;      /* [3*n_mon]                                              */
+  double *fact;  /* [deg+1] exact factorials (deviation: reference int32)  */
+} orc_pexp;
+
+static int pidx(const orc_pexp *p, int i, int j, int k) {
+  int D = p->deg + 1;
+  return p->idx[(i * D + j) * D + k];
+}
+
+/* set_degree_polyexp, mod_multipole.f90:454-501: graded by order o, then i
+ * descending, then j descending, k = o-i-j. */
+static void pexp_init(orc_pexp *p, int deg) {
+  int D = deg + 1;
+  p->deg = deg;
+  p->idx = (int *)malloc(sizeof(int) * D * D * D);
+  for (int q = 0; q < D * D * D; ++q) p->idx[q] = -1;
+  p->n_mon_d = (int *)malloc(sizeof(int) * D);
+  int ipol = 0;
+  for (int o = 0; o <= deg; ++o) {
+    for (int i = o; i >= 0; --i)
+      for (int j = o - i; j >= 0; --j) {
+        int k = o - j - i;
+        p->idx[(i * D + j) * D + k] = ipol;
+        ipol++;
+      }
+    p->n_mon_d[o] = ipol;
+  }
+  p->n_mon = ipol;
+  p->pwr = (int *)malloc(sizeof(int) * 3 * ipol);
+  ipol = 0;
+  for (int o = 0; o <= deg; ++o)
+    for (int i = o; i >= 0; --i)
+      for (int j = o - i; j >= 0; --j) {
+        p->pwr[3 * ipol + 0] = i;
+        p->pwr[3 * ipol + 1] = j;
+        p->pwr[3 * ipol + 2] = o - j - i;
+        ipol++;
+      }
+  p->fact = (double *)malloc(sizeof(double) * D);
+  p->fact[0] = 1.0;
+  for (int o = 1; o <= deg; ++o) p->fact[o] = (double)o * p->fact[o - 1];
+}
+
+static void pexp_free(orc_pexp *p) {
+  free(p->idx); free(p->n_mon_d); free(p->pwr); free(p->fact);
+}
+
+/* nfact_polyexp, mod_multipole.f90:524-536 */
+static double pexp_nfact(const orc_pexp *p, int i, int j, int k) {
+  return p->fact[i] * p->fact[j] * p->fact[k];
+}
+
+/* nbinom_polyexp, mod_multipole.f90:505-519 (product of 1-D binomials) */
+static double pexp_nbinom(const orc_pexp *p, const int m[3], const int s[3]) {
+  double r = 1.0;
+  for (int d = 0; d < 3; ++d)
+    r *= p->fact[m[d]] / (p->fact[s[d]] * p->fact[m[d] - s[d]]);
+  return r;
+}
+
+static double ipow(double x, int n) {
+  double r = 1.0;
+  for (int q = 0; q < n; ++q) r *= x;
+  return r;
+}
+
+/* ------------------------------------------------------------------ */
+/* Kernel derivatives: compute_der_ker, mod_multipole.f90:388-450      */
+/* D  is (3, n_mon_d(deg-1)), Dc is (3,3, n_mon_d(deg-2)), Fortran      */
+/* column-major: D[c + 3*m], Dc[c + 3*i + 9*m].                         */
+/* ------------------------------------------------------------------ */
+void orc_compute_der_ker(int deg, const double diff[3], double delta,
+                         double *D, double *Dc) {
+  orc_pexp pe;
+  pexp_init(&pe, deg);
+  double *bk = (double *)calloc(pe.n_mon, sizeof(double));
+  int nD = pe.n_mon_d[deg - 1], nDc = pe.n_mon_d[deg - 2];
+  memset(D, 0, sizeof(double) * 3 * nD);
+  memset(Dc, 0, sizeof(double) * 9 * nDc);
+  /* Rosenhead-regularised 1/(4 pi sqrt(r^2+delta^2)) */
+  double Rnorm2 = diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2] +
+                  delta * delta;
+  for (int k = 0; k <= deg; ++k)
+    for (int j = 0; j <= deg - k; ++j)
+      for (int i = 0; i <= deg - j - k; ++i) {
+        double kmod = (double)(k + j + i);
+        if (i == 0 && j == 0 && k == 0) {
+          bk[pidx(&pe, 0, 0, 0)] = 1.0 / (4.0 * ORC_PI * sqrt(Rnorm2));
+        } else {
+          double s1 = 0.0, s2 = 0.0;
+          if (i - 1 >= 0) s1 += diff[0] * bk[pidx(&pe, i - 1, j, k)];
+          if (j - 1 >= 0) s1 += diff[1] * bk[pidx(&pe, i, j - 1, k)];
+          if (k - 1 >= 0) s1 += diff[2] * bk[pidx(&pe, i, j, k - 1)];
+          if (i - 2 >= 0) s2 += bk[pidx(&pe, i - 2, j, k)];
+          if (j - 2 >= 0) s2 += bk[pidx(&pe, i, j - 2, k)];
+          if (k - 2 >= 0) s2 += bk[pidx(&pe, i, j, k - 2)];
+          bk[pidx(&pe, i, j, k)] =
+              1.0 / (Rnorm2 * kmod) * ((2.0 * kmod - 1.0) * s1 - (kmod - 1.0) * s2);
+        }
+      }
+  for (int k = 0; k <= deg - 1; ++k)
+    for (int j = 0; j <= deg - 1 - k; ++j)
+      for (int i = 0; i <= deg - 1 - j - k; ++i) {
+        int m = pidx(&pe, i, j, k);
+        D[0 + 3 * m] = -(1.0 + i) * bk[pidx(&pe, i + 1, j, k)];
+        D[1 + 3 * m] = -(1.0 + j) * bk[pidx(&pe, i, j + 1, k)];
+        D[2 + 3 * m] = -(1.0 + k) * bk[pidx(&pe, i, j, k + 1)];
+      }
+  for (int k = 0; k <= deg - 2; ++k)
+    for (int j = 0; j <= deg - 2 - k; ++j)
+      for (int i = 0; i <= deg - 2 - j - k; ++i) {
+        int m = pidx(&pe, i, j, k);
+        int mx = pidx(&pe, i + 1, j, k), my = pidx(&pe, i, j + 1, k),
+            mz = pidx(&pe, i, j, k + 1);
+        for (int c = 0; c < 3; ++c) {
+          Dc[c + 0 * 3 + 9 * m] = -(1.0 + i) * D[c + 3 * mx];
+          Dc[c + 1 * 3 + 9 * m] = -(1.0 + j) * D[c + 3 * my];
+          Dc[c + 2 * 3 + 9 * m] = -(1.0 + k) * D[c + 3 * mz];
+        }
+      }
+  free(bk);
+  pexp_free(&pe);
+}
+
+/* Exported table accessors, used by the tests to pin the ordering. */
+int orc_pexp_table(int deg, int *pwr_out /* 3*n_mon */, int *n_mon_d_out /* deg+1 */) {
+  orc_pexp pe;
+  pexp_init(&pe, deg);
+  if (pwr_out) memcpy(pwr_out, pe.pwr, sizeof(int) * 3 * pe.n_mon);
+  if (n_mon_d_out) memcpy(n_mon_d_out, pe.n_mon_d, sizeof(int) * (deg + 1));
+  int n = pe.n_mon;
+  pexp_free(&pe);
+  return n;
+}
+
+/* ------------------------------------------------------------------ */
+/* Particle kernels: mod_vortpart.f90:139-336                           */
+/* ------------------------------------------------------------------ */
+static inline void cross3(const double a[3], const double b[3], double r[3]) {
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline double norm2_3(const double a[3]) {
+  return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+}
+
+/* kernel_coeffs, mod_vortpart.f90:260-299 (Rosenhead branch) */
+static inline void kernel_coeffs(double r_vort, double rr, double *c, double *d) {
+  double distn = sqrt(rr * rr + r_vort * r_vort);
+  double d3 = distn * distn * distn;
+  *c = -1.0 / d3;
+  *d = 3.0 / (d3 * distn * distn);
+}
+
+typedef struct {
+  const double *cen, *dir, *mag, *rv; /* cen,dir: (3,n) interleaved */
+} orc_parts;
+
+/* compute_vel_vortpart, mod_vortpart.f90:139-155 */
+static inline void p_vel(const orc_parts *P, int q, const double pos[3], double v[3]) {
+  double dist[3] = {pos[0] - P->cen[3 * q], pos[1] - P->cen[3 * q + 1],
+                    pos[2] - P->cen[3 * q + 2]};
+  double c, d;
+  kernel_coeffs(P->rv[q], norm2_3(dist), &c, &d);
+  double cr[3];
+  cross3(dist, &P->dir[3 * q], cr);
+  for (int e = 0; e < 3; ++e) v[e] = c * cr[e] * P->mag[q];
+}
+
+/* compute_stretch_vortpart, mod_vortpart.f90:176-227 ("transpose" scheme) */
+static inline void p_stretch(const orc_parts *P, int q, const double pos[3],
+                             const double alpha[3], double s[3]) {
+  double dist[3] = {pos[0] - P->cen[3 * q], pos[1] - P->cen[3 * q + 1],
+                    pos[2] - P->cen[3 * q + 2]};
+  double distn = norm2_3(dist);
+  double am[3] = {P->dir[3 * q] * P->mag[q], P->dir[3 * q + 1] * P->mag[q],
+                  P->dir[3 * q + 2] * P->mag[q]};
+  double vp[3];
+  cross3(alpha, am, vp);
+  double c, d;
+  kernel_coeffs(P->rv[q], distn, &c, &d);
+  double dv = dist[0] * vp[0] + dist[1] * vp[1] + dist[2] * vp[2];
+  for (int e = 0; e < 3; ++e) s[e] = -(c * vp[e] + d * dist[e] * dv);
+}
+
+/* etaeps, mod_vortpart.f90:328-336 */
+static inline double etaeps(double dist, double eps) {
+  double eta = 105.0 / (8.0 * ORC_PI) / pow((dist / eps) * (dist / eps) + 1.0, 4.5);
+  return eta / (eps * eps * eps);
+}
+
+/* compute_diffusion_vortpart, mod_vortpart.f90:305-324 (both volumes use the
+ * source radius, as in the reference) */
+static inline void p_diff(const orc_parts *P, int q, const double pos[3],
+                          const double alpha[3], double df[3]) {
+  double dist[3] = {pos[0] - P->cen[3 * q], pos[1] - P->cen[3 * q + 1],
+                    pos[2] - P->cen[3 * q + 2]};
+  double distn = norm2_3(dist);
+  double r = P->rv[q];
+  double vol = 4.0 / 3.0 * ORC_PI * r * r * r;
+  double eta = etaeps(distn, r);
+  for (int e = 0; e < 3; ++e)
+    df[e] = 1.0 / (r * r) * (vol * P->dir[3 * q + e] * P->mag[q] - vol * alpha[e]) * eta;
+}
+
+/* Single-pair kernels exported for closed-form tests. */
+void orc_pair_kernels(const double src_cen[3], const double src_dir[3], double src_mag,
+                      double src_r, const double pos[3], const double alpha[3],
+                      double vel[3], double str[3], double dif[3]) {
+  orc_parts P = {src_cen, src_dir, &src_mag, &src_r};
+  p_vel(&P, 0, pos, vel);
+  p_stretch(&P, 0, pos, alpha, str);
+  p_diff(&P, 0, pos, alpha, dif);
+}
+
+/* ------------------------------------------------------------------ */
+/* Direct O(N^2) evaluation: mod_wake.f90:1050-1113 with the particle   */
+/* part of get_vel, plus the uniform free stream (mod_wind.f90:78).     */
+/* ------------------------------------------------------------------ */
+void orc_direct(int n, const double *cen, const double *dir, const double *mag,
+                const double *rv, int use_vs, int use_vd, double nu_inf,
+                const double uinf[3], double *vel, double *stretch) {
+  orc_parts P = {cen, dir, mag, rv};
+  const double fpi = 4.0 * ORC_PI;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int ip = 0; ip < n; ++ip) {
+    const double *pos = &cen[3 * ip];
+    double alpha[3] = {dir[3 * ip] * mag[ip], dir[3 * ip + 1] * mag[ip],
+                       dir[3 * ip + 2] * mag[ip]};
+    double v[3] = {0, 0, 0}, s[3] = {0, 0, 0}, df[3] = {0, 0, 0}, t[3];
+    for (int iq = 0; iq < n; ++iq) {
+      if (iq == ip) continue;
+      p_vel(&P, iq, pos, t);
+      for (int e = 0; e < 3; ++e) v[e] += t[e] / fpi;
+      if (use_vs) {
+        p_stretch(&P, iq, pos, alpha, t);
+        for (int e = 0; e < 3; ++e) s[e] += t[e] / fpi;
+      }
+      if (use_vd) {
+        p_diff(&P, iq, pos, alpha, t);
+        for (int e = 0; e < 3; ++e) df[e] += t[e] * nu_inf;
+      }
+    }
+    for (int e = 0; e < 3; ++e) {
+      vel[3 * ip + e] = v[e] + uinf[e];
+      if (stretch) stretch[3 * ip + e] = s[e] + df[e];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Octree: mod_octree.f90                                              */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int ncl[3];
+  int ncells;
+  double csize;
+  int *npart;     /* per cell                                          */
+  int *pstart;    /* start of the cell's particle list in `plist`       */
+  int *plist;     /* particle indices, cell lists in reference order    */
+  unsigned char *leaf, *branch, *active;
+  int *slot;      /* multipole slot of active cells, -1 otherwise       */
+  double *cen;    /* [3*ncells]                                         */
+  int *ln_start, *ln_count; /* leaf_neigh lists                         */
+  int *ln_lev, *ln_cell;
+  int kdl[3];     /* ker_der half-extent per dimension                 */
+  double *D, *Dc; /* per offset, nD*3 and nDc*9                         */
+} orc_layer;
+
+typedef struct {
+  double xmin[3], xmax[3];
+  int nbox[3], nlevels, min_part, degree;
+  double delta;
+  int use_vs, use_vd, use_tv;
+  double nu_inf, uinf[3];
+  orc_pexp pexp, pexp_der;
+  int nD, nDc;
+  double *mult;   /* [n_mon*n_mon] M2L multiplicities                   */
+  orc_layer *L;   /* [nlevels]                                          */
+  /* particles (borrowed) */
+  int n;
+  orc_parts P;
+  /* leaves */
+  int nleaves;
+  int *leaf_lev, *leaf_cell;
+  /* multipole storage */
+  int nslots;
+  double *a, *b, *c;
+} orc_octree;
+
+static inline int cidx(const orc_layer *L, int i, int j, int k) {
+  /* 1-based (i,j,k), Fortran column-major cell ordering */
+  return (i - 1) + L->ncl[0] * ((j - 1) + L->ncl[1] * (k - 1));
+}
+static inline void cijk(const orc_layer *L, int c, int *i, int *j, int *k) {
+  *i = c % L->ncl[0] + 1;
+  *j = (c / L->ncl[0]) % L->ncl[1] + 1;
+  *k = c / (L->ncl[0] * L->ncl[1]) + 1;
+}
+static inline int kd_off(const orc_layer *L, int i, int j, int k) {
+  int w0 = 2 * L->kdl[0] + 1, w1 = 2 * L->kdl[1] + 1;
+  return (i + L->kdl[0]) + w0 * ((j + L->kdl[1]) + w1 * (k + L->kdl[2]));
+}
+
+/* initialize_octree, mod_octree.f90:251-518 (the lists that the reference
+ * stores as pointers are recomputed on the fly from cell indices). */
+orc_octree *orc_octree_create(double box_length, const int nbox[3], const double origin[3],
+                              int nlevels, int min_part, int degree, double delta,
+                              int use_vs, int use_vd, int use_tv, double nu_inf,
+                              const double uinf[3]) {
+  orc_octree *T = (orc_octree *)calloc(1, sizeof(orc_octree));
+  for (int d = 0; d < 3; ++d) {
+    T->xmin[d] = origin[d];
+    T->xmax[d] = origin[d] + (double)nbox[d] * box_length;
+    T->nbox[d] = nbox[d];
+    T->uinf[d] = uinf ? uinf[d] : 0.0;
+  }
+  T->nlevels = nlevels; T->min_part = min_part; T->degree = degree; T->delta = delta;
+  T->use_vs = use_vs; T->use_vd = use_vd; T->use_tv = use_tv; T->nu_inf = nu_inf;
+  pexp_init(&T->pexp, degree);
+  pexp_init(&T->pexp_der, 2 * degree + 2);
+  T->nD = T->pexp_der.n_mon_d[T->pexp_der.deg - 1];
+  T->nDc = T->pexp_der.n_mon_d[T->pexp_der.deg - 2];
+  int nm = T->pexp.n_mon;
+  T->mult = (double *)malloc(sizeof(double) * nm * nm);
+  for (int m = 0; m < nm; ++m)
+    for (int n = 0; n < nm; ++n) {
+      const int *pm = &T->pexp.pwr[3 * m], *pn = &T->pexp.pwr[3 * n];
+      /* mod_multipole.f90:304-306 */
+      T->mult[m * nm + n] =
+          pexp_nfact(&T->pexp_der, pm[0] + pn[0], pm[1] + pn[1], pm[2] + pn[2]) /
+          (pexp_nfact(&T->pexp, pm[0], pm[1], pm[2]) * pexp_nfact(&T->pexp, pn[0], pn[1], pn[2]));
+    }
+  T->L = (orc_layer *)calloc(nlevels, sizeof(orc_layer));
+  for (int l = 0; l < nlevels; ++l) {
+    orc_layer *L = &T->L[l];
+    for (int d = 0; d < 3; ++d) L->ncl[d] = nbox[d] << l;
+    L->ncells = L->ncl[0] * L->ncl[1] * L->ncl[2];
+    L->csize = box_length / (double)(1 << l); /* mod_octree.f90:324,340 */
+    L->npart = (int *)calloc(L->ncells, sizeof(int));
+    L->pstart = (int *)calloc(L->ncells, sizeof(int));
+    L->leaf = (unsigned char *)calloc(L->ncells, 1);
+    L->branch = (unsigned char *)calloc(L->ncells, 1);
+    L->active = (unsigned char *)calloc(L->ncells, 1);
+    L->slot = (int *)malloc(sizeof(int) * L->ncells);
+    L->cen = (double *)malloc(sizeof(double) * 3 * L->ncells);
+    L->ln_start = (int *)calloc(L->ncells, sizeof(int));
+    L->ln_count = (int *)calloc(L->ncells, sizeof(int));
+    for (int c = 0; c < L->ncells; ++c) {
+      int ijk[3];
+      cijk(L, c, &ijk[0], &ijk[1], &ijk[2]);
+      for (int d = 0; d < 3; ++d) /* mod_octree.f90:332-334,363-366 */
+        L->cen[3 * c + d] = T->xmin[d] + ((double)ijk[d] - 0.5) * (T->xmax[d] - T->xmin[d]) /
+                                             (double)(nbox[d] << l);
+    }
+    /* kernel derivatives, mod_octree.f90:486-516 */
+    for (int d = 0; d < 3; ++d) L->kdl[d] = (l == 0) ? (nbox[d] > 3 ? nbox[d] : 3) : 3;
+    int noff = (2 * L->kdl[0] + 1) * (2 * L->kdl[1] + 1) * (2 * L->kdl[2] + 1);
+    L->D = (double *)calloc((size_t)noff * 3 * T->nD, sizeof(double));
+    L->Dc = (double *)calloc((size_t)noff * 9 * T->nDc, sizeof(double));
+#pragma omp parallel for collapse(3) schedule(dynamic)
+    for (int k = -L->kdl[2]; k <= L->kdl[2]; ++k)
+      for (int j = -L->kdl[1]; j <= L->kdl[1]; ++j)
+        for (int i = -L->kdl[0]; i <= L->kdl[0]; ++i) {
+          if (abs(i) >= 2 || abs(j) >= 2 || abs(k) >= 2) {
+            double diff[3] = {L->csize * (double)(-i), L->csize * (double)(-j),
+                              L->csize * (double)(-k)};
+            int o = kd_off(L, i, j, k);
+            orc_compute_der_ker(T->pexp_der.deg, diff, delta, &L->D[(size_t)o * 3 * T->nD],
+                                &L->Dc[(size_t)o * 9 * T->nDc]);
+          }
+        }
+  }
+  return T;
+}
+
+static void free_lists(orc_octree *T) {
+  for (int l = 0; l < T->nlevels; ++l) {
+    free(T->L[l].plist); T->L[l].plist = NULL;
+    free(T->L[l].ln_lev); T->L[l].ln_lev = NULL;
+    free(T->L[l].ln_cell); T->L[l].ln_cell = NULL;
+  }
+  free(T->leaf_lev); T->leaf_lev = NULL;
+  free(T->leaf_cell); T->leaf_cell = NULL;
+  free(T->a); free(T->b); free(T->c);
+  T->a = T->b = T->c = NULL;
+}
+
+void orc_octree_destroy(orc_octree *T) {
+  if (!T) return;
+  free_lists(T);
+  for (int l = 0; l < T->nlevels; ++l) {
+    orc_layer *L = &T->L[l];
+    free(L->npart); free(L->pstart); free(L->leaf); free(L->branch); free(L->active);
+    free(L->slot); free(L->cen); free(L->ln_start); free(L->ln_count); free(L->D); free(L->Dc);
+  }
+  free(T->L); free(T->mult);
+  pexp_free(&T->pexp); pexp_free(&T->pexp_der);
+  free(T);
+}
+
+/* child p (0..7) of cell (i,j,k) at level l: mod_octree.f90:349-361, ic fastest */
+static inline int child_of(const orc_layer *Lc, int i, int j, int k, int p) {
+  int ic = p & 1, jc = (p >> 1) & 1, kc = (p >> 2) & 1;
+  return cidx(Lc, 2 * (i - 1) + ic + 1, 2 * (j - 1) + jc + 1, 2 * (k - 1) + kc + 1);
+}
+
+/* sort_particles, mod_octree.f90:703-996 (no panels, no redistribution).
+ * Returns 0, or -1 (with *bad set to the particle index) when a particle
+ * falls outside the octree box (mod_octree.f90:757-762). */
+int orc_sort_particles(orc_octree *T, int n, const double *cen, const double *dir,
+                       const double *mag, const double *rv, int *bad) {
+  free_lists(T);
+  T->n = n;
+  T->P.cen = cen; T->P.dir = dir; T->P.mag = mag; T->P.rv = rv;
+  int ll = T->nlevels;
+  orc_layer *LL = &T->L[ll - 1];
+  double csize = LL->csize;
+  for (int l = 0; l < ll; ++l) {
+    orc_layer *L = &T->L[l];
+    memset(L->npart, 0, sizeof(int) * L->ncells);
+    memset(L->leaf, 0, L->ncells);
+    memset(L->branch, 0, L->ncells);
+    memset(L->ln_count, 0, sizeof(int) * L->ncells);
+    /* the reference keeps `active` across calls; cells are re-flagged below */
+    memset(L->active, 0, L->ncells);
+  }
+  /* binning at the finest level, serial in input order (746-773) */
+  int *pc = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+  for (int ip = 0; ip < n; ++ip) {
+    int id[3];
+    for (int d = 0; d < 3; ++d)
+      id[d] = (int)ceil((cen[3 * ip + d] - T->xmin[d]) / csize);
+    if (id[0] <= 0 || id[1] <= 0 || id[2] <= 0 || id[0] > LL->ncl[0] || id[1] > LL->ncl[1] ||
+        id[2] > LL->ncl[2]) {
+      if (bad) *bad = ip;
+      free(pc);
+      return -1;
+    }
+    pc[ip] = cidx(LL, id[0], id[1], id[2]);
+    LL->npart[pc[ip]]++;
+  }
+  /* finest-level lists: stable counting sort = push order of the reference */
+  {
+    int s = 0;
+    for (int c = 0; c < LL->ncells; ++c) { LL->pstart[c] = s; s += LL->npart[c]; }
+    LL->plist = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+    int *fill = (int *)calloc(LL->ncells, sizeof(int));
+    for (int ip = 0; ip < n; ++ip) {
+      int c = pc[ip];
+      LL->plist[LL->pstart[c] + fill[c]++] = ip;
+    }
+    free(fill);
+  }
+  free(pc);
+  /* leaves list capacity: every cell could be a leaf */
+  int cap = 0;
+  for (int l = 0; l < ll; ++l) cap += T->L[l].ncells;
+  T->leaf_lev = (int *)malloc(sizeof(int) * cap);
+  T->leaf_cell = (int *)malloc(sizeof(int) * cap);
+  int nl = 0;
+  /* finest level leaves (898-907) */
+  for (int c = 0; c < LL->ncells; ++c)
+    if (LL->npart[c] >= T->min_part || ll == 1) {
+      LL->leaf[c] = 1; LL->active[c] = 1;
+      T->leaf_lev[nl] = ll - 1; T->leaf_cell[nl] = c; nl++;
+    }
+  /* upwards (911-961) */
+  for (int l = ll - 2; l >= 0; --l) {
+    orc_layer *L = &T->L[l], *Lc = &T->L[l + 1];
+    /* counts and concatenated child lists */
+    int s = 0;
+    for (int c = 0; c < L->ncells; ++c) {
+      int i, j, k;
+      cijk(L, c, &i, &j, &k);
+      L->pstart[c] = s;
+      for (int p = 0; p < 8; ++p) L->npart[c] += Lc->npart[child_of(Lc, i, j, k, p)];
+      s += L->npart[c];
+    }
+    L->plist = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+    for (int c = 0; c < L->ncells; ++c) {
+      int i, j, k;
+      cijk(L, c, &i, &j, &k);
+      int w = L->pstart[c];
+      int got_leaves = 0, got_branches = 0;
+      for (int p = 0; p < 8; ++p) {
+        int ch = child_of(Lc, i, j, k, p);
+        if (Lc->leaf[ch]) got_leaves = 1;
+        if (Lc->branch[ch]) got_branches = 1;
+        memcpy(&L->plist[w], &Lc->plist[Lc->pstart[ch]], sizeof(int) * Lc->npart[ch]);
+        w += Lc->npart[ch];
+      }
+      if (got_leaves || got_branches) {
+        for (int p = 0; p < 8; ++p) {
+          int ch = child_of(Lc, i, j, k, p);
+          if (!Lc->leaf[ch] && !Lc->branch[ch]) {
+            Lc->leaf[ch] = 1; Lc->active[ch] = 1;
+            T->leaf_lev[nl] = l + 1; T->leaf_cell[nl] = ch; nl++;
+          }
+        }
+        L->branch[c] = 1; L->active[c] = 1;
+      } else {
+        for (int p = 0; p < 8; ++p) {
+          int ch = child_of(Lc, i, j, k, p);
+          Lc->branch[ch] = 0; Lc->leaf[ch] = 0; Lc->active[ch] = 0;
+        }
+        if (L->npart[c] >= T->min_part || l == 0) {
+          L->leaf[c] = 1; L->active[c] = 1;
+          T->leaf_lev[nl] = l; T->leaf_cell[nl] = c; nl++;
+        }
+      }
+    }
+  }
+  /* Drop leaves that were later de-activated (a finest cell flagged leaf and
+   * then reset by set_inactive cannot happen: a leaf child makes the parent a
+   * branch).  Keep the list as built. */
+  T->nleaves = nl;
+  /* leaf_neigh, 964-988: parent's list + parent's neighbours that are leaves */
+  for (int l = 1; l < ll; ++l) {
+    orc_layer *L = &T->L[l], *Lp = &T->L[l - 1];
+    int tot = 0;
+    for (int c = 0; c < L->ncells; ++c) {
+      if (!L->active[c]) continue;
+      int i, j, k;
+      cijk(L, c, &i, &j, &k);
+      int pi_ = (i + 1) / 2, pj = (j + 1) / 2, pk = (k + 1) / 2;
+      int pcell = cidx(Lp, pi_, pj, pk);
+      int cnt = Lp->ln_count[pcell];
+      for (int kc = -1; kc <= 1; ++kc)
+        for (int jc = -1; jc <= 1; ++jc)
+          for (int ic = -1; ic <= 1; ++ic) {
+            if (!ic && !jc && !kc) continue;
+            int a = pi_ + ic, b = pj + jc, d = pk + kc;
+            if (a < 1 || b < 1 || d < 1 || a > Lp->ncl[0] || b > Lp->ncl[1] || d > Lp->ncl[2])
+              continue;
+            if (Lp->leaf[cidx(Lp, a, b, d)]) cnt++;
+          }
+      L->ln_count[c] = cnt;
+      L->ln_start[c] = tot;
+      tot += cnt;
+    }
+    L->ln_lev = (int *)malloc(sizeof(int) * (tot > 0 ? tot : 1));
+    L->ln_cell = (int *)malloc(sizeof(int) * (tot > 0 ? tot : 1));
+    for (int c = 0; c < L->ncells; ++c) {
+      if (!L->active[c]) continue;
+      int i, j, k;
+      cijk(L, c, &i, &j, &k);
+      int pi_ = (i + 1) / 2, pj = (j + 1) / 2, pk = (k + 1) / 2;
+      int pcell = cidx(Lp, pi_, pj, pk);
+      int w = L->ln_start[c];
+      if (l >= 2) {
+        for (int q = 0; q < Lp->ln_count[pcell]; ++q) {
+          L->ln_lev[w] = Lp->ln_lev[Lp->ln_start[pcell] + q];
+          L->ln_cell[w] = Lp->ln_cell[Lp->ln_start[pcell] + q];
+          w++;
+        }
+      }
+      for (int kc = -1; kc <= 1; ++kc)
+        for (int jc = -1; jc <= 1; ++jc)
+          for (int ic = -1; ic <= 1; ++ic) {
+            if (!ic && !jc && !kc) continue;
+            int a = pi_ + ic, b = pj + jc, d = pk + kc;
+            if (a < 1 || b < 1 || d < 1 || a > Lp->ncl[0] || b > Lp->ncl[1] || d > Lp->ncl[2])
+              continue;
+            int nc = cidx(Lp, a, b, d);
+            if (Lp->leaf[nc]) { L->ln_lev[w] = l - 1; L->ln_cell[w] = nc; w++; }
+          }
+    }
+  }
+  /* multipole slots for the active cells */
+  int ns = 0;
+  for (int l = 0; l < ll; ++l)
+    for (int c = 0; c < T->L[l].ncells; ++c) T->L[l].slot[c] = T->L[l].active[c] ? ns++ : -1;
+  T->nslots = ns;
+  int nm = T->pexp.n_mon;
+  T->a = (double *)calloc((size_t)ns * 3 * nm + 1, sizeof(double));
+  T->b = (double *)calloc((size_t)ns * 3 * nm + 1, sizeof(double));
+  T->c = (double *)calloc((size_t)ns * 9 * nm + 1, sizeof(double));
+  return 0;
+}
+
+/* leaf_M_multipole, mod_multipole.f90:224-251 */
+static void p2m(const orc_octree *T, const double cen[3], const int *plist, int np, double *a) {
+  int nm = T->pexp.n_mon;
+  memset(a, 0, sizeof(double) * 3 * nm);
+  for (int m = 0; m < nm; ++m) {
+    const int *pw = &T->pexp.pwr[3 * m];
+    for (int q = 0; q < np; ++q) {
+      int ip = plist[q];
+      double pr = ipow(T->P.cen[3 * ip] - cen[0], pw[0]) *
+                  ipow(T->P.cen[3 * ip + 1] - cen[1], pw[1]) *
+                  ipow(T->P.cen[3 * ip + 2] - cen[2], pw[2]);
+      double f = T->P.mag[ip] * pr;
+      for (int e = 0; e < 3; ++e) a[e + 3 * m] += f * T->P.dir[3 * ip + e];
+    }
+  }
+}
+
+/* M2M_multipole, mod_multipole.f90:256-278 */
+static void m2m(const orc_octree *T, const double cen[3], const double *ach,
+                const double ccen[3], double *a) {
+  const orc_pexp *pe = &T->pexp;
+  for (int m = 0; m < pe->n_mon; ++m) {
+    const int *id = &pe->pwr[3 * m];
+    for (int ks = 0; ks <= id[2]; ++ks)
+      for (int js = 0; js <= id[1]; ++js)
+        for (int is = 0; is <= id[0]; ++is) {
+          int s = pidx(pe, is, js, ks);
+          int sv[3] = {is, js, ks};
+          double f = pexp_nbinom(pe, id, sv) * (ipow(ccen[0] - cen[0], id[0] - is) *
+                                                ipow(ccen[1] - cen[1], id[1] - js) *
+                                                ipow(ccen[2] - cen[2], id[2] - ks));
+          for (int e = 0; e < 3; ++e) a[e + 3 * m] += f * ach[e + 3 * s];
+        }
+  }
+}
+
+/* M2L_multipole, mod_multipole.f90:282-356 */
+static void m2l(const orc_octree *T, const double *D, const double *Dc, const double *as,
+                double *b, double *c) {
+  const orc_pexp *pe = &T->pexp, *pd = &T->pexp_der;
+  int nm = pe->n_mon;
+  for (int m = 0; m < nm; ++m) {
+    double sv[3] = {0, 0, 0}, sg[9] = {0};
+    const int *pm = &pe->pwr[3 * m];
+    for (int n = 0; n < nm; ++n) {
+      const int *pn = &pe->pwr[3 * n];
+      int id = pidx(pd, pm[0] + pn[0], pm[1] + pn[1], pm[2] + pn[2]);
+      double mult = T->mult[m * nm + n];
+      double cr[3];
+      cross3(&D[3 * id], &as[3 * n], cr);
+      for (int e = 0; e < 3; ++e) sv[e] += mult * cr[e];
+      if (T->use_vs) {
+        for (int i = 0; i < 3; ++i) {
+          /* Der = Dc(:,i,idx_der); sum_g(i,:) += mult*cross(Der, a) */
+          cross3(&Dc[3 * i + 9 * id], &as[3 * n], cr);
+          for (int e = 0; e < 3; ++e) sg[i + 3 * e] += mult * cr[e];
+        }
+      }
+    }
+    double sgn = ((pm[0] + pm[1] + pm[2]) & 1) ? -1.0 : 1.0;
+    for (int e = 0; e < 3; ++e) b[e + 3 * m] += sgn * sv[e];
+    if (T->use_vs)
+      for (int q = 0; q < 9; ++q) c[q + 9 * m] += sgn * sg[q];
+  }
+}
+
+/* L2L_multipole, mod_multipole.f90:360-384 */
+static void l2l(const orc_octree *T, const double cen[3], const double *bp, const double *cp,
+                const double pcen[3], double *b, double *c) {
+  const orc_pexp *pe = &T->pexp;
+  int deg = pe->deg;
+  for (int m = 0; m < pe->n_mon; ++m) {
+    const int *id = &pe->pwr[3 * m];
+    for (int ks = id[2]; ks <= deg; ++ks)
+      for (int js = id[1]; js <= deg - ks; ++js)
+        for (int is = id[0]; is <= deg - ks - js; ++is) {
+          int s = pidx(pe, is, js, ks);
+          int sv[3] = {is, js, ks};
+          double f = pexp_nbinom(pe, sv, id) * (ipow(cen[0] - pcen[0], is - id[0]) *
+                                                ipow(cen[1] - pcen[1], js - id[1]) *
+                                                ipow(cen[2] - pcen[2], ks - id[2]));
+          for (int e = 0; e < 3; ++e) b[e + 3 * m] += f * bp[e + 3 * s];
+          if (T->use_vs)
+            for (int q = 0; q < 9; ++q) c[q + 9 * m] += f * cp[q + 9 * s];
+        }
+  }
+}
+
+/* calculate_multipole, mod_octree.f90:1002-1133 */
+void orc_calculate_multipole(orc_octree *T) {
+  int ll = T->nlevels, nm = T->pexp.n_mon;
+  /* P2M on all leaves */
+#pragma omp parallel for schedule(guided)
+  for (int lv = 0; lv < T->nleaves; ++lv) {
+    orc_layer *L = &T->L[T->leaf_lev[lv]];
+    int c = T->leaf_cell[lv];
+    p2m(T, &L->cen[3 * c], &L->plist[L->pstart[c]], L->npart[c], &T->a[(size_t)L->slot[c] * 3 * nm]);
+  }
+  /* M2M bottom-up */
+  for (int l = ll - 2; l >= 0; --l) {
+    orc_layer *L = &T->L[l], *Lc = &T->L[l + 1];
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int c = 0; c < L->ncells; ++c) {
+      if (!(L->active[c] && L->branch[c])) continue;
+      int i, j, k;
+      cijk(L, c, &i, &j, &k);
+      for (int p = 0; p < 8; ++p) {
+        int ch = child_of(Lc, i, j, k, p);
+        m2m(T, &L->cen[3 * c], &T->a[(size_t)Lc->slot[ch] * 3 * nm], &Lc->cen[3 * ch],
+            &T->a[(size_t)L->slot[c] * 3 * nm]);
+      }
+    }
+  }
+  /* top-down: M2L over the interaction list, then L2L */
+  for (int l = 0; l < ll; ++l) {
+    orc_layer *L = &T->L[l];
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int c = 0; c < L->ncells; ++c) {
+      if (!L->active[c]) continue;
+      int i, j, k;
+      cijk(L, c, &i, &j, &k);
+      double *b = &T->b[(size_t)L->slot[c] * 3 * nm];
+      double *cc = &T->c[(size_t)L->slot[c] * 9 * nm];
+      if (l == 0) {
+        /* level 1: all non-neighbouring level-1 cells, mod_octree.f90:422-435 */
+        for (int kc = 1; kc <= L->ncl[2]; ++kc)
+          for (int jc = 1; jc <= L->ncl[1]; ++jc)
+            for (int ic = 1; ic <= L->ncl[0]; ++ic) {
+              int di = ic - i, dj = jc - j, dk = kc - k;
+              if (abs(di) > 1 || abs(dj) > 1 || abs(dk) > 1) {
+                int s = cidx(L, ic, jc, kc);
+                if (!L->active[s]) continue;
+                int o = kd_off(L, di, dj, dk);
+                m2l(T, &L->D[(size_t)o * 3 * T->nD], &L->Dc[(size_t)o * 9 * T->nDc],
+                    &T->a[(size_t)L->slot[s] * 3 * nm], b, cc);
+              }
+            }
+      } else {
+        /* children of the parent's neighbours that are not neighbours,
+         * mod_octree.f90:441-476 */
+        orc_layer *Lp = &T->L[l - 1];
+        int pi_ = (i + 1) / 2, pj = (j + 1) / 2, pk = (k + 1) / 2;
+        for (int kc = -1; kc <= 1; ++kc)
+          for (int jc = -1; jc <= 1; ++jc)
+            for (int ic = -1; ic <= 1; ++ic) {
+              if (!ic && !jc && !kc) continue;
+              int a = pi_ + ic, bb = pj + jc, d = pk + kc;
+              if (a < 1 || bb < 1 || d < 1 || a > Lp->ncl[0] || bb > Lp->ncl[1] || d > Lp->ncl[2])
+                continue;
+              for (int p = 0; p < 8; ++p) {
+                int s = child_of(L, a, bb, d, p);
+                int si, sj, sk;
+                cijk(L, s, &si, &sj, &sk);
+                int di = si - i, dj = sj - j, dk = sk - k;
+                if (abs(di) > 1 || abs(dj) > 1 || abs(dk) > 1) {
+                  if (!L->active[s]) continue;
+                  int o = kd_off(L, di, dj, dk);
+                  m2l(T, &L->D[(size_t)o * 3 * T->nD], &L->Dc[(size_t)o * 9 * T->nDc],
+                      &T->a[(size_t)L->slot[s] * 3 * nm], b, cc);
+                }
+              }
+            }
+      }
+    }
+    if (l < ll - 1) {
+      orc_layer *Lc = &T->L[l + 1];
+#pragma omp parallel for schedule(dynamic, 16)
+      for (int c = 0; c < L->ncells; ++c) {
+        if (!L->active[c]) continue;
+        int i, j, k;
+        cijk(L, c, &i, &j, &k);
+        for (int p = 0; p < 8; ++p) {
+          int ch = child_of(Lc, i, j, k, p);
+          if (!Lc->active[ch]) continue;
+          l2l(T, &Lc->cen[3 * ch], &T->b[(size_t)L->slot[c] * 3 * nm],
+              &T->c[(size_t)L->slot[c] * 9 * nm], &L->cen[3 * c],
+              &T->b[(size_t)Lc->slot[ch] * 3 * nm], &T->c[(size_t)Lc->slot[ch] * 9 * nm]);
+        }
+      }
+    }
+  }
+}
+
+/* P2P of target particle ip against the particle list of one cell. */
+static inline void p2p_list(const orc_octree *T, const int *plist, int np, int skip,
+                            const double pos[3], const double alpha[3], double tv,
+                            double vel[3], double str[3]) {
+  const double fpi = 4.0 * ORC_PI;
+  double t[3];
+  for (int q = 0; q < np; ++q) {
+    int iq = plist[q];
+    if (iq == skip) continue;
+    p_vel(&T->P, iq, pos, t);
+    for (int e = 0; e < 3; ++e) vel[e] += t[e] / fpi;
+    if (T->use_vs) {
+      p_stretch(&T->P, iq, pos, alpha, t);
+      for (int e = 0; e < 3; ++e) str[e] += t[e] / fpi;
+    }
+    if (T->use_vd) {
+      p_diff(&T->P, iq, pos, alpha, t);
+      for (int e = 0; e < 3; ++e) str[e] += t[e] * (T->nu_inf + tv);
+    }
+  }
+}
+
+/* apply_multipole, mod_octree.f90:1139-1429, particle terms only (the body
+ * and wake-element terms are a separate entry point) plus the uniform free
+ * stream.  Output in input particle order. */
+void orc_apply_multipole(orc_octree *T, double *vel_out, double *str_out, double *tv_out) {
+  int nm = T->pexp.n_mon;
+#pragma omp parallel for schedule(dynamic)
+  for (int lv = 0; lv < T->nleaves; ++lv) {
+    int l = T->leaf_lev[lv], c = T->leaf_cell[lv];
+    orc_layer *L = &T->L[l];
+    int np = L->npart[c];
+    const int *pl = &L->plist[L->pstart[c]];
+    const double *b = &T->b[(size_t)L->slot[c] * 3 * nm];
+    const double *cc = &T->c[(size_t)L->slot[c] * 9 * nm];
+    const double *lc = &L->cen[3 * c];
+    double *velv = (double *)malloc(sizeof(double) * 3 * (np + 1));
+    double *strv = (double *)calloc(3 * (np + 1), sizeof(double));
+    double ave_ros = 0.0;
+    /* L2P, 1171-1219 */
+    for (int q = 0; q < np; ++q) {
+      int ip = pl[q];
+      const double *pos = &T->P.cen[3 * ip];
+      double alpha[3] = {T->P.mag[ip] * T->P.dir[3 * ip], T->P.mag[ip] * T->P.dir[3 * ip + 1],
+                         T->P.mag[ip] * T->P.dir[3 * ip + 2]};
+      double v[3] = {0, 0, 0}, g[9] = {0};
+      for (int m = 0; m < nm; ++m) {
+        const int *pw = &T->pexp.pwr[3 * m];
+        double pr = ipow(pos[0] - lc[0], pw[0]) * ipow(pos[1] - lc[1], pw[1]) *
+                    ipow(pos[2] - lc[2], pw[2]);
+        for (int e = 0; e < 3; ++e) v[e] += b[e + 3 * m] * pr;
+        if (T->use_vs)
+          for (int e = 0; e < 9; ++e) g[e] += cc[e + 9 * m] * pr;
+      }
+      for (int e = 0; e < 3; ++e) velv[3 * q + e] = v[e];
+      if (T->use_vs) {
+        /* str = matmul(alpha, transpose(grad)) : str_j = sum_i grad(j,i) alpha_i */
+        for (int jj = 0; jj < 3; ++jj)
+          strv[3 * q + jj] = g[jj + 0] * alpha[0] + g[jj + 3] * alpha[1] + g[jj + 6] * alpha[2];
+      }
+      if (T->use_vd) {
+        double s = 0.0;
+        for (int ii = 0; ii < 3; ++ii)
+          for (int jj = 0; jj < 3; ++jj) {
+            double e = 0.5 * (g[ii + 3 * jj] + g[jj + 3 * ii]);
+            s += e * e;
+          }
+        ave_ros += s;
+      }
+    }
+    if (T->use_vd && np > 0) ave_ros /= (double)np;
+    /* P2P, 1234-1348 */
+    for (int q = 0; q < np; ++q) {
+      int ip = pl[q];
+      const double *pos = &T->P.cen[3 * ip];
+      double alpha[3] = {T->P.mag[ip] * T->P.dir[3 * ip], T->P.mag[ip] * T->P.dir[3 * ip + 1],
+                         T->P.mag[ip] * T->P.dir[3 * ip + 2]};
+      double v[3] = {velv[3 * q], velv[3 * q + 1], velv[3 * q + 2]};
+      double s[3] = {strv[3 * q], strv[3 * q + 1], strv[3 * q + 2]};
+      double tv = 0.0;
+      if (T->use_tv) {
+        double rr = 0.11 * T->P.rv[ip];
+        tv = rr * rr * sqrt(2.0 * ave_ros);
+      }
+      int i, j, k;
+      cijk(L, c, &i, &j, &k);
+      for (int kk = -1; kk <= 1; ++kk)
+        for (int jj = -1; jj <= 1; ++jj)
+          for (int ii = -1; ii <= 1; ++ii) {
+            if (!ii && !jj && !kk) continue;
+            int a = i + ii, bb = j + jj, d = k + kk;
+            if (a < 1 || bb < 1 || d < 1 || a > L->ncl[0] || bb > L->ncl[1] || d > L->ncl[2])
+              continue;
+            int nc = cidx(L, a, bb, d);
+            if (!L->active[nc]) continue;
+            p2p_list(T, &L->plist[L->pstart[nc]], L->npart[nc], -1, pos, alpha, tv, v, s);
+          }
+      for (int q2 = 0; q2 < L->ln_count[c]; ++q2) {
+        orc_layer *Ln = &T->L[L->ln_lev[L->ln_start[c] + q2]];
+        int nc = L->ln_cell[L->ln_start[c] + q2];
+        p2p_list(T, &Ln->plist[Ln->pstart[nc]], Ln->npart[nc], -1, pos, alpha, tv, v, s);
+      }
+      p2p_list(T, pl, np, ip, pos, alpha, tv, v, s);
+      for (int e = 0; e < 3; ++e) {
+        vel_out[3 * ip + e] = v[e] + T->uinf[e];
+        if (str_out) str_out[3 * ip + e] = s[e];
+      }
+      if (tv_out) tv_out[ip] = tv;
+    }
+    free(velv);
+    free(strv);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Introspection used by the tests                                     */
+/* ------------------------------------------------------------------ */
+int orc_octree_nleaves(const orc_octree *T) { return T->nleaves; }
+
+/* Leaf membership dump: for each leaf its level (1-based), its cartesian
+ * index (1-based) and its particle count; particle lists concatenated. */
+void orc_octree_leaves(const orc_octree *T, int *lev, int *ijk, int *npart, int *pstart,
+                       int *plist) {
+  int w = 0;
+  for (int lv = 0; lv < T->nleaves; ++lv) {
+    const orc_layer *L = &T->L[T->leaf_lev[lv]];
+    int c = T->leaf_cell[lv];
+    if (lev) lev[lv] = T->leaf_lev[lv] + 1;
+    if (ijk) cijk(L, c, &ijk[3 * lv], &ijk[3 * lv + 1], &ijk[3 * lv + 2]);
+    if (npart) npart[lv] = L->npart[c];
+    if (pstart) pstart[lv] = w;
+    if (plist) memcpy(&plist[w], &L->plist[L->pstart[c]], sizeof(int) * L->npart[c]);
+    w += L->npart[c];
+  }
+}
+
+/* Count of P2P pairs and of active M2L pairs, the algorithmic work units. */
+void orc_octree_work(const orc_octree *T, double *p2p_pairs, double *m2l_pairs) {
+  double pp = 0, mp = 0;
+  for (int lv = 0; lv < T->nleaves; ++lv) {
+    int l = T->leaf_lev[lv], c = T->leaf_cell[lv];
+    const orc_layer *L = &T->L[l];
+    double src = 0;
+    int i, j, k;
+    cijk(L, c, &i, &j, &k);
+    for (int kk = -1; kk <= 1; ++kk)
+      for (int jj = -1; jj <= 1; ++jj)
+        for (int ii = -1; ii <= 1; ++ii) {
+          if (!ii && !jj && !kk) continue;
+          int a = i + ii, bb = j + jj, d = k + kk;
+          if (a < 1 || bb < 1 || d < 1 || a > L->ncl[0] || bb > L->ncl[1] || d > L->ncl[2]) continue;
+          int nc = cidx(L, a, bb, d);
+          if (L->active[nc]) src += L->npart[nc];
+        }
+    for (int q2 = 0; q2 < L->ln_count[c]; ++q2) {
+      const orc_layer *Ln = &T->L[L->ln_lev[L->ln_start[c] + q2]];
+      src += Ln->npart[L->ln_cell[L->ln_start[c] + q2]];
+    }
+    double np = L->npart[c];
+    pp += np * (src + np - 1.0);
+  }
+  for (int l = 0; l < T->nlevels; ++l) {
+    const orc_layer *L = &T->L[l];
+    for (int c = 0; c < L->ncells; ++c) {
+      if (!L->active[c]) continue;
+      int i, j, k;
+      cijk(L, c, &i, &j, &k);
+      if (l == 0) {
+        for (int s = 0; s < L->ncells; ++s) {
+          int si, sj, sk;
+          cijk(L, s, &si, &sj, &sk);
+          if ((abs(si - i) > 1 || abs(sj - j) > 1 || abs(sk - k) > 1) && L->active[s]) mp += 1;
+        }
+      } else {
+        const orc_layer *Lp = &T->L[l - 1];
+        int pi_ = (i + 1) / 2, pj = (j + 1) / 2, pk = (k + 1) / 2;
+        for (int kc = -1; kc <= 1; ++kc)
+          for (int jc = -1; jc <= 1; ++jc)
+            for (int ic = -1; ic <= 1; ++ic) {
+              if (!ic && !jc && !kc) continue;
+              int a = pi_ + ic, bb = pj + jc, d = pk + kc;
+              if (a < 1 || bb < 1 || d < 1 || a > Lp->ncl[0] || bb > Lp->ncl[1] || d > Lp->ncl[2])
+                continue;
+              for (int p = 0; p < 8; ++p) {
+                int s = child_of(L, a, bb, d, p);
+                int si, sj, sk;
+                cijk(L, s, &si, &sj, &sk);
+                if ((abs(si - i) > 1 || abs(sj - j) > 1 || abs(sk - k) > 1) && L->active[s]) mp += 1;
+              }
+            }
+      }
+    }
+  }
+  if (p2p_pairs) *p2p_pairs = pp;
+  if (m2l_pairs) *m2l_pairs = mp;
+}
+
+/* Expansion coefficients of one cell (level 1-based, cartesian 1-based),
+ * for the stage-by-stage tests.  Returns 0 if the cell is active. */
+int orc_octree_cell_mp(const orc_octree *T, int level, const int ijk[3], double *a, double *b,
+                       double *c) {
+  const orc_layer *L = &T->L[level - 1];
+  int cc = cidx(L, ijk[0], ijk[1], ijk[2]);
+  if (!L->active[cc]) return -1;
+  int nm = T->pexp.n_mon;
+  size_t s = (size_t)L->slot[cc];
+  if (a) memcpy(a, &T->a[s * 3 * nm], sizeof(double) * 3 * nm);
+  if (b) memcpy(b, &T->b[s * 3 * nm], sizeof(double) * 3 * nm);
+  if (c) memcpy(c, &T->c[s * 9 * nm], sizeof(double) * 9 * nm);
+  return 0;
+}
+
+/* Stand-alone expansion operators for the exactness tests. */
+void orc_p2m(int degree, const double cen[3], int np, const double *pcen, const double *pdir,
+             const double *pmag, double *a) {
+  orc_octree T;
+  memset(&T, 0, sizeof(T));
+  pexp_init(&T.pexp, degree);
+  int *pl = (int *)malloc(sizeof(int) * (np > 0 ? np : 1));
+  for (int q = 0; q < np; ++q) pl[q] = q;
+  T.P.cen = pcen; T.P.dir = pdir; T.P.mag = pmag;
+  p2m(&T, cen, pl, np, a);
+  free(pl);
+  pexp_free(&T.pexp);
+}
+
+void orc_m2m(int degree, const double cen[3], const double *ach, const double ccen[3], double *a) {
+  orc_octree T;
+  memset(&T, 0, sizeof(T));
+  pexp_init(&T.pexp, degree);
+  m2m(&T, cen, ach, ccen, a);
+  pexp_free(&T.pexp);
+}
+
+void orc_l2l(int degree, int use_vs, const double cen[3], const double *bp, const double *cp,
+             const double pcen[3], double *b, double *c) {
+  orc_octree T;
+  memset(&T, 0, sizeof(T));
+  T.use_vs = use_vs;
+  pexp_init(&T.pexp, degree);
+  l2l(&T, cen, bp, cp, pcen, b, c);
+  pexp_free(&T.pexp);
+}
+
+/* M2L of one source expansion across a displacement diff = x_t - x_s. */
+void orc_m2l(int degree, int use_vs, const double diff[3], double delta, const double *as,
+             double *b, double *c) {
+  orc_octree T;
+  memset(&T, 0, sizeof(T));
+  T.use_vs = use_vs;
+  pexp_init(&T.pexp, degree);
+  pexp_init(&T.pexp_der, 2 * degree + 2);
+  int nm = T.pexp.n_mon;
+  T.mult = (double *)malloc(sizeof(double) * nm * nm);
+  for (int m = 0; m < nm; ++m)
+    for (int n = 0; n < nm; ++n) {
+      const int *pm = &T.pexp.pwr[3 * m], *pn = &T.pexp.pwr[3 * n];
+      T.mult[m * nm + n] =
+          pexp_nfact(&T.pexp_der, pm[0] + pn[0], pm[1] + pn[1], pm[2] + pn[2]) /
+          (pexp_nfact(&T.pexp, pm[0], pm[1], pm[2]) * pexp_nfact(&T.pexp, pn[0], pn[1], pn[2]));
+    }
+  int nD = T.pexp_der.n_mon_d[T.pexp_der.deg - 1], nDc = T.pexp_der.n_mon_d[T.pexp_der.deg - 2];
+  double *D = (double *)malloc(sizeof(double) * 3 * nD);
+  double *Dc = (double *)malloc(sizeof(double) * 9 * nDc);
+  orc_compute_der_ker(T.pexp_der.deg, diff, delta, D, Dc);
+  m2l(&T, D, Dc, as, b, c);
+  free(D); free(Dc); free(T.mult);
+  pexp_free(&T.pexp); pexp_free(&T.pexp_der);
+}
+
+/* L2P of one expansion at one point (velocity and the grad used for the
+ * stretching), mod_octree.f90:1185-1196 */
+void orc_l2p(int degree, const double cen[3], const double *b, const double *c,
+             const double pos[3], double vel[3], double grad[9]) {
+  orc_pexp pe;
+  pexp_init(&pe, degree);
+  for (int e = 0; e < 3; ++e) vel[e] = 0.0;
+  for (int e = 0; e < 9; ++e) grad[e] = 0.0;
+  for (int m = 0; m < pe.n_mon; ++m) {
+    const int *pw = &pe.pwr[3 * m];
+    double pr = ipow(pos[0] - cen[0], pw[0]) * ipow(pos[1] - cen[1], pw[1]) *
+                ipow(pos[2] - cen[2], pw[2]);
+    for (int e = 0; e < 3; ++e) vel[e] += b[e + 3 * m] * pr;
+    if (c)
+      for (int e = 0; e < 9; ++e) grad[e] += c[e + 9 * m] * pr;
+  }
+  pexp_free(&pe);
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -1,0 +1,291 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+where it is the checker (or the timed CPU baseline), never the product path.
+Build with `make -C oracle`.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_lib = None
+_vp = C.c_void_p
+G_SIZE = 74  # panel record length of aic_oracle.c
+G_CEN, G_NOR, G_AREA = 1, 4, 13
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "orc_pexp_table": (C.c_int, [C.c_int, _vp, _vp]),
+            "orc_compute_der_ker": (None, [C.c_int, _vp, C.c_double, _vp, _vp]),
+            "orc_pair_kernels": (None, [_vp, _vp, C.c_double, C.c_double, _vp, _vp, _vp, _vp, _vp]),
+            "orc_direct": (None, [C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_double, _vp,
+                                  _vp, _vp]),
+            "orc_octree_create": (_vp, [C.c_double, _vp, _vp, C.c_int, C.c_int, C.c_int,
+                                        C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, _vp]),
+            "orc_octree_destroy": (None, [_vp]),
+            "orc_sort_particles": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+            "orc_calculate_multipole": (None, [_vp]),
+            "orc_apply_multipole": (None, [_vp, _vp, _vp, _vp]),
+            "orc_octree_nleaves": (C.c_int, [_vp]),
+            "orc_octree_leaves": (None, [_vp, _vp, _vp, _vp, _vp, _vp]),
+            "orc_octree_work": (None, [_vp, _vp, _vp]),
+            "orc_octree_cell_mp": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
+            "orc_p2m": (None, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp]),
+            "orc_m2m": (None, [C.c_int, _vp, _vp, _vp, _vp]),
+            "orc_l2l": (None, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+            "orc_m2l": (None, [C.c_int, C.c_int, _vp, C.c_double, _vp, _vp, _vp]),
+            "orc_l2p": (None, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+            "orc_num_threads": (C.c_int, []),
+            "ao_geo": (None, [C.c_int, _vp, _vp, _vp]),
+            "ao_pot_doublet": (C.c_double, [_vp, _vp, C.c_double]),
+            "ao_pot_source": (C.c_double, [_vp, C.c_double, _vp, _vp]),
+            "ao_vel_doublet": (None, [_vp, _vp, C.c_double, C.c_double, C.c_double, _vp]),
+            "ao_assemble_surfpan": (C.c_int, [C.c_int, _vp, C.c_double, _vp, C.c_int, _vp,
+                                              C.c_int]),
+            "ao_assemble_vortlatt": (None, [C.c_int, _vp, C.c_double, C.c_double, C.c_double, _vp,
+                                            C.c_int, _vp, C.c_int]),
+            "ao_rhs_static": (None, [C.c_int, _vp, _vp, C.c_int, _vp, _vp, _vp]),
+            "ao_dgetrf": (C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp]),
+            "ao_dgetrs": (None, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+class _Ptr(C.c_void_p):
+    """c_void_p that keeps the numpy array it points into alive."""
+
+
+def _p(a):
+    if a is None:
+        return None
+    ptr = _Ptr(a.ctypes.data)
+    ptr._keep = a
+    return ptr
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ---- FMM --------------------------------------------------------------------
+def pexp_table(deg):
+    n = (deg + 1) * (deg + 2) * (deg + 3) // 6
+    pwr = np.zeros((n, 3), np.int32)
+    nmd = np.zeros(deg + 1, np.int32)
+    lib().orc_pexp_table(deg, _p(pwr), _p(nmd))
+    return pwr, nmd
+
+
+def der_ker(deg, diff, delta):
+    """compute_der_ker with pexp_der of degree `deg`: returns D (n,3), Dc (n,3,3)
+    with D[m, c] = D(c, m) and Dc[m, c, i] = Dc(c, i, m)."""
+    nD = deg * (deg + 1) * (deg + 2) // 6
+    nDc = (deg - 1) * deg * (deg + 1) // 6
+    D = np.zeros((nD, 3))
+    Dc = np.zeros((nDc, 3, 3))
+    lib().orc_compute_der_ker(deg, _p(_c(diff)), float(delta), _p(D), _p(Dc))
+    return D, np.ascontiguousarray(Dc.transpose(0, 2, 1))
+
+
+def pair(src_cen, src_dir, src_mag, src_r, pos, alpha):
+    v, s, d = np.zeros(3), np.zeros(3), np.zeros(3)
+    lib().orc_pair_kernels(_p(_c(src_cen)), _p(_c(src_dir)), float(src_mag), float(src_r),
+                           _p(_c(pos)), _p(_c(alpha)), _p(v), _p(s), _p(d))
+    return v, s, d
+
+
+def direct(cen, dir, mag, rv, use_vs=True, use_vd=False, nu_inf=0.0, u_inf=(0, 0, 0)):
+    n = len(mag)
+    vel = np.zeros((n, 3))
+    st = np.zeros((n, 3))
+    lib().orc_direct(n, _p(_c(cen)), _p(_c(dir)), _p(_c(mag)), _p(_c(rv)), int(use_vs),
+                     int(use_vd), float(nu_inf), _p(_c(u_inf)), _p(vel), _p(st))
+    return vel, st
+
+
+class Octree:
+    """CPU restatement of the reference octree FMM."""
+
+    def __init__(self, box_length, nbox, origin, nlevels, min_part, degree, delta, use_vs=True,
+                 use_vd=False, use_tv=False, nu_inf=0.0, u_inf=(0, 0, 0)):
+        self.degree = degree
+        self.nlevels = nlevels
+        self._h = lib().orc_octree_create(float(box_length), _p(_c(nbox, np.int32)),
+                                          _p(_c(origin)), nlevels, min_part, degree,
+                                          float(delta), int(use_vs), int(use_vd), int(use_tv),
+                                          float(nu_inf), _p(_c(u_inf)))
+        self._keep = None
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_octree_destroy(self._h)
+            self._h = None
+
+    def sort(self, cen, dir, mag, rv):
+        self._keep = [_c(cen), _c(dir), _c(mag), _c(rv)]
+        bad = C.c_int(-1)
+        rc = lib().orc_sort_particles(self._h, len(self._keep[2]), *[_p(a) for a in self._keep],
+                                      C.byref(bad))
+        if rc != 0:
+            raise ValueError(f"particle {bad.value} outside the octree")
+        return lib().orc_octree_nleaves(self._h)
+
+    def multipole(self):
+        lib().orc_calculate_multipole(self._h)
+
+    def apply(self):
+        n = len(self._keep[2])
+        vel = np.zeros((n, 3))
+        st = np.zeros((n, 3))
+        tv = np.zeros(n)
+        lib().orc_apply_multipole(self._h, _p(vel), _p(st), _p(tv))
+        return vel, st, tv
+
+    def run(self, cen, dir, mag, rv):
+        self.sort(cen, dir, mag, rv)
+        self.multipole()
+        return self.apply()
+
+    def leaves(self):
+        nl = lib().orc_octree_nleaves(self._h)
+        n = len(self._keep[2])
+        lev = np.zeros(nl, np.int32)
+        ijk = np.zeros((nl, 3), np.int32)
+        npart = np.zeros(nl, np.int32)
+        ps = np.zeros(nl, np.int32)
+        pl = np.zeros(max(n, 1), np.int32)
+        lib().orc_octree_leaves(self._h, _p(lev), _p(ijk), _p(npart), _p(ps), _p(pl))
+        return lev, ijk, npart, pl[:n]
+
+    def work(self):
+        a, b = C.c_double(), C.c_double()
+        lib().orc_octree_work(self._h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def cell_mp(self, level, ijk):
+        nm = (self.degree + 1) * (self.degree + 2) * (self.degree + 3) // 6
+        a, b, c = np.zeros((nm, 3)), np.zeros((nm, 3)), np.zeros((nm, 3, 3))
+        rc = lib().orc_octree_cell_mp(self._h, level, _p(_c(ijk, np.int32)), _p(a), _p(b), _p(c))
+        return None if rc else (a, b, c)
+
+
+def p2m(degree, cen, pcen, pdir, pmag):
+    nm = (degree + 1) * (degree + 2) * (degree + 3) // 6
+    a = np.zeros((nm, 3))
+    lib().orc_p2m(degree, _p(_c(cen)), len(pmag), _p(_c(pcen)), _p(_c(pdir)), _p(_c(pmag)), _p(a))
+    return a
+
+
+def m2m(degree, cen, a_child, ccen, a=None):
+    a = np.zeros_like(a_child) if a is None else a.copy()
+    lib().orc_m2m(degree, _p(_c(cen)), _p(_c(a_child)), _p(_c(ccen)), _p(a))
+    return a
+
+
+def l2l(degree, cen, b_par, c_par, pcen, use_vs=True):
+    b = np.zeros_like(b_par)
+    c = np.zeros_like(c_par)
+    lib().orc_l2l(degree, int(use_vs), _p(_c(cen)), _p(_c(b_par)), _p(_c(c_par)), _p(_c(pcen)),
+                  _p(b), _p(c))
+    return b, c
+
+
+def m2l(degree, diff, delta, a_src, use_vs=True):
+    b = np.zeros_like(a_src)
+    c = np.zeros((a_src.shape[0], 3, 3))
+    lib().orc_m2l(degree, int(use_vs), _p(_c(diff)), float(delta), _p(_c(a_src)), _p(b), _p(c))
+    return b, c
+
+
+def l2p(degree, cen, b, c, pos):
+    v = np.zeros(3)
+    g = np.zeros(9)
+    lib().orc_l2p(degree, _p(_c(cen)), _p(_c(b)), None if c is None else _p(_c(c)), _p(_c(pos)),
+                  _p(v), _p(g))
+    return v, g.reshape(3, 3, order="F")  # grad(i, j)
+
+
+# ---- AIC / LU -------------------------------------------------------------------
+def geo(ver, n_ver):
+    n = len(n_ver)
+    v = np.zeros((n, 4, 3))
+    v[:, : np.asarray(ver).shape[1]] = np.asarray(ver)[:, :4]
+    G = np.zeros((n, G_SIZE))
+    lib().ao_geo(n, _p(_c(n_ver, np.int32)), _p(_c(v)), _p(G))
+    return G
+
+
+def pot_doublet(g, pos, ff_ratio=10.0):
+    return lib().ao_pot_doublet(_p(_c(g)), _p(_c(pos)), float(ff_ratio))
+
+
+def pot_source(g, dou, pos):
+    w = C.c_int(0)
+    return lib().ao_pot_source(_p(_c(g)), float(dou), _p(_c(pos)), C.byref(w))
+
+
+def vel_doublet(g, pos, ff_ratio=10.0, rankine=0.1, cutoff=0.001):
+    v = np.zeros(3)
+    lib().ao_vel_doublet(_p(_c(g)), _p(_c(pos)), float(ff_ratio), float(rankine), float(cutoff),
+                         _p(v))
+    return v
+
+
+def aic_surfpan(G, ff_ratio=10.0, with_source=True):
+    n = G.shape[0]
+    A = np.zeros((n, n), order="F")
+    B = np.zeros((n, n), order="F") if with_source else None
+    lib().ao_assemble_surfpan(n, _p(_c(G)), float(ff_ratio), _p(A), n, _p(B), n)
+    return A, B
+
+
+def aic_vortlatt(G, ff_ratio=10.0, rankine=0.1, cutoff=0.001):
+    n = G.shape[0]
+    A = np.zeros((n, n), order="F")
+    B = np.zeros((n, n), order="F")
+    lib().ao_assemble_vortlatt(n, _p(_c(G)), float(ff_ratio), float(rankine), float(cutoff),
+                               _p(A), n, _p(B), n)
+    return A, B
+
+
+def rhs_static(G, B, u_inf, uvort=None):
+    n = G.shape[0]
+    b = np.zeros(n)
+    lib().ao_rhs_static(n, _p(_c(G)), _p(np.asfortranarray(B)), n, _p(_c(u_inf)),
+                        None if uvort is None else _p(_c(uvort)), _p(b))
+    return b
+
+
+def dgetrf(A):
+    A = np.array(A, order="F", dtype=np.float64)
+    m, n = A.shape
+    ipiv = np.zeros(min(m, n), np.int32)
+    info = lib().ao_dgetrf(m, n, _p(A), m, _p(ipiv))
+    return A, ipiv, info
+
+
+def dgetrs(LU, ipiv, B):
+    B = np.array(B, order="F", dtype=np.float64)
+    n = LU.shape[0]
+    Bv = B.reshape(n, -1, order="F")
+    lib().ao_dgetrs(n, Bv.shape[1], _p(np.asfortranarray(LU)), n, _p(_c(ipiv, np.int32)), _p(B), n)
+    return B
+
+
+def num_threads():
+    return lib().orc_num_threads()
