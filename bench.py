@@ -34,9 +34,11 @@ def fmm_handle(O, tree, degree, n, form):
                                precision=form)
 
 
-def time_fmm(torch, L, O, tree, degree, dev, n, form, steps, warmup):
+def time_fmm(torch, L, O, tree, degree, dev, n, form, steps, warmup, knobs=None):
     """Returns dict with ms per pass, stage times and the work counts."""
     g = fmm_handle(O, tree, degree, n, form)
+    for k, v in (knobs or {}).items():
+        g.set_tuning(k, v)
     lib = L.lib()
     h = g.handle
     L.check(lib.dust_fmm_set_particles_dev(h, n, dev["cen"].data_ptr(), dev["dir"].data_ptr(),
@@ -195,6 +197,9 @@ def main():
     ap.add_argument("--no-lu", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=125_000)
+    ap.add_argument("--near", type=float, default=1.0,
+                    help="tuned form: FP64 for pairs closer than this many core radii")
+    ap.add_argument("--no-fp64", action="store_true", help="skip timing the plain FP64 form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this path has no "
@@ -222,7 +227,8 @@ def main():
     tuned = None
     try:
         log("timing tuned FMM")
-        tuned = time_fmm(torch, L, O, tree, args.degree, dev, n, "tuned", args.steps, args.warmup)
+        tuned = time_fmm(torch, L, O, tree, args.degree, dev, n, "tuned", args.steps, args.warmup,
+                         knobs={"near_factor": args.near})
         log(f"tuned: {tuned['ms']:.2f} ms/pass stages {tuned['stages']}")
     except Exception as e:  # tuned form not built in this library
         log(f"tuned form unavailable: {e}")

@@ -73,6 +73,7 @@ SIGNATURES = [
     ("dust_fmm_apply_multipole_dev", C.c_int, [_vp]),
     ("dust_fmm_results_dev", C.c_int, [_vp, _vp, _vp, _vp]),
     ("dust_fmm_pass_dev", C.c_int, [_vp, _ip]),
+    ("dust_fmm_set_tuning", C.c_int, [_vp, C.c_char_p, C.c_double]),
     ("dust_fmm_stream", _vp, [_vp]),
     ("dust_fmm_synchronize", C.c_int, [_vp]),
     ("dust_fmm_leaves", C.c_int, [_vp, _vp, _vp, _vp, _vp]),

@@ -12,7 +12,7 @@
 #include <cstring>
 #include <mutex>
 
-#include "fmm_internal.h"
+#include "fmm_eval.h"
 
 namespace dust {
 
@@ -27,17 +27,6 @@ void set_error(const char *fmt, ...) {
   va_end(ap);
 }
 
-// ---------------------------------------------------------------------------
-// Graded monomial numbering of set_degree_polyexp (mod_multipole.f90:465-488):
-// order o = i+j+k, then i descending, then j descending.  Closed form of the
-// position: o(o+1)(o+2)/6 monomials of lower order, (o-i)(o-i+1)/2 with a
-// larger i, and o-i-j with the same i and a larger j.
-// ---------------------------------------------------------------------------
-__host__ __device__ inline int mono_index(int i, int j, int k) {
-  int o = i + j + k;
-  return o * (o + 1) * (o + 2) / 6 + (o - i) * (o - i + 1) / 2 + (o - i - j);
-}
-inline int mono_count(int deg) { return (deg + 1) * (deg + 2) * (deg + 3) / 6; }
 
 static void mono_powers(int deg, std::vector<int> &pwr) {
   pwr.clear();
@@ -104,70 +93,7 @@ static void kernel_derivatives(int p, const double diff[3], double delta, int nU
 
 // ---------------------------------------------------------------------------
 // Device helpers for the cell numbering
-// ---------------------------------------------------------------------------
-struct TreeGeom {
-  int nlev;
-  int nbox[3];
-  int ncl[kMaxLevels][3];
-  long off[kMaxLevels];
-  int kdl[kMaxLevels][3];
-  long kd_base[kMaxLevels];
-  double xmin[3], xmax[3];
-  double csize_fine;
-};
-
-__host__ __device__ inline unsigned spread3(unsigned v) {
-  v &= 0x3ffu;
-  v = (v | (v << 16)) & 0x030000FFu;
-  v = (v | (v << 8)) & 0x0300F00Fu;
-  v = (v | (v << 4)) & 0x030C30C3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
-}
-__host__ __device__ inline unsigned compact3(unsigned v) {
-  v &= 0x09249249u;
-  v = (v | (v >> 2)) & 0x030C30C3u;
-  v = (v | (v >> 4)) & 0x0300F00Fu;
-  v = (v | (v >> 8)) & 0x030000FFu;
-  v = (v | (v >> 16)) & 0x000003FFu;
-  return v;
-}
-
-// 0-based cartesian cell coordinates -> cell number within level lvl.
-__host__ __device__ inline long cell_encode(const TreeGeom &G, int lvl, int ix, int iy, int iz) {
-  int bx = ix >> lvl, by = iy >> lvl, bz = iz >> lvl;
-  unsigned msk = (1u << lvl) - 1u;
-  long box = bx + (long)G.nbox[0] * (by + (long)G.nbox[1] * bz);
-  unsigned m = spread3(ix & msk) | (spread3(iy & msk) << 1) | (spread3(iz & msk) << 2);
-  return (box << (3 * lvl)) | (long)m;
-}
-__host__ __device__ inline void cell_decode(const TreeGeom &G, int lvl, long c, int &ix, int &iy,
-                                            int &iz) {
-  long box = c >> (3 * lvl);
-  unsigned m = (unsigned)(c & ((1L << (3 * lvl)) - 1));
-  int bx = (int)(box % G.nbox[0]);
-  int by = (int)((box / G.nbox[0]) % G.nbox[1]);
-  int bz = (int)(box / ((long)G.nbox[0] * G.nbox[1]));
-  ix = (bx << lvl) | (int)compact3(m);
-  iy = (by << lvl) | (int)compact3(m >> 1);
-  iz = (bz << lvl) | (int)compact3(m >> 2);
-}
-// Cell centre, evaluated as mod_octree.f90:332-334,363-366.
-__host__ __device__ inline double cell_center(const TreeGeom &G, int lvl, int d, int i0) {
-  return G.xmin[d] + ((double)(i0 + 1) - 0.5) * (G.xmax[d] - G.xmin[d]) / (double)(G.nbox[d] << lvl);
-}
-__device__ inline int lvl_of(const TreeGeom &G, long g) {
-  int l = 0;
-  while (l + 1 < G.nlev && g >= G.off[l + 1]) ++l;
-  return l;
-}
-// Particle range [s, e) of cell c at level lvl in the sorted arrays.
-__device__ inline int2 cell_range(const TreeGeom &G, const int *fstart, int lvl, long c) {
-  int sh = 3 * (G.nlev - 1 - lvl);
-  return make_int2(fstart[c << sh], fstart[(c + 1) << sh]);
-}
-
-static TreeGeom make_geom(const FmmHandle *h) {
+TreeGeom make_geom(const FmmHandle *h) {
   TreeGeom G;
   memset(&G, 0, sizeof(G));
   G.nlev = h->nlev;
@@ -312,7 +238,8 @@ __global__ void k_level_starts(TreeGeom G, int n_active, const int *__restrict__
 // particles.
 __global__ void k_p2p_lists(TreeGeom G, int n_leaves, const int *__restrict__ leaves,
                             const unsigned char *__restrict__ flags,
-                            const int *__restrict__ fstart, int cap, int2 *rng, int *cnt) {
+                            const int *__restrict__ fstart, int cap, int2 *rng, int *cnt,
+                            int *tot) {
   int lv = blockIdx.x * blockDim.x + threadIdx.x;
   if (lv >= n_leaves) return;
   long g = leaves[lv];
@@ -354,6 +281,9 @@ __global__ void k_p2p_lists(TreeGeom G, int n_leaves, const int *__restrict__ le
   }
   out[w++] = cell_range(G, fstart, lvl, c);
   cnt[lv] = w;
+  int T = 0;
+  for (int q = 0; q < w; ++q) T += out[q].y - out[q].x;
+  tot[lv] = T;
 }
 
 // ---------------------------------------------------------------------------
@@ -621,55 +551,58 @@ __global__ void k_l2l(TreeGeom G, int lvl, int deg, int nmon, int use_vs,
 }
 
 // ---------------------------------------------------------------------------
-// Evaluation: L2P + P2P, plain FP64 (apply_multipole, mod_octree.f90:1139-1429)
+// Evaluation: L2P + P2P (apply_multipole, mod_octree.f90:1139-1429)
 // ---------------------------------------------------------------------------
-struct EvalArgs {
-  TreeGeom G;
-  int deg, nmon, use_vs, use_vd, use_tv;
-  double nu_inf, uinf[3];
-  const int *pwr, *leaves, *slot, *fstart;
-  const double *mp_b, *mp_c;
-  const double *px, *py, *pz, *dx, *dy, *dz, *pm, *pr;
-  const int2 *rng;
-  const int *cnt;
-  int cap;
-  double *vel, *str, *tv;
-};
 
-constexpr int kEvalThreads = 128;
-
-// L2P of one target: vel = sum b(:,m) d^m, grad(i,j) = sum c(i,j,m) d^m.
-__device__ inline void l2p_point(const EvalArgs &A, const double *sb, const double *sc,
-                                 const int *spw, double d0, double d1, double d2, double v[3],
-                                 double g[9]) {
-  double p0[11], p1[11], p2[11];
-  p0[0] = p1[0] = p2[0] = 1.0;
-  for (int e = 1; e <= A.deg; ++e) {
-    p0[e] = p0[e - 1] * d0;
-    p1[e] = p1[e - 1] * d1;
-    p2[e] = p2[e - 1] * d2;
-  }
-  v[0] = v[1] = v[2] = 0.0;
-  for (int q = 0; q < 9; ++q) g[q] = 0.0;
-  for (int m = 0; m < A.nmon; ++m) {
-    double pr = p0[spw[3 * m]] * p1[spw[3 * m + 1]] * p2[spw[3 * m + 2]];
-    v[0] += sb[3 * m] * pr;
-    v[1] += sb[3 * m + 1] * pr;
-    v[2] += sb[3 * m + 2] * pr;
-    if (A.use_vs)
-      for (int q = 0; q < 9; ++q) g[q] += sc[9 * m + q] * pr;
+// Work units: every leaf's targets in chunks of <= 64 (see fmm_eval.h); each
+// unit is cut into items of about kSegIters source iterations per lane.
+__global__ void k_units(TreeGeom G, int n_leaves, const int *__restrict__ leaves,
+                        const int *__restrict__ fstart, const int *__restrict__ tot,
+                        EvalUnit *units, int *n_units, int *n_items) {
+  int lv = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lv >= n_leaves) return;
+  long g = leaves[lv];
+  int lvl = lvl_of(G, g);
+  int2 r = cell_range(G, fstart, lvl, g - G.off[lvl]);
+  int np = r.y - r.x;
+  if (np <= 0) return;
+  const int T = tot[lv];
+  int nu = (np + kWave - 1) / kWave;
+  for (int q = 0; q < nu; ++q) {
+    int nt = min(kWave, np - q * kWave);
+    int lpt = 1;
+    while (lpt * 2 * nt <= kWave) lpt *= 2;
+    int iters = (T + lpt - 1) / lpt;
+    int nseg = max(1, (iters + kSegIters - 1) / kSegIters);
+    int i0 = atomicAdd(n_items, nseg);
+    int w = atomicAdd(n_units, 1);
+    units[w] = EvalUnit{lv, r.x + q * kWave, nt, lpt, i0, nseg, 0, 0};
   }
 }
 
-__global__ void __launch_bounds__(kEvalThreads) k_eval_fp64(EvalArgs A) {
+__global__ void k_items(int n_units, const EvalUnit *__restrict__ units,
+                        const int *__restrict__ tot, EvalItem *items) {
+  int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n_units) return;
+  EvalUnit U = units[u];
+  int T = tot[U.leaf];
+  int len = (T + U.nseg - 1) / U.nseg;
+  for (int k = 0; k < U.nseg; ++k)
+    items[U.item0 + k] = EvalItem{u, min(T, k * len), min(T, (k + 1) * len), 0};
+}
+
+// Leaf-averaged rate of strain of the L2P gradient (mod_octree.f90:1214-1224),
+// needed for the turbulent viscosity; one wavefront per leaf.
+__global__ void k_leaf_ros(EvalArgs A, int n_leaves) {
   extern __shared__ double sm[];
   const int nmon = A.nmon;
-  double *sb = sm;                       // 3*nmon
-  double *sc = sb + 3 * nmon;            // 9*nmon
-  double *ss = sc + 9 * nmon;            // [8][kEvalThreads] source tile
-  int *spw = (int *)(ss + 8 * kEvalThreads);
-  __shared__ double red[kEvalThreads / 64];
-  const int lv = blockIdx.x;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double *sb = sm + wv * 12 * nmon, *sc = sb + 3 * nmon;
+  int *spw = (int *)(sm + kEvalWaves * 12 * nmon);
+  for (int q = threadIdx.x; q < 3 * nmon; q += blockDim.x) spw[q] = A.pwr[q];
+  __syncthreads();
+  int lv = blockIdx.x * kEvalWaves + wv;
+  if (lv >= n_leaves) return;
   long g = A.leaves[lv];
   int lvl = lvl_of(A.G, g);
   long c = g - A.G.off[lvl];
@@ -677,132 +610,183 @@ __global__ void __launch_bounds__(kEvalThreads) k_eval_fp64(EvalArgs A) {
   cell_decode(A.G, lvl, c, ix, iy, iz);
   double cx = cell_center(A.G, lvl, 0, ix), cy = cell_center(A.G, lvl, 1, iy),
          cz = cell_center(A.G, lvl, 2, iz);
-  int2 own = cell_range(A.G, A.fstart, lvl, c);
   long sl = A.slot[g];
-  for (int q = threadIdx.x; q < 3 * nmon; q += blockDim.x) sb[q] = A.mp_b[sl * 3 * nmon + q];
-  if (A.use_vs)
-    for (int q = threadIdx.x; q < 9 * nmon; q += blockDim.x) sc[q] = A.mp_c[sl * 9 * nmon + q];
-  for (int q = threadIdx.x; q < 3 * nmon; q += blockDim.x) spw[q] = A.pwr[q];
-  __syncthreads();
-  const double fpi_inv = 1.0 / (4.0 * kPi);
-  // leaf-averaged rate of strain of the L2P gradient (mod_octree.f90:1214-1224)
-  double ave_ros = 0.0;
-  if (A.use_vd && A.use_tv) {
-    double acc = 0.0;
-    for (int t = own.x + threadIdx.x; t < own.y; t += blockDim.x) {
-      double v[3], gr[9];
-      l2p_point(A, sb, sc, spw, A.px[t] - cx, A.py[t] - cy, A.pz[t] - cz, v, gr);
-      for (int ii = 0; ii < 3; ++ii)
-        for (int jj = 0; jj < 3; ++jj) {
-          double e = 0.5 * (gr[ii + 3 * jj] + gr[jj + 3 * ii]);
-          acc += e * e;
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int w = 0; w < kEvalThreads / 64; ++w) s += red[w];
-      red[0] = s;
-    }
-    __syncthreads();
-    ave_ros = red[0];
-    if (own.y > own.x) ave_ros /= (double)(own.y - own.x);
+  for (int q = lane; q < 3 * nmon; q += kWave) sb[q] = A.mp_b[sl * 3 * nmon + q];
+  for (int q = lane; q < 9 * nmon; q += kWave) sc[q] = A.mp_c[sl * 9 * nmon + q];
+  wave_sync();
+  int2 own = cell_range(A.G, A.fstart, lvl, c);
+  double acc = 0.0;
+  for (int t = own.x + lane; t < own.y; t += kWave) {
+    double v[3], gr[9];
+    l2p_point(A.deg, nmon, 1, sb, sc, spw, A.px[t] - cx, A.py[t] - cy, A.pz[t] - cz, v, gr);
+    for (int ii = 0; ii < 3; ++ii)
+      for (int jj = 0; jj < 3; ++jj) {
+        double e = 0.5 * (gr[ii + 3 * jj] + gr[jj + 3 * ii]);
+        acc += e * e;
+      }
   }
-  const int nr = A.cnt[lv];
-  const int2 *rl = &A.rng[(long)lv * A.cap];
-  for (int tb = own.x; tb < own.y; tb += blockDim.x) {
-    int t = tb + threadIdx.x;
-    bool valid = t < own.y;
-    double x = 0, y = 0, z = 0, al0 = 0, al1 = 0, al2 = 0, tvp = 0.0;
-    double v[3] = {0, 0, 0}, st[3] = {0, 0, 0};
-    if (valid) {
-      x = A.px[t]; y = A.py[t]; z = A.pz[t];
-      double mg = A.pm[t];
-      al0 = mg * A.dx[t]; al1 = mg * A.dy[t]; al2 = mg * A.dz[t];
-      double gr[9];
-      l2p_point(A, sb, sc, spw, x - cx, y - cy, z - cz, v, gr);
-      if (A.use_vs) {
-        // str = matmul(alpha, transpose(grad)): str_j = sum_i grad(j,i) alpha_i
-        for (int jj = 0; jj < 3; ++jj)
-          st[jj] = gr[jj] * al0 + gr[jj + 3] * al1 + gr[jj + 6] * al2;
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+  if (lane == 0) {
+    int np = own.y - own.x;
+    const_cast<double *>(A.leaf_ros)[lv] = np > 0 ? acc / (double)np : 0.0;
+  }
+}
+
+__device__ inline double turbvisc_of(const EvalArgs &A, int leaf, int t) {
+  if (!A.use_tv) return 0.0;
+  double rr = 0.11 * A.pr[t];
+  return rr * rr * sqrt(2.0 * A.leaf_ros[leaf]);
+}
+
+// P2P of one item, plain FP64.  Pair terms are the reference formulas
+// (compute_vel, compute_stretch, compute_diffusion of mod_vortpart.f90) in
+// FP64; the list is walked in the reference order (neighbours, leaf_neigh,
+// own leaf).  Writes the item's partial sums (velocity and stretching,
+// divided by 4 pi; diffusion times (nu + nu_t)).
+__global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_fp64(EvalArgs A) {
+  __shared__ double ssm[kEvalWaves][8 * kWave];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double *ss = ssm[wv];
+  const int it = blockIdx.x * kEvalWaves + wv;
+  if (it >= A.n_items) return;
+  const EvalItem I = A.items[it];
+  const EvalUnit U = A.units[I.unit];
+  const int lpt = U.lpt, tl = lane / lpt, sub = lane - tl * lpt;
+  const bool valid = tl < U.nt;
+  const int t = U.t0 + (valid ? tl : 0);
+  const double x = A.px[t], y = A.py[t], z = A.pz[t];
+  const double mg = A.pm[t];
+  const double al0 = mg * A.dx[t], al1 = mg * A.dy[t], al2 = mg * A.dz[t];
+  const double tvp = turbvisc_of(A, U.leaf, t);
+  double pv0 = 0, pv1 = 0, pv2 = 0, ps0 = 0, ps1 = 0, ps2 = 0, pd0 = 0, pd1 = 0, pd2 = 0;
+  const int nr = A.cnt[U.leaf];
+  const int2 *rl = &A.rng[(long)U.leaf * A.cap];
+  int ri, off;
+  seg_seek(rl, nr, I.lo, ri, off);
+  int left = I.hi - I.lo;
+  for (; ri < nr && left > 0; ++ri, off = 0) {
+    const int2 r = rl[ri];
+    const int e = min(r.y, r.x + off + left);
+    for (int s0 = r.x + off; s0 < e; s0 += kWave) {
+      const int ns = min(kWave, e - s0);
+      if (lane < ns) {
+        const int q = s0 + lane;
+        ss[0 * kWave + lane] = A.px[q];
+        ss[1 * kWave + lane] = A.py[q];
+        ss[2 * kWave + lane] = A.pz[q];
+        ss[3 * kWave + lane] = A.dx[q];
+        ss[4 * kWave + lane] = A.dy[q];
+        ss[5 * kWave + lane] = A.dz[q];
+        ss[6 * kWave + lane] = A.pm[q];
+        ss[7 * kWave + lane] = A.pr[q];
       }
-      if (A.use_tv) {
-        double rr = 0.11 * A.pr[t];
-        tvp = rr * rr * sqrt(2.0 * ave_ros);
-      }
-    }
-    double pv0 = 0, pv1 = 0, pv2 = 0, ps0 = 0, ps1 = 0, ps2 = 0, pd0 = 0, pd1 = 0, pd2 = 0;
-    for (int ri = 0; ri < nr; ++ri) {
-      int2 r = rl[ri];
-      for (int sb0 = r.x; sb0 < r.y; sb0 += kEvalThreads) {
-        int ns = min(kEvalThreads, r.y - sb0);
-        __syncthreads();
-        if (threadIdx.x < ns) {
-          int q = sb0 + threadIdx.x;
-          ss[0 * kEvalThreads + threadIdx.x] = A.px[q];
-          ss[1 * kEvalThreads + threadIdx.x] = A.py[q];
-          ss[2 * kEvalThreads + threadIdx.x] = A.pz[q];
-          ss[3 * kEvalThreads + threadIdx.x] = A.dx[q];
-          ss[4 * kEvalThreads + threadIdx.x] = A.dy[q];
-          ss[5 * kEvalThreads + threadIdx.x] = A.dz[q];
-          ss[6 * kEvalThreads + threadIdx.x] = A.pm[q];
-          ss[7 * kEvalThreads + threadIdx.x] = A.pr[q];
-        }
-        __syncthreads();
-        if (valid) {
-          for (int s = 0; s < ns; ++s) {
-            if (sb0 + s == t) continue;
-            double r0 = x - ss[s], r1 = y - ss[kEvalThreads + s], r2 = z - ss[2 * kEvalThreads + s];
-            double sdx = ss[3 * kEvalThreads + s], sdy = ss[4 * kEvalThreads + s],
-                   sdz = ss[5 * kEvalThreads + s];
-            double smg = ss[6 * kEvalThreads + s], srv = ss[7 * kEvalThreads + s];
-            double d2 = r0 * r0 + r1 * r1 + r2 * r2;
-            // kernel_coeffs (mod_vortpart.f90:269-271)
-            double R2 = d2 + srv * srv;
-            double R = sqrt(R2);
-            double R3 = R2 * R;
-            double cf = -1.0 / R3;
-            // compute_vel: c * (dist x dir) * mag
-            pv0 += cf * (r1 * sdz - r2 * sdy) * smg;
-            pv1 += cf * (r2 * sdx - r0 * sdz) * smg;
-            pv2 += cf * (r0 * sdy - r1 * sdx) * smg;
-            if (A.use_vs) {
-              double df = 3.0 / (R3 * R2);
-              double am0 = sdx * smg, am1 = sdy * smg, am2 = sdz * smg;
-              double vp0 = al1 * am2 - al2 * am1, vp1 = al2 * am0 - al0 * am2,
-                     vp2 = al0 * am1 - al1 * am0;
-              double dv = r0 * vp0 + r1 * vp1 + r2 * vp2;
-              ps0 -= cf * vp0 + df * r0 * dv;
-              ps1 -= cf * vp1 + df * r1 * dv;
-              ps2 -= cf * vp2 + df * r2 * dv;
-            }
-            if (A.use_vd) {
-              // compute_diffusion with etaeps (mod_vortpart.f90:305-336)
-              double vol = 4.0 / 3.0 * kPi * srv * srv * srv;
-              double qq = d2 / (srv * srv) + 1.0;
-              double q2 = qq * qq;
-              double eta = 105.0 / (8.0 * kPi) / (q2 * q2 * sqrt(qq)) / (srv * srv * srv);
-              double f = 1.0 / (srv * srv) * eta * (A.nu_inf + tvp);
-              pd0 += f * (vol * sdx * smg - vol * al0);
-              pd1 += f * (vol * sdy * smg - vol * al1);
-              pd2 += f * (vol * sdz * smg - vol * al2);
-            }
+      wave_sync();
+      if (valid) {
+        for (int s = sub; s < ns; s += lpt) {
+          if (s0 + s == t) continue;
+          double r0 = x - ss[s], r1 = y - ss[kWave + s], r2 = z - ss[2 * kWave + s];
+          double sdx = ss[3 * kWave + s], sdy = ss[4 * kWave + s], sdz = ss[5 * kWave + s];
+          double smg = ss[6 * kWave + s], srv = ss[7 * kWave + s];
+          double d2 = r0 * r0 + r1 * r1 + r2 * r2;
+          // kernel_coeffs (mod_vortpart.f90:269-271)
+          double R2 = d2 + srv * srv;
+          double R3 = R2 * sqrt(R2);
+          double cf = -1.0 / R3;
+          pv0 += cf * (r1 * sdz - r2 * sdy) * smg;
+          pv1 += cf * (r2 * sdx - r0 * sdz) * smg;
+          pv2 += cf * (r0 * sdy - r1 * sdx) * smg;
+          if (A.use_vs) {
+            double df = 3.0 / (R3 * R2);
+            double am0 = sdx * smg, am1 = sdy * smg, am2 = sdz * smg;
+            double vp0 = al1 * am2 - al2 * am1, vp1 = al2 * am0 - al0 * am2, vp2 = al0 * am1 - al1 * am0;
+            double dv = r0 * vp0 + r1 * vp1 + r2 * vp2;
+            ps0 -= cf * vp0 + df * r0 * dv;
+            ps1 -= cf * vp1 + df * r1 * dv;
+            ps2 -= cf * vp2 + df * r2 * dv;
+          }
+          if (A.use_vd) {
+            // compute_diffusion with etaeps (mod_vortpart.f90:305-336)
+            double vol = 4.0 / 3.0 * kPi * srv * srv * srv;
+            double qq = d2 / (srv * srv) + 1.0;
+            double q2 = qq * qq;
+            double eta = 105.0 / (8.0 * kPi) / (q2 * q2 * sqrt(qq)) / (srv * srv * srv);
+            double f = 1.0 / (srv * srv) * eta * (A.nu_inf + tvp);
+            pd0 += f * (vol * sdx * smg - vol * al0);
+            pd1 += f * (vol * sdy * smg - vol * al1);
+            pd2 += f * (vol * sdz * smg - vol * al2);
           }
         }
       }
+      wave_sync();
     }
-    if (valid) {
-      A.vel[3 * t] = v[0] + pv0 * fpi_inv + A.uinf[0];
-      A.vel[3 * t + 1] = v[1] + pv1 * fpi_inv + A.uinf[1];
-      A.vel[3 * t + 2] = v[2] + pv2 * fpi_inv + A.uinf[2];
-      A.str[3 * t] = st[0] + ps0 * fpi_inv + pd0;
-      A.str[3 * t + 1] = st[1] + ps1 * fpi_inv + pd1;
-      A.str[3 * t + 2] = st[2] + ps2 * fpi_inv + pd2;
-      A.tv[t] = tvp;
+    left -= e - (r.x + off);
+  }
+  pv0 = lane_group_sum(pv0, lpt); pv1 = lane_group_sum(pv1, lpt); pv2 = lane_group_sum(pv2, lpt);
+  if (A.use_vs) {
+    ps0 = lane_group_sum(ps0, lpt); ps1 = lane_group_sum(ps1, lpt); ps2 = lane_group_sum(ps2, lpt);
+  }
+  if (A.use_vd) {
+    pd0 = lane_group_sum(pd0, lpt); pd1 = lane_group_sum(pd1, lpt); pd2 = lane_group_sum(pd2, lpt);
+  }
+  if (valid && sub == 0) {
+    const double fi = 1.0 / (4.0 * kPi);
+    double *o = &A.part[((long)it * kWave + tl) * 6];
+    o[0] = pv0 * fi; o[1] = pv1 * fi; o[2] = pv2 * fi;
+    o[3] = ps0 * fi + pd0; o[4] = ps1 * fi + pd1; o[5] = ps2 * fi + pd2;
+  }
+}
+
+// Finish of every unit (both forms): L2P in FP64, the items' P2P sums in
+// item order, the free stream, and the turbulent viscosity.
+__global__ void __launch_bounds__(kWave * kEvalWaves) k_eval_finish(EvalArgs A) {
+  extern __shared__ double sm[];
+  const int nmon = A.nmon;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double *sb = sm + wv * 12 * nmon, *sc = sb + 3 * nmon;
+  int *spw = (int *)(sm + kEvalWaves * 12 * nmon);
+  for (int q = threadIdx.x; q < 3 * nmon; q += blockDim.x) spw[q] = A.pwr[q];
+  __syncthreads();
+  const int u = blockIdx.x * kEvalWaves + wv;
+  if (u >= A.n_units) return;
+  const EvalUnit U = A.units[u];
+  const long g = A.leaves[U.leaf];
+  const int lvl = lvl_of(A.G, g);
+  const long c = g - A.G.off[lvl];
+  int ix, iy, iz;
+  cell_decode(A.G, lvl, c, ix, iy, iz);
+  const double cx = cell_center(A.G, lvl, 0, ix), cy = cell_center(A.G, lvl, 1, iy),
+               cz = cell_center(A.G, lvl, 2, iz);
+  const long sl = A.slot[g];
+  for (int q = lane; q < 3 * nmon; q += kWave) sb[q] = A.mp_b[sl * 3 * nmon + q];
+  if (A.use_vs)
+    for (int q = lane; q < 9 * nmon; q += kWave) sc[q] = A.mp_c[sl * 9 * nmon + q];
+  wave_sync();
+  if (lane >= U.nt) return;
+  const int t = U.t0 + lane;
+  const double x = A.px[t], y = A.py[t], z = A.pz[t];
+  const double mg = A.pm[t];
+  const double al0 = mg * A.dx[t], al1 = mg * A.dy[t], al2 = mg * A.dz[t];
+  double v[3], gr[9], st[3] = {0, 0, 0};
+  l2p_point(A.deg, nmon, A.use_vs, sb, sc, spw, x - cx, y - cy, z - cz, v, gr);
+  if (A.use_vs)  // str = matmul(alpha, transpose(grad)): str_j = sum_i grad(j,i) alpha_i
+    for (int jj = 0; jj < 3; ++jj) st[jj] = gr[jj] * al0 + gr[jj + 3] * al1 + gr[jj + 6] * al2;
+  double pv[3] = {0, 0, 0}, ps[3] = {0, 0, 0};
+  for (int k = 0; k < U.nseg; ++k) {
+    const double *o = &A.part[((long)(U.item0 + k) * kWave + lane) * 6];
+    for (int e = 0; e < 3; ++e) {
+      pv[e] += o[e];
+      ps[e] += o[3 + e];
     }
   }
+  for (int e = 0; e < 3; ++e) {
+    A.vel[3 * t + e] = v[e] + pv[e] + A.uinf[e];
+    A.str[3 * t + e] = st[e] + ps[e];
+  }
+  A.tv[t] = turbvisc_of(A, U.leaf, t);
+}
+
+size_t eval_finish_lds(int nmon) {
+  return sizeof(double) * kEvalWaves * 12 * nmon + sizeof(int) * 3 * nmon;
 }
 
 // Scatter results from sorted to input order.
@@ -820,6 +804,8 @@ __global__ void k_scatter(int n, const int *__restrict__ perm, const double *__r
   s[3 * q + 2] = ss[3 * i + 2];
   tv[q] = ts[i];
 }
+
+constexpr int kEvalThreads = 128;
 
 // Direct O(N^2) particle terms, the fmm=F branch (mod_wake.f90:1050-1113).
 __global__ void __launch_bounds__(kEvalThreads)
@@ -1043,7 +1029,7 @@ static int fmm_create(const dust_fmm_params *prm, int capacity, FmmHandle **out)
       (rc = h->fstart.reserve(h->nfine + 1)) || (rc = h->flags.reserve(h->ncells_tot)) ||
       (rc = h->slot.reserve(h->ncells_tot)) || (rc = h->active.reserve(h->ncells_tot)) ||
       (rc = h->leaves.reserve(h->ncells_tot)) || (rc = h->counts.reserve(kMaxLevels + 4)) ||
-      (rc = h->d_err.reserve(1)))
+      (rc = h->d_err.reserve(1)) || (rc = h->n_units.reserve(2)))
     return fail(rc);
   // scratch for the device-wide primitives, sized once so that later calls
   // never reallocate while work is in flight
@@ -1054,6 +1040,15 @@ static int fmm_create(const dust_fmm_params *prm, int capacity, FmmHandle **out)
         hipFuncSetAttribute((const void *)k_m2l_fp64, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)m2l_lds) != hipSuccess) {
       set_error("cannot reserve %zu bytes of LDS for M2L", m2l_lds);
+      return fail(DUST_E_HIP);
+    }
+  }
+  {
+    size_t ev = eval_finish_lds(nm);
+    if (ev > 64 * 1024 &&
+        hipFuncSetAttribute((const void *)k_eval_finish, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)ev) != hipSuccess) {
+      set_error("cannot reserve %zu bytes of LDS for the evaluation", ev);
       return fail(DUST_E_HIP);
     }
   }
@@ -1073,7 +1068,13 @@ static void fmm_destroy(FmmHandle *h) {
                            &h->vel, &h->str, &h->tv};
   for (auto b : dbl) b->release();
   DevBuf<int> *ints[] = {&h->d_pwr, &h->d_mnT, &h->idx_in, &h->perm, &h->fstart, &h->slot,
-                         &h->active, &h->leaves, &h->counts, &h->p2p_cnt, &h->d_err};
+                         &h->active, &h->leaves, &h->counts, &h->p2p_cnt, &h->d_err,
+                         &h->n_units};
+  h->units.release();
+  h->items.release();
+  h->part.release();
+  h->p2p_tot.release();
+  h->leaf_ros.release();
   for (auto b : ints) b->release();
   h->key.release();
   h->key_s.release();
@@ -1153,16 +1154,36 @@ static int fmm_sort(FmmHandle *h, int *nleaves_out) {
   // P2P lists
   DUST_TRY(h->p2p_rng.reserve((size_t)std::max(h->n_leaves, 1) * h->p2p_cap));
   DUST_TRY(h->p2p_cnt.reserve(std::max(h->n_leaves, 1)));
+  DUST_TRY(h->p2p_tot.reserve(std::max(h->n_leaves, 1)));
   if (h->n_leaves > 0)
     k_p2p_lists<<<ceil_div(h->n_leaves, TB), TB, 0, h->stream>>>(
         G, h->n_leaves, h->leaves.p, h->flags.p, h->fstart.p, h->p2p_cap, h->p2p_rng.p,
-        h->p2p_cnt.p);
+        h->p2p_cnt.p, h->p2p_tot.p);
+  // evaluation work units and their items (sized after the next synchronisation)
+  h->units_cap = h->n_leaves + n / kWave + 1;
+  DUST_TRY(h->units.reserve((size_t)h->units_cap * 2));
+  DUST_TRY(h->leaf_ros.reserve(std::max(h->n_leaves, 1)));
+  DUST_HIP(hipMemsetAsync(h->n_units.p, 0, 2 * sizeof(int), h->stream));
+  if (h->n_leaves > 0)
+    k_units<<<ceil_div(h->n_leaves, TB), TB, 0, h->stream>>>(
+        G, h->n_leaves, h->leaves.p, h->fstart.p, h->p2p_tot.p, (EvalUnit *)h->units.p,
+        h->n_units.p, h->n_units.p + 1);
   // expansion storage
   size_t na = (size_t)std::max(h->n_active, 1);
   DUST_TRY(h->mp_a.reserve(na * 3 * h->nmon));
   DUST_TRY(h->mp_b.reserve(na * 3 * h->nmon));
   DUST_TRY(h->mp_c.reserve(na * 9 * h->nmon));
+  int hu[2] = {0, 0};
+  DUST_HIP(hipMemcpyAsync(hu, h->n_units.p, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
   DUST_HIP(hipStreamSynchronize(h->stream));
+  h->n_unit = hu[0];
+  h->n_item = hu[1];
+  DUST_TRY(h->items.reserve((size_t)std::max(h->n_item, 1) * 4));
+  DUST_TRY(h->part.reserve((size_t)std::max(h->n_item, 1) * kWave * 6));
+  if (h->n_unit > 0)
+    k_items<<<ceil_div(h->n_unit, TB), TB, 0, h->stream>>>(h->n_unit, (const EvalUnit *)h->units.p,
+                                                          h->p2p_tot.p, (EvalItem *)h->items.p);
+  DUST_HIP(hipGetLastError());
   timer_end(h, DUST_FMM_STAGE_SORT);
   h->sorted = true;
   h->computed = false;
@@ -1198,21 +1219,28 @@ static int fmm_multipole(FmmHandle *h) {
                                        h->act_start[l], h->flags.p, h->slot.p, h->mp_a.p);
   }
   timer_end(h, DUST_FMM_STAGE_M2M);
+  const bool tuned = h->prm.precision == DUST_FMM_TUNED;
+  if (tuned) {
+    // translations of every level below the first in one launch (they only
+    // read the multipoles); the L2L chain adds the parents afterwards
+    timer_begin(h, DUST_FMM_STAGE_M2L);
+    DUST_TRY(tuned_pass_prepare(h));
+    DUST_TRY(tuned_m2l_all(h));
+    timer_end(h, DUST_FMM_STAGE_M2L);
+  }
   for (int l = 0; l < h->nlev; ++l) {
     int na = h->act_start[l + 1] - h->act_start[l];
-    timer_begin(h, DUST_FMM_STAGE_M2L);
-    if (na > 0) {
-      if (h->prm.precision == DUST_FMM_TUNED) {
-        DUST_TRY(tuned_m2l_level(h, l));
-      } else {
-        size_t sh = sizeof(double) * (12 * h->nU + 3 * nm);
-        k_m2l_fp64<<<na, tpb, sh, h->stream>>>(G, l, nm, h->nU, use_vs, h->d_pwr.p, h->d_multT.p,
-                                               h->d_mnT.p, h->d_kd.p, h->active.p, h->act_start[l],
-                                               h->flags.p, h->slot.p, h->mp_a.p, h->mp_b.p,
-                                               h->mp_c.p);
-      }
+    // level 1 has no parent: its interaction list (nbox > 1) is irregular
+    // and small, and runs in FP64 in both forms
+    if (na > 0 && (!tuned || l == 0)) {
+      timer_begin(h, DUST_FMM_STAGE_M2L);
+      size_t sh = sizeof(double) * (12 * h->nU + 3 * nm);
+      k_m2l_fp64<<<na, tpb, sh, h->stream>>>(G, l, nm, h->nU, use_vs, h->d_pwr.p, h->d_multT.p,
+                                             h->d_mnT.p, h->d_kd.p, h->active.p, h->act_start[l],
+                                             h->flags.p, h->slot.p, h->mp_a.p, h->mp_b.p,
+                                             h->mp_c.p);
+      timer_end(h, DUST_FMM_STAGE_M2L);
     }
-    timer_end(h, DUST_FMM_STAGE_M2L);
     if (l + 1 < h->nlev) {
       int nc = h->act_start[l + 2] - h->act_start[l + 1];
       timer_begin(h, DUST_FMM_STAGE_L2L);
@@ -1227,6 +1255,54 @@ static int fmm_multipole(FmmHandle *h) {
   return 0;
 }
 
+void make_eval_args(const FmmHandle *h, EvalArgs *Ap) {
+  EvalArgs &A = *Ap;
+  A.G = make_geom(h);
+  A.deg = h->deg;
+  A.nmon = h->nmon;
+  A.use_vs = h->prm.use_vs;
+  A.use_vd = h->prm.use_vd;
+  A.use_tv = h->prm.use_tv;
+  A.nu_inf = h->prm.nu_inf;
+  for (int d = 0; d < 3; ++d) A.uinf[d] = h->prm.u_inf[d];
+  A.pwr = h->d_pwr.p; A.leaves = h->leaves.p; A.slot = h->slot.p; A.fstart = h->fstart.p;
+  A.mp_b = h->mp_b.p; A.mp_c = h->mp_c.p;
+  A.px = h->px.p; A.py = h->py.p; A.pz = h->pz.p; A.dx = h->dx.p; A.dy = h->dy.p; A.dz = h->dz.p;
+  A.pm = h->pm.p; A.pr = h->pr.p; A.rng = h->p2p_rng.p; A.cnt = h->p2p_cnt.p; A.cap = h->p2p_cap;
+  A.units = (const EvalUnit *)h->units.p;
+  A.items = (const EvalItem *)h->items.p;
+  A.n_units = h->n_unit;
+  A.n_items = h->n_item;
+  A.leaf_ros = h->leaf_ros.p;
+  A.near2 = 0.0;
+  A.part = h->part.p;
+  A.vel = h->vel_s.p; A.str = h->str_s.p; A.tv = h->tv_s.p;
+}
+
+int eval_leaf_ros(FmmHandle *h, const EvalArgs &A) {
+  if (!(h->prm.use_vd && h->prm.use_tv) || h->n_leaves == 0) return 0;
+  size_t sh = sizeof(double) * kEvalWaves * 12 * h->nmon + sizeof(int) * 3 * h->nmon;
+  k_leaf_ros<<<ceil_div(h->n_leaves, kEvalWaves), kWave * kEvalWaves, sh, h->stream>>>(A, h->n_leaves);
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
+
+int eval_fp64_launch(FmmHandle *h, const EvalArgs &A) {
+  DUST_TRY(eval_leaf_ros(h, A));
+  if (A.n_items > 0)
+    k_p2p_fp64<<<ceil_div(A.n_items, kEvalWaves), kWave * kEvalWaves, 0, h->stream>>>(A);
+  DUST_HIP(hipGetLastError());
+  return eval_finish_launch(h, A);
+}
+
+int eval_finish_launch(FmmHandle *h, const EvalArgs &A) {
+  size_t sh = eval_finish_lds(h->nmon);
+  if (A.n_units > 0)
+    k_eval_finish<<<ceil_div(A.n_units, kEvalWaves), kWave * kEvalWaves, sh, h->stream>>>(A);
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
+
 // apply_multipole (mod_octree.f90:1139-1429), particle terms + free stream
 static int fmm_apply(FmmHandle *h) {
   DUST_CHECK(h && h->computed, DUST_E_ARG, "calculate_multipole must precede apply_multipole");
@@ -1237,18 +1313,8 @@ static int fmm_apply(FmmHandle *h) {
     DUST_TRY(tuned_eval(h));
   } else {
     EvalArgs A;
-    A.G = make_geom(h);
-    A.deg = h->deg; A.nmon = h->nmon;
-    A.use_vs = h->prm.use_vs; A.use_vd = h->prm.use_vd; A.use_tv = h->prm.use_tv;
-    A.nu_inf = h->prm.nu_inf;
-    for (int d = 0; d < 3; ++d) A.uinf[d] = h->prm.u_inf[d];
-    A.pwr = h->d_pwr.p; A.leaves = h->leaves.p; A.slot = h->slot.p; A.fstart = h->fstart.p;
-    A.mp_b = h->mp_b.p; A.mp_c = h->mp_c.p;
-    A.px = h->px.p; A.py = h->py.p; A.pz = h->pz.p; A.dx = h->dx.p; A.dy = h->dy.p; A.dz = h->dz.p;
-    A.pm = h->pm.p; A.pr = h->pr.p; A.rng = h->p2p_rng.p; A.cnt = h->p2p_cnt.p; A.cap = h->p2p_cap;
-    A.vel = h->vel_s.p; A.str = h->str_s.p; A.tv = h->tv_s.p;
-    size_t sh = sizeof(double) * (12 * h->nmon + 8 * kEvalThreads) + sizeof(int) * 3 * h->nmon;
-    if (h->n_leaves > 0) k_eval_fp64<<<h->n_leaves, kEvalThreads, sh, h->stream>>>(A);
+    make_eval_args(h, &A);
+    DUST_TRY(eval_fp64_launch(h, A));
   }
   timer_end(h, DUST_FMM_STAGE_P2P);
   k_scatter<<<ceil_div(n, 256), 256, 0, h->stream>>>(n, h->perm.p, h->vel_s.p, h->str_s.p,
@@ -1362,6 +1428,11 @@ int dust_fmm_synchronize(dust_fmm_t h) {
   DUST_HIP(hipStreamSynchronize(h->h->stream));
   timer_collect(h->h);
   return 0;
+}
+
+int dust_fmm_set_tuning(dust_fmm_t h, const char *name, double value) {
+  DUST_CHECK(h && name, DUST_E_ARG, "null argument");
+  return tuned_set(h->h, name, value);
 }
 
 int dust_fmm_set_timing(dust_fmm_t h, int enable) {

@@ -73,10 +73,17 @@ struct FmmHandle {
   DevBuf<int> counts;        // device scratch for list sizes
   int n_active = 0, n_leaves = 0;
   int act_start[kMaxLevels + 1];
-  // P2P lists
+  // P2P lists and evaluation work units
   int p2p_cap = 0;
   DevBuf<int2> p2p_rng;
   DevBuf<int> p2p_cnt;
+  DevBuf<int> p2p_tot;     // sources in every leaf's P2P list
+  DevBuf<int4> units;      // EvalUnit records (2 x int4 each)
+  DevBuf<int4> items;      // EvalItem records
+  DevBuf<int> n_units;     // device-side unit and item counts
+  DevBuf<double> part;     // per item partial P2P sums
+  DevBuf<double> leaf_ros;
+  int units_cap = 0, n_unit = 0, n_item = 0;
   // expansions
   DevBuf<double> mp_a, mp_b, mp_c;
   // outputs
@@ -94,9 +101,18 @@ int prim_reserve(FmmHandle *h);
 int prim_sort_pairs(FmmHandle *h, int n, int key_bits);
 int prim_select_flagged_cells(FmmHandle *h, int want_leaf_only, int *out, int *d_count);
 
+// fmm.hip: evaluation helpers shared with the tuned form
+struct EvalArgs;
+void make_eval_args(const FmmHandle *h, EvalArgs *A);
+int eval_leaf_ros(FmmHandle *h, const EvalArgs &A);
+int eval_fp64_launch(FmmHandle *h, const EvalArgs &A);
+int eval_finish_launch(FmmHandle *h, const EvalArgs &A);
+
 // fmm_tuned.hip
 int tuned_prepare(FmmHandle *h);
-int tuned_m2l_level(FmmHandle *h, int lvl);
+int tuned_pass_prepare(FmmHandle *h);
+int tuned_set(FmmHandle *h, const char *name, double value);
+int tuned_m2l_all(FmmHandle *h);
 int tuned_eval(FmmHandle *h);
 void tuned_release(FmmHandle *h);
 

@@ -1,13 +1,534 @@
-// Tuned evaluation form of the FMM (placeholder until the kernels land).
-#include "fmm_internal.h"
+// Tuned evaluation form of the FMM on gfx950.
+//
+//  * M2L on the matrix cores: for each level and each octant of the target
+//    inside its parent, every active target has the same 189 interaction
+//    offsets (children of the parent's neighbours that are not neighbours,
+//    mod_octree.f90:441-476).  The translation of M2L_multipole
+//    (mod_multipole.f90:282-356) for one offset is a fixed matrix
+//        [b(:,m); c(:,:,m)] += (-1)^|m| sum_n mult(m,n) cross(D(m+n) | Dc(:,i,m+n), a(:,n))
+//    of 12 n_mon rows by 3 n_mon columns, so a whole (level, octant) group is
+//    one GEMM  Y(12 n_mon x targets) += [W_o1 ... W_o189] x [A_o1; ...; A_o189]
+//    with K = 189 x 3 n_mon.  The W tiles are generated on the fly in LDS from
+//    the compact kernel-derivative table, the B operand gathers the FP32
+//    source multipoles, and v_mfma_f32_16x16x4f32 accumulates in FP32; the
+//    result is added into the FP64 local expansions.
+//  * P2P in FP32: coordinates relative to the target leaf centre, rsqrt on
+//    the transcendental unit, per-tile FP32 partial sums folded into FP64
+//    accumulators.  Pairs closer than the source core radius (times a
+//    factor) are recomputed in FP64 for the whole wavefront.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "fmm_eval.h"
 
 namespace dust {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kOct = 189;  // interaction offsets per target octant
+
+struct TunedState {
+  DevBuf<float> a32;        // FP32 copy of the multipole coefficients, (3 n_mon) per slot
+  DevBuf<int> grp;          // target slots grouped by (level, octant)
+  DevBuf<int> src;          // per grouped target: 189 source slots (-1 = none)
+  DevBuf<int> gcount;       // [kMaxLevels * 8] group sizes (device)
+  DevBuf<int> offs;         // [8][189][3] offsets of each octant
+  DevBuf<float> W;          // M2L operators of every level and offset, fragment order
+  DevBuf<unsigned char> jobs;  // M2L job list of the current pass
+  int hcount[kMaxLevels * 8];
+  long gbase[kMaxLevels * 8];  // first entry of every group in grp
+  double near_factor = 1.0;
+};
+
+static TunedState *TS(FmmHandle *h) { return (TunedState *)h->tuned; }
+static int build_w(FmmHandle *h);
+
 int tuned_prepare(FmmHandle *h) {
-  (void)h;
-  set_error("tuned FMM form not built");
-  return DUST_E_UNSUPPORTED;
+  TunedState *t = new TunedState();
+  h->tuned = t;
+  // offsets of every octant, in (k, j, i) order of the parent's neighbours
+  std::vector<int> offs(8 * kOct * 3);
+  for (int g = 0; g < 8; ++g) {
+    int gx = g & 1, gy = (g >> 1) & 1, gz = (g >> 2) & 1;
+    int w = 0;
+    for (int dk = -2 - gz; dk <= 3 - gz; ++dk)
+      for (int dj = -2 - gy; dj <= 3 - gy; ++dj)
+        for (int di = -2 - gx; di <= 3 - gx; ++di) {
+          if (std::abs(di) <= 1 && std::abs(dj) <= 1 && std::abs(dk) <= 1) continue;
+          offs[(g * kOct + w) * 3 + 0] = di;
+          offs[(g * kOct + w) * 3 + 1] = dj;
+          offs[(g * kOct + w) * 3 + 2] = dk;
+          ++w;
+        }
+    if (w != kOct) {
+      set_error("internal: octant offset count %d", w);
+      return DUST_E_HIP;
+    }
+  }
+  DUST_TRY(t->offs.reserve(offs.size()));
+  DUST_HIP(hipMemcpy(t->offs.p, offs.data(), offs.size() * sizeof(int), hipMemcpyHostToDevice));
+  DUST_TRY(t->gcount.reserve(kMaxLevels * 8));
+  return build_w(h);
 }
-int tuned_m2l_level(FmmHandle *, int) { return DUST_E_UNSUPPORTED; }
-int tuned_eval(FmmHandle *) { return DUST_E_UNSUPPORTED; }
-void tuned_release(FmmHandle *) {}
+
+// Knobs of the tuned form (set by the autotuner or the caller).
+int tuned_set(FmmHandle *h, const char *name, double value) {
+  TunedState *t = TS(h);
+  DUST_CHECK(t, DUST_E_ARG, "the handle was not created with DUST_FMM_TUNED");
+  if (!strcmp(name, "near_factor")) {
+    DUST_CHECK(value >= 0.0, DUST_E_ARG, "near_factor must be >= 0");
+    t->near_factor = value;
+    return 0;
+  }
+  set_error("unknown tuning knob '%s'", name);
+  return DUST_E_ARG;
+}
+
+void tuned_release(FmmHandle *h) {
+  TunedState *t = TS(h);
+  if (!t) return;
+  t->a32.release();
+  t->grp.release();
+  t->src.release();
+  t->gcount.release();
+  t->offs.release();
+  t->W.release();
+  t->jobs.release();
+  delete t;
+  h->tuned = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Grouping of the targets of every level by octant, with their source slots
+// ---------------------------------------------------------------------------
+__global__ void k_group_targets(TreeGeom G, int lvl, const int *__restrict__ active, int a0, int na,
+                                const int *__restrict__ slot, const unsigned char *__restrict__ flags,
+                                const int *__restrict__ offs, int *grp, int *src, long gb0,
+                                long gstride, int *gcount) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= na) return;
+  long g = active[a0 + q];
+  long c = g - G.off[lvl];
+  int ix, iy, iz;
+  cell_decode(G, lvl, c, ix, iy, iz);
+  int oc = (ix & 1) + 2 * (iy & 1) + 4 * (iz & 1);
+  int w = atomicAdd(&gcount[lvl * 8 + oc], 1);
+  long e = gb0 + oc * gstride + w;
+  grp[e] = slot[g];
+  int *sr = &src[e * kOct];
+  for (int j = 0; j < kOct; ++j) {
+    int sx = ix + offs[(oc * kOct + j) * 3], sy = iy + offs[(oc * kOct + j) * 3 + 1],
+        sz = iz + offs[(oc * kOct + j) * 3 + 2];
+    int s = -1;
+    if (sx >= 0 && sy >= 0 && sz >= 0 && sx < G.ncl[lvl][0] && sy < G.ncl[lvl][1] &&
+        sz < G.ncl[lvl][2]) {
+      long gs = G.off[lvl] + cell_encode(G, lvl, sx, sy, sz);
+      if (flags[gs]) s = slot[gs];
+    }
+    sr[j] = s;
+  }
+}
+
+__global__ void k_to_f32(long n, const double *__restrict__ a, float *b) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+// ---------------------------------------------------------------------------
+// M2L as one GEMM per (level, octant)
+//
+// Operator W_o (rows: rpm per monomial m, rpm = 12 with stretching, 3 without;
+// columns: 3 per monomial n, the components of a(:,n)) is stored once per
+// handle for every level and offset in MFMA fragment order: for row tile rt,
+// k step ks and lane l, element (row 16 rt + l % 16, col 4 ks + l / 16), so
+// that a wavefront loads one fragment with one coalesced 256-byte access.
+// ---------------------------------------------------------------------------
+constexpr int kNT = 64;       // targets per block (4 column tiles of 16)
+constexpr int kRTW = 7;       // row tiles per wavefront
+constexpr int kRTB = 4 * kRTW;
+
+struct WGeom {
+  int rpm, nrows, K, rt, ks;  // rows per monomial, rows, K, row tiles, k steps
+  long per_off;               // floats per offset
+};
+
+static WGeom wgeom(const FmmHandle *h) {
+  WGeom w;
+  w.rpm = h->prm.use_vs ? 12 : 3;
+  w.nrows = w.rpm * h->nmon;
+  w.K = 3 * h->nmon;
+  w.rt = (w.nrows + 15) / 16;
+  w.ks = (w.K + 3) / 4;
+  w.per_off = (long)w.rt * w.ks * 64;
+  return w;
+}
+
+// Levi-Civita symbol for distinct indices in 0..2.
+__device__ inline double eps3(int a, int b) { return ((b - a + 3) % 3 == 1) ? 1.0 : -1.0; }
+
+// W_o(row, col) = (-1)^|m| mult(m,n) eps(o3, c', e) T_(c'[,i])(m+n), with
+// row = (m, comp), col = (n, e): comp < 3 gives b(comp, m) from D(c');
+// comp = 3 + 3 j + i gives c(i, j, m) from Dc(c', i) (cross product index j).
+__global__ void k_build_w(WGeom w, int nmon, int nU, long ntab, const double *__restrict__ kd,
+                          const double *__restrict__ multT, const int *__restrict__ mnT,
+                          const int *__restrict__ pwr, float *W) {
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ntab * w.per_off) return;
+  long tab = q / w.per_off;
+  long r = q % w.per_off;
+  int lane = (int)(r & 63);
+  long fr = r >> 6;
+  int ks = (int)(fr % w.ks), rt = (int)(fr / w.ks);
+  int row = rt * 16 + (lane & 15), col = ks * 4 + (lane >> 4);
+  float v = 0.f;
+  if (row < w.nrows && col < w.K) {
+    int m = row / w.rpm, comp = row % w.rpm;
+    int n = col / 3, e = col % 3;
+    int o3 = comp < 3 ? comp : (comp - 3) / 3;
+    if (o3 != e) {
+      int cp = 3 - o3 - e;
+      int id = mnT[n * nmon + m];
+      double mu = multT[n * nmon + m];
+      int pw = pwr[3 * m] + pwr[3 * m + 1] + pwr[3 * m + 2];
+      int trow = comp < 3 ? cp : 3 + cp + 3 * ((comp - 3) % 3);
+      double sg = (pw & 1) ? -1.0 : 1.0;
+      v = (float)(sg * eps3(o3, cp) * mu * kd[tab * 12 * (long)nU + (long)trow * nU + id]);
+    }
+  }
+  W[q] = v;
+}
+
+// M2L operators of all levels and offsets (level 1 uses the FP64 kernel; its
+// tables are built too so that every table index is valid).
+static int build_w(FmmHandle *h) {
+  TunedState *t = TS(h);
+  WGeom w = wgeom(h);
+  long nf = h->kd_tables * w.per_off;
+  DUST_TRY(t->W.reserve(nf));
+  k_build_w<<<ceil_div(nf, 256), 256, 0, h->stream>>>(w, h->nmon, h->nU, h->kd_tables, h->d_kd.p,
+                                                     h->d_multT.p, h->d_mnT.p, h->d_pwr.p, t->W.p);
+  DUST_HIP(hipGetLastError());
+  DUST_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// One job = (level, octant, 64 targets, a chunk of kJC offsets, a block of
+// row tiles); all jobs of a pass run in one launch and add their partial
+// results into the FP64 local expansions with atomics (M2L only reads the
+// multipoles, so every level can be translated before the L2L chain).
+constexpr int kJC = 21;  // offsets per job (189 = 9 x 21)
+
+struct M2LJob {
+  int lvl, oct, t0, j0, rb, nt;  // nt = targets in the group
+  long grp_off;                  // first grouped target of (lvl, oct)
+  long kd_base;
+  int kdl0, kdl1, kdl2, pad;
+};
+
+struct M2LArgs {
+  WGeom w;
+  int nmon;
+  const M2LJob *jobs;
+  const int *grp, *src, *offs;
+  const float *W;
+  const float *a32;
+  double *mp_b, *mp_c;
+};
+
+__global__ void __launch_bounds__(256) k_m2l_mfma(M2LArgs P) {
+  extern __shared__ float sB[];  // [ks * 4][kNT + 4]
+  __shared__ int sSrc[kNT];
+  const M2LJob J = P.jobs[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int rt0 = J.rb * kRTB + wv * kRTW;                 // first row tile of this wave
+  const int nrt = max(0, min(kRTW, P.w.rt - rt0));         // row tiles of this wave
+  const int K = P.w.K, KS = P.w.ks, ld = kNT + 4;
+  const int *grp = P.grp + J.grp_off;
+  const int *src = P.src + J.grp_off * kOct;
+  f4 acc[kRTW][4];
+#pragma unroll
+  for (int a = 0; a < kRTW; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+  const int j1 = min(kOct, J.j0 + kJC);
+  for (int j = J.j0; j < j1; ++j) {
+    const int di = P.offs[(J.oct * kOct + j) * 3], dj = P.offs[(J.oct * kOct + j) * 3 + 1],
+              dk = P.offs[(J.oct * kOct + j) * 3 + 2];
+    const long tab = J.kd_base + (di + J.kdl0) +
+                     (long)(2 * J.kdl0 + 1) * ((dj + J.kdl1) + (long)(2 * J.kdl1 + 1) * (dk + J.kdl2));
+    const float *Wo = P.W + tab * P.w.per_off;
+    __syncthreads();
+    if (tid < kNT) {
+      int t = J.t0 + tid;
+      sSrc[tid] = (t < J.nt) ? src[(long)t * kOct + j] : -1;
+    }
+    __syncthreads();
+    // gather the source multipoles of the 64 targets: sB[k][t]
+    for (int q = tid; q < kNT * KS * 4; q += 256) {
+      const int tt = q / (KS * 4), k = q % (KS * 4);
+      const int s = sSrc[tt];
+      sB[k * ld + tt] = (s >= 0 && k < K) ? P.a32[(long)s * K + k] : 0.f;
+    }
+    __syncthreads();
+    if (nrt > 0) {
+      const float *Wl = Wo + ((long)rt0 * KS) * 64 + lane;
+      for (int ks = 0; ks < KS; ++ks) {
+        float bf[4];
+        const int kk = ks * 4 + (lane >> 4);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) bf[b] = sB[kk * ld + b * 16 + (lane & 15)];
+        float af[kRTW];
+#pragma unroll
+        for (int a = 0; a < kRTW; ++a) af[a] = (a < nrt) ? Wl[((long)a * KS + ks) * 64] : 0.f;
+#pragma unroll
+        for (int a = 0; a < kRTW; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bf[b], acc[a][b], 0, 0, 0);
+      }
+    }
+  }
+  // FP32 accumulator layout: lane holds column lane % 16, rows 4 (lane / 16) + i
+  const int nm = P.nmon, rpm = P.w.rpm;
+#pragma unroll
+  for (int a = 0; a < kRTW; ++a) {
+    if (a >= nrt) continue;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int t = J.t0 + b * 16 + (lane & 15);
+      if (t >= J.nt) continue;
+      const long sl = grp[t];
+      for (int i = 0; i < 4; ++i) {
+        const int row = (rt0 + a) * 16 + 4 * (lane >> 4) + i;
+        if (row >= P.w.nrows) continue;
+        const int m = row / rpm, comp = row % rpm;
+        const double v = (double)acc[a][b][i];
+        if (comp < 3)
+          atomicAdd(&P.mp_b[sl * 3 * nm + 3 * m + comp], v);
+        else  // comp - 3 = 3 j + i holds c(i, j, m), stored at i + 3 j
+          atomicAdd(&P.mp_c[sl * 9 * nm + 9 * m + (comp - 3)], v);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 P2P with FP64 near pairs
+// ---------------------------------------------------------------------------
+__device__ inline void pair64(double x, double y, double z, double al0, double al1, double al2,
+                              double sx, double sy, double sz, double sdx, double sdy, double sdz,
+                              double smg, double srv, int use_vs, double v[3], double s[3]) {
+  double r0 = x - sx, r1 = y - sy, r2 = z - sz;
+  double R2 = r0 * r0 + r1 * r1 + r2 * r2 + srv * srv;
+  double R3 = R2 * sqrt(R2);
+  double cf = -1.0 / R3;
+  v[0] += cf * (r1 * sdz - r2 * sdy) * smg;
+  v[1] += cf * (r2 * sdx - r0 * sdz) * smg;
+  v[2] += cf * (r0 * sdy - r1 * sdx) * smg;
+  if (use_vs) {
+    double df = 3.0 / (R3 * R2);
+    double am0 = sdx * smg, am1 = sdy * smg, am2 = sdz * smg;
+    double vp0 = al1 * am2 - al2 * am1, vp1 = al2 * am0 - al0 * am2, vp2 = al0 * am1 - al1 * am0;
+    double dv = r0 * vp0 + r1 * vp1 + r2 * vp2;
+    s[0] -= cf * vp0 + df * r0 * dv;
+    s[1] -= cf * vp1 + df * r1 * dv;
+    s[2] -= cf * vp2 + df * r2 * dv;
+  }
+}
+
+// FP32 P2P of one item (tuned form).  Sources are staged as FP32 offsets
+// from the target leaf centre (computed in FP64), strengths alpha = dir*mag
+// and r^2; every tile's FP32 sums are folded into FP64 accumulators.  Pairs
+// with d^2 < near2 r_s^2 are evaluated in FP64 instead.
+__global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
+  __shared__ float ssm[kEvalWaves][7 * kWave];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float *ss = ssm[wv];
+  const int it = blockIdx.x * kEvalWaves + wv;
+  if (it >= A.n_items) return;
+  const EvalItem I = A.items[it];
+  const EvalUnit U = A.units[I.unit];
+  const long g = A.leaves[U.leaf];
+  const int lvl = lvl_of(A.G, g);
+  const long c = g - A.G.off[lvl];
+  int ix, iy, iz;
+  cell_decode(A.G, lvl, c, ix, iy, iz);
+  const double cx = cell_center(A.G, lvl, 0, ix), cy = cell_center(A.G, lvl, 1, iy),
+               cz = cell_center(A.G, lvl, 2, iz);
+  const int lpt = U.lpt, tl = lane / lpt, sub = lane - tl * lpt;
+  const bool valid = tl < U.nt;
+  const int t = U.t0 + (valid ? tl : 0);
+  const double x = A.px[t], y = A.py[t], z = A.pz[t];
+  const double mg = A.pm[t];
+  const double al0 = mg * A.dx[t], al1 = mg * A.dy[t], al2 = mg * A.dz[t];
+  const float fx = (float)(x - cx), fy = (float)(y - cy), fz = (float)(z - cz);
+  const float fa0 = (float)al0, fa1 = (float)al1, fa2 = (float)al2;
+  double pv[3] = {0, 0, 0}, ps[3] = {0, 0, 0};
+  const float near2 = (float)A.near2;
+  const int nr = A.cnt[U.leaf];
+  const int2 *rl = &A.rng[(long)U.leaf * A.cap];
+  int ri, off;
+  seg_seek(rl, nr, I.lo, ri, off);
+  int left = I.hi - I.lo;
+  for (; ri < nr && left > 0; ++ri, off = 0) {
+    const int2 r = rl[ri];
+    const int e = min(r.y, r.x + off + left);
+    for (int s0 = r.x + off; s0 < e; s0 += kWave) {
+      const int ns = min(kWave, e - s0);
+      if (lane < ns) {
+        const int q = s0 + lane;
+        const double smg = A.pm[q];
+        ss[0 * kWave + lane] = (float)(A.px[q] - cx);
+        ss[1 * kWave + lane] = (float)(A.py[q] - cy);
+        ss[2 * kWave + lane] = (float)(A.pz[q] - cz);
+        ss[3 * kWave + lane] = (float)(A.dx[q] * smg);
+        ss[4 * kWave + lane] = (float)(A.dy[q] * smg);
+        ss[5 * kWave + lane] = (float)(A.dz[q] * smg);
+        ss[6 * kWave + lane] = (float)(A.pr[q] * A.pr[q]);
+      }
+      wave_sync();
+      float v0 = 0.f, v1 = 0.f, v2 = 0.f, w0 = 0.f, w1 = 0.f, w2 = 0.f;
+      for (int s = sub; s < ns; s += lpt) {
+        const float r0 = fx - ss[s], r1 = fy - ss[kWave + s], r2 = fz - ss[2 * kWave + s];
+        const float a0 = ss[3 * kWave + s], a1 = ss[4 * kWave + s], a2 = ss[5 * kWave + s];
+        const float rs2 = ss[6 * kWave + s];
+        const float d2 = r0 * r0 + r1 * r1 + r2 * r2;
+        const bool self = (s0 + s == t);
+        const bool near = valid && !self && d2 < near2 * rs2;
+        if (near) {
+          // FP64 for the pairs inside the core radius of this source
+          const int q = s0 + s;
+          pair64(x, y, z, al0, al1, al2, A.px[q], A.py[q], A.pz[q], A.dx[q], A.dy[q], A.dz[q],
+                 A.pm[q], A.pr[q], A.use_vs, pv, ps);
+        }
+        const float R2 = d2 + rs2;
+        const float inv = __builtin_amdgcn_rsqf(R2);
+        const float inv2 = inv * inv;
+        float cf = -(inv2 * inv);
+        cf = (valid && !self && !near) ? cf : 0.f;
+        v0 += cf * (r1 * a2 - r2 * a1);
+        v1 += cf * (r2 * a0 - r0 * a2);
+        v2 += cf * (r0 * a1 - r1 * a0);
+        if (A.use_vs) {
+          const float df = -3.f * cf * inv2;
+          const float vp0 = fa1 * a2 - fa2 * a1, vp1 = fa2 * a0 - fa0 * a2, vp2 = fa0 * a1 - fa1 * a0;
+          const float dv = df * (r0 * vp0 + r1 * vp1 + r2 * vp2);
+          w0 -= cf * vp0 + dv * r0;
+          w1 -= cf * vp1 + dv * r1;
+          w2 -= cf * vp2 + dv * r2;
+        }
+      }
+      pv[0] += v0; pv[1] += v1; pv[2] += v2;
+      ps[0] += w0; ps[1] += w1; ps[2] += w2;
+      wave_sync();
+    }
+    left -= e - (r.x + off);
+  }
+  for (int q = 0; q < 3; ++q) {
+    pv[q] = lane_group_sum(pv[q], lpt);
+    if (A.use_vs) ps[q] = lane_group_sum(ps[q], lpt);
+  }
+  if (valid && sub == 0) {
+    const double fi = 1.0 / (4.0 * kPi);
+    double *o = &A.part[((long)it * kWave + tl) * 6];
+    for (int q = 0; q < 3; ++q) {
+      o[q] = pv[q] * fi;
+      o[3 + q] = ps[q] * fi;
+    }
+  }
+}
+
+static size_t m2l_lds(const WGeom &w) { return sizeof(float) * (size_t)w.ks * 4 * (kNT + 4); }
+
+// Per pass, after M2M: targets of every level grouped by octant with their
+// 189 source slots, and the FP32 copy of the multipoles.
+int tuned_pass_prepare(FmmHandle *h) {
+  TunedState *t = TS(h);
+  TreeGeom G = make_geom(h);
+  DUST_HIP(hipMemsetAsync(t->gcount.p, 0, sizeof(int) * kMaxLevels * 8, h->stream));
+  long tot = 0;
+  for (int l = 0; l < h->nlev; ++l) {
+    int na = h->act_start[l + 1] - h->act_start[l];
+    for (int g = 0; g < 8; ++g) t->gbase[l * 8 + g] = tot + (long)g * na;
+    tot += 8L * na;
+  }
+  DUST_TRY(t->grp.reserve(std::max(tot, 1L)));
+  DUST_TRY(t->src.reserve(std::max(tot, 1L) * kOct));
+  for (int l = 1; l < h->nlev; ++l) {
+    int na = h->act_start[l + 1] - h->act_start[l];
+    if (na == 0) continue;
+    k_group_targets<<<ceil_div(na, 256), 256, 0, h->stream>>>(
+        G, l, h->active.p, h->act_start[l], na, h->slot.p, h->flags.p, t->offs.p, t->grp.p,
+        t->src.p, t->gbase[l * 8], na, t->gcount.p);
+  }
+  DUST_HIP(hipMemcpyAsync(t->hcount, t->gcount.p, sizeof(int) * kMaxLevels * 8,
+                          hipMemcpyDeviceToHost, h->stream));
+  // FP32 copy of the multipoles (P2M and M2M are complete at this point)
+  long na3 = (long)h->n_active * 3 * h->nmon;
+  DUST_TRY(t->a32.reserve(std::max(na3, 1L)));
+  k_to_f32<<<ceil_div(na3, 256), 256, 0, h->stream>>>(na3, h->mp_a.p, t->a32.p);
+  DUST_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// M2L of every level >= 2 (1-based) in one launch.
+int tuned_m2l_all(FmmHandle *h) {
+  TunedState *t = TS(h);
+  const WGeom w = wgeom(h);
+  size_t lds = m2l_lds(w);
+  if (lds > 64 * 1024)
+    DUST_HIP(hipFuncSetAttribute((const void *)k_m2l_mfma,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  std::vector<M2LJob> jobs;
+  const int nrb = ceil_div(w.rt, kRTB);
+  for (int lvl = 1; lvl < h->nlev; ++lvl)
+    for (int g = 0; g < 8; ++g) {
+      int nt = t->hcount[lvl * 8 + g];
+      for (int t0 = 0; t0 < nt; t0 += kNT)
+        for (int j0 = 0; j0 < kOct; j0 += kJC)
+          for (int rb = 0; rb < nrb; ++rb) {
+            M2LJob J;
+            J.lvl = lvl; J.oct = g; J.t0 = t0; J.j0 = j0; J.rb = rb; J.nt = nt;
+            J.grp_off = t->gbase[lvl * 8 + g];
+            J.kd_base = h->lev[lvl].kd_base;
+            J.kdl0 = h->lev[lvl].kdl[0]; J.kdl1 = h->lev[lvl].kdl[1]; J.kdl2 = h->lev[lvl].kdl[2];
+            J.pad = 0;
+            jobs.push_back(J);
+          }
+    }
+  if (jobs.empty()) return 0;
+  DUST_TRY(t->jobs.reserve(jobs.size() * sizeof(M2LJob)));
+  DUST_HIP(hipMemcpyAsync(t->jobs.p, jobs.data(), jobs.size() * sizeof(M2LJob),
+                          hipMemcpyHostToDevice, h->stream));
+  M2LArgs P;
+  P.w = w;
+  P.nmon = h->nmon;
+  P.jobs = (const M2LJob *)t->jobs.p;
+  P.grp = t->grp.p;
+  P.src = t->src.p;
+  P.offs = t->offs.p;
+  P.W = t->W.p;
+  P.a32 = t->a32.p;
+  P.mp_b = h->mp_b.p;
+  P.mp_c = h->mp_c.p;
+  k_m2l_mfma<<<(unsigned)jobs.size(), 256, lds, h->stream>>>(P);
+  DUST_HIP(hipGetLastError());
+  // the host vector must outlive the asynchronous copy
+  DUST_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int tuned_eval(FmmHandle *h) {
+  TunedState *t = TS(h);
+  EvalArgs A;
+  make_eval_args(h, &A);
+  if (h->prm.use_vd || h->prm.use_tv) return eval_fp64_launch(h, A);  // PSE runs in FP64
+  A.near2 = t->near_factor * t->near_factor;
+  if (A.n_items > 0)
+    k_p2p_f32<<<ceil_div(A.n_items, kEvalWaves), kWave * kEvalWaves, 0, h->stream>>>(A);
+  DUST_HIP(hipGetLastError());
+  return eval_finish_launch(h, A);
+}
+
 }  // namespace dust

@@ -31,9 +31,11 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // Spins are bounded; a timed-out barrier raises flag[2] and every block
 // leaves the kernel at the next check.
 // ---------------------------------------------------------------------------
+// Only the arriving thread fences at device scope: the workgroup barrier has
+// already made the block's stores visible in L2, and a device-scope release
+// writes the whole L2 back, so one per block is enough.
 __device__ inline bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &gen) {
   __shared__ int ok;
-  __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) {
     ok = 1;
@@ -62,7 +64,6 @@ __device__ inline bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &gen)
     gen += 1;
   }
   __syncthreads();
-  __threadfence();
   return ok != 0;
 }
 
@@ -401,7 +402,7 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t st,
   for (int k = 0; k < mn && !rc; k += nb) {
     const int jb = std::min(nb, mn - k);
     const int mrows = m - k;
-    int G = std::min(Gmax, std::max(1, (mrows + 31) / 32));
+    int G = std::min(std::min(Gmax, 128), std::max(1, (mrows + 63) / 64));
     int R = (mrows + G - 1) / G;
     G = (mrows + R - 1) / R;
     size_t lds = sizeof(double) * ((size_t)R * jb + 2 * jb);

@@ -115,6 +115,10 @@ class Octree:
         L.check(L.lib().dust_fmm_work(self.handle, C.byref(p), C.byref(m)), "work")
         return p.value, m.value
 
+    def set_tuning(self, name, value):
+        """Knob of the tuned form (e.g. near_factor)."""
+        L.check(L.lib().dust_fmm_set_tuning(self.handle, name.encode(), float(value)), "set_tuning")
+
     def set_timing(self, on):
         L.check(L.lib().dust_fmm_set_timing(self.handle, int(bool(on))), "set_timing")
 

@@ -103,6 +103,10 @@ int dust_fmm_results_dev(dust_fmm_t h, double *vel, double *stretch, double *tur
 /* One full pass (sort + multipole + apply) on device-resident particles. */
 int dust_fmm_pass_dev(dust_fmm_t h, int *nleaves);
 
+/* Knobs of the tuned form: "near_factor" -- pairs closer than near_factor
+ * times the source core radius are evaluated in FP64 (default 1). */
+int dust_fmm_set_tuning(dust_fmm_t h, const char *name, double value);
+
 /* HIP stream (hipStream_t) the handle launches on. */
 void *dust_fmm_stream(dust_fmm_t h);
 int dust_fmm_synchronize(dust_fmm_t h);

@@ -28,8 +28,9 @@ def run_both(orc, cen, alpha, rv, tree, degree, **flags):
     nl = O.sort_particles(parts, g)
     O.calculate_multipole(parts, g)
     O.apply_multipole(parts, g)
+    oflags = {k: v for k, v in flags.items() if k != "precision"}
     c = orc.Octree(tree["box_length"], tree["nbox"], tree["origin"], tree["nlevels"],
-                   tree["min_part"], degree, tree["delta"], **flags)
+                   tree["min_part"], degree, tree["delta"], **oflags)
     v, s, tv = c.run(parts.cen, parts.dir, parts.mag, parts.r_vortex)
     return parts, g, nl, c, v, s, tv
 
@@ -167,3 +168,45 @@ def test_repeat_calls_reuse_handle(orc):
         v, s, _ = c.run(parts.cen, parts.dir, parts.mag, parts.r_vortex)
         assert rel(parts.vel, v) <= TOL
         assert rel(parts.stretch, s) <= TOL
+
+
+# ---- tuned form: FP32 P2P with FP64 near pairs + FP32 MFMA M2L -------------------
+TUNED_VEL = 1e-6   # relative L2 velocity error vs the FP64 oracle
+TUNED_STR = 2e-5   # relative L2 stretching error vs the FP64 oracle
+
+
+@pytest.mark.parametrize("nlev,p", [(4, 4), (5, 4), (4, 6), (4, 8), (3, 2)])
+def test_tuned_vs_oracle(orc, nlev, p):
+    cen, alpha, rv, tree = W.helix_wake(12000, seed=11)
+    tree = dict(tree, nlevels=nlev)
+    parts, g, nl, c, v, s, tv = run_both(orc, cen, alpha, rv, tree, p, use_vs=True,
+                                         precision="tuned")
+    ev, es = rel(parts.vel, v), rel(parts.stretch, s)
+    print(f"tuned nlev={nlev} p={p}: vel {ev:.2e} stretch {es:.2e}")
+    assert ev <= TUNED_VEL
+    assert es <= TUNED_STR
+    check_leaves(parts, g, c)
+
+
+def test_tuned_velocity_only_and_nbox(orc):
+    rng = np.random.default_rng(3)
+    n = 5000
+    cen = rng.uniform([0.0, 0.0, 0.0], [3.0, 2.0, 1.0], (n, 3))
+    alpha = rng.normal(0, 1e-3, (n, 3))
+    rv = np.full(n, 0.03)
+    tree = dict(box_length=1.0, nbox=(3, 2, 1), origin=(-1e-7,) * 3, nlevels=4, min_part=16,
+                delta=0.03)
+    parts, g, nl, c, v, s, tv = run_both(orc, cen, alpha, rv, tree, 4, use_vs=False,
+                                         precision="tuned")
+    assert rel(parts.vel, v) <= TUNED_VEL
+
+
+def test_tuned_with_diffusion_falls_back_exact(orc):
+    """The PSE diffusion and turbulent viscosity run in FP64 in the tuned form."""
+    cen, alpha, rv, tree = W.helix_wake(4000, seed=5)
+    tree = dict(tree, nlevels=4)
+    parts, g, nl, c, v, s, tv = run_both(orc, cen, alpha, rv, tree, 4, use_vs=True, use_vd=True,
+                                         use_tv=True, nu_inf=1e-3, precision="tuned")
+    assert rel(parts.vel, v) <= TUNED_VEL
+    assert rel(parts.stretch, s) <= TUNED_STR
+    assert rel(parts.turbvisc, tv) <= 1e-5
