@@ -27,47 +27,6 @@ namespace dust {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
-// Grid barrier for cooperative launches: arrival counter + generation word.
-// Spins are bounded; a timed-out barrier raises flag[2] and every block
-// leaves the kernel at the next check.
-// ---------------------------------------------------------------------------
-// Only the arriving thread fences at device scope: the workgroup barrier has
-// already made the block's stores visible in L2, and a device-scope release
-// writes the whole L2 back, so one per block is enough.
-__device__ inline bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &gen) {
-  __shared__ int ok;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    ok = 1;
-    __threadfence();
-    unsigned prev = atomicAdd(&bar[0], 1u);
-    if (prev == nblocks - 1) {
-      atomicExch(&bar[0], 0u);
-      __threadfence();
-      atomicAdd(&bar[1], 1u);
-    } else {
-      long spins = 0;
-      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins > (1L << 26)) {
-          atomicExch(&bar[2], 1u);
-          ok = 0;
-          break;
-        }
-        if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-          ok = 0;
-          break;
-        }
-      }
-    }
-    __threadfence();
-    gen += 1;
-  }
-  __syncthreads();
-  return ok != 0;
-}
-
-// ---------------------------------------------------------------------------
 // Panel factorisation: columns [k, k+jb) of A, rows [k, m).  Block b owns
 // rows [k + b R, k + (b+1) R).  Per column: local first-maximum of |a| ->
 // candidates published -> barrier -> global pivot (max value, smallest row

@@ -4,6 +4,48 @@
 
 namespace dust {
 
+// ---------------------------------------------------------------------------
+// Grid barrier for cooperative launches: arrival counter + generation word.
+// Spins are bounded; a timed-out barrier raises flag[2] and every block
+// leaves the kernel at the next check.
+// ---------------------------------------------------------------------------
+// Only the arriving thread fences at device scope: the workgroup barrier has
+// already made the block's stores visible in L2, and a device-scope release
+// writes the whole L2 back, so one per block is enough.
+__device__ inline bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &gen) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ok = 1;
+    __threadfence();
+    unsigned prev = atomicAdd(&bar[0], 1u);
+    if (prev == nblocks - 1) {
+      atomicExch(&bar[0], 0u);
+      __threadfence();
+      atomicAdd(&bar[1], 1u);
+    } else {
+      long spins = 0;
+      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > (1L << 26)) {
+          atomicExch(&bar[2], 1u);
+          ok = 0;
+          break;
+        }
+        if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          ok = 0;
+          break;
+        }
+      }
+    }
+    __threadfence();
+    gen += 1;
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+
 // xGETRF semantics on device memory; *info as LAPACK.  nb is the panel width.
 int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t st, int nb,
                  int *info);

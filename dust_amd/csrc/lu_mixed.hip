@@ -1,17 +1,1217 @@
-// Tuned mixed-precision solve (placeholder until the kernels land).
+// Tuned mixed-precision dense solve A X = B for the AIC systems of
+// src/linsys/mod_linsys.f90 (dgetrf 236/435 + dgetrs 473): the LU runs on the
+// matrix cores in FP32 or in row-scaled FP16, the diagonal blocks are
+// factored in FP64, and FP64 residuals drive iterative refinement.
+//
+// Factorisation (np = n rounded up to 128, padded with an identity block):
+//   * W = R A in FP32, R a power-of-two row scaling (max |w_ij| per row in
+//     [1/2, 1)) so that FP16 operands stay in range.
+//   * Block columns of NB = 128.  The diagonal block is factored by one
+//     workgroup in FP64 with row interchanges among its own 128 rows
+//     (pivot = largest magnitude in the block column part).  The multipliers
+//     of the rows below, L21 = A21 U11^-1, are then checked against a
+//     threshold tau: max |l_ij| <= tau means the block pivot dominates the
+//     whole column to within tau (threshold partial pivoting).  If any block
+//     fails the check, or a pivot is zero, the solve is redone by the plain
+//     FP64 partial-pivoting path.
+//   * L21 and U12 come from MFMA triangular solves that use FP64-inverted
+//     32x32 diagonal sub-blocks; the trailing update is delayed over DEPTH
+//     block columns (K = DEPTH * 128 per pass over the trailing matrix) and
+//     runs as an FP16 or FP32 MFMA GEMM with FP32 accumulation.  Operands are
+//     written once per block column in MFMA fragment order.
+//
+// Refinement: x0 = U^-1 L^-1 P R b; r = b - A x in FP64; d = U^-1 L^-1 P R r;
+// x += d, until ||r||_inf / (||A||_inf ||x||_inf) <= tol.  Triangular
+// solves are one persistent launch each: block row kb waits on a per-block
+// ready flag of every block row it depends on (generation-stamped, no reset).
+//
+// The workspace is process-wide and grows on demand; calls are not
+// re-entrant.
+#include <hip/hip_cooperative_groups.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <utility>
+#include <vector>
+
+#include "common.h"
 #include "lu_internal.h"
+
+namespace dust {
+namespace {
+
+constexpr int NB = 128;         // diagonal block
+constexpr int SP = 32;          // sub-panel / inverted sub-block
+constexpr int LDD = NB + 1;     // LDS leading dimension of the FP64 diagonal block
+constexpr int DEPTH = 4;        // block columns per trailing update
+constexpr int NR = 8;           // right-hand sides per solve pass
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ inline void wsync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline double rdlane(double v, int l) {
+  unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  unsigned lo = __builtin_amdgcn_readlane((unsigned)u, l);
+  unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Wavefront maximum of a 32-bit key by DPP row shifts and row broadcasts.
+__device__ inline unsigned wave_umax(unsigned v) {
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Pivot key: FP32 magnitude with its 7 low mantissa bits replaced by
+// (127 - row), so the maximum picks the largest |a| and, among values equal
+// to 2^-16 relative, the smallest row.
+__device__ inline unsigned piv_key(double v, int row) {
+  return (__float_as_uint(fabsf((float)v)) & ~127u) | (unsigned)(127 - row);
+}
+
+__device__ inline void wave_max_atomic(float v, unsigned *dst) {
+  unsigned k = wave_umax(__float_as_uint(fabsf(v)));
+  if ((threadIdx.x & 63) == 0) atomicMax(dst, k);
+}
+
+// ---------------------------------------------------------------------------
+// Conversion: row maxima of |A|, then W = R A (FP32, padded identity).
+// ---------------------------------------------------------------------------
+__global__ void k_mx_fill_u32(unsigned *p, int n, unsigned v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_mx_rowmax(const double *__restrict__ A, long lda, int n, int cw,
+                            unsigned *rowmax) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int j0 = blockIdx.y * cw, j1 = min(n, j0 + cw);
+  float m = 0.f;
+  for (int j = j0; j < j1; ++j) m = fmaxf(m, fabsf((float)A[i + lda * j]));
+  atomicMax(&rowmax[i], __float_as_uint(m));
+}
+
+__global__ void k_mx_rowscale(const unsigned *rowmax, int n, int np, int scaled, double *rs) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double s = 1.0;
+  if (scaled && i < n) {
+    float m = __uint_as_float(rowmax[i]);
+    if (m > 0.f && isfinite(m)) {
+      int e;
+      (void)frexpf(m, &e);  // m = f 2^e, f in [1/2, 1)
+      s = ldexp(1.0, -e);
+    }
+  }
+  rs[i] = s;
+}
+
+__global__ void k_mx_convert(const double *__restrict__ A, long lda, int n, int np,
+                             const double *__restrict__ rs, float *W) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j0 = blockIdx.y * 4;
+  if (i >= np) return;
+  double s = rs[i];
+  for (int j = j0; j < j0 + 4; ++j) {
+    float v;
+    if (i < n && j < n) v = (float)(A[i + lda * j] * s);
+    else v = (i == j) ? 1.f : 0.f;
+    W[i + (long)np * j] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal block: FP64 LU with interchanges among the block's rows, plus the
+// FP64 inverses of the four 32x32 diagonal sub-blocks of L and U.
+// ---------------------------------------------------------------------------
+struct DiagArgs {
+  float *W;
+  long ld;
+  int k;          // first row/column of the block
+  int *piv;       // [np]: global 0-based pivot row of each row (LAPACK order)
+  int *perm;      // [np]: factored row -> original row
+  int *npiv;      // [nblk]: interchanges in the block
+  float *dinv;    // [nblk][8][32*32]: Linv sub-blocks 0..3, Uinv sub-blocks 4..7
+  unsigned *flags;  // [0] max |l| bits, [1] zero pivot
+};
+
+// Wave 0: factor columns [c0, c0+32) of the block, rows [c0, 128), held in
+// registers (lane l: rows l and l + 64).  One template instance per column
+// keeps every register index a compile-time constant.
+template <int J>
+__device__ __forceinline__ void subpanel_step(double (&a0)[SP], double (&a1)[SP], int c0, int lane,
+                                              int *spiv, unsigned *flags) {
+  const int g = c0 + J;
+  unsigned k0 = lane >= g ? piv_key(a0[J], lane) : 0u;
+  unsigned k1 = lane + 64 >= g ? piv_key(a1[J], lane + 64) : 0u;
+  const unsigned kk = wave_umax(max(k0, k1));
+  const int p = 127 - (int)(kk & 127u);
+  const int lg = g & 63, sg = g >> 6;
+  if (p != g) {
+    const int lp = p & 63, spp = p >> 6;
+#pragma unroll
+    for (int c = 0; c < SP; ++c) {
+      double vg = rdlane(sg ? a1[c] : a0[c], lg);
+      double vp = rdlane(spp ? a1[c] : a0[c], lp);
+      if (lane == lg) {
+        if (sg) a1[c] = vp;
+        else a0[c] = vp;
+      }
+      if (lane == lp) {
+        if (spp) a1[c] = vg;
+        else a0[c] = vg;
+      }
+    }
+  }
+  double u[SP];
+#pragma unroll
+  for (int c = J; c < SP; ++c) u[c] = rdlane(sg ? a1[c] : a0[c], lg);
+  const double piv = u[J];
+  if (lane == 0) {
+    spiv[g] = p;
+    if (piv == 0.0) atomicOr(&flags[1], 1u);
+  }
+  const double rp = 1.0 / piv;
+  if (lane > g) {
+    const double l = a0[J] * rp;
+    a0[J] = l;
+#pragma unroll
+    for (int c = J + 1; c < SP; ++c) a0[c] = fma(-l, u[c], a0[c]);
+  }
+  if (lane + 64 > g) {
+    const double l = a1[J] * rp;
+    a1[J] = l;
+#pragma unroll
+    for (int c = J + 1; c < SP; ++c) a1[c] = fma(-l, u[c], a1[c]);
+  }
+}
+
+template <int... Js>
+__device__ __forceinline__ void subpanel_steps(std::integer_sequence<int, Js...>, double (&a0)[SP],
+                                               double (&a1)[SP], int c0, int lane, int *spiv,
+                                               unsigned *flags) {
+  (subpanel_step<Js>(a0, a1, c0, lane, spiv, flags), ...);
+}
+
+__device__ __forceinline__ void diag_subpanel(double *sD, int c0, int lane, int *spiv,
+                                              unsigned *flags) {
+  double a0[SP], a1[SP];
+#pragma unroll
+  for (int j = 0; j < SP; ++j) {
+    a0[j] = sD[(c0 + j) * LDD + lane];
+    a1[j] = sD[(c0 + j) * LDD + lane + 64];
+  }
+  subpanel_steps(std::make_integer_sequence<int, SP>{}, a0, a1, c0, lane, spiv, flags);
+#pragma unroll
+  for (int j = 0; j < SP; ++j) {
+    sD[(c0 + j) * LDD + lane] = a0[j];
+    sD[(c0 + j) * LDD + lane + 64] = a1[j];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_mx_diag(DiagArgs a) {
+  extern __shared__ double sD[];  // NB x LDD, column-major
+  __shared__ int spiv[NB];
+  __shared__ int sperm[NB];
+  __shared__ int scount;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const float *Wd = a.W + a.k + a.ld * a.k;
+  for (int q = tid; q < NB * NB; q += 256) {
+    int r = q & (NB - 1), c = q >> 7;
+    sD[c * LDD + r] = (double)Wd[r + a.ld * c];
+  }
+  if (tid == 0) scount = 0;
+  __syncthreads();
+  for (int sp = 0; sp < NB / SP; ++sp) {
+    const int c0 = sp * SP;
+    if (wv == 0) diag_subpanel(sD, c0, lane, spiv, a.flags);
+    __syncthreads();
+    // interchanges of this sub-panel on the other 96 columns
+    if (tid < NB - SP) {
+      const int col = tid < c0 ? tid : tid + SP;
+      double *cp = sD + col * LDD;
+      for (int i = 0; i < SP; ++i) {
+        int p = spiv[c0 + i];
+        if (p != c0 + i) {
+          double t = cp[c0 + i];
+          cp[c0 + i] = cp[p];
+          cp[p] = t;
+        }
+      }
+    }
+    __syncthreads();
+    const int m2 = NB - c0 - SP;  // rows/columns after the sub-panel
+    if (m2 == 0) break;
+    // U12 of the sub-panel: unit-lower forward substitution, one column per thread
+    if (tid < m2) {
+      double *cp = sD + (c0 + SP + tid) * LDD + c0;
+      double x[SP];
+#pragma unroll
+      for (int i = 0; i < SP; ++i) x[i] = cp[i];
+#pragma unroll
+      for (int i = 0; i < SP; ++i)
+#pragma unroll
+        for (int r = i + 1; r < SP; ++r) x[r] = fma(-sD[(c0 + i) * LDD + c0 + r], x[i], x[r]);
+#pragma unroll
+      for (int i = 0; i < SP; ++i) cp[i] = x[i];
+    }
+    __syncthreads();
+    // trailing block: D22 -= L21 U12 on FP64 MFMA, 16x16 tiles over the waves
+    {
+      const int nt = m2 / 16;
+      const int base = c0 + SP;
+      for (int t = wv; t < nt * nt; t += 4) {
+        const int ti = t % nt, tj = t / nt;
+        const int r0 = base + 16 * ti, cc0 = base + 16 * tj;
+        d4 acc;
+        // f64 16x16x4 output: column lane % 16, rows 4 i + lane / 16
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = sD[(cc0 + (lane & 15)) * LDD + r0 + 4 * i + (lane >> 4)];
+#pragma unroll
+        for (int kk = 0; kk < SP; kk += 4) {
+          double av = -sD[(c0 + kk + (lane >> 4)) * LDD + r0 + (lane & 15)];
+          double bv = sD[(cc0 + (lane & 15)) * LDD + c0 + kk + (lane >> 4)];
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sD[(cc0 + (lane & 15)) * LDD + r0 + 4 * i + (lane >> 4)] = acc[i];
+      }
+    }
+    __syncthreads();
+  }
+  // inverses of the 32x32 diagonal sub-blocks: wave w takes sub-block w;
+  // lanes 0..31 invert U (back substitution per column), lanes 32..63 invert
+  // the unit-lower L (forward substitution per column).
+  {
+    const int b0 = wv * SP;
+    const int c = lane & 31;
+    float *Li = a.dinv + ((long)(a.k / NB) * 8 + wv) * SP * SP;
+    float *Ui = a.dinv + ((long)(a.k / NB) * 8 + 4 + wv) * SP * SP;
+    double x[SP];
+    if (lane < 32) {
+#pragma unroll
+      for (int i = SP - 1; i >= 0; --i) {
+        double s = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+        for (int kq = i + 1; kq < SP; ++kq) s = fma(-sD[(b0 + kq) * LDD + b0 + i], x[kq], s);
+        x[i] = s / sD[(b0 + i) * LDD + b0 + i];
+      }
+#pragma unroll
+      for (int i = 0; i < SP; ++i) Ui[i + SP * c] = (float)x[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < SP; ++i) {
+        double s = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+        for (int kq = 0; kq < i; ++kq) s = fma(-sD[(b0 + kq) * LDD + b0 + i], x[kq], s);
+        x[i] = s;
+      }
+#pragma unroll
+      for (int i = 0; i < SP; ++i) Li[i + SP * c] = (float)x[i];
+    }
+  }
+  // pivots, permutation, write-back
+  if (tid < NB) {
+    if (spiv[tid] != tid) atomicAdd(&scount, 1);
+    sperm[tid] = tid;
+  }
+  __syncthreads();
+  if (tid == 0 && scount > 0)
+    for (int i = 0; i < NB; ++i) {
+      int p = spiv[i], t = sperm[i];
+      sperm[i] = sperm[p];
+      sperm[p] = t;
+    }
+  __syncthreads();
+  if (tid < NB) {
+    a.piv[a.k + tid] = a.k + spiv[tid];
+    a.perm[a.k + tid] = a.k + sperm[tid];
+  }
+  if (tid == 0) a.npiv[a.k / NB] = scount;
+  float *Wo = a.W + a.k + a.ld * a.k;
+  for (int q = tid; q < NB * NB; q += 256) {
+    int r = q & (NB - 1), c = q >> 7;
+    Wo[r + a.ld * c] = (float)sD[c * LDD + r];
+  }
+}
+
+// Interchanges of diagonal block kb on every column outside the block.
+__global__ void k_mx_laswp(float *W, long ld, int np, int k, const int *__restrict__ piv,
+                          const int *__restrict__ npiv) {
+  if (npiv[k / NB] == 0) return;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= np || (c >= k && c < k + NB)) return;
+  float *col = W + ld * c;
+  for (int i = k; i < k + NB; ++i) {
+    int p = piv[i];
+    if (p != i) {
+      float t = col[i];
+      col[i] = col[p];
+      col[p] = t;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Block-column / block-row triangular solves on FP32 MFMA.
+//   L tiles: X U11 = A21 for 64 rows (X -> W and L fragments), max |x|.
+//   U tiles: L11 Y = A12 for 64 columns (Y -> W and U fragments).
+// mode 1 (inverse): A21 = I / A12 = I, output to the 128x128 Uinv / Linv of
+// the block instead (used by the refinement solves).
+// Fragment order: FP16 (E = 8, KS = 32): frag (tile, s) lane l holds
+// rows/cols 16 tile + l % 16 and k = 32 s + 8 (l / 16) + e; FP32 (E = 1,
+// KS = 4): k = 4 s + l / 16.
+// ---------------------------------------------------------------------------
+constexpr int LDX = 64 + 1;   // LDS leading dimension of the 64-wide tile (floats)
+constexpr int LDT = NB + 1;   // LDS leading dimension of the 128x128 triangle
+
+struct TrsmArgs {
+  float *W;
+  long ld;
+  int k, np, nl;    // nl: number of L tiles (the rest are U tiles)
+  int f16;          // fragment precision
+  int mode;         // 0: factorisation, 1: block inverses
+  const float *dinv;
+  void *Lf, *Uf;    // fragment panels of this block column
+  unsigned *flags;
+  float *Linv, *Uinv;  // [nblk][128*128] (mode 1)
+};
+
+__device__ inline void frag_store_L(void *Lf, int f16, int rt, int lane, const float *sX, int rloc) {
+  // rows rloc.. of sX (column-major, LDX) -> fragments of row tile rt
+  if (f16) {
+    h8 *F = (h8 *)Lf;
+    for (int s = 0; s < NB / 32; ++s) {
+      h8 v;
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        v[e] = (_Float16)sX[(32 * s + 8 * (lane >> 4) + e) * LDX + rloc + (lane & 15)];
+      F[((long)rt * (NB / 32) + s) * 64 + lane] = v;
+    }
+  } else {
+    float *F = (float *)Lf;
+    for (int s = 0; s < NB / 4; ++s)
+      F[((long)rt * (NB / 4) + s) * 64 + lane] = sX[(4 * s + (lane >> 4)) * LDX + rloc + (lane & 15)];
+  }
+}
+
+__device__ inline void frag_store_U(void *Uf, int f16, int ct, int lane, const float *sY, int cloc) {
+  // sY: rows 0..127 of columns cloc.. (column-major, LDT)
+  if (f16) {
+    h8 *F = (h8 *)Uf;
+    for (int s = 0; s < NB / 32; ++s) {
+      h8 v;
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        v[e] = (_Float16)sY[(cloc + (lane & 15)) * LDT + 32 * s + 8 * (lane >> 4) + e];
+      F[((long)ct * (NB / 32) + s) * 64 + lane] = v;
+    }
+  } else {
+    float *F = (float *)Uf;
+    for (int s = 0; s < NB / 4; ++s)
+      F[((long)ct * (NB / 4) + s) * 64 + lane] = sY[(cloc + (lane & 15)) * LDT + 4 * s + (lane >> 4)];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_mx_trsm(TrsmArgs a) {
+  extern __shared__ float sm[];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int k = a.mode ? (int)blockIdx.y * NB : a.k;
+  const int kb = k / NB;
+  const float *Dg = a.W + k + a.ld * k;   // diagonal block
+  const float *inv = a.dinv + (long)kb * 8 * SP * SP;
+  if ((int)blockIdx.x < a.nl) {
+    // ---------------- L tile: rows R0 .. R0+64 ----------------
+    float *sX = sm;                   // 128 cols x LDX
+    float *sU = sX + NB * LDX;        // 128 x LDT (upper triangle used)
+    float *sI = sU + NB * LDT;        // 4 x 32 x 32 (Uinv sub-blocks)
+    const int R0 = (a.mode == 0) ? k + NB + 64 * blockIdx.x : 64 * blockIdx.x;
+    for (int q = tid; q < NB * 64; q += 256) {
+      int r = q & 63, c = q >> 6;
+      float v;
+      if (a.mode == 0) v = a.W[R0 + r + a.ld * (k + c)];
+      else v = (R0 + r == c) ? 1.f : 0.f;
+      sX[c * LDX + r] = v;
+    }
+    for (int q = tid; q < NB * NB; q += 256) {
+      int r = q & (NB - 1), c = q >> 7;
+      sU[c * LDT + r] = Dg[r + a.ld * c];
+    }
+    for (int q = tid; q < 4 * SP * SP; q += 256) sI[q] = inv[4 * SP * SP + q];
+    __syncthreads();
+    const int r16 = 16 * wv;  // the wave's 16 rows
+    for (int s = 0; s < NB / SP; ++s) {
+      f4 acc[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          acc[t][i] = sX[(SP * s + 16 * t + (lane & 15)) * LDX + r16 + 4 * (lane >> 4) + i];
+      for (int kk = 0; kk < SP * s; kk += 4) {
+        float av = -sX[(kk + (lane >> 4)) * LDX + r16 + (lane & 15)];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          float bv = sU[(SP * s + 16 * t + (lane & 15)) * LDT + kk + (lane >> 4)];
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[t], 0, 0, 0);
+        }
+      }
+      wsync();
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          sX[(SP * s + 16 * t + (lane & 15)) * LDX + r16 + 4 * (lane >> 4) + i] = acc[t][i];
+      wsync();
+      f4 out[2] = {(f4){0.f, 0.f, 0.f, 0.f}, (f4){0.f, 0.f, 0.f, 0.f}};
+      const float *Ui = sI + s * SP * SP;
+#pragma unroll
+      for (int kk = 0; kk < SP; kk += 4) {
+        float av = sX[(SP * s + kk + (lane >> 4)) * LDX + r16 + (lane & 15)];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          float bv = Ui[(kk + (lane >> 4)) + SP * (16 * t + (lane & 15))];
+          out[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, out[t], 0, 0, 0);
+        }
+      }
+      wsync();
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          sX[(SP * s + 16 * t + (lane & 15)) * LDX + r16 + 4 * (lane >> 4) + i] = out[t][i];
+      wsync();
+    }
+    // outputs of the wave's 16 rows
+    if (a.mode == 0) {
+      float m = 0.f;
+      bool nan = false;
+      for (int q = lane; q < 16 * NB; q += 64) {
+        int r = q & 15, c = q >> 4;
+        float v = sX[c * LDX + r16 + r];
+        m = fmaxf(m, fabsf(v));
+        nan |= !(v == v);
+        a.W[R0 + r16 + r + a.ld * (k + c)] = v;
+      }
+      wave_max_atomic(nan ? INFINITY : m, &a.flags[0]);
+      frag_store_L(a.Lf, a.f16, (R0 + r16) / 16, lane, sX, r16);
+    } else {
+      float *Ui = a.Uinv + (long)kb * NB * NB;
+      for (int q = lane; q < 16 * NB; q += 64) {
+        int r = q & 15, c = q >> 4;
+        Ui[R0 + r16 + r + NB * c] = sX[c * LDX + r16 + r];
+      }
+    }
+  } else {
+    // ---------------- U tile: columns C0 .. C0+64 ----------------
+    float *sY = sm;                   // 64 cols x LDT (rows 0..127)
+    float *sL = sY + 64 * LDT;        // 128 x LDT (strict lower triangle used)
+    float *sI = sL + NB * LDT;        // 4 x 32 x 32 (Linv sub-blocks)
+    const int ub = blockIdx.x - a.nl;
+    const int C0 = (a.mode == 0) ? k + NB + 64 * ub : 64 * ub;
+    for (int q = tid; q < NB * 64; q += 256) {
+      int r = q & (NB - 1), c = q >> 7;
+      float v;
+      if (a.mode == 0) v = a.W[k + r + a.ld * (C0 + c)];
+      else v = (r == C0 + c) ? 1.f : 0.f;
+      sY[c * LDT + r] = v;
+    }
+    for (int q = tid; q < NB * NB; q += 256) {
+      int r = q & (NB - 1), c = q >> 7;
+      sL[c * LDT + r] = Dg[r + a.ld * c];
+    }
+    for (int q = tid; q < 4 * SP * SP; q += 256) sI[q] = inv[q];
+    __syncthreads();
+    const int c16 = 16 * wv;  // the wave's 16 columns
+    for (int s = 0; s < NB / SP; ++s) {
+      f4 acc[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          acc[t][i] = sY[(c16 + (lane & 15)) * LDT + SP * s + 16 * t + 4 * (lane >> 4) + i];
+      for (int kk = 0; kk < SP * s; kk += 4) {
+        float bv = sY[(c16 + (lane & 15)) * LDT + kk + (lane >> 4)];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          float av = -sL[(kk + (lane >> 4)) * LDT + SP * s + 16 * t + (lane & 15)];
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[t], 0, 0, 0);
+        }
+      }
+      wsync();
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          sY[(c16 + (lane & 15)) * LDT + SP * s + 16 * t + 4 * (lane >> 4) + i] = acc[t][i];
+      wsync();
+      f4 out[2] = {(f4){0.f, 0.f, 0.f, 0.f}, (f4){0.f, 0.f, 0.f, 0.f}};
+      const float *Li = sI + s * SP * SP;
+#pragma unroll
+      for (int kk = 0; kk < SP; kk += 4) {
+        float bv = sY[(c16 + (lane & 15)) * LDT + SP * s + kk + (lane >> 4)];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          float av = Li[(16 * t + (lane & 15)) + SP * (kk + (lane >> 4))];
+          out[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, out[t], 0, 0, 0);
+        }
+      }
+      wsync();
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          sY[(c16 + (lane & 15)) * LDT + SP * s + 16 * t + 4 * (lane >> 4) + i] = out[t][i];
+      wsync();
+    }
+    if (a.mode == 0) {
+      for (int q = lane; q < 16 * NB; q += 64) {
+        int r = q & (NB - 1), c = q >> 7;
+        a.W[k + r + a.ld * (C0 + c16 + c)] = sY[(c16 + c) * LDT + r];
+      }
+      frag_store_U(a.Uf, a.f16, (C0 + c16) / 16, lane, sY, c16);
+    } else {
+      float *Li = a.Linv + (long)kb * NB * NB;
+      for (int q = lane; q < 16 * NB; q += 64) {
+        int r = q & (NB - 1), c = q >> 7;
+        Li[r + NB * (C0 + c16 + c)] = sY[(c16 + c) * LDT + r];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Update GEMM: W[r0.., c0..] -= sum_q Lfrag_q Ufrag_q over P block columns.
+// Block tile 128 x 128, four waves of 64 x 64 (4 x 4 MFMA tiles).
+// ---------------------------------------------------------------------------
+template <int F16>
+__global__ void __launch_bounds__(256) k_mx_gemm(float *W, long ld, int r0, int c0, const void *Lf,
+                                                 const void *Uf, long pbytes, int P) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int rt0 = (r0 + 128 * blockIdx.x + 64 * (wv & 1)) / 16;
+  const int ct0 = (c0 + 128 * blockIdx.y + 64 * (wv >> 1)) / 16;
+  f4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+  for (int q = 0; q < P; ++q) {
+    if (F16) {
+      const h8 *La = (const h8 *)((const char *)Lf + q * pbytes);
+      const h8 *Ub = (const h8 *)((const char *)Uf + q * pbytes);
+#pragma unroll
+      for (int s = 0; s < NB / 32; ++s) {
+        h8 av[4], bv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) av[i] = La[((long)(rt0 + i) * (NB / 32) + s) * 64 + lane];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bv[j] = Ub[((long)(ct0 + j) * (NB / 32) + s) * 64 + lane];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[i], bv[j], acc[i][j], 0, 0, 0);
+      }
+    } else {
+      const float *La = (const float *)((const char *)Lf + q * pbytes);
+      const float *Ub = (const float *)((const char *)Uf + q * pbytes);
+#pragma unroll 4
+      for (int s = 0; s < NB / 4; ++s) {
+        float av[4], bv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) av[i] = La[((long)(rt0 + i) * (NB / 4) + s) * 64 + lane];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bv[j] = Ub[((long)(ct0 + j) * (NB / 4) + s) * 64 + lane];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0);
+      }
+    }
+  }
+  // output layout: row 4 (lane / 16) + i, column lane % 16
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long row = 16L * (rt0 + i) + 4 * (lane >> 4);
+      const long col = 16L * (ct0 + j) + (lane & 15);
+      f4 *p = (f4 *)(W + row + ld * col);
+      f4 v = *p;
+      *p = v - acc[i][j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Refinement pieces.
+// ---------------------------------------------------------------------------
+// y[i][q] = (float)(rs[perm i] * r[perm i, q]); rows >= n and q >= nrhs -> 0.
+__global__ void k_mx_gather(const double *__restrict__ r, long ldr, int n, int np, int nrhs,
+                            const int *__restrict__ perm, const double *__restrict__ rs, float *y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  float v[NR];
+  int p = i < n ? perm[i] : 0;
+  double s = i < n ? rs[p] : 0.0;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) v[q] = (i < n && q < nrhs) ? (float)(s * r[p + ldr * q]) : 0.f;
+  f4 *o = (f4 *)(y + (long)i * NR);
+  o[0] = (f4){v[0], v[1], v[2], v[3]};
+  o[1] = (f4){v[4], v[5], v[6], v[7]};
+}
+
+// x[i, q] (+)= y[i][q]
+__global__ void k_mx_update_x(double *x, long ldx, int n, const float *__restrict__ y,
+                              int accumulate) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int q = 0; q < NR; ++q) {
+    double d = (double)y[(long)i * NR + q];
+    x[i + ldx * q] = accumulate ? x[i + ldx * q] + d : d;
+  }
+}
+
+__device__ inline bool wait_flag(unsigned *f, unsigned gen, unsigned *err) {
+  long spins = 0;
+  while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gen) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins > (1L << 25)) {
+      atomicExch(err, 1u);
+      return false;
+    }
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+  }
+  return true;
+}
+
+struct SolveArgs {
+  const float *W;
+  long ld;
+  const float *Inv;   // Linv (forward) or Uinv (backward), [nblk][128*128]
+  float *y;           // [np][8], in place
+  unsigned *ready;    // [nblk]
+  unsigned gen;
+  int nblk;
+  unsigned *err;
+};
+
+// Forward (upper = 0: L y = c, L unit lower) or backward (upper = 1: U x = y)
+// block substitution.  Block row kb is finished by one workgroup; the
+// workgroups take block rows in dependency order, so the earliest unfinished
+// block row always has its inputs ready.
+template <int UPPER>
+__global__ void __launch_bounds__(256) k_mx_solve(SolveArgs a) {
+  __shared__ float sy[NB * NR];
+  __shared__ float sacc[NB * NR];
+  __shared__ int ok;
+  const int t = threadIdx.x, r = t & (NB - 1), h = t >> 7;
+  for (int it = blockIdx.x; it < a.nblk; it += gridDim.x) {
+    const int kb = UPPER ? a.nblk - 1 - it : it;
+    float acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = a.y[((long)kb * NB + r) * NR + 4 * h + q];
+    const int nj = UPPER ? a.nblk - 1 - kb : kb;
+    for (int jj = 0; jj < nj; ++jj) {
+      const int j = UPPER ? a.nblk - 1 - jj : jj;
+      if (t == 0) ok = wait_flag(&a.ready[j], a.gen, a.err) ? 1 : 0;
+      __syncthreads();
+      if (!ok) return;
+      *(f4 *)(sy + 4 * t) = *(const f4 *)(a.y + (long)j * NB * NR + 4 * t);
+      __syncthreads();
+      const float *Lc = a.W + (long)kb * NB + r + a.ld * (long)j * NB;
+#pragma unroll 8
+      for (int c = 0; c < NB; ++c) {
+        const float l = Lc[a.ld * c];
+        const f4 v = *(const f4 *)(sy + c * NR + 4 * h);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = fmaf(-l, v[q], acc[q]);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sacc[r * NR + 4 * h + q] = acc[q];
+    __syncthreads();
+    const float *Iv = a.Inv + (long)kb * NB * NB + r;
+    float out[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 8
+    for (int c = 0; c < NB; ++c) {
+      const float l = Iv[NB * c];
+      const f4 v = *(const f4 *)(sacc + c * NR + 4 * h);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) out[q] = fmaf(l, v[q], out[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a.y[((long)kb * NB + r) * NR + 4 * h + q] = out[q];
+    __threadfence();
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(&a.ready[kb], a.gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Partial products of r = b - A x over column chunks (deterministic: the
+// chunks are summed in order by k_mx_resid_fin).  Also row sums of |A|.
+__global__ void k_mx_resid_part(const double *__restrict__ A, long lda, int n, int cw,
+                                const double *__restrict__ x, double *part, double *rowabs) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int ch = blockIdx.y;
+  const int j0 = ch * cw, j1 = min(n, j0 + cw);
+  double s[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) s[q] = 0.0;
+  double sa = 0.0;
+  for (int j = j0; j < j1; ++j) {
+    const double av = A[i + lda * j];
+    sa += fabs(av);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) s[q] = fma(av, x[j + (long)n * q], s[q]);
+  }
+  double *p = part + ((long)ch * n + i) * NR;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) p[q] = s[q];
+  if (rowabs) rowabs[(long)ch * n + i] = sa;
+}
+
+__device__ inline unsigned long long wave_max_u64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long w = __shfl_xor(v, o, 64);
+    v = v > w ? v : w;
+  }
+  return v;
+}
+
+// r = b - sum of partials; norms[q] = max |r_q|, norms[8+q] = max |x_q|,
+// norms[16] = max row sum of |A| (when rowabs is given).  Doubles are >= 0,
+// so their bit patterns order like the values.
+__global__ void __launch_bounds__(256) k_mx_resid_fin(int n, int nch, const double *__restrict__ part,
+                                                      const double *__restrict__ b, double *r,
+                                                      const double *__restrict__ x,
+                                                      const double *__restrict__ rowabs,
+                                                      unsigned long long *norms) {
+  __shared__ unsigned long long sm[4][2 * NR + 1];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  unsigned long long v[2 * NR + 1];
+#pragma unroll
+  for (int q = 0; q < 2 * NR + 1; ++q) v[q] = 0ull;
+  if (i < n) {
+    double s[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) s[q] = b[i + (long)n * q];
+    for (int ch = 0; ch < nch; ++ch) {
+      const double *p = part + ((long)ch * n + i) * NR;
+#pragma unroll
+      for (int q = 0; q < NR; ++q) s[q] -= p[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      r[i + (long)n * q] = s[q];
+      v[q] = (unsigned long long)__double_as_longlong(fabs(s[q]));
+      v[NR + q] = (unsigned long long)__double_as_longlong(fabs(x[i + (long)n * q]));
+    }
+    if (rowabs) {
+      double ra = 0.0;
+      for (int ch = 0; ch < nch; ++ch) ra += rowabs[(long)ch * n + i];
+      v[2 * NR] = (unsigned long long)__double_as_longlong(ra);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 2 * NR + 1; ++q) v[q] = wave_max_u64(v[q]);
+  if (lane == 0)
+    for (int q = 0; q < 2 * NR + 1; ++q) sm[wv][q] = v[q];
+  __syncthreads();
+  if (threadIdx.x < 2 * NR + 1) {
+    unsigned long long m = sm[0][threadIdx.x];
+    for (int w = 1; w < 4; ++w) m = m > sm[w][threadIdx.x] ? m : sm[w][threadIdx.x];
+    if (m) atomicMax(&norms[threadIdx.x], m);
+  }
+}
+
+__global__ void k_mx_zero_u64(unsigned long long *p, int n) {
+  int i = threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// Copy an n x nrhs column-major block into an n x 8 one (zero columns past nrhs).
+__global__ void k_mx_pack_rhs(const double *__restrict__ B, long ldb, int n, int nrhs, double *o) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int q = 0; q < NR; ++q) o[i + (long)n * q] = q < nrhs ? B[i + ldb * q] : 0.0;
+}
+
+__global__ void k_mx_unpack_rhs(const double *__restrict__ x, int n, int nrhs, double *B, long ldb) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int q = 0; q < nrhs; ++q) B[i + ldb * q] = x[i + (long)n * q];
+}
+
+__global__ void k_mx_copy_a(const double *__restrict__ A, long lda, int n, double *o) {
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)n * n) return;
+  long i = q % n, j = q / n;
+  o[q] = A[i + lda * j];
+}
+
+// ---------------------------------------------------------------------------
+// Workspace
+// ---------------------------------------------------------------------------
+struct MixedWs {
+  int dev = -1;
+  DevBuf<float> W;
+  DevBuf<double> rs;
+  DevBuf<unsigned> rowmax;
+  DevBuf<char> Lf, Uf;
+  DevBuf<int> piv, perm, npiv;
+  DevBuf<float> dinv, Linv, Uinv;
+  DevBuf<unsigned> flags;       // [0] max |l|, [1] zero pivot, [2] solve error, [3..] ready
+  DevBuf<float> y;
+  DevBuf<double> b8, x8, r8, part, rowabs, A64;
+  DevBuf<unsigned long long> norms;
+  DevBuf<int> ipiv64;
+  unsigned gen = 0;
+};
+MixedWs g_ws;
+
+int mx_cus() {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  hipDeviceProp_t pr;
+  if (hipGetDeviceProperties(&pr, dev) != hipSuccess) return 64;
+  return pr.multiProcessorCount > 0 ? pr.multiProcessorCount : 64;
+}
+
+size_t lds_diag() { return sizeof(double) * (size_t)NB * LDD; }
+size_t lds_trsm() {
+  size_t l = sizeof(float) * ((size_t)NB * LDX + (size_t)NB * LDT + 4 * SP * SP);
+  size_t u = sizeof(float) * ((size_t)64 * LDT + (size_t)NB * LDT + 4 * SP * SP);
+  return std::max(l, u);
+}
+
+#define MX_LAUNCH_CHECK(what)                                                     \
+  do {                                                                            \
+    hipError_t e_ = hipGetLastError();                                            \
+    if (e_ != hipSuccess) {                                                       \
+      set_error("%s launch failed: %s", what, hipGetErrorString(e_));             \
+      return DUST_E_HIP;                                                          \
+    }                                                                             \
+  } while (0)
+
+int mx_gemm(int f16, float *W, long ld, int r0, int r1, int c0, int c1, const void *Lf,
+            const void *Uf, long pbytes, int P, hipStream_t st) {
+  if (r1 <= r0 || c1 <= c0 || P <= 0) return 0;
+  dim3 grid((r1 - r0) / 128, (c1 - c0) / 128);
+  if (f16) k_mx_gemm<1><<<grid, 256, 0, st>>>(W, ld, r0, c0, Lf, Uf, pbytes, P);
+  else k_mx_gemm<0><<<grid, 256, 0, st>>>(W, ld, r0, c0, Lf, Uf, pbytes, P);
+  MX_LAUNCH_CHECK("update gemm");
+  return 0;
+}
+
+// Factorisation of W (np x np).  Returns 0 or a HIP error; *bad is set when a
+// block failed the pivot check or hit a zero pivot.
+int mx_factor(MixedWs &w, int np, int f16, double tau, hipStream_t st, int *bad) {
+  const int nblk = np / NB;
+  const long ld = np;
+  const size_t esz = f16 ? sizeof(_Float16) : sizeof(float);
+  const long pbytes = (long)np * NB * (long)esz;  // bytes per fragment panel
+  k_mx_fill_u32<<<1, 64, 0, st>>>(w.flags.p, 3, 0u);
+  for (int ks = 0; ks < np; ks += DEPTH * NB) {
+    const int nsub = std::min(DEPTH, (np - ks) / NB);
+    for (int a = 0; a < nsub; ++a) {
+      const int k = ks + a * NB;
+      if (a > 0) {
+        // block column a and block row a, updated by the sub-panels before it
+        DUST_TRY(mx_gemm(f16, w.W.p, ld, k, np, k, k + NB, w.Lf.p, w.Uf.p, pbytes, a, st));
+        DUST_TRY(mx_gemm(f16, w.W.p, ld, k, k + NB, k + NB, np, w.Lf.p, w.Uf.p, pbytes, a, st));
+      }
+      DiagArgs da{w.W.p, ld, k, w.piv.p, w.perm.p, w.npiv.p, w.dinv.p, w.flags.p};
+      k_mx_diag<<<1, 256, lds_diag(), st>>>(da);
+      MX_LAUNCH_CHECK("diagonal block");
+      k_mx_laswp<<<ceil_div(np, 256), 256, 0, st>>>(w.W.p, ld, np, k, w.piv.p, w.npiv.p);
+      MX_LAUNCH_CHECK("laswp");
+      const int nt = (np - k - NB) / 64;
+      if (nt > 0) {
+        TrsmArgs ta{w.W.p, ld, k, np, nt, f16, 0, w.dinv.p,
+                    w.Lf.p + (size_t)a * pbytes, w.Uf.p + (size_t)a * pbytes,
+                    w.flags.p, nullptr, nullptr};
+        k_mx_trsm<<<2 * nt, 256, lds_trsm(), st>>>(ta);
+        MX_LAUNCH_CHECK("block triangular solve");
+      }
+    }
+    const int t0 = ks + nsub * NB;
+    if (t0 < np)
+      DUST_TRY(mx_gemm(f16, w.W.p, ld, t0, np, t0, np, w.Lf.p, w.Uf.p, pbytes, nsub, st));
+  }
+  // 128x128 inverses of every diagonal block for the refinement solves
+  TrsmArgs ia{w.W.p, ld, 0, np, 2, f16, 1, w.dinv.p, nullptr, nullptr, w.flags.p, w.Linv.p, w.Uinv.p};
+  k_mx_trsm<<<dim3(4, nblk), 256, lds_trsm(), st>>>(ia);
+  MX_LAUNCH_CHECK("block inverses");
+  unsigned hf[2] = {0, 0};
+  DUST_HIP(hipMemcpyAsync(hf, w.flags.p, sizeof(hf), hipMemcpyDeviceToHost, st));
+  DUST_HIP(hipStreamSynchronize(st));
+  const float lmax = *(const float *)&hf[0];
+  *bad = (hf[1] != 0 || !(lmax <= (float)tau)) ? 1 : 0;
+  return 0;
+}
+
+// y <- U^-1 L^-1 y (y: np x 8 FP32, in place).
+int mx_solve(MixedWs &w, int np, hipStream_t st, int cus) {
+  const int nblk = np / NB;
+  const int G = std::min(nblk, cus);
+  for (int upper = 0; upper < 2; ++upper) {
+    SolveArgs sa{w.W.p, (long)np, upper ? w.Uinv.p : w.Linv.p, w.y.p,
+                 w.flags.p + 3 + upper * nblk, ++w.gen, nblk, w.flags.p + 2};
+    if (w.gen == 0) sa.gen = ++w.gen;  // generation 0 is the cleared state
+    void *args[] = {&sa};
+    hipError_t e = upper ? hipLaunchCooperativeKernel((const void *)k_mx_solve<1>, dim3(G), dim3(256), args, 0, st)
+                         : hipLaunchCooperativeKernel((const void *)k_mx_solve<0>, dim3(G), dim3(256), args, 0, st);
+    if (e != hipSuccess) {
+      set_error("triangular solve launch failed: %s", hipGetErrorString(e));
+      return DUST_E_HIP;
+    }
+  }
+  return 0;
+}
+
+// r = b - A x and the norms; returns the largest backward error over the
+// right-hand sides.
+int mx_residual(MixedWs &w, const double *dA, long lda, int n, int nrhs, bool with_anorm,
+                double *anorm, double *berr, hipStream_t st) {
+  const int cw = 512;
+  const int nch = ceil_div(n, cw);
+  k_mx_zero_u64<<<1, 64, 0, st>>>(w.norms.p, 2 * NR + 1);
+  dim3 g(ceil_div(n, 256), nch);
+  k_mx_resid_part<<<g, 256, 0, st>>>(dA, lda, n, cw, w.x8.p, w.part.p, with_anorm ? w.rowabs.p : nullptr);
+  k_mx_resid_fin<<<ceil_div(n, 256), 256, 0, st>>>(n, nch, w.part.p, w.b8.p, w.r8.p, w.x8.p,
+                                                   with_anorm ? w.rowabs.p : nullptr, w.norms.p);
+  MX_LAUNCH_CHECK("residual");
+  unsigned long long hn[2 * NR + 1];
+  DUST_HIP(hipMemcpyAsync(hn, w.norms.p, sizeof(hn), hipMemcpyDeviceToHost, st));
+  DUST_HIP(hipStreamSynchronize(st));
+  double nm[2 * NR + 1];
+  std::memcpy(nm, hn, sizeof(nm));
+  if (with_anorm) *anorm = nm[2 * NR];
+  double be = 0.0;
+  for (int q = 0; q < nrhs; ++q) {
+    double den = *anorm * nm[NR + q];
+    double e = den > 0.0 ? nm[q] / den : (nm[q] > 0.0 ? INFINITY : 0.0);
+    be = std::max(be, e);
+  }
+  *berr = be;
+  return 0;
+}
+
+int mx_reserve(MixedWs &w, int n, int np) {
+  const size_t nblk = np / NB;
+  DUST_TRY(w.W.reserve((size_t)np * np));
+  DUST_TRY(w.rs.reserve(np));
+  DUST_TRY(w.rowmax.reserve(np));
+  DUST_TRY(w.Lf.reserve((size_t)DEPTH * np * NB * sizeof(float)));
+  DUST_TRY(w.Uf.reserve((size_t)DEPTH * np * NB * sizeof(float)));
+  DUST_TRY(w.piv.reserve(np));
+  DUST_TRY(w.perm.reserve(np));
+  DUST_TRY(w.npiv.reserve(nblk));
+  DUST_TRY(w.dinv.reserve(nblk * 8 * SP * SP));
+  DUST_TRY(w.Linv.reserve(nblk * NB * NB));
+  DUST_TRY(w.Uinv.reserve(nblk * NB * NB));
+  if (w.flags.n < 3 + 2 * nblk) {
+    // ready flags start at generation 0 (never a live generation)
+    w.flags.n = 0;
+    DUST_TRY(w.flags.reserve(3 + 2 * nblk));
+    DUST_HIP(hipMemset(w.flags.p, 0, sizeof(unsigned) * w.flags.n));
+  }
+  DUST_TRY(w.y.reserve((size_t)np * NR));
+  DUST_TRY(w.b8.reserve((size_t)n * NR));
+  DUST_TRY(w.x8.reserve((size_t)n * NR));
+  DUST_TRY(w.r8.reserve((size_t)n * NR));
+  DUST_TRY(w.part.reserve((size_t)ceil_div(n, 512) * n * NR));
+  DUST_TRY(w.rowabs.reserve((size_t)ceil_div(n, 512) * n));
+  DUST_TRY(w.norms.reserve(2 * NR + 1));
+  return 0;
+}
+
+// FP64 partial-pivoting solve of the packed right-hand sides (fallback and
+// DUST_LU_FP64).
+int mx_fp64(MixedWs &w, const double *dA, long lda, int n, hipStream_t st, int *info) {
+  DUST_TRY(w.A64.reserve((size_t)n * n));
+  DUST_TRY(w.ipiv64.reserve(n));
+  long tot = (long)n * n;
+  k_mx_copy_a<<<(int)((tot + 255) / 256), 256, 0, st>>>(dA, lda, n, w.A64.p);
+  MX_LAUNCH_CHECK("copy");
+  DUST_TRY(lu_getrf_dev(n, n, w.A64.p, n, w.ipiv64.p, st, 64, info));
+  if (*info) return 0;
+  DUST_HIP(hipMemcpyAsync(w.x8.p, w.b8.p, sizeof(double) * (size_t)n * NR, hipMemcpyDeviceToDevice, st));
+  DUST_TRY(lu_getrs_dev(n, NR, w.A64.p, n, w.ipiv64.p, w.x8.p, n, st));
+  return 0;
+}
+
+}  // namespace
+
+// Solve for up to 8 right-hand sides held in w.b8; solution in w.x8.
+int mixed_solve8(const double *dA, long lda, int n, int nrhs, int prec, double tol, int max_iter,
+                 int *iters, double *resid, hipStream_t st) {
+  MixedWs &w = g_ws;
+  const int np = ((n + NB - 1) / NB) * NB;
+  int nit = 0;
+  double anorm = 0.0, berr = INFINITY;
+  bool done = false;
+  if (prec != DUST_LU_FP64) {
+    const int f16 = prec == DUST_LU_FP16S;
+    static bool attr = false;
+    if (!attr) {
+      DUST_HIP(hipFuncSetAttribute((const void *)k_mx_diag, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds_diag()));
+      DUST_HIP(hipFuncSetAttribute((const void *)k_mx_trsm, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds_trsm()));
+      attr = true;
+    }
+    // W = R A
+    if (f16) {
+      k_mx_fill_u32<<<ceil_div(np, 256), 256, 0, st>>>(w.rowmax.p, np, 0u);
+      dim3 g(ceil_div(n, 256), ceil_div(n, 256));
+      k_mx_rowmax<<<g, 256, 0, st>>>(dA, lda, n, 256, w.rowmax.p);
+    }
+    k_mx_rowscale<<<ceil_div(np, 256), 256, 0, st>>>(w.rowmax.p, n, np, f16, w.rs.p);
+    k_mx_convert<<<dim3(np / 256 + (np % 256 != 0), np / 4), 256, 0, st>>>(dA, lda, n, np, w.rs.p, w.W.p);
+    MX_LAUNCH_CHECK("convert");
+    int bad = 0;
+    DUST_TRY(mx_factor(w, np, f16, 2.0, st, &bad));
+    if (!bad) {
+      const int cus = mx_cus();
+      double prev = INFINITY;
+      int slow = 0;
+      for (nit = 1; nit <= std::max(1, max_iter); ++nit) {
+        // d = U^-1 L^-1 P R r  (r = b on the first sweep)
+        k_mx_gather<<<ceil_div(np, 256), 256, 0, st>>>(nit == 1 ? w.b8.p : w.r8.p, n, n, np, nrhs,
+                                                       w.perm.p, w.rs.p, w.y.p);
+        MX_LAUNCH_CHECK("gather");
+        DUST_TRY(mx_solve(w, np, st, cus));
+        k_mx_update_x<<<ceil_div(n, 256), 256, 0, st>>>(w.x8.p, n, n, w.y.p, nit > 1);
+        MX_LAUNCH_CHECK("update");
+        DUST_TRY(mx_residual(w, dA, lda, n, nrhs, nit == 1, &anorm, &berr, st));
+        unsigned herr = 0;
+        DUST_HIP(hipMemcpyAsync(&herr, w.flags.p + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        DUST_HIP(hipStreamSynchronize(st));
+        DUST_CHECK(herr == 0, DUST_E_HIP, "triangular solve flag wait timed out");
+        if (berr <= tol) {
+          done = true;
+          break;
+        }
+        if (!(berr < 0.5 * prev)) {
+          if (++slow >= 2) {
+            done = berr <= 1e-14;  // stalled at working accuracy
+            break;
+          }
+        } else {
+          slow = 0;
+        }
+        prev = berr;
+      }
+    }
+  }
+  if (!done) {
+    int info = 0;
+    DUST_TRY(mx_fp64(w, dA, lda, n, st, &info));
+    if (info) return info;
+    DUST_TRY(mx_residual(w, dA, lda, n, nrhs, true, &anorm, &berr, st));
+    nit = prec == DUST_LU_FP64 ? 0 : -1;
+  }
+  if (iters) *iters = nit;
+  if (resid) *resid = berr;
+  return 0;
+}
+
+int mixed_dev(int n, int nrhs, const double *dA, long lda, double *dB, long ldb, int prec, double tol,
+              int max_iter, int *iters, double *resid, hipStream_t st) {
+  DUST_CHECK(n >= 0 && nrhs >= 0 && lda >= std::max(1, n) && ldb >= std::max(1, n), DUST_E_ARG,
+             "invalid dgesv_mixed dimensions");
+  DUST_CHECK(prec == DUST_LU_FP64 || prec == DUST_LU_FP32 || prec == DUST_LU_FP16S, DUST_E_ARG,
+             "unknown precision %d", prec);
+  if (iters) *iters = 0;
+  if (resid) *resid = 0.0;
+  if (n == 0 || nrhs == 0) return 0;
+  MixedWs &w = g_ws;
+  const int np = ((n + NB - 1) / NB) * NB;
+  DUST_TRY(mx_reserve(w, n, np));
+  int it_max = 0;
+  double r_max = 0.0;
+  for (int q0 = 0; q0 < nrhs; q0 += NR) {
+    const int nq = std::min(NR, nrhs - q0);
+    k_mx_pack_rhs<<<ceil_div(n, 256), 256, 0, st>>>(dB + ldb * q0, ldb, n, nq, w.b8.p);
+    MX_LAUNCH_CHECK("pack");
+    int it = 0;
+    double r = 0.0;
+    int rc = mixed_solve8(dA, lda, n, nq, prec, tol, max_iter, &it, &r, st);
+    if (rc) return rc;
+    k_mx_unpack_rhs<<<ceil_div(n, 256), 256, 0, st>>>(w.x8.p, n, nq, dB + ldb * q0, ldb);
+    MX_LAUNCH_CHECK("unpack");
+    it_max = (it < 0 || it_max < 0) ? -1 : std::max(it_max, it);
+    r_max = std::max(r_max, r);
+  }
+  DUST_HIP(hipStreamSynchronize(st));
+  if (iters) *iters = it_max;
+  if (resid) *resid = r_max;
+  return 0;
+}
+
+}  // namespace dust
 
 using namespace dust;
 
 extern "C" {
-int dust_dgesv_mixed(int, int, const double *, int, double *, int, int, double, int, int *,
-                     double *) {
-  set_error("mixed-precision LU not built");
-  return DUST_E_UNSUPPORTED;
+
+int dust_dgesv_mixed_dev(int n, int nrhs, const double *dA, int lda, double *dB, int ldb, int prec,
+                         double tol, int max_iter, int *iters, double *resid, void *stream) {
+  return mixed_dev(n, nrhs, dA, lda, dB, ldb, prec, tol, max_iter, iters, resid,
+                   (hipStream_t)stream);
 }
-int dust_dgesv_mixed_dev(int, int, const double *, int, double *, int, int, double, int, int *,
-                         double *, void *) {
-  set_error("mixed-precision LU not built");
-  return DUST_E_UNSUPPORTED;
+
+int dust_dgesv_mixed(int n, int nrhs, const double *A, int lda, double *B, int ldb, int prec,
+                     double tol, int max_iter, int *iters, double *resid) {
+  DUST_CHECK(A && B && n >= 0 && nrhs >= 0 && lda >= std::max(1, n) && ldb >= std::max(1, n),
+             DUST_E_ARG, "invalid dgesv_mixed arguments");
+  if (n == 0 || nrhs == 0) {
+    if (iters) *iters = 0;
+    if (resid) *resid = 0.0;
+    return 0;
+  }
+  double *dA = nullptr, *dB = nullptr;
+  DUST_HIP(hipMalloc(&dA, sizeof(double) * (size_t)n * n));
+  if (hipMalloc(&dB, sizeof(double) * (size_t)n * nrhs) != hipSuccess) {
+    (void)hipFree(dA);
+    set_error("hipMalloc failed");
+    return DUST_E_NOMEM;
+  }
+  int rc = 0;
+  if (hipMemcpy2D(dA, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, n,
+                  hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy2D(dB, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
+                  hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("upload failed");
+    rc = DUST_E_HIP;
+  }
+  if (!rc) rc = mixed_dev(n, nrhs, dA, n, dB, n, prec, tol, max_iter, iters, resid, nullptr);
+  if (!rc && hipMemcpy2D(B, sizeof(double) * ldb, dB, sizeof(double) * n, sizeof(double) * n, nrhs,
+                         hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("download failed");
+    rc = DUST_E_HIP;
+  }
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  return rc;
 }
-}
+
+}  // extern "C"

@@ -107,10 +107,17 @@ def solve_linsys(A, ipiv, nstatic, b):
 
 
 def dgesv_mixed(A, B, prec="fp16s", tol=1e-12, max_iter=30):
-    """Tuned solve of A X = B: low-precision LU on the matrix cores, FP64
-    residuals and iterative refinement.  A is not modified; B is overwritten
-    with X.  Returns (X, iters, backward_error)."""
+    """Tuned solve of A X = B: FP32 or row-scaled FP16 LU on the matrix cores
+    (FP64 diagonal blocks, threshold-checked block pivoting), FP64 residuals
+    and iterative refinement.  A is not modified; B (Fortran order, (n,) or
+    (n, nrhs)) is overwritten with X.  Returns (X, iters, backward_error);
+    iters == -1 means the block pivots failed the check or refinement
+    stalled and X came from the FP64 partial-pivoting path."""
     n = A.shape[0]
+    if not (A.flags.f_contiguous and A.dtype == np.float64):
+        raise ValueError("A must be a Fortran-ordered float64 array")
+    if not (B.flags.f_contiguous and B.dtype == np.float64):
+        raise ValueError("B must be a Fortran-ordered float64 array")
     Bv = B.reshape(n, -1, order="F")
     it = C.c_int()
     res = C.c_double()

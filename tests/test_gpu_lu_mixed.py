@@ -1,0 +1,126 @@
+"""Tuned mixed-precision solve (dust_dgesv_mixed) against LAPACK / numpy
+FP64 solutions: backward error <= 1e-12 after refinement, solution within
+1e-10 of the FP64 one on well-conditioned systems, the block-pivot check and
+the FP64 fallback."""
+import numpy as np
+import pytest
+
+from dust_amd import linsys as LS
+from dust_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+PRECS = ["fp32", "fp16s"]
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def berr(A, X, B):
+    r = B - A @ X
+    return np.max(np.abs(r).max(axis=0) / (np.abs(A).sum(axis=1).max() * np.abs(X).max(axis=0)))
+
+
+def solve(A, B, prec, tol=1e-12):
+    X = np.asfortranarray(B.copy())
+    print(f"dgesv_mixed n={A.shape[0]} nrhs={X.reshape(A.shape[0], -1).shape[1]} prec={prec}",
+          flush=True)
+    X, it, be = LS.dgesv_mixed(A, X, prec=prec, tol=tol, max_iter=40)
+    return X, it, be
+
+
+@pytest.fixture(scope="module")
+def aic():
+    ver, nv = W.ellipsoid(24, 24)
+    A, B = LS.aic_surfpan(ver, nv)
+    b = LS.rhs_static(ver, nv, B, W.freestreams(8))
+    return A, np.asfortranarray(b)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_aic_ellipsoid(aic, prec):
+    A, b = aic
+    X, it, be = solve(A, b, prec)
+    X0 = np.linalg.solve(A, b)
+    print(f"{prec}: iters {it} berr {be:.2e} rel {rel(X, X0):.2e}")
+    assert it >= 1  # refined, no fallback: the AIC needs no interchanges
+    assert be <= 1e-12
+    assert berr(A, X, b) <= 1e-12
+    assert rel(X, X0) <= 1e-10
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (5, 2), (127, 3), (129, 8), (300, 11), (1000, 8)])
+@pytest.mark.parametrize("prec", PRECS)
+def test_diag_dominant_shapes(n, nrhs, prec):
+    rng = np.random.default_rng(n * 7 + nrhs)
+    A = np.asfortranarray(rng.standard_normal((n, n)) / np.sqrt(n) + 2.0 * np.eye(n))
+    B = np.asfortranarray(rng.standard_normal((n, nrhs)))
+    X, it, be = solve(A, B, prec)
+    assert it >= 1
+    assert be <= 1e-12
+    assert rel(X, np.linalg.solve(A, B)) <= 1e-10
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_interchanges_inside_blocks(prec):
+    """Rows permuted inside each 128-row block: the block pivoting undoes the
+    permutation without falling back."""
+    n = 640
+    rng = np.random.default_rng(5)
+    A0 = rng.standard_normal((n, n)) / np.sqrt(n) + 3.0 * np.eye(n)
+    perm = np.concatenate([128 * k + rng.permutation(128) for k in range(n // 128)])
+    A = np.asfortranarray(A0[perm])
+    B = np.asfortranarray(rng.standard_normal((n, 4)))
+    X, it, be = solve(A, B, prec)
+    assert it >= 1
+    assert be <= 1e-12
+    assert rel(X, np.linalg.solve(A, B)) <= 1e-10
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_global_pivoting_falls_back(prec):
+    """A matrix whose pivots lie outside the diagonal blocks fails the
+    multiplier check and is solved by the FP64 partial-pivoting path."""
+    n = 512
+    rng = np.random.default_rng(9)
+    A0 = rng.standard_normal((n, n)) / np.sqrt(n) + 3.0 * np.eye(n)
+    A = np.asfortranarray(np.roll(A0, 200, axis=0))
+    B = np.asfortranarray(rng.standard_normal((n, 3)))
+    X, it, be = solve(A, B, prec)
+    assert it == -1
+    assert be <= 1e-14
+    assert rel(X, np.linalg.solve(A, B)) <= 1e-10
+
+
+def test_fp64_mode_and_singular():
+    rng = np.random.default_rng(2)
+    n = 200
+    A = np.asfortranarray(rng.standard_normal((n, n)))
+    B = np.asfortranarray(rng.standard_normal((n, 2)))
+    X, it, be = solve(A, B, "fp64")
+    assert it == 0
+    assert rel(X, np.linalg.solve(A, B)) <= 1e-10
+    S = np.asfortranarray(np.ones((150, 150)))
+    with pytest.raises(np.linalg.LinAlgError):
+        solve(S, np.asfortranarray(np.ones((150, 1))), "fp16s")
+
+
+def test_device_entry_point():
+    import ctypes as C
+    import torch
+    from dust_amd import _lib as L
+    ver, nv = W.ellipsoid(40, 40)
+    A, Bs = LS.aic_surfpan(ver, nv)
+    b = LS.rhs_static(ver, nv, Bs, W.freestreams(8))
+    n = A.shape[0]
+    dA = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()  # column-major storage
+    dB = torch.from_numpy(np.ascontiguousarray(b.T)).cuda()
+    it, res = C.c_int(), C.c_double()
+    print(f"dgesv_mixed_dev n={n}", flush=True)
+    L.check(L.lib().dust_dgesv_mixed_dev(n, 8, dA.data_ptr(), n, dB.data_ptr(), n, L.LU_FP16S,
+                                         1e-12, 40, C.byref(it), C.byref(res), None), "mixed_dev")
+    X = dB.cpu().numpy().T
+    assert it.value >= 1 and res.value <= 1e-12
+    assert rel(X, np.linalg.solve(A, b)) <= 1e-10
+    # A untouched
+    assert np.array_equal(dA.cpu().numpy().T, A)
