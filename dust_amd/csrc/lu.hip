@@ -38,16 +38,35 @@ struct PanelArgs {
   int m, k, jb, R;
   int *ipiv;      // 1-based, global row numbers
   int *info;
-  unsigned *bar;  // [3]
+  unsigned *arr;  // [G] arrival words of the grid barrier
+  unsigned *err;  // barrier time-out flag
+  unsigned gen0;  // barriers passed by earlier panel launches
   double *cval;   // [2][G]
   int *crow;      // [2][G]
   double *crowv;  // [2][G][jb]
   double *topv;   // [2][jb]
 };
 
+// (value, row) maximum in idamax order: larger |a|, then the smaller row.
+__device__ inline void pivot_max(double &v, int &i, double ov, int oi) {
+  if (ov > v || (ov == v && oi < i)) {
+    v = ov;
+    i = oi;
+  }
+}
+
+__device__ inline void wave_pivot_max(double &v, int &i) {
+  for (int o = 32; o > 0; o >>= 1) {
+    double ov = __shfl_xor(v, o, 64);
+    int oi = __shfl_xor(i, o, 64);
+    pivot_max(v, i, ov, oi);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_panel(PanelArgs P) {
   extern __shared__ double sp[];  // [R][jb] row-major panel rows + pivot rows
   const int G = gridDim.x, b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wv = tid >> 6;
   const int jb = P.jb, R = P.R;
   const int r0 = P.k + b * R;
   const int r1 = min(P.m, r0 + R);
@@ -55,11 +74,8 @@ __global__ void __launch_bounds__(256) k_panel(PanelArgs P) {
   double *rows = sp;               // nr x jb
   double *prow = sp + R * jb;      // jb
   double *trow = prow + jb;        // jb
-  __shared__ double rv[256];
-  __shared__ int ri[256];
-  __shared__ int win;
-  unsigned gen = 0;
-  if (tid == 0) gen = __hip_atomic_load(&P.bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __shared__ double wvv[4];
+  __shared__ int wvi[4];
   for (int q = tid; q < nr * jb; q += nt) {
     int r = q / jb, c = q % jb;
     rows[q] = P.A[(long)(r0 + r) + P.lda * (P.k + c)];
@@ -68,58 +84,40 @@ __global__ void __launch_bounds__(256) k_panel(PanelArgs P) {
   for (int jj = 0; jj < jb; ++jj) {
     const int par = jj & 1;
     const int gr = P.k + jj;  // global row of the diagonal
-    // local first maximum over rows >= gr
+    // block first maximum over rows >= gr: wave butterflies, then 4 partials
     double best = -1.0;
     int bi = 0x7fffffff;
     for (int r = tid; r < nr; r += nt) {
       if (r0 + r < gr) continue;
-      double a = fabs(rows[r * jb + jj]);
-      if (a > best) { best = a; bi = r0 + r; }
+      pivot_max(best, bi, fabs(rows[r * jb + jj]), r0 + r);
     }
-    rv[tid] = best;
-    ri[tid] = bi;
+    wave_pivot_max(best, bi);
+    if (lane == 0) {
+      wvv[wv] = best;
+      wvi[wv] = bi;
+    }
     __syncthreads();
-    for (int s = nt / 2; s > 0; s >>= 1) {
-      if (tid < s) {
-        double o = rv[tid + s];
-        int oi = ri[tid + s];
-        if (o > rv[tid] || (o == rv[tid] && oi < ri[tid])) { rv[tid] = o; ri[tid] = oi; }
-      }
-      __syncthreads();
-    }
+    best = wvv[0];
+    bi = wvi[0];
+    for (int w = 1; w < 4; ++w) pivot_max(best, bi, wvv[w], wvi[w]);
     if (tid == 0) {
-      P.cval[par * G + b] = rv[0];
-      P.crow[par * G + b] = ri[0];
+      P.cval[par * G + b] = best;
+      P.crow[par * G + b] = bi;
     }
-    if (ri[0] != 0x7fffffff)
-      for (int c = tid; c < jb; c += nt) P.crowv[((long)par * G + b) * jb + c] = rows[(ri[0] - r0) * jb + c];
+    if (bi != 0x7fffffff)
+      for (int c = tid; c < jb; c += nt) P.crowv[((long)par * G + b) * jb + c] = rows[(bi - r0) * jb + c];
     if (gr >= r0 && gr < r1)
       for (int c = tid; c < jb; c += nt) P.topv[par * jb + c] = rows[(gr - r0) * jb + c];
-    if (!grid_sync(P.bar, G, gen)) return;
-    // global pivot
+    if (!grid_sync_arrive(P.arr, P.err, G, P.gen0 + jj + 1)) return;
+    // global pivot, computed by every wave from the G candidates
     double gb = -1.0;
-    int gi = 0x7fffffff, gw = 0;
-    for (int q = tid; q < G; q += nt) {
-      double v = __hip_atomic_load(&P.cval[par * G + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      int i = __hip_atomic_load(&P.crow[par * G + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (v > gb || (v == gb && i < gi)) { gb = v; gi = i; gw = q; }
-    }
-    rv[tid] = gb;
-    ri[tid] = gi;
-    __syncthreads();
-    if (tid == 0) {
-      double bv = rv[0];
-      int bidx = ri[0], bw = 0;
-      // recover the winner block: candidates are owned by row ranges
-      for (int t = 1; t < nt; ++t)
-        if (rv[t] > bv || (rv[t] == bv && ri[t] < bidx)) { bv = rv[t]; bidx = ri[t]; }
-      bw = (bidx - P.k) / R;
-      win = bidx;
-      ri[0] = bw;
-    }
-    __syncthreads();
-    (void)gw;
-    const int p = win, pw = ri[0];
+    int gi = 0x7fffffff;
+    for (int q = lane; q < G; q += 64)
+      pivot_max(gb, gi,
+                __hip_atomic_load(&P.cval[par * G + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                __hip_atomic_load(&P.crow[par * G + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    wave_pivot_max(gb, gi);
+    const int p = gi, pw = (gi - P.k) / R;
     for (int c = tid; c < jb; c += nt) {
       prow[c] = __hip_atomic_load(&P.crowv[((long)par * G + pw) * jb + c], __ATOMIC_RELAXED,
                                   __HIP_MEMORY_SCOPE_AGENT);
@@ -131,8 +129,8 @@ __global__ void __launch_bounds__(256) k_panel(PanelArgs P) {
         for (int c = tid; c < jb; c += nt) rows[(p - r0) * jb + c] = trow[c];
       if (gr >= r0 && gr < r1)
         for (int c = tid; c < jb; c += nt) rows[(gr - r0) * jb + c] = prow[c];
+      __syncthreads();
     }
-    __syncthreads();
     const double piv = prow[jj];
     if (b == 0 && tid == 0) {
       P.ipiv[gr] = p + 1;
@@ -259,66 +257,110 @@ __global__ void __launch_bounds__(256) k_gemm_f64_sub(int M, int N, int K, const
 }
 
 // ---------------------------------------------------------------------------
-// getrs pieces: row interchanges on B, blocked unit-lower and upper solves.
+// getrs: row interchanges on B, then block forward / backward substitution
+// with 128-row blocks whose diagonal triangles are inverted first.
 // ---------------------------------------------------------------------------
-__global__ void k_laswp_rhs(double *B, long ldb, int nrhs, int n, const int *__restrict__ ipiv,
-                            int reverse) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrhs) return;
-  double *b = B + ldb * r;
-  if (!reverse) {
-    for (int i = 0; i < n; ++i) {
-      int p = ipiv[i] - 1;
-      if (p != i) { double t = b[i]; b[i] = b[p]; b[p] = t; }
+constexpr int SB = 128;   // solve block
+constexpr int SR = 8;     // right-hand sides per persistent solve pass
+
+// B rows interchanged by ipiv[0..n) in order (dlaswp, incx = 1).  One block:
+// chunks of 256 pivots are screened with ballots so identity entries cost
+// nothing; thread q then applies the chunk's interchanges to column q.
+__global__ void __launch_bounds__(256) k_laswp_rhs(double *B, long ldb, int nrhs, int n,
+                                                   const int *__restrict__ ipiv) {
+  __shared__ unsigned long long mask[4];
+  __shared__ int pv[256];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int base = 0; base < n; base += 256) {
+    const int i = base + tid;
+    int p = i < n ? ipiv[i] - 1 : i;
+    pv[tid] = p;
+    unsigned long long m = __ballot(i < n && p != i);
+    if (lane == 0) mask[wv] = m;
+    __syncthreads();
+    for (int q = tid; q < nrhs; q += 256) {
+      double *b = B + ldb * q;
+      for (int w = 0; w < 4; ++w) {
+        unsigned long long mm = mask[w];
+        while (mm) {
+          const int t = __builtin_ctzll(mm);
+          mm &= mm - 1;
+          const int ii = base + 64 * w + t, pp = pv[64 * w + t];
+          double x = b[ii];
+          b[ii] = b[pp];
+          b[pp] = x;
+        }
+      }
     }
-  } else {
-    for (int i = n - 1; i >= 0; --i) {
-      int p = ipiv[i] - 1;
-      if (p != i) { double t = b[i]; b[i] = b[p]; b[p] = t; }
-    }
+    __syncthreads();
   }
 }
 
-// Solve the diagonal block [k, k+jb) in place (one block per RHS set).
-__global__ void k_trsv_block(const double *__restrict__ A, long lda, double *B, long ldb, int nrhs,
-                             int k, int jb, int upper) {
-  extern __shared__ double sx[];  // jb x nrhs
-  for (int q = threadIdx.x; q < jb * nrhs; q += blockDim.x)
-    sx[q] = B[(k + q % jb) + ldb * (q / jb)];
+// Inverses of the diagonal blocks of the LU factors (dtrti2 column sweeps in
+// LDS): Linv = L_kk^-1 (unit lower) and Uinv = U_kk^-1, written row-major.
+// Rows/columns past n are padded with the identity.
+__global__ void __launch_bounds__(256) k_diag_inverse(const double *__restrict__ A, long lda, int n,
+                                                      double *Linv, double *Uinv) {
+  extern __shared__ double sd[];  // SB x (SB + 1), column-major
+  __shared__ double tl[SB], tu[SB];
+  constexpr int LD = SB + 1;
+  const int kb = blockIdx.x, k0 = kb * SB, tid = threadIdx.x;
+  const int w = min(SB, n - k0);
+  for (int q = tid; q < SB * SB; q += 256) {
+    int r = q & (SB - 1), c = q >> 7;
+    sd[c * LD + r] = (r < w && c < w) ? A[(k0 + r) + lda * (long)(k0 + c)] : (r == c ? 1.0 : 0.0);
+  }
   __syncthreads();
-  if (!upper) {
-    for (int i = 0; i < jb; ++i) {
-      for (int q = threadIdx.x; q < (jb - 1 - i) * nrhs; q += blockDim.x) {
-        int w = jb - 1 - i, r = i + 1 + q % w, c = q / w;
-        sx[r + jb * c] -= A[(k + r) + lda * (k + i)] * sx[i + jb * c];
-      }
-      __syncthreads();
+  // U: j = 0..SB-1: u_jj <- 1/u_jj; U[0:j, j] <- -u_jj^-1 Uinv[0:j, 0:j] U[0:j, j]
+  // L: j = SB-1..0: L[j+1:, j] <- -Linv[j+1:, j+1:] L[j+1:, j]
+  for (int s = 0; s < SB; ++s) {
+    const int ju = s, jl = SB - 1 - s;
+    if (tid < SB) {
+      if (tid < ju) tu[tid] = sd[ju * LD + tid];
+    } else {
+      const int r = tid - SB;
+      if (r > jl) tl[r] = sd[jl * LD + r];
     }
-  } else {
-    for (int i = jb - 1; i >= 0; --i) {
-      if (threadIdx.x < nrhs) sx[i + jb * threadIdx.x] /= A[(k + i) + lda * (k + i)];
-      __syncthreads();
-      for (int q = threadIdx.x; q < i * nrhs; q += blockDim.x) {
-        int r = q % i, c = q / i;
-        sx[r + jb * c] -= A[(k + r) + lda * (k + i)] * sx[i + jb * c];
+    __syncthreads();
+    if (tid < SB) {
+      const int r = tid;
+      const double dj = 1.0 / sd[ju * LD + ju];
+      if (r < ju) {
+        double acc = 0.0;
+        for (int c = r; c < ju; ++c) acc += sd[c * LD + r] * tu[c];
+        sd[ju * LD + r] = -dj * acc;
       }
-      __syncthreads();
+    } else {
+      const int r = tid - SB;
+      if (r > jl) {
+        double acc = tl[r];
+        for (int c = jl + 1; c < r; ++c) acc += sd[c * LD + r] * tl[c];
+        sd[jl * LD + r] = -acc;
+      }
     }
+    __syncthreads();
+    if (tid == 0) sd[ju * LD + ju] = 1.0 / sd[ju * LD + ju];
+    __syncthreads();
   }
-  for (int q = threadIdx.x; q < jb * nrhs; q += blockDim.x)
-    B[(k + q % jb) + ldb * (q / jb)] = sx[q];
+  double *ol = Linv + (long)kb * SB * SB, *ou = Uinv + (long)kb * SB * SB;
+  for (int q = tid; q < SB * SB; q += 256) {
+    int c = q & (SB - 1), r = q >> 7;
+    const double v = sd[c * LD + r];
+    ol[r * SB + c] = c < r ? v : (c == r ? 1.0 : 0.0);
+    ou[r * SB + c] = c >= r ? v : 0.0;
+  }
 }
 
-// B[rows, :] -= A[rows, k:k+jb] X[k:k+jb, :], rows given by [r0, r1).
-__global__ void k_gemv_update(const double *__restrict__ A, long lda, double *B, long ldb,
-                              int nrhs, int k, int jb, int r0, int r1) {
-  int r = r0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= r1) return;
-  for (int c = 0; c < nrhs; ++c) {
-    double acc = 0.0;
-    for (int t = 0; t < jb; ++t) acc += A[r + lda * (k + t)] * B[(k + t) + ldb * c];
-    B[r + ldb * c] -= acc;
-  }
+// y[i][q] = B[i, q0 + q] (rows past n and columns past nq: 0), and back.
+__global__ void k_pack_y(const double *__restrict__ B, long ldb, int n, int np, int nq, double *y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  for (int q = 0; q < SR; ++q) y[(long)i * SR + q] = (i < n && q < nq) ? B[i + ldb * q] : 0.0;
+}
+__global__ void k_unpack_y(const double *__restrict__ y, int n, int nq, double *B, long ldb) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int q = 0; q < nq; ++q) B[i + ldb * q] = y[(long)i * SR + q];
 }
 
 // ---------------------------------------------------------------------------
@@ -345,7 +387,8 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t st,
   const int Gmax = cus;
   // workspace: barrier + candidates
   size_t wbytes = 256 + sizeof(double) * 2 * Gmax + sizeof(int) * 2 * Gmax +
-                  sizeof(double) * 2 * (size_t)Gmax * nb + sizeof(double) * 2 * nb + 64;
+                  sizeof(double) * 2 * (size_t)Gmax * nb + sizeof(double) * 2 * nb + 64 +
+                  sizeof(unsigned) * (Gmax + 16);
   char *ws = nullptr;
   DUST_HIP(hipMalloc(&ws, wbytes));
   DUST_HIP(hipMemsetAsync(ws, 0, wbytes, st));
@@ -356,6 +399,8 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t st,
   double *crowv = (double *)(crow + 2 * Gmax + 2);
   crowv = (double *)(((uintptr_t)crowv + 15) & ~(uintptr_t)15);
   double *topv = crowv + 2 * (size_t)Gmax * nb;
+  unsigned *arr = (unsigned *)(topv + 2 * nb);
+  unsigned gen0 = 0;
   const int mn = std::min(m, n);
   int rc = 0;
   for (int k = 0; k < mn && !rc; k += nb) {
@@ -370,7 +415,8 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t st,
       rc = DUST_E_UNSUPPORTED;
       break;
     }
-    PanelArgs P{dA, lda, m, k, jb, R, dipiv, dinfo, bar, cval, crow, crowv, topv};
+    PanelArgs P{dA, lda, m, k, jb, R, dipiv, dinfo, arr, bar + 2, gen0, cval, crow, crowv, topv};
+    gen0 += jb;
     void *args[] = {&P};
     if (lds > 64 * 1024)
       (void)hipFuncSetAttribute((const void *)k_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -424,27 +470,63 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t st,
   return 0;
 }
 
+struct GetrsWs {
+  DevBuf<double> linv, uinv, y;
+  DevBuf<unsigned> flags;  // [0] error, [1..] ready flags (forward, backward)
+  unsigned gen = 0;
+};
+static GetrsWs g_getrs;
+
 int lu_getrs_dev(int n, int nrhs, const double *dA, long lda, const int *dipiv, double *dB,
                  long ldb, hipStream_t st) {
   DUST_CHECK(n >= 0 && nrhs >= 0 && lda >= std::max(1, n) && ldb >= std::max(1, n), DUST_E_ARG,
              "invalid getrs dimensions");
   if (n == 0 || nrhs == 0) return 0;
-  const int jb = 64;
-  k_laswp_rhs<<<ceil_div(nrhs, 64), 64, 0, st>>>(dB, ldb, nrhs, n, dipiv, 0);
-  size_t sh = sizeof(double) * jb * nrhs;
-  DUST_CHECK(sh <= 64 * 1024, DUST_E_ARG, "too many right-hand sides for one solve");
-  for (int k = 0; k < n; k += jb) {
-    int w = std::min(jb, n - k);
-    k_trsv_block<<<1, 256, sizeof(double) * w * nrhs, st>>>(dA, lda, dB, ldb, nrhs, k, w, 0);
-    if (k + w < n)
-      k_gemv_update<<<ceil_div(n - k - w, 256), 256, 0, st>>>(dA, lda, dB, ldb, nrhs, k, w, k + w, n);
+  GetrsWs &w = g_getrs;
+  const int nblk = ceil_div(n, SB), np = nblk * SB;
+  DUST_TRY(w.linv.reserve((size_t)nblk * SB * SB));
+  DUST_TRY(w.uinv.reserve((size_t)nblk * SB * SB));
+  DUST_TRY(w.y.reserve((size_t)np * SR));
+  if (w.flags.n < (size_t)(1 + 2 * nblk)) {
+    w.flags.n = 0;
+    DUST_TRY(w.flags.reserve(1 + 2 * nblk));
+    DUST_HIP(hipMemset(w.flags.p, 0, sizeof(unsigned) * w.flags.n));
   }
-  for (int k = ((n - 1) / jb) * jb; k >= 0; k -= jb) {
-    int w = std::min(jb, n - k);
-    k_trsv_block<<<1, 256, sizeof(double) * w * nrhs, st>>>(dA, lda, dB, ldb, nrhs, k, w, 1);
-    if (k > 0) k_gemv_update<<<ceil_div(k, 256), 256, 0, st>>>(dA, lda, dB, ldb, nrhs, k, w, 0, k);
+  static bool attr = false;
+  const size_t lds = sizeof(double) * SB * (SB + 1);
+  if (!attr) {
+    DUST_HIP(hipFuncSetAttribute((const void *)k_diag_inverse,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr = true;
   }
+  k_laswp_rhs<<<1, 256, 0, st>>>(dB, ldb, nrhs, n, dipiv);
+  k_diag_inverse<<<nblk, 256, lds, st>>>(dA, lda, n, w.linv.p, w.uinv.p);
   DUST_HIP(hipGetLastError());
+  const int G = std::min(nblk, device_cus());
+  for (int q0 = 0; q0 < nrhs; q0 += SR) {
+    const int nq = std::min(SR, nrhs - q0);
+    k_pack_y<<<ceil_div(np, 256), 256, 0, st>>>(dB + ldb * q0, ldb, n, np, nq, w.y.p);
+    for (int upper = 0; upper < 2; ++upper) {
+      if (++w.gen == 0) ++w.gen;
+      BlockSolveArgs<double> sa{dA, lda, n, nblk, upper ? w.uinv.p : w.linv.p, w.y.p,
+                                w.flags.p + 1 + upper * nblk, w.gen, w.flags.p};
+      void *args[] = {&sa};
+      hipError_t e = upper ? hipLaunchCooperativeKernel((const void *)k_block_solve<double, 1, 512>,
+                                                        dim3(G), dim3(512), args, 0, st)
+                           : hipLaunchCooperativeKernel((const void *)k_block_solve<double, 0, 512>,
+                                                        dim3(G), dim3(512), args, 0, st);
+      if (e != hipSuccess) {
+        set_error("triangular solve launch failed: %s", hipGetErrorString(e));
+        return DUST_E_HIP;
+      }
+    }
+    k_unpack_y<<<ceil_div(n, 256), 256, 0, st>>>(w.y.p, n, nq, dB + ldb * q0, ldb);
+    DUST_HIP(hipGetLastError());
+  }
+  unsigned herr = 0;
+  DUST_HIP(hipMemcpyAsync(&herr, w.flags.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  DUST_HIP(hipStreamSynchronize(st));
+  DUST_CHECK(herr == 0, DUST_E_HIP, "triangular solve flag wait timed out");
   return 0;
 }
 

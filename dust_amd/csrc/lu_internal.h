@@ -45,6 +45,162 @@ __device__ inline bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &gen)
   return ok != 0;
 }
 
+// Bounded spin until *f == gen (acquire).  On time-out raises *err; returns
+// false when *err is raised by anyone.
+__device__ inline bool wait_flag(unsigned *f, unsigned gen, unsigned *err) {
+  long spins = 0;
+  while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gen) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins > (1L << 25)) {
+      atomicExch(err, 1u);
+      return false;
+    }
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+  }
+  return true;
+}
+// Grid barrier without a shared counter: block b publishes arrival number
+// `target` in arr[b] and wave 0 polls all G words (G <= 128).  A block that
+// has already left this barrier can only have published target + 1, so the
+// test is "arr[q] - target >= 0" in wrap-around arithmetic.  Bounded spin;
+// a time-out raises *err.
+__device__ inline bool grid_sync_arrive(unsigned *arr, unsigned *err, int G, unsigned target) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    __hip_atomic_store(&arr[blockIdx.x], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    long spins = 0;
+    int good = 1;
+    while (true) {
+      bool mine = true;
+      for (int q = lane; q < G; q += 64)
+        mine &= (int)(__hip_atomic_load(&arr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) >= 0;
+      if (__all(mine)) break;
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > (1L << 26)) {
+        if (lane == 0) atomicExch(err, 1u);
+        good = 0;
+        break;
+      }
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        good = 0;
+        break;
+      }
+    }
+    if (lane == 0) {
+      __threadfence();
+      ok = good;
+    }
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+// Block substitution with 128-row blocks for up to 8 right-hand sides held as
+// y[row][8], one workgroup per block row in dependency order (forward: L unit
+// lower from the off-diagonal blocks of A; backward: U).  Every block row
+// waits on the ready flag (generation-stamped) of each block row it reads.
+// Each thread owns CW columns of its row: the row segment of every
+// off-diagonal block and of the diagonal-block inverse is loaded before the
+// wait, so a ready flag is followed only by the 4 KB y block and FMAs.
+// Inv holds the inverse of every diagonal block, row-major (a thread's row
+// segment is contiguous).
+template <typename T>
+struct BlockSolveArgs {
+  const T *A;
+  long lda;
+  int n, nblk;
+  const T *Inv;       // [nblk][128*128], row-major blocks
+  T *y;               // [nblk*128][8], in place
+  unsigned *ready;    // [nblk]
+  unsigned gen;
+  unsigned *err;
+};
+
+template <typename T, int UPPER, int NT>
+__global__ void __launch_bounds__(NT) k_block_solve(BlockSolveArgs<T> a) {
+  constexpr int BS = 128, NQ = NT / BS, CW = BS / NQ, R8 = 8;
+  __shared__ T sy[BS * R8];
+  __shared__ T sp[NQ * BS * R8];
+  __shared__ int ok;
+  const int t = threadIdx.x, r = t & (BS - 1), h = t / BS;
+  for (int it = blockIdx.x; it < a.nblk; it += gridDim.x) {
+    const int kb = UPPER ? a.nblk - 1 - it : it;
+    const int gr = kb * BS + r;
+    const bool rok = gr < a.n;
+    T inv[CW];
+    {
+      const T *Iv = a.Inv + (long)kb * BS * BS + r * BS + h * CW;
+#pragma unroll
+      for (int c = 0; c < CW; ++c) inv[c] = Iv[c];
+    }
+    T acc[R8];
+#pragma unroll
+    for (int q = 0; q < R8; ++q) acc[q] = T(0);
+    const int nj = UPPER ? a.nblk - 1 - kb : kb;
+    for (int jj = 0; jj < nj; ++jj) {
+      const int j = UPPER ? a.nblk - 1 - jj : jj;
+      T l[CW];
+      const int c0 = j * BS + h * CW;
+      const T *Lr = a.A + gr + a.lda * (long)c0;
+      const int ldi = (int)a.lda;  // column offsets inside one block fit in 32 bits
+#pragma unroll
+      for (int c = 0; c < CW; ++c) l[c] = (rok && c0 + c < a.n) ? Lr[ldi * c] : T(0);
+      if (t == 0) ok = wait_flag(&a.ready[j], a.gen, a.err) ? 1 : 0;
+      __syncthreads();
+      if (!ok) return;
+      for (int q = t; q < BS * R8; q += NT) sy[q] = a.y[(long)j * BS * R8 + q];
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < CW; ++c) {
+        if ((c & 3) == 0) __builtin_amdgcn_sched_barrier(0);  // keep the LDS reads near their use
+#pragma unroll
+        for (int q = 0; q < R8; ++q) acc[q] += l[c] * sy[(h * CW + c) * R8 + q];
+      }
+      __syncthreads();
+    }
+    // right-hand side minus the partial sums of the NQ column groups
+#pragma unroll
+    for (int q = 0; q < R8; ++q) sp[(h * BS + r) * R8 + q] = acc[q];
+    __syncthreads();
+    if (h == 0) {
+#pragma unroll
+      for (int q = 0; q < R8; ++q) {
+        T v = a.y[(long)gr * R8 + q];
+        for (int g = 0; g < NQ; ++g) v -= sp[(g * BS + r) * R8 + q];
+        sy[r * R8 + q] = v;
+      }
+    }
+    __syncthreads();
+    // diagonal block: y_kb = Inv_kb (rhs)
+#pragma unroll
+    for (int q = 0; q < R8; ++q) acc[q] = T(0);
+#pragma unroll
+    for (int c = 0; c < CW; ++c) {
+      if ((c & 3) == 0) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < R8; ++q) acc[q] += inv[c] * sy[(h * CW + c) * R8 + q];
+    }
+#pragma unroll
+    for (int q = 0; q < R8; ++q) sp[(h * BS + r) * R8 + q] = acc[q];
+    __syncthreads();
+    if (h == 0) {
+#pragma unroll
+      for (int q = 0; q < R8; ++q) {
+        T v = sp[r * R8 + q];
+        for (int g = 1; g < NQ; ++g) v += sp[(g * BS + r) * R8 + q];
+        a.y[(long)gr * R8 + q] = v;
+      }
+      __threadfence();
+    }
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(&a.ready[kb], a.gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
 
 // xGETRF semantics on device memory; *info as LAPACK.  nb is the panel width.
 int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t st, int nb,

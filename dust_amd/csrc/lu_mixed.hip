@@ -509,10 +509,10 @@ __global__ void __launch_bounds__(256) k_mx_trsm(TrsmArgs a) {
       wave_max_atomic(nan ? INFINITY : m, &a.flags[0]);
       frag_store_L(a.Lf, a.f16, (R0 + r16) / 16, lane, sX, r16);
     } else {
-      float *Ui = a.Uinv + (long)kb * NB * NB;
+      float *Ui = a.Uinv + (long)kb * NB * NB;  // row-major
       for (int q = lane; q < 16 * NB; q += 64) {
-        int r = q & 15, c = q >> 4;
-        Ui[R0 + r16 + r + NB * c] = sX[c * LDX + r16 + r];
+        int c = q & (NB - 1), r = q >> 7;
+        Ui[(R0 + r16 + r) * NB + c] = sX[c * LDX + r16 + r];
       }
     }
   } else {
@@ -584,10 +584,10 @@ __global__ void __launch_bounds__(256) k_mx_trsm(TrsmArgs a) {
       }
       frag_store_U(a.Uf, a.f16, (C0 + c16) / 16, lane, sY, c16);
     } else {
-      float *Li = a.Linv + (long)kb * NB * NB;
+      float *Li = a.Linv + (long)kb * NB * NB;  // row-major
       for (int q = lane; q < 16 * NB; q += 64) {
         int r = q & (NB - 1), c = q >> 7;
-        Li[r + NB * (C0 + c16 + c)] = sY[(c16 + c) * LDT + r];
+        Li[r * NB + C0 + c16 + c] = sY[(c16 + c) * LDT + r];
       }
     }
   }
@@ -682,83 +682,6 @@ __global__ void k_mx_update_x(double *x, long ldx, int n, const float *__restric
   for (int q = 0; q < NR; ++q) {
     double d = (double)y[(long)i * NR + q];
     x[i + ldx * q] = accumulate ? x[i + ldx * q] + d : d;
-  }
-}
-
-__device__ inline bool wait_flag(unsigned *f, unsigned gen, unsigned *err) {
-  long spins = 0;
-  while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gen) {
-    __builtin_amdgcn_s_sleep(1);
-    if (++spins > (1L << 25)) {
-      atomicExch(err, 1u);
-      return false;
-    }
-    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
-  }
-  return true;
-}
-
-struct SolveArgs {
-  const float *W;
-  long ld;
-  const float *Inv;   // Linv (forward) or Uinv (backward), [nblk][128*128]
-  float *y;           // [np][8], in place
-  unsigned *ready;    // [nblk]
-  unsigned gen;
-  int nblk;
-  unsigned *err;
-};
-
-// Forward (upper = 0: L y = c, L unit lower) or backward (upper = 1: U x = y)
-// block substitution.  Block row kb is finished by one workgroup; the
-// workgroups take block rows in dependency order, so the earliest unfinished
-// block row always has its inputs ready.
-template <int UPPER>
-__global__ void __launch_bounds__(256) k_mx_solve(SolveArgs a) {
-  __shared__ float sy[NB * NR];
-  __shared__ float sacc[NB * NR];
-  __shared__ int ok;
-  const int t = threadIdx.x, r = t & (NB - 1), h = t >> 7;
-  for (int it = blockIdx.x; it < a.nblk; it += gridDim.x) {
-    const int kb = UPPER ? a.nblk - 1 - it : it;
-    float acc[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] = a.y[((long)kb * NB + r) * NR + 4 * h + q];
-    const int nj = UPPER ? a.nblk - 1 - kb : kb;
-    for (int jj = 0; jj < nj; ++jj) {
-      const int j = UPPER ? a.nblk - 1 - jj : jj;
-      if (t == 0) ok = wait_flag(&a.ready[j], a.gen, a.err) ? 1 : 0;
-      __syncthreads();
-      if (!ok) return;
-      *(f4 *)(sy + 4 * t) = *(const f4 *)(a.y + (long)j * NB * NR + 4 * t);
-      __syncthreads();
-      const float *Lc = a.W + (long)kb * NB + r + a.ld * (long)j * NB;
-#pragma unroll 8
-      for (int c = 0; c < NB; ++c) {
-        const float l = Lc[a.ld * c];
-        const f4 v = *(const f4 *)(sy + c * NR + 4 * h);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[q] = fmaf(-l, v[q], acc[q]);
-      }
-      __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) sacc[r * NR + 4 * h + q] = acc[q];
-    __syncthreads();
-    const float *Iv = a.Inv + (long)kb * NB * NB + r;
-    float out[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 8
-    for (int c = 0; c < NB; ++c) {
-      const float l = Iv[NB * c];
-      const f4 v = *(const f4 *)(sacc + c * NR + 4 * h);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) out[q] = fmaf(l, v[q], out[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) a.y[((long)kb * NB + r) * NR + 4 * h + q] = out[q];
-    __threadfence();
-    __syncthreads();
-    if (t == 0) __hip_atomic_store(&a.ready[kb], a.gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -972,12 +895,14 @@ int mx_solve(MixedWs &w, int np, hipStream_t st, int cus) {
   const int nblk = np / NB;
   const int G = std::min(nblk, cus);
   for (int upper = 0; upper < 2; ++upper) {
-    SolveArgs sa{w.W.p, (long)np, upper ? w.Uinv.p : w.Linv.p, w.y.p,
-                 w.flags.p + 3 + upper * nblk, ++w.gen, nblk, w.flags.p + 2};
-    if (w.gen == 0) sa.gen = ++w.gen;  // generation 0 is the cleared state
+    if (++w.gen == 0) ++w.gen;  // generation 0 is the cleared state
+    BlockSolveArgs<float> sa{w.W.p, (long)np, np, nblk, upper ? w.Uinv.p : w.Linv.p, w.y.p,
+                             w.flags.p + 3 + upper * nblk, w.gen, w.flags.p + 2};
     void *args[] = {&sa};
-    hipError_t e = upper ? hipLaunchCooperativeKernel((const void *)k_mx_solve<1>, dim3(G), dim3(256), args, 0, st)
-                         : hipLaunchCooperativeKernel((const void *)k_mx_solve<0>, dim3(G), dim3(256), args, 0, st);
+    hipError_t e = upper ? hipLaunchCooperativeKernel((const void *)k_block_solve<float, 1, 256>,
+                                                      dim3(G), dim3(256), args, 0, st)
+                         : hipLaunchCooperativeKernel((const void *)k_block_solve<float, 0, 256>,
+                                                      dim3(G), dim3(256), args, 0, st);
     if (e != hipSuccess) {
       set_error("triangular solve launch failed: %s", hipGetErrorString(e));
       return DUST_E_HIP;
