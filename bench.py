@@ -2,13 +2,23 @@
 
 A step is one full FMM pass -- sort_particles + calculate_multipole +
 apply_multipole (velocity and stretching) -- over the synthetic 3-helix
-vortex wake of 1,000,000 particles, particles resident in HBM.  The metric is
-particle updates per second of the tuned form (the plain FP64 form is timed
-beside it).  The dense AIC solve (N = 16,384 panels of a perturbed ellipsoid,
-8 right-hand sides) is timed after the FMM, plain FP64 LU vs the mixed
-precision LU with FP64 refinement.
+vortex wake of 1,000,000 particles, particles resident in HBM.  `value` is
+particle updates per second of the tuned form; the plain FP64 form is timed
+beside it.  The dense AIC solve (N = 16,384 panels of a perturbed ellipsoid,
+8 right-hand sides) is timed after the FMM: plain FP64 getrf + getrs against
+the mixed-precision LU with FP64 refinement.
 
-Prints ONE JSON line (see README of the driver contract).
+roofline: the dominant kernel of the tuned pass is the FP32 P2P item kernel;
+its algorithmic work is 60 flop per (target, source) pair (velocity +
+stretching, rsqrt and division counted as one each), timed with HIP events
+recorded on the FMM stream around the kernel during the timed steps.
+traffic: HBM bytes per launch from the committed rocprofv3 counter summary
+(profiles/traffic.json), when present.
+
+--copy-probe runs a plain device copy of known size, for calibrating the
+FETCH_SIZE / WRITE_SIZE counters.
+
+Prints ONE JSON line on stdout.
 """
 import argparse
 import json
@@ -21,7 +31,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"fp64": 78.6, "fp32": 157.3, "hbm": 8000.0}  # TFLOP/s, GB/s (MI355X)
+PEAK = {"fp64": 78.6, "fp32": 157.3, "fp16": 2500.0, "hbm": 8000.0}  # TFLOP/s, GB/s (MI355X)
+METRIC = "wake particle updates/s (FMM vel+stretch) & AIC LU time, tuned vs FP64 GPU"
 
 
 def log(*a):
@@ -47,48 +58,50 @@ def time_fmm(torch, L, O, tree, degree, dev, n, form, steps, warmup, knobs=None)
     nl = C.c_int()
     for _ in range(warmup):
         L.check(lib.dust_fmm_pass_dev(h, C.byref(nl)), "pass")
-    L.check(lib.dust_fmm_synchronize(h), "sync")
-    stream = torch.cuda.ExternalStream(lib.dust_fmm_stream(h))
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        L.check(lib.dust_fmm_pass_dev(h, C.byref(nl)), "pass")
-    e1.record(stream)
-    L.check(lib.dust_fmm_synchronize(h), "sync")
-    torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) * 1e3 / steps
-    ev = e0.elapsed_time(e1) / steps
-    # stage timing pass (HIP events around every stage on the handle's stream)
+    # one untimed round with stage events on, so the event pool already holds
+    # every event the timed steps record
     g.set_timing(True)
     for _ in range(steps):
         L.check(lib.dust_fmm_pass_dev(h, C.byref(nl)), "pass")
     L.check(lib.dust_fmm_synchronize(h), "sync")
-    stages = g.stage_times()
+    # timed region; the library records HIP events on its stream around every
+    # stage (the P2P stage brackets exactly the P2P item kernel)
+    g.set_timing(True)  # clears the accumulated stage times
+    before = g.stage_times()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L.check(lib.dust_fmm_pass_dev(h, C.byref(nl)), "pass")
+    L.check(lib.dust_fmm_synchronize(h), "sync")
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    after = g.stage_times()
     g.set_timing(False)
+    stages = {k: ((after[k][0] - before[k][0]), after[k][1] - before[k][1]) for k in after}
     p2p_pairs, m2l_pairs = g.work()
     vel = torch.empty((n, 3), dtype=torch.float64, device="cuda")
     st = torch.empty((n, 3), dtype=torch.float64, device="cuda")
     L.check(lib.dust_fmm_results_dev(h, vel.data_ptr(), st.data_ptr(), None), "results")
     L.check(lib.dust_fmm_synchronize(h), "sync")
-    res = dict(ms=wall, ms_events=ev, n_leaves=nl.value, stages={k: v[0] / steps for k, v in stages.items()},
+    res = dict(ms=wall, n_leaves=nl.value, stages={k: v[0] / steps for k, v in stages.items()},
                stage_launches={k: v[1] for k, v in stages.items()}, p2p_pairs=p2p_pairs,
                m2l_pairs=m2l_pairs, vel=vel.cpu().numpy(), stretch=st.cpu().numpy())
     g.destroy()
     return res
 
 
-def time_lu(torch, L, LS, W, n, nrhs, steps):
-    """AIC assembly + plain FP64 getrf/getrs and the mixed solve, device resident."""
+def time_lu(torch, L, W, n_req, nrhs, steps):
+    """AIC assembly, then plain FP64 getrf + getrs and the mixed solves on the
+    same device-resident matrix and right-hand sides b = B s."""
     import ctypes as C
     lib = L.lib()
-    ver, nv = W.ellipsoid(128, 128) if n == 16384 else W.ellipsoid(int(np.sqrt(n)), int(np.sqrt(n)))
+    m = int(round(np.sqrt(n_req)))
+    ver, nv = W.ellipsoid(m, m)
     n = len(nv)
     dver = torch.from_numpy(np.ascontiguousarray(ver.reshape(n, 12))).cuda()
     dnv = torch.from_numpy(nv.astype(np.int32)).cuda()
-    A = torch.empty((n, n), dtype=torch.float64, device="cuda")  # column-major view: A.T rows
+    # torch row-major (n, n) buffers hold the column-major matrices
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
     B = torch.empty((n, n), dtype=torch.float64, device="cuda")
     prm = L.AicParams()
     prm.ff_ratio_doublet = 10.0
@@ -103,21 +116,24 @@ def time_lu(torch, L, LS, W, n, nrhs, steps):
     t_asm = time.perf_counter() - t0
     out["assembly_ms"] = t_asm * 1e3
     out["assembly_GBps"] = 16.0 * n * n / t_asm / 1e9
-    # right-hand sides b = B s (column-major B stored as B.T in torch)
+    # right-hand sides b_k = B sigma_k, sigma_k(j) = n_j . (-U_k)
     U = torch.from_numpy(W.freestreams(nrhs)).cuda()
-    G = torch.from_numpy(ver.reshape(n, 4, 3)).cuda()
-    # normals as in calc_geo_data
-    v = G
+    v = torch.from_numpy(ver.reshape(n, 4, 3)).cuda()
     q = torch.where(dnv[:, None] == 4, torch.cross(v[:, 2] - v[:, 0], v[:, 3] - v[:, 1], dim=1),
                     torch.cross(v[:, 2] - v[:, 1], v[:, 0] - v[:, 1], dim=1))
     nor = q / q.norm(dim=1, keepdim=True)
-    S = -(nor @ U.T)  # (n, nrhs)
-    Bm = B.T  # logical column-major matrix
-    rhs = (Bm @ S).contiguous()  # (n, nrhs) row-major -> need column-major
-    rhs_cm = rhs.T.contiguous()  # (nrhs, n) = column-major (n, nrhs)
+    S = -(nor @ U.T)                      # (n, nrhs)
+    rhs = (B.T @ S)                       # logical B times S, (n, nrhs)
+    rhs_cm = rhs.T.contiguous()           # column-major (n, nrhs)
     Am = A.clone()
     ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
-    # plain FP64
+
+    def berr(Xcm):
+        X = Xcm.T
+        r = A.T @ X - rhs
+        return float((r.abs().max(dim=0).values /
+                      (A.abs().sum(dim=0).max() * X.abs().max(dim=0).values)).max().item())
+
     ts = []
     for it in range(steps):
         Am.copy_(A)
@@ -131,14 +147,12 @@ def time_lu(torch, L, LS, W, n, nrhs, steps):
         ts.append(time.perf_counter() - t0)
         assert info == 0, info
     out["fp64_ms"] = 1e3 * float(np.median(ts))
-    Xp = X.T  # (n, nrhs)
-    r = (A.T @ Xp - rhs)
-    out["fp64_residual"] = float((r.abs().max() / (A.abs().sum(dim=0).max() * Xp.abs().max())).item())
-    out["getrf_TFLOPs"] = (2.0 / 3.0) * n ** 3 / (out["fp64_ms"] / 1e3) / 1e12
-    # mixed
+    Xp = X.clone()
+    out["fp64_backward_error"] = berr(Xp)
+    out["fp64_getrf_getrs_TFLOPs"] = ((2.0 / 3.0) * n ** 3 + 2.0 * n * n * nrhs) / (out["fp64_ms"] / 1e3) / 1e12
     for prec, code in (("fp16s", L.LU_FP16S), ("fp32", L.LU_FP32)):
         ts = []
-        ok = True
+        rc = 0
         for it in range(steps):
             X2 = rhs_cm.clone()
             iters = C.c_int()
@@ -150,20 +164,19 @@ def time_lu(torch, L, LS, W, n, nrhs, steps):
             torch.cuda.synchronize()
             ts.append(time.perf_counter() - t0)
             if rc != 0:
-                ok = False
                 break
-        if ok:
-            Xm = X2.T
+        if rc == 0:
             out[f"{prec}_ms"] = 1e3 * float(np.median(ts))
             out[f"{prec}_iters"] = iters.value
-            out[f"{prec}_backward_error"] = resid.value
-            out[f"{prec}_rel_diff_vs_fp64"] = float(((Xm - Xp).norm() / Xp.norm()).item())
-            out[f"{prec}_speedup"] = out["fp64_ms"] / out[f"{prec}_ms"]
+            out[f"{prec}_backward_error"] = berr(X2)
+            out[f"{prec}_rel_diff_vs_fp64"] = float(((X2 - Xp).norm() / Xp.norm()).item())
+            out[f"{prec}_speedup_vs_fp64"] = out["fp64_ms"] / out[f"{prec}_ms"]
         else:
             out[f"{prec}_error"] = L.lib().dust_last_error().decode()
     out["n"] = n
     out["nrhs"] = nrhs
-    return out
+    out["steps"] = steps
+    return out, (A, rhs_cm)
 
 
 def cpu_baseline(W, n_sample, degree):
@@ -182,8 +195,112 @@ def cpu_baseline(W, n_sample, degree):
     return dict(value=n_sample / dt, unit="particle-updates/s", cores=orc.num_threads(),
                 kind="port",
                 sample=f"oracle FMM pass (sort+multipole+apply, FP64, OpenMP) on a "
-                       f"{n_sample}-particle draw of the same helix generator, {lev} levels, "
-                       f"order {degree}, {dt:.1f} s")
+                       f"{n_sample}-particle draw of the same helix generator, {lev} levels "
+                       f"(same particles per leaf), order {degree}, {dt:.1f} s")
+
+
+def cpu_lu_baseline(A, rhs_cm):
+    """The reference's own LU call sites are LAPACK dgetrf + dgetrs: timed here
+    with the host LAPACK that scipy links, on the same matrix."""
+    import scipy.linalg.lapack as lapack
+    import threadpoolctl
+    An = np.asfortranarray(A.cpu().numpy().T)
+    Bn = np.asfortranarray(rhs_cm.cpu().numpy().T)
+    info = threadpoolctl.threadpool_info()
+    cores = max([d.get("num_threads", 1) for d in info] or [1])
+    t0 = time.perf_counter()
+    lu, piv, inf = lapack.dgetrf(An, overwrite_a=True)
+    x, inf2 = lapack.dgetrs(lu, piv, Bn)
+    dt = time.perf_counter() - t0
+    lib = ",".join(sorted({d.get("internal_api", "?") for d in info})) or "lapack"
+    return dict(ms=dt * 1e3, cores=cores, kind="reference",
+                sample=f"host LAPACK ({lib}) dgetrf + dgetrs, same N={An.shape[0]} matrix, "
+                       f"{Bn.shape[1]} right-hand sides")
+
+
+def copy_probe(torch, gib=2.0, reps=5):
+    """Plain device copy of a known byte count (counter calibration)."""
+    n = int(gib * (1 << 30)) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8,
+                      "GBps": 2 * n * 8 / dt / 1e9}))
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        e = t["kernels"][kernel]
+        return float(e["fetch_bytes"] + e["write_bytes"]), e
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
+def _pmc_per_dispatch(db, counter):
+    """kernel name -> mean counter value per dispatch (summed over instances)."""
+    import collections
+    import sqlite3
+    con = sqlite3.connect(db)
+    cur = con.cursor()
+    cols = [d[1] for d in cur.execute("pragma table_info(counters_collection)")]
+    ci = {c: i for i, c in enumerate(cols)}
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    for r in cur.execute("select * from counters_collection"):
+        if r[ci["counter_name"]] != counter:
+            continue
+        per[r[ci["kernel_name"]]][r[ci["dispatch_id"]]] += r[ci["value"]]
+    return {k: sum(v.values()) / len(v) for k, v in per.items()}
+
+
+def summarize_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_bytes, out_path):
+    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the bench kernels, scaled
+    by what the same counters read for a device copy of known size."""
+    cf = max(_pmc_per_dispatch(copy_fetch_db, "FETCH_SIZE").values())
+    cw = max(_pmc_per_dispatch(copy_write_db, "WRITE_SIZE").values())
+    kf = copy_bytes / (cf * 1024.0)
+    kw = copy_bytes / (cw * 1024.0)
+    fe = _pmc_per_dispatch(fetch_db, "FETCH_SIZE")
+    wr = _pmc_per_dispatch(write_db, "WRITE_SIZE")
+    kernels = {}
+    for name in fe:
+        short = name.split("(")[0].split("::")[-1].split("<")[0]
+        if short in kernels and kernels[short]["fetch_bytes"] >= fe[name] * 1024.0 * kf:
+            continue
+        kernels[short] = dict(name=name[:160], fetch_KiB=fe[name], write_KiB=wr.get(name, 0.0),
+                              fetch_bytes=fe[name] * 1024.0 * kf,
+                              write_bytes=wr.get(name, 0.0) * 1024.0 * kw)
+    res = dict(copy_bytes=copy_bytes, copy_fetch_KiB=cf, copy_write_KiB=cw,
+               fetch_scale=kf, write_scale=kw, kernels=kernels)
+    with open(out_path, "w") as f:
+        json.dump(res, f, indent=1, sort_keys=True)
+    print(json.dumps({k: (v["fetch_bytes"], v["write_bytes"]) for k, v in kernels.items()}))
+
+
+def summarize_trace(db, pattern=None):
+    """Per-kernel dispatch statistics of a rocprofv3 --kernel-trace database:
+    calls, total, mean, min and max duration."""
+    import re
+    import sqlite3
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, count(*), sum(end-start), avg(end-start), min(end-start), "
+                       "max(end-start) from kernels group by name order by 3 desc").fetchall()
+    tot = sum(r[2] for r in rows)
+    print(f"{'kernel':<78} {'calls':>6} {'total_ms':>10} {'mean_us':>10} {'min_us':>9} "
+          f"{'max_us':>9} {'pct':>6}")
+    for name, cnt, sm, av, mn, mx in rows:
+        if pattern and not re.search(pattern, name):
+            continue
+        print(f"{name[:78]:<78} {cnt:>6d} {sm / 1e6:>10.3f} {av / 1e3:>10.1f} {mn / 1e3:>9.1f} "
+              f"{mx / 1e3:>9.1f} {100.0 * sm / tot:>6.2f}")
 
 
 def main():
@@ -196,19 +313,33 @@ def main():
     ap.add_argument("--lu-n", type=int, default=16384)
     ap.add_argument("--no-lu", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=125_000)
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--near", type=float, default=1.0,
                     help="tuned form: FP64 for pairs closer than this many core radii")
     ap.add_argument("--no-fp64", action="store_true", help="skip timing the plain FP64 form")
+    ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB",
+                                                          "COPY_WRITE_DB"))
+    ap.add_argument("--summarize-trace", metavar="TRACE_DB")
+    ap.add_argument("--kernels", default=None, help="regex filter for --summarize-trace")
     args = ap.parse_args()
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace, args.kernels)
+        return
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc, copy_bytes=2 * (1 << 30),
+                      out_path=os.path.join(ROOT, "profiles", "traffic.json"))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this path has no "
               "multi-GPU form", file=sys.stderr)
         sys.exit(2)
     import torch
 
+    if args.copy_probe:
+        copy_probe(torch)
+        return
     from dust_amd import _lib as L
-    from dust_amd import linsys as LS
     from dust_amd import octree as O
     from dust_amd import workloads as W
 
@@ -221,48 +352,39 @@ def main():
     dev = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda()
            for k, v in dict(cen=cen, dir=dr, mag=mag, rv=rv).items()}
     torch.cuda.synchronize()
-    log("timing plain FP64 FMM")
-    plain = time_fmm(torch, L, O, tree, args.degree, dev, n, "fp64", args.steps, args.warmup)
-    log(f"plain: {plain['ms']:.2f} ms/pass stages {plain['stages']}")
-    tuned = None
-    try:
-        log("timing tuned FMM")
-        tuned = time_fmm(torch, L, O, tree, args.degree, dev, n, "tuned", args.steps, args.warmup,
-                         knobs={"near_factor": args.near})
-        log(f"tuned: {tuned['ms']:.2f} ms/pass stages {tuned['stages']}")
-    except Exception as e:  # tuned form not built in this library
-        log(f"tuned form unavailable: {e}")
-    head = tuned if tuned is not None else plain
-    form = "tuned" if tuned is not None else "fp64"
-    value = n / (head["ms"] / 1e3)
-    # roofline of the dominant kernel (largest stage)
-    st = head["stages"]
-    dom = max(st, key=lambda k: st[k])
-    if dom == "p2p":
-        flops = 60.0 * head["p2p_pairs"]
-        peak = PEAK["fp32"] if form == "tuned" else PEAK["fp64"]
-    elif dom == "m2l":
-        nm = (args.degree + 1) * (args.degree + 2) * (args.degree + 3) // 6
-        flops = 72.0 * nm * nm * head["m2l_pairs"]
-        peak = PEAK["fp32"] if form == "tuned" else PEAK["fp64"]
-    else:
-        flops = None
-    roof = None
-    if flops is not None:
-        ach = flops / (st[dom] / 1e3) / 1e12
-        roof = dict(bound="valu" if dom == "p2p" else "mfma", kernel=dom, achieved=ach, peak=peak,
-                    unit="TFLOP/s", frac=ach / peak, traffic=None,
-                    algorithmic=f"{'60 flop/pair x %.4g pairs' % head['p2p_pairs'] if dom == 'p2p' else '72 n_mon^2 flop x %.4g M2L pairs' % head['m2l_pairs']}")
+    plain = None
+    if not args.no_fp64:
+        log("timing plain FP64 FMM")
+        plain = time_fmm(torch, L, O, tree, args.degree, dev, n, "fp64", args.steps, args.warmup)
+        log(f"plain: {plain['ms']:.2f} ms/pass stages {plain['stages']}")
+    log("timing tuned FMM")
+    tuned = time_fmm(torch, L, O, tree, args.degree, dev, n, "tuned", args.steps, args.warmup,
+                     knobs={"near_factor": args.near})
+    log(f"tuned: {tuned['ms']:.2f} ms/pass stages {tuned['stages']}")
+    value = n / (tuned["ms"] / 1e3)
+    # roofline of the dominant kernel of the tuned pass: FP32 P2P items
+    st = tuned["stages"]
+    flops = 60.0 * tuned["p2p_pairs"]
+    ach = flops / (st["p2p"] / 1e3) / 1e12
+    traffic, tr = load_traffic("k_p2p_f32")
+    roof = dict(bound="valu", kernel="k_p2p_f32 (FP32 P2P items, FP64 near pairs)",
+                achieved=ach, peak=PEAK["fp32"], unit="TFLOP/s", frac=ach / PEAK["fp32"],
+                traffic=traffic, launch_ms=st["p2p"],
+                algorithmic=f"60 flop/pair x {tuned['p2p_pairs']:.6g} pairs per launch",
+                traffic_source=None if tr is None else "profiles/traffic.json")
+    nm = (args.degree + 1) * (args.degree + 2) * (args.degree + 3) // 6
+    m2l_ach = 72.0 * nm * nm * tuned["m2l_pairs"] / (st["m2l"] / 1e3) / 1e12
     acc = None
-    if tuned is not None:
+    if plain is not None:
         dv = np.linalg.norm(tuned["vel"] - plain["vel"]) / np.linalg.norm(plain["vel"])
         ds = np.linalg.norm(tuned["stretch"] - plain["stretch"]) / np.linalg.norm(plain["stretch"])
         acc = dict(vel_rel_l2_vs_fp64=dv, stretch_rel_l2_vs_fp64=ds)
     lu = None
+    lu_mats = None
     if not args.no_lu:
         log(f"timing AIC + LU n={args.lu_n}")
         try:
-            lu = time_lu(torch, L, LS, W, args.lu_n, 8, 3)
+            lu, lu_mats = time_lu(torch, L, W, args.lu_n, 8, 3)
             log(f"lu: {lu}")
         except Exception as e:
             log(f"LU timing failed: {e}")
@@ -272,30 +394,36 @@ def main():
         log("timing CPU oracle sample")
         cpu = cpu_baseline(W, args.cpu_sample, args.degree)
         log(f"cpu: {cpu}")
+        if lu_mats is not None:
+            log("timing host LAPACK on the AIC system")
+            lu["cpu_baseline"] = cpu_lu_baseline(*lu_mats)
+            lu["cpu_baseline"]["unit"] = "ms"
+            log(f"cpu lu: {lu['cpu_baseline']}")
     out = {
-        "metric": "wake particle updates/s (FMM velocity + stretching)",
+        "metric": METRIC,
         "value": value,
         "unit": "particle-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": head["ms"],
+        "ms_per_step": tuned["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64" if form == "tuned" else "fp64",
+        "dtype": "fp32+fp64",
         "data": "synthetic",
-        "config": {"workload": f"3-helix vortex wake, {n} particles, Rosenhead core 0.02, "
-                               f"octree {tree['nlevels']} levels, min_part {tree['min_part']}, "
-                               f"expansion order {args.degree}, vel+stretch",
-                   "form": form, "parallelism": "single GPU"},
-        "fp64_path": {"value": n / (plain["ms"] / 1e3), "ms_per_step": plain["ms"],
-                      "stages_ms": plain["stages"], "n_leaves": plain["n_leaves"]},
-        "tuned_path": None if tuned is None else {"ms_per_step": tuned["ms"],
-                                                  "stages_ms": tuned["stages"],
-                                                  "speedup_vs_fp64": plain["ms"] / tuned["ms"]},
+        "config": {"workload": f"3-helix vortex wake, {n} particles (10% uniform background), "
+                               f"Rosenhead core 0.02, octree {tree['nlevels']} levels, min_part "
+                               f"{tree['min_part']}, expansion order {args.degree}, velocity + "
+                               f"stretching; AIC LU N={args.lu_n}, 8 RHS",
+                   "form": "tuned", "parallelism": "single GPU"},
+        "tuned_path": {"ms_per_step": tuned["ms"], "stages_ms": tuned["stages"],
+                       "n_leaves": tuned["n_leaves"], "m2l_TFLOPs": m2l_ach},
+        "fp64_path": None if plain is None else {
+            "value": n / (plain["ms"] / 1e3), "ms_per_step": plain["ms"],
+            "stages_ms": plain["stages"], "speedup_tuned_vs_fp64": plain["ms"] / tuned["ms"]},
         "accuracy": acc,
-        "work": {"p2p_pairs": head["p2p_pairs"], "m2l_pairs": head["m2l_pairs"]},
+        "work": {"p2p_pairs": tuned["p2p_pairs"], "m2l_pairs": tuned["m2l_pairs"]},
         "roofline": roof,
         "lu": lu,
         "cpu_baseline": cpu,

@@ -24,7 +24,7 @@ LU_FP64 = 0
 LU_FP32 = 1
 LU_FP16S = 2
 
-STAGES = ("sort", "p2m", "m2m", "m2l", "l2l", "p2p")
+STAGES = ("sort", "p2m", "m2m", "m2l", "l2l", "p2p", "l2p")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

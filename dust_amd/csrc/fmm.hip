@@ -885,15 +885,19 @@ __global__ void __launch_bounds__(kEvalThreads)
 // ---------------------------------------------------------------------------
 // Stage timing
 // ---------------------------------------------------------------------------
+// Events come from a pool that is kept between passes (no event creation in
+// a timed loop); marks index pairs of pool events.
 void timer_begin(FmmHandle *h, int stage) {
   if (!h->timer.on) return;
-  hipEvent_t a, b;
-  (void)hipEventCreate(&a);
-  (void)hipEventCreate(&b);
-  h->timer.pool.push_back(a);
-  h->timer.pool.push_back(b);
-  h->timer.marks.push_back({stage, (int)h->timer.pool.size() - 2});
-  (void)hipEventRecord(a, h->stream);
+  const size_t need = 2 * (h->timer.marks.size() + 1);
+  while (h->timer.pool.size() < need) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    h->timer.pool.push_back(e);
+  }
+  const int i = (int)need - 2;
+  h->timer.marks.push_back({stage, i});
+  (void)hipEventRecord(h->timer.pool[i], h->stream);
 }
 void timer_end(FmmHandle *h, int stage) {
   if (!h->timer.on || h->timer.marks.empty()) return;
@@ -911,8 +915,6 @@ static void timer_collect(FmmHandle *h) {
       h->timer.launches[mk.first] += 1;
     }
   }
-  for (auto e : h->timer.pool) (void)hipEventDestroy(e);
-  h->timer.pool.clear();
   h->timer.marks.clear();
 }
 
@@ -1292,7 +1294,7 @@ int eval_fp64_launch(FmmHandle *h, const EvalArgs &A) {
   if (A.n_items > 0)
     k_p2p_fp64<<<ceil_div(A.n_items, kEvalWaves), kWave * kEvalWaves, 0, h->stream>>>(A);
   DUST_HIP(hipGetLastError());
-  return eval_finish_launch(h, A);
+  return 0;
 }
 
 int eval_finish_launch(FmmHandle *h, const EvalArgs &A) {
@@ -1308,15 +1310,16 @@ static int fmm_apply(FmmHandle *h) {
   DUST_CHECK(h && h->computed, DUST_E_ARG, "calculate_multipole must precede apply_multipole");
   const int n = h->n;
   if (n == 0) return 0;
+  EvalArgs A;
+  make_eval_args(h, &A);
   timer_begin(h, DUST_FMM_STAGE_P2P);
-  if (h->prm.precision == DUST_FMM_TUNED) {
-    DUST_TRY(tuned_eval(h));
-  } else {
-    EvalArgs A;
-    make_eval_args(h, &A);
-    DUST_TRY(eval_fp64_launch(h, A));
-  }
+  if (h->prm.precision == DUST_FMM_TUNED) DUST_TRY(tuned_eval(h, A));
+  else DUST_TRY(eval_fp64_launch(h, A));
   timer_end(h, DUST_FMM_STAGE_P2P);
+  // L2P, the sum of the items' partial sums and the free stream
+  timer_begin(h, DUST_FMM_STAGE_L2P);
+  DUST_TRY(eval_finish_launch(h, A));
+  timer_end(h, DUST_FMM_STAGE_L2P);
   k_scatter<<<ceil_div(n, 256), 256, 0, h->stream>>>(n, h->perm.p, h->vel_s.p, h->str_s.p,
                                                      h->tv_s.p, h->vel.p, h->str.p, h->tv.p);
   return 0;

@@ -113,7 +113,7 @@ int tuned_prepare(FmmHandle *h);
 int tuned_pass_prepare(FmmHandle *h);
 int tuned_set(FmmHandle *h, const char *name, double value);
 int tuned_m2l_all(FmmHandle *h);
-int tuned_eval(FmmHandle *h);
+int tuned_eval(FmmHandle *h, EvalArgs &A);
 void tuned_release(FmmHandle *h);
 
 // timing helpers

@@ -519,16 +519,14 @@ int tuned_m2l_all(FmmHandle *h) {
   return 0;
 }
 
-int tuned_eval(FmmHandle *h) {
+int tuned_eval(FmmHandle *h, EvalArgs &A) {
   TunedState *t = TS(h);
-  EvalArgs A;
-  make_eval_args(h, &A);
   if (h->prm.use_vd || h->prm.use_tv) return eval_fp64_launch(h, A);  // PSE runs in FP64
   A.near2 = t->near_factor * t->near_factor;
   if (A.n_items > 0)
     k_p2p_f32<<<ceil_div(A.n_items, kEvalWaves), kWave * kEvalWaves, 0, h->stream>>>(A);
   DUST_HIP(hipGetLastError());
-  return eval_finish_launch(h, A);
+  return 0;
 }
 
 }  // namespace dust

@@ -123,8 +123,8 @@ class Octree:
         L.check(L.lib().dust_fmm_set_timing(self.handle, int(bool(on))), "set_timing")
 
     def stage_times(self):
-        ms = np.zeros(6)
-        cnt = np.zeros(6, np.int32)
+        ms = np.zeros(len(L.STAGES))
+        cnt = np.zeros(len(L.STAGES), np.int32)
         L.check(L.lib().dust_fmm_stage_times(self.handle, L.ptr(ms), L.ptr(cnt)), "stage_times")
         return {s: (ms[i], int(cnt[i])) for i, s in enumerate(L.STAGES)}
 

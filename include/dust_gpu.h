@@ -132,8 +132,9 @@ int dust_fmm_work(dust_fmm_t h, double *p2p_pairs, double *m2l_pairs);
 #define DUST_FMM_STAGE_M2M 2
 #define DUST_FMM_STAGE_M2L 3
 #define DUST_FMM_STAGE_L2L 4
-#define DUST_FMM_STAGE_P2P 5
-#define DUST_FMM_NSTAGES 6
+#define DUST_FMM_STAGE_P2P 5   /* P2P items (partial sums per target) */
+#define DUST_FMM_STAGE_L2P 6   /* L2P + sum of the partials + free stream */
+#define DUST_FMM_NSTAGES 7
 int dust_fmm_set_timing(dust_fmm_t h, int enable);
 int dust_fmm_stage_times(dust_fmm_t h, double *ms, int *launches);
 
