@@ -16,6 +16,9 @@ _lib = None
 _vp = C.c_void_p
 G_SIZE = 74  # panel record length of aic_oracle.c
 G_CEN, G_NOR, G_AREA = 1, 4, 13
+# offsets of the panel record fields (aic_oracle.c enum)
+G = dict(nver=0, cen=1, nor=4, tang=7, area=13, ver=14, evec=26, elen=38, euni=42, cost=54,
+         sint=58, verp=62)
 
 
 def lib():
