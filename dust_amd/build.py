@@ -14,6 +14,12 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdust_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/amdflang")
+ROOT = os.path.dirname(HERE)
+# Fortran binding (ISO_C_BINDING module) and the Fortran test driver
+FORTRAN_MOD = os.path.join(HERE, "fortran", "mod_dust_gpu.f90")
+FORTRAN_DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "dust_gpu_driver.f90")
+FORTRAN_DRIVER = os.path.join(ROOT, "tests", "fortran", "dust_gpu_driver")
 ARCH = "gfx950"
 
 SOURCES = [
@@ -66,5 +72,30 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
+def build_fortran():
+    """Compile the Fortran binding module and link the Fortran test driver
+    against libdust_gpu.so (rpath relative to the driver).  Returns the driver
+    path, or None when no Fortran compiler is installed."""
+    if not os.path.exists(FLANG):
+        return None
+    fdir = os.path.join(BUILD, "fortran")
+    os.makedirs(fdir, exist_ok=True)
+    mod_o = os.path.join(fdir, "mod_dust_gpu.o")
+    drv_o = os.path.join(fdir, "dust_gpu_driver.o")
+    steps = [
+        [FLANG, "-O2", "-J", fdir, "-c", FORTRAN_MOD, "-o", mod_o],
+        [FLANG, "-O2", "-J", fdir, "-c", FORTRAN_DRIVER_SRC, "-o", drv_o],
+        [FLANG, drv_o, mod_o, "-L" + HERE, "-ldust_gpu",
+         "-Wl,-rpath,$ORIGIN/../../dust_amd", "-o", FORTRAN_DRIVER + ".tmp"],
+    ]
+    for cmd in steps:
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            raise RuntimeError("flang failed: " + " ".join(cmd) + "\n" + p.stdout + p.stderr)
+    os.replace(FORTRAN_DRIVER + ".tmp", FORTRAN_DRIVER)
+    return FORTRAN_DRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_fortran())

@@ -196,12 +196,19 @@ int dust_dgetrf_dev(int m, int n, double *dA, int lda, int *dipiv, void *stream)
 int dust_dgetrs_dev(int n, int nrhs, const double *dA, int lda, const int *dipiv, double *dB,
                     int ldb, void *stream);
 
-/* Tuned mixed-precision solve of A X = B (tuned form of the same call
- * sites): LU in `prec` (DUST_LU_FP32, DUST_LU_FP16S = FP16 with scaling)
- * on the matrix cores, FP64 residuals and iterative refinement until
- * ||b - A x||_inf / (||A||_inf ||x||_inf) <= tol or max_iter sweeps.
- * A is not modified.  Host pointers.  *iters and *resid report the
- * refinement sweeps and the final backward error (may be NULL). */
+/* Tuned mixed-precision solve of A X = B (tuned form of the dgetrf 236/435 +
+ * dgetrs 473 call sites): the LU runs on the matrix cores in `prec`
+ * (DUST_LU_FP32, or DUST_LU_FP16S = row-scaled FP16 operands with FP32
+ * accumulation); every 128-wide diagonal block is factored in FP64 with row
+ * interchanges among its rows, and the multipliers below it must satisfy
+ * |l| <= 2 (threshold pivoting) -- otherwise, or on a zero pivot, or when
+ * refinement stalls, the system is solved by the FP64 partial-pivoting path
+ * and *iters is set to -1.  FP64 residuals drive iterative refinement until
+ * max_k ||b_k - A x_k||_inf / (||A||_inf ||x_k||_inf) <= tol or max_iter
+ * sweeps.  DUST_LU_FP64 solves with the FP64 path (*iters = 0).  A is not
+ * modified; B is overwritten by X.  Returns 0 or the FP64 path's info > 0 for
+ * a singular matrix.  *iters and *resid (may be NULL) report the sweeps and
+ * the final backward error.  Process-wide workspace: not re-entrant. */
 #define DUST_LU_FP64 0
 #define DUST_LU_FP32 1
 #define DUST_LU_FP16S 2
