@@ -150,17 +150,22 @@ constexpr int kRTB = 4 * kRTW;
 
 struct WGeom {
   int rpm, nrows, K, rt, ks;  // rows per monomial, rows, K, row tiles, k steps
+  int rtp;                    // row tiles stored: rt rounded up to whole row blocks
   long per_off;               // floats per offset
 };
 
+// The operator of one offset is stored as rtp x ks fragments; ks is even and
+// rows past nrows / columns past K are zero, so every fragment load of a
+// row block is in bounds and the k loop runs in pairs.
 static WGeom wgeom(const FmmHandle *h) {
   WGeom w;
   w.rpm = h->prm.use_vs ? 12 : 3;
   w.nrows = w.rpm * h->nmon;
   w.K = 3 * h->nmon;
   w.rt = (w.nrows + 15) / 16;
-  w.ks = (w.K + 3) / 4;
-  w.per_off = (long)w.rt * w.ks * 64;
+  w.rtp = ceil_div(w.rt, kRTB) * kRTB;
+  w.ks = 2 * ceil_div(w.K, 8);
+  w.per_off = (long)w.rtp * w.ks * 64;
   return w;
 }
 
@@ -236,13 +241,32 @@ struct M2LArgs {
   double *mp_b, *mp_c;
 };
 
+// One k step of a job: B fragments of the 64 targets from LDS, then the MFMAs
+// of the wave's row tiles for the column tiles that have sources.
+__device__ __forceinline__ void m2l_step(f4 (&acc)[kRTW][4], const float (&af)[kRTW],
+                                         const float *sB, int ld, int ks, int lane, int bmask) {
+  float bf[4];
+  const int kk = ks * 4 + (lane >> 4);
+#pragma unroll
+  for (int b = 0; b < 4; ++b) bf[b] = sB[kk * ld + b * 16 + (lane & 15)];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    if (!(bmask & (1 << b))) continue;
+#pragma unroll
+    for (int a = 0; a < kRTW; ++a)  // padded row tiles hold zeros; their rows are not stored
+      acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bf[b], acc[a][b], 0, 0, 0);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_m2l_mfma(M2LArgs P) {
   extern __shared__ float sB[];  // [ks * 4][kNT + 4]
   __shared__ int sSrc[kNT];
+  __shared__ int sMask;
   const M2LJob J = P.jobs[blockIdx.x];
+  if (J.t0 >= J.nt) return;  // padding job
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int rt0 = J.rb * kRTB + wv * kRTW;                 // first row tile of this wave
-  const int nrt = max(0, min(kRTW, P.w.rt - rt0));         // row tiles of this wave
+  const int rt0 = J.rb * kRTB + __builtin_amdgcn_readfirstlane(wv) * kRTW;  // wave's first row tile
+  const int nrt = max(0, min(kRTW, P.w.rt - rt0));         // row tiles with rows < nrows
   const int K = P.w.K, KS = P.w.ks, ld = kNT + 4;
   const int *grp = P.grp + J.grp_off;
   const int *src = P.src + J.grp_off * kOct;
@@ -259,11 +283,24 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(M2LArgs P) {
                      (long)(2 * J.kdl0 + 1) * ((dj + J.kdl1) + (long)(2 * J.kdl1 + 1) * (dk + J.kdl2));
     const float *Wo = P.W + tab * P.w.per_off;
     __syncthreads();
+    int has = 0;
     if (tid < kNT) {
       int t = J.t0 + tid;
       sSrc[tid] = (t < J.nt) ? src[(long)t * kOct + j] : -1;
+      has = sSrc[tid] >= 0;
+    }
+    // column tiles (16 targets) with at least one source at this offset
+    if (tid < 64) {
+      const unsigned long long bal = __ballot(has);
+      if (tid == 0) {
+        int m = 0;
+        for (int b = 0; b < 4; ++b) m |= ((bal >> (16 * b)) & 0xffffull) ? (1 << b) : 0;
+        sMask = m;
+      }
     }
     __syncthreads();
+    const int bmask = sMask;
+    if (bmask == 0) continue;  // no target has a source at this offset
     // gather the source multipoles of the 64 targets: sB[k][t]
     for (int q = tid; q < kNT * KS * 4; q += 256) {
       const int tt = q / (KS * 4), k = q % (KS * 4);
@@ -272,20 +309,21 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(M2LArgs P) {
     }
     __syncthreads();
     if (nrt > 0) {
+      // fragments of row tile a, k step ks at Wl[(a KS + ks) 64]; two register
+      // sets alternate so the loads of one k step overlap the MFMAs of the other
       const float *Wl = Wo + ((long)rt0 * KS) * 64 + lane;
-      for (int ks = 0; ks < KS; ++ks) {
-        float bf[4];
-        const int kk = ks * 4 + (lane >> 4);
+      float f0[kRTW], f1[kRTW];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) bf[b] = sB[kk * ld + b * 16 + (lane & 15)];
-        float af[kRTW];
+      for (int a = 0; a < kRTW; ++a) f0[a] = Wl[((long)a * KS) * 64];
+      for (int ks = 0; ks < KS; ks += 2) {
 #pragma unroll
-        for (int a = 0; a < kRTW; ++a) af[a] = (a < nrt) ? Wl[((long)a * KS + ks) * 64] : 0.f;
+        for (int a = 0; a < kRTW; ++a) f1[a] = Wl[((long)a * KS + ks + 1) * 64];
+        m2l_step(acc, f0, sB, ld, ks, lane, bmask);
+        if (ks + 2 < KS) {
 #pragma unroll
-        for (int a = 0; a < kRTW; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b)
-            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bf[b], acc[a][b], 0, 0, 0);
+          for (int a = 0; a < kRTW; ++a) f0[a] = Wl[((long)a * KS + ks + 2) * 64];
+        }
+        m2l_step(acc, f1, sB, ld, ks + 1, lane, bmask);
       }
     }
   }
@@ -342,9 +380,10 @@ __device__ inline void pair64(double x, double y, double z, double al0, double a
 // and r^2; every tile's FP32 sums are folded into FP64 accumulators.  Pairs
 // with d^2 < near2 r_s^2 are evaluated in FP64 instead.
 __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
-  __shared__ float ssm[kEvalWaves][7 * kWave];
+  // per wavefront: sources as (x, y, z, r^2) and (alpha, 0) records
+  __shared__ f4 ssm[kEvalWaves][2 * kWave];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  float *ss = ssm[wv];
+  f4 *ss = ssm[wv];
   const int it = blockIdx.x * kEvalWaves + wv;
   if (it >= A.n_items) return;
   const EvalItem I = A.items[it];
@@ -379,23 +418,28 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
       if (lane < ns) {
         const int q = s0 + lane;
         const double smg = A.pm[q];
-        ss[0 * kWave + lane] = (float)(A.px[q] - cx);
-        ss[1 * kWave + lane] = (float)(A.py[q] - cy);
-        ss[2 * kWave + lane] = (float)(A.pz[q] - cz);
-        ss[3 * kWave + lane] = (float)(A.dx[q] * smg);
-        ss[4 * kWave + lane] = (float)(A.dy[q] * smg);
-        ss[5 * kWave + lane] = (float)(A.dz[q] * smg);
-        ss[6 * kWave + lane] = (float)(A.pr[q] * A.pr[q]);
+        ss[2 * lane] = (f4){(float)(A.px[q] - cx), (float)(A.py[q] - cy), (float)(A.pz[q] - cz),
+                            (float)(A.pr[q] * A.pr[q])};
+        ss[2 * lane + 1] = (f4){(float)(A.dx[q] * smg), (float)(A.dy[q] * smg),
+                                (float)(A.dz[q] * smg), 0.f};
       }
       wave_sync();
+      // Per pair: u += cf (r x a_s); stretching -[cf (al x a_s) + df r (r.(al x a_s))]
+      // is accumulated as S = sum cf a_s (crossed with al once per tile) and
+      // sum df (r.(al x a_s)) r with r.(al x a_s) = -al.(r x a_s).  Pairs at
+      // zero distance (the self pair) are left out (the reference kernels
+      // give exactly zero for the self pair); lanes without a target are
+      // never written.
       float v0 = 0.f, v1 = 0.f, v2 = 0.f, w0 = 0.f, w1 = 0.f, w2 = 0.f;
+      float S0 = 0.f, S1 = 0.f, S2 = 0.f;
       for (int s = sub; s < ns; s += lpt) {
-        const float r0 = fx - ss[s], r1 = fy - ss[kWave + s], r2 = fz - ss[2 * kWave + s];
-        const float a0 = ss[3 * kWave + s], a1 = ss[4 * kWave + s], a2 = ss[5 * kWave + s];
-        const float rs2 = ss[6 * kWave + s];
+        const f4 sp = ss[2 * s], sa = ss[2 * s + 1];
+        const float r0 = fx - sp[0], r1 = fy - sp[1], r2 = fz - sp[2];
+        const float a0 = sa[0], a1 = sa[1], a2 = sa[2];
+        const float rs2 = sp[3];
         const float d2 = r0 * r0 + r1 * r1 + r2 * r2;
-        const bool self = (s0 + s == t);
-        const bool near = valid && !self && d2 < near2 * rs2;
+        const bool skip = d2 <= near2 * rs2;  // FP64 pairs and d = 0 (self) leave the FP32 sums
+        const bool near = skip && d2 > 0.f;
         if (near) {
           // FP64 for the pairs inside the core radius of this source
           const int q = s0 + s;
@@ -405,19 +449,26 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
         const float R2 = d2 + rs2;
         const float inv = __builtin_amdgcn_rsqf(R2);
         const float inv2 = inv * inv;
-        float cf = -(inv2 * inv);
-        cf = (valid && !self && !near) ? cf : 0.f;
-        v0 += cf * (r1 * a2 - r2 * a1);
-        v1 += cf * (r2 * a0 - r0 * a2);
-        v2 += cf * (r0 * a1 - r1 * a0);
+        const float cf = skip ? 0.f : -(inv2 * inv);
+        const float c0 = r1 * a2 - r2 * a1, c1 = r2 * a0 - r0 * a2, c2 = r0 * a1 - r1 * a0;
+        v0 += cf * c0;
+        v1 += cf * c1;
+        v2 += cf * c2;
         if (A.use_vs) {
-          const float df = -3.f * cf * inv2;
-          const float vp0 = fa1 * a2 - fa2 * a1, vp1 = fa2 * a0 - fa0 * a2, vp2 = fa0 * a1 - fa1 * a0;
-          const float dv = df * (r0 * vp0 + r1 * vp1 + r2 * vp2);
-          w0 -= cf * vp0 + dv * r0;
-          w1 -= cf * vp1 + dv * r1;
-          w2 -= cf * vp2 + dv * r2;
+          S0 += cf * a0;
+          S1 += cf * a1;
+          S2 += cf * a2;
+          // df * (r.(al x a)) with df = -3 cf / R^2 and r.(al x a) = -al.(r x a)
+          const float t = 3.f * cf * inv2 * (fa0 * c0 + fa1 * c1 + fa2 * c2);
+          w0 -= t * r0;
+          w1 -= t * r1;
+          w2 -= t * r2;
         }
+      }
+      if (A.use_vs) {
+        w0 -= fa1 * S2 - fa2 * S1;
+        w1 -= fa2 * S0 - fa0 * S2;
+        w2 -= fa0 * S1 - fa1 * S0;
       }
       pv[0] += v0; pv[1] += v1; pv[2] += v2;
       ps[0] += w0; ps[1] += w1; ps[2] += w2;
@@ -480,24 +531,48 @@ int tuned_m2l_all(FmmHandle *h) {
   if (lds > 64 * 1024)
     DUST_HIP(hipFuncSetAttribute((const void *)k_m2l_mfma,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  std::vector<M2LJob> jobs;
+  // Jobs that read the same operator chunk (level, octant, offset chunk) are
+  // given to one XCD: workgroups are dealt to the 8 XCDs round-robin by
+  // index, so the job of workgroup i comes from queue i % 8, and every chunk
+  // is fetched into one L2 instead of eight.  Chunks go to the least loaded
+  // queue, largest first; short queues are padded with empty jobs.
+  constexpr int kXcd = 8;
   const int nrb = ceil_div(w.rt, kRTB);
+  struct Chunk { int lvl, g, j0, njobs; };
+  std::vector<Chunk> chunks;
   for (int lvl = 1; lvl < h->nlev; ++lvl)
     for (int g = 0; g < 8; ++g) {
       int nt = t->hcount[lvl * 8 + g];
-      for (int t0 = 0; t0 < nt; t0 += kNT)
-        for (int j0 = 0; j0 < kOct; j0 += kJC)
-          for (int rb = 0; rb < nrb; ++rb) {
-            M2LJob J;
-            J.lvl = lvl; J.oct = g; J.t0 = t0; J.j0 = j0; J.rb = rb; J.nt = nt;
-            J.grp_off = t->gbase[lvl * 8 + g];
-            J.kd_base = h->lev[lvl].kd_base;
-            J.kdl0 = h->lev[lvl].kdl[0]; J.kdl1 = h->lev[lvl].kdl[1]; J.kdl2 = h->lev[lvl].kdl[2];
-            J.pad = 0;
-            jobs.push_back(J);
-          }
+      if (nt == 0) continue;
+      for (int j0 = 0; j0 < kOct; j0 += kJC) chunks.push_back({lvl, g, j0, ceil_div(nt, kNT) * nrb});
     }
-  if (jobs.empty()) return 0;
+  std::stable_sort(chunks.begin(), chunks.end(),
+                   [](const Chunk &a, const Chunk &b) { return a.njobs > b.njobs; });
+  std::vector<std::vector<M2LJob>> queue(kXcd);
+  std::vector<long> load(kXcd, 0);
+  for (const Chunk &c : chunks) {
+    int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += c.njobs;
+    const int nt = t->hcount[c.lvl * 8 + c.g];
+    for (int t0 = 0; t0 < nt; t0 += kNT)
+      for (int rb = 0; rb < nrb; ++rb) {
+        M2LJob J;
+        J.lvl = c.lvl; J.oct = c.g; J.t0 = t0; J.j0 = c.j0; J.rb = rb; J.nt = nt;
+        J.grp_off = t->gbase[c.lvl * 8 + c.g];
+        J.kd_base = h->lev[c.lvl].kd_base;
+        J.kdl0 = h->lev[c.lvl].kdl[0]; J.kdl1 = h->lev[c.lvl].kdl[1]; J.kdl2 = h->lev[c.lvl].kdl[2];
+        J.pad = 0;
+        queue[x].push_back(J);
+      }
+  }
+  size_t qmax = 0;
+  for (auto &q : queue) qmax = std::max(qmax, q.size());
+  if (qmax == 0) return 0;
+  M2LJob empty{};
+  empty.nt = 0;  // t0 >= nt: the job does nothing
+  std::vector<M2LJob> jobs(qmax * kXcd, empty);
+  for (int x = 0; x < kXcd; ++x)
+    for (size_t k = 0; k < queue[x].size(); ++k) jobs[k * kXcd + x] = queue[x][k];
   DUST_TRY(t->jobs.reserve(jobs.size() * sizeof(M2LJob)));
   DUST_HIP(hipMemcpyAsync(t->jobs.p, jobs.data(), jobs.size() * sizeof(M2LJob),
                           hipMemcpyHostToDevice, h->stream));
