@@ -354,36 +354,47 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(M2LArgs P) {
 // ---------------------------------------------------------------------------
 // FP32 P2P with FP64 near pairs
 // ---------------------------------------------------------------------------
-__device__ inline void pair64(double x, double y, double z, double al0, double al1, double al2,
-                              double sx, double sy, double sz, double sdx, double sdy, double sdz,
-                              double smg, double srv, int use_vs, double v[3], double s[3]) {
-  double r0 = x - sx, r1 = y - sy, r2 = z - sz;
-  double R2 = r0 * r0 + r1 * r1 + r2 * r2 + srv * srv;
-  double R3 = R2 * sqrt(R2);
-  double cf = -1.0 / R3;
-  v[0] += cf * (r1 * sdz - r2 * sdy) * smg;
-  v[1] += cf * (r2 * sdx - r0 * sdz) * smg;
-  v[2] += cf * (r0 * sdy - r1 * sdx) * smg;
+// Near pair in FP64 (rsq and reciprocal refined by Newton steps instead of
+// IEEE division): r = target - source, alpha_s = dir_s mag_s, core radius rv.
+__device__ inline void near64(double r0, double r1, double r2, double al0, double al1, double al2,
+                              double as0, double as1, double as2, double rv, int use_vs,
+                              double v[3], double s[3]) {
+  const double R2 = r0 * r0 + r1 * r1 + r2 * r2 + rv * rv;
+  double y = __builtin_amdgcn_rsq(R2);
+  y = y * (1.5 - 0.5 * R2 * y * y);
+  y = y * (1.5 - 0.5 * R2 * y * y);
+  const double y2 = y * y;
+  const double cf = -(y2 * y);          // -1/R^3
+  v[0] += cf * (r1 * as2 - r2 * as1);
+  v[1] += cf * (r2 * as0 - r0 * as2);
+  v[2] += cf * (r0 * as1 - r1 * as0);
   if (use_vs) {
-    double df = 3.0 / (R3 * R2);
-    double am0 = sdx * smg, am1 = sdy * smg, am2 = sdz * smg;
-    double vp0 = al1 * am2 - al2 * am1, vp1 = al2 * am0 - al0 * am2, vp2 = al0 * am1 - al1 * am0;
-    double dv = r0 * vp0 + r1 * vp1 + r2 * vp2;
+    const double df = -3.0 * cf * y2;   // 3/R^5
+    const double vp0 = al1 * as2 - al2 * as1, vp1 = al2 * as0 - al0 * as2,
+                 vp2 = al0 * as1 - al1 * as0;
+    const double dv = r0 * vp0 + r1 * vp1 + r2 * vp2;
     s[0] -= cf * vp0 + df * r0 * dv;
     s[1] -= cf * vp1 + df * r1 * dv;
     s[2] -= cf * vp2 + df * r2 * dv;
   }
 }
 
-// FP32 P2P of one item (tuned form).  Sources are staged as FP32 offsets
-// from the target leaf centre (computed in FP64), strengths alpha = dir*mag
-// and r^2; every tile's FP32 sums are folded into FP64 accumulators.  Pairs
-// with d^2 < near2 r_s^2 are evaluated in FP64 instead.
+// FP32 P2P of one item (tuned form).  Sources are staged per wavefront as
+// FP32 records (offset from the target leaf centre, r^2; alpha = dir*mag)
+// and FP64 records (offset, alpha, r).  Pairs with 0 < d^2 <= near2 r_s^2
+// (inside the source core) are left out of the FP32 sums and evaluated in
+// FP64 after the tile, all lanes together (a per-lane mask of tile
+// positions), so the rare FP64 work does not diverge the FP32 loop.  The
+// FP32 sums of every tile are folded into FP64 accumulators.  A partial tile
+// is padded with zero-strength sources far away, so the loop trip count is
+// uniform.
+template <int VS>
 __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
-  // per wavefront: sources as (x, y, z, r^2) and (alpha, 0) records
   __shared__ f4 ssm[kEvalWaves][2 * kWave];
+  __shared__ double sdm[kEvalWaves][8 * kWave];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   f4 *ss = ssm[wv];
+  double *sd = sdm[wv];
   const int it = blockIdx.x * kEvalWaves + wv;
   if (it >= A.n_items) return;
   const EvalItem I = A.items[it];
@@ -398,10 +409,10 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
   const int lpt = U.lpt, tl = lane / lpt, sub = lane - tl * lpt;
   const bool valid = tl < U.nt;
   const int t = U.t0 + (valid ? tl : 0);
-  const double x = A.px[t], y = A.py[t], z = A.pz[t];
+  const double xr = A.px[t] - cx, yr = A.py[t] - cy, zr = A.pz[t] - cz;
   const double mg = A.pm[t];
   const double al0 = mg * A.dx[t], al1 = mg * A.dy[t], al2 = mg * A.dz[t];
-  const float fx = (float)(x - cx), fy = (float)(y - cy), fz = (float)(z - cz);
+  const float fx = (float)xr, fy = (float)yr, fz = (float)zr;
   const float fa0 = (float)al0, fa1 = (float)al1, fa2 = (float)al2;
   double pv[3] = {0, 0, 0}, ps[3] = {0, 0, 0};
   const float near2 = (float)A.near2;
@@ -417,36 +428,35 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
       const int ns = min(kWave, e - s0);
       if (lane < ns) {
         const int q = s0 + lane;
-        const double smg = A.pm[q];
-        ss[2 * lane] = (f4){(float)(A.px[q] - cx), (float)(A.py[q] - cy), (float)(A.pz[q] - cz),
-                            (float)(A.pr[q] * A.pr[q])};
-        ss[2 * lane + 1] = (f4){(float)(A.dx[q] * smg), (float)(A.dy[q] * smg),
-                                (float)(A.dz[q] * smg), 0.f};
+        const double smg = A.pm[q], srv = A.pr[q];
+        const double ox = A.px[q] - cx, oy = A.py[q] - cy, oz = A.pz[q] - cz;
+        const double a0 = A.dx[q] * smg, a1 = A.dy[q] * smg, a2 = A.dz[q] * smg;
+        ss[2 * lane] = (f4){(float)ox, (float)oy, (float)oz, (float)(srv * srv)};
+        ss[2 * lane + 1] = (f4){(float)a0, (float)a1, (float)a2, 0.f};
+        double *d = sd + 8 * lane;
+        d[0] = ox; d[1] = oy; d[2] = oz; d[3] = srv;
+        d[4] = a0; d[5] = a1; d[6] = a2;
+      } else {
+        ss[2 * lane] = (f4){1.0e6f, 1.0e6f, 1.0e6f, 1.f};
+        ss[2 * lane + 1] = (f4){0.f, 0.f, 0.f, 0.f};
       }
       wave_sync();
-      // Per pair: u += cf (r x a_s); stretching -[cf (al x a_s) + df r (r.(al x a_s))]
-      // is accumulated as S = sum cf a_s (crossed with al once per tile) and
-      // sum df (r.(al x a_s)) r with r.(al x a_s) = -al.(r x a_s).  Pairs at
-      // zero distance (the self pair) are left out (the reference kernels
-      // give exactly zero for the self pair); lanes without a target are
-      // never written.
       float v0 = 0.f, v1 = 0.f, v2 = 0.f, w0 = 0.f, w1 = 0.f, w2 = 0.f;
       float S0 = 0.f, S1 = 0.f, S2 = 0.f;
-      for (int s = sub; s < ns; s += lpt) {
+      unsigned long long nmask = 0ull;
+      const int nk = (ns + lpt - 1) / lpt;
+      for (int k = 0; k < nk; ++k) {
+        const int s = sub + k * lpt;
         const f4 sp = ss[2 * s], sa = ss[2 * s + 1];
         const float r0 = fx - sp[0], r1 = fy - sp[1], r2 = fz - sp[2];
         const float a0 = sa[0], a1 = sa[1], a2 = sa[2];
-        const float rs2 = sp[3];
         const float d2 = r0 * r0 + r1 * r1 + r2 * r2;
-        const bool skip = d2 <= near2 * rs2;  // FP64 pairs and d = 0 (self) leave the FP32 sums
-        const bool near = skip && d2 > 0.f;
-        if (near) {
-          // FP64 for the pairs inside the core radius of this source
-          const int q = s0 + s;
-          pair64(x, y, z, al0, al1, al2, A.px[q], A.py[q], A.pz[q], A.dx[q], A.dy[q], A.dz[q],
-                 A.pm[q], A.pr[q], A.use_vs, pv, ps);
+        // pairs inside the source core go to FP64; d = 0 is the self pair
+        const bool skip = d2 <= near2 * sp[3];
+        if (__builtin_expect(__any(skip), 0)) {
+          if (skip && d2 > 0.f) nmask |= 1ull << s;
         }
-        const float R2 = d2 + rs2;
+        const float R2 = d2 + sp[3];
         const float inv = __builtin_amdgcn_rsqf(R2);
         const float inv2 = inv * inv;
         const float cf = skip ? 0.f : -(inv2 * inv);
@@ -454,31 +464,42 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
         v0 += cf * c0;
         v1 += cf * c1;
         v2 += cf * c2;
-        if (A.use_vs) {
+        if (VS) {
+          // stretching -[cf (al x a) + df r (r.(al x a))]: sum cf a is crossed
+          // with al after the tile; r.(al x a) = -al.(r x a), df = -3 cf / R^2
           S0 += cf * a0;
           S1 += cf * a1;
           S2 += cf * a2;
-          // df * (r.(al x a)) with df = -3 cf / R^2 and r.(al x a) = -al.(r x a)
-          const float t = 3.f * cf * inv2 * (fa0 * c0 + fa1 * c1 + fa2 * c2);
-          w0 -= t * r0;
-          w1 -= t * r1;
-          w2 -= t * r2;
+          const float tt = 3.f * cf * inv2 * (fa0 * c0 + fa1 * c1 + fa2 * c2);
+          w0 -= tt * r0;
+          w1 -= tt * r1;
+          w2 -= tt * r2;
         }
       }
-      if (A.use_vs) {
+      if (VS) {
         w0 -= fa1 * S2 - fa2 * S1;
         w1 -= fa2 * S0 - fa0 * S2;
         w2 -= fa0 * S1 - fa1 * S0;
       }
       pv[0] += v0; pv[1] += v1; pv[2] += v2;
       ps[0] += w0; ps[1] += w1; ps[2] += w2;
+      // the FP64 pairs of this tile, one per lane per step
+      while (__any(nmask != 0ull)) {
+        if (nmask) {
+          const int s = __builtin_ctzll(nmask);
+          nmask &= nmask - 1;
+          const double *d = sd + 8 * s;
+          near64(xr - d[0], yr - d[1], zr - d[2], al0, al1, al2, d[4], d[5], d[6], d[3], VS, pv,
+                 ps);
+        }
+      }
       wave_sync();
     }
     left -= e - (r.x + off);
   }
   for (int q = 0; q < 3; ++q) {
     pv[q] = lane_group_sum(pv[q], lpt);
-    if (A.use_vs) ps[q] = lane_group_sum(ps[q], lpt);
+    if (VS) ps[q] = lane_group_sum(ps[q], lpt);
   }
   if (valid && sub == 0) {
     const double fi = 1.0 / (4.0 * kPi);
@@ -598,8 +619,11 @@ int tuned_eval(FmmHandle *h, EvalArgs &A) {
   TunedState *t = TS(h);
   if (h->prm.use_vd || h->prm.use_tv) return eval_fp64_launch(h, A);  // PSE runs in FP64
   A.near2 = t->near_factor * t->near_factor;
-  if (A.n_items > 0)
-    k_p2p_f32<<<ceil_div(A.n_items, kEvalWaves), kWave * kEvalWaves, 0, h->stream>>>(A);
+  if (A.n_items > 0) {
+    const dim3 grid(ceil_div(A.n_items, kEvalWaves)), block(kWave * kEvalWaves);
+    if (A.use_vs) k_p2p_f32<1><<<grid, block, 0, h->stream>>>(A);
+    else k_p2p_f32<0><<<grid, block, 0, h->stream>>>(A);
+  }
   DUST_HIP(hipGetLastError());
   return 0;
 }
