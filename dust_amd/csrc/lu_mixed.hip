@@ -806,6 +806,9 @@ struct MixedWs {
   DevBuf<unsigned long long> norms;
   DevBuf<int> ipiv64;
   unsigned gen = 0;
+  // second stream and events of the lookahead (default priority)
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_start = nullptr, ev_panel = nullptr, ev_rest = nullptr;
 };
 MixedWs g_ws;
 
@@ -850,15 +853,26 @@ int mx_factor(MixedWs &w, int np, int f16, double tau, hipStream_t st, int *bad)
   const long ld = np;
   const size_t esz = f16 ? sizeof(_Float16) : sizeof(float);
   const long pbytes = (long)np * NB * (long)esz;  // bytes per fragment panel
+  // Lookahead: the trailing update of super-panel i is split into the next
+  // super-panel's block column and block row (on `st`, needed at once) and the
+  // remainder (on the second stream `s2`), which runs while `st` factors the
+  // next super-panel.  The two fragment sets alternate between super-panels.
+  hipStream_t s2 = w.s2;
   k_mx_fill_u32<<<1, 64, 0, st>>>(w.flags.p, 3, 0u);
-  for (int ks = 0; ks < np; ks += DEPTH * NB) {
+  DUST_HIP(hipEventRecord(w.ev_start, st));
+  DUST_HIP(hipStreamWaitEvent(s2, w.ev_start, 0));
+  bool pending = false;  // a remainder update is queued on s2
+  int sp = 0;
+  for (int ks = 0; ks < np; ks += DEPTH * NB, ++sp) {
+    char *Lf = w.Lf.p + (size_t)(sp & 1) * DEPTH * pbytes;
+    char *Uf = w.Uf.p + (size_t)(sp & 1) * DEPTH * pbytes;
     const int nsub = std::min(DEPTH, (np - ks) / NB);
     for (int a = 0; a < nsub; ++a) {
       const int k = ks + a * NB;
       if (a > 0) {
         // block column a and block row a, updated by the sub-panels before it
-        DUST_TRY(mx_gemm(f16, w.W.p, ld, k, np, k, k + NB, w.Lf.p, w.Uf.p, pbytes, a, st));
-        DUST_TRY(mx_gemm(f16, w.W.p, ld, k, k + NB, k + NB, np, w.Lf.p, w.Uf.p, pbytes, a, st));
+        DUST_TRY(mx_gemm(f16, w.W.p, ld, k, np, k, k + NB, Lf, Uf, pbytes, a, st));
+        DUST_TRY(mx_gemm(f16, w.W.p, ld, k, k + NB, k + NB, np, Lf, Uf, pbytes, a, st));
       }
       DiagArgs da{w.W.p, ld, k, w.piv.p, w.perm.p, w.npiv.p, w.dinv.p, w.flags.p};
       k_mx_diag<<<1, 256, lds_diag(), st>>>(da);
@@ -867,17 +881,31 @@ int mx_factor(MixedWs &w, int np, int f16, double tau, hipStream_t st, int *bad)
       MX_LAUNCH_CHECK("laswp");
       const int nt = (np - k - NB) / 64;
       if (nt > 0) {
-        TrsmArgs ta{w.W.p, ld, k, np, nt, f16, 0, w.dinv.p,
-                    w.Lf.p + (size_t)a * pbytes, w.Uf.p + (size_t)a * pbytes,
-                    w.flags.p, nullptr, nullptr};
+        TrsmArgs ta{w.W.p, ld, k, np, nt, f16, 0, w.dinv.p, Lf + (size_t)a * pbytes,
+                    Uf + (size_t)a * pbytes, w.flags.p, nullptr, nullptr};
         k_mx_trsm<<<2 * nt, 256, lds_trsm(), st>>>(ta);
         MX_LAUNCH_CHECK("block triangular solve");
       }
     }
     const int t0 = ks + nsub * NB;
-    if (t0 < np)
-      DUST_TRY(mx_gemm(f16, w.W.p, ld, t0, np, t0, np, w.Lf.p, w.Uf.p, pbytes, nsub, st));
+    if (t0 >= np) break;
+    const int t1 = std::min(np, t0 + DEPTH * NB);
+    DUST_HIP(hipEventRecord(w.ev_panel, st));
+    if (pending) DUST_HIP(hipStreamWaitEvent(st, w.ev_rest, 0));
+    // the next super-panel's block column and block row
+    DUST_TRY(mx_gemm(f16, w.W.p, ld, t0, np, t0, t1, Lf, Uf, pbytes, nsub, st));
+    DUST_TRY(mx_gemm(f16, w.W.p, ld, t0, t1, t1, np, Lf, Uf, pbytes, nsub, st));
+    // the remainder on s2
+    if (t1 < np) {
+      DUST_HIP(hipStreamWaitEvent(s2, w.ev_panel, 0));
+      DUST_TRY(mx_gemm(f16, w.W.p, ld, t1, np, t1, np, Lf, Uf, pbytes, nsub, s2));
+      DUST_HIP(hipEventRecord(w.ev_rest, s2));
+      pending = true;
+    } else {
+      pending = false;
+    }
   }
+  if (pending) DUST_HIP(hipStreamWaitEvent(st, w.ev_rest, 0));
   // 128x128 inverses of every diagonal block for the refinement solves
   TrsmArgs ia{w.W.p, ld, 0, np, 2, f16, 1, w.dinv.p, nullptr, nullptr, w.flags.p, w.Linv.p, w.Uinv.p};
   k_mx_trsm<<<dim3(4, nblk), 256, lds_trsm(), st>>>(ia);
@@ -944,8 +972,14 @@ int mx_reserve(MixedWs &w, int n, int np) {
   DUST_TRY(w.W.reserve((size_t)np * np));
   DUST_TRY(w.rs.reserve(np));
   DUST_TRY(w.rowmax.reserve(np));
-  DUST_TRY(w.Lf.reserve((size_t)DEPTH * np * NB * sizeof(float)));
-  DUST_TRY(w.Uf.reserve((size_t)DEPTH * np * NB * sizeof(float)));
+  DUST_TRY(w.Lf.reserve((size_t)2 * DEPTH * np * NB * sizeof(float)));
+  DUST_TRY(w.Uf.reserve((size_t)2 * DEPTH * np * NB * sizeof(float)));
+  if (!w.s2) {
+    DUST_HIP(hipStreamCreateWithFlags(&w.s2, hipStreamNonBlocking));
+    DUST_HIP(hipEventCreateWithFlags(&w.ev_start, hipEventDisableTiming));
+    DUST_HIP(hipEventCreateWithFlags(&w.ev_panel, hipEventDisableTiming));
+    DUST_HIP(hipEventCreateWithFlags(&w.ev_rest, hipEventDisableTiming));
+  }
   DUST_TRY(w.piv.reserve(np));
   DUST_TRY(w.perm.reserve(np));
   DUST_TRY(w.npiv.reserve(nblk));
