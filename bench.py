@@ -179,6 +179,101 @@ def time_lu(torch, L, W, n_req, nrhs, steps):
     return out, (A, rhs_cm)
 
 
+def time_batch(torch, L, O, W, ncase=48, reps=3):
+    """The many-case workload: ncase rotor cases (2-5 blades, 500-4,000
+    panels) with 50k-400k-particle helix wakes.  Tuned FMM passes and tuned
+    8-RHS solves of every case, one call per case and then through the batch
+    entry points (dust_fmm_pass_batch_dev, dust_dgesv_mixed_batch_dev)."""
+    import ctypes as C
+    lib = L.lib()
+    cases = W.batch_cases(ncase)
+    out = dict(cases=ncase)
+    # FMM
+    handles, keep = [], []
+    npart = 0
+    for cs in cases:
+        cen, alpha, rv, tree = cs["cen"], cs["alpha"], cs["rv"], cs["tree"]
+        mag = np.linalg.norm(alpha, axis=1)
+        dr = alpha / mag[:, None]
+        g = O.initialize_octree(tree["box_length"], tree["nbox"], tree["origin"], tree["nlevels"],
+                                tree["min_part"], 4, tree["delta"], capacity=len(cen), use_vs=True,
+                                precision="tuned")
+        dev = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (cen, dr, mag, rv)]
+        L.check(lib.dust_fmm_set_particles_dev(g.handle, len(cen), *[d.data_ptr() for d in dev]),
+                "set")
+        handles.append(g)
+        keep.append(dev)
+        npart += len(cen)
+    arr = (C.c_void_p * ncase)(*[g.handle.value for g in handles])
+    nl = C.c_int()
+    L.check(lib.dust_fmm_pass_batch_dev(ncase, arr, None), "warm-up")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for g in handles:
+            L.check(lib.dust_fmm_pass_dev(g.handle, C.byref(nl)), "pass")
+            L.check(lib.dust_fmm_synchronize(g.handle), "sync")
+    t_seq = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.check(lib.dust_fmm_pass_batch_dev(ncase, arr, None), "batch")
+    t_bat = (time.perf_counter() - t0) / reps
+    out.update(particles=npart, fmm_one_by_one_ms=t_seq * 1e3, fmm_batched_ms=t_bat * 1e3,
+               fmm_batched_updates_per_s=npart / t_bat, fmm_batch_speedup=t_seq / t_bat)
+    for g in handles:
+        g.destroy()
+    del keep
+    # dense solves
+    prm = L.AicParams()
+    prm.ff_ratio_doublet, prm.rankine_rad, prm.cutoff_rad = 10.0, 0.1, 0.001
+    mats, rhs, sols, ns = [], [], [], []
+    for k, cs in enumerate(cases):
+        ver, nv = cs["ver"], cs["n_ver"]
+        n = len(nv)
+        v4 = np.zeros((n, 4, 3))
+        v4[:, :ver.shape[1]] = ver
+        dver = torch.from_numpy(v4.reshape(n, 12)).cuda()
+        dnv = torch.from_numpy(nv.astype(np.int32)).cuda()
+        A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+        L.check(lib.dust_aic_surfpan_dev(n, dnv.data_ptr(), dver.data_ptr(), C.byref(prm),
+                                         A.data_ptr(), n, None, n, None), "aic")
+        xt = torch.from_numpy(np.random.default_rng(k).standard_normal((8, n))).cuda()
+        b = (xt @ A)  # rows: b_q = A x_q in column-major storage
+        mats.append(A)
+        rhs.append(b.contiguous())
+        ns.append(n)
+    torch.cuda.synchronize()
+    nb = ncase
+    n_arr = (C.c_int * nb)(*ns)
+    pa = (C.c_void_p * nb)(*[A.data_ptr() for A in mats])
+    it = (C.c_int * nb)()
+    res = (C.c_double * nb)()
+    ts_seq, ts_bat = [], []
+    for _ in range(reps):
+        X = [b.clone() for b in rhs]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(nb):
+            L.check(lib.dust_dgesv_mixed_dev(ns[i], 8, mats[i].data_ptr(), ns[i], X[i].data_ptr(),
+                                             ns[i], L.LU_FP16S, 1e-12, 50, C.byref(nl),
+                                             C.byref(C.c_double()), None), "mixed")
+        torch.cuda.synchronize()
+        ts_seq.append(time.perf_counter() - t0)
+        X = [b.clone() for b in rhs]
+        pb = (C.c_void_p * nb)(*[x.data_ptr() for x in X])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.check(lib.dust_dgesv_mixed_batch_dev(nb, n_arr, pa, n_arr, pb, n_arr, 8, L.LU_FP16S,
+                                               1e-12, 50, it, res), "mixed batch")
+        torch.cuda.synchronize()
+        ts_bat.append(time.perf_counter() - t0)
+    out.update(aic_sizes=[min(ns), max(ns)], lu_one_by_one_ms=1e3 * float(np.median(ts_seq)),
+               lu_batched_ms=1e3 * float(np.median(ts_bat)),
+               lu_batch_speedup=float(np.median(ts_seq) / np.median(ts_bat)),
+               lu_max_backward_error=float(max(res)), lu_iters=[min(it), max(it)])
+    return out
+
+
 def cpu_baseline(W, n_sample, degree):
     """The CPU oracle (FP64 restatement, OpenMP) on a bounded sample of the same
     generator; particle updates per second."""
@@ -318,6 +413,7 @@ def main():
                     help="tuned form: FP64 for pairs closer than this many core radii")
     ap.add_argument("--no-fp64", action="store_true", help="skip timing the plain FP64 form")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 48-case batch leg")
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB",
                                                           "COPY_WRITE_DB"))
     ap.add_argument("--summarize-trace", metavar="TRACE_DB")
@@ -389,6 +485,15 @@ def main():
         except Exception as e:
             log(f"LU timing failed: {e}")
             lu = dict(error=str(e))
+    batch = None
+    if not args.no_batch:
+        log("timing the 48-case batch")
+        try:
+            batch = time_batch(torch, L, O, W)
+            log(f"batch: {batch}")
+        except Exception as e:
+            log(f"batch timing failed: {e}")
+            batch = dict(error=str(e))
     cpu = None
     if not args.no_cpu:
         log("timing CPU oracle sample")
@@ -426,6 +531,7 @@ def main():
         "work": {"p2p_pairs": tuned["p2p_pairs"], "m2l_pairs": tuned["m2l_pairs"]},
         "roofline": roof,
         "lu": lu,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out, default=float))

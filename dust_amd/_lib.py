@@ -73,6 +73,7 @@ SIGNATURES = [
     ("dust_fmm_apply_multipole_dev", C.c_int, [_vp]),
     ("dust_fmm_results_dev", C.c_int, [_vp, _vp, _vp, _vp]),
     ("dust_fmm_pass_dev", C.c_int, [_vp, _ip]),
+    ("dust_fmm_pass_batch_dev", C.c_int, [C.c_int, _vp, _ip]),
     ("dust_fmm_set_tuning", C.c_int, [_vp, C.c_char_p, C.c_double]),
     ("dust_fmm_stream", _vp, [_vp]),
     ("dust_fmm_synchronize", C.c_int, [_vp]),
@@ -100,6 +101,8 @@ SIGNATURES = [
     ("dust_dgesv_mixed_dev", C.c_int,
      [C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _dp,
       _vp]),
+    ("dust_dgesv_mixed_batch_dev", C.c_int,
+     [C.c_int, _ip, _vp, _ip, _vp, _ip, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _dp]),
     ("dust_solve_linsys", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp]),
     ("dust_test_gemm_sub_dev", C.c_int,
      [C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int]),

@@ -1106,14 +1106,18 @@ static int fmm_set_particles(FmmHandle *h, int n, const double *cen, const doubl
 }
 
 // sort_particles (mod_octree.f90:703-996)
-static int fmm_sort(FmmHandle *h, int *nleaves_out) {
+// sort_particles in three phases separated by host read-backs of device
+// counts (fmm_phase_* below); the read-back targets live in the handle so
+// that several handles can be advanced between synchronisations.
+static int sort_phase_a(FmmHandle *h) {
   DUST_CHECK(h, DUST_E_ARG, "null handle");
   const int n = h->n;
   TreeGeom G = make_geom(h);
   const int TB = 256;
   timer_begin(h, DUST_FMM_STAGE_SORT);
-  const int init_err = 0x7fffffff;
-  DUST_HIP(hipMemcpyAsync(h->d_err.p, &init_err, sizeof(int), hipMemcpyHostToDevice, h->stream));
+  h->hb_init_err = 0x7fffffff;
+  DUST_HIP(hipMemcpyAsync(h->d_err.p, &h->hb_init_err, sizeof(int), hipMemcpyHostToDevice,
+                          h->stream));
   if (n > 0) {
     k_keys<<<ceil_div(n, TB), TB, 0, h->stream>>>(G, n, h->cen_in.p, h->key.p, h->idx_in.p,
                                                   h->d_err.p);
@@ -1135,18 +1139,23 @@ static int fmm_sort(FmmHandle *h, int *nleaves_out) {
                                                                         h->prm.min_part, h->flags.p);
   DUST_TRY(prim_select_flagged_cells(h, 0, h->active.p, h->counts.p));
   DUST_TRY(prim_select_flagged_cells(h, 1, h->leaves.p, h->counts.p + 1));
-  int hc[kMaxLevels + 4];
-  int herr = 0;
-  DUST_HIP(hipMemcpyAsync(hc, h->counts.p, sizeof(int) * 2, hipMemcpyDeviceToHost, h->stream));
-  DUST_HIP(hipMemcpyAsync(&herr, h->d_err.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-  DUST_HIP(hipStreamSynchronize(h->stream));
-  if (herr != init_err) {
-    set_error("particle %d resulted outside the octree box", herr);
+  DUST_HIP(hipMemcpyAsync(h->hb_counts, h->counts.p, sizeof(int) * 2, hipMemcpyDeviceToHost,
+                          h->stream));
+  DUST_HIP(hipMemcpyAsync(&h->hb_err, h->d_err.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  return 0;
+}
+
+static int sort_phase_b(FmmHandle *h) {
+  const int n = h->n;
+  TreeGeom G = make_geom(h);
+  const int TB = 256;
+  if (h->hb_err != h->hb_init_err) {
+    set_error("particle %d resulted outside the octree box", h->hb_err);
     timer_end(h, DUST_FMM_STAGE_SORT);
     return DUST_E_OUTSIDE;
   }
-  h->n_active = hc[0];
-  h->n_leaves = hc[1];
+  h->n_active = h->hb_counts[0];
+  h->n_leaves = h->hb_counts[1];
   k_level_starts<<<1, 64, 0, h->stream>>>(G, h->n_active, h->active.p, h->counts.p + 2);
   DUST_HIP(hipMemcpyAsync(h->act_start, h->counts.p + 2, sizeof(int) * (h->nlev + 1),
                           hipMemcpyDeviceToHost, h->stream));
@@ -1175,11 +1184,15 @@ static int fmm_sort(FmmHandle *h, int *nleaves_out) {
   DUST_TRY(h->mp_a.reserve(na * 3 * h->nmon));
   DUST_TRY(h->mp_b.reserve(na * 3 * h->nmon));
   DUST_TRY(h->mp_c.reserve(na * 9 * h->nmon));
-  int hu[2] = {0, 0};
-  DUST_HIP(hipMemcpyAsync(hu, h->n_units.p, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
-  DUST_HIP(hipStreamSynchronize(h->stream));
-  h->n_unit = hu[0];
-  h->n_item = hu[1];
+  DUST_HIP(hipMemcpyAsync(h->hb_units, h->n_units.p, 2 * sizeof(int), hipMemcpyDeviceToHost,
+                          h->stream));
+  return 0;
+}
+
+static int sort_phase_c(FmmHandle *h) {
+  const int TB = 256;
+  h->n_unit = h->hb_units[0];
+  h->n_item = h->hb_units[1];
   DUST_TRY(h->items.reserve((size_t)std::max(h->n_item, 1) * 4));
   DUST_TRY(h->part.reserve((size_t)std::max(h->n_item, 1) * kWave * 6));
   if (h->n_unit > 0)
@@ -1189,12 +1202,21 @@ static int fmm_sort(FmmHandle *h, int *nleaves_out) {
   timer_end(h, DUST_FMM_STAGE_SORT);
   h->sorted = true;
   h->computed = false;
+  return 0;
+}
+
+static int fmm_sort(FmmHandle *h, int *nleaves_out) {
+  DUST_TRY(sort_phase_a(h));
+  DUST_HIP(hipStreamSynchronize(h->stream));
+  DUST_TRY(sort_phase_b(h));
+  DUST_HIP(hipStreamSynchronize(h->stream));
+  DUST_TRY(sort_phase_c(h));
   if (nleaves_out) *nleaves_out = h->n_leaves;
   return 0;
 }
 
 // calculate_multipole (mod_octree.f90:1002-1133)
-static int fmm_multipole(FmmHandle *h) {
+static int multipole_phase_a(FmmHandle *h) {
   DUST_CHECK(h && h->sorted, DUST_E_ARG, "sort_particles must precede calculate_multipole");
   TreeGeom G = make_geom(h);
   const int nm = h->nmon;
@@ -1221,12 +1243,25 @@ static int fmm_multipole(FmmHandle *h) {
                                        h->act_start[l], h->flags.p, h->slot.p, h->mp_a.p);
   }
   timer_end(h, DUST_FMM_STAGE_M2M);
-  const bool tuned = h->prm.precision == DUST_FMM_TUNED;
-  if (tuned) {
+  if (h->prm.precision == DUST_FMM_TUNED) {
     // translations of every level below the first in one launch (they only
-    // read the multipoles); the L2L chain adds the parents afterwards
+    // read the multipoles); the L2L chain adds the parents afterwards.  The
+    // targets are grouped now; the group sizes are read after the next
+    // synchronisation (multipole_phase_b).
     timer_begin(h, DUST_FMM_STAGE_M2L);
     DUST_TRY(tuned_pass_prepare(h));
+  }
+  return 0;
+}
+
+static int multipole_phase_b(FmmHandle *h) {
+  if (h->n_active == 0) return 0;
+  TreeGeom G = make_geom(h);
+  const int nm = h->nmon;
+  const int tpb = ((nm + 63) / 64) * 64;
+  const int use_vs = h->prm.use_vs;
+  const bool tuned = h->prm.precision == DUST_FMM_TUNED;
+  if (tuned) {
     DUST_TRY(tuned_m2l_all(h));
     timer_end(h, DUST_FMM_STAGE_M2L);
   }
@@ -1303,6 +1338,13 @@ int eval_finish_launch(FmmHandle *h, const EvalArgs &A) {
     k_eval_finish<<<ceil_div(A.n_units, kEvalWaves), kWave * kEvalWaves, sh, h->stream>>>(A);
   DUST_HIP(hipGetLastError());
   return 0;
+}
+
+static int fmm_multipole(FmmHandle *h) {
+  DUST_TRY(multipole_phase_a(h));
+  if (h->prm.precision == DUST_FMM_TUNED && h->n_active > 0)
+    DUST_HIP(hipStreamSynchronize(h->stream));  // target group sizes
+  return multipole_phase_b(h);
 }
 
 // apply_multipole (mod_octree.f90:1139-1429), particle terms + free stream
@@ -1422,6 +1464,34 @@ int dust_fmm_pass_dev(dust_fmm_t hh, int *nleaves) {
   DUST_TRY(fmm_multipole(hh->h));
   DUST_TRY(fmm_apply(hh->h));
   return 0;
+}
+
+// Several independent cases: the passes advance in lockstep, phase by phase
+// (sort a / b / c, multipole a / b, apply), each handle on its own stream; one
+// synchronisation of every stream separates two phases, so the kernels of
+// different cases overlap and the host round trips are shared.
+int dust_fmm_pass_batch_dev(int nh, dust_fmm_t *hs, int *nleaves) {
+  DUST_CHECK(nh >= 0 && (nh == 0 || hs), DUST_E_ARG, "invalid batch arguments");
+  for (int i = 0; i < nh; ++i) DUST_CHECK(hs[i], DUST_E_ARG, "null handle %d in batch", i);
+  auto sync_all = [&]() -> int {
+    for (int i = 0; i < nh; ++i) DUST_HIP(hipStreamSynchronize(hs[i]->h->stream));
+    return 0;
+  };
+  for (int i = 0; i < nh; ++i) DUST_TRY(sort_phase_a(hs[i]->h));
+  DUST_TRY(sync_all());
+  for (int i = 0; i < nh; ++i) DUST_TRY(sort_phase_b(hs[i]->h));
+  DUST_TRY(sync_all());
+  for (int i = 0; i < nh; ++i) {
+    DUST_TRY(sort_phase_c(hs[i]->h));
+    DUST_TRY(multipole_phase_a(hs[i]->h));
+    if (nleaves) nleaves[i] = hs[i]->h->n_leaves;
+  }
+  DUST_TRY(sync_all());
+  for (int i = 0; i < nh; ++i) {
+    DUST_TRY(multipole_phase_b(hs[i]->h));
+    DUST_TRY(fmm_apply(hs[i]->h));
+  }
+  return sync_all();
 }
 
 void *dust_fmm_stream(dust_fmm_t h) { return h ? (void *)h->h->stream : nullptr; }

@@ -94,6 +94,10 @@ struct FmmHandle {
   hipStream_t stream = nullptr;
   StageTimer timer;
   bool sorted = false, computed = false;
+  // host targets of the asynchronous read-backs between pass phases
+  int hb_init_err = 0, hb_err = 0;
+  int hb_counts[2] = {0, 0};   // active cells, leaves
+  int hb_units[2] = {0, 0};    // evaluation units, items
 };
 
 // fmm_prim.hip: device-wide primitives (radix sort, selection).

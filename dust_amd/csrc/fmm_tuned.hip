@@ -28,6 +28,13 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kOct = 189;  // interaction offsets per target octant
 
+struct M2LJob {
+  int lvl, oct, t0, j0, rb, nt;  // nt = targets in the group
+  long grp_off;                  // first grouped target of (lvl, oct)
+  long kd_base;
+  int kdl0, kdl1, kdl2, pad;
+};
+
 struct TunedState {
   DevBuf<float> a32;        // FP32 copy of the multipole coefficients, (3 n_mon) per slot
   DevBuf<int> grp;          // target slots grouped by (level, octant)
@@ -36,6 +43,7 @@ struct TunedState {
   DevBuf<int> offs;         // [8][189][3] offsets of each octant
   DevBuf<float> W;          // M2L operators of every level and offset, fragment order
   DevBuf<unsigned char> jobs;  // M2L job list of the current pass
+  std::vector<M2LJob> host_jobs;
   int hcount[kMaxLevels * 8];
   long gbase[kMaxLevels * 8];  // first entry of every group in grp
   double near_factor = 1.0;
@@ -224,12 +232,6 @@ static int build_w(FmmHandle *h) {
 // multipoles, so every level can be translated before the L2L chain).
 constexpr int kJC = 21;  // offsets per job (189 = 9 x 21)
 
-struct M2LJob {
-  int lvl, oct, t0, j0, rb, nt;  // nt = targets in the group
-  long grp_off;                  // first grouped target of (lvl, oct)
-  long kd_base;
-  int kdl0, kdl1, kdl2, pad;
-};
 
 struct M2LArgs {
   WGeom w;
@@ -540,8 +542,7 @@ int tuned_pass_prepare(FmmHandle *h) {
   long na3 = (long)h->n_active * 3 * h->nmon;
   DUST_TRY(t->a32.reserve(std::max(na3, 1L)));
   k_to_f32<<<ceil_div(na3, 256), 256, 0, h->stream>>>(na3, h->mp_a.p, t->a32.p);
-  DUST_HIP(hipStreamSynchronize(h->stream));
-  return 0;
+  return 0;  // hcount is read after the caller's next synchronisation
 }
 
 // M2L of every level >= 2 (1-based) in one launch.
@@ -591,7 +592,10 @@ int tuned_m2l_all(FmmHandle *h) {
   if (qmax == 0) return 0;
   M2LJob empty{};
   empty.nt = 0;  // t0 >= nt: the job does nothing
-  std::vector<M2LJob> jobs(qmax * kXcd, empty);
+  // the host list stays in the tuned state until the next pass (whose first
+  // phase synchronises the stream), so the asynchronous upload needs no wait
+  std::vector<M2LJob> &jobs = t->host_jobs;
+  jobs.assign(qmax * kXcd, empty);
   for (int x = 0; x < kXcd; ++x)
     for (size_t k = 0; k < queue[x].size(); ++k) jobs[k * kXcd + x] = queue[x][k];
   DUST_TRY(t->jobs.reserve(jobs.size() * sizeof(M2LJob)));
@@ -610,8 +614,6 @@ int tuned_m2l_all(FmmHandle *h) {
   P.mp_c = h->mp_c.p;
   k_m2l_mfma<<<(unsigned)jobs.size(), 256, lds, h->stream>>>(P);
   DUST_HIP(hipGetLastError());
-  // the host vector must outlive the asynchronous copy
-  DUST_HIP(hipStreamSynchronize(h->stream));
   return 0;
 }
 

@@ -8,15 +8,14 @@
 // (src/linsys/mod_pressure_equation.f90:251-390).
 //
 // Structure: right-looking blocked LU.  Each panel is factored by one
-// cooperative launch whose blocks keep their rows of the panel in LDS and
+// launch of resident blocks that keep their rows of the panel in LDS and
 // meet at a grid barrier once per column (pivot search across blocks);
 // the row interchanges are then applied to the rest of the matrix, U12 is
 // obtained by a unit-lower triangular solve in LDS, and the trailing matrix
 // is updated by an FP64 MFMA GEMM.
-#include <hip/hip_cooperative_groups.h>
-
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -417,13 +416,17 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t st,
     }
     PanelArgs P{dA, lda, m, k, jb, R, dipiv, dinfo, arr, bar + 2, gen0, cval, crow, crowv, topv};
     gen0 += jb;
-    void *args[] = {&P};
     if (lds > 64 * 1024)
       (void)hipFuncSetAttribute((const void *)k_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds);
-    hipError_t e = hipLaunchCooperativeKernel((const void *)k_panel, dim3(G), dim3(256), args, lds, st);
+    // G <= 128 blocks of one CU each: all resident at once on an idle GPU (the
+    // grid barrier is bounded and reports a time-out otherwise).  An ordinary
+    // launch: the kernel does its own synchronisation and needs none of the
+    // cooperative-launch resources, which another process may hold.
+    k_panel<<<G, 256, lds, st>>>(P);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
-      set_error("cooperative panel launch failed (G=%d, lds=%zu): %s", G, lds, hipGetErrorString(e));
+      set_error("panel launch failed (G=%d, lds=%zu): %s", G, lds, hipGetErrorString(e));
       rc = DUST_E_HIP;
       break;
     }
@@ -492,13 +495,14 @@ int lu_getrs_dev(int n, int nrhs, const double *dA, long lda, const int *dipiv, 
     DUST_TRY(w.flags.reserve(1 + 2 * nblk));
     DUST_HIP(hipMemset(w.flags.p, 0, sizeof(unsigned) * w.flags.n));
   }
-  static bool attr = false;
   const size_t lds = sizeof(double) * SB * (SB + 1);
-  if (!attr) {
-    DUST_HIP(hipFuncSetAttribute((const void *)k_diag_inverse,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr = true;
-  }
+  static std::once_flag attr;
+  hipError_t ae = hipSuccess;
+  std::call_once(attr, [&] {
+    ae = hipFuncSetAttribute((const void *)k_diag_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds);
+  });
+  DUST_HIP(ae);
   k_laswp_rhs<<<1, 256, 0, st>>>(dB, ldb, nrhs, n, dipiv);
   k_diag_inverse<<<nblk, 256, lds, st>>>(dA, lda, n, w.linv.p, w.uinv.p);
   DUST_HIP(hipGetLastError());
@@ -510,11 +514,11 @@ int lu_getrs_dev(int n, int nrhs, const double *dA, long lda, const int *dipiv, 
       if (++w.gen == 0) ++w.gen;
       BlockSolveArgs<double> sa{dA, lda, n, nblk, upper ? w.uinv.p : w.linv.p, w.y.p,
                                 w.flags.p + 1 + upper * nblk, w.gen, w.flags.p};
-      void *args[] = {&sa};
-      hipError_t e = upper ? hipLaunchCooperativeKernel((const void *)k_block_solve<double, 1, 512>,
-                                                        dim3(G), dim3(512), args, 0, st)
-                           : hipLaunchCooperativeKernel((const void *)k_block_solve<double, 0, 512>,
-                                                        dim3(G), dim3(512), args, 0, st);
+      // G <= #CUs small blocks, all resident at once (ordinary launch; the
+      // flag waits are bounded)
+      if (upper) k_block_solve<double, 1, 512><<<G, 512, 0, st>>>(sa);
+      else k_block_solve<double, 0, 512><<<G, 512, 0, st>>>(sa);
+      hipError_t e = hipGetLastError();
       if (e != hipSuccess) {
         set_error("triangular solve launch failed: %s", hipGetErrorString(e));
         return DUST_E_HIP;

@@ -4,47 +4,6 @@
 
 namespace dust {
 
-// ---------------------------------------------------------------------------
-// Grid barrier for cooperative launches: arrival counter + generation word.
-// Spins are bounded; a timed-out barrier raises flag[2] and every block
-// leaves the kernel at the next check.
-// ---------------------------------------------------------------------------
-// Only the arriving thread fences at device scope: the workgroup barrier has
-// already made the block's stores visible in L2, and a device-scope release
-// writes the whole L2 back, so one per block is enough.
-__device__ inline bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &gen) {
-  __shared__ int ok;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    ok = 1;
-    __threadfence();
-    unsigned prev = atomicAdd(&bar[0], 1u);
-    if (prev == nblocks - 1) {
-      atomicExch(&bar[0], 0u);
-      __threadfence();
-      atomicAdd(&bar[1], 1u);
-    } else {
-      long spins = 0;
-      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins > (1L << 26)) {
-          atomicExch(&bar[2], 1u);
-          ok = 0;
-          break;
-        }
-        if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-          ok = 0;
-          break;
-        }
-      }
-    }
-    __threadfence();
-    gen += 1;
-  }
-  __syncthreads();
-  return ok != 0;
-}
-
 // Bounded spin until *f == gen (acquire).  On time-out raises *err; returns
 // false when *err is raised by anyone.
 __device__ inline bool wait_flag(unsigned *f, unsigned gen, unsigned *err) {
@@ -59,7 +18,9 @@ __device__ inline bool wait_flag(unsigned *f, unsigned gen, unsigned *err) {
   }
   return true;
 }
-// Grid barrier without a shared counter: block b publishes arrival number
+// Grid barrier for a grid whose blocks are all resident at once (a grid of
+// at most one block per CU), without a shared counter: block b publishes
+// arrival number
 // `target` in arr[b] and wave 0 polls all G words (G <= 128).  A block that
 // has already left this barrier can only have published target + 1, so the
 // test is "arr[q] - target >= 0" in wrap-around arithmetic.  Bounded spin;

@@ -27,11 +27,11 @@
 //
 // The workspace is process-wide and grows on demand; calls are not
 // re-entrant.
-#include <hip/hip_cooperative_groups.h>
-
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -806,11 +806,20 @@ struct MixedWs {
   DevBuf<unsigned long long> norms;
   DevBuf<int> ipiv64;
   unsigned gen = 0;
+  // host targets of the asynchronous read-backs
+  unsigned h_flags[2] = {0, 0};
+  unsigned long long h_norms[2 * NR + 1] = {};
+  unsigned h_err = 0;
   // second stream and events of the lookahead (default priority)
   hipStream_t s2 = nullptr;
   hipEvent_t ev_start = nullptr, ev_panel = nullptr, ev_rest = nullptr;
 };
 MixedWs g_ws;
+
+struct BatchSlot {
+  MixedWs ws;
+  hipStream_t stream = nullptr;
+};
 
 int mx_cus() {
   int dev = 0;
@@ -848,7 +857,7 @@ int mx_gemm(int f16, float *W, long ld, int r0, int r1, int c0, int c1, const vo
 
 // Factorisation of W (np x np).  Returns 0 or a HIP error; *bad is set when a
 // block failed the pivot check or hit a zero pivot.
-int mx_factor(MixedWs &w, int np, int f16, double tau, hipStream_t st, int *bad) {
+int mx_factor_launch(MixedWs &w, int np, int f16, hipStream_t st) {
   const int nblk = np / NB;
   const long ld = np;
   const size_t esz = f16 ? sizeof(_Float16) : sizeof(float);
@@ -910,12 +919,17 @@ int mx_factor(MixedWs &w, int np, int f16, double tau, hipStream_t st, int *bad)
   TrsmArgs ia{w.W.p, ld, 0, np, 2, f16, 1, w.dinv.p, nullptr, nullptr, w.flags.p, w.Linv.p, w.Uinv.p};
   k_mx_trsm<<<dim3(4, nblk), 256, lds_trsm(), st>>>(ia);
   MX_LAUNCH_CHECK("block inverses");
-  unsigned hf[2] = {0, 0};
-  DUST_HIP(hipMemcpyAsync(hf, w.flags.p, sizeof(hf), hipMemcpyDeviceToHost, st));
-  DUST_HIP(hipStreamSynchronize(st));
-  const float lmax = *(const float *)&hf[0];
-  *bad = (hf[1] != 0 || !(lmax <= (float)tau)) ? 1 : 0;
+  // max |l| and the zero-pivot flag, read after the caller's synchronisation
+  DUST_HIP(hipMemcpyAsync(w.h_flags, w.flags.p, sizeof(w.h_flags), hipMemcpyDeviceToHost, st));
   return 0;
+}
+
+// After the stream has drained: did a block fail the multiplier check or hit a
+// zero pivot?
+bool mx_factor_bad(const MixedWs &w, double tau) {
+  float lmax;
+  std::memcpy(&lmax, &w.h_flags[0], sizeof(float));
+  return w.h_flags[1] != 0 || !(lmax <= (float)tau);
 }
 
 // y <- U^-1 L^-1 y (y: np x 8 FP32, in place).
@@ -926,11 +940,11 @@ int mx_solve(MixedWs &w, int np, hipStream_t st, int cus) {
     if (++w.gen == 0) ++w.gen;  // generation 0 is the cleared state
     BlockSolveArgs<float> sa{w.W.p, (long)np, np, nblk, upper ? w.Uinv.p : w.Linv.p, w.y.p,
                              w.flags.p + 3 + upper * nblk, w.gen, w.flags.p + 2};
-    void *args[] = {&sa};
-    hipError_t e = upper ? hipLaunchCooperativeKernel((const void *)k_block_solve<float, 1, 256>,
-                                                      dim3(G), dim3(256), args, 0, st)
-                         : hipLaunchCooperativeKernel((const void *)k_block_solve<float, 0, 256>,
-                                                      dim3(G), dim3(256), args, 0, st);
+    // G <= #CUs small blocks, all resident at once (ordinary launch; the flag
+    // waits are bounded)
+    if (upper) k_block_solve<float, 1, 256><<<G, 256, 0, st>>>(sa);
+    else k_block_solve<float, 0, 256><<<G, 256, 0, st>>>(sa);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
       set_error("triangular solve launch failed: %s", hipGetErrorString(e));
       return DUST_E_HIP;
@@ -939,10 +953,10 @@ int mx_solve(MixedWs &w, int np, hipStream_t st, int cus) {
   return 0;
 }
 
-// r = b - A x and the norms; returns the largest backward error over the
-// right-hand sides.
-int mx_residual(MixedWs &w, const double *dA, long lda, int n, int nrhs, bool with_anorm,
-                double *anorm, double *berr, hipStream_t st) {
+// r = b - A x and the norms (read back asynchronously into the workspace,
+// with the triangular-solve time-out flag).
+int mx_residual_launch(MixedWs &w, const double *dA, long lda, int n, bool with_anorm,
+                       hipStream_t st) {
   const int cw = 512;
   const int nch = ceil_div(n, cw);
   k_mx_zero_u64<<<1, 64, 0, st>>>(w.norms.p, 2 * NR + 1);
@@ -951,11 +965,16 @@ int mx_residual(MixedWs &w, const double *dA, long lda, int n, int nrhs, bool wi
   k_mx_resid_fin<<<ceil_div(n, 256), 256, 0, st>>>(n, nch, w.part.p, w.b8.p, w.r8.p, w.x8.p,
                                                    with_anorm ? w.rowabs.p : nullptr, w.norms.p);
   MX_LAUNCH_CHECK("residual");
-  unsigned long long hn[2 * NR + 1];
-  DUST_HIP(hipMemcpyAsync(hn, w.norms.p, sizeof(hn), hipMemcpyDeviceToHost, st));
-  DUST_HIP(hipStreamSynchronize(st));
+  DUST_HIP(hipMemcpyAsync(w.h_norms, w.norms.p, sizeof(w.h_norms), hipMemcpyDeviceToHost, st));
+  DUST_HIP(hipMemcpyAsync(&w.h_err, w.flags.p + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  return 0;
+}
+
+// After the stream has drained: the largest backward error over the
+// right-hand sides, max_q ||r_q||_inf / (||A||_inf ||x_q||_inf).
+double mx_residual_berr(const MixedWs &w, int nrhs, bool with_anorm, double *anorm) {
   double nm[2 * NR + 1];
-  std::memcpy(nm, hn, sizeof(nm));
+  std::memcpy(nm, w.h_norms, sizeof(nm));
   if (with_anorm) *anorm = nm[2 * NR];
   double be = 0.0;
   for (int q = 0; q < nrhs; ++q) {
@@ -963,8 +982,7 @@ int mx_residual(MixedWs &w, const double *dA, long lda, int n, int nrhs, bool wi
     double e = den > 0.0 ? nm[q] / den : (nm[q] > 0.0 ? INFINITY : 0.0);
     be = std::max(be, e);
   }
-  *berr = be;
-  return 0;
+  return be;
 }
 
 int mx_reserve(MixedWs &w, int n, int np) {
@@ -1002,130 +1020,205 @@ int mx_reserve(MixedWs &w, int n, int np) {
   return 0;
 }
 
-// FP64 partial-pivoting solve of the packed right-hand sides (fallback and
-// DUST_LU_FP64).
-int mx_fp64(MixedWs &w, const double *dA, long lda, int n, hipStream_t st, int *info) {
-  DUST_TRY(w.A64.reserve((size_t)n * n));
-  DUST_TRY(w.ipiv64.reserve(n));
-  long tot = (long)n * n;
-  k_mx_copy_a<<<(int)((tot + 255) / 256), 256, 0, st>>>(dA, lda, n, w.A64.p);
+// ---------------------------------------------------------------------------
+// Drivers.  A system is factored once; its right-hand sides are refined in
+// chunks of 8.  Every step is split into launches (ending in asynchronous
+// read-backs into the workspace) and a host-side finish, so that a batch of
+// systems can advance in lockstep with one synchronisation per step.
+// ---------------------------------------------------------------------------
+struct MixedRun {
+  MixedWs *w = nullptr;
+  hipStream_t st = nullptr;
+  const double *dA = nullptr;
+  long lda = 0;
+  double *dB = nullptr;
+  long ldb = 0;
+  int n = 0, np = 0, nrhs = 0, prec = 0, max_iter = 0;
+  double tol = 0.0;
+  bool factored = false;  // tuned factors usable
+  bool fp64 = false;      // FP64 partial-pivoting factors in w->A64 / w->ipiv64
+  // current chunk
+  int q0 = 0, nq = 0, nit = 0, slow = 0, it_report = 0;
+  double anorm = 0.0, berr = INFINITY, prev = INFINITY, resid_report = 0.0;
+  bool refining = false, chunk_done = false;
+  int rc = 0;
+};
+
+int set_attributes() {
+  static std::once_flag attr;
+  hipError_t ae = hipSuccess;
+  std::call_once(attr, [&] {
+    ae = hipFuncSetAttribute((const void *)k_mx_diag, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds_diag());
+    if (ae == hipSuccess)
+      ae = hipFuncSetAttribute((const void *)k_mx_trsm, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_trsm());
+  });
+  DUST_HIP(ae);
+  return 0;
+}
+
+// W = R A and the tuned factorisation (launches only).
+int run_factor_launch(MixedRun &r) {
+  MixedWs &w = *r.w;
+  const int f16 = r.prec == DUST_LU_FP16S;
+  const int n = r.n, np = r.np;
+  DUST_TRY(set_attributes());
+  if (f16) {
+    k_mx_fill_u32<<<ceil_div(np, 256), 256, 0, r.st>>>(w.rowmax.p, np, 0u);
+    dim3 g(ceil_div(n, 256), ceil_div(n, 256));
+    k_mx_rowmax<<<g, 256, 0, r.st>>>(r.dA, r.lda, n, 256, w.rowmax.p);
+  }
+  k_mx_rowscale<<<ceil_div(np, 256), 256, 0, r.st>>>(w.rowmax.p, n, np, f16, w.rs.p);
+  k_mx_convert<<<dim3(np / 256 + (np % 256 != 0), np / 4), 256, 0, r.st>>>(r.dA, r.lda, n, np,
+                                                                         w.rs.p, w.W.p);
+  MX_LAUNCH_CHECK("convert");
+  return mx_factor_launch(w, np, f16, r.st);
+}
+
+// FP64 partial-pivoting factors (fallback and DUST_LU_FP64); synchronous.
+int run_fp64_factor(MixedRun &r) {
+  MixedWs &w = *r.w;
+  DUST_TRY(w.A64.reserve((size_t)r.n * r.n));
+  DUST_TRY(w.ipiv64.reserve(r.n));
+  long tot = (long)r.n * r.n;
+  k_mx_copy_a<<<(int)((tot + 255) / 256), 256, 0, r.st>>>(r.dA, r.lda, r.n, w.A64.p);
   MX_LAUNCH_CHECK("copy");
-  DUST_TRY(lu_getrf_dev(n, n, w.A64.p, n, w.ipiv64.p, st, 64, info));
-  if (*info) return 0;
-  DUST_HIP(hipMemcpyAsync(w.x8.p, w.b8.p, sizeof(double) * (size_t)n * NR, hipMemcpyDeviceToDevice, st));
-  DUST_TRY(lu_getrs_dev(n, NR, w.A64.p, n, w.ipiv64.p, w.x8.p, n, st));
+  int info = 0;
+  DUST_TRY(lu_getrf_dev(r.n, r.n, w.A64.p, r.n, w.ipiv64.p, r.st, 64, &info));
+  if (info) return info;
+  r.fp64 = true;
   return 0;
 }
 
-}  // namespace
-
-// Solve for up to 8 right-hand sides held in w.b8; solution in w.x8.
-int mixed_solve8(const double *dA, long lda, int n, int nrhs, int prec, double tol, int max_iter,
-                 int *iters, double *resid, hipStream_t st) {
-  MixedWs &w = g_ws;
-  const int np = ((n + NB - 1) / NB) * NB;
-  int nit = 0;
-  double anorm = 0.0, berr = INFINITY;
-  bool done = false;
-  if (prec != DUST_LU_FP64) {
-    const int f16 = prec == DUST_LU_FP16S;
-    static bool attr = false;
-    if (!attr) {
-      DUST_HIP(hipFuncSetAttribute((const void *)k_mx_diag, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds_diag()));
-      DUST_HIP(hipFuncSetAttribute((const void *)k_mx_trsm, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds_trsm()));
-      attr = true;
-    }
-    // W = R A
-    if (f16) {
-      k_mx_fill_u32<<<ceil_div(np, 256), 256, 0, st>>>(w.rowmax.p, np, 0u);
-      dim3 g(ceil_div(n, 256), ceil_div(n, 256));
-      k_mx_rowmax<<<g, 256, 0, st>>>(dA, lda, n, 256, w.rowmax.p);
-    }
-    k_mx_rowscale<<<ceil_div(np, 256), 256, 0, st>>>(w.rowmax.p, n, np, f16, w.rs.p);
-    k_mx_convert<<<dim3(np / 256 + (np % 256 != 0), np / 4), 256, 0, st>>>(dA, lda, n, np, w.rs.p, w.W.p);
-    MX_LAUNCH_CHECK("convert");
-    int bad = 0;
-    DUST_TRY(mx_factor(w, np, f16, 2.0, st, &bad));
-    if (!bad) {
-      const int cus = mx_cus();
-      double prev = INFINITY;
-      int slow = 0;
-      for (nit = 1; nit <= std::max(1, max_iter); ++nit) {
-        // d = U^-1 L^-1 P R r  (r = b on the first sweep)
-        k_mx_gather<<<ceil_div(np, 256), 256, 0, st>>>(nit == 1 ? w.b8.p : w.r8.p, n, n, np, nrhs,
-                                                       w.perm.p, w.rs.p, w.y.p);
-        MX_LAUNCH_CHECK("gather");
-        DUST_TRY(mx_solve(w, np, st, cus));
-        k_mx_update_x<<<ceil_div(n, 256), 256, 0, st>>>(w.x8.p, n, n, w.y.p, nit > 1);
-        MX_LAUNCH_CHECK("update");
-        DUST_TRY(mx_residual(w, dA, lda, n, nrhs, nit == 1, &anorm, &berr, st));
-        unsigned herr = 0;
-        DUST_HIP(hipMemcpyAsync(&herr, w.flags.p + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-        DUST_HIP(hipStreamSynchronize(st));
-        DUST_CHECK(herr == 0, DUST_E_HIP, "triangular solve flag wait timed out");
-        if (berr <= tol) {
-          done = true;
-          break;
-        }
-        if (!(berr < 0.5 * prev)) {
-          if (++slow >= 2) {
-            done = berr <= 1e-14;  // stalled at working accuracy
-            break;
-          }
-        } else {
-          slow = 0;
-        }
-        prev = berr;
-      }
-    }
-  }
-  if (!done) {
-    int info = 0;
-    DUST_TRY(mx_fp64(w, dA, lda, n, st, &info));
-    if (info) return info;
-    DUST_TRY(mx_residual(w, dA, lda, n, nrhs, true, &anorm, &berr, st));
-    nit = prec == DUST_LU_FP64 ? 0 : -1;
-  }
-  if (iters) *iters = nit;
-  if (resid) *resid = berr;
+// Start a chunk of right-hand sides: pack them into w.b8.
+int run_chunk_begin(MixedRun &r, int q0) {
+  r.q0 = q0;
+  r.nq = std::min(NR, r.nrhs - q0);
+  r.nit = 0;
+  r.slow = 0;
+  r.berr = r.prev = INFINITY;
+  r.chunk_done = false;
+  k_mx_pack_rhs<<<ceil_div(r.n, 256), 256, 0, r.st>>>(r.dB + r.ldb * q0, r.ldb, r.n, r.nq,
+                                                      r.w->b8.p);
+  MX_LAUNCH_CHECK("pack");
+  r.refining = r.factored && !r.fp64;
   return 0;
 }
 
-int mixed_dev(int n, int nrhs, const double *dA, long lda, double *dB, long ldb, int prec, double tol,
-              int max_iter, int *iters, double *resid, hipStream_t st) {
+// One refinement sweep (launches): d = U^-1 L^-1 P R r (r = b first), x += d,
+// r = b - A x with its norms.
+int run_sweep_launch(MixedRun &r) {
+  MixedWs &w = *r.w;
+  ++r.nit;
+  k_mx_gather<<<ceil_div(r.np, 256), 256, 0, r.st>>>(r.nit == 1 ? w.b8.p : w.r8.p, r.n, r.n, r.np,
+                                                     r.nq, w.perm.p, w.rs.p, w.y.p);
+  MX_LAUNCH_CHECK("gather");
+  DUST_TRY(mx_solve(w, r.np, r.st, mx_cus()));
+  k_mx_update_x<<<ceil_div(r.n, 256), 256, 0, r.st>>>(w.x8.p, r.n, r.n, w.y.p, r.nit > 1);
+  MX_LAUNCH_CHECK("update");
+  return mx_residual_launch(w, r.dA, r.lda, r.n, r.nit == 1, r.st);
+}
+
+// After the stream has drained: convergence test of the sweep.
+int run_sweep_finish(MixedRun &r) {
+  MixedWs &w = *r.w;
+  DUST_CHECK(w.h_err == 0, DUST_E_HIP, "triangular solve flag wait timed out");
+  r.berr = mx_residual_berr(w, r.nq, r.nit == 1, &r.anorm);
+  if (r.berr <= r.tol) {
+    r.refining = false;
+    r.chunk_done = true;
+  } else if (!(r.berr < 0.5 * r.prev)) {
+    if (++r.slow >= 2) {
+      r.refining = false;
+      r.chunk_done = r.berr <= 1e-14;  // stalled at working accuracy
+    }
+  } else {
+    r.slow = 0;
+  }
+  r.prev = r.berr;
+  if (r.refining && r.nit >= std::max(1, r.max_iter)) r.refining = false;
+  return 0;
+}
+
+// Finish a chunk: FP64 path when refinement did not converge, then unpack.
+int run_chunk_end(MixedRun &r) {
+  MixedWs &w = *r.w;
+  int nit = r.nit;
+  if (!r.chunk_done) {
+    if (!r.fp64) {
+      int info = run_fp64_factor(r);
+      if (info > 0) return info;
+      DUST_TRY(info);
+    }
+    DUST_HIP(hipMemcpyAsync(w.x8.p, w.b8.p, sizeof(double) * (size_t)r.n * NR,
+                            hipMemcpyDeviceToDevice, r.st));
+    DUST_TRY(lu_getrs_dev(r.n, NR, w.A64.p, r.n, w.ipiv64.p, w.x8.p, r.n, r.st));
+    DUST_TRY(mx_residual_launch(w, r.dA, r.lda, r.n, true, r.st));
+    DUST_HIP(hipStreamSynchronize(r.st));
+    r.berr = mx_residual_berr(w, r.nq, true, &r.anorm);
+    nit = r.prec == DUST_LU_FP64 ? 0 : -1;
+  }
+  k_mx_unpack_rhs<<<ceil_div(r.n, 256), 256, 0, r.st>>>(w.x8.p, r.n, r.nq, r.dB + r.ldb * r.q0,
+                                                        r.ldb);
+  MX_LAUNCH_CHECK("unpack");
+  r.it_report = (nit < 0 || r.it_report < 0) ? -1 : std::max(r.it_report, nit);
+  r.resid_report = std::max(r.resid_report, r.berr);
+  return 0;
+}
+
+int run_init(MixedRun &r, MixedWs &w, int n, int nrhs, const double *dA, long lda, double *dB,
+             long ldb, int prec, double tol, int max_iter, hipStream_t st) {
   DUST_CHECK(n >= 0 && nrhs >= 0 && lda >= std::max(1, n) && ldb >= std::max(1, n), DUST_E_ARG,
              "invalid dgesv_mixed dimensions");
   DUST_CHECK(prec == DUST_LU_FP64 || prec == DUST_LU_FP32 || prec == DUST_LU_FP16S, DUST_E_ARG,
              "unknown precision %d", prec);
+  r = MixedRun();
+  r.w = &w;
+  r.st = st;
+  r.dA = dA;
+  r.lda = lda;
+  r.dB = dB;
+  r.ldb = ldb;
+  r.n = n;
+  r.np = ((n + NB - 1) / NB) * NB;
+  r.nrhs = nrhs;
+  r.prec = prec;
+  r.tol = tol;
+  r.max_iter = max_iter;
+  if (n == 0 || nrhs == 0) return 0;
+  return mx_reserve(w, n, r.np);
+}
+
+int mixed_dev(MixedWs &w, int n, int nrhs, const double *dA, long lda, double *dB, long ldb, int prec,
+              double tol, int max_iter, int *iters, double *resid, hipStream_t st) {
   if (iters) *iters = 0;
   if (resid) *resid = 0.0;
+  MixedRun r;
+  DUST_TRY(run_init(r, w, n, nrhs, dA, lda, dB, ldb, prec, tol, max_iter, st));
   if (n == 0 || nrhs == 0) return 0;
-  MixedWs &w = g_ws;
-  const int np = ((n + NB - 1) / NB) * NB;
-  DUST_TRY(mx_reserve(w, n, np));
-  int it_max = 0;
-  double r_max = 0.0;
+  if (prec != DUST_LU_FP64) {
+    DUST_TRY(run_factor_launch(r));
+    DUST_HIP(hipStreamSynchronize(st));
+    r.factored = !mx_factor_bad(w, 2.0);
+  }
   for (int q0 = 0; q0 < nrhs; q0 += NR) {
-    const int nq = std::min(NR, nrhs - q0);
-    k_mx_pack_rhs<<<ceil_div(n, 256), 256, 0, st>>>(dB + ldb * q0, ldb, n, nq, w.b8.p);
-    MX_LAUNCH_CHECK("pack");
-    int it = 0;
-    double r = 0.0;
-    int rc = mixed_solve8(dA, lda, n, nq, prec, tol, max_iter, &it, &r, st);
+    DUST_TRY(run_chunk_begin(r, q0));
+    while (r.refining) {
+      DUST_TRY(run_sweep_launch(r));
+      DUST_HIP(hipStreamSynchronize(st));
+      DUST_TRY(run_sweep_finish(r));
+    }
+    int rc = run_chunk_end(r);
     if (rc) return rc;
-    k_mx_unpack_rhs<<<ceil_div(n, 256), 256, 0, st>>>(w.x8.p, n, nq, dB + ldb * q0, ldb);
-    MX_LAUNCH_CHECK("unpack");
-    it_max = (it < 0 || it_max < 0) ? -1 : std::max(it_max, it);
-    r_max = std::max(r_max, r);
   }
   DUST_HIP(hipStreamSynchronize(st));
-  if (iters) *iters = it_max;
-  if (resid) *resid = r_max;
+  if (iters) *iters = r.it_report;
+  if (resid) *resid = r.resid_report;
   return 0;
 }
 
+}  // namespace
 }  // namespace dust
 
 using namespace dust;
@@ -1134,7 +1227,7 @@ extern "C" {
 
 int dust_dgesv_mixed_dev(int n, int nrhs, const double *dA, int lda, double *dB, int ldb, int prec,
                          double tol, int max_iter, int *iters, double *resid, void *stream) {
-  return mixed_dev(n, nrhs, dA, lda, dB, ldb, prec, tol, max_iter, iters, resid,
+  return mixed_dev(g_ws, n, nrhs, dA, lda, dB, ldb, prec, tol, max_iter, iters, resid,
                    (hipStream_t)stream);
 }
 
@@ -1162,7 +1255,7 @@ int dust_dgesv_mixed(int n, int nrhs, const double *A, int lda, double *B, int l
     set_error("upload failed");
     rc = DUST_E_HIP;
   }
-  if (!rc) rc = mixed_dev(n, nrhs, dA, n, dB, n, prec, tol, max_iter, iters, resid, nullptr);
+  if (!rc) rc = mixed_dev(g_ws, n, nrhs, dA, n, dB, n, prec, tol, max_iter, iters, resid, nullptr);
   if (!rc && hipMemcpy2D(B, sizeof(double) * ldb, dB, sizeof(double) * n, sizeof(double) * n, nrhs,
                          hipMemcpyDeviceToHost) != hipSuccess) {
     set_error("download failed");
@@ -1171,6 +1264,67 @@ int dust_dgesv_mixed(int n, int nrhs, const double *A, int lda, double *B, int l
   (void)hipFree(dA);
   (void)hipFree(dB);
   return rc;
+}
+
+// Independent systems in lockstep: each system has its own workspace and
+// stream; the factorisations of all systems are launched, then one
+// synchronisation of every stream, then refinement sweeps of all unfinished
+// systems, one synchronisation per sweep.  Latency-bound parts of different
+// systems (diagonal-block chains, refinement solves) overlap on the GPU.
+// Device pointers.
+int dust_dgesv_mixed_batch_dev(int nb, const int *n, const double *const *dA, const int *lda,
+                               double *const *dB, const int *ldb, int nrhs, int prec, double tol,
+                               int max_iter, int *iters, double *resid) {
+  DUST_CHECK(nb >= 0 && (nb == 0 || (n && dA && lda && dB && ldb)), DUST_E_ARG,
+             "invalid batch arguments");
+  if (nb == 0) return 0;
+  static std::vector<std::unique_ptr<BatchSlot>> slots;
+  while ((int)slots.size() < nb) {
+    slots.emplace_back(new BatchSlot());
+    DUST_HIP(hipStreamCreateWithFlags(&slots.back()->stream, hipStreamNonBlocking));
+  }
+  std::vector<MixedRun> runs(nb);
+  for (int i = 0; i < nb; ++i)
+    DUST_TRY(run_init(runs[i], slots[i]->ws, n[i], nrhs, dA[i], lda[i], dB[i], ldb[i], prec, tol,
+                      max_iter, slots[i]->stream));
+  auto sync_all = [&]() -> int {
+    for (int i = 0; i < nb; ++i) DUST_HIP(hipStreamSynchronize(runs[i].st));
+    return 0;
+  };
+  const bool any = nrhs > 0;
+  if (prec != DUST_LU_FP64 && any) {
+    for (int i = 0; i < nb; ++i)
+      if (runs[i].n > 0) DUST_TRY(run_factor_launch(runs[i]));
+    DUST_TRY(sync_all());
+    for (int i = 0; i < nb; ++i) runs[i].factored = runs[i].n > 0 && !mx_factor_bad(slots[i]->ws, 2.0);
+  }
+  for (int q0 = 0; any && q0 < nrhs; q0 += NR) {
+    for (int i = 0; i < nb; ++i)
+      if (runs[i].n > 0) DUST_TRY(run_chunk_begin(runs[i], q0));
+    while (true) {
+      bool active = false;
+      for (int i = 0; i < nb; ++i)
+        if (runs[i].n > 0 && runs[i].refining) {
+          DUST_TRY(run_sweep_launch(runs[i]));
+          active = true;
+        }
+      if (!active) break;
+      DUST_TRY(sync_all());
+      for (int i = 0; i < nb; ++i)
+        if (runs[i].n > 0 && runs[i].refining) DUST_TRY(run_sweep_finish(runs[i]));
+    }
+    for (int i = 0; i < nb; ++i)
+      if (runs[i].n > 0) {
+        int rc = run_chunk_end(runs[i]);
+        if (rc) return rc;
+      }
+  }
+  DUST_TRY(sync_all());
+  for (int i = 0; i < nb; ++i) {
+    if (iters) iters[i] = runs[i].it_report;
+    if (resid) resid[i] = runs[i].resid_report;
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -7,6 +7,11 @@ ellipsoid    -- perturbed ellipsoid surface mesh for the dense AIC
 wing_case    -- rectangular NACA0012 wing (40 chordwise per side x 20 spanwise)
                 with a trailing vortex-particle sheet (1,600 panels, 8,000
                 particles)
+rotor        -- b rectangular NACA0012 blades (aspect ratio 10) around the z
+                axis, the surface-panel mesh of one rotor case
+batch_cases  -- the many-case workload: 48 rotor cases with 2-5 blades and
+                500-4,000 panels, each with a helix wake of 50k-400k
+                particles (pitch 0.4-0.6)
 Each generator returns plain numpy arrays plus the octree settings that go
 with them.
 """
@@ -28,8 +33,8 @@ def helix_wake(n=1_000_000, seed=HELIX_SEED, tube_fraction=0.9, radius=1.0, pitc
     nb = n - nt
     k = rng.integers(0, 3, nt)
     phi = 2.0 * np.pi * k / 3.0
-    s = rng.uniform(0.0, 14.0 * np.pi, nt)
     c = pitch / (2.0 * np.pi)
+    s = rng.uniform(0.0, 3.5 / c, nt)  # z from -1.75 to 1.75 (14 pi at pitch 0.5)
     a = s + phi
     ca, sa = np.cos(a), np.sin(a)
     centre = np.stack([radius * ca, radius * sa, -1.75 + c * s], axis=1)
@@ -167,3 +172,59 @@ def wing_case(nc=40, ns=20, aspect=8.0, alpha_deg=5.0, u=30.0, nrows=200, seed=7
     tree = dict(box_length=box, nbox=(1, 1, 1), origin=tuple(lo), nlevels=5, min_part=64,
                 delta=0.1)
     return ver, nv, cen, alpha, rv, tree
+
+
+def rotor(blades=3, panels=1000, aspect=10.0, root=0.2, pitch_deg=8.0):
+    """`blades` rectangular NACA0012 blades (chord 1, span = aspect), root at
+    radius `root`, pitched by pitch_deg, evenly spaced around the z axis;
+    about `panels` quads in total (upper and lower surfaces, cosine chordwise
+    spacing).  Returns (ver (n,4,3), n_ver)."""
+    per = max(8, panels // blades)
+    ns = max(2, int(round(np.sqrt(per * aspect / 8.0))))
+    nc = max(2, int(round(per / (2 * ns))))
+    xc = 0.5 * (1.0 - np.cos(np.linspace(0.0, np.pi, nc + 1))) - 0.25
+    rs = np.linspace(root, root + aspect, ns + 1)
+    zt = naca0012(xc + 0.25)
+    cp, sp = np.cos(np.radians(pitch_deg)), np.sin(np.radians(pitch_deg))
+    ver = []
+    for b in range(blades):
+        th = 2.0 * np.pi * b / blades
+        ct, st = np.cos(th), np.sin(th)
+
+        def pt(x, r, z):
+            xx, zz = x * cp + z * sp, -x * sp + z * cp  # blade pitch about its span axis
+            return np.array([r * ct - xx * st, r * st + xx * ct, zz])
+        for j in range(ns):
+            for i in range(nc):
+                a, c = nc - i, nc - i - 1
+                ver.append([pt(xc[a], rs[j], zt[a]), pt(xc[c], rs[j], zt[c]),
+                            pt(xc[c], rs[j + 1], zt[c]), pt(xc[a], rs[j + 1], zt[a])])
+            for i in range(nc):
+                a, c = i, i + 1
+                ver.append([pt(xc[a], rs[j], -zt[a]), pt(xc[c], rs[j], -zt[c]),
+                            pt(xc[c], rs[j + 1], -zt[c]), pt(xc[a], rs[j + 1], -zt[a])])
+    ver = np.array(ver)
+    return ver, np.full(len(ver), 4, np.int32)
+
+
+def batch_cases(ncase=48, seed=20261205):
+    """The many-case workload: ncase rotor cases, each with 2-5 blades and
+    500-4,000 surface panels and a helix wake of 50k-400k particles (pitch
+    0.4-0.6, seed varied per case).  Returns a list of dicts with the rotor
+    mesh (ver, n_ver) and the wake (cen, alpha, rv, tree)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(ncase):
+        blades = int(rng.integers(2, 6))
+        panels = int(rng.integers(500, 4001))
+        n = int(rng.integers(50_000, 400_001))
+        pitch = float(rng.uniform(0.4, 0.6))
+        ver, nv = rotor(blades, panels)
+        cen, alpha, rv, tree = helix_wake(n, seed=seed + 1 + k, pitch=pitch)
+        # the rare tube particles beyond 5 sigma of the cross-section are kept
+        # inside the octree box
+        cen = np.clip(cen, -2.0 + 1e-3, 2.0 - 1e-3)
+        tree = dict(tree, nlevels=7 - int(round(np.log(1_000_000 / n) / np.log(8))))
+        out.append(dict(blades=blades, ver=ver, n_ver=nv, cen=cen, alpha=alpha, rv=rv,
+                        tree=tree))
+    return out

@@ -103,6 +103,16 @@ int dust_fmm_results_dev(dust_fmm_t h, double *vel, double *stretch, double *tur
 /* One full pass (sort + multipole + apply) on device-resident particles. */
 int dust_fmm_pass_dev(dust_fmm_t h, int *nleaves);
 
+/* Several independent cases, one pass each (sort + multipole + apply on the
+ * device-resident particles of every handle), advanced in lockstep: every
+ * phase of the pass is launched for all handles, each on its own stream,
+ * before one synchronisation of all of them, so the cases' kernels overlap
+ * and the host round trips are shared.  Returns when all passes are done;
+ * nleaves[i] (may be NULL) receives each case's leaf count.  The handles must
+ * be distinct.  No counterpart in the reference (its cases are separate
+ * runs); for many-case workloads. */
+int dust_fmm_pass_batch_dev(int nh, dust_fmm_t *handles, int *nleaves);
+
 /* Knobs of the tuned form: "near_factor" -- pairs closer than near_factor
  * times the source core radius are evaluated in FP64 (default 1). */
 int dust_fmm_set_tuning(dust_fmm_t h, const char *name, double value);
@@ -217,6 +227,18 @@ int dust_dgesv_mixed(int n, int nrhs, const double *A, int lda, double *B, int l
 int dust_dgesv_mixed_dev(int n, int nrhs, const double *dA, int lda, double *dB, int ldb,
                          int prec, double tol, int max_iter, int *iters, double *resid,
                          void *stream);
+
+/* Batch of independent tuned solves (device pointers, one system per entry,
+ * sizes may differ), advanced in lockstep: the factorisations of all systems
+ * are launched (each system on its own stream and workspace), then each
+ * refinement sweep of all unfinished systems, with one synchronisation of all
+ * streams per step.  iters[i] / resid[i] as for dust_dgesv_mixed_dev (may be
+ * NULL).  Returns 0 or the first failing system's status.  Process-wide
+ * workspaces: not re-entrant.  No counterpart in the reference; for many-case
+ * runs. */
+int dust_dgesv_mixed_batch_dev(int nb, const int *n, const double *const *dA, const int *lda,
+                               double *const *dB, const int *ldb, int nrhs, int prec, double tol,
+                               int max_iter, int *iters, double *resid);
 
 /* solve_linsys block path (mod_linsys.f90:352-483): the leading nstatic
  * block of A(lda,rank) is already factored (ipiv(1:nstatic) from dgetrf);

@@ -54,6 +54,7 @@ program dust_gpu_driver
   call check(dust_fmm_destroy(h), 'destroy_octree')
 
   info = dust_dgetrf(nlu, nlu, A, nlu, ipiv)
+  if (info < 0) print '(a,i0,a,a)', 'dust_gpu: dgetrf failed (', info, '): ', dust_error_text()
   if (info == 0) call check(dust_dgetrs(nlu, 1, A, nlu, ipiv, b, nlu), 'dgetrs')
 
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')

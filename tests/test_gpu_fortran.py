@@ -53,5 +53,5 @@ def test_fortran_driver_fmm_and_lu(orc, tmp_path):
     assert nleaves == len(c.leaves()[0])
     assert rel(vel, v) <= 1e-10
     assert rel(st, s) <= 1e-10
-    assert info == 0
+    assert info == 0, p.stdout
     assert rel(x, np.linalg.solve(A, b)) <= 1e-10
