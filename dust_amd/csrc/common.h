@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstring>
 #include <string>
 
 #include "../../include/dust_gpu.h"
@@ -62,6 +63,37 @@ struct DevBuf {
   }
   void release() {
     if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// Page-locked host buffer that grows on demand, the source or target of
+// asynchronous copies.  A copy between the device and pageable memory is
+// staged by the runtime and returns only once the stream has reached it,
+// which would tie the host to the GPU at every read-back.  Zero-filled.
+template <typename T>
+struct HostBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  int reserve(size_t count) {
+    if (count <= n) return 0;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    size_t bytes = (count ? count : 1) * sizeof(T);
+    void *q = nullptr;
+    if (hipHostMalloc(&q, bytes, hipHostMallocDefault) != hipSuccess) {
+      set_error("hipHostMalloc of %zu bytes failed", bytes);
+      return DUST_E_NOMEM;
+    }
+    std::memset(q, 0, bytes);
+    p = (T *)q;
+    n = count;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
     p = nullptr;
     n = 0;
   }

@@ -972,6 +972,7 @@ static int fmm_create(const dust_fmm_params *prm, int capacity, FmmHandle **out)
     set_error("hipStreamCreate failed");
     return fail(DUST_E_HIP);
   }
+  if (h->rb.reserve(1)) return fail(DUST_E_NOMEM);
   // polynomial tables
   std::vector<int> pw;
   mono_powers(h->deg, pw);
@@ -1083,6 +1084,7 @@ static void fmm_destroy(FmmHandle *h) {
   h->tmp.release();
   h->flags.release();
   h->p2p_rng.release();
+  h->rb.release();
   for (auto e : h->timer.pool) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -1115,8 +1117,9 @@ static int sort_phase_a(FmmHandle *h) {
   TreeGeom G = make_geom(h);
   const int TB = 256;
   timer_begin(h, DUST_FMM_STAGE_SORT);
-  h->hb_init_err = 0x7fffffff;
-  DUST_HIP(hipMemcpyAsync(h->d_err.p, &h->hb_init_err, sizeof(int), hipMemcpyHostToDevice,
+  FmmReadback *rb = h->rb.p;
+  rb->init_err = 0x7fffffff;
+  DUST_HIP(hipMemcpyAsync(h->d_err.p, &rb->init_err, sizeof(int), hipMemcpyHostToDevice,
                           h->stream));
   if (n > 0) {
     k_keys<<<ceil_div(n, TB), TB, 0, h->stream>>>(G, n, h->cen_in.p, h->key.p, h->idx_in.p,
@@ -1139,9 +1142,9 @@ static int sort_phase_a(FmmHandle *h) {
                                                                         h->prm.min_part, h->flags.p);
   DUST_TRY(prim_select_flagged_cells(h, 0, h->active.p, h->counts.p));
   DUST_TRY(prim_select_flagged_cells(h, 1, h->leaves.p, h->counts.p + 1));
-  DUST_HIP(hipMemcpyAsync(h->hb_counts, h->counts.p, sizeof(int) * 2, hipMemcpyDeviceToHost,
+  DUST_HIP(hipMemcpyAsync(rb->counts, h->counts.p, sizeof(int) * 2, hipMemcpyDeviceToHost,
                           h->stream));
-  DUST_HIP(hipMemcpyAsync(&h->hb_err, h->d_err.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  DUST_HIP(hipMemcpyAsync(&rb->err, h->d_err.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   return 0;
 }
 
@@ -1149,15 +1152,16 @@ static int sort_phase_b(FmmHandle *h) {
   const int n = h->n;
   TreeGeom G = make_geom(h);
   const int TB = 256;
-  if (h->hb_err != h->hb_init_err) {
-    set_error("particle %d resulted outside the octree box", h->hb_err);
+  FmmReadback *rb = h->rb.p;
+  if (rb->err != rb->init_err) {
+    set_error("particle %d resulted outside the octree box", rb->err);
     timer_end(h, DUST_FMM_STAGE_SORT);
     return DUST_E_OUTSIDE;
   }
-  h->n_active = h->hb_counts[0];
-  h->n_leaves = h->hb_counts[1];
+  h->n_active = rb->counts[0];
+  h->n_leaves = rb->counts[1];
   k_level_starts<<<1, 64, 0, h->stream>>>(G, h->n_active, h->active.p, h->counts.p + 2);
-  DUST_HIP(hipMemcpyAsync(h->act_start, h->counts.p + 2, sizeof(int) * (h->nlev + 1),
+  DUST_HIP(hipMemcpyAsync(rb->act_start, h->counts.p + 2, sizeof(int) * (h->nlev + 1),
                           hipMemcpyDeviceToHost, h->stream));
   k_fill_int<<<ceil_div(h->ncells_tot, TB), TB, 0, h->stream>>>(h->ncells_tot, h->slot.p, -1);
   if (h->n_active > 0)
@@ -1184,15 +1188,17 @@ static int sort_phase_b(FmmHandle *h) {
   DUST_TRY(h->mp_a.reserve(na * 3 * h->nmon));
   DUST_TRY(h->mp_b.reserve(na * 3 * h->nmon));
   DUST_TRY(h->mp_c.reserve(na * 9 * h->nmon));
-  DUST_HIP(hipMemcpyAsync(h->hb_units, h->n_units.p, 2 * sizeof(int), hipMemcpyDeviceToHost,
+  DUST_HIP(hipMemcpyAsync(rb->units, h->n_units.p, 2 * sizeof(int), hipMemcpyDeviceToHost,
                           h->stream));
   return 0;
 }
 
 static int sort_phase_c(FmmHandle *h) {
   const int TB = 256;
-  h->n_unit = h->hb_units[0];
-  h->n_item = h->hb_units[1];
+  const FmmReadback *rb = h->rb.p;
+  h->n_unit = rb->units[0];
+  h->n_item = rb->units[1];
+  std::memcpy(h->act_start, rb->act_start, sizeof(int) * (h->nlev + 1));
   DUST_TRY(h->items.reserve((size_t)std::max(h->n_item, 1) * 4));
   DUST_TRY(h->part.reserve((size_t)std::max(h->n_item, 1) * kWave * 6));
   if (h->n_unit > 0)

@@ -38,6 +38,14 @@ struct StageTimer {
   int launches[DUST_FMM_NSTAGES] = {0};
 };
 
+// Host side of the small copies between the phases of a pass.
+struct FmmReadback {
+  int init_err, err;
+  int counts[2];                  // active cells, leaves
+  int units[2];                   // evaluation units, items
+  int act_start[kMaxLevels + 1];  // first active cell of every level
+};
+
 struct FmmHandle {
   dust_fmm_params prm{};
   int cap = 0, n = 0;
@@ -94,10 +102,8 @@ struct FmmHandle {
   hipStream_t stream = nullptr;
   StageTimer timer;
   bool sorted = false, computed = false;
-  // host targets of the asynchronous read-backs between pass phases
-  int hb_init_err = 0, hb_err = 0;
-  int hb_counts[2] = {0, 0};   // active cells, leaves
-  int hb_units[2] = {0, 0};    // evaluation units, items
+  // page-locked source/targets of the asynchronous copies between pass phases
+  HostBuf<FmmReadback> rb;
 };
 
 // fmm_prim.hip: device-wide primitives (radix sort, selection).

@@ -43,8 +43,8 @@ struct TunedState {
   DevBuf<int> offs;         // [8][189][3] offsets of each octant
   DevBuf<float> W;          // M2L operators of every level and offset, fragment order
   DevBuf<unsigned char> jobs;  // M2L job list of the current pass
-  std::vector<M2LJob> host_jobs;
-  int hcount[kMaxLevels * 8];
+  HostBuf<M2LJob> host_jobs;   // page-locked source of the job upload
+  HostBuf<int> hcount;          // [kMaxLevels * 8] group sizes (page-locked)
   long gbase[kMaxLevels * 8];  // first entry of every group in grp
   double near_factor = 1.0;
 };
@@ -77,6 +77,7 @@ int tuned_prepare(FmmHandle *h) {
   DUST_TRY(t->offs.reserve(offs.size()));
   DUST_HIP(hipMemcpy(t->offs.p, offs.data(), offs.size() * sizeof(int), hipMemcpyHostToDevice));
   DUST_TRY(t->gcount.reserve(kMaxLevels * 8));
+  DUST_TRY(t->hcount.reserve(kMaxLevels * 8));
   return build_w(h);
 }
 
@@ -103,6 +104,8 @@ void tuned_release(FmmHandle *h) {
   t->offs.release();
   t->W.release();
   t->jobs.release();
+  t->host_jobs.release();
+  t->hcount.release();
   delete t;
   h->tuned = nullptr;
 }
@@ -536,7 +539,7 @@ int tuned_pass_prepare(FmmHandle *h) {
         G, l, h->active.p, h->act_start[l], na, h->slot.p, h->flags.p, t->offs.p, t->grp.p,
         t->src.p, t->gbase[l * 8], na, t->gcount.p);
   }
-  DUST_HIP(hipMemcpyAsync(t->hcount, t->gcount.p, sizeof(int) * kMaxLevels * 8,
+  DUST_HIP(hipMemcpyAsync(t->hcount.p, t->gcount.p, sizeof(int) * kMaxLevels * 8,
                           hipMemcpyDeviceToHost, h->stream));
   // FP32 copy of the multipoles (P2M and M2M are complete at this point)
   long na3 = (long)h->n_active * 3 * h->nmon;
@@ -564,7 +567,7 @@ int tuned_m2l_all(FmmHandle *h) {
   std::vector<Chunk> chunks;
   for (int lvl = 1; lvl < h->nlev; ++lvl)
     for (int g = 0; g < 8; ++g) {
-      int nt = t->hcount[lvl * 8 + g];
+      int nt = t->hcount.p[lvl * 8 + g];
       if (nt == 0) continue;
       for (int j0 = 0; j0 < kOct; j0 += kJC) chunks.push_back({lvl, g, j0, ceil_div(nt, kNT) * nrb});
     }
@@ -575,7 +578,7 @@ int tuned_m2l_all(FmmHandle *h) {
   for (const Chunk &c : chunks) {
     int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     load[x] += c.njobs;
-    const int nt = t->hcount[c.lvl * 8 + c.g];
+    const int nt = t->hcount.p[c.lvl * 8 + c.g];
     for (int t0 = 0; t0 < nt; t0 += kNT)
       for (int rb = 0; rb < nrb; ++rb) {
         M2LJob J;
@@ -594,13 +597,15 @@ int tuned_m2l_all(FmmHandle *h) {
   empty.nt = 0;  // t0 >= nt: the job does nothing
   // the host list stays in the tuned state until the next pass (whose first
   // phase synchronises the stream), so the asynchronous upload needs no wait
-  std::vector<M2LJob> &jobs = t->host_jobs;
-  jobs.assign(qmax * kXcd, empty);
+  const size_t njob = qmax * kXcd;
+  DUST_TRY(t->host_jobs.reserve(njob));
+  M2LJob *jobs = t->host_jobs.p;
+  std::fill(jobs, jobs + njob, empty);
   for (int x = 0; x < kXcd; ++x)
     for (size_t k = 0; k < queue[x].size(); ++k) jobs[k * kXcd + x] = queue[x][k];
-  DUST_TRY(t->jobs.reserve(jobs.size() * sizeof(M2LJob)));
-  DUST_HIP(hipMemcpyAsync(t->jobs.p, jobs.data(), jobs.size() * sizeof(M2LJob),
-                          hipMemcpyHostToDevice, h->stream));
+  DUST_TRY(t->jobs.reserve(njob * sizeof(M2LJob)));
+  DUST_HIP(hipMemcpyAsync(t->jobs.p, jobs, njob * sizeof(M2LJob), hipMemcpyHostToDevice,
+                          h->stream));
   M2LArgs P;
   P.w = w;
   P.nmon = h->nmon;
@@ -612,7 +617,7 @@ int tuned_m2l_all(FmmHandle *h) {
   P.a32 = t->a32.p;
   P.mp_b = h->mp_b.p;
   P.mp_c = h->mp_c.p;
-  k_m2l_mfma<<<(unsigned)jobs.size(), 256, lds, h->stream>>>(P);
+  k_m2l_mfma<<<(unsigned)njob, 256, lds, h->stream>>>(P);
   DUST_HIP(hipGetLastError());
   return 0;
 }

@@ -61,6 +61,19 @@ __device__ inline bool grid_sync_arrive(unsigned *arr, unsigned *err, int G, uns
   return ok != 0;
 }
 
+// Workgroup `bid` of a batched launch belongs to the item i with
+// starts[i] <= bid < starts[i + 1] (starts ascending, starts[0] = 0; items
+// own at least one workgroup).
+__device__ inline int batch_item(const int *__restrict__ starts, int m, int bid) {
+  int lo = 0, hi = m;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (starts[mid] <= bid) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // Block substitution with 128-row blocks for up to 8 right-hand sides held as
 // y[row][8], one workgroup per block row in dependency order (forward: L unit
 // lower from the off-diagonal blocks of A; backward: U).  Every block row
@@ -82,14 +95,15 @@ struct BlockSolveArgs {
   unsigned *err;
 };
 
+// Block rows bid, bid + G, ... of one system (G workgroups share it).
 template <typename T, int UPPER, int NT>
-__global__ void __launch_bounds__(NT) k_block_solve(BlockSolveArgs<T> a) {
+__device__ __forceinline__ void block_solve_rows(const BlockSolveArgs<T> &a, int bid, int G) {
   constexpr int BS = 128, NQ = NT / BS, CW = BS / NQ, R8 = 8;
   __shared__ T sy[BS * R8];
   __shared__ T sp[NQ * BS * R8];
   __shared__ int ok;
   const int t = threadIdx.x, r = t & (BS - 1), h = t / BS;
-  for (int it = blockIdx.x; it < a.nblk; it += gridDim.x) {
+  for (int it = bid; it < a.nblk; it += G) {
     const int kb = UPPER ? a.nblk - 1 - it : it;
     const int gr = kb * BS + r;
     const bool rok = gr < a.n;
@@ -161,6 +175,24 @@ __global__ void __launch_bounds__(NT) k_block_solve(BlockSolveArgs<T> a) {
     __syncthreads();
     if (t == 0) __hip_atomic_store(&a.ready[kb], a.gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+
+template <typename T, int UPPER, int NT>
+__global__ void __launch_bounds__(NT) k_block_solve(BlockSolveArgs<T> a) {
+  block_solve_rows<T, UPPER, NT>(a, blockIdx.x, gridDim.x);
+}
+
+// Independent systems in one launch: items[i] owns workgroups
+// [starts[i], starts[i+1]).  A workgroup waits only on workgroups of its own
+// system; with workgroups dispatched in index order, every system whose
+// first workgroup is resident has all its workgroups resident once the
+// systems before it have drained, so the waits end.
+template <typename T, int UPPER, int NT>
+__global__ void __launch_bounds__(NT) k_block_solve_batch(const BlockSolveArgs<T> *__restrict__ items,
+                                                          const int *__restrict__ starts, int m) {
+  const int i = batch_item(starts, m, blockIdx.x);
+  const BlockSolveArgs<T> a = items[i];
+  block_solve_rows<T, UPPER, NT>(a, blockIdx.x - starts[i], starts[i + 1] - starts[i]);
 }
 
 // xGETRF semantics on device memory; *info as LAPACK.  nb is the panel width.

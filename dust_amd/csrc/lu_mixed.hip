@@ -10,7 +10,7 @@
 //     workgroup in FP64 with row interchanges among its own 128 rows
 //     (pivot = largest magnitude in the block column part).  The multipliers
 //     of the rows below, L21 = A21 U11^-1, are then checked against a
-//     threshold tau: max |l_ij| <= tau means the block pivot dominates the
+//     threshold tau = TAU: max |l_ij| <= tau means the block pivot dominates the
 //     whole column to within tau (threshold partial pivoting).  If any block
 //     fails the check, or a pivot is zero, the solve is redone by the plain
 //     FP64 partial-pivoting path.
@@ -46,6 +46,11 @@ constexpr int SP = 32;          // sub-panel / inverted sub-block
 constexpr int LDD = NB + 1;     // LDS leading dimension of the FP64 diagonal block
 constexpr int DEPTH = 4;        // block columns per trailing update
 constexpr int NR = 8;           // right-hand sides per solve pass
+// Multiplier bound of the threshold pivoting: |l_ij| <= 10, the threshold
+// u = 0.1 of sparse direct solvers (Duff, Erisman & Reid, "Direct Methods for
+// Sparse Matrices", ch. 7).  Rotor AICs with thin blades reach |l| ~ 2.6
+// below a diagonal block; refinement corrects the extra growth.
+constexpr double TAU = 10.0;
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -223,7 +228,10 @@ __device__ __forceinline__ void diag_subpanel(double *sD, int c0, int lane, int 
   }
 }
 
-__global__ void __launch_bounds__(256) k_mx_diag(DiagArgs a) {
+// BATCH: workgroup b factors the block of items[b] (one block per system).
+template <int BATCH>
+__global__ void __launch_bounds__(256) k_mx_diag(DiagArgs one, const DiagArgs *__restrict__ items) {
+  const DiagArgs a = BATCH ? items[blockIdx.x] : one;
   extern __shared__ double sD[];  // NB x LDD, column-major
   __shared__ int spiv[NB];
   __shared__ int sperm[NB];
@@ -349,13 +357,31 @@ __global__ void __launch_bounds__(256) k_mx_diag(DiagArgs a) {
   }
 }
 
-// Interchanges of diagonal block kb on every column outside the block.
-__global__ void k_mx_laswp(float *W, long ld, int np, int k, const int *__restrict__ piv,
-                          const int *__restrict__ npiv) {
-  if (npiv[k / NB] == 0) return;
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= np || (c >= k && c < k + NB)) return;
-  float *col = W + ld * c;
+// Interchanges of the diagonal block at row k on every column outside the
+// block.  BATCH: items[i] owns workgroups [starts[i], starts[i+1]).
+struct LaswpItem {
+  float *W;
+  long ld;
+  int np, k;
+  const int *piv, *npiv;
+};
+
+template <int BATCH>
+__global__ void k_mx_laswp(LaswpItem one, const LaswpItem *__restrict__ items,
+                           const int *__restrict__ starts, int m) {
+  int bx = blockIdx.x;
+  LaswpItem a = one;
+  if (BATCH) {
+    const int i = batch_item(starts, m, bx);
+    a = items[i];
+    bx -= starts[i];
+  }
+  const int k = a.k;
+  if (a.npiv[k / NB] == 0) return;
+  int c = bx * blockDim.x + threadIdx.x;
+  if (c >= a.np || (c >= k && c < k + NB)) return;
+  float *col = a.W + a.ld * c;
+  const int *piv = a.piv;
   for (int i = k; i < k + NB; ++i) {
     int p = piv[i];
     if (p != i) {
@@ -427,19 +453,32 @@ __device__ inline void frag_store_U(void *Uf, int f16, int ct, int lane, const f
   }
 }
 
-__global__ void __launch_bounds__(256) k_mx_trsm(TrsmArgs a) {
+// BATCH: items[i] owns workgroups [starts[i], starts[i+1]), numbered as the
+// single launch's grid: 2 nl tiles (mode 0) or 4 x nblk (mode 1, x fastest).
+template <int BATCH>
+__global__ void __launch_bounds__(256) k_mx_trsm(TrsmArgs one, const TrsmArgs *__restrict__ items,
+                                                 const int *__restrict__ starts, int m) {
   extern __shared__ float sm[];
+  int bx = blockIdx.x, by = blockIdx.y;
+  TrsmArgs a = one;
+  if (BATCH) {
+    const int i = batch_item(starts, m, bx);
+    a = items[i];
+    const int loc = bx - starts[i];
+    bx = a.mode ? (loc & 3) : loc;
+    by = a.mode ? (loc >> 2) : 0;
+  }
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int k = a.mode ? (int)blockIdx.y * NB : a.k;
+  const int k = a.mode ? by * NB : a.k;
   const int kb = k / NB;
   const float *Dg = a.W + k + a.ld * k;   // diagonal block
   const float *inv = a.dinv + (long)kb * 8 * SP * SP;
-  if ((int)blockIdx.x < a.nl) {
+  if (bx < a.nl) {
     // ---------------- L tile: rows R0 .. R0+64 ----------------
     float *sX = sm;                   // 128 cols x LDX
     float *sU = sX + NB * LDX;        // 128 x LDT (upper triangle used)
     float *sI = sU + NB * LDT;        // 4 x 32 x 32 (Uinv sub-blocks)
-    const int R0 = (a.mode == 0) ? k + NB + 64 * blockIdx.x : 64 * blockIdx.x;
+    const int R0 = (a.mode == 0) ? k + NB + 64 * bx : 64 * bx;
     for (int q = tid; q < NB * 64; q += 256) {
       int r = q & 63, c = q >> 6;
       float v;
@@ -520,7 +559,7 @@ __global__ void __launch_bounds__(256) k_mx_trsm(TrsmArgs a) {
     float *sY = sm;                   // 64 cols x LDT (rows 0..127)
     float *sL = sY + 64 * LDT;        // 128 x LDT (strict lower triangle used)
     float *sI = sL + NB * LDT;        // 4 x 32 x 32 (Linv sub-blocks)
-    const int ub = blockIdx.x - a.nl;
+    const int ub = bx - a.nl;
     const int C0 = (a.mode == 0) ? k + NB + 64 * ub : 64 * ub;
     for (int q = tid; q < NB * 64; q += 256) {
       int r = q & (NB - 1), c = q >> 7;
@@ -597,12 +636,36 @@ __global__ void __launch_bounds__(256) k_mx_trsm(TrsmArgs a) {
 // Update GEMM: W[r0.., c0..] -= sum_q Lfrag_q Ufrag_q over P block columns.
 // Block tile 128 x 128, four waves of 64 x 64 (4 x 4 MFMA tiles).
 // ---------------------------------------------------------------------------
-template <int F16>
-__global__ void __launch_bounds__(256) k_mx_gemm(float *W, long ld, int r0, int c0, const void *Lf,
-                                                 const void *Uf, long pbytes, int P) {
+struct GemmItem {
+  float *W;
+  long ld;
+  int r0, c0;   // first row / column of the updated region
+  int nrt, P;   // 128-row tiles of the region; block columns summed
+  const void *Lf, *Uf;
+  long pbytes;  // bytes per fragment panel
+};
+
+// BATCH: items[i] owns workgroups [starts[i], starts[i+1]), tile (x, y) =
+// (local % nrt, local / nrt); otherwise the grid is (row tiles, column tiles).
+template <int F16, int BATCH>
+__global__ void __launch_bounds__(256) k_mx_gemm(GemmItem one, const GemmItem *__restrict__ items,
+                                                 const int *__restrict__ starts, int m) {
+  int bx = blockIdx.x, by = blockIdx.y;
+  GemmItem g = one;
+  if (BATCH) {
+    const int i = batch_item(starts, m, bx);
+    g = items[i];
+    const int loc = bx - starts[i];
+    bx = loc % g.nrt;
+    by = loc / g.nrt;
+  }
+  float *W = g.W;
+  const long ld = g.ld, pbytes = g.pbytes;
+  const void *Lf = g.Lf, *Uf = g.Uf;
+  const int P = g.P;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int rt0 = (r0 + 128 * blockIdx.x + 64 * (wv & 1)) / 16;
-  const int ct0 = (c0 + 128 * blockIdx.y + 64 * (wv >> 1)) / 16;
+  const int rt0 = (g.r0 + 128 * bx + 64 * (wv & 1)) / 16;
+  const int ct0 = (g.c0 + 128 * by + 64 * (wv >> 1)) / 16;
   f4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -765,7 +828,7 @@ __global__ void __launch_bounds__(256) k_mx_resid_fin(int n, int nch, const doub
 }
 
 __global__ void k_mx_zero_u64(unsigned long long *p, int n) {
-  int i = threadIdx.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
 }
 
@@ -806,20 +869,18 @@ struct MixedWs {
   DevBuf<unsigned long long> norms;
   DevBuf<int> ipiv64;
   unsigned gen = 0;
-  // host targets of the asynchronous read-backs
-  unsigned h_flags[2] = {0, 0};
-  unsigned long long h_norms[2 * NR + 1] = {};
-  unsigned h_err = 0;
+  // page-locked targets of the asynchronous read-backs
+  struct Readback {
+    unsigned flags[2];                    // max |l| bits, zero pivot
+    unsigned long long norms[2 * NR + 1];  // residual and solution norms, ||A||
+    unsigned err;                         // triangular-solve time-out
+  };
+  HostBuf<Readback> hb;
   // second stream and events of the lookahead (default priority)
   hipStream_t s2 = nullptr;
   hipEvent_t ev_start = nullptr, ev_panel = nullptr, ev_rest = nullptr;
 };
 MixedWs g_ws;
-
-struct BatchSlot {
-  MixedWs ws;
-  hipStream_t stream = nullptr;
-};
 
 int mx_cus() {
   int dev = 0;
@@ -849,8 +910,9 @@ int mx_gemm(int f16, float *W, long ld, int r0, int r1, int c0, int c1, const vo
             const void *Uf, long pbytes, int P, hipStream_t st) {
   if (r1 <= r0 || c1 <= c0 || P <= 0) return 0;
   dim3 grid((r1 - r0) / 128, (c1 - c0) / 128);
-  if (f16) k_mx_gemm<1><<<grid, 256, 0, st>>>(W, ld, r0, c0, Lf, Uf, pbytes, P);
-  else k_mx_gemm<0><<<grid, 256, 0, st>>>(W, ld, r0, c0, Lf, Uf, pbytes, P);
+  GemmItem g{W, ld, r0, c0, (r1 - r0) / 128, P, Lf, Uf, pbytes};
+  if (f16) k_mx_gemm<1, 0><<<grid, 256, 0, st>>>(g, nullptr, nullptr, 0);
+  else k_mx_gemm<0, 0><<<grid, 256, 0, st>>>(g, nullptr, nullptr, 0);
   MX_LAUNCH_CHECK("update gemm");
   return 0;
 }
@@ -884,15 +946,16 @@ int mx_factor_launch(MixedWs &w, int np, int f16, hipStream_t st) {
         DUST_TRY(mx_gemm(f16, w.W.p, ld, k, k + NB, k + NB, np, Lf, Uf, pbytes, a, st));
       }
       DiagArgs da{w.W.p, ld, k, w.piv.p, w.perm.p, w.npiv.p, w.dinv.p, w.flags.p};
-      k_mx_diag<<<1, 256, lds_diag(), st>>>(da);
+      k_mx_diag<0><<<1, 256, lds_diag(), st>>>(da, nullptr);
       MX_LAUNCH_CHECK("diagonal block");
-      k_mx_laswp<<<ceil_div(np, 256), 256, 0, st>>>(w.W.p, ld, np, k, w.piv.p, w.npiv.p);
+      LaswpItem li{w.W.p, ld, np, k, w.piv.p, w.npiv.p};
+      k_mx_laswp<0><<<ceil_div(np, 256), 256, 0, st>>>(li, nullptr, nullptr, 0);
       MX_LAUNCH_CHECK("laswp");
       const int nt = (np - k - NB) / 64;
       if (nt > 0) {
         TrsmArgs ta{w.W.p, ld, k, np, nt, f16, 0, w.dinv.p, Lf + (size_t)a * pbytes,
                     Uf + (size_t)a * pbytes, w.flags.p, nullptr, nullptr};
-        k_mx_trsm<<<2 * nt, 256, lds_trsm(), st>>>(ta);
+        k_mx_trsm<0><<<2 * nt, 256, lds_trsm(), st>>>(ta, nullptr, nullptr, 0);
         MX_LAUNCH_CHECK("block triangular solve");
       }
     }
@@ -917,10 +980,10 @@ int mx_factor_launch(MixedWs &w, int np, int f16, hipStream_t st) {
   if (pending) DUST_HIP(hipStreamWaitEvent(st, w.ev_rest, 0));
   // 128x128 inverses of every diagonal block for the refinement solves
   TrsmArgs ia{w.W.p, ld, 0, np, 2, f16, 1, w.dinv.p, nullptr, nullptr, w.flags.p, w.Linv.p, w.Uinv.p};
-  k_mx_trsm<<<dim3(4, nblk), 256, lds_trsm(), st>>>(ia);
+  k_mx_trsm<0><<<dim3(4, nblk), 256, lds_trsm(), st>>>(ia, nullptr, nullptr, 0);
   MX_LAUNCH_CHECK("block inverses");
   // max |l| and the zero-pivot flag, read after the caller's synchronisation
-  DUST_HIP(hipMemcpyAsync(w.h_flags, w.flags.p, sizeof(w.h_flags), hipMemcpyDeviceToHost, st));
+  DUST_HIP(hipMemcpyAsync(w.hb.p->flags, w.flags.p, sizeof(w.hb.p->flags), hipMemcpyDeviceToHost, st));
   return 0;
 }
 
@@ -928,8 +991,8 @@ int mx_factor_launch(MixedWs &w, int np, int f16, hipStream_t st) {
 // zero pivot?
 bool mx_factor_bad(const MixedWs &w, double tau) {
   float lmax;
-  std::memcpy(&lmax, &w.h_flags[0], sizeof(float));
-  return w.h_flags[1] != 0 || !(lmax <= (float)tau);
+  std::memcpy(&lmax, &w.hb.p->flags[0], sizeof(float));
+  return w.hb.p->flags[1] != 0 || !(lmax <= (float)tau);
 }
 
 // y <- U^-1 L^-1 y (y: np x 8 FP32, in place).
@@ -955,18 +1018,25 @@ int mx_solve(MixedWs &w, int np, hipStream_t st, int cus) {
 
 // r = b - A x and the norms (read back asynchronously into the workspace,
 // with the triangular-solve time-out flag).
-int mx_residual_launch(MixedWs &w, const double *dA, long lda, int n, bool with_anorm,
-                       hipStream_t st) {
+// Kernels only: the norms go to `norms` (2 NR + 1 words, zeroed by the caller).
+int mx_residual_kernels(MixedWs &w, const double *dA, long lda, int n, bool with_anorm,
+                        unsigned long long *norms, hipStream_t st) {
   const int cw = 512;
   const int nch = ceil_div(n, cw);
-  k_mx_zero_u64<<<1, 64, 0, st>>>(w.norms.p, 2 * NR + 1);
   dim3 g(ceil_div(n, 256), nch);
   k_mx_resid_part<<<g, 256, 0, st>>>(dA, lda, n, cw, w.x8.p, w.part.p, with_anorm ? w.rowabs.p : nullptr);
   k_mx_resid_fin<<<ceil_div(n, 256), 256, 0, st>>>(n, nch, w.part.p, w.b8.p, w.r8.p, w.x8.p,
-                                                   with_anorm ? w.rowabs.p : nullptr, w.norms.p);
+                                                   with_anorm ? w.rowabs.p : nullptr, norms);
   MX_LAUNCH_CHECK("residual");
-  DUST_HIP(hipMemcpyAsync(w.h_norms, w.norms.p, sizeof(w.h_norms), hipMemcpyDeviceToHost, st));
-  DUST_HIP(hipMemcpyAsync(&w.h_err, w.flags.p + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  return 0;
+}
+
+int mx_residual_launch(MixedWs &w, const double *dA, long lda, int n, bool with_anorm,
+                       hipStream_t st) {
+  k_mx_zero_u64<<<1, 64, 0, st>>>(w.norms.p, 2 * NR + 1);
+  DUST_TRY(mx_residual_kernels(w, dA, lda, n, with_anorm, w.norms.p, st));
+  DUST_HIP(hipMemcpyAsync(w.hb.p->norms, w.norms.p, sizeof(w.hb.p->norms), hipMemcpyDeviceToHost, st));
+  DUST_HIP(hipMemcpyAsync(&w.hb.p->err, w.flags.p + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st));
   return 0;
 }
 
@@ -974,7 +1044,7 @@ int mx_residual_launch(MixedWs &w, const double *dA, long lda, int n, bool with_
 // right-hand sides, max_q ||r_q||_inf / (||A||_inf ||x_q||_inf).
 double mx_residual_berr(const MixedWs &w, int nrhs, bool with_anorm, double *anorm) {
   double nm[2 * NR + 1];
-  std::memcpy(nm, w.h_norms, sizeof(nm));
+  std::memcpy(nm, w.hb.p->norms, sizeof(nm));
   if (with_anorm) *anorm = nm[2 * NR];
   double be = 0.0;
   for (int q = 0; q < nrhs; ++q) {
@@ -1017,6 +1087,7 @@ int mx_reserve(MixedWs &w, int n, int np) {
   DUST_TRY(w.part.reserve((size_t)ceil_div(n, 512) * n * NR));
   DUST_TRY(w.rowabs.reserve((size_t)ceil_div(n, 512) * n));
   DUST_TRY(w.norms.reserve(2 * NR + 1));
+  DUST_TRY(w.hb.reserve(1));
   return 0;
 }
 
@@ -1048,18 +1119,21 @@ int set_attributes() {
   static std::once_flag attr;
   hipError_t ae = hipSuccess;
   std::call_once(attr, [&] {
-    ae = hipFuncSetAttribute((const void *)k_mx_diag, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)lds_diag());
-    if (ae == hipSuccess)
-      ae = hipFuncSetAttribute((const void *)k_mx_trsm, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds_trsm());
+    const void *diag[] = {(const void *)k_mx_diag<0>, (const void *)k_mx_diag<1>};
+    const void *trsm[] = {(const void *)k_mx_trsm<0>, (const void *)k_mx_trsm<1>};
+    for (int b = 0; b < 2 && ae == hipSuccess; ++b) {
+      ae = hipFuncSetAttribute(diag[b], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_diag());
+      if (ae == hipSuccess)
+        ae = hipFuncSetAttribute(trsm[b], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds_trsm());
+    }
   });
   DUST_HIP(ae);
   return 0;
 }
 
-// W = R A and the tuned factorisation (launches only).
-int run_factor_launch(MixedRun &r) {
+// W = R A (launches only).
+int run_convert_launch(MixedRun &r) {
   MixedWs &w = *r.w;
   const int f16 = r.prec == DUST_LU_FP16S;
   const int n = r.n, np = r.np;
@@ -1073,7 +1147,13 @@ int run_factor_launch(MixedRun &r) {
   k_mx_convert<<<dim3(np / 256 + (np % 256 != 0), np / 4), 256, 0, r.st>>>(r.dA, r.lda, n, np,
                                                                          w.rs.p, w.W.p);
   MX_LAUNCH_CHECK("convert");
-  return mx_factor_launch(w, np, f16, r.st);
+  return 0;
+}
+
+// W = R A and the tuned factorisation (launches only).
+int run_factor_launch(MixedRun &r) {
+  DUST_TRY(run_convert_launch(r));
+  return mx_factor_launch(*r.w, r.np, r.prec == DUST_LU_FP16S, r.st);
 }
 
 // FP64 partial-pivoting factors (fallback and DUST_LU_FP64); synchronous.
@@ -1107,23 +1187,34 @@ int run_chunk_begin(MixedRun &r, int q0) {
 }
 
 // One refinement sweep (launches): d = U^-1 L^-1 P R r (r = b first), x += d,
-// r = b - A x with its norms.
-int run_sweep_launch(MixedRun &r) {
+// r = b - A x with its norms.  Split around the triangular solves so that a
+// batch can run the solves of all its systems in one launch.
+int run_sweep_gather(MixedRun &r) {
   MixedWs &w = *r.w;
   ++r.nit;
   k_mx_gather<<<ceil_div(r.np, 256), 256, 0, r.st>>>(r.nit == 1 ? w.b8.p : w.r8.p, r.n, r.n, r.np,
                                                      r.nq, w.perm.p, w.rs.p, w.y.p);
   MX_LAUNCH_CHECK("gather");
-  DUST_TRY(mx_solve(w, r.np, r.st, mx_cus()));
-  k_mx_update_x<<<ceil_div(r.n, 256), 256, 0, r.st>>>(w.x8.p, r.n, r.n, w.y.p, r.nit > 1);
+  return 0;
+}
+
+int run_sweep_update(MixedRun &r) {
+  k_mx_update_x<<<ceil_div(r.n, 256), 256, 0, r.st>>>(r.w->x8.p, r.n, r.n, r.w->y.p, r.nit > 1);
   MX_LAUNCH_CHECK("update");
-  return mx_residual_launch(w, r.dA, r.lda, r.n, r.nit == 1, r.st);
+  return 0;
+}
+
+int run_sweep_launch(MixedRun &r) {
+  DUST_TRY(run_sweep_gather(r));
+  DUST_TRY(mx_solve(*r.w, r.np, r.st, mx_cus()));
+  DUST_TRY(run_sweep_update(r));
+  return mx_residual_launch(*r.w, r.dA, r.lda, r.n, r.nit == 1, r.st);
 }
 
 // After the stream has drained: convergence test of the sweep.
 int run_sweep_finish(MixedRun &r) {
   MixedWs &w = *r.w;
-  DUST_CHECK(w.h_err == 0, DUST_E_HIP, "triangular solve flag wait timed out");
+  DUST_CHECK(w.hb.p->err == 0, DUST_E_HIP, "triangular solve flag wait timed out");
   r.berr = mx_residual_berr(w, r.nq, r.nit == 1, &r.anorm);
   if (r.berr <= r.tol) {
     r.refining = false;
@@ -1200,7 +1291,7 @@ int mixed_dev(MixedWs &w, int n, int nrhs, const double *dA, long lda, double *d
   if (prec != DUST_LU_FP64) {
     DUST_TRY(run_factor_launch(r));
     DUST_HIP(hipStreamSynchronize(st));
-    r.factored = !mx_factor_bad(w, 2.0);
+    r.factored = !mx_factor_bad(w, TAU);
   }
   for (int q0 = 0; q0 < nrhs; q0 += NR) {
     DUST_TRY(run_chunk_begin(r, q0));
@@ -1215,6 +1306,280 @@ int mixed_dev(MixedWs &w, int n, int nrhs, const double *dA, long lda, double *d
   DUST_HIP(hipStreamSynchronize(st));
   if (iters) *iters = r.it_report;
   if (resid) *resid = r.resid_report;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batches of independent systems of different sizes, on one stream.  The
+// factorisation advances every system by one block column per step, and each
+// step is one launch per kernel (panel update, diagonal blocks, interchanges,
+// triangular solves, trailing update) whose workgroups are dealt out to the
+// systems that have that block column: a region table with a prefix sum of
+// workgroup counts, searched by each workgroup.  The refinement solves of all
+// systems are one launch per triangle.  Launch count: ~5 per block column of
+// the largest system instead of ~5 per block column of every system, and the
+// latency-bound diagonal-block chains of all systems run side by side.
+// ---------------------------------------------------------------------------
+enum { BK_GEMM16, BK_GEMM32, BK_DIAG, BK_LASWP, BK_TRSM, BK_SOLVE_L, BK_SOLVE_U };
+
+struct BatchLaunch {
+  int kind, m, blocks;
+  size_t items, starts;  // byte offsets in the plan
+};
+
+struct BatchPlan {
+  std::vector<char> blob;
+  std::vector<BatchLaunch> launches;
+  size_t append(const void *p, size_t bytes) {
+    size_t off = (blob.size() + 15) & ~(size_t)15;
+    blob.resize(off + bytes);
+    std::memcpy(blob.data() + off, p, bytes);
+    return off;
+  }
+  // One launch over `items`; item i gets blocks[i] workgroups (items with
+  // none are dropped).
+  template <class T>
+  void add(int kind, const std::vector<T> &items, const std::vector<int> &blocks) {
+    std::vector<T> it;
+    std::vector<int> st(1, 0);
+    for (size_t i = 0; i < items.size(); ++i)
+      if (blocks[i] > 0) {
+        it.push_back(items[i]);
+        st.push_back(st.back() + blocks[i]);
+      }
+    if (it.empty()) return;
+    BatchLaunch L;
+    L.kind = kind;
+    L.m = (int)it.size();
+    L.blocks = st.back();
+    L.items = append(it.data(), it.size() * sizeof(T));
+    L.starts = append(st.data(), st.size() * sizeof(int));
+    launches.push_back(L);
+  }
+};
+
+constexpr int NW = 2 * NR + 1;  // norm words per system
+
+struct BatchState {
+  std::vector<std::unique_ptr<MixedWs>> ws;
+  hipStream_t st = nullptr;
+  DevBuf<char> dplan;
+  HostBuf<char> hplan;
+  DevBuf<unsigned> flags;            // [3 i ..]: max |l|, zero pivot, solve time-out
+  DevBuf<unsigned long long> norms;  // [NW i ..]
+  HostBuf<unsigned> hflags;
+  HostBuf<unsigned long long> hnorms;
+};
+BatchState g_batch;
+
+// Upload the plan (page-locked staging; the previous plan's copy has been
+// waited for by the caller's last synchronisation) and launch it.
+int plan_run(BatchState &B, const BatchPlan &P, hipStream_t st) {
+  if (P.launches.empty()) return 0;
+  DUST_TRY(B.hplan.reserve(P.blob.size()));
+  DUST_TRY(B.dplan.reserve(P.blob.size()));
+  std::memcpy(B.hplan.p, P.blob.data(), P.blob.size());
+  DUST_HIP(hipMemcpyAsync(B.dplan.p, B.hplan.p, P.blob.size(), hipMemcpyHostToDevice, st));
+  const char *base = B.dplan.p;
+  for (const BatchLaunch &L : P.launches) {
+    const int *starts = (const int *)(base + L.starts);
+    const void *items = base + L.items;
+    switch (L.kind) {
+      case BK_GEMM16:
+        k_mx_gemm<1, 1><<<L.blocks, 256, 0, st>>>(GemmItem{}, (const GemmItem *)items, starts, L.m);
+        break;
+      case BK_GEMM32:
+        k_mx_gemm<0, 1><<<L.blocks, 256, 0, st>>>(GemmItem{}, (const GemmItem *)items, starts, L.m);
+        break;
+      case BK_DIAG:  // one workgroup per item
+        k_mx_diag<1><<<L.blocks, 256, lds_diag(), st>>>(DiagArgs{}, (const DiagArgs *)items);
+        break;
+      case BK_LASWP:
+        k_mx_laswp<1><<<L.blocks, 256, 0, st>>>(LaswpItem{}, (const LaswpItem *)items, starts, L.m);
+        break;
+      case BK_TRSM:
+        k_mx_trsm<1><<<L.blocks, 256, lds_trsm(), st>>>(TrsmArgs{}, (const TrsmArgs *)items, starts,
+                                                       L.m);
+        break;
+      case BK_SOLVE_L:
+        k_block_solve_batch<float, 0, 256><<<L.blocks, 256, 0, st>>>(
+            (const BlockSolveArgs<float> *)items, starts, L.m);
+        break;
+      case BK_SOLVE_U:
+        k_block_solve_batch<float, 1, 256><<<L.blocks, 256, 0, st>>>(
+            (const BlockSolveArgs<float> *)items, starts, L.m);
+        break;
+    }
+    MX_LAUNCH_CHECK("batched launch");
+  }
+  return 0;
+}
+
+// Factorisation of every system of the batch (W already converted): the
+// steps of mx_factor_launch without the lookahead, one fragment set.
+void batch_factor_plan(std::vector<MixedRun> &runs, BatchState &B, int f16, BatchPlan &P) {
+  int maxblk = 0;
+  for (const MixedRun &r : runs) maxblk = std::max(maxblk, r.np / NB);
+  const int gk = f16 ? BK_GEMM16 : BK_GEMM32;
+  const size_t esz = f16 ? sizeof(_Float16) : sizeof(float);
+  for (int kk = 0; kk < maxblk; ++kk) {
+    const int k = kk * NB, a = kk % DEPTH;
+    std::vector<GemmItem> pg, tg;
+    std::vector<DiagArgs> dg;
+    std::vector<LaswpItem> lg;
+    std::vector<TrsmArgs> sg;
+    std::vector<int> pb, tb, db, lb, sb;
+    for (size_t i = 0; i < runs.size(); ++i) {
+      MixedRun &r = runs[i];
+      MixedWs &w = *r.w;
+      const int np = r.np;
+      if (r.n == 0 || kk >= np / NB) continue;
+      const long ld = np;
+      const long pbytes = (long)np * NB * (long)esz;
+      char *Lf = w.Lf.p, *Uf = w.Uf.p;
+      unsigned *fl = B.flags.p + 3 * i;
+      if (a > 0) {
+        // block column and block row of this step, updated by the sub-panels
+        // before it in the super-panel
+        pg.push_back(GemmItem{w.W.p, ld, k, k, (np - k) / NB, a, Lf, Uf, pbytes});
+        pb.push_back((np - k) / NB);
+        pg.push_back(GemmItem{w.W.p, ld, k, k + NB, 1, a, Lf, Uf, pbytes});
+        pb.push_back((np - k - NB) / NB);
+      }
+      dg.push_back(DiagArgs{w.W.p, ld, k, w.piv.p, w.perm.p, w.npiv.p, w.dinv.p, fl});
+      db.push_back(1);
+      lg.push_back(LaswpItem{w.W.p, ld, np, k, w.piv.p, w.npiv.p});
+      lb.push_back(ceil_div(np, 256));
+      const int nt = (np - k - NB) / 64;
+      sg.push_back(TrsmArgs{w.W.p, ld, k, np, nt, f16, 0, w.dinv.p, Lf + (size_t)a * pbytes,
+                            Uf + (size_t)a * pbytes, fl, nullptr, nullptr});
+      sb.push_back(2 * nt);
+      if (a == DEPTH - 1 && k + NB < np) {
+        const int t0 = k + NB, nrt = (np - t0) / NB;
+        tg.push_back(GemmItem{w.W.p, ld, t0, t0, nrt, DEPTH, Lf, Uf, pbytes});
+        tb.push_back(nrt * nrt);
+      }
+    }
+    P.add(gk, pg, pb);
+    P.add(BK_DIAG, dg, db);
+    P.add(BK_LASWP, lg, lb);
+    P.add(BK_TRSM, sg, sb);
+    P.add(gk, tg, tb);
+  }
+  std::vector<TrsmArgs> ig;
+  std::vector<int> ib;
+  for (size_t i = 0; i < runs.size(); ++i) {
+    MixedRun &r = runs[i];
+    MixedWs &w = *r.w;
+    ig.push_back(TrsmArgs{w.W.p, (long)r.np, 0, r.np, 2, f16, 1, w.dinv.p, nullptr, nullptr,
+                          B.flags.p + 3 * i, w.Linv.p, w.Uinv.p});
+    ib.push_back(r.n > 0 ? 4 * (r.np / NB) : 0);
+  }
+  P.add(BK_TRSM, ig, ib);
+}
+
+// Both triangular solves of the systems in `act`, in one plan.  Workgroups
+// per system in proportion to its block rows, about one per CU in total.
+void batch_solve_plan(std::vector<MixedRun> &runs, const std::vector<int> &act, BatchState &B,
+                      int cus, BatchPlan &P) {
+  long tot = 0;
+  for (int i : act) tot += runs[i].np / NB;
+  const long budget = std::max<long>(cus, (long)act.size());
+  for (int upper = 0; upper < 2; ++upper) {
+    std::vector<BlockSolveArgs<float>> items;
+    std::vector<int> blocks;
+    for (int i : act) {
+      MixedWs &w = *runs[i].w;
+      const int np = runs[i].np, nblk = np / NB;
+      if (++w.gen == 0) ++w.gen;  // generation 0 is the cleared state
+      items.push_back(BlockSolveArgs<float>{w.W.p, (long)np, np, nblk,
+                                            upper ? w.Uinv.p : w.Linv.p, w.y.p,
+                                            w.flags.p + 3 + upper * nblk, w.gen,
+                                            B.flags.p + 3 * i + 2});
+      blocks.push_back((int)std::max(1L, std::min<long>(nblk, budget * nblk / std::max(tot, 1L))));
+    }
+    P.add(upper ? BK_SOLVE_U : BK_SOLVE_L, items, blocks);
+  }
+}
+
+int mixed_batch(int nb, const int *n, const double *const *dA, const int *lda, double *const *dB,
+                const int *ldb, int nrhs, int prec, double tol, int max_iter, int *iters,
+                double *resid) {
+  BatchState &B = g_batch;
+  if (!B.st) DUST_HIP(hipStreamCreateWithFlags(&B.st, hipStreamNonBlocking));
+  while ((int)B.ws.size() < nb) B.ws.emplace_back(new MixedWs());
+  hipStream_t st = B.st;
+  std::vector<MixedRun> runs(nb);
+  for (int i = 0; i < nb; ++i)
+    DUST_TRY(run_init(runs[i], *B.ws[i], n[i], nrhs, dA[i], lda[i], dB[i], ldb[i], prec, tol,
+                      max_iter, st));
+  DUST_TRY(B.flags.reserve((size_t)3 * nb));
+  DUST_TRY(B.norms.reserve((size_t)NW * nb));
+  DUST_TRY(B.hflags.reserve((size_t)3 * nb));
+  DUST_TRY(B.hnorms.reserve((size_t)NW * nb));
+  DUST_TRY(set_attributes());
+  const int cus = mx_cus();
+  if (nrhs == 0) return 0;
+  k_mx_fill_u32<<<ceil_div(3 * nb, 256), 256, 0, st>>>(B.flags.p, 3 * nb, 0u);
+  MX_LAUNCH_CHECK("flags");
+  if (prec != DUST_LU_FP64) {
+    const int f16 = prec == DUST_LU_FP16S;
+    for (int i = 0; i < nb; ++i)
+      if (runs[i].n > 0) DUST_TRY(run_convert_launch(runs[i]));
+    BatchPlan P;
+    batch_factor_plan(runs, B, f16, P);
+    DUST_TRY(plan_run(B, P, st));
+    DUST_HIP(hipMemcpyAsync(B.hflags.p, B.flags.p, sizeof(unsigned) * 3 * nb,
+                            hipMemcpyDeviceToHost, st));
+    DUST_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nb; ++i) {
+      if (runs[i].n == 0) continue;  // no workspace
+      std::memcpy(runs[i].w->hb.p->flags, B.hflags.p + 3 * i, sizeof(unsigned) * 2);
+      runs[i].factored = !mx_factor_bad(*runs[i].w, TAU);
+    }
+  }
+  for (int q0 = 0; q0 < nrhs; q0 += NR) {
+    for (int i = 0; i < nb; ++i)
+      if (runs[i].n > 0) DUST_TRY(run_chunk_begin(runs[i], q0));
+    while (true) {
+      std::vector<int> act;
+      for (int i = 0; i < nb; ++i)
+        if (runs[i].n > 0 && runs[i].refining) act.push_back(i);
+      if (act.empty()) break;
+      k_mx_zero_u64<<<ceil_div(NW * nb, 256), 256, 0, st>>>(B.norms.p, NW * nb);
+      for (int i : act) DUST_TRY(run_sweep_gather(runs[i]));
+      BatchPlan P;
+      batch_solve_plan(runs, act, B, cus, P);
+      DUST_TRY(plan_run(B, P, st));
+      for (int i : act) {
+        MixedRun &r = runs[i];
+        DUST_TRY(run_sweep_update(r));
+        DUST_TRY(mx_residual_kernels(*r.w, r.dA, r.lda, r.n, r.nit == 1, B.norms.p + (size_t)NW * i,
+                                     st));
+      }
+      DUST_HIP(hipMemcpyAsync(B.hnorms.p, B.norms.p, sizeof(unsigned long long) * NW * nb,
+                              hipMemcpyDeviceToHost, st));
+      DUST_HIP(hipMemcpyAsync(B.hflags.p, B.flags.p, sizeof(unsigned) * 3 * nb,
+                              hipMemcpyDeviceToHost, st));
+      DUST_HIP(hipStreamSynchronize(st));
+      for (int i : act) {
+        MixedWs &w = *runs[i].w;
+        std::memcpy(w.hb.p->norms, B.hnorms.p + (size_t)NW * i, sizeof(w.hb.p->norms));
+        w.hb.p->err = B.hflags.p[3 * i + 2];
+        DUST_TRY(run_sweep_finish(runs[i]));
+      }
+    }
+    for (int i = 0; i < nb; ++i)
+      if (runs[i].n > 0) {
+        int rc = run_chunk_end(runs[i]);
+        if (rc) return rc;
+      }
+  }
+  DUST_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < nb; ++i) {
+    if (iters) iters[i] = runs[i].it_report;
+    if (resid) resid[i] = runs[i].resid_report;
+  }
   return 0;
 }
 
@@ -1266,65 +1631,23 @@ int dust_dgesv_mixed(int n, int nrhs, const double *A, int lda, double *B, int l
   return rc;
 }
 
-// Independent systems in lockstep: each system has its own workspace and
-// stream; the factorisations of all systems are launched, then one
-// synchronisation of every stream, then refinement sweeps of all unfinished
-// systems, one synchronisation per sweep.  Latency-bound parts of different
-// systems (diagonal-block chains, refinement solves) overlap on the GPU.
-// Device pointers.
+// Independent systems of different sizes on one stream: the factorisations
+// advance together one block column per step (one launch per kernel and step
+// for the whole batch), then refinement sweeps of all unfinished systems with
+// both triangular solves of the batch in one launch each and one
+// synchronisation per sweep.  Device pointers.
 int dust_dgesv_mixed_batch_dev(int nb, const int *n, const double *const *dA, const int *lda,
                                double *const *dB, const int *ldb, int nrhs, int prec, double tol,
                                int max_iter, int *iters, double *resid) {
   DUST_CHECK(nb >= 0 && (nb == 0 || (n && dA && lda && dB && ldb)), DUST_E_ARG,
              "invalid batch arguments");
+  DUST_CHECK(nrhs >= 0, DUST_E_ARG, "invalid batch arguments");
+  if (iters)
+    for (int i = 0; i < nb; ++i) iters[i] = 0;
+  if (resid)
+    for (int i = 0; i < nb; ++i) resid[i] = 0.0;
   if (nb == 0) return 0;
-  static std::vector<std::unique_ptr<BatchSlot>> slots;
-  while ((int)slots.size() < nb) {
-    slots.emplace_back(new BatchSlot());
-    DUST_HIP(hipStreamCreateWithFlags(&slots.back()->stream, hipStreamNonBlocking));
-  }
-  std::vector<MixedRun> runs(nb);
-  for (int i = 0; i < nb; ++i)
-    DUST_TRY(run_init(runs[i], slots[i]->ws, n[i], nrhs, dA[i], lda[i], dB[i], ldb[i], prec, tol,
-                      max_iter, slots[i]->stream));
-  auto sync_all = [&]() -> int {
-    for (int i = 0; i < nb; ++i) DUST_HIP(hipStreamSynchronize(runs[i].st));
-    return 0;
-  };
-  const bool any = nrhs > 0;
-  if (prec != DUST_LU_FP64 && any) {
-    for (int i = 0; i < nb; ++i)
-      if (runs[i].n > 0) DUST_TRY(run_factor_launch(runs[i]));
-    DUST_TRY(sync_all());
-    for (int i = 0; i < nb; ++i) runs[i].factored = runs[i].n > 0 && !mx_factor_bad(slots[i]->ws, 2.0);
-  }
-  for (int q0 = 0; any && q0 < nrhs; q0 += NR) {
-    for (int i = 0; i < nb; ++i)
-      if (runs[i].n > 0) DUST_TRY(run_chunk_begin(runs[i], q0));
-    while (true) {
-      bool active = false;
-      for (int i = 0; i < nb; ++i)
-        if (runs[i].n > 0 && runs[i].refining) {
-          DUST_TRY(run_sweep_launch(runs[i]));
-          active = true;
-        }
-      if (!active) break;
-      DUST_TRY(sync_all());
-      for (int i = 0; i < nb; ++i)
-        if (runs[i].n > 0 && runs[i].refining) DUST_TRY(run_sweep_finish(runs[i]));
-    }
-    for (int i = 0; i < nb; ++i)
-      if (runs[i].n > 0) {
-        int rc = run_chunk_end(runs[i]);
-        if (rc) return rc;
-      }
-  }
-  DUST_TRY(sync_all());
-  for (int i = 0; i < nb; ++i) {
-    if (iters) iters[i] = runs[i].it_report;
-    if (resid) resid[i] = runs[i].resid_report;
-  }
-  return 0;
+  return mixed_batch(nb, n, dA, lda, dB, ldb, nrhs, prec, tol, max_iter, iters, resid);
 }
 
 }  // extern "C"

@@ -211,7 +211,7 @@ int dust_dgetrs_dev(int n, int nrhs, const double *dA, int lda, const int *dipiv
  * (DUST_LU_FP32, or DUST_LU_FP16S = row-scaled FP16 operands with FP32
  * accumulation); every 128-wide diagonal block is factored in FP64 with row
  * interchanges among its rows, and the multipliers below it must satisfy
- * |l| <= 2 (threshold pivoting) -- otherwise, or on a zero pivot, or when
+ * |l| <= 10 (threshold pivoting) -- otherwise, or on a zero pivot, or when
  * refinement stalls, the system is solved by the FP64 partial-pivoting path
  * and *iters is set to -1.  FP64 residuals drive iterative refinement until
  * max_k ||b_k - A x_k||_inf / (||A||_inf ||x_k||_inf) <= tol or max_iter

@@ -101,3 +101,56 @@ def test_mixed_batch_vs_lapack():
         X = dB[i].cpu().numpy().T
         assert it[i] >= 1 and res[i] <= 1e-12
         assert rel(X, np.linalg.solve(A, rhs)) <= 1e-10
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16s"])
+def test_mixed_batch_equals_single(prec):
+    """Uneven batch: empty, tiny, non-multiple-of-128 and multi-chunk systems,
+    a rotor AIC with multipliers above 2 and a system that needs the FP64
+    fallback.  The batched factorisation runs the same tile operations in the
+    same order as the single solve, so solutions, sweep counts and backward
+    errors agree exactly."""
+    import torch
+    lib = L.lib()
+    rng = np.random.default_rng(31)
+    nrhs = 11
+    systems = []
+    for n in (0, 1, 127, 129, 300, 512, 1000):
+        A = rng.standard_normal((n, n)) / np.sqrt(max(n, 1)) + 2.0 * np.eye(n)
+        if n == 512:
+            A = np.roll(A, 200, axis=0)  # pivots outside the diagonal blocks
+        systems.append(A)
+    ver, nv = W.rotor(5, 704)
+    systems.insert(3, LS.aic_surfpan(ver, nv)[0])
+    rhs = [rng.standard_normal((A.shape[0], nrhs)) for A in systems]
+    code = {"fp32": L.LU_FP32, "fp16s": L.LU_FP16S}[prec]
+    single = []
+    for A, b in zip(systems, rhs):
+        if A.shape[0] == 0:
+            single.append((b, 0, 0.0))
+            continue
+        single.append(LS.dgesv_mixed(np.asfortranarray(A), np.asfortranarray(b.copy()), prec=prec,
+                                     tol=1e-12, max_iter=40))
+    nb = len(systems)
+    dA = [torch.from_numpy(np.ascontiguousarray(A.T)).cuda() for A in systems]
+    dB = [torch.from_numpy(np.ascontiguousarray(b.T)).cuda() for b in rhs]
+    n = (C.c_int * nb)(*[A.shape[0] for A in systems])
+    ld = (C.c_int * nb)(*[max(1, A.shape[0]) for A in systems])
+    pa = (C.c_void_p * nb)(*[d.data_ptr() for d in dA])
+    pb = (C.c_void_p * nb)(*[d.data_ptr() for d in dB])
+    it = (C.c_int * nb)()
+    res = (C.c_double * nb)()
+    print(f"batch of {nb} mixed solves ({prec}), n = {list(n)}, nrhs {nrhs}", flush=True)
+    L.check(lib.dust_dgesv_mixed_batch_dev(nb, n, pa, ld, pb, ld, nrhs, code, 1e-12, 40, it, res),
+            "mixed batch")
+    for i, A in enumerate(systems):
+        X = dB[i].cpu().numpy().T
+        Xs, its, bes = single[i]
+        print(f"  n={A.shape[0]}: iters {it[i]} / {its}, berr {res[i]:.2e}")
+        assert it[i] == its
+        assert res[i] == bes
+        assert np.array_equal(X, Xs)
+        if A.shape[0] == 512:
+            assert it[i] == -1
+        elif A.shape[0] > 0:
+            assert it[i] >= 1

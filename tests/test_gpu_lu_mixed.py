@@ -49,6 +49,41 @@ def test_aic_ellipsoid(aic, prec):
     assert rel(X, X0) <= 1e-10
 
 
+@pytest.mark.parametrize("prec", PRECS)
+def test_aic_rotor_multipliers_above_two(prec):
+    """Five-blade rotor AIC (650 panels) whose block pivoting leaves
+    multipliers of 2.6 below a diagonal block (the largest |l| is computed in
+    the test): inside the threshold, refined without the fallback."""
+    ver, nv = W.rotor(5, 704)
+    A, B = LS.aic_surfpan(ver, nv)
+    n = A.shape[0]
+    lmax = block_pivot_lmax(A)
+    assert 2.0 < lmax < 10.0
+    b = np.asfortranarray(LS.rhs_static(ver, nv, B, W.freestreams(3)))
+    X, it, be = solve(A, b, prec)
+    print(f"rotor n={n} max|l| {lmax:.3f} {prec}: iters {it} berr {be:.2e}")
+    assert it >= 1
+    assert be <= 1e-12
+    assert rel(X, np.linalg.solve(A, b)) <= 1e-10
+
+
+def block_pivot_lmax(A, nb=128):
+    """Largest multiplier below the diagonal blocks when each block column
+    pivots only among the rows of its own diagonal block."""
+    A = np.array(A, dtype=float)
+    n, mx = A.shape[0], 0.0
+    for k0 in range(0, n, nb):
+        k1 = min(n, k0 + nb)
+        for j in range(k0, k1):
+            p = j + int(np.argmax(np.abs(A[j:k1, j])))
+            A[[j, p]] = A[[p, j]]
+            A[j + 1:, j] /= A[j, j]
+            if k1 < n:
+                mx = max(mx, float(np.abs(A[k1:, j]).max()))
+            A[j + 1:, j + 1:] -= np.outer(A[j + 1:, j], A[j, j + 1:])
+    return mx
+
+
 @pytest.mark.parametrize("n,nrhs", [(1, 1), (5, 2), (127, 3), (129, 8), (300, 11), (1000, 8)])
 @pytest.mark.parametrize("prec", PRECS)
 def test_diag_dominant_shapes(n, nrhs, prec):
