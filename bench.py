@@ -411,6 +411,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--near", type=float, default=1.0,
                     help="tuned form: FP64 for pairs closer than this many core radii")
+    ap.add_argument("--m2l-chunk", type=int, default=None,
+                    help="tuned form: interaction offsets per M2L job (1..189)")
     ap.add_argument("--no-fp64", action="store_true", help="skip timing the plain FP64 form")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-case batch leg")
@@ -455,7 +457,8 @@ def main():
         log(f"plain: {plain['ms']:.2f} ms/pass stages {plain['stages']}")
     log("timing tuned FMM")
     tuned = time_fmm(torch, L, O, tree, args.degree, dev, n, "tuned", args.steps, args.warmup,
-                     knobs={"near_factor": args.near})
+                     knobs=dict({"near_factor": args.near},
+                                **({"m2l_chunk": args.m2l_chunk} if args.m2l_chunk else {})))
     log(f"tuned: {tuned['ms']:.2f} ms/pass stages {tuned['stages']}")
     value = n / (tuned["ms"] / 1e3)
     # roofline of the dominant kernel of the tuned pass: FP32 P2P items

@@ -42,6 +42,7 @@ struct TunedState {
   DevBuf<int> gcount;       // [kMaxLevels * 8] group sizes (device)
   DevBuf<int> offs;         // [8][189][3] offsets of each octant
   DevBuf<float> W;          // M2L operators of every level and offset, fragment order
+  int m2l_chunk = 21;       // offsets per M2L job
   DevBuf<unsigned char> jobs;  // M2L job list of the current pass
   HostBuf<M2LJob> host_jobs;   // page-locked source of the job upload
   HostBuf<int> hcount;          // [kMaxLevels * 8] group sizes (page-locked)
@@ -88,6 +89,11 @@ int tuned_set(FmmHandle *h, const char *name, double value) {
   if (!strcmp(name, "near_factor")) {
     DUST_CHECK(value >= 0.0, DUST_E_ARG, "near_factor must be >= 0");
     t->near_factor = value;
+    return 0;
+  }
+  if (!strcmp(name, "m2l_chunk")) {
+    DUST_CHECK(value >= 1.0 && value <= kOct, DUST_E_ARG, "m2l_chunk must be in 1..189");
+    t->m2l_chunk = (int)value;
     return 0;
   }
   set_error("unknown tuning knob '%s'", name);
@@ -229,16 +235,15 @@ static int build_w(FmmHandle *h) {
   return 0;
 }
 
-// One job = (level, octant, 64 targets, a chunk of kJC offsets, a block of
+// One job = (level, octant, 64 targets, a chunk of offsets, a block of
 // row tiles); all jobs of a pass run in one launch and add their partial
 // results into the FP64 local expansions with atomics (M2L only reads the
 // multipoles, so every level can be translated before the L2L chain).
-constexpr int kJC = 21;  // offsets per job (189 = 9 x 21)
 
 
 struct M2LArgs {
   WGeom w;
-  int nmon;
+  int nmon, jc;
   const M2LJob *jobs;
   const int *grp, *src, *offs;
   const float *W;
@@ -280,7 +285,7 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(M2LArgs P) {
   for (int a = 0; a < kRTW; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
-  const int j1 = min(kOct, J.j0 + kJC);
+  const int j1 = min(kOct, J.j0 + P.jc);
   for (int j = J.j0; j < j1; ++j) {
     const int di = P.offs[(J.oct * kOct + j) * 3], dj = P.offs[(J.oct * kOct + j) * 3 + 1],
               dk = P.offs[(J.oct * kOct + j) * 3 + 2];
@@ -569,7 +574,8 @@ int tuned_m2l_all(FmmHandle *h) {
     for (int g = 0; g < 8; ++g) {
       int nt = t->hcount.p[lvl * 8 + g];
       if (nt == 0) continue;
-      for (int j0 = 0; j0 < kOct; j0 += kJC) chunks.push_back({lvl, g, j0, ceil_div(nt, kNT) * nrb});
+      for (int j0 = 0; j0 < kOct; j0 += t->m2l_chunk)
+        chunks.push_back({lvl, g, j0, ceil_div(nt, kNT) * nrb});
     }
   std::stable_sort(chunks.begin(), chunks.end(),
                    [](const Chunk &a, const Chunk &b) { return a.njobs > b.njobs; });
@@ -609,6 +615,7 @@ int tuned_m2l_all(FmmHandle *h) {
   M2LArgs P;
   P.w = w;
   P.nmon = h->nmon;
+  P.jc = t->m2l_chunk;
   P.jobs = (const M2LJob *)t->jobs.p;
   P.grp = t->grp.p;
   P.src = t->src.p;

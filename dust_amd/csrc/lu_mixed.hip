@@ -29,6 +29,7 @@
 // re-entrant.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -100,14 +101,62 @@ __global__ void k_mx_fill_u32(unsigned *p, int n, unsigned v) {
   if (i < n) p[i] = v;
 }
 
-__global__ void k_mx_rowmax(const double *__restrict__ A, long lda, int n, int cw,
-                            unsigned *rowmax) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Streaming over A (n x n, column-major) for the row maxima and the residual:
+// a workgroup of 128 threads takes 256 rows (two per thread) and a chunk of
+// `cw` columns, eight columns in flight per thread.  VEC = 1 (one 16-byte
+// load per column and thread) only when A is 16-byte aligned and n and lda
+// are even, so that both rows of every pair exist.
+constexpr int SR = 256;  // rows per streaming workgroup
+constexpr int SU = 8;    // columns in flight per thread
+
+template <int VEC>
+__device__ __forceinline__ double2 load_pair(const double *__restrict__ p, bool two) {
+  if (VEC) return *(const double2 *)p;
+  return make_double2(p[0], two ? p[1] : 0.0);
+}
+
+// Column chunks of the streaming kernels: about 2048 workgroups, chunks of
+// 64..512 columns (multiple of 8).
+inline int stream_chunk(int n, int *nch) {
+  const int rb = ceil_div(n, SR);
+  int c = std::max(1, std::min(ceil_div(n, 64), ceil_div(2048, rb)));
+  int cw = std::min(512, std::max(64, ceil_div(ceil_div(n, c), SU) * SU));
+  *nch = ceil_div(std::max(n, 1), cw);
+  return cw;
+}
+
+inline bool stream_vec(const double *A, long lda, int n) {
+  return ((uintptr_t)A & 15) == 0 && (lda & 1) == 0 && (n & 1) == 0;
+}
+
+// rowmax[i] = max over the chunk of |A(i, j)| (FP32 bits, atomic max).
+template <int VEC>
+__global__ void __launch_bounds__(128) k_mx_rowmax(const double *__restrict__ A, long lda, int n,
+                                                   int cw, unsigned *rowmax) {
+  const int i = blockIdx.x * SR + 2 * threadIdx.x;
   if (i >= n) return;
-  int j0 = blockIdx.y * cw, j1 = min(n, j0 + cw);
-  float m = 0.f;
-  for (int j = j0; j < j1; ++j) m = fmaxf(m, fabsf((float)A[i + lda * j]));
-  atomicMax(&rowmax[i], __float_as_uint(m));
+  const bool two = i + 1 < n;
+  const int j0 = blockIdx.y * cw, nj = min(n - j0, cw);
+  const double *Ap = A + i + lda * (long)j0;
+  double m0 = 0.0, m1 = 0.0;
+  int j = 0;
+  for (; j + SU <= nj; j += SU) {
+    double2 a[SU];
+#pragma unroll
+    for (int u = 0; u < SU; ++u) a[u] = load_pair<VEC>(Ap + lda * (long)(j + u), two);
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      m0 = fmax(m0, fabs(a[u].x));
+      m1 = fmax(m1, fabs(a[u].y));
+    }
+  }
+  for (; j < nj; ++j) {
+    double2 a = load_pair<0>(Ap + lda * (long)j, two);
+    m0 = fmax(m0, fabs(a.x));
+    m1 = fmax(m1, fabs(a.y));
+  }
+  atomicMax(&rowmax[i], __float_as_uint((float)m0));
+  if (two) atomicMax(&rowmax[i + 1], __float_as_uint((float)m1));
 }
 
 __global__ void k_mx_rowscale(const unsigned *rowmax, int n, int np, int scaled, double *rs) {
@@ -155,49 +204,49 @@ struct DiagArgs {
 };
 
 // Wave 0: factor columns [c0, c0+32) of the block, rows [c0, 128), held in
-// registers (lane l: rows l and l + 64).  One template instance per column
-// keeps every register index a compile-time constant.
+// registers: lane l holds the slots of rows l and l + 64 at the start.  Rows
+// are not moved while the sub-panel is factored: each slot carries the row
+// position of its data (pos0, pos1), an interchange of positions g and p
+// only exchanges two labels, and every slot is stored at its final position
+// at the end.  The pivot of column g is the largest |a| among positions >= g
+// (smallest position on ties); the interchanges are recorded LAPACK-style
+// in spiv.  One template instance per column keeps every register index a
+// compile-time constant.
 template <int J>
-__device__ __forceinline__ void subpanel_step(double (&a0)[SP], double (&a1)[SP], int c0, int lane,
-                                              int *spiv, unsigned *flags) {
+__device__ __forceinline__ void subpanel_step(double (&a0)[SP], double (&a1)[SP], int &pos0,
+                                              int &pos1, int c0, int lane, int *spiv,
+                                              unsigned *flags) {
   const int g = c0 + J;
-  unsigned k0 = lane >= g ? piv_key(a0[J], lane) : 0u;
-  unsigned k1 = lane + 64 >= g ? piv_key(a1[J], lane + 64) : 0u;
-  const unsigned kk = wave_umax(max(k0, k1));
-  const int p = 127 - (int)(kk & 127u);
-  const int lg = g & 63, sg = g >> 6;
-  if (p != g) {
-    const int lp = p & 63, spp = p >> 6;
-#pragma unroll
-    for (int c = 0; c < SP; ++c) {
-      double vg = rdlane(sg ? a1[c] : a0[c], lg);
-      double vp = rdlane(spp ? a1[c] : a0[c], lp);
-      if (lane == lg) {
-        if (sg) a1[c] = vp;
-        else a0[c] = vp;
-      }
-      if (lane == lp) {
-        if (spp) a1[c] = vg;
-        else a0[c] = vg;
-      }
-    }
-  }
+  const unsigned k0 = pos0 >= g ? piv_key(a0[J], pos0) : 0u;
+  const unsigned k1 = pos1 >= g ? piv_key(a1[J], pos1) : 0u;
+  const int p = 127 - (int)(wave_umax(max(k0, k1)) & 127u);
+  // the slot holding position p (wave-uniform)
+  const unsigned long long m0 = __ballot(pos0 == p);
+  const unsigned long long m1 = __ballot(pos1 == p);
+  const int ls = __ffsll((long long)(m0 ? m0 : m1)) - 1;
   double u[SP];
+  if (m0) {
 #pragma unroll
-  for (int c = J; c < SP; ++c) u[c] = rdlane(sg ? a1[c] : a0[c], lg);
+    for (int c = J; c < SP; ++c) u[c] = rdlane(a0[c], ls);
+  } else {
+#pragma unroll
+    for (int c = J; c < SP; ++c) u[c] = rdlane(a1[c], ls);
+  }
   const double piv = u[J];
   if (lane == 0) {
     spiv[g] = p;
     if (piv == 0.0) atomicOr(&flags[1], 1u);
   }
+  pos0 = pos0 == g ? p : (pos0 == p ? g : pos0);
+  pos1 = pos1 == g ? p : (pos1 == p ? g : pos1);
   const double rp = 1.0 / piv;
-  if (lane > g) {
+  if (pos0 > g) {
     const double l = a0[J] * rp;
     a0[J] = l;
 #pragma unroll
     for (int c = J + 1; c < SP; ++c) a0[c] = fma(-l, u[c], a0[c]);
   }
-  if (lane + 64 > g) {
+  if (pos1 > g) {
     const double l = a1[J] * rp;
     a1[J] = l;
 #pragma unroll
@@ -207,12 +256,13 @@ __device__ __forceinline__ void subpanel_step(double (&a0)[SP], double (&a1)[SP]
 
 template <int... Js>
 __device__ __forceinline__ void subpanel_steps(std::integer_sequence<int, Js...>, double (&a0)[SP],
-                                               double (&a1)[SP], int c0, int lane, int *spiv,
-                                               unsigned *flags) {
-  (subpanel_step<Js>(a0, a1, c0, lane, spiv, flags), ...);
+                                               double (&a1)[SP], int &pos0, int &pos1, int c0,
+                                               int lane, int *spiv, unsigned *flags) {
+  (subpanel_step<Js>(a0, a1, pos0, pos1, c0, lane, spiv, flags), ...);
 }
 
-__device__ __forceinline__ void diag_subpanel(double *sD, int c0, int lane, int *spiv,
+// sorig[q]: original row (within the block) of the row now at position q.
+__device__ __forceinline__ void diag_subpanel(double *sD, int *sorig, int c0, int lane, int *spiv,
                                               unsigned *flags) {
   double a0[SP], a1[SP];
 #pragma unroll
@@ -220,12 +270,16 @@ __device__ __forceinline__ void diag_subpanel(double *sD, int c0, int lane, int 
     a0[j] = sD[(c0 + j) * LDD + lane];
     a1[j] = sD[(c0 + j) * LDD + lane + 64];
   }
-  subpanel_steps(std::make_integer_sequence<int, SP>{}, a0, a1, c0, lane, spiv, flags);
+  const int o0 = sorig[lane], o1 = sorig[lane + 64];
+  int pos0 = lane, pos1 = lane + 64;
+  subpanel_steps(std::make_integer_sequence<int, SP>{}, a0, a1, pos0, pos1, c0, lane, spiv, flags);
 #pragma unroll
   for (int j = 0; j < SP; ++j) {
-    sD[(c0 + j) * LDD + lane] = a0[j];
-    sD[(c0 + j) * LDD + lane + 64] = a1[j];
+    sD[(c0 + j) * LDD + pos0] = a0[j];
+    sD[(c0 + j) * LDD + pos1] = a1[j];
   }
+  sorig[pos0] = o0;
+  sorig[pos1] = o1;
 }
 
 // BATCH: workgroup b factors the block of items[b] (one block per system).
@@ -234,19 +288,33 @@ __global__ void __launch_bounds__(256) k_mx_diag(DiagArgs one, const DiagArgs *_
   const DiagArgs a = BATCH ? items[blockIdx.x] : one;
   extern __shared__ double sD[];  // NB x LDD, column-major
   __shared__ int spiv[NB];
-  __shared__ int sperm[NB];
-  __shared__ int scount;
+  __shared__ int sorig[NB];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const float *Wd = a.W + a.k + a.ld * a.k;
-  for (int q = tid; q < NB * NB; q += 256) {
-    int r = q & (NB - 1), c = q >> 7;
-    sD[c * LDD + r] = (double)Wd[r + a.ld * c];
+  // the block, 16 bytes per load, all loads in flight before the LDS stores
+  // (W and its leading dimension are multiples of 128 floats)
+  {
+    const float *Wd = a.W + a.k + a.ld * a.k;
+    float4 v[NB * NB / 4 / 256];
+#pragma unroll
+    for (int it = 0; it < NB * NB / 4 / 256; ++it) {
+      const int q = tid + 256 * it, c = q >> 5, r = (q & 31) * 4;
+      v[it] = *(const float4 *)(Wd + r + a.ld * c);
+    }
+#pragma unroll
+    for (int it = 0; it < NB * NB / 4 / 256; ++it) {
+      const int q = tid + 256 * it, c = q >> 5, r = (q & 31) * 4;
+      double *d = sD + c * LDD + r;
+      d[0] = v[it].x;
+      d[1] = v[it].y;
+      d[2] = v[it].z;
+      d[3] = v[it].w;
+    }
   }
-  if (tid == 0) scount = 0;
+  if (tid < NB) sorig[tid] = tid;
   __syncthreads();
   for (int sp = 0; sp < NB / SP; ++sp) {
     const int c0 = sp * SP;
-    if (wv == 0) diag_subpanel(sD, c0, lane, spiv, a.flags);
+    if (wv == 0) diag_subpanel(sD, sorig, c0, lane, spiv, a.flags);
     __syncthreads();
     // interchanges of this sub-panel on the other 96 columns
     if (tid < NB - SP) {
@@ -302,58 +370,56 @@ __global__ void __launch_bounds__(256) k_mx_diag(DiagArgs one, const DiagArgs *_
     __syncthreads();
   }
   // inverses of the 32x32 diagonal sub-blocks: wave w takes sub-block w;
-  // lanes 0..31 invert U (back substitution per column), lanes 32..63 invert
-  // the unit-lower L (forward substitution per column).
+  // lanes 0..31 invert U (column c by back substitution), lanes 32..63 the
+  // unit-lower L (forward substitution), both in column (axpy) order so that
+  // the updates of one step are independent.
   {
     const int b0 = wv * SP;
     const int c = lane & 31;
     float *Li = a.dinv + ((long)(a.k / NB) * 8 + wv) * SP * SP;
     float *Ui = a.dinv + ((long)(a.k / NB) * 8 + 4 + wv) * SP * SP;
     double x[SP];
+#pragma unroll
+    for (int i = 0; i < SP; ++i) x[i] = (i == c) ? 1.0 : 0.0;
     if (lane < 32) {
 #pragma unroll
       for (int i = SP - 1; i >= 0; --i) {
-        double s = (i == c) ? 1.0 : 0.0;
+        const double *col = sD + (b0 + i) * LDD + b0;  // U(0..i, i)
+        x[i] = x[i] / col[i];
 #pragma unroll
-        for (int kq = i + 1; kq < SP; ++kq) s = fma(-sD[(b0 + kq) * LDD + b0 + i], x[kq], s);
-        x[i] = s / sD[(b0 + i) * LDD + b0 + i];
+        for (int r = 0; r < i; ++r) x[r] = fma(-col[r], x[i], x[r]);
       }
 #pragma unroll
       for (int i = 0; i < SP; ++i) Ui[i + SP * c] = (float)x[i];
     } else {
 #pragma unroll
       for (int i = 0; i < SP; ++i) {
-        double s = (i == c) ? 1.0 : 0.0;
+        const double *col = sD + (b0 + i) * LDD + b0;  // L(i+1.., i)
 #pragma unroll
-        for (int kq = 0; kq < i; ++kq) s = fma(-sD[(b0 + kq) * LDD + b0 + i], x[kq], s);
-        x[i] = s;
+        for (int r = i + 1; r < SP; ++r) x[r] = fma(-col[r], x[i], x[r]);
       }
 #pragma unroll
       for (int i = 0; i < SP; ++i) Li[i + SP * c] = (float)x[i];
     }
   }
-  // pivots, permutation, write-back
-  if (tid < NB) {
-    if (spiv[tid] != tid) atomicAdd(&scount, 1);
-    sperm[tid] = tid;
-  }
-  __syncthreads();
-  if (tid == 0 && scount > 0)
-    for (int i = 0; i < NB; ++i) {
-      int p = spiv[i], t = sperm[i];
-      sperm[i] = sperm[p];
-      sperm[p] = t;
-    }
-  __syncthreads();
+  // pivots, permutation (sorig), interchange count, write-back
   if (tid < NB) {
     a.piv[a.k + tid] = a.k + spiv[tid];
-    a.perm[a.k + tid] = a.k + sperm[tid];
+    a.perm[a.k + tid] = a.k + sorig[tid];
   }
-  if (tid == 0) a.npiv[a.k / NB] = scount;
-  float *Wo = a.W + a.k + a.ld * a.k;
-  for (int q = tid; q < NB * NB; q += 256) {
-    int r = q & (NB - 1), c = q >> 7;
-    Wo[r + a.ld * c] = (float)sD[c * LDD + r];
+  if (wv == 0) {
+    const unsigned long long b0 = __ballot(spiv[lane] != lane);
+    const unsigned long long b1 = __ballot(spiv[lane + 64] != lane + 64);
+    if (lane == 0) a.npiv[a.k / NB] = __popcll(b0) + __popcll(b1);
+  }
+  {
+    float *Wo = a.W + a.k + a.ld * a.k;
+#pragma unroll 4
+    for (int it = 0; it < NB * NB / 4 / 256; ++it) {
+      const int q = tid + 256 * it, c = q >> 5, r = (q & 31) * 4;
+      const double *d = sD + c * LDD + r;
+      *(float4 *)(Wo + r + a.ld * c) = make_float4((float)d[0], (float)d[1], (float)d[2], (float)d[3]);
+    }
   }
 }
 
@@ -750,26 +816,68 @@ __global__ void k_mx_update_x(double *x, long ldx, int n, const float *__restric
 
 // Partial products of r = b - A x over column chunks (deterministic: the
 // chunks are summed in order by k_mx_resid_fin).  Also row sums of |A|.
-__global__ void k_mx_resid_part(const double *__restrict__ A, long lda, int n, int cw,
-                                const double *__restrict__ x, double *part, double *rowabs) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// Partial products of r = b - A x over column chunk ch = blockIdx.y:
+// part[ch][i][q] = sum_j A(i, j) x(j, q) and rowabs[ch][i] = sum_j |A(i, j)|,
+// summed in column order (deterministic).  The chunk's rows of x are staged
+// in LDS ([j][q]); streaming as k_mx_rowmax.
+template <int VEC>
+__global__ void __launch_bounds__(128) k_mx_resid_part(const double *__restrict__ A, long lda, int n,
+                                                       int cw, const double *__restrict__ x,
+                                                       double *part, double *rowabs) {
+  extern __shared__ double sx[];  // cw x NR
   const int ch = blockIdx.y;
-  const int j0 = ch * cw, j1 = min(n, j0 + cw);
-  double s[NR];
+  const int j0 = ch * cw, nj = min(n - j0, cw);
+  for (int t = threadIdx.x; t < nj * NR; t += 128) {
+    const int j = t / NR, q = t % NR;
+    sx[t] = x[j0 + j + (long)n * q];
+  }
+  __syncthreads();
+  const int i = blockIdx.x * SR + 2 * threadIdx.x;
+  if (i >= n) return;
+  const bool two = i + 1 < n;
+  const double *Ap = A + i + lda * (long)j0;
+  double s0[NR], s1[NR];
 #pragma unroll
-  for (int q = 0; q < NR; ++q) s[q] = 0.0;
-  double sa = 0.0;
-  for (int j = j0; j < j1; ++j) {
-    const double av = A[i + lda * j];
-    sa += fabs(av);
+  for (int q = 0; q < NR; ++q) s0[q] = s1[q] = 0.0;
+  double sa0 = 0.0, sa1 = 0.0;
+  int j = 0;
+  for (; j + SU <= nj; j += SU) {
+    double2 a[SU];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) s[q] = fma(av, x[j + (long)n * q], s[q]);
+    for (int u = 0; u < SU; ++u) a[u] = load_pair<VEC>(Ap + lda * (long)(j + u), two);
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      sa0 += fabs(a[u].x);
+      sa1 += fabs(a[u].y);
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        const double xv = sx[(j + u) * NR + q];
+        s0[q] = fma(a[u].x, xv, s0[q]);
+        s1[q] = fma(a[u].y, xv, s1[q]);
+      }
+    }
+  }
+  for (; j < nj; ++j) {
+    const double2 a = load_pair<0>(Ap + lda * (long)j, two);
+    sa0 += fabs(a.x);
+    sa1 += fabs(a.y);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const double xv = sx[j * NR + q];
+      s0[q] = fma(a.x, xv, s0[q]);
+      s1[q] = fma(a.y, xv, s1[q]);
+    }
   }
   double *p = part + ((long)ch * n + i) * NR;
 #pragma unroll
-  for (int q = 0; q < NR; ++q) p[q] = s[q];
-  if (rowabs) rowabs[(long)ch * n + i] = sa;
+  for (int q = 0; q < NR; ++q) p[q] = s0[q];
+  if (two)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) p[NR + q] = s1[q];
+  if (rowabs) {
+    rowabs[(long)ch * n + i] = sa0;
+    if (two) rowabs[(long)ch * n + i + 1] = sa1;
+  }
 }
 
 __device__ inline unsigned long long wave_max_u64(unsigned long long v) {
@@ -1021,10 +1129,13 @@ int mx_solve(MixedWs &w, int np, hipStream_t st, int cus) {
 // Kernels only: the norms go to `norms` (2 NR + 1 words, zeroed by the caller).
 int mx_residual_kernels(MixedWs &w, const double *dA, long lda, int n, bool with_anorm,
                         unsigned long long *norms, hipStream_t st) {
-  const int cw = 512;
-  const int nch = ceil_div(n, cw);
-  dim3 g(ceil_div(n, 256), nch);
-  k_mx_resid_part<<<g, 256, 0, st>>>(dA, lda, n, cw, w.x8.p, w.part.p, with_anorm ? w.rowabs.p : nullptr);
+  int nch;
+  const int cw = stream_chunk(n, &nch);
+  dim3 g(ceil_div(n, SR), nch);
+  const size_t lds = sizeof(double) * cw * NR;
+  double *ra = with_anorm ? w.rowabs.p : nullptr;
+  if (stream_vec(dA, lda, n)) k_mx_resid_part<1><<<g, 128, lds, st>>>(dA, lda, n, cw, w.x8.p, w.part.p, ra);
+  else k_mx_resid_part<0><<<g, 128, lds, st>>>(dA, lda, n, cw, w.x8.p, w.part.p, ra);
   k_mx_resid_fin<<<ceil_div(n, 256), 256, 0, st>>>(n, nch, w.part.p, w.b8.p, w.r8.p, w.x8.p,
                                                    with_anorm ? w.rowabs.p : nullptr, norms);
   MX_LAUNCH_CHECK("residual");
@@ -1084,8 +1195,10 @@ int mx_reserve(MixedWs &w, int n, int np) {
   DUST_TRY(w.b8.reserve((size_t)n * NR));
   DUST_TRY(w.x8.reserve((size_t)n * NR));
   DUST_TRY(w.r8.reserve((size_t)n * NR));
-  DUST_TRY(w.part.reserve((size_t)ceil_div(n, 512) * n * NR));
-  DUST_TRY(w.rowabs.reserve((size_t)ceil_div(n, 512) * n));
+  int nch;
+  (void)stream_chunk(n, &nch);
+  DUST_TRY(w.part.reserve((size_t)nch * n * NR));
+  DUST_TRY(w.rowabs.reserve((size_t)nch * n));
   DUST_TRY(w.norms.reserve(2 * NR + 1));
   DUST_TRY(w.hb.reserve(1));
   return 0;
@@ -1140,8 +1253,11 @@ int run_convert_launch(MixedRun &r) {
   DUST_TRY(set_attributes());
   if (f16) {
     k_mx_fill_u32<<<ceil_div(np, 256), 256, 0, r.st>>>(w.rowmax.p, np, 0u);
-    dim3 g(ceil_div(n, 256), ceil_div(n, 256));
-    k_mx_rowmax<<<g, 256, 0, r.st>>>(r.dA, r.lda, n, 256, w.rowmax.p);
+    int nch;
+    const int cw = stream_chunk(n, &nch);
+    dim3 g(ceil_div(n, SR), nch);
+    if (stream_vec(r.dA, r.lda, n)) k_mx_rowmax<1><<<g, 128, 0, r.st>>>(r.dA, r.lda, n, cw, w.rowmax.p);
+    else k_mx_rowmax<0><<<g, 128, 0, r.st>>>(r.dA, r.lda, n, cw, w.rowmax.p);
   }
   k_mx_rowscale<<<ceil_div(np, 256), 256, 0, r.st>>>(w.rowmax.p, n, np, f16, w.rs.p);
   k_mx_convert<<<dim3(np / 256 + (np % 256 != 0), np / 4), 256, 0, r.st>>>(r.dA, r.lda, n, np,

@@ -114,7 +114,10 @@ int dust_fmm_pass_dev(dust_fmm_t h, int *nleaves);
 int dust_fmm_pass_batch_dev(int nh, dust_fmm_t *handles, int *nleaves);
 
 /* Knobs of the tuned form: "near_factor" -- pairs closer than near_factor
- * times the source core radius are evaluated in FP64 (default 1). */
+ * times the source core radius are evaluated in FP64 (default 1);
+ * "m2l_chunk" -- interaction offsets per M2L job, 1..189 (default 21): larger
+ * chunks mean fewer atomic additions into the local expansions and fewer,
+ * longer jobs. */
 int dust_fmm_set_tuning(dust_fmm_t h, const char *name, double value);
 
 /* HIP stream (hipStream_t) the handle launches on. */

@@ -6,6 +6,7 @@ import os
 import numpy as np
 import pytest
 
+from dust_amd import _lib as L
 from dust_amd import octree as O
 from dust_amd import workloads as W
 
@@ -210,3 +211,42 @@ def test_tuned_with_diffusion_falls_back_exact(orc):
     assert rel(parts.vel, v) <= TUNED_VEL
     assert rel(parts.stretch, s) <= TUNED_STR
     assert rel(parts.turbvisc, tv) <= 1e-5
+
+
+def test_tuned_m2l_chunk_knob(orc):
+    """The M2L job size changes how the offsets are split between jobs.  A job
+    sums its offsets in FP32 before adding into the FP64 locals, so results
+    differ at FP32 rounding; each stays within the tuned bounds of the FP64
+    oracle."""
+    cen, alpha, rv, tree = W.helix_wake(12000, seed=11)
+    tree = dict(tree, nlevels=5)
+    out = {}
+    for chunk in (None, 7, 189):
+        parts = O.VortParts.from_alpha(cen, alpha, rv)
+        print(f"m2l_chunk {chunk}", flush=True)
+        g = O.initialize_octree(tree["box_length"], tree["nbox"], tree["origin"], tree["nlevels"],
+                                tree["min_part"], 4, tree["delta"], capacity=parts.n, use_vs=True,
+                                precision="tuned")
+        if chunk:
+            g.set_tuning("m2l_chunk", chunk)
+        O.sort_particles(parts, g)
+        O.calculate_multipole(parts, g)
+        O.apply_multipole(parts, g)
+        out[chunk] = (parts.vel.copy(), parts.stretch.copy())
+        if chunk == 7:
+            with pytest.raises(L.DustError):
+                g.set_tuning("m2l_chunk", 0)
+            with pytest.raises(L.DustError):
+                g.set_tuning("m2l_chunk", 190)
+        g.destroy()
+    c = orc.Octree(tree["box_length"], tree["nbox"], tree["origin"], tree["nlevels"],
+                   tree["min_part"], 4, tree["delta"], use_vs=True)
+    parts = O.VortParts.from_alpha(cen, alpha, rv)
+    v, st, _ = c.run(parts.cen, parts.dir, parts.mag, parts.r_vortex)
+    for chunk, (vel, stretch) in out.items():
+        print(f"chunk {chunk}: vel {rel(vel, v):.2e} stretch {rel(stretch, st):.2e}")
+        assert rel(vel, v) <= TUNED_VEL
+        assert rel(stretch, st) <= TUNED_STR
+    for chunk in (7, 189):
+        assert rel(out[chunk][0], out[None][0]) <= 5e-7
+        assert rel(out[chunk][1], out[None][1]) <= 5e-6
