@@ -39,6 +39,7 @@ struct TunedState {
   DevBuf<float> a32;        // FP32 copy of the multipole coefficients, (3 n_mon) per slot
   DevBuf<int> grp;          // target slots grouped by (level, octant)
   DevBuf<int> src;          // per grouped target: 189 source slots (-1 = none)
+  DevBuf<long> pos;         // per active cell of a level: its grouped entry
   DevBuf<int> gcount;       // [kMaxLevels * 8] group sizes (device)
   DevBuf<int> offs;         // [8][189][3] offsets of each octant
   DevBuf<float> W;          // M2L operators of every level and offset, fragment order
@@ -106,6 +107,7 @@ void tuned_release(FmmHandle *h) {
   t->a32.release();
   t->grp.release();
   t->src.release();
+  t->pos.release();
   t->gcount.release();
   t->offs.release();
   t->W.release();
@@ -119,9 +121,10 @@ void tuned_release(FmmHandle *h) {
 // ---------------------------------------------------------------------------
 // Grouping of the targets of every level by octant, with their source slots
 // ---------------------------------------------------------------------------
+// Targets of level lvl grouped by octant: grp[e] = slot of the target, its
+// entry e recorded in pos[q] for k_group_src.
 __global__ void k_group_targets(TreeGeom G, int lvl, const int *__restrict__ active, int a0, int na,
-                                const int *__restrict__ slot, const unsigned char *__restrict__ flags,
-                                const int *__restrict__ offs, int *grp, int *src, long gb0,
+                                const int *__restrict__ slot, int *grp, long *pos, long gb0,
                                 long gstride, int *gcount) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= na) return;
@@ -133,18 +136,31 @@ __global__ void k_group_targets(TreeGeom G, int lvl, const int *__restrict__ act
   int w = atomicAdd(&gcount[lvl * 8 + oc], 1);
   long e = gb0 + oc * gstride + w;
   grp[e] = slot[g];
-  int *sr = &src[e * kOct];
-  for (int j = 0; j < kOct; ++j) {
-    int sx = ix + offs[(oc * kOct + j) * 3], sy = iy + offs[(oc * kOct + j) * 3 + 1],
-        sz = iz + offs[(oc * kOct + j) * 3 + 2];
-    int s = -1;
-    if (sx >= 0 && sy >= 0 && sz >= 0 && sx < G.ncl[lvl][0] && sy < G.ncl[lvl][1] &&
-        sz < G.ncl[lvl][2]) {
-      long gs = G.off[lvl] + cell_encode(G, lvl, sx, sy, sz);
-      if (flags[gs]) s = slot[gs];
-    }
-    sr[j] = s;
+  pos[q] = e;
+}
+
+// The 189 source slots of every grouped target (-1 = no active cell there):
+// one thread per (target, offset), neighbouring threads on neighbouring
+// offsets of one target.
+__global__ void k_group_src(TreeGeom G, int lvl, const int *__restrict__ active, int a0, int na,
+                            const int *__restrict__ slot, const unsigned char *__restrict__ flags,
+                            const int *__restrict__ offs, const long *__restrict__ pos, int *src) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)na * kOct) return;
+  const int q = (int)(t / kOct), j = (int)(t % kOct);
+  long c = active[a0 + q] - G.off[lvl];
+  int ix, iy, iz;
+  cell_decode(G, lvl, c, ix, iy, iz);
+  const int oc = (ix & 1) + 2 * (iy & 1) + 4 * (iz & 1);
+  const int sx = ix + offs[(oc * kOct + j) * 3], sy = iy + offs[(oc * kOct + j) * 3 + 1],
+            sz = iz + offs[(oc * kOct + j) * 3 + 2];
+  int s = -1;
+  if (sx >= 0 && sy >= 0 && sz >= 0 && sx < G.ncl[lvl][0] && sy < G.ncl[lvl][1] &&
+      sz < G.ncl[lvl][2]) {
+    long gs = G.off[lvl] + cell_encode(G, lvl, sx, sy, sz);
+    if (flags[gs]) s = slot[gs];
   }
+  src[pos[q] * kOct + j] = s;
 }
 
 __global__ void k_to_f32(long n, const double *__restrict__ a, float *b) {
@@ -454,9 +470,8 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
       float v0 = 0.f, v1 = 0.f, v2 = 0.f, w0 = 0.f, w1 = 0.f, w2 = 0.f;
       float S0 = 0.f, S1 = 0.f, S2 = 0.f;
       unsigned long long nmask = 0ull;
-      const int nk = (ns + lpt - 1) / lpt;
-      for (int k = 0; k < nk; ++k) {
-        const int s = sub + k * lpt;
+      // one source pair of the tile: s = sub + k lpt
+      auto pair = [&](int s) {
         const f4 sp = ss[2 * s], sa = ss[2 * s + 1];
         const float r0 = fx - sp[0], r1 = fy - sp[1], r2 = fz - sp[2];
         const float a0 = sa[0], a1 = sa[1], a2 = sa[2];
@@ -477,19 +492,28 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
         if (VS) {
           // stretching -[cf (al x a) + df r (r.(al x a))]: sum cf a is crossed
           // with al after the tile; r.(al x a) = -al.(r x a), df = -3 cf / R^2
+          // (the factor 3 is applied after the tile)
           S0 += cf * a0;
           S1 += cf * a1;
           S2 += cf * a2;
-          const float tt = 3.f * cf * inv2 * (fa0 * c0 + fa1 * c1 + fa2 * c2);
-          w0 -= tt * r0;
-          w1 -= tt * r1;
-          w2 -= tt * r2;
+          const float tt = cf * inv2 * (fa0 * c0 + fa1 * c1 + fa2 * c2);
+          w0 += tt * r0;
+          w1 += tt * r1;
+          w2 += tt * r2;
         }
+      };
+      // wave-uniform trip count: scalar loop control, two pairs per iteration
+      const int nk = __builtin_amdgcn_readfirstlane((ns + lpt - 1) / lpt);
+      int k = 0;
+      for (; k + 2 <= nk; k += 2) {
+        pair(sub + k * lpt);
+        pair(sub + (k + 1) * lpt);
       }
+      if (k < nk) pair(sub + k * lpt);
       if (VS) {
-        w0 -= fa1 * S2 - fa2 * S1;
-        w1 -= fa2 * S0 - fa0 * S2;
-        w2 -= fa0 * S1 - fa1 * S0;
+        w0 = -3.f * w0 - (fa1 * S2 - fa2 * S1);
+        w1 = -3.f * w1 - (fa2 * S0 - fa0 * S2);
+        w2 = -3.f * w2 - (fa0 * S1 - fa1 * S0);
       }
       pv[0] += v0; pv[1] += v1; pv[2] += v2;
       ps[0] += w0; ps[1] += w1; ps[2] += w2;
@@ -537,12 +561,16 @@ int tuned_pass_prepare(FmmHandle *h) {
   }
   DUST_TRY(t->grp.reserve(std::max(tot, 1L)));
   DUST_TRY(t->src.reserve(std::max(tot, 1L) * kOct));
+  DUST_TRY(t->pos.reserve(std::max(h->n_active, 1)));
   for (int l = 1; l < h->nlev; ++l) {
     int na = h->act_start[l + 1] - h->act_start[l];
     if (na == 0) continue;
     k_group_targets<<<ceil_div(na, 256), 256, 0, h->stream>>>(
-        G, l, h->active.p, h->act_start[l], na, h->slot.p, h->flags.p, t->offs.p, t->grp.p,
-        t->src.p, t->gbase[l * 8], na, t->gcount.p);
+        G, l, h->active.p, h->act_start[l], na, h->slot.p, t->grp.p, t->pos.p + h->act_start[l],
+        t->gbase[l * 8], na, t->gcount.p);
+    k_group_src<<<ceil_div((long)na * kOct, 256), 256, 0, h->stream>>>(
+        G, l, h->active.p, h->act_start[l], na, h->slot.p, h->flags.p, t->offs.p,
+        t->pos.p + h->act_start[l], t->src.p);
   }
   DUST_HIP(hipMemcpyAsync(t->hcount.p, t->gcount.p, sizeof(int) * kMaxLevels * 8,
                           hipMemcpyDeviceToHost, h->stream));
