@@ -770,14 +770,30 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_eval_finish(EvalArgs A) 
   l2p_point(A.deg, nmon, A.use_vs, sb, sc, spw, x - cx, y - cy, z - cz, v, gr);
   if (A.use_vs)  // str = matmul(alpha, transpose(grad)): str_j = sum_i grad(j,i) alpha_i
     for (int jj = 0; jj < 3; ++jj) st[jj] = gr[jj] * al0 + gr[jj + 3] * al1 + gr[jj + 6] * al2;
+  // partial sums of the unit's items, added in item order; the loads of four
+  // items are issued together (units of large leaves have hundreds of items)
   double pv[3] = {0, 0, 0}, ps[3] = {0, 0, 0};
-  for (int k = 0; k < U.nseg; ++k) {
-    const double *o = &A.part[((long)(U.item0 + k) * kWave + lane) * 6];
-    for (int e = 0; e < 3; ++e) {
-      pv[e] += o[e];
-      ps[e] += o[3 + e];
-    }
+  const double *o = &A.part[((long)U.item0 * kWave + lane) * 6];
+  int k = 0;
+  for (; k + 4 <= U.nseg; k += 4) {
+    double q[4][6];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int e = 0; e < 6; ++e) q[u][e] = o[(long)(k + u) * kWave * 6 + e];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        pv[e] += q[u][e];
+        ps[e] += q[u][3 + e];
+      }
   }
+  for (; k < U.nseg; ++k)
+    for (int e = 0; e < 3; ++e) {
+      pv[e] += o[(long)k * kWave * 6 + e];
+      ps[e] += o[(long)k * kWave * 6 + 3 + e];
+    }
   for (int e = 0; e < 3; ++e) {
     A.vel[3 * t + e] = v[e] + pv[e] + A.uinf[e];
     A.str[3 * t + e] = st[e] + ps[e];
