@@ -45,7 +45,8 @@ def fmm_handle(O, tree, degree, n, form):
                                precision=form)
 
 
-def time_fmm(torch, L, O, tree, degree, dev, n, form, steps, warmup, knobs=None):
+def time_fmm(torch, L, O, tree, degree, dev, n, form, steps, warmup, knobs=None,
+             autotune=False):
     """Returns dict with ms per pass, stage times and the work counts."""
     g = fmm_handle(O, tree, degree, n, form)
     for k, v in (knobs or {}).items():
@@ -56,6 +57,11 @@ def time_fmm(torch, L, O, tree, degree, dev, n, form, steps, warmup, knobs=None)
                                            dev["mag"].data_ptr(), dev["rv"].data_ptr()), "set")
     import ctypes as C
     nl = C.c_int()
+    tuning = None
+    if autotune:  # untimed: picks the performance knobs for this workload
+        tuning = dict(autotune_ms=g.autotune(reps=2), seg_iters=g.get_tuning("seg_iters"))
+        if form == "tuned":
+            tuning["m2l_chunk"] = g.get_tuning("m2l_chunk")
     for _ in range(warmup):
         L.check(lib.dust_fmm_pass_dev(h, C.byref(nl)), "pass")
     # one untimed round with stage events on, so the event pool already holds
@@ -85,7 +91,8 @@ def time_fmm(torch, L, O, tree, degree, dev, n, form, steps, warmup, knobs=None)
     L.check(lib.dust_fmm_synchronize(h), "sync")
     res = dict(ms=wall, n_leaves=nl.value, stages={k: v[0] / steps for k, v in stages.items()},
                stage_launches={k: v[1] for k, v in stages.items()}, p2p_pairs=p2p_pairs,
-               m2l_pairs=m2l_pairs, vel=vel.cpu().numpy(), stretch=st.cpu().numpy())
+               m2l_pairs=m2l_pairs, vel=vel.cpu().numpy(), stretch=st.cpu().numpy(),
+               tuning=tuning)
     g.destroy()
     return res
 
@@ -411,6 +418,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--near", type=float, default=1.0,
                     help="tuned form: FP64 for pairs closer than this many core radii")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the library's autotuner (untimed) before timing the tuned form")
     ap.add_argument("--m2l-chunk", type=int, default=None,
                     help="tuned form: interaction offsets per M2L job (1..189)")
     ap.add_argument("--no-fp64", action="store_true", help="skip timing the plain FP64 form")
@@ -458,7 +467,8 @@ def main():
     log("timing tuned FMM")
     tuned = time_fmm(torch, L, O, tree, args.degree, dev, n, "tuned", args.steps, args.warmup,
                      knobs=dict({"near_factor": args.near},
-                                **({"m2l_chunk": args.m2l_chunk} if args.m2l_chunk else {})))
+                                **({"m2l_chunk": args.m2l_chunk} if args.m2l_chunk else {})),
+                     autotune=args.autotune)
     log(f"tuned: {tuned['ms']:.2f} ms/pass stages {tuned['stages']}")
     value = n / (tuned["ms"] / 1e3)
     # roofline of the dominant kernel of the tuned pass: FP32 P2P items
@@ -526,6 +536,7 @@ def main():
                                f"stretching; AIC LU N={args.lu_n}, 8 RHS",
                    "form": "tuned", "parallelism": "single GPU"},
         "tuned_path": {"ms_per_step": tuned["ms"], "stages_ms": tuned["stages"],
+                       "tuning": tuned["tuning"],
                        "n_leaves": tuned["n_leaves"], "m2l_TFLOPs": m2l_ach},
         "fp64_path": None if plain is None else {
             "value": n / (plain["ms"] / 1e3), "ms_per_step": plain["ms"],

@@ -555,9 +555,9 @@ __global__ void k_l2l(TreeGeom G, int lvl, int deg, int nmon, int use_vs,
 // ---------------------------------------------------------------------------
 
 // Work units: every leaf's targets in chunks of <= 64 (see fmm_eval.h); each
-// unit is cut into items of about kSegIters source iterations per lane.
+// unit is cut into items of about seg_iters source iterations per lane.
 __global__ void k_units(TreeGeom G, int n_leaves, const int *__restrict__ leaves,
-                        const int *__restrict__ fstart, const int *__restrict__ tot,
+                        const int *__restrict__ fstart, const int *__restrict__ tot, int seg_iters,
                         EvalUnit *units, int *n_units, int *n_items) {
   int lv = blockIdx.x * blockDim.x + threadIdx.x;
   if (lv >= n_leaves) return;
@@ -573,7 +573,7 @@ __global__ void k_units(TreeGeom G, int n_leaves, const int *__restrict__ leaves
     int lpt = 1;
     while (lpt * 2 * nt <= kWave) lpt *= 2;
     int iters = (T + lpt - 1) / lpt;
-    int nseg = max(1, (iters + kSegIters - 1) / kSegIters);
+    int nseg = max(1, (iters + seg_iters - 1) / seg_iters);
     int i0 = atomicAdd(n_items, nseg);
     int w = atomicAdd(n_units, 1);
     units[w] = EvalUnit{lv, r.x + q * kWave, nt, lpt, i0, nseg, 0, 0};
@@ -1197,8 +1197,8 @@ static int sort_phase_b(FmmHandle *h) {
   DUST_HIP(hipMemsetAsync(h->n_units.p, 0, 2 * sizeof(int), h->stream));
   if (h->n_leaves > 0)
     k_units<<<ceil_div(h->n_leaves, TB), TB, 0, h->stream>>>(
-        G, h->n_leaves, h->leaves.p, h->fstart.p, h->p2p_tot.p, (EvalUnit *)h->units.p,
-        h->n_units.p, h->n_units.p + 1);
+        G, h->n_leaves, h->leaves.p, h->fstart.p, h->p2p_tot.p, h->seg_iters,
+        (EvalUnit *)h->units.p, h->n_units.p, h->n_units.p + 1);
   // expansion storage
   size_t na = (size_t)std::max(h->n_active, 1);
   DUST_TRY(h->mp_a.reserve(na * 3 * h->nmon));
@@ -1527,7 +1527,79 @@ int dust_fmm_synchronize(dust_fmm_t h) {
 
 int dust_fmm_set_tuning(dust_fmm_t h, const char *name, double value) {
   DUST_CHECK(h && name, DUST_E_ARG, "null argument");
+  if (!strcmp(name, "seg_iters")) {  // both forms
+    DUST_CHECK(value >= 64.0 && value <= 65536.0, DUST_E_ARG, "seg_iters must be in 64..65536");
+    h->h->seg_iters = (int)value;
+    return 0;
+  }
   return tuned_set(h->h, name, value);
+}
+
+int dust_fmm_get_tuning(dust_fmm_t h, const char *name, double *value) {
+  DUST_CHECK(h && name && value, DUST_E_ARG, "null argument");
+  if (!strcmp(name, "seg_iters")) {
+    *value = h->h->seg_iters;
+    return 0;
+  }
+  return tuned_get(h->h, name, value);
+}
+
+// Times full passes over the particles already in the handle for a small grid
+// of the performance knobs (P2P item length; M2L offsets per job in the tuned
+// form) and keeps the fastest setting.  Knobs that change the numbers
+// (near_factor) are left alone.  The handle's results are those of the last
+// timed pass, not of the chosen setting: run a pass afterwards.
+int dust_fmm_autotune(dust_fmm_t hh, int reps, double *best_ms) {
+  DUST_CHECK(hh, DUST_E_ARG, "null handle");
+  FmmHandle *h = hh->h;
+  DUST_CHECK(h->n > 0, DUST_E_ARG, "autotune needs particles in the handle");
+  reps = std::max(1, std::min(reps, 20));
+  const int segs[] = {768, 1536, 3072};
+  const int chunks[] = {7, 14, 21, 27};
+  const bool tuned = h->tuned != nullptr;
+  const int nchunk = tuned ? 4 : 1;
+  hipEvent_t e0, e1;
+  DUST_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    set_error("hipEventCreate failed");
+    return DUST_E_HIP;
+  }
+  double best = INFINITY, chunk0 = 0.0;
+  int best_seg = h->seg_iters, best_chunk = 0;
+  if (tuned) DUST_TRY(tuned_get(h, "m2l_chunk", &chunk0));
+  best_chunk = (int)chunk0;
+  int rc = 0;
+  for (int a = 0; a < 3 && !rc; ++a)
+    for (int b = 0; b < nchunk && !rc; ++b) {
+      h->seg_iters = segs[a];
+      if (tuned) rc = tuned_set(h, "m2l_chunk", chunks[b]);
+      double tmin = INFINITY;
+      for (int r = 0; r <= reps && !rc; ++r) {  // r = 0 warms the buffers up
+        int nl = 0;
+        if ((rc = hipEventRecord(e0, h->stream) != hipSuccess ? DUST_E_HIP : 0)) break;
+        if ((rc = fmm_sort(h, &nl)) || (rc = fmm_multipole(h)) || (rc = fmm_apply(h))) break;
+        if (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+          rc = DUST_E_HIP;
+          break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (r > 0) tmin = std::min(tmin, (double)ms);
+      }
+      if (!rc && tmin < best) {
+        best = tmin;
+        best_seg = segs[a];
+        best_chunk = tuned ? chunks[b] : 0;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h->seg_iters = best_seg;
+  if (tuned && best_chunk > 0) DUST_TRY(tuned_set(h, "m2l_chunk", best_chunk));
+  if (rc) return rc;
+  if (best_ms) *best_ms = best;
+  return 0;
 }
 
 int dust_fmm_set_timing(dust_fmm_t h, int enable) {

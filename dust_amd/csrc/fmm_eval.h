@@ -12,7 +12,6 @@ namespace dust {
 
 constexpr int kWave = 64;
 constexpr int kEvalWaves = 4;     // wavefronts per block, each on its own item
-constexpr int kSegIters = 1536;   // source iterations per lane in one item
 
 // A unit: up to 64 targets of one leaf, evaluated by nseg items.
 struct EvalUnit {

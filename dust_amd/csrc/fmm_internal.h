@@ -8,6 +8,7 @@
 namespace dust {
 
 constexpr int kMaxLevels = 10;
+constexpr int kSegIters = 1536;  // default P2P source iterations per lane in one item
 
 // Cell flags, the state sort_particles leaves in t_cell%leaf/%branch
 // (src/fmm/mod_octree.f90:93-100); active == leaf or branch.
@@ -101,6 +102,7 @@ struct FmmHandle {
   void *tuned = nullptr;
   hipStream_t stream = nullptr;
   StageTimer timer;
+  int seg_iters = kSegIters;  // P2P source iterations per lane in one item
   bool sorted = false, computed = false;
   // page-locked source/targets of the asynchronous copies between pass phases
   HostBuf<FmmReadback> rb;
@@ -122,6 +124,7 @@ int eval_finish_launch(FmmHandle *h, const EvalArgs &A);
 int tuned_prepare(FmmHandle *h);
 int tuned_pass_prepare(FmmHandle *h);
 int tuned_set(FmmHandle *h, const char *name, double value);
+int tuned_get(FmmHandle *h, const char *name, double *value);
 int tuned_m2l_all(FmmHandle *h);
 int tuned_eval(FmmHandle *h, EvalArgs &A);
 void tuned_release(FmmHandle *h);

@@ -84,6 +84,21 @@ int tuned_prepare(FmmHandle *h) {
 }
 
 // Knobs of the tuned form (set by the autotuner or the caller).
+int tuned_get(FmmHandle *h, const char *name, double *value) {
+  TunedState *t = TS(h);
+  DUST_CHECK(t, DUST_E_ARG, "the handle was not created with DUST_FMM_TUNED");
+  if (!strcmp(name, "near_factor")) {
+    *value = t->near_factor;
+    return 0;
+  }
+  if (!strcmp(name, "m2l_chunk")) {
+    *value = t->m2l_chunk;
+    return 0;
+  }
+  set_error("unknown tuning knob '%s'", name);
+  return DUST_E_ARG;
+}
+
 int tuned_set(FmmHandle *h, const char *name, double value) {
   TunedState *t = TS(h);
   DUST_CHECK(t, DUST_E_ARG, "the handle was not created with DUST_FMM_TUNED");

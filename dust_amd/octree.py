@@ -116,8 +116,21 @@ class Octree:
         return p.value, m.value
 
     def set_tuning(self, name, value):
-        """Knob of the tuned form (e.g. near_factor)."""
+        """Knob: near_factor, m2l_chunk (tuned form), seg_iters (both forms)."""
         L.check(L.lib().dust_fmm_set_tuning(self.handle, name.encode(), float(value)), "set_tuning")
+
+    def get_tuning(self, name):
+        v = C.c_double()
+        L.check(L.lib().dust_fmm_get_tuning(self.handle, name.encode(), C.byref(v)), "get_tuning")
+        return v.value
+
+    def autotune(self, reps=3):
+        """Time full passes over the particles already set for a grid of the
+        performance knobs and keep the fastest; returns its ms per pass.  Run
+        a pass afterwards before reading results."""
+        ms = C.c_double()
+        L.check(L.lib().dust_fmm_autotune(self.handle, int(reps), C.byref(ms)), "autotune")
+        return ms.value
 
     def set_timing(self, on):
         L.check(L.lib().dust_fmm_set_timing(self.handle, int(bool(on))), "set_timing")

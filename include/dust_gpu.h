@@ -113,12 +113,24 @@ int dust_fmm_pass_dev(dust_fmm_t h, int *nleaves);
  * runs); for many-case workloads. */
 int dust_fmm_pass_batch_dev(int nh, dust_fmm_t *handles, int *nleaves);
 
-/* Knobs of the tuned form: "near_factor" -- pairs closer than near_factor
- * times the source core radius are evaluated in FP64 (default 1);
- * "m2l_chunk" -- interaction offsets per M2L job, 1..189 (default 21): larger
- * chunks mean fewer atomic additions into the local expansions and fewer,
- * longer jobs. */
+/* Knobs.  Tuned form: "near_factor" -- pairs closer than near_factor times
+ * the source core radius are evaluated in FP64 (default 1); "m2l_chunk" --
+ * interaction offsets per M2L job, 1..189 (default 21): larger chunks mean
+ * fewer atomic additions into the local expansions and fewer, longer jobs.
+ * Both forms: "seg_iters" -- P2P source iterations per lane in one work
+ * item, 64..65536 (default 1536).  No counterpart in the reference. */
 int dust_fmm_set_tuning(dust_fmm_t h, const char *name, double value);
+int dust_fmm_get_tuning(dust_fmm_t h, const char *name, double *value);
+
+/* Autotuner of the performance knobs (seg_iters; m2l_chunk in the tuned
+ * form): times `reps` full passes (sort + multipole + apply) over the
+ * particles already set in the handle for each setting of a small grid and
+ * keeps the fastest; *best_ms (may be NULL) receives its time per pass.
+ * near_factor, which changes the results, is not touched.  The handle's
+ * outputs afterwards are those of the last timed pass: run a pass before
+ * reading them.  No counterpart in the reference (a one-off cost per run
+ * configuration, e.g. at the first time step). */
+int dust_fmm_autotune(dust_fmm_t h, int reps, double *best_ms);
 
 /* HIP stream (hipStream_t) the handle launches on. */
 void *dust_fmm_stream(dust_fmm_t h);

@@ -250,3 +250,51 @@ def test_tuned_m2l_chunk_knob(orc):
     for chunk in (7, 189):
         assert rel(out[chunk][0], out[None][0]) <= 5e-7
         assert rel(out[chunk][1], out[None][1]) <= 5e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "tuned"])
+def test_autotune(precision):
+    """The autotuner only picks performance knobs from its grid; a pass with
+    the chosen setting gives the default setting's results up to the
+    regrouping of partial sums (FP64 items; FP32 M2L jobs in the tuned
+    form)."""
+    cen, alpha, rv, tree = W.helix_wake(20000, seed=13)
+    tree = dict(tree, nlevels=5)
+
+    def handle():
+        return O.initialize_octree(tree["box_length"], tree["nbox"], tree["origin"],
+                                   tree["nlevels"], tree["min_part"], 4, tree["delta"],
+                                   capacity=len(cen), use_vs=True, precision=precision)
+
+    def run(g):
+        parts = O.VortParts.from_alpha(cen, alpha, rv)
+        O.sort_particles(parts, g)
+        O.calculate_multipole(parts, g)
+        O.apply_multipole(parts, g)
+        return parts.vel.copy(), parts.stretch.copy()
+
+    g0 = handle()
+    v0, s0 = run(g0)
+    g0.destroy()
+    g = handle()
+    parts = O.VortParts.from_alpha(cen, alpha, rv)
+    O.sort_particles(parts, g)
+    print(f"autotune {precision}", flush=True)
+    ms = g.autotune(reps=1)
+    seg = g.get_tuning("seg_iters")
+    print(f"{precision}: {ms:.3f} ms/pass, seg_iters {seg}")
+    assert 0.0 < ms < 1e4
+    assert seg in (768, 1536, 3072)
+    if precision == "tuned":
+        assert g.get_tuning("m2l_chunk") in (7, 14, 21, 27)
+        assert g.get_tuning("near_factor") == 1.0
+    else:
+        with pytest.raises(L.DustError):
+            g.get_tuning("m2l_chunk")
+    with pytest.raises(L.DustError):
+        g.set_tuning("seg_iters", 10)
+    v, s = run(g)
+    g.destroy()
+    tol_v, tol_s = (1e-13, 1e-13) if precision == "fp64" else (5e-7, 5e-6)
+    assert rel(v, v0) <= tol_v
+    assert rel(s, s0) <= tol_s
