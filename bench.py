@@ -97,13 +97,17 @@ def time_fmm(torch, L, O, tree, degree, dev, n, form, steps, warmup, knobs=None,
     return res
 
 
-def time_lu(torch, L, W, n_req, nrhs, steps):
+def time_lu(torch, L, W, n_req, nrhs, steps, mesh=None):
     """AIC assembly, then plain FP64 getrf + getrs and the mixed solves on the
-    same device-resident matrix and right-hand sides b = B s."""
+    same device-resident matrix and right-hand sides b = B s (the ellipsoid of
+    about n_req panels, or `mesh` = (ver, n_ver))."""
     import ctypes as C
     lib = L.lib()
-    m = int(round(np.sqrt(n_req)))
-    ver, nv = W.ellipsoid(m, m)
+    if mesh is None:
+        m = int(round(np.sqrt(n_req)))
+        ver, nv = W.ellipsoid(m, m)
+    else:
+        ver, nv = mesh
     n = len(nv)
     dver = torch.from_numpy(np.ascontiguousarray(ver.reshape(n, 12))).cuda()
     dnv = torch.from_numpy(nv.astype(np.int32)).cuda()
@@ -281,6 +285,25 @@ def time_batch(torch, L, O, W, ncase=48, reps=3):
     return out
 
 
+def time_wing(torch, L, O, W, steps):
+    """The wing configuration: NACA0012 wing of 1,600 surface panels with an
+    8,000-particle trailing sheet (order 4, 5 levels): FMM passes of the sheet
+    in both forms, AIC assembly and the dense solves of the wing system."""
+    ver, nv, cen, alpha, rv, tree = W.wing_case()
+    mag = np.linalg.norm(alpha, axis=1)
+    dev = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda()
+           for k, v in dict(cen=cen, dir=alpha / mag[:, None], mag=mag, rv=rv).items()}
+    n = len(cen)
+    f64 = time_fmm(torch, L, O, tree, 4, dev, n, "fp64", steps, 1)
+    tun = time_fmm(torch, L, O, tree, 4, dev, n, "tuned", steps, 1)
+    lu, _ = time_lu(torch, L, W, None, 8, steps, mesh=(ver, nv))
+    return dict(panels=len(nv), particles=n, fmm_fp64_ms=f64["ms"], fmm_tuned_ms=tun["ms"],
+                fmm_tuned_vel_rel_l2_vs_fp64=float(np.linalg.norm(tun["vel"] - f64["vel"]) /
+                                                   np.linalg.norm(f64["vel"])),
+                aic_ms=lu["assembly_ms"], lu_fp64_ms=lu["fp64_ms"], lu_fp16s_ms=lu["fp16s_ms"],
+                lu_fp16s_iters=lu["fp16s_iters"], lu_fp16s_backward_error=lu["fp16s_backward_error"])
+
+
 def cpu_baseline(W, n_sample, degree):
     """The CPU oracle (FP64 restatement, OpenMP) on a bounded sample of the same
     generator; particle updates per second."""
@@ -425,6 +448,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip timing the plain FP64 form")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-case batch leg")
+    ap.add_argument("--no-wing", action="store_true", help="skip the wing configuration leg")
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB",
                                                           "COPY_WRITE_DB"))
     ap.add_argument("--summarize-trace", metavar="TRACE_DB")
@@ -498,6 +522,11 @@ def main():
         except Exception as e:
             log(f"LU timing failed: {e}")
             lu = dict(error=str(e))
+    wing = None
+    if not args.no_wing:
+        log("timing the wing configuration")
+        wing = time_wing(torch, L, O, W, 3)
+        log(f"wing: {wing}")
     batch = None
     if not args.no_batch:
         log("timing the 48-case batch")
@@ -537,7 +566,8 @@ def main():
                    "form": "tuned", "parallelism": "single GPU"},
         "tuned_path": {"ms_per_step": tuned["ms"], "stages_ms": tuned["stages"],
                        "tuning": tuned["tuning"],
-                       "n_leaves": tuned["n_leaves"], "m2l_TFLOPs": m2l_ach},
+                       "n_leaves": tuned["n_leaves"], "m2l_TFLOPs": m2l_ach,
+                       "eval_only_updates_per_s": n / ((tuned["ms"] - st["sort"]) / 1e3)},
         "fp64_path": None if plain is None else {
             "value": n / (plain["ms"] / 1e3), "ms_per_step": plain["ms"],
             "stages_ms": plain["stages"], "speedup_tuned_vs_fp64": plain["ms"] / tuned["ms"]},
@@ -545,6 +575,7 @@ def main():
         "work": {"p2p_pairs": tuned["p2p_pairs"], "m2l_pairs": tuned["m2l_pairs"]},
         "roofline": roof,
         "lu": lu,
+        "wing": wing,
         "batch": batch,
         "cpu_baseline": cpu,
     }
