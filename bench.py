@@ -435,6 +435,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--degree", type=int, default=4)
+    ap.add_argument("--levels", type=int, default=None, help="octree levels (default 7)")
     ap.add_argument("--lu-n", type=int, default=16384)
     ap.add_argument("--no-lu", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -478,6 +479,8 @@ def main():
     n = args.n
     log(f"generating helix wake n={n}")
     cen, alpha, rv, tree = W.helix_wake(n)
+    if args.levels:
+        tree = dict(tree, nlevels=args.levels)
     mag = np.linalg.norm(alpha, axis=1)
     dr = alpha / mag[:, None]
     dev = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda()
