@@ -471,6 +471,44 @@ __global__ void k_mx_laswp(LaswpItem one, const LaswpItem *__restrict__ items,
 constexpr int LDX = 64 + 1;   // LDS leading dimension of the 64-wide tile (floats)
 constexpr int LDT = NB + 1;   // LDS leading dimension of the 128x128 triangle
 
+// Column-major FP32 tile (ROWS x COLS, ROWS a multiple of 4; source and
+// leading dimension 16-byte aligned) -> LDS with leading dimension sld, by a
+// 256-thread workgroup: 16-byte loads, eight in flight per thread before the
+// LDS stores.
+template <int ROWS, int COLS>
+__device__ __forceinline__ void tile_to_lds(const float *__restrict__ src, long ld, float *dst,
+                                            int sld, int tid) {
+  constexpr int R4 = ROWS / 4, PER = ROWS * COLS / 4 / 256, B = PER < 8 ? PER : 8;
+  static_assert(ROWS % 4 == 0 && (ROWS * COLS / 4) % 256 == 0 && PER % B == 0, "tile shape");
+#pragma unroll
+  for (int b0 = 0; b0 < PER; b0 += B) {
+    float4 v[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int q = tid + 256 * (b0 + u), c = q / R4, r = (q % R4) * 4;
+      v[u] = *(const float4 *)(src + r + ld * c);
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int q = tid + 256 * (b0 + u), c = q / R4, r = (q % R4) * 4;
+      float *d = dst + c * sld + r;
+      d[0] = v[u].x;
+      d[1] = v[u].y;
+      d[2] = v[u].z;
+      d[3] = v[u].w;
+    }
+  }
+}
+
+// 4 x 32 x 32 inverse sub-blocks (contiguous, 16-byte aligned) -> LDS.
+__device__ __forceinline__ void inv_to_lds(const float *__restrict__ src, float *dst, int tid) {
+  float4 v[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) v[u] = ((const float4 *)src)[tid + 256 * u];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) ((float4 *)dst)[tid + 256 * u] = v[u];
+}
+
 struct TrsmArgs {
   float *W;
   long ld;
@@ -545,18 +583,16 @@ __global__ void __launch_bounds__(256) k_mx_trsm(TrsmArgs one, const TrsmArgs *_
     float *sU = sX + NB * LDX;        // 128 x LDT (upper triangle used)
     float *sI = sU + NB * LDT;        // 4 x 32 x 32 (Uinv sub-blocks)
     const int R0 = (a.mode == 0) ? k + NB + 64 * bx : 64 * bx;
-    for (int q = tid; q < NB * 64; q += 256) {
-      int r = q & 63, c = q >> 6;
-      float v;
-      if (a.mode == 0) v = a.W[R0 + r + a.ld * (k + c)];
-      else v = (R0 + r == c) ? 1.f : 0.f;
-      sX[c * LDX + r] = v;
+    if (a.mode == 0) {
+      tile_to_lds<64, NB>(a.W + R0 + a.ld * k, a.ld, sX, LDX, tid);
+    } else {
+      for (int q = tid; q < NB * 64; q += 256) {
+        int r = q & 63, c = q >> 6;
+        sX[c * LDX + r] = (R0 + r == c) ? 1.f : 0.f;
+      }
     }
-    for (int q = tid; q < NB * NB; q += 256) {
-      int r = q & (NB - 1), c = q >> 7;
-      sU[c * LDT + r] = Dg[r + a.ld * c];
-    }
-    for (int q = tid; q < 4 * SP * SP; q += 256) sI[q] = inv[4 * SP * SP + q];
+    tile_to_lds<NB, NB>(Dg, a.ld, sU, LDT, tid);
+    inv_to_lds(inv + 4 * SP * SP, sI, tid);
     __syncthreads();
     const int r16 = 16 * wv;  // the wave's 16 rows
     for (int s = 0; s < NB / SP; ++s) {
@@ -627,18 +663,16 @@ __global__ void __launch_bounds__(256) k_mx_trsm(TrsmArgs one, const TrsmArgs *_
     float *sI = sL + NB * LDT;        // 4 x 32 x 32 (Linv sub-blocks)
     const int ub = bx - a.nl;
     const int C0 = (a.mode == 0) ? k + NB + 64 * ub : 64 * ub;
-    for (int q = tid; q < NB * 64; q += 256) {
-      int r = q & (NB - 1), c = q >> 7;
-      float v;
-      if (a.mode == 0) v = a.W[k + r + a.ld * (C0 + c)];
-      else v = (r == C0 + c) ? 1.f : 0.f;
-      sY[c * LDT + r] = v;
+    if (a.mode == 0) {
+      tile_to_lds<NB, 64>(a.W + k + a.ld * C0, a.ld, sY, LDT, tid);
+    } else {
+      for (int q = tid; q < NB * 64; q += 256) {
+        int r = q & (NB - 1), c = q >> 7;
+        sY[c * LDT + r] = (r == C0 + c) ? 1.f : 0.f;
+      }
     }
-    for (int q = tid; q < NB * NB; q += 256) {
-      int r = q & (NB - 1), c = q >> 7;
-      sL[c * LDT + r] = Dg[r + a.ld * c];
-    }
-    for (int q = tid; q < 4 * SP * SP; q += 256) sI[q] = inv[q];
+    tile_to_lds<NB, NB>(Dg, a.ld, sL, LDT, tid);
+    inv_to_lds(inv, sI, tid);
     __syncthreads();
     const int c16 = 16 * wv;  // the wave's 16 columns
     for (int s = 0; s < NB / SP; ++s) {
