@@ -239,18 +239,23 @@ __device__ __forceinline__ void subpanel_step(double (&a0)[SP], double (&a1)[SP]
   }
   pos0 = pos0 == g ? p : (pos0 == p ? g : pos0);
   pos1 = pos1 == g ? p : (pos1 == p ? g : pos1);
-  const double rp = 1.0 / piv;
-  if (pos0 > g) {
-    const double l = a0[J] * rp;
-    a0[J] = l;
-#pragma unroll
-    for (int c = J + 1; c < SP; ++c) a0[c] = fma(-l, u[c], a0[c]);
+  // 1 / piv: hardware reciprocal and two Newton steps
+  double rp = __builtin_amdgcn_rcp(piv);
+  rp = fma(fma(-piv, rp, 1.0), rp, rp);
+  rp = fma(fma(-piv, rp, 1.0), rp, rp);
+  // multipliers and column J + 1 (the next pivot column) first
+  const double l0 = pos0 > g ? a0[J] * rp : 0.0;
+  const double l1 = pos1 > g ? a1[J] * rp : 0.0;
+  if (pos0 > g) a0[J] = l0;
+  if (pos1 > g) a1[J] = l1;
+  if constexpr (J + 1 < SP) {
+    a0[J + 1] = fma(-l0, u[J + 1], a0[J + 1]);
+    a1[J + 1] = fma(-l1, u[J + 1], a1[J + 1]);
   }
-  if (pos1 > g) {
-    const double l = a1[J] * rp;
-    a1[J] = l;
 #pragma unroll
-    for (int c = J + 1; c < SP; ++c) a1[c] = fma(-l, u[c], a1[c]);
+  for (int c = J + 2; c < SP; ++c) {
+    a0[c] = fma(-l0, u[c], a0[c]);
+    a1[c] = fma(-l1, u[c], a1[c]);
   }
 }
 
@@ -289,6 +294,8 @@ __global__ void __launch_bounds__(256) k_mx_diag(DiagArgs one, const DiagArgs *_
   extern __shared__ double sD[];  // NB x LDD, column-major
   __shared__ int spiv[NB];
   __shared__ int sorig[NB];
+  __shared__ double sLi[SP * SP];  // inverse of the sub-panel's unit-lower L11, column-major
+  __shared__ double srd[NB];       // 1 / U(i, i)
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   // the block, 16 bytes per load, all loads in flight before the LDS stores
   // (W and its leading dimension are multiples of 128 floats)
@@ -316,7 +323,56 @@ __global__ void __launch_bounds__(256) k_mx_diag(DiagArgs one, const DiagArgs *_
     const int c0 = sp * SP;
     if (wv == 0) diag_subpanel(sD, sorig, c0, lane, spiv, a.flags);
     __syncthreads();
-    // interchanges of this sub-panel on the other 96 columns
+    // interchanges of this sub-panel on the other 96 columns; meanwhile wave 3
+    // inverts the sub-panel's L11 (final now: later interchanges only touch
+    // rows below it) blockwise: L11 = [A 0; C D] (16 x 16 blocks), inverse
+    // [A^-1 0; -D^-1 C A^-1  D^-1]; lanes 0-15 / 16-31 take a column of A^-1 /
+    // D^-1 by forward substitution, then two FP64 MFMA products
+    if (wv == 3) {
+      const int h = lane >> 4, c = lane & 15;
+      if (lane < 32) {
+        const int b0 = c0 + 16 * h;
+        double x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const double *col = sD + (b0 + i) * LDD + b0;  // unit lower, rows i+1..
+#pragma unroll
+          for (int r = i + 1; r < 16; ++r) x[r] = fma(-col[r], x[i], x[r]);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sLi[(16 * h + c) * SP + 16 * h + i] = x[i];
+      }
+      // T = C A^-1, then the lower-left block -D^-1 T (f64 16x16x4 layouts:
+      // A operand row lane % 16, k lane / 16; B operand k lane / 16, column
+      // lane % 16; output column lane % 16, rows 4 i + lane / 16)
+      d4 t = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kk = 0; kk < 16; kk += 4) {
+        const double av = sD[(c0 + kk + (lane >> 4)) * LDD + c0 + 16 + (lane & 15)];  // C
+        const double bv = sLi[(lane & 15) * SP + kk + (lane >> 4)];                   // A^-1
+        t = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, t, 0, 0, 0);
+      }
+      // T to the (still zero) upper-right block of sLi as scratch, then -D^-1 T
+#pragma unroll
+      for (int i = 0; i < 4; ++i) sLi[(16 + (lane & 15)) * SP + 4 * i + (lane >> 4)] = t[i];
+      d4 xo = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kk = 0; kk < 16; kk += 4) {
+        const double av = -sLi[(16 + kk + (lane >> 4)) * SP + 16 + (lane & 15)];  // -D^-1
+        const double bv = sLi[(16 + (lane & 15)) * SP + kk + (lane >> 4)];        // T (k, col)
+        xo = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, xo, 0, 0, 0);
+      }
+      // lower-left block (rows 16.., columns ..15); the upper-right block is zero
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        sLi[(lane & 15) * SP + 16 + 4 * i + (lane >> 4)] = xo[i];
+        sLi[(16 + (lane & 15)) * SP + 4 * i + (lane >> 4)] = 0.0;
+      }
+      float *Li = a.dinv + ((long)(a.k / NB) * 8 + sp) * SP * SP;
+      for (int q = lane; q < SP * SP; q += 64) Li[q] = (float)sLi[q];
+    }
     if (tid < NB - SP) {
       const int col = tid < c0 ? tid : tid + SP;
       double *cp = sD + col * LDD;
@@ -332,18 +388,25 @@ __global__ void __launch_bounds__(256) k_mx_diag(DiagArgs one, const DiagArgs *_
     __syncthreads();
     const int m2 = NB - c0 - SP;  // rows/columns after the sub-panel
     if (m2 == 0) break;
-    // U12 of the sub-panel: unit-lower forward substitution, one column per thread
-    if (tid < m2) {
-      double *cp = sD + (c0 + SP + tid) * LDD + c0;
-      double x[SP];
+    // U12 of the sub-panel = L11^-1 A12 on FP64 MFMA; a wave owns whole
+    // 16-column blocks (it reads all 32 rows of a block before writing it)
+    for (int tj = wv; tj < m2 / 16; tj += 4) {
+      const int cc0 = c0 + SP + 16 * tj;
+      d4 acc0 = (d4){0.0, 0.0, 0.0, 0.0}, acc1 = (d4){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int i = 0; i < SP; ++i) x[i] = cp[i];
+      for (int kk = 0; kk < SP; kk += 4) {
+        const double bv = sD[(cc0 + (lane & 15)) * LDD + c0 + kk + (lane >> 4)];
+        const double av0 = sLi[(kk + (lane >> 4)) * SP + (lane & 15)];
+        const double av1 = sLi[(kk + (lane >> 4)) * SP + 16 + (lane & 15)];
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv, acc1, 0, 0, 0);
+      }
+      // f64 16x16x4 output: column lane % 16, rows 4 i + lane / 16
 #pragma unroll
-      for (int i = 0; i < SP; ++i)
-#pragma unroll
-        for (int r = i + 1; r < SP; ++r) x[r] = fma(-sD[(c0 + i) * LDD + c0 + r], x[i], x[r]);
-#pragma unroll
-      for (int i = 0; i < SP; ++i) cp[i] = x[i];
+      for (int i = 0; i < 4; ++i) {
+        sD[(cc0 + (lane & 15)) * LDD + c0 + 4 * i + (lane >> 4)] = acc0[i];
+        sD[(cc0 + (lane & 15)) * LDD + c0 + 16 + 4 * i + (lane >> 4)] = acc1[i];
+      }
     }
     __syncthreads();
     // trailing block: D22 -= L21 U12 on FP64 MFMA, 16x16 tiles over the waves
@@ -369,38 +432,26 @@ __global__ void __launch_bounds__(256) k_mx_diag(DiagArgs one, const DiagArgs *_
     }
     __syncthreads();
   }
-  // inverses of the 32x32 diagonal sub-blocks: wave w takes sub-block w;
-  // lanes 0..31 invert U (column c by back substitution), lanes 32..63 the
-  // unit-lower L (forward substitution), both in column (axpy) order so that
-  // the updates of one step are independent.
-  {
-    const int b0 = wv * SP;
-    const int c = lane & 31;
-    float *Li = a.dinv + ((long)(a.k / NB) * 8 + wv) * SP * SP;
-    float *Ui = a.dinv + ((long)(a.k / NB) * 8 + 4 + wv) * SP * SP;
+  // inverses of the four 32x32 diagonal sub-blocks of U (those of L were
+  // formed after each sub-panel): waves 0 and 1, two sub-blocks each, column
+  // c by back substitution in column order with the diagonal reciprocals
+  if (tid < NB) srd[tid] = 1.0 / sD[tid * LDD + tid];
+  __syncthreads();
+  if (wv < 2) {
+    const int sb = 2 * wv + (lane >> 5), c = lane & 31, b0 = sb * SP;
+    float *Ui = a.dinv + ((long)(a.k / NB) * 8 + 4 + sb) * SP * SP;
     double x[SP];
 #pragma unroll
     for (int i = 0; i < SP; ++i) x[i] = (i == c) ? 1.0 : 0.0;
-    if (lane < 32) {
 #pragma unroll
-      for (int i = SP - 1; i >= 0; --i) {
-        const double *col = sD + (b0 + i) * LDD + b0;  // U(0..i, i)
-        x[i] = x[i] / col[i];
+    for (int i = SP - 1; i >= 0; --i) {
+      const double *col = sD + (b0 + i) * LDD + b0;  // U(0..i, i)
+      x[i] *= srd[b0 + i];
 #pragma unroll
-        for (int r = 0; r < i; ++r) x[r] = fma(-col[r], x[i], x[r]);
-      }
-#pragma unroll
-      for (int i = 0; i < SP; ++i) Ui[i + SP * c] = (float)x[i];
-    } else {
-#pragma unroll
-      for (int i = 0; i < SP; ++i) {
-        const double *col = sD + (b0 + i) * LDD + b0;  // L(i+1.., i)
-#pragma unroll
-        for (int r = i + 1; r < SP; ++r) x[r] = fma(-col[r], x[i], x[r]);
-      }
-#pragma unroll
-      for (int i = 0; i < SP; ++i) Li[i + SP * c] = (float)x[i];
+      for (int r = 0; r < i; ++r) x[r] = fma(-col[r], x[i], x[r]);
     }
+#pragma unroll
+    for (int i = 0; i < SP; ++i) Ui[i + SP * c] = (float)x[i];
   }
   // pivots, permutation (sorig), interchange count, write-back
   if (tid < NB) {
@@ -741,24 +792,14 @@ struct GemmItem {
   long ld;
   int r0, c0;   // first row / column of the updated region
   int nrt, P;   // 128-row tiles of the region; block columns summed
+  int ntiles;   // tiles of the region (the persistent single launch)
   const void *Lf, *Uf;
   long pbytes;  // bytes per fragment panel
 };
 
-// BATCH: items[i] owns workgroups [starts[i], starts[i+1]), tile (x, y) =
-// (local % nrt, local / nrt); otherwise the grid is (row tiles, column tiles).
-template <int F16, int BATCH>
-__global__ void __launch_bounds__(256) k_mx_gemm(GemmItem one, const GemmItem *__restrict__ items,
-                                                 const int *__restrict__ starts, int m) {
-  int bx = blockIdx.x, by = blockIdx.y;
-  GemmItem g = one;
-  if (BATCH) {
-    const int i = batch_item(starts, m, bx);
-    g = items[i];
-    const int loc = bx - starts[i];
-    bx = loc % g.nrt;
-    by = loc / g.nrt;
-  }
+// One 128 x 128 tile (bx, by) of the region.
+template <int F16>
+__device__ __forceinline__ void gemm_tile(const GemmItem &g, int bx, int by) {
   float *W = g.W;
   const long ld = g.ld, pbytes = g.pbytes;
   const void *Lf = g.Lf, *Uf = g.Uf;
@@ -817,6 +858,23 @@ __global__ void __launch_bounds__(256) k_mx_gemm(GemmItem one, const GemmItem *_
       f4 v = *p;
       *p = v - acc[i][j];
     }
+}
+
+// BATCH: items[i] owns workgroups [starts[i], starts[i+1]), tile (x, y) =
+// (local % nrt, local / nrt).  Otherwise the workgroups take tiles t =
+// blockIdx.x, blockIdx.x + gridDim.x, ... of the ntiles, (x, y) = (t % nrt,
+// t / nrt), so that a launch can be held to a given number of workgroups.
+template <int F16, int BATCH>
+__global__ void __launch_bounds__(256) k_mx_gemm(GemmItem one, const GemmItem *__restrict__ items,
+                                                 const int *__restrict__ starts, int m) {
+  if (BATCH) {
+    const int i = batch_item(starts, m, blockIdx.x);
+    const GemmItem g = items[i];
+    const int loc = blockIdx.x - starts[i];
+    gemm_tile<F16>(g, loc % g.nrt, loc / g.nrt);
+  } else {
+    for (int t = blockIdx.x; t < one.ntiles; t += gridDim.x) gemm_tile<F16>(one, t % one.nrt, t / one.nrt);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1018,18 +1076,25 @@ struct MixedWs {
     unsigned err;                         // triangular-solve time-out
   };
   HostBuf<Readback> hb;
-  // second stream and events of the lookahead (default priority)
-  hipStream_t s2 = nullptr;
-  hipEvent_t ev_start = nullptr, ev_panel = nullptr, ev_rest = nullptr;
+  // streams of the single-system solve (default priority): s1 runs the solve,
+  // s2 the lookahead's trailing remainder on all CUs but a reserved few
+  hipStream_t s1 = nullptr, s2 = nullptr;
+  hipEvent_t ev_in = nullptr, ev_start = nullptr, ev_panel = nullptr, ev_rest = nullptr;
 };
 MixedWs g_ws;
 
+// Compute units of the current device (queried once per device).
 int mx_cus() {
+  static int cached[64] = {};
   int dev = 0;
   (void)hipGetDevice(&dev);
+  if (dev >= 0 && dev < 64 && cached[dev] > 0) return cached[dev];
   hipDeviceProp_t pr;
-  if (hipGetDeviceProperties(&pr, dev) != hipSuccess) return 64;
-  return pr.multiProcessorCount > 0 ? pr.multiProcessorCount : 64;
+  int n = 64;
+  if (hipGetDeviceProperties(&pr, dev) == hipSuccess && pr.multiProcessorCount > 0)
+    n = pr.multiProcessorCount;
+  if (dev >= 0 && dev < 64) cached[dev] = n;
+  return n;
 }
 
 size_t lds_diag() { return sizeof(double) * (size_t)NB * LDD; }
@@ -1048,11 +1113,13 @@ size_t lds_trsm() {
     }                                                                             \
   } while (0)
 
+// maxwg caps the workgroups (each then takes several tiles).
 int mx_gemm(int f16, float *W, long ld, int r0, int r1, int c0, int c1, const void *Lf,
-            const void *Uf, long pbytes, int P, hipStream_t st) {
+            const void *Uf, long pbytes, int P, hipStream_t st, int maxwg = 1 << 30) {
   if (r1 <= r0 || c1 <= c0 || P <= 0) return 0;
-  dim3 grid((r1 - r0) / 128, (c1 - c0) / 128);
-  GemmItem g{W, ld, r0, c0, (r1 - r0) / 128, P, Lf, Uf, pbytes};
+  const int nrt = (r1 - r0) / 128, ntiles = nrt * ((c1 - c0) / 128);
+  const int grid = std::min(ntiles, maxwg);
+  GemmItem g{W, ld, r0, c0, nrt, P, ntiles, Lf, Uf, pbytes};
   if (f16) k_mx_gemm<1, 0><<<grid, 256, 0, st>>>(g, nullptr, nullptr, 0);
   else k_mx_gemm<0, 0><<<grid, 256, 0, st>>>(g, nullptr, nullptr, 0);
   MX_LAUNCH_CHECK("update gemm");
@@ -1207,12 +1274,6 @@ int mx_reserve(MixedWs &w, int n, int np) {
   DUST_TRY(w.rowmax.reserve(np));
   DUST_TRY(w.Lf.reserve((size_t)2 * DEPTH * np * NB * sizeof(float)));
   DUST_TRY(w.Uf.reserve((size_t)2 * DEPTH * np * NB * sizeof(float)));
-  if (!w.s2) {
-    DUST_HIP(hipStreamCreateWithFlags(&w.s2, hipStreamNonBlocking));
-    DUST_HIP(hipEventCreateWithFlags(&w.ev_start, hipEventDisableTiming));
-    DUST_HIP(hipEventCreateWithFlags(&w.ev_panel, hipEventDisableTiming));
-    DUST_HIP(hipEventCreateWithFlags(&w.ev_rest, hipEventDisableTiming));
-  }
   DUST_TRY(w.piv.reserve(np));
   DUST_TRY(w.perm.reserve(np));
   DUST_TRY(w.npiv.reserve(nblk));
@@ -1431,10 +1492,43 @@ int run_init(MixedRun &r, MixedWs &w, int n, int nrhs, const double *dA, long ld
   return mx_reserve(w, n, r.np);
 }
 
+// Streams of the single-system solve, created on first use.  The lookahead
+// stream s2 is kept off the first kReservedCUs compute units (a CU-masked
+// stream; an ordinary one if masking is unavailable), so that the
+// latency-bound diagonal-block kernel and the panel updates of the next step
+// always find free CUs while the trailing remainder runs.  A CU-masked stream
+// synchronises with the legacy default stream, so the solve itself runs on
+// its own non-blocking stream s1, ordered after the caller's stream.
+constexpr int kReservedCUs = 32;
+
+int mx_streams(MixedWs &w) {
+  if (w.s1) return 0;
+  DUST_HIP(hipStreamCreateWithFlags(&w.s1, hipStreamNonBlocking));
+  const int cus = mx_cus();
+  std::vector<uint32_t> mask((cus + 31) / 32, 0u);
+  for (int c = kReservedCUs; c < cus; ++c) mask[c / 32] |= 1u << (c % 32);
+  if (cus <= 2 * kReservedCUs ||
+      hipExtStreamCreateWithCUMask(&w.s2, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
+    (void)hipGetLastError();
+    w.s2 = nullptr;
+    DUST_HIP(hipStreamCreateWithFlags(&w.s2, hipStreamNonBlocking));
+  }
+  DUST_HIP(hipEventCreateWithFlags(&w.ev_in, hipEventDisableTiming));
+  DUST_HIP(hipEventCreateWithFlags(&w.ev_start, hipEventDisableTiming));
+  DUST_HIP(hipEventCreateWithFlags(&w.ev_panel, hipEventDisableTiming));
+  DUST_HIP(hipEventCreateWithFlags(&w.ev_rest, hipEventDisableTiming));
+  return 0;
+}
+
 int mixed_dev(MixedWs &w, int n, int nrhs, const double *dA, long lda, double *dB, long ldb, int prec,
-              double tol, int max_iter, int *iters, double *resid, hipStream_t st) {
+              double tol, int max_iter, int *iters, double *resid, hipStream_t cst) {
   if (iters) *iters = 0;
   if (resid) *resid = 0.0;
+  DUST_TRY(mx_streams(w));
+  // the solve runs on s1 after the work already queued on the caller's stream
+  const hipStream_t st = w.s1;
+  DUST_HIP(hipEventRecord(w.ev_in, cst));
+  DUST_HIP(hipStreamWaitEvent(st, w.ev_in, 0));
   MixedRun r;
   DUST_TRY(run_init(r, w, n, nrhs, dA, lda, dB, ldb, prec, tol, max_iter, st));
   if (n == 0 || nrhs == 0) return 0;
@@ -1451,7 +1545,10 @@ int mixed_dev(MixedWs &w, int n, int nrhs, const double *dA, long lda, double *d
       DUST_TRY(run_sweep_finish(r));
     }
     int rc = run_chunk_end(r);
-    if (rc) return rc;
+    if (rc) {
+      (void)hipStreamSynchronize(st);  // leave no work of this call running
+      return rc;
+    }
   }
   DUST_HIP(hipStreamSynchronize(st));
   if (iters) *iters = r.it_report;
@@ -1591,9 +1688,9 @@ void batch_factor_plan(std::vector<MixedRun> &runs, BatchState &B, int f16, Batc
       if (a > 0) {
         // block column and block row of this step, updated by the sub-panels
         // before it in the super-panel
-        pg.push_back(GemmItem{w.W.p, ld, k, k, (np - k) / NB, a, Lf, Uf, pbytes});
+        pg.push_back(GemmItem{w.W.p, ld, k, k, (np - k) / NB, a, (np - k) / NB, Lf, Uf, pbytes});
         pb.push_back((np - k) / NB);
-        pg.push_back(GemmItem{w.W.p, ld, k, k + NB, 1, a, Lf, Uf, pbytes});
+        pg.push_back(GemmItem{w.W.p, ld, k, k + NB, 1, a, (np - k - NB) / NB, Lf, Uf, pbytes});
         pb.push_back((np - k - NB) / NB);
       }
       dg.push_back(DiagArgs{w.W.p, ld, k, w.piv.p, w.perm.p, w.npiv.p, w.dinv.p, fl});
@@ -1606,7 +1703,7 @@ void batch_factor_plan(std::vector<MixedRun> &runs, BatchState &B, int f16, Batc
       sb.push_back(2 * nt);
       if (a == DEPTH - 1 && k + NB < np) {
         const int t0 = k + NB, nrt = (np - t0) / NB;
-        tg.push_back(GemmItem{w.W.p, ld, t0, t0, nrt, DEPTH, Lf, Uf, pbytes});
+        tg.push_back(GemmItem{w.W.p, ld, t0, t0, nrt, DEPTH, nrt * nrt, Lf, Uf, pbytes});
         tb.push_back(nrt * nrt);
       }
     }
