@@ -15,6 +15,10 @@ recorded on the FMM stream around the kernel during the timed steps.
 traffic: HBM bytes per launch from the committed rocprofv3 counter summary
 (profiles/traffic.json), when present.
 
+The direct element <-> particle terms (get_vort_vel at the panel centres and
+the element loops of apply_multipole) are timed at the rotor shape of 5,760
+surface panels with the same particles (`elements`, FP64).
+
 --copy-probe runs a plain device copy of known size, for calibrating the
 FETCH_SIZE / WRITE_SIZE counters.
 
@@ -304,6 +308,106 @@ def time_wing(torch, L, O, W, steps):
                 lu_fp16s_iters=lu["fp16s_iters"], lu_fp16s_backward_error=lu["fp16s_backward_error"])
 
 
+# flop per particle pair of get_vort_vel: distance 3, |d|^2 + r^2 6,
+# rsqrt 1, cube 2, cross product 9, accumulation 6 (rsqrt counted as 1)
+UVORT_FLOP = 27
+
+
+def time_elements(torch, L, W, n, steps, with_cpu=True):
+    """The direct element <-> particle terms at the rotor shape: 4 blades of
+    aspect ratio 12 (about 5,760 surface panels), wake-panel / ring / end
+    vortex rows of 24 each, and n particles of the helix generator scaled to
+    the rotor (x3).  Times dust_vort_vel_dev (uvort at the panel centres,
+    get_vort_vel) and dust_elem_apply_dev (the element loops of
+    apply_multipole, velocity and velocity + vs_elems stretching); with
+    with_cpu, the oracle on a bounded sample of the same work."""
+    import ctypes as C
+    from dust_amd import elements as E
+    ver, nv = W.rotor(4, 5760, aspect=12.0)
+    # trailing edge of the first blade (span along x, chord along y); its
+    # wake rows leave downwards
+    te = np.array([[r, 0.75, 0.0] for r in np.linspace(0.2, 12.2, 25)])
+    groups = W.element_lists(ver, nv, te, wake_dir=(0.0, 0.0, -1.0))
+    cen, alpha, rv, _ = W.helix_wake(n)
+    cen = 3.0 * cen
+    mag = np.linalg.norm(alpha, axis=1)
+    dirs = alpha / mag[:, None]
+    pc = np.ascontiguousarray(ver.mean(axis=1))
+    dv = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    t_cen, t_dir, t_mag, t_rv, t_al = dv(cen), dv(dirs), dv(mag), dv(rv), dv(alpha)
+    t_pc = dv(pc)
+    t_u = torch.zeros((len(pc), 3), dtype=torch.float64, device="cuda")
+    lv, lm = dv(groups[3]["ver"]), dv(groups[3]["mag"])
+    prm = E.params(vortex_rad=0.1)
+    keep = []
+    arr = (L.ElemGroup * len(groups))()
+    for k, g in enumerate(groups):
+        v = np.asarray(g["ver"], dtype=np.float64)
+        tv, tn, tm = dv(v), torch.from_numpy(np.asarray(g["n_ver"], np.int32)).cuda(), dv(g["mag"])
+        ts = dv(g["sigma"]) if g.get("sigma") is not None else None
+        keep += [tv, tn, tm, ts]
+        arr[k].kind, arr[k].n = g["kind"], len(v)
+        arr[k].n_ver, arr[k].ver, arr[k].mag = tn.data_ptr(), tv.data_ptr(), tm.data_ptr()
+        arr[k].sigma = ts.data_ptr() if ts is not None else None
+        arr[k].stretch = 1 if g.get("stretch", True) else 0
+    ne = sum(len(g["ver"]) for g in groups)
+    t_vel = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    t_st = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    lib = L.lib()
+
+    def uvort():
+        L.check(lib.dust_vort_vel_dev(len(pc), t_pc.data_ptr(), n, t_cen.data_ptr(),
+                                      t_dir.data_ptr(), t_mag.data_ptr(), t_rv.data_ptr(),
+                                      len(lm), lv.data_ptr(), lm.data_ptr(), C.byref(prm),
+                                      t_u.data_ptr(), None), "vort_vel_dev")
+
+    def apply(st):
+        L.check(lib.dust_elem_apply_dev(len(groups), arr, C.byref(prm), n, t_cen.data_ptr(),
+                                        t_al.data_ptr() if st else None, t_vel.data_ptr(),
+                                        t_st.data_ptr() if st else None, None), "elem_apply_dev")
+
+    def timed(f, reps):
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms_u = timed(uvort, steps)
+    ms_v = timed(lambda: apply(False), steps)
+    ms_s = timed(lambda: apply(True), max(1, steps // 2))
+    out = dict(panels=len(pc), elements=ne, particles=n, end_vortices=len(lm),
+               uvort_ms=ms_u, uvort_pairs_per_s=len(pc) * n / (ms_u / 1e3),
+               uvort_TFLOPs=UVORT_FLOP * len(pc) * n / (ms_u / 1e3) / 1e12,
+               apply_vel_ms=ms_v, apply_vel_pairs_per_s=ne * n / (ms_v / 1e3),
+               apply_vel_stretch_ms=ms_s, dtype="f64")
+    if with_cpu:
+        from oracle import oracle as orc
+        ns = 4000
+        t0 = time.perf_counter()
+        ov, _ = orc.elem_apply(groups, cen[:ns])
+        t_app = time.perf_counter() - t0
+        u_dev = t_u.cpu().numpy()
+        mu = 256
+        t0 = time.perf_counter()
+        ou = orc.vort_vel(pc[:mu], cen, dirs, mag, rv, groups[3]["ver"], groups[3]["mag"])
+        t_uv = time.perf_counter() - t0
+        g_vel, _ = E.apply_elements(groups, cen[:ns], vortex_rad=0.1)
+        out["cpu_baseline"] = dict(
+            apply_vel_pairs_per_s=ne * ns / t_app, uvort_pairs_per_s=mu * n / t_uv,
+            cores=orc.num_threads(), kind="port",
+            sample=f"oracle element loops at {ns} particles x {ne} elements ({t_app:.1f} s) and "
+                   f"get_vort_vel at {mu} centres x {n} particles ({t_uv:.1f} s), FP64, OpenMP")
+        out["vel_rel_l2_vs_oracle"] = float(np.linalg.norm(g_vel - ov) / np.linalg.norm(ov))
+        out["uvort_rel_l2_vs_oracle"] = float(np.linalg.norm(u_dev[:mu] - ou) /
+                                              np.linalg.norm(ou))
+    return out
+
+
 def cpu_baseline(W, n_sample, degree):
     """The CPU oracle (FP64 restatement, OpenMP) on a bounded sample of the same
     generator; particle updates per second."""
@@ -450,6 +554,10 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-case batch leg")
     ap.add_argument("--no-wing", action="store_true", help="skip the wing configuration leg")
+    ap.add_argument("--no-elements", action="store_true",
+                    help="skip the element <-> particle leg (rotor panels, helix particles)")
+    ap.add_argument("--elements-only", action="store_true",
+                    help="time only the element <-> particle leg and print its JSON")
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB",
                                                           "COPY_WRITE_DB"))
     ap.add_argument("--summarize-trace", metavar="TRACE_DB")
@@ -477,6 +585,10 @@ def main():
 
     L.lib()
     n = args.n
+    if args.elements_only:
+        print(json.dumps(time_elements(torch, L, W, n, args.steps, with_cpu=not args.no_cpu),
+                         default=float))
+        return
     log(f"generating helix wake n={n}")
     cen, alpha, rv, tree = W.helix_wake(n)
     if args.levels:
@@ -530,6 +642,11 @@ def main():
         log("timing the wing configuration")
         wing = time_wing(torch, L, O, W, 3)
         log(f"wing: {wing}")
+    elements = None
+    if not args.no_elements:
+        log("timing the element <-> particle terms")
+        elements = time_elements(torch, L, W, n, 3, with_cpu=not args.no_cpu)
+        log(f"elements: {elements}")
     batch = None
     if not args.no_batch:
         log("timing the 48-case batch")
@@ -579,6 +696,7 @@ def main():
         "roofline": roof,
         "lu": lu,
         "wing": wing,
+        "elements": elements,
         "batch": batch,
         "cpu_baseline": cpu,
     }

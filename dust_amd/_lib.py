@@ -59,6 +59,34 @@ class AicParams(C.Structure):
     ]
 
 
+ELEM_SURFPAN = 0
+ELEM_RING = 1
+ELEM_LINE = 2
+
+
+class ElemGroup(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("n", C.c_int),
+        ("n_ver", _vp),
+        ("ver", _vp),
+        ("mag", _vp),
+        ("sigma", _vp),
+        ("stretch", C.c_int),
+        ("reserved", C.c_int),
+    ]
+
+
+class ElemParams(C.Structure):
+    _fields_ = [
+        ("ff_ratio_doublet", C.c_double),
+        ("rankine_rad", C.c_double),
+        ("cutoff_rad", C.c_double),
+        ("vortex_rad", C.c_double),
+        ("reserved", C.c_int * 4),
+    ]
+
+
 # (name, restype, argtypes) of every exported function in include/dust_gpu.h
 SIGNATURES = [
     ("dust_last_error", C.c_char_p, []),
@@ -86,6 +114,15 @@ SIGNATURES = [
     ("dust_fmm_stage_times", C.c_int, [_vp, _vp, _vp]),
     ("dust_vortpart_direct", C.c_int,
      [C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_double, _vp, _vp, _vp]),
+    ("dust_elem_apply", C.c_int,
+     [C.c_int, C.POINTER(ElemGroup), C.POINTER(ElemParams), C.c_int, _vp, _vp, _vp, _vp]),
+    ("dust_elem_apply_dev", C.c_int,
+     [C.c_int, C.POINTER(ElemGroup), C.POINTER(ElemParams), C.c_int, _vp, _vp, _vp, _vp, _vp]),
+    ("dust_vort_vel", C.c_int,
+     [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, C.POINTER(ElemParams), _vp]),
+    ("dust_vort_vel_dev", C.c_int,
+     [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, C.POINTER(ElemParams), _vp,
+      _vp]),
     ("dust_aic_surfpan", C.c_int,
      [C.c_int, _vp, _vp, C.POINTER(AicParams), _vp, C.c_int, _vp, C.c_int]),
     ("dust_aic_surfpan_dev", C.c_int,

@@ -27,6 +27,7 @@ SOURCES = [
     "fmm_prim.hip",
     "fmm_tuned.hip",
     "aic.hip",
+    "elem.hip",
     "lu.hip",
     "lu_mixed.hip",
     "linsys.hip",

@@ -1,0 +1,695 @@
+// Direct element <-> particle terms of a DUST time step on gfx950:
+//  * the element loops of apply_multipole (src/fmm/mod_octree.f90:1350-1401):
+//    body panels, wake panels, wake rings and end vortices acting on the
+//    vortex particles -- velocity, and with vs_elems the stretching
+//    transpose(grad) alpha;
+//  * get_vort_vel (src/aeroel/mod_surfpan.f90:962-977,
+//    src/aeroel/mod_vortlatt.f90:676-693): particles and end vortices acting
+//    on the panel centres (uvort).
+//
+// Kernels per element (compute_vel / compute_grad, all times 4 pi):
+//   surface panel  velocity_calc_doublet * mag - velocity_calc_sou_surfpan * sigma
+//                  (mod_surfpan.f90:641-658, 268-357); gradient
+//                  gradient_calc_doublet * mag (the source gradient is zero,
+//                  mod_surfpan.f90:362-370)
+//   vortex ring    velocity_calc_doublet * mag, gradient_calc_doublet * mag
+//                  (mod_doublet.f90:178-305)
+//   vortex line    compute_vel_vortline / compute_grad_vortline
+//                  (mod_vortline.f90:126-236)
+//   particle       c (dist x dir) mag, c = -1/(|dist|^2 + r^2)^1.5
+//                  (mod_vortpart.f90:139-153, 260-271)
+//
+// Layout: element records (geometry of calc_geo_data, strength, flags) are
+// built once per call in one array, read by every thread with uniform loads;
+// one thread per target point.  get_vort_vel has few targets (panels) and
+// many sources (particles): the sources are split over blockIdx.y, staged
+// through LDS 256 at a time, and the per-split partial sums are added in a
+// fixed order by a second kernel.
+#include <cfloat>
+#include <cmath>
+#include <hipcub/hipcub.hpp>
+
+#include "panel.h"
+
+namespace dust {
+namespace {
+
+constexpr double kInv4Pi = 1.0 / (4.0 * kPi);
+constexpr int kFlagDoublet = 1, kFlagSource = 2, kFlagStretch = 4;
+constexpr int kSortMin = 4096;  // targets below this count are evaluated in input order
+
+struct ElemRec {
+  Panel g;  // polygons: calc_geo_data; lines: ver[0..1], evec/euni/elen[0], cen
+  double mag, sigma;
+  int kind, flags;
+};
+
+__global__ void k_elem_recs(int n, int kind, const int *__restrict__ nver,
+                            const double *__restrict__ ver, const double *__restrict__ mag,
+                            const double *__restrict__ sigma, int stretch, ElemRec *R) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  ElemRec r{};
+  if (kind == DUST_ELEM_LINE) {
+    // calc_geo_data_vortline (mod_vortline.f90:238-258)
+    r.g.nv = 2;
+    for (int e = 0; e < 3; ++e) {
+      r.g.ver[0][e] = ver[6 * j + e];
+      r.g.ver[1][e] = ver[6 * j + 3 + e];
+      r.g.cen[e] = 0.5 * (r.g.ver[0][e] + r.g.ver[1][e]);
+      r.g.evec[0][e] = r.g.ver[1][e] - r.g.ver[0][e];
+    }
+    double el = sqrt(dot3(r.g.evec[0], r.g.evec[0]));
+    r.g.elen[0] = el;
+    r.g.maxedge = el;
+    for (int e = 0; e < 3; ++e) r.g.euni[0][e] = r.g.evec[0][e] / el;
+  } else {
+    int nv = nver[j] == 3 ? 3 : 4;  // other counts are rejected on the host
+    panel_geo(nv, ver + 12 * j, r.g);
+  }
+  r.mag = mag ? mag[j] : 0.0;
+  r.sigma = sigma ? sigma[j] : 0.0;
+  r.kind = kind;
+  r.flags = (mag ? kFlagDoublet : 0) | (sigma && kind == DUST_ELEM_SURFPAN ? kFlagSource : 0) |
+            (stretch ? kFlagStretch : 0);
+  R[j] = r;
+}
+
+// compute_vel_vortline without the strength: the edge x1 -> x2 (unit vector
+// eu, length el) with the Rankine core and the cut-off (the same expression
+// as an edge of velocity_calc_doublet).
+__device__ inline void line_vel(const double *x1, const double *x2, const double *eu, double el,
+                                const double pos[3], double rran, double rcut, double v[3]) {
+  double av[3], bv[3], hv[3], cr[3];
+  for (int e = 0; e < 3; ++e) {
+    av[e] = pos[e] - x1[e];
+    bv[e] = pos[e] - x2[e];
+  }
+  double ai = dot3(av, eu);
+  double R1 = sqrt(dot3(av, av)), R2 = sqrt(dot3(bv, bv));
+  for (int e = 0; e < 3; ++e) hv[e] = av[e] - ai * eu[e];
+  double hi = sqrt(dot3(hv, hv));
+  crs3(eu, hv, cr);
+  double f = 0.0;
+  if (hi > el * rran)
+    f = ((el - ai) / R2 + ai / R1) / (hi * hi);
+  else if (R1 > el * rcut && R2 > el * rcut)
+    f = ((el - ai) / R2 + ai / R1) / ((rran * el) * (rran * el));
+  for (int e = 0; e < 3; ++e) v[e] = f * cr[e];
+}
+
+// velocity_calc_sou_surfpan (mod_surfpan.f90:268-357); its own far-field
+// ratio is the local parameter 10, the doublet potential of its near field
+// uses FarFieldRatioDoublet.
+__device__ inline void vel_source(const Panel &g, const double pos[3], double ffr_dou,
+                                  double v[3]) {
+  double rv[3] = {pos[0] - g.cen[0], pos[1] - g.cen[1], pos[2] - g.cen[2]};
+  double radius = sqrt(dot3(rv, rv));
+  double phix, phiy, pdou;
+  if (radius > 10.0 * g.maxedge) {
+    double r3 = radius * radius * radius;
+    phix = -g.area * dot3(rv, g.t1) / r3;
+    phiy = -g.area * dot3(rv, g.t2) / r3;
+    pdou = -g.area * dot3(rv, g.nor) / r3;
+  } else {
+    phix = 0.0;
+    phiy = 0.0;
+    const int nv = g.nv;
+    for (int i1 = 0; i1 < nv; ++i1) {
+      int ip1 = (i1 + 1 < nv) ? i1 + 1 : 0;
+      double a[3], b[3];
+      for (int e = 0; e < 3; ++e) {
+        a[e] = pos[e] - g.verp[i1][e];
+        b[e] = pos[e] - g.verp[ip1][e];
+      }
+      double R1 = sqrt(dot3(a, a)), R2 = sqrt(dot3(b, b)), el = g.elen[i1];
+      double sl = (R1 + R2 - el < 1e-12) ? 0.0 : log((R1 + R2 + el) / (R1 + R2 - el));
+      phix += dot3(g.euni[i1], g.t2) * sl;  // sinTi
+      phiy -= dot3(g.euni[i1], g.t1) * sl;  // cosTi
+    }
+    pdou = pot_doublet(g, pos, ffr_dou);
+    phix = -phix;
+    phiy = -phiy;
+    pdou = -pdou;
+  }
+  for (int e = 0; e < 3; ++e) v[e] = g.t1[e] * phix + g.t2[e] * phiy + g.nor[e] * pdou;
+}
+
+// 1/sqrt(x) in FP64: the hardware estimate and two Newton steps.
+__device__ inline double rsqrt64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
+// transpose(grad) alpha for one edge of gradient_calc_doublet /
+// compute_grad_vortline (mod_doublet.f90:254-305, mod_vortline.f90:174-236),
+// accumulated into s.  With a_k = l x R_k, a = |a_1|, A = a^2 + del^2 and the
+// skew matrix lx of l, the reference's endpoint term is
+//   (a lx + (1/a - 3a/A) a_k a_k^T lx)(l.R) + a (a_k l^T - a_k R^T (l.R)/|R|^2)
+// over A^1.5 |R|; its transpose applied to alpha is
+//   (l.R)(a (alpha x l) + (1/a - 3a/A)(a_k.alpha)(a_k x l))
+//   + a (a_k.alpha)(l - R (l.R)/|R|^2),
+// evaluated here without forming the 3x3 matrices, with the reciprocal
+// square roots of a^2, A and |R|^2 and no division.
+__device__ inline void edge_gradT(const double *l, const double *x1, const double *x2,
+                                  const double pos[3], double del2, const double al[3],
+                                  double s[3]) {
+  double R[2][3], av[2][3];
+  for (int e = 0; e < 3; ++e) {
+    R[0][e] = pos[e] - x1[e];
+    R[1][e] = pos[e] - x2[e];
+  }
+  crs3(l, R[0], av[0]);
+  crs3(l, R[1], av[1]);
+  double a2 = dot3(av[0], av[0]);
+  double ia = rsqrt64(a2);  // 1/a
+  double a = a2 * ia;
+  double A = a2 + del2;
+  double iAh = rsqrt64(A);
+  double iA15 = iAh * iAh * iAh;
+  double coef = ia - 3.0 * a * (iAh * iAh);
+  double axl[3];
+  crs3(al, l, axl);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    double nR2 = dot3(R[t], R[t]);
+    double iR = rsqrt64(nR2);
+    double lR = dot3(l, R[t]);
+    double fac = (t ? -iA15 : iA15) * iR;
+    double ava = dot3(av[t], al);
+    double avxl[3];
+    crs3(av[t], l, avxl);
+    double q = lR * (iR * iR);
+    for (int e = 0; e < 3; ++e)
+      s[e] += fac * (lR * (a * axl[e] + coef * ava * avxl[e]) + a * ava * (l[e] - R[t][e] * q));
+  }
+}
+
+__device__ inline void elem_gradT(const ElemRec &E, const double pos[3], double del2,
+                                  const double al[3], double s[3]) {
+  const int nv = E.g.nv;
+  if (nv == 2) {
+    edge_gradT(E.g.euni[0], E.g.ver[0], E.g.ver[1], pos, del2, al, s);
+    return;
+  }
+  for (int i = 0; i < nv; ++i) {
+    int ip1 = (i + 1 < nv) ? i + 1 : 0;
+    edge_gradT(E.g.euni[i], E.g.ver[i], E.g.ver[ip1], pos, del2, al, s);
+  }
+}
+
+// One thread per target point; every thread walks the whole element array
+// (uniform loads of the records).  Far from a panel (beyond both far-field
+// ratios) the doublet and source far fields share one square root and one
+// reciprocal; otherwise the full near-field kernels run.  A wave runs the
+// near-field code for an element if any of its lanes needs it, so the targets
+// are given in Morton order (spatially close targets share waves).
+__global__ void __launch_bounds__(256)
+    k_elem_apply(int ne, const ElemRec *__restrict__ R, double ffr, double rran, double rcut,
+                 double del2, int n, const double *__restrict__ pos,
+                 const double *__restrict__ alpha, double *vel, double *stretch, int acc) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+  const bool do_st = stretch != nullptr;
+  double al[3] = {0.0, 0.0, 0.0};
+  if (do_st)
+    for (int e = 0; e < 3; ++e) al[e] = alpha[3 * p + e];
+  double v[3] = {0.0, 0.0, 0.0}, st[3] = {0.0, 0.0, 0.0};
+  for (int j = 0; j < ne; ++j) {
+    const ElemRec &E = R[j];
+    const int flags = E.flags;
+    double w[3] = {0.0, 0.0, 0.0};
+    const double m = E.mag;
+    if (E.kind == DUST_ELEM_LINE) {
+      if (flags & kFlagDoublet)
+        line_vel(E.g.ver[0], E.g.ver[1], E.g.euni[0], E.g.elen[0], x, rran, rcut, w);
+      for (int e = 0; e < 3; ++e) w[e] *= m;
+    } else {
+      const Panel &g = E.g;
+      double rv[3] = {x[0] - g.cen[0], x[1] - g.cen[1], x[2] - g.cen[2]};
+      double r2 = dot3(rv, rv);
+      double radius = sqrt(r2);
+      const bool far_d = radius > ffr * g.maxedge;       // velocity_calc_doublet switch
+      const bool far_s = radius > 10.0 * g.maxedge;      // velocity_calc_sou_surfpan switch
+      const bool src = flags & kFlagSource;
+      if (far_d && (far_s || !src)) {
+        // both far-field formulas with one reciprocal: doublet
+        //   t1 phix + t2 phiy + nor pdou, rati = 3 area (r.n)/r^5,
+        //   phix = rati (r.t1), phiy = rati (r.t2), pdou = area (3 (r.n)^2 - r^2)/r^5;
+        // source: the same basis with -area (r.t1, r.t2, r.n)/r^3
+        double ir = 1.0 / radius;
+        double ir3 = ir * ir * ir, ir5 = ir3 * ir * ir;
+        double rn = dot3(rv, g.nor), r1 = dot3(rv, g.t1), rt2 = dot3(rv, g.t2);
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        if (flags & kFlagDoublet) {
+          double rati = 3.0 * g.area * rn * ir5;
+          cx = rati * r1 * m;
+          cy = rati * rt2 * m;
+          cz = g.area * (3.0 * rn * rn - r2) * ir5 * m;
+        }
+        if (src) {
+          double fs = g.area * ir3 * E.sigma;  // minus the source velocity times sigma
+          cx += fs * r1;
+          cy += fs * rt2;
+          cz += fs * rn;
+        }
+        for (int e = 0; e < 3; ++e) w[e] = g.t1[e] * cx + g.t2[e] * cy + g.nor[e] * cz;
+      } else {
+        if (flags & kFlagDoublet) vel_doublet(g, x, ffr, rran, rcut, w);
+        for (int e = 0; e < 3; ++e) w[e] *= m;
+        if (src) {
+          double vs[3];
+          vel_source(g, x, ffr, vs);
+          for (int e = 0; e < 3; ++e) w[e] -= vs[e] * E.sigma;
+        }
+      }
+    }
+    for (int e = 0; e < 3; ++e) v[e] += w[e];
+    if (do_st && (flags & kFlagStretch) && (flags & kFlagDoublet)) {
+      double sg[3] = {0.0, 0.0, 0.0};
+      elem_gradT(E, x, del2, al, sg);
+      for (int e = 0; e < 3; ++e) st[e] += sg[e] * m;
+    }
+  }
+  // accumulate: out += term; otherwise out = term (the sorted path adds the
+  // terms into the caller's arrays in a second kernel)
+  for (int e = 0; e < 3; ++e) vel[3 * p + e] = (acc ? vel[3 * p + e] : 0.0) + v[e] * kInv4Pi;
+  if (do_st)
+    for (int e = 0; e < 3; ++e)
+      stretch[3 * p + e] = (acc ? stretch[3 * p + e] : 0.0) + st[e] * kInv4Pi;
+}
+
+// ---- Morton order of the targets -------------------------------------------
+
+// Per-block bounding box of the targets (grid-stride), part[6 b + (lo xyz, hi xyz)].
+__global__ void __launch_bounds__(256) k_bbox(int n, const double *__restrict__ pos,
+                                              double *part) {
+  __shared__ double sm[6][256];
+  const int tid = threadIdx.x;
+  double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (long p = (long)blockIdx.x * 256 + tid; p < n; p += (long)gridDim.x * 256)
+    for (int e = 0; e < 3; ++e) {
+      double x = pos[3 * p + e];
+      v[e] = fmin(v[e], x);
+      v[3 + e] = fmax(v[3 + e], x);
+    }
+  for (int k = 0; k < 6; ++k) sm[k][tid] = v[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w)
+      for (int k = 0; k < 6; ++k)
+        sm[k][tid] = k < 3 ? fmin(sm[k][tid], sm[k][tid + w]) : fmax(sm[k][tid], sm[k][tid + w]);
+    __syncthreads();
+  }
+  if (tid < 6) part[6 * blockIdx.x + tid] = sm[tid][0];
+}
+
+// Reduce nb per-block boxes into part[0..5] (one block).
+__global__ void __launch_bounds__(256) k_bbox_final(int nb, double *part) {
+  __shared__ double sm[6][256];
+  const int tid = threadIdx.x;
+  double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (int b = tid; b < nb; b += 256)
+    for (int k = 0; k < 6; ++k)
+      v[k] = k < 3 ? fmin(v[k], part[6 * b + k]) : fmax(v[k], part[6 * b + k]);
+  for (int k = 0; k < 6; ++k) sm[k][tid] = v[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w)
+      for (int k = 0; k < 6; ++k)
+        sm[k][tid] = k < 3 ? fmin(sm[k][tid], sm[k][tid + w]) : fmax(sm[k][tid], sm[k][tid + w]);
+    __syncthreads();
+  }
+  if (tid < 6) part[tid] = sm[tid][0];
+}
+
+__device__ inline unsigned spread10(unsigned v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// 30-bit Morton key of each target on a 1024^3 grid over the bounding box.
+__global__ void k_morton(int n, const double *__restrict__ pos, const double *__restrict__ bb,
+                         unsigned *key, int *idx) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  unsigned q[3];
+  for (int e = 0; e < 3; ++e) {
+    double w = bb[3 + e] - bb[e];
+    double t = w > 0.0 ? (pos[3 * p + e] - bb[e]) * (1023.0 / w) : 0.0;
+    int c = (int)t;
+    q[e] = (unsigned)(c < 0 ? 0 : (c > 1023 ? 1023 : c));
+  }
+  key[p] = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+  idx[p] = p;
+}
+
+__global__ void k_gather_targets(int n, const int *__restrict__ perm,
+                                 const double *__restrict__ pos, const double *__restrict__ alpha,
+                                 double *spos, double *sal) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  long p = perm[k];
+  for (int e = 0; e < 3; ++e) spos[3 * (long)k + e] = pos[3 * p + e];
+  if (alpha)
+    for (int e = 0; e < 3; ++e) sal[3 * (long)k + e] = alpha[3 * p + e];
+}
+
+// out[perm[k]] += sorted[k] (perm is a permutation: no two threads share a target)
+__global__ void k_scatter_add(int n, const int *__restrict__ perm, const double *__restrict__ svel,
+                              const double *__restrict__ sst, double *vel, double *stretch) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  long p = perm[k];
+  for (int e = 0; e < 3; ++e) vel[3 * p + e] += svel[3 * (long)k + e];
+  if (stretch)
+    for (int e = 0; e < 3; ++e) stretch[3 * p + e] += sst[3 * (long)k + e];
+}
+
+constexpr int kVT = 256;  // targets per block, one per thread
+constexpr int kVS = 256;  // sources per LDS chunk
+
+// Particle source record: x, y, z, r_vortex^2, dir*mag (3), unused.
+__global__ void k_pack_particles(int np, const double *__restrict__ cen,
+                                 const double *__restrict__ dir, const double *__restrict__ mag,
+                                 const double *__restrict__ rv, double *out) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= np) return;
+  double m = mag[j], r = rv[j];
+  double rec[8] = {cen[3 * j], cen[3 * j + 1], cen[3 * j + 2], r * r,
+                   dir[3 * j] * m, dir[3 * j + 1] * m, dir[3 * j + 2] * m, 0.0};
+  double4 *o = reinterpret_cast<double4 *>(out + 8 * (long)j);
+  o[0] = make_double4(rec[0], rec[1], rec[2], rec[3]);
+  o[1] = make_double4(rec[4], rec[5], rec[6], rec[7]);
+}
+
+// Partial uvort sums of the particles [y*per, min(np, (y+1)*per)) at the
+// targets of this block: part[(y*3 + e)*m + i].
+__global__ void __launch_bounds__(kVT)
+    k_vort_particles(int m, const double *__restrict__ cen, int np,
+                     const double *__restrict__ src, int per, double *part) {
+  __shared__ double sx[kVS], sy[kVS], sz[kVS], sr[kVS], swx[kVS], swy[kVS], swz[kVS];
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * kVT + tid;
+  const int ic = i < m ? i : m - 1;
+  const double x0 = cen[3 * ic], x1 = cen[3 * ic + 1], x2 = cen[3 * ic + 2];
+  const long s0 = (long)blockIdx.y * per;
+  const long s1 = s0 + per < np ? s0 + per : np;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (long base = s0; base < s1; base += kVS) {
+    __syncthreads();
+    long j = base + tid;
+    if (j < s1) {
+      const double4 *q = reinterpret_cast<const double4 *>(src + 8 * j);
+      double4 u = q[0], w = q[1];
+      sx[tid] = u.x; sy[tid] = u.y; sz[tid] = u.z; sr[tid] = u.w;
+      swx[tid] = w.x; swy[tid] = w.y; swz[tid] = w.z;
+    } else {  // zero-strength filler: contributes exactly nothing
+      sx[tid] = 0.0; sy[tid] = 0.0; sz[tid] = 0.0; sr[tid] = 1.0;
+      swx[tid] = 0.0; swy[tid] = 0.0; swz[tid] = 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < kVS; ++k) {
+      double d0 = x0 - sx[k], d1 = x1 - sy[k], d2 = x2 - sz[k];
+      double R2 = fma(d0, d0, fma(d1, d1, fma(d2, d2, sr[k])));
+      double ri = rsqrt64(R2);
+      double c = -(ri * ri * ri);
+      double w0 = swx[k], w1 = swy[k], w2 = swz[k];
+      a0 = fma(c, d1 * w2 - d2 * w1, a0);
+      a1 = fma(c, d2 * w0 - d0 * w2, a1);
+      a2 = fma(c, d0 * w1 - d1 * w0, a2);
+    }
+  }
+  if (i < m) {
+    long o = (long)blockIdx.y * 3 * m + i;
+    part[o] = a0;
+    part[o + m] = a1;
+    part[o + 2 * m] = a2;
+  }
+}
+
+// uvort = (sum of the partials in split order + end-vortex lines) / (4 pi)
+__global__ void k_vort_finish(int m, int S, const double *__restrict__ part,
+                              const double *__restrict__ cen, int nl,
+                              const double *__restrict__ lver, const double *__restrict__ lmag,
+                              double rran, double rcut, double *uvort) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double u[3] = {0.0, 0.0, 0.0};
+  for (int s = 0; s < S; ++s)
+    for (int e = 0; e < 3; ++e) u[e] += part[((long)s * 3 + e) * m + i];
+  if (lmag) {
+    double x[3] = {cen[3 * i], cen[3 * i + 1], cen[3 * i + 2]};
+    for (int j = 0; j < nl; ++j) {
+      const double *v1 = lver + 6 * j;
+      double ev[3] = {v1[3] - v1[0], v1[4] - v1[1], v1[5] - v1[2]};
+      double el = sqrt(dot3(ev, ev));
+      double eu[3] = {ev[0] / el, ev[1] / el, ev[2] / el};
+      double w[3];
+      line_vel(v1, v1 + 3, eu, el, x, rran, rcut, w);
+      for (int e = 0; e < 3; ++e) u[e] += w[e] * lmag[j];
+    }
+  }
+  for (int e = 0; e < 3; ++e) uvort[3 * i + e] = u[e] * kInv4Pi;
+}
+
+// Device buffers of the element terms, grown on demand (one caller at a time,
+// like the rest of the library's host entry points).
+struct ElemWs {
+  DevBuf<ElemRec> rec;
+  DevBuf<double> part, packed;
+  DevBuf<unsigned> key, key_s;  // Morton order of the targets
+  DevBuf<int> idx, perm;
+  DevBuf<double> spos, sal, svel, sst, bb;
+  DevBuf<char> tmp;
+  DevBuf<double> h_d;  // staging of the host entry points
+  DevBuf<int> h_i;
+};
+ElemWs g_ws;
+
+int check_groups(int ngroups, const dust_elem_group *groups) {
+  DUST_CHECK(ngroups >= 0 && (ngroups == 0 || groups), DUST_E_ARG, "invalid element groups");
+  for (int k = 0; k < ngroups; ++k) {
+    const dust_elem_group &G = groups[k];
+    DUST_CHECK(G.kind >= DUST_ELEM_SURFPAN && G.kind <= DUST_ELEM_LINE, DUST_E_ARG,
+               "element group %d: unknown kind %d", k, G.kind);
+    DUST_CHECK(G.n >= 0, DUST_E_ARG, "element group %d: negative count", k);
+    if (G.n == 0) continue;
+    DUST_CHECK(G.ver && (G.kind == DUST_ELEM_LINE || G.n_ver), DUST_E_ARG,
+               "element group %d: missing vertices", k);
+  }
+  return 0;
+}
+
+int elem_apply_dev(int ngroups, const dust_elem_group *groups, const dust_elem_params *prm, int n,
+                   const double *pos, const double *alpha, double *vel, double *stretch,
+                   hipStream_t st) {
+  DUST_TRY(check_groups(ngroups, groups));
+  DUST_CHECK(prm && n >= 0 && (n == 0 || (pos && vel)), DUST_E_ARG, "invalid element arguments");
+  if (!alpha) stretch = nullptr;
+  long ne = 0;
+  for (int k = 0; k < ngroups; ++k) ne += groups[k].n;
+  DUST_CHECK(ne < (1L << 31), DUST_E_ARG, "too many elements");
+  if (n == 0 || ne == 0) return 0;
+  DUST_TRY(g_ws.rec.reserve(ne));
+  long off = 0;
+  for (int k = 0; k < ngroups; ++k) {
+    const dust_elem_group &G = groups[k];
+    if (G.n == 0) continue;
+    k_elem_recs<<<ceil_div(G.n, 128), 128, 0, st>>>(G.n, G.kind, G.n_ver, G.ver, G.mag, G.sigma,
+                                                    G.stretch, g_ws.rec.p + off);
+    off += G.n;
+  }
+  const double del2 = prm->vortex_rad * prm->vortex_rad;
+  if (n < kSortMin) {
+    k_elem_apply<<<ceil_div(n, 256), 256, 0, st>>>((int)ne, g_ws.rec.p, prm->ff_ratio_doublet,
+                                                   prm->rankine_rad, prm->cutoff_rad, del2, n,
+                                                   pos, alpha, vel, stretch, 1);
+    DUST_HIP(hipGetLastError());
+    return 0;
+  }
+  // Morton order: bounding box, keys, stable sort, gather, evaluate, scatter-add
+  const int nb = ceil_div(n, 256) < 1024 ? ceil_div(n, 256) : 1024;
+  DUST_TRY(g_ws.bb.reserve(6 * (size_t)nb));
+  DUST_TRY(g_ws.key.reserve(n));
+  DUST_TRY(g_ws.key_s.reserve(n));
+  DUST_TRY(g_ws.idx.reserve(n));
+  DUST_TRY(g_ws.perm.reserve(n));
+  DUST_TRY(g_ws.spos.reserve(3 * (size_t)n));
+  DUST_TRY(g_ws.svel.reserve(3 * (size_t)n));
+  if (stretch) {
+    DUST_TRY(g_ws.sal.reserve(3 * (size_t)n));
+    DUST_TRY(g_ws.sst.reserve(3 * (size_t)n));
+  }
+  size_t bytes = 0;
+  DUST_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, g_ws.key.p, g_ws.key_s.p,
+                                              g_ws.idx.p, g_ws.perm.p, n, 0, 30, st));
+  DUST_TRY(g_ws.tmp.reserve(bytes + 256));
+  k_bbox<<<nb, 256, 0, st>>>(n, pos, g_ws.bb.p);
+  k_bbox_final<<<1, 256, 0, st>>>(nb, g_ws.bb.p);
+  k_morton<<<ceil_div(n, 256), 256, 0, st>>>(n, pos, g_ws.bb.p, g_ws.key.p, g_ws.idx.p);
+  DUST_HIP(hipcub::DeviceRadixSort::SortPairs(g_ws.tmp.p, bytes, g_ws.key.p, g_ws.key_s.p,
+                                              g_ws.idx.p, g_ws.perm.p, n, 0, 30, st));
+  double *sal = stretch ? g_ws.sal.p : nullptr, *sst = stretch ? g_ws.sst.p : nullptr;
+  k_gather_targets<<<ceil_div(n, 256), 256, 0, st>>>(n, g_ws.perm.p, pos, stretch ? alpha : nullptr,
+                                                     g_ws.spos.p, sal);
+  k_elem_apply<<<ceil_div(n, 256), 256, 0, st>>>((int)ne, g_ws.rec.p, prm->ff_ratio_doublet,
+                                                 prm->rankine_rad, prm->cutoff_rad, del2, n,
+                                                 g_ws.spos.p, sal, g_ws.svel.p, sst, 0);
+  k_scatter_add<<<ceil_div(n, 256), 256, 0, st>>>(n, g_ws.perm.p, g_ws.svel.p, sst, vel,
+                                                  stretch);
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
+
+int vort_vel_dev(int m, const double *cen, int np, const double *pcen, const double *pdir,
+                 const double *pmag, const double *prv, int nl, const double *lver,
+                 const double *lmag, const dust_elem_params *prm, double *uvort, hipStream_t st) {
+  DUST_CHECK(prm && m >= 0 && np >= 0 && nl >= 0, DUST_E_ARG, "invalid uvort arguments");
+  DUST_CHECK(m == 0 || (cen && uvort), DUST_E_ARG, "missing centres or output");
+  DUST_CHECK(np == 0 || (pcen && pdir && pmag && prv), DUST_E_ARG, "missing particle arrays");
+  DUST_CHECK(nl == 0 || lver || !lmag, DUST_E_ARG, "missing end-vortex vertices");
+  if (m == 0) return 0;
+  const int tb = ceil_div(m, kVT);
+  // enough blocks to fill the chip several times over; each split a whole
+  // number of LDS chunks
+  int S = 1, per = 0;
+  if (np > 0) {
+    S = ceil_div(4096, tb);
+    int smax = ceil_div(np, kVS);
+    S = S < smax ? S : smax;
+    S = S > 65535 ? 65535 : S;
+    per = ceil_div(ceil_div(np, S), kVS) * kVS;
+    S = ceil_div(np, per);
+    DUST_TRY(g_ws.packed.reserve(8 * (size_t)np));
+    DUST_TRY(g_ws.part.reserve((size_t)S * 3 * m));
+    k_pack_particles<<<ceil_div(np, 256), 256, 0, st>>>(np, pcen, pdir, pmag, prv, g_ws.packed.p);
+    k_vort_particles<<<dim3(tb, S), kVT, 0, st>>>(m, cen, np, g_ws.packed.p, per, g_ws.part.p);
+  } else {
+    S = 0;
+  }
+  k_vort_finish<<<ceil_div(m, 256), 256, 0, st>>>(m, S, g_ws.part.p, cen, nl, lver,
+                                                  nl ? lmag : nullptr, prm->rankine_rad,
+                                                  prm->cutoff_rad, uvort);
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
+
+// Host staging: copy `count` values to the device buffer at offset `off`;
+// a failed copy is remembered in `err`.
+template <typename T>
+const T *stage(DevBuf<T> &b, size_t &off, const T *h, size_t count, hipError_t &err) {
+  if (!h || count == 0) return nullptr;
+  T *d = b.p + off;
+  hipError_t e = hipMemcpy(d, h, sizeof(T) * count, hipMemcpyHostToDevice);
+  if (e != hipSuccess) err = e;
+  off += count;
+  return d;
+}
+
+}  // namespace
+}  // namespace dust
+
+using namespace dust;
+
+extern "C" {
+
+int dust_elem_apply_dev(int ngroups, const dust_elem_group *groups, const dust_elem_params *prm,
+                        int n, const double *pos, const double *alpha, double *vel,
+                        double *stretch, void *stream) {
+  return elem_apply_dev(ngroups, groups, prm, n, pos, alpha, vel, stretch, (hipStream_t)stream);
+}
+
+int dust_elem_apply(int ngroups, const dust_elem_group *groups, const dust_elem_params *prm,
+                    int n, const double *pos, const double *alpha, double *vel, double *stretch) {
+  DUST_TRY(check_groups(ngroups, groups));
+  DUST_CHECK(prm && n >= 0 && (n == 0 || (pos && vel)), DUST_E_ARG, "invalid element arguments");
+  for (int k = 0; k < ngroups; ++k)
+    if (groups[k].kind != DUST_ELEM_LINE)
+      for (int j = 0; j < groups[k].n; ++j)
+        DUST_CHECK(groups[k].n_ver[j] == 3 || groups[k].n_ver[j] == 4, DUST_E_ARG,
+                   "element group %d: element %d has %d vertices", k, j, groups[k].n_ver[j]);
+  if (n == 0) return 0;
+  const bool st = alpha && stretch;
+  size_t nd = 3 * (size_t)n * (st ? 4 : 2), ni = 0;
+  for (int k = 0; k < ngroups; ++k) {
+    size_t g = (size_t)groups[k].n;
+    nd += g * (groups[k].kind == DUST_ELEM_LINE ? 6 : 12) + 2 * g;
+    ni += groups[k].kind == DUST_ELEM_LINE ? 0 : g;
+  }
+  DUST_TRY(g_ws.h_d.reserve(nd));
+  DUST_TRY(g_ws.h_i.reserve(ni + 1));
+  size_t od = 0, oi = 0;
+  hipError_t err = hipSuccess;
+  dust_elem_group dg[64];
+  DUST_CHECK(ngroups <= 64, DUST_E_ARG, "at most 64 element groups per call");
+  for (int k = 0; k < ngroups; ++k) {
+    const dust_elem_group &G = groups[k];
+    dg[k] = G;
+    size_t g = (size_t)G.n;
+    dg[k].ver = stage(g_ws.h_d, od, G.ver, g * (G.kind == DUST_ELEM_LINE ? 6 : 12), err);
+    dg[k].mag = stage(g_ws.h_d, od, G.mag, g, err);
+    dg[k].sigma = stage(g_ws.h_d, od, G.sigma, g, err);
+    dg[k].n_ver = G.kind == DUST_ELEM_LINE ? nullptr : stage(g_ws.h_i, oi, G.n_ver, g, err);
+  }
+  const double *dpos = stage(g_ws.h_d, od, pos, 3 * (size_t)n, err);
+  double *dvel = g_ws.h_d.p + od;
+  stage(g_ws.h_d, od, (const double *)vel, 3 * (size_t)n, err);
+  const double *dal = nullptr;
+  double *dst = nullptr;
+  if (st) {
+    dal = stage(g_ws.h_d, od, alpha, 3 * (size_t)n, err);
+    dst = g_ws.h_d.p + od;
+    stage(g_ws.h_d, od, (const double *)stretch, 3 * (size_t)n, err);
+  }
+  DUST_HIP(err);
+  DUST_TRY(elem_apply_dev(ngroups, dg, prm, n, dpos, dal, dvel, dst, nullptr));
+  DUST_HIP(hipMemcpy(vel, dvel, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+  if (st) DUST_HIP(hipMemcpy(stretch, dst, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int dust_vort_vel_dev(int m, const double *cen, int np, const double *pcen, const double *pdir,
+                      const double *pmag, const double *prv, int nl, const double *lver,
+                      const double *lmag, const dust_elem_params *prm, double *uvort,
+                      void *stream) {
+  return vort_vel_dev(m, cen, np, pcen, pdir, pmag, prv, nl, lver, lmag, prm, uvort,
+                      (hipStream_t)stream);
+}
+
+int dust_vort_vel(int m, const double *cen, int np, const double *pcen, const double *pdir,
+                  const double *pmag, const double *prv, int nl, const double *lver,
+                  const double *lmag, const dust_elem_params *prm, double *uvort) {
+  DUST_CHECK(prm && m >= 0 && np >= 0 && nl >= 0, DUST_E_ARG, "invalid uvort arguments");
+  DUST_CHECK(m == 0 || (cen && uvort), DUST_E_ARG, "missing centres or output");
+  DUST_CHECK(np == 0 || (pcen && pdir && pmag && prv), DUST_E_ARG, "missing particle arrays");
+  DUST_CHECK(nl == 0 || lver || !lmag, DUST_E_ARG, "missing end-vortex vertices");
+  if (m == 0) return 0;
+  size_t nd = 6 * (size_t)m + 8 * (size_t)np + 7 * (size_t)nl;
+  DUST_TRY(g_ws.h_d.reserve(nd));
+  size_t od = 0;
+  hipError_t err = hipSuccess;
+  const double *dc = stage(g_ws.h_d, od, cen, 3 * (size_t)m, err);
+  const double *dpc = stage(g_ws.h_d, od, pcen, 3 * (size_t)np, err);
+  const double *dpd = stage(g_ws.h_d, od, pdir, 3 * (size_t)np, err);
+  const double *dpm = stage(g_ws.h_d, od, pmag, (size_t)np, err);
+  const double *dpr = stage(g_ws.h_d, od, prv, (size_t)np, err);
+  const double *dlv = stage(g_ws.h_d, od, lver, 6 * (size_t)nl, err);
+  const double *dlm = stage(g_ws.h_d, od, lmag, (size_t)nl, err);
+  double *du = g_ws.h_d.p + od;
+  DUST_HIP(err);
+  DUST_TRY(vort_vel_dev(m, dc, np, dpc, dpd, dpm, dpr, nl, dlv, dlm, prm, du, nullptr));
+  DUST_HIP(hipMemcpy(uvort, du, sizeof(double) * 3 * m, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // extern "C"

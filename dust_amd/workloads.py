@@ -9,6 +9,10 @@ wing_case    -- rectangular NACA0012 wing (40 chordwise per side x 20 spanwise)
                 particles)
 rotor        -- b rectangular NACA0012 blades (aspect ratio 10) around the z
                 axis, the surface-panel mesh of one rotor case
+element_lists -- element lists of apply_multipole around a panel mesh: the
+                body surface panels (doublet + source strengths), a first
+                row of wake panels and rings behind given trailing-edge
+                points, and the end vortices closing them
 batch_cases  -- the many-case workload: 48 rotor cases with 2-5 blades and
                 500-4,000 panels, each with a helix wake of 50k-400k
                 particles (pitch 0.4-0.6)
@@ -228,3 +232,53 @@ def batch_cases(ncase=48, seed=20261205):
         out.append(dict(blades=blades, ver=ver, n_ver=nv, cen=cen, alpha=alpha, rv=rv,
                         tree=tree))
     return out
+
+
+def element_lists(ver, n_ver, te, u_inf=(30.0, 0.0, 0.0), dx=0.05, wake_dir=(1.0, 0.0, 0.0),
+                  seed=11):
+    """Element lists of apply_multipole around the surface-panel mesh
+    (ver, n_ver) with the trailing edge te (k+1, 3) (a polyline of k
+    segments): a list of dicts (kind 0 surface panel, 1 vortex ring,
+    2 vortex line; the kinds of dust_amd.elements) in the reference's order
+    elem, wpan, wrin, wvort.
+
+    - elem: the surface panels with a smooth doublet strength and the source
+      strength nor.(-u_inf) of a body at rest;
+    - wpan: one row of wake panels (quads te_i, te_i + d, te_i+1 + d, te_i+1,
+      d = dx wake_dir, wake_dir a unit vector across the trailing edge) with
+      circulation Gamma_i;
+    - wrin: the same row moved one more dx downstream (vortex rings that take
+      no part in the stretching term);
+    - wvort: the k end vortices (te_i + 2 d -> te_i+1 + 2 d) closing it."""
+    rng = np.random.default_rng(seed)
+    ver = np.asarray(ver, dtype=np.float64)
+    n_ver = np.asarray(n_ver, dtype=np.int32)
+    cen = np.array([ver[i, :n_ver[i]].mean(axis=0) for i in range(len(ver))])
+    a = np.where(n_ver[:, None] == 4, ver[:, 2] - ver[:, 0], ver[:, 2] - ver[:, 1])
+    b = np.where(n_ver[:, None] == 4, ver[:, 3] - ver[:, 1], ver[:, 0] - ver[:, 1])
+    nor = np.cross(a, b)
+    nor /= np.linalg.norm(nor, axis=1)[:, None]
+    mag = 0.5 * np.sin(cen[:, 0] * 1.3) + 0.2 * np.cos(cen[:, 1] * 0.7) \
+        + 0.05 * rng.standard_normal(len(cen))
+    sigma = nor @ (-np.asarray(u_inf, dtype=np.float64))
+    te = np.asarray(te, dtype=np.float64)
+    k = len(te) - 1
+    s = np.linspace(-1.0, 1.0, k)
+    gam = np.sqrt(np.maximum(1.0 - s**2, 0.0)) + 0.1
+    d = dx * np.asarray(wake_dir, dtype=np.float64)
+    wp = np.array([[te[i], te[i] + d, te[i + 1] + d, te[i + 1]] for i in range(k)])
+    wr = wp + d
+    wl = np.array([[te[i] + 2 * d, te[i + 1] + 2 * d] for i in range(k)])
+    return [dict(kind=0, ver=ver, n_ver=n_ver, mag=mag, sigma=sigma, stretch=True),
+            dict(kind=1, ver=wp, n_ver=np.full(k, 4, np.int32), mag=gam, stretch=True),
+            dict(kind=1, ver=wr, n_ver=np.full(k, 4, np.int32), mag=0.8 * gam, stretch=False),
+            dict(kind=2, ver=wl, n_ver=np.full(k, 2, np.int32), mag=0.8 * gam, stretch=True)]
+
+
+def wing_elements(nc=40, ns=20, aspect=8.0, alpha_deg=5.0, u=30.0):
+    """element_lists of the wing_case wing (trailing edge at chord 1)."""
+    ver, nv, *_ = wing_case(nc, ns, aspect, alpha_deg, u, nrows=1)
+    ca, sa = np.cos(np.radians(alpha_deg)), np.sin(np.radians(alpha_deg))
+    ys = np.linspace(-aspect / 2, aspect / 2, ns + 1)
+    te = np.array([[ca, y, -sa] for y in ys])
+    return element_lists(ver, nv, te, u_inf=(u, 0.0, 0.0))

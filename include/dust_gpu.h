@@ -170,6 +170,71 @@ int dust_vortpart_direct(int n, const double *cen, const double *dir, const doub
                          const double u_inf[3], double *vel, double *stretch);
 
 /* ======================================================================== */
+/* Element <-> particle direct terms: src/fmm/mod_octree.f90:1350-1401,     */
+/* get_vort_vel (src/aeroel/mod_surfpan.f90:962-977, mod_vortlatt.f90:676) */
+/* ======================================================================== */
+
+/* Element kinds (the element types behind apply_multipole's lists). */
+#define DUST_ELEM_SURFPAN 0 /* t_surfpan: doublet + source (compute_vel_surfpan, mod_surfpan.f90:641) */
+#define DUST_ELEM_RING 1    /* t_vortlatt / t_liftlin / wake panels: vortex ring (velocity_calc_doublet) */
+#define DUST_ELEM_LINE 2    /* t_vortline end vortex (compute_vel_vortline, mod_vortline.f90:126) */
+
+/* One element list of apply_multipole (elem, wpan, wrin or wvort).
+ * Polygons: n_ver(n) in {3,4}, ver(3,4,n) (the 4th vertex of a triangle is
+ * unused); lines: ver(3,2,n), n_ver unused.  mag(n): doublet / circulation
+ * strength, NULL = no doublet or line term (a line whose strength is not
+ * associated induces nothing).  sigma(n), surface panels only: the source
+ * strength nor.(ub - wind(cen) - uvort) of compute_vel_surfpan; NULL = none.
+ * stretch: 1 if the list takes part in the vs_elems stretching term (elem,
+ * wpan, wvort in the reference), 0 if not (wrin). */
+typedef struct {
+  int kind;
+  int n;
+  const int *n_ver;
+  const double *ver;
+  const double *mag;
+  const double *sigma;
+  int stretch;
+  int reserved;
+} dust_elem_group;
+
+typedef struct {
+  double ff_ratio_doublet; /* sim_param%FarFieldRatioDoublet                        */
+  double rankine_rad;      /* sim_param%RankineRad                                  */
+  double cutoff_rad;       /* sim_param%CutoffRad                                   */
+  double vortex_rad;       /* sim_param%VortexRad, core of gradient_calc_doublet    */
+  int reserved[4];
+} dust_elem_params;
+
+/* The element loops of apply_multipole (mod_octree.f90:1350-1401) at n
+ * points pos(3,n): vel(3,n) += sum over the groups, in order, of
+ * compute_vel / (4 pi); if stretch and alpha are given (use_vs and
+ * vs_elems), stretch(3,n) += sum over the groups with `stretch` set of
+ * transpose(compute_grad) alpha / (4 pi).  Host pointers, synchronous. */
+int dust_elem_apply(int ngroups, const dust_elem_group *groups, const dust_elem_params *prm,
+                    int n, const double *pos, const double *alpha, double *vel, double *stretch);
+/* Same with device pointers (group arrays, pos, alpha, vel, stretch; the
+ * group and parameter structs themselves are host memory), ordered on
+ * `stream`; n_ver must hold 3 or 4. */
+int dust_elem_apply_dev(int ngroups, const dust_elem_group *groups, const dust_elem_params *prm,
+                        int n, const double *pos, const double *alpha, double *vel,
+                        double *stretch, void *stream);
+
+/* get_vort_vel (mod_surfpan.f90:962-977, mod_vortlatt.f90:676-693) at m
+ * centres cen(3,m): uvort(3,m) = sum over the np vortex particles
+ * (compute_vel_vortpart, mod_vortpart.f90:139-153: Rosenhead core of the
+ * source radius r_vortex) and then the nl end-vortex lines lver(3,2,nl) of
+ * strength lmag (NULL: none associated) of compute_vel / (4 pi).
+ * Host pointers, synchronous. */
+int dust_vort_vel(int m, const double *cen, int np, const double *pcen, const double *pdir,
+                  const double *pmag, const double *prv, int nl, const double *lver,
+                  const double *lmag, const dust_elem_params *prm, double *uvort);
+int dust_vort_vel_dev(int m, const double *cen, int np, const double *pcen, const double *pdir,
+                      const double *pmag, const double *prv, int nl, const double *lver,
+                      const double *lmag, const dust_elem_params *prm, double *uvort,
+                      void *stream);
+
+/* ======================================================================== */
 /* AIC assembly: src/aeroel/mod_surfpan.f90, mod_vortlatt.f90, mod_doublet   */
 /* ======================================================================== */
 

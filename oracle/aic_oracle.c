@@ -26,37 +26,7 @@
 #include <omp.h>
 #endif
 
-#define AO_PI 3.1415926535897932384626433832795028841971693993751058209749445923
-
-/* Panel geometry record; layout shared with the tests (74 doubles). */
-enum {
-  G_NVER = 0,       /* number of vertices (3 or 4), stored as double     */
-  G_CEN = 1,        /* 3                                                */
-  G_NOR = 4,        /* 3                                                */
-  G_TANG = 7,       /* tang(3,2) column-major: t1 then t2               */
-  G_AREA = 13,      /* 1                                                */
-  G_VER = 14,       /* ver(3,4)                                         */
-  G_EVEC = 26,      /* edge_vec(3,4)                                    */
-  G_ELEN = 38,      /* edge_len(4)                                      */
-  G_EUNI = 42,      /* edge_uni(3,4)                                    */
-  G_COST = 54,      /* cosTi(4)                                         */
-  G_SINT = 58,      /* sinTi(4)                                         */
-  G_VERP = 62,      /* verp(3,4)                                        */
-  G_SIZE = 74
-};
-
-static inline void ao_cross(const double *a, const double *b, double *r) {
-  r[0] = a[1] * b[2] - a[2] * b[1];
-  r[1] = a[2] * b[0] - a[0] * b[2];
-  r[2] = a[0] * b[1] - a[1] * b[0];
-}
-static inline double ao_dot(const double *a, const double *b) {
-  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-}
-static inline double ao_norm(const double *a) { return sqrt(ao_dot(a, a)); }
-
-static const int next_tri[3] = {1, 2, 0}, next_qua[4] = {1, 2, 3, 0};
-static const int prev_tri[3] = {2, 0, 1}, prev_qua[4] = {3, 0, 1, 2};
+#include "aic_oracle.h"
 
 /* calc_geo_data_surfpan (mod_surfpan.f90:890-957); vortex rings use the same
  * fields (mod_vortlatt.f90:601-665) plus a guard on a zero normal. */
@@ -104,13 +74,6 @@ void ao_geo(int n, const int *nver, const double *ver /* (3,4,n) */, double *G /
       for (int e = 0; e < 3; ++e) g[G_VERP + 3 * is + e] = v[3 * is + e] - g[G_NOR + e] * z;
     }
   }
-}
-
-static inline double maxedge(const double *g) {
-  int nv = (int)g[G_NVER];
-  double m = g[G_ELEN];
-  for (int i = 1; i < nv; ++i) m = g[G_ELEN + i] > m ? g[G_ELEN + i] : m;
-  return m;
 }
 
 /* potential_calc_doublet, mod_doublet.f90:113-168 */

@@ -58,6 +58,14 @@ def lib():
             "ao_assemble_vortlatt": (None, [C.c_int, _vp, C.c_double, C.c_double, C.c_double, _vp,
                                             C.c_int, _vp, C.c_int]),
             "ao_rhs_static": (None, [C.c_int, _vp, _vp, C.c_int, _vp, _vp, _vp]),
+            "ao_vel_source": (None, [_vp, _vp, C.c_double, _vp]),
+            "ao_grad_doublet": (None, [_vp, _vp, C.c_double, _vp]),
+            "ao_vel_vortline": (None, [_vp, _vp, C.c_double, C.c_double, _vp]),
+            "ao_grad_vortline": (None, [_vp, _vp, C.c_double, _vp]),
+            "ao_elem_group": (None, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, C.c_int, _vp, _vp, _vp, _vp]),
+            "ao_vort_vel": (None, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp,
+                                   C.c_double, C.c_double, _vp]),
             "ao_dgetrf": (C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp]),
             "ao_dgetrs": (None, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int]),
         }
@@ -272,6 +280,82 @@ def rhs_static(G, B, u_inf, uvort=None):
     lib().ao_rhs_static(n, _p(_c(G)), _p(np.asfortranarray(B)), n, _p(_c(u_inf)),
                         None if uvort is None else _p(_c(uvort)), _p(b))
     return b
+
+
+# ---- element <-> particle direct terms (elem_oracle.c) ------------------------
+SURFPAN, RING, LINE = 0, 1, 2
+
+
+def vel_source(g, pos, ff_ratio=10.0):
+    v = np.zeros(3)
+    lib().ao_vel_source(_p(_c(g)), _p(_c(pos)), float(ff_ratio), _p(v))
+    return v
+
+
+def grad_doublet(g, pos, vortex_rad):
+    """gradient_calc_doublet as a 3x3 array grad[r, c] (Fortran grad(r+1, c+1))."""
+    gr = np.zeros(9)
+    lib().ao_grad_doublet(_p(_c(g)), _p(_c(pos)), float(vortex_rad), _p(gr))
+    return gr.reshape(3, 3, order="F")
+
+
+def vel_vortline(ver, pos, rankine=0.1, cutoff=0.001):
+    v = np.zeros(3)
+    lib().ao_vel_vortline(_p(_c(ver)), _p(_c(pos)), float(rankine), float(cutoff), _p(v))
+    return v
+
+
+def grad_vortline(ver, pos, vortex_rad):
+    gr = np.zeros(9)
+    lib().ao_grad_vortline(_p(_c(ver)), _p(_c(pos)), float(vortex_rad), _p(gr))
+    return gr.reshape(3, 3, order="F")
+
+
+def elem_apply(groups, pos, alpha=None, vel=None, stretch=None, ff_ratio=10.0, rankine=0.1,
+               cutoff=0.001, vortex_rad=0.1):
+    """The element loops of apply_multipole over `groups`, a list of dicts with
+    kind, ver ((n,4,3) polygons or (n,2,3) lines), n_ver, mag, sigma and
+    stretch (bool).  vel / stretch are accumulated (start from zero if None);
+    the stretching of all groups is summed first and then added, as the
+    reference does."""
+    pos = _c(pos).reshape(-1, 3)
+    npts = pos.shape[0]
+    vel = np.zeros((npts, 3)) if vel is None else np.array(vel, dtype=np.float64).reshape(-1, 3)
+    st_el = np.zeros((npts, 3)) if alpha is not None else None
+    for gdef in groups:
+        kind = gdef["kind"]
+        ver = np.asarray(gdef["ver"], dtype=np.float64)
+        n = ver.shape[0]
+        G = geo(ver, np.asarray(gdef["n_ver"], np.int32)) if kind != LINE else None
+        mag = gdef.get("mag")
+        sig = gdef.get("sigma")
+        do_st = alpha is not None and gdef.get("stretch", True)
+        lib().ao_elem_group(kind, n, _p(G), _p(_c(ver)) if kind == LINE else None,
+                            None if mag is None else _p(_c(mag)),
+                            None if sig is None else _p(_c(sig)), float(ff_ratio), float(rankine),
+                            float(cutoff), float(vortex_rad), npts, _p(pos),
+                            None if alpha is None else _p(_c(alpha)), _p(vel),
+                            _p(st_el) if do_st else None)
+    if alpha is not None:
+        stretch = np.zeros((npts, 3)) if stretch is None else np.array(stretch, dtype=np.float64)
+        stretch = stretch.reshape(-1, 3) + st_el
+    return vel, stretch
+
+
+def vort_vel(cen, pcen=None, pdir=None, pmag=None, prv=None, lver=None, lmag=None, rankine=0.1,
+             cutoff=0.001):
+    """get_vort_vel at the centres `cen` (m,3) from particles and end-vortex lines."""
+    cen = _c(cen).reshape(-1, 3)
+    m = cen.shape[0]
+    np_ = 0 if pcen is None else len(pmag)
+    nl = 0 if lver is None else np.asarray(lver).shape[0]
+    u = np.zeros((m, 3))
+    lib().ao_vort_vel(m, _p(cen), np_, _p(_c(pcen)) if np_ else None,
+                      _p(_c(pdir)) if np_ else None, _p(_c(pmag)) if np_ else None,
+                      _p(_c(prv)) if np_ else None, nl, _p(_c(lver)) if nl else None,
+                      None if (lmag is None or not nl) else _p(_c(lmag)), float(rankine),
+                      float(cutoff), _p(u))
+    return u
 
 
 def dgetrf(A):

@@ -43,3 +43,27 @@ def test_version_and_error_text():
     rc = lib.dust_dgetrf(-1, 1, None, 1, None)
     assert rc == L.DUST_E_ARG
     assert lib.dust_last_error().decode()
+
+
+def test_element_argument_errors_without_gpu():
+    """dust_elem_apply / dust_vort_vel reject bad lists before any device work."""
+    import numpy as np
+    lib = L.lib()
+    prm = L.ElemParams(10.0, 0.1, 0.001, 0.1)
+    ver = np.zeros((1, 4, 3))
+    nv = np.array([5], np.int32)
+    grp = (L.ElemGroup * 1)()
+    grp[0].kind, grp[0].n, grp[0].n_ver, grp[0].ver = L.ELEM_RING, 1, nv.ctypes.data, \
+        ver.ctypes.data
+    pos = np.zeros(3)
+    vel = np.zeros(3)
+    assert lib.dust_elem_apply(1, grp, ctypes.byref(prm), 1, pos.ctypes.data, None,
+                               vel.ctypes.data, None) == L.DUST_E_ARG
+    assert "vertices" in lib.dust_last_error().decode()
+    grp[0].kind = 7
+    assert lib.dust_elem_apply(1, grp, ctypes.byref(prm), 1, pos.ctypes.data, None,
+                               vel.ctypes.data, None) == L.DUST_E_ARG
+    assert lib.dust_vort_vel(-1, None, 0, None, None, None, None, 0, None, None,
+                             ctypes.byref(prm), None) == L.DUST_E_ARG
+    assert lib.dust_vort_vel(2, pos.ctypes.data, 3, None, None, None, None, 0, None, None,
+                             ctypes.byref(prm), vel.ctypes.data) == L.DUST_E_ARG
