@@ -32,6 +32,19 @@ module mod_dust_gpu
     integer(c_int) :: reserved(4)
   end type
 
+  ! dust_elem_group / dust_elem_params (include/dust_gpu.h), same field order
+  integer(c_int), parameter :: DUST_ELEM_SURFPAN = 0, DUST_ELEM_RING = 1, DUST_ELEM_LINE = 2
+  type, bind(C) :: t_dust_elem_group
+    integer(c_int) :: kind, n
+    type(c_ptr)    :: n_ver, ver, mag, sigma
+    integer(c_int) :: stretch, reserved
+  end type
+
+  type, bind(C) :: t_dust_elem_params
+    real(c_double) :: ff_ratio_doublet, rankine_rad, cutoff_rad, vortex_rad
+    integer(c_int) :: reserved(4)
+  end type
+
   interface
     integer(c_int) function dust_fmm_create(prm, capacity, h) bind(C)
       import; type(t_dust_fmm_params), intent(in) :: prm
@@ -92,6 +105,23 @@ module mod_dust_gpu
     end function
     integer(c_int) function dust_fmm_work(h, p2p_pairs, m2l_pairs) bind(C)
       import; type(c_ptr), value :: h; real(c_double), intent(out) :: p2p_pairs, m2l_pairs
+    end function
+    integer(c_int) function dust_elem_apply(ngroups, groups, prm, n, pos, alpha, vel, &
+                                            stretch) bind(C)
+      import; integer(c_int), value :: ngroups, n
+      type(t_dust_elem_group), intent(in) :: groups(*)
+      type(t_dust_elem_params), intent(in) :: prm
+      real(c_double), intent(in) :: pos(3,*)
+      type(c_ptr), value :: alpha, stretch
+      real(c_double), intent(inout) :: vel(3,*)
+    end function
+    integer(c_int) function dust_vort_vel(m, cen, np, pcen, pdir, pmag, prv, nl, lver, lmag, &
+                                          prm, uvort) bind(C)
+      import; integer(c_int), value :: m, np, nl
+      real(c_double), intent(in) :: cen(3,*)
+      type(c_ptr), value :: pcen, pdir, pmag, prv, lver, lmag
+      type(t_dust_elem_params), intent(in) :: prm
+      real(c_double), intent(out) :: uvort(3,*)
     end function
   end interface
 
