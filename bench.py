@@ -408,6 +408,68 @@ def time_elements(torch, L, W, n, steps, with_cpu=True):
     return out
 
 
+def time_wake(torch, L, nmax, steps, with_cpu=True):
+    """The device particle pool at the rotor's wake size: nmax slots, 90 %
+    live, one complete_wake particle update (Euler + divergence filter + box
+    culling), the live list of prepare_wake and the shedding of 96 particles,
+    timed with HIP events on the null stream.  HBM bytes per live particle of
+    the update: reads free 1 + cen 24 + vel 24 + dir 24 + mag 8 + stretch 24 +
+    rotu 24, writes cen 24 + dir 24 + stretch 24 = 201 B (1 B per free slot)."""
+    from dust_amd import wake as WK
+    rng = np.random.default_rng(3)
+    P = WK.ParticlePool(nmax)
+    free = (rng.random(nmax) < 0.1).astype(np.uint8)
+    d = rng.standard_normal((nmax, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    host = dict(free=free, cen=rng.uniform(-5.0, 5.0, (nmax, 3)), dir=d,
+                mag=rng.uniform(0.1, 1.0, nmax), vel=1e-3 * rng.standard_normal((nmax, 3)),
+                stretch=1e-3 * rng.standard_normal((nmax, 3)),
+                rotu=rng.standard_normal((nmax, 3)))
+    for k, v in host.items():
+        getattr(P, k).copy_(torch.from_numpy(v))
+    nlive = int(np.count_nonzero(free == 0))
+    P.n_prt.fill_(nlive)
+    box = (np.full(3, -6.0), np.full(3, 6.0))
+    prm = WK.params(*box, dt=1e-3, filt_eta=0.01, use_vs=True, use_divfilt=True)
+    kshed = 96
+    ncen = rng.uniform(-5.0, 5.0, (kshed, 3))
+    npv = rng.standard_normal((kshed, 3))
+
+    def timed(f, reps):
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms_up = timed(lambda: P.update(prm), steps)
+    live_t = torch.zeros(nmax, dtype=torch.int32, device="cuda")
+    n_live = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ms_live = timed(lambda: L.check(L.lib().dust_wake_live_dev(
+        nmax, P.free.data_ptr(), live_t.data_ptr(), n_live.data_ptr(), None), "live"), steps)
+    t0 = time.perf_counter()
+    P.shed(ncen, npv, np.full(kshed, 0.02), np.zeros((kshed, 3)), prm)
+    ms_shed = (time.perf_counter() - t0) * 1e3
+    bytes_up = 201.0 * nlive + 1.0 * (nmax - nlive)
+    out = dict(slots=nmax, live=nlive, update_ms=ms_up, update_GBps=bytes_up / (ms_up / 1e3) / 1e9,
+               update_hbm_frac=bytes_up / (ms_up / 1e3) / 1e9 / PEAK["hbm"], live_list_ms=ms_live,
+               shed96_host_ms=ms_shed, dtype="f64")
+    if with_cpu:
+        from oracle import oracle as orc
+        hp = {k: v.copy() for k, v in host.items()}
+        t0 = time.perf_counter()
+        orc.wake_update(hp, *box, 1e-3, 0.01, use_vs=True, use_divfilt=True)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(update_ms=dt * 1e3, cores=1, kind="port",
+                                   sample=f"oracle particle loop over the same {nmax} slots "
+                                          f"(one thread, {dt:.2f} s)")
+    return out
+
+
 def cpu_baseline(W, n_sample, degree):
     """The CPU oracle (FP64 restatement, OpenMP) on a bounded sample of the same
     generator; particle updates per second."""
@@ -557,7 +619,8 @@ def main():
     ap.add_argument("--no-elements", action="store_true",
                     help="skip the element <-> particle leg (rotor panels, helix particles)")
     ap.add_argument("--elements-only", action="store_true",
-                    help="time only the element <-> particle leg and print its JSON")
+                    help="time only the element <-> particle and particle-pool legs, print "
+                         "their JSON")
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB",
                                                           "COPY_WRITE_DB"))
     ap.add_argument("--summarize-trace", metavar="TRACE_DB")
@@ -586,8 +649,9 @@ def main():
     L.lib()
     n = args.n
     if args.elements_only:
-        print(json.dumps(time_elements(torch, L, W, n, args.steps, with_cpu=not args.no_cpu),
-                         default=float))
+        out = dict(elements=time_elements(torch, L, W, n, args.steps, with_cpu=not args.no_cpu),
+                   wake=time_wake(torch, L, 5_000_000, args.steps, with_cpu=not args.no_cpu))
+        print(json.dumps(out, default=float))
         return
     log(f"generating helix wake n={n}")
     cen, alpha, rv, tree = W.helix_wake(n)
@@ -647,6 +711,9 @@ def main():
         log("timing the element <-> particle terms")
         elements = time_elements(torch, L, W, n, 3, with_cpu=not args.no_cpu)
         log(f"elements: {elements}")
+        log("timing the device particle pool (5M slots)")
+        elements["wake_pool"] = time_wake(torch, L, 5_000_000, 5, with_cpu=not args.no_cpu)
+        log(f"wake pool: {elements['wake_pool']}")
     batch = None
     if not args.no_batch:
         log("timing the 48-case batch")

@@ -87,6 +87,19 @@ class ElemParams(C.Structure):
     ]
 
 
+class WakeParams(C.Structure):
+    _fields_ = [
+        ("box_min", C.c_double * 3),
+        ("box_max", C.c_double * 3),
+        ("dt", C.c_double),
+        ("filt_eta", C.c_double),
+        ("use_vs", C.c_int),
+        ("use_vd", C.c_int),
+        ("use_divfilt", C.c_int),
+        ("reserved", C.c_int * 5),
+    ]
+
+
 # (name, restype, argtypes) of every exported function in include/dust_gpu.h
 SIGNATURES = [
     ("dust_last_error", C.c_char_p, []),
@@ -123,6 +136,12 @@ SIGNATURES = [
     ("dust_vort_vel_dev", C.c_int,
      [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, C.POINTER(ElemParams), _vp,
       _vp]),
+    ("dust_wake_update_dev", C.c_int,
+     [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(WakeParams), _vp, _vp]),
+    ("dust_wake_live_dev", C.c_int, [C.c_int, _vp, _vp, _vp, _vp]),
+    ("dust_wake_shed_dev", C.c_int,
+     [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp, C.POINTER(WakeParams),
+      _vp, _vp]),
     ("dust_aic_surfpan", C.c_int,
      [C.c_int, _vp, _vp, C.POINTER(AicParams), _vp, C.c_int, _vp, C.c_int]),
     ("dust_aic_surfpan_dev", C.c_int,

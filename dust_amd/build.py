@@ -28,6 +28,7 @@ SOURCES = [
     "fmm_tuned.hip",
     "aic.hip",
     "elem.hip",
+    "wake.hip",
     "lu.hip",
     "lu_mixed.hip",
     "linsys.hip",

@@ -1,0 +1,224 @@
+// Device-resident vortex-particle pool of the wake: the per-step particle
+// update of complete_wake, the live-particle list of prepare_wake and the
+// insertion of newly shed particles into free slots.
+//
+// Reference behaviour (src/aeroel/mod_wake.f90):
+//   complete_wake particle loop 1305-1355: pos = cen + vel dt; inside the
+//     particle box -> cen = pos and (use_vs or use_vd) optional divergence
+//     filter stretch -= filt_eta (dir mag - rotu mag/|rotu|), then explicit
+//     Euler alpha = dir mag + stretch dt, dir = alpha/|alpha| if |alpha| /= 0
+//     (mag is left as it is); outside -> the slot is freed, n_prt -= 1;
+//   prepare_wake 826-835: part_p = the non-free slots in slot order;
+//   complete_wake shedding 1888-1967 (and the ring branch after it): every
+//     new particle inside the box takes the lowest free slot, mag = |partvec|,
+//     dir = partvec/mag (partvec itself when mag <= 1e-13), n_prt += 1; no
+//     free slot left is an error.
+//
+// The pool is SoA in HBM (cen(3,nmax), dir(3,nmax), mag, r_vortex, vel,
+// stretch, rotu, one free byte per slot).  The update is one thread per slot
+// (HBM-bound: about 170 bytes per live particle); the list and the insertion
+// use hipcub stream compaction of the slot flags.
+#include <hipcub/hipcub.hpp>
+
+#include "common.h"
+
+namespace dust {
+namespace {
+
+__global__ void __launch_bounds__(256)
+    k_wake_update(int nmax, unsigned char *freef, double *cen, double *dir,
+                  const double *__restrict__ mag, const double *__restrict__ vel,
+                  double *stretch, const double *__restrict__ rotu, dust_wake_params P,
+                  int *n_prt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int freed = 0;
+  if (i < nmax && !freef[i]) {
+    const double dt = P.dt;
+    double pos[3];
+    bool inside = true;
+    for (int e = 0; e < 3; ++e) {
+      pos[e] = cen[3 * (long)i + e] + vel[3 * (long)i + e] * dt;
+      inside = inside && pos[e] >= P.box_min[e] && pos[e] <= P.box_max[e];
+    }
+    if (inside) {
+      for (int e = 0; e < 3; ++e) cen[3 * (long)i + e] = pos[e];
+      if (P.use_vs || P.use_vd) {
+        const double m = mag[i];
+        double d[3], s[3];
+        for (int e = 0; e < 3; ++e) {
+          d[e] = dir[3 * (long)i + e];
+          s[e] = stretch[3 * (long)i + e];
+        }
+        if (P.use_divfilt) {
+          double r[3] = {rotu[3 * (long)i], rotu[3 * (long)i + 1], rotu[3 * (long)i + 2]};
+          double rn = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+          for (int e = 0; e < 3; ++e) {
+            s[e] = s[e] - P.filt_eta * (d[e] * m - r[e] * m / rn);
+            stretch[3 * (long)i + e] = s[e];
+          }
+        }
+        double a[3];
+        for (int e = 0; e < 3; ++e) a[e] = d[e] * m + s[e] * dt;
+        double an = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        if (an != 0.0)
+          for (int e = 0; e < 3; ++e) dir[3 * (long)i + e] = a[e] / an;
+      }
+    } else {
+      freef[i] = 1;
+      freed = 1;
+    }
+  }
+  // one atomic per wave with a freed slot
+  unsigned long long b = __ballot(freed);
+  if (b && (threadIdx.x & 63) == __builtin_ctzll(b)) atomicSub(n_prt, __popcll(b));
+}
+
+struct NotFree {
+  __host__ __device__ bool operator()(const unsigned char &f) const { return f == 0; }
+};
+struct IsFree {
+  __host__ __device__ bool operator()(const unsigned char &f) const { return f != 0; }
+};
+
+// New particle j (in order) inside the box takes free slot rank[j] of the
+// ascending free list; rank = number of inside particles before j.
+__global__ void k_wake_insert(int k, const double *__restrict__ ncen,
+                              const double *__restrict__ npv, const double *__restrict__ nrv,
+                              const double *__restrict__ nvel, const int *__restrict__ free_list,
+                              const int *__restrict__ n_free, dust_wake_params P,
+                              unsigned char *freef, double *cen, double *dir, double *mag,
+                              double *rv, double *vel, int *n_prt, int *status) {
+  // a single block walks the (few) new particles in order
+  __shared__ int s_rank;
+  if (threadIdx.x == 0) s_rank = 0;
+  __syncthreads();
+  const int nf = *n_free;
+  for (int base = 0; base < k; base += blockDim.x) {
+    int j = base + threadIdx.x;
+    bool in = false;
+    if (j < k) {
+      in = true;
+      for (int e = 0; e < 3; ++e) {
+        double x = ncen[3 * j + e];
+        in = in && x >= P.box_min[e] && x <= P.box_max[e];
+      }
+    }
+    // exclusive rank of this particle among the inside ones of the chunk
+    __shared__ int s_flag[1024];
+    s_flag[threadIdx.x] = in ? 1 : 0;
+    __syncthreads();
+    int r = s_rank;
+    for (int t = 0; t < (int)threadIdx.x; ++t) r += s_flag[t];
+    __syncthreads();
+    if (in) {
+      if (r < nf) {
+        int slot = free_list[r];
+        double pv[3] = {npv[3 * j], npv[3 * j + 1], npv[3 * j + 2]};
+        double m = sqrt(pv[0] * pv[0] + pv[1] * pv[1] + pv[2] * pv[2]);
+        freef[slot] = 0;
+        mag[slot] = m;
+        for (int e = 0; e < 3; ++e) {
+          dir[3 * (long)slot + e] = m > 1.0e-13 ? pv[e] / m : pv[e];
+          cen[3 * (long)slot + e] = ncen[3 * j + e];
+          vel[3 * (long)slot + e] = nvel[3 * j + e];
+        }
+        rv[slot] = nrv[j];
+        atomicAdd(n_prt, 1);
+      } else {
+        atomicOr(status, 1);  // more new particles than free slots
+      }
+    }
+    if (threadIdx.x == blockDim.x - 1) s_rank = r + (in ? 1 : 0);
+    __syncthreads();
+  }
+}
+
+struct WakeWs {
+  DevBuf<int> list, count;
+  DevBuf<char> tmp;
+  HostBuf<int> hcount;
+};
+WakeWs g_wws;
+
+int select_slots(int nmax, const unsigned char *freef, bool want_free, int *out, int *d_count,
+                 hipStream_t st) {
+  hipcub::CountingInputIterator<int> it(0);
+  size_t bytes = 0;
+  if (want_free) {
+    hipcub::TransformInputIterator<bool, IsFree, const unsigned char *> fl(freef, IsFree());
+    DUST_HIP(hipcub::DeviceSelect::Flagged(nullptr, bytes, it, fl, out, d_count, nmax, st));
+    DUST_TRY(g_wws.tmp.reserve(bytes + 256));
+    DUST_HIP(hipcub::DeviceSelect::Flagged(g_wws.tmp.p, bytes, it, fl, out, d_count, nmax, st));
+  } else {
+    hipcub::TransformInputIterator<bool, NotFree, const unsigned char *> fl(freef, NotFree());
+    DUST_HIP(hipcub::DeviceSelect::Flagged(nullptr, bytes, it, fl, out, d_count, nmax, st));
+    DUST_TRY(g_wws.tmp.reserve(bytes + 256));
+    DUST_HIP(hipcub::DeviceSelect::Flagged(g_wws.tmp.p, bytes, it, fl, out, d_count, nmax, st));
+  }
+  return 0;
+}
+
+}  // namespace
+}  // namespace dust
+
+using namespace dust;
+
+extern "C" {
+
+int dust_wake_update_dev(int nmax, unsigned char *free_flag, double *cen, double *dir,
+                         const double *mag, const double *vel, double *stretch,
+                         const double *rotu, const dust_wake_params *prm, int *n_prt,
+                         void *stream) {
+  DUST_CHECK(prm && nmax >= 0 && n_prt, DUST_E_ARG, "invalid wake update arguments");
+  if (nmax == 0) return 0;
+  DUST_CHECK(free_flag && cen && dir && mag && vel, DUST_E_ARG, "missing particle arrays");
+  DUST_CHECK(!(prm->use_vs || prm->use_vd) || stretch, DUST_E_ARG, "stretch needed");
+  DUST_CHECK(!((prm->use_vs || prm->use_vd) && prm->use_divfilt) || rotu, DUST_E_ARG,
+             "rotu needed for the divergence filter");
+  hipStream_t st = (hipStream_t)stream;
+  k_wake_update<<<ceil_div(nmax, 256), 256, 0, st>>>(nmax, free_flag, cen, dir, mag, vel, stretch,
+                                                     rotu, *prm, n_prt);
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
+
+int dust_wake_live_dev(int nmax, const unsigned char *free_flag, int *live, int *n_live,
+                       void *stream) {
+  DUST_CHECK(nmax >= 0 && n_live && (nmax == 0 || (free_flag && live)), DUST_E_ARG,
+             "invalid live-list arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (nmax == 0) {
+    DUST_HIP(hipMemsetAsync(n_live, 0, sizeof(int), st));
+    return 0;
+  }
+  return select_slots(nmax, free_flag, false, live, n_live, st);
+}
+
+int dust_wake_shed_dev(int nmax, unsigned char *free_flag, double *cen, double *dir,
+                       double *mag, double *r_vortex, double *vel, int k, const double *new_cen,
+                       const double *new_partvec, const double *new_rv, const double *new_vel,
+                       const dust_wake_params *prm, int *n_prt, void *stream) {
+  DUST_CHECK(prm && nmax >= 0 && k >= 0 && n_prt, DUST_E_ARG, "invalid shedding arguments");
+  if (k == 0) return 0;
+  DUST_CHECK(new_cen && new_partvec && new_rv && new_vel, DUST_E_ARG, "missing new particles");
+  DUST_CHECK(nmax > 0 && free_flag && cen && dir && mag && r_vortex && vel, DUST_E_ARG,
+             "missing particle arrays");
+  hipStream_t st = (hipStream_t)stream;
+  DUST_TRY(g_wws.list.reserve(nmax));
+  DUST_TRY(g_wws.count.reserve(2));
+  DUST_TRY(g_wws.hcount.reserve(1));
+  DUST_TRY(select_slots(nmax, free_flag, true, g_wws.list.p, g_wws.count.p, st));
+  DUST_HIP(hipMemsetAsync(g_wws.count.p + 1, 0, sizeof(int), st));
+  k_wake_insert<<<1, 1024, 0, st>>>(k, new_cen, new_partvec, new_rv, new_vel, g_wws.list.p,
+                                   g_wws.count.p, *prm, free_flag, cen, dir, mag, r_vortex, vel,
+                                   n_prt, g_wws.count.p + 1);
+  DUST_HIP(hipGetLastError());
+  DUST_HIP(hipMemcpyAsync(g_wws.hcount.p, g_wws.count.p + 1, sizeof(int), hipMemcpyDeviceToHost,
+                          st));
+  DUST_HIP(hipStreamSynchronize(st));
+  DUST_CHECK(g_wws.hcount.p[0] == 0, DUST_E_CAPACITY,
+             "Exceeding the maximum number of %d wake particles introduced", nmax);
+  return 0;
+}
+
+}  // extern "C"

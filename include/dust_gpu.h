@@ -235,6 +235,53 @@ int dust_vort_vel_dev(int m, const double *cen, int np, const double *pcen, cons
                       void *stream);
 
 /* ======================================================================== */
+/* Device-resident particle pool: src/aeroel/mod_wake.f90 complete_wake     */
+/* (1305-1355, 1888-1967) and prepare_wake (826-835)                         */
+/* ======================================================================== */
+
+/* The pool is the reference's wake_parts(nmax) as device SoA: cen(3,nmax),
+ * dir(3,nmax), mag(nmax), r_vortex(nmax), vel(3,nmax), stretch(3,nmax),
+ * rotu(3,nmax) and free_flag(nmax) (1 = free slot).  n_prt is a device int
+ * holding wake%n_prt. */
+typedef struct {
+  double box_min[3]; /* wake%part_box_min                                       */
+  double box_max[3]; /* wake%part_box_max                                       */
+  double dt;         /* sim_param%dt * ndt_update_wake                          */
+  double filt_eta;   /* sim_param%filt_eta / ndt_update_wake                    */
+  int use_vs, use_vd, use_divfilt;
+  int reserved[5];
+} dust_wake_params;
+
+/* Particle loop of complete_wake (mod_wake.f90:1305-1355) over the non-free
+ * slots: move by vel*dt; inside the box, optional divergence filter on
+ * stretch and explicit Euler on the direction (mag unchanged); outside, free
+ * the slot and decrement *n_prt.  The collision avoidance of use_pa
+ * (avoid_collision, :2350) is not part of it.  Device pointers, ordered on
+ * `stream`, no host synchronisation. */
+int dust_wake_update_dev(int nmax, unsigned char *free_flag, double *cen, double *dir,
+                         const double *mag, const double *vel, double *stretch,
+                         const double *rotu, const dust_wake_params *prm, int *n_prt,
+                         void *stream);
+
+/* The live list of prepare_wake (mod_wake.f90:826-835): live(0..*n_live-1)
+ * = the non-free slots in ascending order (0-based); n_live is a device
+ * int. */
+int dust_wake_live_dev(int nmax, const unsigned char *free_flag, int *live, int *n_live,
+                       void *stream);
+
+/* Insertion of k shed particles (mod_wake.f90:1888-1967 and the ring branch):
+ * in order, each new particle inside the box takes the lowest free slot:
+ * mag = |partvec|, dir = partvec/mag (partvec when mag <= 1e-13), cen,
+ * r_vortex, vel as given; *n_prt is incremented.  The geometry that forms
+ * partvec (last wake panel row, neighbours, join_te) stays with the caller.
+ * Returns DUST_E_CAPACITY, as the reference stops, when the free slots run
+ * out.  Device pointers; synchronises `stream` to report that error. */
+int dust_wake_shed_dev(int nmax, unsigned char *free_flag, double *cen, double *dir,
+                       double *mag, double *r_vortex, double *vel, int k, const double *new_cen,
+                       const double *new_partvec, const double *new_rv, const double *new_vel,
+                       const dust_wake_params *prm, int *n_prt, void *stream);
+
+/* ======================================================================== */
 /* AIC assembly: src/aeroel/mod_surfpan.f90, mod_vortlatt.f90, mod_doublet   */
 /* ======================================================================== */
 

@@ -66,6 +66,11 @@ def lib():
                                      C.c_double, C.c_double, C.c_int, _vp, _vp, _vp, _vp]),
             "ao_vort_vel": (None, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp,
                                    C.c_double, C.c_double, _vp]),
+            "ao_wake_update": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                         C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]),
+            "ao_wake_live": (C.c_int, [C.c_int, _vp, _vp]),
+            "ao_wake_shed": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp,
+                                       _vp, _vp, _vp]),
             "ao_dgetrf": (C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp]),
             "ao_dgetrs": (None, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int]),
         }
@@ -356,6 +361,35 @@ def vort_vel(cen, pcen=None, pdir=None, pmag=None, prv=None, lver=None, lmag=Non
                       None if (lmag is None or not nl) else _p(_c(lmag)), float(rankine),
                       float(cutoff), _p(u))
     return u
+
+
+# ---- particle pool of the wake (wake_oracle.c) --------------------------------
+def wake_update(pool, box_min, box_max, dt, filt_eta=0.0, use_vs=True, use_vd=False,
+                use_divfilt=False):
+    """complete_wake's particle loop on a pool dict (free uint8, cen, dir, mag,
+    vel, stretch, rotu as numpy arrays, updated in place); returns the number
+    of slots freed."""
+    n = len(pool["free"])
+    return lib().ao_wake_update(n, _p(pool["free"]), _p(pool["cen"]), _p(pool["dir"]),
+                                _p(pool["mag"]), _p(pool["vel"]), _p(pool["stretch"]),
+                                _p(pool["rotu"]), _p(_c(box_min)), _p(_c(box_max)), float(dt),
+                                float(filt_eta), int(use_vs), int(use_vd), int(use_divfilt))
+
+
+def wake_live(free):
+    live = np.zeros(len(free), np.int32)
+    n = lib().ao_wake_live(len(free), _p(_c(free, np.uint8)), _p(live))
+    return live[:n]
+
+
+def wake_shed(pool, ncen, npv, nrv, nvel, box_min, box_max):
+    """Insertion of the new particles into the pool dict (in place); returns
+    the number inserted or -1 when the pool is full."""
+    n = len(pool["free"])
+    return lib().ao_wake_shed(n, _p(pool["free"]), _p(pool["cen"]), _p(pool["dir"]),
+                              _p(pool["mag"]), _p(pool["rv"]), _p(pool["vel"]), len(nrv),
+                              _p(_c(ncen)), _p(_c(npv)), _p(_c(nrv)), _p(_c(nvel)),
+                              _p(_c(box_min)), _p(_c(box_max)))
 
 
 def dgetrf(A):
