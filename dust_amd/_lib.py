@@ -142,6 +142,11 @@ SIGNATURES = [
     ("dust_wake_shed_dev", C.c_int,
      [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp, C.POINTER(WakeParams),
       _vp, _vp]),
+    ("dust_wake_gather_dev", C.c_int,
+     [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("dust_wake_scatter_dev", C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("dust_aic_rhs_dev", C.c_int,
+     [C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp]),
     ("dust_aic_surfpan", C.c_int,
      [C.c_int, _vp, _vp, C.POINTER(AicParams), _vp, C.c_int, _vp, C.c_int]),
     ("dust_aic_surfpan_dev", C.c_int,

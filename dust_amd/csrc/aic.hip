@@ -70,21 +70,62 @@ __global__ void k_rhs_coef(int n, int nrhs, const Panel *__restrict__ P,
   }
 }
 
-// b(i, r) = sum_j B(i, j) s(j, r), one thread per row, B read once.
+// b(i, r) = sum_j B(i, j) s(j, r), B read once.  Workgroup (x, y) takes 64
+// rows and the column range y; its four waves stride over the range's columns
+// (coalesced 512-byte column segments), then the waves' and the ranges'
+// partial sums are added in a fixed order.
 constexpr int kRhsMax = 16;
-__global__ void k_rhs_gemv(int n, int nrhs, const double *__restrict__ B, long ldb,
-                           const double *__restrict__ S, double *b, long ldbo) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+constexpr int kRhsRows = 64;
+__global__ void __launch_bounds__(256)
+    k_rhs_gemv_part(int n, int nrhs, const double *__restrict__ B, long ldb,
+                    const double *__restrict__ S, int per, double *part) {
+  __shared__ double red[4][kRhsRows][kRhsMax];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * kRhsRows + lane;
+  const int j0 = blockIdx.y * per, j1 = j0 + per < n ? j0 + per : n;
   double acc[kRhsMax];
   for (int r = 0; r < kRhsMax; ++r) acc[r] = 0.0;
-  for (int j = 0; j < n; ++j) {
-    double bij = B[i + ldb * j];
+  if (i < n)
+    for (int j = j0 + w; j < j1; j += 4) {
+      double bij = B[i + ldb * j];
 #pragma unroll
-    for (int r = 0; r < kRhsMax; ++r)
-      if (r < nrhs) acc[r] += bij * S[(long)r * n + j];
+      for (int r = 0; r < kRhsMax; ++r)
+        if (r < nrhs) acc[r] += bij * S[(long)r * n + j];
+    }
+  for (int r = 0; r < kRhsMax; ++r) red[w][lane][r] = acc[r];
+  __syncthreads();
+  if (w == 0 && i < n)
+    for (int r = 0; r < nrhs; ++r)
+      part[((long)blockIdx.y * nrhs + r) * n + i] =
+          ((red[0][lane][r] + red[1][lane][r]) + red[2][lane][r]) + red[3][lane][r];
+}
+
+__global__ void k_rhs_gemv_sum(int n, int nrhs, int Y, const double *__restrict__ part,
+                               double *b, long ldbo) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int r = 0; r < nrhs; ++r) {
+    double s = 0.0;
+    for (int y = 0; y < Y; ++y) s += part[((long)y * nrhs + r) * n + i];
+    b[i + ldbo * r] = s;
   }
-  for (int r = 0; r < nrhs; ++r) b[i + ldbo * r] = acc[r];
+}
+
+// Launch of the two kernels; `part` holds Y*nrhs*n doubles (rhs_split).
+static int rhs_split(int n) {
+  int xb = ceil_div(n, kRhsRows);
+  int Y = ceil_div(1024, xb);
+  int ymax = ceil_div(n, 64);  // at least 64 columns per range
+  return Y < ymax ? Y : ymax;
+}
+
+static void rhs_gemv(int n, int nrhs, const double *B, long ldb, const double *S, double *part,
+                     double *b, long ldbo, hipStream_t st) {
+  int Y = rhs_split(n);
+  int per = ceil_div(n, Y);
+  Y = ceil_div(n, per);
+  k_rhs_gemv_part<<<dim3(ceil_div(n, kRhsRows), Y), 256, 0, st>>>(n, nrhs, B, ldb, S, per, part);
+  k_rhs_gemv_sum<<<ceil_div(n, 256), 256, 0, st>>>(n, nrhs, Y, part, b, ldbo);
 }
 
 static int geo_upload(int n, const int *n_ver, const double *ver, Panel **dP, hipStream_t st) {
@@ -194,11 +235,41 @@ int dust_aic_rhs(int n, const int *n_ver, const double *ver, const double *B, in
   DUST_HIP(hipMemcpy(du, u_inf, sizeof(double) * 3 * nrhs, hipMemcpyHostToDevice));
   if (uvort) DUST_HIP(hipMemcpy(dw, uvort, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
   k_rhs_coef<<<ceil_div(n, 256), 256>>>(n, nrhs, dP, du, dw, dS);
-  k_rhs_gemv<<<ceil_div(n, 128), 128>>>(n, nrhs, dB, n, dS, db, n);
+  double *dpart = nullptr;
+  DUST_HIP(hipMalloc(&dpart, sizeof(double) * (size_t)rhs_split(n) * nrhs * n));
+  rhs_gemv(n, nrhs, dB, n, dS, dpart, db, n, nullptr);
   DUST_HIP(hipMemcpy2D(b, sizeof(double) * ldbo, db, sizeof(double) * n, sizeof(double) * n, nrhs,
                        hipMemcpyDeviceToHost));
   (void)hipFree(dP); (void)hipFree(dB); (void)hipFree(dS); (void)hipFree(du); (void)hipFree(db);
+  (void)hipFree(dpart);
   if (dw) (void)hipFree(dw);
+  return 0;
+}
+
+int dust_aic_rhs_dev(int n, const int *n_ver, const double *ver, const double *B, int ldb,
+                     int nrhs, const double *u_inf, const double *uvort, double *b, int ldbo,
+                     double *sigma, void *stream) {
+  DUST_CHECK(n >= 1 && n_ver && ver && B && u_inf && b && nrhs >= 1 && nrhs <= kRhsMax,
+             DUST_E_ARG, "invalid RHS arguments (nrhs must be 1..%d)", kRhsMax);
+  DUST_CHECK(ldb >= n && ldbo >= n, DUST_E_ARG, "leading dimension smaller than n");
+  hipStream_t st = (hipStream_t)stream;
+  static DevBuf<Panel> P;
+  static DevBuf<double> S, U, Pt;
+  DUST_TRY(P.reserve(n));
+  DUST_TRY(U.reserve(3 * (size_t)kRhsMax));
+  double *dS = sigma;
+  if (!dS) {
+    DUST_TRY(S.reserve((size_t)n * nrhs));
+    dS = S.p;
+  }
+  DUST_HIP(hipMemcpyAsync(U.p, u_inf, sizeof(double) * 3 * nrhs, hipMemcpyHostToDevice, st));
+  k_geo<<<ceil_div(n, 128), 128, 0, st>>>(n, n_ver, ver, P.p);
+  k_rhs_coef<<<ceil_div(n, 256), 256, 0, st>>>(n, nrhs, P.p, U.p, uvort, dS);
+  DUST_TRY(Pt.reserve((size_t)rhs_split(n) * nrhs * n));
+  rhs_gemv(n, nrhs, B, ldb, dS, Pt.p, b, ldbo, st);
+  DUST_HIP(hipGetLastError());
+  // the host copy of u_inf may be reused as soon as this returns
+  DUST_HIP(hipStreamSynchronize(st));
   return 0;
 }
 

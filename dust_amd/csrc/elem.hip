@@ -37,6 +37,7 @@ namespace {
 constexpr double kInv4Pi = 1.0 / (4.0 * kPi);
 constexpr int kFlagDoublet = 1, kFlagSource = 2, kFlagStretch = 4;
 constexpr int kSortMin = 4096;  // targets below this count are evaluated in input order
+constexpr int kFillBlocks = 2048, kMinPerSplit = 32;  // element splits for few targets
 
 struct ElemRec {
   Panel g;  // polygons: calc_geo_data; lines: ver[0..1], evec/euni/elen[0], cen
@@ -208,9 +209,11 @@ __device__ inline void elem_gradT(const ElemRec &E, const double pos[3], double 
 // near-field code for an element if any of its lanes needs it, so the targets
 // are given in Morton order (spatially close targets share waves).
 __global__ void __launch_bounds__(256)
-    k_elem_apply(int ne, const ElemRec *__restrict__ R, double ffr, double rran, double rcut,
+    k_elem_apply(int ne, int per, const ElemRec *__restrict__ R, double ffr, double rran,
+                 double rcut,
                  double del2, int n, const double *__restrict__ pos,
-                 const double *__restrict__ alpha, double *vel, double *stretch, int acc) {
+                 const double *__restrict__ alpha, double *vel, double *stretch, int acc,
+                 double *part) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
@@ -219,7 +222,9 @@ __global__ void __launch_bounds__(256)
   if (do_st)
     for (int e = 0; e < 3; ++e) al[e] = alpha[3 * p + e];
   double v[3] = {0.0, 0.0, 0.0}, st[3] = {0.0, 0.0, 0.0};
-  for (int j = 0; j < ne; ++j) {
+  const int j0 = blockIdx.y * per;
+  const int j1 = j0 + per < ne ? j0 + per : ne;
+  for (int j = j0; j < j1; ++j) {
     const ElemRec &E = R[j];
     const int flags = E.flags;
     double w[3] = {0.0, 0.0, 0.0};
@@ -275,12 +280,36 @@ __global__ void __launch_bounds__(256)
       for (int e = 0; e < 3; ++e) st[e] += sg[e] * m;
     }
   }
+  if (part) {  // one element range of several: raw partial sums
+    double *q = part + 6 * ((long)blockIdx.y * n + p);
+    for (int e = 0; e < 3; ++e) q[e] = v[e];
+    if (do_st)
+      for (int e = 0; e < 3; ++e) q[3 + e] = st[e];
+    return;
+  }
   // accumulate: out += term; otherwise out = term (the sorted path adds the
   // terms into the caller's arrays in a second kernel)
   for (int e = 0; e < 3; ++e) vel[3 * p + e] = (acc ? vel[3 * p + e] : 0.0) + v[e] * kInv4Pi;
   if (do_st)
     for (int e = 0; e < 3; ++e)
       stretch[3 * p + e] = (acc ? stretch[3 * p + e] : 0.0) + st[e] * kInv4Pi;
+}
+
+// Sum of the S element-range partials of each target, in range order.
+__global__ void k_elem_reduce(int n, int S, const double *__restrict__ part, double *vel,
+                              double *stretch, int acc) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int nv = stretch ? 6 : 3;
+  for (int y = 0; y < S; ++y) {
+    const double *q = part + 6 * ((long)y * n + p);
+    for (int e = 0; e < nv; ++e) v[e] += q[e];
+  }
+  for (int e = 0; e < 3; ++e) vel[3 * p + e] = (acc ? vel[3 * p + e] : 0.0) + v[e] * kInv4Pi;
+  if (stretch)
+    for (int e = 0; e < 3; ++e)
+      stretch[3 * p + e] = (acc ? stretch[3 * p + e] : 0.0) + v[3 + e] * kInv4Pi;
 }
 
 // ---- Morton order of the targets -------------------------------------------
@@ -471,7 +500,8 @@ struct ElemWs {
   DevBuf<int> idx, perm;
   DevBuf<double> spos, sal, svel, sst, bb;
   DevBuf<char> tmp;
-  DevBuf<double> h_d;  // staging of the host entry points
+  DevBuf<double> epart;  // element-range partial sums (few targets)
+  DevBuf<double> h_d;    // staging of the host entry points
   DevBuf<int> h_i;
 };
 ElemWs g_ws;
@@ -510,13 +540,31 @@ int elem_apply_dev(int ngroups, const dust_elem_group *groups, const dust_elem_p
     off += G.n;
   }
   const double del2 = prm->vortex_rad * prm->vortex_rad;
-  if (n < kSortMin) {
-    k_elem_apply<<<ceil_div(n, 256), 256, 0, st>>>((int)ne, g_ws.rec.p, prm->ff_ratio_doublet,
-                                                   prm->rankine_rad, prm->cutoff_rad, del2, n,
-                                                   pos, alpha, vel, stretch, 1);
+  // Few targets: the element list is also split (blockIdx.y) so that the
+  // launch fills the chip, and the partial sums are added in range order.
+  auto launch = [&](const double *tp, const double *ta, double *vo, double *so, int acc) -> int {
+    const int tb = ceil_div(n, 256);
+    int S = 1;
+    if (tb < kFillBlocks) {
+      S = ceil_div(kFillBlocks, tb);
+      int smax = ceil_div((int)ne, kMinPerSplit);
+      S = S < smax ? S : smax;
+    }
+    int per = ceil_div((int)ne, S);
+    S = ceil_div((int)ne, per);
+    double *part = nullptr;
+    if (S > 1) {
+      DUST_TRY(g_ws.epart.reserve(6 * (size_t)S * n));
+      part = g_ws.epart.p;
+    }
+    k_elem_apply<<<dim3(tb, S), 256, 0, st>>>((int)ne, per, g_ws.rec.p, prm->ff_ratio_doublet,
+                                             prm->rankine_rad, prm->cutoff_rad, del2, n, tp, ta,
+                                             vo, so, acc, part);
+    if (S > 1) k_elem_reduce<<<tb, 256, 0, st>>>(n, S, part, vo, so, acc);
     DUST_HIP(hipGetLastError());
     return 0;
-  }
+  };
+  if (n < kSortMin) return launch(pos, alpha, vel, stretch, 1);
   // Morton order: bounding box, keys, stable sort, gather, evaluate, scatter-add
   const int nb = ceil_div(n, 256) < 1024 ? ceil_div(n, 256) : 1024;
   DUST_TRY(g_ws.bb.reserve(6 * (size_t)nb));
@@ -542,9 +590,7 @@ int elem_apply_dev(int ngroups, const dust_elem_group *groups, const dust_elem_p
   double *sal = stretch ? g_ws.sal.p : nullptr, *sst = stretch ? g_ws.sst.p : nullptr;
   k_gather_targets<<<ceil_div(n, 256), 256, 0, st>>>(n, g_ws.perm.p, pos, stretch ? alpha : nullptr,
                                                      g_ws.spos.p, sal);
-  k_elem_apply<<<ceil_div(n, 256), 256, 0, st>>>((int)ne, g_ws.rec.p, prm->ff_ratio_doublet,
-                                                 prm->rankine_rad, prm->cutoff_rad, del2, n,
-                                                 g_ws.spos.p, sal, g_ws.svel.p, sst, 0);
+  DUST_TRY(launch(g_ws.spos.p, sal, g_ws.svel.p, sst, 0));
   k_scatter_add<<<ceil_div(n, 256), 256, 0, st>>>(n, g_ws.perm.p, g_ws.svel.p, sst, vel,
                                                   stretch);
   DUST_HIP(hipGetLastError());

@@ -73,6 +73,38 @@ __global__ void __launch_bounds__(256)
   if (b && (threadIdx.x & 63) == __builtin_ctzll(b)) atomicSub(n_prt, __popcll(b));
 }
 
+// live -> contiguous arrays: pcen/pdir/pmag/prv (and palpha = dir mag) for
+// the FMM and the element terms
+__global__ void k_wake_gather(int n, const int *__restrict__ live, const double *__restrict__ cen,
+                              const double *__restrict__ dir, const double *__restrict__ mag,
+                              const double *__restrict__ rv, double *pcen, double *pdir,
+                              double *pmag, double *prv, double *palpha) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  long i = live[k];
+  double m = mag[i];
+  for (int e = 0; e < 3; ++e) {
+    double d = dir[3 * i + e];
+    pcen[3 * (long)k + e] = cen[3 * i + e];
+    pdir[3 * (long)k + e] = d;
+    if (palpha) palpha[3 * (long)k + e] = d * m;
+  }
+  pmag[k] = m;
+  prv[k] = rv[i];
+}
+
+// contiguous results -> the pool slots of the live list
+__global__ void k_wake_scatter(int n, const int *__restrict__ live, const double *__restrict__ pvel,
+                               const double *__restrict__ pstr, double *vel, double *stretch) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  long i = live[k];
+  for (int e = 0; e < 3; ++e) {
+    vel[3 * i + e] = pvel[3 * (long)k + e];
+    if (pstr) stretch[3 * i + e] = pstr[3 * (long)k + e];
+  }
+}
+
 struct NotFree {
   __host__ __device__ bool operator()(const unsigned char &f) const { return f == 0; }
 };
@@ -218,6 +250,31 @@ int dust_wake_shed_dev(int nmax, unsigned char *free_flag, double *cen, double *
   DUST_HIP(hipStreamSynchronize(st));
   DUST_CHECK(g_wws.hcount.p[0] == 0, DUST_E_CAPACITY,
              "Exceeding the maximum number of %d wake particles introduced", nmax);
+  return 0;
+}
+
+int dust_wake_gather_dev(int n, const int *live, const double *cen, const double *dir,
+                         const double *mag, const double *r_vortex, double *pcen, double *pdir,
+                         double *pmag, double *prv, double *palpha, void *stream) {
+  DUST_CHECK(n >= 0, DUST_E_ARG, "negative count");
+  if (n == 0) return 0;
+  DUST_CHECK(live && cen && dir && mag && r_vortex && pcen && pdir && pmag && prv, DUST_E_ARG,
+             "missing arrays");
+  k_wake_gather<<<ceil_div(n, 256), 256, 0, (hipStream_t)stream>>>(n, live, cen, dir, mag,
+                                                                   r_vortex, pcen, pdir, pmag,
+                                                                   prv, palpha);
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
+
+int dust_wake_scatter_dev(int n, const int *live, const double *pvel, const double *pstretch,
+                          double *vel, double *stretch, void *stream) {
+  DUST_CHECK(n >= 0, DUST_E_ARG, "negative count");
+  if (n == 0) return 0;
+  DUST_CHECK(live && pvel && vel && (!pstretch || stretch), DUST_E_ARG, "missing arrays");
+  k_wake_scatter<<<ceil_div(n, 256), 256, 0, (hipStream_t)stream>>>(n, live, pvel, pstretch, vel,
+                                                                    stretch);
+  DUST_HIP(hipGetLastError());
   return 0;
 }
 

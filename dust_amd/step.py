@@ -1,0 +1,195 @@
+"""One time step of a body at rest with a vortex-particle wake, device-resident,
+in the order of DUST's time loop (src/main/dust.f90 via src/aeroel/mod_wake.f90
+and src/linsys/mod_linsys.f90):
+
+  prepare_wake      live list of the particle pool, particle arrays for the
+                    FMM and the element terms (mod_wake.f90:785-849)
+  assemble_linsys   uvort at the panel centres from the particles
+                    (get_vort_vel, mod_surfpan.f90:962-977) and the static
+                    right-hand side b = B nor.(-u_inf - uvort)
+                    (mod_surfpan.f90:400-404)
+  solve_linsys      b -> panel intensities mu with the LU factors of the
+                    static AIC computed once (mod_linsys.f90:236, 473)
+  update_wake       particle velocity and stretching: the FMM pass
+                    (sort_particles, calculate_multipole, apply_multipole)
+                    plus the body panels' velocity (mag = mu, source
+                    sigma = nor.(-u_inf - uvort)) (mod_octree.f90:1350-1373)
+  complete_wake     particle update (Euler, box culling) and insertion of the
+                    particles shed at the trailing edge (mod_wake.f90:1305-1355,
+                    1888-1967)
+
+The shedding geometry of the reference (last wake-panel row) is replaced by a
+rule of the same shape: one particle per trailing-edge segment, half a step
+downstream of its midpoint, partvec = (mu_upper - mu_lower) times the segment
+vector, radius `r_shed`, velocity u_inf.  Every stage runs on the GPU through
+the C ABI; the host reads back the live count and the trailing-edge
+intensities (2 x segments values) once per step.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib as L
+from . import elements as E
+from . import octree as O
+from . import wake as WK
+
+
+def trailing_edge(ver, strips, nc):
+    """Trailing-edge segments of a mesh built as `strips` strips of 2*nc quads
+    (nc upper panels from the trailing edge forward, then nc lower panels
+    back to it, as workloads.wing_case / workloads.rotor lay them out):
+    (upper panel index, lower panel index, segment start, segment end)."""
+    up = np.arange(strips) * 2 * nc
+    lo = up + 2 * nc - 1
+    return up, lo, ver[up, 0].copy(), ver[up, 3].copy()
+
+
+class BodyWakeStep:
+    def __init__(self, ver, n_ver, te, u_inf, dt, nmax, tree, degree=4, form="fp64",
+                 aic=None, r_shed=0.1, box=None, vs_elems=False):
+        import torch
+        self.torch = torch
+        self.lib = L.lib()
+        self.n = len(n_ver)
+        self.u_inf = np.asarray(u_inf, dtype=np.float64)
+        self.dt = float(dt)
+        self.form = form
+        self.vs_elems = bool(vs_elems)
+        self.aic = dict(ff_ratio_doublet=10.0, rankine_rad=0.1, cutoff_rad=0.001)
+        self.aic.update(aic or {})
+        self.eprm = E.params(vortex_rad=r_shed, **self.aic)
+        dv = lambda a, dt=np.float64: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).cuda()
+        v4 = np.zeros((self.n, 4, 3))
+        v4[:, : np.asarray(ver).shape[1]] = np.asarray(ver)[:, :4]
+        self.ver = dv(v4)
+        self.nv = dv(n_ver, np.int32)
+        self.cen = dv(np.array([v4[i, :n_ver[i]].mean(axis=0) for i in range(self.n)]))
+        n = self.n
+        # static AIC and its LU factors (initialize_linsys, mod_linsys.f90:236)
+        self.A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+        self.B = torch.empty((n, n), dtype=torch.float64, device="cuda")
+        ap = L.AicParams()
+        ap.ff_ratio_doublet = self.aic["ff_ratio_doublet"]
+        ap.rankine_rad = self.aic["rankine_rad"]
+        ap.cutoff_rad = self.aic["cutoff_rad"]
+        L.check(self.lib.dust_aic_surfpan_dev(n, self.nv.data_ptr(), self.ver.data_ptr(),
+                                              C.byref(ap), self.A.data_ptr(), n,
+                                              self.B.data_ptr(), n, None), "aic")
+        self.LU = self.A.clone()
+        self.ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+        info = self.lib.dust_dgetrf_dev(n, n, self.LU.data_ptr(), n, self.ipiv.data_ptr(), None)
+        if info != 0:
+            raise RuntimeError(f"static AIC is singular (dgetrf info {info})")
+        self.b = torch.zeros(n, dtype=torch.float64, device="cuda")
+        self.sigma = torch.zeros(n, dtype=torch.float64, device="cuda")
+        self.uvort = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+        # wake
+        self.te_up, self.te_lo, te0, te1 = te
+        self.te_mid = 0.5 * (te0 + te1) + 0.5 * self.dt * self.u_inf
+        self.te_vec = te1 - te0
+        self.r_shed = float(r_shed)
+        self.pool = WK.ParticlePool(nmax)
+        if box is None:
+            lo = np.asarray(tree["origin"], dtype=np.float64)
+            box = (lo + 1e-9, lo + np.asarray(tree["nbox"]) * tree["box_length"] - 1e-9)
+        self.wprm = WK.params(box[0], box[1], dt=self.dt, use_vs=True)
+        self.fmm = O.initialize_octree(tree["box_length"], tree["nbox"], tree["origin"],
+                                       tree["nlevels"], tree["min_part"], degree, tree["delta"],
+                                       capacity=nmax, use_vs=True, u_inf=tuple(self.u_inf),
+                                       precision=form)
+        d3 = dict(dtype=torch.float64, device="cuda")
+        self.p = dict(cen=torch.zeros((nmax, 3), **d3), dir=torch.zeros((nmax, 3), **d3),
+                      mag=torch.zeros(nmax, **d3), rv=torch.zeros(nmax, **d3),
+                      alpha=torch.zeros((nmax, 3), **d3), vel=torch.zeros((nmax, 3), **d3),
+                      stretch=torch.zeros((nmax, 3), **d3))
+        self.mu = self.b  # solve_linsys writes res in place
+        self.times = {}
+
+    def seed(self, cen, alpha, rv):
+        """Initial particles (inserted like shed ones)."""
+        self.pool.shed(cen, alpha, rv, np.tile(self.u_inf, (len(rv), 1)), self.wprm)
+
+    def _shed(self):
+        mu = self.b.cpu().numpy()
+        pv = (mu[self.te_up] - mu[self.te_lo])[:, None] * self.te_vec
+        k = len(pv)
+        self.pool.shed(self.te_mid, pv, np.full(k, self.r_shed), np.tile(self.u_inf, (k, 1)),
+                       self.wprm)
+
+    def step(self):
+        torch, lib, p = self.torch, self.lib, self.p
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(7)]
+        ev[0].record()
+        # prepare_wake: live list and the pointer arrays
+        live = self.pool.live()
+        npart = live.numel()
+        P = self.pool
+        L.check(lib.dust_wake_gather_dev(npart, live.data_ptr(), P.cen.data_ptr(),
+                                         P.dir.data_ptr(), P.mag.data_ptr(),
+                                         P.r_vortex.data_ptr(), p["cen"].data_ptr(),
+                                         p["dir"].data_ptr(), p["mag"].data_ptr(),
+                                         p["rv"].data_ptr(), p["alpha"].data_ptr(), None),
+                "gather")
+        ev[1].record()
+        # assemble_linsys: uvort, right-hand side and source strengths
+        L.check(lib.dust_vort_vel_dev(self.n, self.cen.data_ptr(), npart, p["cen"].data_ptr(),
+                                      p["dir"].data_ptr(), p["mag"].data_ptr(),
+                                      p["rv"].data_ptr(), 0, None, None, C.byref(self.eprm),
+                                      self.uvort.data_ptr(), None), "uvort")
+        u = np.ascontiguousarray(self.u_inf)
+        L.check(lib.dust_aic_rhs_dev(self.n, self.nv.data_ptr(), self.ver.data_ptr(),
+                                     self.B.data_ptr(), self.n, 1, u.ctypes.data,
+                                     self.uvort.data_ptr(), self.b.data_ptr(), self.n,
+                                     self.sigma.data_ptr(), None), "rhs")
+        ev[2].record()
+        # solve_linsys: mu = A^-1 b with the static factors
+        L.check(lib.dust_dgetrs_dev(self.n, 1, self.LU.data_ptr(), self.n, self.ipiv.data_ptr(),
+                                    self.b.data_ptr(), self.n, None), "getrs")
+        ev[3].record()
+        # update_wake: FMM pass over the live particles
+        h = self.fmm.handle
+        torch.cuda.synchronize()
+        if npart:
+            L.check(lib.dust_fmm_set_particles_dev(h, npart, p["cen"].data_ptr(),
+                                                   p["dir"].data_ptr(), p["mag"].data_ptr(),
+                                                   p["rv"].data_ptr()), "fmm set")
+            nl = C.c_int()
+            L.check(lib.dust_fmm_pass_dev(h, C.byref(nl)), "fmm pass")
+            L.check(lib.dust_fmm_results_dev(h, p["vel"].data_ptr(), p["stretch"].data_ptr(),
+                                             None), "fmm results")
+            L.check(lib.dust_fmm_synchronize(h), "fmm sync")
+        ev[4].record()
+        # ... plus the body panels (elem list of apply_multipole)
+        grp = (L.ElemGroup * 1)()
+        grp[0].kind, grp[0].n = L.ELEM_SURFPAN, self.n
+        grp[0].n_ver, grp[0].ver = self.nv.data_ptr(), self.ver.data_ptr()
+        grp[0].mag, grp[0].sigma, grp[0].stretch = self.b.data_ptr(), self.sigma.data_ptr(), 1
+        L.check(lib.dust_elem_apply_dev(1, grp, C.byref(self.eprm), npart, p["cen"].data_ptr(),
+                                        p["alpha"].data_ptr() if self.vs_elems else None,
+                                        p["vel"].data_ptr(),
+                                        p["stretch"].data_ptr() if self.vs_elems else None,
+                                        None), "elements")
+        L.check(lib.dust_wake_scatter_dev(npart, live.data_ptr(), p["vel"].data_ptr(),
+                                          p["stretch"].data_ptr(), P.vel.data_ptr(),
+                                          P.stretch.data_ptr(), None), "scatter")
+        ev[5].record()
+        # complete_wake: move, cull, shed
+        P.update(self.wprm)
+        self._shed()
+        ev[6].record()
+        torch.cuda.synchronize()
+        names = ("prepare", "assemble", "solve", "fmm", "elements", "complete")
+        for i, k in enumerate(names):
+            self.times[k] = self.times.get(k, 0.0) + ev[i].elapsed_time(ev[i + 1])
+        return npart
+
+    def state(self):
+        """Host copies: panel intensities, uvort, live slots and the pool."""
+        live = self.pool.live().cpu().numpy()
+        P = self.pool
+        return dict(mu=self.b.cpu().numpy(), uvort=self.uvort.cpu().numpy(), live=live,
+                    cen=P.cen.cpu().numpy(), dir=P.dir.cpu().numpy(), mag=P.mag.cpu().numpy(),
+                    vel=P.vel.cpu().numpy(), stretch=P.stretch.cpu().numpy(),
+                    free=P.free.cpu().numpy())

@@ -178,14 +178,21 @@ def wing_case(nc=40, ns=20, aspect=8.0, alpha_deg=5.0, u=30.0, nrows=200, seed=7
     return ver, nv, cen, alpha, rv, tree
 
 
+def rotor_strips(blades, panels, aspect):
+    """(nc, ns) of rotor(): chordwise panels per surface and spanwise strips
+    per blade.  The mesh is blades x ns strips of 2*nc quads."""
+    per = max(8, panels // blades)
+    ns = max(2, int(round(np.sqrt(per * aspect / 8.0))))
+    nc = max(2, int(round(per / (2 * ns))))
+    return nc, ns
+
+
 def rotor(blades=3, panels=1000, aspect=10.0, root=0.2, pitch_deg=8.0):
     """`blades` rectangular NACA0012 blades (chord 1, span = aspect), root at
     radius `root`, pitched by pitch_deg, evenly spaced around the z axis;
     about `panels` quads in total (upper and lower surfaces, cosine chordwise
     spacing).  Returns (ver (n,4,3), n_ver)."""
-    per = max(8, panels // blades)
-    ns = max(2, int(round(np.sqrt(per * aspect / 8.0))))
-    nc = max(2, int(round(per / (2 * ns))))
+    nc, ns = rotor_strips(blades, panels, aspect)
     xc = 0.5 * (1.0 - np.cos(np.linspace(0.0, np.pi, nc + 1))) - 0.25
     rs = np.linspace(root, root + aspect, ns + 1)
     zt = naca0012(xc + 0.25)

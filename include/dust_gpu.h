@@ -281,6 +281,17 @@ int dust_wake_shed_dev(int nmax, unsigned char *free_flag, double *cen, double *
                        const double *new_partvec, const double *new_rv, const double *new_vel,
                        const dust_wake_params *prm, int *n_prt, void *stream);
 
+/* The pointer arrays the FMM and the element loops read (the reference
+ * passes part_p(:)%p): pcen(3,n), pdir(3,n), pmag(n), prv(n) and optional
+ * palpha(3,n) = dir*mag of the live slots live(0..n-1), and the way back of
+ * the velocities and stretching (apply_multipole writes them through the
+ * same pointers, mod_octree.f90:1377-1397).  Device pointers. */
+int dust_wake_gather_dev(int n, const int *live, const double *cen, const double *dir,
+                         const double *mag, const double *r_vortex, double *pcen, double *pdir,
+                         double *pmag, double *prv, double *palpha, void *stream);
+int dust_wake_scatter_dev(int n, const int *live, const double *pvel, const double *pstretch,
+                          double *vel, double *stretch, void *stream);
+
 /* ======================================================================== */
 /* AIC assembly: src/aeroel/mod_surfpan.f90, mod_vortlatt.f90, mod_doublet   */
 /* ======================================================================== */
@@ -317,6 +328,13 @@ int dust_aic_vortlatt(int n, const int *n_ver, const double *ver, const dust_aic
  * b is (ldb_out, nrhs).  Host pointers. */
 int dust_aic_rhs(int n, const int *n_ver, const double *ver, const double *B, int ldb,
                  int nrhs, const double *u_inf, const double *uvort, double *b, int ldbo);
+/* Same with device pointers (n_ver, ver, B, uvort, b; u_inf(3,nrhs) is host
+ * memory), ordered on `stream`.  If sigma (device, (n,nrhs)) is given it
+ * receives the coefficients nor_j.(-u_inf - uvort_j), which are also the
+ * source strengths of compute_vel_surfpan for a body at rest. */
+int dust_aic_rhs_dev(int n, const int *n_ver, const double *ver, const double *B, int ldb,
+                     int nrhs, const double *u_inf, const double *uvort, double *b, int ldbo,
+                     double *sigma, void *stream);
 
 /* ======================================================================== */
 /* LU: the LAPACK call sites of src/linsys/mod_linsys.f90 (236, 435, 473)    */

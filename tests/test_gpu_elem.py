@@ -181,14 +181,15 @@ def test_empty_inputs():
     assert np.all(E.get_vort_vel(np.ones((4, 3))) == 0.0)
 
 
-def test_morton_order_path_is_bitwise_input_order(wing):
+def test_morton_order_path_matches_input_order(wing):
     """Targets above the sorting threshold are evaluated in Morton order and
-    scattered back; each target's sum runs over the elements in list order
-    either way, so the two paths agree bit for bit."""
+    scattered back; the two paths sum each target's elements in list order
+    (split into element ranges when there are few targets, a split that
+    depends on the target count), so they agree to rounding."""
     groups, pts, al = wing
     big = np.concatenate([pts, pts[:3000] + 0.37])          # sorted path
     print(f"elem_apply sorted n={len(big)} vs input order n=4000", flush=True)
     vb, sb = E.apply_elements(groups, big, alpha=np.concatenate([al, al[:3000]]))
     vs, ss = E.apply_elements(groups, big[:4000], alpha=al[:4000])   # below the threshold
-    assert np.array_equal(vb[:4000], vs)
-    assert np.array_equal(sb[:4000], ss)
+    assert np.allclose(vb[:4000], vs, rtol=1e-13, atol=1e-13 * np.abs(vs).max())
+    assert np.allclose(sb[:4000], ss, rtol=1e-13, atol=1e-13 * np.abs(ss).max())
