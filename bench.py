@@ -470,6 +470,48 @@ def time_wake(torch, L, nmax, steps, with_cpu=True):
     return out
 
 
+def time_wing_run(torch, W, nsteps=200, with_cpu=True):
+    """The wing configuration as a time-marching run on the device
+    (dust_amd.step.BodyWakeStep): 1,600 surface panels, static AIC factored
+    once, u_inf = 30 m/s, dt = chord/(20 U), two particles shed per
+    trailing-edge segment per step (40 per step), nsteps steps from an empty
+    wake; FP64 and tuned FMM forms.  With with_cpu: the CPU oracle pipeline of
+    the same step timed over 3 steps of the seeded 8,000-particle sheet."""
+    from dust_amd.step import BodyWakeStep, trailing_edge
+    ver, nv, cen, alpha, rv, tree = W.wing_case()
+    te = trailing_edge(ver, 20, 40)
+    u, dt = (30.0, 0.0, 0.0), 1.0 / 600.0
+    out = dict(panels=len(nv), steps=nsteps)
+    mus = {}
+    for form in ("fp64", "tuned"):
+        s = BodyWakeStep(ver, nv, te, u, dt, 40 * nsteps + 64, tree, form=form, sub=2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            s.step()
+        torch.cuda.synchronize()
+        tot = time.perf_counter() - t0
+        st = s.state()
+        mus[form] = st["mu"]
+        out[form] = dict(ms_total=tot * 1e3, ms_per_step=tot * 1e3 / nsteps,
+                         particles_end=int(np.count_nonzero(st["free"] == 0)),
+                         stages_ms_per_step={k: v / nsteps for k, v in s.times.items()})
+    out["tuned_mu_rel_l2_vs_fp64"] = float(np.linalg.norm(mus["tuned"] - mus["fp64"]) /
+                                           np.linalg.norm(mus["fp64"]))
+    if with_cpu:
+        from oracle.pipeline import OracleStep
+        from oracle import oracle as orc
+        o = OracleStep((ver, nv, cen, alpha, rv, tree), te, len(rv) + 400, u, dt, sub=2)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            o.step()
+        dtc = (time.perf_counter() - t0) / 3
+        out["cpu_baseline"] = dict(ms_per_step=dtc * 1e3, cores=orc.num_threads(), kind="port",
+                                   sample=f"oracle pipeline of the same step, 3 steps of the "
+                                          f"seeded {len(rv)}-particle sheet")
+    return out
+
+
 def cpu_baseline(W, n_sample, degree):
     """The CPU oracle (FP64 restatement, OpenMP) on a bounded sample of the same
     generator; particle updates per second."""
@@ -650,7 +692,8 @@ def main():
     n = args.n
     if args.elements_only:
         out = dict(elements=time_elements(torch, L, W, n, args.steps, with_cpu=not args.no_cpu),
-                   wake=time_wake(torch, L, 5_000_000, args.steps, with_cpu=not args.no_cpu))
+                   wake=time_wake(torch, L, 5_000_000, args.steps, with_cpu=not args.no_cpu),
+                   wing_run=time_wing_run(torch, W, 200, with_cpu=not args.no_cpu))
         print(json.dumps(out, default=float))
         return
     log(f"generating helix wake n={n}")
@@ -706,6 +749,11 @@ def main():
         log("timing the wing configuration")
         wing = time_wing(torch, L, O, W, 3)
         log(f"wing: {wing}")
+    wing_run = None
+    if not args.no_wing:
+        log("timing the wing time-marching run")
+        wing_run = time_wing_run(torch, W, 200, with_cpu=not args.no_cpu)
+        log(f"wing run: {wing_run}")
     elements = None
     if not args.no_elements:
         log("timing the element <-> particle terms")
@@ -763,6 +811,7 @@ def main():
         "roofline": roof,
         "lu": lu,
         "wing": wing,
+        "wing_run": wing_run,
         "elements": elements,
         "batch": batch,
         "cpu_baseline": cpu,

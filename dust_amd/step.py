@@ -19,9 +19,10 @@ and src/linsys/mod_linsys.f90):
                     1888-1967)
 
 The shedding geometry of the reference (last wake-panel row) is replaced by a
-rule of the same shape: one particle per trailing-edge segment, half a step
-downstream of its midpoint, partvec = (mu_upper - mu_lower) times the segment
-vector, radius `r_shed`, velocity u_inf.  Every stage runs on the GPU through
+rule of the same shape: `sub` particles per trailing-edge segment, half a
+step downstream of the midpoints of its `sub` equal pieces, partvec =
+(mu_upper - mu_lower) times the piece's vector, radius `r_shed`, velocity
+u_inf.  Every stage runs on the GPU through
 the C ABI; the host reads back the live count and the trailing-edge
 intensities (2 x segments values) once per step.
 """
@@ -48,7 +49,7 @@ def trailing_edge(ver, strips, nc):
 
 class BodyWakeStep:
     def __init__(self, ver, n_ver, te, u_inf, dt, nmax, tree, degree=4, form="fp64",
-                 aic=None, r_shed=0.1, box=None, vs_elems=False):
+                 aic=None, r_shed=0.1, box=None, vs_elems=False, sub=1):
         import torch
         self.torch = torch
         self.lib = L.lib()
@@ -86,9 +87,12 @@ class BodyWakeStep:
         self.sigma = torch.zeros(n, dtype=torch.float64, device="cuda")
         self.uvort = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
         # wake
-        self.te_up, self.te_lo, te0, te1 = te
-        self.te_mid = 0.5 * (te0 + te1) + 0.5 * self.dt * self.u_inf
-        self.te_vec = te1 - te0
+        up, lo, te0, te1 = te
+        q = (np.arange(sub) + 0.5) / sub
+        self.te_up, self.te_lo = np.repeat(up, sub), np.repeat(lo, sub)
+        self.te_vec = np.repeat((te1 - te0) / sub, sub, axis=0)
+        self.te_mid = (np.repeat(te0, sub, axis=0) + np.tile(q, len(up))[:, None] *
+                       np.repeat(te1 - te0, sub, axis=0) + 0.5 * self.dt * self.u_inf)
         self.r_shed = float(r_shed)
         self.pool = WK.ParticlePool(nmax)
         if box is None:
