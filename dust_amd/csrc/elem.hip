@@ -60,7 +60,7 @@ __global__ void k_elem_recs(int n, int kind, const int *__restrict__ nver,
       r.g.cen[e] = 0.5 * (r.g.ver[0][e] + r.g.ver[1][e]);
       r.g.evec[0][e] = r.g.ver[1][e] - r.g.ver[0][e];
     }
-    double el = sqrt(dot3(r.g.evec[0], r.g.evec[0]));
+    double el = sqrt(dot3_rn(r.g.evec[0], r.g.evec[0]));
     r.g.elen[0] = el;
     r.g.maxedge = el;
     for (int e = 0; e < 3; ++e) r.g.euni[0][e] = r.g.evec[0][e] / el;
@@ -105,7 +105,7 @@ __device__ inline void line_vel(const double *x1, const double *x2, const double
 __device__ inline void vel_source(const Panel &g, const double pos[3], double ffr_dou,
                                   double v[3]) {
   double rv[3] = {pos[0] - g.cen[0], pos[1] - g.cen[1], pos[2] - g.cen[2]};
-  double radius = sqrt(dot3(rv, rv));
+  double radius = sqrt(dot3_rn(rv, rv));
   double phix, phiy, pdou;
   if (radius > 10.0 * g.maxedge) {
     double r3 = radius * radius * radius;
@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(256)
     } else {
       const Panel &g = E.g;
       double rv[3] = {x[0] - g.cen[0], x[1] - g.cen[1], x[2] - g.cen[2]};
-      double r2 = dot3(rv, rv);
+      double r2 = dot3_rn(rv, rv);
       double radius = sqrt(r2);
       const bool far_d = radius > ffr * g.maxedge;       // velocity_calc_doublet switch
       const bool far_s = radius > 10.0 * g.maxedge;      // velocity_calc_sou_surfpan switch
@@ -481,7 +481,7 @@ __global__ void k_vort_finish(int m, int S, const double *__restrict__ part,
     for (int j = 0; j < nl; ++j) {
       const double *v1 = lver + 6 * j;
       double ev[3] = {v1[3] - v1[0], v1[4] - v1[1], v1[5] - v1[2]};
-      double el = sqrt(dot3(ev, ev));
+      double el = sqrt(dot3_rn(ev, ev));
       double eu[3] = {ev[0] / el, ev[1] / el, ev[2] / el};
       double w[3];
       line_vel(v1, v1 + 3, eu, el, x, rran, rcut, w);

@@ -26,6 +26,15 @@ struct Panel {
 __device__ inline double dot3(const double *a, const double *b) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
+// The same sum rounded term by term, as plain C on the host evaluates it (no
+// fused multiply-add).  Used for the lengths that decide a branch (far-field
+// switches, edge lengths, core radii): on regular meshes many distances sit
+// exactly on a switch (a centre 10 edge lengths away), and a fused rounding
+// would pick the other formula there.
+__device__ inline double dot3_rn(const double *a, const double *b) {
+#pragma clang fp contract(off)
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
 __device__ inline void crs3(const double *a, const double *b, double *r) {
   r[0] = a[1] * b[2] - a[2] * b[1];
   r[1] = a[2] * b[0] - a[0] * b[2];
@@ -66,7 +75,7 @@ __device__ inline void panel_geo(int nv, const double *__restrict__ vp, Panel &g
   for (int i = 0; i < 4; ++i) {
     int nx = (i + 1 < nv) ? i + 1 : 0;
     for (int e = 0; e < 3; ++e) g.evec[i][e] = (i < nv) ? g.ver[nx][e] - g.ver[i][e] : 0.0;
-    g.elen[i] = (i < nv) ? sqrt(dot3(g.evec[i], g.evec[i])) : 0.0;
+    g.elen[i] = (i < nv) ? sqrt(dot3_rn(g.evec[i], g.evec[i])) : 0.0;
     for (int e = 0; e < 3; ++e) g.euni[i][e] = (i < nv) ? g.evec[i][e] / g.elen[i] : 0.0;
     if (i < nv && g.elen[i] > g.maxedge) g.maxedge = g.elen[i];
     double d[3];
@@ -79,7 +88,7 @@ __device__ inline void panel_geo(int nv, const double *__restrict__ vp, Panel &g
 // potential_calc_doublet: solid angle of panel g seen from pos.
 __device__ inline double pot_doublet(const Panel &g, const double pos[3], double ffr) {
   double rv[3] = {pos[0] - g.cen[0], pos[1] - g.cen[1], pos[2] - g.cen[2]};
-  double radius = sqrt(dot3(rv, rv));
+  double radius = sqrt(dot3_rn(rv, rv));
   double zQ = dot3(rv, g.nor);
   if (radius > ffr * g.maxedge) return zQ * g.area / (radius * radius * radius);
   double dou = 0.0;
@@ -113,7 +122,7 @@ __device__ inline double pot_doublet(const Panel &g, const double pos[3], double
 // potential_calc_sou_surfpan (the routine's local far-field ratio is 10)
 __device__ inline double pot_source(const Panel &g, double dou, const double pos[3], int *warn) {
   double rv[3] = {pos[0] - g.cen[0], pos[1] - g.cen[1], pos[2] - g.cen[2]};
-  double radius = sqrt(dot3(rv, rv));
+  double radius = sqrt(dot3_rn(rv, rv));
   if (radius > 10.0 * g.maxedge) return g.area / radius;
   double zQ = dot3(rv, g.nor);
   double Qp[3] = {pos[0] - zQ * g.nor[0], pos[1] - zQ * g.nor[1], pos[2] - zQ * g.nor[2]};
@@ -152,7 +161,7 @@ __device__ inline double pot_source(const Panel &g, double dou, const double pos
 __device__ inline void vel_doublet(const Panel &g, const double pos[3], double ffr, double rran,
                                    double rcut, double v[3]) {
   double rv[3] = {pos[0] - g.cen[0], pos[1] - g.cen[1], pos[2] - g.cen[2]};
-  double radius = sqrt(dot3(rv, rv));
+  double radius = sqrt(dot3_rn(rv, rv));
   v[0] = v[1] = v[2] = 0.0;
   if (radius > ffr * g.maxedge) {
     double r2 = radius * radius, r5 = r2 * r2 * radius, rn = dot3(rv, g.nor);

@@ -25,6 +25,13 @@
 namespace dust {
 namespace {
 
+// x + v dt with the product rounded before the sum, as plain C on the host
+// evaluates it: the box test that frees a slot then decides alike.
+__device__ inline double step_rn(double x, double v, double dt) {
+#pragma clang fp contract(off)
+  return x + v * dt;
+}
+
 __global__ void __launch_bounds__(256)
     k_wake_update(int nmax, unsigned char *freef, double *cen, double *dir,
                   const double *__restrict__ mag, const double *__restrict__ vel,
@@ -37,7 +44,7 @@ __global__ void __launch_bounds__(256)
     double pos[3];
     bool inside = true;
     for (int e = 0; e < 3; ++e) {
-      pos[e] = cen[3 * (long)i + e] + vel[3 * (long)i + e] * dt;
+      pos[e] = step_rn(cen[3 * (long)i + e], vel[3 * (long)i + e], dt);
       inside = inside && pos[e] >= P.box_min[e] && pos[e] <= P.box_max[e];
     }
     if (inside) {

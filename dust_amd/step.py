@@ -1,4 +1,5 @@
-"""One time step of a body at rest with a vortex-particle wake, device-resident,
+"""One time step of a body (at rest, or rotating about z) with a vortex-particle
+wake, device-resident,
 in the order of DUST's time loop (src/main/dust.f90 via src/aeroel/mod_wake.f90
 and src/linsys/mod_linsys.f90):
 
@@ -22,7 +23,11 @@ The shedding geometry of the reference (last wake-panel row) is replaced by a
 rule of the same shape: `sub` particles per trailing-edge segment, half a
 step downstream of the midpoints of its `sub` equal pieces, partvec =
 (mu_upper - mu_lower) times the piece's vector, radius `r_shed`, velocity
-u_inf.  Every stage runs on the GPU through
+u_inf.  A rotating body (omega != 0) turns by omega dt at the start of each
+step (geometry update of the caller), its AIC is assembled again and solved
+with a fresh factorisation (FP64 getrf/getrs, or the mixed solve with
+solver="fp32" / "fp16s"), and the panels' velocity omega x r enters the
+source strengths.  Every stage runs on the GPU through
 the C ABI; the host reads back the live count and the trailing-edge
 intensities (2 x segments values) once per step.
 """
@@ -35,6 +40,7 @@ from . import _lib as L
 from . import elements as E
 from . import octree as O
 from . import wake as WK
+from . import workloads as WL
 
 
 def trailing_edge(ver, strips, nc):
@@ -49,7 +55,8 @@ def trailing_edge(ver, strips, nc):
 
 class BodyWakeStep:
     def __init__(self, ver, n_ver, te, u_inf, dt, nmax, tree, degree=4, form="fp64",
-                 aic=None, r_shed=0.1, box=None, vs_elems=False, sub=1):
+                 aic=None, r_shed=0.1, box=None, vs_elems=False, sub=1, omega=0.0,
+                 solver="fp64", strips=None):
         import torch
         self.torch = torch
         self.lib = L.lib()
@@ -64,9 +71,15 @@ class BodyWakeStep:
         dv = lambda a, dt=np.float64: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).cuda()
         v4 = np.zeros((self.n, 4, 3))
         v4[:, : np.asarray(ver).shape[1]] = np.asarray(ver)[:, :4]
+        self.ver0 = v4
+        self.nv_h = np.asarray(n_ver, dtype=np.int32)
         self.ver = dv(v4)
         self.nv = dv(n_ver, np.int32)
-        self.cen = dv(np.array([v4[i, :n_ver[i]].mean(axis=0) for i in range(self.n)]))
+        self.cen = dv(self._centres(v4))
+        self.omega = float(omega)
+        self.theta = 0.0
+        self.solver = solver
+        self.strips = strips  # (strips, nc) of the trailing-edge layout, moving bodies
         n = self.n
         # static AIC and its LU factors (initialize_linsys, mod_linsys.f90:236)
         self.A = torch.empty((n, n), dtype=torch.float64, device="cuda")
@@ -86,13 +99,11 @@ class BodyWakeStep:
         self.b = torch.zeros(n, dtype=torch.float64, device="cuda")
         self.sigma = torch.zeros(n, dtype=torch.float64, device="cuda")
         self.uvort = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+        self.ub = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+        self.uvort_rel = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
         # wake
-        up, lo, te0, te1 = te
-        q = (np.arange(sub) + 0.5) / sub
-        self.te_up, self.te_lo = np.repeat(up, sub), np.repeat(lo, sub)
-        self.te_vec = np.repeat((te1 - te0) / sub, sub, axis=0)
-        self.te_mid = (np.repeat(te0, sub, axis=0) + np.tile(q, len(up))[:, None] *
-                       np.repeat(te1 - te0, sub, axis=0) + 0.5 * self.dt * self.u_inf)
+        self.sub = sub
+        self._set_te(te)
         self.r_shed = float(r_shed)
         self.pool = WK.ParticlePool(nmax)
         if box is None:
@@ -110,6 +121,33 @@ class BodyWakeStep:
                       stretch=torch.zeros((nmax, 3), **d3))
         self.mu = self.b  # solve_linsys writes res in place
         self.times = {}
+
+    def _centres(self, v4):
+        return np.array([v4[i, :self.nv_h[i]].mean(axis=0) for i in range(self.n)])
+
+    def _set_te(self, te):
+        up, lo, te0, te1 = te
+        sub = self.sub
+        q = (np.arange(sub) + 0.5) / sub
+        self.te_up, self.te_lo = np.repeat(up, sub), np.repeat(lo, sub)
+        self.te_vec = np.repeat((te1 - te0) / sub, sub, axis=0)
+        self.te_mid = (np.repeat(te0, sub, axis=0) + np.tile(q, len(up))[:, None] *
+                       np.repeat(te1 - te0, sub, axis=0) + 0.5 * self.dt * self.u_inf)
+
+    def _move(self):
+        """Rotating body: the geometry of the new step (rotation about z by
+        omega dt; the caller's geometry update, outside the library) and the
+        body velocity omega x r at the panel centres."""
+        torch = self.torch
+        self.theta += self.omega * self.dt
+        v = WL.rotate_z(self.ver0, self.theta)
+        cen = self._centres(v)
+        self.ver.copy_(torch.from_numpy(v))
+        self.cen.copy_(torch.from_numpy(cen))
+        ub = np.stack([-self.omega * cen[:, 1], self.omega * cen[:, 0], np.zeros(self.n)], axis=1)
+        self.ub.copy_(torch.from_numpy(ub))
+        ns, nc = self.strips
+        self._set_te(trailing_edge(v, ns, nc))
 
     def seed(self, cen, alpha, rv):
         """Initial particles (inserted like shed ones)."""
@@ -137,20 +175,53 @@ class BodyWakeStep:
                                          p["rv"].data_ptr(), p["alpha"].data_ptr(), None),
                 "gather")
         ev[1].record()
+        moving = self.omega != 0.0
+        if moving:  # moving body: geometry of the step, AIC re-assembled
+            self._move()
+            ap = L.AicParams()
+            ap.ff_ratio_doublet = self.aic["ff_ratio_doublet"]
+            ap.rankine_rad = self.aic["rankine_rad"]
+            ap.cutoff_rad = self.aic["cutoff_rad"]
+            L.check(lib.dust_aic_surfpan_dev(self.n, self.nv.data_ptr(), self.ver.data_ptr(),
+                                             C.byref(ap), self.A.data_ptr(), self.n,
+                                             self.B.data_ptr(), self.n, None), "aic")
         # assemble_linsys: uvort, right-hand side and source strengths
         L.check(lib.dust_vort_vel_dev(self.n, self.cen.data_ptr(), npart, p["cen"].data_ptr(),
                                       p["dir"].data_ptr(), p["mag"].data_ptr(),
                                       p["rv"].data_ptr(), 0, None, None, C.byref(self.eprm),
                                       self.uvort.data_ptr(), None), "uvort")
         u = np.ascontiguousarray(self.u_inf)
+        # sigma = nor.(ub - u_inf - uvort): the moving panels' velocity enters
+        # as uvort - ub
+        urel = self.uvort
+        if moving:
+            torch.sub(self.uvort, self.ub, out=self.uvort_rel)
+            urel = self.uvort_rel
         L.check(lib.dust_aic_rhs_dev(self.n, self.nv.data_ptr(), self.ver.data_ptr(),
                                      self.B.data_ptr(), self.n, 1, u.ctypes.data,
-                                     self.uvort.data_ptr(), self.b.data_ptr(), self.n,
+                                     urel.data_ptr(), self.b.data_ptr(), self.n,
                                      self.sigma.data_ptr(), None), "rhs")
         ev[2].record()
-        # solve_linsys: mu = A^-1 b with the static factors
-        L.check(lib.dust_dgetrs_dev(self.n, 1, self.LU.data_ptr(), self.n, self.ipiv.data_ptr(),
-                                    self.b.data_ptr(), self.n, None), "getrs")
+        # solve_linsys: the static factors, or (moving) a new factorisation
+        if not moving:
+            L.check(lib.dust_dgetrs_dev(self.n, 1, self.LU.data_ptr(), self.n,
+                                        self.ipiv.data_ptr(), self.b.data_ptr(), self.n, None),
+                    "getrs")
+        elif self.solver == "fp64":
+            self.LU.copy_(self.A)
+            info = lib.dust_dgetrf_dev(self.n, self.n, self.LU.data_ptr(), self.n,
+                                       self.ipiv.data_ptr(), None)
+            if info != 0:
+                raise RuntimeError(f"AIC is singular (dgetrf info {info})")
+            L.check(lib.dust_dgetrs_dev(self.n, 1, self.LU.data_ptr(), self.n,
+                                        self.ipiv.data_ptr(), self.b.data_ptr(), self.n, None),
+                    "getrs")
+        else:
+            it, res = C.c_int(), C.c_double()
+            prec = {"fp32": L.LU_FP32, "fp16s": L.LU_FP16S}[self.solver]
+            L.check(lib.dust_dgesv_mixed_dev(self.n, 1, self.A.data_ptr(), self.n,
+                                             self.b.data_ptr(), self.n, prec, 1e-12, 30,
+                                             C.byref(it), C.byref(res), None), "mixed solve")
         ev[3].record()
         # update_wake: FMM pass over the live particles
         h = self.fmm.handle
@@ -185,6 +256,7 @@ class BodyWakeStep:
         ev[6].record()
         torch.cuda.synchronize()
         names = ("prepare", "assemble", "solve", "fmm", "elements", "complete")
+        # (moving bodies: "assemble" includes the AIC re-assembly)
         for i, k in enumerate(names):
             self.times[k] = self.times.get(k, 0.0) + ev[i].elapsed_time(ev[i + 1])
         return npart

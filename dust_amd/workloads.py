@@ -289,3 +289,13 @@ def wing_elements(nc=40, ns=20, aspect=8.0, alpha_deg=5.0, u=30.0):
     ys = np.linspace(-aspect / 2, aspect / 2, ns + 1)
     te = np.array([[ca, y, -sa] for y in ys])
     return element_lists(ver, nv, te, u_inf=(u, 0.0, 0.0))
+
+
+def rotate_z(x, theta):
+    """Points x (..., 3) rotated by theta about the z axis through the origin."""
+    c, s_ = np.cos(theta), np.sin(theta)
+    x = np.asarray(x, dtype=np.float64)
+    out = x.copy()
+    out[..., 0] = c * x[..., 0] - s_ * x[..., 1]
+    out[..., 1] = s_ * x[..., 0] + c * x[..., 1]
+    return out

@@ -13,8 +13,12 @@ from . import oracle as orc
 class OracleStep:
     """The same step with the CPU oracle, on host copies of the pool."""
 
-    def __init__(self, case, te, nmax, u_inf, dt, sub=1, r_shed=0.1):
+    def __init__(self, case, te, nmax, u_inf, dt, sub=1, r_shed=0.1, omega=0.0, strips=None):
         ver, nv, cen, alpha, rv, tree = case
+        self.omega, self.theta, self.strips, self.sub = float(omega), 0.0, strips, sub
+        v4 = np.zeros((len(nv), 4, 3))
+        v4[:, : np.asarray(ver).shape[1]] = np.asarray(ver)[:, :4]
+        self.ver0 = v4
         self.orc = orc
         self.r_shed = r_shed
         self.G = orc.geo(ver, nv)
@@ -29,12 +33,7 @@ class OracleStep:
         self.tree = tree
         lo = np.asarray(tree["origin"], float)
         self.box = (lo + 1e-9, lo + np.asarray(tree["nbox"]) * tree["box_length"] - 1e-9)
-        up, lo_i, te0, te1 = te
-        q = (np.arange(sub) + 0.5) / sub
-        self.up, self.lo = np.repeat(up, sub), np.repeat(lo_i, sub)
-        self.te_vec = np.repeat((te1 - te0) / sub, sub, axis=0)
-        self.te_mid = (np.repeat(te0, sub, axis=0) + np.tile(q, len(up))[:, None] *
-                       np.repeat(te1 - te0, sub, axis=0) + 0.5 * dt * self.u)
+        self._set_te(te)
         self.pool = dict(free=np.ones(nmax, np.uint8), cen=np.zeros((nmax, 3)),
                          dir=np.zeros((nmax, 3)), mag=np.zeros(nmax), rv=np.zeros(nmax),
                          vel=np.zeros((nmax, 3)), stretch=np.zeros((nmax, 3)),
@@ -42,14 +41,41 @@ class OracleStep:
         assert orc.wake_shed(self.pool, cen, alpha, rv, np.tile(self.u, (len(rv), 1)),
                              *self.box) == len(rv)
 
+    def _set_te(self, te):
+        up, lo_i, te0, te1 = te
+        sub = self.sub
+        q = (np.arange(sub) + 0.5) / sub
+        self.up, self.lo = np.repeat(up, sub), np.repeat(lo_i, sub)
+        self.te_vec = np.repeat((te1 - te0) / sub, sub, axis=0)
+        self.te_mid = (np.repeat(te0, sub, axis=0) + np.tile(q, len(up))[:, None] *
+                       np.repeat(te1 - te0, sub, axis=0) + 0.5 * self.dt * self.u)
+
+    def _move(self):
+        from dust_amd.step import trailing_edge
+        from dust_amd.workloads import rotate_z
+        self.theta += self.omega * self.dt
+        v = rotate_z(self.ver0, self.theta)
+        self.ver = v
+        self.G = orc.geo(v, self.nv)
+        self.cen_p = self.G[:, orc.G_CEN:orc.G_CEN + 3]
+        self.nor = self.G[:, orc.G_NOR:orc.G_NOR + 3]
+        A, self.B = orc.aic_surfpan(self.G)
+        self.LU, self.ipiv, info = orc.dgetrf(A)
+        assert info == 0
+        ns, nc = self.strips
+        self._set_te(trailing_edge(v, ns, nc))
+        c = self.cen_p
+        return np.stack([-self.omega * c[:, 1], self.omega * c[:, 0], np.zeros(len(c))], axis=1)
+
     def step(self):
         orc, P = self.orc, self.pool
         live = orc.wake_live(P["free"])
         pc, pd, pm, pr = P["cen"][live], P["dir"][live], P["mag"][live], P["rv"][live]
+        ub = self._move() if self.omega != 0.0 else np.zeros_like(self.cen_p)
         uvort = orc.vort_vel(self.cen_p, pc, pd, pm, pr)
-        b = orc.rhs_static(self.G, self.B, self.u, uvort)
+        b = orc.rhs_static(self.G, self.B, self.u, uvort - ub)
         mu = orc.dgetrs(self.LU, self.ipiv, b)
-        sigma = np.einsum("ij,ij->i", self.nor, -self.u[None] - uvort)
+        sigma = np.einsum("ij,ij->i", self.nor, ub - self.u[None] - uvort)
         t = self.tree
         fmm = orc.Octree(t["box_length"], t["nbox"], t["origin"], t["nlevels"], t["min_part"], 4,
                          t["delta"], u_inf=tuple(self.u))

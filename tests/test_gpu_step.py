@@ -72,3 +72,63 @@ def test_tuned_fmm_steps_match_fp64(case):
     assert rel(gb["mu"], ga["mu"]) <= 1e-6
     assert rel(gb["vel"][lv], ga["vel"][lv]) <= 1e-6
     assert rel(gb["cen"][lv], ga["cen"][lv]) <= 1e-9
+
+
+@pytest.fixture(scope="module")
+def rotor_case():
+    """Two blades of aspect ratio 4 (512 panels) turning at 20 rad/s, 2 degrees
+    per step, in an axial flow of 5 m/s, with 5,000 particles of the helix
+    generator (x1.5) around and through the rotor disk."""
+    blades, panels, aspect = 2, 512, 4.0
+    ver, nv = W.rotor(blades, panels, aspect=aspect)
+    nc, ns = W.rotor_strips(blades, panels, aspect)
+    cen, alpha, rv, _ = W.helix_wake(5000, seed=9)
+    tree = dict(box_length=12.0, nbox=(1, 1, 1), origin=(-6.0, -6.0, -6.0), nlevels=4,
+                min_part=32, delta=0.05)
+    return (ver, nv, 1.5 * cen, alpha, rv, tree), (blades * ns, nc)
+
+
+def make_rotor(rotor_case, form, solver):
+    from dust_amd.step import BodyWakeStep, trailing_edge
+    (ver, nv, cen, alpha, rv, tree), strips = rotor_case
+    te = trailing_edge(ver, *strips)
+    s = BodyWakeStep(ver, nv, te, (0.0, 0.0, -5.0), np.radians(2.0) / 20.0, len(rv) + 400, tree,
+                     form=form, r_shed=0.05, omega=20.0, solver=solver, strips=strips)
+    s.seed(cen, alpha, rv)
+    return s, te
+
+
+def test_rotor_steps_match_oracle(orc, rotor_case):
+    from oracle.pipeline import OracleStep
+    case, strips = rotor_case
+    s, te = make_rotor(rotor_case, "fp64", "fp64")
+    o = OracleStep(case, te, s.pool.nmax, (0.0, 0.0, -5.0), np.radians(2.0) / 20.0, r_shed=0.05,
+                   omega=20.0, strips=strips)
+    for k in range(2):
+        print(f"rotor step {k}", flush=True)
+        s.step()
+        g = s.state()
+        r = o.step()
+        print(f"mu {rel(g['mu'], r['mu']):.2e} uvort {rel(g['uvort'], r['uvort']):.2e} "
+              f"vel {rel(g['vel'][r['live']], r['vel']):.2e}")
+        assert rel(g["mu"], r["mu"]) <= TOL
+        assert rel(g["uvort"], r["uvort"]) <= TOL
+        assert rel(g["vel"][r["live"]], r["vel"]) <= TOL
+        assert np.array_equal(g["free"], o.pool["free"])
+        lv = np.flatnonzero(o.pool["free"] == 0)
+        assert rel(g["cen"][lv], o.pool["cen"][lv]) <= TOL
+
+
+def test_rotor_tuned_matches_fp64(rotor_case):
+    a, _ = make_rotor(rotor_case, "fp64", "fp64")
+    b, _ = make_rotor(rotor_case, "tuned", "fp16s")
+    for k in range(2):
+        print(f"rotor tuned step {k}", flush=True)
+        a.step()
+        b.step()
+    ga, gb = a.state(), b.state()
+    lv = np.flatnonzero(ga["free"] == 0)
+    print(f"mu {rel(gb['mu'], ga['mu']):.2e} vel {rel(gb['vel'][lv], ga['vel'][lv]):.2e}")
+    assert np.array_equal(ga["free"], gb["free"])
+    assert rel(gb["mu"], ga["mu"]) <= 1e-8
+    assert rel(gb["vel"][lv], ga["vel"][lv]) <= 1e-6
