@@ -512,6 +512,48 @@ def time_wing_run(torch, W, nsteps=200, with_cpu=True):
     return out
 
 
+def time_rotor_run(torch, W, n, nsteps=4):
+    """The time-marching rotor at its per-step shape: 4 blades of aspect ratio
+    12 (5,888 surface panels) turning at 200 rad/s, 2 degrees per step, axial
+    flow of 10 m/s, n particles of the helix generator (x3, inside the rotor
+    disk), AIC assembled and factorised every step.  FP64 form (FP64 FMM,
+    FP64 getrf/getrs) and tuned form (tuned FMM, FP16-scaled LU with FP64
+    refinement); nsteps timed steps after one untimed step each."""
+    from dust_amd.step import BodyWakeStep, trailing_edge
+    blades, panels, aspect = 4, 5760, 12.0
+    ver, nv = W.rotor(blades, panels, aspect=aspect)
+    nc, ns = W.rotor_strips(blades, panels, aspect)
+    strips = (blades * ns, nc)
+    te = trailing_edge(ver, *strips)
+    cen, alpha, rv, _ = W.helix_wake(n)
+    tree = dict(box_length=30.0, nbox=(1, 1, 1), origin=(-15.0, -15.0, -15.0), nlevels=7,
+                min_part=64, delta=0.06)
+    omega, dt = 200.0, np.radians(2.0) / 200.0
+    out = dict(panels=len(nv), particles=n, omega=omega, dt=dt)
+    mus = {}
+    for form, solver in (("fp64", "fp64"), ("tuned", "fp16s")):
+        s = BodyWakeStep(ver, nv, te, (0.0, 0.0, -10.0), dt, n + strips[0] * (nsteps + 2), tree,
+                         form=form, r_shed=0.06, omega=omega, solver=solver, strips=strips)
+        s.seed(3.0 * cen, alpha, 3.0 * rv)
+        s.step()
+        s.times = {}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            s.step()
+        torch.cuda.synchronize()
+        tot = time.perf_counter() - t0
+        mus[form] = s.state()["mu"]
+        out[form] = dict(ms_per_step=tot * 1e3 / nsteps,
+                         stages_ms_per_step={k: v / nsteps for k, v in s.times.items()})
+        del s
+        torch.cuda.empty_cache()
+    out["speedup_tuned_vs_fp64"] = out["fp64"]["ms_per_step"] / out["tuned"]["ms_per_step"]
+    out["tuned_mu_rel_l2_vs_fp64"] = float(np.linalg.norm(mus["tuned"] - mus["fp64"]) /
+                                           np.linalg.norm(mus["fp64"]))
+    return out
+
+
 def cpu_baseline(W, n_sample, degree):
     """The CPU oracle (FP64 restatement, OpenMP) on a bounded sample of the same
     generator; particle updates per second."""
@@ -660,6 +702,7 @@ def main():
     ap.add_argument("--no-wing", action="store_true", help="skip the wing configuration leg")
     ap.add_argument("--no-elements", action="store_true",
                     help="skip the element <-> particle leg (rotor panels, helix particles)")
+    ap.add_argument("--no-rotor", action="store_true", help="skip the rotating-rotor steps")
     ap.add_argument("--elements-only", action="store_true",
                     help="time only the element <-> particle and particle-pool legs, print "
                          "their JSON")
@@ -693,7 +736,8 @@ def main():
     if args.elements_only:
         out = dict(elements=time_elements(torch, L, W, n, args.steps, with_cpu=not args.no_cpu),
                    wake=time_wake(torch, L, 5_000_000, args.steps, with_cpu=not args.no_cpu),
-                   wing_run=time_wing_run(torch, W, 200, with_cpu=not args.no_cpu))
+                   wing_run=time_wing_run(torch, W, 200, with_cpu=not args.no_cpu),
+                   rotor_run=time_rotor_run(torch, W, n))
         print(json.dumps(out, default=float))
         return
     log(f"generating helix wake n={n}")
@@ -754,6 +798,11 @@ def main():
         log("timing the wing time-marching run")
         wing_run = time_wing_run(torch, W, 200, with_cpu=not args.no_cpu)
         log(f"wing run: {wing_run}")
+    rotor_run = None
+    if not args.no_rotor:
+        log("timing the rotating-rotor steps")
+        rotor_run = time_rotor_run(torch, W, n)
+        log(f"rotor run: {rotor_run}")
     elements = None
     if not args.no_elements:
         log("timing the element <-> particle terms")
@@ -812,6 +861,7 @@ def main():
         "lu": lu,
         "wing": wing,
         "wing_run": wing_run,
+        "rotor_run": rotor_run,
         "elements": elements,
         "batch": batch,
         "cpu_baseline": cpu,

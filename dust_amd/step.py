@@ -123,7 +123,12 @@ class BodyWakeStep:
         self.times = {}
 
     def _centres(self, v4):
-        return np.array([v4[i, :self.nv_h[i]].mean(axis=0) for i in range(self.n)])
+        """Vertex means (calc_geo_data): the first 3 or 4 vertices, summed in
+        vertex order."""
+        s = v4[:, 0] + v4[:, 1] + v4[:, 2]
+        quad = self.nv_h == 4
+        s[quad] = s[quad] + v4[quad, 3]
+        return s / self.nv_h[:, None].astype(np.float64)
 
     def _set_te(self, te):
         up, lo, te0, te1 = te
