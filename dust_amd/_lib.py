@@ -100,6 +100,17 @@ class WakeParams(C.Structure):
     ]
 
 
+class PaParams(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double),
+        ("dt_wake", C.c_double),
+        ("u_ref", C.c_double),
+        ("rad_mult", C.c_double),
+        ("elrad_mult", C.c_double),
+        ("reserved", C.c_int * 4),
+    ]
+
+
 # (name, restype, argtypes) of every exported function in include/dust_gpu.h
 SIGNATURES = [
     ("dust_last_error", C.c_char_p, []),
@@ -139,6 +150,9 @@ SIGNATURES = [
     ("dust_wake_update_dev", C.c_int,
      [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(WakeParams), _vp, _vp]),
     ("dust_wake_live_dev", C.c_int, [C.c_int, _vp, _vp, _vp, _vp]),
+    ("dust_wake_avoid_collision_dev", C.c_int,
+     [C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(ElemGroup), _vp, C.POINTER(PaParams),
+      _vp]),
     ("dust_wake_shed_dev", C.c_int,
      [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp, C.POINTER(WakeParams),
       _vp, _vp]),

@@ -20,7 +20,7 @@
 // use hipcub stream compaction of the slot flags.
 #include <hipcub/hipcub.hpp>
 
-#include "common.h"
+#include "panel.h"
 
 namespace dust {
 namespace {
@@ -112,6 +112,126 @@ __global__ void k_wake_scatter(int n, const int *__restrict__ live, const double
   }
 }
 
+// ---- avoid_collision (mod_wake.f90:2350-2534) ------------------------------
+
+// Body element seen by the collision avoidance: kind 0 surface panel, 1 vortex
+// lattice (other kinds are skipped), centre, unit normal, largest edge and
+// velocity.
+struct PaElem {
+  double cen[3], nor[3], ub[3];
+  double maxedge;
+  int kind, pad;
+};
+
+__global__ void k_pa_elems(int n, int kind, const int *__restrict__ nver,
+                           const double *__restrict__ ver, const double *__restrict__ ub,
+                           PaElem *out) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  PaElem r{};
+  r.kind = kind == DUST_ELEM_SURFPAN ? 0 : (kind == DUST_ELEM_RING ? 1 : -1);
+  if (r.kind >= 0) {
+    Panel g;
+    panel_geo(nver[j] == 3 ? 3 : 4, ver + 12 * (long)j, g);
+    for (int e = 0; e < 3; ++e) {
+      r.cen[e] = g.cen[e];
+      r.nor[e] = g.nor[e];
+      r.ub[e] = ub ? ub[3 * (long)j + e] : 0.0;
+    }
+    r.maxedge = g.maxedge;
+  }
+  out[j] = r;
+}
+
+// The reference's expressions, each rounded as written (no fused
+// multiply-add): the branch tests compare distances with thresholds.
+__device__ inline double dot_rn(const double *a, const double *b) {
+#pragma clang fp contract(off)
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+__device__ void pa_correct(int ne, const PaElem *__restrict__ E, const double pos1[3],
+                           double r_vortex, const dust_pa_params &P, double vel[3]) {
+#pragma clang fp contract(off)
+  double blthick = r_vortex;
+  const double r2d2 = sqrt(2.0) / 2.0;
+  for (int ie = 0; ie < ne; ++ie) {
+    const PaElem &el = E[ie];
+    const int kind = el.kind;
+    if (kind < 0) continue;
+    double dist1[3], relvel[3], n[3];
+    for (int k = 0; k < 3; ++k) {
+      dist1[k] = pos1[k] - el.cen[k];
+      n[k] = el.nor[k];
+    }
+    double elrad = el.maxedge * r2d2;
+    double check_radius = P.dt * P.u_ref * P.rad_mult + elrad;
+    double distn = sqrt(dot_rn(dist1, dist1));
+    if (!(distn < check_radius)) continue;
+    for (int k = 0; k < 3; ++k) relvel[k] = vel[k] - el.ub[k];
+    double normvel = dot_rn(relvel, n);
+    double dist1_nor, dt_part;
+    if (kind == 0) {
+      dist1_nor = dot_rn(dist1, n);
+      if (dist1_nor < 0.0 || normvel >= 0.0) continue;
+      dt_part = fmin(fmax(dist1_nor / (-normvel), 0.0), P.dt);
+    } else {
+      double q[3] = {dist1[0] * n[0], dist1[1] * n[1], dist1[2] * n[2]};
+      dist1_nor = sqrt(dot_rn(q, q));
+      if (dot_rn(dist1, n) * normvel >= 0.0) continue;
+      dt_part = fmin(fmax(fabs(dist1_nor) / fabs(normvel), 0.0), P.dt);
+    }
+    double dist2[3];
+    for (int k = 0; k < 3; ++k) dist2[k] = (pos1[k] + relvel[k] * dt_part) - el.cen[k];
+    double dist2_nor;
+    if (kind == 0) {
+      dist2_nor = dot_rn(dist2, n);
+    } else {
+      double q[3] = {dist2[0] * n[0], dist2[1] * n[1], dist2[2] * n[2]};
+      dist2_nor = sqrt(dot_rn(q, q));
+      blthick = 2.0 * blthick;  // the reference doubles its local thickness here
+    }
+    if (!(dist2_nor <= blthick)) continue;
+    double t2[3];
+    for (int k = 0; k < 3; ++k) t2[k] = dist2[k] - n[k] * dist2_nor;
+    if (!(sqrt(dot_rn(t2, t2)) < P.elrad_mult * elrad)) continue;
+    double normvel_corr;
+    if (dt_part < P.dt) {
+      normvel_corr = -dist1_nor / P.dt_wake;
+    } else {
+      double dampf = dist2_nor / blthick;
+      normvel_corr = normvel + (blthick - dist2_nor) / P.dt_wake * dampf;
+    }
+    double newvel[3];
+    for (int k = 0; k < 3; ++k) newvel[k] = vel[k] + (normvel_corr - normvel) * n[k];
+    normvel_corr = dot_rn(newvel, n);
+    double nv = dot_rn(vel, n);
+    double tanvel[3];
+    for (int k = 0; k < 3; ++k) tanvel[k] = vel[k] - nv * n[k];
+    double tanvel2 = dot_rn(tanvel, tanvel);
+    double nveldiff = nv * nv - normvel_corr * normvel_corr + tanvel2;
+    if (tanvel2 > 0.0 && normvel_corr * nv >= 0.0 && nveldiff >= 0.0) {
+      double f = sqrt(nveldiff) / sqrt(tanvel2);
+      for (int k = 0; k < 3; ++k) tanvel[k] = tanvel[k] * f;
+    }
+    for (int k = 0; k < 3; ++k) vel[k] = tanvel[k] + normvel_corr * n[k];
+  }
+}
+
+// One thread per live slot; every thread walks the element list in order
+// (uniform loads), almost always leaving at the check-radius test.
+__global__ void __launch_bounds__(256)
+    k_wake_avoid(int nmax, const unsigned char *__restrict__ freef,
+                 const double *__restrict__ cen, const double *__restrict__ rv, double *vel,
+                 int ne, const PaElem *__restrict__ E, dust_pa_params P) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nmax || freef[i]) return;
+  double pos[3] = {cen[3 * (long)i], cen[3 * (long)i + 1], cen[3 * (long)i + 2]};
+  double v[3] = {vel[3 * (long)i], vel[3 * (long)i + 1], vel[3 * (long)i + 2]};
+  pa_correct(ne, E, pos, rv[i], P, v);
+  for (int k = 0; k < 3; ++k) vel[3 * (long)i + k] = v[k];
+}
+
 struct NotFree {
   __host__ __device__ bool operator()(const unsigned char &f) const { return f == 0; }
 };
@@ -173,6 +293,7 @@ __global__ void k_wake_insert(int k, const double *__restrict__ ncen,
 }
 
 struct WakeWs {
+  DevBuf<PaElem> pa;
   DevBuf<int> list, count;
   DevBuf<char> tmp;
   HostBuf<int> hcount;
@@ -281,6 +402,37 @@ int dust_wake_scatter_dev(int n, const int *live, const double *pvel, const doub
   DUST_CHECK(live && pvel && vel && (!pstretch || stretch), DUST_E_ARG, "missing arrays");
   k_wake_scatter<<<ceil_div(n, 256), 256, 0, (hipStream_t)stream>>>(n, live, pvel, pstretch, vel,
                                                                     stretch);
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
+
+int dust_wake_avoid_collision_dev(int nmax, const unsigned char *free_flag, const double *cen,
+                                  const double *r_vortex, double *vel, int ngroups,
+                                  const dust_elem_group *groups, const double *const *ub,
+                                  const dust_pa_params *prm, void *stream) {
+  DUST_CHECK(prm && nmax >= 0 && ngroups >= 0 && (ngroups == 0 || groups), DUST_E_ARG,
+             "invalid collision-avoidance arguments");
+  long ne = 0;
+  for (int k = 0; k < ngroups; ++k) {
+    DUST_CHECK(groups[k].n >= 0 && (groups[k].n == 0 || groups[k].kind == DUST_ELEM_LINE ||
+                                    (groups[k].ver && groups[k].n_ver)),
+               DUST_E_ARG, "element group %d: missing vertices", k);
+    ne += groups[k].n;
+  }
+  if (nmax == 0 || ne == 0) return 0;
+  DUST_CHECK(free_flag && cen && r_vortex && vel, DUST_E_ARG, "missing particle arrays");
+  hipStream_t st = (hipStream_t)stream;
+  DUST_TRY(g_wws.pa.reserve(ne));
+  long off = 0;
+  for (int k = 0; k < ngroups; ++k) {
+    const dust_elem_group &G = groups[k];
+    if (G.n == 0) continue;
+    k_pa_elems<<<ceil_div(G.n, 128), 128, 0, st>>>(G.n, G.kind, G.n_ver, G.ver,
+                                                   ub ? ub[k] : nullptr, g_wws.pa.p + off);
+    off += G.n;
+  }
+  k_wake_avoid<<<ceil_div(nmax, 256), 256, 0, st>>>(nmax, free_flag, cen, r_vortex, vel, (int)ne,
+                                                    g_wws.pa.p, *prm);
   DUST_HIP(hipGetLastError());
   return 0;
 }

@@ -43,9 +43,10 @@ def _f64(a, shape=None):
     return a if shape is None else a.reshape(shape)
 
 
-def pack_groups(groups):
-    """ctypes array of dust_elem_group plus the numpy arrays it points into
-    (kept alive by the caller)."""
+def pack_groups(groups, device=False):
+    """ctypes array of dust_elem_group plus the arrays it points into (kept
+    alive by the caller): numpy host arrays, or with device=True torch tensors
+    on the GPU."""
     keep = []
     arr = (L.ElemGroup * max(1, len(groups)))()
     for k, g in enumerate(groups):
@@ -61,6 +62,10 @@ def pack_groups(groups):
             nv = np.ascontiguousarray(g["n_ver"], dtype=np.int32)
         mag = _f64(g.get("mag"))
         sig = _f64(g.get("sigma"))
+        if device:
+            import torch
+            v, nv, mag, sig = [None if a is None else torch.from_numpy(a).cuda()
+                               for a in (v, nv, mag, sig)]
         keep += [v, nv, mag, sig]
         e = arr[k]
         e.kind, e.n = kind, n

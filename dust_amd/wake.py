@@ -72,3 +72,25 @@ class ParticlePool:
             self.mag.data_ptr(), self.r_vortex.data_ptr(), self.vel.data_ptr(), k,
             dv[0].data_ptr(), dv[1].data_ptr(), dv[2].data_ptr(), dv[3].data_ptr(), C.byref(prm),
             self.n_prt.data_ptr(), None), "wake_shed")
+
+    def avoid_collision(self, groups, ub, prm):
+        """avoid_collision (mod_wake.f90:2350-2534) on the live slots' vel.
+        groups: the body's element lists (dust_amd.elements dicts; surface
+        panels and vortex-lattice rings); ub: per group a device (n,3) tensor
+        of the panels' velocity, or None."""
+        from . import elements as E
+        arr, keep = E.pack_groups(groups, device=True)
+        ptrs = (C.c_void_p * max(1, len(groups)))(
+            *[None if u is None else u.data_ptr() for u in ub])
+        L.check(L.lib().dust_wake_avoid_collision_dev(
+            self.nmax, self.free.data_ptr(), self.cen.data_ptr(), self.r_vortex.data_ptr(),
+            self.vel.data_ptr(), len(groups), arr, ptrs, C.byref(prm), None), "avoid_collision")
+        del keep
+
+
+def pa_params(dt, dt_wake, u_ref, rad_mult=1.0, elrad_mult=1.0):
+    """sim_param%dt, dt * ndt_update_wake, u_ref, pa_rad_mult, pa_elrad_mult."""
+    p = L.PaParams()
+    p.dt, p.dt_wake, p.u_ref = float(dt), float(dt_wake), float(u_ref)
+    p.rad_mult, p.elrad_mult = float(rad_mult), float(elrad_mult)
+    return p

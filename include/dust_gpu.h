@@ -255,8 +255,8 @@ typedef struct {
 /* Particle loop of complete_wake (mod_wake.f90:1305-1355) over the non-free
  * slots: move by vel*dt; inside the box, optional divergence filter on
  * stretch and explicit Euler on the direction (mag unchanged); outside, free
- * the slot and decrement *n_prt.  The collision avoidance of use_pa
- * (avoid_collision, :2350) is not part of it.  Device pointers, ordered on
+ * the slot and decrement *n_prt.  The collision avoidance of use_pa runs
+ * before it (dust_wake_avoid_collision_dev).  Device pointers, ordered on
  * `stream`, no host synchronisation. */
 int dust_wake_update_dev(int nmax, unsigned char *free_flag, double *cen, double *dir,
                          const double *mag, const double *vel, double *stretch,
@@ -280,6 +280,29 @@ int dust_wake_shed_dev(int nmax, unsigned char *free_flag, double *cen, double *
                        double *mag, double *r_vortex, double *vel, int k, const double *new_cen,
                        const double *new_partvec, const double *new_rv, const double *new_vel,
                        const dust_wake_params *prm, int *n_prt, void *stream);
+
+/* Collision avoidance of use_pa (avoid_collision, mod_wake.f90:2350-2534,
+ * called from complete_wake :1313-1318 before the particle update): for
+ * every non-free slot, the velocity is corrected element by element, in list
+ * order, against the body's surface panels (DUST_ELEM_SURFPAN lists) and
+ * vortex-lattice panels (DUST_ELEM_RING lists; lifting lines take no part,
+ * DUST_ELEM_LINE lists are skipped).  ub[k] (device, (3,n) of group k, or
+ * NULL) is the panels' velocity; mag and sigma of the groups are unused.
+ * The reference's quirks are kept: the vortex-lattice normal distance is
+ * |dist*n| elementwise, and the layer thickness (the particle's r_Vortex)
+ * doubles each time that branch reaches its test.  Device pointers. */
+typedef struct {
+  double dt;         /* sim_param%dt                     */
+  double dt_wake;    /* sim_param%dt * ndt_update_wake   */
+  double u_ref;      /* sim_param%u_ref                  */
+  double rad_mult;   /* sim_param%pa_rad_mult            */
+  double elrad_mult; /* sim_param%pa_elrad_mult          */
+  int reserved[4];
+} dust_pa_params;
+int dust_wake_avoid_collision_dev(int nmax, const unsigned char *free_flag, const double *cen,
+                                  const double *r_vortex, double *vel, int ngroups,
+                                  const dust_elem_group *groups, const double *const *ub,
+                                  const dust_pa_params *prm, void *stream);
 
 /* The pointer arrays the FMM and the element loops read (the reference
  * passes part_p(:)%p): pcen(3,n), pdir(3,n), pmag(n), prv(n) and optional

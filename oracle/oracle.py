@@ -69,6 +69,9 @@ def lib():
             "ao_wake_update": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                          C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]),
             "ao_wake_live": (C.c_int, [C.c_int, _vp, _vp]),
+            "ao_avoid_collision": (None, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,
+                                          C.c_double, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, _vp, _vp]),
             "ao_wake_shed": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp,
                                        _vp, _vp, _vp]),
             "ao_dgetrf": (C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp]),
@@ -390,6 +393,27 @@ def wake_shed(pool, ncen, npv, nrv, nvel, box_min, box_max):
                               _p(pool["mag"]), _p(pool["rv"]), _p(pool["vel"]), len(nrv),
                               _p(_c(ncen)), _p(_c(npv)), _p(_c(nrv)), _p(_c(nvel)),
                               _p(_c(box_min)), _p(_c(box_max)))
+
+
+def avoid_collision(kind, ecen, enor, emaxedge, eub, pos, r_vortex, vel, dt, dtw, u_ref,
+                    rad_mult, elrad_mult):
+    """avoid_collision for one particle; element kinds 0 (surface panel) and 1
+    (vortex lattice).  Returns the corrected velocity."""
+    out = np.zeros(3)
+    lib().ao_avoid_collision(len(kind), _p(_c(kind, np.int32)), _p(_c(ecen)), _p(_c(enor)),
+                             _p(_c(emaxedge)), _p(_c(eub)), _p(_c(pos)), float(r_vortex),
+                             float(dt), float(dtw), float(u_ref), float(rad_mult),
+                             float(elrad_mult), _p(_c(vel)), _p(out))
+    return out
+
+
+def avoid_collision_pool(kind, ecen, enor, emaxedge, eub, pool, dt, dtw, u_ref, rad_mult,
+                         elrad_mult):
+    """avoid_collision applied to every live slot of a pool dict (vel updated)."""
+    for i in np.flatnonzero(pool["free"] == 0):
+        pool["vel"][i] = avoid_collision(kind, ecen, enor, emaxedge, eub, pool["cen"][i],
+                                         pool["rv"][i], pool["vel"][i], dt, dtw, u_ref, rad_mult,
+                                         elrad_mult)
 
 
 def dgetrf(A):

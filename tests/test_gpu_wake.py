@@ -8,6 +8,7 @@ import pytest
 
 from dust_amd import _lib as L
 from dust_amd import wake as WK
+from dust_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 BOX = (np.array([-1.0, -1.0, -1.0]), np.array([1.0, 1.0, 1.0]))
@@ -106,3 +107,45 @@ def test_empty_pool_and_no_new_particles():
     Q = WK.ParticlePool(10)
     Q.shed(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0), np.zeros((0, 3)), prm)
     assert Q.live().numel() == 0
+
+
+def test_avoid_collision_vs_oracle(orc):
+    """avoid_collision on a pool around a two-blade rotor mesh given once as
+    surface panels and once as vortex-lattice panels (both branches, the
+    thickness doubling), panels moving with omega x r, particles placed just
+    off the panels on both sides with velocities towards them, plus far
+    ones."""
+    import torch
+    ver, nv = W.rotor(2, 300, aspect=3.0)
+    G = orc.geo(ver, nv)
+    cen = G[:, orc.G_CEN:orc.G_CEN + 3]
+    nor = G[:, orc.G_NOR:orc.G_NOR + 3]
+    me = np.array([G[j, orc.G["elen"]:orc.G["elen"] + 4].max() for j in range(len(nv))])
+    ub = np.stack([-5.0 * cen[:, 1], 5.0 * cen[:, 0], np.zeros(len(nv))], axis=1)
+    rng = np.random.default_rng(21)
+    m = 6000
+    pick = rng.integers(0, len(nv), m)
+    h = rng.uniform(-0.03, 0.03, m)
+    pos = cen[pick] + h[:, None] * nor[pick] + 0.02 * rng.standard_normal((m, 3))
+    vel = -np.sign(h)[:, None] * nor[pick] * rng.uniform(0.5, 5.0, (m, 1)) + \
+        rng.standard_normal((m, 3))
+    far = rng.uniform(-6.0, 6.0, (2000, 3))
+    hp = host_pool(m + 2000, 4, free_frac=0.1)
+    hp["cen"] = np.concatenate([pos, far])
+    hp["vel"] = np.concatenate([vel, rng.standard_normal((2000, 3))])
+    hp["rv"] = rng.uniform(0.005, 0.03, m + 2000)
+    P = to_device(hp)
+    kinds = np.concatenate([np.zeros(len(nv), np.int32), np.ones(len(nv), np.int32)])
+    groups = [dict(kind=0, ver=ver, n_ver=nv), dict(kind=1, ver=ver, n_ver=nv)]
+    tub = torch.from_numpy(np.ascontiguousarray(ub)).cuda()
+    prm = WK.pa_params(0.01, 0.02, 10.0, 1.0, 1.0)
+    print(f"avoid_collision n={m + 2000} elements={2 * len(nv)}", flush=True)
+    P.avoid_collision(groups, [tub, tub], prm)
+    orc.avoid_collision_pool(kinds, np.concatenate([cen, cen]), np.concatenate([nor, nor]),
+                             np.concatenate([me, me]), np.concatenate([ub, ub]), hp, 0.01, 0.02,
+                             10.0, 1.0, 1.0)
+    gv = P.vel.cpu().numpy()
+    changed = np.any(hp["vel"] != np.concatenate([vel, hp["vel"][m:]]), axis=1)
+    print(f"changed {np.count_nonzero(changed)}")
+    assert np.count_nonzero(changed[:m]) > 100
+    assert close(gv, hp["vel"], 1e-13)

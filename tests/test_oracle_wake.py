@@ -10,7 +10,12 @@ properties that do not restate its loops (CPU only):
   filt_eta mag (dir - rotu/|rotu|);
 - prepare_wake's list (:826-835) is the ascending list of non-free slots;
 - shedding (:1888-1967): the inside particles occupy, in order, the lowest
-  free slots; outside ones take none; a full pool is an error.
+  free slots; outside ones take none; a full pool is an error;
+- avoid_collision (:2350-2534): untouched far away, below a surface panel or
+  moving away; a particle crossing a surface panel within dt lands on its
+  plane after dt*ndt with its speed kept; inside the layer the damped
+  correction; the vortex-lattice branch's elementwise normal distance and
+  its layer thickness doubling per panel, each from a hand-worked case.
 """
 import numpy as np
 import pytest
@@ -132,3 +137,76 @@ def test_shed_full_pool_is_an_error(orc):
     k = 3
     assert orc.wake_shed(p, np.zeros((k, 3)), np.ones((k, 3)), np.full(k, 0.1), np.zeros((k, 3)),
                          *BOX) == -1
+
+
+# ---- avoid_collision (mod_wake.f90:2350-2534) --------------------------------
+PA = dict(dt=0.01, dtw=0.02, u_ref=10.0, rad_mult=1.0, elrad_mult=1.0)
+
+
+def flat(kind, z=0.0, maxedge=0.4):
+    return (np.array([kind]), np.array([[0.0, 0.0, z]]), np.array([[0.0, 0.0, 1.0]]),
+            np.array([maxedge]), np.zeros((1, 3)))
+
+
+def test_collision_far_or_leaving_is_untouched(orc):
+    el = flat(0)
+    v = np.array([0.3, -0.2, -2.0])
+    far = orc.avoid_collision(*el, np.array([5.0, 0.0, 0.05]), 0.02, v, **PA)
+    assert np.array_equal(far, v)
+    below = orc.avoid_collision(*el, np.array([0.05, 0.0, -0.05]), 0.02, v, **PA)
+    assert np.array_equal(below, v)
+    away = orc.avoid_collision(*el, np.array([0.05, 0.0, 0.05]), 0.02, -v, **PA)
+    assert np.array_equal(away, -v)
+
+
+def test_collision_lands_on_the_panel_with_the_same_speed(orc):
+    """Surface panel: a particle that would cross the panel within dt gets the
+    normal velocity that brings it exactly onto the plane in dt*ndt, and the
+    tangential part is rescaled so that the speed is unchanged."""
+    el = flat(0)
+    h, w = 0.01, 2.0                       # crosses after h/w = 0.005 < dt
+    v = np.array([0.3, -0.2, -w])
+    out = orc.avoid_collision(*el, np.array([0.05, 0.02, h]), 0.02, v, **PA)
+    assert out[2] == pytest.approx(-h / PA["dtw"], rel=1e-14)
+    assert np.linalg.norm(out) == pytest.approx(np.linalg.norm(v), rel=1e-14)
+    assert np.allclose(out[:2] / np.linalg.norm(out[:2]), v[:2] / np.linalg.norm(v[:2]))
+
+
+def test_collision_damped_inside_the_layer(orc):
+    """Surface panel, not reached within dt but ending inside the layer of
+    thickness r_vortex: normvel + (b - d2)/dtw * d2/b."""
+    el = flat(0)
+    b, h, w = 0.02, 0.025, 1.0             # d2 = h - w dt = 0.015 <= b
+    v = np.array([0.0, 0.0, -w])
+    out = orc.avoid_collision(*el, np.array([0.0, 0.0, h]), b, v, **PA)
+    d2 = h - w * PA["dt"]
+    assert out[2] == pytest.approx(-w + (b - d2) / PA["dtw"] * (d2 / b), rel=1e-13)
+    assert out[0] == 0.0 and out[1] == 0.0
+
+
+def test_vortex_lattice_layer_doubles_per_panel(orc):
+    """Vortex-lattice branch: the layer thickness doubles each time the branch
+    reaches its test (the reference's local blthick).  With the particle
+    ending at d2 in (2b, 4b], the first of two coincident panels leaves the
+    velocity alone and the second corrects it with thickness 4b."""
+    b, h, w = 0.01, 0.04, 1.0              # d2 = 0.03 in (0.02, 0.04]
+    el = tuple(np.concatenate([a, a]) for a in flat(1))
+    v = np.array([0.0, 0.0, -w])
+    out = orc.avoid_collision(*el, np.array([0.0, 0.0, h]), b, v, **PA)
+    d2 = h - w * PA["dt"]
+    assert out[2] == pytest.approx(-w + (4 * b - d2) / PA["dtw"] * (d2 / (4 * b)), rel=1e-13)
+    one = orc.avoid_collision(*flat(1), np.array([0.0, 0.0, h]), b, v, **PA)
+    assert np.array_equal(one, v)
+
+
+def test_vortex_lattice_normal_distance_is_elementwise_norm(orc):
+    """Vortex-lattice branch: the normal distance is |dist * n| (elementwise),
+    so with a tilted normal it differs from the dot product; a particle on the
+    far side is corrected too (the test is on the sign of (dist.n)(v.n))."""
+    n = np.array([0.0, 0.6, 0.8])
+    el = (np.array([1]), np.zeros((1, 3)), n[None], np.array([0.4]), np.zeros((1, 3)))
+    pos = -0.01 * n                         # below, moving up towards the panel
+    v = 2.0 * n
+    out = orc.avoid_collision(*el, pos, 0.02, v, **PA)
+    d1 = np.linalg.norm(pos * n)            # elementwise, as the reference
+    assert np.dot(out, n) == pytest.approx(-d1 / PA["dtw"], rel=1e-13)
