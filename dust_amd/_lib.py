@@ -124,6 +124,8 @@ SIGNATURES = [
     ("dust_fmm_apply_multipole", C.c_int, [_vp, _vp, _vp, _vp]),
     ("dust_fmm_apply_multipole_dev", C.c_int, [_vp]),
     ("dust_fmm_results_dev", C.c_int, [_vp, _vp, _vp, _vp]),
+    ("dust_fmm_apply_multipole_panels", C.c_int, [_vp, C.c_int, _vp, _vp]),
+    ("dust_fmm_apply_multipole_panels_dev", C.c_int, [_vp, C.c_int, _vp, _vp]),
     ("dust_fmm_pass_dev", C.c_int, [_vp, _ip]),
     ("dust_fmm_pass_batch_dev", C.c_int, [C.c_int, _vp, _ip]),
     ("dust_fmm_set_tuning", C.c_int, [_vp, C.c_char_p, C.c_double]),

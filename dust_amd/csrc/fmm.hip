@@ -1389,6 +1389,124 @@ static int fmm_apply(FmmHandle *h) {
   return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// apply_multipole_panels (mod_octree.f90:1434-1510) with the panel binning of
+// sort_particles (777-820): the particle velocity at panel centres from the
+// tree of the last pass, added into uvort.
+// ---------------------------------------------------------------------------
+
+// Leaf of every centre: its finest cell (ceiling, as for the particles), then
+// the first ancestor-or-self flagged leaf (every point of the box lies under
+// one: a level-1 cell with no active child is a leaf; -1 is kept for an empty
+// tree).
+__global__ void k_pan_leaf(TreeGeom G, int m, const double *__restrict__ cen,
+                           const unsigned char *__restrict__ flags,
+                           const int *__restrict__ leaves, int n_leaves, int *pleaf, int *err) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int L = G.nlev - 1;
+  int id[3];
+  bool bad = false;
+  for (int d = 0; d < 3; ++d) {
+    double q = ceil((cen[3 * i + d] - G.xmin[d]) / G.csize_fine);
+    if (!(q >= 1.0 && q <= (double)G.ncl[L][d])) bad = true;
+    id[d] = bad ? 1 : (int)q - 1;
+  }
+  pleaf[i] = -1;
+  if (bad) {
+    atomicMin(err, i);
+    return;
+  }
+  for (int l = L; l >= 0; --l) {
+    int sh = L - l;
+    long g = G.off[l] + cell_encode(G, l, id[0] >> sh, id[1] >> sh, id[2] >> sh);
+    if (flags[g] == kLeaf) {
+      int lo = 0, hi = n_leaves - 1;  // leaves are in ascending cell order
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (leaves[mid] < g) lo = mid + 1;
+        else hi = mid;
+      }
+      if (n_leaves > 0 && leaves[lo] == g) pleaf[i] = lo;
+      return;
+    }
+  }
+}
+
+// One wavefront per panel: the lanes split the leaf's P2P list (velocity of
+// compute_vel_vortpart, Rosenhead core of the source), a fixed-order
+// reduction, then lane 0 adds the L2P velocity and writes uvort += .
+__global__ void __launch_bounds__(256)
+    k_pan_eval(EvalArgs A, int m, const double *__restrict__ cen, const int *__restrict__ pleaf,
+               double *uvort) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= m) return;
+  const int lv = pleaf[i];
+  if (lv < 0) return;
+  const double x = cen[3 * i], y = cen[3 * i + 1], z = cen[3 * i + 2];
+  const int2 *rl = &A.rng[(long)lv * A.cap];
+  const int nr = A.cnt[lv];
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  for (int r = 0; r < nr; ++r) {
+    const int2 rg = rl[r];
+    for (int q = rg.x + lane; q < rg.y; q += kWave) {
+      double d0 = x - A.px[q], d1 = y - A.py[q], d2 = z - A.pz[q];
+      double rr = A.pr[q];
+      double R2 = d0 * d0 + d1 * d1 + d2 * d2 + rr * rr;
+      double c = -A.pm[q] / (R2 * sqrt(R2));
+      double a0 = A.dx[q], a1 = A.dy[q], a2 = A.dz[q];
+      v0 += c * (d1 * a2 - d2 * a1);
+      v1 += c * (d2 * a0 - d0 * a2);
+      v2 += c * (d0 * a1 - d1 * a0);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    v0 += __shfl_xor(v0, o, kWave);
+    v1 += __shfl_xor(v1, o, kWave);
+    v2 += __shfl_xor(v2, o, kWave);
+  }
+  if (lane != 0) return;
+  const long g = A.leaves[lv];
+  const int lvl = lvl_of(A.G, g);
+  int ix, iy, iz;
+  cell_decode(A.G, lvl, g - A.G.off[lvl], ix, iy, iz);
+  const double cx = cell_center(A.G, lvl, 0, ix), cy = cell_center(A.G, lvl, 1, iy),
+               cz = cell_center(A.G, lvl, 2, iz);
+  const double *sb = A.mp_b + (long)A.slot[g] * 3 * A.nmon;
+  double v[3], gr[9];
+  l2p_point(A.deg, A.nmon, 0, sb, sb, A.pwr, x - cx, y - cy, z - cz, v, gr);
+  const double f = 1.0 / (4.0 * kPi);
+  uvort[3 * i] += v[0] + v0 * f;
+  uvort[3 * i + 1] += v[1] + v1 * f;
+  uvort[3 * i + 2] += v[2] + v2 * f;
+}
+
+static int fmm_apply_panels(FmmHandle *h, int m, const double *cen, double *uvort) {
+  DUST_CHECK(h && h->computed, DUST_E_ARG,
+             "calculate_multipole must precede apply_multipole_panels");
+  DUST_CHECK(m >= 0 && (m == 0 || (cen && uvort)), DUST_E_ARG, "invalid panel arguments");
+  if (m == 0) return 0;
+  static DevBuf<int> pleaf, perr;
+  DUST_TRY(pleaf.reserve(m));
+  DUST_TRY(perr.reserve(1));
+  DUST_TRY(h->rb.reserve(1));
+  TreeGeom G = make_geom(h);
+  k_fill_int<<<1, 32, 0, h->stream>>>(1, perr.p, 0x7fffffff);
+  k_pan_leaf<<<ceil_div(m, 256), 256, 0, h->stream>>>(G, m, cen, h->flags.p, h->leaves.p,
+                                                     h->n_leaves, pleaf.p, perr.p);
+  DUST_HIP(hipMemcpyAsync(&h->rb.p->err, perr.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  DUST_HIP(hipStreamSynchronize(h->stream));
+  DUST_CHECK(h->rb.p->err == 0x7fffffff, DUST_E_OUTSIDE,
+             "panel centre %d outside the octree box", h->rb.p->err);
+  if (h->n == 0 || h->n_leaves == 0) return 0;
+  EvalArgs A;
+  make_eval_args(h, &A);
+  k_pan_eval<<<ceil_div(m, 4), 256, 0, h->stream>>>(A, m, cen, pleaf.p, uvort);
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
 }  // namespace dust
 
 // ===========================================================================
@@ -1478,6 +1596,38 @@ int dust_fmm_results_dev(dust_fmm_t hh, double *vel, double *stretch, double *tu
   if (turbvisc)
     DUST_HIP(hipMemcpyAsync(turbvisc, h->tv.p, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
   return 0;
+}
+
+int dust_fmm_apply_multipole_panels_dev(dust_fmm_t hh, int m, const double *cen,
+                                        double *uvort) {
+  DUST_CHECK(hh, DUST_E_ARG, "null handle");
+  return fmm_apply_panels(hh->h, m, cen, uvort);
+}
+
+int dust_fmm_apply_multipole_panels(dust_fmm_t hh, int m, const double *cen, double *uvort) {
+  DUST_CHECK(hh, DUST_E_ARG, "null handle");
+  FmmHandle *h = hh->h;
+  DUST_CHECK(m >= 0 && (m == 0 || (cen && uvort)), DUST_E_ARG, "invalid panel arguments");
+  if (m == 0) return 0;
+  double *d = nullptr;
+  DUST_HIP(hipMalloc(&d, sizeof(double) * 6 * (size_t)m));
+  int rc = 0;
+  if (hipMemcpy(d, cen, sizeof(double) * 3 * m, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d + 3 * (size_t)m, uvort, sizeof(double) * 3 * m, hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    set_error("copy of the panel centres failed");
+    rc = DUST_E_HIP;
+  }
+  if (rc == 0) rc = fmm_apply_panels(h, m, d, d + 3 * (size_t)m);
+  if (rc == 0 &&
+      hipMemcpyAsync(uvort, d + 3 * (size_t)m, sizeof(double) * 3 * m, hipMemcpyDeviceToHost,
+                     h->stream) != hipSuccess) {
+    set_error("copy of uvort failed");
+    rc = DUST_E_HIP;
+  }
+  (void)hipStreamSynchronize(h->stream);
+  (void)hipFree(d);
+  return rc;
 }
 
 int dust_fmm_pass_dev(dust_fmm_t hh, int *nleaves) {

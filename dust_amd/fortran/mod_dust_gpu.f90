@@ -123,6 +123,10 @@ module mod_dust_gpu
       type(t_dust_elem_params), intent(in) :: prm
       real(c_double), intent(out) :: uvort(3,*)
     end function
+    integer(c_int) function dust_fmm_apply_multipole_panels(h, m, cen, uvort) bind(C)
+      import; type(c_ptr), value :: h; integer(c_int), value :: m
+      real(c_double), intent(in) :: cen(3,*); real(c_double), intent(inout) :: uvort(3,*)
+    end function
   end interface
 
 contains

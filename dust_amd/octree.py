@@ -197,6 +197,17 @@ def apply_multipole(parts, octree):
                                              L.ptr(parts.turbvisc)), "apply_multipole")
 
 
+def apply_multipole_panels(octree, cen, uvort=None):
+    """apply_multipole_panels (mod_octree.f90:1434-1510): uvort (m,3) +=
+    the particle velocity at the panel centres cen (m,3) from the last tree."""
+    cen = np.ascontiguousarray(cen, dtype=np.float64).reshape(-1, 3)
+    u = (np.zeros((len(cen), 3)) if uvort is None
+         else np.array(uvort, dtype=np.float64).reshape(-1, 3))
+    L.check(L.lib().dust_fmm_apply_multipole_panels(octree.handle, len(cen), L.ptr(cen),
+                                                    L.ptr(u)), "apply_multipole_panels")
+    return u
+
+
 def destroy_octree(octree):
     """destroy_octree (mod_octree.f90:694)."""
     octree.destroy()

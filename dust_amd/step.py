@@ -27,7 +27,10 @@ u_inf.  A rotating body (omega != 0) turns by omega dt at the start of each
 step (geometry update of the caller), its AIC is assembled again and solved
 with a fresh factorisation (FP64 getrf/getrs, or the mixed solve with
 solver="fp32" / "fp16s"), and the panels' velocity omega x r enters the
-source strengths.  Every stage runs on the GPU through
+source strengths.  With fmm_pan (the reference's use_fmm_pan) the tree and
+expansions are built first and uvort comes from apply_multipole_panels on
+them instead of the direct get_vort_vel sum.  Every stage runs on the GPU
+through
 the C ABI; the host reads back the live count and the trailing-edge
 intensities (2 x segments values) once per step.
 """
@@ -56,7 +59,7 @@ def trailing_edge(ver, strips, nc):
 class BodyWakeStep:
     def __init__(self, ver, n_ver, te, u_inf, dt, nmax, tree, degree=4, form="fp64",
                  aic=None, r_shed=0.1, box=None, vs_elems=False, sub=1, omega=0.0,
-                 solver="fp64", strips=None):
+                 solver="fp64", strips=None, fmm_pan=False):
         import torch
         self.torch = torch
         self.lib = L.lib()
@@ -80,6 +83,7 @@ class BodyWakeStep:
         self.theta = 0.0
         self.solver = solver
         self.strips = strips  # (strips, nc) of the trailing-edge layout, moving bodies
+        self.fmm_pan = bool(fmm_pan)
         n = self.n
         # static AIC and its LU factors (initialize_linsys, mod_linsys.f90:236)
         self.A = torch.empty((n, n), dtype=torch.float64, device="cuda")
@@ -179,6 +183,18 @@ class BodyWakeStep:
                                          p["dir"].data_ptr(), p["mag"].data_ptr(),
                                          p["rv"].data_ptr(), p["alpha"].data_ptr(), None),
                 "gather")
+        h = self.fmm.handle
+        if self.fmm_pan and npart:
+            # use_fmm_pan: the tree and the expansions are built in prepare_wake
+            # (mod_wake.f90:797-801) and serve both uvort and the particles
+            torch.cuda.synchronize()
+            L.check(lib.dust_fmm_set_particles_dev(h, npart, p["cen"].data_ptr(),
+                                                   p["dir"].data_ptr(), p["mag"].data_ptr(),
+                                                   p["rv"].data_ptr()), "fmm set")
+            nl = C.c_int()
+            L.check(lib.dust_fmm_sort_particles(h, C.byref(nl)), "fmm sort")
+            L.check(lib.dust_fmm_calculate_multipole(h), "fmm multipole")
+            L.check(lib.dust_fmm_synchronize(h), "fmm sync")
         ev[1].record()
         moving = self.omega != 0.0
         if moving:  # moving body: geometry of the step, AIC re-assembled
@@ -191,10 +207,22 @@ class BodyWakeStep:
                                              C.byref(ap), self.A.data_ptr(), self.n,
                                              self.B.data_ptr(), self.n, None), "aic")
         # assemble_linsys: uvort, right-hand side and source strengths
-        L.check(lib.dust_vort_vel_dev(self.n, self.cen.data_ptr(), npart, p["cen"].data_ptr(),
-                                      p["dir"].data_ptr(), p["mag"].data_ptr(),
-                                      p["rv"].data_ptr(), 0, None, None, C.byref(self.eprm),
-                                      self.uvort.data_ptr(), None), "uvort")
+        if self.fmm_pan:
+            # apply_multipole_panels on the tree of this step (no end vortices
+            # here, so get_vort_vel contributes nothing)
+            self.uvort.zero_()
+            if npart:
+                torch.cuda.synchronize()
+                L.check(lib.dust_fmm_apply_multipole_panels_dev(h, self.n, self.cen.data_ptr(),
+                                                                self.uvort.data_ptr()),
+                        "fmm panels")
+                L.check(lib.dust_fmm_synchronize(h), "fmm sync")
+        else:
+            L.check(lib.dust_vort_vel_dev(self.n, self.cen.data_ptr(), npart,
+                                          p["cen"].data_ptr(), p["dir"].data_ptr(),
+                                          p["mag"].data_ptr(), p["rv"].data_ptr(), 0, None,
+                                          None, C.byref(self.eprm), self.uvort.data_ptr(),
+                                          None), "uvort")
         u = np.ascontiguousarray(self.u_inf)
         # sigma = nor.(ub - u_inf - uvort): the moving panels' velocity enters
         # as uvort - ub
@@ -229,14 +257,16 @@ class BodyWakeStep:
                                              C.byref(it), C.byref(res), None), "mixed solve")
         ev[3].record()
         # update_wake: FMM pass over the live particles
-        h = self.fmm.handle
         torch.cuda.synchronize()
-        if npart:
+        if npart and self.fmm_pan:
+            L.check(lib.dust_fmm_apply_multipole_dev(h), "fmm apply")
+        elif npart:
             L.check(lib.dust_fmm_set_particles_dev(h, npart, p["cen"].data_ptr(),
                                                    p["dir"].data_ptr(), p["mag"].data_ptr(),
                                                    p["rv"].data_ptr()), "fmm set")
             nl = C.c_int()
             L.check(lib.dust_fmm_pass_dev(h, C.byref(nl)), "fmm pass")
+        if npart:
             L.check(lib.dust_fmm_results_dev(h, p["vel"].data_ptr(), p["stretch"].data_ptr(),
                                              None), "fmm results")
             L.check(lib.dust_fmm_synchronize(h), "fmm sync")

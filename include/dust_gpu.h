@@ -100,6 +100,18 @@ int dust_fmm_apply_multipole_dev(dust_fmm_t h);
 /* Device-to-device copy of the last results (input particle order). */
 int dust_fmm_results_dev(dust_fmm_t h, double *vel, double *stretch, double *turbvisc);
 
+/* apply_multipole_panels (mod_octree.f90:1434-1510) with the panel binning
+ * of sort_particles (777-820, the use_fmm_pan path): the particle velocity at
+ * m panel centres cen(3,m) from the tree and expansions of the last
+ * sort_particles + calculate_multipole, added into uvort(3,m): L2P of the
+ * leaf that contains the centre's finest cell plus the particles of that
+ * leaf's P2P list (compute_vel / 4 pi).  DUST_E_OUTSIDE for a centre outside the
+ * box.  Host pointers (synchronous) / device pointers on the handle's
+ * stream (synchronises to report DUST_E_OUTSIDE). */
+int dust_fmm_apply_multipole_panels(dust_fmm_t h, int m, const double *cen, double *uvort);
+int dust_fmm_apply_multipole_panels_dev(dust_fmm_t h, int m, const double *cen,
+                                        double *uvort);
+
 /* One full pass (sort + multipole + apply) on device-resident particles. */
 int dust_fmm_pass_dev(dust_fmm_t h, int *nleaves);
 

@@ -918,6 +918,85 @@ void orc_apply_multipole(orc_octree *T, double *vel_out, double *str_out, double
   }
 }
 
+/* apply_multipole_panels, mod_octree.f90:1434-1510, with the panel binning
+ * of sort_particles (777-820, use_fmm_pan: a centre outside the box is an
+ * error).  A panel belongs to the leaf that contains its finest cell (every
+ * point of the box lies under one).  uvort(3,m) is accumulated: L2P of
+ * the leaf's local expansion, then the particles of the leaf's active
+ * neighbours, of its leaf_neigh list and of the leaf itself, each
+ * compute_vel / (4 pi).  Returns 0, or -1 - i for the first centre i
+ * outside the box. */
+int orc_apply_multipole_panels(orc_octree *T, int m, const double *cen, double *uvort) {
+  const int Lf = T->nlevels - 1;
+  const orc_layer *F = &T->L[Lf];
+  for (int ip = 0; ip < m; ++ip)
+    for (int d = 0; d < 3; ++d) {
+      double q = ceil((cen[3 * ip + d] - T->xmin[d]) / F->csize);
+      if (!(q >= 1.0 && q <= (double)F->ncl[d])) return -1 - ip;
+    }
+  const int nm = T->pexp.n_mon;
+  const double fpi = 4.0 * ORC_PI;
+#pragma omp parallel for schedule(dynamic)
+  for (int ip = 0; ip < m; ++ip) {
+    const double *pos = &cen[3 * ip];
+    int id[3];
+    for (int d = 0; d < 3; ++d) id[d] = (int)ceil((pos[d] - T->xmin[d]) / F->csize);
+    int l = Lf, c = -1;
+    for (; l >= 0; --l) {
+      int cc = cidx(&T->L[l], id[0], id[1], id[2]);
+      if (T->L[l].leaf[cc]) {
+        c = cc;
+        break;
+      }
+      for (int d = 0; d < 3; ++d) id[d] = (id[d] + 1) / 2;
+    }
+    if (c < 0) continue;
+    const orc_layer *L = &T->L[l];
+    const double *b = &T->b[(size_t)L->slot[c] * 3 * nm];
+    const double *lc = &L->cen[3 * c];
+    double vel[3] = {0.0, 0.0, 0.0}, t[3];
+    for (int q = 0; q < nm; ++q) {
+      const int *pw = &T->pexp.pwr[3 * q];
+      double pr = ipow(pos[0] - lc[0], pw[0]) * ipow(pos[1] - lc[1], pw[1]) *
+                  ipow(pos[2] - lc[2], pw[2]);
+      for (int e = 0; e < 3; ++e) vel[e] += b[e + 3 * q] * pr;
+    }
+    int i, j, k;
+    cijk(L, c, &i, &j, &k);
+    for (int kk = -1; kk <= 1; ++kk)
+      for (int jj = -1; jj <= 1; ++jj)
+        for (int ii = -1; ii <= 1; ++ii) {
+          if (!ii && !jj && !kk) continue;
+          int a = i + ii, bb = j + jj, d = k + kk;
+          if (a < 1 || bb < 1 || d < 1 || a > L->ncl[0] || bb > L->ncl[1] || d > L->ncl[2])
+            continue;
+          int nc = cidx(L, a, bb, d);
+          if (!L->active[nc]) continue;
+          const int *pl = &L->plist[L->pstart[nc]];
+          for (int q = 0; q < L->npart[nc]; ++q) {
+            p_vel(&T->P, pl[q], pos, t);
+            for (int e = 0; e < 3; ++e) vel[e] = vel[e] + t[e] / fpi;
+          }
+        }
+    for (int q2 = 0; q2 < L->ln_count[c]; ++q2) {
+      const orc_layer *Ln = &T->L[L->ln_lev[L->ln_start[c] + q2]];
+      int nc = L->ln_cell[L->ln_start[c] + q2];
+      const int *pl = &Ln->plist[Ln->pstart[nc]];
+      for (int q = 0; q < Ln->npart[nc]; ++q) {
+        p_vel(&T->P, pl[q], pos, t);
+        for (int e = 0; e < 3; ++e) vel[e] = vel[e] + t[e] / fpi;
+      }
+    }
+    const int *pl = &L->plist[L->pstart[c]];
+    for (int q = 0; q < L->npart[c]; ++q) {
+      p_vel(&T->P, pl[q], pos, t);
+      for (int e = 0; e < 3; ++e) vel[e] = vel[e] + t[e] / fpi;
+    }
+    for (int e = 0; e < 3; ++e) uvort[3 * ip + e] = uvort[3 * ip + e] + vel[e];
+  }
+  return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* Introspection used by the tests                                     */
 /* ------------------------------------------------------------------ */

@@ -42,6 +42,7 @@ def lib():
             "orc_octree_nleaves": (C.c_int, [_vp]),
             "orc_octree_leaves": (None, [_vp, _vp, _vp, _vp, _vp, _vp]),
             "orc_octree_work": (None, [_vp, _vp, _vp]),
+            "orc_apply_multipole_panels": (C.c_int, [_vp, C.c_int, _vp, _vp]),
             "orc_octree_cell_mp": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
             "orc_p2m": (None, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp]),
             "orc_m2m": (None, [C.c_int, _vp, _vp, _vp, _vp]),
@@ -174,6 +175,15 @@ class Octree:
         tv = np.zeros(n)
         lib().orc_apply_multipole(self._h, _p(vel), _p(st), _p(tv))
         return vel, st, tv
+
+    def apply_panels(self, cen, uvort=None):
+        """apply_multipole_panels at the centres cen (m,3); uvort accumulated."""
+        cen = _c(cen).reshape(-1, 3)
+        u = np.zeros((len(cen), 3)) if uvort is None else np.array(uvort, dtype=np.float64)
+        rc = lib().orc_apply_multipole_panels(self._h, len(cen), _p(cen), _p(u))
+        if rc != 0:
+            raise ValueError(f"panel {-1 - rc} outside the octree")
+        return u
 
     def run(self, cen, dir, mag, rv):
         self.sort(cen, dir, mag, rv)

@@ -13,9 +13,11 @@ from . import oracle as orc
 class OracleStep:
     """The same step with the CPU oracle, on host copies of the pool."""
 
-    def __init__(self, case, te, nmax, u_inf, dt, sub=1, r_shed=0.1, omega=0.0, strips=None):
+    def __init__(self, case, te, nmax, u_inf, dt, sub=1, r_shed=0.1, omega=0.0, strips=None,
+                 fmm_pan=False):
         ver, nv, cen, alpha, rv, tree = case
         self.omega, self.theta, self.strips, self.sub = float(omega), 0.0, strips, sub
+        self.fmm_pan = fmm_pan
         v4 = np.zeros((len(nv), 4, 3))
         v4[:, : np.asarray(ver).shape[1]] = np.asarray(ver)[:, :4]
         self.ver0 = v4
@@ -72,14 +74,22 @@ class OracleStep:
         live = orc.wake_live(P["free"])
         pc, pd, pm, pr = P["cen"][live], P["dir"][live], P["mag"][live], P["rv"][live]
         ub = self._move() if self.omega != 0.0 else np.zeros_like(self.cen_p)
-        uvort = orc.vort_vel(self.cen_p, pc, pd, pm, pr)
-        b = orc.rhs_static(self.G, self.B, self.u, uvort - ub)
-        mu = orc.dgetrs(self.LU, self.ipiv, b)
-        sigma = np.einsum("ij,ij->i", self.nor, ub - self.u[None] - uvort)
         t = self.tree
         fmm = orc.Octree(t["box_length"], t["nbox"], t["origin"], t["nlevels"], t["min_part"], 4,
                          t["delta"], u_inf=tuple(self.u))
-        vel, st, _ = fmm.run(pc, pd, pm, pr)
+        if self.fmm_pan:
+            fmm.sort(pc, pd, pm, pr)
+            fmm.multipole()
+            uvort = fmm.apply_panels(self.cen_p)
+        else:
+            uvort = orc.vort_vel(self.cen_p, pc, pd, pm, pr)
+        b = orc.rhs_static(self.G, self.B, self.u, uvort - ub)
+        mu = orc.dgetrs(self.LU, self.ipiv, b)
+        sigma = np.einsum("ij,ij->i", self.nor, ub - self.u[None] - uvort)
+        if self.fmm_pan:
+            vel, st, _ = fmm.apply()
+        else:
+            vel, st, _ = fmm.run(pc, pd, pm, pr)
         grp = [dict(kind=orc.SURFPAN, ver=self.ver, n_ver=self.nv, mag=mu, sigma=sigma)]
         vel, _ = orc.elem_apply(grp, pc, vel=vel)
         P["vel"][live] = vel

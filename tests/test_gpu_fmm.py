@@ -298,3 +298,34 @@ def test_autotune(precision):
     tol_v, tol_s = (1e-13, 1e-13) if precision == "fp64" else (5e-7, 5e-6)
     assert rel(v, v0) <= tol_v
     assert rel(s, s0) <= tol_s
+
+
+@pytest.mark.parametrize("form", ["fp64", "tuned"])
+def test_apply_multipole_panels_vs_oracle(orc, form):
+    """apply_multipole_panels (use_fmm_pan) at panel-like points in and around
+    the helix wake, after a pass: FP64 form to 1e-10, tuned form to 1e-6 of
+    the oracle; uvort is accumulated; host and device entry points agree."""
+    import torch
+    cen, alpha, rv, tree = W.helix_wake(20000, seed=12)
+    tree = dict(tree, nlevels=4)
+    parts, g, nl, c, v, s, tv = run_both(orc, cen, alpha, rv, tree, 4, use_vs=True,
+                                         precision=form)
+    rng = np.random.default_rng(13)
+    pts = np.clip(cen[rng.choice(len(cen), 3000, replace=False)] +
+                  0.02 * rng.standard_normal((3000, 3)), -1.999, 1.999)
+    pts = np.concatenate([pts, rng.uniform(-1.999, 1.999, (500, 3))])
+    u0 = rng.standard_normal(pts.shape)
+    print(f"apply_multipole_panels form={form} m={len(pts)}", flush=True)
+    u = O.apply_multipole_panels(g, pts, u0)
+    uo = c.apply_panels(pts, u0)
+    tol = TOL if form == "fp64" else 1e-6
+    assert rel(u - u0, uo - u0) <= tol
+    tp = torch.from_numpy(np.ascontiguousarray(pts)).cuda()
+    tu = torch.from_numpy(np.ascontiguousarray(u0)).cuda()
+    L.check(L.lib().dust_fmm_apply_multipole_panels_dev(g.handle, len(pts), tp.data_ptr(),
+                                                        tu.data_ptr()), "panels_dev")
+    L.check(L.lib().dust_fmm_synchronize(g.handle), "sync")
+    assert np.array_equal(tu.cpu().numpy(), u)
+    with pytest.raises(L.DustError) as ei:
+        O.apply_multipole_panels(g, np.array([[5.0, 0.0, 0.0]]))
+    assert ei.value.code == L.DUST_E_OUTSIDE

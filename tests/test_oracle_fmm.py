@@ -293,3 +293,61 @@ def test_fmm_converges_with_order(orc):
         v, s, _ = c.run(cen, dr, mag, rv)
         ev.append(np.linalg.norm(v - vd) / np.linalg.norm(vd))
     assert ev[1] < 0.3 * ev[0] and ev[2] < 0.3 * ev[1] and ev[2] < 1e-4, ev
+
+
+# ---- apply_multipole_panels (mod_octree.f90:1434-1510) ------------------------
+def _panel_points(rng, cen, m, lo=-2.0, hi=2.0):
+    """Points inside the particle cloud (near particles, so inside leaves),
+    kept inside the box [lo, hi]^3."""
+    p = cen[rng.choice(len(cen), m, replace=False)] + 0.01 * rng.standard_normal((m, 3))
+    return np.clip(p, lo + 1e-3, hi - 1e-3)
+
+
+@pytest.mark.parametrize("nlev", [1, 2])
+def test_panels_without_far_field_equal_direct(orc, nlev):
+    """One or two levels: every panel's leaf sees all particles through its
+    neighbour lists, so the result is the direct get_vort_vel sum (plus the
+    uvort it started from)."""
+    d = np.load(os.path.join(HERE, "golden", "reg_test_b_particles.npz"))
+    cen, alpha = d["cen"][:1500], d["alpha"][:1500]
+    mag = np.linalg.norm(alpha, axis=1)
+    dr = alpha / mag[:, None]
+    rv = np.full(len(mag), 0.05)
+    c = orc.Octree(6.0, (1, 1, 1), (-3.0, -3.0, -5.0), nlev, 5, 2, 0.005, use_vs=True)
+    c.run(cen, dr, mag, rv)
+    pts = _panel_points(np.random.default_rng(2), cen, 200, -3.0, 1.0)
+    u0 = np.random.default_rng(3).standard_normal((200, 3))
+    u = c.apply_panels(pts, u0)
+    ud = orc.vort_vel(pts, cen, dr, mag, rv)
+    assert np.linalg.norm(u - u0 - ud) / np.linalg.norm(ud) < 1e-13
+
+
+def test_panels_converge_with_order(orc):
+    cen, alpha, rv, tree = W.helix_wake(8000, seed=4)
+    mag = np.linalg.norm(alpha, axis=1)
+    dr = alpha / mag[:, None]
+    pts = _panel_points(np.random.default_rng(5), cen, 300)
+    ud = orc.vort_vel(pts, cen, dr, mag, rv)
+    ev = []
+    for p in (2, 4, 6):
+        c = orc.Octree(tree["box_length"], tree["nbox"], tree["origin"], 4, 64, p, tree["delta"])
+        c.run(cen, dr, mag, rv)
+        ev.append(np.linalg.norm(c.apply_panels(pts) - ud) / np.linalg.norm(ud))
+    assert ev[1] < 0.3 * ev[0] and ev[2] < 0.3 * ev[1] and ev[2] < 1e-4, ev
+
+
+def test_panels_outside_the_box(orc):
+    """A centre outside the box is an error (use_fmm_pan, mod_octree.f90:785-788);
+    inside, every point lies under one leaf (a level-1 cell whose children are
+    all inactive is itself a leaf, :953-957), so a point far from the
+    particles still receives their far field."""
+    cen, alpha, rv, tree = W.helix_wake(3000, seed=6)
+    mag = np.linalg.norm(alpha, axis=1)
+    c = orc.Octree(tree["box_length"], tree["nbox"], tree["origin"], 5, 64, 4, tree["delta"])
+    c.run(cen, alpha / mag[:, None], mag, rv)
+    with pytest.raises(ValueError):
+        c.apply_panels(np.array([[100.0, 0.0, 0.0]]))
+    corner = np.asarray(tree["origin"]) + 1e-3
+    u = c.apply_panels(corner[None])
+    ud = orc.vort_vel(corner[None], cen, alpha / mag[:, None], mag, rv)
+    assert np.linalg.norm(u - ud) < 0.1 * np.linalg.norm(ud)  # p = 4 at a leaf corner
