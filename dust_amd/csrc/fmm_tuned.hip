@@ -421,8 +421,8 @@ __device__ inline void near64(double r0, double r1, double r2, double al0, doubl
 }
 
 // FP32 P2P of one item (tuned form).  Sources are staged per wavefront as
-// FP32 records (offset from the target leaf centre, r^2; alpha = dir*mag)
-// and FP64 records (offset, alpha, r).  Pairs with 0 < d^2 <= near2 r_s^2
+// FP32 records (offset from the target leaf centre, r^2; alpha = dir*mag,
+// near2 r^2) and FP64 records (offset, alpha, r).  Pairs with 0 < d^2 <= near2 r_s^2
 // (inside the source core) are left out of the FP32 sums and evaluated in
 // FP64 after the tile, all lanes together (a per-lane mask of tile
 // positions), so the rare FP64 work does not diverge the FP32 loop.  The
@@ -456,7 +456,7 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
   const float fx = (float)xr, fy = (float)yr, fz = (float)zr;
   const float fa0 = (float)al0, fa1 = (float)al1, fa2 = (float)al2;
   double pv[3] = {0, 0, 0}, ps[3] = {0, 0, 0};
-  const float near2 = (float)A.near2;
+  const double near2d = A.near2;  // staged per source as near2 r_s^2
   const int nr = A.cnt[U.leaf];
   const int2 *rl = &A.rng[(long)U.leaf * A.cap];
   int ri, off;
@@ -473,7 +473,7 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
         const double ox = A.px[q] - cx, oy = A.py[q] - cy, oz = A.pz[q] - cz;
         const double a0 = A.dx[q] * smg, a1 = A.dy[q] * smg, a2 = A.dz[q] * smg;
         ss[2 * lane] = (f4){(float)ox, (float)oy, (float)oz, (float)(srv * srv)};
-        ss[2 * lane + 1] = (f4){(float)a0, (float)a1, (float)a2, 0.f};
+        ss[2 * lane + 1] = (f4){(float)a0, (float)a1, (float)a2, (float)(near2d * srv * srv)};
         double *d = sd + 8 * lane;
         d[0] = ox; d[1] = oy; d[2] = oz; d[3] = srv;
         d[4] = a0; d[5] = a1; d[6] = a2;
@@ -492,7 +492,7 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
         const float a0 = sa[0], a1 = sa[1], a2 = sa[2];
         const float d2 = r0 * r0 + r1 * r1 + r2 * r2;
         // pairs inside the source core go to FP64; d = 0 is the self pair
-        const bool skip = d2 <= near2 * sp[3];
+        const bool skip = d2 <= sa[3];
         if (__builtin_expect(__any(skip), 0)) {
           if (skip && d2 > 0.f) nmask |= 1ull << s;
         }
