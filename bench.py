@@ -380,11 +380,24 @@ def time_elements(torch, L, W, n, steps, with_cpu=True):
     ms_u = timed(uvort, steps)
     ms_v = timed(lambda: apply(False), steps)
     ms_s = timed(lambda: apply(True), max(1, steps // 2))
+    prm.precision = L.ELEM_TUNED
+    t_vel.zero_()
+    ms_vt = timed(lambda: apply(False), steps)
+    t_vel.zero_()
+    apply(False)
+    torch.cuda.synchronize()
+    prm.precision = L.ELEM_FP64
+    vt = t_vel.clone()
+    t_vel.zero_()
+    apply(False)
+    torch.cuda.synchronize()
+    tuned_err = float((vt - t_vel).norm() / t_vel.norm())
     out = dict(panels=len(pc), elements=ne, particles=n, end_vortices=len(lm),
                uvort_ms=ms_u, uvort_pairs_per_s=len(pc) * n / (ms_u / 1e3),
                uvort_TFLOPs=UVORT_FLOP * len(pc) * n / (ms_u / 1e3) / 1e12,
                apply_vel_ms=ms_v, apply_vel_pairs_per_s=ne * n / (ms_v / 1e3),
-               apply_vel_stretch_ms=ms_s, dtype="f64")
+               apply_vel_stretch_ms=ms_s, apply_vel_tuned_ms=ms_vt,
+               apply_vel_tuned_rel_l2_vs_fp64=tuned_err, dtype="f64 (tuned: f32 far field)")
     if with_cpu:
         from oracle import oracle as orc
         ns = 4000

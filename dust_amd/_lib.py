@@ -62,6 +62,8 @@ class AicParams(C.Structure):
 ELEM_SURFPAN = 0
 ELEM_RING = 1
 ELEM_LINE = 2
+ELEM_FP64 = 0
+ELEM_TUNED = 1
 
 
 class ElemGroup(C.Structure):
@@ -83,7 +85,8 @@ class ElemParams(C.Structure):
         ("rankine_rad", C.c_double),
         ("cutoff_rad", C.c_double),
         ("vortex_rad", C.c_double),
-        ("reserved", C.c_int * 4),
+        ("precision", C.c_int),
+        ("reserved", C.c_int * 3),
     ]
 
 

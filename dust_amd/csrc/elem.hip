@@ -202,19 +202,32 @@ __device__ inline void elem_gradT(const ElemRec &E, const double pos[3], double 
   }
 }
 
+// sqrt(r2) > T, as the reference decides its far-field switches (radius >
+// ratio * max edge, with radius = norm2 rounded once): outside a relative
+// band of 1e-14 around T^2 the answer follows from r2 and the rounded T^2
+// alone (the roundings of the square root and of T^2 move it by less than
+// 1e-15); inside the band the correctly rounded square root decides.
+__device__ inline bool beyond(double r2, double T) {
+  const double T2 = T * T;
+  if (r2 > T2 * (1.0 + 1e-14)) return true;
+  if (r2 < T2 * (1.0 - 1e-14)) return false;
+  return sqrt(r2) > T;
+}
+
 // One thread per target point; every thread walks the whole element array
 // (uniform loads of the records).  Far from a panel (beyond both far-field
 // ratios) the doublet and source far fields share one square root and one
 // reciprocal; otherwise the full near-field kernels run.  A wave runs the
 // near-field code for an element if any of its lanes needs it, so the targets
 // are given in Morton order (spatially close targets share waves).
+template <int TUNED>
 __global__ void __launch_bounds__(256)
     k_elem_apply(int ne, int per, const ElemRec *__restrict__ R, double ffr, double rran,
                  double rcut,
                  double del2, int n, const double *__restrict__ pos,
                  const double *__restrict__ alpha, double *vel, double *stretch, int acc,
                  double *part) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
   const bool do_st = stretch != nullptr;
@@ -237,32 +250,58 @@ __global__ void __launch_bounds__(256)
       const Panel &g = E.g;
       double rv[3] = {x[0] - g.cen[0], x[1] - g.cen[1], x[2] - g.cen[2]};
       double r2 = dot3_rn(rv, rv);
-      double radius = sqrt(r2);
-      const bool far_d = radius > ffr * g.maxedge;       // velocity_calc_doublet switch
-      const bool far_s = radius > 10.0 * g.maxedge;      // velocity_calc_sou_surfpan switch
+      const bool far_d = beyond(r2, ffr * g.maxedge);   // velocity_calc_doublet switch
+      const bool far_s = beyond(r2, 10.0 * g.maxedge);  // velocity_calc_sou_surfpan switch
       const bool src = flags & kFlagSource;
       if (far_d && (far_s || !src)) {
-        // both far-field formulas with one reciprocal: doublet
+        // both far-field formulas with one reciprocal square root: doublet
         //   t1 phix + t2 phiy + nor pdou, rati = 3 area (r.n)/r^5,
         //   phix = rati (r.t1), phiy = rati (r.t2), pdou = area (3 (r.n)^2 - r^2)/r^5;
         // source: the same basis with -area (r.t1, r.t2, r.n)/r^3
-        double ir = 1.0 / radius;
-        double ir3 = ir * ir * ir, ir5 = ir3 * ir * ir;
-        double rn = dot3(rv, g.nor), r1 = dot3(rv, g.t1), rt2 = dot3(rv, g.t2);
-        double cx = 0.0, cy = 0.0, cz = 0.0;
-        if (flags & kFlagDoublet) {
-          double rati = 3.0 * g.area * rn * ir5;
-          cx = rati * r1 * m;
-          cy = rati * rt2 * m;
-          cz = g.area * (3.0 * rn * rn - r2) * ir5 * m;
+        if (TUNED) {  // FP32 formulas on the FP64 offset, FP64 sums
+          const float fr0 = (float)rv[0], fr1 = (float)rv[1], fr2 = (float)rv[2];
+          const float fr2s = (float)r2;
+          const float ir = __builtin_amdgcn_rsqf(fr2s);
+          const float ir3 = ir * ir * ir, ir5 = ir3 * ir * ir;
+          const float fa = (float)g.area;
+          const float rn = fr0 * (float)g.nor[0] + fr1 * (float)g.nor[1] + fr2 * (float)g.nor[2];
+          const float r1 = fr0 * (float)g.t1[0] + fr1 * (float)g.t1[1] + fr2 * (float)g.t1[2];
+          const float rt = fr0 * (float)g.t2[0] + fr1 * (float)g.t2[1] + fr2 * (float)g.t2[2];
+          float cx = 0.f, cy = 0.f, cz = 0.f;
+          if (flags & kFlagDoublet) {
+            const float fm = (float)m;
+            const float rati = 3.f * fa * rn * ir5 * fm;
+            cx = rati * r1;
+            cy = rati * rt;
+            cz = fa * (3.f * rn * rn - fr2s) * ir5 * fm;
+          }
+          if (src) {
+            const float fs = fa * ir3 * (float)E.sigma;
+            cx += fs * r1;
+            cy += fs * rt;
+            cz += fs * rn;
+          }
+          for (int e = 0; e < 3; ++e)
+            w[e] = (double)((float)g.t1[e] * cx + (float)g.t2[e] * cy + (float)g.nor[e] * cz);
+        } else {
+          double ir = rsqrt64(r2);
+          double ir3 = ir * ir * ir, ir5 = ir3 * ir * ir;
+          double rn = dot3(rv, g.nor), r1 = dot3(rv, g.t1), rt2 = dot3(rv, g.t2);
+          double cx = 0.0, cy = 0.0, cz = 0.0;
+          if (flags & kFlagDoublet) {
+            double rati = 3.0 * g.area * rn * ir5;
+            cx = rati * r1 * m;
+            cy = rati * rt2 * m;
+            cz = g.area * (3.0 * rn * rn - r2) * ir5 * m;
+          }
+          if (src) {
+            double fs = g.area * ir3 * E.sigma;  // minus the source velocity times sigma
+            cx += fs * r1;
+            cy += fs * rt2;
+            cz += fs * rn;
+          }
+          for (int e = 0; e < 3; ++e) w[e] = g.t1[e] * cx + g.t2[e] * cy + g.nor[e] * cz;
         }
-        if (src) {
-          double fs = g.area * ir3 * E.sigma;  // minus the source velocity times sigma
-          cx += fs * r1;
-          cy += fs * rt2;
-          cz += fs * rn;
-        }
-        for (int e = 0; e < 3; ++e) w[e] = g.t1[e] * cx + g.t2[e] * cy + g.nor[e] * cz;
       } else {
         if (flags & kFlagDoublet) vel_doublet(g, x, ffr, rran, rcut, w);
         for (int e = 0; e < 3; ++e) w[e] *= m;
@@ -557,9 +596,16 @@ int elem_apply_dev(int ngroups, const dust_elem_group *groups, const dust_elem_p
       DUST_TRY(g_ws.epart.reserve(6 * (size_t)S * n));
       part = g_ws.epart.p;
     }
-    k_elem_apply<<<dim3(tb, S), 256, 0, st>>>((int)ne, per, g_ws.rec.p, prm->ff_ratio_doublet,
-                                             prm->rankine_rad, prm->cutoff_rad, del2, n, tp, ta,
-                                             vo, so, acc, part);
+    if (prm->precision == DUST_ELEM_TUNED)
+      k_elem_apply<1><<<dim3(tb, S), 256, 0, st>>>((int)ne, per, g_ws.rec.p,
+                                                  prm->ff_ratio_doublet, prm->rankine_rad,
+                                                  prm->cutoff_rad, del2, n, tp, ta, vo, so, acc,
+                                                  part);
+    else
+      k_elem_apply<0><<<dim3(tb, S), 256, 0, st>>>((int)ne, per, g_ws.rec.p,
+                                                  prm->ff_ratio_doublet, prm->rankine_rad,
+                                                  prm->cutoff_rad, del2, n, tp, ta, vo, so, acc,
+                                                  part);
     if (S > 1) k_elem_reduce<<<tb, 256, 0, st>>>(n, S, part, vo, so, acc);
     DUST_HIP(hipGetLastError());
     return 0;

@@ -27,8 +27,10 @@ from . import _lib as L
 SURFPAN, RING, LINE = L.ELEM_SURFPAN, L.ELEM_RING, L.ELEM_LINE
 
 
-def params(ff_ratio_doublet=10.0, rankine_rad=0.1, cutoff_rad=0.001, vortex_rad=0.1):
+def params(ff_ratio_doublet=10.0, rankine_rad=0.1, cutoff_rad=0.001, vortex_rad=0.1,
+           precision="fp64"):
     p = L.ElemParams()
+    p.precision = {"fp64": L.ELEM_FP64, "tuned": L.ELEM_TUNED}[precision]
     p.ff_ratio_doublet = float(ff_ratio_doublet)
     p.rankine_rad = float(rankine_rad)
     p.cutoff_rad = float(cutoff_rad)

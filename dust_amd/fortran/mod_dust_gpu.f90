@@ -42,7 +42,7 @@ module mod_dust_gpu
 
   type, bind(C) :: t_dust_elem_params
     real(c_double) :: ff_ratio_doublet, rankine_rad, cutoff_rad, vortex_rad
-    integer(c_int) :: reserved(4)
+    integer(c_int) :: precision, reserved(3)
   end type
 
   interface

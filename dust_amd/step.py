@@ -70,7 +70,9 @@ class BodyWakeStep:
         self.vs_elems = bool(vs_elems)
         self.aic = dict(ff_ratio_doublet=10.0, rankine_rad=0.1, cutoff_rad=0.001)
         self.aic.update(aic or {})
-        self.eprm = E.params(vortex_rad=r_shed, **self.aic)
+        # the tuned form also evaluates the element far fields in FP32
+        self.eprm = E.params(vortex_rad=r_shed, precision="tuned" if form == "tuned" else "fp64",
+                             **self.aic)
         dv = lambda a, dt=np.float64: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).cuda()
         v4 = np.zeros((self.n, 4, 3))
         v4[:, : np.asarray(ver).shape[1]] = np.asarray(ver)[:, :4]

@@ -215,8 +215,13 @@ typedef struct {
   double rankine_rad;      /* sim_param%RankineRad                                  */
   double cutoff_rad;       /* sim_param%CutoffRad                                   */
   double vortex_rad;       /* sim_param%VortexRad, core of gradient_calc_doublet    */
-  int reserved[4];
+  int precision;           /* DUST_ELEM_FP64 (the drop-in) or DUST_ELEM_TUNED:      */
+                           /* far-field element velocities in FP32 (FP64 offsets,  */
+                           /* switches and sums; near field and stretching FP64)   */
+  int reserved[3];
 } dust_elem_params;
+#define DUST_ELEM_FP64 0
+#define DUST_ELEM_TUNED 1
 
 /* The element loops of apply_multipole (mod_octree.f90:1350-1401) at n
  * points pos(3,n): vel(3,n) += sum over the groups, in order, of

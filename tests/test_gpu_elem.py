@@ -193,3 +193,18 @@ def test_morton_order_path_matches_input_order(wing):
     vs, ss = E.apply_elements(groups, big[:4000], alpha=al[:4000])   # below the threshold
     assert np.allclose(vb[:4000], vs, rtol=1e-13, atol=1e-13 * np.abs(vs).max())
     assert np.allclose(sb[:4000], ss, rtol=1e-13, atol=1e-13 * np.abs(ss).max())
+
+
+def test_tuned_far_field_vs_oracle(orc, wing):
+    """precision = tuned: far-field element velocities in FP32 (FP64 offsets,
+    switches and sums), near field FP64: within 1e-6 of the oracle, and the
+    near-field-only case (every pair near) equal to the FP64 form."""
+    groups, pts, al = wing
+    print(f"elem_apply tuned n={len(pts)}", flush=True)
+    vel, _ = E.apply_elements(groups, pts, precision="tuned")
+    ov, _ = orc.elem_apply(oracle_groups(groups), pts)
+    print(f"tuned vel rel {rel(vel, ov):.2e}")
+    assert rel(vel, ov) <= 1e-6
+    vn, _ = E.apply_elements(groups, pts, precision="tuned", ff_ratio_doublet=1e9)
+    v64, _ = E.apply_elements(groups, pts, ff_ratio_doublet=1e9)
+    assert np.array_equal(vn, v64)
