@@ -39,15 +39,54 @@ constexpr int kFlagDoublet = 1, kFlagSource = 2, kFlagStretch = 4;
 constexpr int kSortMin = 4096;  // targets below this count are evaluated in input order
 constexpr int kFillBlocks = 2048, kMinPerSplit = 32;  // element splits for few targets
 
+// The far fields of a polygon, doublet plus source, as one formula: with
+// r = x - cen, rn = r.n and the PANAIR basis t1, t2, n (t1 need not be normal
+// to n on a warped panel, so the basis is not orthonormal),
+//   t1 phix + t2 phiy + n pdou - sigma (t1 sx + t2 sy + n sz)
+//     = (3 A m rn / r^5 + A sigma / r^3) P r - (A m / r^3) n,
+//   P = t1 t1^T + t2 t2^T + n n^T  (symmetric: xx yy zz xy xz yz),
+// and the two far-field switches with their thresholds squared.
+struct FarCoef {
+  double cen[3], nor[3];
+  double P[6];
+  double am3, am, as;       // 3 A m, A m, A sigma (zero without the term)
+  double td, td_lo, td_hi;  // doublet switch: ffr maxedge, its square -/+ a band
+  double ts, ts_lo, ts_hi;  // source switch: 10 maxedge, likewise
+};
+
+struct FarCoef32 {  // the same in FP32 for the tuned form
+  float nor[3], P[6], am3, am, as;
+};
+
 struct ElemRec {
+  FarCoef f;
+  FarCoef32 h;
   Panel g;  // polygons: calc_geo_data; lines: ver[0..1], evec/euni/elen[0], cen
+  double sinT[4], cosT[4];  // edge directions on the PANAIR basis: euni.t2, euni.t1
   double mag, sigma;
   int kind, flags;
 };
 
+// sqrt(r2) > T, as the reference decides its far-field switches (radius >
+// ratio * max edge, with radius = norm2 rounded once): outside a relative
+// band of 1e-14 around T^2 the answer follows from r2 and the rounded T^2
+// alone (the roundings of the square root and of T^2 move it by less than
+// 1e-15); inside the band the correctly rounded square root decides.
+__device__ inline void switch_band(double T, double &lo, double &hi) {
+  const double T2 = T * T;
+  lo = T2 * (1.0 - 1e-14);
+  hi = T2 * (1.0 + 1e-14);
+}
+__device__ inline bool beyond(double r2, double T, double lo, double hi) {
+  if (r2 > hi) return true;
+  if (r2 < lo) return false;
+  return sqrt(r2) > T;
+}
+
 __global__ void k_elem_recs(int n, int kind, const int *__restrict__ nver,
                             const double *__restrict__ ver, const double *__restrict__ mag,
-                            const double *__restrict__ sigma, int stretch, ElemRec *R) {
+                            const double *__restrict__ sigma, int stretch, double ffr,
+                            ElemRec *R) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   ElemRec r{};
@@ -73,6 +112,36 @@ __global__ void k_elem_recs(int n, int kind, const int *__restrict__ nver,
   r.kind = kind;
   r.flags = (mag ? kFlagDoublet : 0) | (sigma && kind == DUST_ELEM_SURFPAN ? kFlagSource : 0) |
             (stretch ? kFlagStretch : 0);
+  FarCoef &f = r.f;
+  const Panel &g = r.g;
+  for (int i = 0; i < 4; ++i) {
+    r.sinT[i] = dot3(g.euni[i], g.t2);
+    r.cosT[i] = dot3(g.euni[i], g.t1);
+  }
+  for (int e = 0; e < 3; ++e) {
+    f.cen[e] = g.cen[e];
+    f.nor[e] = g.nor[e];
+  }
+  const double *b[3] = {g.t1, g.t2, g.nor};
+  const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+  for (int q = 0; q < 6; ++q) {
+    double acc = 0.0;
+    for (int k = 0; k < 3; ++k) acc += b[k][ia[q]] * b[k][ib[q]];
+    f.P[q] = acc;
+  }
+  const double am = (r.flags & kFlagDoublet) ? g.area * r.mag : 0.0;
+  f.am3 = 3.0 * am;
+  f.am = am;
+  f.as = (r.flags & kFlagSource) ? g.area * r.sigma : 0.0;
+  f.td = ffr * g.maxedge;
+  switch_band(f.td, f.td_lo, f.td_hi);
+  f.ts = 10.0 * g.maxedge;
+  switch_band(f.ts, f.ts_lo, f.ts_hi);
+  for (int e = 0; e < 3; ++e) r.h.nor[e] = (float)f.nor[e];
+  for (int q = 0; q < 6; ++q) r.h.P[q] = (float)f.P[q];
+  r.h.am3 = (float)f.am3;
+  r.h.am = (float)f.am;
+  r.h.as = (float)f.as;
   R[j] = r;
 }
 
@@ -97,43 +166,6 @@ __device__ inline void line_vel(const double *x1, const double *x2, const double
   else if (R1 > el * rcut && R2 > el * rcut)
     f = ((el - ai) / R2 + ai / R1) / ((rran * el) * (rran * el));
   for (int e = 0; e < 3; ++e) v[e] = f * cr[e];
-}
-
-// velocity_calc_sou_surfpan (mod_surfpan.f90:268-357); its own far-field
-// ratio is the local parameter 10, the doublet potential of its near field
-// uses FarFieldRatioDoublet.
-__device__ inline void vel_source(const Panel &g, const double pos[3], double ffr_dou,
-                                  double v[3]) {
-  double rv[3] = {pos[0] - g.cen[0], pos[1] - g.cen[1], pos[2] - g.cen[2]};
-  double radius = sqrt(dot3_rn(rv, rv));
-  double phix, phiy, pdou;
-  if (radius > 10.0 * g.maxedge) {
-    double r3 = radius * radius * radius;
-    phix = -g.area * dot3(rv, g.t1) / r3;
-    phiy = -g.area * dot3(rv, g.t2) / r3;
-    pdou = -g.area * dot3(rv, g.nor) / r3;
-  } else {
-    phix = 0.0;
-    phiy = 0.0;
-    const int nv = g.nv;
-    for (int i1 = 0; i1 < nv; ++i1) {
-      int ip1 = (i1 + 1 < nv) ? i1 + 1 : 0;
-      double a[3], b[3];
-      for (int e = 0; e < 3; ++e) {
-        a[e] = pos[e] - g.verp[i1][e];
-        b[e] = pos[e] - g.verp[ip1][e];
-      }
-      double R1 = sqrt(dot3(a, a)), R2 = sqrt(dot3(b, b)), el = g.elen[i1];
-      double sl = (R1 + R2 - el < 1e-12) ? 0.0 : log((R1 + R2 + el) / (R1 + R2 - el));
-      phix += dot3(g.euni[i1], g.t2) * sl;  // sinTi
-      phiy -= dot3(g.euni[i1], g.t1) * sl;  // cosTi
-    }
-    pdou = pot_doublet(g, pos, ffr_dou);
-    phix = -phix;
-    phiy = -phiy;
-    pdou = -pdou;
-  }
-  for (int e = 0; e < 3; ++e) v[e] = g.t1[e] * phix + g.t2[e] * phiy + g.nor[e] * pdou;
 }
 
 // 1/sqrt(x) in FP64: the hardware estimate and two Newton steps.
@@ -202,136 +234,294 @@ __device__ inline void elem_gradT(const ElemRec &E, const double pos[3], double 
   }
 }
 
-// sqrt(r2) > T, as the reference decides its far-field switches (radius >
-// ratio * max edge, with radius = norm2 rounded once): outside a relative
-// band of 1e-14 around T^2 the answer follows from r2 and the rounded T^2
-// alone (the roundings of the square root and of T^2 move it by less than
-// 1e-15); inside the band the correctly rounded square root decides.
-__device__ inline bool beyond(double r2, double T) {
-  const double T2 = T * T;
-  if (r2 > T2 * (1.0 + 1e-14)) return true;
-  if (r2 < T2 * (1.0 - 1e-14)) return false;
-  return sqrt(r2) > T;
+// The far-field test of a polygon at x: r = x - cen, r2 = |r|^2 rounded as
+// the reference's norm2, and true when both far fields apply (doublet beyond
+// its ratio, and the source -- if any -- beyond its own).
+__device__ inline bool far_pair(const FarCoef &f, int flags, const double x[3], double rv[3],
+                                double &r2) {
+  for (int e = 0; e < 3; ++e) rv[e] = x[e] - f.cen[e];
+  r2 = dot3_rn(rv, rv);
+  const bool far_d = beyond(r2, f.td, f.td_lo, f.td_hi);  // velocity_calc_doublet switch
+  const bool far_s = beyond(r2, f.ts, f.ts_lo, f.ts_hi);  // velocity_calc_sou_surfpan switch
+  return far_d && (far_s || !(flags & kFlagSource));
 }
 
-// One thread per target point; every thread walks the whole element array
-// (uniform loads of the records).  Far from a panel (beyond both far-field
-// ratios) the doublet and source far fields share one square root and one
-// reciprocal; otherwise the full near-field kernels run.  A wave runs the
-// near-field code for an element if any of its lanes needs it, so the targets
-// are given in Morton order (spatially close targets share waves).
+// Both far fields with one reciprocal square root:
+//   w = (3 A m rn / r^5 + A sigma / r^3) P r - (A m / r^3) n.
 template <int TUNED>
+__device__ inline void far_vel(const ElemRec &E, const double rv[3], double r2, double w[3]) {
+  if (TUNED) {  // FP32 on the FP64 offset, FP64 sums
+    const FarCoef32 &h = E.h;
+    const float r0 = (float)rv[0], r1 = (float)rv[1], rz = (float)rv[2];
+    const float ir = __builtin_amdgcn_rsqf((float)r2);
+    const float ir2 = ir * ir, ir3 = ir2 * ir;
+    const float rn = r0 * h.nor[0] + r1 * h.nor[1] + rz * h.nor[2];
+    const float a = (h.am3 * rn * ir2 + h.as) * ir3, b = h.am * ir3;
+    const float p0 = h.P[0] * r0 + h.P[3] * r1 + h.P[4] * rz;
+    const float p1 = h.P[3] * r0 + h.P[1] * r1 + h.P[5] * rz;
+    const float p2 = h.P[4] * r0 + h.P[5] * r1 + h.P[2] * rz;
+    w[0] = (double)(a * p0 - b * h.nor[0]);
+    w[1] = (double)(a * p1 - b * h.nor[1]);
+    w[2] = (double)(a * p2 - b * h.nor[2]);
+  } else {
+    const FarCoef &f = E.f;
+    const double ir = rsqrt64(r2);
+    const double ir2 = ir * ir, ir3 = ir2 * ir;
+    const double rn = dot3(rv, f.nor);
+    const double a = (f.am3 * rn * ir2 + f.as) * ir3, b = f.am * ir3;
+    const double p0 = f.P[0] * rv[0] + f.P[3] * rv[1] + f.P[4] * rv[2];
+    const double p1 = f.P[3] * rv[0] + f.P[1] * rv[1] + f.P[5] * rv[2];
+    const double p2 = f.P[4] * rv[0] + f.P[5] * rv[1] + f.P[2] * rv[2];
+    w[0] = a * p0 - b * f.nor[0];
+    w[1] = a * p1 - b * f.nor[1];
+    w[2] = a * p2 - b * f.nor[2];
+  }
+}
+
+// The full kernels of a polygon (near field of either term), times strength:
+// velocity_calc_doublet (mod_doublet.f90:178-252: the ring of the edges with
+// the Rankine core and the cut-off) times mag, minus velocity_calc_sou_surfpan
+// (mod_surfpan.f90:268-357: edge logarithms on the projected vertices and the
+// solid angle of potential_calc_doublet, mod_doublet.f90:100-176) times sigma,
+// in one pass:
+//  * the distances R_i to the vertices are taken once and serve the ring's
+//    two edges at each vertex and the solid angle (the reference recomputes
+//    them per edge and per routine, with the same expression);
+//  * the ring edge factor (el - a)/R_2 + a/R_1 uses the reciprocals 1/R_i;
+//  * the Rankine switch hi > el rran compares squares (the two branches
+//    agree at the switch);
+//  * each solid-angle term is atan2(u.(m x p), m.p), the reference's
+//    atan2(sinB, cosB) without the common positive factor 1/(|m||p|), and the
+//    unit vector u to the vertex is a product with 1/R_i.
+// Rounding differs from the reference's order by a few ulps per term.
+__device__ inline void near_vel(const ElemRec &E, const double x[3], double ffr, double rran,
+                                double rcut, double w[3]) {
+  const Panel &g = E.g;
+  const int nv = g.nv;
+  const bool dbl = E.flags & kFlagDoublet, src = E.flags & kFlagSource;
+  const double rv[3] = {x[0] - g.cen[0], x[1] - g.cen[1], x[2] - g.cen[2]};
+  const double radius = sqrt(dot3_rn(rv, rv));
+  const bool dfar = radius > ffr * g.maxedge;   // doublet velocity and potential switch
+  const bool sfar = radius > 10.0 * g.maxedge;  // source velocity switch
+  const double rn = dot3(rv, g.nor);
+  double vd[3] = {0.0, 0.0, 0.0};
+  double R[4], iR[4];  // the offsets x - ver_i are recomputed where used (exact)
+  if (!dfar && (dbl || (src && !sfar)))
+    for (int i = 0; i < nv; ++i) {
+      const double d[3] = {x[0] - g.ver[i][0], x[1] - g.ver[i][1], x[2] - g.ver[i][2]};
+      R[i] = sqrt(dot3(d, d));
+      iR[i] = 1.0 / R[i];
+    }
+  if (dbl) {
+    if (dfar) {
+      const double r2 = radius * radius, r5 = r2 * r2 * radius;
+      const double rati = 3.0 * g.area * rn / r5;
+      const double px = rati * dot3(rv, g.t1), py = rati * dot3(rv, g.t2);
+      const double pd = g.area * (-r2 + 3.0 * rn * rn) / r5;
+      for (int e = 0; e < 3; ++e) vd[e] = g.t1[e] * px + g.t2[e] * py + g.nor[e] * pd;
+    } else {
+      for (int i = 0; i < nv; ++i) {
+        const int ip1 = (i + 1 < nv) ? i + 1 : 0;
+        const double el = g.elen[i];
+        const double d[3] = {x[0] - g.ver[i][0], x[1] - g.ver[i][1], x[2] - g.ver[i][2]};
+        const double ai = dot3(d, g.euni[i]);
+        double hv[3], cr[3];
+        for (int e = 0; e < 3; ++e) hv[e] = d[e] - ai * g.euni[i][e];
+        const double h2 = dot3(hv, hv);
+        crs3(g.euni[i], hv, cr);
+        const double rr = rran * el;
+        double f = 0.0;
+        if (h2 > rr * rr)
+          f = ((el - ai) * iR[ip1] + ai * iR[i]) / h2;
+        else if (R[i] > el * rcut && R[ip1] > el * rcut)
+          f = ((el - ai) * iR[ip1] + ai * iR[i]) / (rr * rr);
+        for (int e = 0; e < 3; ++e) vd[e] += f * cr[e];
+      }
+    }
+  }
+  double vs[3] = {0.0, 0.0, 0.0};
+  if (src) {
+    double phix, phiy, pdou;
+    if (sfar) {
+      const double r3 = radius * radius * radius;
+      phix = -g.area * dot3(rv, g.t1) / r3;
+      phiy = -g.area * dot3(rv, g.t2) / r3;
+      pdou = -g.area * rn / r3;
+    } else {
+      double Rp[4];
+      for (int i = 0; i < nv; ++i) {
+        double a[3];
+        for (int e = 0; e < 3; ++e) a[e] = x[e] - g.verp[i][e];
+        Rp[i] = sqrt(dot3(a, a));
+      }
+      double sx = 0.0, sy = 0.0;
+      for (int i = 0; i < nv; ++i) {
+        const int ip1 = (i + 1 < nv) ? i + 1 : 0;
+        const double el = g.elen[i], sr = Rp[i] + Rp[ip1];
+        const double sl = (sr - el < 1e-12) ? 0.0 : log((sr + el) / (sr - el));
+        sx += E.sinT[i] * sl;
+        sy -= E.cosT[i] * sl;
+      }
+      double dou;
+      if (dfar) {
+        dou = rn * g.area / (radius * radius * radius);
+      } else {
+        dou = 0.0;
+        for (int i = 0; i < nv; ++i) {
+          const int im1 = (i == 0) ? nv - 1 : i - 1;
+          double u[3], ap1[3], am1[3], cr[3];
+          for (int e = 0; e < 3; ++e) u[e] = (g.ver[i][e] - x[e]) * iR[i];
+          const double d1 = dot3(u, g.evec[i]), d2 = dot3(u, g.evec[im1]);
+          for (int e = 0; e < 3; ++e) {
+            ap1[e] = g.evec[i][e] - u[e] * d1;
+            am1[e] = -g.evec[im1][e] + u[e] * d2;
+          }
+          crs3(am1, ap1, cr);
+          dou += atan2(dot3(u, cr), dot3(am1, ap1));
+        }
+        const double lim = (double)(nv - 2) * kPi;
+        if (dou < -lim + 1.0e-5)
+          dou += lim;
+        else if (dou > lim - 1.0e-5)
+          dou -= lim;
+      }
+      phix = -sx;
+      phiy = -sy;
+      pdou = -dou;
+    }
+    for (int e = 0; e < 3; ++e) vs[e] = g.t1[e] * phix + g.t2[e] * phiy + g.nor[e] * pdou;
+  }
+  for (int e = 0; e < 3; ++e) w[e] = vd[e] * E.mag - vs[e] * E.sigma;
+}
+
+// A line's velocity (all distances; lines have no far-field form).
+__device__ inline void line_term(const ElemRec &E, const double x[3], double rran, double rcut,
+                                 double w[3]) {
+  w[0] = w[1] = w[2] = 0.0;
+  if (E.flags & kFlagDoublet)
+    line_vel(E.g.ver[0], E.g.ver[1], E.g.euni[0], E.g.elen[0], x, rran, rcut, w);
+  for (int e = 0; e < 3; ++e) w[e] *= E.mag;
+}
+
+// The vs_elems stretching transpose(grad) alpha: one thread per target
+// point, every thread walks its element range (uniform loads of the
+// records).  The reference has no far-field form for it (the edge gradients
+// run for every pair).  Writes slots 3..5 of the partials, or stretch.
 __global__ void __launch_bounds__(256)
-    k_elem_apply(int ne, int per, const ElemRec *__restrict__ R, double ffr, double rran,
-                 double rcut,
-                 double del2, int n, const double *__restrict__ pos,
-                 const double *__restrict__ alpha, double *vel, double *stretch, int acc,
-                 double *part) {
+    k_elem_stretch(int ne, int per, const ElemRec *__restrict__ R, double del2, int n,
+                   const double *__restrict__ pos, const double *__restrict__ alpha,
+                   double *stretch, int acc, double *part) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
-  const bool do_st = stretch != nullptr;
-  double al[3] = {0.0, 0.0, 0.0};
-  if (do_st)
-    for (int e = 0; e < 3; ++e) al[e] = alpha[3 * p + e];
-  double v[3] = {0.0, 0.0, 0.0}, st[3] = {0.0, 0.0, 0.0};
+  const double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+  const double al[3] = {alpha[3 * p], alpha[3 * p + 1], alpha[3 * p + 2]};
+  double t[3] = {0.0, 0.0, 0.0};
   const int j0 = blockIdx.y * per;
   const int j1 = j0 + per < ne ? j0 + per : ne;
   for (int j = j0; j < j1; ++j) {
     const ElemRec &E = R[j];
-    const int flags = E.flags;
-    double w[3] = {0.0, 0.0, 0.0};
-    const double m = E.mag;
-    if (E.kind == DUST_ELEM_LINE) {
-      if (flags & kFlagDoublet)
-        line_vel(E.g.ver[0], E.g.ver[1], E.g.euni[0], E.g.elen[0], x, rran, rcut, w);
-      for (int e = 0; e < 3; ++e) w[e] *= m;
-    } else {
-      const Panel &g = E.g;
-      double rv[3] = {x[0] - g.cen[0], x[1] - g.cen[1], x[2] - g.cen[2]};
-      double r2 = dot3_rn(rv, rv);
-      const bool far_d = beyond(r2, ffr * g.maxedge);   // velocity_calc_doublet switch
-      const bool far_s = beyond(r2, 10.0 * g.maxedge);  // velocity_calc_sou_surfpan switch
-      const bool src = flags & kFlagSource;
-      if (far_d && (far_s || !src)) {
-        // both far-field formulas with one reciprocal square root: doublet
-        //   t1 phix + t2 phiy + nor pdou, rati = 3 area (r.n)/r^5,
-        //   phix = rati (r.t1), phiy = rati (r.t2), pdou = area (3 (r.n)^2 - r^2)/r^5;
-        // source: the same basis with -area (r.t1, r.t2, r.n)/r^3
-        if (TUNED) {  // FP32 formulas on the FP64 offset, FP64 sums
-          const float fr0 = (float)rv[0], fr1 = (float)rv[1], fr2 = (float)rv[2];
-          const float fr2s = (float)r2;
-          const float ir = __builtin_amdgcn_rsqf(fr2s);
-          const float ir3 = ir * ir * ir, ir5 = ir3 * ir * ir;
-          const float fa = (float)g.area;
-          const float rn = fr0 * (float)g.nor[0] + fr1 * (float)g.nor[1] + fr2 * (float)g.nor[2];
-          const float r1 = fr0 * (float)g.t1[0] + fr1 * (float)g.t1[1] + fr2 * (float)g.t1[2];
-          const float rt = fr0 * (float)g.t2[0] + fr1 * (float)g.t2[1] + fr2 * (float)g.t2[2];
-          float cx = 0.f, cy = 0.f, cz = 0.f;
-          if (flags & kFlagDoublet) {
-            const float fm = (float)m;
-            const float rati = 3.f * fa * rn * ir5 * fm;
-            cx = rati * r1;
-            cy = rati * rt;
-            cz = fa * (3.f * rn * rn - fr2s) * ir5 * fm;
-          }
-          if (src) {
-            const float fs = fa * ir3 * (float)E.sigma;
-            cx += fs * r1;
-            cy += fs * rt;
-            cz += fs * rn;
-          }
-          for (int e = 0; e < 3; ++e)
-            w[e] = (double)((float)g.t1[e] * cx + (float)g.t2[e] * cy + (float)g.nor[e] * cz);
-        } else {
-          double ir = rsqrt64(r2);
-          double ir3 = ir * ir * ir, ir5 = ir3 * ir * ir;
-          double rn = dot3(rv, g.nor), r1 = dot3(rv, g.t1), rt2 = dot3(rv, g.t2);
-          double cx = 0.0, cy = 0.0, cz = 0.0;
-          if (flags & kFlagDoublet) {
-            double rati = 3.0 * g.area * rn * ir5;
-            cx = rati * r1 * m;
-            cy = rati * rt2 * m;
-            cz = g.area * (3.0 * rn * rn - r2) * ir5 * m;
-          }
-          if (src) {
-            double fs = g.area * ir3 * E.sigma;  // minus the source velocity times sigma
-            cx += fs * r1;
-            cy += fs * rt2;
-            cz += fs * rn;
-          }
-          for (int e = 0; e < 3; ++e) w[e] = g.t1[e] * cx + g.t2[e] * cy + g.nor[e] * cz;
-        }
-      } else {
-        if (flags & kFlagDoublet) vel_doublet(g, x, ffr, rran, rcut, w);
-        for (int e = 0; e < 3; ++e) w[e] *= m;
-        if (src) {
-          double vs[3];
-          vel_source(g, x, ffr, vs);
-          for (int e = 0; e < 3; ++e) w[e] -= vs[e] * E.sigma;
-        }
-      }
-    }
-    for (int e = 0; e < 3; ++e) v[e] += w[e];
-    if (do_st && (flags & kFlagStretch) && (flags & kFlagDoublet)) {
+    if ((E.flags & kFlagStretch) && (E.flags & kFlagDoublet)) {
       double sg[3] = {0.0, 0.0, 0.0};
       elem_gradT(E, x, del2, al, sg);
-      for (int e = 0; e < 3; ++e) st[e] += sg[e] * m;
+      for (int e = 0; e < 3; ++e) t[e] += sg[e] * E.mag;
+    }
+  }
+  if (part) {
+    double *q = part + 6 * ((long)blockIdx.y * n + p) + 3;
+    for (int e = 0; e < 3; ++e) q[e] = t[e];
+    return;
+  }
+  for (int e = 0; e < 3; ++e)
+    stretch[3 * p + e] = (acc ? stretch[3 * p + e] : 0.0) + t[e] * kInv4Pi;
+}
+
+// The velocity, in two kernels, so that the far fields (most pairs) run
+// with few registers and many waves per SIMD while the near-field code
+// (several edges with square roots, logarithms and arctangents) keeps its
+// own occupancy.  k_elem_far: lines and the far pairs; for each polygon a wave
+// sets one bit of its near mask when any of its lanes is inside the switch.
+// k_elem_near: per wave, the polygons of its mask, near lanes only.  Per
+// target the velocity is the far terms in element order plus the near terms
+// in element order (whatever the other lanes of the wave are).  Mask words:
+// [split][wave][ceil(per / 64)], wave = global thread index / 64.
+template <int TUNED>
+__global__ void __launch_bounds__(256)
+    k_elem_far(int ne, int per, const ElemRec *__restrict__ R, double rran, double rcut, int n,
+               const double *__restrict__ pos, double *vel, int acc, double *part,
+               unsigned long long *mask) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;  // lane 0 of a wave is live whenever the wave is
+  const double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+  double t[3] = {0.0, 0.0, 0.0};
+  const int j0 = blockIdx.y * per;
+  const int j1 = j0 + per < ne ? j0 + per : ne;
+  const int nw = (per + 63) >> 6;
+  const int wave = (blockIdx.x * blockDim.x + __builtin_amdgcn_readfirstlane(threadIdx.x)) >> 6;
+  unsigned long long *mw = mask + ((long)blockIdx.y * (gridDim.x * (blockDim.x >> 6)) + wave) * nw;
+  const bool lane0 = (threadIdx.x & 63) == 0;
+  unsigned long long word = 0;
+  for (int j = j0; j < j1; ++j) {
+    const ElemRec &E = R[j];
+    double w[3] = {0.0, 0.0, 0.0};
+    if (E.kind == DUST_ELEM_LINE) {
+      line_term(E, x, rran, rcut, w);
+    } else {
+      double rv[3], r2;
+      const bool far = far_pair(E.f, E.flags, x, rv, r2);
+      if (far) far_vel<TUNED>(E, rv, r2, w);
+      if (__ballot(!far)) word |= 1ull << ((j - j0) & 63);
+    }
+    for (int e = 0; e < 3; ++e) t[e] += w[e];
+    if (((j - j0) & 63) == 63 || j == j1 - 1) {
+      if (lane0) mw[(j - j0) >> 6] = word;
+      word = 0;
     }
   }
   if (part) {  // one element range of several: raw partial sums
     double *q = part + 6 * ((long)blockIdx.y * n + p);
-    for (int e = 0; e < 3; ++e) q[e] = v[e];
-    if (do_st)
-      for (int e = 0; e < 3; ++e) q[3 + e] = st[e];
+    for (int e = 0; e < 3; ++e) q[e] = t[e];
     return;
   }
-  // accumulate: out += term; otherwise out = term (the sorted path adds the
-  // terms into the caller's arrays in a second kernel)
-  for (int e = 0; e < 3; ++e) vel[3 * p + e] = (acc ? vel[3 * p + e] : 0.0) + v[e] * kInv4Pi;
-  if (do_st)
-    for (int e = 0; e < 3; ++e)
-      stretch[3 * p + e] = (acc ? stretch[3 * p + e] : 0.0) + st[e] * kInv4Pi;
+  for (int e = 0; e < 3; ++e) vel[3 * p + e] = (acc ? vel[3 * p + e] : 0.0) + t[e] * kInv4Pi;
+}
+
+__global__ void __launch_bounds__(256)
+    k_elem_near(int ne, int per, const ElemRec *__restrict__ R, double ffr, double rran,
+                double rcut, int n, const double *__restrict__ pos, double *vel, double *part,
+                const unsigned long long *__restrict__ mask) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+  double t[3] = {0.0, 0.0, 0.0};
+  const int j0 = blockIdx.y * per;
+  const int j1 = j0 + per < ne ? j0 + per : ne;
+  const int nw = (per + 63) >> 6, nwr = (j1 - j0 + 63) >> 6;
+  const int wave = (blockIdx.x * blockDim.x + __builtin_amdgcn_readfirstlane(threadIdx.x)) >> 6;
+  const unsigned long long *mw =
+      mask + ((long)blockIdx.y * (gridDim.x * (blockDim.x >> 6)) + wave) * nw;
+  bool any = false;
+  for (int k = 0; k < nwr; ++k) {
+    unsigned long long word = mw[k];
+    while (word) {
+      const int j = j0 + 64 * k + __builtin_ctzll(word);
+      word &= word - 1;
+      const ElemRec &E = R[j];
+      double rv[3], r2;
+      if (!far_pair(E.f, E.flags, x, rv, r2)) {
+        double w[3];
+        near_vel(E, x, ffr, rran, rcut, w);
+        for (int e = 0; e < 3; ++e) t[e] += w[e];
+        any = true;
+      }
+    }
+  }
+  if (!any) return;
+  if (part) {
+    double *q = part + 6 * ((long)blockIdx.y * n + p);
+    for (int e = 0; e < 3; ++e) q[e] += t[e];
+  } else {
+    for (int e = 0; e < 3; ++e) vel[3 * p + e] += t[e] * kInv4Pi;
+  }
 }
 
 // Sum of the S element-range partials of each target, in range order.
@@ -534,6 +724,7 @@ __global__ void k_vort_finish(int m, int S, const double *__restrict__ part,
 // like the rest of the library's host entry points).
 struct ElemWs {
   DevBuf<ElemRec> rec;
+  DevBuf<unsigned long long> nmask;  // near masks of the velocity-only path
   DevBuf<double> part, packed;
   DevBuf<unsigned> key, key_s;  // Morton order of the targets
   DevBuf<int> idx, perm;
@@ -575,7 +766,8 @@ int elem_apply_dev(int ngroups, const dust_elem_group *groups, const dust_elem_p
     const dust_elem_group &G = groups[k];
     if (G.n == 0) continue;
     k_elem_recs<<<ceil_div(G.n, 128), 128, 0, st>>>(G.n, G.kind, G.n_ver, G.ver, G.mag, G.sigma,
-                                                    G.stretch, g_ws.rec.p + off);
+                                                    G.stretch, prm->ff_ratio_doublet,
+                                                    g_ws.rec.p + off);
     off += G.n;
   }
   const double del2 = prm->vortex_rad * prm->vortex_rad;
@@ -596,16 +788,20 @@ int elem_apply_dev(int ngroups, const dust_elem_group *groups, const dust_elem_p
       DUST_TRY(g_ws.epart.reserve(6 * (size_t)S * n));
       part = g_ws.epart.p;
     }
-    if (prm->precision == DUST_ELEM_TUNED)
-      k_elem_apply<1><<<dim3(tb, S), 256, 0, st>>>((int)ne, per, g_ws.rec.p,
-                                                  prm->ff_ratio_doublet, prm->rankine_rad,
-                                                  prm->cutoff_rad, del2, n, tp, ta, vo, so, acc,
-                                                  part);
+    const bool tuned = prm->precision == DUST_ELEM_TUNED;
+    const double ffr = prm->ff_ratio_doublet, rran = prm->rankine_rad, rcut = prm->cutoff_rad;
+    DUST_TRY(g_ws.nmask.reserve((size_t)S * tb * 4 * ceil_div(per, 64)));
+    if (tuned)
+      k_elem_far<1><<<dim3(tb, S), 256, 0, st>>>((int)ne, per, g_ws.rec.p, rran, rcut, n, tp, vo,
+                                                acc, part, g_ws.nmask.p);
     else
-      k_elem_apply<0><<<dim3(tb, S), 256, 0, st>>>((int)ne, per, g_ws.rec.p,
-                                                  prm->ff_ratio_doublet, prm->rankine_rad,
-                                                  prm->cutoff_rad, del2, n, tp, ta, vo, so, acc,
-                                                  part);
+      k_elem_far<0><<<dim3(tb, S), 256, 0, st>>>((int)ne, per, g_ws.rec.p, rran, rcut, n, tp, vo,
+                                                acc, part, g_ws.nmask.p);
+    k_elem_near<<<dim3(tb, S), 256, 0, st>>>((int)ne, per, g_ws.rec.p, ffr, rran, rcut, n, tp, vo,
+                                             part, g_ws.nmask.p);
+    if (so)
+      k_elem_stretch<<<dim3(tb, S), 256, 0, st>>>((int)ne, per, g_ws.rec.p, del2, n, tp, ta, so,
+                                                  acc, part);
     if (S > 1) k_elem_reduce<<<tb, 256, 0, st>>>(n, S, part, vo, so, acc);
     DUST_HIP(hipGetLastError());
     return 0;
