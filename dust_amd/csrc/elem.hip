@@ -47,6 +47,7 @@ constexpr int kFillBlocks = 2048, kMinPerSplit = 32;  // element splits for few 
 //   P = t1 t1^T + t2 t2^T + n n^T  (symmetric: xx yy zz xy xz yz),
 // and the two far-field switches with their thresholds squared.
 struct FarCoef {
+  int kind, flags;
   double cen[3], nor[3];
   double P[6];
   double am3, am, as;       // 3 A m, A m, A sigma (zero without the term)
@@ -64,7 +65,6 @@ struct ElemRec {
   Panel g;  // polygons: calc_geo_data; lines: ver[0..1], evec/euni/elen[0], cen
   double sinT[4], cosT[4];  // edge directions on the PANAIR basis: euni.t2, euni.t1
   double mag, sigma;
-  int kind, flags;
 };
 
 // sqrt(r2) > T, as the reference decides its far-field switches (radius >
@@ -109,8 +109,8 @@ __global__ void k_elem_recs(int n, int kind, const int *__restrict__ nver,
   }
   r.mag = mag ? mag[j] : 0.0;
   r.sigma = sigma ? sigma[j] : 0.0;
-  r.kind = kind;
-  r.flags = (mag ? kFlagDoublet : 0) | (sigma && kind == DUST_ELEM_SURFPAN ? kFlagSource : 0) |
+  r.f.kind = kind;
+  r.f.flags = (mag ? kFlagDoublet : 0) | (sigma && kind == DUST_ELEM_SURFPAN ? kFlagSource : 0) |
             (stretch ? kFlagStretch : 0);
   FarCoef &f = r.f;
   const Panel &g = r.g;
@@ -129,10 +129,10 @@ __global__ void k_elem_recs(int n, int kind, const int *__restrict__ nver,
     for (int k = 0; k < 3; ++k) acc += b[k][ia[q]] * b[k][ib[q]];
     f.P[q] = acc;
   }
-  const double am = (r.flags & kFlagDoublet) ? g.area * r.mag : 0.0;
+  const double am = (r.f.flags & kFlagDoublet) ? g.area * r.mag : 0.0;
   f.am3 = 3.0 * am;
   f.am = am;
-  f.as = (r.flags & kFlagSource) ? g.area * r.sigma : 0.0;
+  f.as = (r.f.flags & kFlagSource) ? g.area * r.sigma : 0.0;
   f.td = ffr * g.maxedge;
   switch_band(f.td, f.td_lo, f.td_hi);
   f.ts = 10.0 * g.maxedge;
@@ -249,9 +249,9 @@ __device__ inline bool far_pair(const FarCoef &f, int flags, const double x[3], 
 // Both far fields with one reciprocal square root:
 //   w = (3 A m rn / r^5 + A sigma / r^3) P r - (A m / r^3) n.
 template <int TUNED>
-__device__ inline void far_vel(const ElemRec &E, const double rv[3], double r2, double w[3]) {
+__device__ inline void far_vel(const FarCoef &f, const FarCoef32 &h, const double rv[3], double r2,
+                               double w[3]) {
   if (TUNED) {  // FP32 on the FP64 offset, FP64 sums
-    const FarCoef32 &h = E.h;
     const float r0 = (float)rv[0], r1 = (float)rv[1], rz = (float)rv[2];
     const float ir = __builtin_amdgcn_rsqf((float)r2);
     const float ir2 = ir * ir, ir3 = ir2 * ir;
@@ -264,7 +264,6 @@ __device__ inline void far_vel(const ElemRec &E, const double rv[3], double r2, 
     w[1] = (double)(a * p1 - b * h.nor[1]);
     w[2] = (double)(a * p2 - b * h.nor[2]);
   } else {
-    const FarCoef &f = E.f;
     const double ir = rsqrt64(r2);
     const double ir2 = ir * ir, ir3 = ir2 * ir;
     const double rn = dot3(rv, f.nor);
@@ -298,7 +297,7 @@ __device__ inline void near_vel(const ElemRec &E, const double x[3], double ffr,
                                 double rcut, double w[3]) {
   const Panel &g = E.g;
   const int nv = g.nv;
-  const bool dbl = E.flags & kFlagDoublet, src = E.flags & kFlagSource;
+  const bool dbl = E.f.flags & kFlagDoublet, src = E.f.flags & kFlagSource;
   const double rv[3] = {x[0] - g.cen[0], x[1] - g.cen[1], x[2] - g.cen[2]};
   const double radius = sqrt(dot3_rn(rv, rv));
   const bool dfar = radius > ffr * g.maxedge;   // doublet velocity and potential switch
@@ -398,7 +397,7 @@ __device__ inline void near_vel(const ElemRec &E, const double x[3], double ffr,
 __device__ inline void line_term(const ElemRec &E, const double x[3], double rran, double rcut,
                                  double w[3]) {
   w[0] = w[1] = w[2] = 0.0;
-  if (E.flags & kFlagDoublet)
+  if (E.f.flags & kFlagDoublet)
     line_vel(E.g.ver[0], E.g.ver[1], E.g.euni[0], E.g.elen[0], x, rran, rcut, w);
   for (int e = 0; e < 3; ++e) w[e] *= E.mag;
 }
@@ -420,7 +419,8 @@ __global__ void __launch_bounds__(256)
   const int j1 = j0 + per < ne ? j0 + per : ne;
   for (int j = j0; j < j1; ++j) {
     const ElemRec &E = R[j];
-    if ((E.flags & kFlagStretch) && (E.flags & kFlagDoublet)) {
+    const int flags = E.f.flags;
+    if ((flags & kFlagStretch) && (flags & kFlagDoublet)) {
       double sg[3] = {0.0, 0.0, 0.0};
       elem_gradT(E, x, del2, al, sg);
       for (int e = 0; e < 3; ++e) t[e] += sg[e] * E.mag;
@@ -462,13 +462,21 @@ __global__ void __launch_bounds__(256)
   unsigned long long word = 0;
   for (int j = j0; j < j1; ++j) {
     const ElemRec &E = R[j];
+    const FarCoef f = E.f;  // one batch of uniform loads, then one wait
     double w[3] = {0.0, 0.0, 0.0};
-    if (E.kind == DUST_ELEM_LINE) {
+    if (f.kind == DUST_ELEM_LINE) {
       line_term(E, x, rran, rcut, w);
     } else {
       double rv[3], r2;
-      const bool far = far_pair(E.f, E.flags, x, rv, r2);
-      if (far) far_vel<TUNED>(E, rv, r2, w);
+      const bool far = far_pair(f, f.flags, x, rv, r2);
+      if (far) {
+        if (TUNED) {
+          const FarCoef32 h = E.h;
+          far_vel<1>(f, h, rv, r2, w);
+        } else {
+          far_vel<0>(f, E.h, rv, r2, w);
+        }
+      }
       if (__ballot(!far)) word |= 1ull << ((j - j0) & 63);
     }
     for (int e = 0; e < 3; ++e) t[e] += w[e];
@@ -507,7 +515,7 @@ __global__ void __launch_bounds__(256)
       word &= word - 1;
       const ElemRec &E = R[j];
       double rv[3], r2;
-      if (!far_pair(E.f, E.flags, x, rv, r2)) {
+      if (!far_pair(E.f, E.f.flags, x, rv, r2)) {
         double w[3];
         near_vel(E, x, ffr, rran, rcut, w);
         for (int e = 0; e < 3; ++e) t[e] += w[e];

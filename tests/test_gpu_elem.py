@@ -87,6 +87,31 @@ def test_elem_apply_kinds_and_missing_strengths(orc, wing):
                 assert np.all(st == 0.0)
 
 
+@pytest.mark.parametrize("n", [900, 9000], ids=["input-order", "morton"])
+def test_elem_apply_triangles_vs_oracle(orc, wing, n):
+    """Triangular surface panels (each wing quad cut along a diagonal, both
+    strengths) and a mixed list with the quads: velocity and stretching with
+    probes in the near field and on the vertices' neighbourhood."""
+    groups, pts, al = wing
+    body = groups[0]
+    q = np.asarray(body["ver"])
+    tri = np.zeros((2 * len(q), 4, 3))
+    tri[0::2, :3] = q[:, [0, 1, 2]]
+    tri[1::2, :3] = q[:, [0, 2, 3]]
+    rng = np.random.default_rng(6)
+    tg = dict(kind=E.SURFPAN, ver=tri, n_ver=np.full(len(tri), 3, np.int32),
+              mag=np.repeat(body["mag"], 2), sigma=np.repeat(body["sigma"], 2))
+    cen = tri[:, :3].mean(axis=1)
+    probes = cen[rng.choice(len(cen), n, replace=True)] + 0.02 * rng.standard_normal((n, 3))
+    a = 1e-3 * rng.standard_normal((n, 3))
+    for gs in ([tg], [tg, groups[1]]):
+        print(f"elem_apply triangles n={n} groups={len(gs)}", flush=True)
+        vel, st = E.apply_elements(gs, probes, alpha=a, vortex_rad=0.05)
+        ov, os_ = orc.elem_apply(oracle_groups(gs), probes, alpha=a, vortex_rad=0.05)
+        print(f"vel rel {rel(vel, ov):.2e} stretch rel {rel(st, os_):.2e}")
+        assert rel(vel, ov) <= TOL and rel(st, os_) <= TOL
+
+
 def test_vort_vel_wing_vs_oracle(orc, wing):
     groups, pts, al = wing
     ver, nv, cen, alpha, rv, tree = W.wing_case()
