@@ -748,9 +748,11 @@ def main():
     n = args.n
     if args.elements_only:
         out = dict(elements=time_elements(torch, L, W, n, args.steps, with_cpu=not args.no_cpu),
-                   wake=time_wake(torch, L, 5_000_000, args.steps, with_cpu=not args.no_cpu),
-                   wing_run=time_wing_run(torch, W, 200, with_cpu=not args.no_cpu),
-                   rotor_run=time_rotor_run(torch, W, n))
+                   wake=time_wake(torch, L, 5_000_000, args.steps, with_cpu=not args.no_cpu))
+        if not args.no_wing:
+            out["wing_run"] = time_wing_run(torch, W, 200, with_cpu=not args.no_cpu)
+        if not args.no_rotor:
+            out["rotor_run"] = time_rotor_run(torch, W, n)
         print(json.dumps(out, default=float))
         return
     log(f"generating helix wake n={n}")

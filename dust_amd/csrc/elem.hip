@@ -19,9 +19,12 @@
 //   particle       c (dist x dir) mag, c = -1/(|dist|^2 + r^2)^1.5
 //                  (mod_vortpart.f90:139-153, 260-271)
 //
-// Layout: element records (geometry of calc_geo_data, strength, flags) are
-// built once per call in one array, read by every thread with uniform loads;
-// one thread per target point.  get_vort_vel has few targets (panels) and
+// Layout: element records (a far-field block, then the geometry of
+// calc_geo_data, strength, flags) are built once per call in one array, read
+// by every thread with uniform loads; one thread per target point, targets in
+// Morton order.  The velocity runs as a far kernel (every pair's far field,
+// few registers) and a near kernel (the elements each wave flagged); the
+// stretching as a kernel of its own.  get_vort_vel has few targets (panels) and
 // many sources (particles): the sources are split over blockIdx.y, staged
 // through LDS 256 at a time, and the per-split partial sums are added in a
 // fixed order by a second kernel.

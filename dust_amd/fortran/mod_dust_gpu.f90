@@ -34,6 +34,7 @@ module mod_dust_gpu
 
   ! dust_elem_group / dust_elem_params (include/dust_gpu.h), same field order
   integer(c_int), parameter :: DUST_ELEM_SURFPAN = 0, DUST_ELEM_RING = 1, DUST_ELEM_LINE = 2
+  integer(c_int), parameter :: DUST_ELEM_FP64 = 0, DUST_ELEM_TUNED = 1
   type, bind(C) :: t_dust_elem_group
     integer(c_int) :: kind, n
     type(c_ptr)    :: n_ver, ver, mag, sigma
