@@ -186,6 +186,13 @@ SIGNATURES = [
     ("dust_dgesv_mixed_batch_dev", C.c_int,
      [C.c_int, _ip, _vp, _ip, _vp, _ip, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _dp]),
     ("dust_solve_linsys", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp]),
+    ("dust_solve_pressure_sys", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp]),
+    ("dust_pres_rhs_rot", C.c_int,
+     [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp,
+      _vp, _vp, _vp]),
+    ("dust_pres_rhs_rot_dev", C.c_int,
+     [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp,
+      _vp, _vp, _vp, _vp]),
     ("dust_test_gemm_sub_dev", C.c_int,
      [C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int]),
     ("dust_test_trsm_llnu_dev", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int]),

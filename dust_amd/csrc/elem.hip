@@ -171,15 +171,6 @@ __device__ inline void line_vel(const double *x1, const double *x2, const double
   for (int e = 0; e < 3; ++e) v[e] = f * cr[e];
 }
 
-// 1/sqrt(x) in FP64: the hardware estimate and two Newton steps.
-__device__ inline double rsqrt64(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  double h = 0.5 * x;
-  y = y * fma(-h * y, y, 1.5);
-  y = y * fma(-h * y, y, 1.5);
-  return y;
-}
-
 // transpose(grad) alpha for one edge of gradient_calc_doublet /
 // compute_grad_vortline (mod_doublet.f90:254-305, mod_vortline.f90:174-236),
 // accumulated into s.  With a_k = l x R_k, a = |a_1|, A = a^2 + del^2 and the

@@ -77,10 +77,13 @@ static int solve_linsys_dev(int rank, int ns, double *dA, long lda, int *dp, dou
 
 using namespace dust;
 
-extern "C" {
+namespace dust {
+namespace {
 
-int dust_solve_linsys(int rank, int nstatic, double *A, int lda, int *ipiv, const double *b,
-                      double *res) {
+// Host staging around solve_linsys_dev: A(lda, rank) with its static block
+// factored, ipiv, b in; A, ipiv and res out.
+int solve_block_host(int rank, int nstatic, double *A, int lda, int *ipiv, const double *b,
+                     double *res) {
   DUST_CHECK(rank >= 1 && nstatic >= 0 && nstatic <= rank && lda >= rank && A && ipiv && b && res,
              DUST_E_ARG, "invalid solve_linsys arguments");
   double *dA = nullptr, *dr = nullptr;
@@ -107,6 +110,25 @@ int dust_solve_linsys(int rank, int nstatic, double *A, int lda, int *ipiv, cons
   (void)hipFree(dr);
   (void)hipFree(dp);
   return rc ? rc : info;
+}
+
+}  // namespace
+}  // namespace dust
+
+extern "C" {
+
+int dust_solve_linsys(int rank, int nstatic, double *A, int lda, int *ipiv, const double *b,
+                      double *res) {
+  return dust::solve_block_host(rank, nstatic, A, lda, ipiv, b, res);
+}
+
+// solve_pressure_sys (mod_pressure_equation.f90:251-390) is the same block
+// elimination on the surface-panel system A_pres: its side-band condition
+// reads the main system's nstatic / nmoving, which for the surface-panel
+// counts only differs where a side band is empty and the operation a no-op.
+int dust_solve_pressure_sys(int n_sp, int nstatic_sp, double *A_pres, int lda, int *P_pres,
+                            const double *b_pres, double *res_pres) {
+  return dust::solve_block_host(n_sp, nstatic_sp, A_pres, lda, P_pres, b_pres, res_pres);
 }
 
 }  // extern "C"

@@ -85,6 +85,15 @@ __device__ inline void panel_geo(int nv, const double *__restrict__ vp, Panel &g
   }
 }
 
+// 1/sqrt(x) in FP64: the hardware estimate and two Newton steps.
+__device__ inline double rsqrt64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
 // potential_calc_doublet: solid angle of panel g seen from pos.
 __device__ inline double pot_doublet(const Panel &g, const double pos[3], double ffr) {
   double rv[3] = {pos[0] - g.cen[0], pos[1] - g.cen[1], pos[2] - g.cen[2]};

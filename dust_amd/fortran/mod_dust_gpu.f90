@@ -88,6 +88,20 @@ module mod_dust_gpu
       real(c_double), intent(inout) :: A(lda,*); integer(c_int), intent(inout) :: ipiv(*)
       real(c_double), intent(in) :: b(*); real(c_double), intent(out) :: res(*)
     end function
+    integer(c_int) function dust_solve_pressure_sys(n_sp, nstatic_sp, A_pres, lda, P_pres, &
+                                                    b_pres, res_pres) bind(C)
+      import; integer(c_int), value :: n_sp, nstatic_sp, lda
+      real(c_double), intent(inout) :: A_pres(lda,*); integer(c_int), intent(inout) :: P_pres(*)
+      real(c_double), intent(in) :: b_pres(*); real(c_double), intent(out) :: res_pres(*)
+    end function
+    integer(c_int) function dust_pres_rhs_rot(m, cen, np, pcen, pdir, pmag, prv, pvel, nl, &
+                                              lver, lmag, lvel, nw, wnver, wver, wmag, wvel, &
+                                              b_pres) bind(C)
+      import; integer(c_int), value :: m, np, nl, nw
+      real(c_double), intent(in) :: cen(3,*)
+      type(c_ptr), value :: pcen, pdir, pmag, prv, pvel, lver, lmag, lvel, wnver, wver, wmag, wvel
+      real(c_double), intent(inout) :: b_pres(*)
+    end function
     integer(c_int) function dust_dgesv_mixed(n, nrhs, A, lda, B, ldb, prec, tol, max_iter, &
                                              iters, resid) bind(C)
       import; integer(c_int), value :: n, nrhs, lda, ldb, prec, max_iter

@@ -434,6 +434,45 @@ int dust_dgesv_mixed_batch_dev(int nb, const int *n, const double *const *dA, co
 int dust_solve_linsys(int rank, int nstatic, double *A, int lda, int *ipiv, const double *b,
                       double *res);
 
+/* ======================================================================== */
+/* Pressure (Bernoulli) integral equation: src/linsys/mod_pressure_equation */
+/* ======================================================================== */
+
+/* solve_pressure_sys (mod_pressure_equation.f90:251-390): the same block
+ * elimination as dust_solve_linsys on the surface-panel system A_pres
+ * (n_sp x n_sp, static block of nstatic_sp rows factored by dgetrf as
+ * initialize_linsys does, mod_linsys.f90:247-262), pivots P_pres; A_pres and
+ * P_pres are left as the reference leaves them, res_pres = A_pres^-1 b_pres.
+ * Host pointers. */
+int dust_solve_pressure_sys(int n_sp, int nstatic_sp, double *A_pres, int lda, int *P_pres,
+                            const double *b_pres, double *res_pres);
+
+/* assemble_pressure_sys (mod_pressure_equation.f90:139-236), rotational part
+ * (c) of the right-hand side at the m surface-panel centres cen(3,m),
+ * accumulated into b_pres(m) in the reference's order (particles, then
+ * lines, then wake panels):
+ *   + the np particles (live slots only; 171-187): pcen, pdir, pmag,
+ *     prv (r_Vortex) and pvel (3,np), d.(dir x vel) mag / (|d|^2+r^2)^1.5,
+ *     d = pcen - cen;
+ *   - the nl end-vortex lines lver(3,2,nl) of strength lmag (NULL: none
+ *     associated) with vertex velocities lvel(3,2,nl) (ver_vel; 191-203);
+ *   - the nw wake panels wver(3,4,nw) (wnver vertices each, triangles
+ *     padded) of strength wmag with vertex velocities wvel(3,4,nw) in the
+ *     vertex order (pan_w_vel at the panel's points; 204-225), edge by edge.
+ * Host pointers, synchronous. */
+int dust_pres_rhs_rot(int m, const double *cen, int np, const double *pcen, const double *pdir,
+                      const double *pmag, const double *prv, const double *pvel, int nl,
+                      const double *lver, const double *lmag, const double *lvel, int nw,
+                      const int *wnver, const double *wver, const double *wmag,
+                      const double *wvel, double *b_pres);
+/* Same with device pointers, ordered on `stream`. */
+int dust_pres_rhs_rot_dev(int m, const double *cen, int np, const double *pcen,
+                          const double *pdir, const double *pmag, const double *prv,
+                          const double *pvel, int nl, const double *lver, const double *lmag,
+                          const double *lvel, int nw, const int *wnver, const double *wver,
+                          const double *wmag, const double *wvel, double *b_pres,
+                          void *stream);
+
 /* Test hooks (no counterpart in the reference): the building blocks of
  * dust_dgetrf on device pointers, synchronous.  C(M,N) -= A(M,K) B(K,N) in
  * FP64 on the matrix cores; B(m,n) <- L^-1 B with L unit lower (m x m). */

@@ -75,6 +75,8 @@ def lib():
                                           C.c_double, _vp, _vp]),
             "ao_wake_shed": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp,
                                        _vp, _vp, _vp]),
+            "ao_pres_rhs_rot": (None, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int,
+                                       _vp, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
             "ao_dgetrf": (C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp]),
             "ao_dgetrs": (None, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int]),
         }
@@ -374,6 +376,31 @@ def vort_vel(cen, pcen=None, pdir=None, pmag=None, prv=None, lver=None, lmag=Non
                       None if (lmag is None or not nl) else _p(_c(lmag)), float(rankine),
                       float(cutoff), _p(u))
     return u
+
+
+# ---- pressure equation (pres_oracle.c) --------------------------------------
+def pres_rhs_rot(cen, b=None, pcen=None, pdir=None, pmag=None, prv=None, pvel=None, lver=None,
+                 lmag=None, lvel=None, wver=None, wnver=None, wmag=None, wvel=None):
+    """assemble_pressure_sys's rotational right-hand side at the centres cen
+    (m,3), added to b (zeros if None); returns b."""
+    cen = _c(cen).reshape(-1, 3)
+    m = cen.shape[0]
+    b = np.zeros(m) if b is None else np.array(b, dtype=np.float64)
+    np_ = 0 if pmag is None else len(pmag)
+    nl = 0 if lver is None else np.asarray(lver).shape[0]
+    nw = 0 if wver is None else np.asarray(wver).shape[0]
+    if nw:  # polygons padded to four vertices, as the C arrays are laid out
+        pad = lambda a: np.concatenate(
+            [np.asarray(a, np.float64)[:, :4],
+             np.zeros((nw, max(0, 4 - np.asarray(a).shape[1]), 3))], axis=1)
+        wver, wvel = pad(wver), pad(wvel)
+    opt = lambda a, k: _p(_c(a)) if k else None
+    lib().ao_pres_rhs_rot(m, _p(cen), np_, opt(pcen, np_), opt(pdir, np_), opt(pmag, np_),
+                          opt(prv, np_), opt(pvel, np_), nl, opt(lver, nl),
+                          None if (lmag is None or not nl) else _p(_c(lmag)), opt(lvel, nl), nw,
+                          _p(_c(wnver, np.int32)) if nw else None, opt(wver, nw), opt(wmag, nw),
+                          opt(wvel, nw), _p(b))
+    return b
 
 
 # ---- particle pool of the wake (wake_oracle.c) --------------------------------

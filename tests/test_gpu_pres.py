@@ -1,0 +1,139 @@
+"""The pressure (Bernoulli) integral equation on the GPU against the FP64 CPU
+oracle and closed forms: the rotational right-hand side of
+assemble_pressure_sys (src/linsys/mod_pressure_equation.f90:139-236) to
+1e-10 relative L2, and solve_pressure_sys (:251-390) against a direct solve
+of the surface-panel system."""
+import numpy as np
+import pytest
+
+from dust_amd import _lib as L
+from dust_amd import linsys as LS
+from dust_amd import pressure as P
+from dust_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def wing():
+    ver, nv, cen, alpha, rv, _ = W.wing_case()
+    rng = np.random.default_rng(11)
+    pc = ver[:, :4].mean(axis=1)
+    mag = np.linalg.norm(alpha, axis=1)
+    d = alpha / mag[:, None]
+    pvel = 30.0 * np.array([1.0, 0.0, 0.0]) + rng.standard_normal(cen.shape)
+    # end vortices along the two wing tips and wake panels behind the trailing edge
+    nl = 20
+    lver = np.stack([np.stack([[1.0 + 0.05 * k, -4.0, 0.0], [1.0 + 0.05 * (k + 1), -4.0, 0.0]])
+                     for k in range(nl)])
+    lvel = rng.standard_normal((nl, 2, 3))
+    ys = np.linspace(-4.0, 4.0, 21)
+    wver = np.array([[[1.0, ys[k], 0.0], [1.0, ys[k + 1], 0.0], [1.05, ys[k + 1], -0.01],
+                      [1.05, ys[k], -0.01]] for k in range(20)])
+    wvel = rng.standard_normal((20, 4, 3))
+    return dict(cen=pc, pcen=cen, pdir=d, pmag=mag, prv=rv, pvel=pvel, lver=lver,
+                lmag=rng.standard_normal(nl), lvel=lvel, wver=wver, wnver=np.full(20, 4),
+                wmag=rng.standard_normal(20), wvel=wvel)
+
+
+def test_rhs_rot_wing_vs_oracle(orc, wing):
+    b0 = np.random.default_rng(2).standard_normal(len(wing["cen"]))
+    print(f"pres_rhs_rot wing m={len(wing['cen'])} np={len(wing['pmag'])}", flush=True)
+    b = P.pres_rhs_rot(b=b0, **wing)
+    ob = orc.pres_rhs_rot(b=b0, **wing)
+    print(f"rel {rel(b - b0, ob - b0):.2e}")
+    assert rel(b - b0, ob - b0) <= TOL
+    # each part alone
+    parts = [dict(pcen=wing["pcen"], pdir=wing["pdir"], pmag=wing["pmag"], prv=wing["prv"],
+                  pvel=wing["pvel"]),
+             dict(lver=wing["lver"], lmag=wing["lmag"], lvel=wing["lvel"]),
+             dict(wver=wing["wver"], wnver=wing["wnver"], wmag=wing["wmag"], wvel=wing["wvel"])]
+    for k, kw in enumerate(parts):
+        print(f"pres_rhs_rot part {k}", flush=True)
+        g = P.pres_rhs_rot(wing["cen"], **kw)
+        o = orc.pres_rhs_rot(wing["cen"], **kw)
+        assert rel(g, o) <= TOL
+
+
+@pytest.mark.parametrize("m,npart", [(1, 1), (1, 70_001), (300, 200_000), (5000, 3)])
+def test_rhs_rot_split_sources(orc, m, npart):
+    """Few targets with many particles (the source splits and their fixed-
+    order sum), and the other way round."""
+    rng = np.random.default_rng(m + npart)
+    cen = rng.standard_normal((m, 3))
+    pc, pd, pv = rng.standard_normal((npart, 3)), rng.standard_normal((npart, 3)), \
+        rng.standard_normal((npart, 3))
+    pm, pr = rng.uniform(0.1, 1.0, npart), rng.uniform(0.01, 0.1, npart)
+    print(f"pres_rhs_rot m={m} np={npart}", flush=True)
+    b = P.pres_rhs_rot(cen, pcen=pc, pdir=pd, pmag=pm, prv=pr, pvel=pv)
+    o = orc.pres_rhs_rot(cen, pcen=pc, pdir=pd, pmag=pm, prv=pr, pvel=pv)
+    assert rel(b, o) <= TOL
+
+
+def test_rhs_rot_triangles_and_empty(orc, wing):
+    rng = np.random.default_rng(5)
+    cen = wing["cen"][:200]
+    wver = 1.0 + rng.standard_normal((7, 3, 3))
+    wvel = rng.standard_normal((7, 3, 3))
+    kw = dict(wver=wver, wnver=np.full(7, 3), wmag=rng.standard_normal(7), wvel=wvel)
+    print("pres_rhs_rot triangles", flush=True)
+    assert rel(P.pres_rhs_rot(cen, **kw), orc.pres_rhs_rot(cen, **kw)) <= TOL
+    b0 = rng.standard_normal(200)
+    # nothing to add: b_pres unchanged, bit for bit
+    assert np.array_equal(P.pres_rhs_rot(cen, b=b0), b0)
+    assert np.array_equal(P.pres_rhs_rot(cen, b=b0, lver=wing["lver"], lmag=None,
+                                         lvel=wing["lvel"]), b0)
+    assert P.pres_rhs_rot(np.zeros((0, 3))).shape == (0,)
+
+
+def test_rhs_rot_device_matches_host(wing):
+    import ctypes as C
+    import torch
+    dv = lambda a, t=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to("cuda", t)
+    w = wing
+    m, npart, nl, nw = len(w["cen"]), len(w["pmag"]), len(w["lmag"]), len(w["wmag"])
+    t = {k: dv(w[k]) for k in ("cen", "pcen", "pdir", "pmag", "prv", "pvel", "lver", "lmag",
+                               "lvel", "wver", "wmag", "wvel")}
+    wn = dv(w["wnver"].astype(np.int32), torch.int32)
+    tb = torch.zeros(m, dtype=torch.float64, device="cuda")
+    print("pres_rhs_rot device entry", flush=True)
+    L.check(L.lib().dust_pres_rhs_rot_dev(
+        m, t["cen"].data_ptr(), npart, t["pcen"].data_ptr(), t["pdir"].data_ptr(),
+        t["pmag"].data_ptr(), t["prv"].data_ptr(), t["pvel"].data_ptr(), nl, t["lver"].data_ptr(),
+        t["lmag"].data_ptr(), t["lvel"].data_ptr(), nw, wn.data_ptr(), t["wver"].data_ptr(),
+        t["wmag"].data_ptr(), t["wvel"].data_ptr(), tb.data_ptr(), None), "pres_rhs_rot_dev")
+    torch.cuda.synchronize()
+    assert np.array_equal(tb.cpu().numpy(), P.pres_rhs_rot(**w))
+
+
+@pytest.mark.parametrize("ns", [0, 1000, 1600])
+def test_solve_pressure_sys_wing(wing, ns):
+    """The surface-panel system of the wing (A_pres = the surface-panel AIC
+    before the wake correction, mod_linsys.f90:183-187) with a static block
+    of ns panels factored by dgetrf, right-hand side from the rotational
+    terms: solve_pressure_sys's solution is that of the full system, and it
+    leaves the same factors and pivots as solve_linsys."""
+    ver, nv = W.wing_case()[:2]
+    A0, _ = LS.aic_surfpan(ver, nv, with_source=False)
+    n = A0.shape[0]
+    b = P.pres_rhs_rot(**wing)
+    A = np.asfortranarray(A0.copy())
+    piv = np.zeros(n, np.int32)
+    if ns:
+        As = np.asfortranarray(A[:ns, :ns].copy())
+        p, info = LS.dgetrf(As)
+        assert info == 0
+        A[:ns, :ns] = As
+        piv[:ns] = p
+    A2, piv2 = A.copy(order="F"), piv.copy()
+    print(f"solve_pressure_sys n_sp={n} nstatic_sp={ns}", flush=True)
+    res, info = P.solve_pressure_sys(A, piv, ns, b)
+    assert info == 0
+    assert rel(res, np.linalg.solve(A0, b)) <= TOL
+    res2, info2 = LS.solve_linsys(A2, piv2, ns, b)
+    assert np.array_equal(res, res2) and np.array_equal(A, A2) and np.array_equal(piv, piv2)
