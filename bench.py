@@ -377,7 +377,17 @@ def time_elements(torch, L, W, n, steps, with_cpu=True):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 
+    t_pv = dv(np.random.default_rng(9).standard_normal((n, 3)))
+    t_b = torch.zeros(len(pc), dtype=torch.float64, device="cuda")
+
+    def pres():
+        L.check(lib.dust_pres_rhs_rot_dev(len(pc), t_pc.data_ptr(), n, t_cen.data_ptr(),
+                                          t_dir.data_ptr(), t_mag.data_ptr(), t_rv.data_ptr(),
+                                          t_pv.data_ptr(), 0, None, None, None, 0, None, None,
+                                          None, None, t_b.data_ptr(), None), "pres_rhs_rot_dev")
+
     ms_u = timed(uvort, steps)
+    ms_p = timed(pres, steps)
     ms_v = timed(lambda: apply(False), steps)
     ms_s = timed(lambda: apply(True), max(1, steps // 2))
     prm.precision = L.ELEM_TUNED
@@ -397,7 +407,8 @@ def time_elements(torch, L, W, n, steps, with_cpu=True):
                uvort_TFLOPs=UVORT_FLOP * len(pc) * n / (ms_u / 1e3) / 1e12,
                apply_vel_ms=ms_v, apply_vel_pairs_per_s=ne * n / (ms_v / 1e3),
                apply_vel_stretch_ms=ms_s, apply_vel_tuned_ms=ms_vt,
-               apply_vel_tuned_rel_l2_vs_fp64=tuned_err, dtype="f64 (tuned: f32 far field)")
+               apply_vel_tuned_rel_l2_vs_fp64=tuned_err, dtype="f64 (tuned: f32 far field)",
+               pres_rhs_rot_ms=ms_p, pres_rhs_rot_pairs_per_s=len(pc) * n / (ms_p / 1e3))
     if with_cpu:
         from oracle import oracle as orc
         ns = 4000
