@@ -187,6 +187,8 @@ SIGNATURES = [
      [C.c_int, _ip, _vp, _ip, _vp, _ip, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _dp]),
     ("dust_solve_linsys", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp]),
     ("dust_solve_pressure_sys", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp]),
+    ("dust_pres_rhs_src", C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp]),
+    ("dust_pres_rhs_src_dev", C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp, _vp]),
     ("dust_pres_rhs_rot", C.c_int,
      [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp,
       _vp, _vp, _vp]),

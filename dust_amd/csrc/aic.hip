@@ -273,4 +273,44 @@ int dust_aic_rhs_dev(int n, const int *n_ver, const double *ver, const double *B
   return 0;
 }
 
+// Source part (b) of the pressure right-hand side: b_pres = B_sp src with the
+// surface panels' source matrix and their bernoulli_source.
+int dust_pres_rhs_src_dev(int n_sp, const double *B, int ldb, const double *src, double *b_pres,
+                          void *stream) {
+  DUST_CHECK(n_sp >= 1 && B && src && b_pres && ldb >= n_sp, DUST_E_ARG,
+             "invalid pressure source-part arguments");
+  static DevBuf<double> Pt;
+  DUST_TRY(Pt.reserve((size_t)rhs_split(n_sp) * n_sp));
+  rhs_gemv(n_sp, 1, B, ldb, src, Pt.p, b_pres, n_sp, (hipStream_t)stream);
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
+
+int dust_pres_rhs_src(int n_sp, const double *B, int ldb, const double *src, double *b_pres) {
+  DUST_CHECK(n_sp >= 1 && B && src && b_pres && ldb >= n_sp, DUST_E_ARG,
+             "invalid pressure source-part arguments");
+  double *dB = nullptr, *dv = nullptr;
+  DUST_HIP(hipMalloc(&dB, sizeof(double) * (size_t)n_sp * n_sp));
+  if (hipMalloc(&dv, sizeof(double) * 2 * (size_t)n_sp) != hipSuccess) {
+    (void)hipFree(dB);
+    DUST_HIP(hipErrorOutOfMemory);
+  }
+  int rc = 0;
+  if (hipMemcpy2D(dB, sizeof(double) * n_sp, B, sizeof(double) * ldb, sizeof(double) * n_sp, n_sp,
+                  hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dv, src, sizeof(double) * n_sp, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("upload failed");
+    rc = DUST_E_HIP;
+  }
+  if (!rc) rc = dust_pres_rhs_src_dev(n_sp, dB, n_sp, dv, dv + n_sp, nullptr);
+  if (!rc && hipMemcpy(b_pres, dv + n_sp, sizeof(double) * n_sp, hipMemcpyDeviceToHost) !=
+                 hipSuccess) {
+    set_error("download failed");
+    rc = DUST_E_HIP;
+  }
+  (void)hipFree(dB);
+  (void)hipFree(dv);
+  return rc;
+}
+
 }  // extern "C"

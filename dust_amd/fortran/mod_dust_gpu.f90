@@ -94,6 +94,10 @@ module mod_dust_gpu
       real(c_double), intent(inout) :: A_pres(lda,*); integer(c_int), intent(inout) :: P_pres(*)
       real(c_double), intent(in) :: b_pres(*); real(c_double), intent(out) :: res_pres(*)
     end function
+    integer(c_int) function dust_pres_rhs_src(n_sp, B, ldb, src, b_pres) bind(C)
+      import; integer(c_int), value :: n_sp, ldb
+      real(c_double), intent(in) :: B(ldb,*), src(*); real(c_double), intent(out) :: b_pres(*)
+    end function
     integer(c_int) function dust_pres_rhs_rot(m, cen, np, pcen, pdir, pmag, prv, pvel, nl, &
                                               lver, lmag, lvel, nw, wnver, wver, wmag, wvel, &
                                               b_pres) bind(C)

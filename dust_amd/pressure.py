@@ -9,10 +9,12 @@
                                    static/moving elimination of the surface-
                                    panel system, as solve_linsys
 
-The source part (b) of the right-hand side, B_pres times the panels'
-bernoulli_source (mod_surfpan.f90:405-437), is a product with the source
-matrix of the AIC assembly; press_normvel_der (the panels' bernoulli_source
-from the neighbour stencils of the geometry) stays with the caller.
+    pres_rhs_src(B, src)           the source part (b): the source matrix of
+                                   the surface panels times their
+                                   bernoulli_source (mod_surfpan.f90:405-437)
+
+press_normvel_der (the panels' bernoulli_source from the neighbour stencils
+of the geometry) stays with the caller.
 """
 import numpy as np
 
@@ -56,6 +58,19 @@ def pres_rhs_rot(cen, b=None, pcen=None, pdir=None, pmag=None, prv=None, pvel=No
                                       L.ptr(pr), L.ptr(pv), nl, L.ptr(lv), L.ptr(lm), L.ptr(lu), nw,
                                       L.ptr(wn), L.ptr(wv), L.ptr(wm), L.ptr(wu), L.ptr(b)),
             "pres_rhs_rot")
+    return b
+
+
+def pres_rhs_src(B, src):
+    """The source part (b) of b_pres (mod_surfpan.f90:405-437): B_sp src, with
+    B the surface panels' source matrix (Fortran order) and src their
+    bernoulli_source.  Returns b_pres."""
+    n = B.shape[0]
+    if not (B.flags.f_contiguous and B.dtype == np.float64):
+        raise ValueError("B must be a Fortran-ordered float64 array")
+    src = _f64(src)
+    b = np.zeros(n)
+    L.check(L.lib().dust_pres_rhs_src(n, L.ptr(B), n, L.ptr(src), L.ptr(b)), "pres_rhs_src")
     return b
 
 

@@ -447,6 +447,16 @@ int dust_solve_linsys(int rank, int nstatic, double *A, int lda, int *ipiv, cons
 int dust_solve_pressure_sys(int n_sp, int nstatic_sp, double *A_pres, int lda, int *P_pres,
                             const double *b_pres, double *res_pres);
 
+/* Source part (b) of the pressure right-hand side, formed row by row in
+ * build_row_surfpan (mod_surfpan.f90:405-437): b_pres = B_sp src, with B_sp
+ * (ldb, n_sp) the surface panels' source matrix (b_static_pres for the static
+ * columns, the b1 of compute_pot for the moving ones) and src(n_sp) their
+ * bernoulli_source (press_normvel_der).  b_pres is overwritten.  Host
+ * pointers / device pointers ordered on `stream`. */
+int dust_pres_rhs_src(int n_sp, const double *B, int ldb, const double *src, double *b_pres);
+int dust_pres_rhs_src_dev(int n_sp, const double *B, int ldb, const double *src, double *b_pres,
+                          void *stream);
+
 /* assemble_pressure_sys (mod_pressure_equation.f90:139-236), rotational part
  * (c) of the right-hand side at the m surface-panel centres cen(3,m),
  * accumulated into b_pres(m) in the reference's order (particles, then

@@ -137,3 +137,22 @@ def test_solve_pressure_sys_wing(wing, ns):
     assert rel(res, np.linalg.solve(A0, b)) <= TOL
     res2, info2 = LS.solve_linsys(A2, piv2, ns, b)
     assert np.array_equal(res, res2) and np.array_equal(A, A2) and np.array_equal(piv, piv2)
+
+
+def test_pres_rhs_src_and_full_chain(wing):
+    """The source part (b) b_pres = B_sp src (mod_surfpan.f90:405-437) against
+    numpy, then the whole right-hand side (b) + (c) through
+    solve_pressure_sys against a direct solve."""
+    ver, nv = W.wing_case()[:2]
+    A0, B = LS.aic_surfpan(ver, nv)
+    n = A0.shape[0]
+    src = np.random.default_rng(4).standard_normal(n)
+    print(f"pres_rhs_src n_sp={n}", flush=True)
+    bs = P.pres_rhs_src(B, src)
+    assert rel(bs, B @ src) <= 1e-13
+    b = P.pres_rhs_rot(b=bs, **wing)
+    assert rel(b - bs, P.pres_rhs_rot(**wing)) <= 1e-11  # the terms add onto (b)
+    A = np.asfortranarray(A0.copy())
+    piv = np.zeros(n, np.int32)
+    res, info = P.solve_pressure_sys(A, piv, 0, b)
+    assert info == 0 and rel(res, np.linalg.solve(A0, b)) <= TOL
