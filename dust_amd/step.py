@@ -35,6 +35,7 @@ the C ABI; the host reads back the live count and the trailing-edge
 intensities (2 x segments values) once per step.
 """
 import ctypes as C
+import threading
 import time
 
 import numpy as np
@@ -59,7 +60,7 @@ def trailing_edge(ver, strips, nc):
 class BodyWakeStep:
     def __init__(self, ver, n_ver, te, u_inf, dt, nmax, tree, degree=4, form="fp64",
                  aic=None, r_shed=0.1, box=None, vs_elems=False, sub=1, omega=0.0,
-                 solver="fp64", strips=None, fmm_pan=False):
+                 solver="fp64", strips=None, fmm_pan=False, overlap=True):
         import torch
         self.torch = torch
         self.lib = L.lib()
@@ -86,6 +87,10 @@ class BodyWakeStep:
         self.solver = solver
         self.strips = strips  # (strips, nc) of the trailing-edge layout, moving bodies
         self.fmm_pan = bool(fmm_pan)
+        # the FMM pass depends only on the particles: with overlap it runs on
+        # its own (non-blocking) stream from a second host thread while this
+        # thread assembles and solves the panel system
+        self.overlap = bool(overlap) and not self.fmm_pan
         n = self.n
         # static AIC and its LU factors (initialize_linsys, mod_linsys.f90:236)
         self.A = torch.empty((n, n), dtype=torch.float64, device="cuda")
@@ -171,6 +176,22 @@ class BodyWakeStep:
         self.pool.shed(self.te_mid, pv, np.full(k, self.r_shed), np.tile(self.u_inf, (k, 1)),
                        self.wprm)
 
+    def _fmm_pass(self, npart, err):
+        """update_wake's FMM pass over the gathered particles into p[vel],
+        p[stretch]; returns when its stream is done.  Exceptions go to err."""
+        try:
+            lib, p, h = self.lib, self.p, self.fmm.handle
+            L.check(lib.dust_fmm_set_particles_dev(h, npart, p["cen"].data_ptr(),
+                                                   p["dir"].data_ptr(), p["mag"].data_ptr(),
+                                                   p["rv"].data_ptr()), "fmm set")
+            nl = C.c_int()
+            L.check(lib.dust_fmm_pass_dev(h, C.byref(nl)), "fmm pass")
+            L.check(lib.dust_fmm_results_dev(h, p["vel"].data_ptr(), p["stretch"].data_ptr(),
+                                             None), "fmm results")
+            L.check(lib.dust_fmm_synchronize(h), "fmm sync")
+        except BaseException as e:  # re-raised by the stepping thread
+            err.append(e)
+
     def step(self):
         torch, lib, p = self.torch, self.lib, self.p
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(7)]
@@ -198,6 +219,11 @@ class BodyWakeStep:
             L.check(lib.dust_fmm_calculate_multipole(h), "fmm multipole")
             L.check(lib.dust_fmm_synchronize(h), "fmm sync")
         ev[1].record()
+        fmm_job, fmm_err = None, []
+        if npart and self.overlap:
+            torch.cuda.synchronize()  # the gathered particle arrays
+            fmm_job = threading.Thread(target=self._fmm_pass, args=(npart, fmm_err))
+            fmm_job.start()
         moving = self.omega != 0.0
         if moving:  # moving body: geometry of the step, AIC re-assembled
             self._move()
@@ -259,16 +285,20 @@ class BodyWakeStep:
                                              C.byref(it), C.byref(res), None), "mixed solve")
         ev[3].record()
         # update_wake: FMM pass over the live particles
+        if fmm_job is not None:
+            fmm_job.join()
+            if fmm_err:
+                raise fmm_err[0]
         torch.cuda.synchronize()
-        if npart and self.fmm_pan:
-            L.check(lib.dust_fmm_apply_multipole_dev(h), "fmm apply")
-        elif npart:
-            L.check(lib.dust_fmm_set_particles_dev(h, npart, p["cen"].data_ptr(),
-                                                   p["dir"].data_ptr(), p["mag"].data_ptr(),
-                                                   p["rv"].data_ptr()), "fmm set")
-            nl = C.c_int()
-            L.check(lib.dust_fmm_pass_dev(h, C.byref(nl)), "fmm pass")
-        if npart:
+        if npart and fmm_job is None:
+            if self.fmm_pan:
+                L.check(lib.dust_fmm_apply_multipole_dev(h), "fmm apply")
+            else:
+                L.check(lib.dust_fmm_set_particles_dev(h, npart, p["cen"].data_ptr(),
+                                                       p["dir"].data_ptr(), p["mag"].data_ptr(),
+                                                       p["rv"].data_ptr()), "fmm set")
+                nl = C.c_int()
+                L.check(lib.dust_fmm_pass_dev(h, C.byref(nl)), "fmm pass")
             L.check(lib.dust_fmm_results_dev(h, p["vel"].data_ptr(), p["stretch"].data_ptr(),
                                              None), "fmm results")
             L.check(lib.dust_fmm_synchronize(h), "fmm sync")
@@ -293,7 +323,9 @@ class BodyWakeStep:
         ev[6].record()
         torch.cuda.synchronize()
         names = ("prepare", "assemble", "solve", "fmm", "elements", "complete")
-        # (moving bodies: "assemble" includes the AIC re-assembly)
+        # (moving bodies: "assemble" includes the AIC re-assembly; with
+        # overlap "fmm" is the part of the FMM pass not hidden behind the
+        # assembly and the solve, which it slows down while they share the GPU)
         for i, k in enumerate(names):
             self.times[k] = self.times.get(k, 0.0) + ev[i].elapsed_time(ev[i + 1])
         return npart
