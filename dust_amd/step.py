@@ -60,7 +60,7 @@ def trailing_edge(ver, strips, nc):
 class BodyWakeStep:
     def __init__(self, ver, n_ver, te, u_inf, dt, nmax, tree, degree=4, form="fp64",
                  aic=None, r_shed=0.1, box=None, vs_elems=False, sub=1, omega=0.0,
-                 solver="fp64", strips=None, fmm_pan=False, overlap=True):
+                 solver="fp64", strips=None, fmm_pan=False, overlap=False):
         import torch
         self.torch = torch
         self.lib = L.lib()
@@ -89,7 +89,10 @@ class BodyWakeStep:
         self.fmm_pan = bool(fmm_pan)
         # the FMM pass depends only on the particles: with overlap it runs on
         # its own (non-blocking) stream from a second host thread while this
-        # thread assembles and solves the panel system
+        # thread assembles and solves the panel system.  Off by default: both
+        # sides keep the chip busy on their own, so at 1M particles the step
+        # gains under 1 %, and at a few thousand particles the thread start
+        # costs more than it hides
         self.overlap = bool(overlap) and not self.fmm_pan
         n = self.n
         # static AIC and its LU factors (initialize_linsys, mod_linsys.f90:236)
