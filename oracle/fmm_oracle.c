@@ -1,7 +1,7 @@
 /*
  * fmm_oracle.c -- CPU restatement of DUST's Cartesian FMM for vortex particles.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP path: only
+ * TEST INFRASTRUCTURE ONLY.  The HIP path is compared against it: only
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
  * The product library (dust_amd/libdust_gpu.so) never links or calls it.
  *

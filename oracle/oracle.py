@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so) -- TEST INFRASTRUCTURE ONLY.
 
 Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
-where it is the checker (or the timed CPU baseline), never the product path.
+where it is what results are compared against (or the timed CPU baseline),
+never the product path.
 Build with `make -C oracle`.
 """
 import ctypes as C
