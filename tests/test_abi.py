@@ -67,3 +67,27 @@ def test_element_argument_errors_without_gpu():
                              ctypes.byref(prm), None) == L.DUST_E_ARG
     assert lib.dust_vort_vel(2, pos.ctypes.data, 3, None, None, None, None, 0, None, None,
                              ctypes.byref(prm), vel.ctypes.data) == L.DUST_E_ARG
+
+
+def test_pressure_argument_errors_without_gpu():
+    """The pressure entry points reject bad sizes and missing arrays before any
+    device work."""
+    import numpy as np
+    lib = L.lib()
+    cen = np.zeros(3)
+    b = np.zeros(1)
+    null = [None] * 5
+    assert lib.dust_pres_rhs_rot(-1, None, 0, *null, 0, None, None, None, 0, None, None, None,
+                                 None, None) == L.DUST_E_ARG
+    # particles announced without their arrays
+    assert lib.dust_pres_rhs_rot(1, cen.ctypes.data, 4, *null, 0, None, None, None, 0, None,
+                                 None, None, None, b.ctypes.data) == L.DUST_E_ARG
+    assert "particle" in lib.dust_last_error().decode()
+    # wake panels announced without their arrays
+    assert lib.dust_pres_rhs_rot(1, cen.ctypes.data, 0, *null, 0, None, None, None, 2, None,
+                                 None, None, None, b.ctypes.data) == L.DUST_E_ARG
+    A = np.zeros((2, 2), order="F")
+    piv = np.zeros(2, np.int32)
+    assert lib.dust_solve_pressure_sys(2, 3, A.ctypes.data, 2, piv.ctypes.data, b.ctypes.data,
+                                       b.ctypes.data) == L.DUST_E_ARG
+    assert lib.dust_pres_rhs_src(2, A.ctypes.data, 1, b.ctypes.data, b.ctypes.data) == L.DUST_E_ARG
