@@ -165,6 +165,7 @@ def time_lu(torch, L, W, n_req, nrhs, steps, mesh=None):
     Xp = X.clone()
     out["fp64_backward_error"] = berr(Xp)
     out["fp64_getrf_getrs_TFLOPs"] = ((2.0 / 3.0) * n ** 3 + 2.0 * n * n * nrhs) / (out["fp64_ms"] / 1e3) / 1e12
+    sols = {"fp64": Xp}
     for prec, code in (("fp16s", L.LU_FP16S), ("fp32", L.LU_FP32)):
         ts = []
         rc = 0
@@ -186,12 +187,26 @@ def time_lu(torch, L, W, n_req, nrhs, steps, mesh=None):
             out[f"{prec}_backward_error"] = berr(X2)
             out[f"{prec}_rel_diff_vs_fp64"] = float(((X2 - Xp).norm() / Xp.norm()).item())
             out[f"{prec}_speedup_vs_fp64"] = out["fp64_ms"] / out[f"{prec}_ms"]
+            sols[prec] = X2
         else:
             out[f"{prec}_error"] = L.lib().dust_last_error().decode()
     out["n"] = n
     out["nrhs"] = nrhs
     out["steps"] = steps
-    return out, (A, rhs_cm)
+    # roofline of the solves: the getrf work (2/3) N^3 over the whole solve
+    # (factorisation, triangular solves and refinement sweeps)
+    gf = (2.0 / 3.0) * n ** 3
+    roof = {"fp64": dict(bound="mfma", kernel="dgetrf + dgetrs (FP64 matrix cores)",
+                         achieved=gf / (out["fp64_ms"] / 1e3) / 1e12, peak=PEAK["fp64"],
+                         unit="TFLOP/s", algorithmic=f"(2/3) N^3 = {gf:.4g} flop per solve")}
+    if "fp16s_ms" in out:
+        roof["fp16s"] = dict(bound="mfma", kernel="scaled-FP16 LU + FP64 refinement",
+                             achieved=gf / (out["fp16s_ms"] / 1e3) / 1e12, peak=PEAK["fp16"],
+                             unit="TFLOP/s", algorithmic=f"(2/3) N^3 = {gf:.4g} flop per solve")
+    for v in roof.values():
+        v["frac"] = v["achieved"] / v["peak"]
+    out["roofline"] = roof
+    return out, (A, rhs_cm, sols)
 
 
 def time_batch(torch, L, O, W, ncase=48, reps=3):
@@ -584,27 +599,29 @@ def time_rotor_run(torch, W, n, nsteps=4):
     return out
 
 
-def cpu_baseline(W, n_sample, degree):
+def cpu_baseline(W, n_sample, degree, nlevels=7):
     """The CPU oracle (FP64 restatement, OpenMP) on a bounded sample of the same
-    generator; particle updates per second."""
+    generator; particle updates per second.  Also returns the oracle's
+    velocity and stretching (the GPU results are compared with them when the
+    sample is the timed workload itself)."""
     from oracle import oracle as orc
     cen, alpha, rv, tree = W.helix_wake(n_sample)
     mag = np.linalg.norm(alpha, axis=1)
     dr = alpha / mag[:, None]
-    lev = tree["nlevels"] - max(0, int(round(np.log(1_000_000 / n_sample) / np.log(8))))
+    lev = nlevels - max(0, int(round(np.log(1_000_000 / n_sample) / np.log(8))))
     c = orc.Octree(tree["box_length"], tree["nbox"], tree["origin"], lev, tree["min_part"], degree,
                    tree["delta"])
     t0 = time.perf_counter()
-    c.run(cen, dr, mag, rv)
+    vel, stretch, _ = c.run(cen, dr, mag, rv)
     dt = time.perf_counter() - t0
     return dict(value=n_sample / dt, unit="particle-updates/s", cores=orc.num_threads(),
                 kind="port",
                 sample=f"oracle FMM pass (sort+multipole+apply, FP64, OpenMP) on a "
                        f"{n_sample}-particle draw of the same helix generator, {lev} levels "
-                       f"(same particles per leaf), order {degree}, {dt:.1f} s")
+                       f"(same particles per leaf), order {degree}, {dt:.1f} s"), (vel, stretch)
 
 
-def cpu_lu_baseline(A, rhs_cm):
+def cpu_lu_baseline(A, rhs_cm, sols=None):
     """The reference's own LU call sites are LAPACK dgetrf + dgetrs: timed here
     with the host LAPACK that scipy links, on the same matrix."""
     import scipy.linalg.lapack as lapack
@@ -618,9 +635,14 @@ def cpu_lu_baseline(A, rhs_cm):
     x, inf2 = lapack.dgetrs(lu, piv, Bn)
     dt = time.perf_counter() - t0
     lib = ",".join(sorted({d.get("internal_api", "?") for d in info})) or "lapack"
-    return dict(ms=dt * 1e3, cores=cores, kind="reference",
-                sample=f"host LAPACK ({lib}) dgetrf + dgetrs, same N={An.shape[0]} matrix, "
-                       f"{Bn.shape[1]} right-hand sides")
+    out = dict(ms=dt * 1e3, cores=cores, kind="reference",
+               sample=f"host LAPACK ({lib}) dgetrf + dgetrs, same N={An.shape[0]} matrix, "
+                      f"{Bn.shape[1]} right-hand sides")
+    # the GPU solutions against the host LAPACK solution of the same system
+    for k, X in (sols or {}).items():
+        xg = X.cpu().numpy().T
+        out[f"x_rel_vs_lapack_{k}"] = float(np.linalg.norm(xg - x) / np.linalg.norm(x))
+    return out
 
 
 def copy_probe(torch, gib=2.0, reps=5):
@@ -855,8 +877,19 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("timing CPU oracle sample")
-        cpu = cpu_baseline(W, args.cpu_sample, args.degree)
+        cpu, (ovel, ost) = cpu_baseline(W, args.cpu_sample, args.degree, tree["nlevels"])
         log(f"cpu: {cpu}")
+        if args.cpu_sample == n:
+            # the oracle ran on the timed workload itself: full-size parity
+            acc = acc or {}
+            for key, res in (("fp64", plain), ("tuned", tuned)):
+                if res is None:
+                    continue
+                acc[f"{key}_vel_rel_l2_vs_oracle"] = float(
+                    np.linalg.norm(res["vel"] - ovel) / np.linalg.norm(ovel))
+                acc[f"{key}_stretch_rel_l2_vs_oracle"] = float(
+                    np.linalg.norm(res["stretch"] - ost) / np.linalg.norm(ost))
+            log(f"accuracy vs oracle: {acc}")
         if lu_mats is not None:
             log("timing host LAPACK on the AIC system")
             lu["cpu_baseline"] = cpu_lu_baseline(*lu_mats)
@@ -890,6 +923,7 @@ def main():
         "accuracy": acc,
         "work": {"p2p_pairs": tuned["p2p_pairs"], "m2l_pairs": tuned["m2l_pairs"]},
         "roofline": roof,
+        "roofline_lu": None if not lu else lu.get("roofline"),
         "lu": lu,
         "wing": wing,
         "wing_run": wing_run,
@@ -898,7 +932,10 @@ def main():
         "batch": batch,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out, default=float))
+    print(json.dumps(out, default=float), flush=True)
+    # release the library's workspaces, streams and events before the HIP
+    # runtime is torn down at exit
+    L.check(L.lib().dust_shutdown(), "shutdown")
 
 
 if __name__ == "__main__":

@@ -198,6 +198,8 @@ SIGNATURES = [
     ("dust_test_gemm_sub_dev", C.c_int,
      [C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int]),
     ("dust_test_trsm_llnu_dev", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int]),
+    ("dust_test_getrs_set_error_word", C.c_int, [C.c_uint]),
+    ("dust_shutdown", C.c_int, []),
 ]
 
 _lib = None

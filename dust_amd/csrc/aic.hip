@@ -128,20 +128,21 @@ static void rhs_gemv(int n, int nrhs, const double *B, long ldb, const double *S
   k_rhs_gemv_sum<<<ceil_div(n, 256), 256, 0, st>>>(n, nrhs, Y, part, b, ldbo);
 }
 
-static int geo_upload(int n, const int *n_ver, const double *ver, Panel **dP, hipStream_t st) {
+// Panel records built on the device from host vertex lists; *dP is owned
+// by the caller's TmpDev.
+static int geo_upload(int n, const int *n_ver, const double *ver, TmpDev<Panel> &dP,
+                      hipStream_t st) {
   for (int p = 0; p < n; ++p)
     DUST_CHECK(n_ver[p] == 3 || n_ver[p] == 4, DUST_E_ARG, "panel %d has %d vertices", p, n_ver[p]);
-  int *dn = nullptr;
-  double *dv = nullptr;
-  DUST_HIP(hipMalloc(&dn, sizeof(int) * n));
-  DUST_HIP(hipMalloc(&dv, sizeof(double) * 12 * n));
-  DUST_HIP(hipMalloc(dP, sizeof(Panel) * n));
-  DUST_HIP(hipMemcpyAsync(dn, n_ver, sizeof(int) * n, hipMemcpyHostToDevice, st));
-  DUST_HIP(hipMemcpyAsync(dv, ver, sizeof(double) * 12 * n, hipMemcpyHostToDevice, st));
-  k_geo<<<ceil_div(n, 128), 128, 0, st>>>(n, dn, dv, *dP);
+  TmpDev<int> dn;
+  TmpDev<double> dv;
+  DUST_TRY(dn.alloc(n));
+  DUST_TRY(dv.alloc(12 * (size_t)n));
+  DUST_TRY(dP.alloc(n));
+  DUST_HIP(hipMemcpyAsync(dn.p, n_ver, sizeof(int) * n, hipMemcpyHostToDevice, st));
+  DUST_HIP(hipMemcpyAsync(dv.p, ver, sizeof(double) * 12 * n, hipMemcpyHostToDevice, st));
+  k_geo<<<ceil_div(n, 128), 128, 0, st>>>(n, dn.p, dv.p, dP.p);
   DUST_HIP(hipStreamSynchronize(st));
-  (void)hipFree(dn);
-  (void)hipFree(dv);
   return 0;
 }
 
@@ -150,32 +151,33 @@ static int aic_common(bool surfpan, int n, const int *n_ver, const double *ver,
                       bool dev, hipStream_t st) {
   DUST_CHECK(n >= 1 && n_ver && ver && A && prm, DUST_E_ARG, "invalid AIC arguments");
   DUST_CHECK(lda >= n && (!B || ldb >= n), DUST_E_ARG, "leading dimension smaller than n");
-  Panel *dP = nullptr;
-  int *dn = nullptr;
-  double *dv = nullptr;
+  TmpDev<Panel> dP;
   if (dev) {
     // n_ver and ver are device pointers
-    DUST_HIP(hipMalloc(&dP, sizeof(Panel) * n));
-    k_geo<<<ceil_div(n, 128), 128, 0, st>>>(n, n_ver, ver, dP);
+    DUST_TRY(dP.alloc(n));
+    k_geo<<<ceil_div(n, 128), 128, 0, st>>>(n, n_ver, ver, dP.p);
   } else {
-    DUST_TRY(geo_upload(n, n_ver, ver, &dP, st));
+    DUST_TRY(geo_upload(n, n_ver, ver, dP, st));
   }
-  (void)dn;
-  (void)dv;
+  TmpDev<double> tA, tB;
   double *dA = A, *dB = B;
   if (!dev) {
-    DUST_HIP(hipMalloc(&dA, sizeof(double) * (size_t)n * n));
-    if (B) DUST_HIP(hipMalloc(&dB, sizeof(double) * (size_t)n * n));
+    DUST_TRY(tA.alloc((size_t)n * n));
+    dA = tA.p;
+    if (B) {
+      DUST_TRY(tB.alloc((size_t)n * n));
+      dB = tB.p;
+    }
   }
   long la = dev ? lda : n, lb = dev ? ldb : n;
-  int *dwarn = nullptr;
-  DUST_HIP(hipMalloc(&dwarn, sizeof(int)));
-  DUST_HIP(hipMemsetAsync(dwarn, 0, sizeof(int), st));
+  TmpDev<int> dwarn;
+  DUST_TRY(dwarn.alloc(1));
+  DUST_HIP(hipMemsetAsync(dwarn.p, 0, sizeof(int), st));
   dim3 grid(ceil_div(n, 256), n);
   if (surfpan)
-    k_aic_surfpan<<<grid, 256, 0, st>>>(n, dP, prm->ff_ratio_doublet, dA, la, dB, lb, dwarn);
+    k_aic_surfpan<<<grid, 256, 0, st>>>(n, dP.p, prm->ff_ratio_doublet, dA, la, dB, lb, dwarn.p);
   else
-    k_aic_vortlatt<<<grid, 256, 0, st>>>(n, dP, prm->ff_ratio_doublet, prm->rankine_rad,
+    k_aic_vortlatt<<<grid, 256, 0, st>>>(n, dP.p, prm->ff_ratio_doublet, prm->rankine_rad,
                                          prm->cutoff_rad, dA, la, dB, lb);
   DUST_HIP(hipGetLastError());
   if (!dev) {
@@ -186,13 +188,22 @@ static int aic_common(bool surfpan, int n, const int *n_ver, const double *ver,
                                 sizeof(double) * n, n, hipMemcpyDeviceToHost, st));
   }
   DUST_HIP(hipStreamSynchronize(st));
-  if (!dev) {
-    (void)hipFree(dA);
-    if (B) (void)hipFree(dB);
-  }
-  (void)hipFree(dP);
-  (void)hipFree(dwarn);
   return 0;
+}
+
+// Workspaces of the device right-hand-side entry points, reused across calls.
+struct AicWs {
+  DevBuf<Panel> P;
+  DevBuf<double> S, U, Pt, Ptp;
+};
+AicWs g_aws;
+
+void aic_release() {
+  g_aws.P.release();
+  g_aws.S.release();
+  g_aws.U.release();
+  g_aws.Pt.release();
+  g_aws.Ptp.release();
 }
 
 }  // namespace dust
@@ -222,27 +233,24 @@ int dust_aic_rhs(int n, const int *n_ver, const double *ver, const double *B, in
              "invalid RHS arguments (nrhs must be 1..%d)", kRhsMax);
   DUST_CHECK(ldb >= n && ldbo >= n, DUST_E_ARG, "leading dimension smaller than n");
   hipStream_t st = nullptr;
-  Panel *dP = nullptr;
-  DUST_TRY(geo_upload(n, n_ver, ver, &dP, st));
-  double *dB = nullptr, *dS = nullptr, *du = nullptr, *dw = nullptr, *db = nullptr;
-  DUST_HIP(hipMalloc(&dB, sizeof(double) * (size_t)n * n));
-  DUST_HIP(hipMalloc(&dS, sizeof(double) * (size_t)n * nrhs));
-  DUST_HIP(hipMalloc(&du, sizeof(double) * 3 * nrhs));
-  DUST_HIP(hipMalloc(&db, sizeof(double) * (size_t)n * nrhs));
-  if (uvort) DUST_HIP(hipMalloc(&dw, sizeof(double) * 3 * n));
-  DUST_HIP(hipMemcpy2D(dB, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, n,
+  TmpDev<Panel> dP;
+  DUST_TRY(geo_upload(n, n_ver, ver, dP, st));
+  TmpDev<double> dB, dS, du, dw, db, dpart;
+  DUST_TRY(dB.alloc((size_t)n * n));
+  DUST_TRY(dS.alloc((size_t)n * nrhs));
+  DUST_TRY(du.alloc(3 * (size_t)nrhs));
+  DUST_TRY(db.alloc((size_t)n * nrhs));
+  if (uvort) DUST_TRY(dw.alloc(3 * (size_t)n));
+  DUST_TRY(dpart.alloc((size_t)rhs_split(n) * nrhs * n));
+  DUST_HIP(hipMemcpy2D(dB.p, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, n,
                        hipMemcpyHostToDevice));
-  DUST_HIP(hipMemcpy(du, u_inf, sizeof(double) * 3 * nrhs, hipMemcpyHostToDevice));
-  if (uvort) DUST_HIP(hipMemcpy(dw, uvort, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-  k_rhs_coef<<<ceil_div(n, 256), 256>>>(n, nrhs, dP, du, dw, dS);
-  double *dpart = nullptr;
-  DUST_HIP(hipMalloc(&dpart, sizeof(double) * (size_t)rhs_split(n) * nrhs * n));
-  rhs_gemv(n, nrhs, dB, n, dS, dpart, db, n, nullptr);
-  DUST_HIP(hipMemcpy2D(b, sizeof(double) * ldbo, db, sizeof(double) * n, sizeof(double) * n, nrhs,
+  DUST_HIP(hipMemcpy(du.p, u_inf, sizeof(double) * 3 * nrhs, hipMemcpyHostToDevice));
+  if (uvort) DUST_HIP(hipMemcpy(dw.p, uvort, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  k_rhs_coef<<<ceil_div(n, 256), 256>>>(n, nrhs, dP.p, du.p, dw.p, dS.p);
+  rhs_gemv(n, nrhs, dB.p, n, dS.p, dpart.p, db.p, n, nullptr);
+  DUST_HIP(hipGetLastError());
+  DUST_HIP(hipMemcpy2D(b, sizeof(double) * ldbo, db.p, sizeof(double) * n, sizeof(double) * n, nrhs,
                        hipMemcpyDeviceToHost));
-  (void)hipFree(dP); (void)hipFree(dB); (void)hipFree(dS); (void)hipFree(du); (void)hipFree(db);
-  (void)hipFree(dpart);
-  if (dw) (void)hipFree(dw);
   return 0;
 }
 
@@ -253,8 +261,8 @@ int dust_aic_rhs_dev(int n, const int *n_ver, const double *ver, const double *B
              DUST_E_ARG, "invalid RHS arguments (nrhs must be 1..%d)", kRhsMax);
   DUST_CHECK(ldb >= n && ldbo >= n, DUST_E_ARG, "leading dimension smaller than n");
   hipStream_t st = (hipStream_t)stream;
-  static DevBuf<Panel> P;
-  static DevBuf<double> S, U, Pt;
+  DevBuf<Panel> &P = g_aws.P;
+  DevBuf<double> &S = g_aws.S, &U = g_aws.U, &Pt = g_aws.Pt;
   DUST_TRY(P.reserve(n));
   DUST_TRY(U.reserve(3 * (size_t)kRhsMax));
   double *dS = sigma;
@@ -279,7 +287,7 @@ int dust_pres_rhs_src_dev(int n_sp, const double *B, int ldb, const double *src,
                           void *stream) {
   DUST_CHECK(n_sp >= 1 && B && src && b_pres && ldb >= n_sp, DUST_E_ARG,
              "invalid pressure source-part arguments");
-  static DevBuf<double> Pt;
+  DevBuf<double> &Pt = g_aws.Ptp;
   DUST_TRY(Pt.reserve((size_t)rhs_split(n_sp) * n_sp));
   rhs_gemv(n_sp, 1, B, ldb, src, Pt.p, b_pres, n_sp, (hipStream_t)stream);
   DUST_HIP(hipGetLastError());
@@ -289,28 +297,15 @@ int dust_pres_rhs_src_dev(int n_sp, const double *B, int ldb, const double *src,
 int dust_pres_rhs_src(int n_sp, const double *B, int ldb, const double *src, double *b_pres) {
   DUST_CHECK(n_sp >= 1 && B && src && b_pres && ldb >= n_sp, DUST_E_ARG,
              "invalid pressure source-part arguments");
-  double *dB = nullptr, *dv = nullptr;
-  DUST_HIP(hipMalloc(&dB, sizeof(double) * (size_t)n_sp * n_sp));
-  if (hipMalloc(&dv, sizeof(double) * 2 * (size_t)n_sp) != hipSuccess) {
-    (void)hipFree(dB);
-    DUST_HIP(hipErrorOutOfMemory);
-  }
-  int rc = 0;
-  if (hipMemcpy2D(dB, sizeof(double) * n_sp, B, sizeof(double) * ldb, sizeof(double) * n_sp, n_sp,
-                  hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(dv, src, sizeof(double) * n_sp, hipMemcpyHostToDevice) != hipSuccess) {
-    set_error("upload failed");
-    rc = DUST_E_HIP;
-  }
-  if (!rc) rc = dust_pres_rhs_src_dev(n_sp, dB, n_sp, dv, dv + n_sp, nullptr);
-  if (!rc && hipMemcpy(b_pres, dv + n_sp, sizeof(double) * n_sp, hipMemcpyDeviceToHost) !=
-                 hipSuccess) {
-    set_error("download failed");
-    rc = DUST_E_HIP;
-  }
-  (void)hipFree(dB);
-  (void)hipFree(dv);
-  return rc;
+  TmpDev<double> dB, dv;
+  DUST_TRY(dB.alloc((size_t)n_sp * n_sp));
+  DUST_TRY(dv.alloc(2 * (size_t)n_sp));
+  DUST_HIP(hipMemcpy2D(dB.p, sizeof(double) * n_sp, B, sizeof(double) * ldb, sizeof(double) * n_sp,
+                       n_sp, hipMemcpyHostToDevice));
+  DUST_HIP(hipMemcpy(dv.p, src, sizeof(double) * n_sp, hipMemcpyHostToDevice));
+  DUST_TRY(dust_pres_rhs_src_dev(n_sp, dB.p, n_sp, dv.p, dv.p + n_sp, nullptr));
+  DUST_HIP(hipMemcpy(b_pres, dv.p + n_sp, sizeof(double) * n_sp, hipMemcpyDeviceToHost));
+  return 0;
 }
 
 }  // extern "C"

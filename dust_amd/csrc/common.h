@@ -39,6 +39,17 @@ void set_error(const char *fmt, ...);
     if (s_ != 0) return s_;                                                       \
   } while (0)
 
+// Release of the library-global workspaces of each translation unit
+// (buffers, page-locked records, streams, events), for dust_shutdown.  The
+// caller has synchronised the device.
+void aic_release();
+void elem_release();
+void fmm_release();
+void lu_release();
+void mixed_release();
+void pres_release();
+void wake_release();
+
 constexpr double kPi = 3.1415926535897932384626433832795028841971693993751;
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
@@ -65,6 +76,28 @@ struct DevBuf {
     if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
+  }
+};
+
+// Device allocation owned by one scope (the staging buffers of the
+// host-pointer entry points): freed on every return path.
+template <typename T>
+struct TmpDev {
+  T *p = nullptr;
+  TmpDev() = default;
+  TmpDev(const TmpDev &) = delete;
+  TmpDev &operator=(const TmpDev &) = delete;
+  ~TmpDev() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t count) {
+    size_t bytes = (count ? count : 1) * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      p = nullptr;
+      set_error("hipMalloc of %zu bytes failed", bytes);
+      return DUST_E_NOMEM;
+    }
+    return 0;
   }
 };
 

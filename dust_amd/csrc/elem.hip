@@ -738,6 +738,16 @@ struct ElemWs {
 };
 ElemWs g_ws;
 
+}  // namespace
+void elem_release() {
+  ElemWs &w = g_ws;
+  w.rec.release(); w.nmask.release(); w.part.release(); w.packed.release();
+  w.key.release(); w.key_s.release(); w.idx.release(); w.perm.release();
+  w.spos.release(); w.sal.release(); w.svel.release(); w.sst.release(); w.bb.release();
+  w.tmp.release(); w.epart.release(); w.h_d.release(); w.h_i.release();
+}
+namespace {
+
 int check_groups(int ngroups, const dust_elem_group *groups) {
   DUST_CHECK(ngroups >= 0 && (ngroups == 0 || groups), DUST_E_ARG, "invalid element groups");
   for (int k = 0; k < ngroups; ++k) {

@@ -1483,12 +1483,20 @@ __global__ void __launch_bounds__(256)
   uvort[3 * i + 2] += v[2] + v2 * f;
 }
 
+// leaf of every panel centre and the out-of-box report of apply_multipole_panels
+static DevBuf<int> g_pan_leaf, g_pan_err;
+
+void fmm_release() {
+  g_pan_leaf.release();
+  g_pan_err.release();
+}
+
 static int fmm_apply_panels(FmmHandle *h, int m, const double *cen, double *uvort) {
   DUST_CHECK(h && h->computed, DUST_E_ARG,
              "calculate_multipole must precede apply_multipole_panels");
   DUST_CHECK(m >= 0 && (m == 0 || (cen && uvort)), DUST_E_ARG, "invalid panel arguments");
   if (m == 0) return 0;
-  static DevBuf<int> pleaf, perr;
+  DevBuf<int> &pleaf = g_pan_leaf, &perr = g_pan_err;
   DUST_TRY(pleaf.reserve(m));
   DUST_TRY(perr.reserve(1));
   DUST_TRY(h->rb.reserve(1));
@@ -1522,7 +1530,20 @@ extern "C" {
 
 const char *dust_last_error(void) { return g_err; }
 
-const char *dust_version(void) { return "libdust_gpu 0.1 (gfx950)"; }
+const char *dust_version(void) { return "libdust_gpu 0.2 (gfx950)"; }
+
+int dust_shutdown(void) {
+  // the workspaces may still be in use by queued work
+  DUST_HIP(hipDeviceSynchronize());
+  aic_release();
+  elem_release();
+  fmm_release();
+  lu_release();
+  mixed_release();
+  pres_release();
+  wake_release();
+  return 0;
+}
 
 int dust_fmm_create(const dust_fmm_params *prm, int capacity, dust_fmm_t *out) {
   DUST_CHECK(out, DUST_E_ARG, "null output handle");

@@ -8,10 +8,11 @@
 //        [b(:,m); c(:,:,m)] += (-1)^|m| sum_n mult(m,n) cross(D(m+n) | Dc(:,i,m+n), a(:,n))
 //    of 12 n_mon rows by 3 n_mon columns, so a whole (level, octant) group is
 //    one GEMM  Y(12 n_mon x targets) += [W_o1 ... W_o189] x [A_o1; ...; A_o189]
-//    with K = 189 x 3 n_mon.  The W tiles are generated on the fly in LDS from
-//    the compact kernel-derivative table, the B operand gathers the FP32
-//    source multipoles, and v_mfma_f32_16x16x4f32 accumulates in FP32; the
-//    result is added into the FP64 local expansions.
+//    with K = 189 x 3 n_mon.  The operators W are precomputed per (level,
+//    offset) in FP32 and streamed in MFMA fragment order, the B operand
+//    gathers the FP32 source multipoles, and v_mfma_f32_16x16x4f32
+//    accumulates in FP32; the result is added into the FP64 local
+//    expansions.
 //  * P2P in FP32: coordinates relative to the target leaf centre, rsqrt on
 //    the transcendental unit, per-tile FP32 partial sums folded into FP64
 //    accumulators.  Pairs closer than the source core radius (times a
@@ -422,8 +423,9 @@ __device__ inline void near64(double r0, double r1, double r2, double al0, doubl
 
 // FP32 P2P of one item (tuned form).  Sources are staged per wavefront as
 // FP32 records (offset from the target leaf centre, r^2; alpha = dir*mag,
-// near2 r^2) and FP64 records (offset, alpha, r).  Pairs with 0 < d^2 <= near2 r_s^2
-// (inside the source core) are left out of the FP32 sums and evaluated in
+// near2 r^2) and FP64 records (offset, alpha, r).  Pairs with d^2 <= near2 r_s^2
+// (inside the source core; the self pair is dropped by index) are left out
+// of the FP32 sums and evaluated in
 // FP64 after the tile, all lanes together (a per-lane mask of tile
 // positions), so the rare FP64 work does not diverge the FP32 loop.  The
 // FP32 sums of every tile are folded into FP64 accumulators.  A partial tile
@@ -491,10 +493,12 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
         const float r0 = fx - sp[0], r1 = fy - sp[1], r2 = fz - sp[2];
         const float a0 = sa[0], a1 = sa[1], a2 = sa[2];
         const float d2 = r0 * r0 + r1 * r1 + r2 * r2;
-        // pairs inside the source core go to FP64; d = 0 is the self pair
+        // pairs inside the source core go to FP64, all but the self pair
+        // (skipped by index, as the reference does: a distinct particle at
+        // the same position still adds its stretching term)
         const bool skip = d2 <= sa[3];
         if (__builtin_expect(__any(skip), 0)) {
-          if (skip && d2 > 0.f) nmask |= 1ull << s;
+          if (skip && s0 + s != t) nmask |= 1ull << s;
         }
         const float R2 = d2 + sp[3];
         const float inv = __builtin_amdgcn_rsqf(R2);

@@ -86,30 +86,22 @@ int solve_block_host(int rank, int nstatic, double *A, int lda, int *ipiv, const
                      double *res) {
   DUST_CHECK(rank >= 1 && nstatic >= 0 && nstatic <= rank && lda >= rank && A && ipiv && b && res,
              DUST_E_ARG, "invalid solve_linsys arguments");
-  double *dA = nullptr, *dr = nullptr;
-  int *dp = nullptr;
-  DUST_HIP(hipMalloc(&dA, sizeof(double) * (size_t)rank * rank));
-  DUST_HIP(hipMalloc(&dr, sizeof(double) * rank));
-  DUST_HIP(hipMalloc(&dp, sizeof(int) * rank));
-  DUST_HIP(hipMemcpy2D(dA, sizeof(double) * rank, A, sizeof(double) * lda, sizeof(double) * rank,
+  TmpDev<double> dA, dr;
+  TmpDev<int> dp;
+  DUST_TRY(dA.alloc((size_t)rank * rank));
+  DUST_TRY(dr.alloc(rank));
+  DUST_TRY(dp.alloc(rank));
+  DUST_HIP(hipMemcpy2D(dA.p, sizeof(double) * rank, A, sizeof(double) * lda, sizeof(double) * rank,
                        rank, hipMemcpyHostToDevice));
-  DUST_HIP(hipMemcpy(dp, ipiv, sizeof(int) * rank, hipMemcpyHostToDevice));
-  DUST_HIP(hipMemcpy(dr, b, sizeof(double) * rank, hipMemcpyHostToDevice));
+  DUST_HIP(hipMemcpy(dp.p, ipiv, sizeof(int) * rank, hipMemcpyHostToDevice));
+  DUST_HIP(hipMemcpy(dr.p, b, sizeof(double) * rank, hipMemcpyHostToDevice));
   int info = 0;
-  int rc = solve_linsys_dev(rank, nstatic, dA, rank, dp, dr, nullptr, &info);
-  if (!rc) {
-    if (hipMemcpy2D(A, sizeof(double) * lda, dA, sizeof(double) * rank, sizeof(double) * rank, rank,
-                    hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(ipiv, dp, sizeof(int) * rank, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(res, dr, sizeof(double) * rank, hipMemcpyDeviceToHost) != hipSuccess) {
-      set_error("download failed");
-      rc = DUST_E_HIP;
-    }
-  }
-  (void)hipFree(dA);
-  (void)hipFree(dr);
-  (void)hipFree(dp);
-  return rc ? rc : info;
+  DUST_TRY(solve_linsys_dev(rank, nstatic, dA.p, rank, dp.p, dr.p, nullptr, &info));
+  DUST_HIP(hipMemcpy2D(A, sizeof(double) * lda, dA.p, sizeof(double) * rank, sizeof(double) * rank,
+                       rank, hipMemcpyDeviceToHost));
+  DUST_HIP(hipMemcpy(ipiv, dp.p, sizeof(int) * rank, hipMemcpyDeviceToHost));
+  DUST_HIP(hipMemcpy(res, dr.p, sizeof(double) * rank, hipMemcpyDeviceToHost));
+  return info;
 }
 
 }  // namespace

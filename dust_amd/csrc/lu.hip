@@ -18,6 +18,7 @@
 #include <mutex>
 #include <vector>
 
+#include "../../include/dust_gpu_test.h"
 #include "common.h"
 #include "lu_internal.h"
 
@@ -362,6 +363,12 @@ __global__ void k_unpack_y(const double *__restrict__ y, int n, int nq, double *
   for (int q = 0; q < nq; ++q) B[i + ldb * q] = y[(long)i * SR + q];
 }
 
+// One word set on the stream (a kernel, not a memset: the solve may be
+// recorded into a graph).
+__global__ void k_set_word(unsigned *p, unsigned v) {
+  if (threadIdx.x == 0) *p = v;
+}
+
 // ---------------------------------------------------------------------------
 // Host drivers
 // ---------------------------------------------------------------------------
@@ -388,8 +395,9 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t st,
   size_t wbytes = 256 + sizeof(double) * 2 * Gmax + sizeof(int) * 2 * Gmax +
                   sizeof(double) * 2 * (size_t)Gmax * nb + sizeof(double) * 2 * nb + 64 +
                   sizeof(unsigned) * (Gmax + 16);
-  char *ws = nullptr;
-  DUST_HIP(hipMalloc(&ws, wbytes));
+  TmpDev<char> wsb;
+  DUST_TRY(wsb.alloc(wbytes));
+  char *ws = wsb.p;
   DUST_HIP(hipMemsetAsync(ws, 0, wbytes, st));
   unsigned *bar = (unsigned *)ws;
   int *dinfo = (int *)(ws + 16);
@@ -466,7 +474,6 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t st,
     DUST_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, st));
   }
   DUST_HIP(hipStreamSynchronize(st));
-  (void)hipFree(ws);
   if (rc) return rc;
   DUST_CHECK(hb[2] == 0, DUST_E_HIP, "panel grid barrier timed out");
   *info_out = hinfo;
@@ -479,6 +486,21 @@ struct GetrsWs {
   unsigned gen = 0;
 };
 static GetrsWs g_getrs;
+
+void lu_release() {
+  g_getrs.linv.release();
+  g_getrs.uinv.release();
+  g_getrs.y.release();
+  g_getrs.flags.release();
+  g_getrs.gen = 0;
+}
+
+// Test hook: the error word of the getrs workspace (when allocated).
+int lu_getrs_set_error_word(unsigned v) {
+  if (!g_getrs.flags.p) return 0;
+  DUST_HIP(hipMemcpy(g_getrs.flags.p, &v, sizeof(unsigned), hipMemcpyHostToDevice));
+  return 0;
+}
 
 int lu_getrs_dev(int n, int nrhs, const double *dA, long lda, const int *dipiv, double *dB,
                  long ldb, hipStream_t st) {
@@ -503,6 +525,9 @@ int lu_getrs_dev(int n, int nrhs, const double *dA, long lda, const int *dipiv, 
                              (int)lds);
   });
   DUST_HIP(ae);
+  // the error word starts every call at zero: a time-out of an earlier call
+  // must not fail this one (the ready flags are generation-stamped)
+  k_set_word<<<1, 64, 0, st>>>(w.flags.p, 0u);
   k_laswp_rhs<<<1, 256, 0, st>>>(dB, ldb, nrhs, n, dipiv);
   k_diag_inverse<<<nblk, 256, lds, st>>>(dA, lda, n, w.linv.p, w.uinv.p);
   DUST_HIP(hipGetLastError());
@@ -586,26 +611,19 @@ int dust_dgetrf(int m, int n, double *A, int lda, int *ipiv) {
   DUST_CHECK(A && ipiv && m >= 0 && n >= 0 && lda >= std::max(1, m), DUST_E_ARG,
              "invalid dgetrf arguments");
   if (m == 0 || n == 0) return 0;
-  double *dA = nullptr;
-  int *dp = nullptr;
-  int mn = std::min(m, n);
-  DUST_HIP(hipMalloc(&dA, sizeof(double) * (size_t)m * n));
-  DUST_HIP(hipMalloc(&dp, sizeof(int) * mn));
-  DUST_HIP(hipMemcpy2D(dA, sizeof(double) * m, A, sizeof(double) * lda, sizeof(double) * m, n,
+  const int mn = std::min(m, n);
+  TmpDev<double> dA;
+  TmpDev<int> dp;
+  DUST_TRY(dA.alloc((size_t)m * n));
+  DUST_TRY(dp.alloc(mn));
+  DUST_HIP(hipMemcpy2D(dA.p, sizeof(double) * m, A, sizeof(double) * lda, sizeof(double) * m, n,
                        hipMemcpyHostToDevice));
   int info = 0;
-  int rc = lu_getrf_dev(m, n, dA, m, dp, nullptr, 64, &info);
-  if (!rc) {
-    if (hipMemcpy2D(A, sizeof(double) * lda, dA, sizeof(double) * m, sizeof(double) * m, n,
-                    hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(ipiv, dp, sizeof(int) * mn, hipMemcpyDeviceToHost) != hipSuccess) {
-      set_error("download failed");
-      rc = DUST_E_HIP;
-    }
-  }
-  (void)hipFree(dA);
-  (void)hipFree(dp);
-  return rc ? rc : info;
+  DUST_TRY(lu_getrf_dev(m, n, dA.p, m, dp.p, nullptr, 64, &info));
+  DUST_HIP(hipMemcpy2D(A, sizeof(double) * lda, dA.p, sizeof(double) * m, sizeof(double) * m, n,
+                       hipMemcpyDeviceToHost));
+  DUST_HIP(hipMemcpy(ipiv, dp.p, sizeof(int) * mn, hipMemcpyDeviceToHost));
+  return info;
 }
 
 // Test hooks on device pointers: the trailing-update GEMM, the unit-lower
@@ -621,32 +639,27 @@ int dust_test_trsm_llnu_dev(int m, int n, const double *L, int ldl, double *B, i
   DUST_HIP(hipDeviceSynchronize());
   return 0;
 }
+int dust_test_getrs_set_error_word(unsigned v) { return lu_getrs_set_error_word(v); }
 
 int dust_dgetrs(int n, int nrhs, const double *A, int lda, const int *ipiv, double *B, int ldb) {
   DUST_CHECK(A && ipiv && B && n >= 0 && nrhs >= 0 && lda >= std::max(1, n) &&
                  ldb >= std::max(1, n),
              DUST_E_ARG, "invalid dgetrs arguments");
   if (n == 0 || nrhs == 0) return 0;
-  double *dA = nullptr, *dB = nullptr;
-  int *dp = nullptr;
-  DUST_HIP(hipMalloc(&dA, sizeof(double) * (size_t)n * n));
-  DUST_HIP(hipMalloc(&dB, sizeof(double) * (size_t)n * nrhs));
-  DUST_HIP(hipMalloc(&dp, sizeof(int) * n));
-  DUST_HIP(hipMemcpy2D(dA, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, n,
+  TmpDev<double> dA, dB;
+  TmpDev<int> dp;
+  DUST_TRY(dA.alloc((size_t)n * n));
+  DUST_TRY(dB.alloc((size_t)n * nrhs));
+  DUST_TRY(dp.alloc(n));
+  DUST_HIP(hipMemcpy2D(dA.p, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, n,
                        hipMemcpyHostToDevice));
-  DUST_HIP(hipMemcpy2D(dB, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
+  DUST_HIP(hipMemcpy2D(dB.p, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
                        hipMemcpyHostToDevice));
-  DUST_HIP(hipMemcpy(dp, ipiv, sizeof(int) * n, hipMemcpyHostToDevice));
-  int rc = lu_getrs_dev(n, nrhs, dA, n, dp, dB, n, nullptr);
-  if (!rc && hipMemcpy2D(B, sizeof(double) * ldb, dB, sizeof(double) * n, sizeof(double) * n, nrhs,
-                         hipMemcpyDeviceToHost) != hipSuccess) {
-    set_error("download failed");
-    rc = DUST_E_HIP;
-  }
-  (void)hipFree(dA);
-  (void)hipFree(dB);
-  (void)hipFree(dp);
-  return rc;
+  DUST_HIP(hipMemcpy(dp.p, ipiv, sizeof(int) * n, hipMemcpyHostToDevice));
+  DUST_TRY(lu_getrs_dev(n, nrhs, dA.p, n, dp.p, dB.p, n, nullptr));
+  DUST_HIP(hipMemcpy2D(B, sizeof(double) * ldb, dB.p, sizeof(double) * n, sizeof(double) * n, nrhs,
+                       hipMemcpyDeviceToHost));
+  return 0;
 }
 
 }  // extern "C"

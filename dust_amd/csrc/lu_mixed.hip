@@ -1830,7 +1830,43 @@ int mixed_batch(int nb, const int *n, const double *const *dA, const int *lda, d
   return 0;
 }
 
+void ws_release(MixedWs &w) {
+  w.W.release(); w.rs.release(); w.rowmax.release(); w.Lf.release(); w.Uf.release();
+  w.piv.release(); w.perm.release(); w.npiv.release(); w.dinv.release(); w.Linv.release();
+  w.Uinv.release(); w.flags.release(); w.y.release(); w.b8.release(); w.x8.release();
+  w.r8.release(); w.part.release(); w.rowabs.release(); w.A64.release(); w.norms.release();
+  w.ipiv64.release(); w.hb.release();
+  w.gen = 0;
+  w.dev = -1;
+  for (hipEvent_t *e : {&w.ev_in, &w.ev_start, &w.ev_panel, &w.ev_rest})
+    if (*e) {
+      (void)hipEventDestroy(*e);
+      *e = nullptr;
+    }
+  for (hipStream_t *q : {&w.s1, &w.s2})
+    if (*q) {
+      (void)hipStreamDestroy(*q);
+      *q = nullptr;
+    }
+}
+
 }  // namespace
+
+void mixed_release() {
+  ws_release(g_ws);
+  BatchState &B = g_batch;
+  for (auto &w : B.ws) ws_release(*w);
+  B.ws.clear();
+  if (B.st) (void)hipStreamDestroy(B.st);
+  B.st = nullptr;
+  B.dplan.release();
+  B.hplan.release();
+  B.flags.release();
+  B.norms.release();
+  B.hflags.release();
+  B.hnorms.release();
+}
+
 }  // namespace dust
 
 using namespace dust;

@@ -135,6 +135,15 @@ struct PresWs {
 };
 PresWs g_pws;
 
+}  // namespace
+void pres_release() {
+  g_pws.packed.release();
+  g_pws.part.release();
+  g_pws.h_d.release();
+  g_pws.h_i.release();
+}
+namespace {
+
 int pres_rhs_rot_dev(int m, const double *cen, int np, const double *pcen, const double *pdir,
                      const double *pmag, const double *prv, const double *pvel, int nl,
                      const double *lver, const double *lmag, const double *lvel, int nw,
@@ -147,6 +156,9 @@ int pres_rhs_rot_dev(int m, const double *cen, int np, const double *pcen, const
              "missing particle arrays");
   DUST_CHECK(nl == 0 || !lmag || (lver && lvel), DUST_E_ARG, "missing end-vortex arrays");
   DUST_CHECK(nw == 0 || (wnver && wver && wmag && wvel), DUST_E_ARG, "missing wake-panel arrays");
+  for (int w = 0; w < nw; ++w)
+    DUST_CHECK(wnver[w] == 3 || wnver[w] == 4, DUST_E_ARG, "wake panel %d has %d vertices", w,
+               wnver[w]);
   if (m == 0) return 0;
   const int tb = ceil_div(m, kPT);
   int S = 0, per = 0;
@@ -210,6 +222,9 @@ int dust_pres_rhs_rot(int m, const double *cen, int np, const double *pcen, cons
              "missing particle arrays");
   DUST_CHECK(nl == 0 || !lmag || (lver && lvel), DUST_E_ARG, "missing end-vortex arrays");
   DUST_CHECK(nw == 0 || (wnver && wver && wmag && wvel), DUST_E_ARG, "missing wake-panel arrays");
+  for (int w = 0; w < nw; ++w)
+    DUST_CHECK(wnver[w] == 3 || wnver[w] == 4, DUST_E_ARG, "wake panel %d has %d vertices", w,
+               wnver[w]);
   if (m == 0) return 0;
   const bool lines = nl > 0 && lmag;
   size_t nd = 4 * (size_t)m + 11 * (size_t)np + (lines ? 13 * (size_t)nl : 0) + 25 * (size_t)nw;

@@ -300,6 +300,16 @@ struct WakeWs {
 };
 WakeWs g_wws;
 
+}  // namespace
+void wake_release() {
+  g_wws.pa.release();
+  g_wws.list.release();
+  g_wws.count.release();
+  g_wws.tmp.release();
+  g_wws.hcount.release();
+}
+namespace {
+
 int select_slots(int nmax, const unsigned char *freef, bool want_free, int *out, int *d_count,
                  hipStream_t st) {
   hipcub::CountingInputIterator<int> it(0);

@@ -469,6 +469,9 @@ int dust_pres_rhs_src_dev(int n_sp, const double *B, int ldb, const double *src,
  *   - the nw wake panels wver(3,4,nw) (wnver vertices each, triangles
  *     padded) of strength wmag with vertex velocities wvel(3,4,nw) in the
  *     vertex order (pan_w_vel at the panel's points; 204-225), edge by edge.
+ *     Every wnver entry must be 3 or 4: the host entry point returns
+ *     DUST_E_ARG otherwise; the device entry point does not read the list on
+ *     the host and treats any value other than 3 as 4.
  * Host pointers, synchronous. */
 int dust_pres_rhs_rot(int m, const double *cen, int np, const double *pcen, const double *pdir,
                       const double *pmag, const double *prv, const double *pvel, int nl,
@@ -483,12 +486,13 @@ int dust_pres_rhs_rot_dev(int m, const double *cen, int np, const double *pcen,
                           const double *wmag, const double *wvel, double *b_pres,
                           void *stream);
 
-/* Test hooks (no counterpart in the reference): the building blocks of
- * dust_dgetrf on device pointers, synchronous.  C(M,N) -= A(M,K) B(K,N) in
- * FP64 on the matrix cores; B(m,n) <- L^-1 B with L unit lower (m x m). */
-int dust_test_gemm_sub_dev(int M, int N, int K, const double *A, int lda, const double *B,
-                           int ldb, double *C, int ldc);
-int dust_test_trsm_llnu_dev(int m, int n, const double *L, int ldl, double *B, int ldb);
+/* Release every library-global workspace (device buffers, page-locked
+ * records, streams and events of the solvers and element kernels) after
+ * synchronising the device; they are re-created on the next call that needs
+ * them.  FMM handles are not touched (dust_fmm_destroy releases each).  For
+ * callers that tear the GPU runtime down before process exit.  No
+ * counterpart in the reference (its LAPACK calls hold no state). */
+int dust_shutdown(void);
 
 /* Library build/version information. */
 const char *dust_version(void);

@@ -228,6 +228,29 @@ int ao_assemble_surfpan(int n, const double *G, double ff_ratio_dou, double *A, 
   return warn;
 }
 
+/* Selected rows of the same matrices (the rows of build_row_static_surfpan
+ * for the panels rows[0..nr)): Ar(r,j) = A(rows[r],j), Br(r,j) =
+ * B(rows[r],j), both nr x n column-major with leading dimension nr. */
+int ao_assemble_surfpan_rows(int n, const double *G, double ff_ratio_dou, int nr, const int *rows,
+                             double *Ar, double *Br) {
+  int warn = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : warn)
+  for (int j = 0; j < n; ++j) {
+    const double *gj = &G[(size_t)G_SIZE * j];
+    for (int r = 0; r < nr; ++r) {
+      const int i = rows[r];
+      const double *pos = &G[(size_t)G_SIZE * i + G_CEN];
+      double dou = (i != j) ? ao_pot_doublet(gj, pos, ff_ratio_dou) : -2.0 * AO_PI;
+      Ar[(size_t)r + (size_t)nr * j] = -dou;
+      int w = 0;
+      double sou = ao_pot_source(gj, dou, pos, &w);
+      warn += w;
+      Br[(size_t)r + (size_t)nr * j] = sou;
+    }
+  }
+  return warn;
+}
+
 /* Static AIC of a vortex-lattice system (build_row_static_vortlatt with
  * compute_psi_vortlatt): A(i,j) = v_ring_j(cen_i).nor_i, B(i,i) = 4 pi. */
 void ao_assemble_vortlatt(int n, const double *G, double ff_ratio_dou, double r_rankine,

@@ -57,6 +57,8 @@ def lib():
             "ao_vel_doublet": (None, [_vp, _vp, C.c_double, C.c_double, C.c_double, _vp]),
             "ao_assemble_surfpan": (C.c_int, [C.c_int, _vp, C.c_double, _vp, C.c_int, _vp,
                                               C.c_int]),
+            "ao_assemble_surfpan_rows": (C.c_int, [C.c_int, _vp, C.c_double, C.c_int, _vp, _vp,
+                                                   _vp]),
             "ao_assemble_vortlatt": (None, [C.c_int, _vp, C.c_double, C.c_double, C.c_double, _vp,
                                             C.c_int, _vp, C.c_int]),
             "ao_rhs_static": (None, [C.c_int, _vp, _vp, C.c_int, _vp, _vp, _vp]),
@@ -284,6 +286,17 @@ def aic_surfpan(G, ff_ratio=10.0, with_source=True):
     B = np.zeros((n, n), order="F") if with_source else None
     lib().ao_assemble_surfpan(n, _p(_c(G)), float(ff_ratio), _p(A), n, _p(B), n)
     return A, B
+
+
+def aic_surfpan_rows(G, rows, ff_ratio=10.0):
+    """Rows `rows` of the surface-panel A and B (len(rows) x n each)."""
+    n = G.shape[0]
+    rows = _c(rows, np.int32)
+    Ar = np.zeros((len(rows), n), order="F")
+    Br = np.zeros((len(rows), n), order="F")
+    lib().ao_assemble_surfpan_rows(n, _p(_c(G)), float(ff_ratio), len(rows), _p(rows), _p(Ar),
+                                   _p(Br))
+    return Ar, Br
 
 
 def aic_vortlatt(G, ff_ratio=10.0, rankine=0.1, cutoff=0.001):

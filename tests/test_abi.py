@@ -9,10 +9,11 @@ from dust_amd import _lib as L
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 HEADER = os.path.join(HERE, "..", "include", "dust_gpu.h")
+TEST_HEADER = os.path.join(HERE, "..", "include", "dust_gpu_test.h")
 
 
-def declared():
-    src = open(HEADER).read()
+def declared(header=HEADER):
+    src = open(header).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(dust_[a-z0-9_]+)\s*\(", src)))
 
@@ -27,13 +28,20 @@ def test_header_declares_entry_points():
 
 def test_library_exports_every_declared_symbol():
     lib = L.lib()
-    missing = [n for n in declared() if not hasattr(lib, n)]
+    missing = [n for n in declared() + declared(TEST_HEADER) if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_test_hooks_stay_out_of_the_product_header():
+    hooks = declared(TEST_HEADER)
+    assert hooks and all(n.startswith("dust_test_") for n in hooks)
+    assert not [n for n in declared() if n.startswith("dust_test_")]
 
 
 def test_python_signatures_match_header():
     sig = {s[0] for s in L.SIGNATURES}
-    assert set(declared()) <= sig | {"dust_last_error", "dust_version"}, set(declared()) - sig
+    names = set(declared()) | set(declared(TEST_HEADER))
+    assert names <= sig | {"dust_last_error", "dust_version"}, names - sig
 
 
 def test_version_and_error_text():
@@ -86,6 +94,14 @@ def test_pressure_argument_errors_without_gpu():
     # wake panels announced without their arrays
     assert lib.dust_pres_rhs_rot(1, cen.ctypes.data, 0, *null, 0, None, None, None, 2, None,
                                  None, None, None, b.ctypes.data) == L.DUST_E_ARG
+    # a wake panel with a vertex count other than 3 or 4
+    wn = np.array([4, 5], np.int32)
+    wv = np.zeros((2, 4, 3))
+    wm = np.zeros(2)
+    assert lib.dust_pres_rhs_rot(1, cen.ctypes.data, 0, *null, 0, None, None, None, 2,
+                                 wn.ctypes.data, wv.ctypes.data, wm.ctypes.data, wv.ctypes.data,
+                                 b.ctypes.data) == L.DUST_E_ARG
+    assert "vertices" in lib.dust_last_error().decode()
     A = np.zeros((2, 2), order="F")
     piv = np.zeros(2, np.int32)
     assert lib.dust_solve_pressure_sys(2, 3, A.ctypes.data, 2, piv.ctypes.data, b.ctypes.data,

@@ -189,6 +189,26 @@ def test_tuned_vs_oracle(orc, nlev, p):
     check_leaves(parts, g, c)
 
 
+def test_tuned_coincident_particles(orc):
+    """Distinct particles at the same position: the tuned P2P drops only the
+    self pair (by index) and evaluates the coincident pair in FP64, whose
+    stretching term is not zero under the Rosenhead core."""
+    cen, alpha, rv, tree = W.helix_wake(6000, seed=21)
+    rng = np.random.default_rng(22)
+    k = 300
+    cen = np.concatenate([cen, cen[:k]])
+    alpha = np.concatenate([alpha, rng.normal(0.0, 1e-3, (k, 3))])
+    rv = np.concatenate([rv, rv[:k]])
+    tree = dict(tree, nlevels=4)
+    parts, g, nl, c, v, s, tv = run_both(orc, cen, alpha, rv, tree, 4, use_vs=True,
+                                         precision="tuned")
+    dup = np.r_[0:k, len(rv) - k:len(rv)]
+    ev, es, ed = rel(parts.vel, v), rel(parts.stretch, s), rel(parts.stretch[dup], s[dup])
+    print(f"coincident: vel {ev:.2e} stretch {es:.2e} duplicated particles {ed:.2e}")
+    assert ev <= TUNED_VEL
+    assert es <= TUNED_STR and ed <= TUNED_STR
+
+
 def test_tuned_velocity_only_and_nbox(orc):
     rng = np.random.default_rng(3)
     n = 5000

@@ -55,3 +55,49 @@ def test_panel_tall(m, n):
     assert info == info_s
     assert np.array_equal(ipiv - 1, piv)
     assert np.abs(A - lu).max() <= 1e-12 * np.abs(lu).max()
+
+
+def test_getrs_recovers_after_error_word():
+    """The dgetrs error word starts every call at zero: an earlier time-out
+    (simulated by writing the word from the host) does not fail later
+    solves."""
+    import scipy.linalg.lapack as lapack
+    from dust_amd import linsys as LS
+    rng = np.random.default_rng(5)
+    n = 300
+    A0 = rng.standard_normal((n, n)) + 0.1 * np.eye(n)
+    b = rng.standard_normal((n, 2))
+    A = np.asfortranarray(A0.copy())
+    ipiv, info = LS.dgetrf(A)
+    assert info == 0
+    x = np.asfortranarray(b.copy())
+    LS.dgetrs(A, ipiv, x)  # allocates the workspace
+    L.check(L.lib().dust_test_getrs_set_error_word(1), "poison")
+    y = np.asfortranarray(b.copy())
+    LS.dgetrs(A, ipiv, y)
+    assert np.array_equal(x, y)
+    lu, piv, _ = lapack.dgetrf(A0)
+    ref, _ = lapack.dgetrs(lu, piv, b)
+    assert np.abs(y - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_shutdown_then_reuse():
+    """dust_shutdown releases the library workspaces; the next calls
+    re-create them and give the same results."""
+    from dust_amd import linsys as LS
+    rng = np.random.default_rng(6)
+    n = 200
+    A0 = rng.standard_normal((n, n)) + 0.1 * np.eye(n)
+    b = rng.standard_normal((n, 1))
+    A = np.asfortranarray(A0.copy())
+    ipiv, _ = LS.dgetrf(A)
+    x = np.asfortranarray(b.copy())
+    LS.dgetrs(A, ipiv, x)
+    A0 = np.asfortranarray(A0)
+    xm = LS.dgesv_mixed(A0, np.asfortranarray(b.copy()), prec="fp16s")[0].copy()
+    L.check(L.lib().dust_shutdown(), "shutdown")
+    y = np.asfortranarray(b.copy())
+    LS.dgetrs(A, ipiv, y)
+    assert np.array_equal(x, y)
+    ym = LS.dgesv_mixed(A0, np.asfortranarray(b.copy()), prec="fp16s")[0]
+    assert np.array_equal(xm, ym)

@@ -159,10 +159,12 @@ def test_fmm_pan_steps_match_oracle(orc, case):
 
 
 @pytest.mark.parametrize("form,solver", [("fp64", "fp64"), ("tuned", "fp16s")])
-def test_rotor_overlap_is_bit_identical(rotor_case, form, solver):
+def test_rotor_overlap_matches_sequential(rotor_case, form, solver):
     """The FMM pass run from a second host thread on its own stream while the
-    panel system is assembled and solved gives the same state, bit for bit,
-    as the sequential order."""
+    panel system is assembled and solved gives the same state as the
+    sequential order: bit for bit in the FP64 form; within 1e-12 in the tuned
+    form, whose M2L adds into the FP64 locals with atomics from several jobs
+    per target cell (the order of those additions varies from run to run)."""
     a, _ = make_rotor(rotor_case, form, solver, overlap=False)
     b, _ = make_rotor(rotor_case, form, solver, overlap=True)
     assert not a.overlap and b.overlap
@@ -171,5 +173,9 @@ def test_rotor_overlap_is_bit_identical(rotor_case, form, solver):
         a.step()
         b.step()
         ga, gb = a.state(), b.state()
-        for key in ("mu", "uvort", "vel", "cen", "free"):
-            assert np.array_equal(ga[key], gb[key]), key
+        assert np.array_equal(ga["free"], gb["free"])
+        for key in ("mu", "uvort", "vel", "cen"):
+            if form == "fp64":
+                assert np.array_equal(ga[key], gb[key]), key
+            else:
+                assert rel(gb[key], ga[key]) <= 1e-12, key
