@@ -114,6 +114,26 @@ class PaParams(C.Structure):
     ]
 
 
+ROW_SURFPAN = 0
+ROW_VORTLATT = 1
+
+
+class LinsysDesc(C.Structure):
+    _fields_ = [
+        ("rank", C.c_int), ("nstatic", C.c_int),
+        ("kind", _vp), ("n_ver", _vp), ("ver", _vp), ("ub", _vp), ("uvort", _vp),
+        ("bern", _vp),
+        ("u_inf", C.c_double * 3),
+        ("n_sp", C.c_int), ("nstatic_sp", C.c_int), ("id_sp", _vp),
+        ("n_wake", C.c_int), ("n_impl", C.c_int),
+        ("wake_nver", _vp), ("wake_ver", _vp), ("wake_mag", _vp), ("impl_ind", _vp),
+        ("n_expl", C.c_int), ("nstatic_expl", C.c_int),
+        ("expl_nver", _vp), ("expl_ver", _vp), ("expl_mag", _vp),
+        ("ff_ratio_doublet", C.c_double), ("rankine_rad", C.c_double),
+        ("cutoff_rad", C.c_double),
+    ]
+
+
 # (name, restype, argtypes) of every exported function in include/dust_gpu.h
 SIGNATURES = [
     ("dust_last_error", C.c_char_p, []),
@@ -186,6 +206,14 @@ SIGNATURES = [
     ("dust_dgesv_mixed_batch_dev", C.c_int,
      [C.c_int, _ip, _vp, _ip, _vp, _ip, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _dp]),
     ("dust_solve_linsys", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp]),
+    ("dust_initialize_linsys", C.c_int,
+     [_vp, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int, _vp, _vp]),
+    ("dust_initialize_linsys_dev", C.c_int,
+     [_vp, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp]),
+    ("dust_assemble_linsys", C.c_int,
+     [_vp, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, _vp]),
+    ("dust_assemble_linsys_dev", C.c_int,
+     [_vp, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, _vp, _vp]),
     ("dust_solve_pressure_sys", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp]),
     ("dust_pres_rhs_src", C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp]),
     ("dust_pres_rhs_src_dev", C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp, _vp]),

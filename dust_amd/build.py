@@ -32,6 +32,7 @@ SOURCES = [
     "lu.hip",
     "lu_mixed.hip",
     "linsys.hip",
+    "linsys_asm.hip",
     "pressure.hip",
 ]
 

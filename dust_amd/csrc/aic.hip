@@ -12,6 +12,7 @@
 // (mod_surfpan.f90:400-404, 448-481; mod_vortlatt.f90:158-164, 201-237).
 #include <cmath>
 
+#include "aic_internal.h"
 #include "panel.h"
 
 namespace dust {
@@ -112,15 +113,15 @@ __global__ void k_rhs_gemv_sum(int n, int nrhs, int Y, const double *__restrict_
 }
 
 // Launch of the two kernels; `part` holds Y*nrhs*n doubles (rhs_split).
-static int rhs_split(int n) {
+int rhs_split(int n) {
   int xb = ceil_div(n, kRhsRows);
   int Y = ceil_div(1024, xb);
   int ymax = ceil_div(n, 64);  // at least 64 columns per range
   return Y < ymax ? Y : ymax;
 }
 
-static void rhs_gemv(int n, int nrhs, const double *B, long ldb, const double *S, double *part,
-                     double *b, long ldbo, hipStream_t st) {
+void rhs_gemv(int n, int nrhs, const double *B, long ldb, const double *S, double *part,
+              double *b, long ldbo, hipStream_t st) {
   int Y = rhs_split(n);
   int per = ceil_div(n, Y);
   Y = ceil_div(n, per);

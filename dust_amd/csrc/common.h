@@ -43,6 +43,7 @@ void set_error(const char *fmt, ...);
 // (buffers, page-locked records, streams, events), for dust_shutdown.  The
 // caller has synchronised the device.
 void aic_release();
+void asm_release();
 void elem_release();
 void fmm_release();
 void lu_release();

@@ -1536,6 +1536,7 @@ int dust_shutdown(void) {
   // the workspaces may still be in use by queued work
   DUST_HIP(hipDeviceSynchronize());
   aic_release();
+  asm_release();
   elem_release();
   fmm_release();
   lu_release();

@@ -207,4 +207,41 @@ __device__ inline void vel_doublet(const Panel &g, const double pos[3], double f
   }
 }
 
+// velocity_calc_sou_surfpan (mod_surfpan.f90:268-357): the routine's own
+// far-field ratio is the constant 10; its near field takes the doublet
+// potential with the doublet far-field ratio ffr.  The PANAIR edge cosines
+// cosTi = edge_uni.t1, sinTi = edge_uni.t2 of calc_geo_data_surfpan are formed
+// here.
+__device__ inline void vel_source(const Panel &g, const double pos[3], double ffr, double v[3]) {
+  const double rv[3] = {pos[0] - g.cen[0], pos[1] - g.cen[1], pos[2] - g.cen[2]};
+  const double radius = sqrt(dot3_rn(rv, rv));
+  double phix, phiy, pdou;
+  if (radius > 10.0 * g.maxedge) {
+    const double r3 = radius * radius * radius;
+    phix = -g.area * dot3(rv, g.t1) / r3;
+    phiy = -g.area * dot3(rv, g.t2) / r3;
+    pdou = -g.area * dot3(rv, g.nor) / r3;
+  } else {
+    phix = 0.0;
+    phiy = 0.0;
+    const int nv = g.nv;
+    for (int i1 = 0; i1 < nv; ++i1) {
+      const int ip1 = (i1 + 1 < nv) ? i1 + 1 : 0;
+      double a[3], b[3];
+      for (int e = 0; e < 3; ++e) {
+        a[e] = pos[e] - g.verp[i1][e];
+        b[e] = pos[e] - g.verp[ip1][e];
+      }
+      const double R1 = sqrt(dot3(a, a)), R2 = sqrt(dot3(b, b)), el = g.elen[i1];
+      const double sl = (R1 + R2 - el < 1e-12) ? 0.0 : log((R1 + R2 + el) / (R1 + R2 - el));
+      phix += dot3(g.euni[i1], g.t2) * sl;
+      phiy -= dot3(g.euni[i1], g.t1) * sl;
+    }
+    pdou = -pot_doublet(g, pos, ffr);
+    phix = -phix;
+    phiy = -phiy;
+  }
+  for (int e = 0; e < 3; ++e) v[e] = g.t1[e] * phix + g.t2[e] * phiy + g.nor[e] * pdou;
+}
+
 }  // namespace dust

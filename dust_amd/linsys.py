@@ -128,3 +128,107 @@ def dgesv_mixed(A, B, prec="fp16s", tol=1e-12, max_iter=30):
     if info > 0:
         raise np.linalg.LinAlgError(f"dgesv_mixed: info = {info}")
     return B, it.value, res.value
+
+
+# ---- initialize_linsys / assemble_linsys with the wake ------------------------
+ROW_SURFPAN = L.ROW_SURFPAN
+ROW_VORTLATT = L.ROW_VORTLATT
+
+
+class System:
+    """Inputs of initialize_linsys / assemble_linsys (src/linsys/mod_linsys.f90:
+    109-272, 279-344): the implicit elements (row kinds ROW_SURFPAN /
+    ROW_VORTLATT, vertices, the nstatic static ones first), their body
+    velocity ub and uvort, the Bernoulli source bern of the surface panels,
+    the surface-panel list id_sp (0-based, the nstatic_sp static ones first),
+    the wake panels (`wake`: dict ver, n_ver, mag, n_impl, impl_ind (n_impl, 2)
+    1-based pan_gen_elems_id with 0 for a missing lower element) and the
+    explicit elements (`expl`: dict ver, n_ver, mag, nstatic)."""
+
+    def __init__(self, kind, ver, n_ver, nstatic, ub=None, uvort=None, bern=None,
+                 u_inf=(0.0, 0.0, 0.0), id_sp=None, nstatic_sp=0, wake=None, expl=None,
+                 ff_ratio=10.0, rankine=0.1, cutoff=0.001):
+        self.ver, self.n_ver = _panels(ver, n_ver)
+        self.rank = len(self.n_ver)
+        self.kind = np.ascontiguousarray(kind, dtype=np.int32)
+        self.nstatic = int(nstatic)
+        f = lambda a, sh: None if a is None else np.ascontiguousarray(
+            np.asarray(a, np.float64).reshape(sh))
+        self.ub = f(ub, (self.rank, 3))
+        self.uvort = f(uvort, (self.rank, 3))
+        self.bern = f(bern, (self.rank,))
+        self.u_inf = tuple(float(x) for x in u_inf)
+        self.id_sp = np.zeros(0, np.int32) if id_sp is None else np.ascontiguousarray(
+            id_sp, dtype=np.int32)
+        self.n_sp = len(self.id_sp)
+        self.nstatic_sp = int(nstatic_sp)
+        w = wake or {}
+        if w:
+            self.wake_ver, self.wake_nver = _panels(w["ver"], w["n_ver"])
+            self.wake_mag = f(w.get("mag", np.zeros(len(self.wake_nver))), (-1,))
+            self.n_impl = int(w.get("n_impl", 0))
+            self.impl_ind = np.ascontiguousarray(
+                np.asarray(w.get("impl_ind", np.zeros((0, 2)))).reshape(-1, 2), dtype=np.int32)
+        else:
+            self.wake_ver, self.wake_nver, self.wake_mag = None, None, None
+            self.n_impl, self.impl_ind = 0, np.zeros((0, 2), np.int32)
+        self.n_wake = 0 if self.wake_nver is None else len(self.wake_nver)
+        e = expl or {}
+        if e:
+            self.expl_ver, self.expl_nver = _panels(e["ver"], e["n_ver"])
+            self.expl_mag = f(e["mag"], (-1,))
+            self.nstatic_expl = int(e.get("nstatic", 0))
+        else:
+            self.expl_ver, self.expl_nver, self.expl_mag = None, None, None
+            self.nstatic_expl = 0
+        self.n_expl = 0 if self.expl_nver is None else len(self.expl_nver)
+        self.ff_ratio, self.rankine, self.cutoff = float(ff_ratio), float(rankine), float(cutoff)
+
+    ARRAYS = ("kind", "n_ver", "ver", "ub", "uvort", "bern", "id_sp", "wake_nver", "wake_ver",
+              "wake_mag", "impl_ind", "expl_nver", "expl_ver", "expl_mag")
+
+    def desc(self, ptrs=None):
+        """LinsysDesc over the host arrays (or over `ptrs`: name -> address)."""
+        d = L.LinsysDesc()
+        d.rank, d.nstatic = self.rank, self.nstatic
+        for k in self.ARRAYS:
+            a = getattr(self, k)
+            setattr(d, k, (ptrs or {}).get(k) if ptrs is not None else
+                    (None if a is None or a.size == 0 else a.ctypes.data))
+        d.u_inf = (C.c_double * 3)(*self.u_inf)
+        d.n_sp, d.nstatic_sp = self.n_sp, self.nstatic_sp
+        d.n_wake, d.n_impl = self.n_wake, self.n_impl
+        d.n_expl, d.nstatic_expl = self.n_expl, self.nstatic_expl
+        d.ff_ratio_doublet, d.rankine_rad, d.cutoff_rad = self.ff_ratio, self.rankine, self.cutoff
+        return d
+
+
+def initialize_linsys(S):
+    """Returns dict(A, B, L_static, A_pres, ipiv, ipiv_pres, info); A and
+    A_pres hold the LU factors of their static blocks."""
+    n, nst = S.rank, S.nstatic
+    st = dict(A=np.zeros((n, n), order="F"), B=np.zeros((n, n), order="F"),
+              L_static=np.zeros((max(nst, 1), max(S.nstatic_expl, 1)), order="F"),
+              A_pres=np.zeros((max(S.n_sp, 1), max(S.n_sp, 1)), order="F"),
+              ipiv=np.zeros(max(nst, 1), np.int32), ipiv_pres=np.zeros(max(S.nstatic_sp, 1),
+                                                                        np.int32))
+    d = S.desc()
+    st["info"] = L.check(L.lib().dust_initialize_linsys(
+        C.byref(d), L.ptr(st["A"]), n, L.ptr(st["B"]), n, L.ptr(st["L_static"]),
+        st["L_static"].shape[0], L.ptr(st["A_pres"]), st["A_pres"].shape[0], L.ptr(st["ipiv"]),
+        L.ptr(st["ipiv_pres"])), "initialize_linsys")
+    return st
+
+
+def assemble_linsys(S, st, pressure=False):
+    """One assembly into the state of initialize_linsys (A, B, A_pres updated
+    in place).  Returns b, or (b, b_pres) with pressure=True."""
+    n = S.rank
+    b = np.zeros(n)
+    bp = np.zeros(max(S.n_sp, 1)) if pressure else None
+    d = S.desc()
+    L.check(L.lib().dust_assemble_linsys(
+        C.byref(d), L.ptr(st["A"]), n, L.ptr(st["B"]), n, L.ptr(st["L_static"]),
+        st["L_static"].shape[0], L.ptr(b), L.ptr(st["A_pres"]) if S.n_sp else None,
+        st["A_pres"].shape[0], L.ptr(bp)), "assemble_linsys")
+    return (b, bp[:S.n_sp]) if pressure else b

@@ -435,6 +435,84 @@ int dust_solve_linsys(int rank, int nstatic, double *A, int lda, int *ipiv, cons
                       double *res);
 
 /* ======================================================================== */
+/* Assembly of the system with its wake: src/linsys/mod_linsys.f90          */
+/* initialize_linsys (109-272) and assemble_linsys (279-344)                 */
+/* ======================================================================== */
+
+#define DUST_ROW_SURFPAN 0  /* t_surfpan: compute_pot rows (mod_surfpan.f90:377-481) */
+#define DUST_ROW_VORTLATT 1 /* t_vortlatt: compute_psi rows (mod_vortlatt.f90:143-237) */
+
+/* The implicit elements (linsys%rank of them, the nstatic static ones
+ * first), the wake panels of wake%pan_p (the n_impl implicit first-row
+ * panels first, with pan_gen_elems_id), the explicit elements (lifting-line
+ * panels, the nstatic_expl static ones first) and the surface-panel index
+ * list idSurfPan of the pressure system.  Vertex lists are (3,4,n) with
+ * triangles padded.  For the _dev entry points every pointer is device
+ * memory (the struct itself is host memory); for the host entry points,
+ * host memory. */
+typedef struct dust_linsys_desc {
+  int rank, nstatic;
+  const int *kind;         /* (rank) DUST_ROW_*                                    */
+  const int *n_ver;        /* (rank) 3 or 4                                         */
+  const double *ver;       /* (3,4,rank)                                            */
+  const double *ub;        /* (3,rank) body velocity at the centres, or NULL (0);
+                              read for the moving elements only (zero for static) */
+  const double *uvort;     /* (3,rank) get_vort_vel result, or NULL (0)             */
+  const double *bern;      /* (rank) bernoulli_source of the surface panels, or NULL */
+  double u_inf[3];         /* uniform wind (variable_wind without gusts)           */
+  int n_sp, nstatic_sp;    /* surface panels, the static ones first                */
+  const int *id_sp;        /* (n_sp) 0-based element index of each (idSurfPan - 1)  */
+  int n_wake, n_impl;      /* wake panels, the implicit first row first             */
+  const int *wake_nver;    /* (n_wake)                                              */
+  const double *wake_ver;  /* (3,4,n_wake)                                          */
+  const double *wake_mag;  /* (n_wake) doublet intensity (read for the explicit ones) */
+  const int *impl_ind;     /* (2,n_impl) pan_gen_elems_id: 1-based, 0 = no lower   */
+  int n_expl, nstatic_expl;
+  const int *expl_nver;    /* (n_expl)                                              */
+  const double *expl_ver;  /* (3,4,n_expl)                                          */
+  const double *expl_mag;  /* (n_expl)                                              */
+  double ff_ratio_doublet, rankine_rad, cutoff_rad;
+} dust_linsys_desc;
+
+/* initialize_linsys (mod_linsys.f90:109-272): the static rows
+ * (build_row_static, 174-179) into A(1:nst,1:nst) and the static source
+ * coefficients b_static into B(1:nst,1:nst); L_static(nst, nstatic_expl);
+ * the pressure matrix A_pres(n_sp,n_sp) copied by idSurfPan before the wake
+ * correction (its static block; the moving blocks are written by every
+ * assembly); the implicit wake panels on static columns added into A
+ * (191-198); then dgetrf of A(1:nst,1:nst) into ipiv(nst) and of
+ * A_pres(1:nstatic_sp,1:nstatic_sp) into ipiv_pres (236, 255).  Returns the
+ * first nonzero dgetrf info, as the reference stops on it.  B(ldb, rank) is
+ * the whole source-coefficient matrix of this library: its static block is
+ * b_static and dust_assemble_linsys_dev writes the moving blocks (the
+ * reference forms those products inside its row loops). */
+int dust_initialize_linsys(const dust_linsys_desc *desc, double *A, int lda, double *B, int ldb,
+                           double *L_static, int ldl, double *A_pres, int ldp, int *ipiv,
+                           int *ipiv_pres);
+int dust_initialize_linsys_dev(const dust_linsys_desc *desc, double *A, int lda, double *B, int ldb,
+                               double *L_static, int ldl, double *A_pres, int ldp, int *ipiv,
+                               int *ipiv_pres, void *stream);
+
+/* assemble_linsys (mod_linsys.f90:279-344) after get_vort_vel (desc->uvort):
+ *   (2) build_row of the static rows over the moving columns and of the
+ *       moving rows over all columns: A entries and their B coefficients;
+ *       b(ie) = sum_j B(ie,j) nor_j.(ub_j - u_inf - uvort_j), ub taken for
+ *       the moving columns only; b_pres(n_sp) (optional, needs desc->bern) =
+ *       the Bernoulli right-hand side of build_row_surfpan (405-437);
+ *   (3) A_pres blocks of the moving surface panels copied from A (318-324);
+ *   (4) add_wake and add_expl: implicit wake panels whose generating
+ *       elements are columns of the row's range add +-a into A(ie,ind1/ind2);
+ *       explicit wake panels and explicit elements subtract a*mag from b(ie)
+ *       (static explicit elements of static rows through L_static).
+ * The static block of A keeps its LU factors.  A_pres may be NULL. */
+int dust_assemble_linsys(const dust_linsys_desc *desc, double *A, int lda, double *B, int ldb,
+                         const double *L_static, int ldl, double *b, double *A_pres, int ldp,
+                         double *b_pres);
+int dust_assemble_linsys_dev(const dust_linsys_desc *desc, double *A, int lda, double *B, int ldb,
+                             const double *L_static, int ldl, double *b, double *A_pres, int ldp,
+                             double *b_pres, void *stream);
+
+/* ======================================================================== */
 /* Pressure (Bernoulli) integral equation: src/linsys/mod_pressure_equation */
 /* ======================================================================== */
 

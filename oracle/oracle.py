@@ -80,6 +80,9 @@ def lib():
                                        _vp, _vp, _vp]),
             "ao_pres_rhs_rot": (None, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int,
                                        _vp, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+            "ao_initialize_linsys": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                               _vp]),
+            "ao_assemble_linsys": (None, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
             "ao_dgetrf": (C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp]),
             "ao_dgetrs": (None, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int]),
         }
@@ -485,3 +488,81 @@ def dgetrs(LU, ipiv, B):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+# ---- initialize_linsys / assemble_linsys with the wake (linsys_oracle.c) ------
+class _Linsys(C.Structure):
+    _fields_ = [
+        ("rank", C.c_int), ("nstatic", C.c_int),
+        ("kind", _vp), ("G", _vp), ("ub", _vp), ("uvort", _vp), ("bern", _vp),
+        ("uinf", C.c_double * 3),
+        ("n_sp", C.c_int), ("nstatic_sp", C.c_int), ("id_sp", _vp),
+        ("n_wake", C.c_int), ("n_impl", C.c_int), ("Gw", _vp), ("wmag", _vp), ("impl_ind", _vp),
+        ("n_expl", C.c_int), ("nstatic_expl", C.c_int), ("Ge", _vp), ("emag", _vp),
+        ("ff_ratio", C.c_double), ("rankine", C.c_double), ("cutoff", C.c_double),
+    ]
+
+
+def _linsys_struct(S):
+    """The oracle's description of a system given with the attributes of
+    dust_amd.linsys.System (numpy arrays; the arrays are kept on the
+    returned holder)."""
+    keep = {}
+
+    def arr(name, a, dt=np.float64):
+        if a is None or np.asarray(a).size == 0:
+            return None
+        keep[name] = _c(a, dt)
+        return keep[name].ctypes.data
+
+    d = _Linsys()
+    d.rank, d.nstatic = S.rank, S.nstatic
+    d.kind = arr("kind", S.kind, np.int32)
+    keep["G"] = geo(S.ver, S.n_ver)
+    d.G = keep["G"].ctypes.data
+    d.ub, d.uvort, d.bern = arr("ub", S.ub), arr("uvort", S.uvort), arr("bern", S.bern)
+    d.uinf = (C.c_double * 3)(*S.u_inf)
+    d.n_sp, d.nstatic_sp, d.id_sp = S.n_sp, S.nstatic_sp, arr("id_sp", S.id_sp, np.int32)
+    d.n_wake, d.n_impl = S.n_wake, S.n_impl
+    if S.n_wake:
+        keep["Gw"] = geo(S.wake_ver, S.wake_nver)
+        d.Gw = keep["Gw"].ctypes.data
+        d.wmag = arr("wmag", S.wake_mag)
+        d.impl_ind = arr("impl_ind", S.impl_ind, np.int32)
+    d.n_expl, d.nstatic_expl = S.n_expl, S.nstatic_expl
+    if S.n_expl:
+        keep["Ge"] = geo(S.expl_ver, S.expl_nver)
+        d.Ge = keep["Ge"].ctypes.data
+        d.emag = arr("emag", S.expl_mag)
+    d.ff_ratio, d.rankine, d.cutoff = S.ff_ratio, S.rankine, S.cutoff
+    return d, keep
+
+
+def initialize_linsys(S):
+    """initialize_linsys restated (mod_linsys.f90:109-272).  Returns dict(A,
+    Bst (nst, nst), L_static, A_pres, Bst_pres, ipiv, ipiv_pres, info)."""
+    d, keep = _linsys_struct(S)
+    n, nst, nsp, nss = S.rank, S.nstatic, S.n_sp, S.nstatic_sp
+    st = dict(A=np.zeros((n, n), order="F"), Bst=np.zeros((max(nst, 1), max(nst, 1)), order="F"),
+              L_static=np.zeros((max(nst, 1), max(S.nstatic_expl, 1)), order="F"),
+              A_pres=np.zeros((max(nsp, 1), max(nsp, 1)), order="F"),
+              Bst_pres=np.zeros((max(nss, 1), max(nss, 1)), order="F"),
+              ipiv=np.zeros(max(nst, 1), np.int32), ipiv_pres=np.zeros(max(nss, 1), np.int32))
+    b = np.zeros(n)
+    st["info"] = lib().ao_initialize_linsys(
+        C.byref(d), _p(st["A"]), n, _p(st["Bst"]), _p(st["L_static"]), _p(st["A_pres"]),
+        _p(st["Bst_pres"]), _p(st["ipiv"]), _p(st["ipiv_pres"]), _p(b))
+    return st
+
+
+def assemble_linsys(S, st):
+    """assemble_linsys restated (mod_linsys.f90:279-344); updates st["A"],
+    st["A_pres"]; returns (b, b_pres)."""
+    d, keep = _linsys_struct(S)
+    n = S.rank
+    b = np.zeros(n)
+    bp = np.zeros(max(S.n_sp, 1))
+    lib().ao_assemble_linsys(C.byref(d), _p(st["A"]), n, _p(st["Bst"]), _p(st["L_static"]),
+                             _p(b), _p(st["A_pres"]) if S.n_sp else None, _p(st["Bst_pres"]),
+                             _p(bp) if S.bern is not None else None)
+    return b, bp[:S.n_sp]

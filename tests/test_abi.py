@@ -107,3 +107,33 @@ def test_pressure_argument_errors_without_gpu():
     assert lib.dust_solve_pressure_sys(2, 3, A.ctypes.data, 2, piv.ctypes.data, b.ctypes.data,
                                        b.ctypes.data) == L.DUST_E_ARG
     assert lib.dust_pres_rhs_src(2, A.ctypes.data, 1, b.ctypes.data, b.ctypes.data) == L.DUST_E_ARG
+
+
+def test_assembly_argument_errors_without_gpu():
+    """dust_initialize_linsys / dust_assemble_linsys validate the description
+    (vertex counts, index ranges) before any device work."""
+    import numpy as np
+    import sys
+    sys.path.insert(0, HERE)
+    import linsys_cases as LC
+    from dust_amd import linsys as LS
+    c = LC.mixed_case(nc=4, ns=2)
+    S = LC.mixed_system(c, 0, 0, 0)
+    n = S.rank
+    A = np.zeros((n, n), order="F")
+    b = np.zeros(n)
+    lib = L.lib()
+    for field, bad in (("impl_ind", n + 5), ("id_sp", -1), ("n_ver", 5)):
+        T = LC.mixed_system(c, 0, 0, 0)
+        arr = getattr(T, field).copy()
+        arr.flat[0] = bad
+        setattr(T, field, arr)
+        d = T.desc()
+        rc = lib.dust_assemble_linsys(ctypes.byref(d), A.ctypes.data, n, A.ctypes.data, n, None,
+                                      1, b.ctypes.data, None, 1, None)
+        assert rc == L.DUST_E_ARG, field
+        assert lib.dust_last_error().decode()
+    d = S.desc()
+    d.nstatic = n + 1
+    assert lib.dust_initialize_linsys(ctypes.byref(d), A.ctypes.data, n, A.ctypes.data, n, None, 1,
+                                      None, 1, None, None) == L.DUST_E_ARG
