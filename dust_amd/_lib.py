@@ -198,6 +198,8 @@ SIGNATURES = [
     ("dust_dgetrs", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int]),
     ("dust_dgetrf_dev", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp]),
     ("dust_dgetrs_dev", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, _vp]),
+    ("dust_dgetrs_noalloc_dev", C.c_int,
+     [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, _vp]),
     ("dust_dgesv_mixed", C.c_int,
      [C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _dp]),
     ("dust_dgesv_mixed_dev", C.c_int,

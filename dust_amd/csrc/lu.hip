@@ -502,6 +502,60 @@ int lu_getrs_set_error_word(unsigned v) {
   return 0;
 }
 
+// Workspace of one getrs: inverted diagonal blocks, the packed right-hand
+// sides, the error word and the ready flags, and the generation counter of
+// the flags.
+struct GetrsPtrs {
+  double *linv, *uinv, *y;
+  unsigned *flags;  // [0] error, [1..2 nblk] ready flags (forward, backward)
+  unsigned *gen;
+};
+
+static int getrs_core(int n, int nrhs, const double *dA, long lda, const int *dipiv, double *dB,
+                      long ldb, hipStream_t st, const GetrsPtrs &w) {
+  const int nblk = ceil_div(n, SB), np = nblk * SB;
+  const size_t lds = sizeof(double) * SB * (SB + 1);
+  static std::once_flag attr;
+  hipError_t ae = hipSuccess;
+  std::call_once(attr, [&] {
+    ae = hipFuncSetAttribute((const void *)k_diag_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds);
+  });
+  DUST_HIP(ae);
+  // the error word starts every call at zero: a time-out of an earlier call
+  // must not fail this one (the ready flags are generation-stamped)
+  k_set_word<<<1, 64, 0, st>>>(w.flags, 0u);
+  k_laswp_rhs<<<1, 256, 0, st>>>(dB, ldb, nrhs, n, dipiv);
+  k_diag_inverse<<<nblk, 256, lds, st>>>(dA, lda, n, w.linv, w.uinv);
+  DUST_HIP(hipGetLastError());
+  const int G = std::min(nblk, device_cus());
+  for (int q0 = 0; q0 < nrhs; q0 += SR) {
+    const int nq = std::min(SR, nrhs - q0);
+    k_pack_y<<<ceil_div(np, 256), 256, 0, st>>>(dB + ldb * q0, ldb, n, np, nq, w.y);
+    for (int upper = 0; upper < 2; ++upper) {
+      if (++*w.gen == 0) ++*w.gen;
+      BlockSolveArgs<double> sa{dA, lda, n, nblk, upper ? w.uinv : w.linv, w.y,
+                                w.flags + 1 + upper * nblk, *w.gen, w.flags};
+      // G <= #CUs small blocks, all resident at once (ordinary launch; the
+      // flag waits are bounded)
+      if (upper) k_block_solve<double, 1, 512><<<G, 512, 0, st>>>(sa);
+      else k_block_solve<double, 0, 512><<<G, 512, 0, st>>>(sa);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) {
+        set_error("triangular solve launch failed: %s", hipGetErrorString(e));
+        return DUST_E_HIP;
+      }
+    }
+    k_unpack_y<<<ceil_div(n, 256), 256, 0, st>>>(w.y, n, nq, dB + ldb * q0, ldb);
+    DUST_HIP(hipGetLastError());
+  }
+  unsigned herr = 0;
+  DUST_HIP(hipMemcpyAsync(&herr, w.flags, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  DUST_HIP(hipStreamSynchronize(st));
+  DUST_CHECK(herr == 0, DUST_E_HIP, "triangular solve flag wait timed out");
+  return 0;
+}
+
 int lu_getrs_dev(int n, int nrhs, const double *dA, long lda, const int *dipiv, double *dB,
                  long ldb, hipStream_t st) {
   DUST_CHECK(n >= 0 && nrhs >= 0 && lda >= std::max(1, n) && ldb >= std::max(1, n), DUST_E_ARG,
@@ -517,46 +571,67 @@ int lu_getrs_dev(int n, int nrhs, const double *dA, long lda, const int *dipiv, 
     DUST_TRY(w.flags.reserve(1 + 2 * nblk));
     DUST_HIP(hipMemset(w.flags.p, 0, sizeof(unsigned) * w.flags.n));
   }
-  const size_t lds = sizeof(double) * SB * (SB + 1);
-  static std::once_flag attr;
-  hipError_t ae = hipSuccess;
-  std::call_once(attr, [&] {
-    ae = hipFuncSetAttribute((const void *)k_diag_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)lds);
-  });
-  DUST_HIP(ae);
-  // the error word starts every call at zero: a time-out of an earlier call
-  // must not fail this one (the ready flags are generation-stamped)
-  k_set_word<<<1, 64, 0, st>>>(w.flags.p, 0u);
-  k_laswp_rhs<<<1, 256, 0, st>>>(dB, ldb, nrhs, n, dipiv);
-  k_diag_inverse<<<nblk, 256, lds, st>>>(dA, lda, n, w.linv.p, w.uinv.p);
-  DUST_HIP(hipGetLastError());
-  const int G = std::min(nblk, device_cus());
-  for (int q0 = 0; q0 < nrhs; q0 += SR) {
-    const int nq = std::min(SR, nrhs - q0);
-    k_pack_y<<<ceil_div(np, 256), 256, 0, st>>>(dB + ldb * q0, ldb, n, np, nq, w.y.p);
-    for (int upper = 0; upper < 2; ++upper) {
-      if (++w.gen == 0) ++w.gen;
-      BlockSolveArgs<double> sa{dA, lda, n, nblk, upper ? w.uinv.p : w.linv.p, w.y.p,
-                                w.flags.p + 1 + upper * nblk, w.gen, w.flags.p};
-      // G <= #CUs small blocks, all resident at once (ordinary launch; the
-      // flag waits are bounded)
-      if (upper) k_block_solve<double, 1, 512><<<G, 512, 0, st>>>(sa);
-      else k_block_solve<double, 0, 512><<<G, 512, 0, st>>>(sa);
-      hipError_t e = hipGetLastError();
-      if (e != hipSuccess) {
-        set_error("triangular solve launch failed: %s", hipGetErrorString(e));
-        return DUST_E_HIP;
-      }
+  return getrs_core(n, nrhs, dA, lda, dipiv, dB, ldb, st,
+                    GetrsPtrs{w.linv.p, w.uinv.p, w.y.p, w.flags.p, &w.gen});
+}
+
+// ---------------------------------------------------------------------------
+// getrs without allocation: the workspace is the file-scope device array
+// `scratch` (every GPU holds one copy until the process ends), laid out as
+// linv | uinv | y (doubles) | error word + ready flags (unsigned), sized for
+// n <= kNoAllocMaxN.  Its address on each device comes from
+// hipGetSymbolAddress on the first call with that device current.  Calls on
+// one GPU share the array: calls that overlap in time on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxN = 6144;
+constexpr int kNoAllocBlk = kNoAllocMaxN / SB;
+constexpr size_t kScrInv = (size_t)kNoAllocBlk * SB * SB;           // doubles per inverse set
+constexpr size_t kScrY = (size_t)kNoAllocBlk * SB * SR;             // doubles
+constexpr size_t kScrDoubles = 2 * kScrInv + kScrY;
+constexpr size_t kScrFlags = 1 + 2 * (size_t)kNoAllocBlk;           // unsigned
+constexpr size_t kScrBytes = sizeof(double) * kScrDoubles + sizeof(unsigned) * kScrFlags;
+static_assert(kNoAllocMaxN % SB == 0, "whole solve blocks");
+
+static __device__ __align__(256) unsigned char scratch[kScrBytes];
+
+constexpr int kMaxDevices = 64;
+struct ScratchDev {
+  unsigned char *ptr = nullptr;  // device address of `scratch` on this device
+  unsigned gen = 0;              // generation of the ready flags
+};
+static ScratchDev g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_ptrs(GetrsPtrs *out) {
+  int dev = 0;
+  DUST_HIP(hipGetDevice(&dev));
+  DUST_CHECK(dev >= 0 && dev < kMaxDevices, DUST_E_UNSUPPORTED, "device %d beyond %d devices",
+             dev, kMaxDevices);
+  ScratchDev &s = g_scratch[dev];
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!s.ptr) {
+      void *p = nullptr;
+      DUST_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      DUST_CHECK(p != nullptr, DUST_E_HIP, "no device address for the getrs scratch array");
+      s.ptr = (unsigned char *)p;
     }
-    k_unpack_y<<<ceil_div(n, 256), 256, 0, st>>>(w.y.p, n, nq, dB + ldb * q0, ldb);
-    DUST_HIP(hipGetLastError());
   }
-  unsigned herr = 0;
-  DUST_HIP(hipMemcpyAsync(&herr, w.flags.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-  DUST_HIP(hipStreamSynchronize(st));
-  DUST_CHECK(herr == 0, DUST_E_HIP, "triangular solve flag wait timed out");
+  double *d = (double *)s.ptr;
+  *out = GetrsPtrs{d, d + kScrInv, d + 2 * kScrInv, (unsigned *)(d + kScrDoubles), &s.gen};
   return 0;
+}
+
+int lu_getrs_noalloc_dev(int n, int nrhs, const double *dA, long lda, const int *dipiv,
+                         double *dB, long ldb, hipStream_t st) {
+  DUST_CHECK(n >= 0 && nrhs >= 0 && lda >= std::max(1, n) && ldb >= std::max(1, n), DUST_E_ARG,
+             "invalid getrs dimensions");
+  DUST_CHECK(n <= kNoAllocMaxN, DUST_E_CAPACITY,
+             "dgetrs_noalloc: n = %d exceeds the static workspace (n <= %d)", n, kNoAllocMaxN);
+  if (n == 0 || nrhs == 0) return 0;
+  GetrsPtrs w;
+  DUST_TRY(scratch_ptrs(&w));
+  return getrs_core(n, nrhs, dA, lda, dipiv, dB, ldb, st, w);
 }
 
 int lu_gemm_sub_dev(int M, int N, int K, const double *A, long lda, const double *B, long ldb,
@@ -640,6 +715,12 @@ int dust_test_trsm_llnu_dev(int m, int n, const double *L, int ldl, double *B, i
   return 0;
 }
 int dust_test_getrs_set_error_word(unsigned v) { return lu_getrs_set_error_word(v); }
+
+int dust_dgetrs_noalloc_dev(int n, int nrhs, const double *A, int lda, const int *ipiv,
+                            double *B, int ldb, void *stream) {
+  DUST_CHECK(A && ipiv && B, DUST_E_ARG, "null dgetrs_noalloc argument");
+  return lu_getrs_noalloc_dev(n, nrhs, A, lda, ipiv, B, ldb, (hipStream_t)stream);
+}
 
 int dust_dgetrs(int n, int nrhs, const double *A, int lda, const int *ipiv, double *B, int ldb) {
   DUST_CHECK(A && ipiv && B && n >= 0 && nrhs >= 0 && lda >= std::max(1, n) &&

@@ -390,6 +390,16 @@ int dust_dgetrs(int n, int nrhs, const double *A, int lda, const int *ipiv, doub
 int dust_dgetrf_dev(int m, int n, double *dA, int lda, int *dipiv, void *stream);
 int dust_dgetrs_dev(int n, int nrhs, const double *dA, int lda, const int *dipiv, double *dB,
                     int ldb, void *stream);
+/* dust_dgetrs_dev without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight per-step solve loops, harnesses that forbid hipMalloc).
+ * Same kernels and results as dust_dgetrs_dev; its workspace is a static
+ * device array of about 12.4 MB that every GPU the library runs on holds
+ * until the process ends, shared by all calls on that GPU (calls that
+ * overlap in time on one GPU race; each call clears what must start at
+ * zero).  n <= 6144, else DUST_E_CAPACITY.  Synchronises `stream`. */
+int dust_dgetrs_noalloc_dev(int n, int nrhs, const double *dA, int lda, const int *dipiv,
+                            double *dB, int ldb, void *stream);
 
 /* Tuned mixed-precision solve of A X = B (tuned form of the dgetrf 236/435 +
  * dgetrs 473 call sites): the LU runs on the matrix cores in `prec`
