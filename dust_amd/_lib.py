@@ -114,6 +114,19 @@ class PaParams(C.Structure):
     ]
 
 
+class PanwakeParams(C.Structure):
+    _fields_ = [
+        ("n_stripes", C.c_int), ("n_points", C.c_int), ("nmax_pan", C.c_int),
+        ("start_points", _vp), ("neigh", _vp), ("neigh_o", _vp), ("gen_elems", _vp),
+        ("dt", C.c_double), ("ndt_update_wake", C.c_int), ("min_vel_at_te", C.c_double),
+        ("join_te", C.c_int), ("join_te_factor", C.c_double),
+        ("refine_wake", C.c_int), ("k_refine", C.c_int),
+        ("k_vortex_rad", C.c_double), ("vortex_rad", C.c_double),
+        ("box_min", C.c_double * 3), ("box_max", C.c_double * 3),
+        ("reserved", C.c_int * 4),
+    ]
+
+
 ROW_SURFPAN = 0
 ROW_VORTLATT = 1
 
@@ -230,6 +243,17 @@ SIGNATURES = [
     ("dust_test_trsm_llnu_dev", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int]),
     ("dust_test_getrs_set_error_word", C.c_int, [C.c_uint]),
     ("dust_shutdown", C.c_int, []),
+    ("dust_panwake_create", C.c_int, [_vp, _vp]),
+    ("dust_panwake_destroy", C.c_int, [_vp]),
+    ("dust_panwake_start_dev", C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _dp, _vp]),
+    ("dust_panwake_panels", C.c_int, [_vp, _ip, _ip, _vp, _vp, _vp, _vp]),
+    ("dust_panwake_points", C.c_int, [_vp, _ip, _vp]),
+    ("dust_panwake_update_dev", C.c_int, [_vp, _vp, _vp, _vp]),
+    ("dust_panwake_shed_dev", C.c_int,
+     [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("dust_panwake_end_vortices", C.c_int, [_vp, _ip, _vp, _vp, _vp]),
+    ("dust_panwake_state", C.c_int, [_vp, _ip, _ip, _ip]),
+    ("dust_panwake_get", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 ]
 
 _lib = None

@@ -29,6 +29,7 @@ SOURCES = [
     "aic.hip",
     "elem.hip",
     "wake.hip",
+    "panwake.hip",
     "lu.hip",
     "lu_mixed.hip",
     "linsys.hip",

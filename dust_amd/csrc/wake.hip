@@ -241,7 +241,7 @@ struct IsFree {
 
 // New particle j (in order) inside the box takes free slot rank[j] of the
 // ascending free list; rank = number of inside particles before j.
-__global__ void k_wake_insert(int k, const double *__restrict__ ncen,
+__global__ void k_wake_insert(int kmax, const int *__restrict__ kdev, const double *__restrict__ ncen,
                               const double *__restrict__ npv, const double *__restrict__ nrv,
                               const double *__restrict__ nvel, const int *__restrict__ free_list,
                               const int *__restrict__ n_free, dust_wake_params P,
@@ -252,6 +252,7 @@ __global__ void k_wake_insert(int k, const double *__restrict__ ncen,
   if (threadIdx.x == 0) s_rank = 0;
   __syncthreads();
   const int nf = *n_free;
+  const int k = kdev ? min(*kdev, kmax) : kmax;
   for (int base = 0; base < k; base += blockDim.x) {
     int j = base + threadIdx.x;
     bool in = false;
@@ -298,6 +299,7 @@ struct WakeWs {
   DevBuf<char> tmp;
   HostBuf<int> hcount;
 };
+
 WakeWs g_wws;
 
 }  // namespace
@@ -329,6 +331,27 @@ int select_slots(int nmax, const unsigned char *freef, bool want_free, int *out,
 }
 
 }  // namespace
+}  // namespace dust
+
+namespace dust {
+// Insertion of the first min(*kdev, kmax) new particles (all kmax when kdev
+// is NULL) into the lowest free slots, in order; `status` (device int) is
+// or-ed with 1 when the free slots run out.  No host synchronisation.
+int wake_insert_dev(int nmax, unsigned char *free_flag, double *cen, double *dir, double *mag,
+                    double *r_vortex, double *vel, int kmax, const int *kdev,
+                    const double *new_cen, const double *new_partvec, const double *new_rv,
+                    const double *new_vel, const dust_wake_params *prm, int *n_prt, int *status,
+                    hipStream_t st) {
+  if (kmax <= 0) return 0;
+  DUST_TRY(g_wws.list.reserve(nmax));
+  DUST_TRY(g_wws.count.reserve(2));
+  DUST_TRY(select_slots(nmax, free_flag, true, g_wws.list.p, g_wws.count.p, st));
+  k_wake_insert<<<1, 1024, 0, st>>>(kmax, kdev, new_cen, new_partvec, new_rv, new_vel,
+                                   g_wws.list.p, g_wws.count.p, *prm, free_flag, cen, dir, mag,
+                                   r_vortex, vel, n_prt, status);
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
 }  // namespace dust
 
 using namespace dust;
@@ -379,7 +402,7 @@ int dust_wake_shed_dev(int nmax, unsigned char *free_flag, double *cen, double *
   DUST_TRY(g_wws.hcount.reserve(1));
   DUST_TRY(select_slots(nmax, free_flag, true, g_wws.list.p, g_wws.count.p, st));
   DUST_HIP(hipMemsetAsync(g_wws.count.p + 1, 0, sizeof(int), st));
-  k_wake_insert<<<1, 1024, 0, st>>>(k, new_cen, new_partvec, new_rv, new_vel, g_wws.list.p,
+  k_wake_insert<<<1, 1024, 0, st>>>(k, nullptr, new_cen, new_partvec, new_rv, new_vel, g_wws.list.p,
                                    g_wws.count.p, *prm, free_flag, cen, dir, mag, r_vortex, vel,
                                    n_prt, g_wws.count.p + 1);
   DUST_HIP(hipGetLastError());

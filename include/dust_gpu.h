@@ -574,6 +574,98 @@ int dust_pres_rhs_rot_dev(int m, const double *cen, int np, const double *pcen,
                           const double *wmag, const double *wvel, double *b_pres,
                           void *stream);
 
+/* ======================================================================== */
+/* Panel wake: src/aeroel/mod_wake.f90 initialize_wake (331-579) and the    */
+/* panel parts of update_wake (860-1214) and complete_wake (1222-2105)       */
+/* ======================================================================== */
+
+/* Static description of the panel wake (host memory; copied at creation). */
+typedef struct {
+  int n_stripes;            /* wake%n_pan_stripes (trailing-edge segments)        */
+  int n_points;             /* wake%n_pan_points (trailing-edge points)           */
+  int nmax_pan;             /* sim_param%n_wake_panels (rows of panels)           */
+  const int *start_points;  /* (2,n_stripes) i_start_points, 1-based              */
+  const int *neigh;         /* (2,n_stripes) pan_neigh, 1-based stripe, 0 = none  */
+  const int *neigh_o;       /* (2,n_stripes) pan_neigh_o, +1 / -1                 */
+  const int *gen_elems;     /* (2,n_stripes) pan_gen_elems_id: implicit elements, 1-based, 0 = none */
+  double dt;                /* sim_param%dt                                        */
+  int ndt_update_wake;      /* sim_param%ndt_update_wake                           */
+  double min_vel_at_te;     /* sim_param%min_vel_at_te                             */
+  int join_te;              /* sim_param%join_te                                   */
+  double join_te_factor;    /* sim_param%join_te_factor                            */
+  int refine_wake, k_refine;/* sim_param%refine_wake, k_refine                    */
+  double k_vortex_rad;      /* sim_param%KVortexRad (< 1e-10: use vortex_rad)      */
+  double vortex_rad;        /* sim_param%VortexRad                                 */
+  double box_min[3], box_max[3]; /* particles_box_min / max                      */
+  int reserved[4];
+} dust_panwake_params;
+
+typedef struct dust_panwake_s *dust_panwake_t;
+
+/* Device state of one panel wake: points pan_w_points(3,n_points,nmax_pan+1)
+ * and velocities pan_w_vel, intensities, end and last intensities, the
+ * junction history of join_te, the panels' vertex lists; pan_wake_len = 1. */
+int dust_panwake_create(const dust_panwake_params *prm, dust_panwake_t *out);
+int dust_panwake_destroy(dust_panwake_t h);
+
+/* Rows 1 and 2 of points from the trailing edge: first = 1 as in
+ * initialize_wake (512-546), first = 0 as in complete_wake (1250-1280, then
+ * the junction history moves back one row); join_first_panels (2199-2313)
+ * when join_te; then the vertex lists of the pan_wake_len rows of panels.
+ * start_points(3,n_points) = w_start_points (mean of the two generating mesh
+ * points), gen_dir(3,n_points) = R_g t_hinged, scaling(n_points) =
+ * pan_gen_scaling, vel_te(3,n_points) = calc_node_vel at the points, u_inf
+ * the wind (host, 3 doubles).  Device pointers, ordered on `stream`. */
+int dust_panwake_start_dev(dust_panwake_t h, int first, const double *start_points,
+                           const double *gen_dir, const double *scaling, const double *vel_te,
+                           const double u_inf[3], void *stream);
+
+/* The wake panels of the assembly (wake%pan_p): n_wake = n_stripes *
+ * pan_wake_len panels in (row, stripe) order, the first n_impl = n_stripes
+ * implicit, with device pointers to their vertex counts, vertex lists
+ * (3,4,n_wake), intensities and pan_gen_elems_id (2,n_impl) -- the fields of
+ * dust_linsys_desc. */
+int dust_panwake_panels(dust_panwake_t h, int *n_wake, int *n_impl, const int **n_ver,
+                        const double **ver, const double **mag, const int **impl_ind);
+
+/* The points at which update_wake evaluates velocities (rows 2 ..
+ * pan_wake_len+1 of pan_w_points, (3,n) device memory, n = pan_wake_len *
+ * n_points). */
+int dust_panwake_points(dust_panwake_t h, int *n, const double **pts);
+
+/* update_wake's panel part: first-row intensities from the solution mu
+ * (implicit elements, device) and pan_gen_elems_id (884-894); the points of
+ * rows 3.. moved with vel(3,n) = the velocity (compute_vel_from_all + wind)
+ * at dust_panwake_points, the end points when the wake is full (910-955);
+ * pan_wake_len grown; intensities shifted back one row (1177-1187). */
+int dust_panwake_update_dev(dust_panwake_t h, const double *mu, const double *vel, void *stream);
+
+/* complete_wake's shedding when the panel wake is full (classical, or
+ * refined with k_refine; 1798-1969): per stripe the vortex vector of the
+ * last panel row from its left, right and end sides with the neighbours'
+ * end intensities (or the joined intensity of join_te), inserted into the
+ * particle pool as dust_wake_shed_dev does; then the end vortices (2089-2103).
+ * *status (device int) is or-ed with 1 when the pool's free slots run out.
+ * Device pointers, no host synchronisation. */
+int dust_panwake_shed_dev(dust_panwake_t h, int nmax, unsigned char *free_flag, double *cen,
+                          double *dir, double *mag, double *r_vortex, double *vel,
+                          const dust_wake_params *prm, int *n_prt, int *status, void *stream);
+
+/* The end vortices (n = n_stripes once the wake is full, else 0): device
+ * vertex lists (3,2,n), intensities (last_pan_idou) and vertex velocities. */
+int dust_panwake_end_vortices(dust_panwake_t h, int *n, const double **ver, const double **mag,
+                              const double **vel);
+
+/* pan_wake_len, full_panels and the most particles one shedding creates. */
+int dust_panwake_state(dust_panwake_t h, int *len, int *full, int *nshed_max);
+
+/* Host copies of the state (each pointer may be NULL): points and
+ * velocities (3,n_points,nmax_pan+1), intensities (n_stripes,nmax_pan)
+ * stored row by row, end and last intensities (n_stripes), end points
+ * (3,n_points).  Synchronises the device. */
+int dust_panwake_get(dust_panwake_t h, double *points, double *pvel, double *idou,
+                     double *end_idou, double *last_idou, double *points_end);
+
 /* Release every library-global workspace (device buffers, page-locked
  * records, streams and events of the solvers and element kernels) after
  * synchronising the device; they are re-created on the next call that needs

@@ -83,6 +83,10 @@ def lib():
             "ao_initialize_linsys": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
                                                _vp]),
             "ao_assemble_linsys": (None, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+            "ao_pw_start": (None, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+            "ao_pw_update": (None, [_vp, _vp, _vp]),
+            "ao_pw_shed": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+            "ao_pw_end_vortices": (None, [_vp, _vp, _vp]),
             "ao_dgetrf": (C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp]),
             "ao_dgetrs": (None, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int]),
         }
@@ -566,3 +570,95 @@ def assemble_linsys(S, st):
                              _p(b), _p(st["A_pres"]) if S.n_sp else None, _p(st["Bst_pres"]),
                              _p(bp) if S.bern is not None else None)
     return b, bp[:S.n_sp]
+
+
+# ---- panel wake (panwake_oracle.c) ----------------------------------------------
+class _Pw(C.Structure):
+    _fields_ = [
+        ("ns", C.c_int), ("np", C.c_int), ("nmax", C.c_int), ("len", C.c_int), ("full", C.c_int),
+        ("nend", C.c_int),
+        ("start", _vp), ("neigh", _vp), ("neigh_o", _vp), ("gen", _vp), ("ends", _vp),
+        ("joined", _vp),
+        ("pts", _vp), ("pvel", _vp), ("wstart", _vp), ("pend", _vp), ("idou", _vp),
+        ("end_idou", _vp), ("last_idou", _vp),
+        ("dt", C.c_double), ("ndt", C.c_int), ("min_vel", C.c_double), ("join_te", C.c_int),
+        ("join_fact", C.c_double), ("refine", C.c_int), ("k_refine", C.c_int),
+        ("kvr", C.c_double), ("vr", C.c_double),
+    ]
+
+
+class PanWake:
+    """The panel wake restated (initialize_wake / update_wake / complete_wake
+    panel parts, mod_wake.f90).  Tables (n_stripes, 2) 1-based as the
+    reference's; state in numpy arrays: pts / pvel (nmax+1, np, 3), idou
+    (nmax, ns), end_idou / last_idou (ns,), pend (np, 3)."""
+
+    def __init__(self, start, neigh, neigh_o, gen, np_, nmax, dt, ndt=1, min_vel=1e-3,
+                 join_te=False, join_fact=0.1, refine=False, k_refine=1, kvr=0.0, vr=0.1):
+        self.tab = [np.ascontiguousarray(a, dtype=np.int32) for a in (start, neigh, neigh_o, gen)]
+        ns = len(self.tab[0])
+        ends = [iw for iw in range(ns) if self.tab[1][iw, 0] == 0 or self.tab[1][iw, 1] == 0]
+        self.ends = np.array(ends, np.int32)
+        self.joined = np.zeros((nmax + 1, max(len(ends), 1), 2), np.int32)
+        self.pts = np.zeros((nmax + 1, np_, 3))
+        self.pvel = np.zeros((nmax + 1, np_, 3))
+        self.wstart = np.zeros((np_, 3))
+        self.pend = np.zeros((np_, 3))
+        self.idou = np.zeros((nmax, ns))
+        self.end_idou = np.zeros(ns)
+        self.last_idou = np.zeros(ns)
+        s = _Pw()
+        s.ns, s.np, s.nmax, s.len, s.full, s.nend = ns, np_, nmax, 1, 0, len(ends)
+        s.start, s.neigh, s.neigh_o, s.gen = [t.ctypes.data for t in self.tab]
+        s.ends = self.ends.ctypes.data if len(ends) else None
+        for k in ("joined", "pts", "pvel", "wstart", "pend", "idou", "end_idou", "last_idou"):
+            setattr(s, k, getattr(self, k).ctypes.data)
+        s.dt, s.ndt, s.min_vel = dt, ndt, min_vel
+        s.join_te, s.join_fact = int(join_te), join_fact
+        s.refine, s.k_refine, s.kvr, s.vr = int(refine), k_refine, kvr, vr
+        self.s = s
+
+    @property
+    def len(self):
+        return self.s.len
+
+    @property
+    def full(self):
+        return bool(self.s.full)
+
+    def start(self, first, spts, gdir, scal, vte, uinf):
+        keep = [_c(spts), _c(gdir), _c(scal), _c(vte), _c(uinf)]
+        lib().ao_pw_start(C.byref(self.s), int(first), *[_p(a) for a in keep])
+
+    def points(self):
+        return self.pts[1:self.s.len + 1].reshape(-1, 3).copy()
+
+    def update(self, mu, vel):
+        lib().ao_pw_update(C.byref(self.s), _p(_c(mu)), _p(_c(vel)))
+
+    def panels(self):
+        """(ver (n,4,3), mag (n,)) of the pan_p panels in (row, stripe) order."""
+        st = self.tab[0] - 1
+        out = []
+        for r in range(self.s.len):
+            for iw in range(self.s.ns):
+                p1, p2 = st[iw]
+                out.append([self.pts[r, p1], self.pts[r, p2], self.pts[r + 1, p2],
+                            self.pts[r + 1, p1]])
+        return np.array(out), self.idou[:self.s.len].reshape(-1).copy()
+
+    def shed(self, pool, box_min, box_max):
+        """Shedding into pool (dict of numpy arrays free, cen, dir, mag, rv,
+        vel; n_prt int); returns the status (0 or -4 for a full pool)."""
+        n = C.c_int(int(pool["n_prt"]))
+        rc = lib().ao_pw_shed(C.byref(self.s), len(pool["free"]), _p(pool["free"]),
+                              _p(pool["cen"]), _p(pool["dir"]), _p(pool["mag"]), _p(pool["rv"]),
+                              _p(pool["vel"]), _p(_c(box_min)), _p(_c(box_max)), C.byref(n))
+        pool["n_prt"] = n.value
+        return rc
+
+    def end_vortices(self):
+        ns = self.s.ns
+        lver, lvel = np.zeros((ns, 2, 3)), np.zeros((ns, 2, 3))
+        lib().ao_pw_end_vortices(C.byref(self.s), _p(lver), _p(lvel))
+        return lver, self.last_idou.copy(), lvel
