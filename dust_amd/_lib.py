@@ -249,6 +249,7 @@ SIGNATURES = [
     ("dust_panwake_panels", C.c_int, [_vp, _ip, _ip, _vp, _vp, _vp, _vp]),
     ("dust_panwake_points", C.c_int, [_vp, _ip, _vp]),
     ("dust_panwake_update_dev", C.c_int, [_vp, _vp, _vp, _vp]),
+    ("dust_panwake_first_row_dev", C.c_int, [_vp, _vp, _vp]),
     ("dust_panwake_shed_dev", C.c_int,
      [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("dust_panwake_end_vortices", C.c_int, [_vp, _ip, _vp, _vp, _vp]),

@@ -192,6 +192,15 @@ __global__ void k_pw_idou(PwArgs A, const double *__restrict__ mu, int new_len) 
     A.idou[(long)(row - 1) * A.ns + iw] = A.idou[(long)(row - 2) * A.ns + iw];
 }
 
+// First-row intensities alone (update_wake 884-894), before the velocities
+// of the step are evaluated with them.
+__global__ void k_pw_first(PwArgs A, const double *__restrict__ mu) {
+  const int iw = blockIdx.x * blockDim.x + threadIdx.x;
+  if (iw >= A.ns) return;
+  const int g1 = A.gen[2 * iw], g2 = A.gen[2 * iw + 1];
+  A.idou[iw] = g2 > 0 ? mu[g1 - 1] - mu[g2 - 1] : mu[g1 - 1];
+}
+
 // get_joined_intensity (2315-2345).
 __device__ double joined_intensity(const PwArgs &A, int iw, int side) {
   int iend = 0;
@@ -510,6 +519,14 @@ int dust_panwake_points(dust_panwake_t h, int *n, const double **pts) {
   DUST_CHECK(h && n && pts, DUST_E_ARG, "null argument");
   *n = h->len * h->p.n_points;
   *pts = h->pts.p + 3 * (size_t)h->p.n_points;
+  return 0;
+}
+
+int dust_panwake_first_row_dev(dust_panwake_t h, const double *mu, void *stream) {
+  DUST_CHECK(h && mu, DUST_E_ARG, "invalid panel-wake first-row arguments");
+  PwArgs A = args(h);
+  k_pw_first<<<ceil_div(A.ns, 128), 128, 0, (hipStream_t)stream>>>(A, mu);
+  DUST_HIP(hipGetLastError());
   return 0;
 }
 

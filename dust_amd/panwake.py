@@ -71,15 +71,20 @@ class PanWake:
         L.check(L.lib().dust_panwake_points(self.h, C.byref(n), C.byref(p)), "panwake_points")
         return n.value, p.value
 
+    def first_row(self, mu, stream=None):
+        L.check(L.lib().dust_panwake_first_row_dev(self.h, _addr(mu), stream), "first_row")
+
     def update(self, mu, vel, stream=None):
         L.check(L.lib().dust_panwake_update_dev(self.h, _addr(mu), _addr(vel), stream),
                 "panwake_update")
 
     def shed(self, pool, prm, n_prt, status, stream=None):
-        """pool: object with device tensors free, cen, dir, mag, rv, vel."""
+        """pool: object with device tensors free, cen, dir, mag, r_vortex (or
+        rv), vel."""
+        rv = pool.r_vortex if hasattr(pool, "r_vortex") else pool.rv
         L.check(L.lib().dust_panwake_shed_dev(
             self.h, int(pool.free.numel()), _addr(pool.free), _addr(pool.cen), _addr(pool.dir),
-            _addr(pool.mag), _addr(pool.rv), _addr(pool.vel), C.byref(prm), _addr(n_prt),
+            _addr(pool.mag), _addr(rv), _addr(pool.vel), C.byref(prm), _addr(n_prt),
             _addr(status), stream), "panwake_shed")
 
     def end_vortices(self):

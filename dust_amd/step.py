@@ -1,66 +1,129 @@
-"""One time step of a body (at rest, or rotating about z) with a vortex-particle
-wake, device-resident,
-in the order of DUST's time loop (src/main/dust.f90 via src/aeroel/mod_wake.f90
-and src/linsys/mod_linsys.f90):
+"""One time step of a body (at rest, or rotating about z) with DUST's wake
+model -- a row of implicit wake panels at the trailing edge, n_wake_panels
+rows of panels behind it, vortex particles shed from the last row -- every
+stage on the device, in the order of DUST's time loop (src/main/dust.f90
+537-1108):
 
-  prepare_wake      live list of the particle pool, particle arrays for the
-                    FMM and the element terms (mod_wake.f90:785-849)
-  assemble_linsys   uvort at the panel centres from the particles
-                    (get_vort_vel, mod_surfpan.f90:962-977) and the static
-                    right-hand side b = B nor.(-u_inf - uvort)
-                    (mod_surfpan.f90:400-404)
-  solve_linsys      b -> panel intensities mu with the LU factors of the
-                    static AIC computed once (mod_linsys.f90:236, 473)
-  update_wake       particle velocity and stretching: the FMM pass
-                    (sort_particles, calculate_multipole, apply_multipole)
-                    plus the body panels' velocity (mag = mu, source
-                    sigma = nor.(-u_inf - uvort)) (mod_octree.f90:1350-1373)
-  complete_wake     particle update (Euler, box culling) and insertion of the
-                    particles shed at the trailing edge (mod_wake.f90:1305-1355,
-                    1888-1967)
+  prepare_wake      live list of the particle pool, the particle arrays of
+                    the FMM and of the element terms (mod_wake.f90:785-849)
+  assemble_linsys   uvort at the panel centres from the particles and the
+                    end vortices (get_vort_vel), then the rows with the
+                    implicit first wake row in A and the explicit wake panels
+                    in b (mod_linsys.f90:279-344)
+  solve_linsys      static body: dgetrs with the factors of initialize_linsys
+                    (the implicit wake row included); rotating body: a new
+                    factorisation every step (FP64 getrf/getrs, or the mixed
+                    solve) (mod_linsys.f90:352-483)
+  update_wake       first-row intensities mu_upper - mu_lower; the velocity
+                    at the panel points (body panels, wake panels, end
+                    vortices, particles, wind); the particles' velocity and
+                    stretching (FMM pass + element loops); the points
+                    convected, the intensities shifted back one row
+                    (mod_wake.f90:860-1214)
+  complete_wake     (after the caller's geometry update: rotation by
+                    omega dt) rows 1 and 2 from the trailing edge, the
+                    particle update, the particles shed from the last panel
+                    row, the end vortices (mod_wake.f90:1222-2105)
 
-The shedding geometry of the reference (last wake-panel row) is replaced by a
-rule of the same shape: `sub` particles per trailing-edge segment, half a
-step downstream of the midpoints of its `sub` equal pieces, partvec =
-(mu_upper - mu_lower) times the piece's vector, radius `r_shed`, velocity
-u_inf.  A rotating body (omega != 0) turns by omega dt at the start of each
-step (geometry update of the caller), its AIC is assembled again and solved
-with a fresh factorisation (FP64 getrf/getrs, or the mixed solve with
-solver="fp32" / "fp16s"), and the panels' velocity omega x r enters the
-source strengths.  With fmm_pan (the reference's use_fmm_pan) the tree and
-expansions are built first and uvort comes from apply_multipole_panels on
-them instead of the direct get_vort_vel sum.  Every stage runs on the GPU
-through
-the C ABI; the host reads back the live count and the trailing-edge
-intensities (2 x segments values) once per step.
+The trailing edge of a mesh laid out as `strips` strips of 2 nc quads (nc
+upper panels from the trailing edge forward, then nc lower panels back to
+it, as workloads.wing_case / workloads.rotor build them) is described by
+trailing_edge(): the generating panels of every stripe, the trailing-edge
+points (the vertices 1 and 4 of the upper panels, shared between strips), the
+neighbours across shared points and their orientation, and the direction of
+the first wake row.  With fmm_pan (use_fmm_pan) the tree and expansions are
+built first and uvort comes from apply_multipole_panels plus the end
+vortices.
 """
 import ctypes as C
 import threading
-import time
 
 import numpy as np
 
 from . import _lib as L
 from . import elements as E
 from . import octree as O
+from . import panwake as PW
 from . import wake as WK
 from . import workloads as WL
 
 
 def trailing_edge(ver, strips, nc):
-    """Trailing-edge segments of a mesh built as `strips` strips of 2*nc quads
-    (nc upper panels from the trailing edge forward, then nc lower panels
-    back to it, as workloads.wing_case / workloads.rotor lay them out):
-    (upper panel index, lower panel index, segment start, segment end)."""
+    """Trailing edge of the strip layout: dict with up, lo (generating upper
+    and lower panels, 0-based), start (strips, 2) 1-based trailing-edge
+    points of every stripe (i_start_points: at vertex 1 and vertex 4 of the
+    upper panel), ref (points, 4) = (upper panel, vertex, lower panel, its
+    nearest vertex) -- a point is the midpoint of its upper and lower mesh
+    points, as w_start_points (mod_wake.f90:512-514) --, neigh / o (strips, 2)
+    the stripe sharing each end point (1-based, 0 = none) and +1 when the
+    shared point is the other end of that stripe, -1 when the same end."""
+    ver = np.asarray(ver, dtype=np.float64)
     up = np.arange(strips) * 2 * nc
     lo = up + 2 * nc - 1
-    return up, lo, ver[up, 0].copy(), ver[up, 3].copy()
+    key, ref, start = {}, [], np.zeros((strips, 2), np.int32)
+    for s in range(strips):
+        for side, vtx in ((0, 0), (1, 3)):
+            k = tuple(ver[up[s], vtx])
+            if k not in key:
+                key[k] = len(ref)
+                dl = np.linalg.norm(ver[lo[s]] - ver[up[s], vtx], axis=1)
+                ref.append((up[s], vtx, lo[s], int(np.argmin(dl))))
+            start[s, side] = key[k] + 1
+    neigh = np.zeros((strips, 2), np.int32)
+    o = np.zeros((strips, 2), np.int32)
+    owner = {}
+    for s in range(strips):
+        for side in range(2):
+            owner.setdefault(start[s, side], []).append((s, side))
+    for s in range(strips):
+        for side in range(2):
+            for t, tside in owner[start[s, side]]:
+                if t != s:
+                    neigh[s, side] = t + 1
+                    o[s, side] = 1 if tside != side else -1
+    return dict(up=up, lo=lo, start=start, ref=np.array(ref), neigh=neigh, o=o,
+                gen=np.stack([up + 1, lo + 1], 1).astype(np.int32))
+
+
+def te_geometry(ver, te):
+    """Trailing-edge points and the direction of the first wake row at each:
+    the mean over the adjacent strips of the offsets from the upper and lower
+    generating panels' centres, without its component along the trailing
+    edge, normalised."""
+    ver = np.asarray(ver, dtype=np.float64)
+    r = te["ref"]
+    pts = 0.5 * (ver[r[:, 0], r[:, 1]] + ver[r[:, 2], r[:, 3]])
+    npt = len(pts)
+    d = np.zeros((npt, 3))
+    e = np.zeros((npt, 3))
+    st = te["start"] - 1
+    for s in range(len(st)):
+        p1, p2 = st[s]
+        c = 0.125 * (ver[te["up"][s]].sum(0) + ver[te["lo"][s]].sum(0))
+        edge = pts[p2] - pts[p1]
+        for p in (p1, p2):
+            d[p] += pts[p] - c
+            e[p] += edge
+    e /= np.linalg.norm(e, axis=1)[:, None]
+    d -= np.einsum("ij,ij->i", d, e)[:, None] * e
+    return pts, d / np.linalg.norm(d, axis=1)[:, None]
+
+
+def _normals(v4, nv):
+    """calc_geo_data_surfpan normals: (v3 - v1) x (v4 - v2), triangles
+    (v3 - v2) x (v1 - v2)."""
+    q = np.cross(v4[:, 2] - v4[:, 0], v4[:, 3] - v4[:, 1])
+    t = np.cross(v4[:, 2] - v4[:, 1], v4[:, 0] - v4[:, 1])
+    n = np.where((nv == 4)[:, None], q, t)
+    return n / np.linalg.norm(n, axis=1)[:, None]
 
 
 class BodyWakeStep:
-    def __init__(self, ver, n_ver, te, u_inf, dt, nmax, tree, degree=4, form="fp64",
-                 aic=None, r_shed=0.1, box=None, vs_elems=False, sub=1, omega=0.0,
-                 solver="fp64", strips=None, fmm_pan=False, overlap=False):
+    def __init__(self, ver, n_ver, strips, u_inf, dt, nmax, tree, degree=4, form="fp64",
+                 aic=None, vortex_rad=0.1, box=None, vs_elems=False, omega=0.0,
+                 solver="fp64", fmm_pan=False, overlap=False, n_wake_panels=1,
+                 refine_wake=False, k_refine=1, k_vortex_rad=0.0, join_te=False,
+                 join_te_factor=0.1, min_vel_at_te=1e-3):
         import torch
         self.torch = torch
         self.lib = L.lib()
@@ -72,70 +135,79 @@ class BodyWakeStep:
         self.aic = dict(ff_ratio_doublet=10.0, rankine_rad=0.1, cutoff_rad=0.001)
         self.aic.update(aic or {})
         # the tuned form also evaluates the element far fields in FP32
-        self.eprm = E.params(vortex_rad=r_shed, precision="tuned" if form == "tuned" else "fp64",
-                             **self.aic)
+        self.eprm = E.params(vortex_rad=vortex_rad,
+                             precision="tuned" if form == "tuned" else "fp64", **self.aic)
+        self.eprm64 = E.params(vortex_rad=vortex_rad, precision="fp64", **self.aic)
         dv = lambda a, dt=np.float64: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).cuda()
+        self.dv = dv
         v4 = np.zeros((self.n, 4, 3))
         v4[:, : np.asarray(ver).shape[1]] = np.asarray(ver)[:, :4]
         self.ver0 = v4
         self.nv_h = np.asarray(n_ver, dtype=np.int32)
         self.ver = dv(v4)
         self.nv = dv(n_ver, np.int32)
+        self.kind = dv(np.full(self.n, L.ROW_SURFPAN), np.int32)
         self.cen = dv(self._centres(v4))
+        self.nor = dv(_normals(v4, self.nv_h))
         self.omega = float(omega)
+        self.moving = self.omega != 0.0
         self.theta = 0.0
         self.solver = solver
-        self.strips = strips  # (strips, nc) of the trailing-edge layout, moving bodies
         self.fmm_pan = bool(fmm_pan)
-        # the FMM pass depends only on the particles: with overlap it runs on
-        # its own (non-blocking) stream from a second host thread while this
-        # thread assembles and solves the panel system.  Off by default: both
-        # sides keep the chip busy on their own, so at 1M particles the step
-        # gains under 1 %, and at a few thousand particles the thread start
-        # costs more than it hides
         self.overlap = bool(overlap) and not self.fmm_pan
         n = self.n
-        # static AIC and its LU factors (initialize_linsys, mod_linsys.f90:236)
-        self.A = torch.empty((n, n), dtype=torch.float64, device="cuda")
-        self.B = torch.empty((n, n), dtype=torch.float64, device="cuda")
-        ap = L.AicParams()
-        ap.ff_ratio_doublet = self.aic["ff_ratio_doublet"]
-        ap.rankine_rad = self.aic["rankine_rad"]
-        ap.cutoff_rad = self.aic["cutoff_rad"]
-        L.check(self.lib.dust_aic_surfpan_dev(n, self.nv.data_ptr(), self.ver.data_ptr(),
-                                              C.byref(ap), self.A.data_ptr(), n,
-                                              self.B.data_ptr(), n, None), "aic")
-        self.LU = self.A.clone()
+        d3 = dict(dtype=torch.float64, device="cuda")
+        self.uvort = torch.zeros((n, 3), **d3)
+        self.uline = torch.zeros((n, 3), **d3)
+        self.ub = torch.zeros((n, 3), **d3)
+        self.sigma = torch.zeros(n, **d3)
+        self.b = torch.zeros(n, **d3)
+        self.A = torch.zeros((n, n), **d3)
+        self.B = torch.zeros((n, n), **d3)
         self.ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
-        info = self.lib.dust_dgetrf_dev(n, n, self.LU.data_ptr(), n, self.ipiv.data_ptr(), None)
-        if info != 0:
-            raise RuntimeError(f"static AIC is singular (dgetrf info {info})")
-        self.b = torch.zeros(n, dtype=torch.float64, device="cuda")
-        self.sigma = torch.zeros(n, dtype=torch.float64, device="cuda")
-        self.uvort = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
-        self.ub = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
-        self.uvort_rel = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
-        # wake
-        self.sub = sub
-        self._set_te(te)
-        self.r_shed = float(r_shed)
+        self.L0 = torch.zeros(1, **d3)
+        if self.moving:
+            self.LU = torch.zeros((n, n), **d3)
+            self._move(advance=False)
+        # trailing edge and panel wake (initialize_wake)
+        self.te = trailing_edge(v4, *strips)
+        self.strips = strips
+        te = self.te
+        self.npt = len(te["ref"])
         self.pool = WK.ParticlePool(nmax)
         if box is None:
             lo = np.asarray(tree["origin"], dtype=np.float64)
             box = (lo + 1e-9, lo + np.asarray(tree["nbox"]) * tree["box_length"] - 1e-9)
         self.wprm = WK.params(box[0], box[1], dt=self.dt, use_vs=True)
+        self.pw = PW.PanWake(te["start"], te["neigh"], te["o"], te["gen"], self.npt,
+                             n_wake_panels, self.dt, min_vel=min_vel_at_te, join_te=join_te,
+                             join_fact=join_te_factor, refine=refine_wake, k_refine=k_refine,
+                             kvr=k_vortex_rad, vr=vortex_rad, box=box)
+        self.scal = dv(np.ones(self.npt))
+        self._te_rows(first=True)
+        self.status = torch.zeros(1, dtype=torch.int32, device="cuda")
+        # initialize_linsys: static body -> static rows with the implicit wake
+        # row, factorised once; rotating body -> everything per step
+        d = self._desc()
+        info = self.lib.dust_initialize_linsys_dev(C.byref(d), self.A.data_ptr(), n,
+                                                   self.B.data_ptr(), n, self.L0.data_ptr(), 1,
+                                                   None, 1, self.ipiv.data_ptr(), None, None)
+        L.check(info, "initialize_linsys")
+        if info:
+            raise RuntimeError(f"static AIC is singular (dgetrf info {info})")
         self.fmm = O.initialize_octree(tree["box_length"], tree["nbox"], tree["origin"],
                                        tree["nlevels"], tree["min_part"], degree, tree["delta"],
                                        capacity=nmax, use_vs=True, u_inf=tuple(self.u_inf),
                                        precision=form)
-        d3 = dict(dtype=torch.float64, device="cuda")
         self.p = dict(cen=torch.zeros((nmax, 3), **d3), dir=torch.zeros((nmax, 3), **d3),
                       mag=torch.zeros(nmax, **d3), rv=torch.zeros(nmax, **d3),
                       alpha=torch.zeros((nmax, 3), **d3), vel=torch.zeros((nmax, 3), **d3),
                       stretch=torch.zeros((nmax, 3), **d3))
-        self.mu = self.b  # solve_linsys writes res in place
+        self.ptvel = torch.zeros((self.npt * (n_wake_panels + 1), 3), **d3)
+        self.mu = self.b
         self.times = {}
 
+    # ---- geometry -----------------------------------------------------------
     def _centres(self, v4):
         """Vertex means (calc_geo_data): the first 3 or 4 vertices, summed in
         vertex order."""
@@ -144,40 +216,70 @@ class BodyWakeStep:
         s[quad] = s[quad] + v4[quad, 3]
         return s / self.nv_h[:, None].astype(np.float64)
 
-    def _set_te(self, te):
-        up, lo, te0, te1 = te
-        sub = self.sub
-        q = (np.arange(sub) + 0.5) / sub
-        self.te_up, self.te_lo = np.repeat(up, sub), np.repeat(lo, sub)
-        self.te_vec = np.repeat((te1 - te0) / sub, sub, axis=0)
-        self.te_mid = (np.repeat(te0, sub, axis=0) + np.tile(q, len(up))[:, None] *
-                       np.repeat(te1 - te0, sub, axis=0) + 0.5 * self.dt * self.u_inf)
+    def _te_rows(self, first):
+        """Rows 1 and 2 of the panel wake from the current geometry (the
+        trailing-edge points, the first-row direction, the trailing edge's
+        own velocity omega x r)."""
+        v = WL.rotate_z(self.ver0, self.theta) if self.moving else self.ver0
+        pts, gdir = te_geometry(v, self.te)
+        vte = np.stack([-self.omega * pts[:, 1], self.omega * pts[:, 0],
+                        np.zeros(self.npt)], 1)
+        self._te_keep = [self.dv(pts), self.dv(gdir), self.dv(vte)]
+        self.pw.start(first, self._te_keep[0], self._te_keep[1], self.scal, self._te_keep[2],
+                      self.u_inf)
 
-    def _move(self):
-        """Rotating body: the geometry of the new step (rotation about z by
-        omega dt; the caller's geometry update, outside the library) and the
-        body velocity omega x r at the panel centres."""
+    def _move(self, advance=True):
+        """The caller's geometry update of a rotating body: rotation about z by
+        omega dt (advance), the panel centres, normals and body velocity
+        omega x r."""
         torch = self.torch
-        self.theta += self.omega * self.dt
+        if advance:
+            self.theta += self.omega * self.dt
         v = WL.rotate_z(self.ver0, self.theta)
         cen = self._centres(v)
         self.ver.copy_(torch.from_numpy(v))
         self.cen.copy_(torch.from_numpy(cen))
-        ub = np.stack([-self.omega * cen[:, 1], self.omega * cen[:, 0], np.zeros(self.n)], axis=1)
+        self.nor.copy_(torch.from_numpy(_normals(v, self.nv_h)))
+        ub = np.stack([-self.omega * cen[:, 1], self.omega * cen[:, 0], np.zeros(self.n)],
+                      axis=1)
         self.ub.copy_(torch.from_numpy(ub))
-        ns, nc = self.strips
-        self._set_te(trailing_edge(v, ns, nc))
+
+    def _desc(self):
+        d = L.LinsysDesc()
+        d.rank = self.n
+        d.nstatic = 0 if self.moving else self.n
+        d.kind, d.n_ver, d.ver = self.kind.data_ptr(), self.nv.data_ptr(), self.ver.data_ptr()
+        d.ub = self.ub.data_ptr() if self.moving else None
+        d.uvort = self.uvort.data_ptr()
+        d.u_inf = (C.c_double * 3)(*self.u_inf)
+        pn = self.pw.panels()
+        d.n_wake, d.n_impl = pn["n_wake"], pn["n_impl"]
+        d.wake_nver, d.wake_ver, d.wake_mag = pn["n_ver"], pn["ver"], pn["mag"]
+        d.impl_ind = pn["impl_ind"]
+        d.ff_ratio_doublet = self.aic["ff_ratio_doublet"]
+        d.rankine_rad = self.aic["rankine_rad"]
+        d.cutoff_rad = self.aic["cutoff_rad"]
+        return d
+
+    def _groups(self):
+        """The element lists of apply_multipole and compute_vel_from_all: body
+        panels (mag mu, source sigma), wake panels, end vortices."""
+        pn = self.pw.panels()
+        nl, lver, lmag, _ = self.pw.end_vortices()
+        grp = (L.ElemGroup * 3)()
+        grp[0].kind, grp[0].n = L.ELEM_SURFPAN, self.n
+        grp[0].n_ver, grp[0].ver = self.nv.data_ptr(), self.ver.data_ptr()
+        grp[0].mag, grp[0].sigma, grp[0].stretch = self.b.data_ptr(), self.sigma.data_ptr(), 1
+        grp[1].kind, grp[1].n = L.ELEM_RING, pn["n_wake"]
+        grp[1].n_ver, grp[1].ver, grp[1].mag, grp[1].stretch = (pn["n_ver"], pn["ver"],
+                                                                 pn["mag"], 1)
+        grp[2].kind, grp[2].n = L.ELEM_LINE, nl
+        grp[2].ver, grp[2].mag, grp[2].stretch = lver, lmag, 1
+        return grp, (3 if nl else 2)
 
     def seed(self, cen, alpha, rv):
         """Initial particles (inserted like shed ones)."""
         self.pool.shed(cen, alpha, rv, np.tile(self.u_inf, (len(rv), 1)), self.wprm)
-
-    def _shed(self):
-        mu = self.b.cpu().numpy()
-        pv = (mu[self.te_up] - mu[self.te_lo])[:, None] * self.te_vec
-        k = len(pv)
-        self.pool.shed(self.te_mid, pv, np.full(k, self.r_shed), np.tile(self.u_inf, (k, 1)),
-                       self.wprm)
 
     def _fmm_pass(self, npart, err):
         """update_wake's FMM pass over the gathered particles into p[vel],
@@ -195,8 +297,10 @@ class BodyWakeStep:
         except BaseException as e:  # re-raised by the stepping thread
             err.append(e)
 
+    # ---- the step -------------------------------------------------------------
     def step(self):
         torch, lib, p = self.torch, self.lib, self.p
+        n = self.n
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(7)]
         ev[0].record()
         # prepare_wake: live list and the pointer arrays
@@ -211,8 +315,6 @@ class BodyWakeStep:
                 "gather")
         h = self.fmm.handle
         if self.fmm_pan and npart:
-            # use_fmm_pan: the tree and the expansions are built in prepare_wake
-            # (mod_wake.f90:797-801) and serve both uvort and the particles
             torch.cuda.synchronize()
             L.check(lib.dust_fmm_set_particles_dev(h, npart, p["cen"].data_ptr(),
                                                    p["dir"].data_ptr(), p["mag"].data_ptr(),
@@ -227,67 +329,66 @@ class BodyWakeStep:
             torch.cuda.synchronize()  # the gathered particle arrays
             fmm_job = threading.Thread(target=self._fmm_pass, args=(npart, fmm_err))
             fmm_job.start()
-        moving = self.omega != 0.0
-        if moving:  # moving body: geometry of the step, AIC re-assembled
-            self._move()
-            ap = L.AicParams()
-            ap.ff_ratio_doublet = self.aic["ff_ratio_doublet"]
-            ap.rankine_rad = self.aic["rankine_rad"]
-            ap.cutoff_rad = self.aic["cutoff_rad"]
-            L.check(lib.dust_aic_surfpan_dev(self.n, self.nv.data_ptr(), self.ver.data_ptr(),
-                                             C.byref(ap), self.A.data_ptr(), self.n,
-                                             self.B.data_ptr(), self.n, None), "aic")
-        # assemble_linsys: uvort, right-hand side and source strengths
+        # assemble_linsys: uvort from the particles (or the tree) and the end
+        # vortices, then the rows with the wake
+        nl, lver, lmag, _ = self.pw.end_vortices()
         if self.fmm_pan:
-            # apply_multipole_panels on the tree of this step (no end vortices
-            # here, so get_vort_vel contributes nothing)
             self.uvort.zero_()
             if npart:
                 torch.cuda.synchronize()
-                L.check(lib.dust_fmm_apply_multipole_panels_dev(h, self.n, self.cen.data_ptr(),
+                L.check(lib.dust_fmm_apply_multipole_panels_dev(h, n, self.cen.data_ptr(),
                                                                 self.uvort.data_ptr()),
                         "fmm panels")
                 L.check(lib.dust_fmm_synchronize(h), "fmm sync")
+            if nl:
+                L.check(lib.dust_vort_vel_dev(n, self.cen.data_ptr(), 0, None, None, None, None,
+                                              nl, lver, lmag, C.byref(self.eprm64),
+                                              self.uline.data_ptr(), None), "uvort lines")
+                self.uvort.add_(self.uline)
         else:
-            L.check(lib.dust_vort_vel_dev(self.n, self.cen.data_ptr(), npart,
-                                          p["cen"].data_ptr(), p["dir"].data_ptr(),
-                                          p["mag"].data_ptr(), p["rv"].data_ptr(), 0, None,
-                                          None, C.byref(self.eprm), self.uvort.data_ptr(),
-                                          None), "uvort")
-        u = np.ascontiguousarray(self.u_inf)
-        # sigma = nor.(ub - u_inf - uvort): the moving panels' velocity enters
-        # as uvort - ub
-        urel = self.uvort
-        if moving:
-            torch.sub(self.uvort, self.ub, out=self.uvort_rel)
-            urel = self.uvort_rel
-        L.check(lib.dust_aic_rhs_dev(self.n, self.nv.data_ptr(), self.ver.data_ptr(),
-                                     self.B.data_ptr(), self.n, 1, u.ctypes.data,
-                                     urel.data_ptr(), self.b.data_ptr(), self.n,
-                                     self.sigma.data_ptr(), None), "rhs")
+            L.check(lib.dust_vort_vel_dev(n, self.cen.data_ptr(), npart, p["cen"].data_ptr(),
+                                          p["dir"].data_ptr(), p["mag"].data_ptr(),
+                                          p["rv"].data_ptr(), nl, lver if nl else None,
+                                          lmag if nl else None, C.byref(self.eprm64),
+                                          self.uvort.data_ptr(), None), "uvort")
+        d = self._desc()
+        L.check(lib.dust_assemble_linsys_dev(C.byref(d), self.A.data_ptr(), n, self.B.data_ptr(),
+                                             n, self.L0.data_ptr(), 1, self.b.data_ptr(), None,
+                                             1, None, None), "assemble_linsys")
         ev[2].record()
-        # solve_linsys: the static factors, or (moving) a new factorisation
-        if not moving:
-            L.check(lib.dust_dgetrs_dev(self.n, 1, self.LU.data_ptr(), self.n,
-                                        self.ipiv.data_ptr(), self.b.data_ptr(), self.n, None),
-                    "getrs")
+        # solve_linsys
+        if not self.moving:
+            L.check(lib.dust_dgetrs_dev(n, 1, self.A.data_ptr(), n, self.ipiv.data_ptr(),
+                                        self.b.data_ptr(), n, None), "getrs")
         elif self.solver == "fp64":
             self.LU.copy_(self.A)
-            info = lib.dust_dgetrf_dev(self.n, self.n, self.LU.data_ptr(), self.n,
-                                       self.ipiv.data_ptr(), None)
+            info = lib.dust_dgetrf_dev(n, n, self.LU.data_ptr(), n, self.ipiv.data_ptr(), None)
             if info != 0:
                 raise RuntimeError(f"AIC is singular (dgetrf info {info})")
-            L.check(lib.dust_dgetrs_dev(self.n, 1, self.LU.data_ptr(), self.n,
-                                        self.ipiv.data_ptr(), self.b.data_ptr(), self.n, None),
-                    "getrs")
+            L.check(lib.dust_dgetrs_dev(n, 1, self.LU.data_ptr(), n, self.ipiv.data_ptr(),
+                                        self.b.data_ptr(), n, None), "getrs")
         else:
             it, res = C.c_int(), C.c_double()
             prec = {"fp32": L.LU_FP32, "fp16s": L.LU_FP16S}[self.solver]
-            L.check(lib.dust_dgesv_mixed_dev(self.n, 1, self.A.data_ptr(), self.n,
-                                             self.b.data_ptr(), self.n, prec, 1e-12, 30,
-                                             C.byref(it), C.byref(res), None), "mixed solve")
+            L.check(lib.dust_dgesv_mixed_dev(n, 1, self.A.data_ptr(), n, self.b.data_ptr(), n,
+                                             prec, 1e-12, 30, C.byref(it), C.byref(res), None),
+                    "mixed solve")
         ev[3].record()
-        # update_wake: FMM pass over the live particles
+        # update_wake: first-row intensities, then the velocities at the panel
+        # points and at the particles, then the panels move
+        self.pw.first_row(self.b)
+        u = torch.from_numpy(self.u_inf).cuda()
+        torch.sum(self.nor * (self.ub - u - self.uvort), dim=1, out=self.sigma)
+        grp, ng = self._groups()
+        npts, pts = self.pw.points()
+        vpt = self.ptvel[:npts]
+        L.check(lib.dust_vort_vel_dev(npts, pts, npart, p["cen"].data_ptr(),
+                                      p["dir"].data_ptr(), p["mag"].data_ptr(),
+                                      p["rv"].data_ptr(), 0, None, None, C.byref(self.eprm64),
+                                      vpt.data_ptr(), None), "point velocity (particles)")
+        vpt.add_(u)
+        L.check(lib.dust_elem_apply_dev(ng, grp, C.byref(self.eprm64), npts, pts, None,
+                                        vpt.data_ptr(), None, None), "point velocity (elements)")
         if fmm_job is not None:
             fmm_job.join()
             if fmm_err:
@@ -300,18 +401,14 @@ class BodyWakeStep:
                 L.check(lib.dust_fmm_set_particles_dev(h, npart, p["cen"].data_ptr(),
                                                        p["dir"].data_ptr(), p["mag"].data_ptr(),
                                                        p["rv"].data_ptr()), "fmm set")
-                nl = C.c_int()
-                L.check(lib.dust_fmm_pass_dev(h, C.byref(nl)), "fmm pass")
+                nlv = C.c_int()
+                L.check(lib.dust_fmm_pass_dev(h, C.byref(nlv)), "fmm pass")
             L.check(lib.dust_fmm_results_dev(h, p["vel"].data_ptr(), p["stretch"].data_ptr(),
                                              None), "fmm results")
             L.check(lib.dust_fmm_synchronize(h), "fmm sync")
         ev[4].record()
-        # ... plus the body panels (elem list of apply_multipole)
-        grp = (L.ElemGroup * 1)()
-        grp[0].kind, grp[0].n = L.ELEM_SURFPAN, self.n
-        grp[0].n_ver, grp[0].ver = self.nv.data_ptr(), self.ver.data_ptr()
-        grp[0].mag, grp[0].sigma, grp[0].stretch = self.b.data_ptr(), self.sigma.data_ptr(), 1
-        L.check(lib.dust_elem_apply_dev(1, grp, C.byref(self.eprm), npart, p["cen"].data_ptr(),
+        # ... plus the element loops of apply_multipole
+        L.check(lib.dust_elem_apply_dev(ng, grp, C.byref(self.eprm), npart, p["cen"].data_ptr(),
                                         p["alpha"].data_ptr() if self.vs_elems else None,
                                         p["vel"].data_ptr(),
                                         p["stretch"].data_ptr() if self.vs_elems else None,
@@ -319,25 +416,31 @@ class BodyWakeStep:
         L.check(lib.dust_wake_scatter_dev(npart, live.data_ptr(), p["vel"].data_ptr(),
                                           p["stretch"].data_ptr(), P.vel.data_ptr(),
                                           P.stretch.data_ptr(), None), "scatter")
+        self.pw.update(self.b, vpt)
         ev[5].record()
-        # complete_wake: move, cull, shed
+        # geometry update, then complete_wake: rows 1-2, particles, shedding
+        if self.moving:
+            self._move()
+        self._te_rows(first=False)
         P.update(self.wprm)
-        self._shed()
+        self.pw.shed(P, self.wprm, P.n_prt, self.status)
         ev[6].record()
         torch.cuda.synchronize()
+        if int(self.status.item()) != 0:
+            raise L.DustError(L.DUST_E_CAPACITY, "shedding: the particle pool is full")
         names = ("prepare", "assemble", "solve", "fmm", "elements", "complete")
-        # (moving bodies: "assemble" includes the AIC re-assembly; with
-        # overlap "fmm" is the part of the FMM pass not hidden behind the
-        # assembly and the solve, which it slows down while they share the GPU)
         for i, k in enumerate(names):
             self.times[k] = self.times.get(k, 0.0) + ev[i].elapsed_time(ev[i + 1])
         return npart
 
     def state(self):
-        """Host copies: panel intensities, uvort, live slots and the pool."""
+        """Host copies: panel intensities, uvort, live slots, the pool and the
+        panel wake."""
         live = self.pool.live().cpu().numpy()
         P = self.pool
-        return dict(mu=self.b.cpu().numpy(), uvort=self.uvort.cpu().numpy(), live=live,
-                    cen=P.cen.cpu().numpy(), dir=P.dir.cpu().numpy(), mag=P.mag.cpu().numpy(),
-                    vel=P.vel.cpu().numpy(), stretch=P.stretch.cpu().numpy(),
-                    free=P.free.cpu().numpy())
+        out = dict(mu=self.b.cpu().numpy(), uvort=self.uvort.cpu().numpy(), live=live,
+                   cen=P.cen.cpu().numpy(), dir=P.dir.cpu().numpy(), mag=P.mag.cpu().numpy(),
+                   vel=P.vel.cpu().numpy(), stretch=P.stretch.cpu().numpy(),
+                   free=P.free.cpu().numpy())
+        out.update({"pw_" + k: v for k, v in self.pw.get().items()})
+        return out

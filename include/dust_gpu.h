@@ -633,6 +633,11 @@ int dust_panwake_panels(dust_panwake_t h, int *n_wake, int *n_impl, const int **
  * n_points). */
 int dust_panwake_points(dust_panwake_t h, int *n, const double **pts);
 
+/* The first-row intensities alone (update_wake 884-894): mu(ind1) -
+ * mu(ind2); update_wake evaluates the velocities of the step with them,
+ * before dust_panwake_update_dev moves the panels. */
+int dust_panwake_first_row_dev(dust_panwake_t h, const double *mu, void *stream);
+
 /* update_wake's panel part: first-row intensities from the solution mu
  * (implicit elements, device) and pan_gen_elems_id (884-894); the points of
  * rows 3.. moved with vel(3,n) = the velocity (compute_vel_from_all + wind)

@@ -1,10 +1,13 @@
-"""A whole body + wake time step on the device (dust_amd.step.BodyWakeStep:
-uvort, right-hand side, solve with the static LU, FMM pass, body panel
-velocity, particle update, shedding) against the same sequence of CPU oracle
-calls, three steps of the 1,600-panel wing with its 8,000-particle sheet.
-FP64: panel intensities, uvort, velocities, stretching and particle states to
-1e-10 relative L2, slots bit-exact; the tuned FMM form to 1e-6 of the FP64
-form."""
+"""Whole time steps with DUST's wake model on the device
+(dust_amd.step.BodyWakeStep: uvort with the end vortices, assembly with the
+implicit wake row and the explicit wake panels, solve, first-row
+intensities, velocities at the panel points and at the particles, panel
+convection, trailing-edge rows, particle update, shedding from the last panel
+row) against the CPU oracle pipeline doing the same (oracle/pipeline.py):
+the 1,600-panel wing with its 8,000-particle sheet and a two-blade rotor.
+FP64: panel intensities, uvort, velocities, stretching, panel points and
+particle states to 1e-10 relative L2, slots bit-exact; the tuned forms to
+1e-6 of the FP64 form."""
 import numpy as np
 import pytest
 
@@ -12,11 +15,14 @@ from dust_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-10
-STEPS = 3
+STEPS = 4          # the panel wake (2 rows) is full from the second step on: shedding
+NWP = 2
+WING = dict(u=(30.0, 0.0, 0.0), dt=1.0 / 600.0, strips=(20, 40))
 
 
 def rel(a, b):
-    return np.linalg.norm(a - b) / np.linalg.norm(b)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
 
 
 @pytest.fixture(scope="module")
@@ -25,42 +31,54 @@ def case():
     return ver, nv, cen, alpha, rv, tree
 
 
-def make_step(case, form):
-    from dust_amd.step import BodyWakeStep, trailing_edge
+def make_step(case, form, **kw):
+    from dust_amd.step import BodyWakeStep
     ver, nv, cen, alpha, rv, tree = case
-    te = trailing_edge(ver, 20, 40)
-    s = BodyWakeStep(ver, nv, te, (30.0, 0.0, 0.0), 1.0 / 600.0, len(rv) + 400, tree,
-                     form=form)
+    # refined shedding: 2 x 16 particles per stripe and step
+    extra = 4000 if kw.get("refine_wake") else 400
+    s = BodyWakeStep(ver, nv, WING["strips"], WING["u"], WING["dt"], len(rv) + extra, tree,
+                     form=form, n_wake_panels=NWP, **kw)
     s.seed(cen, alpha, rv)
-    return s, te
+    return s
 
 
-def test_fp64_steps_match_oracle(orc, case):
-    s, te = make_step(case, "fp64")
+def compare(g, o, r, tol=TOL):
+    assert rel(g["mu"], r["mu"]) <= tol
+    assert rel(g["uvort"], r["uvort"]) <= tol
+    assert rel(g["vel"][r["live"]], r["vel"]) <= tol
+    assert rel(g["stretch"][r["live"]], r["stretch"]) <= tol
+    assert np.array_equal(g["free"], o.pool["free"])
+    lv = np.flatnonzero(o.pool["free"] == 0)
+    assert rel(g["cen"][lv], o.pool["cen"][lv]) <= tol
+    assert rel(g["dir"][lv], o.pool["dir"][lv]) <= tol
+    assert rel(g["mag"][lv], o.pool["mag"][lv]) <= tol
+    for key in ("pts", "idou", "last_idou"):
+        assert rel(g["pw_" + key], getattr(o.pw, key)) <= tol, key
+
+
+@pytest.mark.parametrize("kw", [{}, dict(refine_wake=True, k_refine=2, k_vortex_rad=0.5)])
+def test_wing_steps_match_oracle(orc, case, kw):
     from oracle.pipeline import OracleStep
-    o = OracleStep(case, te, s.pool.nmax, (30.0, 0.0, 0.0), 1.0 / 600.0)
+    s = make_step(case, "fp64", **kw)
+    o = OracleStep(case, WING["strips"], s.pool.nmax, WING["u"], WING["dt"], n_wake_panels=NWP,
+                   **kw)
     for k in range(STEPS):
-        print(f"step {k}", flush=True)
+        print(f"wing step {k} {kw}", flush=True)
         live_g = s.pool.live().cpu().numpy()
         s.step()
         g = s.state()
         r = o.step()
         assert np.array_equal(live_g, r["live"])
-        print(f"mu {rel(g['mu'], r['mu']):.2e} uvort {rel(g['uvort'], r['uvort']):.2e}")
-        assert rel(g["mu"], r["mu"]) <= TOL
-        assert rel(g["uvort"], r["uvort"]) <= TOL
-        assert rel(g["vel"][r["live"]], r["vel"]) <= TOL
-        assert rel(g["stretch"][r["live"]], r["stretch"]) <= TOL
-        assert np.array_equal(g["free"], o.pool["free"])
-        lv = np.flatnonzero(o.pool["free"] == 0)
-        assert rel(g["cen"][lv], o.pool["cen"][lv]) <= TOL
-        assert rel(g["dir"][lv], o.pool["dir"][lv]) <= TOL
+        print(f"mu {rel(g['mu'], r['mu']):.2e} uvort {rel(g['uvort'], r['uvort']):.2e} "
+              f"pts {rel(g['pw_pts'], o.pw.pts):.2e} live {len(r['live'])}")
+        compare(g, o, r)
+    assert o.pw.full
     print("stage ms", {k: round(v / STEPS, 3) for k, v in s.times.items()})
 
 
 def test_tuned_fmm_steps_match_fp64(case):
-    a, _ = make_step(case, "fp64")
-    b, _ = make_step(case, "tuned")
+    a = make_step(case, "fp64")
+    b = make_step(case, "tuned")
     for k in range(STEPS):
         print(f"tuned step {k}", flush=True)
         a.step()
@@ -88,41 +106,39 @@ def rotor_case():
     return (ver, nv, 1.5 * cen, alpha, rv, tree), (blades * ns, nc)
 
 
+ROTOR = dict(u=(0.0, 0.0, -5.0), dt=np.radians(2.0) / 20.0, omega=20.0)
+
+
 def make_rotor(rotor_case, form, solver, **kw):
-    from dust_amd.step import BodyWakeStep, trailing_edge
+    from dust_amd.step import BodyWakeStep
     (ver, nv, cen, alpha, rv, tree), strips = rotor_case
-    te = trailing_edge(ver, *strips)
-    s = BodyWakeStep(ver, nv, te, (0.0, 0.0, -5.0), np.radians(2.0) / 20.0, len(rv) + 400, tree,
-                     form=form, r_shed=0.05, omega=20.0, solver=solver, strips=strips, **kw)
+    s = BodyWakeStep(ver, nv, strips, ROTOR["u"], ROTOR["dt"], len(rv) + 400, tree, form=form,
+                     vortex_rad=0.05, omega=ROTOR["omega"], solver=solver, n_wake_panels=NWP,
+                     **kw)
     s.seed(cen, alpha, rv)
-    return s, te
+    return s
 
 
 def test_rotor_steps_match_oracle(orc, rotor_case):
     from oracle.pipeline import OracleStep
     case, strips = rotor_case
-    s, te = make_rotor(rotor_case, "fp64", "fp64")
-    o = OracleStep(case, te, s.pool.nmax, (0.0, 0.0, -5.0), np.radians(2.0) / 20.0, r_shed=0.05,
-                   omega=20.0, strips=strips)
-    for k in range(2):
+    s = make_rotor(rotor_case, "fp64", "fp64")
+    o = OracleStep(case, strips, s.pool.nmax, ROTOR["u"], ROTOR["dt"], vortex_rad=0.05,
+                   omega=ROTOR["omega"], n_wake_panels=NWP)
+    for k in range(3):
         print(f"rotor step {k}", flush=True)
         s.step()
         g = s.state()
         r = o.step()
         print(f"mu {rel(g['mu'], r['mu']):.2e} uvort {rel(g['uvort'], r['uvort']):.2e} "
               f"vel {rel(g['vel'][r['live']], r['vel']):.2e}")
-        assert rel(g["mu"], r["mu"]) <= TOL
-        assert rel(g["uvort"], r["uvort"]) <= TOL
-        assert rel(g["vel"][r["live"]], r["vel"]) <= TOL
-        assert np.array_equal(g["free"], o.pool["free"])
-        lv = np.flatnonzero(o.pool["free"] == 0)
-        assert rel(g["cen"][lv], o.pool["cen"][lv]) <= TOL
+        compare(g, o, r)
 
 
 def test_rotor_tuned_matches_fp64(rotor_case):
-    a, _ = make_rotor(rotor_case, "fp64", "fp64")
-    b, _ = make_rotor(rotor_case, "tuned", "fp16s")
-    for k in range(2):
+    a = make_rotor(rotor_case, "fp64", "fp64")
+    b = make_rotor(rotor_case, "tuned", "fp16s")
+    for k in range(3):
         print(f"rotor tuned step {k}", flush=True)
         a.step()
         b.step()
@@ -136,26 +152,19 @@ def test_rotor_tuned_matches_fp64(rotor_case):
 
 def test_fmm_pan_steps_match_oracle(orc, case):
     """use_fmm_pan: the tree and expansions of prepare_wake give uvort through
-    apply_multipole_panels and then the particle velocities; three wing steps
-    against the oracle pipeline doing the same."""
-    from dust_amd.step import BodyWakeStep, trailing_edge
+    apply_multipole_panels (plus the end vortices) and then the particle
+    velocities."""
     from oracle.pipeline import OracleStep
-    ver, nv, cen, alpha, rv, tree = case
-    te = trailing_edge(ver, 20, 40)
-    s = BodyWakeStep(ver, nv, te, (30.0, 0.0, 0.0), 1.0 / 600.0, len(rv) + 400, tree,
-                     fmm_pan=True)
-    s.seed(cen, alpha, rv)
-    o = OracleStep(case, te, s.pool.nmax, (30.0, 0.0, 0.0), 1.0 / 600.0, fmm_pan=True)
-    for k in range(STEPS):
+    s = make_step(case, "fp64", fmm_pan=True)
+    o = OracleStep(case, WING["strips"], s.pool.nmax, WING["u"], WING["dt"], n_wake_panels=NWP,
+                   fmm_pan=True)
+    for k in range(3):
         print(f"fmm_pan step {k}", flush=True)
         s.step()
         g = s.state()
         r = o.step()
         print(f"mu {rel(g['mu'], r['mu']):.2e} uvort {rel(g['uvort'], r['uvort']):.2e}")
-        assert rel(g["mu"], r["mu"]) <= TOL
-        assert rel(g["uvort"], r["uvort"]) <= TOL
-        assert rel(g["vel"][r["live"]], r["vel"]) <= TOL
-        assert np.array_equal(g["free"], o.pool["free"])
+        compare(g, o, r)
 
 
 @pytest.mark.parametrize("form,solver", [("fp64", "fp64"), ("tuned", "fp16s")])
@@ -165,8 +174,8 @@ def test_rotor_overlap_matches_sequential(rotor_case, form, solver):
     sequential order: bit for bit in the FP64 form; within 1e-12 in the tuned
     form, whose M2L adds into the FP64 locals with atomics from several jobs
     per target cell (the order of those additions varies from run to run)."""
-    a, _ = make_rotor(rotor_case, form, solver, overlap=False)
-    b, _ = make_rotor(rotor_case, form, solver, overlap=True)
+    a = make_rotor(rotor_case, form, solver, overlap=False)
+    b = make_rotor(rotor_case, form, solver, overlap=True)
     assert not a.overlap and b.overlap
     for k in range(3):
         print(f"rotor overlap {form}/{solver} step {k}", flush=True)
