@@ -45,6 +45,8 @@ struct PanelArgs {
   int *crow;      // [2][G]
   double *crowv;  // [2][G][jb]
   double *topv;   // [2][jb]
+  const int *spec_fail;  // NULL, or the speculative factorisation's verdict
+                         // (0: it stands and this launch does nothing)
 };
 
 // (value, row) maximum in idamax order: larger |a|, then the smaller row.
@@ -76,6 +78,7 @@ __global__ void __launch_bounds__(256) k_panel(PanelArgs P) {
   double *trow = prow + jb;        // jb
   __shared__ double wvv[4];
   __shared__ int wvi[4];
+  if (P.spec_fail && *P.spec_fail == 0) return;  // uniform over the grid
   for (int q = tid; q < nr * jb; q += nt) {
     int r = q / jb, c = q % jb;
     rows[q] = P.A[(long)(r0 + r) + P.lda * (P.k + c)];
@@ -161,6 +164,135 @@ __global__ void __launch_bounds__(256) k_panel(PanelArgs P) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Speculative panel: the 64 x jb diagonal block factored with interchanges
+// among its own rows, then L21 = A21 U11^-1 for all rows below.  Partial
+// pivoting over the whole panel would have chosen the same pivots exactly
+// when no |l21| exceeds 1 (a row below wins only with a strictly larger
+// candidate |a_ij^(j)| = |l_ij u_jj|; on a tie the first maximum, a block
+// row, wins), so then the result stands; otherwise (or on a zero pivot) the
+// panel is factored again by k_panel from the untouched matrix.
+// ---------------------------------------------------------------------------
+constexpr int SP = 64;  // speculative block rows = panel width
+
+// One wavefront: factor the 64 x jb diagonal block with interchanges among
+// its rows.  Lane r holds row r of the block in registers, shifted left by
+// one column per step so that the current column is always a[0] (constant
+// register indices, no scratch); rows are never moved: every lane carries the
+// current position of its row (an interchange swaps two labels), the pivot
+// row is broadcast through LDS, each finished column entry is kept per lane
+// in LDS, and the rows are stored at their final positions at the end.  The
+// factored block and the interchanges go to the workspace; a zero pivot
+// marks the speculation failed.
+__global__ void __launch_bounds__(64) k_spec_diag(const double *__restrict__ A, long lda, int k,
+                                                  int jb, double *D, int *lp, int *fail) {
+  __shared__ double prow[SP];
+  __shared__ double out[SP][SP + 1];  // [lane][column]
+  const int lane = threadIdx.x;
+  double a[SP];
+#pragma unroll
+  for (int c = 0; c < SP; ++c) a[c] = c < jb ? A[(long)(k + lane) + lda * (k + c)] : 0.0;
+  int pos = lane;  // current position of this lane's row
+  for (int j = 0; j < jb; ++j) {
+    // first maximum of |a_j| over the positions j.. (idamax: smallest position on ties)
+    double best = pos >= j ? fabs(a[0]) : -1.0;
+    int bp = pos >= j ? pos : 0x7fffffff;
+    int bl = lane;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(best, o, 64);
+      const int op = __shfl_xor(bp, o, 64), ol = __shfl_xor(bl, o, 64);
+      if (ov > best || (ov == best && op < bp)) {
+        best = ov;
+        bp = op;
+        bl = ol;
+      }
+    }
+    // bp: position of the pivot row, bl: its lane
+    if (lane == 0) lp[j] = bp;
+    const double piv = __shfl(a[0], bl, 64);
+    if (piv == 0.0) {
+      if (lane == 0) atomicOr(fail, 1);
+      return;  // uniform: every lane holds the same pivot
+    }
+    if (lane == bl) {
+#pragma unroll
+      for (int c = 1; c < SP; ++c) prow[c] = a[c];
+    }
+    __syncthreads();
+    // the row at position j takes the pivot row's old position
+    if (pos == j) pos = bp;
+    if (lane == bl) pos = j;
+    if (pos > j) {
+      const bool recip = fabs(piv) >= 2.2250738585072014e-308;
+      const double l = recip ? a[0] * (1.0 / piv) : a[0] / piv;
+      a[0] = l;
+#pragma unroll
+      for (int c = 1; c < SP; ++c) a[c] -= l * prow[c];
+    }
+    out[lane][j] = a[0];
+#pragma unroll
+    for (int c = 0; c < SP - 1; ++c) a[c] = a[c + 1];
+    a[SP - 1] = 0.0;
+    __syncthreads();
+  }
+  for (int c = 0; c < jb; ++c) D[pos + SP * c] = out[lane][c];
+}
+
+// L21 = A21 U11^-1 (row-wise forward substitution, one thread per row below
+// the block), into the workspace W(ldw, jb); any |l| > 1 fails the
+// speculation.
+template <int JB>
+__global__ void __launch_bounds__(256) k_spec_l21(const double *__restrict__ A, long lda, int k,
+                                                  int nrow, const double *__restrict__ D,
+                                                  double *W, long ldw, int *fail) {
+  __shared__ double su[JB][JB + 1];
+  __shared__ double sr[JB];
+  for (int q = threadIdx.x; q < JB * JB; q += 256) su[q % JB][q / JB] = D[(q % JB) + SP * (q / JB)];
+  __syncthreads();
+  if (threadIdx.x < JB) {
+    const double u = su[threadIdx.x][threadIdx.x];
+    sr[threadIdx.x] = fabs(u) >= 2.2250738585072014e-308 ? 1.0 / u : 0.0;
+  }
+  __syncthreads();
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= nrow) return;
+  const long row = k + SP + r;
+  double x[JB];
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < JB; ++j) {
+    double a = A[row + lda * (k + j)];
+#pragma unroll
+    for (int i = 0; i < j; ++i) a -= x[i] * su[i][j];
+    x[j] = a * sr[j];
+    bad |= !(fabs(x[j]) <= 1.0);
+  }
+#pragma unroll
+  for (int j = 0; j < JB; ++j) W[r + ldw * j] = x[j];
+  if (bad) atomicOr(fail, 1);
+}
+
+// The speculation stands (fail == 0): factored block, L21 and the
+// interchanges (global, 1-based) into A and ipiv.
+__global__ void k_spec_commit(double *A, long lda, int k, int jb, int nrow,
+                              const double *__restrict__ D, const int *__restrict__ lp,
+                              const double *__restrict__ W, long ldw, int *ipiv,
+                              const int *__restrict__ fail) {
+  if (*fail) return;
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nd = (long)SP * jb, nl = (long)nrow * jb;
+  if (q < nd) {
+    const int r = (int)(q % SP), c = (int)(q / SP);
+    A[(long)(k + r) + lda * (k + c)] = D[r + SP * c];
+  } else if (q < nd + nl) {
+    const long t = q - nd;
+    const int r = (int)(t % nrow), c = (int)(t / nrow);
+    A[(long)(k + SP + r) + lda * (k + c)] = W[r + ldw * c];
+  }
+  if (q < jb) ipiv[k + q] = k + lp[q] + 1;
+}
+
 // Row interchanges ipiv[k..k+jb) applied to the columns outside the panel
 // (dlaswp): one thread per column.
 __global__ void k_laswp(double *A, long lda, int ncol, int c0, int c1, int k, int jb,
@@ -178,82 +310,139 @@ __global__ void k_laswp(double *A, long lda, int ncol, int c0, int c1, int k, in
   }
 }
 
-// U12 = L11^-1 A12 for 64 columns per block; L11 unit lower (jb x jb) in LDS.
-__global__ void __launch_bounds__(256) k_trsm_llnu(const double *__restrict__ L, long ldl, double *B,
-                                                   long ldb, int jb, int ncol) {
-  extern __shared__ double st[];
-  double *sL = st;              // jb x jb, column-major
-  double *sX = st + jb * jb;    // jb x 64, column-major
-  const int c0 = blockIdx.x * 64;
-  const int nc = min(64, ncol - c0);
-  for (int q = threadIdx.x; q < jb * jb; q += blockDim.x) sL[q] = L[(q % jb) + ldl * (q / jb)];
-  for (int q = threadIdx.x; q < jb * nc; q += blockDim.x)
-    sX[q] = B[(q % jb) + ldb * (long)(c0 + q / jb)];
-  __syncthreads();
-  for (int i = 0; i < jb - 1; ++i) {
-    const int w = jb - 1 - i;
-    for (int q = threadIdx.x; q < w * nc; q += blockDim.x) {
-      int r = i + 1 + q % w, c = q / w;
-      sX[r + jb * c] -= sL[r + jb * i] * sX[i + jb * c];
-    }
-    __syncthreads();
+// B(jb, ncol) <- L^-1 B with L unit lower (jb <= 64): one thread per column,
+// its column in registers shifted by one row per step (the finished entry is
+// always x[0], constant register indices), L in LDS (every lane reads the
+// same element: broadcast); no barrier inside the substitution.
+__global__ void __launch_bounds__(64) k_trsm_llnu(const double *__restrict__ L, long ldl, double *B,
+                                                  long ldb, int jb, int ncol) {
+  __shared__ double sL[2 * 64][65];  // rows past jb are zero
+  for (int q = threadIdx.x; q < 2 * 64 * 64; q += 64) {
+    const int r = q & 127, c = q >> 7;
+    sL[r][c] = (r < jb && c < jb) ? L[r + ldl * c] : 0.0;
   }
-  for (int q = threadIdx.x; q < jb * nc; q += blockDim.x)
-    B[(q % jb) + ldb * (long)(c0 + q / jb)] = sX[q];
+  __syncthreads();
+  const int col = blockIdx.x * 64 + threadIdx.x;
+  if (col >= ncol) return;
+  double *b = B + ldb * (long)col;
+  double x[64];
+#pragma unroll
+  for (int i = 0; i < 64; ++i) x[i] = i < jb ? b[i] : 0.0;
+  for (int i = 0; i < jb; ++i) {
+    // x[0] is the finished entry i; eliminate it from the rows below
+#pragma unroll
+    for (int r = 1; r < 64; ++r) x[r] -= sL[i + r][i] * x[0];
+    b[i] = x[0];
+#pragma unroll
+    for (int r = 0; r < 63; ++r) x[r] = x[r + 1];
+    x[63] = 0.0;
+  }
 }
 
-// C(M,N) -= A(M,K) B(K,N), FP64 on the matrix cores (v_mfma_f64_16x16x4f64).
-// Block tile 64x64, four waves in 2x2, each wave a 32x32 tile of 2x2 MFMA
-// tiles; K staged through LDS in chunks of 16.
-__global__ void __launch_bounds__(256) k_gemm_f64_sub(int M, int N, int K, const double *__restrict__ A,
-                                                      long lda, const double *__restrict__ B, long ldb,
-                                                      double *C, long ldc) {
-  __shared__ double sA[16][64 + 1];  // [k][m]
-  __shared__ double sB[16][64 + 1];  // [k][n]
-  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+// C(M,N) -= A(M,K) B(K,N), column-major FP64, on the matrix cores
+// (v_mfma_f64_16x16x4f64).  Block tile 128 x 128, four waves in 2 x 2, each a
+// 64 x 64 tile of 4 x 4 MFMA tiles; K in chunks of 16 staged through LDS in
+// two buffers, so the next chunk's global loads are in flight while the
+// current chunk's MFMAs run (one barrier per chunk).  VEC: 16-byte loads
+// (even leading dimensions, 16-byte aligned A and B).
+constexpr int GT = 128, GK = 16, GPAD = 4;
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+template <int VEC>
+__global__ void __launch_bounds__(256, 2) k_dgemm_sub(int M, int N, int K, const double *__restrict__ A,
+                                                   long lda, const double *__restrict__ B, long ldb,
+                                                   double *C, long ldc) {
+  __shared__ double sA[2][GK][GT + GPAD];  // [k][m]
+  __shared__ double sB[2][GK][GT + GPAD];  // [k][n]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wr = wv & 1, wc = wv >> 1;
-  d4 acc[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    // A chunk 64 x 16: thread -> (m = tid % 64, k = tid / 64 + 4 t)
-    for (int t = 0; t < 4; ++t) {
-      int kk = tid / 64 + 4 * t, mm = tid % 64;
-      int gm = m0 + mm, gk = k0 + kk;
-      sA[kk][mm] = (gm < M && gk < K) ? A[gm + lda * gk] : 0.0;
-    }
-    // B chunk 16 x 64: thread -> (k = tid % 16, n = tid / 16 + 16 t)
-    for (int t = 0; t < 4; ++t) {
-      int kk = tid % 16, nn = tid / 16 + 16 * t;
-      int gn = n0 + nn, gk = k0 + kk;
-      sB[kk][nn] = (gn < N && gk < K) ? B[gk + ldb * gn] : 0.0;
-    }
-    __syncthreads();
+  const int m0 = blockIdx.x * GT, n0 = blockIdx.y * GT;
+  d4 acc[4][4];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+  // staging: A chunk rows ar, ar + 1 of columns k0 + ak + 4 q; B chunk rows
+  // k0 + bk, bk + 1 of columns bn + 32 q
+  const int ar = 2 * (tid & 63), ak = tid >> 6;
+  const int bk = 2 * (tid & 7), bn = tid >> 3;
+  dbl2 ra[4], rb[4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = k0 + ak + 4 * q, r = m0 + ar;
+      if (VEC && k < K && r + 1 < M) {
+        ra[q] = *(const dbl2 *)&A[r + lda * k];
+      } else {
+        ra[q].x = (k < K && r < M) ? A[r + lda * k] : 0.0;
+        ra[q].y = (k < K && r + 1 < M) ? A[r + 1 + lda * k] : 0.0;
+      }
+      const int kb = k0 + bk, c = n0 + bn + 32 * q;
+      if (VEC && kb + 1 < K && c < N) {
+        rb[q] = *(const dbl2 *)&B[kb + ldb * c];
+      } else {
+        rb[q].x = (kb < K && c < N) ? B[kb + ldb * c] : 0.0;
+        rb[q].y = (kb + 1 < K && c < N) ? B[kb + 1 + ldb * c] : 0.0;
+      }
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      sA[buf][ak + 4 * q][ar] = ra[q].x;
+      sA[buf][ak + 4 * q][ar + 1] = ra[q].y;
+      sB[buf][bk][bn + 32 * q] = rb[q].x;
+      sB[buf][bk + 1][bn + 32 * q] = rb[q].y;
+    }
+  };
+  const int nch = (K + GK - 1) / GK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int c = 0; c < nch; ++c) {
+    const int buf = c & 1;
+    if (c + 1 < nch) gload((c + 1) * GK);
+#pragma unroll
+    for (int ks = 0; ks < GK / 4; ++ks) {
       const int kk = ks * 4 + (lane >> 4);
-      double af[2], bf[2];
-      for (int a = 0; a < 2; ++a) af[a] = sA[kk][wr * 32 + a * 16 + (lane & 15)];
-      for (int b = 0; b < 2; ++b) bf[b] = sB[kk][wc * 32 + b * 16 + (lane & 15)];
-      for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b)
+      double af[4], bf[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) af[a] = sA[buf][kk][wr * 64 + a * 16 + (lane & 15)];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bf[b] = sB[buf][kk][wc * 64 + b * 16 + (lane & 15)];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
     }
+    if (c + 1 < nch) sstore(buf ^ 1);
     __syncthreads();
   }
   // FP64 accumulator layout of v_mfma_f64_16x16x4f64: lane holds column
-  // (lane % 16) and rows 4 i + lane / 16 (i = register index); the FP32
-  // variants use rows 4 (lane / 16) + i instead.
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) {
-      int gn = n0 + wc * 32 + b * 16 + (lane & 15);
+  // (lane % 16) and rows 4 i + lane / 16 (i = register index)
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int gn = n0 + wc * 64 + b * 16 + (lane & 15);
       if (gn >= N) continue;
+#pragma unroll
       for (int i = 0; i < 4; ++i) {
-        int gm = m0 + wr * 32 + a * 16 + 4 * i + (lane >> 4);
+        const int gm = m0 + wr * 64 + a * 16 + 4 * i + (lane >> 4);
         if (gm < M) C[gm + ldc * gn] -= acc[a][b][i];
       }
     }
+}
+
+static void gemm_sub(int M, int N, int K, const double *A, long lda, const double *B, long ldb,
+                     double *C, long ldc, hipStream_t st) {
+  if (M <= 0 || N <= 0 || K <= 0) return;
+  dim3 grid(ceil_div(M, GT), ceil_div(N, GT));
+  const bool vec = (lda % 2 == 0) && (ldb % 2 == 0) && (((uintptr_t)A & 15) == 0) &&
+                   (((uintptr_t)B & 15) == 0);
+  if (vec) k_dgemm_sub<1><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
+  else k_dgemm_sub<0><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
 }
 
 // ---------------------------------------------------------------------------
@@ -384,21 +573,96 @@ static int device_cus() {
   return g_cus;
 }
 
-int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t st, int nb,
+// Blocked right-looking LU with LAPACK's pivots: outer panels of NB = 256
+// columns factored as 64-column sub-panels (speculative first; k_panel when
+// the speculation fails), the sub-panels' updates inside the outer panel,
+// then a trailing update of K = 256 per outer panel (a quarter of the
+// trailing-matrix traffic of 64-column panels).  One panel of lookahead:
+// the next outer panel's columns are updated first and that panel is factored
+// on stream s1 while the rest of the trailing matrix is updated on stream s2,
+// which is kept off the first kLuReservedCUs compute units (a CU-masked
+// stream) so that the single-workgroup panel kernels always find a CU.  The
+// speculation's verdict is read back per sub-panel (s1 only; s2 keeps
+// running); a failed one waits for s2 and refactors the sub-panel with
+// k_panel, whose grid barrier needs all its blocks resident at once.
+constexpr int LU_NB = 256;
+constexpr int kLuReservedCUs = 32;
+
+struct GetrfWs {
+  int dev = -1;
+  hipStream_t s1 = nullptr, s2 = nullptr;
+  hipEvent_t ev_in = nullptr, ev_u12 = nullptr, ev_rest = nullptr, ev_done = nullptr;
+  HostBuf<int> hflag;
+};
+static GetrfWs g_getrf;
+
+static int getrf_streams(GetrfWs &w) {
+  if (w.s1) return 0;
+  DUST_HIP(hipStreamCreateWithFlags(&w.s1, hipStreamNonBlocking));
+  const int cus = device_cus();
+  std::vector<uint32_t> mask((cus + 31) / 32, 0u);
+  for (int c = kLuReservedCUs; c < cus; ++c) mask[c / 32] |= 1u << (c % 32);
+  if (cus <= 2 * kLuReservedCUs ||
+      hipExtStreamCreateWithCUMask(&w.s2, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
+    (void)hipGetLastError();
+    w.s2 = nullptr;
+    DUST_HIP(hipStreamCreateWithFlags(&w.s2, hipStreamNonBlocking));
+  }
+  DUST_HIP(hipEventCreateWithFlags(&w.ev_in, hipEventDisableTiming));
+  DUST_HIP(hipEventCreateWithFlags(&w.ev_u12, hipEventDisableTiming));
+  DUST_HIP(hipEventCreateWithFlags(&w.ev_rest, hipEventDisableTiming));
+  DUST_HIP(hipEventCreateWithFlags(&w.ev_done, hipEventDisableTiming));
+  DUST_TRY(w.hflag.reserve(4));
+  return 0;
+}
+
+static void getrf_release() {
+  GetrfWs &w = g_getrf;
+  for (hipEvent_t *e : {&w.ev_in, &w.ev_u12, &w.ev_rest, &w.ev_done})
+    if (*e) {
+      (void)hipEventDestroy(*e);
+      *e = nullptr;
+    }
+  for (hipStream_t *q : {&w.s1, &w.s2})
+    if (*q) {
+      (void)hipStreamDestroy(*q);
+      *q = nullptr;
+    }
+  w.hflag.release();
+}
+
+int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t cst, int nb,
                  int *info_out) {
   DUST_CHECK(m >= 0 && n >= 0 && lda >= std::max(1, m), DUST_E_ARG, "invalid getrf dimensions");
   *info_out = 0;
   if (m == 0 || n == 0) return 0;
+  nb = 64;  // the sub-panel width (the speculative block and k_panel's)
+  GetrfWs &W = g_getrf;
+  DUST_TRY(getrf_streams(W));
+  hipStream_t s1 = W.s1, s2 = W.s2;
   const int cus = device_cus();
   const int Gmax = cus;
-  // workspace: barrier + candidates
+  const int mn = std::min(m, n);
+  const int npanel = ceil_div(mn, nb);
+  // workspace: barrier + candidates | speculation verdicts | block | interchanges | L21
   size_t wbytes = 256 + sizeof(double) * 2 * Gmax + sizeof(int) * 2 * Gmax +
                   sizeof(double) * 2 * (size_t)Gmax * nb + sizeof(double) * 2 * nb + 64 +
                   sizeof(unsigned) * (Gmax + 16);
+  wbytes = (wbytes + 255) & ~(size_t)255;
+  const size_t off_fail = wbytes;
+  wbytes += ((sizeof(int) * npanel + 255) & ~(size_t)255);
+  const size_t off_d = wbytes;
+  wbytes += sizeof(double) * SP * SP;
+  const size_t off_lp = wbytes;
+  wbytes += 256;
+  const size_t off_w = wbytes;
+  wbytes += sizeof(double) * (size_t)std::max(m, 1) * SP;
   TmpDev<char> wsb;
   DUST_TRY(wsb.alloc(wbytes));
   char *ws = wsb.p;
-  DUST_HIP(hipMemsetAsync(ws, 0, wbytes, st));
+  DUST_HIP(hipEventRecord(W.ev_in, cst));
+  DUST_HIP(hipStreamWaitEvent(s1, W.ev_in, 0));
+  DUST_HIP(hipMemsetAsync(ws, 0, off_d, s1));
   unsigned *bar = (unsigned *)ws;
   int *dinfo = (int *)(ws + 16);
   double *cval = (double *)(ws + 256);
@@ -407,73 +671,127 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t st,
   crowv = (double *)(((uintptr_t)crowv + 15) & ~(uintptr_t)15);
   double *topv = crowv + 2 * (size_t)Gmax * nb;
   unsigned *arr = (unsigned *)(topv + 2 * nb);
+  int *spec_fail = (int *)(ws + off_fail);
+  double *Dblk = (double *)(ws + off_d);
+  int *lp = (int *)(ws + off_lp);
+  double *Wl = (double *)(ws + off_w);
   unsigned gen0 = 0;
-  const int mn = std::min(m, n);
   int rc = 0;
-  for (int k = 0; k < mn && !rc; k += nb) {
-    const int jb = std::min(nb, mn - k);
-    const int mrows = m - k;
-    int G = std::min(std::min(Gmax, 128), std::max(1, (mrows + 63) / 64));
-    int R = (mrows + G - 1) / G;
-    G = (mrows + R - 1) / R;
-    size_t lds = sizeof(double) * ((size_t)R * jb + 2 * jb);
-    if (lds > 150 * 1024) {
-      set_error("panel of %d x %d does not fit in LDS", R, jb);
-      rc = DUST_E_UNSUPPORTED;
-      break;
-    }
-    PanelArgs P{dA, lda, m, k, jb, R, dipiv, dinfo, arr, bar + 2, gen0, cval, crow, crowv, topv};
-    gen0 += jb;
-    if (lds > 64 * 1024)
-      (void)hipFuncSetAttribute((const void *)k_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
-    // G <= 128 blocks of one CU each: all resident at once on an idle GPU (the
-    // grid barrier is bounded and reports a time-out otherwise).  An ordinary
-    // launch: the kernel does its own synchronisation and needs none of the
-    // cooperative-launch resources, which another process may hold.
-    k_panel<<<G, 256, lds, st>>>(P);
+  bool s2_busy = false;  // ev_rest marks the last trailing update on s2
+  auto launch_check = [&](const char *what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
-      set_error("panel launch failed (G=%d, lds=%zu): %s", G, lds, hipGetErrorString(e));
+      set_error("%s launch failed: %s", what, hipGetErrorString(e));
       rc = DUST_E_HIP;
-      break;
     }
-    // interchanges on the other columns
-    k_laswp<<<ceil_div(n, 256), 256, 0, st>>>(dA, lda, n, k, k + jb, k, jb, dipiv);
-    if ((e = hipGetLastError()) != hipSuccess) {
-      set_error("laswp launch failed: %s", hipGetErrorString(e));
-      rc = DUST_E_HIP;
-      break;
-    }
-    const int nrest = n - k - jb;
-    if (nrest > 0) {
-      size_t tl = sizeof(double) * ((size_t)jb * jb + (size_t)jb * 64);
-      if (tl > 64 * 1024)
-        (void)hipFuncSetAttribute((const void *)k_trsm_llnu,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)tl);
-      k_trsm_llnu<<<ceil_div(nrest, 64), 256, tl, st>>>(dA + k + lda * k, lda,
-                                                        dA + k + lda * (k + jb), lda, jb, nrest);
-      const int mrest = m - k - jb;
-      if (mrest > 0) {
-        dim3 grid(ceil_div(mrest, 64), ceil_div(nrest, 64));
-        k_gemm_f64_sub<<<grid, 256, 0, st>>>(mrest, nrest, jb, dA + (k + jb) + lda * k, lda,
-                                             dA + k + lda * (k + jb), lda,
-                                             dA + (k + jb) + lda * (k + jb), lda);
+  };
+  // factor the outer panel [K0, K0 + JB) on s1 (its columns up to date)
+  auto factor_panel = [&](int K0, int JB) -> int {
+    for (int k = K0; k < K0 + JB; k += nb) {
+      const int jb = std::min(nb, K0 + JB - k);
+      const int mrows = m - k;
+      int *fail = spec_fail + (k / nb);
+      // speculative sub-panel: a full 64-row block and at least one row below
+      bool exact = true;
+      if (jb == nb && mrows > SP) {
+        const int nrow = mrows - SP;
+        k_spec_diag<<<1, 64, 0, s1>>>(dA, lda, k, jb, Dblk, lp, fail);
+        k_spec_l21<SP><<<ceil_div(nrow, 256), 256, 0, s1>>>(dA, lda, k, nrow, Dblk, Wl, nrow,
+                                                           fail);
+        const long tot = (long)SP * jb + (long)nrow * jb;
+        k_spec_commit<<<ceil_div(tot, 256), 256, 0, s1>>>(dA, lda, k, jb, nrow, Dblk, lp, Wl,
+                                                          nrow, dipiv, fail);
+        launch_check("speculative panel");
+        if (rc) return rc;
+        DUST_HIP(hipMemcpyAsync(W.hflag.p, fail, sizeof(int), hipMemcpyDeviceToHost, s1));
+        DUST_HIP(hipStreamSynchronize(s1));
+        exact = W.hflag.p[0] != 0;
       }
-      if ((e = hipGetLastError()) != hipSuccess) {
-        set_error("trailing update launch failed: %s", hipGetErrorString(e));
-        rc = DUST_E_HIP;
-        break;
+      if (exact) {
+        // all CUs for k_panel's resident grid: the trailing update first
+        if (s2_busy) DUST_HIP(hipStreamWaitEvent(s1, W.ev_rest, 0));
+        int G = std::min(std::min(Gmax, 128), std::max(1, (mrows + 63) / 64));
+        int R = (mrows + G - 1) / G;
+        G = (mrows + R - 1) / R;
+        size_t lds = sizeof(double) * ((size_t)R * jb + 2 * jb);
+        DUST_CHECK(lds <= 150 * 1024, DUST_E_UNSUPPORTED, "panel of %d x %d does not fit in LDS",
+                   R, jb);
+        PanelArgs P{dA, lda, m, k, jb, R, dipiv, dinfo, arr, bar + 2, gen0, cval, crow, crowv,
+                    topv, nullptr};
+        gen0 += jb;
+        if (lds > 64 * 1024)
+          (void)hipFuncSetAttribute((const void *)k_panel,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        // G <= 128 blocks of one CU each, all resident at once (the grid
+        // barrier is bounded and reports a time-out otherwise).  An ordinary
+        // launch: the kernel does its own synchronisation and needs none of
+        // the cooperative-launch resources, which another process may hold.
+        k_panel<<<G, 256, lds, s1>>>(P);
+        launch_check("panel");
+        if (rc) return rc;
       }
+      // interchanges on the other columns of the outer panel
+      if (JB > jb)
+        k_laswp<<<ceil_div(JB, 256), 256, 0, s1>>>(dA + lda * K0, lda, JB, k - K0, k - K0 + jb, k,
+                                                   jb, dipiv);
+      const int wrest = K0 + JB - k - jb;
+      if (wrest > 0) {
+        k_trsm_llnu<<<ceil_div(wrest, 64), 64, 0, s1>>>(dA + k + lda * k, lda,
+                                                        dA + k + lda * (k + jb), lda, jb, wrest);
+        gemm_sub(m - k - jb, wrest, jb, dA + (k + jb) + lda * k, lda, dA + k + lda * (k + jb), lda,
+                 dA + (k + jb) + lda * (k + jb), lda, s1);
+      }
+      launch_check("sub-panel update");
+      if (rc) return rc;
     }
+    return 0;
+  };
+  rc = factor_panel(0, std::min(LU_NB, mn));
+  for (int K0 = 0; K0 < mn && !rc; K0 += LU_NB) {
+    const int JB = std::min(LU_NB, mn - K0);
+    // the panel's interchanges on the columns outside it (after the last
+    // trailing update)
+    if (s2_busy) DUST_HIP(hipStreamWaitEvent(s1, W.ev_rest, 0));
+    s2_busy = false;
+    k_laswp<<<ceil_div(n, 256), 256, 0, s1>>>(dA, lda, n, K0, K0 + JB, K0, JB, dipiv);
+    launch_check("laswp");
+    if (rc) break;
+    const int nrest = n - K0 - JB;
+    if (nrest <= 0) break;
+    // U12 = L11^-1 A12 (blocked by 64)
+    rc = lu_trsm_llnu_dev(JB, nrest, dA + K0 + lda * K0, lda, dA + K0 + lda * (K0 + JB), lda, s1);
+    if (rc) break;
+    const int K1 = K0 + JB;
+    const int JB2 = K1 < mn ? std::min(LU_NB, mn - K1) : 0;
+    const int mrest = m - K1;
+    // the next panel's columns first (s1), the rest on s2 meanwhile
+    if (JB2 > 0)
+      gemm_sub(mrest, JB2, JB, dA + K1 + lda * K0, lda, dA + K0 + lda * K1, lda,
+               dA + K1 + lda * K1, lda, s1);
+    if (nrest > JB2) {
+      DUST_HIP(hipEventRecord(W.ev_u12, s1));
+      DUST_HIP(hipStreamWaitEvent(s2, W.ev_u12, 0));
+      gemm_sub(mrest, nrest - JB2, JB, dA + K1 + lda * K0, lda, dA + K0 + lda * (K1 + JB2), lda,
+               dA + K1 + lda * (K1 + JB2), lda, s2);
+      DUST_HIP(hipEventRecord(W.ev_rest, s2));
+      s2_busy = true;
+    }
+    launch_check("trailing update");
+    if (rc) break;
+    if (JB2 > 0) rc = factor_panel(K1, JB2);
   }
+  if (s2_busy) (void)hipStreamWaitEvent(s1, W.ev_rest, 0);
   unsigned hb[3] = {0, 0, 0};
   int hinfo = 0;
   if (!rc) {
-    DUST_HIP(hipMemcpyAsync(hb, bar, sizeof(hb), hipMemcpyDeviceToHost, st));
-    DUST_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, st));
+    DUST_HIP(hipMemcpyAsync(hb, bar, sizeof(hb), hipMemcpyDeviceToHost, s1));
+    DUST_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, s1));
   }
-  DUST_HIP(hipStreamSynchronize(st));
+  DUST_HIP(hipStreamSynchronize(s1));
+  DUST_HIP(hipStreamSynchronize(s2));
+  // the caller's stream continues after the factorisation
+  DUST_HIP(hipEventRecord(W.ev_done, s1));
+  DUST_HIP(hipStreamWaitEvent(cst, W.ev_done, 0));
   if (rc) return rc;
   DUST_CHECK(hb[2] == 0, DUST_E_HIP, "panel grid barrier timed out");
   *info_out = hinfo;
@@ -488,6 +806,7 @@ struct GetrsWs {
 static GetrsWs g_getrs;
 
 void lu_release() {
+  getrf_release();
   g_getrs.linv.release();
   g_getrs.uinv.release();
   g_getrs.y.release();
@@ -637,8 +956,7 @@ int lu_getrs_noalloc_dev(int n, int nrhs, const double *dA, long lda, const int 
 int lu_gemm_sub_dev(int M, int N, int K, const double *A, long lda, const double *B, long ldb,
                     double *C, long ldc, hipStream_t st) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
-  dim3 grid(ceil_div(M, 64), ceil_div(N, 64));
-  k_gemm_f64_sub<<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
+  gemm_sub(M, N, K, A, lda, B, ldb, C, ldc, st);
   DUST_HIP(hipGetLastError());
   return 0;
 }
@@ -651,11 +969,11 @@ int lu_laswp_dev(double *A, long lda, int ncol, int k, int jb, const int *dipiv,
 }
 
 int lu_trsm_llnu_dev(int m, int n, const double *L, long ldl, double *B, long ldb, hipStream_t st) {
-  // blocked: diagonal blocks of 64 solved in LDS, off-diagonal updates by GEMM
+  // blocked: diagonal blocks of 64 (one thread per column), off-diagonal
+  // updates by GEMM
   for (int k = 0; k < m; k += 64) {
     int jb = std::min(64, m - k);
-    size_t tl = sizeof(double) * ((size_t)jb * jb + (size_t)jb * 64);
-    k_trsm_llnu<<<ceil_div(n, 64), 256, tl, st>>>(L + k + ldl * k, ldl, B + k, ldb, jb, n);
+    k_trsm_llnu<<<ceil_div(n, 64), 64, 0, st>>>(L + k + ldl * k, ldl, B + k, ldb, jb, n);
     if (k + jb < m) DUST_TRY(lu_gemm_sub_dev(m - k - jb, n, jb, L + (k + jb) + ldl * k, ldl, B + k, ldb,
                                              B + k + jb, ldb, st));
   }

@@ -152,7 +152,7 @@ def wing_case(nc=40, ns=20, aspect=8.0, alpha_deg=5.0, u=30.0, nrows=200, seed=7
     # reversed vertex order: outward normals (calc_geo_data_surfpan's
     # (v3 - v1) x (v4 - v2)); the trailing-edge points stay vertices 1 and 4
     # of the upper panels
-    ver = np.array(ver)[:, ::-1]
+    ver = np.ascontiguousarray(np.array(ver)[:, ::-1])
     nv = np.full(len(ver), 4, np.int32)
     # particle sheet
     rng = np.random.default_rng(seed)

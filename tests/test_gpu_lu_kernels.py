@@ -101,3 +101,27 @@ def test_shutdown_then_reuse():
     assert np.array_equal(x, y)
     ym = LS.dgesv_mixed(A0, np.asfortranarray(b.copy()), prec="fp16s")[0]
     assert np.array_equal(xm, ym)
+
+
+@pytest.mark.parametrize("case", ["dominant", "mixed", "ties"])
+def test_dgetrf_speculative_and_exact_panels(case):
+    """Panels whose own-block pivoting matches partial pivoting (the
+    speculative path) and panels that need interchanges with rows below
+    (factored again exactly), against LAPACK: same pivots, same factors."""
+    import scipy.linalg.lapack as lapack
+    from dust_amd import linsys as LS
+    rng = np.random.default_rng(17)
+    n = 700
+    A0 = rng.standard_normal((n, n)) + 4.0 * np.sqrt(n) * np.eye(n)
+    if case == "mixed":
+        # rows of the third 64-column panel need a row from far below
+        A0[[140, 600]] = A0[[600, 140]]
+    if case == "ties":
+        # a row below the block whose candidate equals the block's pivot
+        A0[300, 0] = A0[0, 0]
+    A = np.asfortranarray(A0.copy())
+    ipiv, info = LS.dgetrf(A)
+    lu, piv, info_s = lapack.dgetrf(A0)
+    assert info == info_s == 0
+    assert np.array_equal(ipiv - 1, piv)
+    assert np.abs(A - lu).max() <= 1e-12 * np.abs(lu).max()
