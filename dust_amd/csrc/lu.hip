@@ -165,132 +165,317 @@ __global__ void __launch_bounds__(256) k_panel(PanelArgs P) {
 }
 
 // ---------------------------------------------------------------------------
-// Speculative panel: the 64 x jb diagonal block factored with interchanges
+// Speculative panel: the 64 x 64 diagonal block factored with interchanges
 // among its own rows, then L21 = A21 U11^-1 for all rows below.  Partial
 // pivoting over the whole panel would have chosen the same pivots exactly
-// when no |l21| exceeds 1 (a row below wins only with a strictly larger
-// candidate |a_ij^(j)| = |l_ij u_jj|; on a tie the first maximum, a block
-// row, wins), so then the result stands; otherwise (or on a zero pivot) the
-// panel is factored again by k_panel from the untouched matrix.
+// when no row below ever offers a strictly larger candidate, |a_ij^(j)| >
+// |u_jj| (idamax takes the first maximum, and the block rows come first), so
+// then the result stands.  Both kernels work in place and keep the original
+// values in a backup; when the speculation fails (or meets a pivot below the
+// smallest normal number, an infinity or a NaN) the host restores the panel
+// and factors it again with k_panel.
 // ---------------------------------------------------------------------------
 constexpr int SP = 64;  // speculative block rows = panel width
+constexpr double kSfmin = 2.2250738585072014e-308;  // dlamch('S')
 
-// One wavefront: factor the 64 x jb diagonal block with interchanges among
-// its rows.  Lane r holds row r of the block in registers, shifted left by
-// one column per step so that the current column is always a[0] (constant
-// register indices, no scratch); rows are never moved: every lane carries the
-// current position of its row (an interchange swaps two labels), the pivot
-// row is broadcast through LDS, each finished column entry is kept per lane
-// in LDS, and the rows are stored at their final positions at the end.  The
-// factored block and the interchanges go to the workspace; a zero pivot
-// marks the speculation failed.
-__global__ void __launch_bounds__(64) k_spec_diag(const double *__restrict__ A, long lda, int k,
-                                                  int jb, double *D, int *lp, int *fail) {
-  __shared__ double prow[SP];
-  __shared__ double out[SP][SP + 1];  // [lane][column]
-  const int lane = threadIdx.x;
-  double a[SP];
-#pragma unroll
-  for (int c = 0; c < SP; ++c) a[c] = c < jb ? A[(long)(k + lane) + lda * (k + c)] : 0.0;
-  int pos = lane;  // current position of this lane's row
-  for (int j = 0; j < jb; ++j) {
-    // first maximum of |a_j| over the positions j.. (idamax: smallest position on ties)
-    double best = pos >= j ? fabs(a[0]) : -1.0;
-    int bp = pos >= j ? pos : 0x7fffffff;
-    int bl = lane;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ov = __shfl_xor(best, o, 64);
-      const int op = __shfl_xor(bp, o, 64), ol = __shfl_xor(bl, o, 64);
-      if (ov > best || (ov == best && op < bp)) {
-        best = ov;
-        bp = op;
-        bl = ol;
-      }
-    }
-    // bp: position of the pivot row, bl: its lane
-    if (lane == 0) lp[j] = bp;
-    const double piv = __shfl(a[0], bl, 64);
-    if (piv == 0.0) {
-      if (lane == 0) atomicOr(fail, 1);
-      return;  // uniform: every lane holds the same pivot
-    }
-    if (lane == bl) {
-#pragma unroll
-      for (int c = 1; c < SP; ++c) prow[c] = a[c];
-    }
-    __syncthreads();
-    // the row at position j takes the pivot row's old position
-    if (pos == j) pos = bp;
-    if (lane == bl) pos = j;
-    if (pos > j) {
-      const bool recip = fabs(piv) >= 2.2250738585072014e-308;
-      const double l = recip ? a[0] * (1.0 / piv) : a[0] / piv;
-      a[0] = l;
-#pragma unroll
-      for (int c = 1; c < SP; ++c) a[c] -= l * prow[c];
-    }
-    out[lane][j] = a[0];
-#pragma unroll
-    for (int c = 0; c < SP - 1; ++c) a[c] = a[c + 1];
-    a[SP - 1] = 0.0;
-    __syncthreads();
-  }
-  for (int c = 0; c < jb; ++c) D[pos + SP * c] = out[lane][c];
+typedef unsigned long long u64;
+
+// Cross-lane moves inside rows of 16 lanes (DPP): quad_perm [1,0,3,2] and
+// [2,3,0,1], row_half_mirror, row_mirror.  All four are permutations, so
+// every lane receives a value.
+template <int CTRL>
+__device__ inline u64 dpp64(u64 v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)v, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), CTRL, 0xf, 0xf, false);
+  return ((u64)(unsigned)hi << 32) | (unsigned)lo;
 }
 
-// L21 = A21 U11^-1 (row-wise forward substitution, one thread per row below
-// the block), into the workspace W(ldw, jb); any |l| > 1 fails the
-// speculation.
-template <int JB>
-__global__ void __launch_bounds__(256) k_spec_l21(const double *__restrict__ A, long lda, int k,
-                                                  int nrow, const double *__restrict__ D,
-                                                  double *W, long ldw, int *fail) {
-  __shared__ double su[JB][JB + 1];
-  __shared__ double sr[JB];
-  for (int q = threadIdx.x; q < JB * JB; q += 256) su[q % JB][q / JB] = D[(q % JB) + SP * (q / JB)];
-  __syncthreads();
-  if (threadIdx.x < JB) {
-    const double u = su[threadIdx.x][threadIdx.x];
-    sr[threadIdx.x] = fabs(u) >= 2.2250738585072014e-308 ? 1.0 / u : 0.0;
+__device__ inline u64 readlane64(u64 v, int l) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), l);
+  return ((u64)hi << 32) | lo;
+}
+
+__device__ inline double readlane_d(double v, int l) {
+  return __longlong_as_double((long long)readlane64((u64)__double_as_longlong(v), l));
+}
+
+// Maximum (MAX) or minimum over the wavefront, the same value in every lane.
+template <bool MAX>
+__device__ inline u64 wave_ext_u64(u64 v) {
+  auto pick = [](u64 a, u64 b) { return MAX ? (a > b ? a : b) : (a < b ? a : b); };
+  v = pick(v, dpp64<0xB1>(v));
+  v = pick(v, dpp64<0x4E>(v));
+  v = pick(v, dpp64<0x141>(v));
+  v = pick(v, dpp64<0x140>(v));
+  return pick(pick(readlane64(v, 0), readlane64(v, 16)), pick(readlane64(v, 32), readlane64(v, 48)));
+}
+
+// X = L^-1 for the unit lower triangle of sD (rows and columns < 64; the
+// diagonal and upper part are not read), by 256 threads: lane c is column c,
+// wave q rows [16q, 16q+16), one 16-row block substitution per wave and the
+// updates from earlier blocks through sX.  X(64 x 64) column-major.
+__device__ inline void unit_lower_inverse(const double (*sD)[SP + 1], double (*sX)[SP + 1],
+                                          double *X) {
+  const int q = threadIdx.x >> 6, c = threadIdx.x & 63;
+  double x[16];
+#pragma unroll
+  for (int ii = 0; ii < 16; ++ii) x[ii] = (16 * q + ii == c) ? 1.0 : 0.0;
+  for (int p = 0; p < 4; ++p) {
+    if (q == p) {
+#pragma unroll
+      for (int kk = 0; kk < 15; ++kk)
+#pragma unroll
+        for (int ii = kk + 1; ii < 16; ++ii) x[ii] -= sD[16 * p + ii][16 * p + kk] * x[kk];
+#pragma unroll
+      for (int ii = 0; ii < 16; ++ii) sX[16 * p + ii][c] = x[ii];
+    }
+    __syncthreads();
+    if (q > p) {
+#pragma unroll 2
+      for (int kk = 0; kk < 16; ++kk) {
+        const double y = sX[16 * p + kk][c];
+#pragma unroll
+        for (int ii = 0; ii < 16; ++ii) x[ii] -= sD[16 * q + ii][16 * p + kk] * y;
+      }
+    }
+  }
+#pragma unroll
+  for (int ii = 0; ii < 16; ++ii) X[(16 * q + ii) + SP * c] = x[ii];
+}
+
+// Factor the 64 x 64 diagonal block at (k, k) in place with interchanges
+// among its rows (dgetf2 order: per column, the first maximum of |a| among
+// the rows not yet pivoted, scaling by the reciprocal pivot, rank-1 update).
+// 256 threads: wave w holds columns [16w, 16w+16) of the block, lane r row r,
+// in registers; rows never move, every lane carries the current position of
+// its row (an interchange swaps two labels).  In phase p wave p factors its
+// 16 columns alone (DPP maximum of |a| as 64-bit keys, ties to the smallest
+// position; the pivot row from its lane by readlane), then publishes the
+// pivot lanes, positions and multipliers through LDS; after the barrier the
+// waves to its right replay the 16 eliminations on their columns and the
+// waves to its left follow the interchanges: four barriers in all.  Writes
+// the original block to Dorig, the factored block to A, the interchanges to
+// ipiv (1-based, global rows) and L^-1 to Linv; a bad pivot sets *fail and
+// leaves A unchanged.
+__global__ void __launch_bounds__(256) k_spec_diag(double *A, long lda, int k, double *Dorig,
+                                                   int *ipiv, double *Linv, int *fail) {
+  __shared__ double sl[2][SP][17];  // multipliers of a phase, [row][column in phase]
+  __shared__ int sbl[SP], sbp[SP];  // pivot lane, pivot position per column
+  __shared__ int sbad;
+  __shared__ double sD[SP][SP + 1];
+  __shared__ double sX[SP][SP + 1];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (threadIdx.x == 0) sbad = 0;
+  double a[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) a[c] = A[(long)(k + lane) + lda * (k + 16 * w + c)];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) Dorig[lane + SP * (16 * w + c)] = a[c];
+  int pos = lane;
+  for (int ph = 0; ph < 4; ++ph) {
+    if (w == ph) {
+      bool bad = false;
+#pragma unroll
+      for (int jl = 0; jl < 16; ++jl) {
+        const int j = 16 * ph + jl;
+        const bool elig = pos >= j;
+        const double v = a[jl];
+        const u64 key = elig ? (u64)__double_as_longlong(fabs(v)) : 0ull;
+        const u64 kmax = wave_ext_u64<true>(key);
+        const u64 m = __ballot(elig && key == kmax);
+        int bl;
+        if (__popcll(m) == 1) {
+          bl = __ffsll((long long)m) - 1;
+        } else {  // tie: the smallest position among the maxima
+          const bool in = (m >> lane) & 1ull;
+          const u64 pmin = wave_ext_u64<false>(in ? (u64)pos : ~0ull);
+          bl = __ffsll((long long)__ballot(in && (u64)pos == pmin)) - 1;
+        }
+        bl = __builtin_amdgcn_readfirstlane(bl);
+        const double piv = readlane_d(v, bl);
+        const int bp = __builtin_amdgcn_readlane(pos, bl);
+        bad |= kmax == 0ull || kmax >= 0x7ff0000000000000ull || !(fabs(piv) >= kSfmin);
+        double prow[16];
+#pragma unroll
+        for (int c = jl + 1; c < 16; ++c) prow[c] = readlane_d(a[c], bl);
+        if (pos == j)
+          pos = bp;
+        if (lane == bl) pos = j;
+        if (pos > j) {
+          const double l = v * (1.0 / piv);
+          a[jl] = l;
+#pragma unroll
+          for (int c = jl + 1; c < 16; ++c) a[c] -= l * prow[c];
+        }
+        if (lane == 0) {
+          sbl[j] = bl;
+          sbp[j] = bp;
+          ipiv[k + j] = k + bp + 1;
+        }
+      }
+#pragma unroll
+      for (int jl = 0; jl < 16; ++jl) sl[ph & 1][lane][jl] = a[jl];
+      if (bad && lane == 0) sbad = 1;
+    }
+    __syncthreads();
+    if (sbad) {
+      if (threadIdx.x == 0) atomicOr(fail, 1);
+      return;  // uniform over the block
+    }
+    if (w != ph) {
+      for (int jl = 0; jl < 16; ++jl) {
+        const int j = 16 * ph + jl;
+        const int bl = __builtin_amdgcn_readfirstlane(sbl[j]);
+        const int bp = sbp[j];
+        if (w > ph) {
+          double prow[16];
+#pragma unroll
+          for (int c = 0; c < 16; ++c) prow[c] = readlane_d(a[c], bl);
+          if (pos == j)
+            pos = bp;
+          if (lane == bl) pos = j;
+          if (pos > j) {
+            const double l = sl[ph & 1][lane][jl];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) a[c] -= l * prow[c];
+          }
+        } else {
+          if (pos == j)
+            pos = bp;
+          if (lane == bl) pos = j;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 16; ++c) {
+    A[(long)(k + pos) + lda * (k + 16 * w + c)] = a[c];
+    sD[pos][16 * w + c] = a[c];
   }
   __syncthreads();
-  const int r = blockIdx.x * 256 + threadIdx.x;
+  unit_lower_inverse(sD, sX, Linv);
+}
+
+// L^-1 of the unit lower triangle of the jb x jb block at A (jb <= 64; the
+// rest of the 64 x 64 result is the identity), for a panel that k_panel
+// factored.
+__global__ void __launch_bounds__(256) k_trtri_unit(const double *__restrict__ A, long lda, int jb,
+                                                    double *Linv) {
+  __shared__ double sD[SP][SP + 1];
+  __shared__ double sX[SP][SP + 1];
+  {
+    // rows and columns clamped to the block (loads without branches), then
+    // the strict lower triangle kept
+    const int r = threadIdx.x & 63, rr = min(r, jb - 1);
+    double t[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) t[i] = A[rr + lda * min((int)(threadIdx.x >> 6) + 4 * i, jb - 1)];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int c = (int)(threadIdx.x >> 6) + 4 * i;
+      sD[r][c] = (r < jb && c < r) ? t[i] : 0.0;
+    }
+  }
+  __syncthreads();
+  unit_lower_inverse(sD, sX, Linv);
+}
+
+// L21 = A21 U11^-1 in place for the nrow rows below the factored diagonal
+// block, one thread per row (dgetf2 order: scale by the reciprocal pivot,
+// then subtract from the later columns), the original rows into W(ldw, 64).
+// A row whose candidate |a_ij^(j)| exceeds |u_jj| fails the speculation.
+__global__ void __launch_bounds__(64) k_spec_l21(double *__restrict__ A, long lda, int k, int nrow,
+                                                 double *__restrict__ W, long ldw, int *fail) {
+  __shared__ double su[SP][SP + 1];
+  __shared__ double sr[SP];
+  for (int c0 = 0; c0 < SP; c0 += 16) {
+    double t[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) t[c] = A[(long)(k + threadIdx.x) + lda * (k + c0 + c)];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) su[threadIdx.x][c0 + c] = (int)threadIdx.x <= c0 + c ? t[c] : 0.0;
+  }
+  __syncthreads();
+  sr[threadIdx.x] = 1.0 / su[threadIdx.x][threadIdx.x];
+  __syncthreads();
+  const int r = blockIdx.x * 64 + threadIdx.x;
   if (r >= nrow) return;
   const long row = k + SP + r;
-  double x[JB];
+  double a[SP];
+#pragma unroll
+  for (int j = 0; j < SP; ++j) a[j] = A[row + lda * (k + j)];
+#pragma unroll
+  for (int j = 0; j < SP; ++j) W[r + ldw * j] = a[j];
   bool bad = false;
 #pragma unroll
-  for (int j = 0; j < JB; ++j) {
-    double a = A[row + lda * (k + j)];
+  for (int i = 0; i < SP; ++i) {
+    bad |= !(fabs(a[i]) <= fabs(su[i][i]));
+    a[i] *= sr[i];
 #pragma unroll
-    for (int i = 0; i < j; ++i) a -= x[i] * su[i][j];
-    x[j] = a * sr[j];
-    bad |= !(fabs(x[j]) <= 1.0);
+    for (int j = i + 1; j < SP; ++j) a[j] -= a[i] * su[i][j];
   }
 #pragma unroll
-  for (int j = 0; j < JB; ++j) W[r + ldw * j] = x[j];
+  for (int j = 0; j < SP; ++j) A[row + lda * (k + j)] = a[j];
   if (bad) atomicOr(fail, 1);
 }
 
-// The speculation stands (fail == 0): factored block, L21 and the
-// interchanges (global, 1-based) into A and ipiv.
-__global__ void k_spec_commit(double *A, long lda, int k, int jb, int nrow,
-                              const double *__restrict__ D, const int *__restrict__ lp,
-                              const double *__restrict__ W, long ldw, int *ipiv,
-                              const int *__restrict__ fail) {
-  if (*fail) return;
+// The speculation failed: the panel's original columns back into A.
+__global__ void k_spec_restore(double *A, long lda, int k, int nrow, const double *__restrict__ Dorig,
+                               const double *__restrict__ W, long ldw) {
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long nd = (long)SP * jb, nl = (long)nrow * jb;
+  const long nd = (long)SP * SP, nl = (long)nrow * SP;
   if (q < nd) {
     const int r = (int)(q % SP), c = (int)(q / SP);
-    A[(long)(k + r) + lda * (k + c)] = D[r + SP * c];
+    A[(long)(k + r) + lda * (k + c)] = Dorig[r + SP * c];
   } else if (q < nd + nl) {
     const long t = q - nd;
     const int r = (int)(t % nrow), c = (int)(t / nrow);
     A[(long)(k + SP + r) + lda * (k + c)] = W[r + ldw * c];
   }
-  if (q < jb) ipiv[k + q] = k + lp[q] + 1;
+}
+
+// B(jb, ncol) <- L^-1 B with X = L^-1 (64 x 64 column-major, from
+// k_spec_diag or k_trtri_unit), jb <= 64: 256 threads per 64 columns, the
+// column tile staged in LDS, lane c computing column c's rows [16q, 16q+16)
+// for wave q.
+__global__ void __launch_bounds__(256) k_trsm_inv(const double *__restrict__ X, int jb,
+                                                  double *__restrict__ B, long ldb, int ncol) {
+  __shared__ double sX[SP][SP + 1];
+  __shared__ double sB[SP][SP + 1];
+  const int c0 = blockIdx.x * 64;
+  {
+    // rows and columns clamped to the tile (loads without branches)
+    const int r = threadIdx.x & 63, rr = min(r, jb - 1);
+    double tx[16], tb[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int c = (int)(threadIdx.x >> 6) + 4 * i;
+      tx[i] = X[r + SP * c];
+      tb[i] = B[rr + ldb * (long)min(c0 + c, ncol - 1)];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int c = (int)(threadIdx.x >> 6) + 4 * i;
+      sX[r][c] = tx[i];
+      sB[r][c] = (r < jb && c0 + c < ncol) ? tb[i] : 0.0;
+    }
+  }
+  __syncthreads();
+  const int q = threadIdx.x >> 6, c = threadIdx.x & 63;
+  double y[16];
+#pragma unroll
+  for (int ii = 0; ii < 16; ++ii) y[ii] = 0.0;
+  const int kend = min(16 * q + 16, jb);
+  for (int kk = 0; kk < kend; ++kk) {
+    const double b = sB[kk][c];
+#pragma unroll
+    for (int ii = 0; ii < 16; ++ii) y[ii] += sX[16 * q + ii][kk] * b;
+  }
+  if (c0 + c >= ncol) return;
+#pragma unroll
+  for (int ii = 0; ii < 16; ++ii)
+    if (16 * q + ii < jb) B[(16 * q + ii) + ldb * (long)(c0 + c)] = y[ii];
 }
 
 // Row interchanges ipiv[k..k+jb) applied to the columns outside the panel
@@ -344,7 +529,7 @@ __global__ void __launch_bounds__(64) k_trsm_llnu(const double *__restrict__ L, 
 // 64 x 64 tile of 4 x 4 MFMA tiles; K in chunks of 16 staged through LDS in
 // two buffers, so the next chunk's global loads are in flight while the
 // current chunk's MFMAs run (one barrier per chunk).  VEC: 16-byte loads
-// (even leading dimensions, 16-byte aligned A and B).
+// (even M, K and leading dimensions, 16-byte aligned A and B).
 constexpr int GT = 128, GK = 16, GPAD = 4;
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
@@ -367,37 +552,41 @@ __global__ void __launch_bounds__(256, 2) k_dgemm_sub(int M, int N, int K, const
   const int ar = 2 * (tid & 63), ak = tid >> 6;
   const int bk = 2 * (tid & 7), bn = tid >> 3;
   dbl2 ra[4], rb[4];
+  // addresses clamped into the matrices, out-of-range values replaced by
+  // zeros only when they are stored to LDS: no branches or uses next to the
+  // loads, so the next chunk's loads stay in flight while the current
+  // chunk's MFMAs run
   auto gload = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int k = k0 + ak + 4 * q, r = m0 + ar;
-      if (VEC && k < K && r + 1 < M) {
-        ra[q] = *(const dbl2 *)&A[r + lda * k];
+      const int kc = min(k0 + ak + 4 * q, K - 1), r = m0 + ar;
+      const int kb = k0 + bk, cc = min(n0 + bn + 32 * q, N - 1);
+      if (VEC) {
+        // M and K even: a pair starting at an even index is in range or not
+        ra[q] = *(const dbl2 *)&A[min(r, M - 2) + lda * kc];
+        rb[q] = *(const dbl2 *)&B[min(kb, K - 2) + ldb * cc];
       } else {
-        ra[q].x = (k < K && r < M) ? A[r + lda * k] : 0.0;
-        ra[q].y = (k < K && r + 1 < M) ? A[r + 1 + lda * k] : 0.0;
-      }
-      const int kb = k0 + bk, c = n0 + bn + 32 * q;
-      if (VEC && kb + 1 < K && c < N) {
-        rb[q] = *(const dbl2 *)&B[kb + ldb * c];
-      } else {
-        rb[q].x = (kb < K && c < N) ? B[kb + ldb * c] : 0.0;
-        rb[q].y = (kb + 1 < K && c < N) ? B[kb + 1 + ldb * c] : 0.0;
+        ra[q].x = A[min(r, M - 1) + lda * kc];
+        ra[q].y = A[min(r + 1, M - 1) + lda * kc];
+        rb[q].x = B[min(kb, K - 1) + ldb * cc];
+        rb[q].y = B[min(kb + 1, K - 1) + ldb * cc];
       }
     }
   };
-  auto sstore = [&](int buf) {
+  auto sstore = [&](int buf, int k0) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      sA[buf][ak + 4 * q][ar] = ra[q].x;
-      sA[buf][ak + 4 * q][ar + 1] = ra[q].y;
-      sB[buf][bk][bn + 32 * q] = rb[q].x;
-      sB[buf][bk + 1][bn + 32 * q] = rb[q].y;
+      const int k = k0 + ak + 4 * q, r = m0 + ar;
+      const int kb = k0 + bk, c = n0 + bn + 32 * q;
+      sA[buf][ak + 4 * q][ar] = (k < K && r < M) ? ra[q].x : 0.0;
+      sA[buf][ak + 4 * q][ar + 1] = (k < K && r + 1 < M) ? ra[q].y : 0.0;
+      sB[buf][bk][bn + 32 * q] = (kb < K && c < N) ? rb[q].x : 0.0;
+      sB[buf][bk + 1][bn + 32 * q] = (kb + 1 < K && c < N) ? rb[q].y : 0.0;
     }
   };
   const int nch = (K + GK - 1) / GK;
   gload(0);
-  sstore(0);
+  sstore(0, 0);
   __syncthreads();
   for (int c = 0; c < nch; ++c) {
     const int buf = c & 1;
@@ -416,31 +605,43 @@ __global__ void __launch_bounds__(256, 2) k_dgemm_sub(int M, int N, int K, const
         for (int b = 0; b < 4; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
     }
-    if (c + 1 < nch) sstore(buf ^ 1);
+    if (c + 1 < nch) sstore(buf ^ 1, (c + 1) * GK);
     __syncthreads();
   }
   // FP64 accumulator layout of v_mfma_f64_16x16x4f64: lane holds column
-  // (lane % 16) and rows 4 i + lane / 16 (i = register index)
+  // (lane % 16) and rows 4 i + lane / 16 (i = register index).  C in four
+  // batches of 16 values per lane: all loads of a batch (clamped addresses),
+  // then its stores.
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 4; ++a) {
+    double cv[4][4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int gn = min(n0 + wc * 64 + b * 16 + (lane & 15), N - 1);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int gm = min(m0 + wr * 64 + a * 16 + 4 * i + (lane >> 4), M - 1);
+        cv[b][i] = C[gm + ldc * gn];
+      }
+    }
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int gn = n0 + wc * 64 + b * 16 + (lane & 15);
-      if (gn >= N) continue;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int gm = m0 + wr * 64 + a * 16 + 4 * i + (lane >> 4);
-        if (gm < M) C[gm + ldc * gn] -= acc[a][b][i];
+        if (gm < M && gn < N) C[gm + ldc * gn] = cv[b][i] - acc[a][b][i];
       }
     }
+  }
 }
 
 static void gemm_sub(int M, int N, int K, const double *A, long lda, const double *B, long ldb,
                      double *C, long ldc, hipStream_t st) {
   if (M <= 0 || N <= 0 || K <= 0) return;
   dim3 grid(ceil_div(M, GT), ceil_div(N, GT));
-  const bool vec = (lda % 2 == 0) && (ldb % 2 == 0) && (((uintptr_t)A & 15) == 0) &&
-                   (((uintptr_t)B & 15) == 0);
+  const bool vec = (M % 2 == 0) && (K % 2 == 0) && (lda % 2 == 0) && (ldb % 2 == 0) &&
+                   (((uintptr_t)A & 15) == 0) && (((uintptr_t)B & 15) == 0);
   if (vec) k_dgemm_sub<1><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
   else k_dgemm_sub<0><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc);
 }
@@ -644,7 +845,8 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t cst
   const int Gmax = cus;
   const int mn = std::min(m, n);
   const int npanel = ceil_div(mn, nb);
-  // workspace: barrier + candidates | speculation verdicts | block | interchanges | L21
+  // workspace: barrier + candidates | speculation verdicts | original block |
+  // original rows below it | L^-1 of every sub-panel's diagonal block
   size_t wbytes = 256 + sizeof(double) * 2 * Gmax + sizeof(int) * 2 * Gmax +
                   sizeof(double) * 2 * (size_t)Gmax * nb + sizeof(double) * 2 * nb + 64 +
                   sizeof(unsigned) * (Gmax + 16);
@@ -653,10 +855,10 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t cst
   wbytes += ((sizeof(int) * npanel + 255) & ~(size_t)255);
   const size_t off_d = wbytes;
   wbytes += sizeof(double) * SP * SP;
-  const size_t off_lp = wbytes;
-  wbytes += 256;
   const size_t off_w = wbytes;
   wbytes += sizeof(double) * (size_t)std::max(m, 1) * SP;
+  const size_t off_li = wbytes;
+  wbytes += sizeof(double) * SP * SP * (size_t)npanel;
   TmpDev<char> wsb;
   DUST_TRY(wsb.alloc(wbytes));
   char *ws = wsb.p;
@@ -672,9 +874,9 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t cst
   double *topv = crowv + 2 * (size_t)Gmax * nb;
   unsigned *arr = (unsigned *)(topv + 2 * nb);
   int *spec_fail = (int *)(ws + off_fail);
-  double *Dblk = (double *)(ws + off_d);
-  int *lp = (int *)(ws + off_lp);
+  double *Dorig = (double *)(ws + off_d);
   double *Wl = (double *)(ws + off_w);
+  double *Linv = (double *)(ws + off_li);  // [npanel][64 x 64]
   unsigned gen0 = 0;
   int rc = 0;
   bool s2_busy = false;  // ev_rest marks the last trailing update on s2
@@ -692,20 +894,21 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t cst
       const int mrows = m - k;
       int *fail = spec_fail + (k / nb);
       // speculative sub-panel: a full 64-row block and at least one row below
+      double *Li = Linv + (size_t)(k / nb) * SP * SP;
       bool exact = true;
       if (jb == nb && mrows > SP) {
         const int nrow = mrows - SP;
-        k_spec_diag<<<1, 64, 0, s1>>>(dA, lda, k, jb, Dblk, lp, fail);
-        k_spec_l21<SP><<<ceil_div(nrow, 256), 256, 0, s1>>>(dA, lda, k, nrow, Dblk, Wl, nrow,
-                                                           fail);
-        const long tot = (long)SP * jb + (long)nrow * jb;
-        k_spec_commit<<<ceil_div(tot, 256), 256, 0, s1>>>(dA, lda, k, jb, nrow, Dblk, lp, Wl,
-                                                          nrow, dipiv, fail);
+        k_spec_diag<<<1, 256, 0, s1>>>(dA, lda, k, Dorig, dipiv, Li, fail);
+        k_spec_l21<<<ceil_div(nrow, 64), 64, 0, s1>>>(dA, lda, k, nrow, Wl, nrow, fail);
         launch_check("speculative panel");
         if (rc) return rc;
         DUST_HIP(hipMemcpyAsync(W.hflag.p, fail, sizeof(int), hipMemcpyDeviceToHost, s1));
         DUST_HIP(hipStreamSynchronize(s1));
         exact = W.hflag.p[0] != 0;
+        if (exact) {
+          const long tot = (long)SP * SP + (long)nrow * SP;
+          k_spec_restore<<<ceil_div(tot, 256), 256, 0, s1>>>(dA, lda, k, nrow, Dorig, Wl, nrow);
+        }
       }
       if (exact) {
         // all CUs for k_panel's resident grid: the trailing update first
@@ -727,6 +930,7 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t cst
         // launch: the kernel does its own synchronisation and needs none of
         // the cooperative-launch resources, which another process may hold.
         k_panel<<<G, 256, lds, s1>>>(P);
+        k_trtri_unit<<<1, 256, 0, s1>>>(dA + k + lda * k, lda, jb, Li);
         launch_check("panel");
         if (rc) return rc;
       }
@@ -736,8 +940,8 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t cst
                                                    jb, dipiv);
       const int wrest = K0 + JB - k - jb;
       if (wrest > 0) {
-        k_trsm_llnu<<<ceil_div(wrest, 64), 64, 0, s1>>>(dA + k + lda * k, lda,
-                                                        dA + k + lda * (k + jb), lda, jb, wrest);
+        k_trsm_inv<<<ceil_div(wrest, 64), 256, 0, s1>>>(Li, jb, dA + k + lda * (k + jb), lda,
+                                                        wrest);
         gemm_sub(m - k - jb, wrest, jb, dA + (k + jb) + lda * k, lda, dA + k + lda * (k + jb), lda,
                  dA + (k + jb) + lda * (k + jb), lda, s1);
       }
@@ -758,8 +962,18 @@ int lu_getrf_dev(int m, int n, double *dA, long lda, int *dipiv, hipStream_t cst
     if (rc) break;
     const int nrest = n - K0 - JB;
     if (nrest <= 0) break;
-    // U12 = L11^-1 A12 (blocked by 64)
-    rc = lu_trsm_llnu_dev(JB, nrest, dA + K0 + lda * K0, lda, dA + K0 + lda * (K0 + JB), lda, s1);
+    // U12 = L11^-1 A12, blocked by the sub-panels: their inverse diagonal
+    // blocks, then the update of the rows below by GEMM
+    for (int b = 0; b < JB; b += nb) {
+      const int jb = std::min(nb, JB - b);
+      double *Bb = dA + (K0 + b) + lda * (K0 + JB);
+      k_trsm_inv<<<ceil_div(nrest, 64), 256, 0, s1>>>(Linv + (size_t)((K0 + b) / nb) * SP * SP,
+                                                      jb, Bb, lda, nrest);
+      if (b + jb < JB)
+        gemm_sub(JB - b - jb, nrest, jb, dA + (K0 + b + jb) + lda * (K0 + b), lda, Bb, lda,
+                 Bb + jb, lda, s1);
+    }
+    launch_check("U12");
     if (rc) break;
     const int K1 = K0 + JB;
     const int JB2 = K1 < mn ? std::min(LU_NB, mn - K1) : 0;
