@@ -812,24 +812,44 @@ __device__ __forceinline__ void gemm_tile(const GemmItem &g, int bx, int by) {
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
-  for (int q = 0; q < P; ++q) {
-    if (F16) {
+  if (F16) {
+    // fragments of step t = 4 q + s (block column q, 32-wide k slice s) read
+    // three steps ahead into a ring of four register sets; the last steps
+    // re-read step T - 1 (clamped index, no branches between the loads)
+    const int T = P * (NB / 32);
+    auto load = [&](int t, h8 (&av)[4], h8 (&bv)[4]) {
+      t = min(t, T - 1);
+      const int q = t >> 2, s = t & 3;
       const h8 *La = (const h8 *)((const char *)Lf + q * pbytes);
       const h8 *Ub = (const h8 *)((const char *)Uf + q * pbytes);
 #pragma unroll
-      for (int s = 0; s < NB / 32; ++s) {
-        h8 av[4], bv[4];
+      for (int i = 0; i < 4; ++i) av[i] = La[((long)(rt0 + i) * (NB / 32) + s) * 64 + lane];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) av[i] = La[((long)(rt0 + i) * (NB / 32) + s) * 64 + lane];
+      for (int j = 0; j < 4; ++j) bv[j] = Ub[((long)(ct0 + j) * (NB / 32) + s) * 64 + lane];
+    };
+    auto mma = [&](const h8 (&av)[4], const h8 (&bv)[4]) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) bv[j] = Ub[((long)(ct0 + j) * (NB / 32) + s) * 64 + lane];
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[i], bv[j], acc[i][j], 0, 0, 0);
-      }
-    } else {
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[i], bv[j], acc[i][j], 0, 0, 0);
+    };
+    h8 a0[4], b0[4], a1[4], b1[4], a2[4], b2[4], a3[4], b3[4];
+    load(0, a0, b0);
+    load(1, a1, b1);
+    load(2, a2, b2);
+    for (int t = 0; t < T; t += 4) {
+      load(t + 3, a3, b3);
+      mma(a0, b0);
+      load(t + 4, a0, b0);
+      mma(a1, b1);
+      load(t + 5, a1, b1);
+      mma(a2, b2);
+      load(t + 6, a2, b2);
+      mma(a3, b3);
+    }
+  } else {
+    for (int q = 0; q < P; ++q) {
       const float *La = (const float *)((const char *)Lf + q * pbytes);
       const float *Ub = (const float *)((const char *)Uf + q * pbytes);
 #pragma unroll 4
@@ -847,16 +867,24 @@ __device__ __forceinline__ void gemm_tile(const GemmItem &g, int bx, int by) {
       }
     }
   }
-  // output layout: row 4 (lane / 16) + i, column lane % 16
+  // output layout: row 4 (lane / 16) + i, column lane % 16; all sixteen
+  // loads of the tile in flight before its stores
+  f4 cv[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const long row = 16L * (rt0 + i) + 4 * (lane >> 4);
       const long col = 16L * (ct0 + j) + (lane & 15);
-      f4 *p = (f4 *)(W + row + ld * col);
-      f4 v = *p;
-      *p = v - acc[i][j];
+      cv[i][j] = *(const f4 *)(W + row + ld * col);
+    }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long row = 16L * (rt0 + i) + 4 * (lane >> 4);
+      const long col = 16L * (ct0 + j) + (lane & 15);
+      *(f4 *)(W + row + ld * col) = cv[i][j] - acc[i][j];
     }
 }
 
