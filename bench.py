@@ -43,16 +43,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def fmm_handle(O, tree, degree, n, form):
+def fmm_handle(O, tree, degree, n, form, kernel="rosenhead"):
     return O.initialize_octree(tree["box_length"], tree["nbox"], tree["origin"], tree["nlevels"],
                                tree["min_part"], degree, tree["delta"], capacity=n, use_vs=True,
-                               precision=form)
+                               precision=form, kernel=kernel)
 
 
 def time_fmm(torch, L, O, tree, degree, dev, n, form, steps, warmup, knobs=None,
-             autotune=False):
+             autotune=False, kernel="rosenhead"):
     """Returns dict with ms per pass, stage times and the work counts."""
-    g = fmm_handle(O, tree, degree, n, form)
+    g = fmm_handle(O, tree, degree, n, form, kernel)
     for k, v in (knobs or {}).items():
         g.set_tuning(k, v)
     lib = L.lib()
@@ -763,6 +763,9 @@ def main():
     ap.add_argument("--no-elements", action="store_true",
                     help="skip the element <-> particle leg (rotor panels, helix particles)")
     ap.add_argument("--no-rotor", action="store_true", help="skip the rotating-rotor steps")
+    ap.add_argument("--no-orders", action="store_true", help="skip the order 6 / 8 FMM legs")
+    ap.add_argument("--no-gaussian", action="store_true",
+                    help="skip the Gaussian-core FMM leg")
     ap.add_argument("--elements-only", action="store_true",
                     help="time only the element <-> particle and particle-pool legs, print "
                          "their JSON")
@@ -840,6 +843,43 @@ def main():
         dv = np.linalg.norm(tuned["vel"] - plain["vel"]) / np.linalg.norm(plain["vel"])
         ds = np.linalg.norm(tuned["stretch"] - plain["stretch"]) / np.linalg.norm(plain["stretch"])
         acc = dict(vel_rel_l2_vs_fp64=dv, stretch_rel_l2_vs_fp64=ds)
+    def rel_l2(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    orders = None
+    if not args.no_orders and plain is not None:
+        # the same draw at expansion orders 6 and 8, both forms
+        orders = {}
+        for p in (6, 8):
+            log(f"timing FMM order {p}")
+            fp = time_fmm(torch, L, O, tree, p, dev, n, "fp64", 2, 1)
+            tp = time_fmm(torch, L, O, tree, p, dev, n, "tuned", 2, 1,
+                          knobs={"near_factor": args.near})
+            orders[f"p{p}"] = dict(fp64_ms=fp["ms"], tuned_ms=tp["ms"],
+                                   speedup_tuned_vs_fp64=fp["ms"] / tp["ms"],
+                                   tuned_updates_per_s=n / (tp["ms"] / 1e3),
+                                   vel_rel_l2_vs_fp64=rel_l2(tp["vel"], fp["vel"]),
+                                   stretch_rel_l2_vs_fp64=rel_l2(tp["stretch"], fp["stretch"]),
+                                   m2l_ms=tp["stages"]["m2l"], p2p_ms=tp["stages"]["p2p"])
+            log(f"order {p}: {orders[f'p{p}']}")
+    gauss = None
+    if not args.no_gaussian and plain is not None:
+        # labelled deviation: the same draw with the Gaussian particle core (the
+        # configuration's wording; the reference runs Rosenhead)
+        log("timing FMM with the Gaussian core")
+        fg = time_fmm(torch, L, O, tree, args.degree, dev, n, "fp64", 2, 1, kernel="gaussian")
+        tg = time_fmm(torch, L, O, tree, args.degree, dev, n, "tuned", args.steps, args.warmup,
+                      knobs={"near_factor": args.near}, kernel="gaussian")
+        gauss = dict(kernel="gaussian (kernel_coeffs alternative, mod_vortpart.f90:283-291)",
+                     core_radius=float(rv[0]), fp64_ms=fg["ms"], tuned_ms=tg["ms"],
+                     tuned_updates_per_s=n / (tg["ms"] / 1e3),
+                     speedup_tuned_vs_fp64=fg["ms"] / tg["ms"],
+                     vel_rel_l2_tuned_vs_fp64=rel_l2(tg["vel"], fg["vel"]),
+                     stretch_rel_l2_tuned_vs_fp64=rel_l2(tg["stretch"], fg["stretch"]),
+                     vel_rel_l2_gaussian_vs_rosenhead=rel_l2(fg["vel"], plain["vel"]),
+                     stretch_rel_l2_gaussian_vs_rosenhead=rel_l2(fg["stretch"], plain["stretch"]),
+                     p2p_ms=tg["stages"]["p2p"])
+        log(f"gaussian: {gauss}")
     lu = None
     lu_mats = None
     if not args.no_lu:
@@ -932,6 +972,8 @@ def main():
         "work": {"p2p_pairs": tuned["p2p_pairs"], "m2l_pairs": tuned["m2l_pairs"]},
         "roofline": roof,
         "roofline_lu": None if not lu else lu.get("roofline"),
+        "orders": orders,
+        "gaussian": gauss,
         "lu": lu,
         "wing": wing,
         "wing_run": wing_run,

@@ -19,6 +19,8 @@ DUST_E_UNSUPPORTED = -6
 
 FMM_FP64 = 0
 FMM_TUNED = 1
+KERNEL_ROSENHEAD = 0
+KERNEL_GAUSSIAN = 1
 
 LU_FP64 = 0
 LU_FP32 = 1
@@ -46,7 +48,8 @@ class FmmParams(C.Structure):
         ("nu_inf", C.c_double),
         ("u_inf", C.c_double * 3),
         ("precision", C.c_int),
-        ("reserved", C.c_int * 7),
+        ("kernel", C.c_int),
+        ("reserved", C.c_int * 6),
     ]
 
 
@@ -176,6 +179,8 @@ SIGNATURES = [
     ("dust_fmm_stage_times", C.c_int, [_vp, _vp, _vp]),
     ("dust_vortpart_direct", C.c_int,
      [C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_double, _vp, _vp, _vp]),
+    ("dust_vortpart_direct_k", C.c_int,
+     [C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_double, _vp, C.c_int, _vp, _vp]),
     ("dust_elem_apply", C.c_int,
      [C.c_int, C.POINTER(ElemGroup), C.POINTER(ElemParams), C.c_int, _vp, _vp, _vp, _vp]),
     ("dust_elem_apply_dev", C.c_int,

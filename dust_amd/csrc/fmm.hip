@@ -688,15 +688,20 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_fp64(EvalArgs A) {
           double sdx = ss[3 * kWave + s], sdy = ss[4 * kWave + s], sdz = ss[5 * kWave + s];
           double smg = ss[6 * kWave + s], srv = ss[7 * kWave + s];
           double d2 = r0 * r0 + r1 * r1 + r2 * r2;
-          // kernel_coeffs (mod_vortpart.f90:269-271)
-          double R2 = d2 + srv * srv;
-          double R3 = R2 * sqrt(R2);
-          double cf = -1.0 / R3;
+          // kernel_coeffs (mod_vortpart.f90:269-271; Gaussian: 283-291)
+          double cf, df;
+          if (A.kernel == DUST_KERNEL_GAUSSIAN) {
+            gauss_cd(d2, srv, cf, df);
+          } else {
+            double R2 = d2 + srv * srv;
+            double R3 = R2 * sqrt(R2);
+            cf = -1.0 / R3;
+            df = 3.0 / (R3 * R2);
+          }
           pv0 += cf * (r1 * sdz - r2 * sdy) * smg;
           pv1 += cf * (r2 * sdx - r0 * sdz) * smg;
           pv2 += cf * (r0 * sdy - r1 * sdx) * smg;
           if (A.use_vs) {
-            double df = 3.0 / (R3 * R2);
             double am0 = sdx * smg, am1 = sdy * smg, am2 = sdz * smg;
             double vp0 = al1 * am2 - al2 * am1, vp1 = al2 * am0 - al0 * am2, vp2 = al0 * am1 - al1 * am0;
             double dv = r0 * vp0 + r1 * vp1 + r2 * vp2;
@@ -827,7 +832,7 @@ constexpr int kEvalThreads = 128;
 __global__ void __launch_bounds__(kEvalThreads)
     k_direct(int n, const double *__restrict__ cen, const double *__restrict__ dir,
              const double *__restrict__ mag, const double *__restrict__ rv, int use_vs, int use_vd,
-             double nu_inf, double u0, double u1, double u2, double *vel, double *str) {
+             double nu_inf, double u0, double u1, double u2, int kernel, double *vel, double *str) {
   __shared__ double ss[8][kEvalThreads];
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   bool valid = t < n;
@@ -858,14 +863,19 @@ __global__ void __launch_bounds__(kEvalThreads)
       double r0 = x - ss[0][s], r1 = y - ss[1][s], r2 = z - ss[2][s];
       double sdx = ss[3][s], sdy = ss[4][s], sdz = ss[5][s], smg = ss[6][s], srv = ss[7][s];
       double d2 = r0 * r0 + r1 * r1 + r2 * r2;
-      double R2 = d2 + srv * srv;
-      double R3 = R2 * sqrt(R2);
-      double cf = -1.0 / R3;
+      double cf, df;
+      if (kernel == DUST_KERNEL_GAUSSIAN) {
+        gauss_cd(d2, srv, cf, df);
+      } else {
+        double R2 = d2 + srv * srv;
+        double R3 = R2 * sqrt(R2);
+        cf = -1.0 / R3;
+        df = 3.0 / (R3 * R2);
+      }
       pv0 += cf * (r1 * sdz - r2 * sdy) * smg;
       pv1 += cf * (r2 * sdx - r0 * sdz) * smg;
       pv2 += cf * (r0 * sdy - r1 * sdx) * smg;
       if (use_vs) {
-        double df = 3.0 / (R3 * R2);
         double am0 = sdx * smg, am1 = sdy * smg, am2 = sdz * smg;
         double vp0 = al1 * am2 - al2 * am1, vp1 = al2 * am0 - al0 * am2, vp2 = al0 * am1 - al1 * am0;
         double dv = r0 * vp0 + r1 * vp1 + r2 * vp2;
@@ -948,6 +958,8 @@ static int fmm_create(const dust_fmm_params *prm, int capacity, FmmHandle **out)
   DUST_CHECK(capacity >= 1, DUST_E_ARG, "capacity must be positive");
   DUST_CHECK(prm->precision == DUST_FMM_FP64 || prm->precision == DUST_FMM_TUNED, DUST_E_ARG,
              "unknown precision %d", prm->precision);
+  DUST_CHECK(prm->kernel == DUST_KERNEL_ROSENHEAD || prm->kernel == DUST_KERNEL_GAUSSIAN, DUST_E_ARG,
+             "kernel must be DUST_KERNEL_ROSENHEAD or DUST_KERNEL_GAUSSIAN");
   long nb = (long)prm->nbox[0] * prm->nbox[1] * prm->nbox[2];
   long nfine = nb << (3 * (prm->nlevels - 1));
   DUST_CHECK(nfine < (1L << 31) - 1, DUST_E_ARG, "finest level has too many cells");
@@ -1321,6 +1333,7 @@ void make_eval_args(const FmmHandle *h, EvalArgs *Ap) {
   A.nmon = h->nmon;
   A.use_vs = h->prm.use_vs;
   A.use_vd = h->prm.use_vd;
+  A.kernel = h->prm.kernel;
   A.use_tv = h->prm.use_tv;
   A.nu_inf = h->prm.nu_inf;
   for (int d = 0; d < 3; ++d) A.uinf[d] = h->prm.u_inf[d];
@@ -1435,7 +1448,7 @@ __global__ void k_pan_leaf(TreeGeom G, int m, const double *__restrict__ cen,
 }
 
 // One wavefront per panel: the lanes split the leaf's P2P list (velocity of
-// compute_vel_vortpart, Rosenhead core of the source), a fixed-order
+// compute_vel_vortpart with the handle's particle core), a fixed-order
 // reduction, then lane 0 adds the L2P velocity and writes uvort += .
 __global__ void __launch_bounds__(256)
     k_pan_eval(EvalArgs A, int m, const double *__restrict__ cen, const int *__restrict__ pleaf,
@@ -1454,8 +1467,15 @@ __global__ void __launch_bounds__(256)
     for (int q = rg.x + lane; q < rg.y; q += kWave) {
       double d0 = x - A.px[q], d1 = y - A.py[q], d2 = z - A.pz[q];
       double rr = A.pr[q];
-      double R2 = d0 * d0 + d1 * d1 + d2 * d2 + rr * rr;
-      double c = -A.pm[q] / (R2 * sqrt(R2));
+      double c;
+      if (A.kernel == DUST_KERNEL_GAUSSIAN) {
+        double cg, dg;
+        gauss_cd(d0 * d0 + d1 * d1 + d2 * d2, rr, cg, dg);
+        c = A.pm[q] * cg;
+      } else {
+        double R2 = d0 * d0 + d1 * d1 + d2 * d2 + rr * rr;
+        c = -A.pm[q] / (R2 * sqrt(R2));
+      }
       double a0 = A.dx[q], a1 = A.dy[q], a2 = A.dz[q];
       v0 += c * (d1 * a2 - d2 * a1);
       v1 += c * (d2 * a0 - d0 * a2);
@@ -1906,7 +1926,16 @@ int dust_fmm_work(dust_fmm_t hh, double *p2p_pairs, double *m2l_pairs) {
 int dust_vortpart_direct(int n, const double *cen, const double *dir, const double *mag,
                          const double *rv, int use_vs, int use_vd, double nu_inf,
                          const double u_inf[3], double *vel, double *stretch) {
+  return dust_vortpart_direct_k(n, cen, dir, mag, rv, use_vs, use_vd, nu_inf, u_inf,
+                                DUST_KERNEL_ROSENHEAD, vel, stretch);
+}
+
+int dust_vortpart_direct_k(int n, const double *cen, const double *dir, const double *mag,
+                           const double *rv, int use_vs, int use_vd, double nu_inf,
+                           const double u_inf[3], int kernel, double *vel, double *stretch) {
   DUST_CHECK(n >= 0, DUST_E_ARG, "negative particle count");
+  DUST_CHECK(kernel == DUST_KERNEL_ROSENHEAD || kernel == DUST_KERNEL_GAUSSIAN, DUST_E_ARG,
+             "kernel must be DUST_KERNEL_ROSENHEAD or DUST_KERNEL_GAUSSIAN");
   if (n == 0) return 0;
   DUST_CHECK(cen && dir && mag && rv && vel, DUST_E_ARG, "null argument");
   double *d = nullptr;
@@ -1925,7 +1954,7 @@ int dust_vortpart_direct(int n, const double *cen, const double *dir, const doub
   if (!rc) {
     double u0 = u_inf ? u_inf[0] : 0.0, u1 = u_inf ? u_inf[1] : 0.0, u2 = u_inf ? u_inf[2] : 0.0;
     k_direct<<<ceil_div(n, kEvalThreads), kEvalThreads>>>(n, dc, dd, dm, dr, use_vs, use_vd,
-                                                          nu_inf, u0, u1, u2, dv, ds);
+                                                          nu_inf, u0, u1, u2, kernel, dv, ds);
     if (hipMemcpy(vel, dv, sizeof(double) * 3 * n, hipMemcpyDeviceToHost) ||
         (stretch && hipMemcpy(stretch, ds, sizeof(double) * 3 * n, hipMemcpyDeviceToHost))) {
       set_error("direct evaluation failed: %s", hipGetErrorString(hipGetLastError()));

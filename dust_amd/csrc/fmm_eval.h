@@ -34,6 +34,7 @@ struct EvalItem {
 struct EvalArgs {
   TreeGeom G;
   int deg, nmon, use_vs, use_vd, use_tv;
+  int kernel;  // DUST_KERNEL_ROSENHEAD or DUST_KERNEL_GAUSSIAN
   double nu_inf, uinf[3];
   const int *pwr, *leaves, *slot, *fstart;
   const double *mp_b, *mp_c;
@@ -49,6 +50,25 @@ struct EvalArgs {
   double *part;            // per item and target: P2P velocity (3) and stretching (3)
   double *vel, *str, *tv;
 };
+
+// Gaussian core of kernel_coeffs (the commented alternative of
+// src/aeroel/mod_vortpart.f90:283-291, same expression order): velocity
+// coefficient c and stretching coefficient d = (1/r) dc/dr at distance
+// r = sqrt(d2) from a source of core radius rv; zero at r <= 1e-13.
+__device__ inline void gauss_cd(double d2, double rv, double &c, double &d) {
+  const double r = sqrt(d2);
+  if (r > 1e-13) {
+    const double s2p = 2.5066282746310002;  // sqrt(2 pi)
+    const double e = erf(r / (1.4142135623730951 * rv));
+    const double x = exp(-(r * r) / (2.0 * rv * rv));
+    const double r2 = r * r, r3 = r2 * r, r4 = r2 * r2, r5 = r4 * r;
+    c = -e / r3 + 2.0 / (r2 * s2p * rv) * x;
+    d = 3.0 / r5 * e + x * (-6.0 / (r4 * rv * s2p) - 2.0 / (r2 * rv * rv * rv * s2p));
+  } else {
+    c = 0.0;
+    d = 0.0;
+  }
+}
 
 // Ordering of LDS traffic inside one wavefront (no block barrier needed).
 __device__ inline void wave_sync() {

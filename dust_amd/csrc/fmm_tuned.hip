@@ -398,20 +398,26 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(M2LArgs P) {
 // ---------------------------------------------------------------------------
 // Near pair in FP64 (rsq and reciprocal refined by Newton steps instead of
 // IEEE division): r = target - source, alpha_s = dir_s mag_s, core radius rv.
+template <int GAUSS>
 __device__ inline void near64(double r0, double r1, double r2, double al0, double al1, double al2,
                               double as0, double as1, double as2, double rv, int use_vs,
                               double v[3], double s[3]) {
-  const double R2 = r0 * r0 + r1 * r1 + r2 * r2 + rv * rv;
-  double y = __builtin_amdgcn_rsq(R2);
-  y = y * (1.5 - 0.5 * R2 * y * y);
-  y = y * (1.5 - 0.5 * R2 * y * y);
-  const double y2 = y * y;
-  const double cf = -(y2 * y);          // -1/R^3
+  double cf, df;
+  if (GAUSS) {
+    gauss_cd(r0 * r0 + r1 * r1 + r2 * r2, rv, cf, df);
+  } else {
+    const double R2 = r0 * r0 + r1 * r1 + r2 * r2 + rv * rv;
+    double y = __builtin_amdgcn_rsq(R2);
+    y = y * (1.5 - 0.5 * R2 * y * y);
+    y = y * (1.5 - 0.5 * R2 * y * y);
+    const double y2 = y * y;
+    cf = -(y2 * y);  // -1/R^3
+    df = -3.0 * cf * y2;  // 3/R^5
+  }
   v[0] += cf * (r1 * as2 - r2 * as1);
   v[1] += cf * (r2 * as0 - r0 * as2);
   v[2] += cf * (r0 * as1 - r1 * as0);
   if (use_vs) {
-    const double df = -3.0 * cf * y2;   // 3/R^5
     const double vp0 = al1 * as2 - al2 * as1, vp1 = al2 * as0 - al0 * as2,
                  vp2 = al0 * as1 - al1 * as0;
     const double dv = r0 * vp0 + r1 * vp1 + r2 * vp2;
@@ -431,7 +437,7 @@ __device__ inline void near64(double r0, double r1, double r2, double al0, doubl
 // FP32 sums of every tile are folded into FP64 accumulators.  A partial tile
 // is padded with zero-strength sources far away, so the loop trip count is
 // uniform.
-template <int VS>
+template <int VS, int GAUSS>
 __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
   __shared__ f4 ssm[kEvalWaves][2 * kWave];
   __shared__ double sdm[kEvalWaves][8 * kWave];
@@ -474,7 +480,9 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
         const double smg = A.pm[q], srv = A.pr[q];
         const double ox = A.px[q] - cx, oy = A.py[q] - cy, oz = A.pz[q] - cz;
         const double a0 = A.dx[q] * smg, a1 = A.dy[q] * smg, a2 = A.dz[q] * smg;
-        ss[2 * lane] = (f4){(float)ox, (float)oy, (float)oz, (float)(srv * srv)};
+        // Rosenhead: r^2 of the core; Gaussian: 1 / (sqrt2 r)
+        ss[2 * lane] = (f4){(float)ox, (float)oy, (float)oz,
+                            GAUSS ? (float)(0.70710678118654752 / srv) : (float)(srv * srv)};
         ss[2 * lane + 1] = (f4){(float)a0, (float)a1, (float)a2, (float)(near2d * srv * srv)};
         double *d = sd + 8 * lane;
         d[0] = ox; d[1] = oy; d[2] = oz; d[3] = srv;
@@ -500,10 +508,23 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
         if (__builtin_expect(__any(skip), 0)) {
           if (skip && s0 + s != t) nmask |= 1ull << s;
         }
-        const float R2 = d2 + sp[3];
-        const float inv = __builtin_amdgcn_rsqf(R2);
-        const float inv2 = inv * inv;
-        const float cf = skip ? 0.f : -(inv2 * inv);
+        // cf = c of kernel_coeffs; tf = -d / 3 (so that the stretching sum
+        // below is shared by both cores: Rosenhead d = -3 cf / R^2)
+        float cf, tf;
+        if (GAUSS) {
+          const float r = sqrtf(d2), x = r * sp[3];
+          const float E = erff(x), ex = expf(-x * x);
+          const float ir = 1.f / r, ir3 = ir * ir * ir;
+          cf = skip ? 0.f : -(E - 1.1283791670955126f * x * ex) * ir3;
+          tf = skip ? 0.f
+                    : -(E - 0.18806319451591877f * ex * x * (6.f + 4.f * x * x)) * ir3 * ir * ir;
+        } else {
+          const float R2 = d2 + sp[3];
+          const float inv = __builtin_amdgcn_rsqf(R2);
+          const float inv2 = inv * inv;
+          cf = skip ? 0.f : -(inv2 * inv);
+          tf = cf * inv2;
+        }
         const float c0 = r1 * a2 - r2 * a1, c1 = r2 * a0 - r0 * a2, c2 = r0 * a1 - r1 * a0;
         v0 += cf * c0;
         v1 += cf * c1;
@@ -515,7 +536,7 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
           S0 += cf * a0;
           S1 += cf * a1;
           S2 += cf * a2;
-          const float tt = cf * inv2 * (fa0 * c0 + fa1 * c1 + fa2 * c2);
+          const float tt = tf * (fa0 * c0 + fa1 * c1 + fa2 * c2);
           w0 += tt * r0;
           w1 += tt * r1;
           w2 += tt * r2;
@@ -542,8 +563,8 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
           const int s = __builtin_ctzll(nmask);
           nmask &= nmask - 1;
           const double *d = sd + 8 * s;
-          near64(xr - d[0], yr - d[1], zr - d[2], al0, al1, al2, d[4], d[5], d[6], d[3], VS, pv,
-                 ps);
+          near64<GAUSS>(xr - d[0], yr - d[1], zr - d[2], al0, al1, al2, d[4], d[5], d[6], d[3], VS,
+                        pv, ps);
         }
       }
       wave_sync();
@@ -682,8 +703,14 @@ int tuned_eval(FmmHandle *h, EvalArgs &A) {
   A.near2 = t->near_factor * t->near_factor;
   if (A.n_items > 0) {
     const dim3 grid(ceil_div(A.n_items, kEvalWaves)), block(kWave * kEvalWaves);
-    if (A.use_vs) k_p2p_f32<1><<<grid, block, 0, h->stream>>>(A);
-    else k_p2p_f32<0><<<grid, block, 0, h->stream>>>(A);
+    const bool g = A.kernel == DUST_KERNEL_GAUSSIAN;
+    if (A.use_vs) {
+      if (g) k_p2p_f32<1, 1><<<grid, block, 0, h->stream>>>(A);
+      else k_p2p_f32<1, 0><<<grid, block, 0, h->stream>>>(A);
+    } else {
+      if (g) k_p2p_f32<0, 1><<<grid, block, 0, h->stream>>>(A);
+      else k_p2p_f32<0, 0><<<grid, block, 0, h->stream>>>(A);
+    }
   }
   DUST_HIP(hipGetLastError());
   return 0;

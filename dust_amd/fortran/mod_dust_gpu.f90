@@ -11,6 +11,7 @@ module mod_dust_gpu
   integer(c_int), parameter :: DUST_OK = 0, DUST_E_ARG = -1, DUST_E_HIP = -2, &
        DUST_E_OUTSIDE = -3, DUST_E_CAPACITY = -4, DUST_E_NOMEM = -5
   integer(c_int), parameter :: DUST_FMM_FP64 = 0, DUST_FMM_TUNED = 1
+  integer(c_int), parameter :: DUST_KERNEL_ROSENHEAD = 0, DUST_KERNEL_GAUSSIAN = 1
   integer(c_int), parameter :: DUST_LU_FP64 = 0, DUST_LU_FP32 = 1, DUST_LU_FP16S = 2
 
   ! dust_fmm_params (include/dust_gpu.h), same field order
@@ -24,7 +25,8 @@ module mod_dust_gpu
     real(c_double)    :: nu_inf
     real(c_double)    :: u_inf(3)
     integer(c_int)    :: precision
-    integer(c_int)    :: reserved(7)
+    integer(c_int)    :: kernel = 0   ! DUST_KERNEL_ROSENHEAD
+    integer(c_int)    :: reserved(6)
   end type
 
   type, bind(C) :: t_dust_aic_params

@@ -148,7 +148,10 @@ class Octree:
 
 def make_params(box_length, nbox, origin, nlevels, min_part, degree, delta, use_vs=True,
                 use_vd=False, use_tv=False, nu_inf=0.0, u_inf=(0.0, 0.0, 0.0),
-                precision="fp64"):
+                precision="fp64", kernel="rosenhead"):
+    """kernel: the particle core of kernel_coeffs, "rosenhead" (the
+    reference's) or "gaussian" (its commented alternative,
+    src/aeroel/mod_vortpart.f90:283-291)."""
     p = L.FmmParams()
     p.box_length = float(box_length)
     for d in range(3):
@@ -164,7 +167,11 @@ def make_params(box_length, nbox, origin, nlevels, min_part, degree, delta, use_
     p.use_tv = int(bool(use_tv))
     p.nu_inf = float(nu_inf)
     p.precision = {"fp64": L.FMM_FP64, "tuned": L.FMM_TUNED}[precision]
+    p.kernel = KERNELS[kernel]
     return p
+
+
+KERNELS = {"rosenhead": L.KERNEL_ROSENHEAD, "gaussian": L.KERNEL_GAUSSIAN}
 
 
 def initialize_octree(box_length, nbox, origin, nlevels, min_part, degree, delta, *,
@@ -213,14 +220,15 @@ def destroy_octree(octree):
     octree.destroy()
 
 
-def direct(parts, use_vs=True, use_vd=False, nu_inf=0.0, u_inf=(0.0, 0.0, 0.0)):
+def direct(parts, use_vs=True, use_vd=False, nu_inf=0.0, u_inf=(0.0, 0.0, 0.0),
+           kernel="rosenhead"):
     """Direct O(N^2) particle terms, the fmm=F branch of update_wake
     (src/aeroel/mod_wake.f90:1050-1113).  Returns (vel, stretch)."""
     vel = np.zeros((parts.n, 3))
     st = np.zeros((parts.n, 3))
     u = np.asarray(u_inf, dtype=np.float64)
-    L.check(L.lib().dust_vortpart_direct(parts.n, L.ptr(parts.cen), L.ptr(parts.dir),
-                                         L.ptr(parts.mag), L.ptr(parts.r_vortex), int(use_vs),
-                                         int(use_vd), float(nu_inf), L.ptr(u), L.ptr(vel),
-                                         L.ptr(st)), "direct")
+    L.check(L.lib().dust_vortpart_direct_k(parts.n, L.ptr(parts.cen), L.ptr(parts.dir),
+                                           L.ptr(parts.mag), L.ptr(parts.r_vortex), int(use_vs),
+                                           int(use_vd), float(nu_inf), L.ptr(u),
+                                           KERNELS[kernel], L.ptr(vel), L.ptr(st)), "direct")
     return vel, st

@@ -40,6 +40,16 @@ const char *dust_last_error(void);
 /* FMM: src/fmm/mod_octree.f90 + src/fmm/mod_multipole.f90                   */
 /* ======================================================================== */
 
+/* Particle core (dust_fmm_params.kernel): kernel_coeffs,
+ * src/aeroel/mod_vortpart.f90:260-299.  Rosenhead is the reference's active
+ * branch (269-271); Gaussian is its commented alternative "from Alvarez"
+ * (283-291), c = -erf(r/(sqrt2 s))/r^3 + 2 exp(-r^2/(2 s^2))/(r^2 sqrt(2 pi) s),
+ * d = (1/r) dc/dr, zero at r <= 1e-13.  Only the particle-particle terms
+ * (P2P, direct sum) depend on it; the expansions use the Rosenhead-regularised
+ * Green's function of `delta` either way, as in the reference. */
+#define DUST_KERNEL_ROSENHEAD 0
+#define DUST_KERNEL_GAUSSIAN 1
+
 /* Evaluation forms (dust_fmm_params.precision). */
 #define DUST_FMM_FP64 0   /* plain FP64 path: the drop-in                    */
 #define DUST_FMM_TUNED 1  /* FP32 P2P with FP64 near pairs + GEMM M2L        */
@@ -61,7 +71,8 @@ typedef struct {
   double nu_inf;         /* kinematic viscosity                              */
   double u_inf[3];       /* uniform free stream (variable_wind without gust) */
   int precision;         /* DUST_FMM_FP64 or DUST_FMM_TUNED                  */
-  int reserved[7];
+  int kernel;            /* DUST_KERNEL_ROSENHEAD (0) or DUST_KERNEL_GAUSSIAN */
+  int reserved[6];
 } dust_fmm_params;
 
 typedef struct dust_fmm_s *dust_fmm_t;
@@ -180,6 +191,10 @@ int dust_fmm_stage_times(dust_fmm_t h, double *ms, int *launches);
 int dust_vortpart_direct(int n, const double *cen, const double *dir, const double *mag,
                          const double *r_vortex, int use_vs, int use_vd, double nu_inf,
                          const double u_inf[3], double *vel, double *stretch);
+/* Same with the particle core chosen (DUST_KERNEL_*). */
+int dust_vortpart_direct_k(int n, const double *cen, const double *dir, const double *mag,
+                           const double *r_vortex, int use_vs, int use_vd, double nu_inf,
+                           const double u_inf[3], int kernel, double *vel, double *stretch);
 
 /* ======================================================================== */
 /* Element <-> particle direct terms: src/fmm/mod_octree.f90:1350-1401,     */

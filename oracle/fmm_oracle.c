@@ -186,8 +186,31 @@ static inline double norm2_3(const double a[3]) {
   return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
 }
 
-/* kernel_coeffs, mod_vortpart.f90:260-299 (Rosenhead branch) */
+/* Particle core of kernel_coeffs: 0 = Rosenhead (the reference's active
+ * branch), 1 = the Gaussian "from Alvarez" of its commented alternative.
+ * Process-wide; set before an evaluation. */
+static int orc_kernel = 0;
+void orc_set_kernel(int k) { orc_kernel = k; }
+int orc_get_kernel(void) { return orc_kernel; }
+
+/* kernel_coeffs, mod_vortpart.f90:260-299: Rosenhead 269-271, Gaussian
+ * 283-291 (same expressions, r <= 1e-13 gives zero) */
 static inline void kernel_coeffs(double r_vort, double rr, double *c, double *d) {
+  if (orc_kernel == 1) {
+    double r = rr;
+    if (r > 1e-13) {
+      double e = erf(r / (sqrt(2.0) * r_vort));
+      double x = exp(-(r * r) / (2.0 * r_vort * r_vort));
+      *c = -e / (r * r * r) + 2.0 / (r * r * sqrt(2.0 * ORC_PI) * r_vort) * x;
+      *d = 3.0 / (r * r * r * r * r) * e +
+           x * (-6.0 / (r * r * r * r * r_vort * sqrt(2.0 * ORC_PI)) -
+                2.0 / (r * r * r_vort * r_vort * r_vort * sqrt(2.0 * ORC_PI)));
+    } else {
+      *c = 0.0;
+      *d = 0.0;
+    }
+    return;
+  }
   double distn = sqrt(rr * rr + r_vort * r_vort);
   double d3 = distn * distn * distn;
   *c = -1.0 / d3;

@@ -32,6 +32,8 @@ def lib():
             "orc_pexp_table": (C.c_int, [C.c_int, _vp, _vp]),
             "orc_compute_der_ker": (None, [C.c_int, _vp, C.c_double, _vp, _vp]),
             "orc_pair_kernels": (None, [_vp, _vp, C.c_double, C.c_double, _vp, _vp, _vp, _vp, _vp]),
+            "orc_set_kernel": (None, [C.c_int]),
+            "orc_get_kernel": (C.c_int, []),
             "orc_direct": (None, [C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_double, _vp,
                                   _vp, _vp]),
             "orc_octree_create": (_vp, [C.c_double, _vp, _vp, C.c_int, C.c_int, C.c_int,
@@ -134,19 +136,39 @@ def der_ker(deg, diff, delta):
     return D, np.ascontiguousarray(Dc.transpose(0, 2, 1))
 
 
-def pair(src_cen, src_dir, src_mag, src_r, pos, alpha):
+ROSENHEAD, GAUSSIAN = 0, 1  # particle cores of kernel_coeffs
+
+
+class _Core:
+    """Selects the oracle's particle core for the duration of a call."""
+
+    def __init__(self, kernel):
+        self.k = int(kernel)
+
+    def __enter__(self):
+        self.old = lib().orc_get_kernel()
+        lib().orc_set_kernel(self.k)
+
+    def __exit__(self, *a):
+        lib().orc_set_kernel(self.old)
+
+
+def pair(src_cen, src_dir, src_mag, src_r, pos, alpha, kernel=ROSENHEAD):
     v, s, d = np.zeros(3), np.zeros(3), np.zeros(3)
-    lib().orc_pair_kernels(_p(_c(src_cen)), _p(_c(src_dir)), float(src_mag), float(src_r),
-                           _p(_c(pos)), _p(_c(alpha)), _p(v), _p(s), _p(d))
+    with _Core(kernel):
+        lib().orc_pair_kernels(_p(_c(src_cen)), _p(_c(src_dir)), float(src_mag), float(src_r),
+                               _p(_c(pos)), _p(_c(alpha)), _p(v), _p(s), _p(d))
     return v, s, d
 
 
-def direct(cen, dir, mag, rv, use_vs=True, use_vd=False, nu_inf=0.0, u_inf=(0, 0, 0)):
+def direct(cen, dir, mag, rv, use_vs=True, use_vd=False, nu_inf=0.0, u_inf=(0, 0, 0),
+           kernel=ROSENHEAD):
     n = len(mag)
     vel = np.zeros((n, 3))
     st = np.zeros((n, 3))
-    lib().orc_direct(n, _p(_c(cen)), _p(_c(dir)), _p(_c(mag)), _p(_c(rv)), int(use_vs),
-                     int(use_vd), float(nu_inf), _p(_c(u_inf)), _p(vel), _p(st))
+    with _Core(kernel):
+        lib().orc_direct(n, _p(_c(cen)), _p(_c(dir)), _p(_c(mag)), _p(_c(rv)), int(use_vs),
+                         int(use_vd), float(nu_inf), _p(_c(u_inf)), _p(vel), _p(st))
     return vel, st
 
 
@@ -154,7 +176,8 @@ class Octree:
     """CPU restatement of the reference octree FMM."""
 
     def __init__(self, box_length, nbox, origin, nlevels, min_part, degree, delta, use_vs=True,
-                 use_vd=False, use_tv=False, nu_inf=0.0, u_inf=(0, 0, 0)):
+                 use_vd=False, use_tv=False, nu_inf=0.0, u_inf=(0, 0, 0), kernel=ROSENHEAD):
+        self.kernel = kernel
         self.degree = degree
         self.nlevels = nlevels
         self._h = lib().orc_octree_create(float(box_length), _p(_c(nbox, np.int32)),
@@ -185,14 +208,16 @@ class Octree:
         vel = np.zeros((n, 3))
         st = np.zeros((n, 3))
         tv = np.zeros(n)
-        lib().orc_apply_multipole(self._h, _p(vel), _p(st), _p(tv))
+        with _Core(self.kernel):
+            lib().orc_apply_multipole(self._h, _p(vel), _p(st), _p(tv))
         return vel, st, tv
 
     def apply_panels(self, cen, uvort=None):
         """apply_multipole_panels at the centres cen (m,3); uvort accumulated."""
         cen = _c(cen).reshape(-1, 3)
         u = np.zeros((len(cen), 3)) if uvort is None else np.array(uvort, dtype=np.float64)
-        rc = lib().orc_apply_multipole_panels(self._h, len(cen), _p(cen), _p(u))
+        with _Core(self.kernel):
+            rc = lib().orc_apply_multipole_panels(self._h, len(cen), _p(cen), _p(u))
         if rc != 0:
             raise ValueError(f"panel {-1 - rc} outside the octree")
         return u

@@ -137,3 +137,19 @@ def test_assembly_argument_errors_without_gpu():
     d.nstatic = n + 1
     assert lib.dust_initialize_linsys(ctypes.byref(d), A.ctypes.data, n, A.ctypes.data, n, None, 1,
                                       None, 1, None, None) == L.DUST_E_ARG
+
+
+def test_particle_core_argument_errors_without_gpu():
+    """An unknown particle core is refused before any device work."""
+    import numpy as np
+
+    from dust_amd import octree as O
+    lib = L.lib()
+    p = O.make_params(4.0, (1, 1, 1), (-2.0, -2.0, -2.0), 3, 32, 4, 0.005)
+    p.kernel = 7
+    h = ctypes.c_void_p()
+    assert lib.dust_fmm_create(ctypes.byref(p), 100, ctypes.byref(h)) == L.DUST_E_ARG
+    assert b"kernel" in lib.dust_last_error()
+    u = np.zeros(3)
+    assert lib.dust_vortpart_direct_k(0, None, None, None, None, 1, 0, 0.0, L.ptr(u), 3, None,
+                                      None) == L.DUST_E_ARG

@@ -30,6 +30,8 @@ def run_both(orc, cen, alpha, rv, tree, degree, **flags):
     O.calculate_multipole(parts, g)
     O.apply_multipole(parts, g)
     oflags = {k: v for k, v in flags.items() if k != "precision"}
+    if "kernel" in oflags:
+        oflags["kernel"] = {"rosenhead": orc.ROSENHEAD, "gaussian": orc.GAUSSIAN}[oflags["kernel"]]
     c = orc.Octree(tree["box_length"], tree["nbox"], tree["origin"], tree["nlevels"],
                    tree["min_part"], degree, tree["delta"], **oflags)
     v, s, tv = c.run(parts.cen, parts.dir, parts.mag, parts.r_vortex)
@@ -150,6 +152,37 @@ def test_direct_vs_oracle(orc):
     vg, sg = O.direct(parts, use_vs=True, use_vd=True, nu_inf=1e-2, u_inf=(10.0, 0.0, 1.0))
     vc, sc = orc.direct(parts.cen, parts.dir, parts.mag, parts.r_vortex, True, True, 1e-2,
                         (10.0, 0.0, 1.0))
+    assert rel(vg, vc) <= TOL
+    assert rel(sg, sc) <= TOL
+
+
+@pytest.mark.parametrize("form", ["fp64", "tuned"])
+def test_gaussian_core_vs_oracle(orc, form):
+    """The Gaussian particle core (kernel_coeffs' commented alternative,
+    mod_vortpart.f90:283-291) in P2P: FP64 form <= 1e-10, tuned form (FP32
+    erf pairs beyond one core radius, FP64 inside) within the tuned bounds."""
+    cen, alpha, rv, tree = W.helix_wake(12000, seed=11)
+    tree = dict(tree, nlevels=4)
+    parts, g, nl, c, v, s, tv = run_both(orc, cen, alpha, rv, tree, 4, use_vs=True,
+                                         precision=form, kernel="gaussian")
+    ev, es = rel(parts.vel, v), rel(parts.stretch, s)
+    print(f"gaussian {form}: vel {ev:.2e} stretch {es:.2e}")
+    assert ev <= (TOL if form == "fp64" else TUNED_VEL)
+    assert es <= (TOL if form == "fp64" else TUNED_STR)
+    check_leaves(parts, g, c)
+    # the core matters: the Rosenhead oracle differs
+    cr = orc.Octree(tree["box_length"], tree["nbox"], tree["origin"], 4, tree["min_part"], 4,
+                    tree["delta"], use_vs=True)
+    vr, _, _ = cr.run(parts.cen, parts.dir, parts.mag, parts.r_vortex)
+    assert rel(vr, v) > 1e-4
+
+
+def test_gaussian_direct_vs_oracle(orc):
+    d = np.load(os.path.join(HERE, "golden", "mir_static_p_particles.npz"))
+    parts = O.VortParts.from_alpha(d["cen"], d["alpha"], d["r_vortex"])
+    vg, sg = O.direct(parts, use_vs=True, u_inf=(10.0, 0.0, 1.0), kernel="gaussian")
+    vc, sc = orc.direct(parts.cen, parts.dir, parts.mag, parts.r_vortex, True, False, 0.0,
+                        (10.0, 0.0, 1.0), kernel=orc.GAUSSIAN)
     assert rel(vg, vc) <= TOL
     assert rel(sg, sc) <= TOL
 
