@@ -880,6 +880,20 @@ def main():
                      stretch_rel_l2_gaussian_vs_rosenhead=rel_l2(fg["stretch"], plain["stretch"]),
                      p2p_ms=tg["stages"]["p2p"])
         log(f"gaussian: {gauss}")
+    sched = None
+    if args.autotune:
+        # the project autotuner's schedule for this draw at the headline's
+        # accuracy (untimed search; the headline keeps the configured schedule)
+        from dust_amd import autotune as AT
+        log("autotuning the FMM schedule")
+        r = AT.tune_fmm(torch, dev, tree, tol=1e-6, degrees=(3, 4, 5),
+                        levels=(tree["nlevels"] - 1, tree["nlevels"], tree["nlevels"] + 1),
+                        min_parts=(32, 64, 128))
+        sched = dict(tol=1e-6, best=r["best"], candidates=len(r["table"]),
+                     reference=r["ref"], table=[{k: row[k] for k in
+                                                 ("degree", "nlevels", "min_part", "ms", "err")}
+                                                for row in r["table"]])
+        log(f"autotuned schedule: {r['best']}")
     lu = None
     lu_mats = None
     if not args.no_lu:
@@ -974,6 +988,7 @@ def main():
         "roofline_lu": None if not lu else lu.get("roofline"),
         "orders": orders,
         "gaussian": gauss,
+        "autotuned_schedule": sched,
         "lu": lu,
         "wing": wing,
         "wing_run": wing_run,

@@ -170,6 +170,8 @@ SIGNATURES = [
     ("dust_fmm_set_tuning", C.c_int, [_vp, C.c_char_p, C.c_double]),
     ("dust_fmm_get_tuning", C.c_int, [_vp, C.c_char_p, _dp]),
     ("dust_fmm_autotune", C.c_int, [_vp, C.c_int, _dp]),
+    ("dust_lu_set_tuning", C.c_int, [C.c_char_p, C.c_int]),
+    ("dust_lu_get_tuning", C.c_int, [C.c_char_p, _vp]),
     ("dust_fmm_stream", _vp, [_vp]),
     ("dust_fmm_synchronize", C.c_int, [_vp]),
     ("dust_fmm_leaves", C.c_int, [_vp, _vp, _vp, _vp, _vp]),

@@ -48,6 +48,8 @@ void elem_release();
 void fmm_release();
 void lu_release();
 void mixed_release();
+int mixed_reserved_cus();
+int mixed_set_reserved_cus(int v);
 void pres_release();
 void wake_release();
 

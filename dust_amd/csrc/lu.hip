@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/dust_gpu_test.h"
@@ -781,13 +782,13 @@ static int device_cus() {
 // trailing-matrix traffic of 64-column panels).  One panel of lookahead:
 // the next outer panel's columns are updated first and that panel is factored
 // on stream s1 while the rest of the trailing matrix is updated on stream s2,
-// which is kept off the first kLuReservedCUs compute units (a CU-masked
+// which is kept off the first g_lu_reserved_cus compute units (a CU-masked
 // stream) so that the single-workgroup panel kernels always find a CU.  The
 // speculation's verdict is read back per sub-panel (s1 only; s2 keeps
 // running); a failed one waits for s2 and refactors the sub-panel with
 // k_panel, whose grid barrier needs all its blocks resident at once.
 constexpr int LU_NB = 256;
-constexpr int kLuReservedCUs = 32;
+int g_lu_reserved_cus = 32;  // tuning knob "getrf_reserved_cus"
 
 struct GetrfWs {
   int dev = -1;
@@ -802,8 +803,8 @@ static int getrf_streams(GetrfWs &w) {
   DUST_HIP(hipStreamCreateWithFlags(&w.s1, hipStreamNonBlocking));
   const int cus = device_cus();
   std::vector<uint32_t> mask((cus + 31) / 32, 0u);
-  for (int c = kLuReservedCUs; c < cus; ++c) mask[c / 32] |= 1u << (c % 32);
-  if (cus <= 2 * kLuReservedCUs ||
+  for (int c = g_lu_reserved_cus; c < cus; ++c) mask[c / 32] |= 1u << (c % 32);
+  if (g_lu_reserved_cus <= 0 || cus <= 2 * g_lu_reserved_cus ||
       hipExtStreamCreateWithCUMask(&w.s2, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
     (void)hipGetLastError();
     w.s2 = nullptr;
@@ -1200,6 +1201,38 @@ int lu_trsm_llnu_dev(int m, int n, const double *L, long ldl, double *B, long ld
 using namespace dust;
 
 extern "C" {
+
+int dust_lu_set_tuning(const char *name, int value) {
+  DUST_CHECK(name, DUST_E_ARG, "null knob name");
+  const std::string k(name);
+  const int cus = device_cus();
+  DUST_CHECK(value >= 0 && value <= cus / 2, DUST_E_ARG,
+             "reserved compute units must be in 0..%d", cus / 2);
+  if (k == "getrf_reserved_cus") {
+    DUST_HIP(hipDeviceSynchronize());
+    getrf_release();  // the lookahead streams are rebuilt with the new mask
+    g_lu_reserved_cus = value;
+    return 0;
+  }
+  if (k == "mixed_reserved_cus") return mixed_set_reserved_cus(value);
+  set_error("unknown LU knob '%s'", name);
+  return DUST_E_ARG;
+}
+
+int dust_lu_get_tuning(const char *name, int *value) {
+  DUST_CHECK(name && value, DUST_E_ARG, "null argument");
+  const std::string k(name);
+  if (k == "getrf_reserved_cus") {
+    *value = g_lu_reserved_cus;
+    return 0;
+  }
+  if (k == "mixed_reserved_cus") {
+    *value = mixed_reserved_cus();
+    return 0;
+  }
+  set_error("unknown LU knob '%s'", name);
+  return DUST_E_ARG;
+}
 
 int dust_dgetrf_dev(int m, int n, double *dA, int lda, int *dipiv, void *stream) {
   int info = 0;

@@ -1521,21 +1521,21 @@ int run_init(MixedRun &r, MixedWs &w, int n, int nrhs, const double *dA, long ld
 }
 
 // Streams of the single-system solve, created on first use.  The lookahead
-// stream s2 is kept off the first kReservedCUs compute units (a CU-masked
+// stream s2 is kept off the first g_reserved_cus compute units (a CU-masked
 // stream; an ordinary one if masking is unavailable), so that the
 // latency-bound diagonal-block kernel and the panel updates of the next step
 // always find free CUs while the trailing remainder runs.  A CU-masked stream
 // synchronises with the legacy default stream, so the solve itself runs on
 // its own non-blocking stream s1, ordered after the caller's stream.
-constexpr int kReservedCUs = 32;
+int g_reserved_cus = 32;  // tuning knob "mixed_reserved_cus"
 
 int mx_streams(MixedWs &w) {
   if (w.s1) return 0;
   DUST_HIP(hipStreamCreateWithFlags(&w.s1, hipStreamNonBlocking));
   const int cus = mx_cus();
   std::vector<uint32_t> mask((cus + 31) / 32, 0u);
-  for (int c = kReservedCUs; c < cus; ++c) mask[c / 32] |= 1u << (c % 32);
-  if (cus <= 2 * kReservedCUs ||
+  for (int c = g_reserved_cus; c < cus; ++c) mask[c / 32] |= 1u << (c % 32);
+  if (g_reserved_cus <= 0 || cus <= 2 * g_reserved_cus ||
       hipExtStreamCreateWithCUMask(&w.s2, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
     (void)hipGetLastError();
     w.s2 = nullptr;
@@ -1893,6 +1893,16 @@ void mixed_release() {
   B.norms.release();
   B.hflags.release();
   B.hnorms.release();
+}
+
+int mixed_reserved_cus() { return g_reserved_cus; }
+
+// The workspaces (and their CU-masked streams) are rebuilt on the next solve.
+int mixed_set_reserved_cus(int v) {
+  DUST_HIP(hipDeviceSynchronize());
+  mixed_release();
+  g_reserved_cus = v;
+  return 0;
 }
 
 }  // namespace dust

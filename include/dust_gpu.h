@@ -438,6 +438,15 @@ int dust_dgesv_mixed_dev(int n, int nrhs, const double *dA, int lda, double *dB,
                          int prec, double tol, int max_iter, int *iters, double *resid,
                          void *stream);
 
+/* Performance knobs of the LU solvers (no counterpart in the reference):
+ * "getrf_reserved_cus" / "mixed_reserved_cus" = compute units kept free of
+ * the trailing updates for the diagonal-block chain (CU-masked lookahead
+ * stream; 0 = no mask; default 32, at most half the CUs).  Results do not
+ * depend on them.  Setting one waits for the device and rebuilds that
+ * solver's streams (and, for the mixed solve, its workspaces). */
+int dust_lu_set_tuning(const char *name, int value);
+int dust_lu_get_tuning(const char *name, int *value);
+
 /* Batch of independent tuned solves (device pointers, one system per entry,
  * sizes may differ), advanced in lockstep: the factorisations of all systems
  * are launched (each system on its own stream and workspace), then each
