@@ -237,6 +237,10 @@ SIGNATURES = [
     ("dust_assemble_linsys_dev", C.c_int,
      [_vp, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, _vp, _vp]),
     ("dust_solve_pressure_sys", C.c_int, [C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp]),
+    ("dust_press_normvel_der", C.c_int,
+     [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _vp]),
+    ("dust_press_normvel_der_dev", C.c_int,
+     [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _vp, _vp]),
     ("dust_pres_rhs_src", C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp]),
     ("dust_pres_rhs_src_dev", C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp, _vp]),
     ("dust_pres_rhs_rot", C.c_int,

@@ -198,6 +198,61 @@ const T *stage(DevBuf<T> &b, size_t &off, const T *h, size_t count, hipError_t &
 
 using namespace dust;
 
+// press_normvel_der (mod_pressure_equation.f90:396-462), one thread per
+// surface panel, the edges in the reference's order; the R products as
+// matmul forms them (row i of R times the vector).
+__global__ void k_normvel_der(int n, const double *__restrict__ nor, const double *__restrict__ ub,
+                              const double *__restrict__ sv, const double *__restrict__ svo,
+                              const double *__restrict__ st, const int *__restrict__ nv,
+                              const int *__restrict__ ng, const double *__restrict__ R, double dt,
+                              double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *ni = nor + 3 * i, *u = sv + 3 * i, *b = ub + 3 * i, *Ri = R + 9 * (long)i;
+  const double dun = (ni[0] * (b[0] - svo[3 * i]) + ni[1] * (b[1] - svo[3 * i + 1]) +
+                      ni[2] * (b[2] - svo[3 * i + 2])) / dt;
+  double g[3] = {0.0, 0.0, 0.0};
+  double div = 0.0;
+  const int ne = min(max(nv[i], 0), 4);
+  // GradS_Un (a neighbour index outside -1..n-1 adds nothing, as -2 does)
+  for (int e = 0; e < ne; ++e) {
+    const int j = ng[4 * i + e];
+    if (j < -1 || j >= n) continue;
+    double d[3];
+    for (int k = 0; k < 3; ++k) d[k] = j >= 0 ? sv[3 * j + k] - u[k] : -2.0 * u[k];
+    const double dn = ni[0] * d[0] + ni[1] * d[1] + ni[2] * d[2];
+    const double *s = st + 12 * (long)i + 3 * e;
+    double v[3] = {s[0] * dn, s[1] * dn, s[2] * dn};
+    for (int r = 0; r < 3; ++r) g[r] += Ri[r] * v[0] + Ri[3 + r] * v[1] + Ri[6 + r] * v[2];
+  }
+  // DivS_U
+  for (int e = 0; e < ne; ++e) {
+    const int j = ng[4 * i + e];
+    if (j < -1 || j >= n) continue;
+    const double *s = st + 12 * (long)i + 3 * e;
+    double q = 0.0;
+    for (int r = 0; r < 3; ++r) {
+      const double rs = Ri[r] * s[0] + Ri[3 + r] * s[1] + Ri[6 + r] * s[2];
+      q += rs * (j >= 0 ? sv[3 * j + r] - u[r] : -2.0 * u[r]);
+    }
+    div += q;
+  }
+  out[i] = dun - (g[0] * b[0] + g[1] * b[1] + g[2] * b[2]) +
+           div * (b[0] * ni[0] + b[1] * ni[1] + b[2] * ni[2]);
+}
+
+static int normvel_der_dev(int n, const double *nor, const double *ub, const double *sv,
+                           const double *svo, const double *st, const int *nv, const int *ng,
+                           const double *R, double dt, double *out, hipStream_t stream) {
+  DUST_CHECK(n >= 0, DUST_E_ARG, "negative panel count");
+  DUST_CHECK(dt != 0.0, DUST_E_ARG, "dt must not be zero");
+  if (n == 0) return 0;
+  DUST_CHECK(nor && ub && sv && svo && st && nv && ng && R && out, DUST_E_ARG, "null argument");
+  k_normvel_der<<<ceil_div(n, 128), 128, 0, stream>>>(n, nor, ub, sv, svo, st, nv, ng, R, dt, out);
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
+
 extern "C" {
 
 int dust_pres_rhs_rot_dev(int m, const double *cen, int np, const double *pcen,
@@ -251,6 +306,53 @@ int dust_pres_rhs_rot(int m, const double *cen, int np, const double *pcen, cons
   DUST_TRY(pres_rhs_rot_dev(m, dc, np, dpc, dpd, dpm, dpr, dpv, lines ? nl : 0, dlv, dlm, dlu, nw,
                             dwn, dwv, dwm, dwu, db, nullptr));
   DUST_HIP(hipMemcpy(b_pres, db, sizeof(double) * m, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int dust_press_normvel_der_dev(int n_sp, const double *nor, const double *ub,
+                               const double *surf_vel, const double *surf_vel_old,
+                               const double *stencil, const int *n_ver, const int *neigh,
+                               const double *R, double dt, double *bernoulli_source,
+                               void *stream) {
+  return normvel_der_dev(n_sp, nor, ub, surf_vel, surf_vel_old, stencil, n_ver, neigh, R, dt,
+                         bernoulli_source, (hipStream_t)stream);
+}
+
+int dust_press_normvel_der(int n_sp, const double *nor, const double *ub,
+                           const double *surf_vel, const double *surf_vel_old,
+                           const double *stencil, const int *n_ver, const int *neigh,
+                           const double *R, double dt, double *bernoulli_source) {
+  DUST_CHECK(n_sp >= 0, DUST_E_ARG, "negative panel count");
+  DUST_CHECK(dt != 0.0, DUST_E_ARG, "dt must not be zero");
+  if (n_sp == 0) return 0;
+  DUST_CHECK(nor && ub && surf_vel && surf_vel_old && stencil && n_ver && neigh && R &&
+                 bernoulli_source,
+             DUST_E_ARG, "null argument");
+  for (int i = 0; i < n_sp; ++i) {
+    DUST_CHECK(n_ver[i] == 3 || n_ver[i] == 4, DUST_E_ARG, "panel %d has %d vertices", i,
+               n_ver[i]);
+    for (int e = 0; e < n_ver[i]; ++e)
+      DUST_CHECK(neigh[4 * i + e] >= -2 && neigh[4 * i + e] < n_sp, DUST_E_ARG,
+                 "panel %d: neighbour %d out of range", i, neigh[4 * i + e]);
+  }
+  const size_t n = (size_t)n_sp;
+  const size_t nd = 3 * n * 4 + 12 * n + 9 * n + n;
+  DUST_TRY(g_pws.h_d.reserve(nd));
+  DUST_TRY(g_pws.h_i.reserve(5 * n));
+  size_t od = 0, oi = 0;
+  hipError_t err = hipSuccess;
+  const double *dn = stage(g_pws.h_d, od, nor, 3 * n, err);
+  const double *du = stage(g_pws.h_d, od, ub, 3 * n, err);
+  const double *ds = stage(g_pws.h_d, od, surf_vel, 3 * n, err);
+  const double *dso = stage(g_pws.h_d, od, surf_vel_old, 3 * n, err);
+  const double *dst = stage(g_pws.h_d, od, stencil, 12 * n, err);
+  const double *dR = stage(g_pws.h_d, od, R, 9 * n, err);
+  const int *dnv = stage(g_pws.h_i, oi, n_ver, n, err);
+  const int *dng = stage(g_pws.h_i, oi, neigh, 4 * n, err);
+  double *dout = g_pws.h_d.p + od;
+  DUST_HIP(err);
+  DUST_TRY(normvel_der_dev(n_sp, dn, du, ds, dso, dst, dnv, dng, dR, dt, dout, nullptr));
+  DUST_HIP(hipMemcpy(bernoulli_source, dout, sizeof(double) * n, hipMemcpyDeviceToHost));
   return 0;
 }
 

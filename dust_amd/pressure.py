@@ -13,8 +13,9 @@
                                    the surface panels times their
                                    bernoulli_source (mod_surfpan.f90:405-437)
 
-press_normvel_der (the panels' bernoulli_source from the neighbour stencils
-of the geometry) stays with the caller.
+    press_normvel_der(...)         press_normvel_der (:396-462): the
+                                   panels' bernoulli_source from their
+                                   velocities and neighbour stencils
 """
 import numpy as np
 
@@ -89,3 +90,23 @@ def solve_pressure_sys(A, P, nstatic_sp, b):
     info = L.check(L.lib().dust_solve_pressure_sys(n, nstatic_sp, L.ptr(A), n, L.ptr(P), L.ptr(b),
                                                    L.ptr(res)), "solve_pressure_sys")
     return res, info
+
+
+def press_normvel_der(nor, ub, surf_vel, surf_vel_old, stencil, n_ver, neigh, R, dt):
+    """press_normvel_der (mod_pressure_equation.f90:396-462) for the n surface
+    panels in idSurfPan order: nor, ub, surf_vel, surf_vel_old (n,3);
+    stencil (n,4,3) = pot_vel_stencil; n_ver (n); neigh (n,4) the 0-based
+    surface-panel index across each edge, -1 for no neighbour (the edge takes
+    -2 surf_vel), -2 for a neighbour that is not a surface panel (no term);
+    R (n,3,3) the rotation R_g applied to panel i (R[i][row, column]; the
+    reference takes that of the component of elems(i)).  Returns the
+    bernoulli_source (n)."""
+    n = len(n_ver)
+    out = np.zeros(n)
+    Rc = np.ascontiguousarray(np.transpose(np.asarray(R, np.float64), (0, 2, 1)))
+    L.check(L.lib().dust_press_normvel_der(
+        n, L.ptr(_f64(nor)), L.ptr(_f64(ub)), L.ptr(_f64(surf_vel)), L.ptr(_f64(surf_vel_old)),
+        L.ptr(_f64(stencil)), L.ptr(np.ascontiguousarray(n_ver, np.int32)),
+        L.ptr(np.ascontiguousarray(neigh, np.int32)), L.ptr(Rc), float(dt), L.ptr(out)),
+        "press_normvel_der")
+    return out

@@ -569,6 +569,32 @@ int dust_pres_rhs_src(int n_sp, const double *B, int ldb, const double *src, dou
 int dust_pres_rhs_src_dev(int n_sp, const double *B, int ldb, const double *src, double *b_pres,
                           void *stream);
 
+/* press_normvel_der (mod_pressure_equation.f90:396-462): the source
+ * strength bernoulli_source(n_sp) of every surface panel,
+ *   n.(ub - surf_vel_old)/dt - GradS_Un . ub + DivS_U (ub . n),
+ * GradS_Un = sum_e R (stencil_e n.(U_e - U)), DivS_U = sum_e (R stencil_e).(U_e - U)
+ * over the panel's n_ver edges, U = surf_vel, U_e = the neighbour's surf_vel;
+ * an edge without neighbour (neigh = -1) takes U_e - U = -2 U, an edge whose
+ * neighbour is not a surface panel (neigh = -2) adds nothing.  Arrays over
+ * the surface panels in idSurfPan order: nor, ub, surf_vel, surf_vel_old
+ * (3,n_sp); stencil (3,4,n_sp) = pot_vel_stencil; n_ver (n_sp); neigh
+ * (4,n_sp) 0-based surface-panel index of the neighbour across each edge
+ * (or -1 / -2); R (3,3,n_sp) column-major, the R_g the reference applies to
+ * panel i (that of the component of elems(i), as the reference indexes it).
+ * bernoulli_source is overwritten.  Host pointers (n_ver and neigh checked:
+ * DUST_E_ARG) / device pointers ordered on `stream` (not checked on the host:
+ * the kernel reads at most 4 edges and treats a neighbour index outside
+ * -1..n_sp-1 as -2). */
+int dust_press_normvel_der(int n_sp, const double *nor, const double *ub,
+                           const double *surf_vel, const double *surf_vel_old,
+                           const double *stencil, const int *n_ver, const int *neigh,
+                           const double *R, double dt, double *bernoulli_source);
+int dust_press_normvel_der_dev(int n_sp, const double *nor, const double *ub,
+                               const double *surf_vel, const double *surf_vel_old,
+                               const double *stencil, const int *n_ver, const int *neigh,
+                               const double *R, double dt, double *bernoulli_source,
+                               void *stream);
+
 /* assemble_pressure_sys (mod_pressure_equation.f90:139-236), rotational part
  * (c) of the right-hand side at the m surface-panel centres cen(3,m),
  * accumulated into b_pres(m) in the reference's order (particles, then

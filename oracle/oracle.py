@@ -80,6 +80,8 @@ def lib():
                                           C.c_double, _vp, _vp]),
             "ao_wake_shed": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp,
                                        _vp, _vp, _vp]),
+            "ao_press_normvel_der": (None, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                             C.c_double, _vp]),
             "ao_pres_rhs_rot": (None, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int,
                                        _vp, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
             "ao_initialize_linsys": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
@@ -447,6 +449,20 @@ def pres_rhs_rot(cen, b=None, pcen=None, pdir=None, pmag=None, prv=None, pvel=No
                           _p(_c(wnver, np.int32)) if nw else None, opt(wver, nw), opt(wmag, nw),
                           opt(wvel, nw), _p(b))
     return b
+
+
+def press_normvel_der(nor, ub, sv, svo, stencil, n_ver, neigh, R, dt):
+    """press_normvel_der (mod_pressure_equation.f90:396-462): the
+    bernoulli_source of the n surface panels; nor, ub, sv, svo (n,3), stencil
+    (n,4,3), n_ver (n), neigh (n,4) (-1: none, -2: not a surface panel), R
+    (n,3,3) with R[i] the matrix (row, column)."""
+    n = len(n_ver)
+    out = np.zeros(n)
+    Rc = np.ascontiguousarray(np.transpose(np.asarray(R, np.float64), (0, 2, 1)))  # column-major
+    lib().ao_press_normvel_der(n, _p(_c(nor)), _p(_c(ub)), _p(_c(sv)), _p(_c(svo)),
+                               _p(_c(stencil)), _p(_c(n_ver, np.int32)), _p(_c(neigh, np.int32)),
+                               _p(Rc), float(dt), _p(out))
+    return out
 
 
 # ---- particle pool of the wake (wake_oracle.c) --------------------------------

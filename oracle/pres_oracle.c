@@ -9,6 +9,7 @@
  *   (c.1) particles            171-187
  *   (c.2) end-vortex lines     191-203
  *   (c.3) wake panels          204-225
+ * and press_normvel_der (396-462), the panels' bernoulli_source.
  * Each panel's b_pres receives the terms one at a time in the reference's
  * loop order (all particles first, in a loop of its own, then the lines and
  * the wake panels edge by edge).
@@ -80,5 +81,52 @@ void ao_pres_rhs_rot(int m, const double *cen, int np, const double *pcen, const
                                                  u + 3 * nx, x);
       }
     }
+  }
+}
+
+/* press_normvel_der, mod_pressure_equation.f90:396-462: bernoulli_source of
+ * every surface panel (arrays as in dust_press_normvel_der: nor, ub, sv,
+ * svo (3,n); st (3,4,n); nv (n); ng (4,n) 0-based neighbour or -1 (none:
+ * -2 U) or -2 (not a surface panel: nothing); R (3,3,n) column-major). */
+void ao_press_normvel_der(int n, const double *nor, const double *ub, const double *sv,
+                          const double *svo, const double *st, const int *nv, const int *ng,
+                          const double *R, double dt, double *out) {
+  for (int i = 0; i < n; ++i) {
+    const double *ni = nor + 3 * i, *u = sv + 3 * i, *b = ub + 3 * i, *Ri = R + 9 * i;
+    double dun = 0.0;
+    for (int k = 0; k < 3; ++k) dun += ni[k] * (b[k] - svo[3 * i + k]);
+    dun = dun / dt;
+    double g[3] = {0.0, 0.0, 0.0};
+    for (int e = 0; e < nv[i]; ++e) {
+      int j = ng[4 * i + e];
+      if (j == -2) continue;
+      double d[3], dn = 0.0;
+      for (int k = 0; k < 3; ++k) d[k] = j >= 0 ? sv[3 * j + k] - u[k] : -2.0 * u[k];
+      for (int k = 0; k < 3; ++k) dn += ni[k] * d[k];
+      const double *s = st + 12 * i + 3 * e;
+      double v[3] = {s[0] * dn, s[1] * dn, s[2] * dn};
+      for (int r = 0; r < 3; ++r) {
+        double m = 0.0;
+        for (int k = 0; k < 3; ++k) m += Ri[r + 3 * k] * v[k];
+        g[r] += m;
+      }
+    }
+    double div = 0.0;
+    for (int e = 0; e < nv[i]; ++e) {
+      int j = ng[4 * i + e];
+      if (j == -2) continue;
+      const double *s = st + 12 * i + 3 * e;
+      double q = 0.0;
+      for (int r = 0; r < 3; ++r) {
+        double rs = 0.0;
+        for (int k = 0; k < 3; ++k) rs += Ri[r + 3 * k] * s[k];
+        q += rs * (j >= 0 ? sv[3 * j + r] - u[r] : -2.0 * u[r]);
+      }
+      div += q;
+    }
+    double gu = 0.0, un = 0.0;
+    for (int k = 0; k < 3; ++k) gu += g[k] * b[k];
+    for (int k = 0; k < 3; ++k) un += b[k] * ni[k];
+    out[i] = dun - gu + div * un;
   }
 }

@@ -153,3 +153,21 @@ def test_particle_core_argument_errors_without_gpu():
     u = np.zeros(3)
     assert lib.dust_vortpart_direct_k(0, None, None, None, None, 1, 0, 0.0, L.ptr(u), 3, None,
                                       None) == L.DUST_E_ARG
+
+
+def test_normvel_der_argument_errors_without_gpu():
+    import numpy as np
+    lib = L.lib()
+    n = 2
+    z3 = np.zeros((n, 3))
+    st = np.zeros((n, 4, 3))
+    R = np.tile(np.eye(3), (n, 1, 1))
+    out = np.zeros(n)
+    args = lambda nv, ng, dt: (n, L.ptr(z3), L.ptr(z3), L.ptr(z3), L.ptr(z3), L.ptr(st),
+                               L.ptr(np.asarray(nv, np.int32)), L.ptr(np.asarray(ng, np.int32)),
+                               L.ptr(R), dt, L.ptr(out))
+    ok_ng = [[1, -1, -1, -2], [0, -1, -1, -1]]
+    assert lib.dust_press_normvel_der(*args([4, 5], ok_ng, 0.1)) == L.DUST_E_ARG
+    assert lib.dust_press_normvel_der(*args([4, 4], [[2, -1, -1, -1], [0, -1, -1, -1]],
+                                            0.1)) == L.DUST_E_ARG
+    assert lib.dust_press_normvel_der(*args([4, 4], ok_ng, 0.0)) == L.DUST_E_ARG

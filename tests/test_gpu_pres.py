@@ -156,3 +156,40 @@ def test_pres_rhs_src_and_full_chain(wing):
     piv = np.zeros(n, np.int32)
     res, info = P.solve_pressure_sys(A, piv, 0, b)
     assert info == 0 and rel(res, np.linalg.solve(A0, b)) <= TOL
+
+
+def test_press_normvel_der_vs_oracle(orc):
+    """press_normvel_der on the device (host and device entry points) against
+    the oracle: random velocities, rotations and neighbour codes on the flat
+    grid plus a few triangles."""
+    import torch
+
+    import normvel_cases as NC
+    from dust_amd import _lib as L
+    from dust_amd import pressure as PR
+    g = NC.flat_grid(9, 7, 0.05)
+    n = g["n"]
+    rng = np.random.default_rng(11)
+    U, Uold, ub = (rng.standard_normal((n, 3)) for _ in range(3))
+    nor = rng.standard_normal((n, 3))
+    nor /= np.linalg.norm(nor, axis=1)[:, None]
+    st = g["st"] + rng.uniform(-1.0, 1.0, g["st"].shape)
+    ng = g["neigh"].copy()
+    ng[rng.random(ng.shape) < 0.1] = -2
+    nv = g["n_ver"].copy()
+    nv[::7] = 3
+    Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    R = np.tile(Q, (n, 1, 1))
+    ref = orc.press_normvel_der(nor, ub, U, Uold, st, nv, ng, R, 0.004)
+    got = PR.press_normvel_der(nor, ub, U, Uold, st, nv, ng, R, 0.004)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    # device entry point
+    # the device copies are held until the kernel has finished
+    keep = [torch.from_numpy(np.ascontiguousarray(a)).cuda()
+            for a in (nor, ub, U, Uold, st, np.asarray(nv, np.int32), np.asarray(ng, np.int32),
+                      np.ascontiguousarray(np.transpose(R, (0, 2, 1))))]
+    out = torch.zeros(n, dtype=torch.float64, device="cuda")
+    L.check(L.lib().dust_press_normvel_der_dev(
+        n, *[t.data_ptr() for t in keep[:8]], 0.004, out.data_ptr(), None), "normvel_der_dev")
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), got)
