@@ -24,6 +24,8 @@
 #include <algorithm>
 #include <vector>
 
+#include <cstring>
+
 #include "common.h"
 #include "wake_internal.h"
 
@@ -407,6 +409,49 @@ PwArgs args(dust_panwake_s *h) {
 
 using namespace dust;
 
+// Checkpoint of the device state (restart): a header of the dimensions and
+// the row counters, then every state array in a fixed order.
+namespace {
+struct PwCkHeader {
+  char magic[4];
+  int version, ns, np, nmax, nend, len, full, pad;
+};
+template <typename T>
+size_t ck_part(dust::DevBuf<T> &b, size_t count, char *out, size_t off, bool save,
+               const char *in, hipError_t &err) {
+  const size_t bytes = sizeof(T) * count;
+  if (count) {
+    hipError_t e = save ? hipMemcpy(out + off, b.p, bytes, hipMemcpyDeviceToHost)
+                        : hipMemcpy(b.p, in + off, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) err = e;
+  }
+  return off + bytes;
+}
+size_t ck_arrays(dust_panwake_s *h, char *out, const char *in, bool save, hipError_t &err) {
+  const size_t rows = (size_t)h->p.nmax_pan + 1, np = h->p.n_points, ns = h->p.n_stripes;
+  const size_t nmax = h->p.nmax_pan, nend = h->nend;
+  size_t off = sizeof(PwCkHeader);
+  const bool go = out || in;
+  auto part = [&](auto &buf, size_t count) {
+    if (go) off = ck_part(buf, count, out, off, save, in, err);
+    else off += sizeof(*buf.p) * count;
+  };
+  part(h->joined, 2 * nend * rows);
+  part(h->pts, 3 * rows * np);
+  part(h->pvel, 3 * rows * np);
+  part(h->wstart, 3 * np);
+  part(h->pend, 3 * np);
+  part(h->idou, nmax * ns);
+  part(h->end_idou, ns);
+  part(h->last_idou, ns);
+  part(h->ver, 12 * nmax * ns);
+  part(h->nver, nmax * ns);
+  part(h->lver, 6 * ns);
+  part(h->lvel, 6 * ns);
+  return off;
+}
+}  // namespace
+
 extern "C" {
 
 int dust_panwake_create(const dust_panwake_params *p, dust_panwake_t *out) {
@@ -590,6 +635,49 @@ int dust_panwake_get(dust_panwake_t h, double *points, double *pvel, double *ido
   if (end_idou) DUST_HIP(hipMemcpy(end_idou, h->end_idou.p, sizeof(double) * ns, hipMemcpyDeviceToHost));
   if (last_idou) DUST_HIP(hipMemcpy(last_idou, h->last_idou.p, sizeof(double) * ns, hipMemcpyDeviceToHost));
   if (points_end) DUST_HIP(hipMemcpy(points_end, h->pend.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int dust_panwake_checkpoint(dust_panwake_t h, void *buf, size_t *size) {
+  DUST_CHECK(h && size, DUST_E_ARG, "null argument");
+  hipError_t err = hipSuccess;
+  const size_t need = ck_arrays(h, nullptr, nullptr, true, err);
+  if (!buf) {
+    *size = need;
+    return 0;
+  }
+  DUST_CHECK(*size >= need, DUST_E_CAPACITY, "checkpoint buffer of %zu bytes, %zu needed", *size,
+             need);
+  DUST_HIP(hipDeviceSynchronize());
+  PwCkHeader hd{{'D', 'P', 'W', 'K'}, 1, h->p.n_stripes, h->p.n_points, h->p.nmax_pan, h->nend,
+                h->len, h->full, 0};
+  std::memcpy(buf, &hd, sizeof(hd));
+  ck_arrays(h, (char *)buf, nullptr, true, err);
+  DUST_HIP(err);
+  *size = need;
+  return 0;
+}
+
+int dust_panwake_restore(dust_panwake_t h, const void *buf, size_t size) {
+  DUST_CHECK(h && buf, DUST_E_ARG, "null argument");
+  DUST_CHECK(size >= sizeof(PwCkHeader), DUST_E_ARG, "checkpoint too short");
+  PwCkHeader hd;
+  std::memcpy(&hd, buf, sizeof(hd));
+  DUST_CHECK(std::memcmp(hd.magic, "DPWK", 4) == 0 && hd.version == 1, DUST_E_ARG,
+             "not a panel-wake checkpoint");
+  DUST_CHECK(hd.ns == h->p.n_stripes && hd.np == h->p.n_points && hd.nmax == h->p.nmax_pan &&
+                 hd.nend == h->nend,
+             DUST_E_ARG, "checkpoint of a different panel wake (%d stripes, %d points, %d rows)",
+             hd.ns, hd.np, hd.nmax);
+  DUST_CHECK(hd.len >= 1 && hd.len <= h->p.nmax_pan, DUST_E_ARG, "bad row count in checkpoint");
+  hipError_t err = hipSuccess;
+  const size_t need = ck_arrays(h, nullptr, nullptr, false, err);
+  DUST_CHECK(size == need, DUST_E_ARG, "checkpoint of %zu bytes, %zu expected", size, need);
+  DUST_HIP(hipDeviceSynchronize());
+  ck_arrays(h, nullptr, (const char *)buf, false, err);
+  DUST_HIP(err);
+  h->len = hd.len;
+  h->full = hd.full;
   return 0;
 }
 

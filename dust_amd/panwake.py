@@ -108,3 +108,16 @@ class PanWake:
                                                    ("pts", "pvel", "idou", "end_idou",
                                                     "last_idou", "pend")]), "panwake_get")
         return out
+
+    def checkpoint(self):
+        """The device state as bytes (dust_panwake_checkpoint)."""
+        n = C.c_size_t()
+        L.check(L.lib().dust_panwake_checkpoint(self.h, None, C.byref(n)), "checkpoint")
+        buf = np.zeros(n.value, np.uint8)
+        L.check(L.lib().dust_panwake_checkpoint(self.h, buf.ctypes.data, C.byref(n)),
+                "checkpoint")
+        return buf
+
+    def restore(self, buf):
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        L.check(L.lib().dust_panwake_restore(self.h, buf.ctypes.data, buf.size), "restore")

@@ -444,3 +444,44 @@ class BodyWakeStep:
                    free=P.free.cpu().numpy())
         out.update({"pw_" + k: v for k, v in self.pw.get().items()})
         return out
+
+    # ---- restart --------------------------------------------------------------
+    _POOL = ("free", "cen", "dir", "mag", "r_vortex", "vel", "stretch", "rotu", "n_prt")
+
+    def save_restart(self, path):
+        """Writes the state a restart needs to continue bit for bit: the
+        particle pool slot by slot, the panel wake's device state, the body's
+        rotation angle and the last solution; plus the live particles under
+        the names of the reference's ParticleWake output (WakePoints,
+        WakeVort, VortexRad; src/io/mod_dust_io.f90:391-413) for readers of
+        the results.  numpy .npz (no HDF5 library here)."""
+        P = self.pool
+        live = P.live().cpu().numpy()
+        cen, dr, mg = P.cen.cpu().numpy(), P.dir.cpu().numpy(), P.mag.cpu().numpy()
+        data = {"pool_" + k: getattr(P, k).cpu().numpy() for k in self._POOL}
+        data.update(panel_wake=self.pw.checkpoint(), theta=np.float64(self.theta),
+                    mu=self.b.cpu().numpy(), n_panels=np.int64(self.n),
+                    WakePoints=cen[live], WakeVort=dr[live] * mg[live][:, None],
+                    VortexRad=P.r_vortex.cpu().numpy()[live])
+        np.savez(path, **data)
+
+    def load_restart(self, path):
+        """Restores a state written by save_restart into this step object
+        (built with the same geometry, wake tables and pool size)."""
+        torch = self.torch
+        with np.load(path, allow_pickle=False) as d:
+            if int(d["n_panels"]) != self.n:
+                raise ValueError("restart file of a different body")
+            P = self.pool
+            for k in self._POOL:
+                a = d["pool_" + k]
+                t = getattr(P, k)
+                if tuple(a.shape) != tuple(t.shape):
+                    raise ValueError(f"restart pool field {k} has shape {a.shape}")
+                t.copy_(torch.from_numpy(a))
+            self.pw.restore(d["panel_wake"])
+            self.theta = float(d["theta"])
+            self.b.copy_(torch.from_numpy(d["mu"]))
+        if self.moving:
+            self._move(advance=False)
+        torch.cuda.synchronize()

@@ -721,6 +721,17 @@ int dust_panwake_state(dust_panwake_t h, int *len, int *full, int *nshed_max);
 int dust_panwake_get(dust_panwake_t h, double *points, double *pvel, double *idou,
                      double *end_idou, double *last_idou, double *points_end);
 
+/* Checkpoint / restart of the panel wake's device state (points, velocities,
+ * intensities, junction history, panels, end vortices, row counters), for a
+ * restart of the time loop that continues bit for bit.  checkpoint: with
+ * buf == NULL writes the size needed to *size; else fills buf (host memory,
+ * *size bytes available) and synchronises the device.  restore: the buffer
+ * of a checkpoint of a wake with the same tables and dimensions.  No
+ * counterpart in the reference's API (its restart reads the HDF5 result
+ * files, mod_wake.f90:597-778). */
+int dust_panwake_checkpoint(dust_panwake_t h, void *buf, size_t *size);
+int dust_panwake_restore(dust_panwake_t h, const void *buf, size_t size);
+
 /* Release every library-global workspace (device buffers, page-locked
  * records, streams and events of the solvers and element kernels) after
  * synchronising the device; they are re-created on the next call that needs
