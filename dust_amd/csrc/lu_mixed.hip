@@ -905,6 +905,16 @@ __global__ void __launch_bounds__(256) k_mx_gemm(GemmItem one, const GemmItem *_
   }
 }
 
+// Two regions in one launch (a block column and a block row of the same
+// sub-panels): workgroups [0, a.ntiles) take the tiles of a, the rest those
+// of b.
+template <int F16>
+__global__ void __launch_bounds__(256) k_mx_gemm2(GemmItem a, GemmItem b) {
+  const int t = blockIdx.x;
+  if (t < a.ntiles) gemm_tile<F16>(a, t % a.nrt, t / a.nrt);
+  else if (t - a.ntiles < b.ntiles) gemm_tile<F16>(b, (t - a.ntiles) % b.nrt, (t - a.ntiles) / b.nrt);
+}
+
 // ---------------------------------------------------------------------------
 // Refinement pieces.
 // ---------------------------------------------------------------------------
@@ -1154,6 +1164,24 @@ int mx_gemm(int f16, float *W, long ld, int r0, int r1, int c0, int c1, const vo
   return 0;
 }
 
+// Two update regions of the same P block columns in one launch.
+int mx_gemm2(int f16, float *W, long ld, int r0, int r1, int c0, int c1, int s0, int s1, int d0,
+             int d1, const void *Lf, const void *Uf, long pbytes, int P, hipStream_t st) {
+  if (P <= 0) return 0;
+  auto item = [&](int ra, int rb, int ca, int cb) {
+    const int nrt = rb > ra ? (rb - ra) / 128 : 0;
+    const int nt = (rb > ra && cb > ca) ? nrt * ((cb - ca) / 128) : 0;
+    return GemmItem{W, ld, ra, ca, std::max(nrt, 1), P, nt, Lf, Uf, pbytes};
+  };
+  const GemmItem a = item(r0, r1, c0, c1), b = item(s0, s1, d0, d1);
+  const int grid = a.ntiles + b.ntiles;
+  if (grid == 0) return 0;
+  if (f16) k_mx_gemm2<1><<<grid, 256, 0, st>>>(a, b);
+  else k_mx_gemm2<0><<<grid, 256, 0, st>>>(a, b);
+  MX_LAUNCH_CHECK("update gemm");
+  return 0;
+}
+
 // Factorisation of W (np x np).  Returns 0 or a HIP error; *bad is set when a
 // block failed the pivot check or hit a zero pivot.
 int mx_factor_launch(MixedWs &w, int np, int f16, hipStream_t st) {
@@ -1179,8 +1207,9 @@ int mx_factor_launch(MixedWs &w, int np, int f16, hipStream_t st) {
       const int k = ks + a * NB;
       if (a > 0) {
         // block column a and block row a, updated by the sub-panels before it
-        DUST_TRY(mx_gemm(f16, w.W.p, ld, k, np, k, k + NB, Lf, Uf, pbytes, a, st));
-        DUST_TRY(mx_gemm(f16, w.W.p, ld, k, k + NB, k + NB, np, Lf, Uf, pbytes, a, st));
+        // (one launch: the two regions write disjoint tiles)
+        DUST_TRY(mx_gemm2(f16, w.W.p, ld, k, np, k, k + NB, k, k + NB, k + NB, np, Lf, Uf, pbytes,
+                          a, st));
       }
       DiagArgs da{w.W.p, ld, k, w.piv.p, w.perm.p, w.npiv.p, w.dinv.p, w.flags.p};
       k_mx_diag<0><<<1, 256, lds_diag(), st>>>(da, nullptr);
@@ -1202,8 +1231,7 @@ int mx_factor_launch(MixedWs &w, int np, int f16, hipStream_t st) {
     DUST_HIP(hipEventRecord(w.ev_panel, st));
     if (pending) DUST_HIP(hipStreamWaitEvent(st, w.ev_rest, 0));
     // the next super-panel's block column and block row
-    DUST_TRY(mx_gemm(f16, w.W.p, ld, t0, np, t0, t1, Lf, Uf, pbytes, nsub, st));
-    DUST_TRY(mx_gemm(f16, w.W.p, ld, t0, t1, t1, np, Lf, Uf, pbytes, nsub, st));
+    DUST_TRY(mx_gemm2(f16, w.W.p, ld, t0, np, t0, t1, t0, t1, t1, np, Lf, Uf, pbytes, nsub, st));
     // the remainder on s2
     if (t1 < np) {
       DUST_HIP(hipStreamWaitEvent(s2, w.ev_panel, 0));
