@@ -121,14 +121,26 @@ __device__ __forceinline__ void block_solve_rows(const BlockSolveArgs<T> &a, int
       const int j = UPPER ? a.nblk - 1 - jj : jj;
       T l[CW];
       const int c0 = j * BS + h * CW;
-      const T *Lr = a.A + gr + a.lda * (long)c0;
       const int ldi = (int)a.lda;  // column offsets inside one block fit in 32 bits
+      // clamped addresses, no branches between the loads; columns past n
+      // (the last block) are zeroed afterwards
+      const T *Lc = a.A + (rok ? gr : 0) + a.lda * (long)min(c0, a.n - 1);
+      const int cmax = a.n - 1 - min(c0, a.n - 1);
 #pragma unroll
-      for (int c = 0; c < CW; ++c) l[c] = (rok && c0 + c < a.n) ? Lr[ldi * c] : T(0);
+      for (int c = 0; c < CW; ++c) l[c] = Lc[ldi * min(c, cmax)];
+#pragma unroll
+      for (int c = 0; c < CW; ++c)
+        if (!rok || c0 + c >= a.n) l[c] = T(0);
       if (t == 0) ok = wait_flag(&a.ready[j], a.gen, a.err) ? 1 : 0;
       __syncthreads();
       if (!ok) return;
-      for (int q = t; q < BS * R8; q += NT) sy[q] = a.y[(long)j * BS * R8 + q];
+      {
+        T yv[BS * R8 / NT];
+#pragma unroll
+        for (int u = 0; u < BS * R8 / NT; ++u) yv[u] = a.y[(long)j * BS * R8 + t + NT * u];
+#pragma unroll
+        for (int u = 0; u < BS * R8 / NT; ++u) sy[t + NT * u] = yv[u];
+      }
       __syncthreads();
 #pragma unroll
       for (int c = 0; c < CW; ++c) {
