@@ -720,6 +720,56 @@ def summarize_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_bytes, 
     print(json.dumps({k: (v["fetch_bytes"], v["write_bytes"]) for k, v in kernels.items()}))
 
 
+def summarize_counters(db, pattern=None, cus=256, simds=1024):
+    """Per-kernel utilisation from a rocprofv3 --pmc database with
+    SQ_VALU_MFMA_BUSY_CYCLES, SQ_INSTS_VALU_MFMA_MOPS_F64/F32/F16,
+    SQ_ACTIVE_INST_VALU, SQ_INSTS_VALU and GRBM_GUI_ACTIVE: mean per dispatch of
+    MfmaUtil = MFMA busy / (GRBM_GUI_ACTIVE x SIMDs), VALUBusy =
+    SQ_ACTIVE_INST_VALU / CUs / GRBM_GUI_ACTIVE (rocprofv3's derived-metric
+    formulas), and the MFMA flops per second (MOPS x 512 / duration)."""
+    import collections
+    import re
+    import sqlite3
+    con = sqlite3.connect(db)
+    rows = con.execute("select dispatch_id, name, counter_name, sum(counter_value), "
+                       "max(counter_value), min(duration) from pmc_events "
+                       "group by dispatch_id, counter_name").fetchall()
+    per = collections.defaultdict(dict)
+    names, dur = {}, {}
+    for d, name, cn, sv, mv, du in rows:
+        short = re.sub(r"\(.*", "", name.replace("(anonymous namespace)::", ""))
+        short = short.replace("void ", "").replace("dust::", "")
+        if pattern and not re.search(pattern, short):
+            continue
+        per[d][cn] = mv if cn == "GRBM_GUI_ACTIVE" else sv
+        names[d], dur[d] = short, du
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    for d, c in per.items():
+        g = c.get("GRBM_GUI_ACTIVE", 0.0)
+        if g <= 0:
+            continue
+        a = agg[names[d]]
+        a["n"].append(1)
+        a["us"].append(dur[d] / 1e3)
+        a["mfma"].append(100.0 * c.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) / (g * simds))
+        a["valu"].append(100.0 * c.get("SQ_ACTIVE_INST_VALU", 0.0) / cus / g)
+        mops = sum(c.get(k, 0.0) for k in ("SQ_INSTS_VALU_MFMA_MOPS_F64",
+                                             "SQ_INSTS_VALU_MFMA_MOPS_F32",
+                                             "SQ_INSTS_VALU_MFMA_MOPS_F16"))
+        a["tflops"].append(mops * 512.0 / (dur[d] * 1e-9) / 1e12 if dur[d] > 0 else 0.0)
+        a["valu_insts"].append(c.get("SQ_INSTS_VALU", 0.0))
+    # time-weighted over the dispatches (a kernel's long launches dominate)
+    print(f"{'kernel':<44} {'calls':>6} {'total_ms':>9} {'mean_us':>9} {'MfmaUtil%':>10} "
+          f"{'VALUBusy%':>10} {'MFMA TF/s':>10} {'VALU insts':>12}")
+    for k, a in sorted(agg.items(), key=lambda kv: -sum(kv[1]["us"])):
+        w = a["us"]
+        tw = lambda key: sum(x * y for x, y in zip(a[key], w)) / max(sum(w), 1e-30)
+        m = lambda key: sum(a[key]) / len(a[key])
+        print(f"{k[:44]:<44} {len(a['n']):>6d} {sum(w) / 1e3:>9.2f} {m('us'):>9.1f} "
+              f"{tw('mfma'):>10.1f} {tw('valu'):>10.1f} {tw('tflops'):>10.1f} "
+              f"{m('valu_insts'):>12.3g}")
+
+
 def summarize_trace(db, pattern=None):
     """Per-kernel dispatch statistics of a rocprofv3 --kernel-trace database:
     calls, total, mean, min and max duration."""
@@ -772,10 +822,14 @@ def main():
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB",
                                                           "COPY_WRITE_DB"))
     ap.add_argument("--summarize-trace", metavar="TRACE_DB")
+    ap.add_argument("--summarize-counters", metavar="PMC_DB")
     ap.add_argument("--kernels", default=None, help="regex filter for --summarize-trace")
     args = ap.parse_args()
     if args.summarize_trace:
         summarize_trace(args.summarize_trace, args.kernels)
+        return
+    if args.summarize_counters:
+        summarize_counters(args.summarize_counters, args.kernels)
         return
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, copy_bytes=2 * (1 << 30),
