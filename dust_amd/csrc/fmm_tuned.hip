@@ -318,6 +318,10 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(M2LArgs P) {
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
   const int j1 = min(kOct, J.j0 + P.jc);
+  // the source slot of this thread's target at the next offset, loaded one
+  // offset ahead so that its latency hides behind the current offset's work
+  const int tme = J.t0 + tid;
+  int cur = (tid < kNT && tme < J.nt) ? src[(long)tme * kOct + J.j0] : -1;
   for (int j = J.j0; j < j1; ++j) {
     const int di = P.offs[(J.oct * kOct + j) * 3], dj = P.offs[(J.oct * kOct + j) * 3 + 1],
               dk = P.offs[(J.oct * kOct + j) * 3 + 2];
@@ -327,9 +331,9 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(M2LArgs P) {
     __syncthreads();
     int has = 0;
     if (tid < kNT) {
-      int t = J.t0 + tid;
-      sSrc[tid] = (t < J.nt) ? src[(long)t * kOct + j] : -1;
-      has = sSrc[tid] >= 0;
+      sSrc[tid] = cur;
+      has = cur >= 0;
+      cur = (j + 1 < j1 && tme < J.nt) ? src[(long)tme * kOct + j + 1] : -1;
     }
     // column tiles (16 targets) with at least one source at this offset
     if (tid < 64) {
@@ -343,11 +347,29 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(M2LArgs P) {
     __syncthreads();
     const int bmask = sMask;
     if (bmask == 0) continue;  // no target has a source at this offset
-    // gather the source multipoles of the 64 targets: sB[k][t]
-    for (int q = tid; q < kNT * KS * 4; q += 256) {
-      const int tt = q / (KS * 4), k = q % (KS * 4);
-      const int s = sSrc[tt];
-      sB[k * ld + tt] = (s >= 0 && k < K) ? P.a32[(long)s * K + k] : 0.f;
+    // gather the source multipoles of the 64 targets: sB[k][t], sixteen
+    // loads per thread in flight at a time (clamped addresses; the missing
+    // sources and the padding rows k >= K become zeros at the LDS store)
+    {
+      const int KS4 = KS * 4, tot = kNT * KS4;
+      for (int q0 = 0; q0 < tot; q0 += 256 * 16) {
+        float v[16];
+        int dst[16];
+        unsigned okm = 0u;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int q = q0 + tid + 256 * u;
+          const int qq = min(q, tot - 1);
+          const int tt = qq / KS4, k = qq - tt * KS4;
+          const int s = sSrc[tt];
+          v[u] = P.a32[(long)max(s, 0) * K + min(k, K - 1)];
+          dst[u] = q < tot ? k * ld + tt : -1;
+          if (q < tot && s >= 0 && k < K) okm |= 1u << u;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          if (dst[u] >= 0) sB[dst[u]] = ((okm >> u) & 1u) ? v[u] : 0.f;
+      }
     }
     __syncthreads();
     if (nrt > 0) {
