@@ -267,6 +267,7 @@ SIGNATURES = [
     ("dust_panwake_state", C.c_int, [_vp, _ip, _ip, _ip]),
     ("dust_panwake_checkpoint", C.c_int, [_vp, _vp, C.POINTER(C.c_size_t)]),
     ("dust_panwake_restore", C.c_int, [_vp, _vp, C.c_size_t]),
+    ("dust_panwake_load", C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     ("dust_panwake_get", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 ]
 

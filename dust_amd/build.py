@@ -21,6 +21,10 @@ FORTRAN_MOD = os.path.join(HERE, "fortran", "mod_dust_gpu.f90")
 FORTRAN_DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "dust_gpu_driver.f90")
 FORTRAN_DRIVER = os.path.join(ROOT, "tests", "fortran", "dust_gpu_driver")
 ARCH = "gfx950"
+# HDF5 result files (libdust_h5.so, host code): the HDF5 C library of the image
+HDF5_ROOT = os.environ.get("HDF5_ROOT", "/opt/conda")
+H5_SRC = os.path.join(CSRC, "h5io.cpp")
+H5_LIB = os.path.join(HERE, "libdust_h5.so")
 
 SOURCES = [
     "fmm.hip",
@@ -101,6 +105,24 @@ def build_fortran():
     return FORTRAN_DRIVER
 
 
+def build_h5():
+    """Compile libdust_h5.so (host C++ against the HDF5 C library, rpath to
+    it).  Returns its path, or None when no HDF5 installation is found."""
+    inc = os.path.join(HDF5_ROOT, "include")
+    libdir = os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        return None
+    tmp = H5_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + inc, H5_SRC,
+           "-L" + libdir, "-lhdf5", "-Wl,-rpath," + libdir, "-o", tmp]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("g++ failed on h5io.cpp:\n" + p.stdout + p.stderr)
+    os.replace(tmp, H5_LIB)
+    return H5_LIB
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
     print(build_fortran())
+    print(build_h5())

@@ -157,6 +157,22 @@ __global__ void k_pw_geom(PwArgs A) {
   A.nver[q] = 4;
 }
 
+// End vortices of a loaded full wake (load_wake 766-775): the segment
+// between the stripe's points of the last row, with their velocities;
+// the intensity is last_idou.
+__global__ void k_pw_endvort(PwArgs A) {
+  const int iw = blockIdx.x * blockDim.x + threadIdx.x;
+  if (iw >= A.ns) return;
+  const long r = (long)A.len * A.np;
+  const int p1 = A.start[2 * iw], p2 = A.start[2 * iw + 1];
+  for (int e = 0; e < 3; ++e) {
+    A.lver[6 * iw + e] = A.pts[3 * (r + p1) + e];
+    A.lver[6 * iw + 3 + e] = A.pts[3 * (r + p2) + e];
+    A.lvel[6 * iw + e] = A.pvel[3 * (r + p1) + e];
+    A.lvel[6 * iw + 3 + e] = A.pvel[3 * (r + p2) + e];
+  }
+}
+
 // Convection of the points (910-955): rows 3..np from the old rows 2..np-1
 // with the velocities at them (vel[(k - 2) np + ip] at the 1-based row k),
 // and the end points when the wake is full.
@@ -678,6 +694,33 @@ int dust_panwake_restore(dust_panwake_t h, const void *buf, size_t size) {
   DUST_HIP(err);
   h->len = hd.len;
   h->full = hd.full;
+  return 0;
+}
+
+int dust_panwake_load(dust_panwake_t h, int len, const double *points, const double *pvel,
+                      const double *idou, const double *last_idou, void *stream) {
+  DUST_CHECK(h && points && pvel && idou && last_idou, DUST_E_ARG, "null argument");
+  DUST_CHECK(len >= 1, DUST_E_ARG, "a panel wake of %d rows cannot be loaded", len);
+  hipStream_t st = (hipStream_t)stream;
+  const int nl = std::min(len, h->p.nmax_pan);  // pan_wake_len = min(nmax_pan, size(wvort,2))
+  const size_t np = h->p.n_points, ns = h->p.n_stripes;
+  const size_t rows = (size_t)nl + 1;
+  // the rows the file holds beyond the kept ones are dropped (the file's
+  // arrays are [len+1][np][3] and [len][ns]; only the leading rows are read)
+  DUST_HIP(hipMemcpyAsync(h->pts.p, points, sizeof(double) * 3 * np * rows, hipMemcpyHostToDevice,
+                          st));
+  DUST_HIP(hipMemcpyAsync(h->pvel.p, pvel, sizeof(double) * 3 * np * rows, hipMemcpyHostToDevice,
+                          st));
+  DUST_HIP(hipMemcpyAsync(h->idou.p, idou, sizeof(double) * ns * nl, hipMemcpyHostToDevice, st));
+  DUST_HIP(hipMemcpyAsync(h->last_idou.p, last_idou, sizeof(double) * ns, hipMemcpyHostToDevice,
+                          st));
+  h->len = nl;
+  h->full = nl == h->p.nmax_pan ? 1 : 0;
+  PwArgs A = args(h);
+  k_pw_geom<<<ceil_div(A.len * A.ns, 128), 128, 0, st>>>(A);
+  if (h->full) k_pw_endvort<<<ceil_div(A.ns, 128), 128, 0, st>>>(A);
+  DUST_HIP(hipGetLastError());
+  DUST_HIP(hipStreamSynchronize(st));  // the host arrays may be released on return
   return 0;
 }
 

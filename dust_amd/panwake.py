@@ -118,6 +118,20 @@ class PanWake:
                 "checkpoint")
         return buf
 
+    def load(self, length, points, pvel, idou, last_idou, stream=None):
+        """The panel part of load_wake from a result file's arrays
+        (dust_panwake_load): points / pvel [length+1][n_points][3], idou
+        [length][n_stripes], last_idou [n_stripes]."""
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (points, pvel, idou, last_idou)]
+        length = int(length)
+        if a[0].shape != (length + 1, self.np, 3) or a[1].shape != a[0].shape:
+            raise ValueError(f"panel wake points of shape {a[0].shape}, "
+                             f"({length + 1}, {self.np}, 3) expected")
+        if a[2].shape != (length, self.ns) or a[3].shape != (self.ns,):
+            raise ValueError("panel wake intensities of the wrong shape")
+        L.check(L.lib().dust_panwake_load(self.h, length, *[x.ctypes.data for x in a], stream),
+                "panwake_load")
+
     def restore(self, buf):
         buf = np.ascontiguousarray(buf, dtype=np.uint8)
         L.check(L.lib().dust_panwake_restore(self.h, buf.ctypes.data, buf.size), "restore")

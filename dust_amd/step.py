@@ -206,6 +206,21 @@ class BodyWakeStep:
         self.ptvel = torch.zeros((self.npt * (n_wake_panels + 1), 3), **d3)
         self.mu = self.b
         self.times = {}
+        self.t = 0.0  # sim_param%time
+        # the sim_param values a result file records (save_sim_param)
+        self.sim = dict(dt=self.dt, u_inf=self.u_inf.copy(), n_wake_panels=int(n_wake_panels),
+                        n_wake_particles=int(nmax), particles_box_min=np.asarray(box[0], float),
+                        particles_box_max=np.asarray(box[1], float), join_te=bool(join_te),
+                        min_vel_at_te=float(min_vel_at_te),
+                        FarFieldRatioDoublet=self.aic["ff_ratio_doublet"],
+                        RankineRad=self.aic["rankine_rad"], VortexRad=float(vortex_rad),
+                        KVortexRad=float(k_vortex_rad), CutoffRad=self.aic["cutoff_rad"],
+                        Vortstretch=True, vortdiff=False, turbvort=False, use_fmm=True,
+                        use_fmm_panels=self.fmm_pan, BoxLength=float(tree["box_length"]),
+                        Nbox=np.asarray(tree["nbox"], np.int32),
+                        OctreeOrigin=np.asarray(tree["origin"], float),
+                        NOctreeLevels=int(tree["nlevels"]), MinOctreePart=int(tree["min_part"]),
+                        MultipoleDegree=int(degree))
 
     # ---- geometry -----------------------------------------------------------
     def _centres(self, v4):
@@ -425,6 +440,7 @@ class BodyWakeStep:
         P.update(self.wprm)
         self.pw.shed(P, self.wprm, P.n_prt, self.status)
         ev[6].record()
+        self.t += self.dt
         torch.cuda.synchronize()
         if int(self.status.item()) != 0:
             raise L.DustError(L.DUST_E_CAPACITY, "shedding: the particle pool is full")
@@ -444,6 +460,85 @@ class BodyWakeStep:
                    free=P.free.cpu().numpy())
         out.update({"pw_" + k: v for k, v in self.pw.get().items()})
         return out
+
+    # ---- result files (HDF5) ------------------------------------------------
+    def save_status(self, path, comp_name="body", comp_id=1, extra=None):
+        """Writes this step's state as one of the reference's result files
+        (save_status, src/io/mod_dust_io.f90:127-437; dust_amd.h5io): the
+        body's intensities as Components/CompNNN/Solution/Vort, the panel
+        wake (PanelWake), the live particles in slot order (ParticleWake,
+        with LastPanIdou), an empty ring wake, the time and the sim_param
+        values of this step (the ring wake's length is the panel wake's, as
+        in the reference's files without actuator disks).  `extra`: more root attributes / Parameters
+        (P_inf, rho_inf, mu_inf, a_inf ...) and element results (a dict
+        'solution' of pres, dforce, dmom, surf_vel, didou_dt arrays)."""
+        from . import h5io as H
+        extra = dict(extra or {})
+        sol = extra.pop("solution", {})
+        P = self.pool
+        live = P.live().cpu().numpy()
+        cen, dr, mg = P.cen.cpu().numpy()[live], P.dir.cpu().numpy()[live], P.mag.cpu().numpy()[live]
+        vel, rv = P.vel.cpu().numpy()[live], P.r_vortex.cpu().numpy()[live]
+        pw = self.pw.get()
+        ln, _, _ = self.pw.state()
+        params = dict(self.sim)
+        params.update(extra)
+        comp = dict(comp_id=comp_id, name=comp_name, ref_tag="0", ref_id=0,
+                    vort=self.b.cpu().numpy())
+        comp.update(sol)
+        H.save_status(path, self.t, params=params, components=[comp],
+                      panel_wake=dict(points=pw["pts"][:ln + 1], vels=pw["pvel"][:ln + 1],
+                                      start_points=self.te["start"], vort=pw["idou"][:ln]),
+                      particles=dict(points=cen, vels=vel, vort=dr * mg[:, None],
+                                     turbvisc=np.zeros(len(live)), vortex_rad=rv,
+                                     last_pan_idou=pw["last_idou"]),
+                      ring_wake_len=ln)  # the ring wake grows with the panel wake
+
+    def load_wake(self, path, comp_id=1):
+        """Restart from a result file, as load_wake (src/aeroel/mod_wake.f90:
+        597-778) and load_solution (src/io/mod_dust_io.f90:444-543) do: the
+        trailing-edge connectivity is checked, the panel wake takes the
+        file's rows (dust_panwake_load), the particles fill slots 0..n-1 with
+        mag = |WakeVort| and dir = WakeVort / mag (WakeVort itself below
+        1e-13), the body's intensities and the time are the file's.  Not bit
+        for bit: mag and dir are recomputed from their product, as the
+        reference recomputes them."""
+        from . import h5io as H
+        torch = self.torch
+        w = H.read_wake(path)
+        sp = np.asarray(w["start_points"])
+        if sp.shape != self.te["start"].shape or not np.array_equal(sp, self.te["start"]):
+            raise ValueError("Different wake trailing edge connectivity between the loaded "
+                             "and built geometry")
+        if w["conn_pe"].size:
+            raise ValueError("Different wake disk connectivity between the loaded and built "
+                             "geometry")
+        ln = int(w["pan_vort"].shape[0])
+        self.pw.load(ln, w["pan_points"], w["pan_vels"], w["pan_vort"], w["last_pan_idou"])
+        n = int(w["vort"].shape[0])
+        P = self.pool
+        if n > P.nmax:
+            raise ValueError("Loading a number of particles greater than the maximum allowed "
+                             "for this run")
+        mag, d = H.particles_from_file(w["vort"])
+        vel = w["vels"] if w["vels"] is not None else np.zeros((n, 3))
+        for name, src in (("cen", w["points"]), ("dir", d), ("mag", mag),
+                          ("r_vortex", w["vortex_rad"]), ("vel", vel)):
+            t = getattr(P, name)
+            t.zero_()
+            if n:
+                t[:n].copy_(torch.from_numpy(np.ascontiguousarray(src, dtype=np.float64)))
+        P.stretch.zero_()
+        P.rotu.zero_()
+        P.free.fill_(1)
+        P.free[:n] = 0
+        P.n_prt.fill_(n)
+        self.b.copy_(torch.from_numpy(np.ascontiguousarray(H.read_solution(path, comp_id))))
+        self.t = w["time"]
+        if self.moving:
+            self.theta = self.omega * self.t
+            self._move(advance=False)
+        torch.cuda.synchronize()
 
     # ---- restart --------------------------------------------------------------
     _POOL = ("free", "cen", "dir", "mag", "r_vortex", "vel", "stretch", "rotu", "n_prt")

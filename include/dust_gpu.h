@@ -732,6 +732,20 @@ int dust_panwake_get(dust_panwake_t h, double *points, double *pvel, double *ido
 int dust_panwake_checkpoint(dust_panwake_t h, void *buf, size_t *size);
 int dust_panwake_restore(dust_panwake_t h, const void *buf, size_t size);
 
+/* The panel part of load_wake (src/aeroel/mod_wake.f90:611-655, 766-775):
+ * the wake of a result file's PanelWake group, host arrays as the file holds
+ * them -- points and velocities [len+1][n_points][3] (WakePoints, WakeVels),
+ * intensities [len][n_stripes] (WakeVort) -- and ParticleWake/LastPanIdou
+ * [n_stripes].  pan_wake_len becomes min(nmax_pan, len) (the leading rows
+ * are kept), the panels are rebuilt from the points, and a wake of
+ * nmax_pan rows is full with its end vortices on the last row of points.
+ * The trailing-edge rows of a start(first = 1) call and the junction tables
+ * are kept, as the reference keeps them.  The caller checks StartPoints
+ * against its own table (the reference's connectivity check, 627-630).
+ * Synchronises `stream`. */
+int dust_panwake_load(dust_panwake_t h, int len, const double *points, const double *pvel,
+                      const double *idou, const double *last_idou, void *stream);
+
 /* Release every library-global workspace (device buffers, page-locked
  * records, streams and events of the solvers and element kernels) after
  * synchronising the device; they are re-created on the next call that needs

@@ -10,10 +10,15 @@ regression tests hold and writes small .npz fixtures next to this script:
                                tests/reg_test_b/dust_base.in)
   mir_static_p_particles.npz -- ParticleWake of tests/mir_test/static/p (the
                                direct-sum case with stretching and diffusion)
+  reference_h5_layout.json  -- the dataset paths, shapes (as h5py reports
+                               them) and element types of three result files
+                               (reg_test_a static, reg_test_b, mir_test
+                               static/p): the layout save_status writes
 
 Run with an interpreter that has h5py (e.g. /opt/conda/bin/python3.9):
     /opt/conda/bin/python3.9 tests/golden/make_golden.py /root/reference
 """
+import json
 import os
 import sys
 
@@ -33,6 +38,19 @@ def particles(path):
         return out
 
 
+def layout(path):
+    """{dataset path: [shape, dtype]} of one result file, plus the root
+    attributes' names."""
+    out = {}
+    with h5py.File(path, "r") as f:
+        def visit(name, obj):
+            if isinstance(obj, h5py.Dataset):
+                out[name] = [list(obj.shape), obj.dtype.str]
+        f.visititems(visit)
+        attrs = sorted(f.attrs.keys())
+    return {"datasets": out, "attrs": attrs}
+
+
 def main(ref):
     b = particles(os.path.join(ref, "tests/reg_test_b/Output/ref_base_0-6-0_res_0011.h5"))
     np.savez_compressed(os.path.join(HERE, "reg_test_b_particles.npz"), **b,
@@ -41,6 +59,12 @@ def main(ref):
                         vortex_rad=0.05, rankine_rad=0.005)
     m = particles(os.path.join(ref, "tests/mir_test/static/p/Output/ref_res_0011.h5"))
     np.savez_compressed(os.path.join(HERE, "mir_static_p_particles.npz"), **m, vortex_rad=0.1)
+    files = {"reg_test_a_static": "tests/reg_test_a/Output/ref_static_base_0-6-0_res_0011.h5",
+             "reg_test_b": "tests/reg_test_b/Output/ref_base_0-6-0_res_0011.h5",
+             "mir_test_static_p": "tests/mir_test/static/p/Output/ref_res_0011.h5"}
+    lay = {k: layout(os.path.join(ref, v)) for k, v in files.items()}
+    with open(os.path.join(HERE, "reference_h5_layout.json"), "w") as fo:
+        json.dump({"files": files, "layout": lay}, fo, indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":
