@@ -166,7 +166,10 @@ def time_lu(torch, L, W, n_req, nrhs, steps, mesh=None):
     out["fp64_backward_error"] = berr(Xp)
     out["fp64_getrf_getrs_TFLOPs"] = ((2.0 / 3.0) * n ** 3 + 2.0 * n * n * nrhs) / (out["fp64_ms"] / 1e3) / 1e12
     sols = {"fp64": Xp}
-    for prec, code in (("fp16s", L.LU_FP16S), ("fp32", L.LU_FP32)):
+    # refinement to 1e-12 (the default legs), and the FP16-scaled solve refined
+    # only to the 1e-10 residual bar of the tuned form's accuracy target
+    for prec, code, tol in (("fp16s", L.LU_FP16S, 1e-12), ("fp32", L.LU_FP32, 1e-12),
+                            ("fp16s_tol1e-10", L.LU_FP16S, 1e-10)):
         ts = []
         rc = 0
         for it in range(steps):
@@ -175,7 +178,7 @@ def time_lu(torch, L, W, n_req, nrhs, steps, mesh=None):
             resid = C.c_double()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            rc = lib.dust_dgesv_mixed_dev(n, nrhs, A.data_ptr(), n, X2.data_ptr(), n, code, 1e-12,
+            rc = lib.dust_dgesv_mixed_dev(n, nrhs, A.data_ptr(), n, X2.data_ptr(), n, code, tol,
                                           50, C.byref(iters), C.byref(resid), None)
             torch.cuda.synchronize()
             ts.append(time.perf_counter() - t0)
@@ -185,6 +188,7 @@ def time_lu(torch, L, W, n_req, nrhs, steps, mesh=None):
             out[f"{prec}_ms"] = 1e3 * float(np.median(ts))
             out[f"{prec}_iters"] = iters.value
             out[f"{prec}_backward_error"] = berr(X2)
+            out[f"{prec}_residual_inf"] = resid.value
             out[f"{prec}_rel_diff_vs_fp64"] = float(((X2 - Xp).norm() / Xp.norm()).item())
             out[f"{prec}_speedup_vs_fp64"] = out["fp64_ms"] / out[f"{prec}_ms"]
             sols[prec] = X2
