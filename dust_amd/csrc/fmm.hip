@@ -1296,7 +1296,7 @@ static int multipole_phase_b(FmmHandle *h) {
   const int use_vs = h->prm.use_vs;
   const bool tuned = h->prm.precision == DUST_FMM_TUNED;
   if (tuned) {
-    DUST_TRY(tuned_m2l_all(h));
+    if (!h->batched) DUST_TRY(tuned_m2l_all(h));  // else in the batch's shared launch
     timer_end(h, DUST_FMM_STAGE_M2L);
   }
   for (int l = 0; l < h->nlev; ++l) {
@@ -1390,8 +1390,13 @@ static int fmm_apply(FmmHandle *h) {
   EvalArgs A;
   make_eval_args(h, &A);
   timer_begin(h, DUST_FMM_STAGE_P2P);
-  if (h->prm.precision == DUST_FMM_TUNED) DUST_TRY(tuned_eval(h, A));
-  else DUST_TRY(eval_fp64_launch(h, A));
+  if (h->batched) {
+    // the P2P items ran in the batch's shared launch
+  } else if (h->prm.precision == DUST_FMM_TUNED) {
+    DUST_TRY(tuned_eval(h, A));
+  } else {
+    DUST_TRY(eval_fp64_launch(h, A));
+  }
   timer_end(h, DUST_FMM_STAGE_P2P);
   // L2P, the sum of the items' partial sums and the free stream
   timer_begin(h, DUST_FMM_STAGE_L2P);
@@ -1509,6 +1514,7 @@ static DevBuf<int> g_pan_leaf, g_pan_err;
 void fmm_release() {
   g_pan_leaf.release();
   g_pan_err.release();
+  tuned_batch_release();
 }
 
 static int fmm_apply_panels(FmmHandle *h, int m, const double *cen, double *uvort) {
@@ -1683,7 +1689,11 @@ int dust_fmm_pass_dev(dust_fmm_t hh, int *nleaves) {
 // Several independent cases: the passes advance in lockstep, phase by phase
 // (sort a / b / c, multipole a / b, apply), each handle on its own stream; one
 // synchronisation of every stream separates two phases, so the kernels of
-// different cases overlap and the host round trips are shared.
+// different cases overlap and the host round trips are shared.  The tuned
+// passes (FP32 P2P, same use_vs and core) share two persistent launches: the
+// M2L jobs of all of them and the P2P items of all of them, on the first such
+// handle's stream; every handle's L2L chain waits for the M2L launch and its
+// L2P / sums for the P2P launch.
 int dust_fmm_pass_batch_dev(int nh, dust_fmm_t *hs, int *nleaves) {
   DUST_CHECK(nh >= 0 && (nh == 0 || hs), DUST_E_ARG, "invalid batch arguments");
   for (int i = 0; i < nh; ++i) DUST_CHECK(hs[i], DUST_E_ARG, "null handle %d in batch", i);
@@ -1701,11 +1711,57 @@ int dust_fmm_pass_batch_dev(int nh, dust_fmm_t *hs, int *nleaves) {
     if (nleaves) nleaves[i] = hs[i]->h->n_leaves;
   }
   DUST_TRY(sync_all());
+  // the passes of the shared launches
+  std::vector<FmmHandle *> shared;
   for (int i = 0; i < nh; ++i) {
-    DUST_TRY(multipole_phase_b(hs[i]->h));
-    DUST_TRY(fmm_apply(hs[i]->h));
+    FmmHandle *h = hs[i]->h;
+    if (!tuned_batchable(h)) continue;
+    if (!shared.empty() && (h->prm.use_vs != shared[0]->prm.use_vs ||
+                            h->prm.kernel != shared[0]->prm.kernel))
+      continue;
+    shared.push_back(h);
   }
-  return sync_all();
+  struct Events {
+    hipEvent_t m2l = nullptr, p2p = nullptr;
+    ~Events() {
+      if (m2l) (void)hipEventDestroy(m2l);
+      if (p2p) (void)hipEventDestroy(p2p);
+    }
+  } ev;
+  auto clear = [&]() {
+    for (FmmHandle *h : shared) h->batched = false;
+  };
+  if (!shared.empty()) {
+    DUST_HIP(hipEventCreateWithFlags(&ev.m2l, hipEventDisableTiming));
+    DUST_HIP(hipEventCreateWithFlags(&ev.p2p, hipEventDisableTiming));
+    const int rc = tuned_batch_launch(shared.data(), (int)shared.size(), shared[0]->stream, ev.m2l,
+                                      ev.p2p);
+    if (rc) {
+      (void)sync_all();
+      return rc;
+    }
+    for (FmmHandle *h : shared) h->batched = true;
+  }
+  int rc = 0;
+  for (int i = 0; i < nh && rc == 0; ++i) {
+    FmmHandle *h = hs[i]->h;
+    if (h->batched) {
+      if (hipStreamWaitEvent(h->stream, ev.m2l, 0) != hipSuccess) {
+        set_error("hipStreamWaitEvent failed");
+        rc = DUST_E_HIP;
+        break;
+      }
+    }
+    rc = multipole_phase_b(h);
+    if (rc == 0 && h->batched && hipStreamWaitEvent(h->stream, ev.p2p, 0) != hipSuccess) {
+      set_error("hipStreamWaitEvent failed");
+      rc = DUST_E_HIP;
+    }
+    if (rc == 0) rc = fmm_apply(h);
+  }
+  const int rs = sync_all();  // also before the events are destroyed
+  clear();
+  return rc ? rc : rs;
 }
 
 void *dust_fmm_stream(dust_fmm_t h) { return h ? (void *)h->h->stream : nullptr; }

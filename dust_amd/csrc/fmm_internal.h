@@ -104,6 +104,9 @@ struct FmmHandle {
   StageTimer timer;
   int seg_iters = kSegIters;  // P2P source iterations per lane in one item
   bool sorted = false, computed = false;
+  // set by the batch driver while the tuned M2L and P2P of this pass run in
+  // the batch's shared launches (the pass skips its own)
+  bool batched = false;
   // page-locked source/targets of the asynchronous copies between pass phases
   HostBuf<FmmReadback> rb;
 };
@@ -127,6 +130,10 @@ int tuned_set(FmmHandle *h, const char *name, double value);
 int tuned_get(FmmHandle *h, const char *name, double *value);
 int tuned_m2l_all(FmmHandle *h);
 int tuned_eval(FmmHandle *h, EvalArgs &A);
+bool tuned_batchable(const FmmHandle *h);
+int tuned_batch_launch(FmmHandle *const *hs, int nh, hipStream_t st, hipEvent_t ev_m2l,
+                       hipEvent_t ev_p2p);
+void tuned_batch_release();
 void tuned_release(FmmHandle *h);
 
 // timing helpers

@@ -53,6 +53,7 @@ struct TunedState {
 };
 
 static TunedState *TS(FmmHandle *h) { return (TunedState *)h->tuned; }
+static const TunedState *TS(const FmmHandle *h) { return (const TunedState *)h->tuned; }
 static int build_w(FmmHandle *h);
 
 int tuned_prepare(FmmHandle *h) {
@@ -300,15 +301,14 @@ __device__ __forceinline__ void m2l_step(f4 (&acc)[kRTW][4], const float (&af)[k
   }
 }
 
-__global__ void __launch_bounds__(256) k_m2l_mfma(M2LArgs P) {
-  extern __shared__ float sB[];  // [ks * 4][kNT + 4]
-  __shared__ int sSrc[kNT];
-  __shared__ int sMask;
-  const M2LJob J = P.jobs[blockIdx.x];
-  if (J.t0 >= J.nt) return;  // padding job
+// One job of the M2L GEMMs (J.t0 < J.nt) by the 256 threads of the
+// workgroup; sB, sSrc, sMask are the workgroup's LDS.
+__device__ __forceinline__ void m2l_job(const M2LArgs &P, const M2LJob &J, float *sB, int *sSrc,
+                                        int &sMask) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int rt0 = J.rb * kRTB + __builtin_amdgcn_readfirstlane(wv) * kRTW;  // wave's first row tile
-  const int nrt = max(0, min(kRTW, P.w.rt - rt0));         // row tiles with rows < nrows
+  // the wave's first row tile, and its row tiles with rows < nrows
+  const int rt0 = J.rb * kRTB + __builtin_amdgcn_readfirstlane(wv) * kRTW;
+  const int nrt = max(0, min(kRTW, P.w.rt - rt0));
   const int K = P.w.K, KS = P.w.ks, ld = kNT + 4;
   const int *grp = P.grp + J.grp_off;
   const int *src = P.src + J.grp_off * kOct;
@@ -325,8 +325,9 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(M2LArgs P) {
   for (int j = J.j0; j < j1; ++j) {
     const int di = P.offs[(J.oct * kOct + j) * 3], dj = P.offs[(J.oct * kOct + j) * 3 + 1],
               dk = P.offs[(J.oct * kOct + j) * 3 + 2];
-    const long tab = J.kd_base + (di + J.kdl0) +
-                     (long)(2 * J.kdl0 + 1) * ((dj + J.kdl1) + (long)(2 * J.kdl1 + 1) * (dk + J.kdl2));
+    const long tab =
+        J.kd_base + (di + J.kdl0) +
+        (long)(2 * J.kdl0 + 1) * ((dj + J.kdl1) + (long)(2 * J.kdl1 + 1) * (dk + J.kdl2));
     const float *Wo = P.W + tab * P.w.per_off;
     __syncthreads();
     int has = 0;
@@ -415,6 +416,49 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(M2LArgs P) {
   }
 }
 
+// Workgroup b runs job b of one pass.
+__global__ void __launch_bounds__(256) k_m2l_mfma(M2LArgs P) {
+  extern __shared__ float sB[];  // [ks * 4][kNT + 4]
+  __shared__ int sSrc[kNT];
+  __shared__ int sMask;
+  const M2LJob J = P.jobs[blockIdx.x];
+  if (J.t0 >= J.nt) return;  // padding job
+  m2l_job(P, J, sB, sSrc, sMask);
+}
+
+// The jobs of several passes in one persistent launch.  The job list holds 8
+// queues interleaved (job k of queue x at k * 8 + x, each queue the jobs of
+// the operator chunks dealt to XCD x; the case of a job in J.pad, its
+// arguments in cases[J.pad]); workgroups are dispatched to the XCDs
+// round-robin, so workgroup b runs on XCD b % 8 and takes the next job of
+// queue b % 8 (counter qcnt[b % 8]) until the queue is empty: dynamic load
+// balance inside an XCD, every operator chunk read into one L2.  Two
+// workgroups per SIMD pair, as the one-pass kernel reaches.
+__global__ void __launch_bounds__(256, 2) k_m2l_mfma_batch(const M2LJob *__restrict__ jobs,
+                                                           const M2LArgs *__restrict__ cases,
+                                                           int qlen, int *qcnt) {
+  extern __shared__ float sB[];  // [ks * 4][kNT + 4]
+  __shared__ int sSrc[kNT];
+  __shared__ int sMask;
+  __shared__ int sJob;
+  const int x = blockIdx.x & 7;
+  for (;;) {
+    if (threadIdx.x == 0) sJob = atomicAdd(&qcnt[x], 1);
+    __syncthreads();
+    const int k = sJob;
+    __syncthreads();  // read by every thread before thread 0 takes the next
+    if (k >= qlen) break;
+    const M2LJob J = jobs[(long)k * 8 + x];
+    if (J.t0 >= J.nt) continue;  // padding job
+    const M2LArgs P = cases[J.pad];
+    m2l_job(P, J, sB, sSrc, sMask);
+  }
+}
+
+__global__ void k_batch_clear(int *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // FP32 P2P with FP64 near pairs
 // ---------------------------------------------------------------------------
@@ -459,152 +503,184 @@ __device__ inline void near64(double r0, double r1, double r2, double al0, doubl
 // FP32 sums of every tile are folded into FP64 accumulators.  A partial tile
 // is padded with zero-strength sources far away, so the loop trip count is
 // uniform.
-template <int VS, int GAUSS>
-__global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_f32(EvalArgs A) {
+//
+// BATCH = 0: wavefront w of the grid evaluates item w of one pass (A0).
+// BATCH = 1: the items of several passes in one persistent launch: every
+// wavefront takes the next global item number from the counter *cnt until
+// all are taken; item g belongs to the case c with off[c] <= g < off[c + 1]
+// (off[ncase] = total) and is item g - off[c] of that case's pass (cases[c]).
+// The body is written once, inside the kernel: as a called function it was
+// compiled to a different register allocation and ran 15 % slower.
+template <int VS, int GAUSS, int BATCH>
+__global__ void __launch_bounds__(kWave * kEvalWaves)
+    k_p2p_f32(EvalArgs A0, const EvalArgs *__restrict__ cases, const int *__restrict__ off,
+              int ncase, int *cnt) {
   __shared__ f4 ssm[kEvalWaves][2 * kWave];
   __shared__ double sdm[kEvalWaves][8 * kWave];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   f4 *ss = ssm[wv];
   double *sd = sdm[wv];
-  const int it = blockIdx.x * kEvalWaves + wv;
-  if (it >= A.n_items) return;
-  const EvalItem I = A.items[it];
-  const EvalUnit U = A.units[I.unit];
-  const long g = A.leaves[U.leaf];
-  const int lvl = lvl_of(A.G, g);
-  const long c = g - A.G.off[lvl];
-  int ix, iy, iz;
-  cell_decode(A.G, lvl, c, ix, iy, iz);
-  const double cx = cell_center(A.G, lvl, 0, ix), cy = cell_center(A.G, lvl, 1, iy),
-               cz = cell_center(A.G, lvl, 2, iz);
-  const int lpt = U.lpt, tl = lane / lpt, sub = lane - tl * lpt;
-  const bool valid = tl < U.nt;
-  const int t = U.t0 + (valid ? tl : 0);
-  const double xr = A.px[t] - cx, yr = A.py[t] - cy, zr = A.pz[t] - cz;
-  const double mg = A.pm[t];
-  const double al0 = mg * A.dx[t], al1 = mg * A.dy[t], al2 = mg * A.dz[t];
-  const float fx = (float)xr, fy = (float)yr, fz = (float)zr;
-  const float fa0 = (float)al0, fa1 = (float)al1, fa2 = (float)al2;
-  double pv[3] = {0, 0, 0}, ps[3] = {0, 0, 0};
-  const double near2d = A.near2;  // staged per source as near2 r_s^2
-  const int nr = A.cnt[U.leaf];
-  const int2 *rl = &A.rng[(long)U.leaf * A.cap];
-  int ri, off;
-  seg_seek(rl, nr, I.lo, ri, off);
-  int left = I.hi - I.lo;
-  for (; ri < nr && left > 0; ++ri, off = 0) {
-    const int2 r = rl[ri];
-    const int e = min(r.y, r.x + off + left);
-    for (int s0 = r.x + off; s0 < e; s0 += kWave) {
-      const int ns = min(kWave, e - s0);
-      if (lane < ns) {
-        const int q = s0 + lane;
-        const double smg = A.pm[q], srv = A.pr[q];
-        const double ox = A.px[q] - cx, oy = A.py[q] - cy, oz = A.pz[q] - cz;
-        const double a0 = A.dx[q] * smg, a1 = A.dy[q] * smg, a2 = A.dz[q] * smg;
-        // Rosenhead: r^2 of the core; Gaussian: 1 / (sqrt2 r)
-        ss[2 * lane] = (f4){(float)ox, (float)oy, (float)oz,
-                            GAUSS ? (float)(0.70710678118654752 / srv) : (float)(srv * srv)};
-        ss[2 * lane + 1] = (f4){(float)a0, (float)a1, (float)a2, (float)(near2d * srv * srv)};
-        double *d = sd + 8 * lane;
-        d[0] = ox; d[1] = oy; d[2] = oz; d[3] = srv;
-        d[4] = a0; d[5] = a1; d[6] = a2;
-      } else {
-        ss[2 * lane] = (f4){1.0e6f, 1.0e6f, 1.0e6f, 1.f};
-        ss[2 * lane + 1] = (f4){0.f, 0.f, 0.f, 0.f};
+  const int total = BATCH ? off[ncase] : A0.n_items;
+  // the next item of the batch, taken by lane 0 for the wavefront
+  auto next_item = [&]() {
+    int v = 0;
+    if (lane == 0) v = atomicAdd(cnt, 1);
+    return __builtin_amdgcn_readfirstlane(v);
+  };
+  int gi = BATCH ? next_item() : blockIdx.x * kEvalWaves + wv;
+  if (gi >= total) return;
+  do {
+    int cs = 0;
+    if (BATCH) {
+      int lo = 0, hi = ncase - 1;  // the last case with off[c] <= gi
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= gi) lo = mid;
+        else hi = mid - 1;
       }
-      wave_sync();
-      float v0 = 0.f, v1 = 0.f, v2 = 0.f, w0 = 0.f, w1 = 0.f, w2 = 0.f;
-      float S0 = 0.f, S1 = 0.f, S2 = 0.f;
-      unsigned long long nmask = 0ull;
-      // one source pair of the tile: s = sub + k lpt
-      auto pair = [&](int s) {
-        const f4 sp = ss[2 * s], sa = ss[2 * s + 1];
-        const float r0 = fx - sp[0], r1 = fy - sp[1], r2 = fz - sp[2];
-        const float a0 = sa[0], a1 = sa[1], a2 = sa[2];
-        const float d2 = r0 * r0 + r1 * r1 + r2 * r2;
-        // pairs inside the source core go to FP64, all but the self pair
-        // (skipped by index, as the reference does: a distinct particle at
-        // the same position still adds its stretching term)
-        const bool skip = d2 <= sa[3];
-        if (__builtin_expect(__any(skip), 0)) {
-          if (skip && s0 + s != t) nmask |= 1ull << s;
-        }
-        // cf = c of kernel_coeffs; tf = -d / 3 (so that the stretching sum
-        // below is shared by both cores: Rosenhead d = -3 cf / R^2)
-        float cf, tf;
-        if (GAUSS) {
-          const float r = sqrtf(d2), x = r * sp[3];
-          const float E = erff(x), ex = expf(-x * x);
-          const float ir = 1.f / r, ir3 = ir * ir * ir;
-          cf = skip ? 0.f : -(E - 1.1283791670955126f * x * ex) * ir3;
-          tf = skip ? 0.f
-                    : -(E - 0.18806319451591877f * ex * x * (6.f + 4.f * x * x)) * ir3 * ir * ir;
+      cs = __builtin_amdgcn_readfirstlane(lo);
+    }
+    const EvalArgs &A = BATCH ? cases[cs] : A0;
+    const int it = BATCH ? gi - off[cs] : gi;
+    const EvalItem I = A.items[it];
+    const EvalUnit U = A.units[I.unit];
+    const long g = A.leaves[U.leaf];
+    const int lvl = lvl_of(A.G, g);
+    const long c = g - A.G.off[lvl];
+    int ix, iy, iz;
+    cell_decode(A.G, lvl, c, ix, iy, iz);
+    const double cx = cell_center(A.G, lvl, 0, ix), cy = cell_center(A.G, lvl, 1, iy),
+                 cz = cell_center(A.G, lvl, 2, iz);
+    const int lpt = U.lpt, tl = lane / lpt, sub = lane - tl * lpt;
+    const bool valid = tl < U.nt;
+    const int t = U.t0 + (valid ? tl : 0);
+    const double xr = A.px[t] - cx, yr = A.py[t] - cy, zr = A.pz[t] - cz;
+    const double mg = A.pm[t];
+    const double al0 = mg * A.dx[t], al1 = mg * A.dy[t], al2 = mg * A.dz[t];
+    const float fx = (float)xr, fy = (float)yr, fz = (float)zr;
+    const float fa0 = (float)al0, fa1 = (float)al1, fa2 = (float)al2;
+    double pv[3] = {0, 0, 0}, ps[3] = {0, 0, 0};
+    const double near2d = A.near2;  // staged per source as near2 r_s^2
+    const int nr = A.cnt[U.leaf];
+    const int2 *rl = &A.rng[(long)U.leaf * A.cap];
+    int ri, off;
+    seg_seek(rl, nr, I.lo, ri, off);
+    int left = I.hi - I.lo;
+    for (; ri < nr && left > 0; ++ri, off = 0) {
+      const int2 r = rl[ri];
+      const int e = min(r.y, r.x + off + left);
+      for (int s0 = r.x + off; s0 < e; s0 += kWave) {
+        const int ns = min(kWave, e - s0);
+        if (lane < ns) {
+          const int q = s0 + lane;
+          const double smg = A.pm[q], srv = A.pr[q];
+          const double ox = A.px[q] - cx, oy = A.py[q] - cy, oz = A.pz[q] - cz;
+          const double a0 = A.dx[q] * smg, a1 = A.dy[q] * smg, a2 = A.dz[q] * smg;
+          // Rosenhead: r^2 of the core; Gaussian: 1 / (sqrt2 r)
+          ss[2 * lane] = (f4){(float)ox, (float)oy, (float)oz,
+                              GAUSS ? (float)(0.70710678118654752 / srv) : (float)(srv * srv)};
+          ss[2 * lane + 1] = (f4){(float)a0, (float)a1, (float)a2, (float)(near2d * srv * srv)};
+          double *d = sd + 8 * lane;
+          d[0] = ox; d[1] = oy; d[2] = oz; d[3] = srv;
+          d[4] = a0; d[5] = a1; d[6] = a2;
         } else {
-          const float R2 = d2 + sp[3];
-          const float inv = __builtin_amdgcn_rsqf(R2);
-          const float inv2 = inv * inv;
-          cf = skip ? 0.f : -(inv2 * inv);
-          tf = cf * inv2;
+          ss[2 * lane] = (f4){1.0e6f, 1.0e6f, 1.0e6f, 1.f};
+          ss[2 * lane + 1] = (f4){0.f, 0.f, 0.f, 0.f};
         }
-        const float c0 = r1 * a2 - r2 * a1, c1 = r2 * a0 - r0 * a2, c2 = r0 * a1 - r1 * a0;
-        v0 += cf * c0;
-        v1 += cf * c1;
-        v2 += cf * c2;
+        wave_sync();
+        float v0 = 0.f, v1 = 0.f, v2 = 0.f, w0 = 0.f, w1 = 0.f, w2 = 0.f;
+        float S0 = 0.f, S1 = 0.f, S2 = 0.f;
+        unsigned long long nmask = 0ull;
+        // one source pair of the tile: s = sub + k lpt
+        auto pair = [&](int s) {
+          const f4 sp = ss[2 * s], sa = ss[2 * s + 1];
+          const float r0 = fx - sp[0], r1 = fy - sp[1], r2 = fz - sp[2];
+          const float a0 = sa[0], a1 = sa[1], a2 = sa[2];
+          const float d2 = r0 * r0 + r1 * r1 + r2 * r2;
+          // pairs inside the source core go to FP64, all but the self pair
+          // (skipped by index, as the reference does: a distinct particle at
+          // the same position still adds its stretching term)
+          const bool skip = d2 <= sa[3];
+          if (__builtin_expect(__any(skip), 0)) {
+            if (skip && s0 + s != t) nmask |= 1ull << s;
+          }
+          // cf = c of kernel_coeffs; tf = -d / 3 (so that the stretching sum
+          // below is shared by both cores: Rosenhead d = -3 cf / R^2)
+          float cf, tf;
+          if (GAUSS) {
+            const float r = sqrtf(d2), x = r * sp[3];
+            const float E = erff(x), ex = expf(-x * x);
+            const float ir = 1.f / r, ir3 = ir * ir * ir;
+            cf = skip ? 0.f : -(E - 1.1283791670955126f * x * ex) * ir3;
+            tf = skip ? 0.f
+                      : -(E - 0.18806319451591877f * ex * x * (6.f + 4.f * x * x)) * ir3 * ir * ir;
+          } else {
+            const float R2 = d2 + sp[3];
+            const float inv = __builtin_amdgcn_rsqf(R2);
+            const float inv2 = inv * inv;
+            cf = skip ? 0.f : -(inv2 * inv);
+            tf = cf * inv2;
+          }
+          const float c0 = r1 * a2 - r2 * a1, c1 = r2 * a0 - r0 * a2, c2 = r0 * a1 - r1 * a0;
+          v0 += cf * c0;
+          v1 += cf * c1;
+          v2 += cf * c2;
+          if (VS) {
+            // stretching -[cf (al x a) + df r (r.(al x a))]: sum cf a is crossed
+            // with al after the tile; r.(al x a) = -al.(r x a), df = -3 cf / R^2
+            // (the factor 3 is applied after the tile)
+            S0 += cf * a0;
+            S1 += cf * a1;
+            S2 += cf * a2;
+            const float tt = tf * (fa0 * c0 + fa1 * c1 + fa2 * c2);
+            w0 += tt * r0;
+            w1 += tt * r1;
+            w2 += tt * r2;
+          }
+        };
+        // wave-uniform trip count: scalar loop control, two pairs per iteration
+        const int nk = __builtin_amdgcn_readfirstlane((ns + lpt - 1) / lpt);
+        int k = 0;
+        for (; k + 2 <= nk; k += 2) {
+          pair(sub + k * lpt);
+          pair(sub + (k + 1) * lpt);
+        }
+        if (k < nk) pair(sub + k * lpt);
         if (VS) {
-          // stretching -[cf (al x a) + df r (r.(al x a))]: sum cf a is crossed
-          // with al after the tile; r.(al x a) = -al.(r x a), df = -3 cf / R^2
-          // (the factor 3 is applied after the tile)
-          S0 += cf * a0;
-          S1 += cf * a1;
-          S2 += cf * a2;
-          const float tt = tf * (fa0 * c0 + fa1 * c1 + fa2 * c2);
-          w0 += tt * r0;
-          w1 += tt * r1;
-          w2 += tt * r2;
+          w0 = -3.f * w0 - (fa1 * S2 - fa2 * S1);
+          w1 = -3.f * w1 - (fa2 * S0 - fa0 * S2);
+          w2 = -3.f * w2 - (fa0 * S1 - fa1 * S0);
         }
-      };
-      // wave-uniform trip count: scalar loop control, two pairs per iteration
-      const int nk = __builtin_amdgcn_readfirstlane((ns + lpt - 1) / lpt);
-      int k = 0;
-      for (; k + 2 <= nk; k += 2) {
-        pair(sub + k * lpt);
-        pair(sub + (k + 1) * lpt);
-      }
-      if (k < nk) pair(sub + k * lpt);
-      if (VS) {
-        w0 = -3.f * w0 - (fa1 * S2 - fa2 * S1);
-        w1 = -3.f * w1 - (fa2 * S0 - fa0 * S2);
-        w2 = -3.f * w2 - (fa0 * S1 - fa1 * S0);
-      }
-      pv[0] += v0; pv[1] += v1; pv[2] += v2;
-      ps[0] += w0; ps[1] += w1; ps[2] += w2;
-      // the FP64 pairs of this tile, one per lane per step
-      while (__any(nmask != 0ull)) {
-        if (nmask) {
-          const int s = __builtin_ctzll(nmask);
-          nmask &= nmask - 1;
-          const double *d = sd + 8 * s;
-          near64<GAUSS>(xr - d[0], yr - d[1], zr - d[2], al0, al1, al2, d[4], d[5], d[6], d[3], VS,
-                        pv, ps);
+        pv[0] += v0; pv[1] += v1; pv[2] += v2;
+        ps[0] += w0; ps[1] += w1; ps[2] += w2;
+        // the FP64 pairs of this tile, one per lane per step
+        while (__any(nmask != 0ull)) {
+          if (nmask) {
+            const int s = __builtin_ctzll(nmask);
+            nmask &= nmask - 1;
+            const double *d = sd + 8 * s;
+            near64<GAUSS>(xr - d[0], yr - d[1], zr - d[2], al0, al1, al2, d[4], d[5], d[6], d[3],
+                          VS, pv, ps);
+          }
         }
+        wave_sync();
       }
-      wave_sync();
+      left -= e - (r.x + off);
     }
-    left -= e - (r.x + off);
-  }
-  for (int q = 0; q < 3; ++q) {
-    pv[q] = lane_group_sum(pv[q], lpt);
-    if (VS) ps[q] = lane_group_sum(ps[q], lpt);
-  }
-  if (valid && sub == 0) {
-    const double fi = 1.0 / (4.0 * kPi);
-    double *o = &A.part[((long)it * kWave + tl) * 6];
     for (int q = 0; q < 3; ++q) {
-      o[q] = pv[q] * fi;
-      o[3 + q] = ps[q] * fi;
+      pv[q] = lane_group_sum(pv[q], lpt);
+      if (VS) ps[q] = lane_group_sum(ps[q], lpt);
     }
-  }
+    if (valid && sub == 0) {
+      const double fi = 1.0 / (4.0 * kPi);
+      double *o = &A.part[((long)it * kWave + tl) * 6];
+      for (int q = 0; q < 3; ++q) {
+        o[q] = pv[q] * fi;
+        o[3 + q] = ps[q] * fi;
+      }
+    }
+    if (BATCH) gi = next_item();
+  } while (BATCH && gi < total);
 }
 
 static size_t m2l_lds(const WGeom &w) { return sizeof(float) * (size_t)w.ks * 4 * (kNT + 4); }
@@ -643,35 +719,42 @@ int tuned_pass_prepare(FmmHandle *h) {
   return 0;  // hcount is read after the caller's next synchronisation
 }
 
-// M2L of every level >= 2 (1-based) in one launch.
-int tuned_m2l_all(FmmHandle *h) {
-  TunedState *t = TS(h);
-  const WGeom w = wgeom(h);
-  size_t lds = m2l_lds(w);
-  if (lds > 64 * 1024)
-    DUST_HIP(hipFuncSetAttribute((const void *)k_m2l_mfma,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  // Jobs that read the same operator chunk (level, octant, offset chunk) are
-  // given to one XCD: workgroups are dealt to the 8 XCDs round-robin by
-  // index, so the job of workgroup i comes from queue i % 8, and every chunk
-  // is fetched into one L2 instead of eight.  Chunks go to the least loaded
-  // queue, largest first; short queues are padded with empty jobs.
-  constexpr int kXcd = 8;
-  const int nrb = ceil_div(w.rt, kRTB);
-  struct Chunk { int lvl, g, j0, njobs; };
-  std::vector<Chunk> chunks;
+// Chunks of M2L jobs: the targets of one (level, octant) group of case c
+// against the offsets [j0, j0 + m2l_chunk), njobs = target tiles x row blocks.
+struct M2LChunk {
+  int c, lvl, g, j0, njobs;
+};
+
+static void m2l_chunks(const FmmHandle *h, int c, std::vector<M2LChunk> &out) {
+  const TunedState *t = TS(h);
+  const int nrb = ceil_div(wgeom(h).rt, kRTB);
   for (int lvl = 1; lvl < h->nlev; ++lvl)
     for (int g = 0; g < 8; ++g) {
       int nt = t->hcount.p[lvl * 8 + g];
       if (nt == 0) continue;
       for (int j0 = 0; j0 < kOct; j0 += t->m2l_chunk)
-        chunks.push_back({lvl, g, j0, ceil_div(nt, kNT) * nrb});
+        out.push_back({c, lvl, g, j0, ceil_div(nt, kNT) * nrb});
     }
+}
+
+// The job list of the chunks (case c of a chunk is hs[c], recorded in
+// M2LJob.pad) in `jobs`, njob entries.  Jobs that read the same operator chunk
+// (case, level, octant, offset chunk) are given to one XCD: workgroups are
+// dealt to the 8 XCDs round-robin by index, so the job of workgroup i comes
+// from queue i % 8, and every chunk is fetched into one L2 instead of eight.
+// Chunks go to the least loaded queue, largest first; short queues are padded
+// with empty jobs.
+static int m2l_job_list(FmmHandle *const *hs, std::vector<M2LChunk> &chunks,
+                        HostBuf<M2LJob> &jobs, size_t *njob) {
+  constexpr int kXcd = 8;
   std::stable_sort(chunks.begin(), chunks.end(),
-                   [](const Chunk &a, const Chunk &b) { return a.njobs > b.njobs; });
+                   [](const M2LChunk &a, const M2LChunk &b) { return a.njobs > b.njobs; });
   std::vector<std::vector<M2LJob>> queue(kXcd);
   std::vector<long> load(kXcd, 0);
-  for (const Chunk &c : chunks) {
+  for (const M2LChunk &c : chunks) {
+    const FmmHandle *h = hs[c.c];
+    const TunedState *t = TS(h);
+    const int nrb = ceil_div(wgeom(h).rt, kRTB);
     int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     load[x] += c.njobs;
     const int nt = t->hcount.p[c.lvl * 8 + c.g];
@@ -682,28 +765,27 @@ int tuned_m2l_all(FmmHandle *h) {
         J.grp_off = t->gbase[c.lvl * 8 + c.g];
         J.kd_base = h->lev[c.lvl].kd_base;
         J.kdl0 = h->lev[c.lvl].kdl[0]; J.kdl1 = h->lev[c.lvl].kdl[1]; J.kdl2 = h->lev[c.lvl].kdl[2];
-        J.pad = 0;
+        J.pad = c.c;
         queue[x].push_back(J);
       }
   }
   size_t qmax = 0;
   for (auto &q : queue) qmax = std::max(qmax, q.size());
+  *njob = qmax * kXcd;
   if (qmax == 0) return 0;
   M2LJob empty{};
   empty.nt = 0;  // t0 >= nt: the job does nothing
-  // the host list stays in the tuned state until the next pass (whose first
-  // phase synchronises the stream), so the asynchronous upload needs no wait
-  const size_t njob = qmax * kXcd;
-  DUST_TRY(t->host_jobs.reserve(njob));
-  M2LJob *jobs = t->host_jobs.p;
-  std::fill(jobs, jobs + njob, empty);
+  DUST_TRY(jobs.reserve(*njob));
+  std::fill(jobs.p, jobs.p + *njob, empty);
   for (int x = 0; x < kXcd; ++x)
-    for (size_t k = 0; k < queue[x].size(); ++k) jobs[k * kXcd + x] = queue[x][k];
-  DUST_TRY(t->jobs.reserve(njob * sizeof(M2LJob)));
-  DUST_HIP(hipMemcpyAsync(t->jobs.p, jobs, njob * sizeof(M2LJob), hipMemcpyHostToDevice,
-                          h->stream));
+    for (size_t k = 0; k < queue[x].size(); ++k) jobs.p[k * kXcd + x] = queue[x][k];
+  return 0;
+}
+
+static M2LArgs m2l_args(FmmHandle *h) {
+  TunedState *t = TS(h);
   M2LArgs P;
-  P.w = w;
+  P.w = wgeom(h);
   P.nmon = h->nmon;
   P.jc = t->m2l_chunk;
   P.jobs = (const M2LJob *)t->jobs.p;
@@ -714,8 +796,157 @@ int tuned_m2l_all(FmmHandle *h) {
   P.a32 = t->a32.p;
   P.mp_b = h->mp_b.p;
   P.mp_c = h->mp_c.p;
+  return P;
+}
+
+static int m2l_lds_attr(const void *kernel, size_t lds) {
+  if (lds > 64 * 1024)
+    DUST_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  return 0;
+}
+
+// M2L of every level >= 2 (1-based) in one launch.
+int tuned_m2l_all(FmmHandle *h) {
+  TunedState *t = TS(h);
+  const size_t lds = m2l_lds(wgeom(h));
+  DUST_TRY(m2l_lds_attr((const void *)k_m2l_mfma, lds));
+  std::vector<M2LChunk> chunks;
+  m2l_chunks(h, 0, chunks);
+  // the host list stays in the tuned state until the next pass (whose first
+  // phase synchronises the stream), so the asynchronous upload needs no wait
+  size_t njob = 0;
+  FmmHandle *hs[1] = {h};
+  DUST_TRY(m2l_job_list(hs, chunks, t->host_jobs, &njob));
+  if (njob == 0) return 0;
+  DUST_TRY(t->jobs.reserve(njob * sizeof(M2LJob)));
+  DUST_HIP(hipMemcpyAsync(t->jobs.p, t->host_jobs.p, njob * sizeof(M2LJob),
+                          hipMemcpyHostToDevice, h->stream));
+  const M2LArgs P = m2l_args(h);
   k_m2l_mfma<<<(unsigned)njob, 256, lds, h->stream>>>(P);
   DUST_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Several tuned passes: their M2L jobs in one persistent launch and their P2P
+// items in another (the shared work lists of many independent cases).
+// ---------------------------------------------------------------------------
+struct BatchBufs {
+  DevBuf<int> cnt;  // [0..7] M2L queue counters, [8] P2P item counter
+  HostBuf<M2LJob> hjobs;
+  DevBuf<M2LJob> jobs;
+  HostBuf<M2LArgs> hm2l;
+  DevBuf<M2LArgs> m2l;
+  HostBuf<EvalArgs> heval;
+  DevBuf<EvalArgs> eval;
+  HostBuf<int> hoff;
+  DevBuf<int> off;
+};
+static BatchBufs g_batch;
+
+void tuned_batch_release() {
+  g_batch.cnt.release();
+  g_batch.hjobs.release();
+  g_batch.jobs.release();
+  g_batch.hm2l.release();
+  g_batch.m2l.release();
+  g_batch.heval.release();
+  g_batch.eval.release();
+  g_batch.hoff.release();
+  g_batch.off.release();
+}
+
+// A tuned pass joins the shared launches when its P2P runs in FP32 (no PSE).
+bool tuned_batchable(const FmmHandle *h) {
+  return h->tuned && !h->prm.use_vd && !h->prm.use_tv && h->n_active > 0;
+}
+
+// Workgroups of a persistent launch: as many as the CUs hold at once (by the
+// kernel's occupancy), a multiple of the 8 XCDs.
+static int persistent_grid(const void *kernel, int threads, size_t lds, unsigned *grid) {
+  int dev = 0, cus = 0, per_cu = 0;
+  DUST_HIP(hipGetDevice(&dev));
+  DUST_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  DUST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds));
+  const long g = (long)std::max(cus, 1) * std::max(per_cu, 1);
+  *grid = (unsigned)((g + 7) / 8 * 8);
+  return 0;
+}
+
+// hs: passes whose sort and multipole_phase_a are complete (the caller has
+// synchronised their streams), all tuned_batchable with the same use_vs and
+// particle core.  Launches on st: the M2L of all of them (then records
+// ev_m2l), then their P2P items (then ev_p2p).  The passes' own M2L and P2P
+// are skipped (FmmHandle.batched).  (The two launches on two streams, side by
+// side, measured no faster.)
+int tuned_batch_launch(FmmHandle *const *hs, int nh, hipStream_t st, hipEvent_t ev_m2l,
+                       hipEvent_t ev_p2p) {
+  BatchBufs &B = g_batch;
+  if (nh == 0) {
+    DUST_HIP(hipEventRecord(ev_m2l, st));
+    DUST_HIP(hipEventRecord(ev_p2p, st));
+    return 0;
+  }
+  // ---- M2L ----
+  std::vector<M2LChunk> chunks;
+  size_t lds = 0;
+  DUST_TRY(B.hm2l.reserve(nh));
+  for (int c = 0; c < nh; ++c) {
+    m2l_chunks(hs[c], c, chunks);
+    B.hm2l.p[c] = m2l_args(hs[c]);
+    lds = std::max(lds, m2l_lds(wgeom(hs[c])));
+  }
+  size_t njob = 0;
+  DUST_TRY(m2l_job_list(hs, chunks, B.hjobs, &njob));
+  DUST_TRY(B.cnt.reserve(9));
+  k_batch_clear<<<1, 64, 0, st>>>(B.cnt.p, 9);
+  if (njob > 0) {
+    const void *kern = (const void *)k_m2l_mfma_batch;
+    DUST_TRY(m2l_lds_attr(kern, lds));
+    DUST_TRY(B.jobs.reserve(njob));
+    DUST_TRY(B.m2l.reserve(nh));
+    DUST_HIP(hipMemcpyAsync(B.jobs.p, B.hjobs.p, njob * sizeof(M2LJob), hipMemcpyHostToDevice,
+                            st));
+    DUST_HIP(hipMemcpyAsync(B.m2l.p, B.hm2l.p, nh * sizeof(M2LArgs), hipMemcpyHostToDevice, st));
+    unsigned grid = 0;
+    DUST_TRY(persistent_grid(kern, 256, lds, &grid));
+    k_m2l_mfma_batch<<<grid, 256, lds, st>>>(B.jobs.p, B.m2l.p, (int)(njob / 8), B.cnt.p);
+    DUST_HIP(hipGetLastError());
+  }
+  DUST_HIP(hipEventRecord(ev_m2l, st));
+  // ---- P2P ----
+  DUST_TRY(B.heval.reserve(nh));
+  DUST_TRY(B.hoff.reserve(nh + 1));
+  long total = 0;
+  for (int c = 0; c < nh; ++c) {
+    EvalArgs &A = B.heval.p[c];
+    make_eval_args(hs[c], &A);
+    const double nf = TS(hs[c])->near_factor;
+    A.near2 = nf * nf;
+    B.hoff.p[c] = (int)total;
+    total += A.n_items;
+  }
+  DUST_CHECK(total < 0x7fffffffL, DUST_E_CAPACITY, "%ld P2P items in one batch", total);
+  B.hoff.p[nh] = (int)total;
+  if (total > 0) {
+    DUST_TRY(B.eval.reserve(nh));
+    DUST_TRY(B.off.reserve(nh + 1));
+    DUST_HIP(hipMemcpyAsync(B.eval.p, B.heval.p, nh * sizeof(EvalArgs), hipMemcpyHostToDevice,
+                            st));
+    DUST_HIP(hipMemcpyAsync(B.off.p, B.hoff.p, (nh + 1) * sizeof(int), hipMemcpyHostToDevice,
+                            st));
+    const bool vs = hs[0]->prm.use_vs, g = hs[0]->prm.kernel == DUST_KERNEL_GAUSSIAN;
+    using Kern = void (*)(EvalArgs, const EvalArgs *, const int *, int, int *);
+    const Kern kern = vs ? (g ? k_p2p_f32<1, 1, 1> : k_p2p_f32<1, 0, 1>)
+                         : (g ? k_p2p_f32<0, 1, 1> : k_p2p_f32<0, 0, 1>);
+    unsigned grid = 0;
+    DUST_TRY(persistent_grid((const void *)kern, kWave * kEvalWaves, 0, &grid));
+    grid = std::min(grid, (unsigned)ceil_div(total, kEvalWaves));
+    const EvalArgs &A0 = B.heval.p[0];  // not read by the batch instances
+    kern<<<grid, kWave * kEvalWaves, 0, st>>>(A0, B.eval.p, B.off.p, nh, B.cnt.p + 8);
+    DUST_HIP(hipGetLastError());
+  }
+  DUST_HIP(hipEventRecord(ev_p2p, st));
   return 0;
 }
 
@@ -727,11 +958,11 @@ int tuned_eval(FmmHandle *h, EvalArgs &A) {
     const dim3 grid(ceil_div(A.n_items, kEvalWaves)), block(kWave * kEvalWaves);
     const bool g = A.kernel == DUST_KERNEL_GAUSSIAN;
     if (A.use_vs) {
-      if (g) k_p2p_f32<1, 1><<<grid, block, 0, h->stream>>>(A);
-      else k_p2p_f32<1, 0><<<grid, block, 0, h->stream>>>(A);
+      if (g) k_p2p_f32<1, 1, 0><<<grid, block, 0, h->stream>>>(A, nullptr, nullptr, 0, nullptr);
+      else k_p2p_f32<1, 0, 0><<<grid, block, 0, h->stream>>>(A, nullptr, nullptr, 0, nullptr);
     } else {
-      if (g) k_p2p_f32<0, 1><<<grid, block, 0, h->stream>>>(A);
-      else k_p2p_f32<0, 0><<<grid, block, 0, h->stream>>>(A);
+      if (g) k_p2p_f32<0, 1, 0><<<grid, block, 0, h->stream>>>(A, nullptr, nullptr, 0, nullptr);
+      else k_p2p_f32<0, 0, 0><<<grid, block, 0, h->stream>>>(A, nullptr, nullptr, 0, nullptr);
     }
   }
   DUST_HIP(hipGetLastError());
