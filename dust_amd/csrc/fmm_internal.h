@@ -115,6 +115,8 @@ struct FmmHandle {
 int prim_reserve(FmmHandle *h);
 int prim_sort_pairs(FmmHandle *h, int n, int key_bits);
 int prim_select_flagged_cells(FmmHandle *h, int want_leaf_only, int *out, int *d_count);
+int prim_sort_pairs_u64(FmmHandle *h, const unsigned long long *kin, unsigned long long *kout,
+                        const int *vin, int *vout, int n, int end_bit);
 
 // fmm.hip: evaluation helpers shared with the tuned form
 struct EvalArgs;

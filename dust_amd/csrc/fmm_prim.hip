@@ -28,6 +28,19 @@ int prim_sort_pairs(FmmHandle *h, int n, int key_bits) {
   return 0;
 }
 
+// Stable LSD radix sort of (64-bit key, int) pairs over the low `end_bit`
+// bits, in the handle's primitive scratch.
+int prim_sort_pairs_u64(FmmHandle *h, const unsigned long long *kin, unsigned long long *kout,
+                        const int *vin, int *vout, int n, int end_bit) {
+  size_t bytes = 0;
+  DUST_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, n, 0, end_bit,
+                                              h->stream));
+  DUST_TRY(h->tmp.reserve(bytes));
+  DUST_HIP(hipcub::DeviceRadixSort::SortPairs(h->tmp.p, bytes, kin, kout, vin, vout, n, 0, end_bit,
+                                              h->stream));
+  return 0;
+}
+
 // Global ids of active cells (or of leaves), in ascending order.
 int prim_select_flagged_cells(FmmHandle *h, int leaf_only, int *out, int *d_count) {
   hipcub::CountingInputIterator<int> it(0);

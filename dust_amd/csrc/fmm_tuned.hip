@@ -41,6 +41,9 @@ struct TunedState {
   DevBuf<int> grp;          // target slots grouped by (level, octant)
   DevBuf<int> src;          // per grouped target: 189 source slots (-1 = none)
   DevBuf<long> pos;         // per active cell of a level: its grouped entry
+  DevBuf<unsigned long long> gkey;  // group sort keys of the targets, in and sorted
+  DevBuf<int> gidx;         // target numbers, in and sorted
+  DevBuf<int> gstart;       // [kMaxLevels * 8] first sorted target of every group
   DevBuf<int> gcount;       // [kMaxLevels * 8] group sizes (device)
   DevBuf<int> offs;         // [8][189][3] offsets of each octant
   DevBuf<float> W;          // M2L operators of every level and offset, fragment order
@@ -135,25 +138,71 @@ void tuned_release(FmmHandle *h) {
   h->tuned = nullptr;
 }
 
-// ---------------------------------------------------------------------------
-// Grouping of the targets of every level by octant, with their source slots
-// ---------------------------------------------------------------------------
-// Targets of level lvl grouped by octant: grp[e] = slot of the target, its
-// entry e recorded in pos[q] for k_group_src.
-__global__ void k_group_targets(TreeGeom G, int lvl, const int *__restrict__ active, int a0, int na,
-                                const int *__restrict__ slot, int *grp, long *pos, long gb0,
-                                long gstride, int *gcount) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= na) return;
-  long g = active[a0 + q];
-  long c = g - G.off[lvl];
+// Targets of an M2L group (level, octant) are ordered by the pattern of
+// active cells around them, so that the 16 targets of an MFMA column tile
+// tend to have sources at the same offsets (tiles with no source at an
+// offset are skipped).  Key of active cell a1 + q (levels >= 1): level
+// (bits 30..33), octant (27..29) and 27 bits, bit 9 px + 3 py + pz set when
+// an active source cell lies at one of the interaction offsets under the
+// parent neighbour (px - 1, py - 1, pz - 1) of the target's parent.  Counted
+// on the 1M-particle helix wake, issued / useful M2L work is 1.85 with the
+// targets in Morton order and 1.43 in this order.
+constexpr int kKeyBits = 34;
+__global__ void k_group_key(TreeGeom G, const int *__restrict__ active, int a1, int n,
+                            const unsigned char *__restrict__ flags, const int *__restrict__ offs,
+                            unsigned long long *key, int *idx, int *gcount) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const long g = active[a1 + q];
+  const int lvl = lvl_of(G, g);
+  const long c = g - G.off[lvl];
   int ix, iy, iz;
   cell_decode(G, lvl, c, ix, iy, iz);
-  int oc = (ix & 1) + 2 * (iy & 1) + 4 * (iz & 1);
-  int w = atomicAdd(&gcount[lvl * 8 + oc], 1);
-  long e = gb0 + oc * gstride + w;
-  grp[e] = slot[g];
-  pos[q] = e;
+  const int bx = ix & 1, by = iy & 1, bz = iz & 1, oc = bx + 2 * by + 4 * bz;
+  unsigned pat = 0;
+  for (int j = 0; j < kOct; ++j) {
+    const int *o = offs + (oc * kOct + j) * 3;
+    const int sx = ix + o[0], sy = iy + o[1], sz = iz + o[2];
+    if (sx < 0 || sy < 0 || sz < 0 || sx >= G.ncl[lvl][0] || sy >= G.ncl[lvl][1] ||
+        sz >= G.ncl[lvl][2])
+      continue;
+    if (!flags[G.off[lvl] + cell_encode(G, lvl, sx, sy, sz)]) continue;
+    // offset + octant bit in -2..3: parent neighbour index 0..2
+    const int px = (o[0] + bx + 2) >> 1, py = (o[1] + by + 2) >> 1, pz = (o[2] + bz + 2) >> 1;
+    pat |= 1u << (9 * px + 3 * py + pz);
+  }
+  key[q] = ((unsigned long long)lvl << 30) | ((unsigned long long)oc << 27) | pat;
+  idx[q] = q;
+  atomicAdd(&gcount[lvl * 8 + oc], 1);
+}
+
+// First sorted target of every group: exclusive prefix of the group sizes
+// in (level, octant) order, the order of the sorted keys.
+__global__ void k_group_starts(const int *__restrict__ gcount, int *gstart) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int s = 0;
+  for (int e = 0; e < kMaxLevels * 8; ++e) {
+    gstart[e] = s;
+    s += gcount[e];
+  }
+}
+
+struct GroupBase {
+  long b[kMaxLevels * 8];  // first entry of every group in grp
+};
+
+// Sorted target i: entry gbase + (i - first sorted target of its group).
+__global__ void k_group_place(int a1, int n, const int *__restrict__ active,
+                              const unsigned long long *__restrict__ key,
+                              const int *__restrict__ idx, const int *__restrict__ gstart,
+                              GroupBase gb, const int *__restrict__ slot, int *grp, long *pos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int e8 = (int)(key[i] >> 27);  // level * 8 + octant
+  const int q = idx[i];
+  const long e = gb.b[e8] + (i - gstart[e8]);
+  grp[e] = slot[active[a1 + q]];
+  pos[a1 + q] = e;
 }
 
 // The 189 source slots of every grouped target (-1 = no active cell there):
@@ -700,12 +749,28 @@ int tuned_pass_prepare(FmmHandle *h) {
   DUST_TRY(t->grp.reserve(std::max(tot, 1L)));
   DUST_TRY(t->src.reserve(std::max(tot, 1L) * kOct));
   DUST_TRY(t->pos.reserve(std::max(h->n_active, 1)));
+  // targets of every group in the order of their source patterns
+  const int a1 = h->nlev > 1 ? h->act_start[1] : h->n_active;
+  const int nt = h->n_active - a1;
+  if (nt > 0) {
+    DUST_TRY(t->gkey.reserve(2 * (size_t)nt));
+    DUST_TRY(t->gidx.reserve(2 * (size_t)nt));
+    DUST_TRY(t->gstart.reserve(kMaxLevels * 8));
+    k_group_key<<<ceil_div(nt, 256), 256, 0, h->stream>>>(G, h->active.p, a1, nt, h->flags.p,
+                                                          t->offs.p, t->gkey.p, t->gidx.p,
+                                                          t->gcount.p);
+    DUST_TRY(prim_sort_pairs_u64(h, t->gkey.p, t->gkey.p + nt, t->gidx.p, t->gidx.p + nt, nt,
+                                 kKeyBits));
+    k_group_starts<<<1, 64, 0, h->stream>>>(t->gcount.p, t->gstart.p);
+    GroupBase gb;
+    for (int e = 0; e < kMaxLevels * 8; ++e) gb.b[e] = e < h->nlev * 8 ? t->gbase[e] : 0;
+    k_group_place<<<ceil_div(nt, 256), 256, 0, h->stream>>>(a1, nt, h->active.p, t->gkey.p + nt,
+                                                            t->gidx.p + nt, t->gstart.p, gb,
+                                                            h->slot.p, t->grp.p, t->pos.p);
+  }
   for (int l = 1; l < h->nlev; ++l) {
     int na = h->act_start[l + 1] - h->act_start[l];
     if (na == 0) continue;
-    k_group_targets<<<ceil_div(na, 256), 256, 0, h->stream>>>(
-        G, l, h->active.p, h->act_start[l], na, h->slot.p, t->grp.p, t->pos.p + h->act_start[l],
-        t->gbase[l * 8], na, t->gcount.p);
     k_group_src<<<ceil_div((long)na * kOct, 256), 256, 0, h->stream>>>(
         G, l, h->active.p, h->act_start[l], na, h->slot.p, h->flags.p, t->offs.p,
         t->pos.p + h->act_start[l], t->src.p);
