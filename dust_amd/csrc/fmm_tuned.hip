@@ -141,16 +141,19 @@ void tuned_release(FmmHandle *h) {
 // Targets of an M2L group (level, octant) are ordered by the pattern of
 // active cells around them, so that the 16 targets of an MFMA column tile
 // tend to have sources at the same offsets (tiles with no source at an
-// offset are skipped).  Key of active cell a1 + q (levels >= 1): level
-// (bits 30..33), octant (27..29) and 27 bits, bit 9 px + 3 py + pz set when
-// an active source cell lies at one of the interaction offsets under the
-// parent neighbour (px - 1, py - 1, pz - 1) of the target's parent.  Counted
-// on the 1M-particle helix wake, issued / useful M2L work is 1.85 with the
+// offset are skipped).  The interaction offsets of a target are the
+// children of its parent's neighbours, and the children of a cell are all
+// active when it is a branch and all inactive otherwise, so the sources of
+// a target are fixed by which of the 27 cells around its parent are
+// branches.  Key of active cell a1 + q (levels >= 1): level (bits 30..33),
+// octant (27..29) and 27 bits, bit 9 (px + 1) + 3 (py + 1) + (pz + 1) set
+// when the cell (px, py, pz) away from the parent is a branch.  Counted on
+// the 1M-particle helix wake, issued / useful M2L work is 1.85 with the
 // targets in Morton order and 1.43 in this order.
 constexpr int kKeyBits = 34;
 __global__ void k_group_key(TreeGeom G, const int *__restrict__ active, int a1, int n,
-                            const unsigned char *__restrict__ flags, const int *__restrict__ offs,
-                            unsigned long long *key, int *idx, int *gcount) {
+                            const unsigned char *__restrict__ flags, unsigned long long *key,
+                            int *idx, int *gcount) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const long g = active[a1 + q];
@@ -158,19 +161,19 @@ __global__ void k_group_key(TreeGeom G, const int *__restrict__ active, int a1, 
   const long c = g - G.off[lvl];
   int ix, iy, iz;
   cell_decode(G, lvl, c, ix, iy, iz);
-  const int bx = ix & 1, by = iy & 1, bz = iz & 1, oc = bx + 2 * by + 4 * bz;
+  const int oc = (ix & 1) + 2 * (iy & 1) + 4 * (iz & 1);
+  const int pl = lvl - 1, qx = ix >> 1, qy = iy >> 1, qz = iz >> 1;
   unsigned pat = 0;
-  for (int j = 0; j < kOct; ++j) {
-    const int *o = offs + (oc * kOct + j) * 3;
-    const int sx = ix + o[0], sy = iy + o[1], sz = iz + o[2];
-    if (sx < 0 || sy < 0 || sz < 0 || sx >= G.ncl[lvl][0] || sy >= G.ncl[lvl][1] ||
-        sz >= G.ncl[lvl][2])
-      continue;
-    if (!flags[G.off[lvl] + cell_encode(G, lvl, sx, sy, sz)]) continue;
-    // offset + octant bit in -2..3: parent neighbour index 0..2
-    const int px = (o[0] + bx + 2) >> 1, py = (o[1] + by + 2) >> 1, pz = (o[2] + bz + 2) >> 1;
-    pat |= 1u << (9 * px + 3 * py + pz);
-  }
+  for (int px = -1; px <= 1; ++px)
+    for (int py = -1; py <= 1; ++py)
+      for (int pz = -1; pz <= 1; ++pz) {
+        const int sx = qx + px, sy = qy + py, sz = qz + pz;
+        if (sx < 0 || sy < 0 || sz < 0 || sx >= G.ncl[pl][0] || sy >= G.ncl[pl][1] ||
+            sz >= G.ncl[pl][2])
+          continue;
+        if (flags[G.off[pl] + cell_encode(G, pl, sx, sy, sz)] == kBranch)
+          pat |= 1u << (9 * (px + 1) + 3 * (py + 1) + (pz + 1));
+      }
   key[q] = ((unsigned long long)lvl << 30) | ((unsigned long long)oc << 27) | pat;
   idx[q] = q;
   atomicAdd(&gcount[lvl * 8 + oc], 1);
@@ -757,8 +760,7 @@ int tuned_pass_prepare(FmmHandle *h) {
     DUST_TRY(t->gidx.reserve(2 * (size_t)nt));
     DUST_TRY(t->gstart.reserve(kMaxLevels * 8));
     k_group_key<<<ceil_div(nt, 256), 256, 0, h->stream>>>(G, h->active.p, a1, nt, h->flags.p,
-                                                          t->offs.p, t->gkey.p, t->gidx.p,
-                                                          t->gcount.p);
+                                                          t->gkey.p, t->gidx.p, t->gcount.p);
     DUST_TRY(prim_sort_pairs_u64(h, t->gkey.p, t->gkey.p + nt, t->gidx.p, t->gidx.p + nt, nt,
                                  kKeyBits));
     k_group_starts<<<1, 64, 0, h->stream>>>(t->gcount.p, t->gstart.p);
