@@ -222,8 +222,9 @@ def test_morton_order_path_matches_input_order(wing):
 
 def test_tuned_far_field_vs_oracle(orc, wing):
     """precision = tuned: far-field element velocities in FP32 (FP64 offsets,
-    switches and sums), near field FP64: within 1e-6 of the oracle, and the
-    near-field-only case (every pair near) equal to the FP64 form."""
+    switches and sums), near field with FP64 geometry and sums and FP32
+    logarithms and arctangents: within 1e-6 of the oracle, also when every
+    pair is near (the near field alone against the oracle and the FP64 form)."""
     groups, pts, al = wing
     print(f"elem_apply tuned n={len(pts)}", flush=True)
     vel, _ = E.apply_elements(groups, pts, precision="tuned")
@@ -232,4 +233,7 @@ def test_tuned_far_field_vs_oracle(orc, wing):
     assert rel(vel, ov) <= 1e-6
     vn, _ = E.apply_elements(groups, pts, precision="tuned", ff_ratio_doublet=1e9)
     v64, _ = E.apply_elements(groups, pts, ff_ratio_doublet=1e9)
-    assert np.array_equal(vn, v64)
+    on, _ = orc.elem_apply(oracle_groups(groups), pts, ff_ratio=1e9)
+    print(f"near only: tuned vs fp64 {rel(vn, v64):.2e}, fp64 vs oracle {rel(v64, on):.2e}")
+    assert rel(v64, on) <= TOL
+    assert 0 < rel(vn, v64) <= 1e-6
