@@ -861,7 +861,8 @@ def main():
             out["wing_run"] = time_wing_run(torch, W, 200, with_cpu=not args.no_cpu)
         if not args.no_rotor:
             out["rotor_run"] = time_rotor_run(torch, W, n)
-        print(json.dumps(out, default=float))
+        print(json.dumps(out, default=float), flush=True)
+        L.check(L.lib().dust_shutdown(), "shutdown")  # before the runtime's teardown
         return
     log(f"generating helix wake n={n}")
     cen, alpha, rv, tree = W.helix_wake(n)
