@@ -159,6 +159,10 @@ SIGNATURES = [
     ("dust_fmm_set_particles", C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
     ("dust_fmm_set_particles_dev", C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
     ("dust_fmm_sort_particles", C.c_int, [_vp, _ip]),
+    ("dust_fmm_set_redistribution", C.c_int,
+     [_vp, C.c_int, C.c_int, C.c_double, C.c_int, _vp]),
+    ("dust_fmm_redistributed", C.c_int, [_vp, _ip, _vp, _vp, _vp]),
+    ("dust_fmm_redistributed_dev", C.c_int, [_vp, _vp, _vp, _vp]),
     ("dust_fmm_calculate_multipole", C.c_int, [_vp]),
     ("dust_fmm_apply_multipole", C.c_int, [_vp, _vp, _vp, _vp]),
     ("dust_fmm_apply_multipole_dev", C.c_int, [_vp]),

@@ -160,16 +160,151 @@ __global__ void k_gather(int n, const int *__restrict__ perm, const double *__re
   pr[i] = rv[q];
 }
 
-// fstart[c] = first sorted particle with key >= c, for c = 0..nfine.
-__global__ void k_fstart(long nfine, int n, const unsigned *__restrict__ ks, int *fstart) {
+// fstart[c] = first sorted particle with key >= c, for c = 0..nfine (the
+// particle count read from n_dev when given: after a redistribution).
+__global__ void k_fstart(long nfine, int n, const int *__restrict__ n_dev,
+                         const unsigned *__restrict__ ks, int *fstart) {
   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c > nfine) return;
+  if (n_dev) n = *n_dev;
   int lo = 0, hi = n;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
     if ((long)ks[mid] < c) lo = mid + 1; else hi = mid;
   }
   fstart[c] = lo;
+}
+
+// ---------------------------------------------------------------------------
+// Particle redistribution (sort_particles, mod_octree.f90:827-894)
+// ---------------------------------------------------------------------------
+
+// Element centres binned with the particles' ceiling rule at the finest level
+// (mod_octree.f90:777-800; centres outside the box are skipped, the warning
+// branch of 784-795) and marked on their ancestor at level ls (the upward
+// panel counts of 801-817, of which only "> 0" is read).
+__global__ void k_pr_bin(TreeGeom G, int m, const double *__restrict__ cen, int ls,
+                         unsigned char *has) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  int L = G.nlev - 1;
+  int id[3];
+  for (int d = 0; d < 3; ++d) {
+    double q = ceil((cen[3 * e + d] - G.xmin[d]) / G.csize_fine);
+    if (!(q >= 1.0 && q <= (double)G.ncl[L][d])) return;
+    id[d] = (int)q - 1;
+  }
+  has[cell_encode(G, L, id[0], id[1], id[2]) >> (3 * (L - ls))] = 1;
+}
+
+// near_wall of the level-ls cells: an element in the cell or in one of its
+// neighbours inside the grid (mod_octree.f90:829-843).
+__global__ void k_pr_near(TreeGeom G, int ls, long nc, const unsigned char *__restrict__ has,
+                          unsigned char *near) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  int ix, iy, iz;
+  cell_decode(G, ls, c, ix, iy, iz);
+  unsigned char v = 0;
+  for (int kc = -1; kc <= 1; ++kc)
+    for (int jc = -1; jc <= 1; ++jc)
+      for (int ic = -1; ic <= 1; ++ic) {
+        int a = ix + ic, b = iy + jc, d = iz + kc;
+        if (a < 0 || b < 0 || d < 0 || a >= G.ncl[ls][0] || b >= G.ncl[ls][1] ||
+            d >= G.ncl[ls][2])
+          continue;
+        v |= has[cell_encode(G, ls, a, b, d)];
+      }
+  near[c] = v;
+}
+
+// The redistribution loop of one finest cell (mod_octree.f90:845-891), one
+// thread per cell over its sorted particles, which keep the push order of the
+// reference's lists.  Walking the list, a particle with
+// mag < (1/ratio) * (sum of the cell's mag) / npart is freed; redistr =
+// (mag * dir) / nprt, nprt counting the freed particle, is added to each
+// other particle still in the cell, whose mag = norm2(vort) and
+// dir = vort / norm2(vort) (0 for a zero vector) are recomputed.  Pulling the
+// freed particle from the list leaves the next one at the same position, so
+// one ordered pass over the positions visits the particles as the reference's
+// while-loop does.  Unfused FP64 arithmetic, norm2 = sqrt(x*x + y*y + z*z).
+__global__ void k_pr_cells(TreeGeom G, int ls, const int *__restrict__ fstart,
+                           const unsigned char *__restrict__ near, double inv_ratio, double *pm,
+                           double *dx, double *dy, double *dz, unsigned char *keep) {
+#pragma clang fp contract(off)
+  int L = G.nlev - 1;
+  long nc = (long)G.ncl[L][0] * G.ncl[L][1] * G.ncl[L][2];
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const int s = fstart[c], e = fstart[c + 1];
+  for (int t = s; t < e; ++t) keep[t] = 1;
+  if (e == s || near[c >> (3 * (L - ls))]) return;
+  double ave = 0.0;
+  for (int t = s; t < e; ++t) ave += pm[t];
+  ave = ave / (double)(e - s);
+  const double thr = inv_ratio * ave;
+  int cnt = e - s;
+  for (int t = s; t < e; ++t) {
+    const double mt = pm[t];
+    if (!(mt < thr)) continue;
+    const double rx = (mt * dx[t]) / (double)cnt;
+    const double ry = (mt * dy[t]) / (double)cnt;
+    const double rz = (mt * dz[t]) / (double)cnt;
+    keep[t] = 0;
+    --cnt;
+    for (int q = s; q < e; ++q) {
+      if (!keep[q]) continue;
+      const double mq = pm[q];
+      const double vx = mq * dx[q] + rx, vy = mq * dy[q] + ry, vz = mq * dz[q] + rz;
+      const double vn = sqrt(vx * vx + vy * vy + vz * vz);
+      pm[q] = vn;
+      dx[q] = vn > 0.0 ? vx / vn : 0.0;
+      dy[q] = vn > 0.0 ? vy / vn : 0.0;
+      dz[q] = vn > 0.0 ? vz / vn : 0.0;
+    }
+  }
+}
+
+// Outcome of the redistribution in input order: kept flag and strengths.
+__global__ void k_pr_out(int n, const int *__restrict__ perm, const unsigned char *__restrict__ ks,
+                         const double *__restrict__ pm, const double *__restrict__ dx,
+                         const double *__restrict__ dy, const double *__restrict__ dz,
+                         unsigned char *keep, double *dir, double *mag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int q = perm[i];
+  keep[q] = ks[i];
+  mag[q] = pm[i];
+  dir[3 * q] = dx[i];
+  dir[3 * q + 1] = dy[i];
+  dir[3 * q + 2] = dz[i];
+}
+
+// The kept particles moved to the front of the sorted arrays, in order
+// (sel = their sorted positions, *ns of them).
+struct PrArrays {
+  const double *src[8];
+  double *dst[8];
+};
+__global__ void k_pr_compact(int n, const int *__restrict__ ns, const int *__restrict__ sel,
+                             PrArrays A, const unsigned *__restrict__ ks, unsigned *ks_out,
+                             const int *__restrict__ perm, int *perm_out) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n || j >= *ns) return;
+  int i = sel[j];
+  for (int a = 0; a < 8; ++a) A.dst[a][j] = A.src[a][i];
+  ks_out[j] = ks[i];
+  perm_out[j] = perm[i];
+}
+
+// Outputs of the freed particles (no evaluation) set to zero.
+__global__ void k_pr_zero(int n, const unsigned char *__restrict__ keep, double *v, double *s,
+                          double *tv) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n || keep[q]) return;
+  v[3 * q] = v[3 * q + 1] = v[3 * q + 2] = 0.0;
+  s[3 * q] = s[3 * q + 1] = s[3 * q + 2] = 0.0;
+  tv[q] = 0.0;
 }
 
 // Finest level: leaf iff npart >= min_part, or the tree has one level
@@ -208,6 +343,11 @@ __global__ void k_flags_level(TreeGeom G, int lvl, const int *__restrict__ fstar
 }
 
 __global__ void k_fill_int(long n, int *p, int v) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u8(long n, unsigned char *p, unsigned char v) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
@@ -1113,6 +1253,14 @@ static void fmm_destroy(FmmHandle *h) {
   h->flags.release();
   h->p2p_rng.release();
   h->rb.release();
+  DevBuf<double> *prd[] = {&h->pr_cen, &h->pr_dir, &h->pr_mag};
+  for (auto b : prd) b->release();
+  for (auto &b : h->pr_stage) b.release();
+  DevBuf<unsigned char> *pru[] = {&h->pr_has, &h->pr_near, &h->pr_keep_s, &h->pr_keep};
+  for (auto b : pru) b->release();
+  DevBuf<int> *pri[] = {&h->pr_sel, &h->pr_ns, &h->pr_perm};
+  for (auto b : pri) b->release();
+  h->pr_key.release();
   for (auto e : h->timer.pool) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -1132,6 +1280,58 @@ static int fmm_set_particles(FmmHandle *h, int n, const double *cen, const doubl
   DUST_HIP(hipMemcpyAsync(h->mag_in.p, mag, sizeof(double) * n, kind, h->stream));
   DUST_HIP(hipMemcpyAsync(h->rv_in.p, rv, sizeof(double) * n, kind, h->stream));
   if (kind == hipMemcpyHostToDevice) DUST_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// The redistribution block of sort_particles (mod_octree.f90:827-894) on the
+// sorted particles, with fstart over all of them: near-wall marks, the
+// per-cell loop, the outcome in input order, then the kept particles
+// compacted to the front of the sorted arrays (their order, hence the push
+// order of every cell, unchanged) and fstart recomputed over them.  The kept
+// count stays on the device (pr_ns) until the phase's read-back.
+static int redistribute(FmmHandle *h) {
+  const int n = h->n;
+  TreeGeom G = make_geom(h);
+  const int TB = 256;
+  const int ls = std::min(std::max(h->pr_lvl, 0), h->nlev - 1);
+  const long nls = h->lev[ls].ncells;
+  DUST_TRY(h->pr_has.reserve(nls));
+  DUST_TRY(h->pr_near.reserve(nls));
+  DUST_TRY(h->pr_keep_s.reserve(h->cap));
+  DUST_TRY(h->pr_keep.reserve(h->cap));
+  DUST_TRY(h->pr_dir.reserve((size_t)3 * h->cap));
+  DUST_TRY(h->pr_mag.reserve(h->cap));
+  DUST_TRY(h->pr_sel.reserve(h->cap));
+  DUST_TRY(h->pr_ns.reserve(1));
+  DUST_TRY(h->pr_perm.reserve(h->cap));
+  DUST_TRY(h->pr_key.reserve(h->cap));
+  for (auto &b : h->pr_stage) DUST_TRY(b.reserve(h->cap));
+  k_fill_u8<<<ceil_div(nls, TB), TB, 0, h->stream>>>(nls, h->pr_has.p, 0);
+  if (h->pr_m > 0)
+    k_pr_bin<<<ceil_div(h->pr_m, TB), TB, 0, h->stream>>>(G, h->pr_m, h->pr_cen.p, ls,
+                                                          h->pr_has.p);
+  k_pr_near<<<ceil_div(nls, TB), TB, 0, h->stream>>>(G, ls, nls, h->pr_has.p, h->pr_near.p);
+  k_pr_cells<<<ceil_div(h->nfine, TB), TB, 0, h->stream>>>(G, ls, h->fstart.p, h->pr_near.p,
+                                                           1.0 / h->pr_ratio, h->pm.p, h->dx.p,
+                                                           h->dy.p, h->dz.p, h->pr_keep_s.p);
+  k_pr_out<<<ceil_div(n, TB), TB, 0, h->stream>>>(n, h->perm.p, h->pr_keep_s.p, h->pm.p, h->dx.p,
+                                                  h->dy.p, h->dz.p, h->pr_keep.p, h->pr_dir.p,
+                                                  h->pr_mag.p);
+  DUST_TRY(prim_select_kept(h, h->pr_keep_s.p, n, h->pr_sel.p, h->pr_ns.p));
+  DevBuf<double> *arr[8] = {&h->px, &h->py, &h->pz, &h->dx, &h->dy, &h->dz, &h->pm, &h->pr};
+  PrArrays A;
+  for (int a = 0; a < 8; ++a) {
+    A.src[a] = arr[a]->p;
+    A.dst[a] = h->pr_stage[a].p;
+  }
+  k_pr_compact<<<ceil_div(n, TB), TB, 0, h->stream>>>(n, h->pr_ns.p, h->pr_sel.p, A, h->key_s.p,
+                                                      h->pr_key.p, h->perm.p, h->pr_perm.p);
+  for (int a = 0; a < 8; ++a) std::swap(*arr[a], h->pr_stage[a]);
+  std::swap(h->key_s, h->pr_key);
+  std::swap(h->perm, h->pr_perm);
+  k_fstart<<<ceil_div(h->nfine + 1, TB), TB, 0, h->stream>>>(h->nfine, n, h->pr_ns.p, h->key_s.p,
+                                                             h->fstart.p);
+  h->pr_done = true;
   return 0;
 }
 
@@ -1161,8 +1361,10 @@ static int sort_phase_a(FmmHandle *h) {
                                                     h->pz.p, h->dx.p, h->dy.p, h->dz.p, h->pm.p,
                                                     h->pr.p);
   }
-  k_fstart<<<ceil_div(h->nfine + 1, TB), TB, 0, h->stream>>>(h->nfine, n, h->key_s.p,
+  k_fstart<<<ceil_div(h->nfine + 1, TB), TB, 0, h->stream>>>(h->nfine, n, nullptr, h->key_s.p,
                                                              h->fstart.p);
+  h->pr_done = false;
+  if (n > 0 && h->pr_on) DUST_TRY(redistribute(h));
   k_flags_fine<<<ceil_div(h->nfine, TB), TB, 0, h->stream>>>(G, h->fstart.p, h->prm.min_part,
                                                              h->flags.p);
   for (int l = h->nlev - 2; l >= 0; --l)
@@ -1173,6 +1375,8 @@ static int sort_phase_a(FmmHandle *h) {
   DUST_HIP(hipMemcpyAsync(rb->counts, h->counts.p, sizeof(int) * 2, hipMemcpyDeviceToHost,
                           h->stream));
   DUST_HIP(hipMemcpyAsync(&rb->err, h->d_err.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  if (h->pr_done)
+    DUST_HIP(hipMemcpyAsync(&rb->ns, h->pr_ns.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   return 0;
 }
 
@@ -1188,6 +1392,7 @@ static int sort_phase_b(FmmHandle *h) {
   }
   h->n_active = rb->counts[0];
   h->n_leaves = rb->counts[1];
+  h->ns = h->pr_done ? rb->ns : n;
   k_level_starts<<<1, 64, 0, h->stream>>>(G, h->n_active, h->active.p, h->counts.p + 2);
   DUST_HIP(hipMemcpyAsync(rb->act_start, h->counts.p + 2, sizeof(int) * (h->nlev + 1),
                           hipMemcpyDeviceToHost, h->stream));
@@ -1387,6 +1592,10 @@ static int fmm_apply(FmmHandle *h) {
   DUST_CHECK(h && h->computed, DUST_E_ARG, "calculate_multipole must precede apply_multipole");
   const int n = h->n;
   if (n == 0) return 0;
+  if (h->ns < n)  // freed by the redistribution: not evaluated
+    k_pr_zero<<<ceil_div(n, 256), 256, 0, h->stream>>>(n, h->pr_keep.p, h->vel.p, h->str.p,
+                                                       h->tv.p);
+  if (h->ns == 0) return 0;
   EvalArgs A;
   make_eval_args(h, &A);
   timer_begin(h, DUST_FMM_STAGE_P2P);
@@ -1402,8 +1611,8 @@ static int fmm_apply(FmmHandle *h) {
   timer_begin(h, DUST_FMM_STAGE_L2P);
   DUST_TRY(eval_finish_launch(h, A));
   timer_end(h, DUST_FMM_STAGE_L2P);
-  k_scatter<<<ceil_div(n, 256), 256, 0, h->stream>>>(n, h->perm.p, h->vel_s.p, h->str_s.p,
-                                                     h->tv_s.p, h->vel.p, h->str.p, h->tv.p);
+  k_scatter<<<ceil_div(h->ns, 256), 256, 0, h->stream>>>(h->ns, h->perm.p, h->vel_s.p, h->str_s.p,
+                                                         h->tv_s.p, h->vel.p, h->str.p, h->tv.p);
   return 0;
 }
 
@@ -1534,7 +1743,7 @@ static int fmm_apply_panels(FmmHandle *h, int m, const double *cen, double *uvor
   DUST_HIP(hipStreamSynchronize(h->stream));
   DUST_CHECK(h->rb.p->err == 0x7fffffff, DUST_E_OUTSIDE,
              "panel centre %d outside the octree box", h->rb.p->err);
-  if (h->n == 0 || h->n_leaves == 0) return 0;
+  if (h->ns == 0 || h->n_leaves == 0) return 0;
   EvalArgs A;
   make_eval_args(h, &A);
   k_pan_eval<<<ceil_div(m, 4), 256, 0, h->stream>>>(A, m, cen, pleaf.p, uvort);
@@ -1603,6 +1812,68 @@ int dust_fmm_set_particles_dev(dust_fmm_t h, int n, const double *cen, const dou
 int dust_fmm_sort_particles(dust_fmm_t h, int *nleaves) {
   DUST_CHECK(h, DUST_E_ARG, "null handle");
   return fmm_sort(h->h, nleaves);
+}
+
+int dust_fmm_set_redistribution(dust_fmm_t hh, int on, int lvl_solid, double ratio, int m,
+                                const double *elem_cen) {
+  DUST_CHECK(hh, DUST_E_ARG, "null handle");
+  FmmHandle *h = hh->h;
+  if (!on) {
+    h->pr_on = 0;
+    h->pr_m = 0;
+    return 0;
+  }
+  DUST_CHECK(lvl_solid >= 1 && lvl_solid <= h->nlev, DUST_E_ARG,
+             "octree_level_solid %d outside 1..%d", lvl_solid, h->nlev);
+  DUST_CHECK(ratio > 0.0, DUST_E_ARG, "particles_redistribution_ratio must be positive");
+  DUST_CHECK(m >= 0 && (m == 0 || elem_cen), DUST_E_ARG, "missing element centres");
+  DUST_TRY(h->pr_cen.reserve((size_t)3 * std::max(m, 1)));
+  if (m > 0) {
+    DUST_HIP(hipMemcpyAsync(h->pr_cen.p, elem_cen, sizeof(double) * 3 * m, hipMemcpyDefault,
+                            h->stream));
+    DUST_HIP(hipStreamSynchronize(h->stream));
+  }
+  h->pr_on = 1;
+  h->pr_lvl = lvl_solid - 1;
+  h->pr_ratio = ratio;
+  h->pr_m = m;
+  return 0;
+}
+
+static int redistributed_copy(FmmHandle *h, unsigned char *keep, double *dir, double *mag,
+                              hipMemcpyKind kind) {
+  DUST_CHECK(h->sorted, DUST_E_ARG, "sort_particles must be called first");
+  const int n = h->n;
+  if (n == 0) return 0;
+  if (h->pr_done) {
+    if (keep) DUST_HIP(hipMemcpyAsync(keep, h->pr_keep.p, n, kind, h->stream));
+    if (dir) DUST_HIP(hipMemcpyAsync(dir, h->pr_dir.p, sizeof(double) * 3 * n, kind, h->stream));
+    if (mag) DUST_HIP(hipMemcpyAsync(mag, h->pr_mag.p, sizeof(double) * n, kind, h->stream));
+  } else {
+    // no redistribution in the last sort: every particle kept, as given
+    if (keep) {
+      if (kind == hipMemcpyDeviceToHost) std::memset(keep, 1, n);
+      else k_fill_u8<<<ceil_div(n, 256), 256, 0, h->stream>>>(n, keep, 1);
+    }
+    if (dir) DUST_HIP(hipMemcpyAsync(dir, h->dir_in.p, sizeof(double) * 3 * n, kind, h->stream));
+    if (mag) DUST_HIP(hipMemcpyAsync(mag, h->mag_in.p, sizeof(double) * n, kind, h->stream));
+  }
+  return 0;
+}
+
+int dust_fmm_redistributed(dust_fmm_t hh, int *n_kept, unsigned char *keep, double *dir,
+                           double *mag) {
+  DUST_CHECK(hh, DUST_E_ARG, "null handle");
+  FmmHandle *h = hh->h;
+  DUST_TRY(redistributed_copy(h, keep, dir, mag, hipMemcpyDeviceToHost));
+  DUST_HIP(hipStreamSynchronize(h->stream));
+  if (n_kept) *n_kept = h->ns;
+  return 0;
+}
+
+int dust_fmm_redistributed_dev(dust_fmm_t hh, unsigned char *keep, double *dir, double *mag) {
+  DUST_CHECK(hh, DUST_E_ARG, "null handle");
+  return redistributed_copy(hh->h, keep, dir, mag, hipMemcpyDeviceToDevice);
 }
 
 int dust_fmm_calculate_multipole(dust_fmm_t h) {
@@ -1876,11 +2147,11 @@ int dust_fmm_leaves(dust_fmm_t hh, int *level, int *ijk, int *npart, int *member
   FmmHandle *h = hh->h;
   DUST_CHECK(h->sorted, DUST_E_ARG, "sort_particles must be called first");
   TreeGeom G = make_geom(h);
-  std::vector<int> lv(h->n_leaves), fs(h->nfine + 1), perm(h->n);
+  std::vector<int> lv(h->n_leaves), fs(h->nfine + 1), perm(h->ns);
   DUST_HIP(hipMemcpy(lv.data(), h->leaves.p, sizeof(int) * h->n_leaves, hipMemcpyDeviceToHost));
   DUST_HIP(hipMemcpy(fs.data(), h->fstart.p, sizeof(int) * (h->nfine + 1), hipMemcpyDeviceToHost));
-  if (h->n > 0)
-    DUST_HIP(hipMemcpy(perm.data(), h->perm.p, sizeof(int) * h->n, hipMemcpyDeviceToHost));
+  if (h->ns > 0)
+    DUST_HIP(hipMemcpy(perm.data(), h->perm.p, sizeof(int) * h->ns, hipMemcpyDeviceToHost));
   int w = 0;
   for (int q = 0; q < h->n_leaves; ++q) {
     long g = lv[q];
@@ -1905,12 +2176,14 @@ int dust_fmm_cell_index(dust_fmm_t hh, int *ijk) {
   FmmHandle *h = hh->h;
   DUST_CHECK(h->sorted, DUST_E_ARG, "sort_particles must be called first");
   TreeGeom G = make_geom(h);
-  std::vector<unsigned> ks(h->n);
-  std::vector<int> perm(h->n);
+  std::vector<unsigned> ks(h->ns);
+  std::vector<int> perm(h->ns);
   if (h->n == 0) return 0;
-  DUST_HIP(hipMemcpy(ks.data(), h->key_s.p, sizeof(unsigned) * h->n, hipMemcpyDeviceToHost));
-  DUST_HIP(hipMemcpy(perm.data(), h->perm.p, sizeof(int) * h->n, hipMemcpyDeviceToHost));
-  for (int t = 0; t < h->n; ++t) {
+  std::fill(ijk, ijk + 3 * (size_t)h->n, 0);  // particles freed by a redistribution
+  if (h->ns == 0) return 0;
+  DUST_HIP(hipMemcpy(ks.data(), h->key_s.p, sizeof(unsigned) * h->ns, hipMemcpyDeviceToHost));
+  DUST_HIP(hipMemcpy(perm.data(), h->perm.p, sizeof(int) * h->ns, hipMemcpyDeviceToHost));
+  for (int t = 0; t < h->ns; ++t) {
     int ix, iy, iz;
     cell_decode(G, h->nlev - 1, (long)ks[t], ix, iy, iz);
     int q = perm[t];

@@ -45,6 +45,7 @@ struct FmmReadback {
   int counts[2];                  // active cells, leaves
   int units[2];                   // evaluation units, items
   int act_start[kMaxLevels + 1];  // first active cell of every level
+  int ns;                         // particles kept by a redistribution
 };
 
 struct FmmHandle {
@@ -109,12 +110,27 @@ struct FmmHandle {
   bool batched = false;
   // page-locked source/targets of the asynchronous copies between pass phases
   HostBuf<FmmReadback> rb;
+  // particle redistribution of sort_particles (mod_octree.f90:827-894):
+  // settings, element centres, near-wall marks of level pr_lvl (0-based),
+  // the kept flags in sorted and input order, the strengths after it in input
+  // order, and the staging of the compaction of the sorted arrays
+  int pr_on = 0, pr_lvl = 0, pr_m = 0;
+  double pr_ratio = 3.0;
+  DevBuf<double> pr_cen;
+  DevBuf<unsigned char> pr_has, pr_near, pr_keep_s, pr_keep;
+  DevBuf<double> pr_dir, pr_mag;
+  DevBuf<int> pr_sel, pr_ns, pr_perm;
+  DevBuf<unsigned> pr_key;
+  DevBuf<double> pr_stage[8];
+  bool pr_done = false;  // the last sort ran the redistribution
+  int ns = 0;            // particles in the tree after the last sort
 };
 
 // fmm_prim.hip: device-wide primitives (radix sort, selection).
 int prim_reserve(FmmHandle *h);
 int prim_sort_pairs(FmmHandle *h, int n, int key_bits);
 int prim_select_flagged_cells(FmmHandle *h, int want_leaf_only, int *out, int *d_count);
+int prim_select_kept(FmmHandle *h, const unsigned char *keep, int n, int *out, int *d_count);
 int prim_sort_pairs_u64(FmmHandle *h, const unsigned long long *kin, unsigned long long *kout,
                         const int *vin, int *vout, int n, int end_bit);
 

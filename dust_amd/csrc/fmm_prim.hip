@@ -53,6 +53,16 @@ int prim_select_flagged_cells(FmmHandle *h, int leaf_only, int *out, int *d_coun
   return 0;
 }
 
+// Sorted positions of the particles a redistribution kept, in order.
+int prim_select_kept(FmmHandle *h, const unsigned char *keep, int n, int *out, int *d_count) {
+  hipcub::CountingInputIterator<int> it(0);
+  size_t bytes = 0;
+  DUST_HIP(hipcub::DeviceSelect::Flagged(nullptr, bytes, it, keep, out, d_count, n, h->stream));
+  DUST_TRY(h->tmp.reserve(bytes));
+  DUST_HIP(hipcub::DeviceSelect::Flagged(h->tmp.p, bytes, it, keep, out, d_count, n, h->stream));
+  return 0;
+}
+
 }  // namespace dust
 
 namespace dust {

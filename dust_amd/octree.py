@@ -30,14 +30,19 @@ class VortParts:
     """Vortex particles (t_vortpart fields used by the FMM)."""
 
     def __init__(self, cen, dir, mag, r_vortex):
-        self.cen = np.ascontiguousarray(cen, dtype=np.float64).reshape(-1, 3)
+        # own copies: sort_particles writes the redistributed strengths back
+        self.cen = np.array(cen, dtype=np.float64, order="C").reshape(-1, 3)
         n = self.cen.shape[0]
-        self.dir = np.ascontiguousarray(dir, dtype=np.float64).reshape(n, 3)
-        self.mag = np.ascontiguousarray(np.broadcast_to(mag, (n,)), dtype=np.float64)
-        self.r_vortex = np.ascontiguousarray(np.broadcast_to(r_vortex, (n,)), dtype=np.float64)
+        self.dir = np.array(dir, dtype=np.float64, order="C").reshape(n, 3)
+        self.mag = np.array(np.broadcast_to(mag, (n,)), dtype=np.float64, order="C")
+        self.r_vortex = np.array(np.broadcast_to(r_vortex, (n,)), dtype=np.float64, order="C")
         self.vel = np.zeros((n, 3))
         self.stretch = np.zeros((n, 3))
         self.turbvisc = np.zeros(n)
+        # particles freed by a redistribution in sort_particles, and the
+        # count of the others (the reference's n_prt)
+        self.free = np.zeros(n, bool)
+        self.n_prt = n
 
     @classmethod
     def from_alpha(cls, cen, alpha, r_vortex):
@@ -66,6 +71,11 @@ class Octree:
         L.check(L.lib().dust_fmm_create(C.byref(params), capacity, C.byref(h)), "initialize_octree")
         self._h = h
         self.n_leaves = 0
+        # particle redistribution switches (sim_param use_pr, lvl_solid,
+        # part_redist_ratio; mod_sim_param.f90:792-800)
+        self.use_pr = False
+        self.lvl_solid = max(params.nlevels - 2, 1)
+        self.part_redist_ratio = 3.0
 
     @property
     def handle(self):
@@ -175,20 +185,49 @@ KERNELS = {"rosenhead": L.KERNEL_ROSENHEAD, "gaussian": L.KERNEL_GAUSSIAN}
 
 
 def initialize_octree(box_length, nbox, origin, nlevels, min_part, degree, delta, *,
-                      capacity=1 << 20, **flags):
-    """initialize_octree (mod_octree.f90:251-518)."""
-    return Octree(make_params(box_length, nbox, origin, nlevels, min_part, degree, delta, **flags),
-                  capacity)
+                      capacity=1 << 20, particles_redistribution=False, octree_level_solid=None,
+                      particles_redistribution_ratio=3.0, **flags):
+    """initialize_octree (mod_octree.f90:251-518).  The redistribution
+    options are the sim_param ones of mod_sim_param.f90:462-467,792-800
+    (octree_level_solid defaults to max(nlevels - 2, 1))."""
+    o = Octree(make_params(box_length, nbox, origin, nlevels, min_part, degree, delta, **flags),
+               capacity)
+    o.use_pr = bool(particles_redistribution)
+    if octree_level_solid is not None:
+        o.lvl_solid = int(octree_level_solid)
+    o.part_redist_ratio = float(particles_redistribution_ratio)
+    return o
 
 
-def sort_particles(parts, octree):
-    """sort_particles (mod_octree.f90:703-996); returns the number of leaves."""
+def sort_particles(parts, octree, elem=None):
+    """sort_particles (mod_octree.f90:703-996); returns the number of leaves.
+
+    elem: the element centres (m,3) the reference sorts into the tree with the
+    particles (777-820); with particles_redistribution on they mark the cells
+    near a wall, and the redistribution (827-894) updates parts.mag, parts.dir
+    and parts.free (the freed particles) in place and sets parts.n_prt."""
     lib = L.lib()
     L.check(lib.dust_fmm_set_particles(octree.handle, parts.n, L.ptr(parts.cen), L.ptr(parts.dir),
                                        L.ptr(parts.mag), L.ptr(parts.r_vortex)), "sort_particles")
+    if octree.use_pr:
+        cen = np.ascontiguousarray(np.zeros((0, 3)) if elem is None else elem,
+                                   dtype=np.float64).reshape(-1, 3)
+        L.check(lib.dust_fmm_set_redistribution(octree.handle, 1, octree.lvl_solid,
+                                                octree.part_redist_ratio, len(cen), L.ptr(cen)),
+                "sort_particles")
+    else:
+        L.check(lib.dust_fmm_set_redistribution(octree.handle, 0, 1, 1.0, 0, None),
+                "sort_particles")
     nl = C.c_int()
     L.check(lib.dust_fmm_sort_particles(octree.handle, C.byref(nl)), "sort_particles")
     octree.n_leaves = nl.value
+    if octree.use_pr and parts.n > 0:
+        keep = np.zeros(parts.n, np.uint8)
+        nk = C.c_int()
+        L.check(lib.dust_fmm_redistributed(octree.handle, C.byref(nk), L.ptr(keep),
+                                           L.ptr(parts.dir), L.ptr(parts.mag)), "sort_particles")
+        parts.free = keep == 0
+    parts.n_prt = int(parts.n - np.count_nonzero(parts.free))
     return nl.value
 
 

@@ -99,6 +99,29 @@ int dust_fmm_set_particles_dev(dust_fmm_t h, int n, const double *cen, const dou
  * outside the box (mod_octree.f90:757-762). */
 int dust_fmm_sort_particles(dust_fmm_t h, int *nleaves);
 
+/* Particle redistribution of sort_particles (mod_octree.f90:827-894; the
+ * sim_param switches particles_redistribution, octree_level_solid and
+ * particles_redistribution_ratio of mod_sim_param.f90:462-467,792-800).
+ * While on, every later sort frees, in each finest cell not near a wall, the
+ * particles whose magnitude falls below the cell's mean / ratio, spreading
+ * each one's vorticity over the cell's other particles; freed particles are
+ * left out of the tree and get zero results from apply_multipole.  A level
+ * lvl_solid (1-based) cell is near a wall when it or a neighbour holds one of
+ * the m element centres elem_cen(3,m) (host or device memory, copied here;
+ * centres outside the box are skipped).  on = 0 switches it off.
+ * DUST_E_ARG for lvl_solid outside 1..nlevels or ratio <= 0. */
+int dust_fmm_set_redistribution(dust_fmm_t h, int on, int lvl_solid, double ratio, int m,
+                                const double *elem_cen);
+/* Outcome of the last sort in input order: keep[i] = 0 for a particle the
+ * redistribution freed (the reference sets its free flag), the strengths
+ * dir(3,n), mag(n) after it (the input ones without a redistribution), and
+ * *n_kept = the reference's n_prt after sort_particles.  Host pointers, any
+ * may be NULL. */
+int dust_fmm_redistributed(dust_fmm_t h, int *n_kept, unsigned char *keep, double *dir,
+                           double *mag);
+/* Same into device memory, on the handle's stream. */
+int dust_fmm_redistributed_dev(dust_fmm_t h, unsigned char *keep, double *dir, double *mag);
+
 /* calculate_multipole (mod_octree.f90:1002): P2M, M2M, M2L, L2L. */
 int dust_fmm_calculate_multipole(dust_fmm_t h);
 

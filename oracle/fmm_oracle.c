@@ -351,6 +351,12 @@ typedef struct {
   /* multipole storage */
   int nslots;
   double *a, *b, *c;
+  /* particle redistribution of sort_particles (mod_octree.f90:827-894) */
+  int pr_on, pr_lvl, pr_m, pr_nkept;
+  double pr_ratio;
+  double *pr_cen;           /* element centres [3*pr_m]                    */
+  double *pr_mag, *pr_dir;  /* working copies of the particle strengths    */
+  unsigned char *pr_keep;   /* 0 for the particles the redistribution freed */
 } orc_octree;
 
 static inline int cidx(const orc_layer *L, int i, int j, int k) {
@@ -460,6 +466,7 @@ void orc_octree_destroy(orc_octree *T) {
     free(L->slot); free(L->cen); free(L->ln_start); free(L->ln_count); free(L->D); free(L->Dc);
   }
   free(T->L); free(T->mult);
+  free(T->pr_cen); free(T->pr_mag); free(T->pr_dir); free(T->pr_keep);
   pexp_free(&T->pexp); pexp_free(&T->pexp_der);
   free(T);
 }
@@ -470,7 +477,149 @@ static inline int child_of(const orc_layer *Lc, int i, int j, int k, int p) {
   return cidx(Lc, 2 * (i - 1) + ic + 1, 2 * (j - 1) + jc + 1, 2 * (k - 1) + kc + 1);
 }
 
-/* sort_particles, mod_octree.f90:703-996 (no panels, no redistribution).
+/* Particle redistribution settings (sim_param use_pr, lvl_solid 1-based,
+ * part_redist_ratio) and the element centres that mark the cells near a wall;
+ * on = 0 switches it off.  Used by the next orc_sort_particles. */
+void orc_octree_set_redistribution(orc_octree *T, int on, int lvl_solid, double ratio, int m,
+                                   const double *elem_cen) {
+  T->pr_on = on;
+  T->pr_lvl = lvl_solid;
+  T->pr_ratio = ratio;
+  T->pr_m = on ? m : 0;
+  free(T->pr_cen);
+  T->pr_cen = (double *)malloc(sizeof(double) * 3 * (T->pr_m > 0 ? T->pr_m : 1));
+  if (T->pr_m > 0) memcpy(T->pr_cen, elem_cen, sizeof(double) * 3 * T->pr_m);
+}
+
+/* Outcome of the last redistribution, in input order: keep[i] = 0 for a
+ * freed particle, the strengths (dir, mag) after it.  Returns the number of
+ * particles kept (the reference's n_prt after sort_particles). */
+int orc_octree_redistributed(const orc_octree *T, unsigned char *keep, double *dir, double *mag) {
+  int n = T->n;
+  if (keep) {
+    if (T->pr_keep) memcpy(keep, T->pr_keep, n);
+    else memset(keep, 1, n);
+  }
+  if (dir) memcpy(dir, T->P.dir, sizeof(double) * 3 * n);
+  if (mag) memcpy(mag, T->P.mag, sizeof(double) * n);
+  return T->pr_nkept;
+}
+
+/* The redistribution block of sort_particles (mod_octree.f90:827-894), on
+ * the finest-level lists in push order:
+ *   - element centres are binned at the finest level with the particles'
+ *     ceiling rule, skipping those outside the box (784-795, the warning
+ *     branch), and counted upwards to level lvl_solid (801-817);
+ *   - a cell of level lvl_solid is near a wall when it or one of its (up to
+ *     26) neighbours holds an element (829-843); set_near_wall marks its
+ *     whole subtree (1655-1667);
+ *   - in every finest cell not near a wall, ave = (sum of mag in push order)
+ *     / npart (768-771, 848-852); walking the list, a particle with
+ *     mag < (1/ratio) * ave is freed and pulled from the list (the rest
+ *     shifts down, 1829-1844), and redistr = (mag * dir) / nprt, with nprt
+ *     counting it, is added to each remaining particle of the cell, whose
+ *     mag = norm2(vort) and dir = vort / norm2(vort) (0 for a zero vector)
+ *     are recomputed (854-885).
+ * norm2 is sqrt(x*x + y*y + z*z) rounded once, as everywhere in this file. */
+static void redistribute(orc_octree *T, int n, const double *dir, const double *mag) {
+  int ll = T->nlevels;
+  orc_layer *LL = &T->L[ll - 1];
+  free(T->pr_mag); free(T->pr_dir); free(T->pr_keep);
+  T->pr_mag = T->pr_dir = NULL;
+  T->pr_keep = NULL;
+  T->pr_nkept = n;
+  if (!T->pr_on) return;
+  T->pr_mag = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+  T->pr_dir = (double *)malloc(sizeof(double) * 3 * (n > 0 ? n : 1));
+  T->pr_keep = (unsigned char *)malloc(n > 0 ? n : 1);
+  if (n > 0) {
+    memcpy(T->pr_mag, mag, sizeof(double) * n);
+    memcpy(T->pr_dir, dir, sizeof(double) * 3 * n);
+    memset(T->pr_keep, 1, n);
+  }
+  T->P.mag = T->pr_mag;
+  T->P.dir = T->pr_dir;
+  int ls = T->pr_lvl < 1 ? 1 : (T->pr_lvl > ll ? ll : T->pr_lvl);
+  orc_layer *LS = &T->L[ls - 1];
+  int sh = ll - ls;
+  int *npan = (int *)calloc(LS->ncells, sizeof(int));
+  for (int e = 0; e < T->pr_m; ++e) {
+    int id[3], out = 0;
+    for (int d = 0; d < 3; ++d) {
+      double q = ceil((T->pr_cen[3 * e + d] - T->xmin[d]) / LL->csize);
+      if (!(q >= 1.0 && q <= (double)LL->ncl[d])) out = 1;
+      id[d] = out ? 1 : (int)q;
+    }
+    if (out) continue;
+    npan[cidx(LS, ((id[0] - 1) >> sh) + 1, ((id[1] - 1) >> sh) + 1, ((id[2] - 1) >> sh) + 1)]++;
+  }
+  unsigned char *near = (unsigned char *)calloc(LS->ncells, 1);
+  for (int c = 0; c < LS->ncells; ++c) {
+    int i, j, k;
+    cijk(LS, c, &i, &j, &k);
+    for (int kc = -1; kc <= 1; ++kc)
+      for (int jc = -1; jc <= 1; ++jc)
+        for (int ic = -1; ic <= 1; ++ic) {
+          int a = i + ic, b = j + jc, d = k + kc;
+          if (a < 1 || b < 1 || d < 1 || a > LS->ncl[0] || b > LS->ncl[1] || d > LS->ncl[2])
+            continue;
+          if (npan[cidx(LS, a, b, d)] > 0) near[c] = 1;
+        }
+  }
+  double *pm = T->pr_mag, *pd = T->pr_dir;
+  const double inv_ratio = 1.0 / T->pr_ratio;
+  int *list = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+  int w = 0, removed = 0;
+  for (int c = 0; c < LL->ncells; ++c) {
+    int i, j, k;
+    cijk(LL, c, &i, &j, &k);
+    int s = LL->pstart[c], np = LL->npart[c];
+    LL->pstart[c] = w;
+    const int *pl = &LL->plist[s];
+    int cnt = np;
+    memcpy(&list[w], pl, sizeof(int) * np);
+    int nearc = near[cidx(LS, ((i - 1) >> sh) + 1, ((j - 1) >> sh) + 1, ((k - 1) >> sh) + 1)];
+    if (!nearc && np > 0) {
+      int *cl = &list[w];
+      double ave = 0.0;
+      for (int q = 0; q < np; ++q) ave += mag[cl[q]];
+      ave = ave / (double)np;
+      const double thr = inv_ratio * ave;
+      int ip = 0;
+      while (ip < cnt) {
+        int r = cl[ip];
+        if (pm[r] < thr) {
+          double red[3];
+          for (int d = 0; d < 3; ++d) red[d] = (pm[r] * pd[3 * r + d]) / (double)cnt;
+          T->pr_keep[r] = 0;
+          for (int q = ip; q < cnt - 1; ++q) cl[q] = cl[q + 1];
+          cnt--;
+          removed++;
+          for (int q = 0; q < cnt; ++q) {
+            int t = cl[q];
+            double v[3];
+            for (int d = 0; d < 3; ++d) v[d] = pm[t] * pd[3 * t + d] + red[d];
+            double vn = norm2_3(v);
+            pm[t] = vn;
+            for (int d = 0; d < 3; ++d) pd[3 * t + d] = vn > 0.0 ? v[d] / vn : 0.0;
+          }
+        } else {
+          ip++;
+        }
+      }
+    }
+    LL->npart[c] = cnt;
+    w += cnt;
+  }
+  memcpy(LL->plist, list, sizeof(int) * (w > 0 ? w : 1));
+  free(list);
+  free(npan);
+  free(near);
+  T->pr_nkept = n - removed;
+}
+
+/* sort_particles, mod_octree.f90:703-996 (no panels; the redistribution
+ * block when orc_octree_set_redistribution switched it on).
  * Returns 0, or -1 (with *bad set to the particle index) when a particle
  * falls outside the octree box (mod_octree.f90:757-762). */
 int orc_sort_particles(orc_octree *T, int n, const double *cen, const double *dir,
@@ -510,7 +659,7 @@ int orc_sort_particles(orc_octree *T, int n, const double *cen, const double *di
     int s = 0;
     for (int c = 0; c < LL->ncells; ++c) { LL->pstart[c] = s; s += LL->npart[c]; }
     LL->plist = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
-    int *fill = (int *)calloc(LL->ncells, sizeof(int));
+    int *fill = (int *)calloc((size_t)(LL->ncells > 0 ? LL->ncells : 1), sizeof(int));
     for (int ip = 0; ip < n; ++ip) {
       int c = pc[ip];
       LL->plist[LL->pstart[c] + fill[c]++] = ip;
@@ -518,6 +667,7 @@ int orc_sort_particles(orc_octree *T, int n, const double *cen, const double *di
     free(fill);
   }
   free(pc);
+  redistribute(T, n, dir, mag);
   /* leaves list capacity: every cell could be a leaf */
   int cap = 0;
   for (int l = 0; l < ll; ++l) cap += T->L[l].ncells;

@@ -40,6 +40,9 @@ def lib():
                                         C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, _vp]),
             "orc_octree_destroy": (None, [_vp]),
             "orc_sort_particles": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+            "orc_octree_set_redistribution": (None, [_vp, C.c_int, C.c_int, C.c_double, C.c_int,
+                                                     _vp]),
+            "orc_octree_redistributed": (C.c_int, [_vp, _vp, _vp, _vp]),
             "orc_calculate_multipole": (None, [_vp]),
             "orc_apply_multipole": (None, [_vp, _vp, _vp, _vp]),
             "orc_octree_nleaves": (C.c_int, [_vp]),
@@ -201,6 +204,22 @@ class Octree:
         if rc != 0:
             raise ValueError(f"particle {bad.value} outside the octree")
         return lib().orc_octree_nleaves(self._h)
+
+    def set_redistribution(self, lvl_solid, ratio, elem_cen, on=True):
+        """Particle redistribution of the next sorts (sort_particles,
+        mod_octree.f90:827-894): lvl_solid 1-based, elem_cen (m,3)."""
+        cen = _c(np.zeros((0, 3)) if elem_cen is None else elem_cen).reshape(-1, 3)
+        lib().orc_octree_set_redistribution(self._h, int(bool(on)), int(lvl_solid), float(ratio),
+                                            len(cen), _p(cen))
+
+    def redistributed(self):
+        """(keep (n,) bool, dir (n,3), mag (n,), n_kept) after the last sort."""
+        n = len(self._keep[2])
+        keep = np.zeros(max(n, 1), np.uint8)
+        d = np.zeros((max(n, 1), 3))
+        m = np.zeros(max(n, 1))
+        nk = lib().orc_octree_redistributed(self._h, _p(keep), _p(d), _p(m))
+        return keep[:n].astype(bool), d[:n], m[:n], nk
 
     def multipole(self):
         lib().orc_calculate_multipole(self._h)
