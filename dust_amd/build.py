@@ -16,10 +16,15 @@ LIB = os.path.join(HERE, "libdust_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/amdflang")
 ROOT = os.path.dirname(HERE)
-# Fortran binding (ISO_C_BINDING module) and the Fortran test driver
+# Fortran binding (ISO_C_BINDING module), the modules with the reference's
+# procedure names on top of it, and the Fortran test drivers
 FORTRAN_MOD = os.path.join(HERE, "fortran", "mod_dust_gpu.f90")
+FORTRAN_SHIMS = [os.path.join(HERE, "fortran", f) for f in ("mod_octree_gpu.f90",
+                                                          "mod_linsys_gpu.f90")]
 FORTRAN_DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "dust_gpu_driver.f90")
 FORTRAN_DRIVER = os.path.join(ROOT, "tests", "fortran", "dust_gpu_driver")
+SHIM_DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "dust_shim_driver.f90")
+SHIM_DRIVER = os.path.join(ROOT, "tests", "fortran", "dust_shim_driver")
 ARCH = "gfx950"
 # HDF5 result files (libdust_h5.so, host code): the HDF5 C library of the image
 HDF5_ROOT = os.environ.get("HDF5_ROOT", "/opt/conda")
@@ -82,26 +87,32 @@ def build(verbose=False, jobs=None):
 
 
 def build_fortran():
-    """Compile the Fortran binding module and link the Fortran test driver
-    against libdust_gpu.so (rpath relative to the driver).  Returns the driver
-    path, or None when no Fortran compiler is installed."""
+    """Compile the Fortran binding module and the reference-named modules on
+    it, and link the two Fortran test drivers against libdust_gpu.so (rpath
+    relative to the drivers).  Returns the first driver's path, or None when
+    no Fortran compiler is installed."""
     if not os.path.exists(FLANG):
         return None
     fdir = os.path.join(BUILD, "fortran")
     os.makedirs(fdir, exist_ok=True)
-    mod_o = os.path.join(fdir, "mod_dust_gpu.o")
-    drv_o = os.path.join(fdir, "dust_gpu_driver.o")
-    steps = [
-        [FLANG, "-O2", "-J", fdir, "-c", FORTRAN_MOD, "-o", mod_o],
-        [FLANG, "-O2", "-J", fdir, "-c", FORTRAN_DRIVER_SRC, "-o", drv_o],
-        [FLANG, drv_o, mod_o, "-L" + HERE, "-ldust_gpu",
-         "-Wl,-rpath,$ORIGIN/../../dust_amd", "-o", FORTRAN_DRIVER + ".tmp"],
-    ]
+
+    def obj(src):
+        return os.path.join(fdir, os.path.splitext(os.path.basename(src))[0] + ".o")
+
+    mods = [FORTRAN_MOD] + FORTRAN_SHIMS
+    steps = [[FLANG, "-O2", "-J", fdir, "-c", m, "-o", obj(m)] for m in mods]
+    for src, exe, deps in ((FORTRAN_DRIVER_SRC, FORTRAN_DRIVER, [FORTRAN_MOD]),
+                           (SHIM_DRIVER_SRC, SHIM_DRIVER, mods)):
+        steps.append([FLANG, "-O2", "-J", fdir, "-c", src, "-o", obj(src)])
+        steps.append([FLANG, obj(src)] + [obj(m) for m in deps] +
+                     ["-L" + HERE, "-ldust_gpu", "-Wl,-rpath,$ORIGIN/../../dust_amd",
+                      "-o", exe + ".tmp"])
     for cmd in steps:
         p = subprocess.run(cmd, capture_output=True, text=True)
         if p.returncode != 0:
             raise RuntimeError("flang failed: " + " ".join(cmd) + "\n" + p.stdout + p.stderr)
-    os.replace(FORTRAN_DRIVER + ".tmp", FORTRAN_DRIVER)
+    for exe in (FORTRAN_DRIVER, SHIM_DRIVER):
+        os.replace(exe + ".tmp", exe)
     return FORTRAN_DRIVER
 
 

@@ -63,6 +63,16 @@ module mod_dust_gpu
     integer(c_int) function dust_fmm_sort_particles(h, nleaves) bind(C)
       import; type(c_ptr), value :: h; integer(c_int), intent(out) :: nleaves
     end function
+    integer(c_int) function dust_fmm_set_redistribution(h, on, lvl_solid, ratio, m, elem_cen) &
+        bind(C)
+      import; type(c_ptr), value :: h; integer(c_int), value :: on, lvl_solid, m
+      real(c_double), value :: ratio; real(c_double), intent(in) :: elem_cen(3,*)
+    end function
+    integer(c_int) function dust_fmm_redistributed(h, n_kept, keep, dir, mag) bind(C)
+      import; type(c_ptr), value :: h; integer(c_int), intent(out) :: n_kept
+      integer(c_int8_t), intent(out) :: keep(*)
+      real(c_double), intent(inout) :: dir(3,*), mag(*)
+    end function
     integer(c_int) function dust_fmm_calculate_multipole(h) bind(C)
       import; type(c_ptr), value :: h
     end function

@@ -2,7 +2,9 @@
 build() with AMD flang against dust_amd/fortran/mod_dust_gpu.f90) calls
 initialize_octree -> sort_particles -> calculate_multipole -> apply_multipole
 and dgetrf + dgetrs through ISO_C_BINDING, as DUST's mod_octree / mod_linsys
-would; results are compared with the CPU oracle and with LAPACK."""
+would; results are compared with the CPU oracle and with LAPACK.  The second
+driver (tests/fortran/dust_shim_driver) goes through the modules with the
+reference's procedure names, mod_octree_gpu and mod_linsys_gpu."""
 import os
 import subprocess
 
@@ -55,3 +57,94 @@ def test_fortran_driver_fmm_and_lu(orc, tmp_path):
     assert rel(st, s) <= 1e-10
     assert info == 0, p.stdout
     assert rel(x, np.linalg.solve(A, b)) <= 1e-10
+
+
+SHIM = os.path.join(HERE, "fortran", "dust_shim_driver")
+
+
+@pytest.mark.skipif(not os.path.exists(SHIM), reason="Fortran shim driver not built (no flang)")
+def test_reference_named_modules(orc, tmp_path):
+    """tests/fortran/dust_shim_driver calls mod_octree_gpu / mod_linsys_gpu as
+    DUST's prepare_wake, update_wake and time step do: sort_particles over
+    wake slots with free slots and the particle redistribution, then
+    calculate_multipole, apply_multipole_panels, apply_multipole with the
+    four element lists; factorize_static, solve_linsys, and solve_linsys
+    again with skip.  Against the oracle on the same live particles (freed
+    flags and strengths bit-exact, velocities and uvort within 1e-10) and
+    LAPACK."""
+    rng = np.random.default_rng(17)
+    groups = W.wing_elements()
+    ver, nv, cen, alpha, rv, tree = W.wing_case()
+    alpha = alpha * rng.lognormal(0.0, 1.0, size=(len(alpha), 1))
+    mag = np.linalg.norm(alpha, axis=1)
+    dr = alpha / mag[:, None]
+    n = len(cen)
+    nfree = 700
+    nmax = n + nfree
+    slots = np.sort(rng.choice(nmax, n, replace=False))   # live slots, in order
+    free = np.ones(nmax, np.int32)
+    free[slots] = 0
+    S = dict(cen=rng.uniform(-1, 1, (nmax, 3)), dir=np.zeros((nmax, 3)), mag=np.zeros(nmax),
+             rv=np.full(nmax, 0.1))
+    S["dir"][:, 0] = 1.0
+    S["cen"][slots], S["dir"][slots], S["mag"][slots], S["rv"][slots] = cen, dr, mag, rv
+    el, wpn, wrn, wvt = groups
+    pc = np.array([el["ver"][i].mean(axis=0) for i in range(len(el["ver"]))])
+    rank, nstatic = 320, 200
+    A = rng.normal(size=(rank, rank)) + rank * np.eye(rank)
+    b, b2 = rng.normal(size=rank), rng.normal(size=rank)
+    u_inf = (30.0, 0.0, 0.0)
+    ff, rankine, cutoff, vrad, ratio, lvl = 10.0, 0.1, 0.001, 0.1, 3.0, 3
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([nmax, n, tree["nlevels"], tree["min_part"], 4, 1, lvl, len(el["ver"]),
+                  len(wpn["ver"]), len(wrn["ver"]), len(wvt["ver"]), rank, nstatic],
+                 np.int32).tofile(f)
+        np.array([tree["box_length"], *tree["origin"], tree["delta"], ratio, *u_inf, ff, rankine,
+                  cutoff, vrad], np.float64).tofile(f)
+        for k in ("cen", "dir", "mag", "rv"):
+            np.ascontiguousarray(S[k], np.float64).tofile(f)
+        free.tofile(f)
+        np.ascontiguousarray(el["ver"], np.float64).tofile(f)
+        np.asarray(el["n_ver"], np.int32).tofile(f)
+        for a in (el["mag"], el["sigma"], pc):
+            np.ascontiguousarray(a, np.float64).tofile(f)
+        for g in (wpn, wrn, wvt):
+            np.ascontiguousarray(g["ver"], np.float64).tofile(f)
+            np.ascontiguousarray(g["mag"], np.float64).tofile(f)
+        np.asfortranarray(A).T.tofile(f)
+        b.tofile(f)
+        b2.tofile(f)
+    print(f"fortran shim driver nmax={nmax} n={n} panels={len(pc)} rank={rank}", flush=True)
+    p = subprocess.run([SHIM, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    ne = len(pc)
+    with open(fout, "rb") as f:
+        n_prt, nleaves = np.fromfile(f, np.int32, 2)
+        free_out = np.fromfile(f, np.int32, nmax)
+        d_out = np.fromfile(f, np.float64, 3 * nmax).reshape(nmax, 3)
+        m_out = np.fromfile(f, np.float64, nmax)
+        vel = np.fromfile(f, np.float64, 3 * nmax).reshape(nmax, 3)
+        st = np.fromfile(f, np.float64, 3 * nmax).reshape(nmax, 3)
+        uvort = np.fromfile(f, np.float64, 3 * ne).reshape(ne, 3)
+        res1 = np.fromfile(f, np.float64, rank)
+        res2 = np.fromfile(f, np.float64, rank)
+    c = orc.Octree(tree["box_length"], tree["nbox"], tree["origin"], tree["nlevels"],
+                   tree["min_part"], 4, tree["delta"], u_inf=u_inf)
+    c.set_redistribution(lvl, ratio, pc)
+    v, s, _ = c.run(cen, dr, mag, rv)
+    keep, d, m, nk = c.redistributed()
+    assert 0 < nk < n and n_prt == nk
+    assert nleaves == orc.lib().orc_octree_nleaves(c._h)
+    assert np.array_equal(free_out[slots] == 0, keep)
+    assert np.array_equal(free_out[free == 1], np.ones(nfree, np.int32))
+    assert np.array_equal(d_out[slots], d) and np.array_equal(m_out[slots], m)
+    kept = slots[keep]
+    v_tot, _ = orc.elem_apply([dict(g, ver=np.asarray(g["ver"])) for g in groups], cen[keep],
+                           vel=v[keep], ff_ratio=ff, rankine=rankine, cutoff=cutoff,
+                           vortex_rad=vrad)
+    assert rel(vel[kept], v_tot) <= 1e-10
+    assert rel(st[kept], s[keep]) <= 1e-10
+    assert rel(uvort, c.apply_panels(pc)) <= 1e-10
+    assert rel(res1, np.linalg.solve(A, b)) <= 1e-12
+    assert rel(res2, np.linalg.solve(A, b2)) <= 1e-12
