@@ -49,7 +49,8 @@ class FmmParams(C.Structure):
         ("u_inf", C.c_double * 3),
         ("precision", C.c_int),
         ("kernel", C.c_int),
-        ("reserved", C.c_int * 6),
+        ("use_divfilt", C.c_int),
+        ("reserved", C.c_int * 5),
     ]
 
 
@@ -166,6 +167,8 @@ SIGNATURES = [
     ("dust_fmm_calculate_multipole", C.c_int, [_vp]),
     ("dust_fmm_apply_multipole", C.c_int, [_vp, _vp, _vp, _vp]),
     ("dust_fmm_apply_multipole_dev", C.c_int, [_vp]),
+    ("dust_fmm_rotu", C.c_int, [_vp, _vp]),
+    ("dust_fmm_rotu_dev", C.c_int, [_vp, _vp]),
     ("dust_fmm_results_dev", C.c_int, [_vp, _vp, _vp, _vp]),
     ("dust_fmm_apply_multipole_panels", C.c_int, [_vp, C.c_int, _vp, _vp]),
     ("dust_fmm_apply_multipole_panels_dev", C.c_int, [_vp, C.c_int, _vp, _vp]),

@@ -299,12 +299,13 @@ __global__ void k_pr_compact(int n, const int *__restrict__ ns, const int *__res
 
 // Outputs of the freed particles (no evaluation) set to zero.
 __global__ void k_pr_zero(int n, const unsigned char *__restrict__ keep, double *v, double *s,
-                          double *tv) {
+                          double *tv, double *rot) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n || keep[q]) return;
   v[3 * q] = v[3 * q + 1] = v[3 * q + 2] = 0.0;
   s[3 * q] = s[3 * q + 1] = s[3 * q + 2] = 0.0;
   tv[q] = 0.0;
+  if (rot) rot[3 * q] = rot[3 * q + 1] = rot[3 * q + 2] = 0.0;
 }
 
 // Finest level: leaf iff npart >= min_part, or the tree has one level
@@ -799,6 +800,7 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_fp64(EvalArgs A) {
   const double al0 = mg * A.dx[t], al1 = mg * A.dy[t], al2 = mg * A.dz[t];
   const double tvp = turbvisc_of(A, U.leaf, t);
   double pv0 = 0, pv1 = 0, pv2 = 0, ps0 = 0, ps1 = 0, ps2 = 0, pd0 = 0, pd1 = 0, pd2 = 0;
+  double pr0 = 0, pr1 = 0, pr2 = 0;
   const int nr = A.cnt[U.leaf];
   const int2 *rl = &A.rng[(long)U.leaf * A.cap];
   int ri, off;
@@ -848,6 +850,14 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_fp64(EvalArgs A) {
             ps0 -= cf * vp0 + df * r0 * dv;
             ps1 -= cf * vp1 + df * r1 * dv;
             ps2 -= cf * vp2 + df * r2 * dv;
+            if (A.divf) {
+              // compute_rotu (mod_vortpart.f90:236-256): -2 c alpha_s + d r x (r x alpha_s)
+              const double i0 = r1 * am2 - r2 * am1, i1 = r2 * am0 - r0 * am2,
+                           i2 = r0 * am1 - r1 * am0;
+              pr0 += -2.0 * cf * sdx * smg + df * (r1 * i2 - r2 * i1);
+              pr1 += -2.0 * cf * sdy * smg + df * (r2 * i0 - r0 * i2);
+              pr2 += -2.0 * cf * sdz * smg + df * (r0 * i1 - r1 * i0);
+            }
           }
           if (A.use_vd) {
             // compute_diffusion with etaeps (mod_vortpart.f90:305-336)
@@ -870,14 +880,20 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_fp64(EvalArgs A) {
   if (A.use_vs) {
     ps0 = lane_group_sum(ps0, lpt); ps1 = lane_group_sum(ps1, lpt); ps2 = lane_group_sum(ps2, lpt);
   }
+  if (A.divf) {
+    pr0 = lane_group_sum(pr0, lpt); pr1 = lane_group_sum(pr1, lpt); pr2 = lane_group_sum(pr2, lpt);
+  }
   if (A.use_vd) {
     pd0 = lane_group_sum(pd0, lpt); pd1 = lane_group_sum(pd1, lpt); pd2 = lane_group_sum(pd2, lpt);
   }
   if (valid && sub == 0) {
     const double fi = 1.0 / (4.0 * kPi);
-    double *o = &A.part[((long)it * kWave + tl) * 6];
+    double *o = &A.part[((long)it * kWave + tl) * A.pstride];
     o[0] = pv0 * fi; o[1] = pv1 * fi; o[2] = pv2 * fi;
     o[3] = ps0 * fi + pd0; o[4] = ps1 * fi + pd1; o[5] = ps2 * fi + pd2;
+    if (A.divf) {
+      o[6] = pr0 * fi; o[7] = pr1 * fi; o[8] = pr2 * fi;
+    }
   }
 }
 
@@ -918,14 +934,15 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_eval_finish(EvalArgs A) 
   // partial sums of the unit's items, added in item order; the loads of four
   // items are issued together (units of large leaves have hundreds of items)
   double pv[3] = {0, 0, 0}, ps[3] = {0, 0, 0};
-  const double *o = &A.part[((long)U.item0 * kWave + lane) * 6];
+  const int ps_ = A.pstride;
+  const double *o = &A.part[((long)U.item0 * kWave + lane) * ps_];
   int k = 0;
   for (; k + 4 <= U.nseg; k += 4) {
     double q[4][6];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int e = 0; e < 6; ++e) q[u][e] = o[(long)(k + u) * kWave * 6 + e];
+      for (int e = 0; e < 6; ++e) q[u][e] = o[(long)(k + u) * kWave * ps_ + e];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -936,12 +953,21 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_eval_finish(EvalArgs A) 
   }
   for (; k < U.nseg; ++k)
     for (int e = 0; e < 3; ++e) {
-      pv[e] += o[(long)k * kWave * 6 + e];
-      ps[e] += o[(long)k * kWave * 6 + 3 + e];
+      pv[e] += o[(long)k * kWave * ps_ + e];
+      ps[e] += o[(long)k * kWave * ps_ + 3 + e];
     }
   for (int e = 0; e < 3; ++e) {
     A.vel[3 * t + e] = v[e] + pv[e] + A.uinf[e];
     A.str[3 * t + e] = st[e] + ps[e];
+  }
+  if (A.divf) {
+    // rotu = curl of the local expansion's velocity, grad(2,3) - grad(3,2),
+    // ... (mod_octree.f90:1207-1212; gr[i + 3 j] = grad(i+1, j+1)), then
+    // the items' P2P sums in item order
+    double ru[3] = {gr[7] - gr[5], gr[2] - gr[6], gr[3] - gr[1]};
+    for (int q = 0; q < U.nseg; ++q)
+      for (int e = 0; e < 3; ++e) ru[e] += o[(long)q * kWave * ps_ + 6 + e];
+    for (int e = 0; e < 3; ++e) A.rot[3 * t + e] = ru[e];
   }
   A.tv[t] = turbvisc_of(A, U.leaf, t);
 }
@@ -964,6 +990,17 @@ __global__ void k_scatter(int n, const int *__restrict__ perm, const double *__r
   s[3 * q + 1] = ss[3 * i + 1];
   s[3 * q + 2] = ss[3 * i + 2];
   tv[q] = ts[i];
+}
+
+// Scatter a 3-vector per particle from sorted to input order.
+__global__ void k_scatter3(int n, const int *__restrict__ perm, const double *__restrict__ src,
+                           double *dst) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int q = perm[i];
+  dst[3 * q] = src[3 * i];
+  dst[3 * q + 1] = src[3 * i + 1];
+  dst[3 * q + 2] = src[3 * i + 2];
 }
 
 constexpr int kEvalThreads = 128;
@@ -1202,6 +1239,9 @@ static int fmm_create(const dust_fmm_params *prm, int capacity, FmmHandle **out)
       (rc = h->leaves.reserve(h->ncells_tot)) || (rc = h->counts.reserve(kMaxLevels + 4)) ||
       (rc = h->d_err.reserve(1)) || (rc = h->n_units.reserve(2)))
     return fail(rc);
+  if (prm->use_vs && prm->use_divfilt &&
+      ((rc = h->rot_s.reserve(3 * C)) || (rc = h->rot.reserve(3 * C))))
+    return fail(rc);
   // scratch for the device-wide primitives, sized once so that later calls
   // never reallocate while work is in flight
   if ((rc = prim_reserve(h))) return fail(rc);
@@ -1236,7 +1276,7 @@ static void fmm_destroy(FmmHandle *h) {
   DevBuf<double> *dbl[] = {&h->d_multT, &h->d_kd, &h->d_fact, &h->cen_in, &h->dir_in, &h->mag_in,
                            &h->rv_in, &h->px, &h->py, &h->pz, &h->dx, &h->dy, &h->dz, &h->pm,
                            &h->pr, &h->mp_a, &h->mp_b, &h->mp_c, &h->vel_s, &h->str_s, &h->tv_s,
-                           &h->vel, &h->str, &h->tv};
+                           &h->vel, &h->str, &h->tv, &h->rot_s, &h->rot};
   for (auto b : dbl) b->release();
   DevBuf<int> *ints[] = {&h->d_pwr, &h->d_mnT, &h->idx_in, &h->perm, &h->fstart, &h->slot,
                          &h->active, &h->leaves, &h->counts, &h->p2p_cnt, &h->d_err,
@@ -1433,7 +1473,8 @@ static int sort_phase_c(FmmHandle *h) {
   h->n_item = rb->units[1];
   std::memcpy(h->act_start, rb->act_start, sizeof(int) * (h->nlev + 1));
   DUST_TRY(h->items.reserve((size_t)std::max(h->n_item, 1) * 4));
-  DUST_TRY(h->part.reserve((size_t)std::max(h->n_item, 1) * kWave * 6));
+  DUST_TRY(h->part.reserve((size_t)std::max(h->n_item, 1) * kWave *
+                           ((h->prm.use_vs && h->prm.use_divfilt) ? 9 : 6)));
   if (h->n_unit > 0)
     k_items<<<ceil_div(h->n_unit, TB), TB, 0, h->stream>>>(h->n_unit, (const EvalUnit *)h->units.p,
                                                           h->p2p_tot.p, (EvalItem *)h->items.p);
@@ -1554,6 +1595,9 @@ void make_eval_args(const FmmHandle *h, EvalArgs *Ap) {
   A.near2 = 0.0;
   A.part = h->part.p;
   A.vel = h->vel_s.p; A.str = h->str_s.p; A.tv = h->tv_s.p;
+  A.divf = (h->prm.use_vs && h->prm.use_divfilt) ? 1 : 0;
+  A.pstride = A.divf ? 9 : 6;
+  A.rot = h->rot_s.p;
 }
 
 int eval_leaf_ros(FmmHandle *h, const EvalArgs &A) {
@@ -1594,7 +1638,7 @@ static int fmm_apply(FmmHandle *h) {
   if (n == 0) return 0;
   if (h->ns < n)  // freed by the redistribution: not evaluated
     k_pr_zero<<<ceil_div(n, 256), 256, 0, h->stream>>>(n, h->pr_keep.p, h->vel.p, h->str.p,
-                                                       h->tv.p);
+                                                       h->tv.p, h->rot.p);
   if (h->ns == 0) return 0;
   EvalArgs A;
   make_eval_args(h, &A);
@@ -1613,6 +1657,9 @@ static int fmm_apply(FmmHandle *h) {
   timer_end(h, DUST_FMM_STAGE_L2P);
   k_scatter<<<ceil_div(h->ns, 256), 256, 0, h->stream>>>(h->ns, h->perm.p, h->vel_s.p, h->str_s.p,
                                                          h->tv_s.p, h->vel.p, h->str.p, h->tv.p);
+  if (A.divf)
+    k_scatter3<<<ceil_div(h->ns, 256), 256, 0, h->stream>>>(h->ns, h->perm.p, h->rot_s.p,
+                                                            h->rot.p);
   return 0;
 }
 
@@ -1904,6 +1951,27 @@ int dust_fmm_apply_multipole(dust_fmm_t hh, double *vel, double *stretch, double
   return 0;
 }
 
+static int rotu_copy(FmmHandle *h, double *rotu, hipMemcpyKind kind) {
+  DUST_CHECK(h->prm.use_vs && h->prm.use_divfilt, DUST_E_ARG,
+             "rotu needs a handle created with use_vs and use_divfilt");
+  DUST_CHECK(h->computed, DUST_E_ARG, "apply_multipole must precede rotu");
+  if (h->n > 0 && rotu)
+    DUST_HIP(hipMemcpyAsync(rotu, h->rot.p, sizeof(double) * 3 * h->n, kind, h->stream));
+  return 0;
+}
+
+int dust_fmm_rotu(dust_fmm_t hh, double *rotu) {
+  DUST_CHECK(hh, DUST_E_ARG, "null handle");
+  DUST_TRY(rotu_copy(hh->h, rotu, hipMemcpyDeviceToHost));
+  DUST_HIP(hipStreamSynchronize(hh->h->stream));
+  return 0;
+}
+
+int dust_fmm_rotu_dev(dust_fmm_t hh, double *rotu) {
+  DUST_CHECK(hh, DUST_E_ARG, "null handle");
+  return rotu_copy(hh->h, rotu, hipMemcpyDeviceToDevice);
+}
+
 int dust_fmm_results_dev(dust_fmm_t hh, double *vel, double *stretch, double *turbvisc) {
   DUST_CHECK(hh, DUST_E_ARG, "null handle");
   FmmHandle *h = hh->h;
@@ -1988,7 +2056,9 @@ int dust_fmm_pass_batch_dev(int nh, dust_fmm_t *hs, int *nleaves) {
     FmmHandle *h = hs[i]->h;
     if (!tuned_batchable(h)) continue;
     if (!shared.empty() && (h->prm.use_vs != shared[0]->prm.use_vs ||
-                            h->prm.kernel != shared[0]->prm.kernel))
+                            h->prm.kernel != shared[0]->prm.kernel ||
+                            (h->prm.use_vs && h->prm.use_divfilt) !=
+                                (shared[0]->prm.use_vs && shared[0]->prm.use_divfilt)))
       continue;
     shared.push_back(h);
   }

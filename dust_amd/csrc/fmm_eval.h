@@ -47,8 +47,10 @@ struct EvalArgs {
   int n_units, n_items;
   const double *leaf_ros;  // leaf-averaged rate of strain (use_vd && use_tv)
   double near2;            // tuned form: pairs with d^2 < near2 * r^2 run in FP64
-  double *part;            // per item and target: P2P velocity (3) and stretching (3)
-  double *vel, *str, *tv;
+  double *part;            // per item and target: P2P velocity (3), stretching (3)
+                           // and, with divf, rotu (3): pstride doubles
+  int divf, pstride;       // divergence filter's rotu (use_vs && use_divfilt)
+  double *vel, *str, *tv, *rot;
 };
 
 // Gaussian core of kernel_coeffs (the commented alternative of

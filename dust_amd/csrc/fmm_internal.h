@@ -96,8 +96,8 @@ struct FmmHandle {
   int units_cap = 0, n_unit = 0, n_item = 0;
   // expansions
   DevBuf<double> mp_a, mp_b, mp_c;
-  // outputs
-  DevBuf<double> vel_s, str_s, tv_s, vel, str, tv;
+  // outputs (rot_s / rot: the divergence filter's rotu, with use_divfilt)
+  DevBuf<double> vel_s, str_s, tv_s, vel, str, tv, rot_s, rot;
   DevBuf<int> d_err;
   // tuned form state
   void *tuned = nullptr;

@@ -519,7 +519,7 @@ __global__ void k_batch_clear(int *p, int n) {
 template <int GAUSS>
 __device__ inline void near64(double r0, double r1, double r2, double al0, double al1, double al2,
                               double as0, double as1, double as2, double rv, int use_vs,
-                              double v[3], double s[3]) {
+                              double v[3], double s[3], double *ru) {
   double cf, df;
   if (GAUSS) {
     gauss_cd(r0 * r0 + r1 * r1 + r2 * r2, rv, cf, df);
@@ -542,6 +542,12 @@ __device__ inline void near64(double r0, double r1, double r2, double al0, doubl
     s[0] -= cf * vp0 + df * r0 * dv;
     s[1] -= cf * vp1 + df * r1 * dv;
     s[2] -= cf * vp2 + df * r2 * dv;
+    if (ru) {  // compute_rotu: -2 c alpha_s + d r x (r x alpha_s)
+      const double i0 = r1 * as2 - r2 * as1, i1 = r2 * as0 - r0 * as2, i2 = r0 * as1 - r1 * as0;
+      ru[0] += -2.0 * cf * as0 + df * (r1 * i2 - r2 * i1);
+      ru[1] += -2.0 * cf * as1 + df * (r2 * i0 - r0 * i2);
+      ru[2] += -2.0 * cf * as2 + df * (r0 * i1 - r1 * i0);
+    }
   }
 }
 
@@ -556,6 +562,9 @@ __device__ inline void near64(double r0, double r1, double r2, double al0, doubl
 // is padded with zero-strength sources far away, so the loop trip count is
 // uniform.
 //
+// VS = 0: velocity; 1: velocity and stretching; 2: also the divergence
+// filter's rotu, -2 c alpha_s + d r x (r x alpha_s) = alpha_s (3 t d^2 - 2 c)
+// - 3 t (r.alpha_s) r with t = -d / 3 (the tf of the stretching sums).
 // BATCH = 0: wavefront w of the grid evaluates item w of one pass (A0).
 // BATCH = 1: the items of several passes in one persistent launch: every
 // wavefront takes the next global item number from the counter *cnt until
@@ -611,7 +620,7 @@ __global__ void __launch_bounds__(kWave * kEvalWaves)
     const double al0 = mg * A.dx[t], al1 = mg * A.dy[t], al2 = mg * A.dz[t];
     const float fx = (float)xr, fy = (float)yr, fz = (float)zr;
     const float fa0 = (float)al0, fa1 = (float)al1, fa2 = (float)al2;
-    double pv[3] = {0, 0, 0}, ps[3] = {0, 0, 0};
+    double pv[3] = {0, 0, 0}, ps[3] = {0, 0, 0}, pr[3] = {0, 0, 0};
     const double near2d = A.near2;  // staged per source as near2 r_s^2
     const int nr = A.cnt[U.leaf];
     const int2 *rl = &A.rng[(long)U.leaf * A.cap];
@@ -642,6 +651,7 @@ __global__ void __launch_bounds__(kWave * kEvalWaves)
         wave_sync();
         float v0 = 0.f, v1 = 0.f, v2 = 0.f, w0 = 0.f, w1 = 0.f, w2 = 0.f;
         float S0 = 0.f, S1 = 0.f, S2 = 0.f;
+        float T0 = 0.f, T1 = 0.f, T2 = 0.f, Q0 = 0.f, Q1 = 0.f, Q2 = 0.f;
         unsigned long long nmask = 0ull;
         // one source pair of the tile: s = sub + k lpt
         auto pair = [&](int s) {
@@ -688,6 +698,15 @@ __global__ void __launch_bounds__(kWave * kEvalWaves)
             w0 += tt * r0;
             w1 += tt * r1;
             w2 += tt * r2;
+            if (VS == 2) {
+              const float td = tf * d2, tq = tf * (r0 * a0 + r1 * a1 + r2 * a2);
+              T0 += td * a0;
+              T1 += td * a1;
+              T2 += td * a2;
+              Q0 += tq * r0;
+              Q1 += tq * r1;
+              Q2 += tq * r2;
+            }
           }
         };
         // wave-uniform trip count: scalar loop control, two pairs per iteration
@@ -703,6 +722,11 @@ __global__ void __launch_bounds__(kWave * kEvalWaves)
           w1 = -3.f * w1 - (fa2 * S0 - fa0 * S2);
           w2 = -3.f * w2 - (fa0 * S1 - fa1 * S0);
         }
+        if (VS == 2) {
+          pr[0] += -2.f * S0 + 3.f * (T0 - Q0);
+          pr[1] += -2.f * S1 + 3.f * (T1 - Q1);
+          pr[2] += -2.f * S2 + 3.f * (T2 - Q2);
+        }
         pv[0] += v0; pv[1] += v1; pv[2] += v2;
         ps[0] += w0; ps[1] += w1; ps[2] += w2;
         // the FP64 pairs of this tile, one per lane per step
@@ -712,7 +736,7 @@ __global__ void __launch_bounds__(kWave * kEvalWaves)
             nmask &= nmask - 1;
             const double *d = sd + 8 * s;
             near64<GAUSS>(xr - d[0], yr - d[1], zr - d[2], al0, al1, al2, d[4], d[5], d[6], d[3],
-                          VS, pv, ps);
+                          VS, pv, ps, VS == 2 ? pr : nullptr);
           }
         }
         wave_sync();
@@ -722,13 +746,15 @@ __global__ void __launch_bounds__(kWave * kEvalWaves)
     for (int q = 0; q < 3; ++q) {
       pv[q] = lane_group_sum(pv[q], lpt);
       if (VS) ps[q] = lane_group_sum(ps[q], lpt);
+      if (VS == 2) pr[q] = lane_group_sum(pr[q], lpt);
     }
     if (valid && sub == 0) {
       const double fi = 1.0 / (4.0 * kPi);
-      double *o = &A.part[((long)it * kWave + tl) * 6];
+      double *o = &A.part[((long)it * kWave + tl) * (VS == 2 ? 9 : 6)];
       for (int q = 0; q < 3; ++q) {
         o[q] = pv[q] * fi;
         o[3 + q] = ps[q] * fi;
+        if (VS == 2) o[6 + q] = pr[q] * fi;
       }
     }
     if (BATCH) gi = next_item();
@@ -1003,9 +1029,11 @@ int tuned_batch_launch(FmmHandle *const *hs, int nh, hipStream_t st, hipEvent_t 
     DUST_HIP(hipMemcpyAsync(B.off.p, B.hoff.p, (nh + 1) * sizeof(int), hipMemcpyHostToDevice,
                             st));
     const bool vs = hs[0]->prm.use_vs, g = hs[0]->prm.kernel == DUST_KERNEL_GAUSSIAN;
+    const bool dv = vs && hs[0]->prm.use_divfilt;
     using Kern = void (*)(EvalArgs, const EvalArgs *, const int *, int, int *);
-    const Kern kern = vs ? (g ? k_p2p_f32<1, 1, 1> : k_p2p_f32<1, 0, 1>)
-                         : (g ? k_p2p_f32<0, 1, 1> : k_p2p_f32<0, 0, 1>);
+    const Kern kern = dv ? (g ? k_p2p_f32<2, 1, 1> : k_p2p_f32<2, 0, 1>)
+                      : vs ? (g ? k_p2p_f32<1, 1, 1> : k_p2p_f32<1, 0, 1>)
+                           : (g ? k_p2p_f32<0, 1, 1> : k_p2p_f32<0, 0, 1>);
     unsigned grid = 0;
     DUST_TRY(persistent_grid((const void *)kern, kWave * kEvalWaves, 0, &grid));
     grid = std::min(grid, (unsigned)ceil_div(total, kEvalWaves));
@@ -1024,7 +1052,10 @@ int tuned_eval(FmmHandle *h, EvalArgs &A) {
   if (A.n_items > 0) {
     const dim3 grid(ceil_div(A.n_items, kEvalWaves)), block(kWave * kEvalWaves);
     const bool g = A.kernel == DUST_KERNEL_GAUSSIAN;
-    if (A.use_vs) {
+    if (A.divf) {
+      if (g) k_p2p_f32<2, 1, 0><<<grid, block, 0, h->stream>>>(A, nullptr, nullptr, 0, nullptr);
+      else k_p2p_f32<2, 0, 0><<<grid, block, 0, h->stream>>>(A, nullptr, nullptr, 0, nullptr);
+    } else if (A.use_vs) {
       if (g) k_p2p_f32<1, 1, 0><<<grid, block, 0, h->stream>>>(A, nullptr, nullptr, 0, nullptr);
       else k_p2p_f32<1, 0, 0><<<grid, block, 0, h->stream>>>(A, nullptr, nullptr, 0, nullptr);
     } else {

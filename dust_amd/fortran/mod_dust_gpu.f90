@@ -26,7 +26,8 @@ module mod_dust_gpu
     real(c_double)    :: u_inf(3)
     integer(c_int)    :: precision
     integer(c_int)    :: kernel = 0   ! DUST_KERNEL_ROSENHEAD
-    integer(c_int)    :: reserved(6)
+    integer(c_int)    :: use_divfilt = 0
+    integer(c_int)    :: reserved(5)
   end type
 
   type, bind(C) :: t_dust_aic_params

@@ -43,6 +43,7 @@ class VortParts:
         # count of the others (the reference's n_prt)
         self.free = np.zeros(n, bool)
         self.n_prt = n
+        self.rotu = np.zeros((n, 3))
 
     @classmethod
     def from_alpha(cls, cen, alpha, r_vortex):
@@ -158,10 +159,12 @@ class Octree:
 
 def make_params(box_length, nbox, origin, nlevels, min_part, degree, delta, use_vs=True,
                 use_vd=False, use_tv=False, nu_inf=0.0, u_inf=(0.0, 0.0, 0.0),
-                precision="fp64", kernel="rosenhead"):
+                precision="fp64", kernel="rosenhead", use_divfilt=False):
     """kernel: the particle core of kernel_coeffs, "rosenhead" (the
     reference's) or "gaussian" (its commented alternative,
-    src/aeroel/mod_vortpart.f90:283-291)."""
+    src/aeroel/mod_vortpart.f90:283-291).  use_divfilt (with use_vs):
+    apply_multipole also fills parts.rotu, the divergence filter's curl of
+    the velocity (mod_octree.f90:1207-1212, 1274-1336)."""
     p = L.FmmParams()
     p.box_length = float(box_length)
     for d in range(3):
@@ -178,6 +181,7 @@ def make_params(box_length, nbox, origin, nlevels, min_part, degree, delta, use_
     p.nu_inf = float(nu_inf)
     p.precision = {"fp64": L.FMM_FP64, "tuned": L.FMM_TUNED}[precision]
     p.kernel = KERNELS[kernel]
+    p.use_divfilt = int(bool(use_divfilt))
     return p
 
 
@@ -241,6 +245,8 @@ def apply_multipole(parts, octree):
     """apply_multipole (mod_octree.f90:1139-1429), particle terms + free stream."""
     L.check(L.lib().dust_fmm_apply_multipole(octree.handle, L.ptr(parts.vel), L.ptr(parts.stretch),
                                              L.ptr(parts.turbvisc)), "apply_multipole")
+    if octree.params.use_divfilt and octree.params.use_vs:
+        L.check(L.lib().dust_fmm_rotu(octree.handle, L.ptr(parts.rotu)), "apply_multipole")
 
 
 def apply_multipole_panels(octree, cen, uvort=None):

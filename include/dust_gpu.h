@@ -72,7 +72,11 @@ typedef struct {
   double u_inf[3];       /* uniform free stream (variable_wind without gust) */
   int precision;         /* DUST_FMM_FP64 or DUST_FMM_TUNED                  */
   int kernel;            /* DUST_KERNEL_ROSENHEAD (0) or DUST_KERNEL_GAUSSIAN */
-  int reserved[6];
+  int use_divfilt;       /* with use_vs: apply_multipole also forms rotu, the */
+                         /* divergence filter's curl of the velocity          */
+                         /* (mod_octree.f90:1207-1212,1274-1336), read with   */
+                         /* dust_fmm_rotu                                     */
+  int reserved[5];
 } dust_fmm_params;
 
 typedef struct dust_fmm_s *dust_fmm_t;
@@ -131,6 +135,14 @@ int dust_fmm_calculate_multipole(dust_fmm_t h);
 int dust_fmm_apply_multipole(dust_fmm_t h, double *vel, double *stretch, double *turbvisc);
 /* Asynchronous form: results stay on the device (see dust_fmm_results_dev). */
 int dust_fmm_apply_multipole_dev(dust_fmm_t h);
+/* rotu(3,n) of the last apply_multipole, input particle order (the
+ * t_vortpart%rotu the reference writes with use_divfilt: the curl of the
+ * local expansion's velocity plus compute_rotu / (4 pi) of every P2P source,
+ * mod_vortpart.f90:236-256; zero for particles freed by a redistribution).
+ * DUST_E_ARG unless the handle was created with use_vs and use_divfilt.
+ * Host pointer (synchronous) / device pointer on the handle's stream. */
+int dust_fmm_rotu(dust_fmm_t h, double *rotu);
+int dust_fmm_rotu_dev(dust_fmm_t h, double *rotu);
 /* Device-to-device copy of the last results (input particle order). */
 int dust_fmm_results_dev(dust_fmm_t h, double *vel, double *stretch, double *turbvisc);
 

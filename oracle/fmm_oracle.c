@@ -248,6 +248,21 @@ static inline void p_stretch(const orc_parts *P, int q, const double pos[3],
   for (int e = 0; e < 3; ++e) s[e] = -(c * vp[e] + d * dist[e] * dv);
 }
 
+/* compute_rotu_vortpart, mod_vortpart.f90:236-256: the curl of the
+ * particle's velocity field at pos, -2 c alpha_s + d dist x (dist x alpha_s) */
+static inline void p_rotu(const orc_parts *P, int q, const double pos[3], double ru[3]) {
+  double dist[3] = {pos[0] - P->cen[3 * q], pos[1] - P->cen[3 * q + 1],
+                    pos[2] - P->cen[3 * q + 2]};
+  double c, d;
+  kernel_coeffs(P->rv[q], norm2_3(dist), &c, &d);
+  double am[3] = {P->dir[3 * q] * P->mag[q], P->dir[3 * q + 1] * P->mag[q],
+                  P->dir[3 * q + 2] * P->mag[q]};
+  double in[3], out[3];
+  cross3(dist, am, in);
+  cross3(dist, in, out);
+  for (int e = 0; e < 3; ++e) ru[e] = -2.0 * c * P->dir[3 * q + e] * P->mag[q] + d * out[e];
+}
+
 /* etaeps, mod_vortpart.f90:328-336 */
 static inline double etaeps(double dist, double eps) {
   double eta = 105.0 / (8.0 * ORC_PI) / pow((dist / eps) * (dist / eps) + 1.0, 4.5);
@@ -266,6 +281,13 @@ static inline void p_diff(const orc_parts *P, int q, const double pos[3],
   double eta = etaeps(distn, r);
   for (int e = 0; e < 3; ++e)
     df[e] = 1.0 / (r * r) * (vol * P->dir[3 * q + e] * P->mag[q] - vol * alpha[e]) * eta;
+}
+
+/* compute_rotu of one source at pos (closed-form tests). */
+void orc_pair_rotu(const double src_cen[3], const double src_dir[3], double src_mag, double src_r,
+                   const double pos[3], double rot[3]) {
+  orc_parts P = {src_cen, src_dir, &src_mag, &src_r};
+  p_rotu(&P, 0, pos, rot);
 }
 
 /* Single-pair kernels exported for closed-form tests. */
@@ -980,7 +1002,7 @@ void orc_calculate_multipole(orc_octree *T) {
 /* P2P of target particle ip against the particle list of one cell. */
 static inline void p2p_list(const orc_octree *T, const int *plist, int np, int skip,
                             const double pos[3], const double alpha[3], double tv,
-                            double vel[3], double str[3]) {
+                            double vel[3], double str[3], double rot[3]) {
   const double fpi = 4.0 * ORC_PI;
   double t[3];
   for (int q = 0; q < np; ++q) {
@@ -991,6 +1013,10 @@ static inline void p2p_list(const orc_octree *T, const int *plist, int np, int s
     if (T->use_vs) {
       p_stretch(&T->P, iq, pos, alpha, t);
       for (int e = 0; e < 3; ++e) str[e] += t[e] / fpi;
+      if (rot) {
+        p_rotu(&T->P, iq, pos, t);
+        for (int e = 0; e < 3; ++e) rot[e] += t[e] / fpi;
+      }
     }
     if (T->use_vd) {
       p_diff(&T->P, iq, pos, alpha, t);
@@ -1001,8 +1027,12 @@ static inline void p2p_list(const orc_octree *T, const int *plist, int np, int s
 
 /* apply_multipole, mod_octree.f90:1139-1429, particle terms only (the body
  * and wake-element terms are a separate entry point) plus the uniform free
- * stream.  Output in input particle order. */
-void orc_apply_multipole(orc_octree *T, double *vel_out, double *str_out, double *tv_out) {
+ * stream.  Output in input particle order.  rot_out (may be NULL): the
+ * divergence filter's rotu (use_divfilt with use_vs, 1207-1212, 1274-1336):
+ * the curl of the local expansion's velocity, grad(2,3) - grad(3,2), ...,
+ * plus compute_rotu / (4 pi) of every P2P source. */
+void orc_apply_multipole_rotu(orc_octree *T, double *vel_out, double *str_out, double *tv_out,
+                              double *rot_out) {
   int nm = T->pexp.n_mon;
 #pragma omp parallel for schedule(dynamic)
   for (int lv = 0; lv < T->nleaves; ++lv) {
@@ -1015,7 +1045,9 @@ void orc_apply_multipole(orc_octree *T, double *vel_out, double *str_out, double
     const double *lc = &L->cen[3 * c];
     double *velv = (double *)malloc(sizeof(double) * 3 * (np + 1));
     double *strv = (double *)calloc(3 * (np + 1), sizeof(double));
+    double *rotv = (double *)calloc(3 * (np + 1), sizeof(double));
     double ave_ros = 0.0;
+    double *rot = (rot_out && T->use_vs) ? rotv : NULL;
     /* L2P, 1171-1219 */
     for (int q = 0; q < np; ++q) {
       int ip = pl[q];
@@ -1036,6 +1068,11 @@ void orc_apply_multipole(orc_octree *T, double *vel_out, double *str_out, double
         /* str = matmul(alpha, transpose(grad)) : str_j = sum_i grad(j,i) alpha_i */
         for (int jj = 0; jj < 3; ++jj)
           strv[3 * q + jj] = g[jj + 0] * alpha[0] + g[jj + 3] * alpha[1] + g[jj + 6] * alpha[2];
+        if (rot) {  /* grad(i,j) = g[(i-1) + 3 (j-1)] */
+          rotv[3 * q] = g[7] - g[5];
+          rotv[3 * q + 1] = g[2] - g[6];
+          rotv[3 * q + 2] = g[3] - g[1];
+        }
       }
       if (T->use_vd) {
         double s = 0.0;
@@ -1056,6 +1093,8 @@ void orc_apply_multipole(orc_octree *T, double *vel_out, double *str_out, double
                          T->P.mag[ip] * T->P.dir[3 * ip + 2]};
       double v[3] = {velv[3 * q], velv[3 * q + 1], velv[3 * q + 2]};
       double s[3] = {strv[3 * q], strv[3 * q + 1], strv[3 * q + 2]};
+      double rt[3] = {rotv[3 * q], rotv[3 * q + 1], rotv[3 * q + 2]};
+      double *rq = rot ? rt : NULL;
       double tv = 0.0;
       if (T->use_tv) {
         double rr = 0.11 * T->P.rv[ip];
@@ -1072,23 +1111,30 @@ void orc_apply_multipole(orc_octree *T, double *vel_out, double *str_out, double
               continue;
             int nc = cidx(L, a, bb, d);
             if (!L->active[nc]) continue;
-            p2p_list(T, &L->plist[L->pstart[nc]], L->npart[nc], -1, pos, alpha, tv, v, s);
+            p2p_list(T, &L->plist[L->pstart[nc]], L->npart[nc], -1, pos, alpha, tv, v, s, rq);
           }
       for (int q2 = 0; q2 < L->ln_count[c]; ++q2) {
         orc_layer *Ln = &T->L[L->ln_lev[L->ln_start[c] + q2]];
         int nc = L->ln_cell[L->ln_start[c] + q2];
-        p2p_list(T, &Ln->plist[Ln->pstart[nc]], Ln->npart[nc], -1, pos, alpha, tv, v, s);
+        p2p_list(T, &Ln->plist[Ln->pstart[nc]], Ln->npart[nc], -1, pos, alpha, tv, v, s, rq);
       }
-      p2p_list(T, pl, np, ip, pos, alpha, tv, v, s);
+      p2p_list(T, pl, np, ip, pos, alpha, tv, v, s, rq);
       for (int e = 0; e < 3; ++e) {
         vel_out[3 * ip + e] = v[e] + T->uinf[e];
         if (str_out) str_out[3 * ip + e] = s[e];
       }
       if (tv_out) tv_out[ip] = tv;
+      if (rot)
+        for (int e = 0; e < 3; ++e) rot_out[3 * ip + e] = rt[e];
     }
     free(velv);
     free(strv);
+    free(rotv);
   }
+}
+
+void orc_apply_multipole(orc_octree *T, double *vel_out, double *str_out, double *tv_out) {
+  orc_apply_multipole_rotu(T, vel_out, str_out, tv_out, NULL);
 }
 
 /* apply_multipole_panels, mod_octree.f90:1434-1510, with the panel binning

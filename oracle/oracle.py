@@ -45,6 +45,8 @@ def lib():
             "orc_octree_redistributed": (C.c_int, [_vp, _vp, _vp, _vp]),
             "orc_calculate_multipole": (None, [_vp]),
             "orc_apply_multipole": (None, [_vp, _vp, _vp, _vp]),
+            "orc_apply_multipole_rotu": (None, [_vp, _vp, _vp, _vp, _vp]),
+            "orc_pair_rotu": (None, [_vp, _vp, C.c_double, C.c_double, _vp, _vp]),
             "orc_octree_nleaves": (C.c_int, [_vp]),
             "orc_octree_leaves": (None, [_vp, _vp, _vp, _vp, _vp, _vp]),
             "orc_octree_work": (None, [_vp, _vp, _vp]),
@@ -233,6 +235,15 @@ class Octree:
             lib().orc_apply_multipole(self._h, _p(vel), _p(st), _p(tv))
         return vel, st, tv
 
+    def apply_rotu(self):
+        """apply_multipole with the divergence filter's rotu: (vel, stretch,
+        turbvisc, rotu)."""
+        n = len(self._keep[2])
+        vel, st, tv, ru = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n), np.zeros((n, 3))
+        with _Core(self.kernel):
+            lib().orc_apply_multipole_rotu(self._h, _p(vel), _p(st), _p(tv), _p(ru))
+        return vel, st, tv, ru
+
     def apply_panels(self, cen, uvort=None):
         """apply_multipole_panels at the centres cen (m,3); uvort accumulated."""
         cen = _c(cen).reshape(-1, 3)
@@ -269,6 +280,15 @@ class Octree:
         a, b, c = np.zeros((nm, 3)), np.zeros((nm, 3)), np.zeros((nm, 3, 3))
         rc = lib().orc_octree_cell_mp(self._h, level, _p(_c(ijk, np.int32)), _p(a), _p(b), _p(c))
         return None if rc else (a, b, c)
+
+
+def pair_rotu(src_cen, src_dir, src_mag, src_r, pos, kernel=ROSENHEAD):
+    """compute_rotu of one source particle at pos."""
+    out = np.zeros(3)
+    with _Core(kernel):
+        lib().orc_pair_rotu(_p(_c(src_cen)), _p(_c(src_dir)), float(src_mag), float(src_r),
+                            _p(_c(pos)), _p(out))
+    return out
 
 
 def p2m(degree, cen, pcen, pdir, pmag):
