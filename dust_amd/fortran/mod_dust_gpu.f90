@@ -74,6 +74,9 @@ module mod_dust_gpu
       integer(c_int8_t), intent(out) :: keep(*)
       real(c_double), intent(inout) :: dir(3,*), mag(*)
     end function
+    integer(c_int) function dust_fmm_rotu(h, rotu) bind(C)
+      import; type(c_ptr), value :: h; real(c_double), intent(out) :: rotu(3,*)
+    end function
     integer(c_int) function dust_fmm_calculate_multipole(h) bind(C)
       import; type(c_ptr), value :: h
     end function

@@ -31,6 +31,7 @@ module mod_octree_gpu
     logical  :: use_vs = .true.          ! vortstretch
     logical  :: use_vd = .true.          ! diffusion
     logical  :: use_tv = .false.         ! turbulent_viscosity
+    logical  :: use_divfilt = .false.    ! divergence_filtering (rotu)
     real(wp) :: nu_inf = 0.0_wp          ! mu_inf / rho_inf of the case
     real(wp) :: u_inf(3) = 0.0_wp        ! uniform wind (variable_wind without gust)
     logical  :: vs_elems = .false.       ! vortstretch_from_elems
@@ -59,6 +60,7 @@ module mod_octree_gpu
   type :: t_vortparts
     real(wp), allocatable :: cen(:,:), dir(:,:), mag(:), r_Vortex(:)
     real(wp), allocatable :: vel(:,:), stretch(:,:), turbvisc(:)
+    real(wp), allocatable :: rotu(:,:)   ! with use_divfilt and use_vs
     logical, allocatable :: free(:)
   end type
 
@@ -108,6 +110,7 @@ contains
     prm%u_inf = gpu_sim_param%u_inf
     prm%precision = merge(DUST_FMM_TUNED, DUST_FMM_FP64, gpu_sim_param%tuned)
     prm%kernel = DUST_KERNEL_ROSENHEAD
+    prm%use_divfilt = merge(1_c_int, 0_c_int, gpu_sim_param%use_divfilt)
     prm%reserved = 0
     octree%nlevels = nlevels
     octree%lvl_solid = gpu_sim_param%lvl_solid
@@ -221,7 +224,7 @@ contains
     type(t_vortparts), intent(inout) :: part
     type(t_octree), intent(inout) :: octree
     type(t_elems), intent(in), target :: elem, wpan, wrin, wvort
-    real(wp), allocatable, target :: vel(:,:), stretch(:,:), tv(:), alpha(:,:)
+    real(wp), allocatable, target :: vel(:,:), stretch(:,:), tv(:), alpha(:,:), ru(:,:)
     type(t_dust_elem_group) :: grp(4)
     type(t_dust_elem_params) :: eprm
     integer :: ip, ipp, n, ng
@@ -229,6 +232,10 @@ contains
     n = octree%n
     allocate(vel(3, max(n, 1)), stretch(3, max(n, 1)), tv(max(n, 1)), alpha(3, max(n, 1)))
     call check(dust_fmm_apply_multipole(octree%gpu, vel, stretch, tv), 'apply_multipole')
+    if (octree%prm%use_divfilt /= 0 .and. octree%prm%use_vs /= 0) then
+      allocate(ru(3, max(n, 1)))
+      call check(dust_fmm_rotu(octree%gpu, ru), 'apply_multipole')
+    end if
     ng = 0
     call add_group(elem, 1, gpu_sim_param%vs_elems)
     call add_group(wpan, 1, gpu_sim_param%vs_elems)
@@ -259,6 +266,7 @@ contains
       part%vel(:, ipp) = vel(:, ip)
       part%stretch(:, ipp) = stretch(:, ip)
       part%turbvisc(ipp) = tv(ip)
+      if (allocated(ru) .and. allocated(part%rotu)) part%rotu(:, ipp) = ru(:, ip)
     end do
   contains
     subroutine add_group(e, st, want)
