@@ -54,6 +54,25 @@ class FmmParams(C.Structure):
     ]
 
 
+class WindParams(C.Structure):
+    _fields_ = [
+        ("u_inf", C.c_double * 3),
+        ("gust_type", C.c_int),
+        ("reserved0", C.c_int),
+        ("origin", C.c_double * 3),
+        ("front_direction", C.c_double * 3),
+        ("front_speed", C.c_double),
+        ("u_des", C.c_double),
+        ("perturb_direction", C.c_double * 3),
+        ("gradient", C.c_double),
+        ("start_time", C.c_double),
+        ("reserved", C.c_int * 4),
+    ]
+
+
+GUST_NONE, GUST_ACM, GUST_LINEAR = 0, 1, 2
+
+
 class AicParams(C.Structure):
     _fields_ = [
         ("ff_ratio_doublet", C.c_double),
@@ -168,6 +187,10 @@ SIGNATURES = [
     ("dust_fmm_apply_multipole", C.c_int, [_vp, _vp, _vp, _vp]),
     ("dust_fmm_apply_multipole_dev", C.c_int, [_vp]),
     ("dust_fmm_rotu", C.c_int, [_vp, _vp]),
+    ("dust_fmm_set_wind", C.c_int, [_vp, C.POINTER(WindParams), C.c_double]),
+    ("dust_variable_wind", C.c_int, [C.c_int, _vp, C.POINTER(WindParams), C.c_double, _vp]),
+    ("dust_variable_wind_dev", C.c_int,
+     [C.c_int, _vp, C.POINTER(WindParams), C.c_double, _vp, _vp]),
     ("dust_fmm_rotu_dev", C.c_int, [_vp, _vp]),
     ("dust_fmm_results_dev", C.c_int, [_vp, _vp, _vp, _vp]),
     ("dust_fmm_apply_multipole_panels", C.c_int, [_vp, C.c_int, _vp, _vp]),

@@ -44,6 +44,7 @@ SOURCES = [
     "linsys.hip",
     "linsys_asm.hip",
     "pressure.hip",
+    "wind.hip",
 ]
 
 CFLAGS = [

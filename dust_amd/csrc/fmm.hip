@@ -956,8 +956,12 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_eval_finish(EvalArgs A) 
       pv[e] += o[(long)k * kWave * ps_ + e];
       ps[e] += o[(long)k * kWave * ps_ + 3 + e];
     }
+  // the wind at the particle (mod_octree.f90:1376): u_inf without a gust
+  double wnd[3];
+  const double pos[3] = {x, y, z};
+  variable_wind(A.wind, pos, A.time, wnd);
   for (int e = 0; e < 3; ++e) {
-    A.vel[3 * t + e] = v[e] + pv[e] + A.uinf[e];
+    A.vel[3 * t + e] = v[e] + pv[e] + wnd[e];
     A.str[3 * t + e] = st[e] + ps[e];
   }
   if (A.divf) {
@@ -1142,6 +1146,8 @@ static int fmm_create(const dust_fmm_params *prm, int capacity, FmmHandle **out)
   DUST_CHECK(nfine < (1L << 31) - 1, DUST_E_ARG, "finest level has too many cells");
   FmmHandle *h = new FmmHandle();
   h->prm = *prm;
+  for (int e = 0; e < 3; ++e) h->wind.u_inf[e] = prm->u_inf[e];
+  h->wind.type = DUST_GUST_NONE;
   h->cap = capacity;
   h->nlev = prm->nlevels;
   h->deg = prm->degree;
@@ -1595,6 +1601,8 @@ void make_eval_args(const FmmHandle *h, EvalArgs *Ap) {
   A.near2 = 0.0;
   A.part = h->part.p;
   A.vel = h->vel_s.p; A.str = h->str_s.p; A.tv = h->tv_s.p;
+  A.wind = h->wind;
+  A.time = h->wind_time;
   A.divf = (h->prm.use_vs && h->prm.use_divfilt) ? 1 : 0;
   A.pstride = A.divf ? 9 : 6;
   A.rot = h->rot_s.p;
@@ -1970,6 +1978,23 @@ int dust_fmm_rotu(dust_fmm_t hh, double *rotu) {
 int dust_fmm_rotu_dev(dust_fmm_t hh, double *rotu) {
   DUST_CHECK(hh, DUST_E_ARG, "null handle");
   return rotu_copy(hh->h, rotu, hipMemcpyDeviceToDevice);
+}
+
+int dust_fmm_set_wind(dust_fmm_t hh, const dust_wind_params *w, double time) {
+  DUST_CHECK(hh, DUST_E_ARG, "null handle");
+  FmmHandle *h = hh->h;
+  if (!w) {
+    h->wind = WindDev{};
+    for (int e = 0; e < 3; ++e) h->wind.u_inf[e] = h->prm.u_inf[e];
+    h->wind.type = DUST_GUST_NONE;
+    h->wind_time = 0.0;
+    return 0;
+  }
+  DUST_CHECK(w->gust_type >= DUST_GUST_NONE && w->gust_type <= DUST_GUST_LINEAR, DUST_E_ARG,
+             "unknown gust type %d", w->gust_type);
+  h->wind = wind_dev(*w);
+  h->wind_time = time;
+  return 0;
 }
 
 int dust_fmm_results_dev(dust_fmm_t hh, double *vel, double *stretch, double *turbvisc) {

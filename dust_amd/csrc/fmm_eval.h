@@ -7,6 +7,7 @@
 // list between them, so that small leaves still fill the wavefront.
 #pragma once
 #include "fmm_geom.h"
+#include "wind.h"
 
 namespace dust {
 
@@ -51,6 +52,8 @@ struct EvalArgs {
                            // and, with divf, rotu (3): pstride doubles
   int divf, pstride;       // divergence filter's rotu (use_vs && use_divfilt)
   double *vel, *str, *tv, *rot;
+  WindDev wind;            // the free-stream term: variable_wind(pos, time)
+  double time;
 };
 
 // Gaussian core of kernel_coeffs (the commented alternative of

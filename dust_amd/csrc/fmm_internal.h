@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "common.h"
+#include "wind.h"
 
 namespace dust {
 
@@ -123,6 +124,9 @@ struct FmmHandle {
   DevBuf<unsigned> pr_key;
   DevBuf<double> pr_stage[8];
   bool pr_done = false;  // the last sort ran the redistribution
+  // the free-stream term of apply_multipole (variable_wind, time)
+  WindDev wind{};
+  double wind_time = 0.0;
   int ns = 0;            // particles in the tree after the last sort
 };
 

@@ -156,6 +156,13 @@ class Octree:
     def stream(self):
         return L.lib().dust_fmm_stream(self.handle)
 
+    def set_wind(self, wind, time):
+        """The free-stream term of the next passes: variable_wind(pos, time)
+        at every particle (mod_octree.f90:1376); wind from wind_params, None
+        for the uniform u_inf of the parameters."""
+        L.check(L.lib().dust_fmm_set_wind(self.handle, None if wind is None else C.byref(wind),
+                                          float(time)), "set_wind")
+
 
 def make_params(box_length, nbox, origin, nlevels, min_part, degree, delta, use_vs=True,
                 use_vd=False, use_tv=False, nu_inf=0.0, u_inf=(0.0, 0.0, 0.0),
@@ -186,6 +193,39 @@ def make_params(box_length, nbox, origin, nlevels, min_part, degree, delta, use_
 
 
 KERNELS = {"rosenhead": L.KERNEL_ROSENHEAD, "gaussian": L.KERNEL_GAUSSIAN}
+GUSTS = {None: L.GUST_NONE, "ACM": L.GUST_ACM, "linear": L.GUST_LINEAR}
+
+
+def wind_params(u_inf, gust_type=None, origin=(0.0, 0.0, 0.0), front_direction=None,
+                front_speed=None, u_des=0.0, perturb_direction=(0.0, 0.0, 1.0), gradient=1.0,
+                start_time=0.0):
+    """variable_wind's parameters (the sim_param gust options,
+    mod_sim_param.f90:476-485, 814-835): gust_type "ACM", "linear" or None
+    (any other string of the reference adds nothing).  front_direction and
+    front_speed default to u_inf and |u_inf|, as the reference fills them."""
+    w = L.WindParams()
+    u = np.asarray(u_inf, dtype=np.float64)
+    fd = u if front_direction is None else np.asarray(front_direction, dtype=np.float64)
+    for d in range(3):
+        w.u_inf[d] = float(u[d])
+        w.origin[d] = float(origin[d])
+        w.front_direction[d] = float(fd[d])
+        w.perturb_direction[d] = float(perturb_direction[d])
+    w.gust_type = GUSTS.get(gust_type, L.GUST_NONE)
+    w.front_speed = float(np.linalg.norm(u) if front_speed is None else front_speed)
+    w.u_des = float(u_des)
+    w.gradient = float(gradient)
+    w.start_time = float(start_time)
+    return w
+
+
+def variable_wind(wind, pos, time):
+    """variable_wind (src/aeroel/mod_wind.f90:66-121) at the points pos (n,3)."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros_like(pos)
+    L.check(L.lib().dust_variable_wind(len(pos), L.ptr(pos), C.byref(wind), float(time),
+                                       L.ptr(out)), "variable_wind")
+    return out
 
 
 def initialize_octree(box_length, nbox, origin, nlevels, min_part, degree, delta, *,

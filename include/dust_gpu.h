@@ -232,6 +232,50 @@ int dust_vortpart_direct_k(int n, const double *cen, const double *dir, const do
                            const double u_inf[3], int kernel, double *vel, double *stretch);
 
 /* ======================================================================== */
+/* Wind: variable_wind, src/aeroel/mod_wind.f90:66-121                      */
+/* ======================================================================== */
+
+/* Gust forms (sim_param%GustType with sim_param%use_gust): the reference's
+ * 'ACM' one-minus-cosine gust and its 'linear' test form; any other type,
+ * including the option's default string 'AMC', adds nothing (the select
+ * case of mod_wind.f90:82-117 matches only 'ACM' and 'linear'). */
+#define DUST_GUST_NONE 0
+#define DUST_GUST_ACM 1
+#define DUST_GUST_LINEAR 2
+
+typedef struct {
+  double u_inf[3];               /* sim_param%u_inf                                  */
+  int gust_type;                 /* DUST_GUST_*                                      */
+  int reserved0;
+  double origin[3];              /* gust_origin                                      */
+  double front_direction[3];     /* gust_front_direction (normalised here)          */
+  double front_speed;            /* gust_front_speed                                 */
+  double u_des;                  /* gust_u_des                                       */
+  double perturb_direction[3];   /* gust_perturb_direction: read, unused by both    */
+                                 /* forms, as in the reference                       */
+  double gradient;               /* gust_gradient                                    */
+  double start_time;             /* gust_time (gust_start_time)                      */
+  int reserved[4];
+} dust_wind_params;
+
+/* wind(3,n) = variable_wind(pos(:,i), time) at n points pos(3,n): u_inf,
+ * plus for an ACM gust u_des/2 (1 - cos(pi s / gradient)) on every component
+ * where the penetration s = -(pos - (origin + front_speed fd (time -
+ * start_time))) . fd behind the moving front lies in [0, 2 gradient)
+ * (fd = front_direction normalised), or for the linear form 0.1 pos_x on z.
+ * Host pointers / device pointers on `stream`. */
+int dust_variable_wind(int n, const double *pos, const dust_wind_params *w, double time,
+                       double *wind);
+int dust_variable_wind_dev(int n, const double *pos, const dust_wind_params *w, double time,
+                           double *wind, void *stream);
+
+/* FMM: the free-stream term of apply_multipole as the reference forms it,
+ * variable_wind(pos, time) at every particle (mod_octree.f90:1376), for the
+ * next passes: w->u_inf replaces the params' u_inf; time = sim_param%time.
+ * w = NULL returns to the params' uniform u_inf. */
+int dust_fmm_set_wind(dust_fmm_t h, const dust_wind_params *w, double time);
+
+/* ======================================================================== */
 /* Element <-> particle direct terms: src/fmm/mod_octree.f90:1350-1401,     */
 /* get_vort_vel (src/aeroel/mod_surfpan.f90:962-977, mod_vortlatt.f90:676) */
 /* ======================================================================== */

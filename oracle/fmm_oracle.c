@@ -379,6 +379,11 @@ typedef struct {
   double *pr_cen;           /* element centres [3*pr_m]                    */
   double *pr_mag, *pr_dir;  /* working copies of the particle strengths    */
   unsigned char *pr_keep;   /* 0 for the particles the redistribution freed */
+  /* variable_wind of the free-stream term (gust type 0 = none) */
+  int gust_type;
+  double gust[13];          /* origin(3), front_direction(3), front_speed, u_des,
+                               gradient, start_time, u_inf(3) */
+  double wind_time;
 } orc_octree;
 
 static inline int cidx(const orc_layer *L, int i, int j, int k) {
@@ -497,6 +502,40 @@ void orc_octree_destroy(orc_octree *T) {
 static inline int child_of(const orc_layer *Lc, int i, int j, int k, int p) {
   int ic = p & 1, jc = (p >> 1) & 1, kc = (p >> 2) & 1;
   return cidx(Lc, 2 * (i - 1) + ic + 1, 2 * (j - 1) + jc + 1, 2 * (k - 1) + kc + 1);
+}
+
+/* variable_wind, mod_wind.f90:66-121, at one point: u_inf; 'ACM' gust
+ * (type 1): fd = front_direction / norm2(front_direction), s = -sum((pos -
+ * (origin + speed * fd * (time - start))) * fd), and for 0 <= s < 2 gradient
+ * the scalar u_des/2 * (1 - cos(pi * s / gradient)) added to every component
+ * (92-110); 'linear' (type 2): + pos(1) * (0, 0, 0.1) (112-124).
+ * g = origin(3), front_direction(3), front_speed, u_des, gradient,
+ * start_time, u_inf(3). */
+void orc_variable_wind(int type, const double *g, const double pos[3], double time,
+                       double wind[3]) {
+  for (int e = 0; e < 3; ++e) wind[e] = g[10 + e];
+  if (type == 1) {
+    double n = sqrt(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]);
+    double fd[3] = {g[3] / n, g[4] / n, g[5] / n};
+    double s = 0.0;
+    for (int e = 0; e < 3; ++e) s += (pos[e] - (g[e] + g[6] * fd[e] * (time - g[9]))) * fd[e];
+    s = -s;
+    if (s >= 0.0 && s < 2.0 * g[8]) {
+      double add = g[7] / 2.0 * (1.0 - cos(ORC_PI * s / g[8]));
+      for (int e = 0; e < 3; ++e) wind[e] += add;
+    }
+  } else if (type == 2) {
+    wind[2] += pos[0] * 0.1;
+  }
+}
+
+/* The free-stream term of the next orc_apply_multipole: variable_wind at
+ * each particle (mod_octree.f90:1376) instead of the uniform uinf. */
+void orc_octree_set_wind(orc_octree *T, int type, const double *g, double time) {
+  T->gust_type = type;
+  memcpy(T->gust, g, sizeof(double) * 13);
+  T->wind_time = time;
+  for (int e = 0; e < 3; ++e) T->uinf[e] = g[10 + e];
 }
 
 /* Particle redistribution settings (sim_param use_pr, lvl_solid 1-based,
@@ -1119,8 +1158,10 @@ void orc_apply_multipole_rotu(orc_octree *T, double *vel_out, double *str_out, d
         p2p_list(T, &Ln->plist[Ln->pstart[nc]], Ln->npart[nc], -1, pos, alpha, tv, v, s, rq);
       }
       p2p_list(T, pl, np, ip, pos, alpha, tv, v, s, rq);
+      double wnd[3] = {T->uinf[0], T->uinf[1], T->uinf[2]};
+      if (T->gust_type) orc_variable_wind(T->gust_type, T->gust, pos, T->wind_time, wnd);
       for (int e = 0; e < 3; ++e) {
-        vel_out[3 * ip + e] = v[e] + T->uinf[e];
+        vel_out[3 * ip + e] = v[e] + wnd[e];
         if (str_out) str_out[3 * ip + e] = s[e];
       }
       if (tv_out) tv_out[ip] = tv;

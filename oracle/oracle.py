@@ -47,6 +47,8 @@ def lib():
             "orc_apply_multipole": (None, [_vp, _vp, _vp, _vp]),
             "orc_apply_multipole_rotu": (None, [_vp, _vp, _vp, _vp, _vp]),
             "orc_pair_rotu": (None, [_vp, _vp, C.c_double, C.c_double, _vp, _vp]),
+            "orc_variable_wind": (None, [C.c_int, _vp, _vp, C.c_double, _vp]),
+            "orc_octree_set_wind": (None, [_vp, C.c_int, _vp, C.c_double]),
             "orc_octree_nleaves": (C.c_int, [_vp]),
             "orc_octree_leaves": (None, [_vp, _vp, _vp, _vp, _vp, _vp]),
             "orc_octree_work": (None, [_vp, _vp, _vp]),
@@ -235,6 +237,10 @@ class Octree:
             lib().orc_apply_multipole(self._h, _p(vel), _p(st), _p(tv))
         return vel, st, tv
 
+    def set_wind(self, gust_type, gust, time):
+        """variable_wind as the free-stream term (gust: see wind_params)."""
+        lib().orc_octree_set_wind(self._h, int(gust_type), _p(_c(gust)), float(time))
+
     def apply_rotu(self):
         """apply_multipole with the divergence filter's rotu: (vel, stretch,
         turbvisc, rotu)."""
@@ -280,6 +286,25 @@ class Octree:
         a, b, c = np.zeros((nm, 3)), np.zeros((nm, 3)), np.zeros((nm, 3, 3))
         rc = lib().orc_octree_cell_mp(self._h, level, _p(_c(ijk, np.int32)), _p(a), _p(b), _p(c))
         return None if rc else (a, b, c)
+
+
+def wind_params(u_inf, origin=(0, 0, 0), front_direction=(1, 0, 0), front_speed=0.0, u_des=0.0,
+                gradient=1.0, start_time=0.0):
+    """The 13 numbers of the oracle's gust description."""
+    return np.array([*origin, *front_direction, front_speed, u_des, gradient, start_time, *u_inf],
+                    dtype=np.float64)
+
+
+def variable_wind(gust_type, gust, pos, time):
+    """variable_wind (mod_wind.f90:66-121) at the points pos (n,3)."""
+    pos = _c(pos).reshape(-1, 3)
+    out = np.zeros_like(pos)
+    g = _c(gust)
+    for i in range(len(pos)):
+        w = np.zeros(3)
+        lib().orc_variable_wind(int(gust_type), _p(g), _p(_c(pos[i])), float(time), _p(w))
+        out[i] = w
+    return out
 
 
 def pair_rotu(src_cen, src_dir, src_mag, src_r, pos, kernel=ROSENHEAD):
