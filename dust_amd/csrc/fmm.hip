@@ -504,8 +504,13 @@ __global__ void k_m2m(TreeGeom G, int lvl, int deg, int nmon, const int *__restr
                       const unsigned char *__restrict__ flags, const int *__restrict__ slot,
                       double *mp_a) {
   __shared__ double dp[8][3][11];
+  __shared__ double sB[11 * 11];  // binomials fact[n] / (fact[r] fact[n-r]), as below
   long g = active[a0i + blockIdx.x];
   if (flags[g] != kBranch) return;
+  for (int t = threadIdx.x; t < (deg + 1) * (deg + 1); t += blockDim.x) {
+    const int n = t / (deg + 1), r = t - n * (deg + 1);
+    sB[t] = r <= n ? fact[n] / (fact[r] * fact[n - r]) : 0.0;
+  }
   long c = g - G.off[lvl];
   int ix, iy, iz;
   cell_decode(G, lvl, c, ix, iy, iz);
@@ -532,8 +537,8 @@ __global__ void k_m2m(TreeGeom G, int lvl, int deg, int nmon, const int *__restr
       for (int js = 0; js <= j; ++js)
         for (int is = 0; is <= i; ++is) {
           int s = mono_index(is, js, ks);
-          double nb = (fact[i] / (fact[is] * fact[i - is])) * (fact[j] / (fact[js] * fact[j - js])) *
-                      (fact[k] / (fact[ks] * fact[k - ks]));
+          const int w = deg + 1;
+          double nb = sB[i * w + is] * sB[j * w + js] * sB[k * w + ks];
           double f = nb * (dp[p][0][i - is] * dp[p][1][j - js] * dp[p][2][k - ks]);
           s0 += f * ac[3 * s];
           s1 += f * ac[3 * s + 1];
@@ -651,6 +656,11 @@ __global__ void k_l2l(TreeGeom G, int lvl, int deg, int nmon, int use_vs,
                       const int *__restrict__ active, int a0i, const int *__restrict__ slot,
                       double *mp_b, double *mp_c) {
   __shared__ double dp[3][11];
+  __shared__ double sB[11 * 11];  // binomials fact[n] / (fact[r] fact[n-r]), as below
+  for (int t = threadIdx.x; t < (deg + 1) * (deg + 1); t += blockDim.x) {
+    const int n = t / (deg + 1), r = t - n * (deg + 1);
+    sB[t] = r <= n ? fact[n] / (fact[r] * fact[n - r]) : 0.0;
+  }
   long g = active[a0i + blockIdx.x];  // child at level lvl+1
   long c = g - G.off[lvl + 1];
   int ix, iy, iz;
@@ -674,8 +684,8 @@ __global__ void k_l2l(TreeGeom G, int lvl, int deg, int nmon, int use_vs,
     for (int js = j; js <= deg - ks; ++js)
       for (int is = i; is <= deg - ks - js; ++is) {
         int s = mono_index(is, js, ks);
-        double nb = (fact[is] / (fact[i] * fact[is - i])) * (fact[js] / (fact[j] * fact[js - j])) *
-                    (fact[ks] / (fact[k] * fact[ks - k]));
+        const int w = deg + 1;
+        double nb = sB[is * w + i] * sB[js * w + j] * sB[ks * w + k];
         double f = nb * (dp[0][is - i] * dp[1][js - j] * dp[2][ks - k]);
         s0 += f * bp[3 * s];
         s1 += f * bp[3 * s + 1];
@@ -1319,7 +1329,7 @@ static int fmm_set_particles(FmmHandle *h, int n, const double *cen, const doubl
   DUST_CHECK(n <= h->cap, DUST_E_CAPACITY, "%d particles exceed the handle capacity %d", n, h->cap);
   DUST_CHECK(n == 0 || (cen && dir && mag && rv), DUST_E_ARG, "null particle array");
   h->n = n;
-  h->sorted = h->computed = false;
+  h->sorted = h->computed = h->applied = false;
   if (n == 0) return 0;
   DUST_HIP(hipMemcpyAsync(h->cen_in.p, cen, sizeof(double) * 3 * n, kind, h->stream));
   DUST_HIP(hipMemcpyAsync(h->dir_in.p, dir, sizeof(double) * 3 * n, kind, h->stream));
@@ -1487,7 +1497,7 @@ static int sort_phase_c(FmmHandle *h) {
   DUST_HIP(hipGetLastError());
   timer_end(h, DUST_FMM_STAGE_SORT);
   h->sorted = true;
-  h->computed = false;
+  h->computed = h->applied = false;
   return 0;
 }
 
@@ -1508,7 +1518,7 @@ static int multipole_phase_a(FmmHandle *h) {
   const int nm = h->nmon;
   const int tpb = ((nm + 63) / 64) * 64;
   const int use_vs = h->prm.use_vs;
-  if (h->n_active == 0) { h->computed = true; return 0; }
+  if (h->n_active == 0) { h->computed = true; h->applied = false; return 0; }
   // local expansions start from zero (set_leaf/set_branch reset, 1596-1603)
   DUST_HIP(hipMemsetAsync(h->mp_b.p, 0, sizeof(double) * 3 * nm * h->n_active, h->stream));
   if (use_vs)
@@ -1575,6 +1585,7 @@ static int multipole_phase_b(FmmHandle *h) {
     }
   }
   h->computed = true;
+  h->applied = false;
   return 0;
 }
 
@@ -1643,6 +1654,7 @@ static int fmm_multipole(FmmHandle *h) {
 static int fmm_apply(FmmHandle *h) {
   DUST_CHECK(h && h->computed, DUST_E_ARG, "calculate_multipole must precede apply_multipole");
   const int n = h->n;
+  h->applied = true;
   if (n == 0) return 0;
   if (h->ns < n)  // freed by the redistribution: not evaluated
     k_pr_zero<<<ceil_div(n, 256), 256, 0, h->stream>>>(n, h->pr_keep.p, h->vel.p, h->str.p,
@@ -1962,7 +1974,7 @@ int dust_fmm_apply_multipole(dust_fmm_t hh, double *vel, double *stretch, double
 static int rotu_copy(FmmHandle *h, double *rotu, hipMemcpyKind kind) {
   DUST_CHECK(h->prm.use_vs && h->prm.use_divfilt, DUST_E_ARG,
              "rotu needs a handle created with use_vs and use_divfilt");
-  DUST_CHECK(h->computed, DUST_E_ARG, "apply_multipole must precede rotu");
+  DUST_CHECK(h->applied, DUST_E_ARG, "apply_multipole must precede rotu");
   if (h->n > 0 && rotu)
     DUST_HIP(hipMemcpyAsync(rotu, h->rot.p, sizeof(double) * 3 * h->n, kind, h->stream));
   return 0;

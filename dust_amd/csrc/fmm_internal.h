@@ -105,7 +105,7 @@ struct FmmHandle {
   hipStream_t stream = nullptr;
   StageTimer timer;
   int seg_iters = kSegIters;  // P2P source iterations per lane in one item
-  bool sorted = false, computed = false;
+  bool sorted = false, computed = false, applied = false;
   // set by the batch driver while the tuned M2L and P2P of this pass run in
   // the batch's shared launches (the pass skips its own)
   bool batched = false;
