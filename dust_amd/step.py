@@ -123,8 +123,13 @@ class BodyWakeStep:
                  aic=None, vortex_rad=0.1, box=None, vs_elems=False, omega=0.0,
                  solver="fp64", fmm_pan=False, overlap=False, n_wake_panels=1,
                  refine_wake=False, k_refine=1, k_vortex_rad=0.0, join_te=False,
-                 join_te_factor=0.1, min_vel_at_te=1e-3):
+                 join_te_factor=0.1, min_vel_at_te=1e-3, wind=None):
         import torch
+        # variable_wind with a gust (octree.wind_params), or None for the
+        # uniform u_inf: evaluated at the panel centres, the panel-wake points
+        # and the particles at the step's time (mod_wind.f90:66-121)
+        self.wind = wind
+        self.t = 0.0  # sim_param%time
         self.torch = torch
         self.lib = L.lib()
         self.n = len(n_ver)
@@ -206,7 +211,10 @@ class BodyWakeStep:
         self.ptvel = torch.zeros((self.npt * (n_wake_panels + 1), 3), **d3)
         self.mu = self.b
         self.times = {}
-        self.t = 0.0  # sim_param%time
+        if wind is not None:
+            self.wcen = torch.zeros((self.n, 3), **d3)
+            self.usys = torch.zeros((self.n, 3), **d3)
+            self.wpt = torch.zeros_like(self.ptvel)
         # the sim_param values a result file records (save_sim_param)
         self.sim = dict(dt=self.dt, u_inf=self.u_inf.copy(), n_wake_panels=int(n_wake_panels),
                         n_wake_particles=int(nmax), particles_box_min=np.asarray(box[0], float),
@@ -239,6 +247,10 @@ class BodyWakeStep:
         pts, gdir = te_geometry(v, self.te)
         vte = np.stack([-self.omega * pts[:, 1], self.omega * pts[:, 0],
                         np.zeros(self.npt)], 1)
+        if self.wind is not None:
+            # the rows take |wind - vel_te| with a uniform wind: the gust at
+            # the start points goes into vel_te with the opposite sign
+            vte = vte - (O.variable_wind(self.wind, pts, self.t) - self.u_inf)
         self._te_keep = [self.dv(pts), self.dv(gdir), self.dv(vte)]
         self.pw.start(first, self._te_keep[0], self._te_keep[1], self.scal, self._te_keep[2],
                       self.u_inf)
@@ -259,13 +271,17 @@ class BodyWakeStep:
                       axis=1)
         self.ub.copy_(torch.from_numpy(ub))
 
-    def _desc(self):
+    def _wind_dev(self, npts, pts, out):
+        L.check(self.lib.dust_variable_wind_dev(npts, pts, C.byref(self.wind), self.t,
+                                                out.data_ptr(), None), "variable_wind")
+
+    def _desc(self, uvort=None):
         d = L.LinsysDesc()
         d.rank = self.n
         d.nstatic = 0 if self.moving else self.n
         d.kind, d.n_ver, d.ver = self.kind.data_ptr(), self.nv.data_ptr(), self.ver.data_ptr()
         d.ub = self.ub.data_ptr() if self.moving else None
-        d.uvort = self.uvort.data_ptr()
+        d.uvort = (self.uvort if uvort is None else uvort).data_ptr()
         d.u_inf = (C.c_double * 3)(*self.u_inf)
         pn = self.pw.panels()
         d.n_wake, d.n_impl = pn["n_wake"], pn["n_impl"]
@@ -318,6 +334,8 @@ class BodyWakeStep:
         n = self.n
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(7)]
         ev[0].record()
+        if self.wind is not None:
+            self.fmm.set_wind(self.wind, self.t)
         # prepare_wake: live list and the pointer arrays
         live = self.pool.live()
         npart = live.numel()
@@ -366,7 +384,15 @@ class BodyWakeStep:
                                           p["rv"].data_ptr(), nl, lver if nl else None,
                                           lmag if nl else None, C.byref(self.eprm64),
                                           self.uvort.data_ptr(), None), "uvort")
-        d = self._desc()
+        usys = None
+        if self.wind is not None:
+            # the gust at the panel centres joins uvort in the right-hand side
+            # (b = B nor.(ub - u_inf - uvort)); sigma below takes the wind whole
+            self._wind_dev(n, self.cen.data_ptr(), self.wcen)
+            u0 = torch.from_numpy(self.u_inf).cuda()
+            torch.add(self.uvort, self.wcen - u0, out=self.usys)
+            usys = self.usys
+        d = self._desc(usys)
         L.check(lib.dust_assemble_linsys_dev(C.byref(d), self.A.data_ptr(), n, self.B.data_ptr(),
                                              n, self.L0.data_ptr(), 1, self.b.data_ptr(), None,
                                              1, None, None), "assemble_linsys")
@@ -393,7 +419,8 @@ class BodyWakeStep:
         # points and at the particles, then the panels move
         self.pw.first_row(self.b)
         u = torch.from_numpy(self.u_inf).cuda()
-        torch.sum(self.nor * (self.ub - u - self.uvort), dim=1, out=self.sigma)
+        wc = self.wcen if self.wind is not None else u
+        torch.sum(self.nor * (self.ub - wc - self.uvort), dim=1, out=self.sigma)
         grp, ng = self._groups()
         npts, pts = self.pw.points()
         vpt = self.ptvel[:npts]
@@ -401,7 +428,11 @@ class BodyWakeStep:
                                       p["dir"].data_ptr(), p["mag"].data_ptr(),
                                       p["rv"].data_ptr(), 0, None, None, C.byref(self.eprm64),
                                       vpt.data_ptr(), None), "point velocity (particles)")
-        vpt.add_(u)
+        if self.wind is not None:
+            self._wind_dev(npts, pts, self.wpt)
+            vpt.add_(self.wpt[:npts])
+        else:
+            vpt.add_(u)
         L.check(lib.dust_elem_apply_dev(ng, grp, C.byref(self.eprm64), npts, pts, None,
                                         vpt.data_ptr(), None, None), "point velocity (elements)")
         if fmm_job is not None:

@@ -92,8 +92,11 @@ class OracleStep:
     def __init__(self, case, strips, nmax, u_inf, dt, vortex_rad=0.1, omega=0.0,
                  fmm_pan=False, n_wake_panels=1, refine_wake=False, k_refine=1,
                  k_vortex_rad=0.0, join_te=False, join_te_factor=0.1, min_vel_at_te=1e-3,
-                 solver="fp64"):
+                 solver="fp64", wind=None):
         ver, nv, cen, alpha, rv, tree = case
+        # wind = (gust type, the 13 numbers of oracle.wind_params) or None:
+        # variable_wind at the step's time where the reference calls it
+        self.wind, self.t = wind, 0.0
         v4 = np.zeros((len(nv), 4, 3))
         v4[:, : np.asarray(ver).shape[1]] = np.asarray(ver)[:, :4]
         self.ver0, self.nv = v4, np.asarray(nv, np.int32)
@@ -137,9 +140,15 @@ class OracleStep:
         c = self.cen_p
         self.ub = np.stack([-self.omega * c[:, 1], self.omega * c[:, 0], np.zeros(len(c))], 1)
 
+    def _wind(self, pts):
+        return orc.variable_wind(self.wind[0], self.wind[1], pts, self.t)
+
     def _rows(self, first):
         pts, gdir = te_rows_input(self.ver, self.te)
         vte = np.stack([-self.omega * pts[:, 1], self.omega * pts[:, 0], np.zeros(len(pts))], 1)
+        if self.wind is not None:
+            # norm2(wind - vel_te) of the reference with the gust moved into vel_te
+            vte = vte - (self._wind(pts) - self.u)
         self.pw.start(first, pts, gdir, np.ones(len(pts)), vte, self.u)
 
     def _wake_to_system(self):
@@ -166,6 +175,9 @@ class OracleStep:
         t = self.tree
         fmm = orc.Octree(t["box_length"], t["nbox"], t["origin"], t["nlevels"], t["min_part"], 4,
                          t["delta"], u_inf=tuple(self.u))
+        if self.wind is not None:
+            fmm.set_wind(self.wind[0], self.wind[1], self.t)
+        wc = self._wind(self.cen_p) if self.wind is not None else self.u[None]
         lver = lmag = None
         if self.pw.full:
             lver, lmag, _ = self.pw.end_vortices()
@@ -178,7 +190,7 @@ class OracleStep:
                 uvort = uvort + orc.vort_vel(self.cen_p, lver=lver, lmag=lmag)
         else:
             uvort = orc.vort_vel(self.cen_p, pc, pd, pm, pr, lver=lver, lmag=lmag)
-        S.uvort = uvort
+        S.uvort = uvort if self.wind is None else uvort + (wc - self.u)
         S.ub = self.ub if self.moving else None
         S.ver = self.ver
         self._wake_to_system()
@@ -196,10 +208,10 @@ class OracleStep:
         # update_wake
         g1, g2 = self.te["gen"][:, 0] - 1, self.te["gen"][:, 1] - 1
         self.pw.idou[0] = mu[g1] - np.where(g2 >= 0, mu[np.maximum(g2, 0)], 0.0)
-        sigma = np.einsum("ij,ij->i", self.nor, self.ub - self.u[None] - uvort)
+        sigma = np.einsum("ij,ij->i", self.nor, self.ub - wc - uvort)
         groups = self._groups(mu, sigma)
         pts = self.pw.points()
-        vpt = orc.vort_vel(pts, pc, pd, pm, pr) + self.u
+        vpt = orc.vort_vel(pts, pc, pd, pm, pr) + (self.u if self.wind is None else self._wind(pts))
         vpt, _ = orc.elem_apply(groups, pts, vel=vpt, vortex_rad=self.vr)
         if self.fmm_pan:
             vel, st, _ = fmm.apply()
@@ -217,4 +229,5 @@ class OracleStep:
         orc.wake_update(P, *self.box, self.dt, use_vs=True)
         P["n_prt"] = int((P["free"] == 0).sum())
         assert self.pw.shed(P, *self.box) == 0
+        self.t += self.dt
         return dict(mu=mu, uvort=uvort, vel=vel, stretch=st, live=live)

@@ -56,6 +56,33 @@ def compare(g, o, r, tol=TOL):
         assert rel(g["pw_" + key], getattr(o.pw, key)) <= tol, key
 
 
+def test_wing_steps_with_a_gust_match_oracle(orc, case):
+    """An 'ACM' gust front sweeping the wing along x at the wind speed
+    (variable_wind, mod_wind.f90:66-121) in the panel sources, the assembled
+    right-hand side, the panel-wake points and rows, and the particles."""
+    from dust_amd import octree as O
+    from oracle.pipeline import OracleStep
+    w = O.wind_params(WING["u"], "ACM", origin=(0.5, 0.0, 0.0), front_direction=(1.0, 0.0, 0.0),
+                      front_speed=30.0, u_des=3.0, gradient=0.5)
+    g13 = orc.wind_params(WING["u"], origin=(0.5, 0.0, 0.0), front_direction=(1.0, 0.0, 0.0),
+                          front_speed=30.0, u_des=3.0, gradient=0.5)
+    s = make_step(case, "fp64", wind=w)
+    o = OracleStep(case, WING["strips"], s.pool.nmax, WING["u"], WING["dt"], n_wake_panels=NWP,
+                   wind=(1, g13))
+    plain = make_step(case, "fp64")
+    for k in range(STEPS):
+        print(f"wing step {k} with a gust", flush=True)
+        live_g = s.pool.live().cpu().numpy()
+        s.step()
+        plain.step()
+        g = s.state()
+        r = o.step()
+        assert np.array_equal(live_g, r["live"])
+        compare(g, o, r)
+    # the gust changed the solution
+    assert rel(plain.state()["mu"], g["mu"]) > 1e-4
+
+
 @pytest.mark.parametrize("kw", [{}, dict(refine_wake=True, k_refine=2, k_vortex_rad=0.5)])
 def test_wing_steps_match_oracle(orc, case, kw):
     from oracle.pipeline import OracleStep
