@@ -525,14 +525,19 @@ __global__ void k_m2m(TreeGeom G, int lvl, int deg, int nmon, const int *__restr
       for (int e = 0; e <= deg; ++e) { dp[p][d][e] = v; v *= dd; }
     }
   }
+  // the children's expansions (8 x 3 nmon doubles), loaded together
+  extern __shared__ double sA[];
+  for (int q = threadIdx.x; q < 8 * 3 * nmon; q += blockDim.x) {
+    const int p = q / (3 * nmon), r = q - p * 3 * nmon;
+    sA[q] = mp_a[(long)slot[G.off[lvl + 1] + 8 * c + p] * 3 * nmon + r];
+  }
   __syncthreads();
   int m = threadIdx.x;
   if (m >= nmon) return;
   int i = pwr[3 * m], j = pwr[3 * m + 1], k = pwr[3 * m + 2];
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   for (int p = 0; p < 8; ++p) {
-    long gc = G.off[lvl + 1] + 8 * c + p;
-    const double *ac = &mp_a[(long)slot[gc] * 3 * nmon];
+    const double *ac = &sA[p * 3 * nmon];
     for (int ks = 0; ks <= k; ++ks)
       for (int js = 0; js <= j; ++js)
         for (int is = 0; is <= i; ++is) {
@@ -673,12 +678,21 @@ __global__ void k_l2l(TreeGeom G, int lvl, int deg, int nmon, int use_vs,
     double v = 1.0;
     for (int e = 0; e <= deg; ++e) { dp[d][e] = v; v *= dd; }
   }
+  // the parent's local expansions (3 nmon + 9 nmon doubles), loaded together
+  extern __shared__ double sP[];
+  {
+    const double *gb = &mp_b[(long)slot[gp] * 3 * nmon];
+    const double *gc = &mp_c[(long)slot[gp] * 9 * nmon];
+    for (int q = threadIdx.x; q < 3 * nmon; q += blockDim.x) sP[q] = gb[q];
+    if (use_vs)
+      for (int q = threadIdx.x; q < 9 * nmon; q += blockDim.x) sP[3 * nmon + q] = gc[q];
+  }
   __syncthreads();
   int m = threadIdx.x;
   if (m >= nmon) return;
   int i = pwr[3 * m], j = pwr[3 * m + 1], k = pwr[3 * m + 2];
-  const double *bp = &mp_b[(long)slot[gp] * 3 * nmon];
-  const double *cp = &mp_c[(long)slot[gp] * 9 * nmon];
+  const double *bp = sP;
+  const double *cp = sP + 3 * nmon;
   double s0 = 0, s1 = 0, s2 = 0, sc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int ks = k; ks <= deg; ++ks)
     for (int js = j; js <= deg - ks; ++js)
@@ -1535,7 +1549,8 @@ static int multipole_phase_a(FmmHandle *h) {
   for (int l = h->nlev - 2; l >= 0; --l) {
     int na = h->act_start[l + 1] - h->act_start[l];
     if (na > 0)
-      k_m2m<<<na, tpb, 0, h->stream>>>(G, l, h->deg, nm, h->d_pwr.p, h->d_fact.p, h->active.p,
+      k_m2m<<<na, tpb, sizeof(double) * 24 * nm, h->stream>>>(G, l, h->deg, nm, h->d_pwr.p,
+                                                                h->d_fact.p, h->active.p,
                                        h->act_start[l], h->flags.p, h->slot.p, h->mp_a.p);
   }
   timer_end(h, DUST_FMM_STAGE_M2M);
@@ -1578,7 +1593,8 @@ static int multipole_phase_b(FmmHandle *h) {
       int nc = h->act_start[l + 2] - h->act_start[l + 1];
       timer_begin(h, DUST_FMM_STAGE_L2L);
       if (nc > 0)
-        k_l2l<<<nc, tpb, 0, h->stream>>>(G, l, h->deg, nm, use_vs, h->d_pwr.p, h->d_fact.p,
+        k_l2l<<<nc, tpb, sizeof(double) * 12 * nm, h->stream>>>(G, l, h->deg, nm, use_vs,
+                                                                h->d_pwr.p, h->d_fact.p,
                                          h->active.p, h->act_start[l + 1], h->slot.p, h->mp_b.p,
                                          h->mp_c.p);
       timer_end(h, DUST_FMM_STAGE_L2L);
