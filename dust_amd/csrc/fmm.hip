@@ -377,11 +377,15 @@ __global__ void k_level_starts(TreeGeom G, int n_active, const int *__restrict__
 // every ancestor from level 1 down to the parent (964-988), then the leaf
 // itself (the target is skipped by index).  Each entry is a range of sorted
 // particles.
-__global__ void k_p2p_lists(TreeGeom G, int n_leaves, const int *__restrict__ leaves,
-                            const unsigned char *__restrict__ flags,
-                            const int *__restrict__ fstart, int cap, int2 *rng, int *cnt,
-                            int *tot) {
-  int lv = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256)
+    k_p2p_lists(TreeGeom G, int n_leaves, const int *__restrict__ leaves,
+                const unsigned char *__restrict__ flags, const int *__restrict__ fstart, int cap,
+                int2 *rng, int *cnt, int *tot) {
+  // one wavefront per leaf: lane q tests candidate q of the list in order
+  // (the 26 same-level offsets in (k, j, i) order, then 26 per ancestor
+  // level); the hits are compacted in that order by ballot
+  const int lane = threadIdx.x & 63;
+  const int lv = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (lv >= n_leaves) return;
   long g = leaves[lv];
   int lvl = lvl_of(G, g);
@@ -390,41 +394,45 @@ __global__ void k_p2p_lists(TreeGeom G, int n_leaves, const int *__restrict__ le
   cell_decode(G, lvl, c, ix, iy, iz);
   int2 *out = &rng[(long)lv * cap];
   int w = 0;
-  for (int kk = -1; kk <= 1; ++kk)
-    for (int jj = -1; jj <= 1; ++jj)
-      for (int ii = -1; ii <= 1; ++ii) {
-        if (!ii && !jj && !kk) continue;
-        int a = ix + ii, b = iy + jj, d = iz + kk;
-        if (a < 0 || b < 0 || d < 0 || a >= G.ncl[lvl][0] || b >= G.ncl[lvl][1] ||
-            d >= G.ncl[lvl][2])
-          continue;
-        long nc = cell_encode(G, lvl, a, b, d);
-        if (flags[G.off[lvl] + nc] == 0) continue;
-        int2 r = cell_range(G, fstart, lvl, nc);
-        if (r.y > r.x) out[w++] = r;
-      }
-  for (int al = 0; al < lvl; ++al) {
-    int sh = lvl - al;
-    int ax = ix >> sh, ay = iy >> sh, az = iz >> sh;
-    for (int kk = -1; kk <= 1; ++kk)
-      for (int jj = -1; jj <= 1; ++jj)
-        for (int ii = -1; ii <= 1; ++ii) {
-          if (!ii && !jj && !kk) continue;
-          int a = ax + ii, b = ay + jj, d = az + kk;
-          if (a < 0 || b < 0 || d < 0 || a >= G.ncl[al][0] || b >= G.ncl[al][1] ||
-              d >= G.ncl[al][2])
-            continue;
-          long nc = cell_encode(G, al, a, b, d);
-          if (flags[G.off[al] + nc] != kLeaf) continue;
-          int2 r = cell_range(G, fstart, al, nc);
-          if (r.y > r.x) out[w++] = r;
+  long T = 0;
+  const int ncand = 26 * (1 + lvl);
+  for (int base = 0; base < ncand; base += 64) {
+    const int q = base + lane;
+    int2 r = make_int2(0, 0);
+    bool ok = false;
+    if (q < ncand) {
+      const int blk = q / 26, idx = q - 26 * blk;
+      const int t = idx < 13 ? idx : idx + 1;  // skip the centre offset
+      const int ii = t % 3 - 1, jj = (t / 3) % 3 - 1, kk = t / 9 - 1;
+      // block 0: the leaf's own level; block 1 + al: ancestor level al
+      const int al = blk == 0 ? lvl : blk - 1;
+      const int sh = lvl - al;
+      const int a = (ix >> sh) + ii, b = (iy >> sh) + jj, d = (iz >> sh) + kk;
+      if (a >= 0 && b >= 0 && d >= 0 && a < G.ncl[al][0] && b < G.ncl[al][1] &&
+          d < G.ncl[al][2]) {
+        const long nc = cell_encode(G, al, a, b, d);
+        const unsigned char f = flags[G.off[al] + nc];
+        if (blk == 0 ? f != 0 : f == kLeaf) {
+          r = cell_range(G, fstart, al, nc);
+          ok = r.y > r.x;
         }
+      }
+    }
+    const unsigned long long m = __ballot(ok);
+    if (ok) {
+      const int pos = w + __popcll(m & ((1ull << lane) - 1ull));
+      out[pos] = r;
+      T += r.y - r.x;
+    }
+    w += __popcll(m);
   }
-  out[w++] = cell_range(G, fstart, lvl, c);
-  cnt[lv] = w;
-  int T = 0;
-  for (int q = 0; q < w; ++q) T += out[q].y - out[q].x;
-  tot[lv] = T;
+  for (int o = 32; o > 0; o >>= 1) T += __shfl_xor(T, o, 64);
+  if (lane == 0) {
+    const int2 own = cell_range(G, fstart, lvl, c);
+    out[w] = own;
+    cnt[lv] = w + 1;
+    tot[lv] = (int)(T + (own.y - own.x));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1474,7 +1482,7 @@ static int sort_phase_b(FmmHandle *h) {
   DUST_TRY(h->p2p_cnt.reserve(std::max(h->n_leaves, 1)));
   DUST_TRY(h->p2p_tot.reserve(std::max(h->n_leaves, 1)));
   if (h->n_leaves > 0)
-    k_p2p_lists<<<ceil_div(h->n_leaves, TB), TB, 0, h->stream>>>(
+    k_p2p_lists<<<ceil_div(h->n_leaves, 4), 256, 0, h->stream>>>(
         G, h->n_leaves, h->leaves.p, h->flags.p, h->fstart.p, h->p2p_cap, h->p2p_rng.p,
         h->p2p_cnt.p, h->p2p_tot.p);
   // evaluation work units and their items (sized after the next synchronisation)
