@@ -69,7 +69,7 @@ typedef struct {
   int use_vd;            /* PSE vortex diffusion                             */
   int use_tv;            /* turbulent viscosity (needs use_vd)               */
   double nu_inf;         /* kinematic viscosity                              */
-  double u_inf[3];       /* uniform free stream (variable_wind without gust) */
+  double u_inf[3];       /* uniform free stream; gusts: dust_fmm_set_wind   */
   int precision;         /* DUST_FMM_FP64 or DUST_FMM_TUNED                  */
   int kernel;            /* DUST_KERNEL_ROSENHEAD (0) or DUST_KERNEL_GAUSSIAN */
   int use_divfilt;       /* with use_vs: apply_multipole also forms rotu, the */
@@ -572,7 +572,7 @@ typedef struct dust_linsys_desc {
                               read for the moving elements only (zero for static) */
   const double *uvort;     /* (3,rank) get_vort_vel result, or NULL (0)             */
   const double *bern;      /* (rank) bernoulli_source of the surface panels, or NULL */
-  double u_inf[3];         /* uniform wind (variable_wind without gusts)           */
+  double u_inf[3];         /* uniform wind (gusts: fold dust_variable_wind into uvort) */
   int n_sp, nstatic_sp;    /* surface panels, the static ones first                */
   const int *id_sp;        /* (n_sp) 0-based element index of each (idSurfPan - 1)  */
   int n_wake, n_impl;      /* wake panels, the implicit first row first             */
