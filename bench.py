@@ -308,6 +308,44 @@ def time_batch(torch, L, O, W, ncase=48, reps=3):
     return out
 
 
+def time_fmm_graph(torch, L, O, tree, degree, dev, n, form, steps, warmup, ref=None):
+    """The graph form of the pass: capacities reserved from one host-ordered
+    pass (slack 1.25), the pass captured once and replayed; ms per replay
+    (wall clock over `steps` replays, one synchronisation at the end) and the
+    replay's velocity against the host-ordered pass's (ref, same draw)."""
+    import ctypes as C
+    g = fmm_handle(O, tree, degree, n, form)
+    lib = L.lib()
+    h = g.handle
+    L.check(lib.dust_fmm_set_particles_dev(h, n, dev["cen"].data_ptr(), dev["dir"].data_ptr(),
+                                           dev["mag"].data_ptr(), dev["rv"].data_ptr()), "set")
+    nl = C.c_int()
+    L.check(lib.dust_fmm_pass_dev(h, C.byref(nl)), "pass")
+    L.check(lib.dust_fmm_graph_reserve(h, 1.25), "graph reserve")
+    nd = torch.full((1,), n, dtype=torch.int32, device="cuda")
+    vel = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+    st = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    L.check(lib.dust_fmm_graph_capture(h, nd.data_ptr(), dev["cen"].data_ptr(),
+                                       dev["dir"].data_ptr(), dev["mag"].data_ptr(),
+                                       dev["rv"].data_ptr(), vel.data_ptr(), st.data_ptr()),
+            "graph capture")
+    for _ in range(warmup):
+        L.check(lib.dust_fmm_graph_launch(h), "graph launch")
+    L.check(lib.dust_fmm_graph_status(h, None), "graph status")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L.check(lib.dust_fmm_graph_launch(h), "graph launch")
+    L.check(lib.dust_fmm_graph_status(h, None), "graph status")
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    out = dict(ms=ms)
+    if ref is not None:
+        v = vel.cpu().numpy()
+        out["vel_rel_l2_vs_host_ordered"] = float(np.linalg.norm(v - ref) / np.linalg.norm(ref))
+    g.destroy()
+    return out
+
+
 def time_wing(torch, L, O, W, steps):
     """The wing configuration: NACA0012 wing of 1,600 surface panels with an
     8,000-particle trailing sheet (order 4, 5 levels): FMM passes of the sheet
@@ -319,10 +357,15 @@ def time_wing(torch, L, O, W, steps):
     n = len(cen)
     f64 = time_fmm(torch, L, O, tree, 4, dev, n, "fp64", steps, 1)
     tun = time_fmm(torch, L, O, tree, 4, dev, n, "tuned", steps, 1)
+    gf = time_fmm_graph(torch, L, O, tree, 4, dev, n, "fp64", max(steps, 20), 2, f64["vel"])
+    gt = time_fmm_graph(torch, L, O, tree, 4, dev, n, "tuned", max(steps, 20), 2, tun["vel"])
     lu, _ = time_lu(torch, L, W, None, 8, steps, mesh=(ver, nv))
     return dict(panels=len(nv), particles=n, fmm_fp64_ms=f64["ms"], fmm_tuned_ms=tun["ms"],
                 fmm_tuned_vel_rel_l2_vs_fp64=float(np.linalg.norm(tun["vel"] - f64["vel"]) /
                                                    np.linalg.norm(f64["vel"])),
+                fmm_graph_fp64_ms=gf["ms"], fmm_graph_tuned_ms=gt["ms"],
+                fmm_graph_fp64_vel_rel_l2_vs_host_ordered=gf["vel_rel_l2_vs_host_ordered"],
+                fmm_graph_tuned_vel_rel_l2_vs_host_ordered=gt["vel_rel_l2_vs_host_ordered"],
                 aic_ms=lu["assembly_ms"], lu_fp64_ms=lu["fp64_ms"], lu_fp16s_ms=lu["fp16s_ms"],
                 lu_fp16s_iters=lu["fp16s_iters"], lu_fp16s_backward_error=lu["fp16s_backward_error"])
 
@@ -905,6 +948,18 @@ def main():
     def rel_l2(a, b):
         return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
+    graph = None
+    if plain is not None:
+        # the graph form of the same pass (captured once, replayed)
+        log("timing the graph form of the pass")
+        gt = time_fmm_graph(torch, L, O, tree, args.degree, dev, n, "tuned", args.steps,
+                            args.warmup, tuned["vel"])
+        gf = time_fmm_graph(torch, L, O, tree, args.degree, dev, n, "fp64", 2, 1, plain["vel"])
+        graph = dict(tuned_ms=gt["ms"], fp64_ms=gf["ms"],
+                     tuned_updates_per_s=n / (gt["ms"] / 1e3),
+                     tuned_vel_rel_l2_vs_host_ordered=gt["vel_rel_l2_vs_host_ordered"],
+                     fp64_vel_rel_l2_vs_host_ordered=gf["vel_rel_l2_vs_host_ordered"])
+        log(f"graph form: {graph}")
     orders = None
     if not args.no_orders and plain is not None:
         # the same draw at expansion orders 6 and 8, both forms
@@ -1046,6 +1101,7 @@ def main():
         "roofline": roof,
         "roofline_lu": None if not lu else lu.get("roofline"),
         "orders": orders,
+        "graph": graph,
         "gaussian": gauss,
         "autotuned_schedule": sched,
         "lu": lu,

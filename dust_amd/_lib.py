@@ -197,6 +197,11 @@ SIGNATURES = [
     ("dust_fmm_apply_multipole_panels_dev", C.c_int, [_vp, C.c_int, _vp, _vp]),
     ("dust_fmm_pass_dev", C.c_int, [_vp, _ip]),
     ("dust_fmm_pass_batch_dev", C.c_int, [C.c_int, _vp, _ip]),
+    ("dust_fmm_graph_reserve", C.c_int, [_vp, C.c_double]),
+    ("dust_fmm_pass_graph_dev", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("dust_fmm_graph_capture", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("dust_fmm_graph_launch", C.c_int, [_vp]),
+    ("dust_fmm_graph_status", C.c_int, [_vp, _ip]),
     ("dust_fmm_set_tuning", C.c_int, [_vp, C.c_char_p, C.c_double]),
     ("dust_fmm_get_tuning", C.c_int, [_vp, C.c_char_p, _dp]),
     ("dust_fmm_autotune", C.c_int, [_vp, C.c_int, _dp]),

@@ -34,6 +34,7 @@ H5_LIB = os.path.join(HERE, "libdust_h5.so")
 SOURCES = [
     "fmm.hip",
     "fmm_prim.hip",
+    "fmm_graph.hip",
     "fmm_tuned.hip",
     "aic.hip",
     "elem.hip",

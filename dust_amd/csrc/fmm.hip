@@ -111,6 +111,12 @@ TreeGeom make_geom(const FmmHandle *h) {
     G.kd_base[l] = h->lev[l].kd_base;
   }
   G.csize_fine = h->lev[h->nlev - 1].csize;
+  if (h->gr.active) {
+    G.dc = h->gr.dc.p;
+    G.act_cap = h->gr.act_cap;
+    G.leaf_cap = h->gr.leaf_cap;
+    G.item_cap = h->gr.item_cap;
+  }
   return G;
 }
 
@@ -123,6 +129,7 @@ TreeGeom make_geom(const FmmHandle *h) {
 __global__ void k_keys(TreeGeom G, int n, const double *__restrict__ cen, unsigned *key, int *idx,
                        int *err) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (G.dc) n = G.dc[kGcN];
   if (i >= n) return;
   int L = G.nlev - 1;
   int id[3];
@@ -146,8 +153,10 @@ __global__ void k_keys(TreeGeom G, int n, const double *__restrict__ cen, unsign
 __global__ void k_gather(int n, const int *__restrict__ perm, const double *__restrict__ cen,
                          const double *__restrict__ dir, const double *__restrict__ mag,
                          const double *__restrict__ rv, double *px, double *py, double *pz,
-                         double *dx, double *dy, double *dz, double *pm, double *pr) {
+                         double *dx, double *dy, double *dz, double *pm, double *pr,
+                         const int *__restrict__ n_dev) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n_dev) n = *n_dev;
   if (i >= n) return;
   int q = perm[i];
   px[i] = cen[3 * q];
@@ -353,8 +362,10 @@ __global__ void k_fill_u8(long n, unsigned char *p, unsigned char v) {
   if (i < n) p[i] = v;
 }
 
-__global__ void k_slots(int n_active, const int *__restrict__ active, int *slot) {
+__global__ void k_slots(int n_active, const int *__restrict__ active, int *slot,
+                        const int *__restrict__ dc) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dc) n_active = dc[kGcOvf] ? 0 : dc[kGcActive];
   if (i < n_active) slot[active[i]] = i;
 }
 
@@ -362,6 +373,7 @@ __global__ void k_slots(int n_active, const int *__restrict__ active, int *slot)
 __global__ void k_level_starts(TreeGeom G, int n_active, const int *__restrict__ active, int *out) {
   int l = threadIdx.x;
   if (l > G.nlev) return;
+  if (G.dc) n_active = min(G.dc[kGcActive], G.act_cap);
   long key = (l == G.nlev) ? 0x7fffffffffffL : G.off[l];
   int lo = 0, hi = n_active;
   while (lo < hi) {
@@ -386,6 +398,7 @@ __global__ void __launch_bounds__(256)
   // level); the hits are compacted in that order by ballot
   const int lane = threadIdx.x & 63;
   const int lv = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (G.dc) n_leaves = g_stop(G) ? 0 : G.dc[kGcLeaves];
   if (lv >= n_leaves) return;
   long g = leaves[lv];
   int lvl = lvl_of(G, g);
@@ -455,6 +468,7 @@ __global__ void k_p2m(TreeGeom G, int deg, int nmon, const int *__restrict__ pwr
                       const double *__restrict__ dx, const double *__restrict__ dy,
                       const double *__restrict__ dz, const double *__restrict__ pm, double *mp_a) {
   extern __shared__ double sm[];
+  if (G.dc && (g_stop(G) || (int)blockIdx.x >= G.dc[kGcLeaves])) return;
   const int P1 = deg + 1;
   double *pw = sm;                         // [kP2MChunk][3][P1]
   double *al = sm + kP2MChunk * 3 * P1;    // [kP2MChunk][4] mag, dir
@@ -513,6 +527,11 @@ __global__ void k_m2m(TreeGeom G, int lvl, int deg, int nmon, const int *__restr
                       double *mp_a) {
   __shared__ double dp[8][3][11];
   __shared__ double sB[11 * 11];  // binomials fact[n] / (fact[r] fact[n-r]), as below
+  if (G.dc) {
+    if (g_stop(G)) return;
+    a0i = G.dc[kGcActStart + lvl];
+    if ((int)blockIdx.x >= G.dc[kGcActStart + lvl + 1] - a0i) return;
+  }
   long g = active[a0i + blockIdx.x];
   if (flags[g] != kBranch) return;
   for (int t = threadIdx.x; t < (deg + 1) * (deg + 1); t += blockDim.x) {
@@ -585,6 +604,11 @@ __global__ void k_m2l_fp64(TreeGeom G, int lvl, int nmon, int nU, int use_vs,
   double *sD = sm;                 // [12][nU]
   double *sa = sm + 12 * nU;       // [3*nmon]
   const int ntab = use_vs ? 12 : 3;
+  if (G.dc) {
+    if (g_stop(G)) return;
+    a0i = G.dc[kGcActStart + lvl];
+    if ((int)blockIdx.x >= G.dc[kGcActStart + lvl + 1] - a0i) return;
+  }
   long g = active[a0i + blockIdx.x];
   long c = g - G.off[lvl];
   int ix, iy, iz;
@@ -670,6 +694,11 @@ __global__ void k_l2l(TreeGeom G, int lvl, int deg, int nmon, int use_vs,
                       double *mp_b, double *mp_c) {
   __shared__ double dp[3][11];
   __shared__ double sB[11 * 11];  // binomials fact[n] / (fact[r] fact[n-r]), as below
+  if (G.dc) {
+    if (g_stop(G)) return;
+    a0i = G.dc[kGcActStart + lvl + 1];
+    if ((int)blockIdx.x >= G.dc[kGcActStart + lvl + 2] - a0i) return;
+  }
   for (int t = threadIdx.x; t < (deg + 1) * (deg + 1); t += blockDim.x) {
     const int n = t / (deg + 1), r = t - n * (deg + 1);
     sB[t] = r <= n ? fact[n] / (fact[r] * fact[n - r]) : 0.0;
@@ -733,6 +762,7 @@ __global__ void k_units(TreeGeom G, int n_leaves, const int *__restrict__ leaves
                         const int *__restrict__ fstart, const int *__restrict__ tot, int seg_iters,
                         EvalUnit *units, int *n_units, int *n_items) {
   int lv = blockIdx.x * blockDim.x + threadIdx.x;
+  if (G.dc) n_leaves = g_stop(G) ? 0 : G.dc[kGcLeaves];
   if (lv >= n_leaves) return;
   long g = leaves[lv];
   int lvl = lvl_of(G, g);
@@ -749,13 +779,15 @@ __global__ void k_units(TreeGeom G, int n_leaves, const int *__restrict__ leaves
     int nseg = max(1, (iters + seg_iters - 1) / seg_iters);
     int i0 = atomicAdd(n_items, nseg);
     int w = atomicAdd(n_units, 1);
+    if (G.dc && i0 + nseg > G.item_cap) atomicOr(&G.dc[kGcOvf], 8);
     units[w] = EvalUnit{lv, r.x + q * kWave, nt, lpt, i0, nseg, 0, 0};
   }
 }
 
 __global__ void k_items(int n_units, const EvalUnit *__restrict__ units,
-                        const int *__restrict__ tot, EvalItem *items) {
+                        const int *__restrict__ tot, EvalItem *items, const int *__restrict__ dc) {
   int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dc) n_units = dc[kGcOvf] ? 0 : dc[kGcUnits];
   if (u >= n_units) return;
   EvalUnit U = units[u];
   int T = tot[U.leaf];
@@ -775,6 +807,7 @@ __global__ void k_leaf_ros(EvalArgs A, int n_leaves) {
   for (int q = threadIdx.x; q < 3 * nmon; q += blockDim.x) spw[q] = A.pwr[q];
   __syncthreads();
   int lv = blockIdx.x * kEvalWaves + wv;
+  if (A.G.dc) n_leaves = g_stop(A.G) ? 0 : A.G.dc[kGcLeaves];
   if (lv >= n_leaves) return;
   long g = A.leaves[lv];
   int lvl = lvl_of(A.G, g);
@@ -821,7 +854,7 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_p2p_fp64(EvalArgs A) {
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double *ss = ssm[wv];
   const int it = blockIdx.x * kEvalWaves + wv;
-  if (it >= A.n_items) return;
+  if (it >= (A.G.dc ? (g_stop(A.G) ? 0 : A.G.dc[kGcItems]) : A.n_items)) return;
   const EvalItem I = A.items[it];
   const EvalUnit U = A.units[I.unit];
   const int lpt = U.lpt, tl = lane / lpt, sub = lane - tl * lpt;
@@ -940,7 +973,7 @@ __global__ void __launch_bounds__(kWave * kEvalWaves) k_eval_finish(EvalArgs A) 
   for (int q = threadIdx.x; q < 3 * nmon; q += blockDim.x) spw[q] = A.pwr[q];
   __syncthreads();
   const int u = blockIdx.x * kEvalWaves + wv;
-  if (u >= A.n_units) return;
+  if (u >= (A.G.dc ? (g_stop(A.G) ? 0 : A.G.dc[kGcUnits]) : A.n_units)) return;
   const EvalUnit U = A.units[u];
   const long g = A.leaves[U.leaf];
   const int lvl = lvl_of(A.G, g);
@@ -1015,8 +1048,9 @@ size_t eval_finish_lds(int nmon) {
 // Scatter results from sorted to input order.
 __global__ void k_scatter(int n, const int *__restrict__ perm, const double *__restrict__ vs,
                           const double *__restrict__ ss, const double *__restrict__ ts,
-                          double *v, double *s, double *tv) {
+                          double *v, double *s, double *tv, const int *__restrict__ n_dev) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n_dev) n = *n_dev;
   if (i >= n) return;
   int q = perm[i];
   v[3 * q] = vs[3 * i];
@@ -1030,8 +1064,9 @@ __global__ void k_scatter(int n, const int *__restrict__ perm, const double *__r
 
 // Scatter a 3-vector per particle from sorted to input order.
 __global__ void k_scatter3(int n, const int *__restrict__ perm, const double *__restrict__ src,
-                           double *dst) {
+                           double *dst, const int *__restrict__ n_dev) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n_dev) n = *n_dev;
   if (i >= n) return;
   int q = perm[i];
   dst[3 * q] = src[3 * i];
@@ -1339,6 +1374,11 @@ static void fmm_destroy(FmmHandle *h) {
   DevBuf<int> *pri[] = {&h->pr_sel, &h->pr_ns, &h->pr_perm};
   for (auto b : pri) b->release();
   h->pr_key.release();
+  if (h->gr.exec) (void)hipGraphExecDestroy(h->gr.exec);
+  if (h->gr.graph) (void)hipGraphDestroy(h->gr.graph);
+  h->gr.dc.release();
+  h->gr.rs_hist.release();
+  h->gr.sel_cnt.release();
   for (auto e : h->timer.pool) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -1437,7 +1477,7 @@ static int sort_phase_a(FmmHandle *h) {
     k_gather<<<ceil_div(n, TB), TB, 0, h->stream>>>(n, h->perm.p, h->cen_in.p, h->dir_in.p,
                                                     h->mag_in.p, h->rv_in.p, h->px.p, h->py.p,
                                                     h->pz.p, h->dx.p, h->dy.p, h->dz.p, h->pm.p,
-                                                    h->pr.p);
+                                                    h->pr.p, nullptr);
   }
   k_fstart<<<ceil_div(h->nfine + 1, TB), TB, 0, h->stream>>>(h->nfine, n, nullptr, h->key_s.p,
                                                              h->fstart.p);
@@ -1476,7 +1516,8 @@ static int sort_phase_b(FmmHandle *h) {
                           hipMemcpyDeviceToHost, h->stream));
   k_fill_int<<<ceil_div(h->ncells_tot, TB), TB, 0, h->stream>>>(h->ncells_tot, h->slot.p, -1);
   if (h->n_active > 0)
-    k_slots<<<ceil_div(h->n_active, TB), TB, 0, h->stream>>>(h->n_active, h->active.p, h->slot.p);
+    k_slots<<<ceil_div(h->n_active, TB), TB, 0, h->stream>>>(h->n_active, h->active.p, h->slot.p,
+                                                                  nullptr);
   // P2P lists
   DUST_TRY(h->p2p_rng.reserve((size_t)std::max(h->n_leaves, 1) * h->p2p_cap));
   DUST_TRY(h->p2p_cnt.reserve(std::max(h->n_leaves, 1)));
@@ -1515,7 +1556,7 @@ static int sort_phase_c(FmmHandle *h) {
                            ((h->prm.use_vs && h->prm.use_divfilt) ? 9 : 6)));
   if (h->n_unit > 0)
     k_items<<<ceil_div(h->n_unit, TB), TB, 0, h->stream>>>(h->n_unit, (const EvalUnit *)h->units.p,
-                                                          h->p2p_tot.p, (EvalItem *)h->items.p);
+                                                          h->p2p_tot.p, (EvalItem *)h->items.p, nullptr);
   DUST_HIP(hipGetLastError());
   timer_end(h, DUST_FMM_STAGE_SORT);
   h->sorted = true;
@@ -1644,9 +1685,10 @@ void make_eval_args(const FmmHandle *h, EvalArgs *Ap) {
 }
 
 int eval_leaf_ros(FmmHandle *h, const EvalArgs &A) {
-  if (!(h->prm.use_vd && h->prm.use_tv) || h->n_leaves == 0) return 0;
+  const int nl = h->gr.active ? h->gr.leaf_cap : h->n_leaves;  // graph form: by capacity
+  if (!(h->prm.use_vd && h->prm.use_tv) || nl == 0) return 0;
   size_t sh = sizeof(double) * kEvalWaves * 12 * h->nmon + sizeof(int) * 3 * h->nmon;
-  k_leaf_ros<<<ceil_div(h->n_leaves, kEvalWaves), kWave * kEvalWaves, sh, h->stream>>>(A, h->n_leaves);
+  k_leaf_ros<<<ceil_div(nl, kEvalWaves), kWave * kEvalWaves, sh, h->stream>>>(A, nl);
   DUST_HIP(hipGetLastError());
   return 0;
 }
@@ -1700,10 +1742,11 @@ static int fmm_apply(FmmHandle *h) {
   DUST_TRY(eval_finish_launch(h, A));
   timer_end(h, DUST_FMM_STAGE_L2P);
   k_scatter<<<ceil_div(h->ns, 256), 256, 0, h->stream>>>(h->ns, h->perm.p, h->vel_s.p, h->str_s.p,
-                                                         h->tv_s.p, h->vel.p, h->str.p, h->tv.p);
+                                                         h->tv_s.p, h->vel.p, h->str.p, h->tv.p,
+                                                         nullptr);
   if (A.divf)
     k_scatter3<<<ceil_div(h->ns, 256), 256, 0, h->stream>>>(h->ns, h->perm.p, h->rot_s.p,
-                                                            h->rot.p);
+                                                            h->rot.p, nullptr);
   return 0;
 }
 
@@ -1839,6 +1882,296 @@ static int fmm_apply_panels(FmmHandle *h, int m, const double *cen, double *uvor
   make_eval_args(h, &A);
   k_pan_eval<<<ceil_div(m, 4), 256, 0, h->stream>>>(A, m, cen, pleaf.p, uvort);
   DUST_HIP(hipGetLastError());
+  return 0;
+}
+// ---------------------------------------------------------------------------
+// Graph form of the pass: sort_particles, calculate_multipole and
+// apply_multipole (mod_octree.f90:703-1429) enqueued with every count on the
+// device.  The grids and buffers are sized by the capacities of
+// dust_fmm_graph_reserve; each kernel reads its count (particles, active
+// cells, leaves, level starts, units, items, M2L queue length) from
+// GraphState::dc and does nothing past it, or at all once the stop word is
+// set (a capacity exceeded), which the status call reports after the run.
+// Same kernels, same summation order as the host-ordered pass.
+// ---------------------------------------------------------------------------
+__global__ void k_graph_begin(const int *__restrict__ n_in, int cap, int *dc) {
+  const int t = threadIdx.x;
+  if (t >= kGcSize) return;
+  const int n = *n_in;
+  const bool bad = n < 0 || n > cap;
+  int v = 0;
+  if (t == kGcN) v = bad ? 0 : n;
+  else if (t == kGcOvf) v = bad ? 1 : 0;
+  else if (t == kGcErr) v = 0x7fffffff;
+  dc[t] = v;
+}
+
+struct LevCaps {
+  int c[kMaxLevels];
+};
+
+// The lists against the capacities the grids and buffers were sized for.
+__global__ void k_graph_check(int nlev, LevCaps lc, int act_cap, int leaf_cap, int *dc) {
+  if (threadIdx.x != 0) return;
+  int bits = 0;
+  if (dc[kGcActive] > act_cap || dc[kGcLeaves] > leaf_cap) bits |= 2;
+  for (int l = 0; l < nlev; ++l)
+    if (dc[kGcActStart + l + 1] - dc[kGcActStart + l] > lc.c[l]) bits |= 4;
+  if (bits) atomicOr(&dc[kGcOvf], bits);
+}
+
+// Zero the first dc[idx] * per entries (the local expansions of the active
+// cells).
+__global__ void k_zero_dc(double *p, long per, int idx, const int *__restrict__ dc) {
+  const long n = dc[kGcOvf] ? 0 : (long)dc[idx] * per;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+struct GraphActive {
+  FmmHandle *h;
+  explicit GraphActive(FmmHandle *hh) : h(hh) { h->gr.active = true; }
+  ~GraphActive() { h->gr.active = false; }
+};
+
+static std::vector<const void *> graph_buffers(const FmmHandle *h) {
+  std::vector<const void *> v = {
+      h->key.p, h->key_s.p, h->idx_in.p, h->perm.p, h->px.p, h->py.p, h->pz.p, h->dx.p,
+      h->dy.p, h->dz.p, h->pm.p, h->pr.p, h->fstart.p, h->flags.p, h->slot.p, h->active.p,
+      h->leaves.p, h->p2p_rng.p, h->p2p_cnt.p, h->p2p_tot.p, h->units.p, h->items.p, h->part.p,
+      h->leaf_ros.p, h->mp_a.p, h->mp_b.p, h->mp_c.p, h->vel_s.p, h->str_s.p, h->tv_s.p,
+      h->rot_s.p, h->rot.p, h->vel.p, h->str.p, h->tv.p, h->d_pwr.p, h->d_multT.p, h->d_mnT.p,
+      h->d_kd.p, h->d_fact.p, h->gr.dc.p, h->gr.rs_hist.p, h->gr.sel_cnt.p};
+  if (h->prm.precision == DUST_FMM_TUNED) tuned_graph_buffers(h, v);
+  return v;
+}
+
+static int graph_reserve(FmmHandle *h, double slack) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  DUST_CHECK(h->sorted && h->computed, DUST_E_ARG,
+             "graph reserve sizes its lists from a completed pass on the handle");
+  DUST_CHECK(slack >= 1.0 && slack <= 64.0, DUST_E_ARG, "slack %g outside [1, 64]", slack);
+  DUST_CHECK(!h->pr_on, DUST_E_ARG,
+             "graph form: the particle redistribution is not available (set_redistribution off)");
+  DUST_CHECK(h->wind.type == DUST_GUST_NONE, DUST_E_ARG,
+             "graph form: a gust's time would be fixed at capture (uniform free stream only)");
+  GraphState &R = h->gr;
+  if (R.exec) (void)hipGraphExecDestroy(R.exec);
+  if (R.graph) (void)hipGraphDestroy(R.graph);
+  R.exec = nullptr;
+  R.graph = nullptr;
+  R.reserved = false;
+  R.fp.clear();
+  auto grow = [&](long v, long hi) {
+    const long c = (long)std::ceil((double)v * slack) + 64;
+    return (int)std::max(0L, std::min(c, hi));
+  };
+  R.n_cap = h->cap;
+  R.act_cap = grow(h->n_active, h->ncells_tot);
+  R.leaf_cap = grow(h->n_leaves, h->ncells_tot);
+  // a level may hold any share of the active cells: its grids are sized by
+  // the smaller of its cell count and the active-cell capacity (the blocks
+  // past the level's count return at once)
+  for (int l = 0; l < kMaxLevels; ++l)
+    R.lev_cap[l] = l < h->nlev ? (int)std::min<long>(h->lev[l].ncells, R.act_cap) : 0;
+  R.unit_cap = R.leaf_cap + R.n_cap / kWave + 1;
+  R.item_cap = grow(h->n_item, (long)INT32_MAX / 2);
+  const int pstride = (h->prm.use_vs && h->prm.use_divfilt) ? 9 : 6;
+  const size_t na = std::max(R.act_cap, 1), nl = std::max(R.leaf_cap, 1);
+  DUST_TRY(R.dc.reserve(kGcSize));
+  DUST_TRY(R.rs_hist.reserve(gprim_hist_ints(std::max(R.n_cap, R.act_cap))));
+  DUST_TRY(R.sel_cnt.reserve(gprim_sel_ints(h->ncells_tot)));
+  DUST_TRY(h->active.reserve(na));
+  DUST_TRY(h->leaves.reserve(nl));
+  DUST_TRY(h->p2p_rng.reserve(nl * h->p2p_cap));
+  DUST_TRY(h->p2p_cnt.reserve(nl));
+  DUST_TRY(h->p2p_tot.reserve(nl));
+  DUST_TRY(h->leaf_ros.reserve(nl));
+  DUST_TRY(h->units.reserve((size_t)R.unit_cap * 2));
+  DUST_TRY(h->items.reserve((size_t)std::max(R.item_cap, 1)));
+  DUST_TRY(h->part.reserve((size_t)std::max(R.item_cap, 1) * kWave * pstride));
+  DUST_TRY(h->mp_a.reserve(na * 3 * h->nmon));
+  DUST_TRY(h->mp_b.reserve(na * 3 * h->nmon));
+  DUST_TRY(h->mp_c.reserve(na * 9 * h->nmon));
+  if (h->prm.precision == DUST_FMM_TUNED) DUST_TRY(tuned_graph_reserve(h));
+  R.fp = graph_buffers(h);
+  R.reserved = true;
+  return 0;
+}
+
+static int graph_enqueue(FmmHandle *h, const int *d_n, const double *cen, const double *dir,
+                         const double *mag, const double *rv, double *vel, double *str) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  GraphState &R = h->gr;
+  DUST_CHECK(R.reserved, DUST_E_ARG, "dust_fmm_graph_reserve must precede the graph form");
+  DUST_CHECK(d_n && cen && dir && mag && rv && vel, DUST_E_ARG, "null device array");
+  DUST_CHECK(graph_buffers(h) == R.fp, DUST_E_ARG,
+             "handle buffers were reallocated since dust_fmm_graph_reserve");
+  GraphActive on(h);
+  h->sorted = h->computed = h->applied = false;  // host-side counts are not this pass's
+  const TreeGeom G = make_geom(h);
+  int *dc = R.dc.p;
+  const int TB = 256, cap = std::max(R.n_cap, 1);
+  const int nm = h->nmon, tpb = ((nm + 63) / 64) * 64;
+  const bool tuned = h->prm.precision == DUST_FMM_TUNED;
+  // sort_particles
+  k_graph_begin<<<1, 64, 0, h->stream>>>(d_n, R.n_cap, dc);
+  k_keys<<<ceil_div(cap, TB), TB, 0, h->stream>>>(G, 0, cen, h->key.p, h->idx_in.p, dc + kGcErr);
+  int bits = 0;
+  while ((1L << bits) < h->nfine) ++bits;
+  if (bits == 0) bits = 1;
+  DCount cn{dc + kGcN, nullptr, nullptr, R.n_cap};
+  DUST_TRY(gprim_sort_pairs_u32(h, h->key.p, h->idx_in.p, h->key_s.p, h->perm.p, cn, bits));
+  k_gather<<<ceil_div(cap, TB), TB, 0, h->stream>>>(0, h->perm.p, cen, dir, mag, rv, h->px.p,
+                                                    h->py.p, h->pz.p, h->dx.p, h->dy.p, h->dz.p,
+                                                    h->pm.p, h->pr.p, dc + kGcN);
+  k_fstart<<<ceil_div(h->nfine + 1, TB), TB, 0, h->stream>>>(h->nfine, 0, dc + kGcN, h->key_s.p,
+                                                             h->fstart.p);
+  k_flags_fine<<<ceil_div(h->nfine, TB), TB, 0, h->stream>>>(G, h->fstart.p, h->prm.min_part,
+                                                             h->flags.p);
+  for (int l = h->nlev - 2; l >= 0; --l)
+    k_flags_level<<<ceil_div(h->lev[l].ncells, TB), TB, 0, h->stream>>>(G, l, h->fstart.p,
+                                                                        h->prm.min_part, h->flags.p);
+  DUST_TRY(gprim_select_flagged_cells(h, 0, h->active.p, R.act_cap, dc + kGcActive));
+  DUST_TRY(gprim_select_flagged_cells(h, 1, h->leaves.p, R.leaf_cap, dc + kGcLeaves));
+  k_level_starts<<<1, 64, 0, h->stream>>>(G, 0, h->active.p, dc + kGcActStart);
+  LevCaps lc;
+  for (int l = 0; l < kMaxLevels; ++l) lc.c[l] = R.lev_cap[l];
+  k_graph_check<<<1, 64, 0, h->stream>>>(h->nlev, lc, R.act_cap, R.leaf_cap, dc);
+  k_fill_int<<<ceil_div(h->ncells_tot, TB), TB, 0, h->stream>>>(h->ncells_tot, h->slot.p, -1);
+  if (R.act_cap > 0)
+    k_slots<<<ceil_div(R.act_cap, TB), TB, 0, h->stream>>>(0, h->active.p, h->slot.p, dc);
+  if (R.leaf_cap > 0) {
+    k_p2p_lists<<<ceil_div(R.leaf_cap, 4), 256, 0, h->stream>>>(
+        G, 0, h->leaves.p, h->flags.p, h->fstart.p, h->p2p_cap, h->p2p_rng.p, h->p2p_cnt.p,
+        h->p2p_tot.p);
+    k_units<<<ceil_div(R.leaf_cap, TB), TB, 0, h->stream>>>(
+        G, 0, h->leaves.p, h->fstart.p, h->p2p_tot.p, h->seg_iters, (EvalUnit *)h->units.p,
+        dc + kGcUnits, dc + kGcItems);
+  }
+  k_items<<<ceil_div(R.unit_cap, TB), TB, 0, h->stream>>>(0, (const EvalUnit *)h->units.p,
+                                                          h->p2p_tot.p, (EvalItem *)h->items.p, dc);
+  // calculate_multipole
+  k_zero_dc<<<2048, TB, 0, h->stream>>>(h->mp_b.p, 3L * nm, kGcActive, dc);
+  if (h->prm.use_vs) k_zero_dc<<<2048, TB, 0, h->stream>>>(h->mp_c.p, 9L * nm, kGcActive, dc);
+  if (R.leaf_cap > 0) {
+    size_t sh = sizeof(double) * (kP2MChunk * 3 * (h->deg + 1) + kP2MChunk * 4);
+    k_p2m<<<R.leaf_cap, tpb, sh, h->stream>>>(G, h->deg, nm, h->d_pwr.p, h->leaves.p, h->slot.p,
+                                              h->fstart.p, h->px.p, h->py.p, h->pz.p, h->dx.p,
+                                              h->dy.p, h->dz.p, h->pm.p, h->mp_a.p);
+  }
+  for (int l = h->nlev - 2; l >= 0; --l)
+    if (R.lev_cap[l] > 0)
+      k_m2m<<<R.lev_cap[l], tpb, sizeof(double) * 24 * nm, h->stream>>>(
+          G, l, h->deg, nm, h->d_pwr.p, h->d_fact.p, h->active.p, 0, h->flags.p, h->slot.p,
+          h->mp_a.p);
+  if (tuned) DUST_TRY(tuned_graph_m2l(h));
+  for (int l = 0; l < h->nlev; ++l) {
+    if ((!tuned || l == 0) && R.lev_cap[l] > 0) {
+      size_t sh = sizeof(double) * (12 * h->nU + 3 * nm);
+      k_m2l_fp64<<<R.lev_cap[l], tpb, sh, h->stream>>>(
+          G, l, nm, h->nU, h->prm.use_vs, h->d_pwr.p, h->d_multT.p, h->d_mnT.p, h->d_kd.p,
+          h->active.p, 0, h->flags.p, h->slot.p, h->mp_a.p, h->mp_b.p, h->mp_c.p);
+    }
+    if (l + 1 < h->nlev && R.lev_cap[l + 1] > 0)
+      k_l2l<<<R.lev_cap[l + 1], tpb, sizeof(double) * 12 * nm, h->stream>>>(
+          G, l, h->deg, nm, h->prm.use_vs, h->d_pwr.p, h->d_fact.p, h->active.p, 0, h->slot.p,
+          h->mp_b.p, h->mp_c.p);
+  }
+  // apply_multipole: grids by capacity, counts from dc
+  EvalArgs A;
+  make_eval_args(h, &A);
+  A.n_units = R.unit_cap;
+  A.n_items = R.item_cap;
+  if (tuned) DUST_TRY(tuned_eval(h, A));
+  else DUST_TRY(eval_fp64_launch(h, A));
+  DUST_TRY(eval_finish_launch(h, A));
+  k_scatter<<<ceil_div(cap, 256), 256, 0, h->stream>>>(0, h->perm.p, h->vel_s.p, h->str_s.p,
+                                                       h->tv_s.p, vel, str ? str : h->str.p,
+                                                       h->tv.p, dc + kGcN);
+  if (A.divf)
+    k_scatter3<<<ceil_div(cap, 256), 256, 0, h->stream>>>(0, h->perm.p, h->rot_s.p, h->rot.p,
+                                                          dc + kGcN);
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
+
+static void graph_drop(GraphState &R) {
+  if (R.exec) (void)hipGraphExecDestroy(R.exec);
+  if (R.graph) (void)hipGraphDestroy(R.graph);
+  R.exec = nullptr;
+  R.graph = nullptr;
+}
+
+static int graph_capture(FmmHandle *h, const int *d_n, const double *cen, const double *dir,
+                         const double *mag, const double *rv, double *vel, double *str) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  GraphState &R = h->gr;
+  graph_drop(R);
+  DUST_HIP(hipStreamSynchronize(h->stream));
+  DUST_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = graph_enqueue(h, d_n, cen, dir, mag, rv, vel, str);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    set_error("stream capture of the pass failed: %s", hipGetErrorString(e));
+    return DUST_E_HIP;
+  }
+  R.graph = g;
+  const hipError_t ei = hipGraphInstantiate(&R.exec, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    graph_drop(R);
+    set_error("graph instantiation failed: %s", hipGetErrorString(ei));
+    return DUST_E_HIP;
+  }
+  return 0;
+}
+
+static int graph_launch(FmmHandle *h) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  GraphState &R = h->gr;
+  DUST_CHECK(R.exec, DUST_E_ARG, "no captured pass on the handle");
+  DUST_CHECK(graph_buffers(h) == R.fp, DUST_E_ARG,
+             "handle buffers were reallocated since the capture");
+  h->sorted = h->computed = h->applied = false;
+  DUST_HIP(hipGraphLaunch(R.exec, h->stream));
+  return 0;
+}
+
+static int graph_status(FmmHandle *h, int *counts) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  GraphState &R = h->gr;
+  DUST_CHECK(R.reserved, DUST_E_ARG, "no graph form reserved on the handle");
+  int c[kGcSize];
+  DUST_HIP(hipMemcpyAsync(c, R.dc.p, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+  DUST_HIP(hipStreamSynchronize(h->stream));
+  if (counts) {
+    counts[0] = c[kGcN];
+    counts[1] = c[kGcActive];
+    counts[2] = c[kGcLeaves];
+    counts[3] = c[kGcItems];
+  }
+  if (c[kGcOvf] & 1) {
+    set_error("graph pass: particle count outside 0..%d", R.n_cap);
+    return DUST_E_CAPACITY;
+  }
+  if (c[kGcErr] != 0x7fffffff) {
+    set_error("particle %d resulted outside the octree box", c[kGcErr]);
+    return DUST_E_OUTSIDE;
+  }
+  if (c[kGcOvf]) {
+    set_error("graph pass: capacity exceeded (%s%s%s%s; %d active cells / %d, %d leaves / %d, "
+              "%d items / %d): reserve again with more slack and capture again",
+              (c[kGcOvf] & 2) ? "cells " : "", (c[kGcOvf] & 4) ? "level " : "",
+              (c[kGcOvf] & 8) ? "items " : "", (c[kGcOvf] & 16) ? "M2L jobs" : "", c[kGcActive],
+              R.act_cap, c[kGcLeaves], R.leaf_cap, c[kGcItems], R.item_cap);
+    return DUST_E_CAPACITY;
+  }
   return 0;
 }
 }  // namespace dust
@@ -2167,6 +2500,33 @@ int dust_fmm_pass_batch_dev(int nh, dust_fmm_t *hs, int *nleaves) {
 }
 
 void *dust_fmm_stream(dust_fmm_t h) { return h ? (void *)h->h->stream : nullptr; }
+
+int dust_fmm_graph_reserve(dust_fmm_t h, double slack) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  return graph_reserve(h->h, slack);
+}
+
+int dust_fmm_pass_graph_dev(dust_fmm_t h, const int *n_dev, const double *cen, const double *dir,
+                            const double *mag, const double *rv, double *vel, double *stretch) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  return graph_enqueue(h->h, n_dev, cen, dir, mag, rv, vel, stretch);
+}
+
+int dust_fmm_graph_capture(dust_fmm_t h, const int *n_dev, const double *cen, const double *dir,
+                           const double *mag, const double *rv, double *vel, double *stretch) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  return graph_capture(h->h, n_dev, cen, dir, mag, rv, vel, stretch);
+}
+
+int dust_fmm_graph_launch(dust_fmm_t h) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  return graph_launch(h->h);
+}
+
+int dust_fmm_graph_status(dust_fmm_t h, int *counts) {
+  DUST_CHECK(h, DUST_E_ARG, "null handle");
+  return graph_status(h->h, counts);
+}
 
 int dust_fmm_synchronize(dust_fmm_t h) {
   DUST_CHECK(h, DUST_E_ARG, "null handle");

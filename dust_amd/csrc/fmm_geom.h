@@ -27,7 +27,16 @@ struct TreeGeom {
   long kd_base[kMaxLevels];
   double xmin[3], xmax[3];
   double csize_fine;
+  // Graph form of a pass: the counts live on the device (dc, indexed by
+  // kGc*, null in the host-ordered pass) and the kernels size their work
+  // from them; the capacities bound what the fixed grids and buffers hold.
+  int *dc;
+  int act_cap, leaf_cap, item_cap;
 };
+
+// The graph form's stop word: a capacity was exceeded or the count is
+// invalid, and every later kernel of the pass does nothing.
+__device__ inline bool g_stop(const TreeGeom &G) { return G.dc && G.dc[kGcOvf]; }
 
 __host__ __device__ inline unsigned spread3(unsigned v) {
   v &= 0x3ffu;

@@ -16,6 +16,25 @@ constexpr int kSegIters = 1536;  // default P2P source iterations per lane in on
 constexpr unsigned char kLeaf = 1;
 constexpr unsigned char kBranch = 2;
 
+// Device counts of the graph form of a pass (GraphState::dc): the particles,
+// the active cells and leaves, the first active cell of every level, the
+// evaluation units and items, the out-of-box particle (atomicMin), the stop
+// word (bit 0: particle count outside 0..capacity, 1: active cells or leaves,
+// 2: a level, 3: items, 4: M2L jobs over their capacity) and the M2L queue
+// length.
+enum {
+  kGcN = 0,
+  kGcActive = 1,
+  kGcLeaves = 2,
+  kGcActStart = 3,  // kMaxLevels + 1 entries
+  kGcUnits = kGcActStart + kMaxLevels + 1,
+  kGcItems,
+  kGcErr,
+  kGcOvf,
+  kGcQlen,
+  kGcSize = 32
+};
+
 // Geometry of one octree level.  Cells of a level are numbered
 //   c = box * 8^lvl + morton(ix mod 2^lvl, iy mod 2^lvl, iz mod 2^lvl)
 // with box the level-1 box (x fastest) and morton bits interleaved x,y,z from
@@ -47,6 +66,23 @@ struct FmmReadback {
   int units[2];                   // evaluation units, items
   int act_start[kMaxLevels + 1];  // first active cell of every level
   int ns;                         // particles kept by a redistribution
+};
+
+// Graph form of the pass (dust_fmm_graph_*): capacities fixed by
+// dust_fmm_graph_reserve, the pass enqueued with every count on the device
+// and no host synchronisation, allocation or memset, and optionally captured
+// into a graph owned by the handle.
+struct GraphState {
+  bool reserved = false;
+  bool active = false;  // set while the graph form is being enqueued
+  int n_cap = 0, act_cap = 0, leaf_cap = 0, unit_cap = 0, item_cap = 0, job_cap = 0;
+  int lev_cap[kMaxLevels] = {0};
+  DevBuf<int> dc;       // kGcSize device counts
+  DevBuf<int> rs_hist;  // radix-sort digit counts (256 per tile) and digit totals
+  DevBuf<int> sel_cnt;  // selection counts per tile
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<const void *> fp;  // buffers the captured graph addresses
 };
 
 struct FmmHandle {
@@ -128,6 +164,7 @@ struct FmmHandle {
   WindDev wind{};
   double wind_time = 0.0;
   int ns = 0;            // particles in the tree after the last sort
+  GraphState gr;
 };
 
 // fmm_prim.hip: device-wide primitives (radix sort, selection).
@@ -137,6 +174,21 @@ int prim_select_flagged_cells(FmmHandle *h, int want_leaf_only, int *out, int *d
 int prim_select_kept(FmmHandle *h, const unsigned char *keep, int n, int *out, int *d_count);
 int prim_sort_pairs_u64(FmmHandle *h, const unsigned long long *kin, unsigned long long *kout,
                         const int *vin, int *vout, int n, int end_bit);
+
+// fmm_graph.hip: the same primitives with the element count on the device
+// (n = *a - (b ? *b : 0), clamped to 0..cap, 0 once *stop is set), sized by
+// capacity, with no allocation or memset: the graph form's sort and selection.
+struct DCount {
+  const int *a, *b, *stop;
+  int cap;
+};
+int gprim_sort_pairs_u32(FmmHandle *h, unsigned *k0, int *v0, unsigned *k1, int *v1, DCount c,
+                         int key_bits);
+int gprim_sort_pairs_u64(FmmHandle *h, unsigned long long *k0, int *v0, unsigned long long *k1,
+                         int *v1, DCount c, int key_bits);
+int gprim_select_flagged_cells(FmmHandle *h, int leaf_only, int *out, int cap, int *d_count);
+size_t gprim_hist_ints(int cap);
+size_t gprim_sel_ints(long n);
 
 // fmm.hip: evaluation helpers shared with the tuned form
 struct EvalArgs;
@@ -152,6 +204,9 @@ int tuned_set(FmmHandle *h, const char *name, double value);
 int tuned_get(FmmHandle *h, const char *name, double *value);
 int tuned_m2l_all(FmmHandle *h);
 int tuned_eval(FmmHandle *h, EvalArgs &A);
+int tuned_graph_reserve(FmmHandle *h);
+int tuned_graph_m2l(FmmHandle *h);
+void tuned_graph_buffers(const FmmHandle *h, std::vector<const void *> &v);
 bool tuned_batchable(const FmmHandle *h);
 int tuned_batch_launch(FmmHandle *const *hs, int nh, hipStream_t st, hipEvent_t ev_m2l,
                        hipEvent_t ev_p2p);

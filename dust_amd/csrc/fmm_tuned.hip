@@ -53,6 +53,12 @@ struct TunedState {
   HostBuf<int> hcount;          // [kMaxLevels * 8] group sizes (page-locked)
   long gbase[kMaxLevels * 8];  // first entry of every group in grp
   double near_factor = 1.0;
+  // graph form: group entries placed by level capacities, the job list built
+  // on the device, the persistent M2L's arguments and its 8 queue counters
+  long ggbase[kMaxLevels * 8];
+  DevBuf<unsigned char> gjobs, gargs;
+  DevBuf<int> gq;
+  unsigned gm2l_grid = 0;
 };
 
 static TunedState *TS(FmmHandle *h) { return (TunedState *)h->tuned; }
@@ -155,6 +161,10 @@ __global__ void k_group_key(TreeGeom G, const int *__restrict__ active, int a1, 
                             const unsigned char *__restrict__ flags, unsigned long long *key,
                             int *idx, int *gcount) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (G.dc) {
+    a1 = G.dc[kGcActStart + 1];
+    n = g_stop(G) ? 0 : G.dc[kGcActive] - a1;
+  }
   if (q >= n) return;
   const long g = active[a1 + q];
   const int lvl = lvl_of(G, g);
@@ -198,8 +208,13 @@ struct GroupBase {
 __global__ void k_group_place(int a1, int n, const int *__restrict__ active,
                               const unsigned long long *__restrict__ key,
                               const int *__restrict__ idx, const int *__restrict__ gstart,
-                              GroupBase gb, const int *__restrict__ slot, int *grp, long *pos) {
+                              GroupBase gb, const int *__restrict__ slot, int *grp, long *pos,
+                              const int *__restrict__ dc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dc) {
+    a1 = dc[kGcActStart + 1];
+    n = dc[kGcOvf] ? 0 : dc[kGcActive] - a1;
+  }
   if (i >= n) return;
   const int e8 = (int)(key[i] >> 27);  // level * 8 + octant
   const int q = idx[i];
@@ -215,6 +230,11 @@ __global__ void k_group_src(TreeGeom G, int lvl, const int *__restrict__ active,
                             const int *__restrict__ slot, const unsigned char *__restrict__ flags,
                             const int *__restrict__ offs, const long *__restrict__ pos, int *src) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (G.dc) {
+    a0 = G.dc[kGcActStart + lvl];
+    na = g_stop(G) ? 0 : G.dc[kGcActStart + lvl + 1] - a0;
+    pos += a0;
+  }
   if (t >= (long)na * kOct) return;
   const int q = (int)(t / kOct), j = (int)(t % kOct);
   long c = active[a0 + q] - G.off[lvl];
@@ -235,6 +255,14 @@ __global__ void k_group_src(TreeGeom G, int lvl, const int *__restrict__ active,
 __global__ void k_to_f32(long n, const double *__restrict__ a, float *b) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = (float)a[i];
+}
+
+// Same over the multipoles of the active cells counted on the device.
+__global__ void k_to_f32_dc(long per, const double *__restrict__ a, float *b,
+                            const int *__restrict__ dc) {
+  const long n = dc[kGcOvf] ? 0 : (long)dc[kGcActive] * per;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -488,12 +516,14 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(M2LArgs P) {
 // workgroups per SIMD pair, as the one-pass kernel reaches.
 __global__ void __launch_bounds__(256, 2) k_m2l_mfma_batch(const M2LJob *__restrict__ jobs,
                                                            const M2LArgs *__restrict__ cases,
-                                                           int qlen, int *qcnt) {
+                                                           int qlen, int *qcnt,
+                                                           const int *__restrict__ dqlen) {
   extern __shared__ float sB[];  // [ks * 4][kNT + 4]
   __shared__ int sSrc[kNT];
   __shared__ int sMask;
   __shared__ int sJob;
   const int x = blockIdx.x & 7;
+  if (dqlen) qlen = *dqlen;
   for (;;) {
     if (threadIdx.x == 0) sJob = atomicAdd(&qcnt[x], 1);
     __syncthreads();
@@ -581,7 +611,8 @@ __global__ void __launch_bounds__(kWave * kEvalWaves)
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   f4 *ss = ssm[wv];
   double *sd = sdm[wv];
-  const int total = BATCH ? off[ncase] : A0.n_items;
+  const int total = BATCH ? off[ncase]
+                          : (A0.G.dc ? (g_stop(A0.G) ? 0 : A0.G.dc[kGcItems]) : A0.n_items);
   // the next item of the batch, taken by lane 0 for the wavefront
   auto next_item = [&]() {
     int v = 0;
@@ -794,7 +825,7 @@ int tuned_pass_prepare(FmmHandle *h) {
     for (int e = 0; e < kMaxLevels * 8; ++e) gb.b[e] = e < h->nlev * 8 ? t->gbase[e] : 0;
     k_group_place<<<ceil_div(nt, 256), 256, 0, h->stream>>>(a1, nt, h->active.p, t->gkey.p + nt,
                                                             t->gidx.p + nt, t->gstart.p, gb,
-                                                            h->slot.p, t->grp.p, t->pos.p);
+                                                            h->slot.p, t->grp.p, t->pos.p, nullptr);
   }
   for (int l = 1; l < h->nlev; ++l) {
     int na = h->act_start[l + 1] - h->act_start[l];
@@ -1003,7 +1034,8 @@ int tuned_batch_launch(FmmHandle *const *hs, int nh, hipStream_t st, hipEvent_t 
     DUST_HIP(hipMemcpyAsync(B.m2l.p, B.hm2l.p, nh * sizeof(M2LArgs), hipMemcpyHostToDevice, st));
     unsigned grid = 0;
     DUST_TRY(persistent_grid(kern, 256, lds, &grid));
-    k_m2l_mfma_batch<<<grid, 256, lds, st>>>(B.jobs.p, B.m2l.p, (int)(njob / 8), B.cnt.p);
+    k_m2l_mfma_batch<<<grid, 256, lds, st>>>(B.jobs.p, B.m2l.p, (int)(njob / 8), B.cnt.p,
+                                              nullptr);
     DUST_HIP(hipGetLastError());
   }
   DUST_HIP(hipEventRecord(ev_m2l, st));
@@ -1063,6 +1095,184 @@ int tuned_eval(FmmHandle *h, EvalArgs &A) {
       else k_p2p_f32<0, 0, 0><<<grid, block, 0, h->stream>>>(A, nullptr, nullptr, 0, nullptr);
     }
   }
+  DUST_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Graph form of the tuned M2L: targets grouped as above with every count on
+// the device, the job list built on the device in the order of
+// m2l_job_list (chunks in (level, octant, offset chunk) order, stably sorted
+// by job count, largest first; each to the least loaded of the 8 XCD queues;
+// jobs of a chunk target tile by target tile, row blocks inside; short
+// queues padded with empty jobs), then the persistent M2L over the queues.
+// ---------------------------------------------------------------------------
+constexpr int kGraphChunks = 2048;
+
+__global__ void __launch_bounds__(256)
+    k_m2l_jobs(TreeGeom G, const int *__restrict__ gcount, int jc, int nrb, GroupBase gb,
+               M2LJob *jobs, int job_cap, int *dc) {
+  __shared__ int cnj[kGraphChunks];  // jobs of every chunk
+  __shared__ int ord[kGraphChunks];  // chunks in dealing order
+  __shared__ int cq[kGraphChunks];   // queue of every chunk
+  __shared__ int cst[kGraphChunks];  // first queue position of every chunk
+  __shared__ int qlen[8];
+  __shared__ int sqmax;
+  const int t = threadIdx.x;
+  if (g_stop(G)) {
+    if (t == 0) dc[kGcQlen] = 0;
+    return;
+  }
+  const int nj = (kOct + jc - 1) / jc;
+  const int nc = (G.nlev - 1) * 8 * nj;
+  for (int q = t; q < nc; q += blockDim.x) {
+    const int e = q / nj;
+    const int nt = gcount[(1 + e / 8) * 8 + (e & 7)];
+    cnj[q] = nt > 0 ? ((nt + kNT - 1) / kNT) * nrb : 0;
+  }
+  __syncthreads();
+  for (int q = t; q < nc; q += blockDim.x) {
+    const int v = cnj[q];
+    int r = 0;
+    for (int p = 0; p < nc; ++p) {
+      const int u = cnj[p];
+      r += (u > v || (u == v && p < q)) ? 1 : 0;
+    }
+    ord[r] = q;
+  }
+  __syncthreads();
+  if (t == 0) {
+    long load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int r = 0; r < nc; ++r) {
+      const int q = ord[r];
+      if (cnj[q] == 0) break;  // the rest hold no jobs (sorted)
+      int x = 0;
+      for (int y = 1; y < 8; ++y)
+        if (load[y] < load[x]) x = y;
+      cq[q] = x;
+      cst[q] = (int)load[x];
+      load[x] += cnj[q];
+    }
+    long qm = 0;
+    for (int y = 0; y < 8; ++y) {
+      qlen[y] = (int)load[y];
+      qm = load[y] > qm ? load[y] : qm;
+    }
+    if (qm * 8 > job_cap) {
+      atomicOr(&dc[kGcOvf], 16);
+      qm = 0;
+    }
+    sqmax = (int)qm;
+    dc[kGcQlen] = (int)qm;
+  }
+  __syncthreads();
+  const int qmax = sqmax;
+  if (qmax == 0) return;
+  for (int q = 0; q < nc; ++q) {
+    const int nq = cnj[q];
+    if (nq == 0) continue;
+    const int e = q / nj, lvl = 1 + e / 8, g = e & 7;
+    const int j0 = (q - e * nj) * jc;
+    const int nt = gcount[lvl * 8 + g];
+    for (int k = t; k < nq; k += blockDim.x) {
+      M2LJob J;
+      J.lvl = lvl; J.oct = g; J.t0 = (k / nrb) * kNT; J.j0 = j0; J.rb = k % nrb; J.nt = nt;
+      J.grp_off = gb.b[lvl * 8 + g];
+      J.kd_base = G.kd_base[lvl];
+      J.kdl0 = G.kdl[lvl][0]; J.kdl1 = G.kdl[lvl][1]; J.kdl2 = G.kdl[lvl][2];
+      J.pad = 0;
+      jobs[(long)(cst[q] + k) * 8 + cq[q]] = J;
+    }
+  }
+  for (int y = 0; y < 8; ++y)
+    for (int k = qlen[y] + t; k < qmax; k += blockDim.x) {
+      M2LJob J{};
+      J.nt = 0;  // t0 >= nt: the job does nothing
+      jobs[(long)k * 8 + y] = J;
+    }
+}
+
+// Buffers of the graph form, sized by the handle's graph capacities; the
+// persistent M2L's arguments uploaded once.
+int tuned_graph_reserve(FmmHandle *h) {
+  TunedState *t = TS(h);
+  GraphState &R = h->gr;
+  const int nj = ceil_div(kOct, t->m2l_chunk);
+  DUST_CHECK((h->nlev - 1) * 8 * nj <= kGraphChunks, DUST_E_ARG,
+             "graph form: %d M2L chunks exceed %d (raise m2l_chunk)", (h->nlev - 1) * 8 * nj,
+             kGraphChunks);
+  long tot = 0;
+  for (int l = 0; l < h->nlev; ++l) {
+    for (int g = 0; g < 8; ++g) t->ggbase[l * 8 + g] = tot + (long)g * R.lev_cap[l];
+    tot += 8L * R.lev_cap[l];
+  }
+  for (int e = h->nlev * 8; e < kMaxLevels * 8; ++e) t->ggbase[e] = 0;
+  const long na = std::max(R.act_cap, 1);
+  DUST_TRY(t->grp.reserve(std::max(tot, 1L)));
+  DUST_TRY(t->src.reserve(std::max(tot, 1L) * kOct));
+  DUST_TRY(t->pos.reserve(na));
+  DUST_TRY(t->gkey.reserve(2 * na));
+  DUST_TRY(t->gidx.reserve(2 * na));
+  DUST_TRY(t->gstart.reserve(kMaxLevels * 8));
+  DUST_TRY(t->a32.reserve(na * 3 * h->nmon));
+  DUST_TRY(t->gq.reserve(8));
+  // jobs: every group's target tiles times row blocks times chunks, with
+  // one padding row per queue
+  const int nrb = ceil_div(wgeom(h).rt, kRTB);
+  long jobs = 0;
+  for (int l = 1; l < h->nlev; ++l) jobs += 8L * (ceil_div(R.lev_cap[l], kNT) + 1) * nrb * nj;
+  jobs = std::min(jobs, 8L * ((long)INT32_MAX / 16));
+  R.job_cap = (int)std::max(jobs + 8L * nrb * nj * 8, 8L);
+  DUST_TRY(t->gjobs.reserve((size_t)R.job_cap * sizeof(M2LJob)));
+  DUST_TRY(t->gargs.reserve(sizeof(M2LArgs)));
+  M2LArgs P = m2l_args(h);
+  P.jobs = (const M2LJob *)t->gjobs.p;
+  DUST_HIP(hipMemcpy(t->gargs.p, &P, sizeof(M2LArgs), hipMemcpyHostToDevice));
+  DUST_TRY(m2l_lds_attr((const void *)k_m2l_mfma_batch, m2l_lds(wgeom(h))));
+  DUST_TRY(persistent_grid((const void *)k_m2l_mfma_batch, 256, m2l_lds(wgeom(h)), &t->gm2l_grid));
+  return 0;
+}
+
+void tuned_graph_buffers(const FmmHandle *h, std::vector<const void *> &v) {
+  const TunedState *t = TS(h);
+  const void *bufs[] = {t->grp.p, t->src.p, t->pos.p, t->gkey.p, t->gidx.p, t->gstart.p,
+                        t->a32.p, t->gq.p, t->gjobs.p, t->gargs.p, t->gcount.p, t->offs.p, t->W.p};
+  for (const void *b : bufs) v.push_back(b);
+}
+
+// Tuned M2L of every level below the first, enqueued with device counts.
+int tuned_graph_m2l(FmmHandle *h) {
+  TunedState *t = TS(h);
+  GraphState &R = h->gr;
+  TreeGeom G = make_geom(h);
+  int *dc = R.dc.p;
+  const int cap = std::max(R.act_cap, 1);
+  k_batch_clear<<<1, 128, 0, h->stream>>>(t->gcount.p, kMaxLevels * 8);
+  k_batch_clear<<<1, 64, 0, h->stream>>>(t->gq.p, 8);
+  k_group_key<<<ceil_div(cap, 256), 256, 0, h->stream>>>(G, h->active.p, 0, 0, h->flags.p,
+                                                         t->gkey.p, t->gidx.p, t->gcount.p);
+  DCount c{dc + kGcActive, dc + kGcActStart + 1, dc + kGcOvf, cap};
+  DUST_TRY(gprim_sort_pairs_u64(h, t->gkey.p, t->gidx.p, t->gkey.p + cap, t->gidx.p + cap, c,
+                                kKeyBits));
+  k_group_starts<<<1, 64, 0, h->stream>>>(t->gcount.p, t->gstart.p);
+  GroupBase gb;
+  for (int e = 0; e < kMaxLevels * 8; ++e) gb.b[e] = t->ggbase[e];
+  k_group_place<<<ceil_div(cap, 256), 256, 0, h->stream>>>(0, 0, h->active.p, t->gkey.p + cap,
+                                                           t->gidx.p + cap, t->gstart.p, gb,
+                                                           h->slot.p, t->grp.p, t->pos.p, dc);
+  for (int l = 1; l < h->nlev; ++l) {
+    if (R.lev_cap[l] == 0) continue;
+    k_group_src<<<ceil_div((long)R.lev_cap[l] * kOct, 256L), 256, 0, h->stream>>>(
+        G, l, h->active.p, 0, 0, h->slot.p, h->flags.p, t->offs.p, t->pos.p, t->src.p);
+  }
+  k_to_f32_dc<<<2048, 256, 0, h->stream>>>(3L * h->nmon, h->mp_a.p, t->a32.p, dc);
+  const int nrb = ceil_div(wgeom(h).rt, kRTB);
+  k_m2l_jobs<<<1, 256, 0, h->stream>>>(G, t->gcount.p, t->m2l_chunk, nrb, gb,
+                                       (M2LJob *)t->gjobs.p, R.job_cap, dc);
+  const size_t lds = m2l_lds(wgeom(h));
+  k_m2l_mfma_batch<<<t->gm2l_grid, 256, lds, h->stream>>>((const M2LJob *)t->gjobs.p,
+                                                  (const M2LArgs *)t->gargs.p, 0, t->gq.p,
+                                                  dc + kGcQlen);
   DUST_HIP(hipGetLastError());
   return 0;
 }

@@ -171,6 +171,48 @@ int dust_fmm_pass_dev(dust_fmm_t h, int *nleaves);
  * runs); for many-case workloads. */
 int dust_fmm_pass_batch_dev(int nh, dust_fmm_t *handles, int *nleaves);
 
+/* Graph form of the pass: the sequence of dust_fmm_pass_dev
+ * (sort_particles, calculate_multipole, apply_multipole,
+ * src/fmm/mod_octree.f90:703-1429, as called once per step from the time
+ * loop, src/dust.f90:504-1160) with every count on the device, so that a
+ * stream capture can record it and a graph can replay it for a wake whose
+ * particle count and tree change from step to step.  No counterpart in the
+ * reference; for time loops captured in a HIP graph.
+ *
+ * dust_fmm_graph_reserve: fixes the capacities from the last completed pass
+ * on the handle (its active cells, leaves, cells per level and P2P items,
+ * each times `slack` plus 64; particles up to the handle capacity) and
+ * allocates for them.  Not with the particle redistribution or a gust
+ * (DUST_E_ARG).
+ *
+ * dust_fmm_pass_graph_dev: enqueues the pass on the handle's stream without
+ * host synchronisation, allocation or memset: particle count read from
+ * *n_dev, inputs read from cen/dir/mag/rv (device, the layouts of
+ * dust_fmm_set_particles), velocity and stretching written to vel/stretch
+ * (device, (3, n); stretch may be NULL).  Capturable; the FP64 form gives the
+ * same values as dust_fmm_pass_dev bit for bit, the tuned form to its M2L's
+ * atomic order.  The host-side entry points that read a pass's lists
+ * (dust_fmm_leaves, dust_fmm_results_dev, apply_multipole_panels) need a
+ * pass of dust_fmm_pass_dev after it.
+ *
+ * dust_fmm_graph_capture: captures that sequence into a graph owned by the
+ * handle (the pointers are fixed in it: refill the same buffers between
+ * launches); dust_fmm_graph_launch replays it on the handle's stream.
+ *
+ * dust_fmm_graph_status: synchronises the stream and reports the last graph
+ * pass: DUST_E_CAPACITY when a list outgrew its capacity or *n_dev was
+ * outside 0..capacity (every kernel then did nothing past that point: reserve
+ * again after a dust_fmm_pass_dev and capture again), DUST_E_OUTSIDE for a
+ * particle outside the box (results invalid).  counts (may be NULL, 4 ints):
+ * particles, active cells, leaves, P2P items of that pass. */
+int dust_fmm_graph_reserve(dust_fmm_t h, double slack);
+int dust_fmm_pass_graph_dev(dust_fmm_t h, const int *n_dev, const double *cen, const double *dir,
+                            const double *mag, const double *rv, double *vel, double *stretch);
+int dust_fmm_graph_capture(dust_fmm_t h, const int *n_dev, const double *cen, const double *dir,
+                           const double *mag, const double *rv, double *vel, double *stretch);
+int dust_fmm_graph_launch(dust_fmm_t h);
+int dust_fmm_graph_status(dust_fmm_t h, int *counts);
+
 /* Knobs.  Tuned form: "near_factor" -- pairs closer than near_factor times
  * the source core radius are evaluated in FP64 (default 1); "m2l_chunk" --
  * interaction offsets per M2L job, 1..189 (default 21): larger chunks mean
