@@ -1112,12 +1112,14 @@ constexpr int kGraphChunks = 2048;
 __global__ void __launch_bounds__(256)
     k_m2l_jobs(TreeGeom G, const int *__restrict__ gcount, int jc, int nrb, GroupBase gb,
                M2LJob *jobs, int job_cap, int *dc) {
-  __shared__ int cnj[kGraphChunks];  // jobs of every chunk
-  __shared__ int ord[kGraphChunks];  // chunks in dealing order
-  __shared__ int cq[kGraphChunks];   // queue of every chunk
-  __shared__ int cst[kGraphChunks];  // first queue position of every chunk
+  __shared__ int cnj[kGraphChunks];       // jobs of every chunk
+  __shared__ int ord[kGraphChunks];       // chunks in dealing order
+  __shared__ int sn[kGraphChunks];        // jobs of the r-th dealt chunk
+  __shared__ int rq[kGraphChunks];        // queue of the r-th dealt chunk
+  __shared__ int rst[kGraphChunks];       // its first position in that queue
+  __shared__ int cpre[kGraphChunks + 1];  // jobs before the r-th dealt chunk
   __shared__ int qlen[8];
-  __shared__ int sqmax;
+  __shared__ int sqmax, snd;
   const int t = threadIdx.x;
   if (g_stop(G)) {
     if (t == 0) dc[kGcQlen] = 0;
@@ -1131,6 +1133,7 @@ __global__ void __launch_bounds__(256)
     cnj[q] = nt > 0 ? ((nt + kNT - 1) / kNT) * nrb : 0;
   }
   __syncthreads();
+  // stable order by job count, largest first
   for (int q = t; q < nc; q += blockDim.x) {
     const int v = cnj[q];
     int r = 0;
@@ -1139,20 +1142,32 @@ __global__ void __launch_bounds__(256)
       r += (u > v || (u == v && p < q)) ? 1 : 0;
     }
     ord[r] = q;
+    sn[r] = v;
   }
   __syncthreads();
+  // each chunk to the least loaded queue (lowest index on ties); the job
+  // counts are read in order, independently of the loop-carried loads
   if (t == 0) {
     long load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int nd = nc, sum = 0;
+    cpre[0] = 0;
     for (int r = 0; r < nc; ++r) {
-      const int q = ord[r];
-      if (cnj[q] == 0) break;  // the rest hold no jobs (sorted)
+      const int v = sn[r];
+      if (v == 0) {  // the rest hold no jobs (sorted)
+        nd = r;
+        break;
+      }
       int x = 0;
+#pragma unroll
       for (int y = 1; y < 8; ++y)
         if (load[y] < load[x]) x = y;
-      cq[q] = x;
-      cst[q] = (int)load[x];
-      load[x] += cnj[q];
+      rq[r] = x;
+      rst[r] = (int)load[x];
+      load[x] += v;
+      sum += v;
+      cpre[r + 1] = sum;
     }
+    snd = nd;
     long qm = 0;
     for (int y = 0; y < 8; ++y) {
       qlen[y] = (int)load[y];
@@ -1168,21 +1183,25 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
   const int qmax = sqmax;
   if (qmax == 0) return;
-  for (int q = 0; q < nc; ++q) {
-    const int nq = cnj[q];
-    if (nq == 0) continue;
-    const int e = q / nj, lvl = 1 + e / 8, g = e & 7;
-    const int j0 = (q - e * nj) * jc;
-    const int nt = gcount[lvl * 8 + g];
-    for (int k = t; k < nq; k += blockDim.x) {
-      M2LJob J;
-      J.lvl = lvl; J.oct = g; J.t0 = (k / nrb) * kNT; J.j0 = j0; J.rb = k % nrb; J.nt = nt;
-      J.grp_off = gb.b[lvl * 8 + g];
-      J.kd_base = G.kd_base[lvl];
-      J.kdl0 = G.kdl[lvl][0]; J.kdl1 = G.kdl[lvl][1]; J.kdl2 = G.kdl[lvl][2];
-      J.pad = 0;
-      jobs[(long)(cst[q] + k) * 8 + cq[q]] = J;
+  // job j of the dealing order: the r-th dealt chunk with cpre[r] <= j < cpre[r + 1]
+  const int nd = snd, njob = cpre[nd];
+  for (int j = t; j < njob; j += blockDim.x) {
+    int lo = 0, hi = nd - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (cpre[mid] <= j) lo = mid;
+      else hi = mid - 1;
     }
+    const int q = ord[lo], k = j - cpre[lo];
+    const int e = q / nj, lvl = 1 + e / 8, g = e & 7;
+    M2LJob J;
+    J.lvl = lvl; J.oct = g; J.t0 = (k / nrb) * kNT; J.j0 = (q - e * nj) * jc; J.rb = k % nrb;
+    J.nt = gcount[lvl * 8 + g];
+    J.grp_off = gb.b[lvl * 8 + g];
+    J.kd_base = G.kd_base[lvl];
+    J.kdl0 = G.kdl[lvl][0]; J.kdl1 = G.kdl[lvl][1]; J.kdl2 = G.kdl[lvl][2];
+    J.pad = 0;
+    jobs[(long)(rst[lo] + k) * 8 + rq[lo]] = J;
   }
   for (int y = 0; y < 8; ++y)
     for (int k = qlen[y] + t; k < qmax; k += blockDim.x) {
