@@ -57,7 +57,7 @@ struct TunedState {
   // on the device, the persistent M2L's arguments and its 8 queue counters
   long ggbase[kMaxLevels * 8];
   DevBuf<unsigned char> gjobs, gargs;
-  DevBuf<int> gq;
+  DevBuf<int> gq, gtab;
   unsigned gm2l_grid = 0;
 };
 
@@ -1109,20 +1109,27 @@ int tuned_eval(FmmHandle *h, EvalArgs &A) {
 // ---------------------------------------------------------------------------
 constexpr int kGraphChunks = 2048;
 
+// Dealing table of the graph form (int offsets in TunedState::gtab): chunk,
+// queue, first queue position and jobs before, per dealt chunk; then the 8
+// queue lengths, the number of dealt chunks, the job total and the longest
+// queue.
+constexpr int kTabOrd = 0, kTabQ = kGraphChunks, kTabSt = 2 * kGraphChunks,
+              kTabPre = 3 * kGraphChunks, kTabQlen = 4 * kGraphChunks + 1,
+              kTabNd = kTabQlen + 8, kTabNjob = kTabNd + 1, kTabQmax = kTabNjob + 1,
+              kTabSize = kTabQmax + 1;
+
 __global__ void __launch_bounds__(256)
-    k_m2l_jobs(TreeGeom G, const int *__restrict__ gcount, int jc, int nrb, GroupBase gb,
-               M2LJob *jobs, int job_cap, int *dc) {
-  __shared__ int cnj[kGraphChunks];       // jobs of every chunk
-  __shared__ int ord[kGraphChunks];       // chunks in dealing order
-  __shared__ int sn[kGraphChunks];        // jobs of the r-th dealt chunk
-  __shared__ int rq[kGraphChunks];        // queue of the r-th dealt chunk
-  __shared__ int rst[kGraphChunks];       // its first position in that queue
-  __shared__ int cpre[kGraphChunks + 1];  // jobs before the r-th dealt chunk
-  __shared__ int qlen[8];
-  __shared__ int sqmax, snd;
+    k_m2l_jobs(TreeGeom G, const int *__restrict__ gcount, int jc, int nrb, int job_cap, int *tab,
+               int *dc) {
+  __shared__ int cnj[kGraphChunks];  // jobs of every chunk
+  __shared__ int sn[kGraphChunks];   // jobs of the r-th dealt chunk
   const int t = threadIdx.x;
   if (g_stop(G)) {
-    if (t == 0) dc[kGcQlen] = 0;
+    if (t == 0) {
+      dc[kGcQlen] = 0;
+      tab[kTabQmax] = 0;
+      tab[kTabNjob] = 0;
+    }
     return;
   }
   const int nj = (kOct + jc - 1) / jc;
@@ -1141,16 +1148,15 @@ __global__ void __launch_bounds__(256)
       const int u = cnj[p];
       r += (u > v || (u == v && p < q)) ? 1 : 0;
     }
-    ord[r] = q;
+    tab[kTabOrd + r] = q;
     sn[r] = v;
   }
   __syncthreads();
-  // each chunk to the least loaded queue (lowest index on ties); the job
-  // counts are read in order, independently of the loop-carried loads
+  // each chunk to the least loaded queue (lowest index on ties)
   if (t == 0) {
     long load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int nd = nc, sum = 0;
-    cpre[0] = 0;
+    tab[kTabPre] = 0;
     for (int r = 0; r < nc; ++r) {
       const int v = sn[r];
       if (v == 0) {  // the rest hold no jobs (sorted)
@@ -1161,38 +1167,51 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int y = 1; y < 8; ++y)
         if (load[y] < load[x]) x = y;
-      rq[r] = x;
-      rst[r] = (int)load[x];
+      tab[kTabQ + r] = x;
+      tab[kTabSt + r] = (int)load[x];
       load[x] += v;
       sum += v;
-      cpre[r + 1] = sum;
+      tab[kTabPre + r + 1] = sum;
     }
-    snd = nd;
     long qm = 0;
     for (int y = 0; y < 8; ++y) {
-      qlen[y] = (int)load[y];
+      tab[kTabQlen + y] = (int)load[y];
       qm = load[y] > qm ? load[y] : qm;
     }
     if (qm * 8 > job_cap) {
       atomicOr(&dc[kGcOvf], 16);
       qm = 0;
+      sum = 0;
     }
-    sqmax = (int)qm;
+    tab[kTabNd] = nd;
+    tab[kTabNjob] = sum;
+    tab[kTabQmax] = (int)qm;
     dc[kGcQlen] = (int)qm;
   }
-  __syncthreads();
-  const int qmax = sqmax;
+}
+
+// The jobs of the dealt chunks (job j: the r-th dealt chunk with
+// pre[r] <= j < pre[r + 1], searched in an LDS copy of pre) and the padding
+// of the short queues, over the whole grid.
+__global__ void __launch_bounds__(256)
+    k_m2l_fill(TreeGeom G, const int *__restrict__ gcount, int jc, int nrb, GroupBase gb,
+               const int *__restrict__ tab, M2LJob *jobs) {
+  __shared__ int pre[kGraphChunks + 1];
+  const int qmax = tab[kTabQmax];
   if (qmax == 0) return;
-  // job j of the dealing order: the r-th dealt chunk with cpre[r] <= j < cpre[r + 1]
-  const int nd = snd, njob = cpre[nd];
-  for (int j = t; j < njob; j += blockDim.x) {
+  const int nd = tab[kTabNd], njob = tab[kTabNjob];
+  for (int r = threadIdx.x; r <= nd; r += blockDim.x) pre[r] = tab[kTabPre + r];
+  __syncthreads();
+  const int nj = (kOct + jc - 1) / jc;
+  const int stride = gridDim.x * blockDim.x;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < njob; j += stride) {
     int lo = 0, hi = nd - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (cpre[mid] <= j) lo = mid;
+      if (pre[mid] <= j) lo = mid;
       else hi = mid - 1;
     }
-    const int q = ord[lo], k = j - cpre[lo];
+    const int q = tab[kTabOrd + lo], k = j - pre[lo];
     const int e = q / nj, lvl = 1 + e / 8, g = e & 7;
     M2LJob J;
     J.lvl = lvl; J.oct = g; J.t0 = (k / nrb) * kNT; J.j0 = (q - e * nj) * jc; J.rb = k % nrb;
@@ -1201,14 +1220,15 @@ __global__ void __launch_bounds__(256)
     J.kd_base = G.kd_base[lvl];
     J.kdl0 = G.kdl[lvl][0]; J.kdl1 = G.kdl[lvl][1]; J.kdl2 = G.kdl[lvl][2];
     J.pad = 0;
-    jobs[(long)(rst[lo] + k) * 8 + rq[lo]] = J;
+    jobs[(long)(tab[kTabSt + lo] + k) * 8 + tab[kTabQ + lo]] = J;
   }
-  for (int y = 0; y < 8; ++y)
-    for (int k = qlen[y] + t; k < qmax; k += blockDim.x) {
-      M2LJob J{};
-      J.nt = 0;  // t0 >= nt: the job does nothing
-      jobs[(long)k * 8 + y] = J;
-    }
+  for (long p = blockIdx.x * blockDim.x + threadIdx.x; p < 8L * qmax; p += stride) {
+    const int y = (int)(p & 7), k = (int)(p >> 3);
+    if (k < tab[kTabQlen + y]) continue;
+    M2LJob J{};
+    J.nt = 0;  // t0 >= nt: the job does nothing
+    jobs[p] = J;
+  }
 }
 
 // Buffers of the graph form, sized by the handle's graph capacities; the
@@ -1235,6 +1255,7 @@ int tuned_graph_reserve(FmmHandle *h) {
   DUST_TRY(t->gstart.reserve(kMaxLevels * 8));
   DUST_TRY(t->a32.reserve(na * 3 * h->nmon));
   DUST_TRY(t->gq.reserve(8));
+  DUST_TRY(t->gtab.reserve(kTabSize));
   // jobs: every group's target tiles times row blocks times chunks, with
   // one padding row per queue
   const int nrb = ceil_div(wgeom(h).rt, kRTB);
@@ -1255,7 +1276,8 @@ int tuned_graph_reserve(FmmHandle *h) {
 void tuned_graph_buffers(const FmmHandle *h, std::vector<const void *> &v) {
   const TunedState *t = TS(h);
   const void *bufs[] = {t->grp.p, t->src.p, t->pos.p, t->gkey.p, t->gidx.p, t->gstart.p,
-                        t->a32.p, t->gq.p, t->gjobs.p, t->gargs.p, t->gcount.p, t->offs.p, t->W.p};
+                        t->a32.p, t->gq.p, t->gjobs.p, t->gargs.p, t->gcount.p, t->offs.p, t->W.p,
+                        t->gtab.p};
   for (const void *b : bufs) v.push_back(b);
 }
 
@@ -1286,8 +1308,10 @@ int tuned_graph_m2l(FmmHandle *h) {
   }
   k_to_f32_dc<<<2048, 256, 0, h->stream>>>(3L * h->nmon, h->mp_a.p, t->a32.p, dc);
   const int nrb = ceil_div(wgeom(h).rt, kRTB);
-  k_m2l_jobs<<<1, 256, 0, h->stream>>>(G, t->gcount.p, t->m2l_chunk, nrb, gb,
-                                       (M2LJob *)t->gjobs.p, R.job_cap, dc);
+  k_m2l_jobs<<<1, 256, 0, h->stream>>>(G, t->gcount.p, t->m2l_chunk, nrb, R.job_cap, t->gtab.p,
+                                       dc);
+  k_m2l_fill<<<512, 256, 0, h->stream>>>(G, t->gcount.p, t->m2l_chunk, nrb, gb, t->gtab.p,
+                                         (M2LJob *)t->gjobs.p);
   const size_t lds = m2l_lds(wgeom(h));
   k_m2l_mfma_batch<<<t->gm2l_grid, 256, lds, h->stream>>>((const M2LJob *)t->gjobs.p,
                                                   (const M2LArgs *)t->gargs.p, 0, t->gq.p,
