@@ -164,3 +164,37 @@ def test_graph_capacity_and_outside_reported():
     L.check(lib.dust_fmm_graph_launch(g.handle), "launch")
     L.check(lib.dust_fmm_graph_status(g.handle, None), "status")
     g.destroy()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "tuned"])
+def test_graph_replay_against_oracle(orc, precision):
+    """A replay captured on one draw, run on another, against the FP64 oracle
+    on that draw: FP64 form within 1e-10, tuned within 1e-6 (relative L2 of
+    velocity; stretching 1e-10 / 1e-5)."""
+    import torch
+    lib = L.lib()
+    cap = 12000
+    first = W.helix_wake(9000, seed=501)
+    tree = dict(first[3], nlevels=5)
+    g = _tree_handle(tree, precision, cap)
+    b = Bufs(torch, cap)
+    b.load(torch, *first[:3])
+    _eager(torch, g, b, 9000)
+    L.check(lib.dust_fmm_graph_reserve(g.handle, 2.5), "reserve")
+    L.check(lib.dust_fmm_graph_capture(g.handle, *b.args()), "capture")
+    cen, alpha, rv, _ = W.helix_wake(11000, seed=502, pitch=0.55)
+    n = len(cen)
+    print(f"graph form ({precision}) vs oracle, n={n}", flush=True)
+    b.load(torch, cen, alpha, rv)
+    L.check(lib.dust_fmm_graph_launch(g.handle), "launch")
+    L.check(lib.dust_fmm_graph_status(g.handle, None), "status")
+    vel = b.vel[:n].cpu().numpy()
+    st = b.st[:n].cpu().numpy()
+    g.destroy()
+    mag = np.linalg.norm(alpha, axis=1)
+    c = orc.Octree(tree["box_length"], tree["nbox"], tree["origin"], tree["nlevels"],
+                   tree["min_part"], 4, tree["delta"])
+    v, s, _ = c.run(cen, alpha / mag[:, None], mag, rv)
+    tv, ts = (1e-10, 1e-10) if precision == "fp64" else (1e-6, 1e-5)
+    assert np.linalg.norm(vel - v) / np.linalg.norm(v) < tv
+    assert np.linalg.norm(st - s) / np.linalg.norm(s) < ts
